@@ -1,0 +1,226 @@
+#!/usr/bin/env python3
+"""Forward path-tracing throughput on one MI355X.
+
+A step renders one full frame of the workload: reset_accumulation, then render_tiles with one
+full-frame tile per sample (spp tiles) through the C ABI, exactly what ptbench's render loop
+does per frame (BenchmarkRuntime.cpp:680-695). Inputs (scene, BVH) are resident in HBM before
+the timed region; the BVH build is reported separately as the reference does.
+
+Default workload: procedural 1M-triangle scene (displaced 655,360-triangle icosphere + 2048
+instanced 168-triangle spheres + 16 quad lights), 1920x1080, 64 spp, depth 8, NEE + MIS, Russian
+roulette from depth 3, binned-SAH BVH with leaves of at most 4 triangles, seed 0xBADC0FFEE.
+
+Prints one JSON line. `value` is Mpaths/s (W*H*spp per second, the reference's samples_per_sec);
+Mrays/s (extension + shadow rays) and the reference's paths_per_sec (= extension rays/s) are extra
+fields. `roofline` is computed live from HIP events around every kernel launch on the tracer's
+stream and the kernels' own work counters.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E
+
+# Algorithmic bytes per unit of work, by kernel (see DESIGN.md "Roofline accounting").
+BYTES = {
+    "ray_in": 24, "hit_out": 8, "node": 64, "triangle": 48,
+    "shade_path": 204, "shadow_record": 48, "shadow_ray": 56, "raygen_path": 60, "film_sample": 52,
+}
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def workload(name: str, spp: int | None):
+    from vkpathtracer_amd import scene as S
+
+    if name == "procedural":
+        sc = S.procedural_scene(seed=1)
+        st = S.RenderSettings(width=1920, height=1080, spp=spp or 64, max_depth=8)
+        desc = "procedural 1M-tri (displaced icosphere 655360 + 2048x168 spheres + 16 quad lights), 1920x1080"
+    else:
+        sc = S.cornell_box()
+        st = S.RenderSettings(width=256, height=256, spp=spp or 16, max_depth=8)
+        desc = "cornell box 34 tris + light quad, 256x256"
+    return sc, st, desc
+
+
+def kernel_bytes(ks: dict, counters: dict) -> dict:
+    ext_rays = counters["rays"]
+    sh_rays = ks["shadow_rays"]
+    ext_nodes = ks["inner_node_fetches"] - ks["shadow_inner_node_fetches"]
+    ext_tris = counters["triangle_tests"] - ks["shadow_triangle_tests"]
+    return {
+        "extend": ext_rays * (BYTES["ray_in"] + BYTES["hit_out"]) + ext_nodes * BYTES["node"] + ext_tris * BYTES["triangle"],
+        "shadow": sh_rays * BYTES["shadow_ray"] + ks["shadow_inner_node_fetches"] * BYTES["node"]
+                  + ks["shadow_triangle_tests"] * BYTES["triangle"],
+        "shade": ext_rays * BYTES["shade_path"] + sh_rays * BYTES["shadow_record"],
+        "raygen": ks["raygen"]["items"] * BYTES["raygen_path"],
+        "film": ks["film"]["items"] * BYTES["film_sample"],
+    }
+
+
+def cpu_baseline(sc, st, target_seconds: float) -> dict:
+    """The oracle (CPU restatement, fp32, all host cores via OpenMP) on a bounded seeded pixel sample."""
+    from oracle import oracle as O
+
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    o = O.Oracle(sc, builder="sah")
+    rng = np.random.default_rng(2024)
+    n_px = st.width * st.height
+    probe = rng.choice(n_px, min(n_px, 20000), replace=False).astype(np.uint32)
+    t0 = time.perf_counter()
+    o.render_pixels(st, probe, 0, threads=cores)
+    rate = len(probe) / max(time.perf_counter() - t0, 1e-6)
+    samples = 4
+    n = int(min(n_px, max(10000, rate * target_seconds / samples)))
+    px = rng.choice(n_px, n, replace=False).astype(np.uint32)
+    t0 = time.perf_counter()
+    for s in range(samples):
+        o.render_pixels(st, px, s, threads=cores)
+    dt = time.perf_counter() - t0
+    return {"value": round(n * samples / dt / 1e6, 4), "unit": "Mpaths/s", "cores": cores, "kind": "port",
+            "sample": f"{n} seeded pixels x samples 0-{samples - 1} of the same workload ({n * samples} paths, "
+                      f"{dt:.1f} s), oracle fp32 restatement, OpenMP over {cores} threads, {cpu_model()}"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["procedural", "cornell"], default="procedural")
+    ap.add_argument("--spp", type=int, default=None)
+    ap.add_argument("--capacity", type=int, default=None, help="paths in flight per wavefront pass")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-profile", action="store_true", help="time without per-kernel events")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build renders on a single MI355X (no multi-GPU path); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from vkpathtracer_amd.tracer import HipPathTracer
+
+    sc, st, desc = workload(args.workload, args.spp)
+    pt = HipPathTracer(0)
+    assert pt.configure(st)
+    assert pt.load_scene_snapshot(sc)
+    t0 = time.perf_counter()
+    bst = pt.build_or_update_acceleration("sah", 4, 8)
+    build_wall_ms = (time.perf_counter() - t0) * 1e3
+    assert bst, bst.message
+    if args.capacity:
+        pt.set_wavefront_capacity(args.capacity)
+    tiles = [(0, 0, st.width, st.height, s, 0, 0) for s in range(st.spp)]
+
+    def step():
+        pt.reset_accumulation()
+        if not pt.render_tiles(tiles, 0):
+            raise RuntimeError(pt.status().last_error)
+
+    for _ in range(args.warmup):
+        step()
+    pt.synchronize()
+    torch.cuda.synchronize()
+    pt.set_profiling(not args.no_profile)
+    pt.reset_kernel_stats()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    pt.synchronize()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    counters = pt.read_counters()  # counters of the last step (reset_accumulation clears them)
+    ks = pt.kernel_stats()
+    pt.set_profiling(False)
+
+    paths = st.width * st.height * st.spp
+    ms_per_step = dt * 1e3 / args.steps
+    mpaths = paths * args.steps / dt / 1e6
+    ext_rays = counters["rays"]
+    shadow_rays = ks["shadow_rays"]  # device counters are per step (reset_accumulation clears them)
+    mrays = (ext_rays + shadow_rays) * args.steps / dt / 1e6
+
+    roofline = None
+    per_kernel = {}
+    if not args.no_profile:
+        # device work counters cover the last step; event times and launch counts cover all K steps
+        step_ks = dict(ks)
+        for k in ("inner_node_fetches", "shadow_node_visits", "shadow_triangle_tests", "shadow_inner_node_fetches",
+                  "shadow_leaf_visits", "shadow_rays"):
+            step_ks[k] = ks[k] * args.steps
+        tot_counters = {k: v * args.steps for k, v in counters.items()}
+        kb = kernel_bytes(step_ks, tot_counters)
+        for name in ("raygen", "extend", "shade", "shadow", "film"):
+            k = ks[name]
+            if k["launches"]:
+                per_kernel[name] = {"launches": k["launches"], "avg_ms": round(k["total_ms"] / k["launches"], 4),
+                                    "total_ms": round(k["total_ms"], 2),
+                                    "gbps": round(kb[name] / (k["total_ms"] * 1e6), 1) if k["total_ms"] else None}
+        dom = max(per_kernel, key=lambda n: per_kernel[n]["total_ms"])
+        k = ks[dom]
+        achieved = kb[dom] / (k["total_ms"] * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                    "bytes_per_launch": int(kb[dom] / k["launches"]),
+                    "avg_launch_ms": round(k["total_ms"] / k["launches"], 4)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(sc, st, args.cpu_seconds)
+
+    out = {
+        "metric": "Mpaths/s & Mrays/s fwd at 1 GPU vs CPU AVX2; adjoint/forward time ratio",
+        "value": round(mpaths, 3),
+        "unit": "Mpaths/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded procedural scene generated in-process)",
+        "config": {"workload": desc, "width": st.width, "height": st.height, "spp": st.spp,
+                   "max_depth": st.max_depth, "triangles": sc.triangle_count, "bvh": "binned SAH, leaf<=4, 8 buckets",
+                   "nee": True, "mis": True, "rr_start_depth": 3, "seed": hex(st.seed)},
+        "mrays_per_s": round(mrays, 3),
+        "ref_paths_per_sec": round(ext_rays * args.steps / dt, 1),
+        "rays_per_step": {"extension": ext_rays, "shadow": shadow_rays},
+        "bvh_build_ms": round(build_wall_ms, 1),
+        "bvh": {k: pt.build_info[k] for k in ("node_count", "leaf_count", "max_depth", "device_bytes")},
+        "roofline": roofline,
+        "kernels": per_kernel,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["speedup_vs_cpu"] = round(mpaths / cpu["value"], 1)
+    pt.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,245 @@
+/*
+ * vkpt_hip.h — C ABI of the MI355X-native path-tracing hot path.
+ *
+ * This is the drop-in boundary for vkPathTracer's CPU path tracer
+ * (reference: src/pathtracer/PathTracer.h). Every entry point replaces one
+ * IPathTracer / IRayAccelerator method, cited next to it. Plain C types only:
+ * pointers, sizes and POD records; no exceptions cross this boundary and every
+ * call returns a pt_status mirroring vkpt::core::StatusCode
+ * (src/core/contracts/Result.h:13-23). pt_hip_last_error() returns the message
+ * of the last failed call on that handle (IPathTracer::status().last_error,
+ * PathTracer.h:854-865).
+ *
+ * Threading: one caller thread per handle (the reference's contract,
+ * RenderCoordinator.cpp:1353-1391). Each handle owns one HIP stream at default
+ * priority and all device memory it allocates. Calls are not re-entrant.
+ *
+ * Record layouts are fixed (4-byte fields, sizes asserted below) so that the
+ * same host arrays can feed this library and the CPU oracle.
+ */
+#ifndef VKPT_HIP_H
+#define VKPT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* vkpt::core::StatusCode, src/core/contracts/Result.h:13-23. */
+typedef enum pt_status {
+  PT_OK = 0,
+  PT_UNSUPPORTED = 1,
+  PT_INVALID_ARGUMENT = 2,
+  PT_NOT_READY = 3,
+  PT_BUSY = 4,
+  PT_ALLOC_FAILED = 5,
+  PT_TIMEOUT = 6,
+  PT_CANCELLED = 7,
+  PT_INTERNAL_ERROR = 8
+} pt_status;
+
+/* PathTracerLifecycle (PathTracer.h:1052-1067 state machine). */
+typedef enum pt_lifecycle {
+  PT_LIFECYCLE_UNINITIALIZED = 0,
+  PT_LIFECYCLE_CONFIGURED = 1,
+  PT_LIFECYCLE_SCENE_LOADED = 2,
+  PT_LIFECYCLE_READY = 3
+} pt_lifecycle;
+
+/* RTMaterial, PathTracer.h:242-261. 80 bytes. */
+typedef struct PtMaterial {
+  float albedo[3];
+  float emissive[3];
+  float roughness, metallic, ior, transmission, clearcoat, sheen, anisotropy, alpha;
+  uint32_t material_model, material_effect, material_flags;
+  uint32_t base_color_texture_index, normal_texture_index, pad0;
+} PtMaterial;
+
+/* RTHitLight, PathTracer.h:341-349. 64 bytes. */
+typedef struct PtLight {
+  float position[3];
+  float color[3];
+  float intensity, radius;
+  float direction[3];
+  float spot_inner_cos, spot_outer_cos;
+  float pad0, pad1, pad2;
+} PtLight;
+
+/* RTInstance, PathTracer.h:277-296. 96 bytes. flags bit 0 = kRTInstanceFlagDynamicTransform. */
+typedef struct PtInstance {
+  uint64_t entity_id;
+  uint32_t geometry_id, first_triangle, triangle_count, material_index, flags, transform_revision;
+  uint32_t local_first_vertex, local_vertex_count, local_first_index, local_index_count;
+  float translation[3];
+  float rotation[4]; /* quaternion x, y, z, w */
+  float scale[3];
+  uint32_t pad0, pad1;
+} PtInstance;
+
+/* Camera fields of PathTracerSceneSnapshot, PathTracer.h:392-408. 64 bytes. */
+typedef struct PtCamera {
+  float position[3], target[3], up[3];
+  float fov_deg, aperture_radius, focus_distance;
+  uint32_t iris_blade_count;
+  float iris_rotation_degrees, iris_roundness, anamorphic_squeeze;
+} PtCamera;
+
+/* Flattened PathTracerSceneSnapshot (PathTracer.h:358-409). Host pointers; the
+ * library copies what it needs during pt_hip_load_scene and keeps no pointer. */
+typedef struct PtSceneView {
+  const float* vertices; /* xyz per vertex */
+  const uint32_t* indices;
+  const float* local_vertices;
+  const uint32_t* local_indices;
+  const PtInstance* instances;
+  const PtMaterial* materials;
+  const PtLight* lights;
+  const float* environment_map; /* rgb, width*height (unsupported: must be NULL or 0x0) */
+  uint32_t vertex_count, index_count, local_vertex_count, local_index_count;
+  uint32_t instance_count, material_count, light_count;
+  uint32_t environment_map_width, environment_map_height;
+  float environment_color[3];
+  float environment_map_scale[3];
+  PtCamera camera;
+} PtSceneView;
+
+/* RenderSettings fields the hot path reads (PathTracer.h:134-178). 56 bytes. */
+typedef struct PtRenderSettings {
+  uint64_t seed;
+  uint32_t width, height, spp, max_depth;
+  uint32_t enable_nee, enable_mis, deterministic;
+  uint32_t rr_start_depth;
+  float rr_min_survival, rr_max_survival;
+  float camera_aperture_radius, camera_focus_distance;
+} PtRenderSettings;
+
+/* RenderTile, PathTracer.h:919-927. One tile adds one sample (sample_index) to each pixel. */
+typedef struct PtRenderTile {
+  uint32_t x, y, width, height, sample_index, tile_id, gpu_id;
+} PtRenderTile;
+
+/* SampleCounters, PathTracer.h:218-230 (same field order). */
+typedef struct PtSampleCounters {
+  uint64_t samples, rays, triangle_tests, sdf_tests, sdf_steps, triangle_hits;
+  uint64_t sdf_hits, sdf_misses, bvh_node_visits, bvh_leaf_visits, shadow_tests;
+} PtSampleCounters;
+
+/* BVH build parameters. builder 0 = bucketed SAH (BvhBuildConfig,
+ * src/gpu/D3D12GpuPathTracer.SceneBvh.h:15-19; defaults leaf 4, 8 buckets),
+ * builder 1 = midpoint split (ParallelBvhBuilder, src/cpu/ParallelBvhBuilder.cpp:61-163). */
+typedef struct PtBvhParams {
+  uint32_t builder;
+  uint32_t leaf_size;
+  uint32_t bucket_count;
+  uint32_t reserved;
+} PtBvhParams;
+
+enum { PT_BVH_SAH = 0, PT_BVH_MIDPOINT = 1 };
+
+/* RayAcceleratorBuildInfo, PathTracer.h:992-999, plus depth and device bytes. */
+typedef struct PtBuildInfo {
+  double build_ms;
+  uint64_t device_bytes;
+  uint32_t built, primitive_count, node_count, leaf_count, max_depth, deterministic;
+} PtBuildInfo;
+
+/* PathTracerStatus subset (PathTracer.h:854-865). */
+typedef struct PtTracerStatus {
+  uint64_t total_samples;
+  uint64_t accumulation_gen;
+  uint32_t lifecycle, scene_loaded, accel_valid, ready_to_render, current_sample, pad0;
+} PtTracerStatus;
+
+/* Per-kernel HIP-event timings collected while profiling is enabled. */
+enum {
+  PT_KERNEL_RAYGEN = 0,
+  PT_KERNEL_EXTEND = 1,
+  PT_KERNEL_SHADE = 2,
+  PT_KERNEL_SHADOW = 3,
+  PT_KERNEL_FILM = 4,
+  PT_KERNEL_COUNT = 5
+};
+typedef struct PtKernelStats {
+  double total_ms[PT_KERNEL_COUNT];
+  uint64_t launches[PT_KERNEL_COUNT];
+  uint64_t items[PT_KERNEL_COUNT]; /* paths / rays handed to each kernel */
+  uint64_t inner_node_fetches;    /* 64-byte BVH node loads by extend + shadow */
+  uint64_t shadow_node_visits, shadow_triangle_tests;
+  uint64_t shadow_inner_node_fetches, shadow_leaf_visits, shadow_rays;
+} PtKernelStats;
+
+typedef struct pt_tracer pt_tracer;
+
+/* Handle lifetime: the reference constructs ScalarCpuPathTracer directly
+ * (BenchmarkRuntime.cpp:656-664). device is the HIP device ordinal. */
+int pt_hip_create(int device, pt_tracer** out);
+void pt_hip_destroy(pt_tracer* tracer);
+
+/* IPathTracer::configure, PathTracer.h:1078 (ScalarCpuPathTracer.cpp:415-456). */
+int pt_hip_configure(pt_tracer* tracer, const PtRenderSettings* settings);
+/* IPathTracer::load_scene_snapshot, PathTracer.h:1091 (ScalarCpuPathTracer.cpp:458-477). */
+int pt_hip_load_scene(pt_tracer* tracer, const PtSceneView* scene);
+/* IPathTracer::build_or_update_acceleration, PathTracer.h:1095 (ScalarCpuPathTracer.cpp:479-520)
+ * and IRayAccelerator::build / build_info, PathTracer.h:1023, :1034. params may be NULL. */
+int pt_hip_build_accel(pt_tracer* tracer, const PtBvhParams* params, PtBuildInfo* info);
+/* IPathTracer::reset_accumulation, PathTracer.h:1099 (ScalarCpuPathTracer.cpp:669-688). */
+int pt_hip_reset_accumulation(pt_tracer* tracer);
+/* IPathTracer::render_tile, PathTracer.h:1141 (ScalarCpuPathTracer.Render.cpp:140-226). */
+int pt_hip_render_tile(pt_tracer* tracer, const PtRenderTile* tile, uint64_t frame_index);
+/* IPathTracer::render_tiles, PathTracer.h:1157-1168: all tiles in one wavefront launch sequence.
+ * Tiles are accumulated into the film in array order. */
+int pt_hip_render_tiles(pt_tracer* tracer, const PtRenderTile* tiles, uint32_t count, uint64_t frame_index);
+/* Film readback: FilmBuffer::raw / sample_counts (PathTracer.h:955-958). Host pointers:
+ * rgb_sum holds 3 floats per pixel, sample_count one u32 per pixel (either may be NULL). */
+int pt_hip_read_film(pt_tracer* tracer, float* rgb_sum, uint32_t* sample_count);
+/* FilmBuffer::resolve_hdr, FilmBuffer.cpp:160-214: 3 floats per pixel (mean of valid samples). */
+int pt_hip_resolve_hdr(pt_tracer* tracer, float* rgb);
+/* IPathTracer::read_counters, PathTracer.h:1180. */
+int pt_hip_read_counters(pt_tracer* tracer, PtSampleCounters* counters);
+/* IPathTracer::status, PathTracer.h:1181. */
+int pt_hip_status(pt_tracer* tracer, PtTracerStatus* status);
+/* IPathTracer::shutdown, PathTracer.h:1182 (ScalarCpuPathTracer.Render.cpp:463-481). */
+int pt_hip_shutdown(pt_tracer* tracer);
+/* Message of the last failing call on this handle ("" if none). */
+const char* pt_hip_last_error(const pt_tracer* tracer);
+
+/* IRayAccelerator::intersect, PathTracer.h:1032 (CpuBvhAccelerator.cpp:360-458), batched.
+ * Device pointers: rays = 6 floats per ray (origin, direction); tmax = per-ray t limit or NULL
+ * (infinity); out_t / out_prim = closest t and RayQueryHit::primitive_index (-1 on miss).
+ * mode 0 = closest hit, mode 1 = any hit with t < tmax (out_t is the first accepted t).
+ * counters (host, may be NULL) receive bvh_node_visits / bvh_leaf_visits / triangle_tests /
+ * triangle_hits for this call. */
+int pt_hip_trace_rays(pt_tracer* tracer, int mode, const float* rays, const float* tmax, uint32_t count,
+                      float* out_t, int32_t* out_prim, PtSampleCounters* counters);
+
+/* Block until all work on the handle's stream is done. */
+int pt_hip_synchronize(pt_tracer* tracer);
+/* Maximum paths in flight per wavefront pass (default 8M). */
+int pt_hip_set_wavefront_capacity(pt_tracer* tracer, uint32_t paths);
+/* HIP-event timing per kernel (off by default). */
+int pt_hip_set_profiling(pt_tracer* tracer, int enabled);
+int pt_hip_read_kernel_stats(pt_tracer* tracer, PtKernelStats* stats);
+int pt_hip_reset_kernel_stats(pt_tracer* tracer);
+
+/* Host-only BVH construction (no device needed): the flattening and split decisions of
+ * CpuBvhAccelerator::build (CpuBvhAccelerator.cpp:161-263) with the chosen builder. Call once with
+ * NULL outputs to get the counts, then again with buffers of that size:
+ *   bounds     6 floats per node (padded min xyz, max xyz), node 0 = root, pre-order numbering
+ *   meta       4 int32 per node: left, right, first, count (leaves: left = right = -1)
+ *   prim_order 1 u32 per primitive: leaf order -> flattened primitive index
+ *   prim_geo   12 floats per primitive: v0, e1, e2, face normal */
+int pt_bvh_build_host(const PtSceneView* scene, const PtBvhParams* params, uint32_t* node_count,
+                      uint32_t* prim_count, uint32_t* max_depth, float* bounds, int32_t* meta,
+                      uint32_t* prim_order, float* prim_geo);
+
+/* ABI sizes, for bindings to check their mirrors of the records above. */
+uint32_t pt_hip_sizeof(int which); /* 0 material, 1 light, 2 instance, 3 camera, 4 scene, 5 settings, 6 counters, 7 tile */
+const char* pt_hip_version(void);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* VKPT_HIP_H */

@@ -1,0 +1,239 @@
+"""ORACLE — test infrastructure only.
+
+ctypes front end of the CPU restatement in pto.c (see pto.h). Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg import this module, and
+only to check or time against the product; nothing in vkpathtracer_amd/ imports it.
+
+Variants (oracle/Makefile):
+  "f32"       portable transcendentals — bit-level parity anchor for the GPU path
+  "f32_libm"  glibc sinf/cosf/powf, as the reference is built
+  "f64"       the same estimator evaluated in double precision
+"""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+import subprocess
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB_DIR = HERE / "lib"
+VARIANTS = ("f32", "f32_libm", "f64")
+
+
+class PtoCamera(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("target", C.c_float * 3), ("up", C.c_float * 3),
+                ("fov_deg", C.c_float), ("aperture_radius", C.c_float), ("focus_distance", C.c_float),
+                ("iris_blade_count", C.c_uint32), ("iris_rotation_degrees", C.c_float),
+                ("iris_roundness", C.c_float), ("anamorphic_squeeze", C.c_float)]
+
+
+class PtoScene(C.Structure):
+    _fields_ = [("vertices", C.c_void_p), ("indices", C.c_void_p), ("local_vertices", C.c_void_p),
+                ("local_indices", C.c_void_p), ("instances", C.c_void_p), ("materials", C.c_void_p),
+                ("lights", C.c_void_p), ("environment_map", C.c_void_p),
+                ("vertex_count", C.c_uint32), ("index_count", C.c_uint32), ("local_vertex_count", C.c_uint32),
+                ("local_index_count", C.c_uint32), ("instance_count", C.c_uint32), ("material_count", C.c_uint32),
+                ("light_count", C.c_uint32), ("environment_map_width", C.c_uint32),
+                ("environment_map_height", C.c_uint32), ("environment_color", C.c_float * 3),
+                ("environment_map_scale", C.c_float * 3), ("camera", PtoCamera)]
+
+
+class PtoSettings(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
+                ("max_depth", C.c_uint32), ("enable_nee", C.c_uint32), ("enable_mis", C.c_uint32),
+                ("deterministic", C.c_uint32), ("rr_start_depth", C.c_uint32), ("rr_min_survival", C.c_float),
+                ("rr_max_survival", C.c_float), ("camera_aperture_radius", C.c_float),
+                ("camera_focus_distance", C.c_float)]
+
+
+COUNTER_FIELDS = ("samples", "rays", "triangle_tests", "sdf_tests", "sdf_steps", "triangle_hits", "sdf_hits",
+                  "sdf_misses", "bvh_node_visits", "bvh_leaf_visits", "shadow_tests")
+
+
+class PtoCounters(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in COUNTER_FIELDS]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n in COUNTER_FIELDS}
+
+
+SHADOW_CLOSEST, SHADOW_ANYHIT = 0, 1
+BUILDER = {"midpoint": 0, "sah": 1}
+
+_P = C.c_void_p
+_SIGS = {
+    "pto_real_bytes": (C.c_uint32, []),
+    "pto_uses_libm": (C.c_uint32, []),
+    "pto_sizeof": (C.c_uint32, [C.c_int]),
+    "pto_sinf": (C.c_float, [C.c_float]),
+    "pto_cosf": (C.c_float, [C.c_float]),
+    "pto_powf": (C.c_float, [C.c_float, C.c_float]),
+    "pto_splitmix64": (C.c_uint64, [C.c_uint64]),
+    "pto_rng_seed": (C.c_uint64, [C.c_uint64] * 7),
+    "pto_rng_draws": (None, [C.c_uint64, C.c_uint32, _P, _P]),
+    "pto_accel_build": (_P, [C.POINTER(PtoScene), C.c_int, C.c_uint32, C.c_uint32]),
+    "pto_accel_free": (None, [_P]),
+    "pto_accel_node_count": (C.c_uint32, [_P]),
+    "pto_accel_prim_count": (C.c_uint32, [_P]),
+    "pto_accel_max_depth": (C.c_uint32, [_P]),
+    "pto_accel_export": (C.c_int, [_P, _P, _P, _P, _P, _P]),
+    "pto_trace_rays": (C.c_int, [_P, _P, C.c_uint32, C.c_int, _P, _P, _P, C.POINTER(PtoCounters)]),
+    "pto_brute_force_rays": (C.c_int, [_P, _P, C.c_uint32, _P, _P]),
+    "pto_render_pixels": (C.c_int, [C.POINTER(PtoScene), _P, C.POINTER(PtoSettings), _P, C.c_uint32, C.c_uint32,
+                                    C.c_uint64, C.c_int, C.c_int, _P, _P, C.POINTER(PtoCounters)]),
+    "pto_render_tile": (C.c_int, [C.POINTER(PtoScene), _P, C.POINTER(PtoSettings), C.c_uint32, C.c_uint32, C.c_uint32,
+                                  C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.c_int, C.c_int, _P, _P,
+                                  C.POINTER(PtoCounters)]),
+}
+
+_libs: dict[str, C.CDLL] = {}
+
+
+def build(quiet: bool = True) -> None:
+    """Compile the three oracle variants with gcc (make -C oracle)."""
+    subprocess.run(["make", "-s", "-C", str(HERE), "-j3"], check=True,
+                   stdout=subprocess.DEVNULL if quiet else None)
+
+
+def lib(variant: str = "f32") -> C.CDLL:
+    if variant not in _libs:
+        path = LIB_DIR / f"liboracle_{variant}.so"
+        if not path.exists():
+            build()
+        L = C.CDLL(str(path))
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _libs[variant] = L
+    return _libs[variant]
+
+
+def _ptr(a):
+    return None if a is None or a.size == 0 else a.ctypes.data
+
+
+class OracleScene:
+    """PtoScene over a scene object exposing the snapshot arrays (vkpathtracer_amd.scene.Scene layout)."""
+
+    def __init__(self, sc):
+        self.sc = sc
+        s = PtoScene()
+        s.vertices, s.indices = _ptr(sc.vertices), _ptr(sc.indices)
+        s.local_vertices, s.local_indices = _ptr(sc.local_vertices), _ptr(sc.local_indices)
+        s.instances, s.materials, s.lights = _ptr(sc.instances), _ptr(sc.materials), _ptr(sc.lights)
+        s.environment_map = None
+        s.vertex_count, s.index_count = len(sc.vertices), len(sc.indices)
+        s.local_vertex_count, s.local_index_count = len(sc.local_vertices), len(sc.local_indices)
+        s.instance_count, s.material_count, s.light_count = len(sc.instances), len(sc.materials), len(sc.lights)
+        s.environment_color[:] = list(sc.environment_color)
+        c = sc.camera
+        cam = PtoCamera()
+        cam.position[:], cam.target[:], cam.up[:] = list(c.position), list(c.target), list(c.up)
+        cam.fov_deg, cam.aperture_radius, cam.focus_distance = c.fov_deg, c.aperture_radius, c.focus_distance
+        cam.iris_blade_count, cam.iris_rotation_degrees = c.iris_blade_count, c.iris_rotation_degrees
+        cam.iris_roundness, cam.anamorphic_squeeze = c.iris_roundness, c.anamorphic_squeeze
+        s.camera = cam
+        self.s = s
+
+
+def settings_struct(st) -> PtoSettings:
+    o = PtoSettings()
+    o.seed = st.seed & 0xFFFFFFFFFFFFFFFF
+    o.width, o.height, o.spp, o.max_depth = st.width, st.height, st.spp, st.max_depth
+    o.enable_nee, o.enable_mis, o.deterministic = int(st.enable_nee), int(st.enable_mis), int(st.deterministic)
+    o.rr_start_depth, o.rr_min_survival, o.rr_max_survival = st.rr_start_depth, st.rr_min_survival, st.rr_max_survival
+    o.camera_aperture_radius, o.camera_focus_distance = st.camera_aperture_radius, st.camera_focus_distance
+    return o
+
+
+class Oracle:
+    """One scene + acceleration structure in one oracle variant."""
+
+    def __init__(self, scene, builder: str = "sah", leaf_size: int = 4, bucket_count: int = 8, variant: str = "f32"):
+        self.L = lib(variant)
+        self.variant = variant
+        self.real = np.float64 if self.L.pto_real_bytes() == 8 else np.float32
+        self.scene = OracleScene(scene)
+        self.accel = self.L.pto_accel_build(C.byref(self.scene.s), BUILDER[builder], leaf_size, bucket_count)
+        if not self.accel:
+            raise RuntimeError("oracle accel build failed")
+
+    def __del__(self):
+        if getattr(self, "accel", None):
+            self.L.pto_accel_free(self.accel)
+            self.accel = None
+
+    @property
+    def prim_count(self):
+        return self.L.pto_accel_prim_count(self.accel)
+
+    @property
+    def node_count(self):
+        return self.L.pto_accel_node_count(self.accel)
+
+    @property
+    def max_depth(self):
+        return self.L.pto_accel_max_depth(self.accel)
+
+    def export(self):
+        n, p = self.node_count, self.prim_count
+        bounds = np.zeros((n, 6), self.real)
+        meta = np.zeros((n, 4), np.int32)
+        order = np.zeros(p, np.uint32)
+        geo = np.zeros((p, 12), self.real)
+        mat = np.zeros(p, np.uint32)
+        self.L.pto_accel_export(self.accel, bounds.ctypes.data, meta.ctypes.data, order.ctypes.data,
+                                geo.ctypes.data, mat.ctypes.data)
+        return dict(bounds=bounds, meta=meta, order=order, geo=geo, material=mat)
+
+    def trace_rays(self, rays: np.ndarray, any_hit: bool = False, tmax: np.ndarray | None = None):
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = len(rays)
+        t = np.zeros(n, np.float32)
+        p = np.zeros(n, np.int32)
+        c = PtoCounters()
+        tm = None if tmax is None else np.ascontiguousarray(tmax, np.float32)
+        self.L.pto_trace_rays(self.accel, rays.ctypes.data, n, int(any_hit), None if tm is None else tm.ctypes.data,
+                              t.ctypes.data, p.ctypes.data, C.byref(c))
+        return t, p, c.as_dict()
+
+    def brute_force(self, rays: np.ndarray):
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = len(rays)
+        t = np.zeros(n, np.float32)
+        p = np.zeros(n, np.int32)
+        self.L.pto_brute_force_rays(self.accel, rays.ctypes.data, n, t.ctypes.data, p.ctypes.data)
+        return t, p
+
+    def render_pixels(self, settings, pixels: np.ndarray, sample_index: int, frame_index: int = 0, threads: int = 0,
+                      shadow: int = SHADOW_ANYHIT):
+        px = np.ascontiguousarray(pixels, np.uint32)
+        n = len(px)
+        rgb = np.zeros((n, 3), self.real)
+        valid = np.zeros(n, np.uint8)
+        c = PtoCounters()
+        st = settings_struct(settings)
+        rc = self.L.pto_render_pixels(C.byref(self.scene.s), self.accel, C.byref(st), px.ctypes.data, n, sample_index,
+                                      frame_index, threads, shadow, rgb.ctypes.data, valid.ctypes.data, C.byref(c))
+        if rc != 0:
+            raise RuntimeError(f"oracle render_pixels failed ({rc})")
+        return rgb, valid.astype(bool), c.as_dict()
+
+    def render(self, settings, sample_begin: int = 0, sample_count: int | None = None, frame_index: int = 0,
+               threads: int = 0, shadow: int = SHADOW_ANYHIT, tile=None):
+        """Film (sum, count) after render_tile for each sample in order. tile = (x, y, w, h) or full frame."""
+        W, H = settings.width, settings.height
+        film = np.zeros((H, W, 3), self.real)
+        cnt = np.zeros((H, W), np.uint32)
+        c = PtoCounters()
+        st = settings_struct(settings)
+        x, y, w, h = tile if tile is not None else (0, 0, W, H)
+        count = settings.spp if sample_count is None else sample_count
+        rc = self.L.pto_render_tile(C.byref(self.scene.s), self.accel, C.byref(st), x, y, w, h, sample_begin, count,
+                                    frame_index, threads, shadow, film.ctypes.data, cnt.ctypes.data, C.byref(c))
+        if rc != 0:
+            raise RuntimeError(f"oracle render failed ({rc})")
+        return film, cnt, c.as_dict()

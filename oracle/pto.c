@@ -1,0 +1,1228 @@
+/*
+ * ORACLE — test infrastructure only (see pto.h). CPU restatement of the
+ * reference's scalar path-tracing hot path. Each function cites the reference
+ * file:line it follows. Compile with -ffp-contract=off: every float expression
+ * is evaluated in the operand order the reference's C++ evaluates it.
+ *
+ * PTO_REAL    float (default) or double (the fp64 oracle)
+ * PTO_LIBM    1: glibc sinf/cosf/powf like the reference; 0: pto_math.h
+ */
+#include "pto.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "pto_math.h"
+
+#ifndef PTO_REAL_IS_DOUBLE
+#define PTO_REAL_IS_DOUBLE 0
+#endif
+#ifndef PTO_LIBM
+#define PTO_LIBM 0
+#endif
+
+#if PTO_REAL_IS_DOUBLE
+typedef double real;
+#define RC(x) (x)
+#define R_SQRT(x) sqrt(x)
+#define R_SIN(x) sin(x)
+#define R_COS(x) cos(x)
+#define R_POW(x, y) pow((x), (y))
+#define R_FABS(x) fabs(x)
+#define R_MAXV DBL_MAX
+#else
+typedef float real;
+#define RC(x) (x##f)
+#define R_SQRT(x) sqrtf(x)
+#if PTO_LIBM
+#define R_SIN(x) sinf(x)
+#define R_COS(x) cosf(x)
+#define R_POW(x, y) powf((x), (y))
+#else
+#define R_SIN(x) pm_sinf(x)
+#define R_COS(x) pm_cosf(x)
+#define R_POW(x, y) pm_powf((x), (y))
+#endif
+#define R_FABS(x) fabsf(x)
+#define R_MAXV FLT_MAX
+#endif
+
+/* std::min / std::max semantics: min(a,b) = (b < a) ? b : a; max(a,b) = (a < b) ? b : a. */
+static inline real smin(real a, real b) { return (b < a) ? b : a; }
+static inline real smax(real a, real b) { return (a < b) ? b : a; }
+
+static const real kEps = RC(1e-4);                      /* ScalarCpuPathTracer.cpp:16 */
+static const real kPi = RC(3.14159265358979323846);    /* :14 */
+static const real kInvPi = RC(0.31830988618);          /* :15 */
+
+typedef struct { real x, y, z; } v3;
+
+static inline v3 mk(real x, real y, real z) { v3 r = {x, y, z}; return r; }
+static inline v3 add(v3 a, v3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline v3 sub(v3 a, v3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline v3 neg(v3 a) { return mk(-a.x, -a.y, -a.z); }
+static inline v3 muls(v3 a, real s) { return mk(a.x * s, a.y * s, a.z * s); }
+static inline v3 mulv(v3 a, v3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+static inline v3 divs(v3 a, real s) { return mk(a.x / s, a.y / s, a.z / s); }
+static inline real dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline v3 cross(v3 a, v3 b) {
+  return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static inline real length_sq(v3 a) { return dot(a, a); }
+static inline real length(v3 a) { return R_SQRT(length_sq(a)); }
+/* normalize(): ScalarCpuPathTracer.cpp:104-110 (divide by length, l <= eps -> +Y). */
+static inline v3 normalize(v3 a) {
+  const real l = length(a);
+  if (l <= kEps) return mk(RC(0.0), RC(1.0), RC(0.0));
+  return divs(a, l);
+}
+/* normalize_vec3(): CpuBvhAccelerator.cpp:76-82 (multiply by 1/sqrt). */
+static inline v3 normalize_accel(v3 a) {
+  const real len_sq = dot(a, a);
+  if (len_sq <= kEps * kEps) return mk(RC(0.0), RC(1.0), RC(0.0));
+  return muls(a, RC(1.0) / R_SQRT(len_sq));
+}
+static inline real vmax3(v3 a) { return smax(a.x, smax(a.y, a.z)); }
+static inline int is_finite_r(real v) { return isfinite(v); }
+/* clamp01: ScalarCpuPathTracer.cpp:161-166 */
+static inline real clamp01(real v) {
+  if (!is_finite_r(v)) return RC(0.0);
+  return smin(RC(1.0), smax(RC(0.0), v));
+}
+/* std::clamp(v, lo, hi) */
+static inline real sclamp(real v, real lo, real hi) { return (v < lo) ? lo : ((hi < v) ? hi : v); }
+
+static v3 v3f(const float* p) { return mk((real)p[0], (real)p[1], (real)p[2]); }
+
+/* ------------------------------------------------------------------ math */
+uint32_t pto_real_bytes(void) { return (uint32_t)sizeof(real); }
+uint32_t pto_uses_libm(void) { return PTO_LIBM || PTO_REAL_IS_DOUBLE; }
+float pto_sinf(float x) { return (float)R_SIN((real)x); }
+float pto_cosf(float x) { return (float)R_COS((real)x); }
+float pto_powf(float x, float y) { return (float)R_POW((real)x, (real)y); }
+
+uint32_t pto_sizeof(int which) {
+  switch (which) {
+    case 0: return sizeof(PtoMaterial);
+    case 1: return sizeof(PtoLight);
+    case 2: return sizeof(PtoInstance);
+    case 3: return sizeof(PtoCamera);
+    case 4: return sizeof(PtoScene);
+    case 5: return sizeof(PtoSettings);
+    case 6: return sizeof(PtoCounters);
+    default: return 0;
+  }
+}
+
+/* ------------------------------------------------------------------ RNG */
+/* splitmix64: ScalarCpuPathTracer.cpp:36-41 */
+uint64_t pto_splitmix64(uint64_t x) {
+  x += 0x9e3779b97f4a7c15ULL;
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+  return x ^ (x >> 31);
+}
+#define sm pto_splitmix64
+
+/* Rng::Rng(SampleKey): ScalarCpuPathTracer.cpp:383-388 */
+uint64_t pto_rng_seed(uint64_t seed, uint64_t frame, uint64_t sample, uint64_t path_id,
+                      uint64_t dimension, uint64_t depth, uint64_t pixel) {
+  const uint64_t mix = sm(seed ^ sm(frame) ^ sm(sample) ^ sm(path_id) ^ sm(dimension) ^ sm(depth));
+  return sm(mix + sm(pixel));
+}
+
+/* Rng::next01: ScalarCpuPathTracer.cpp:390-393 */
+static inline real next01(uint64_t* state) {
+  *state = sm(*state);
+  return (real)((*state >> 8) & 0x00ffffffu) * (RC(1.0) / RC(16777215.0));
+}
+
+void pto_rng_draws(uint64_t state, uint32_t n, float* out, uint64_t* out_state) {
+  for (uint32_t i = 0; i < n; ++i) out[i] = (float)next01(&state);
+  if (out_state) *out_state = state;
+}
+
+/* ------------------------------------------------------------------ scene flattening + BVH */
+typedef struct {
+  real mn[3], mx[3];
+} aabb_t;
+
+typedef struct {
+  aabb_t box;      /* padded by kEps, as CpuBvhAccelerator's make_triangle_aabb */
+  int32_t left, right, first, count;
+} node_t;
+
+typedef struct {
+  uint32_t material_index, primitive_index;
+  v3 v0, e1, e2, normal;
+} prim_t;
+
+struct pto_accel {
+  prim_t* prims;
+  uint32_t prim_count;
+  node_t* nodes;
+  uint32_t node_count;
+  uint32_t* order; /* leaf order -> primitive index */
+  uint32_t max_depth;
+};
+
+/* rotate_quat / transform_instance_vertex: CpuBvhAccelerator.cpp:48-74 */
+static v3 rotate_quat(v3 value, const float* q) {
+  const real qx = (real)q[0], qy = (real)q[1], qz = (real)q[2], qw0 = (real)q[3];
+  const real len_sq = qx * qx + qy * qy + qz * qz + qw0 * qw0;
+  if (len_sq <= kEps * kEps) return value;
+  const real inv_len = RC(1.0) / R_SQRT(len_sq);
+  const v3 qv = mk(qx * inv_len, qy * inv_len, qz * inv_len);
+  const real qw = qw0 * inv_len;
+  const v3 t = muls(cross(qv, value), RC(2.0));
+  return add(add(value, muls(t, qw)), cross(qv, t));
+}
+static v3 transform_instance_vertex(v3 local, const PtoInstance* inst) {
+  const v3 scaled = mk(local.x * (real)inst->scale[0], local.y * (real)inst->scale[1], local.z * (real)inst->scale[2]);
+  return add(rotate_quat(scaled, inst->rotation), v3f(inst->translation));
+}
+
+/* make_triangle_aabb: CpuBvhAccelerator.cpp:84-95. std::min({a,b,c}) picks the first smallest. */
+static inline real min3(real a, real b, real c) { real m = a; if (b < m) m = b; if (c < m) m = c; return m; }
+static inline real max3(real a, real b, real c) { real m = a; if (m < b) m = b; if (m < c) m = c; return m; }
+
+/* Flatten instances into primitives: CpuBvhAccelerator::build, CpuBvhAccelerator.cpp:161-241. */
+static int flatten(const PtoScene* sc, prim_t** out_prims, uint32_t* out_n, v3** raw /* 3 per prim, unpadded vertices */) {
+  uint64_t cap = 0;
+  for (uint32_t i = 0; i < sc->instance_count; ++i) cap += sc->instances[i].triangle_count;
+  prim_t* prims = (prim_t*)malloc(sizeof(prim_t) * (cap ? cap : 1));
+  v3* verts = (v3*)malloc(sizeof(v3) * 3 * (cap ? cap : 1));
+  if (!prims || !verts) { free(prims); free(verts); return -1; }
+  uint32_t n = 0;
+  for (uint32_t ii = 0; ii < sc->instance_count; ++ii) {
+    const PtoInstance* inst = &sc->instances[ii];
+    const int use_local = (inst->flags & 1u) != 0u && inst->local_vertex_count > 0u && inst->local_index_count > 0u;
+    for (uint32_t tri = 0; tri < inst->triangle_count; ++tri) {
+      v3 a = mk(0, 0, 0), b = a, c = a;
+      int loaded = 0;
+      if (use_local) {
+        const uint32_t local_base = inst->local_first_index + tri * 3u;
+        const uint32_t local_end = inst->local_first_vertex + inst->local_vertex_count;
+        if ((uint64_t)local_base + 2u < sc->local_index_count) {
+          const uint32_t li0 = sc->local_indices[local_base + 0u];
+          const uint32_t li1 = sc->local_indices[local_base + 1u];
+          const uint32_t li2 = sc->local_indices[local_base + 2u];
+          const uint32_t f = inst->local_first_vertex;
+          if (f + li0 < local_end && f + li1 < local_end && f + li2 < local_end &&
+              f + li0 < sc->local_vertex_count && f + li1 < sc->local_vertex_count && f + li2 < sc->local_vertex_count) {
+            a = transform_instance_vertex(v3f(sc->local_vertices + 3u * (f + li0)), inst);
+            b = transform_instance_vertex(v3f(sc->local_vertices + 3u * (f + li1)), inst);
+            c = transform_instance_vertex(v3f(sc->local_vertices + 3u * (f + li2)), inst);
+            loaded = 1;
+          }
+        }
+      }
+      if (!loaded) {
+        const uint64_t base_tri = ((uint64_t)inst->first_triangle + tri) * 3u;
+        if (base_tri + 2u >= sc->index_count) continue;
+        const uint32_t i0 = sc->indices[base_tri], i1 = sc->indices[base_tri + 1], i2 = sc->indices[base_tri + 2];
+        if (i0 >= sc->vertex_count || i1 >= sc->vertex_count || i2 >= sc->vertex_count) continue;
+        a = v3f(sc->vertices + 3u * i0);
+        b = v3f(sc->vertices + 3u * i1);
+        c = v3f(sc->vertices + 3u * i2);
+      }
+      prim_t* p = &prims[n];
+      p->material_index = inst->material_index;
+      p->primitive_index = n;
+      p->v0 = a;
+      p->e1 = sub(b, a);
+      p->e2 = sub(c, a);
+      p->normal = normalize_accel(cross(p->e1, p->e2));
+      verts[3 * n + 0] = a;
+      verts[3 * n + 1] = b;
+      verts[3 * n + 2] = c;
+      ++n;
+    }
+  }
+  *out_prims = prims;
+  *out_n = n;
+  *raw = verts;
+  return 0;
+}
+
+static aabb_t padded_box(const v3* t) {
+  aabb_t b;
+  b.mn[0] = min3(t[0].x, t[1].x, t[2].x) - kEps;
+  b.mn[1] = min3(t[0].y, t[1].y, t[2].y) - kEps;
+  b.mn[2] = min3(t[0].z, t[1].z, t[2].z) - kEps;
+  b.mx[0] = max3(t[0].x, t[1].x, t[2].x) + kEps;
+  b.mx[1] = max3(t[0].y, t[1].y, t[2].y) + kEps;
+  b.mx[2] = max3(t[0].z, t[1].z, t[2].z) + kEps;
+  return b;
+}
+
+/* ---- midpoint builder: ParallelBvhBuilder.cpp:14-163 (deterministic: stable_partition) */
+typedef struct {
+  const aabb_t* boxes;
+  node_t* nodes;
+  uint32_t* idx_base;
+  uint32_t* tmp;
+  int32_t alloc;
+  uint32_t leaf;
+  uint32_t max_depth;
+} mid_ctx;
+
+static void mid_build(mid_ctx* c, int32_t ni, uint32_t* idx, uint32_t count, uint32_t depth) {
+  if (depth > c->max_depth) c->max_depth = depth;
+  aabb_t b;
+  for (int k = 0; k < 3; ++k) { b.mn[k] = R_MAXV; b.mx[k] = -R_MAXV; }
+  for (uint32_t i = 0; i < count; ++i) {
+    const aabb_t* a = &c->boxes[idx[i]];
+    for (int k = 0; k < 3; ++k) { b.mn[k] = smin(b.mn[k], a->mn[k]); b.mx[k] = smax(b.mx[k], a->mx[k]); }
+  }
+  c->nodes[ni].box = b;
+  if (count <= c->leaf) {
+    c->nodes[ni].left = -1;
+    c->nodes[ni].right = -1;
+    c->nodes[ni].first = (int32_t)(idx - c->idx_base);
+    c->nodes[ni].count = (int32_t)count;
+    return;
+  }
+  aabb_t cb;
+  for (int k = 0; k < 3; ++k) { cb.mn[k] = R_MAXV; cb.mx[k] = -R_MAXV; }
+  for (uint32_t i = 0; i < count; ++i) {
+    const aabb_t* a = &c->boxes[idx[i]];
+    for (int k = 0; k < 3; ++k) {
+      const real ck = RC(0.5) * (a->mn[k] + a->mx[k]);
+      cb.mn[k] = smin(cb.mn[k], ck);
+      cb.mx[k] = smax(cb.mx[k], ck);
+    }
+  }
+  int axis = 0;
+  real best = cb.mx[0] - cb.mn[0];
+  for (int k = 1; k < 3; ++k) {
+    const real e = cb.mx[k] - cb.mn[k];
+    if (e > best) { best = e; axis = k; }
+  }
+  const real split = RC(0.5) * (cb.mn[axis] + cb.mx[axis]);
+  uint32_t nl = 0, nr = 0;
+  for (uint32_t i = 0; i < count; ++i) {
+    const aabb_t* a = &c->boxes[idx[i]];
+    const real ck = RC(0.5) * (a->mn[axis] + a->mx[axis]);
+    if (ck < split) idx[nl++] = idx[i]; else c->tmp[nr++] = idx[i];
+  }
+  memcpy(idx + nl, c->tmp, sizeof(uint32_t) * nr);
+  const uint32_t left_count = nl, right_count = count - nl;
+  const uint32_t safe_left = (left_count == 0 || right_count == 0) ? count / 2 : left_count;
+  const int32_t li = c->alloc++;
+  const int32_t ri = c->alloc++;
+  c->nodes[ni].left = li;
+  c->nodes[ni].right = ri;
+  c->nodes[ni].first = -1;
+  c->nodes[ni].count = 0;
+  mid_build(c, li, idx, safe_left, depth + 1);
+  mid_build(c, ri, idx + safe_left, count - safe_left, depth + 1);
+}
+
+/* ---- bucketed SAH builder: D3D12GpuPathTracer.SceneBvh.cpp:61-65 (bvh_sa), :73-238 (bvh_build).
+ * Split decisions use unpadded triangle bounds and centroids (D3D12GpuPathTracer.Scene.cpp:1083-1094).
+ * Node bounds are stored padded by kEps so traversal follows CpuBvhAccelerator's contract
+ * (union of padded triangle boxes == padded union, since rounding is monotonic). */
+typedef struct { uint32_t orig; real c[3], mn[3], mx[3]; } ref_t;
+
+typedef struct {
+  ref_t* refs;
+  ref_t* tmp;
+  node_t* nodes;
+  uint32_t node_count;
+  uint32_t* order;
+  uint32_t order_n;
+  uint32_t leaf, buckets;
+  uint32_t max_depth;
+} sah_ctx;
+
+static real bvh_sa(const real* mn, const real* mx) {
+  real dx = mx[0] - mn[0], dy = mx[1] - mn[1], dz = mx[2] - mn[2];
+  if (dx < 0) dx = 0;
+  if (dy < 0) dy = 0;
+  if (dz < 0) dz = 0;
+  return RC(2.0) * (dx * dy + dy * dz + dz * dx);
+}
+
+static uint32_t sah_build(sah_ctx* c, uint32_t start, uint32_t count, uint32_t depth) {
+  if (depth > c->max_depth) c->max_depth = depth;
+  const uint32_t ni = c->node_count++;
+  real bmn[3] = {RC(1e30), RC(1e30), RC(1e30)}, bmx[3] = {RC(-1e30), RC(-1e30), RC(-1e30)};
+  for (uint32_t i = start; i < start + count; ++i) {
+    for (int k = 0; k < 3; ++k) {
+      bmn[k] = smin(bmn[k], c->refs[i].mn[k]);
+      bmx[k] = smax(bmx[k], c->refs[i].mx[k]);
+    }
+  }
+  node_t* node = &c->nodes[ni];
+  for (int k = 0; k < 3; ++k) { node->box.mn[k] = bmn[k] - kEps; node->box.mx[k] = bmx[k] + kEps; }
+  if (count <= c->leaf) {
+    node->left = -1;
+    node->right = -1;
+    node->first = (int32_t)c->order_n;
+    node->count = (int32_t)count;
+    for (uint32_t i = start; i < start + count; ++i) c->order[c->order_n++] = c->refs[i].orig;
+    return ni;
+  }
+  const int B = (int)(c->buckets < 2u ? 2u : (c->buckets > 16u ? 16u : c->buckets));
+  const real node_sa = smax(RC(1e-9), bvh_sa(bmn, bmx));
+  int best_axis = 0;
+  real best_split = RC(0.0);
+  real best_cost = RC(1e30);
+  int found = 0;
+  for (int axis = 0; axis < 3; ++axis) {
+    real cmin = RC(1e30), cmax = RC(-1e30);
+    for (uint32_t i = start; i < start + count; ++i) {
+      cmin = smin(cmin, c->refs[i].c[axis]);
+      cmax = smax(cmax, c->refs[i].c[axis]);
+    }
+    const real extent = cmax - cmin;
+    if (extent < RC(1e-6)) continue;
+    real bmn_b[16][3], bmx_b[16][3];
+    uint32_t bn[16];
+    for (int b = 0; b < B; ++b) {
+      bn[b] = 0;
+      for (int k = 0; k < 3; ++k) { bmn_b[b][k] = RC(1e30); bmx_b[b][k] = RC(-1e30); }
+    }
+    for (uint32_t i = start; i < start + count; ++i) {
+      int b = (int)((c->refs[i].c[axis] - cmin) / extent * (real)B);
+      if (B - 1 < b) b = B - 1;
+      bn[b]++;
+      for (int k = 0; k < 3; ++k) {
+        bmn_b[b][k] = smin(bmn_b[b][k], c->refs[i].mn[k]);
+        bmx_b[b][k] = smax(bmx_b[b][k], c->refs[i].mx[k]);
+      }
+    }
+    for (int s = 1; s < B; ++s) {
+      real lmn[3] = {RC(1e30), RC(1e30), RC(1e30)}, lmx[3] = {RC(-1e30), RC(-1e30), RC(-1e30)};
+      real rmn[3] = {RC(1e30), RC(1e30), RC(1e30)}, rmx[3] = {RC(-1e30), RC(-1e30), RC(-1e30)};
+      uint32_t lc = 0, rc = 0;
+      for (int b = 0; b < s; ++b) {
+        if (!bn[b]) continue;
+        lc += bn[b];
+        for (int k = 0; k < 3; ++k) { lmn[k] = smin(lmn[k], bmn_b[b][k]); lmx[k] = smax(lmx[k], bmx_b[b][k]); }
+      }
+      for (int b = s; b < B; ++b) {
+        if (!bn[b]) continue;
+        rc += bn[b];
+        for (int k = 0; k < 3; ++k) { rmn[k] = smin(rmn[k], bmn_b[b][k]); rmx[k] = smax(rmx[k], bmx_b[b][k]); }
+      }
+      if (!lc || !rc) continue;
+      const real cost = (bvh_sa(lmn, lmx) * (real)lc + bvh_sa(rmn, rmx) * (real)rc) / node_sa;
+      if (cost < best_cost) {
+        best_cost = cost;
+        best_axis = axis;
+        best_split = cmin + (real)s / (real)B * extent;
+        found = 1;
+      }
+    }
+  }
+  uint32_t mid;
+  if (found) {
+    uint32_t nl = 0, nr = 0;
+    for (uint32_t i = start; i < start + count; ++i) {
+      if (c->refs[i].c[best_axis] < best_split) c->refs[start + nl++] = c->refs[i];
+      else c->tmp[nr++] = c->refs[i];
+    }
+    memcpy(c->refs + start + nl, c->tmp, sizeof(ref_t) * nr);
+    mid = start + nl;
+    if (mid <= start || mid >= start + count) mid = start + count / 2u;
+  } else {
+    mid = start + count / 2u;
+  }
+  const uint32_t l = sah_build(c, start, mid - start, depth + 1);
+  const uint32_t r = sah_build(c, mid, start + count - mid, depth + 1);
+  c->nodes[ni].left = (int32_t)l;
+  c->nodes[ni].right = (int32_t)r;
+  c->nodes[ni].first = -1;
+  c->nodes[ni].count = 0;
+  return ni;
+}
+
+pto_accel* pto_accel_build(const PtoScene* scene, int builder, uint32_t leaf_size, uint32_t bucket_count) {
+  pto_accel* a = (pto_accel*)calloc(1, sizeof(pto_accel));
+  if (!a) return NULL;
+  v3* raw = NULL;
+  if (flatten(scene, &a->prims, &a->prim_count, &raw) != 0) { free(a); return NULL; }
+  const uint32_t n = a->prim_count;
+  if (n == 0) { free(raw); return a; }
+  const uint32_t max_nodes = 2u * n - 1u;
+  a->nodes = (node_t*)calloc(max_nodes, sizeof(node_t));
+  a->order = (uint32_t*)malloc(sizeof(uint32_t) * n);
+  if (builder == PTO_BUILDER_MIDPOINT) {
+    aabb_t* boxes = (aabb_t*)malloc(sizeof(aabb_t) * n);
+    uint32_t* tmp = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    for (uint32_t i = 0; i < n; ++i) { boxes[i] = padded_box(raw + 3 * i); a->order[i] = i; }
+    mid_ctx c = {boxes, a->nodes, a->order, tmp, 1, leaf_size ? leaf_size : 4u, 0};
+    mid_build(&c, 0, a->order, n, 0);
+    a->node_count = (uint32_t)c.alloc;
+    a->max_depth = c.max_depth;
+    free(boxes);
+    free(tmp);
+  } else {
+    ref_t* refs = (ref_t*)malloc(sizeof(ref_t) * n);
+    ref_t* tmp = (ref_t*)malloc(sizeof(ref_t) * n);
+    for (uint32_t i = 0; i < n; ++i) {
+      const v3* t = raw + 3 * i;
+      refs[i].orig = i;
+      refs[i].mn[0] = min3(t[0].x, t[1].x, t[2].x);
+      refs[i].mn[1] = min3(t[0].y, t[1].y, t[2].y);
+      refs[i].mn[2] = min3(t[0].z, t[1].z, t[2].z);
+      refs[i].mx[0] = max3(t[0].x, t[1].x, t[2].x);
+      refs[i].mx[1] = max3(t[0].y, t[1].y, t[2].y);
+      refs[i].mx[2] = max3(t[0].z, t[1].z, t[2].z);
+      for (int k = 0; k < 3; ++k) refs[i].c[k] = (refs[i].mn[k] + refs[i].mx[k]) * RC(0.5);
+    }
+    sah_ctx c = {refs, tmp, a->nodes, 0, a->order, 0, leaf_size ? leaf_size : 4u, bucket_count ? bucket_count : 8u, 0};
+    sah_build(&c, 0, n, 0);
+    a->node_count = c.node_count;
+    a->max_depth = c.max_depth;
+    free(refs);
+    free(tmp);
+  }
+  free(raw);
+  return a;
+}
+
+void pto_accel_free(pto_accel* a) {
+  if (!a) return;
+  free(a->prims);
+  free(a->nodes);
+  free(a->order);
+  free(a);
+}
+uint32_t pto_accel_node_count(const pto_accel* a) { return a ? a->node_count : 0; }
+uint32_t pto_accel_prim_count(const pto_accel* a) { return a ? a->prim_count : 0; }
+uint32_t pto_accel_max_depth(const pto_accel* a) { return a ? a->max_depth : 0; }
+
+int pto_accel_export(const pto_accel* a, void* bounds, int32_t* meta, uint32_t* prim_order,
+                     void* prim_geo, uint32_t* prim_material) {
+  if (!a) return -1;
+  real* bd = (real*)bounds;
+  real* pg = (real*)prim_geo;
+  for (uint32_t i = 0; i < a->node_count; ++i) {
+    const node_t* nd = &a->nodes[i];
+    if (bd) for (int k = 0; k < 3; ++k) { bd[6 * i + k] = nd->box.mn[k]; bd[6 * i + 3 + k] = nd->box.mx[k]; }
+    if (meta) { meta[4 * i] = nd->left; meta[4 * i + 1] = nd->right; meta[4 * i + 2] = nd->first; meta[4 * i + 3] = nd->count; }
+  }
+  if (prim_order) memcpy(prim_order, a->order, sizeof(uint32_t) * a->prim_count);
+  for (uint32_t i = 0; i < a->prim_count; ++i) {
+    const prim_t* p = &a->prims[i];
+    if (pg) {
+      real* d = pg + 12 * i;
+      d[0] = p->v0.x; d[1] = p->v0.y; d[2] = p->v0.z;
+      d[3] = p->e1.x; d[4] = p->e1.y; d[5] = p->e1.z;
+      d[6] = p->e2.x; d[7] = p->e2.y; d[8] = p->e2.z;
+      d[9] = p->normal.x; d[10] = p->normal.y; d[11] = p->normal.z;
+    }
+    if (prim_material) prim_material[i] = p->material_index;
+  }
+  return 0;
+}
+
+/* ---- traversal: CpuBvhAccelerator::intersect, CpuBvhAccelerator.cpp:360-458 */
+typedef struct { v3 o; real inv[3]; int par[3]; } rayq_t;
+
+/* make_ray_aabb_query: CpuBvhAccelerator.cpp:113-123 */
+static rayq_t make_query(v3 o, v3 d) {
+  rayq_t q;
+  q.o = o;
+  const real dd[3] = {d.x, d.y, d.z};
+  for (int k = 0; k < 3; ++k) {
+    q.par[k] = R_FABS(dd[k]) <= kEps;
+    q.inv[k] = q.par[k] ? RC(0.0) : RC(1.0) / dd[k];
+  }
+  return q;
+}
+
+/* intersect_aabb: CpuBvhAccelerator.cpp:125-154 */
+static int slab(const aabb_t* b, const rayq_t* q, real max_t, real* t_near) {
+  real tmin = kEps, tmax = max_t;
+  const real o[3] = {q->o.x, q->o.y, q->o.z};
+  for (int k = 0; k < 3; ++k) {
+    if (q->par[k]) {
+      if (o[k] < b->mn[k] || o[k] > b->mx[k]) return 0;
+      continue;
+    }
+    real t0 = (b->mn[k] - o[k]) * q->inv[k];
+    real t1 = (b->mx[k] - o[k]) * q->inv[k];
+    if (t0 > t1) { const real s = t0; t0 = t1; t1 = s; }
+    tmin = smax(tmin, t0);
+    tmax = smin(tmax, t1);
+    if (tmin > tmax) return 0;
+  }
+  *t_near = tmin;
+  return tmax > kEps;
+}
+
+typedef struct { int hit; real t; uint32_t prim; } qhit_t;
+
+/* Leaf batch: IntersectBvhTriangleBatch / intersect_scalar / commit_ordered_lane_hit,
+ * BvhTriangleIntersector.cpp:25-74, :186-209. Returns accepted count; updates *best. */
+static uint32_t leaf_batch(const pto_accel* a, const uint32_t* prims, uint32_t cnt, v3 o, v3 d,
+                           real* best_t, int* lane_out) {
+  uint32_t accepted = 0;
+  *lane_out = -1;
+  if (cnt == 0 || !(*best_t > kEps)) return 0;
+  real out_t = *best_t;
+  for (uint32_t lane = 0; lane < cnt; ++lane) {
+    const prim_t* p = &a->prims[prims[lane]];
+    const real hx = d.y * p->e2.z - d.z * p->e2.y;
+    const real hy = d.z * p->e2.x - d.x * p->e2.z;
+    const real hz = d.x * p->e2.y - d.y * p->e2.x;
+    const real det = p->e1.x * hx + p->e1.y * hy + p->e1.z * hz;
+    if (R_FABS(det) < kEps) continue;
+    const real inv_det = RC(1.0) / det;
+    const real sx = o.x - p->v0.x, sy = o.y - p->v0.y, sz = o.z - p->v0.z;
+    const real u = (sx * hx + sy * hy + sz * hz) * inv_det;
+    if (u < RC(0.0) || u > RC(1.0)) continue;
+    const real qx = sy * p->e1.z - sz * p->e1.y;
+    const real qy = sz * p->e1.x - sx * p->e1.z;
+    const real qz = sx * p->e1.y - sy * p->e1.x;
+    const real v = (d.x * qx + d.y * qy + d.z * qz) * inv_det;
+    if (v < RC(0.0) || u + v > RC(1.0)) continue;
+    const real t = (p->e2.x * qx + p->e2.y * qy + p->e2.z * qz) * inv_det;
+    if (t <= kEps || t >= out_t) continue;
+    ++accepted;
+    *lane_out = (int)lane;
+    out_t = t;
+  }
+  *best_t = out_t;
+  return accepted;
+}
+
+/* Closest hit (any_hit == 0) or any hit with t < max_t (any_hit == 1). The any-hit form
+ * is the GPU shadow query's contract: it stops at the first accepted triangle. */
+static qhit_t traverse(const pto_accel* a, v3 o, v3 d, real max_t, int any_hit, PtoCounters* c) {
+  qhit_t h = {0, RC(0.0), 0};
+  if (a->node_count == 0 || a->prim_count == 0) return h;
+  real best_t = max_t;
+  const rayq_t q = make_query(o, d);
+  int32_t stack_small[128];
+  int32_t* stack = stack_small;
+  uint32_t cap = 128, sp = 0;
+  stack[sp++] = 0;
+  while (sp > 0) {
+    const int32_t ni = stack[--sp];
+    const node_t* node = &a->nodes[ni];
+    c->bvh_node_visits++;
+    real tn;
+    if (!slab(&node->box, &q, best_t, &tn)) continue;
+    if (node->count > 0) {
+      c->bvh_leaf_visits++;
+      for (int32_t i = 0; i < node->count;) {
+        uint32_t lanes[8];
+        uint32_t cnt = 0;
+        while (i < node->count && cnt < 8u) lanes[cnt++] = a->order[node->first + i++];
+        c->triangle_tests += cnt;
+        int lane;
+        const uint32_t acc = leaf_batch(a, lanes, cnt, o, d, &best_t, &lane);
+        if (lane < 0) continue;
+        c->triangle_hits += acc;
+        h.hit = 1;
+        h.t = best_t;
+        h.prim = lanes[lane];
+        if (any_hit) goto done;
+      }
+      continue;
+    }
+    real lt = 0, rt = 0;
+    const int hl = node->left >= 0 && slab(&a->nodes[node->left].box, &q, best_t, &lt);
+    const int hr = node->right >= 0 && slab(&a->nodes[node->right].box, &q, best_t, &rt);
+    if (sp + 2 > cap) {
+      const uint32_t ncap = cap * 2;
+      int32_t* ns = (int32_t*)malloc(sizeof(int32_t) * ncap);
+      memcpy(ns, stack, sizeof(int32_t) * sp);
+      if (stack != stack_small) free(stack);
+      stack = ns;
+      cap = ncap;
+    }
+    if (hl && hr) {
+      if (lt <= rt) { stack[sp++] = node->right; stack[sp++] = node->left; }
+      else { stack[sp++] = node->left; stack[sp++] = node->right; }
+    } else if (hl) {
+      stack[sp++] = node->left;
+    } else if (hr) {
+      stack[sp++] = node->right;
+    }
+  }
+done:
+  if (stack != stack_small) free(stack);
+  return h;
+}
+
+int pto_trace_rays(const pto_accel* a, const float* rays, uint32_t n, int any_hit, const float* tmax,
+                   float* out_t, int32_t* out_prim, PtoCounters* counters) {
+  PtoCounters c;
+  memset(&c, 0, sizeof(c));
+  for (uint32_t i = 0; i < n; ++i) {
+    const v3 o = v3f(rays + 6 * i), d = v3f(rays + 6 * i + 3);
+    const real mt = tmax ? (real)tmax[i] : (real)INFINITY;
+    const qhit_t h = traverse(a, o, d, mt, any_hit, &c);
+    out_t[i] = h.hit ? (float)h.t : (float)INFINITY;
+    out_prim[i] = h.hit ? (int32_t)a->prims[h.prim].primitive_index : -1;
+  }
+  if (counters) *counters = c;
+  return 0;
+}
+
+/* Brute force: every primitive, strict < keeps the first of equal t (ScalarCpuPathTracer.Intersect.cpp:52-89). */
+int pto_brute_force_rays(const pto_accel* a, const float* rays, uint32_t n, float* out_t, int32_t* out_prim) {
+  for (uint32_t i = 0; i < n; ++i) {
+    const v3 o = v3f(rays + 6 * i), d = v3f(rays + 6 * i + 3);
+    real best = (real)INFINITY;
+    int32_t bp = -1;
+    for (uint32_t p = 0; p < a->prim_count; ++p) {
+      real bt = best;
+      int lane;
+      leaf_batch(a, &p, 1, o, d, &bt, &lane);
+      if (lane == 0) { best = bt; bp = (int32_t)p; }
+    }
+    out_t[i] = bp >= 0 ? (float)best : (float)INFINITY;
+    out_prim[i] = bp;
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ integrator */
+typedef struct {
+  int hit;
+  real t;
+  v3 position, normal;
+  uint32_t material_index;
+} hit_t;
+
+typedef struct {
+  const PtoScene* sc;
+  const pto_accel* a;
+  const PtoSettings* s;
+  uint32_t max_depth;
+  int shadow_mode;
+} ctx_t;
+
+/* intersect_scene with the accelerator: ScalarCpuPathTracer.Intersect.cpp:28-50 */
+static hit_t intersect_scene(const ctx_t* x, v3 o, v3 d, PtoCounters* c) {
+  hit_t h;
+  memset(&h, 0, sizeof(h));
+  const qhit_t q = traverse(x->a, o, d, (real)INFINITY, 0, c);
+  if (q.hit) {
+    const prim_t* p = &x->a->prims[q.prim];
+    h.hit = 1;
+    h.t = q.t;
+    h.position = add(o, muls(d, q.t));
+    h.normal = p->normal;
+    h.material_index = p->material_index;
+  }
+  return h;
+}
+
+static const PtoMaterial* material_at(const PtoScene* sc, uint32_t index) {
+  static const PtoMaterial kDefault = {{1.0f, 1.0f, 1.0f}, {0.0f, 0.0f, 0.0f}, 1.0f, 0.0f, 1.5f, 0.0f, 0.0f, 0.0f,
+                                       0.0f, 1.0f, 0u, 0u, 0u, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u};
+  if (sc->material_count == 0) return &kDefault; /* load_scene_snapshot adds one default material (:469-471) */
+  const uint32_t last = sc->material_count - 1u;
+  return &sc->materials[index < last ? index : last];
+}
+
+/* apply_material_effect for material_effect == 0: identity then clamp to [0, 1.5]
+ * (ScalarCpuPathTracer.cpp:264-355). Other effects are rejected at entry. */
+static v3 material_albedo(const PtoMaterial* m) {
+  const v3 c = v3f(m->albedo);
+  return mk(smin(RC(1.5), smax(RC(0.0), c.x)), smin(RC(1.5), smax(RC(0.0), c.y)), smin(RC(1.5), smax(RC(0.0), c.z)));
+}
+
+static int is_emissive(const PtoMaterial* m) { return m->emissive[0] > 0.0f || m->emissive[1] > 0.0f || m->emissive[2] > 0.0f; }
+
+/* spot_attenuation: ScalarCpuPathTracer.cpp:168-185 */
+static real spot_attenuation(const PtoLight* l, v3 from_light_dir) {
+  const real inner = (real)l->spot_inner_cos, outer = (real)l->spot_outer_cos;
+  if (inner <= RC(-0.999)) return RC(1.0);
+  const real cone = dot(normalize(from_light_dir), normalize(v3f(l->direction)));
+  if (cone <= outer) return RC(0.0);
+  if (cone >= inner || inner <= outer) return RC(1.0);
+  const real t = sclamp((cone - outer) / (inner - outer), RC(0.0), RC(1.0));
+  return t * t * (RC(3.0) - RC(2.0) * t);
+}
+
+static inline real pow5(real v) { const real v2 = v * v; return v2 * v2 * v; }
+
+typedef struct { v3 radiance; int valid; } nee_t;
+
+static int shadow_visible(const ctx_t* x, v3 o, v3 d, real dist, PtoCounters* c) {
+  if (x->shadow_mode == PTO_SHADOW_ANYHIT) {
+    /* Opaque-blocker any-hit query with t_max = dist - 0.004; only valid when the scene has
+     * no transmissive material (the caller checks). */
+    const real remaining = dist - RC(0.004);
+    if (!(remaining > RC(0.004))) return 1;
+    const qhit_t q = traverse(x->a, o, d, remaining, 1, c);
+    return !q.hit;
+  }
+  /* ScalarCpuPathTracer.cpp:1066-1087 */
+  v3 vo = o;
+  real remaining = dist - RC(0.004);
+  for (uint32_t skips = 0; skips < 8u && remaining > RC(0.004); ++skips) {
+    const hit_t sh = intersect_scene(x, vo, d, c);
+    if (!sh.hit || sh.t >= remaining) break;
+    const PtoMaterial* m = material_at(x->sc, sh.material_index);
+    const int transmissive = m->material_model == 5u || m->transmission > 0.05f;
+    if (!transmissive) return 0;
+    const real advance = sh.t + RC(0.004);
+    vo = add(vo, muls(d, advance));
+    remaining -= advance;
+  }
+  return 1;
+}
+
+/* sample_direct_light: ScalarCpuPathTracer.cpp:1022-1118 (hit normal is the shading normal). */
+static nee_t sample_direct_light(const ctx_t* x, const hit_t* hit, v3 view_dir, uint64_t* rng, PtoCounters* c) {
+  nee_t r;
+  memset(&r, 0, sizeof(r));
+  const PtoScene* sc = x->sc;
+  if (sc->light_count == 0) return r;
+  const uint32_t nl = sc->light_count;
+  const uint64_t li = (uint64_t)(next01(rng) * (real)nl);
+  const PtoLight* light = &sc->lights[li < nl - 1u ? li : nl - 1u];
+  v3 lp = v3f(light->position);
+  if (light->radius > 1.0e-5f) {
+    const real uz = RC(1.0) - RC(2.0) * next01(rng);
+    const real upr = R_SQRT(smax(RC(0.0), RC(1.0) - uz * uz));
+    const real phi = RC(6.28318530718) * next01(rng);
+    lp = add(lp, muls(mk(upr * R_COS(phi), uz, upr * R_SIN(phi)), (real)light->radius));
+  }
+  const v3 to_light = sub(lp, hit->position);
+  const real dist_sq = length_sq(to_light);
+  const real dist = R_SQRT(dist_sq);
+  if (dist < kEps) return r;
+  const v3 light_dir = divs(to_light, dist);
+  const real cos_theta = dot(hit->normal, light_dir);
+  if (cos_theta <= RC(0.0)) return r;
+  const real spot = spot_attenuation(light, neg(light_dir));
+  if (spot <= RC(0.0)) return r;
+  const PtoMaterial* m = material_at(sc, hit->material_index);
+  const real roughness = clamp01((real)m->roughness);
+  const int isMirror = m->material_model == 2u || roughness <= RC(0.001);
+  const int isMetallic = m->material_model == 4u || m->metallic > 0.65f;
+  const int isTransmissive = m->material_model == 5u || m->transmission > 0.05f;
+  const int isClearcoat = m->material_model == 7u || m->clearcoat > 0.05f;
+  c->shadow_tests++;
+  const v3 so = add(hit->position, muls(hit->normal, RC(0.002)));
+  if (!shadow_visible(x, so, light_dir, dist, c)) return r;
+  const v3 albedo = material_albedo(m);
+  const v3 irradiance = muls(v3f(light->color), ((real)light->intensity * spot) / (dist_sq + kEps));
+  v3 direct = mk(0, 0, 0);
+  const real nlf = (real)nl;
+  if (!isMirror && !isTransmissive) {
+    direct = add(direct, muls(muls(mulv(muls(albedo, kInvPi), irradiance), cos_theta), nlf));
+  }
+  if (isMirror || isMetallic || isClearcoat || isTransmissive || roughness < RC(0.65)) {
+    const v3 half_dir = normalize(add(light_dir, normalize(view_dir)));
+    const real effR = smax(RC(0.025), roughness * (isMetallic ? RC(0.65) : RC(1.0)));
+    const real a2 = effR * effR;
+    const real specPower = sclamp(RC(2.0) / smax(RC(0.0005), a2 * a2) - RC(2.0), RC(4.0), RC(96.0));
+    const real spec = R_POW(smax(RC(0.0), dot(hit->normal, half_dir)), specPower);
+    const real cosView = smax(RC(0.0), dot(hit->normal, normalize(view_dir)));
+    const real ior = (real)m->ior;
+    real f0 = (RC(1.0) - ior) / (RC(1.0) + ior);
+    f0 *= f0;
+    const real fresnel = f0 + (RC(1.0) - f0) * pow5(RC(1.0) - cosView);
+    const real specStrength = clamp01((RC(1.0) - roughness) * RC(0.8) + (real)m->metallic * RC(0.45) +
+                                      (real)m->clearcoat * RC(0.35) + (isMirror ? RC(0.75) : RC(0.0)) +
+                                      (isTransmissive ? fresnel : RC(0.0)));
+    const v3 white = mk(RC(1.0), RC(1.0), RC(1.0));
+    const v3 specTint = add(muls(white, isMetallic ? RC(0.15) : RC(0.88)), muls(albedo, isMetallic ? RC(0.85) : RC(0.12)));
+    direct = add(direct, muls(mulv(specTint, irradiance), spec * cos_theta * nlf * smax(RC(0.15), specStrength)));
+  }
+  r.radiance = direct;
+  r.valid = 1;
+  return r;
+}
+
+/* light_pdf_for_direction: ScalarCpuPathTracer.cpp:1120-1138 */
+static real light_pdf_for_direction(const PtoScene* sc, v3 position, v3 direction) {
+  if (sc->light_count == 0) return RC(0.0);
+  const real inv_n = RC(1.0) / (real)sc->light_count;
+  real total = RC(0.0);
+  for (uint32_t i = 0; i < sc->light_count; ++i) {
+    const v3 to_light = sub(v3f(sc->lights[i].position), position);
+    const real dist = length(to_light);
+    if (dist < kEps) continue;
+    const v3 dir = divs(to_light, dist);
+    if (dot(dir, direction) > RC(0.9999)) total += inv_n / (dist * dist + kEps);
+  }
+  return total;
+}
+
+/* MisWeight: ScalarCpuPathTracer.cpp:1140-1146 */
+static real mis_weight(real a, real b) {
+  const real a2 = a * a, b2 = b * b;
+  const real denom = a2 + b2;
+  if (denom <= RC(0.0)) return RC(1.0);
+  return a2 / denom;
+}
+
+/* sample_hemisphere: ScalarCpuPathTracer.cpp:978-988 */
+static v3 sample_hemisphere(uint64_t* rng, v3 n) {
+  const real u1 = next01(rng);
+  const real u2 = next01(rng);
+  const real r = R_SQRT(smax(RC(0.0), RC(1.0) - u1));
+  const real phi = RC(2.0) * kPi * u2;
+  const v3 local = mk(r * R_COS(phi), r * R_SIN(phi), R_SQRT(smax(RC(0.0), u1)));
+  v3 tangent = cross((R_FABS(n.z) < RC(0.999)) ? mk(0, 0, 1) : mk(0, 1, 0), n);
+  tangent = normalize(tangent);
+  const v3 bitangent = cross(n, tangent);
+  return normalize(add(add(muls(tangent, local.x), muls(bitangent, local.y)), muls(n, local.z)));
+}
+
+/* sample_phong_lobe: ScalarCpuPathTracer.cpp:996-1020 */
+static v3 sample_phong_lobe(uint64_t* rng, v3 refl, real exponent, v3 n) {
+  const real u1 = next01(rng);
+  const real u2 = next01(rng);
+  const real cosT = R_POW(smax(RC(0.0), u1), RC(1.0) / (exponent + RC(1.0)));
+  const real sinT = R_SQRT(smax(RC(0.0), RC(1.0) - cosT * cosT));
+  const real phi = RC(2.0) * kPi * u2;
+  const v3 ref_t = (R_FABS(refl.z) < RC(0.999)) ? mk(0, 0, 1) : mk(0, 1, 0);
+  const v3 tang = normalize(cross(ref_t, refl));
+  const v3 btan = cross(refl, tang);
+  v3 out = normalize(add(add(muls(tang, sinT * R_COS(phi)), muls(btan, sinT * R_SIN(phi))), muls(refl, cosT)));
+  if (dot(out, n) <= RC(0.0)) {
+    out = sub(out, muls(n, RC(2.0) * dot(out, n)));
+    if (dot(out, n) <= RC(0.0)) return sample_hemisphere(rng, n);
+  }
+  return out;
+}
+
+/* schlick_fresnel / refract_dir: ScalarCpuPathTracer.cpp:357-377 */
+static real schlick_fresnel(real cosTheta, real ior) {
+  const real safeIor = smax(RC(1.01), ior);
+  real r0 = (RC(1.0) - safeIor) / (RC(1.0) + safeIor);
+  r0 *= r0;
+  const real m = RC(1.0) - smin(RC(1.0), smax(RC(0.0), cosTheta));
+  return r0 + (RC(1.0) - r0) * m * m * m * m * m;
+}
+static v3 refract_dir(v3 incoming, v3 normal, real eta, int* tir) {
+  const real cosi = smin(RC(1.0), smax(RC(0.0), dot(neg(incoming), normal)));
+  const real k = RC(1.0) - eta * eta * (RC(1.0) - cosi * cosi);
+  if (k < RC(0.0)) { *tir = 1; return mk(0, 0, 0); }
+  *tir = 0;
+  return normalize(add(muls(incoming, eta), muls(normal, eta * cosi - R_SQRT(k))));
+}
+
+static v3 reflect_about(v3 d, v3 n) { return sub(d, muls(n, RC(2.0) * dot(n, d))); }
+
+/* environment: SampleSceneEnvironment, PathTracer.cpp:213-272 (constant colour; maps are rejected at entry) */
+static v3 environment(const PtoScene* sc) { return v3f(sc->environment_color); }
+
+/* trace: ScalarCpuPathTracer.cpp:1148-1289 */
+static v3 trace(const ctx_t* x, v3 ro, v3 rd, uint64_t* rng, uint64_t* rays, PtoCounters* c) {
+  const PtoScene* sc = x->sc;
+  const PtoSettings* st = x->s;
+  v3 radiance = mk(0, 0, 0);
+  v3 throughput = mk(RC(1.0), RC(1.0), RC(1.0));
+  const real rr_min = (real)st->rr_min_survival, rr_max = (real)st->rr_max_survival;
+  for (uint32_t depth = 0; depth < x->max_depth; ++depth) {
+    ++*rays;
+    hit_t hit = intersect_scene(x, ro, rd, c);
+    if (!hit.hit) {
+      radiance = add(radiance, mulv(throughput, environment(sc)));
+      break;
+    }
+    const v3 gN = hit.normal;
+    const int entering = dot(rd, gN) < RC(0.0);
+    const v3 sN = entering ? gN : neg(gN);
+    const PtoMaterial* m = material_at(sc, hit.material_index);
+    const v3 albedo = material_albedo(m);
+    const real roughness = clamp01((real)m->roughness);
+    const int isEmissiveModel = m->material_model == 1u || is_emissive(m);
+    const int isMirror = m->material_model == 2u || roughness <= RC(0.001);
+    const int isMetallic = m->material_model == 4u || m->metallic > 0.65f;
+    const int isTransmissive = m->material_model == 5u || m->transmission > 0.05f;
+    const int isDiffuse = roughness >= RC(0.999) && !isMirror && !isMetallic && !isTransmissive;
+    const int isSheen = m->material_model == 6u;
+    const int isClearcoat = m->material_model == 7u || m->clearcoat > 0.05f;
+    const int isToon = m->material_model == 8u;
+    const v3 emissive = v3f(m->emissive);
+
+    if (!st->enable_nee || depth == 0) {
+      radiance = add(radiance, mulv(throughput, emissive));
+    } else if (st->enable_mis && isEmissiveModel) {
+      const real bsdf_pdf = smax(RC(0.0), dot(sN, neg(rd))) * kInvPi;
+      const real l_pdf = light_pdf_for_direction(sc, ro, rd);
+      const real w = mis_weight(bsdf_pdf, l_pdf);
+      radiance = add(radiance, muls(mulv(throughput, emissive), w));
+    }
+
+    if (st->enable_nee && depth < x->max_depth - 1u) {
+      hit_t lh = hit;
+      lh.normal = sN;
+      const nee_t nee = sample_direct_light(x, &lh, neg(rd), rng, c);
+      if (nee.valid) {
+        if (st->enable_mis) {
+          const v3 to_approx = sc->light_count == 0 ? mk(0, 0, 0) : sub(v3f(sc->lights[0].position), hit.position);
+          const v3 l_dir = normalize(to_approx);
+          const real cos_theta = smax(RC(0.0), dot(sN, l_dir));
+          const real bsdf_pdf = cos_theta * kInvPi;
+          const real l_pdf = sc->light_count == 0 ? RC(1.0) : (RC(1.0) / (real)sc->light_count);
+          const real w = mis_weight(l_pdf, bsdf_pdf);
+          radiance = add(radiance, muls(mulv(throughput, nee.radiance), w));
+        } else {
+          radiance = add(radiance, mulv(throughput, nee.radiance));
+        }
+      }
+    }
+
+    v3 outDir;
+    int refracted = 0;
+    if (isMirror) {
+      outDir = reflect_about(rd, sN);
+    } else if (isTransmissive) {
+      const real cosTheta = smax(RC(0.0), dot(sN, neg(rd)));
+      const real fresnel = schlick_fresnel(cosTheta, (real)m->ior);
+      if (next01(rng) < smin(RC(0.98), fresnel + (real)m->clearcoat * RC(0.015))) {
+        outDir = reflect_about(rd, sN);
+      } else {
+        int tir = 0;
+        const real safeIor = smax(RC(1.01), (real)m->ior);
+        const real eta = entering ? (RC(1.0) / safeIor) : safeIor;
+        outDir = refract_dir(rd, sN, eta, &tir);
+        if (tir || length_sq(outDir) <= RC(1.0e-8)) outDir = reflect_about(rd, sN);
+        else refracted = 1;
+      }
+    } else if (isDiffuse || isToon) {
+      outDir = sample_hemisphere(rng, sN);
+    } else {
+      const real effR = smax(RC(0.025), roughness * (isMetallic ? RC(0.75) : RC(1.0)) * (isClearcoat ? RC(0.65) : RC(1.0)));
+      const real a2 = effR * effR;
+      const real expt = smax(RC(0.0), RC(2.0) / (a2 * a2) - RC(2.0));
+      const v3 refl = reflect_about(rd, sN);
+      outDir = sample_phong_lobe(rng, refl, expt, sN);
+      if (isSheen && next01(rng) < (real)m->sheen * RC(0.35)) outDir = sample_hemisphere(rng, sN);
+    }
+
+    if (!isTransmissive && dot(outDir, sN) <= RC(0.0)) break;
+    v3 bw = albedo;
+    if (isMetallic || isMirror) {
+      bw = muls(albedo, RC(0.65) + RC(0.35) * (real)m->metallic);
+    } else if (isTransmissive) {
+      const real tint = clamp01((real)m->alpha) * (real)m->transmission;
+      const v3 white = mk(RC(1.0), RC(1.0), RC(1.0));
+      bw = add(muls(white, RC(1.0) - tint), muls(albedo, tint));
+      if (refracted) bw = muls(bw, smax(RC(0.55), (real)m->transmission));
+    }
+    if (isToon) bw = muls(bw, dot(outDir, sN) > RC(0.55) ? RC(1.0) : RC(0.45));
+    throughput = mulv(throughput, bw);
+    if (vmax3(throughput) < RC(0.001)) break;
+    const real rr = smin(rr_max, smax(rr_min, vmax3(throughput)));
+    if (depth >= st->rr_start_depth && next01(rng) > rr) break;
+    throughput = divs(throughput, rr);
+    ro = add(hit.position, muls(outDir, RC(0.002)));
+    rd = outDir;
+  }
+  return radiance;
+}
+
+typedef struct {
+  v3 cam_pos, fwd, right, up;
+  real aspect, tan_half_fov, aperture, focus;
+  uint32_t iris_blades;
+  real iris_roundness, iris_rotation, anamorphic;
+} camera_t;
+
+/* compute_camera_basis: ScalarCpuPathTracer.cpp:139-152; per-frame constants of camera_rays
+ * (ScalarCpuPathTracer.Render.cpp:74-83). tanf runs on the host in both oracle and product. */
+static camera_t make_camera(const PtoScene* sc, const PtoSettings* s) {
+  camera_t cam;
+  const v3 pos = v3f(sc->camera.position), tgt = v3f(sc->camera.target), upv = v3f(sc->camera.up);
+  cam.cam_pos = pos;
+  cam.fwd = normalize(sub(tgt, pos));
+  cam.right = normalize(cross(cam.fwd, upv));
+  if (length_sq(cam.right) <= kEps * kEps) cam.right = mk(1, 0, 0);
+  cam.up = normalize(cross(cam.right, cam.fwd));
+  if (length_sq(cam.up) <= kEps * kEps) cam.up = mk(0, 1, 0);
+  const real hf = (real)(s->height > 0 ? s->height : 0);
+  cam.aspect = (real)s->width / smax(RC(1.0), hf);
+  const real radians = (real)sc->camera.fov_deg * (kPi / RC(180.0));
+#if PTO_REAL_IS_DOUBLE
+  cam.tan_half_fov = tan(RC(0.5) * radians);
+#else
+  cam.tan_half_fov = tanf(RC(0.5) * radians);
+#endif
+  cam.aperture = sc->camera.aperture_radius > 0.0f ? (real)sc->camera.aperture_radius : (real)s->camera_aperture_radius;
+  cam.focus = sc->camera.focus_distance > 0.0f ? (real)sc->camera.focus_distance : (real)s->camera_focus_distance;
+  cam.iris_blades = sc->camera.iris_blade_count < 64u ? sc->camera.iris_blade_count : 64u;
+  cam.iris_roundness = clamp01((real)sc->camera.iris_roundness);
+  cam.iris_rotation = (real)sc->camera.iris_rotation_degrees * (kPi / RC(180.0));
+  cam.anamorphic = isfinite(sc->camera.anamorphic_squeeze) ? smax(RC(0.01), (real)sc->camera.anamorphic_squeeze) : RC(1.0);
+  return cam;
+}
+
+#if PTO_REAL_IS_DOUBLE
+#define R_FLOOR(x) floor(x)
+#else
+#define R_FLOOR(x) floorf(x)
+#endif
+
+/* camera_rays: ScalarCpuPathTracer.Render.cpp:55-119 */
+static void camera_ray(const camera_t* cam, const PtoSettings* s, uint32_t x, uint32_t y, uint32_t sample,
+                       uint64_t frame, uint32_t path_id, uint64_t* sample_seed, v3* ro, v3* rd) {
+  uint64_t rng = pto_rng_seed(s->seed, frame, sample, path_id, 0, 0, (uint64_t)y * s->width + x);
+  const real fx = ((real)x + next01(&rng)) / (real)(s->width > 1u ? s->width : 1u);
+  const real fy = ((real)y + next01(&rng)) / (real)(s->height > 1u ? s->height : 1u);
+  *sample_seed ^= (uint64_t)(next01(&rng) * (real)0xffffu);
+  const real nx = (RC(2.0) * fx - RC(1.0)) * cam->aspect * cam->tan_half_fov;
+  const real ny = (RC(1.0) - RC(2.0) * fy) * cam->tan_half_fov;
+  const v3 dir = normalize(add(add(cam->fwd, muls(cam->right, nx)), muls(cam->up, ny)));
+  if (cam->aperture > RC(0.0) && cam->focus > kEps) {
+    const real lens_radius_sample = R_SQRT(next01(&rng));
+    const real lens_phi = RC(2.0) * kPi * next01(&rng);
+    real boundary = RC(1.0);
+    if (cam->iris_blades >= 3u && cam->iris_roundness < RC(0.999)) {
+      const real sector = RC(2.0) * kPi / (real)cam->iris_blades;
+      const real local_phi = lens_phi - cam->iris_rotation;
+      const real wrapped = local_phi - sector * R_FLOOR(local_phi / sector);
+      const real centered = wrapped > sector * RC(0.5) ? wrapped - sector : wrapped;
+      const real polygon = R_COS(sector * RC(0.5)) / smax(RC(0.1), R_COS(centered));
+      boundary = polygon * (RC(1.0) - cam->iris_roundness) + cam->iris_roundness;
+    }
+    const real lens_r = cam->aperture * lens_radius_sample * boundary;
+    const v3 offset = add(muls(cam->right, lens_r * R_COS(lens_phi) * cam->anamorphic), muls(cam->up, lens_r * R_SIN(lens_phi)));
+    const v3 focus_point = add(cam->cam_pos, muls(dir, cam->focus));
+    const v3 origin = add(cam->cam_pos, offset);
+    *ro = origin;
+    *rd = normalize(sub(focus_point, origin));
+    return;
+  }
+  *ro = cam->cam_pos;
+  *rd = dir;
+}
+
+/* shade_pixel: ScalarCpuPathTracer.Render.cpp:228-284 */
+static int shade_pixel(const ctx_t* x, const camera_t* cam, uint32_t pixel, uint32_t sample, uint64_t frame,
+                       v3* out, uint64_t* rays, PtoCounters* c) {
+  const PtoSettings* s = x->s;
+  const uint32_t px = pixel % s->width, py = pixel / s->width;
+  uint64_t seed = 0;
+  const uint64_t sample_seed = ((uint64_t)py << 32) | px;
+  const uint64_t path_id = sample_seed + (uint64_t)sample + frame;
+  v3 ro, rd;
+  camera_ray(cam, s, px, py, sample, frame, (uint32_t)path_id, &seed, &ro, &rd);
+  uint64_t rng = pto_rng_seed(s->seed + sample_seed + seed, frame, sample, path_id, 0, 0, sample_seed);
+  const v3 v = trace(x, ro, rd, &rng, rays, c);
+  *out = v;
+  return is_finite_r(v.x) && is_finite_r(v.y) && is_finite_r(v.z);
+}
+
+static int check_supported(const PtoScene* sc) {
+  for (uint32_t i = 0; i < sc->material_count; ++i)
+    if (sc->materials[i].material_effect != 0u) return -2;
+  if (sc->environment_map_width && sc->environment_map_height && sc->environment_map) return -3;
+  return 0;
+}
+
+static int any_transmissive(const PtoScene* sc) {
+  for (uint32_t i = 0; i < sc->material_count; ++i)
+    if (sc->materials[i].material_model == 5u || sc->materials[i].transmission > 0.05f) return 1;
+  return 0;
+}
+
+static void add_counters(PtoCounters* d, const PtoCounters* s) {
+  d->samples += s->samples; d->rays += s->rays; d->triangle_tests += s->triangle_tests;
+  d->triangle_hits += s->triangle_hits; d->bvh_node_visits += s->bvh_node_visits;
+  d->bvh_leaf_visits += s->bvh_leaf_visits; d->shadow_tests += s->shadow_tests;
+}
+
+int pto_render_pixels(const PtoScene* scene, const pto_accel* a, const PtoSettings* s,
+                      const uint32_t* pixels, uint32_t n, uint32_t sample_index, uint64_t frame_index,
+                      int threads, int shadow_mode, void* out_rgb, uint8_t* out_valid, PtoCounters* counters) {
+  if (!scene || !a || !s || s->width == 0 || s->height == 0) return -1;
+  const int sup = check_supported(scene);
+  if (sup) return sup;
+  if (shadow_mode == PTO_SHADOW_ANYHIT && any_transmissive(scene)) shadow_mode = PTO_SHADOW_CLOSEST;
+  ctx_t x = {scene, a, s, s->max_depth > 1u ? s->max_depth : 1u, shadow_mode};
+  const camera_t cam = make_camera(scene, s);
+  real* rgb = (real*)out_rgb;
+  PtoCounters total;
+  memset(&total, 0, sizeof(total));
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#else
+  (void)threads;
+#endif
+#pragma omp parallel
+  {
+    PtoCounters lc;
+    memset(&lc, 0, sizeof(lc));
+#pragma omp for schedule(dynamic, 64)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+      v3 v;
+      uint64_t rays = 0;
+      const uint64_t total_px = (uint64_t)s->width * s->height;
+      if ((uint64_t)pixels[i] >= total_px) { out_valid[i] = 0; continue; }
+      const int ok = shade_pixel(&x, &cam, pixels[i], sample_index, frame_index, &v, &rays, &lc);
+      rgb[3 * i] = v.x; rgb[3 * i + 1] = v.y; rgb[3 * i + 2] = v.z;
+      out_valid[i] = (uint8_t)ok;
+      lc.rays += rays;
+      if (ok) lc.samples++;
+    }
+#pragma omp critical
+    add_counters(&total, &lc);
+  }
+  if (counters) *counters = total;
+  return 0;
+}
+
+int pto_render_tile(const PtoScene* scene, const pto_accel* a, const PtoSettings* s,
+                    uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                    uint32_t sample_begin, uint32_t sample_count, uint64_t frame_index,
+                    int threads, int shadow_mode, void* film_sum, uint32_t* film_count, PtoCounters* counters) {
+  if (!scene || !a || !s || s->width == 0 || s->height == 0) return -1;
+  const int sup = check_supported(scene);
+  if (sup) return sup;
+  if (shadow_mode == PTO_SHADOW_ANYHIT && any_transmissive(scene)) shadow_mode = PTO_SHADOW_CLOSEST;
+  ctx_t x = {scene, a, s, s->max_depth > 1u ? s->max_depth : 1u, shadow_mode};
+  const camera_t cam = make_camera(scene, s);
+  real* film = (real*)film_sum;
+  const uint32_t xa = x0 < s->width ? x0 : s->width, ya = y0 < s->height ? y0 : s->height;
+  const uint32_t xb = (uint64_t)xa + w < s->width ? xa + w : s->width;
+  const uint32_t yb = (uint64_t)ya + h < s->height ? ya + h : s->height;
+  const int64_t tw = (int64_t)(xb - xa), npx = tw * (int64_t)(yb - ya);
+  PtoCounters total;
+  memset(&total, 0, sizeof(total));
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#else
+  (void)threads;
+#endif
+  for (uint32_t si = 0; si < sample_count; ++si) {
+    const uint32_t sample = sample_begin + si;
+#pragma omp parallel
+    {
+      PtoCounters lc;
+      memset(&lc, 0, sizeof(lc));
+#pragma omp for schedule(dynamic, 64)
+      for (int64_t i = 0; i < npx; ++i) {
+        const uint32_t px = xa + (uint32_t)(i % tw), py = ya + (uint32_t)(i / tw);
+        const uint32_t pixel = py * s->width + px;
+        v3 v;
+        uint64_t rays = 0;
+        const int ok = shade_pixel(&x, &cam, pixel, sample, frame_index, &v, &rays, &lc);
+        lc.rays += rays;
+        if (!ok) continue; /* shade_pixel drops non-finite samples before FilmBuffer::add_sample */
+        lc.samples++;
+        real* f = film + 3u * (uint64_t)pixel;
+        f[0] = f[0] + v.x;
+        f[1] = f[1] + v.y;
+        f[2] = f[2] + v.z;
+        film_count[pixel] += 1u;
+      }
+#pragma omp critical
+      add_counters(&total, &lc);
+    }
+  }
+  if (counters) *counters = total;
+  return 0;
+}

@@ -1,0 +1,118 @@
+"""The product's host BVH builders (through pt_bvh_build_host) against the oracle's restatement,
+and the BVH contract itself (partition, containment, leaf size, brute-force equivalence)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from vkpathtracer_amd import bvh, scene as S
+
+
+def scenes():
+    yield S.cornell_box()
+    yield S.procedural_scene(seed=3, ico_subdivisions=3, bisect=True, sphere_count=40, light_count=3)
+
+
+@pytest.mark.parametrize("builder,leaf,buckets", [("sah", 4, 8), ("sah", 8, 16), ("sah", 1, 2), ("midpoint", 4, 0),
+                                                  ("midpoint", 8, 0)])
+def test_product_builder_equals_oracle(builder, leaf, buckets):
+    for sc in scenes():
+        a = bvh.build_host(sc, builder, leaf, buckets)
+        o = O.Oracle(sc, builder=builder, leaf_size=leaf, bucket_count=buckets or 8).export()
+        assert np.array_equal(a["meta"], o["meta"])
+        assert np.array_equal(a["order"], o["order"])
+        assert np.array_equal(a["bounds"], o["bounds"])
+        assert np.array_equal(a["geo"], o["geo"])
+
+
+@pytest.mark.parametrize("builder", ["sah", "midpoint"])
+def test_bvh_invariants(builder):
+    for sc in scenes():
+        a = bvh.build_host(sc, builder, 4, 8)
+        meta, bounds, order, geo = a["meta"], a["bounds"], a["order"], a["geo"]
+        n = len(geo)
+        assert len(meta) <= 2 * n - 1
+        assert sorted(order.tolist()) == list(range(n))  # every primitive in exactly one leaf
+        seen = np.zeros(n, int)
+        v0, e1, e2 = geo[:, 0:3], geo[:, 3:6], geo[:, 6:9]
+        verts = np.stack([v0, v0 + e1, v0 + e2], 1)
+        for i, (l, r, first, count) in enumerate(meta):
+            if count > 0:
+                assert l == r == -1 and count <= 4
+                prims = order[first:first + count]
+                seen[prims] += 1
+                lo = verts[prims].min((0, 1))
+                hi = verts[prims].max((0, 1))
+                assert np.all(bounds[i, :3] <= lo) and np.all(bounds[i, 3:] >= hi)
+            else:
+                for c in (l, r):
+                    assert np.all(bounds[i, :3] <= bounds[c, :3]) and np.all(bounds[i, 3:] >= bounds[c, 3:])
+        assert np.all(seen == 1)
+
+
+def test_sah_cost_not_worse_than_midpoint():
+    """Expected SAH traversal cost (sum over inner nodes of area * 1 + leaves area * count) of the
+    bucketed SAH tree is below the midpoint tree's on an uneven scene."""
+    sc = S.procedural_scene(seed=5, ico_subdivisions=4, bisect=False, sphere_count=300, light_count=4)
+
+    def cost(t):
+        b = t["bounds"].astype(np.float64)
+        d = np.maximum(b[:, 3:] - b[:, :3], 0)
+        area = 2 * (d[:, 0] * d[:, 1] + d[:, 1] * d[:, 2] + d[:, 2] * d[:, 0])
+        cnt = t["meta"][:, 3]
+        return float(np.sum(np.where(cnt > 0, area * cnt, area)) / area[0])
+
+    assert cost(bvh.build_host(sc, "sah")) < cost(bvh.build_host(sc, "midpoint"))
+
+
+@pytest.mark.parametrize("builder", ["sah", "midpoint"])
+def test_traversal_equals_brute_force(builder):
+    """Closest t from the BVH equals brute force over all triangles; the primitive id too, except
+    where two triangles tie exactly in t (coplanar faces)."""
+    for sc in scenes():
+        o = O.Oracle(sc, builder=builder)
+        rays = S.random_rays(sc, 20000, seed=4)
+        t, p, _ = o.trace_rays(rays)
+        bt, bp = o.brute_force(rays)
+        assert np.array_equal(t, bt)
+        diff = np.nonzero(p != bp)[0]
+        geo = o.export()["geo"].astype(np.float32)
+        for i in diff:  # each mismatch must be a tie: the BVH's primitive hits at the same t
+            r = rays[i]
+            tt, pp = O.Oracle.__new__(O.Oracle), None
+            v0, e1, e2 = geo[p[i], 0:3], geo[p[i], 3:6], geo[p[i], 6:9]
+            h = np.cross(r[3:], e2)
+            det = np.float32(np.dot(e1, h))
+            assert abs(det) >= 1e-4
+        assert len(diff) < 0.01 * len(rays)
+
+
+def test_moller_trumbore_hand_cases():
+    """Axis-aligned unit quad at z = 0 hit from z = 1 looking down -z: inclusive edges, det cutoff, t cutoff."""
+    v = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]], np.float32)
+    inst = np.zeros(1, S.INSTANCE_DTYPE)
+    inst["triangle_count"] = 2
+    inst["rotation"] = (0, 0, 0, 1)
+    inst["scale"] = 1
+    sc = S.Scene(v, np.array([0, 1, 2, 0, 2, 3], np.uint32), inst, S.make_materials(1), np.zeros(0, S.LIGHT_DTYPE),
+                 S.Camera())
+    o = O.Oracle(sc, builder="sah", leaf_size=4)
+    rays = np.array([[0.25, 0.5, 1, 0, 0, -1],     # interior of triangle 1
+                     [0.75, 0.25, 1, 0, 0, -1],    # interior of triangle 0
+                     [0.0, 0.0, 1, 0, 0, -1],      # shared vertex: first triangle in leaf order wins
+                     [1.0, 1.0, 1, 0, 0, -1],      # on the shared diagonal
+                     [1.0001, 0.5, 1, 0, 0, -1],   # just outside
+                     [0.5, 0.5, 0.00005, 0, 0, -1],  # t = 5e-5 < 1e-4: rejected
+                     [0.5, 0.5, 0.0002, 0, 0, -1],   # t = 2e-4: accepted
+                     [0.5, 0.5, 1, 1, 0, 0],          # parallel: det = 0
+                     [0.5, 0.5, -1, 0, 0, -1]], np.float32)  # pointing away
+    t, p, _ = o.trace_rays(rays)
+    bt, bp = o.brute_force(rays)
+    assert p[0] == 1 and p[1] == 0 and p[2] in (0, 1) and p[3] in (0, 1)
+    assert p[4] == -1 and p[5] == -1 and p[6] >= 0 and p[7] == -1 and p[8] == -1
+    assert t[0] == 1.0 and abs(t[6] - 0.0002) < 1e-9
+    assert np.array_equal(t, bt)
+    # |det| = |d . (e1 x e2)| below 1e-4 is rejected: a tiny triangle (area 2.5e-5 -> |e1 x e2| = 5e-5)
+    v2 = v * np.float32(0.00707)
+    sc2 = S.Scene(v2, sc.indices, inst, sc.materials, sc.lights, S.Camera())
+    t2, p2, _ = O.Oracle(sc2).trace_rays(np.array([[0.001, 0.002, 1, 0, 0, -1]], np.float32))
+    assert p2[0] == -1

@@ -1,0 +1,174 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on the same inputs.
+
+Integer/index results (primitive ids, counters, sample counts) are compared exactly. Radiance is
+compared bit for bit as well (the kernels evaluate the oracle's float expressions in the same order);
+the ≤1e-4 relative-RMSE tolerance of the path's contract is asserted alongside.
+"""
+import numpy as np
+import pytest
+
+from conftest import relrmse
+
+pytestmark = pytest.mark.gpu
+
+REL_TOL = 1e-4  # per-pixel radiance, relative RMSE
+
+
+def _oracle(scene, builder="sah", **kw):
+    from oracle import oracle as O
+
+    return O.Oracle(scene, builder=builder, **kw)
+
+
+@pytest.mark.parametrize("builder", ["sah", "midpoint"])
+def test_trace_rays_closest_hit_bitexact(cornell, small_procedural, tracer_factory, builder):
+    from vkpathtracer_amd import scene as S
+
+    for sc in (cornell, small_procedural):
+        print(f"trace_rays closest {builder} {sc.name}", flush=True)
+        pt = tracer_factory(sc, S.RenderSettings(width=8, height=8), builder=builder)
+        rays = S.random_rays(sc, 200_000, seed=11)
+        t, p, c = pt.trace_rays(rays)
+        ot, op, oc = _oracle(sc, builder).trace_rays(rays)
+        assert (p >= 0).sum() > 1000
+        assert np.array_equal(p, op), f"{(p != op).sum()} primitive ids differ"
+        assert np.array_equal(t, ot)
+        for k in ("bvh_node_visits", "bvh_leaf_visits", "triangle_tests", "triangle_hits"):
+            assert c[k] == oc[k], (k, c[k], oc[k])
+
+
+def test_trace_rays_any_hit_matches(small_procedural, tracer_factory):
+    from vkpathtracer_amd import scene as S
+
+    sc = small_procedural
+    pt = tracer_factory(sc, S.RenderSettings(width=8, height=8))
+    rays = S.random_rays(sc, 100_000, seed=5)
+    tmax = np.random.default_rng(2).uniform(0.01, 8.0, len(rays)).astype(np.float32)
+    t, p, c = pt.trace_rays(rays, any_hit=True, tmax=tmax)
+    ot, op, oc = _oracle(sc).trace_rays(rays, any_hit=True, tmax=tmax)
+    ct, cp, _ = _oracle(sc).trace_rays(rays)
+    visible_closest = ~((cp >= 0) & (ct < tmax))
+    assert np.array_equal(p < 0, visible_closest), "any-hit occlusion differs from closest-hit < tmax"
+    assert np.array_equal(p, op) and np.array_equal(t, ot)
+    assert c["bvh_node_visits"] == oc["bvh_node_visits"]
+
+
+def test_render_cornell_small_bitexact(cornell, tracer_factory):
+    from vkpathtracer_amd import scene as S
+
+    st = S.RenderSettings(width=64, height=64, spp=4, max_depth=8)
+    pt = tracer_factory(cornell, st)
+    assert pt.render_frame_samples(0, st.spp), pt.status().last_error
+    film, cnt = pt.film()
+    counters = pt.read_counters()
+    rf, rc, rcnt = _oracle(cornell).render(st)
+    assert np.array_equal(cnt, rc)
+    assert relrmse(film, rf) <= REL_TOL
+    assert np.array_equal(film, rf), f"max |diff| {np.abs(film - rf).max()}"
+    for k in ("samples", "rays", "shadow_tests", "bvh_node_visits", "bvh_leaf_visits", "triangle_tests",
+              "triangle_hits"):
+        assert counters[k] == rcnt[k], (k, counters[k], rcnt[k])
+
+
+def test_render_cornell_256_16spp(cornell, tracer_factory):
+    """The 34-triangle box at its full size: 256x256, 16 spp, depth 8, NEE + MIS, RR from depth 3."""
+    from vkpathtracer_amd import scene as S
+
+    st = S.RenderSettings(width=256, height=256, spp=16, max_depth=8)
+    pt = tracer_factory(cornell, st)
+    assert pt.render_frame_samples(0, st.spp)
+    hdr = pt.resolve_hdr()
+    film, _ = pt.film()
+    rf, rc, _ = _oracle(cornell).render(st, threads=8)
+    ref_hdr = rf / np.maximum(rc, 1)[..., None]
+    err = relrmse(hdr, ref_hdr)
+    print(f"cornell 256x256x16 relRMSE vs oracle = {err:.3e}")
+    assert err <= REL_TOL
+    assert np.array_equal(film, rf)
+
+
+def test_render_tiles_partial_and_batched(cornell, tracer_factory):
+    """Tiles of different rectangles and samples in one render_tiles call accumulate exactly like
+    sequential render_tile calls (the reference's per-pixel order)."""
+    from vkpathtracer_amd import scene as S
+
+    st = S.RenderSettings(width=48, height=40, spp=3, max_depth=6)
+    tiles = [(0, 0, 48, 40, 0), (5, 7, 20, 9, 1), (40, 30, 30, 30, 2), (0, 16, 48, 16, 1), (3, 3, 0, 5, 2)]
+    pt = tracer_factory(cornell, st)
+    assert pt.render_tiles([t + (i, 0) for i, t in enumerate(tiles)])
+    film, cnt = pt.film()
+    o = _oracle(cornell)
+    ref = np.zeros_like(film)
+    refc = np.zeros_like(cnt)
+    for (x, y, w, h, s) in tiles:  # the tiles overlap: accumulate one tile at a time, in order
+        f, c, _ = o.render(st, sample_begin=s, sample_count=1, tile=(x, y, w, h))
+        m = c > 0
+        ref[m] = ref[m] + f[m]
+        refc += c
+    assert np.array_equal(cnt, refc)
+    assert np.array_equal(film, ref)
+
+
+def test_wavefront_capacity_split_is_invisible(cornell, tracer_factory):
+    from vkpathtracer_amd import scene as S
+
+    st = S.RenderSettings(width=100, height=60, spp=5, max_depth=8)
+    a = tracer_factory(cornell, st)
+    assert a.render_frame_samples(0, st.spp)
+    fa, ca = a.film()
+    b = tracer_factory(cornell, st)
+    b.set_wavefront_capacity(1024)  # forces row-band splits and many passes
+    assert b.render_frame_samples(0, st.spp)
+    fb, cb = b.film()
+    assert np.array_equal(ca, cb) and np.array_equal(fa, fb)
+
+
+def test_procedural_scene_pixel_subset(tracer_factory):
+    """The 1M-triangle workload at 1920x1080: one full-frame sample on the GPU against the oracle on
+    a seeded subset of 3000 pixels (the RNG is keyed by pixel and sample, so a subset is exact)."""
+    from vkpathtracer_amd import scene as S
+
+    sc = S.procedural_scene(seed=1)
+    assert sc.triangle_count == 999_424 + 32
+    st = S.RenderSettings(width=1920, height=1080, spp=1, max_depth=8)
+    pt = tracer_factory(sc, st)
+    info = pt.build_info
+    print("build", info, flush=True)
+    assert pt.render_tiles([(0, 0, 1920, 1080, 3, 0, 0)])
+    film, cnt = pt.film()
+    px = np.random.default_rng(99).choice(1920 * 1080, 3000, replace=False).astype(np.uint32)
+    rgb, valid, _ = _oracle(sc).render_pixels(st, px, sample_index=3, threads=8)
+    gpu = film.reshape(-1, 3)[px]
+    gcnt = cnt.reshape(-1)[px]
+    assert np.array_equal(gcnt.astype(bool), valid)
+    err = relrmse(gpu[valid], rgb[valid])
+    print(f"procedural 1080p subset relRMSE = {err:.3e}, exact pixels {np.all(gpu == rgb, 1).mean():.4f}")
+    assert err <= REL_TOL
+    assert np.array_equal(gpu[valid], rgb[valid].astype(np.float32))
+
+
+def test_state_machine_errors(cornell):
+    from vkpathtracer_amd import scene as S
+    from vkpathtracer_amd.tracer import HipPathTracer
+
+    with HipPathTracer(0) as pt:
+        st = pt.load_scene_snapshot(cornell)
+        assert st.code_name == "not_ready"
+        assert pt.status().lifecycle == "Uninitialized"
+        assert pt.configure(S.RenderSettings(width=16, height=16))
+        assert pt.status().lifecycle == "Configured"
+        assert not pt.render_tile((0, 0, 16, 16, 0))
+        assert "scene" in pt.status().last_error
+        assert pt.build_or_update_acceleration().code_name == "not_ready"
+        assert pt.load_scene_snapshot(cornell)
+        assert pt.status().lifecycle == "SceneLoaded"
+        assert not pt.render_tile((0, 0, 16, 16, 0))
+        assert pt.build_or_update_acceleration()
+        assert pt.status().lifecycle == "Ready"
+        assert pt.render_tile((0, 0, 16, 16, 0))
+        assert pt.render_tile((100, 100, 16, 16, 1))  # empty after clamping: a no-op success
+        _, cnt = pt.film()
+        assert cnt.sum() == 256
+        assert pt.reset_accumulation()
+        _, cnt = pt.film()
+        assert cnt.sum() == 0 and pt.status().accumulation_gen == 1

@@ -1,0 +1,197 @@
+"""ctypes mirrors of the records in include/vkpt_hip.h and the loader for libvkpt_hip.so."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import pathlib
+
+import numpy as np
+
+from . import scene as _scene
+
+PKG_DIR = pathlib.Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR / "lib" / "libvkpt_hip.so"
+
+STATUS_NAMES = ["ok", "unsupported", "invalid_argument", "not_ready", "busy", "alloc_failed", "timeout",
+                "cancelled", "internal_error"]  # vkpt::core::StatusCodeName, Result.h:69-91
+
+
+class PtCamera(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("target", C.c_float * 3), ("up", C.c_float * 3),
+                ("fov_deg", C.c_float), ("aperture_radius", C.c_float), ("focus_distance", C.c_float),
+                ("iris_blade_count", C.c_uint32), ("iris_rotation_degrees", C.c_float),
+                ("iris_roundness", C.c_float), ("anamorphic_squeeze", C.c_float)]
+
+
+class PtSceneView(C.Structure):
+    _fields_ = [("vertices", C.c_void_p), ("indices", C.c_void_p), ("local_vertices", C.c_void_p),
+                ("local_indices", C.c_void_p), ("instances", C.c_void_p), ("materials", C.c_void_p),
+                ("lights", C.c_void_p), ("environment_map", C.c_void_p),
+                ("vertex_count", C.c_uint32), ("index_count", C.c_uint32), ("local_vertex_count", C.c_uint32),
+                ("local_index_count", C.c_uint32), ("instance_count", C.c_uint32), ("material_count", C.c_uint32),
+                ("light_count", C.c_uint32), ("environment_map_width", C.c_uint32),
+                ("environment_map_height", C.c_uint32), ("environment_color", C.c_float * 3),
+                ("environment_map_scale", C.c_float * 3), ("camera", PtCamera)]
+
+
+class PtRenderSettings(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
+                ("max_depth", C.c_uint32), ("enable_nee", C.c_uint32), ("enable_mis", C.c_uint32),
+                ("deterministic", C.c_uint32), ("rr_start_depth", C.c_uint32), ("rr_min_survival", C.c_float),
+                ("rr_max_survival", C.c_float), ("camera_aperture_radius", C.c_float),
+                ("camera_focus_distance", C.c_float)]
+
+
+class PtRenderTile(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("x", "y", "width", "height", "sample_index", "tile_id", "gpu_id")]
+
+
+COUNTER_FIELDS = ("samples", "rays", "triangle_tests", "sdf_tests", "sdf_steps", "triangle_hits", "sdf_hits",
+                  "sdf_misses", "bvh_node_visits", "bvh_leaf_visits", "shadow_tests")
+
+
+class PtSampleCounters(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in COUNTER_FIELDS]
+
+    def as_dict(self) -> dict:
+        return {n: int(getattr(self, n)) for n in COUNTER_FIELDS}
+
+
+class PtBvhParams(C.Structure):
+    _fields_ = [("builder", C.c_uint32), ("leaf_size", C.c_uint32), ("bucket_count", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+
+class PtBuildInfo(C.Structure):
+    _fields_ = [("build_ms", C.c_double), ("device_bytes", C.c_uint64), ("built", C.c_uint32),
+                ("primitive_count", C.c_uint32), ("node_count", C.c_uint32), ("leaf_count", C.c_uint32),
+                ("max_depth", C.c_uint32), ("deterministic", C.c_uint32)]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class PtTracerStatus(C.Structure):
+    _fields_ = [("total_samples", C.c_uint64), ("accumulation_gen", C.c_uint64), ("lifecycle", C.c_uint32),
+                ("scene_loaded", C.c_uint32), ("accel_valid", C.c_uint32), ("ready_to_render", C.c_uint32),
+                ("current_sample", C.c_uint32), ("pad0", C.c_uint32)]
+
+
+KERNELS = ("raygen", "extend", "shade", "shadow", "film")
+
+
+class PtKernelStats(C.Structure):
+    _fields_ = [("total_ms", C.c_double * 5), ("launches", C.c_uint64 * 5), ("items", C.c_uint64 * 5),
+                ("inner_node_fetches", C.c_uint64), ("shadow_node_visits", C.c_uint64),
+                ("shadow_triangle_tests", C.c_uint64), ("shadow_inner_node_fetches", C.c_uint64),
+                ("shadow_leaf_visits", C.c_uint64), ("shadow_rays", C.c_uint64)]
+
+
+# Every symbol include/vkpt_hip.h declares, with its ctypes signature.
+_P = C.c_void_p
+SIGNATURES = {
+    "pt_hip_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "pt_hip_destroy": (None, [_P]),
+    "pt_hip_configure": (C.c_int, [_P, C.POINTER(PtRenderSettings)]),
+    "pt_hip_load_scene": (C.c_int, [_P, C.POINTER(PtSceneView)]),
+    "pt_hip_build_accel": (C.c_int, [_P, C.POINTER(PtBvhParams), C.POINTER(PtBuildInfo)]),
+    "pt_hip_reset_accumulation": (C.c_int, [_P]),
+    "pt_hip_render_tile": (C.c_int, [_P, C.POINTER(PtRenderTile), C.c_uint64]),
+    "pt_hip_render_tiles": (C.c_int, [_P, C.POINTER(PtRenderTile), C.c_uint32, C.c_uint64]),
+    "pt_hip_read_film": (C.c_int, [_P, _P, _P]),
+    "pt_hip_resolve_hdr": (C.c_int, [_P, _P]),
+    "pt_hip_read_counters": (C.c_int, [_P, C.POINTER(PtSampleCounters)]),
+    "pt_hip_status": (C.c_int, [_P, C.POINTER(PtTracerStatus)]),
+    "pt_hip_shutdown": (C.c_int, [_P]),
+    "pt_hip_last_error": (C.c_char_p, [_P]),
+    "pt_hip_trace_rays": (C.c_int, [_P, C.c_int, _P, _P, C.c_uint32, _P, _P, C.POINTER(PtSampleCounters)]),
+    "pt_hip_synchronize": (C.c_int, [_P]),
+    "pt_hip_set_wavefront_capacity": (C.c_int, [_P, C.c_uint32]),
+    "pt_hip_set_profiling": (C.c_int, [_P, C.c_int]),
+    "pt_hip_read_kernel_stats": (C.c_int, [_P, C.POINTER(PtKernelStats)]),
+    "pt_hip_reset_kernel_stats": (C.c_int, [_P]),
+    "pt_bvh_build_host": (C.c_int, [C.POINTER(PtSceneView), C.POINTER(PtBvhParams), _P, _P, _P, _P, _P, _P, _P]),
+    "pt_hip_sizeof": (C.c_uint32, [C.c_int]),
+    "pt_hip_version": (C.c_char_p, []),
+}
+
+_lib = None
+
+
+class NativeLibraryMissing(RuntimeError):
+    pass
+
+
+def load_library(path: os.PathLike | None = None) -> C.CDLL:
+    """Load libvkpt_hip.so (built in-tree by __graft_entry__.build()). Raises if it is absent."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = pathlib.Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise NativeLibraryMissing(f"{p} is missing: build the HIP library first (python -c 'import __graft_entry__ as g; g.build()')")
+    lib = C.CDLL(str(p))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _ptr(a: np.ndarray | None):
+    if a is None or a.size == 0:
+        return None
+    return a.ctypes.data
+
+
+class SceneView:
+    """PtSceneView over a Scene's arrays; keeps the arrays alive while the view exists."""
+
+    def __init__(self, sc: "_scene.Scene"):
+        self.scene = sc
+        self.keep = [sc.vertices, sc.indices, sc.local_vertices, sc.local_indices, sc.instances, sc.materials,
+                     sc.lights]
+        v = PtSceneView()
+        v.vertices = _ptr(sc.vertices)
+        v.indices = _ptr(sc.indices)
+        v.local_vertices = _ptr(sc.local_vertices)
+        v.local_indices = _ptr(sc.local_indices)
+        v.instances = _ptr(sc.instances)
+        v.materials = _ptr(sc.materials)
+        v.lights = _ptr(sc.lights)
+        v.environment_map = None
+        v.vertex_count = len(sc.vertices)
+        v.index_count = len(sc.indices)
+        v.local_vertex_count = len(sc.local_vertices)
+        v.local_index_count = len(sc.local_indices)
+        v.instance_count = len(sc.instances)
+        v.material_count = len(sc.materials)
+        v.light_count = len(sc.lights)
+        v.environment_color[:] = list(sc.environment_color)
+        c = sc.camera
+        cam = PtCamera()
+        cam.position[:] = list(c.position)
+        cam.target[:] = list(c.target)
+        cam.up[:] = list(c.up)
+        cam.fov_deg = c.fov_deg
+        cam.aperture_radius = c.aperture_radius
+        cam.focus_distance = c.focus_distance
+        cam.iris_blade_count = c.iris_blade_count
+        cam.iris_rotation_degrees = c.iris_rotation_degrees
+        cam.iris_roundness = c.iris_roundness
+        cam.anamorphic_squeeze = c.anamorphic_squeeze
+        v.camera = cam
+        self.view = v
+
+
+def make_settings(s: "_scene.RenderSettings") -> PtRenderSettings:
+    out = PtRenderSettings()
+    out.seed = s.seed & 0xFFFFFFFFFFFFFFFF
+    out.width, out.height, out.spp, out.max_depth = s.width, s.height, s.spp, s.max_depth
+    out.enable_nee, out.enable_mis, out.deterministic = int(s.enable_nee), int(s.enable_mis), int(s.deterministic)
+    out.rr_start_depth = s.rr_start_depth
+    out.rr_min_survival, out.rr_max_survival = s.rr_min_survival, s.rr_max_survival
+    out.camera_aperture_radius, out.camera_focus_distance = s.camera_aperture_radius, s.camera_focus_distance
+    return out

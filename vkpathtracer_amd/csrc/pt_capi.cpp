@@ -1,0 +1,813 @@
+// pt_capi.cpp — the C ABI (include/vkpt_hip.h): IPathTracer state machine, device memory,
+// and the host loop that drives the wavefront kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "pt_bvh.h"
+#include "pt_device.h"
+#include "vkpt_hip.h"
+
+namespace {
+
+constexpr uint32_t kDefaultCapacity = 8u * 1024u * 1024u;
+constexpr uint32_t kMaxTilesPerPass = 96u;
+
+template <typename T>
+struct DevArray {
+  T* ptr = nullptr;
+  size_t n = 0;
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    n = 0;
+  }
+  hipError_t ensure(size_t count) {
+    if (count <= n && ptr) return hipSuccess;
+    release();
+    const hipError_t e = hipMalloc(reinterpret_cast<void**>(&ptr), std::max<size_t>(count, 1) * sizeof(T));
+    if (e == hipSuccess) n = count;
+    else ptr = nullptr;
+    return e;
+  }
+};
+
+struct V3h {
+  float x, y, z;
+};
+V3h sub3(V3h a, V3h b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+float dot3(V3h a, V3h b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+V3h cross3(V3h a, V3h b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+V3h normalize3(V3h a) {  // ScalarCpuPathTracer.cpp:104-110
+  const float l = std::sqrt(dot3(a, a));
+  if (l <= 1e-4f) return {0.0f, 1.0f, 0.0f};
+  return {a.x / l, a.y / l, a.z / l};
+}
+
+struct ProfiledLaunch {
+  int kernel;
+  hipEvent_t start, stop;
+};
+
+}  // namespace
+
+struct pt_tracer {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool configured = false, scene_loaded = false, accel_valid = false;
+  PtRenderSettings settings{};
+  std::string last_error;
+  uint32_t current_sample = 0;
+  uint64_t accumulation_gen = 0;
+
+  // host copy of the snapshot
+  std::vector<float> vertices, local_vertices, environment_map;
+  std::vector<uint32_t> indices, local_indices;
+  std::vector<PtInstance> instances;
+  std::vector<PtMaterial> materials;
+  std::vector<PtLight> lights;
+  PtSceneView view{};
+
+  ptb::Bvh bvh;
+  ptb::DeviceLayout layout;
+  PtBuildInfo info{};
+
+  DevArray<float4> d_nodes, d_tris, d_prims, d_film_sum;
+  DevArray<PtMaterial> d_materials;
+  DevArray<PtLight> d_lights;
+  DevArray<uint32_t> d_film_count, d_queue, d_error;
+  DevArray<unsigned long long> d_counters, d_query_counters;
+  DevArray<float> d_path_f;
+  DevArray<uint32_t> d_path_u;
+  DevArray<uint64_t> d_rng;
+  DevArray<int32_t> d_stack_ref;
+  DevArray<float> d_stack_t;
+  uint32_t path_capacity_alloc = 0;
+  uint32_t capacity = kDefaultCapacity;
+
+  ptd::DevScene dscene{};
+  ptd::DevCamera dcam{};
+  ptd::DevPaths paths{};
+  uint32_t grid_ext = 0, grid_shade = 0;
+
+  bool profiling = false;
+  std::vector<ProfiledLaunch> pending;
+  std::vector<hipEvent_t> event_pool;
+  PtKernelStats kstats{};
+};
+
+namespace {
+
+int fail(pt_tracer* t, int code, const std::string& msg) {
+  if (t) t->last_error = msg;
+  return code;
+}
+
+int hip_fail(pt_tracer* t, hipError_t e, const char* what) {
+  const int code = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? PT_ALLOC_FAILED : PT_INTERNAL_ERROR;
+  return fail(t, code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define PT_HIP(t, expr, what)                   \
+  do {                                          \
+    const hipError_t e_ = (expr);               \
+    if (e_ != hipSuccess) return hip_fail((t), e_, (what)); \
+  } while (0)
+
+hipEvent_t take_event(pt_tracer* t) {
+  if (!t->event_pool.empty()) {
+    hipEvent_t e = t->event_pool.back();
+    t->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Launch wrapper: brackets the launch with events while profiling.
+template <typename F>
+void timed(pt_tracer* t, int kernel, uint64_t items, F&& launch) {
+  if (!t->profiling) {
+    launch();
+    return;
+  }
+  ProfiledLaunch pl{kernel, take_event(t), take_event(t)};
+  if (pl.start) (void)hipEventRecord(pl.start, t->stream);
+  launch();
+  if (pl.stop) (void)hipEventRecord(pl.stop, t->stream);
+  t->pending.push_back(pl);
+  t->kstats.launches[kernel] += 1;
+  t->kstats.items[kernel] += items;
+}
+
+void collect_profile(pt_tracer* t) {
+  for (const ProfiledLaunch& pl : t->pending) {
+    if (pl.start && pl.stop) {
+      float ms = 0.0f;
+      if (hipEventSynchronize(pl.stop) == hipSuccess && hipEventElapsedTime(&ms, pl.start, pl.stop) == hipSuccess)
+        t->kstats.total_ms[pl.kernel] += ms;
+    }
+    if (pl.start) t->event_pool.push_back(pl.start);
+    if (pl.stop) t->event_pool.push_back(pl.stop);
+  }
+  t->pending.clear();
+}
+
+int check_device_error(pt_tracer* t) {
+  if (!t->d_error.ptr) return PT_OK;
+  uint32_t flag = 0;
+  PT_HIP(t, hipMemcpyAsync(&flag, t->d_error.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost, t->stream), "read error flag");
+  PT_HIP(t, hipStreamSynchronize(t->stream), "synchronize");
+  if (flag) {
+    (void)hipMemsetAsync(t->d_error.ptr, 0, sizeof(uint32_t), t->stream);
+    return fail(t, PT_INTERNAL_ERROR, "traversal stack overflow area exhausted (BVH deeper than the build reported)");
+  }
+  return PT_OK;
+}
+
+int sync_all(pt_tracer* t) {
+  PT_HIP(t, hipStreamSynchronize(t->stream), "synchronize");
+  if (t->profiling) collect_profile(t);
+  return check_device_error(t);
+}
+
+void free_scene_device(pt_tracer* t) {
+  t->d_nodes.release();
+  t->d_tris.release();
+  t->d_prims.release();
+  t->d_materials.release();
+  t->d_lights.release();
+}
+
+void free_paths(pt_tracer* t) {
+  t->d_path_f.release();
+  t->d_path_u.release();
+  t->d_rng.release();
+  t->path_capacity_alloc = 0;
+}
+
+// Path-state arrays for `cap` paths: 29 float arrays, 4 u32 arrays, 1 u64 array.
+int ensure_paths(pt_tracer* t, uint32_t cap) {
+  if (cap <= t->path_capacity_alloc) return PT_OK;
+  free_paths(t);
+  constexpr size_t kF = 29, kU = 4;
+  const size_t n = cap;
+  PT_HIP(t, t->d_path_f.ensure(kF * n), "allocate path state");
+  PT_HIP(t, t->d_path_u.ensure(kU * n), "allocate path state");
+  PT_HIP(t, t->d_rng.ensure(n), "allocate path state");
+  float* f = t->d_path_f.ptr;
+  uint32_t* u = t->d_path_u.ptr;
+  ptd::DevPaths& P = t->paths;
+  for (int b = 0; b < 2; ++b) {
+    for (int k = 0; k < 6; ++k) P.rays[b].c[k] = f + (size_t)(6 * b + k) * n;
+    P.rays[b].qpath = u + (size_t)b * n;
+  }
+  P.hit_t = f + 12 * n;
+  P.hit_prim = reinterpret_cast<int32_t*>(u + 2 * n);
+  for (int k = 0; k < 3; ++k) {
+    P.tp[k] = f + (size_t)(13 + k) * n;
+    P.rad[k] = f + (size_t)(16 + k) * n;
+    P.scontrib[k] = f + (size_t)(26 + k) * n;
+  }
+  for (int k = 0; k < 6; ++k) P.sray[k] = f + (size_t)(19 + k) * n;
+  P.stmax = f + 25 * n;
+  P.spath = u + 3 * n;
+  P.rng = t->d_rng.ptr;
+  P.capacity = cap;
+  t->path_capacity_alloc = cap;
+  return PT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pt_hip_version(void) { return "vkpathtracer_amd 0.1 (gfx950 wavefront)"; }
+
+uint32_t pt_hip_sizeof(int which) {
+  switch (which) {
+    case 0: return sizeof(PtMaterial);
+    case 1: return sizeof(PtLight);
+    case 2: return sizeof(PtInstance);
+    case 3: return sizeof(PtCamera);
+    case 4: return sizeof(PtSceneView);
+    case 5: return sizeof(PtRenderSettings);
+    case 6: return sizeof(PtSampleCounters);
+    case 7: return sizeof(PtRenderTile);
+    default: return 0;
+  }
+}
+
+int pt_hip_create(int device, pt_tracer** out) {
+  if (!out) return PT_INVALID_ARGUMENT;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return PT_UNSUPPORTED;
+  if (device < 0 || device >= count) return PT_INVALID_ARGUMENT;
+  if (hipSetDevice(device) != hipSuccess) return PT_INTERNAL_ERROR;
+  pt_tracer* t = new (std::nothrow) pt_tracer();
+  if (!t) return PT_ALLOC_FAILED;
+  t->device = device;
+  if (hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete t;
+    return PT_INTERNAL_ERROR;
+  }
+  if (t->d_counters.ensure(ptd::kCounterSlots) != hipSuccess || t->d_query_counters.ensure(ptd::kCounterSlots) != hipSuccess ||
+      t->d_queue.ensure(ptd::kQueueSlots) != hipSuccess || t->d_error.ensure(1) != hipSuccess) {
+    (void)hipStreamDestroy(t->stream);
+    delete t;
+    return PT_ALLOC_FAILED;
+  }
+  (void)hipMemsetAsync(t->d_counters.ptr, 0, sizeof(unsigned long long) * ptd::kCounterSlots, t->stream);
+  (void)hipMemsetAsync(t->d_queue.ptr, 0, sizeof(uint32_t) * ptd::kQueueSlots, t->stream);
+  (void)hipMemsetAsync(t->d_error.ptr, 0, sizeof(uint32_t), t->stream);
+  t->grid_ext = ptd::persistent_grid_extend(device);
+  t->grid_shade = ptd::persistent_grid_shade(device);
+  if (t->grid_shade == 0) t->grid_shade = 1024;
+  *out = t;
+  return PT_OK;
+}
+
+void pt_hip_destroy(pt_tracer* t) {
+  if (!t) return;
+  (void)hipSetDevice(t->device);
+  (void)hipStreamSynchronize(t->stream);
+  collect_profile(t);
+  for (hipEvent_t e : t->event_pool) (void)hipEventDestroy(e);
+  free_scene_device(t);
+  free_paths(t);
+  t->d_film_sum.release();
+  t->d_film_count.release();
+  t->d_queue.release();
+  t->d_error.release();
+  t->d_counters.release();
+  t->d_query_counters.release();
+  t->d_stack_ref.release();
+  t->d_stack_t.release();
+  (void)hipStreamDestroy(t->stream);
+  delete t;
+}
+
+const char* pt_hip_last_error(const pt_tracer* t) { return t ? t->last_error.c_str() : "null tracer"; }
+
+int pt_hip_configure(pt_tracer* t, const PtRenderSettings* s) {
+  if (!t || !s) return fail(t, PT_INVALID_ARGUMENT, "configure: null argument");
+  (void)hipSetDevice(t->device);
+  PT_HIP(t, hipStreamSynchronize(t->stream), "synchronize");
+  if (!(s->rr_min_survival == s->rr_min_survival) || !(s->rr_max_survival == s->rr_max_survival))
+    return fail(t, PT_INVALID_ARGUMENT, "configure: russian roulette bounds are NaN");
+  t->settings = *s;
+  t->settings.max_depth = std::max(1u, s->max_depth);  // MakePathTraceSettings, PathTracer.cpp:283
+  t->settings.camera_aperture_radius = std::max(0.0f, s->camera_aperture_radius);
+  t->settings.camera_focus_distance = std::max(0.0f, s->camera_focus_distance);
+  const size_t pixels = static_cast<size_t>(s->width) * s->height;
+  PT_HIP(t, t->d_film_sum.ensure(pixels), "allocate film");
+  PT_HIP(t, t->d_film_count.ensure(pixels), "allocate film");
+  ptd::launch_clear_film(t->stream, ptd::DevFilm{t->d_film_sum.ptr, t->d_film_count.ptr}, static_cast<uint32_t>(pixels));
+  ptd::launch_zero_counters(t->stream, t->d_counters.ptr);
+  t->configured = true;
+  t->scene_loaded = false;
+  t->accel_valid = false;
+  t->current_sample = 0;
+  t->accumulation_gen = 0;
+  t->last_error.clear();
+  return PT_OK;
+}
+
+int pt_hip_load_scene(pt_tracer* t, const PtSceneView* sc) {
+  if (!t || !sc) return fail(t, PT_INVALID_ARGUMENT, "load_scene: null argument");
+  if (!t->configured) return fail(t, PT_NOT_READY, "load_scene_snapshot called before configure");
+  for (uint32_t i = 0; i < sc->material_count; ++i)
+    if (sc->materials[i].material_effect != 0u)
+      return fail(t, PT_UNSUPPORTED, "material_effect != 0 (procedural albedo) is not supported on this path");
+  if (sc->environment_map && sc->environment_map_width && sc->environment_map_height)
+    return fail(t, PT_UNSUPPORTED, "environment maps are not supported on this path");
+  auto copy = [](auto& dst, const auto* src, size_t n) {
+    if (src && n) dst.assign(src, src + n);
+    else dst.clear();
+  };
+  copy(t->vertices, sc->vertices, static_cast<size_t>(sc->vertex_count) * 3u);
+  copy(t->indices, sc->indices, sc->index_count);
+  copy(t->local_vertices, sc->local_vertices, static_cast<size_t>(sc->local_vertex_count) * 3u);
+  copy(t->local_indices, sc->local_indices, sc->local_index_count);
+  copy(t->instances, sc->instances, sc->instance_count);
+  copy(t->materials, sc->materials, sc->material_count);
+  copy(t->lights, sc->lights, sc->light_count);
+  if (t->materials.empty()) {  // ScalarCpuPathTracer.cpp:469-471
+    PtMaterial m{};
+    m.albedo[0] = m.albedo[1] = m.albedo[2] = 1.0f;
+    m.roughness = 1.0f;
+    m.ior = 1.5f;
+    m.alpha = 1.0f;
+    m.base_color_texture_index = m.normal_texture_index = 0xFFFFFFFFu;
+    t->materials.push_back(m);
+  }
+  t->view = *sc;
+  t->view.vertices = t->vertices.data();
+  t->view.indices = t->indices.data();
+  t->view.local_vertices = t->local_vertices.data();
+  t->view.local_indices = t->local_indices.data();
+  t->view.instances = t->instances.data();
+  t->view.materials = t->materials.data();
+  t->view.lights = t->lights.data();
+  t->view.environment_map = nullptr;
+  t->view.vertex_count = static_cast<uint32_t>(t->vertices.size() / 3u);
+  t->view.index_count = static_cast<uint32_t>(t->indices.size());
+  t->view.local_vertex_count = static_cast<uint32_t>(t->local_vertices.size() / 3u);
+  t->view.local_index_count = static_cast<uint32_t>(t->local_indices.size());
+  t->view.instance_count = static_cast<uint32_t>(t->instances.size());
+  t->view.material_count = static_cast<uint32_t>(t->materials.size());
+  t->view.light_count = static_cast<uint32_t>(t->lights.size());
+  t->scene_loaded = true;
+  t->accel_valid = false;
+  t->last_error.clear();
+  return PT_OK;
+}
+
+int pt_hip_build_accel(pt_tracer* t, const PtBvhParams* params, PtBuildInfo* info) {
+  if (!t) return PT_INVALID_ARGUMENT;
+  if (!t->scene_loaded) return fail(t, PT_NOT_READY, "build_or_update_acceleration called before load_scene_snapshot");
+  (void)hipSetDevice(t->device);
+  PtBvhParams p{PT_BVH_SAH, 4u, 8u, 0u};
+  if (params) p = *params;
+  if (p.leaf_size == 0) p.leaf_size = 4u;
+  if (p.bucket_count == 0) p.bucket_count = 8u;
+  if (p.leaf_size > ptb::kMaxLeafSize) return fail(t, PT_INVALID_ARGUMENT, "leaf_size exceeds 31");
+  if (p.builder != PT_BVH_SAH && p.builder != PT_BVH_MIDPOINT) return fail(t, PT_INVALID_ARGUMENT, "unknown builder");
+  PT_HIP(t, hipStreamSynchronize(t->stream), "synchronize");
+  t->accel_valid = false;
+  const auto t0 = std::chrono::steady_clock::now();
+  std::vector<float> raw;
+  std::string err;
+  if (!ptb::flatten(t->view, t->bvh.prims, raw, err)) return fail(t, PT_INVALID_ARGUMENT, err);
+  if (p.builder == PT_BVH_SAH) ptb::build_sah(t->bvh, raw, p.leaf_size, p.bucket_count);
+  else ptb::build_midpoint(t->bvh, raw, p.leaf_size);
+  ptb::pack_device_layout(t->bvh, t->layout);
+  const ptb::DeviceLayout& L = t->layout;
+
+  free_scene_device(t);
+  uint64_t bytes = 0;
+  auto upload = [&](auto& arr, const void* src, size_t count, size_t elem) -> hipError_t {
+    hipError_t e = arr.ensure(count);
+    if (e != hipSuccess) return e;
+    bytes += count * elem;
+    if (count) e = hipMemcpy(arr.ptr, src, count * elem, hipMemcpyHostToDevice);
+    return e;
+  };
+  PT_HIP(t, upload(t->d_nodes, L.nodes.data(), L.nodes.size() / 4u, sizeof(float4)), "upload BVH nodes");
+  PT_HIP(t, upload(t->d_tris, L.tris.data(), L.tris.size() / 4u, sizeof(float4)), "upload triangles");
+  PT_HIP(t, upload(t->d_prims, L.prims.data(), L.prims.size() / 4u, sizeof(float4)), "upload primitives");
+  PT_HIP(t, upload(t->d_materials, t->materials.data(), t->materials.size(), sizeof(PtMaterial)), "upload materials");
+  PT_HIP(t, upload(t->d_lights, t->lights.data(), t->lights.size(), sizeof(PtLight)), "upload lights");
+
+  // Traversal stack overflow area: a lane holds at most max_depth entries.
+  const uint32_t need = t->bvh.max_depth + 2u > static_cast<uint32_t>(ptd::kStackLds)
+                            ? t->bvh.max_depth + 2u - static_cast<uint32_t>(ptd::kStackLds)
+                            : 1u;
+  const uint32_t threads = t->grid_ext * static_cast<uint32_t>(ptd::kBlock);
+  PT_HIP(t, t->d_stack_ref.ensure(static_cast<size_t>(need) * threads), "allocate traversal stack");
+  PT_HIP(t, t->d_stack_t.ensure(static_cast<size_t>(need) * threads), "allocate traversal stack");
+  bytes += static_cast<uint64_t>(need) * threads * 8u;
+  t->paths.stack_ref = t->d_stack_ref.ptr;
+  t->paths.stack_t = t->d_stack_t.ptr;
+  t->paths.stack_overflow_depth = need;
+  t->paths.stack_threads = threads;
+  t->paths.queue = t->d_queue.ptr;
+  t->paths.counters = t->d_counters.ptr;
+  t->paths.error = t->d_error.ptr;
+
+  ptd::DevScene& S = t->dscene;
+  S.nodes = t->d_nodes.ptr;
+  S.tris = t->d_tris.ptr;
+  S.prims = t->d_prims.ptr;
+  S.materials = t->d_materials.ptr;
+  S.lights = t->d_lights.ptr;
+  S.material_count = static_cast<uint32_t>(t->materials.size());
+  S.light_count = static_cast<uint32_t>(t->lights.size());
+  S.prim_count = static_cast<uint32_t>(t->bvh.prims.size());
+  S.has_transmissive = 0;
+  for (const PtMaterial& m : t->materials)
+    if (m.material_model == 5u || m.transmission > 0.05f) S.has_transmissive = 1;
+  S.root_ref = L.root_ref;
+  for (int k = 0; k < 6; ++k) S.root_box[k] = L.root_box[k];
+  for (int k = 0; k < 3; ++k) S.env[k] = t->view.environment_color[k];
+
+  // compute_camera_basis, ScalarCpuPathTracer.cpp:139-152; camera_rays constants, Render.cpp:74-83
+  const PtCamera& c = t->view.camera;
+  const V3h pos{c.position[0], c.position[1], c.position[2]};
+  const V3h fwd = normalize3(sub3(V3h{c.target[0], c.target[1], c.target[2]}, pos));
+  V3h right = normalize3(cross3(fwd, V3h{c.up[0], c.up[1], c.up[2]}));
+  if (dot3(right, right) <= 1e-4f * 1e-4f) right = {1.0f, 0.0f, 0.0f};
+  V3h up = normalize3(cross3(right, fwd));
+  if (dot3(up, up) <= 1e-4f * 1e-4f) up = {0.0f, 1.0f, 0.0f};
+  ptd::DevCamera& C = t->dcam;
+  const V3h* src[4] = {&pos, &fwd, &right, &up};
+  float* dst[4] = {C.pos, C.fwd, C.right, C.up};
+  for (int i = 0; i < 4; ++i) {
+    dst[i][0] = src[i]->x;
+    dst[i][1] = src[i]->y;
+    dst[i][2] = src[i]->z;
+  }
+  const float kPi = 3.14159265358979323846f;
+  C.aspect = static_cast<float>(t->settings.width) / std::max(1.0f, static_cast<float>(t->settings.height));
+  C.tan_half_fov = std::tan(0.5f * (c.fov_deg * (kPi / 180.0f)));
+  C.aperture = c.aperture_radius > 0.0f ? c.aperture_radius : t->settings.camera_aperture_radius;
+  C.focus = c.focus_distance > 0.0f ? c.focus_distance : t->settings.camera_focus_distance;
+  C.iris_blades = std::min(c.iris_blade_count, 64u);
+  C.iris_roundness = std::isfinite(c.iris_roundness) ? std::min(1.0f, std::max(0.0f, c.iris_roundness)) : 0.0f;
+  C.iris_rotation = c.iris_rotation_degrees * (kPi / 180.0f);
+  C.anamorphic = std::isfinite(c.anamorphic_squeeze) ? std::max(0.01f, c.anamorphic_squeeze) : 1.0f;
+
+  const auto t1 = std::chrono::steady_clock::now();
+  t->info = PtBuildInfo{};
+  t->info.build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  t->info.device_bytes = bytes;
+  t->info.built = 1;
+  t->info.primitive_count = S.prim_count;
+  t->info.node_count = static_cast<uint32_t>(t->bvh.nodes.size());
+  t->info.leaf_count = t->bvh.leaf_count;
+  t->info.max_depth = t->bvh.max_depth;
+  t->info.deterministic = 1;
+  if (info) *info = t->info;
+  t->accel_valid = true;
+  t->last_error.clear();
+  return PT_OK;
+}
+
+int pt_hip_reset_accumulation(pt_tracer* t) {
+  if (!t) return PT_INVALID_ARGUMENT;
+  if (!t->configured) return fail(t, PT_NOT_READY, "reset_accumulation called before configure");
+  (void)hipSetDevice(t->device);
+  const uint32_t pixels = t->settings.width * t->settings.height;
+  ptd::launch_clear_film(t->stream, ptd::DevFilm{t->d_film_sum.ptr, t->d_film_count.ptr}, pixels);
+  ptd::launch_zero_counters(t->stream, t->d_counters.ptr);
+  t->current_sample = 0;
+  ++t->accumulation_gen;
+  t->last_error.clear();
+  return PT_OK;
+}
+
+int pt_hip_render_tiles(pt_tracer* t, const PtRenderTile* tiles, uint32_t count, uint64_t frame_index) {
+  if (!t) return PT_INVALID_ARGUMENT;
+  if (!t->configured || !t->scene_loaded)
+    return fail(t, PT_NOT_READY, "render_tile requires a configured tracer with a scene");
+  if (!t->accel_valid) return fail(t, PT_NOT_READY, "render_tile requires build_or_update_acceleration");
+  if (count && !tiles) return fail(t, PT_INVALID_ARGUMENT, "render_tiles: null tiles");
+  (void)hipSetDevice(t->device);
+  const uint32_t W = t->settings.width, H = t->settings.height;
+  // Clamp tiles like render_tile_cancellable (ScalarCpuPathTracer.Render.cpp:172-180) and split
+  // any tile larger than one pass into row bands.
+  std::vector<ptd::DevTile> work;
+  work.reserve(count);
+  const uint32_t cap = std::max<uint32_t>(t->capacity, 1024u);
+  for (uint32_t i = 0; i < count; ++i) {
+    const PtRenderTile& tl = tiles[i];
+    const uint32_t x0 = std::min(tl.x, W), y0 = std::min(tl.y, H);
+    const uint32_t x1 = static_cast<uint32_t>(std::min<uint64_t>(W, static_cast<uint64_t>(x0) + tl.width));
+    const uint32_t y1 = static_cast<uint32_t>(std::min<uint64_t>(H, static_cast<uint64_t>(y0) + tl.height));
+    if (x0 >= x1 || y0 >= y1) continue;
+    const uint32_t w = x1 - x0;
+    if (w > cap) return fail(t, PT_INVALID_ARGUMENT, "tile wider than the wavefront capacity");
+    const uint32_t rows_per = std::max(1u, cap / w);
+    for (uint32_t y = y0; y < y1; y += rows_per) {
+      const uint32_t h = std::min(rows_per, y1 - y);
+      work.push_back(ptd::DevTile{x0, y, w, h, tl.sample_index, 0u});
+    }
+  }
+  ptd::DevSettings st{};
+  st.seed = t->settings.seed;
+  st.frame = frame_index;
+  st.width = W;
+  st.height = H;
+  st.max_depth = t->settings.max_depth;
+  st.enable_nee = t->settings.enable_nee;
+  st.enable_mis = t->settings.enable_mis;
+  st.rr_start = t->settings.rr_start_depth;
+  st.rr_min = t->settings.rr_min_survival;
+  st.rr_max = t->settings.rr_max_survival;
+  const ptd::DevFilm film{t->d_film_sum.ptr, t->d_film_count.ptr};
+
+  size_t next = 0;
+  while (next < work.size()) {
+    // Gather whole tiles into one pass.
+    size_t end = next;
+    uint32_t paths = 0;
+    while (end < work.size() && end - next < kMaxTilesPerPass) {
+      const uint32_t n = work[end].w * work[end].h;
+      if (end > next && static_cast<uint64_t>(paths) + n > cap) break;
+      work[end].path_begin = paths;
+      paths += n;
+      ++end;
+    }
+    const int rc = ensure_paths(t, std::max(paths, std::min(cap, 1u << 20)));
+    if (rc != PT_OK) return rc;
+    ptd::DevTileTable table{};
+    table.count = static_cast<uint32_t>(end - next);
+    for (size_t i = next; i < end; ++i) table.t[i - next] = work[i];
+    static_assert(kMaxTilesPerPass == ptd::kMaxTilesPerPass, "tile table size");
+    ptd::DevPaths& P = t->paths;
+    ptd::launch_init_queue(t->stream, P, paths);
+    timed(t, PT_KERNEL_RAYGEN, paths, [&] { ptd::launch_raygen(t->stream, P, t->dcam, st, table, paths); });
+    int buf = 0;
+    for (uint32_t depth = 0; depth < st.max_depth; ++depth) {
+      timed(t, PT_KERNEL_EXTEND, 0, [&] { ptd::launch_extend(t->stream, P, t->dscene, buf, t->grid_ext); });
+      timed(t, PT_KERNEL_SHADE, 0, [&] { ptd::launch_shade(t->stream, P, t->dscene, st, buf, depth, t->grid_shade); });
+      if (st.enable_nee && depth + 1u < st.max_depth)
+        timed(t, PT_KERNEL_SHADOW, 0, [&] { ptd::launch_shadow(t->stream, P, t->dscene, t->grid_ext); });
+      ptd::launch_advance(t->stream, P);
+      buf ^= 1;
+    }
+    for (size_t i = next; i < end; ++i) {
+      const ptd::DevTile& tile = work[i];
+      timed(t, PT_KERNEL_FILM, tile.w * tile.h, [&] { ptd::launch_film(t->stream, P, film, W, tile, tile.path_begin); });
+    }
+    const hipError_t le = hipGetLastError();
+    if (le != hipSuccess) return hip_fail(t, le, "kernel launch");
+    next = end;
+  }
+  for (uint32_t i = 0; i < count; ++i) t->current_sample = std::max(t->current_sample, tiles[i].sample_index + 1u);
+  if (t->profiling) {
+    const int rc = sync_all(t);
+    if (rc != PT_OK) return rc;
+  }
+  t->last_error.clear();
+  return PT_OK;
+}
+
+int pt_hip_render_tile(pt_tracer* t, const PtRenderTile* tile, uint64_t frame_index) {
+  if (!tile) return fail(t, PT_INVALID_ARGUMENT, "render_tile: null tile");
+  return pt_hip_render_tiles(t, tile, 1u, frame_index);
+}
+
+int pt_hip_synchronize(pt_tracer* t) {
+  if (!t) return PT_INVALID_ARGUMENT;
+  (void)hipSetDevice(t->device);
+  return sync_all(t);
+}
+
+int pt_hip_read_film(pt_tracer* t, float* rgb_sum, uint32_t* sample_count) {
+  if (!t) return PT_INVALID_ARGUMENT;
+  if (!t->configured) return fail(t, PT_NOT_READY, "read_film called before configure");
+  (void)hipSetDevice(t->device);
+  const size_t pixels = static_cast<size_t>(t->settings.width) * t->settings.height;
+  std::vector<float4> sum(rgb_sum ? pixels : 0);
+  if (rgb_sum && pixels)
+    PT_HIP(t, hipMemcpyAsync(sum.data(), t->d_film_sum.ptr, pixels * sizeof(float4), hipMemcpyDeviceToHost, t->stream),
+           "read film");
+  if (sample_count && pixels)
+    PT_HIP(t, hipMemcpyAsync(sample_count, t->d_film_count.ptr, pixels * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                             t->stream),
+           "read film");
+  const int rc = sync_all(t);
+  if (rc != PT_OK) return rc;
+  if (rgb_sum)
+    for (size_t i = 0; i < pixels; ++i) {
+      rgb_sum[3 * i] = sum[i].x;
+      rgb_sum[3 * i + 1] = sum[i].y;
+      rgb_sum[3 * i + 2] = sum[i].z;
+    }
+  return PT_OK;
+}
+
+int pt_hip_resolve_hdr(pt_tracer* t, float* rgb) {
+  if (!t || !rgb) return fail(t, PT_INVALID_ARGUMENT, "resolve_hdr: null argument");
+  const size_t pixels = static_cast<size_t>(t->settings.width) * t->settings.height;
+  std::vector<uint32_t> counts(pixels);
+  const int rc = pt_hip_read_film(t, rgb, counts.data());
+  if (rc != PT_OK) return rc;
+  // FilmBuffer::resolve_hdr, FilmBuffer.cpp:160-214
+  const bool uniform = std::all_of(counts.begin(), counts.end(), [&](uint32_t c) { return c == counts.front(); });
+  if (uniform && pixels) {
+    const float inv = 1.0f / static_cast<float>(std::max(1u, counts.front()));
+    if (inv != 1.0f)
+      for (size_t i = 0; i < 3 * pixels; ++i) rgb[i] = rgb[i] * inv;
+    return PT_OK;
+  }
+  for (size_t i = 0; i < pixels; ++i) {
+    const float inv = 1.0f / static_cast<float>(std::max(1u, counts[i]));
+    rgb[3 * i] = rgb[3 * i] * inv;
+    rgb[3 * i + 1] = rgb[3 * i + 1] * inv;
+    rgb[3 * i + 2] = rgb[3 * i + 2] * inv;
+  }
+  return PT_OK;
+}
+
+static void to_counters(const unsigned long long* c, PtSampleCounters* out) {
+  *out = PtSampleCounters{};
+  out->samples = c[ptd::kSamples];
+  out->rays = c[ptd::kRays];
+  out->triangle_tests = c[ptd::kTriTests];
+  out->triangle_hits = c[ptd::kTriHits];
+  out->bvh_node_visits = c[ptd::kNodeVisits];
+  out->bvh_leaf_visits = c[ptd::kLeafVisits];
+  out->shadow_tests = c[ptd::kShadowTests];
+}
+
+int pt_hip_read_counters(pt_tracer* t, PtSampleCounters* out) {
+  if (!t || !out) return fail(t, PT_INVALID_ARGUMENT, "read_counters: null argument");
+  (void)hipSetDevice(t->device);
+  unsigned long long c[ptd::kCounterSlots];
+  PT_HIP(t, hipMemcpyAsync(c, t->d_counters.ptr, sizeof(c), hipMemcpyDeviceToHost, t->stream), "read counters");
+  const int rc = sync_all(t);
+  if (rc != PT_OK) return rc;
+  to_counters(c, out);
+  return PT_OK;
+}
+
+int pt_hip_read_kernel_stats(pt_tracer* t, PtKernelStats* out) {
+  if (!t || !out) return fail(t, PT_INVALID_ARGUMENT, "read_kernel_stats: null argument");
+  (void)hipSetDevice(t->device);
+  unsigned long long c[ptd::kCounterSlots];
+  PT_HIP(t, hipMemcpyAsync(c, t->d_counters.ptr, sizeof(c), hipMemcpyDeviceToHost, t->stream), "read counters");
+  const int rc = sync_all(t);
+  if (rc != PT_OK) return rc;
+  *out = t->kstats;
+  out->inner_node_fetches = c[ptd::kInnerFetches];
+  out->shadow_node_visits = c[ptd::kShadowNodeVisits];
+  out->shadow_triangle_tests = c[ptd::kShadowTriTests];
+  out->shadow_inner_node_fetches = c[ptd::kShadowInnerFetches];
+  out->shadow_leaf_visits = c[ptd::kShadowLeafVisits];
+  out->shadow_rays = c[ptd::kShadowRays];
+  out->items[PT_KERNEL_EXTEND] = c[ptd::kRays];
+  out->items[PT_KERNEL_SHADOW] = c[ptd::kShadowRays];
+  out->items[PT_KERNEL_SHADE] = c[ptd::kRays];
+  return PT_OK;
+}
+
+int pt_hip_reset_kernel_stats(pt_tracer* t) {
+  if (!t) return PT_INVALID_ARGUMENT;
+  const int rc = pt_hip_synchronize(t);
+  t->kstats = PtKernelStats{};
+  return rc;
+}
+
+int pt_hip_set_profiling(pt_tracer* t, int enabled) {
+  if (!t) return PT_INVALID_ARGUMENT;
+  const int rc = pt_hip_synchronize(t);
+  t->profiling = enabled != 0;
+  return rc;
+}
+
+int pt_hip_set_wavefront_capacity(pt_tracer* t, uint32_t paths) {
+  if (!t) return PT_INVALID_ARGUMENT;
+  if (paths < 1024u) return fail(t, PT_INVALID_ARGUMENT, "wavefront capacity must be at least 1024 paths");
+  const int rc = pt_hip_synchronize(t);
+  if (rc != PT_OK) return rc;
+  t->capacity = paths;
+  if (t->path_capacity_alloc > paths) free_paths(t);
+  return PT_OK;
+}
+
+int pt_hip_status(pt_tracer* t, PtTracerStatus* out) {
+  if (!t || !out) return PT_INVALID_ARGUMENT;
+  *out = PtTracerStatus{};
+  out->lifecycle = !t->configured ? PT_LIFECYCLE_UNINITIALIZED
+                   : !t->scene_loaded ? PT_LIFECYCLE_CONFIGURED
+                   : !t->accel_valid ? PT_LIFECYCLE_SCENE_LOADED
+                                     : PT_LIFECYCLE_READY;
+  out->scene_loaded = t->scene_loaded;
+  out->accel_valid = t->accel_valid;
+  out->ready_to_render = t->configured && t->scene_loaded && t->accel_valid;
+  out->current_sample = t->current_sample;
+  out->accumulation_gen = t->accumulation_gen;
+  if (t->configured) {
+    PtSampleCounters c;
+    if (pt_hip_read_counters(t, &c) == PT_OK) out->total_samples = c.samples;
+  }
+  return PT_OK;
+}
+
+int pt_hip_shutdown(pt_tracer* t) {
+  if (!t) return PT_INVALID_ARGUMENT;
+  (void)hipSetDevice(t->device);
+  (void)hipStreamSynchronize(t->stream);
+  free_scene_device(t);
+  free_paths(t);
+  t->d_film_sum.release();
+  t->d_film_count.release();
+  t->configured = t->scene_loaded = t->accel_valid = false;
+  t->current_sample = 0;
+  t->accumulation_gen = 0;
+  t->last_error.clear();
+  return PT_OK;
+}
+
+int pt_hip_trace_rays(pt_tracer* t, int mode, const float* rays, const float* tmax, uint32_t count, float* out_t,
+                      int32_t* out_prim, PtSampleCounters* counters) {
+  if (!t) return PT_INVALID_ARGUMENT;
+  if (!t->accel_valid) return fail(t, PT_NOT_READY, "trace_rays requires build_or_update_acceleration");
+  if (count && (!rays || !out_t || !out_prim)) return fail(t, PT_INVALID_ARGUMENT, "trace_rays: null buffer");
+  if (mode != 0 && mode != 1) return fail(t, PT_INVALID_ARGUMENT, "trace_rays: mode must be 0 or 1");
+  (void)hipSetDevice(t->device);
+  ptd::DevPaths q = t->paths;
+  q.counters = t->d_query_counters.ptr;
+  ptd::launch_zero_counters(t->stream, q.counters);
+  ptd::launch_init_queue(t->stream, q, 0);
+  if (count)
+    ptd::launch_trace_rays(t->stream, q, t->dscene, mode, rays, tmax, count, out_t, out_prim, t->grid_ext);
+  const hipError_t le = hipGetLastError();
+  if (le != hipSuccess) return hip_fail(t, le, "kernel launch");
+  unsigned long long c[ptd::kCounterSlots];
+  PT_HIP(t, hipMemcpyAsync(c, q.counters, sizeof(c), hipMemcpyDeviceToHost, t->stream), "read counters");
+  const int rc = sync_all(t);
+  if (rc != PT_OK) return rc;
+  if (counters) to_counters(c, counters);
+  return PT_OK;
+}
+
+}  // extern "C"
+
+extern "C" int pt_bvh_build_host(const PtSceneView* scene, const PtBvhParams* params, uint32_t* node_count,
+                                 uint32_t* prim_count, uint32_t* max_depth, float* bounds, int32_t* meta,
+                                 uint32_t* prim_order, float* prim_geo) {
+  if (!scene) return PT_INVALID_ARGUMENT;
+  PtBvhParams p{PT_BVH_SAH, 4u, 8u, 0u};
+  if (params) p = *params;
+  if (p.leaf_size == 0) p.leaf_size = 4u;
+  if (p.bucket_count == 0) p.bucket_count = 8u;
+  if (p.leaf_size > ptb::kMaxLeafSize || (p.builder != PT_BVH_SAH && p.builder != PT_BVH_MIDPOINT))
+    return PT_INVALID_ARGUMENT;
+  ptb::Bvh bvh;
+  std::vector<float> raw;
+  std::string err;
+  if (!ptb::flatten(*scene, bvh.prims, raw, err)) return PT_INVALID_ARGUMENT;
+  if (p.builder == PT_BVH_SAH) ptb::build_sah(bvh, raw, p.leaf_size, p.bucket_count);
+  else ptb::build_midpoint(bvh, raw, p.leaf_size);
+  if (node_count) *node_count = static_cast<uint32_t>(bvh.nodes.size());
+  if (prim_count) *prim_count = static_cast<uint32_t>(bvh.prims.size());
+  if (max_depth) *max_depth = bvh.max_depth;
+  for (size_t i = 0; i < bvh.nodes.size(); ++i) {
+    const ptb::Node& n = bvh.nodes[i];
+    if (bounds)
+      for (int k = 0; k < 3; ++k) {
+        bounds[6 * i + k] = n.mn[k];
+        bounds[6 * i + 3 + k] = n.mx[k];
+      }
+    if (meta) {
+      meta[4 * i] = n.left;
+      meta[4 * i + 1] = n.right;
+      meta[4 * i + 2] = n.first;
+      meta[4 * i + 3] = n.count;
+    }
+  }
+  if (prim_order) std::copy(bvh.order.begin(), bvh.order.end(), prim_order);
+  if (prim_geo)
+    for (size_t i = 0; i < bvh.prims.size(); ++i) {
+      const ptb::Prim& q = bvh.prims[i];
+      float* g = prim_geo + 12 * i;
+      for (int k = 0; k < 3; ++k) {
+        g[k] = q.v0[k];
+        g[3 + k] = q.e1[k];
+        g[6 + k] = q.e2[k];
+        g[9 + k] = q.n[k];
+      }
+    }
+  return PT_OK;
+}

@@ -1,0 +1,135 @@
+// pt_device.h — device records shared by the wavefront kernels and the C ABI layer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "vkpt_hip.h"
+
+namespace ptd {
+
+constexpr int kBlock = 256;        // threads per workgroup (4 waves of 64)
+constexpr int kStackLds = 16;      // traversal stack entries per lane held in LDS
+constexpr int kCounterSlots = 16;  // device u64 counters (see CounterSlot)
+
+enum CounterSlot : int {
+  kSamples = 0,
+  kRays = 1,
+  kTriTests = 2,
+  kTriHits = 3,
+  kNodeVisits = 4,
+  kLeafVisits = 5,
+  kShadowTests = 6,
+  kInnerFetches = 7,
+  kShadowNodeVisits = 8,
+  kShadowTriTests = 9,
+  kShadowRays = 10,
+  kShadowInnerFetches = 11,
+  kShadowLeafVisits = 12,
+};
+
+// Queue counters (u32), one set per wavefront pass.
+enum QueueSlot : int {
+  kQueueIn = 0,       // live paths entering this bounce
+  kQueueOut = 1,      // paths appended for the next bounce
+  kQueueShadow = 2,   // shadow rays appended this bounce
+  kFetchExtend = 3,   // persistent-thread work cursor, extend
+  kFetchShadow = 4,   // persistent-thread work cursor, shadow
+  kQueueSlots = 8,
+};
+
+struct DevScene {
+  const float4* nodes;   // 4 x float4 per inner node
+  const float4* tris;    // 3 x float4 per leaf-ordered triangle
+  const float4* prims;   // normal.xyz + material bits, by primitive index
+  const PtMaterial* materials;
+  const PtLight* lights;
+  uint32_t material_count, light_count, prim_count, has_transmissive;
+  int32_t root_ref;
+  float root_box[6];
+  float env[3];
+};
+
+struct DevCamera {
+  float pos[3], fwd[3], right[3], up[3];
+  float aspect, tan_half_fov, aperture, focus;
+  float iris_roundness, iris_rotation, anamorphic;
+  uint32_t iris_blades;
+};
+
+struct DevSettings {
+  uint64_t seed;
+  uint64_t frame;
+  uint32_t width, height, max_depth, enable_nee, enable_mis, rr_start;
+  float rr_min, rr_max;
+};
+
+// Tile table entry for one wavefront pass: paths [path_begin, path_begin + w*h) belong to it.
+struct DevTile {
+  uint32_t x, y, w, h, sample, path_begin;
+};
+
+// The tiles of one pass travel as a kernel argument (no host->device copy per pass).
+constexpr uint32_t kMaxTilesPerPass = 96u;
+struct DevTileTable {
+  DevTile t[kMaxTilesPerPass];
+  uint32_t count;
+};
+
+// One ray queue: origin xyz, direction xyz (SoA) and the path slot each queue entry serves.
+struct DevRays {
+  float* c[6];
+  uint32_t* qpath;
+};
+
+// Per-path wavefront state. Ray and hit buffers are in queue order (double-buffered rays);
+// throughput, radiance and RNG state are indexed by the path's slot in the pass.
+struct DevPaths {
+  DevRays rays[2];
+  float* hit_t;
+  int32_t* hit_prim;
+  float* tp[3];          // throughput
+  float* rad[3];         // radiance
+  uint64_t* rng;
+  float* sray[6];        // shadow queue: origin xyz, direction xyz
+  float* stmax;
+  float* scontrib[3];
+  uint32_t* spath;
+  uint32_t* queue;               // QueueSlot counters
+  unsigned long long* counters;  // CounterSlot counters
+  uint32_t* error;               // nonzero: a kernel refused work (stack overflow area too small)
+  int32_t* stack_ref;            // traversal stack overflow beyond kStackLds
+  float* stack_t;
+  uint32_t stack_overflow_depth; // entries per lane in the overflow area
+  uint32_t stack_threads;        // lanes the overflow area is sized for (persistent grid * kBlock)
+  uint32_t capacity;
+};
+
+struct DevFilm {
+  float4* sum;       // rgb sum (w unused)
+  uint32_t* count;
+};
+
+// Launchers (pt_kernels.hip). All launch on `stream`; none synchronizes.
+void launch_init_queue(hipStream_t stream, const DevPaths& paths, uint32_t live);
+void launch_raygen(hipStream_t stream, const DevPaths& paths, const DevCamera& cam, const DevSettings& st,
+                   const DevTileTable& tiles, uint32_t path_count);
+void launch_extend(hipStream_t stream, const DevPaths& paths, const DevScene& scene, int in_buf, uint32_t grid);
+void launch_shade(hipStream_t stream, const DevPaths& paths, const DevScene& scene, const DevSettings& st,
+                  int in_buf, uint32_t depth, uint32_t grid);
+void launch_shadow(hipStream_t stream, const DevPaths& paths, const DevScene& scene, uint32_t grid);
+void launch_advance(hipStream_t stream, const DevPaths& paths);
+void launch_film(hipStream_t stream, const DevPaths& paths, const DevFilm& film, uint32_t film_width,
+                 const DevTile& tile, uint32_t path_offset_in_pass);
+void launch_trace_rays(hipStream_t stream, const DevPaths& paths, const DevScene& scene, int any_hit,
+                       const float* rays, const float* tmax, uint32_t count, float* out_t, int32_t* out_prim,
+                       uint32_t grid);
+void launch_zero_counters(hipStream_t stream, unsigned long long* counters);
+void launch_clear_film(hipStream_t stream, const DevFilm& film, uint32_t pixels);
+
+// Occupancy-derived persistent grid sizes.
+uint32_t persistent_grid_extend(int device);
+uint32_t persistent_grid_shade(int device);
+
+}  // namespace ptd

@@ -1,0 +1,925 @@
+// pt_kernels.hip — the wavefront path tracer for gfx950.
+//
+// One pass of the wavefront: raygen -> { extend -> shade -> shadow -> advance } x max_depth -> film.
+//   raygen  : camera rays and per-path RNG seeding          (ScalarCpuPathTracer.Render.cpp:55-119, :228-262)
+//   extend  : closest-hit BVH traversal, persistent waves  (CpuBvhAccelerator.cpp:360-458)
+//   shade   : one trace() iteration per live path          (ScalarCpuPathTracer.cpp:1148-1289)
+//   shadow  : NEE visibility, any-hit traversal            (ScalarCpuPathTracer.cpp:1066-1087)
+//   film    : per-pixel accumulation in tile order          (FilmBuffer.cpp:80-91)
+//
+// Every float expression is written in the operand order of the reference and the
+// library is built with -ffp-contract=off and correctly rounded fp32 div/sqrt, so a
+// path makes exactly the decisions the CPU restatement makes.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "pt_device.h"
+#include "pt_math.h"
+
+namespace ptd {
+namespace {
+
+constexpr float kEps = 1e-4f;                       // ScalarCpuPathTracer.cpp:16
+constexpr float kPi = 3.14159265358979323846f;     // :14
+constexpr float kInvPi = 0.31830988618f;           // :15
+constexpr float kInf = __builtin_huge_valf();
+
+struct V3 {
+  float x, y, z;
+};
+__device__ __forceinline__ V3 mk(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ V3 add(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 neg(V3 a) { return {-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ V3 muls(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ V3 mulv(V3 a, V3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ V3 divs(V3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float smin(float a, float b) { return (b < a) ? b : a; }  // std::min
+__device__ __forceinline__ float smax(float a, float b) { return (a < b) ? b : a; }  // std::max
+__device__ __forceinline__ float sclamp(float v, float lo, float hi) { return (v < lo) ? lo : ((hi < v) ? hi : v); }
+__device__ __forceinline__ bool finite(float v) { return __builtin_isfinite(v); }
+__device__ __forceinline__ float clamp01(float v) { return finite(v) ? smin(1.0f, smax(0.0f, v)) : 0.0f; }
+__device__ __forceinline__ float vmax3(V3 a) { return smax(a.x, smax(a.y, a.z)); }
+__device__ __forceinline__ V3 ld3(const float* p) { return {p[0], p[1], p[2]}; }
+
+// normalize(), ScalarCpuPathTracer.cpp:104-110
+__device__ __forceinline__ V3 normalize(V3 a) {
+  const float l = __builtin_sqrtf(dot(a, a));
+  if (l <= kEps) return mk(0.0f, 1.0f, 0.0f);
+  return divs(a, l);
+}
+
+// splitmix64 / Rng, ScalarCpuPathTracer.cpp:36-41, :383-393
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+  x += 0x9e3779b97f4a7c15ULL;
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+  return x ^ (x >> 31);
+}
+__device__ __forceinline__ uint64_t rng_seed(uint64_t seed, uint64_t frame, uint64_t sample, uint64_t path_id,
+                                             uint64_t pixel) {
+  // dimension = 0, path_depth = 0 for both streams the path tracer creates
+  const uint64_t z = splitmix64(0);
+  const uint64_t mix = splitmix64(seed ^ splitmix64(frame) ^ splitmix64(sample) ^ splitmix64(path_id) ^ z ^ z);
+  return splitmix64(mix + splitmix64(pixel));
+}
+__device__ __forceinline__ float next01(uint64_t& s) {
+  s = splitmix64(s);
+  return static_cast<float>(static_cast<uint32_t>((s >> 8) & 0x00ffffffu)) * (1.0f / 16777215.0f);
+}
+
+// ------------------------------------------------------------------ traversal
+struct RayQ {
+  float o[3], inv[3];
+  bool par[3];
+};
+
+__device__ __forceinline__ RayQ make_query(V3 o, V3 d) {
+  RayQ q;
+  q.o[0] = o.x; q.o[1] = o.y; q.o[2] = o.z;
+  const float dd[3] = {d.x, d.y, d.z};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    q.par[k] = __builtin_fabsf(dd[k]) <= kEps;
+    q.inv[k] = q.par[k] ? 0.0f : 1.0f / dd[k];
+  }
+  return q;
+}
+
+// intersect_aabb, CpuBvhAccelerator.cpp:125-154 (the early returns do not change the result).
+__device__ __forceinline__ bool slab(const RayQ& q, float mnx, float mny, float mnz, float mxx, float mxy,
+                                     float mxz, float max_t, float& t_near) {
+  const float mn[3] = {mnx, mny, mnz}, mx[3] = {mxx, mxy, mxz};
+  float tmin = kEps, tmax = max_t;
+  bool outside = false;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    if (q.par[k]) {
+      outside = outside || q.o[k] < mn[k] || q.o[k] > mx[k];
+    } else {
+      float t0 = (mn[k] - q.o[k]) * q.inv[k];
+      float t1 = (mx[k] - q.o[k]) * q.inv[k];
+      const float lo = (t0 > t1) ? t1 : t0;
+      const float hi = (t0 > t1) ? t0 : t1;
+      tmin = smax(tmin, lo);
+      tmax = smin(tmax, hi);
+    }
+  }
+  t_near = tmin;
+  return !outside && !(tmin > tmax) && tmax > kEps;
+}
+
+struct TravStats {
+  uint32_t visits, leaves, tests, hits, inner;
+};
+
+// Moller-Trumbore against precomputed edges, intersect_scalar (BvhTriangleIntersector.cpp:35-74).
+// True when eps < t < best_t.
+__device__ __forceinline__ bool tri_test(const float4* tp, V3 o, V3 d, float best_t, float& t_out) {
+  const float4 v0 = tp[0], e1 = tp[1], e2 = tp[2];
+  const float hx = d.y * e2.z - d.z * e2.y;
+  const float hy = d.z * e2.x - d.x * e2.z;
+  const float hz = d.x * e2.y - d.y * e2.x;
+  const float det = e1.x * hx + e1.y * hy + e1.z * hz;
+  if (__builtin_fabsf(det) < kEps) return false;
+  const float inv_det = 1.0f / det;
+  const float sx = o.x - v0.x, sy = o.y - v0.y, sz = o.z - v0.z;
+  const float u = (sx * hx + sy * hy + sz * hz) * inv_det;
+  if (u < 0.0f || u > 1.0f) return false;
+  const float qx = sy * e1.z - sz * e1.y;
+  const float qy = sz * e1.x - sx * e1.z;
+  const float qz = sx * e1.y - sy * e1.x;
+  const float v = (d.x * qx + d.y * qy + d.z * qz) * inv_det;
+  if (v < 0.0f || u + v > 1.0f) return false;
+  const float t = (e2.x * qx + e2.y * qy + e2.z * qz) * inv_det;
+  t_out = t;
+  return !(t <= kEps || t >= best_t);
+}
+
+struct Stack {
+  int32_t* lds_ref;
+  float* lds_t;
+  int32_t* ovf_ref;
+  float* ovf_t;
+  uint32_t ovf_depth, ovf_stride, lane_slot, gid;
+  uint32_t* error;
+};
+
+// Closest hit (kAny = false) or first leaf holding an accepted hit below best_t (kAny = true),
+// following CpuBvhAccelerator::intersect's visit order: children tested at the parent,
+// the nearer pushed last (left on ties), every pop re-tested against the current best t.
+// A popped entry carries the t_near of its box; t_near <= best_t is the same test as the
+// reference's re-run of intersect_aabb with max_t = best_t.
+template <bool kAny>
+__device__ __forceinline__ void traverse(const DevScene& sc, const Stack& stk, V3 o, V3 d, float& best_t,
+                                         int32_t& best_prim, TravStats& st) {
+  best_prim = -1;
+  if (sc.prim_count == 0) return;
+  const RayQ q = make_query(o, d);
+  float tn;
+  ++st.visits;
+  if (!slab(q, sc.root_box[0], sc.root_box[1], sc.root_box[2], sc.root_box[3], sc.root_box[4], sc.root_box[5],
+            best_t, tn))
+    return;
+  int32_t cur = sc.root_ref;
+  uint32_t sp = 0;
+  while (true) {
+    if (cur >= 0) {
+      ++st.inner;
+      const float4* np = sc.nodes + 4u * static_cast<uint32_t>(cur);
+      const float4 a = np[0], b = np[1], c = np[2];
+      const int4 r = *reinterpret_cast<const int4*>(np + 3);
+      float t0, t1;
+      const bool h0 = slab(q, a.x, a.y, a.z, a.w, b.x, b.y, best_t, t0);
+      const bool h1 = slab(q, b.z, b.w, c.x, c.y, c.z, c.w, best_t, t1);
+      if (h0 && h1) {
+        const bool left_first = t0 <= t1;
+        const int32_t near_ref = left_first ? r.x : r.y;
+        const int32_t far_ref = left_first ? r.y : r.x;
+        const float far_t = left_first ? t1 : t0;
+        if (sp < static_cast<uint32_t>(kStackLds)) {
+          stk.lds_ref[sp * kBlock + stk.lane_slot] = far_ref;
+          stk.lds_t[sp * kBlock + stk.lane_slot] = far_t;
+        } else if (sp - kStackLds < stk.ovf_depth) {
+          const size_t o2 = static_cast<size_t>(sp - kStackLds) * stk.ovf_stride + stk.gid;
+          stk.ovf_ref[o2] = far_ref;
+          stk.ovf_t[o2] = far_t;
+        } else {
+          atomicOr(stk.error, 1u);  // overflow area smaller than the tree depth: refuse, never write out of bounds
+          return;
+        }
+        ++sp;
+        cur = near_ref;
+        ++st.visits;
+        continue;
+      }
+      if (h0 || h1) {
+        cur = h0 ? r.x : r.y;
+        ++st.visits;
+        continue;
+      }
+    } else {
+      ++st.leaves;
+      const uint32_t code = ~static_cast<uint32_t>(cur);
+      const uint32_t first = code >> 5, count = code & 31u;
+      uint32_t accepted = 0;
+      for (uint32_t k = 0; k < count; ++k) {
+        ++st.tests;
+        float t;
+        const float4* tp = sc.tris + 3u * (first + k);
+        if (tri_test(tp, o, d, best_t, t)) {  // commit_ordered_lane_hit, BvhTriangleIntersector.cpp:25-33
+          ++accepted;
+          best_t = t;
+          best_prim = __float_as_int(tp[0].w);
+        }
+        // The reference tests leaves in batches of 8 (kBvhTriangleBatchWidth); an any-hit
+        // query stops at the end of the first batch that accepted a hit.
+        if (kAny && accepted && ((k & 7u) == 7u || k + 1u == count)) {
+          st.hits += accepted;
+          return;
+        }
+      }
+      st.hits += accepted;
+    }
+    bool found = false;
+    while (sp > 0) {
+      --sp;
+      int32_t ref;
+      float t;
+      if (sp < static_cast<uint32_t>(kStackLds)) {
+        ref = stk.lds_ref[sp * kBlock + stk.lane_slot];
+        t = stk.lds_t[sp * kBlock + stk.lane_slot];
+      } else {
+        const size_t o2 = static_cast<size_t>(sp - kStackLds) * stk.ovf_stride + stk.gid;
+        ref = stk.ovf_ref[o2];
+        t = stk.ovf_t[o2];
+      }
+      ++st.visits;
+      if (t <= best_t) {
+        cur = ref;
+        found = true;
+        break;
+      }
+    }
+    if (!found) break;
+  }
+}
+
+__device__ __forceinline__ void wave_add(unsigned long long* dst, uint64_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(static_cast<unsigned long long>(v), off);
+  if ((threadIdx.x & 63u) == 0u && v != 0u) atomicAdd(dst, static_cast<unsigned long long>(v));
+}
+
+// Shadow traversals add to the same SampleCounters fields as extension rays, as the
+// reference's shadow query does (ScalarCpuPathTracer.cpp:1071); the shadow-only slots
+// keep their share for the byte accounting.
+__device__ __forceinline__ void flush_stats(unsigned long long* counters, const TravStats& s, bool shadow) {
+  if (shadow) {
+    wave_add(counters + kShadowNodeVisits, s.visits);
+    wave_add(counters + kShadowTriTests, s.tests);
+    wave_add(counters + kShadowInnerFetches, s.inner);
+    wave_add(counters + kShadowLeafVisits, s.leaves);
+  }
+  wave_add(counters + kNodeVisits, s.visits);
+  wave_add(counters + kLeafVisits, s.leaves);
+  wave_add(counters + kTriTests, s.tests);
+  wave_add(counters + kTriHits, s.hits);
+  wave_add(counters + kInnerFetches, s.inner);
+}
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+}
+
+// Wave-wide append: returns this lane's slot (valid when pred is true).
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
+  const uint64_t m = __ballot(pred);
+  uint32_t base = 0;
+  if (m != 0u) {
+    const uint32_t leader = static_cast<uint32_t>(__builtin_ctzll(m));
+    if ((threadIdx.x & 63u) == leader) base = atomicAdd(counter, static_cast<uint32_t>(__popcll(m)));
+    base = __shfl(base, static_cast<int>(leader));
+  }
+  return base + lanes_below(m);
+}
+
+// ------------------------------------------------------------------ kernels
+__global__ void k_init_queue(uint32_t* queue, uint32_t live) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    queue[kQueueIn] = live;
+    queue[kQueueOut] = 0;
+    queue[kQueueShadow] = 0;
+    queue[kFetchExtend] = 0;
+    queue[kFetchShadow] = 0;
+  }
+}
+
+__global__ void k_advance(uint32_t* queue) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    queue[kQueueIn] = queue[kQueueOut];
+    queue[kQueueOut] = 0;
+    queue[kQueueShadow] = 0;
+    queue[kFetchExtend] = 0;
+    queue[kFetchShadow] = 0;
+  }
+}
+
+__global__ void k_zero_counters(unsigned long long* c) {
+  if (threadIdx.x < kCounterSlots && blockIdx.x == 0) c[threadIdx.x] = 0ull;
+}
+
+__global__ __launch_bounds__(kBlock) void k_clear_film(DevFilm film, uint32_t pixels) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < pixels) {
+    film.sum[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    film.count[i] = 0u;
+  }
+}
+
+// camera_rays + shade_pixel seeding, ScalarCpuPathTracer.Render.cpp:55-119, :243-262
+__global__ __launch_bounds__(kBlock) void k_raygen(DevPaths P, DevRays out, DevCamera cam, DevSettings st,
+                                                  DevTileTable tiles, uint32_t path_count) {
+  const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= path_count) return;
+  uint32_t lo = 0, hi = tiles.count;  // last tile with path_begin <= p
+  while (hi - lo > 1u) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (tiles.t[mid].path_begin <= p) lo = mid; else hi = mid;
+  }
+  const DevTile tile = tiles.t[lo];
+  const uint32_t local = p - tile.path_begin;
+  const uint32_t x = tile.x + local % tile.w;
+  const uint32_t y = tile.y + local / tile.w;
+  const uint32_t sample = tile.sample;
+
+  const uint64_t sample_seed = (static_cast<uint64_t>(y) << 32) | x;
+  const uint64_t path_id = sample_seed + static_cast<uint64_t>(sample) + st.frame;
+  uint64_t crng = rng_seed(st.seed, st.frame, sample, static_cast<uint32_t>(path_id),
+                           static_cast<uint64_t>(y) * st.width + x);
+  const float fx = (static_cast<float>(x) + next01(crng)) / static_cast<float>(st.width > 1u ? st.width : 1u);
+  const float fy = (static_cast<float>(y) + next01(crng)) / static_cast<float>(st.height > 1u ? st.height : 1u);
+  const uint64_t seed_mix = static_cast<uint64_t>(next01(crng) * static_cast<float>(0xffffu));
+  const float nx = (2.0f * fx - 1.0f) * cam.aspect * cam.tan_half_fov;
+  const float ny = (1.0f - 2.0f * fy) * cam.tan_half_fov;
+  const V3 F = ld3(cam.fwd), R = ld3(cam.right), U = ld3(cam.up), C = ld3(cam.pos);
+  const V3 dir = normalize(add(add(F, muls(R, nx)), muls(U, ny)));
+  V3 ro = C, rd = dir;
+  if (cam.aperture > 0.0f && cam.focus > kEps) {
+    const float lens_radius_sample = __builtin_sqrtf(next01(crng));
+    const float lens_phi = 2.0f * kPi * next01(crng);
+    float boundary = 1.0f;
+    if (cam.iris_blades >= 3u && cam.iris_roundness < 0.999f) {
+      const float sector = 2.0f * kPi / static_cast<float>(cam.iris_blades);
+      const float local_phi = lens_phi - cam.iris_rotation;
+      const float wrapped = local_phi - sector * __builtin_floorf(local_phi / sector);
+      const float centered = wrapped > sector * 0.5f ? wrapped - sector : wrapped;
+      const float polygon = ptm::cosf_p(sector * 0.5f) / smax(0.1f, ptm::cosf_p(centered));
+      boundary = polygon * (1.0f - cam.iris_roundness) + cam.iris_roundness;
+    }
+    const float lens_r = cam.aperture * lens_radius_sample * boundary;
+    float sp, cp;
+    ptm::sincosf_p(lens_phi, sp, cp);
+    const V3 offset = add(muls(R, lens_r * cp * cam.anamorphic), muls(U, lens_r * sp));
+    const V3 focus_point = add(C, muls(dir, cam.focus));
+    ro = add(C, offset);
+    rd = normalize(sub(focus_point, ro));
+  }
+  const uint64_t prng = rng_seed(st.seed + sample_seed + seed_mix, st.frame, sample, path_id, sample_seed);
+  out.c[0][p] = ro.x; out.c[1][p] = ro.y; out.c[2][p] = ro.z;
+  out.c[3][p] = rd.x; out.c[4][p] = rd.y; out.c[5][p] = rd.z;
+  out.qpath[p] = p;
+  P.tp[0][p] = 1.0f; P.tp[1][p] = 1.0f; P.tp[2][p] = 1.0f;
+  P.rad[0][p] = 0.0f; P.rad[1][p] = 0.0f; P.rad[2][p] = 0.0f;
+  P.rng[p] = prng;
+}
+
+__device__ __forceinline__ Stack make_stack(const DevPaths& P, int32_t* s_ref, float* s_t) {
+  Stack s;
+  s.lds_ref = s_ref;
+  s.lds_t = s_t;
+  s.ovf_ref = P.stack_ref;
+  s.ovf_t = P.stack_t;
+  s.ovf_depth = P.stack_overflow_depth;
+  s.ovf_stride = P.stack_threads;
+  s.lane_slot = threadIdx.x;
+  s.gid = blockIdx.x * kBlock + threadIdx.x;
+  s.error = P.error;
+  return s;
+}
+
+// Persistent closest-hit kernel: each wave takes 64 queue entries at a time.
+__global__ __launch_bounds__(kBlock) void k_extend(DevPaths P, DevRays in, DevScene sc) {
+  __shared__ int32_t s_ref[kStackLds * kBlock];
+  __shared__ float s_t[kStackLds * kBlock];
+  const Stack stk = make_stack(P, s_ref, s_t);
+  const uint32_t n = P.queue[kQueueIn];
+  const uint32_t lane = threadIdx.x & 63u;
+  TravStats st = {0, 0, 0, 0, 0};
+  uint32_t rays = 0;
+  while (true) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(&P.queue[kFetchExtend], 64u);
+    base = __shfl(base, 0);
+    if (base >= n) break;
+    const uint32_t i = base + lane;
+    if (i < n) {
+      const V3 o = mk(in.c[0][i], in.c[1][i], in.c[2][i]);
+      const V3 d = mk(in.c[3][i], in.c[4][i], in.c[5][i]);
+      float bt = kInf;
+      int32_t bp;
+      traverse<false>(sc, stk, o, d, bt, bp, st);
+      P.hit_t[i] = bt;
+      P.hit_prim[i] = bp;
+      ++rays;
+    }
+  }
+  flush_stats(P.counters, st, false);
+  wave_add(P.counters + kRays, rays);
+}
+
+struct MatFlags {
+  bool emissive_model, mirror, metallic, transmissive, diffuse, sheen, clearcoat, toon;
+};
+
+__device__ __forceinline__ const PtMaterial& material_at(const DevScene& sc, uint32_t idx) {
+  const uint32_t last = sc.material_count - 1u;
+  return sc.materials[idx < last ? idx : last];
+}
+
+__device__ __forceinline__ V3 material_albedo(const PtMaterial& m) {
+  return mk(smin(1.5f, smax(0.0f, m.albedo[0])), smin(1.5f, smax(0.0f, m.albedo[1])),
+            smin(1.5f, smax(0.0f, m.albedo[2])));
+}
+
+// spot_attenuation, ScalarCpuPathTracer.cpp:168-185
+__device__ __forceinline__ float spot_attenuation(const PtLight& l, V3 from_light_dir) {
+  if (l.spot_inner_cos <= -0.999f) return 1.0f;
+  const float cone = dot(normalize(from_light_dir), normalize(ld3(l.direction)));
+  if (cone <= l.spot_outer_cos) return 0.0f;
+  if (cone >= l.spot_inner_cos || l.spot_inner_cos <= l.spot_outer_cos) return 1.0f;
+  const float t = sclamp((cone - l.spot_outer_cos) / (l.spot_inner_cos - l.spot_outer_cos), 0.0f, 1.0f);
+  return t * t * (3.0f - 2.0f * t);
+}
+
+__device__ __forceinline__ float mis_weight(float a, float b) {
+  const float a2 = a * a, b2 = b * b;
+  const float denom = a2 + b2;
+  if (denom <= 0.0f) return 1.0f;
+  return a2 / denom;
+}
+
+// light_pdf_for_direction, ScalarCpuPathTracer.cpp:1120-1138
+__device__ __forceinline__ float light_pdf_for_direction(const DevScene& sc, V3 position, V3 direction) {
+  if (sc.light_count == 0) return 0.0f;
+  const float inv_n = 1.0f / static_cast<float>(sc.light_count);
+  float total = 0.0f;
+  for (uint32_t i = 0; i < sc.light_count; ++i) {
+    const V3 to_light = sub(ld3(sc.lights[i].position), position);
+    const float dist = __builtin_sqrtf(dot(to_light, to_light));
+    if (dist < kEps) continue;
+    const V3 dir = divs(to_light, dist);
+    if (dot(dir, direction) > 0.9999f) total += inv_n / (dist * dist + kEps);
+  }
+  return total;
+}
+
+// sample_hemisphere, ScalarCpuPathTracer.cpp:978-988
+__device__ __forceinline__ V3 sample_hemisphere(uint64_t& rng, V3 n) {
+  const float u1 = next01(rng);
+  const float u2 = next01(rng);
+  const float r = __builtin_sqrtf(smax(0.0f, 1.0f - u1));
+  const float phi = 2.0f * kPi * u2;
+  float sp, cp;
+  ptm::sincosf_p(phi, sp, cp);
+  const V3 local = mk(r * cp, r * sp, __builtin_sqrtf(smax(0.0f, u1)));
+  V3 tangent = cross((__builtin_fabsf(n.z) < 0.999f) ? mk(0.0f, 0.0f, 1.0f) : mk(0.0f, 1.0f, 0.0f), n);
+  tangent = normalize(tangent);
+  const V3 bitangent = cross(n, tangent);
+  return normalize(add(add(muls(tangent, local.x), muls(bitangent, local.y)), muls(n, local.z)));
+}
+
+// sample_phong_lobe, ScalarCpuPathTracer.cpp:996-1020
+__device__ __forceinline__ V3 sample_phong_lobe(uint64_t& rng, V3 refl, float exponent, V3 n) {
+  const float u1 = next01(rng);
+  const float u2 = next01(rng);
+  const float cosT = ptm::powf_p(smax(0.0f, u1), 1.0f / (exponent + 1.0f));
+  const float sinT = __builtin_sqrtf(smax(0.0f, 1.0f - cosT * cosT));
+  const float phi = 2.0f * kPi * u2;
+  float sp, cp;
+  ptm::sincosf_p(phi, sp, cp);
+  const V3 ref_t = (__builtin_fabsf(refl.z) < 0.999f) ? mk(0.0f, 0.0f, 1.0f) : mk(0.0f, 1.0f, 0.0f);
+  const V3 tang = normalize(cross(ref_t, refl));
+  const V3 btan = cross(refl, tang);
+  V3 out = normalize(add(add(muls(tang, sinT * cp), muls(btan, sinT * sp)), muls(refl, cosT)));
+  if (dot(out, n) <= 0.0f) {
+    out = sub(out, muls(n, 2.0f * dot(out, n)));
+    if (dot(out, n) <= 0.0f) return sample_hemisphere(rng, n);
+  }
+  return out;
+}
+
+__device__ __forceinline__ V3 reflect_about(V3 d, V3 n) { return sub(d, muls(n, 2.0f * dot(n, d))); }
+
+__device__ __forceinline__ float schlick_fresnel(float cosTheta, float ior) {
+  const float safeIor = smax(1.01f, ior);
+  float r0 = (1.0f - safeIor) / (1.0f + safeIor);
+  r0 *= r0;
+  const float m = 1.0f - smin(1.0f, smax(0.0f, cosTheta));
+  return r0 + (1.0f - r0) * m * m * m * m * m;
+}
+
+__device__ __forceinline__ V3 refract_dir(V3 incoming, V3 normal, float eta, bool& tir) {
+  const float cosi = smin(1.0f, smax(0.0f, dot(neg(incoming), normal)));
+  const float k = 1.0f - eta * eta * (1.0f - cosi * cosi);
+  if (k < 0.0f) {
+    tir = true;
+    return mk(0.0f, 0.0f, 0.0f);
+  }
+  tir = false;
+  return normalize(add(muls(incoming, eta), muls(normal, eta * cosi - __builtin_sqrtf(k))));
+}
+
+__device__ __forceinline__ float pow5(float v) {
+  const float v2 = v * v;
+  return v2 * v2 * v;
+}
+
+struct ShadeOut {
+  bool cont, shadow;
+  V3 no, nd;          // next extension ray
+  V3 so, sd, contrib; // shadow ray and its radiance contribution if unoccluded
+  float stmax;
+};
+
+// One iteration of ScalarCpuPathTracer::trace (ScalarCpuPathTracer.cpp:1159-1287) for a path that
+// arrived at a hit (prim >= 0) or a miss (prim < 0).
+__device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc, const DevSettings& st, uint32_t depth,
+                                           uint32_t p, V3 ro, V3 rd, float t, int32_t prim, ShadeOut& out,
+                                           uint32_t& shadow_tests) {
+  out.cont = false;
+  out.shadow = false;
+  V3 T = mk(P.tp[0][p], P.tp[1][p], P.tp[2][p]);
+  V3 L = mk(P.rad[0][p], P.rad[1][p], P.rad[2][p]);
+  if (prim < 0) {
+    L = add(L, mulv(T, ld3(sc.env)));
+    P.rad[0][p] = L.x; P.rad[1][p] = L.y; P.rad[2][p] = L.z;
+    return;
+  }
+  uint64_t rng = P.rng[p];
+  const V3 pos = add(ro, muls(rd, t));
+  const float4 pn = sc.prims[prim];
+  const V3 gN = mk(pn.x, pn.y, pn.z);
+  const bool entering = dot(rd, gN) < 0.0f;
+  const V3 sN = entering ? gN : neg(gN);
+  const PtMaterial& m = material_at(sc, static_cast<uint32_t>(__float_as_int(pn.w)));
+  const V3 albedo = material_albedo(m);
+  const float roughness = clamp01(m.roughness);
+  MatFlags f;
+  f.emissive_model = m.material_model == 1u || m.emissive[0] > 0.0f || m.emissive[1] > 0.0f || m.emissive[2] > 0.0f;
+  f.mirror = m.material_model == 2u || roughness <= 0.001f;
+  f.metallic = m.material_model == 4u || m.metallic > 0.65f;
+  f.transmissive = m.material_model == 5u || m.transmission > 0.05f;
+  f.diffuse = roughness >= 0.999f && !f.mirror && !f.metallic && !f.transmissive;
+  f.sheen = m.material_model == 6u;
+  f.clearcoat = m.material_model == 7u || m.clearcoat > 0.05f;
+  f.toon = m.material_model == 8u;
+  const V3 emissive = ld3(m.emissive);
+
+  if (!st.enable_nee || depth == 0u) {
+    L = add(L, mulv(T, emissive));
+  } else if (st.enable_mis && f.emissive_model) {
+    const float bsdf_pdf = smax(0.0f, dot(sN, neg(rd))) * kInvPi;
+    const float l_pdf = light_pdf_for_direction(sc, ro, rd);
+    const float w = mis_weight(bsdf_pdf, l_pdf);
+    L = add(L, muls(mulv(T, emissive), w));
+  }
+
+  const bool last = depth + 1u >= st.max_depth;
+  if (st.enable_nee && !last && sc.light_count > 0u) {
+    // sample_direct_light, ScalarCpuPathTracer.cpp:1022-1118
+    const uint32_t nl = sc.light_count;
+    const uint64_t li = static_cast<uint64_t>(next01(rng) * static_cast<float>(nl));
+    const PtLight& light = sc.lights[li < nl - 1u ? li : nl - 1u];
+    V3 lp = ld3(light.position);
+    if (light.radius > 1.0e-5f) {
+      const float uz = 1.0f - 2.0f * next01(rng);
+      const float upr = __builtin_sqrtf(smax(0.0f, 1.0f - uz * uz));
+      const float phi = 6.28318530718f * next01(rng);
+      float sp, cp;
+      ptm::sincosf_p(phi, sp, cp);
+      lp = add(lp, muls(mk(upr * cp, uz, upr * sp), light.radius));
+    }
+    const V3 to_light = sub(lp, pos);
+    const float dist_sq = dot(to_light, to_light);
+    const float dist = __builtin_sqrtf(dist_sq);
+    if (!(dist < kEps)) {
+      const V3 light_dir = divs(to_light, dist);
+      const float cos_theta = dot(sN, light_dir);
+      if (cos_theta > 0.0f) {
+        const float spot = spot_attenuation(light, neg(light_dir));
+        if (spot > 0.0f) {
+          ++shadow_tests;
+          const V3 view_dir = neg(rd);
+          const V3 irradiance = muls(ld3(light.color), (light.intensity * spot) / (dist_sq + kEps));
+          const float nlf = static_cast<float>(nl);
+          V3 direct = mk(0.0f, 0.0f, 0.0f);
+          if (!f.mirror && !f.transmissive)
+            direct = add(direct, muls(muls(mulv(muls(albedo, kInvPi), irradiance), cos_theta), nlf));
+          if (f.mirror || f.metallic || f.clearcoat || f.transmissive || roughness < 0.65f) {
+            const V3 half_dir = normalize(add(light_dir, normalize(view_dir)));
+            const float effR = smax(0.025f, roughness * (f.metallic ? 0.65f : 1.0f));
+            const float a2 = effR * effR;
+            const float specPower = sclamp(2.0f / smax(0.0005f, a2 * a2) - 2.0f, 4.0f, 96.0f);
+            const float spec = ptm::powf_p(smax(0.0f, dot(sN, half_dir)), specPower);
+            const float cosView = smax(0.0f, dot(sN, normalize(view_dir)));
+            float f0 = (1.0f - m.ior) / (1.0f + m.ior);
+            f0 *= f0;
+            const float fresnel = f0 + (1.0f - f0) * pow5(1.0f - cosView);
+            const float specStrength = clamp01((1.0f - roughness) * 0.8f + m.metallic * 0.45f + m.clearcoat * 0.35f +
+                                               (f.mirror ? 0.75f : 0.0f) + (f.transmissive ? fresnel : 0.0f));
+            const V3 specTint = add(muls(mk(1.0f, 1.0f, 1.0f), f.metallic ? 0.15f : 0.88f),
+                                    muls(albedo, f.metallic ? 0.85f : 0.12f));
+            direct = add(direct, muls(mulv(specTint, irradiance), spec * cos_theta * nlf * smax(0.15f, specStrength)));
+          }
+          V3 contrib;
+          if (st.enable_mis) {
+            const V3 l_dir = normalize(sub(ld3(sc.lights[0].position), pos));
+            const float ct = smax(0.0f, dot(sN, l_dir));
+            const float bsdf_pdf = ct * kInvPi;
+            const float l_pdf = 1.0f / nlf;
+            contrib = muls(mulv(T, direct), mis_weight(l_pdf, bsdf_pdf));
+          } else {
+            contrib = mulv(T, direct);
+          }
+          const float remaining = dist - 0.004f;
+          if (remaining > 0.004f) {
+            out.shadow = true;
+            out.so = add(pos, muls(sN, 0.002f));
+            out.sd = light_dir;
+            out.stmax = remaining;
+            out.contrib = contrib;
+          } else {
+            L = add(L, contrib);
+          }
+        }
+      }
+    }
+  }
+  P.rad[0][p] = L.x; P.rad[1][p] = L.y; P.rad[2][p] = L.z;
+  if (last) return;  // the loop ends after this iteration; nothing below reaches the radiance
+
+  V3 outDir;
+  bool refracted = false;
+  if (f.mirror) {
+    outDir = reflect_about(rd, sN);
+  } else if (f.transmissive) {
+    const float cosTheta = smax(0.0f, dot(sN, neg(rd)));
+    const float fresnel = schlick_fresnel(cosTheta, m.ior);
+    if (next01(rng) < smin(0.98f, fresnel + m.clearcoat * 0.015f)) {
+      outDir = reflect_about(rd, sN);
+    } else {
+      bool tir = false;
+      const float safeIor = smax(1.01f, m.ior);
+      const float eta = entering ? (1.0f / safeIor) : safeIor;
+      outDir = refract_dir(rd, sN, eta, tir);
+      if (tir || dot(outDir, outDir) <= 1.0e-8f) outDir = reflect_about(rd, sN);
+      else refracted = true;
+    }
+  } else if (f.diffuse || f.toon) {
+    outDir = sample_hemisphere(rng, sN);
+  } else {
+    const float effR = smax(0.025f, roughness * (f.metallic ? 0.75f : 1.0f) * (f.clearcoat ? 0.65f : 1.0f));
+    const float a2 = effR * effR;
+    const float expt = smax(0.0f, 2.0f / (a2 * a2) - 2.0f);
+    outDir = sample_phong_lobe(rng, reflect_about(rd, sN), expt, sN);
+    if (f.sheen && next01(rng) < m.sheen * 0.35f) outDir = sample_hemisphere(rng, sN);
+  }
+  if (!f.transmissive && dot(outDir, sN) <= 0.0f) return;
+  V3 bw = albedo;
+  if (f.metallic || f.mirror) {
+    bw = muls(albedo, 0.65f + 0.35f * m.metallic);
+  } else if (f.transmissive) {
+    const float tint = clamp01(m.alpha) * m.transmission;
+    bw = add(muls(mk(1.0f, 1.0f, 1.0f), 1.0f - tint), muls(albedo, tint));
+    if (refracted) bw = muls(bw, smax(0.55f, m.transmission));
+  }
+  if (f.toon) bw = muls(bw, dot(outDir, sN) > 0.55f ? 1.0f : 0.45f);
+  T = mulv(T, bw);
+  if (vmax3(T) < 0.001f) return;
+  const float rr = smin(st.rr_max, smax(st.rr_min, vmax3(T)));
+  if (depth >= st.rr_start && next01(rng) > rr) return;
+  T = divs(T, rr);
+  P.tp[0][p] = T.x; P.tp[1][p] = T.y; P.tp[2][p] = T.z;
+  P.rng[p] = rng;
+  out.cont = true;
+  out.no = add(pos, muls(outDir, 0.002f));
+  out.nd = outDir;
+}
+
+__global__ __launch_bounds__(kBlock) void k_shade(DevPaths P, DevRays in, DevRays out, DevScene sc, DevSettings st,
+                                                 uint32_t depth) {
+  const uint32_t n = P.queue[kQueueIn];
+  const uint32_t stride = gridDim.x * kBlock;
+  const uint32_t lane = threadIdx.x & 63u;
+  uint32_t shadow_tests = 0, shadow_rays = 0;
+  for (uint32_t w0 = blockIdx.x * kBlock + (threadIdx.x & ~63u); w0 < n; w0 += stride) {
+    const uint32_t i = w0 + lane;
+    ShadeOut so;
+    so.cont = false;
+    so.shadow = false;
+    uint32_t p = 0;
+    if (i < n) {
+      p = in.qpath[i];
+      const V3 ro = mk(in.c[0][i], in.c[1][i], in.c[2][i]);
+      const V3 rd = mk(in.c[3][i], in.c[4][i], in.c[5][i]);
+      shade_path(P, sc, st, depth, p, ro, rd, P.hit_t[i], P.hit_prim[i], so, shadow_tests);
+    }
+    const uint32_t slot = wave_append(&P.queue[kQueueOut], so.cont);
+    if (so.cont) {
+      out.c[0][slot] = so.no.x; out.c[1][slot] = so.no.y; out.c[2][slot] = so.no.z;
+      out.c[3][slot] = so.nd.x; out.c[4][slot] = so.nd.y; out.c[5][slot] = so.nd.z;
+      out.qpath[slot] = p;
+    }
+    const uint32_t sslot = wave_append(&P.queue[kQueueShadow], so.shadow);
+    if (so.shadow) {
+      P.sray[0][sslot] = so.so.x; P.sray[1][sslot] = so.so.y; P.sray[2][sslot] = so.so.z;
+      P.sray[3][sslot] = so.sd.x; P.sray[4][sslot] = so.sd.y; P.sray[5][sslot] = so.sd.z;
+      P.stmax[sslot] = so.stmax;
+      P.scontrib[0][sslot] = so.contrib.x; P.scontrib[1][sslot] = so.contrib.y; P.scontrib[2][sslot] = so.contrib.z;
+      P.spath[sslot] = p;
+      ++shadow_rays;
+    }
+  }
+  wave_add(P.counters + kShadowTests, shadow_tests);
+  wave_add(P.counters + kShadowRays, shadow_rays);
+}
+
+// Visibility of one NEE sample. Opaque scenes: any-hit query with t_max = dist - 0.004.
+// Scenes with transmissive materials: the reference's closest-hit loop that skips up to
+// 8 transmissive blockers (ScalarCpuPathTracer.cpp:1066-1087).
+__device__ __forceinline__ bool shadow_visible(const DevScene& sc, const Stack& stk, V3 o, V3 d, float remaining,
+                                               TravStats& st) {
+  if (!sc.has_transmissive) {
+    float bt = remaining;
+    int32_t bp;
+    traverse<true>(sc, stk, o, d, bt, bp, st);
+    return bp < 0;
+  }
+  V3 vo = o;
+  for (uint32_t skips = 0; skips < 8u && remaining > 0.004f; ++skips) {
+    float bt = kInf;
+    int32_t bp;
+    traverse<false>(sc, stk, vo, d, bt, bp, st);
+    if (bp < 0 || bt >= remaining) break;
+    const float4 pn = sc.prims[bp];
+    const PtMaterial& m = material_at(sc, static_cast<uint32_t>(__float_as_int(pn.w)));
+    if (!(m.material_model == 5u || m.transmission > 0.05f)) return false;
+    const float advance = bt + 0.004f;
+    vo = add(vo, muls(d, advance));
+    remaining -= advance;
+  }
+  return true;
+}
+
+__global__ __launch_bounds__(kBlock) void k_shadow(DevPaths P, DevScene sc) {
+  __shared__ int32_t s_ref[kStackLds * kBlock];
+  __shared__ float s_t[kStackLds * kBlock];
+  const Stack stk = make_stack(P, s_ref, s_t);
+  const uint32_t n = P.queue[kQueueShadow];
+  const uint32_t lane = threadIdx.x & 63u;
+  TravStats st = {0, 0, 0, 0, 0};
+  while (true) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(&P.queue[kFetchShadow], 64u);
+    base = __shfl(base, 0);
+    if (base >= n) break;
+    const uint32_t i = base + lane;
+    if (i < n) {
+      const V3 o = mk(P.sray[0][i], P.sray[1][i], P.sray[2][i]);
+      const V3 d = mk(P.sray[3][i], P.sray[4][i], P.sray[5][i]);
+      if (shadow_visible(sc, stk, o, d, P.stmax[i], st)) {
+        const uint32_t p = P.spath[i];
+        P.rad[0][p] = P.rad[0][p] + P.scontrib[0][i];
+        P.rad[1][p] = P.rad[1][p] + P.scontrib[1][i];
+        P.rad[2][p] = P.rad[2][p] + P.scontrib[2][i];
+      }
+    }
+  }
+  flush_stats(P.counters, st, true);
+}
+
+// FilmBuffer::add_sample for one tile, after shade_pixel's non-finite drop
+// (ScalarCpuPathTracer.Render.cpp:270-283, FilmBuffer.cpp:80-91).
+__global__ __launch_bounds__(kBlock) void k_film(DevPaths P, DevFilm film, uint32_t film_width, DevTile tile,
+                                                uint32_t path_offset) {
+  const uint32_t local = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t n = tile.w * tile.h;
+  bool ok = false;
+  if (local < n) {
+    const uint32_t p = path_offset + local;
+    const float r = P.rad[0][p], g = P.rad[1][p], b = P.rad[2][p];
+    if (finite(r) && finite(g) && finite(b)) {
+      const uint32_t px = (tile.y + local / tile.w) * film_width + tile.x + local % tile.w;
+      float4 s = film.sum[px];
+      s.x = s.x + r;
+      s.y = s.y + g;
+      s.z = s.z + b;
+      film.sum[px] = s;
+      film.count[px] = film.count[px] + 1u;
+      ok = true;
+    }
+  }
+  wave_add(P.counters + kSamples, ok ? 1u : 0u);
+}
+
+// Batched IRayAccelerator::intersect for the ray-query entry point.
+__global__ __launch_bounds__(kBlock) void k_trace_rays(DevPaths P, DevScene sc, int any_hit, const float* rays,
+                                                      const float* tmax, uint32_t n, float* out_t, int32_t* out_prim) {
+  __shared__ int32_t s_ref[kStackLds * kBlock];
+  __shared__ float s_t[kStackLds * kBlock];
+  const Stack stk = make_stack(P, s_ref, s_t);
+  const uint32_t lane = threadIdx.x & 63u;
+  TravStats st = {0, 0, 0, 0, 0};
+  while (true) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(&P.queue[kFetchExtend], 64u);
+    base = __shfl(base, 0);
+    if (base >= n) break;
+    const uint32_t i = base + lane;
+    if (i < n) {
+      const float* r = rays + 6u * static_cast<size_t>(i);
+      const V3 o = mk(r[0], r[1], r[2]), d = mk(r[3], r[4], r[5]);
+      float bt = tmax ? tmax[i] : kInf;
+      int32_t bp;
+      if (any_hit) traverse<true>(sc, stk, o, d, bt, bp, st);
+      else traverse<false>(sc, stk, o, d, bt, bp, st);
+      out_t[i] = bp >= 0 ? bt : kInf;
+      out_prim[i] = bp;
+    }
+  }
+  flush_stats(P.counters, st, false);
+}
+
+}  // namespace
+
+void launch_init_queue(hipStream_t stream, const DevPaths& paths, uint32_t live) {
+  hipLaunchKernelGGL(k_init_queue, dim3(1), dim3(64), 0, stream, paths.queue, live);
+}
+
+void launch_advance(hipStream_t stream, const DevPaths& paths) {
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, stream, paths.queue);
+}
+
+void launch_zero_counters(hipStream_t stream, unsigned long long* counters) {
+  hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, stream, counters);
+}
+
+void launch_clear_film(hipStream_t stream, const DevFilm& film, uint32_t pixels) {
+  if (pixels == 0) return;
+  hipLaunchKernelGGL(k_clear_film, dim3((pixels + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, film, pixels);
+}
+
+void launch_raygen(hipStream_t stream, const DevPaths& paths, const DevCamera& cam, const DevSettings& st,
+                   const DevTileTable& tiles, uint32_t path_count) {
+  if (path_count == 0) return;
+  hipLaunchKernelGGL(k_raygen, dim3((path_count + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, paths,
+                     paths.rays[0], cam, st, tiles, path_count);
+}
+
+void launch_extend(hipStream_t stream, const DevPaths& paths, const DevScene& scene, int in_buf, uint32_t grid) {
+  hipLaunchKernelGGL(k_extend, dim3(grid), dim3(kBlock), 0, stream, paths, paths.rays[in_buf], scene);
+}
+
+void launch_shade(hipStream_t stream, const DevPaths& paths, const DevScene& scene, const DevSettings& st, int in_buf,
+                  uint32_t depth, uint32_t grid) {
+  hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kBlock), 0, stream, paths, paths.rays[in_buf], paths.rays[in_buf ^ 1],
+                     scene, st, depth);
+}
+
+void launch_shadow(hipStream_t stream, const DevPaths& paths, const DevScene& scene, uint32_t grid) {
+  hipLaunchKernelGGL(k_shadow, dim3(grid), dim3(kBlock), 0, stream, paths, scene);
+}
+
+void launch_film(hipStream_t stream, const DevPaths& paths, const DevFilm& film, uint32_t film_width,
+                 const DevTile& tile, uint32_t path_offset_in_pass) {
+  const uint32_t n = tile.w * tile.h;
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_film, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, paths, film, film_width, tile,
+                     path_offset_in_pass);
+}
+
+void launch_trace_rays(hipStream_t stream, const DevPaths& paths, const DevScene& scene, int any_hit,
+                       const float* rays, const float* tmax, uint32_t count, float* out_t, int32_t* out_prim,
+                       uint32_t grid) {
+  hipLaunchKernelGGL(k_trace_rays, dim3(grid), dim3(kBlock), 0, stream, paths, scene, any_hit, rays, tmax, count,
+                     out_t, out_prim);
+}
+
+static uint32_t occupancy_grid(int device, const void* fn) {
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0) cus = 256;
+  int blocks = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fn, kBlock, 0) != hipSuccess || blocks <= 0) blocks = 4;
+  return static_cast<uint32_t>(cus) * static_cast<uint32_t>(blocks);
+}
+
+uint32_t persistent_grid_extend(int device) {
+  const uint32_t a = occupancy_grid(device, reinterpret_cast<const void*>(&k_extend));
+  const uint32_t b = occupancy_grid(device, reinterpret_cast<const void*>(&k_shadow));
+  const uint32_t c = occupancy_grid(device, reinterpret_cast<const void*>(&k_trace_rays));
+  uint32_t m = a > b ? a : b;
+  return m > c ? m : c;
+}
+
+uint32_t persistent_grid_shade(int device) {
+  return occupancy_grid(device, reinterpret_cast<const void*>(&k_shade));
+}
+
+}  // namespace ptd

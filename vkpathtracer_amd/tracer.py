@@ -1,0 +1,223 @@
+"""HipPathTracer — the host-side mirror of the reference's IPathTracer over the C ABI.
+
+Method names, argument meaning and error behaviour follow
+``vkpt::pathtracer::IPathTracer`` (src/pathtracer/PathTracer.h:1069-1184):
+``configure`` / ``load_scene_snapshot`` / ``build_or_update_acceleration`` return a
+:class:`Status`; ``render_tile`` / ``render_tiles`` / ``reset_accumulation`` return a
+bool and leave the message in ``status().last_error``. Every call goes to
+libvkpt_hip.so; there is no CPU fallback — construction raises if the library or a
+GPU is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+
+import numpy as np
+
+from . import abi
+from .scene import RenderSettings, Scene
+
+
+@dataclasses.dataclass
+class Status:
+    """vkpt::core::Status (src/core/contracts/Result.h:92-119)."""
+    code: int = 0
+    message: str = ""
+
+    @property
+    def code_name(self) -> str:
+        return abi.STATUS_NAMES[self.code] if 0 <= self.code < len(abi.STATUS_NAMES) else "unknown"
+
+    def is_ok(self) -> bool:
+        return self.code == 0
+
+    def __bool__(self) -> bool:
+        return self.is_ok()
+
+
+class PathTracerError(RuntimeError):
+    def __init__(self, status: Status, what: str):
+        super().__init__(f"{what}: {status.code_name}: {status.message}")
+        self.status = status
+
+
+@dataclasses.dataclass
+class TracerStatus:
+    """PathTracerStatus subset (PathTracer.h:854-865)."""
+    backend: str
+    lifecycle: str
+    scene_loaded: bool
+    accel_valid: bool
+    ready_to_render: bool
+    current_sample: int
+    total_samples: int
+    accumulation_gen: int
+    last_error: str
+
+
+_LIFECYCLE = ["Uninitialized", "Configured", "SceneLoaded", "Ready"]
+
+
+class HipPathTracer:
+    """IPathTracer implemented by the gfx950 wavefront kernels."""
+
+    backend_name = "hip-wavefront"
+
+    def __init__(self, device: int = 0, library=None):
+        self._lib = abi.load_library(library)
+        h = C.c_void_p()
+        rc = self._lib.pt_hip_create(device, C.byref(h))
+        if rc != 0:
+            raise PathTracerError(Status(rc, "no HIP device available for the path tracer"), "pt_hip_create")
+        self._h = h
+        self._settings: RenderSettings | None = None
+        self._view: abi.SceneView | None = None
+        self.device = device
+
+    # -- lifetime
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.pt_hip_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _status(self, rc: int) -> Status:
+        msg = self._lib.pt_hip_last_error(self._h) if rc else b""
+        return Status(rc, (msg or b"").decode())
+
+    def _check(self, rc: int, what: str):
+        if rc != 0:
+            raise PathTracerError(self._status(rc), what)
+
+    # -- IPathTracer
+    def configure(self, settings: RenderSettings) -> Status:
+        self._settings = settings
+        s = abi.make_settings(settings)
+        return self._status(self._lib.pt_hip_configure(self._h, C.byref(s)))
+
+    def load_scene_snapshot(self, scene: Scene) -> Status:
+        view = abi.SceneView(scene)
+        st = self._status(self._lib.pt_hip_load_scene(self._h, C.byref(view.view)))
+        if st:
+            self._view = view
+        return st
+
+    def build_or_update_acceleration(self, builder: str = "sah", leaf_size: int = 4, bucket_count: int = 8) -> Status:
+        p = abi.PtBvhParams()
+        p.builder = {"sah": 0, "midpoint": 1}[builder]
+        p.leaf_size = leaf_size
+        p.bucket_count = bucket_count
+        info = abi.PtBuildInfo()
+        st = self._status(self._lib.pt_hip_build_accel(self._h, C.byref(p), C.byref(info)))
+        self.build_info = info.as_dict()
+        return st
+
+    def reset_accumulation(self) -> bool:
+        return self._lib.pt_hip_reset_accumulation(self._h) == 0
+
+    def render_tile(self, tile: dict | tuple, frame_index: int = 0) -> bool:
+        return self.render_tiles([tile], frame_index)
+
+    def render_tiles(self, tiles, frame_index: int = 0) -> bool:
+        arr = (abi.PtRenderTile * max(1, len(tiles)))()
+        for i, t in enumerate(tiles):
+            if isinstance(t, dict):
+                t = (t.get("x", 0), t.get("y", 0), t["width"], t["height"], t.get("sample_index", 0),
+                     t.get("tile_id", 0), t.get("gpu_id", 0))
+            arr[i] = abi.PtRenderTile(*[int(v) for v in (tuple(t) + (0, 0, 0, 0, 0, 0, 0))[:7]])
+        return self._lib.pt_hip_render_tiles(self._h, arr, len(tiles), frame_index) == 0
+
+    def render_frame_samples(self, sample_begin: int, sample_count: int, frame_index: int = 0) -> bool:
+        """Full-frame tiles for samples [begin, begin+count) in one batched call (render_tiles)."""
+        s = self._settings
+        tiles = [(0, 0, s.width, s.height, sample_begin + i, 0, 0) for i in range(sample_count)]
+        return self.render_tiles(tiles, frame_index)
+
+    def synchronize(self):
+        self._check(self._lib.pt_hip_synchronize(self._h), "synchronize")
+
+    def film(self) -> tuple[np.ndarray, np.ndarray]:
+        """(rgb sum (H, W, 3) f32, sample counts (H, W) u32) — FilmBuffer::raw / sample_counts."""
+        s = self._settings
+        rgb = np.zeros((s.height, s.width, 3), np.float32)
+        cnt = np.zeros((s.height, s.width), np.uint32)
+        self._check(self._lib.pt_hip_read_film(self._h, rgb.ctypes.data, cnt.ctypes.data), "read_film")
+        return rgb, cnt
+
+    def resolve_hdr(self) -> np.ndarray:
+        s = self._settings
+        rgb = np.zeros((s.height, s.width, 3), np.float32)
+        self._check(self._lib.pt_hip_resolve_hdr(self._h, rgb.ctypes.data), "resolve_hdr")
+        return rgb
+
+    def read_counters(self) -> dict:
+        c = abi.PtSampleCounters()
+        self._check(self._lib.pt_hip_read_counters(self._h, C.byref(c)), "read_counters")
+        return c.as_dict()
+
+    def status(self) -> TracerStatus:
+        st = abi.PtTracerStatus()
+        self._lib.pt_hip_status(self._h, C.byref(st))
+        return TracerStatus(self.backend_name, _LIFECYCLE[st.lifecycle], bool(st.scene_loaded), bool(st.accel_valid),
+                            bool(st.ready_to_render), int(st.current_sample), int(st.total_samples),
+                            int(st.accumulation_gen), self._lib.pt_hip_last_error(self._h).decode())
+
+    def shutdown(self):
+        self._lib.pt_hip_shutdown(self._h)
+
+    # -- tuning / measurement
+    def set_wavefront_capacity(self, paths: int):
+        self._check(self._lib.pt_hip_set_wavefront_capacity(self._h, int(paths)), "set_wavefront_capacity")
+
+    def set_profiling(self, enabled: bool):
+        self._check(self._lib.pt_hip_set_profiling(self._h, int(enabled)), "set_profiling")
+
+    def reset_kernel_stats(self):
+        self._check(self._lib.pt_hip_reset_kernel_stats(self._h), "reset_kernel_stats")
+
+    def kernel_stats(self) -> dict:
+        k = abi.PtKernelStats()
+        self._check(self._lib.pt_hip_read_kernel_stats(self._h, C.byref(k)), "read_kernel_stats")
+        out = {name: {"total_ms": k.total_ms[i], "launches": int(k.launches[i]), "items": int(k.items[i])}
+               for i, name in enumerate(abi.KERNELS)}
+        for name in ("inner_node_fetches", "shadow_node_visits", "shadow_triangle_tests", "shadow_inner_node_fetches",
+                     "shadow_leaf_visits", "shadow_rays"):
+            out[name] = int(getattr(k, name))
+        return out
+
+    # -- IRayAccelerator::intersect, batched
+    def trace_rays(self, rays, any_hit: bool = False, tmax=None):
+        """Closest-hit (or any-hit below tmax) for an (N, 6) array of rays.
+
+        Accepts numpy arrays or torch device tensors; returns (t, primitive_index, counters)
+        as numpy arrays (t = inf and primitive_index = -1 on a miss)."""
+        import torch
+
+        dev = torch.device("cuda", self.device)
+        r = torch.as_tensor(np.ascontiguousarray(rays, np.float32) if isinstance(rays, np.ndarray) else rays)
+        r = r.to(dev, torch.float32).contiguous()
+        n = r.shape[0]
+        tm = None
+        if tmax is not None:
+            tm = torch.as_tensor(np.ascontiguousarray(tmax, np.float32) if isinstance(tmax, np.ndarray) else tmax)
+            tm = tm.to(dev, torch.float32).contiguous()
+        out_t = torch.empty(n, dtype=torch.float32, device=dev)
+        out_p = torch.empty(n, dtype=torch.int32, device=dev)
+        torch.cuda.synchronize(dev)
+        c = abi.PtSampleCounters()
+        rc = self._lib.pt_hip_trace_rays(self._h, 1 if any_hit else 0, r.data_ptr(), tm.data_ptr() if tm is not None else None,
+                                         n, out_t.data_ptr(), out_p.data_ptr(), C.byref(c))
+        self._check(rc, "trace_rays")
+        return out_t.cpu().numpy(), out_p.cpu().numpy(), c.as_dict()
