@@ -31,7 +31,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E
 
-# Algorithmic bytes per unit of work, by kernel (see DESIGN.md "Roofline accounting").
+# Algorithmic bytes per unit of work, by kernel: extension ray in (24) + hit out (8), one 64-byte
+# two-child BVH node per inner-node fetch, one 48-byte triangle record per test; shade reads the ray,
+# hit, path state, primitive and material and writes state and the next ray (204); shadow rays carry
+# origin, direction, t_max, contribution and path slot plus the radiance update (56); raygen writes
+# 60 bytes per path; film reads the radiance and updates sum and count (52).
 BYTES = {
     "ray_in": 24, "hit_out": 8, "node": 64, "triangle": 48,
     "shade_path": 204, "shadow_record": 48, "shadow_ray": 56, "raygen_path": 60, "film_sample": 52,
@@ -123,7 +127,11 @@ def main():
 
     from vkpathtracer_amd.tracer import HipPathTracer
 
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
     sc, st, desc = workload(args.workload, args.spp)
+    log(f"scene {sc.name}: {sc.triangle_count} triangles")
     pt = HipPathTracer(0)
     assert pt.configure(st)
     assert pt.load_scene_snapshot(sc)
@@ -140,8 +148,10 @@ def main():
         if not pt.render_tiles(tiles, 0):
             raise RuntimeError(pt.status().last_error)
 
+    log(f"bvh built in {build_wall_ms:.0f} ms: {pt.build_info}")
     for _ in range(args.warmup):
         step()
+    log("warmup done")
     pt.synchronize()
     torch.cuda.synchronize()
     pt.set_profiling(not args.no_profile)
@@ -187,6 +197,7 @@ def main():
                     "bytes_per_launch": int(kb[dom] / k["launches"]),
                     "avg_launch_ms": round(k["total_ms"] / k["launches"], 4)}
 
+    log(f"timed {args.steps} steps: {dt:.3f} s")
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(sc, st, args.cpu_seconds)

@@ -214,6 +214,22 @@ const char* pt_hip_last_error(const pt_tracer* tracer);
 int pt_hip_trace_rays(pt_tracer* tracer, int mode, const float* rays, const float* tmax, uint32_t count,
                       float* out_t, int32_t* out_prim, PtSampleCounters* counters);
 
+/* IPathTracer::update_scene_delta (PathTracer.h:1139; ApplySceneDeltaUpdate): replace the material
+ * and / or light records without rebuilding geometry. Counts must equal the loaded scene's. Either
+ * pointer may be NULL. Host pointers. */
+int pt_hip_update_scene_delta(pt_tracer* tracer, const PtMaterial* materials, uint32_t material_count,
+                              const PtLight* lights, uint32_t light_count);
+
+/* Path-replay adjoint of render_tiles (no counterpart in the reference, which has no
+ * differentiation). For the same tiles and frame, replays every path with the same random
+ * numbers and accumulates, with fp32 atomics, the gradient of
+ *   loss = sum over pixels p, channels c of film_grad[p][c] * (film sum contributed by these tiles)[p][c]
+ * into grad_albedo and grad_emission (3 floats per material) and grad_light (3 floats per light
+ * colour). All four are device pointers; gradients are added to the buffers' contents. The film
+ * and the render counters are not modified. Requires rr_start_depth <= 4. */
+int pt_hip_render_adjoint(pt_tracer* tracer, const PtRenderTile* tiles, uint32_t count, uint64_t frame_index,
+                          const float* film_grad, float* grad_albedo, float* grad_emission, float* grad_light);
+
 /* Block until all work on the handle's stream is done. */
 int pt_hip_synchronize(pt_tracer* tracer);
 /* Maximum paths in flight per wavefront pass (default 8M). */

@@ -237,3 +237,27 @@ class Oracle:
         if rc != 0:
             raise RuntimeError(f"oracle render failed ({rc})")
         return film, cnt, c.as_dict()
+
+
+_SIGS_ADJ = (C.c_int, [C.POINTER(PtoScene), _P, C.POINTER(PtoSettings), _P, C.c_uint32, C.c_uint32, C.c_uint64, C.c_int,
+                       C.c_int, _P, _P, _P, _P, _P])
+
+
+def adjoint_pixels(orc: Oracle, settings, pixels, sample_index, delta, frame_index=0, threads=0,
+                   shadow=SHADOW_ANYHIT):
+    """Path-replay adjoint (pto_adjoint_pixels): returns (g_albedo (M,3), g_emission (M,3), g_light (Lt,3), rgb)."""
+    fn = orc.L.pto_adjoint_pixels
+    fn.restype, fn.argtypes = _SIGS_ADJ
+    px = np.ascontiguousarray(pixels, np.uint32)
+    d = np.ascontiguousarray(delta, np.float32).reshape(-1, 3)
+    nm, nl = orc.scene.s.material_count, orc.scene.s.light_count
+    ga = np.zeros((nm, 3), np.float64)
+    ge = np.zeros((nm, 3), np.float64)
+    gl = np.zeros((max(nl, 1), 3), np.float64)
+    rgb = np.zeros((len(px), 3), orc.real)
+    st = settings_struct(settings)
+    rc = fn(C.byref(orc.scene.s), orc.accel, C.byref(st), px.ctypes.data, len(px), sample_index, frame_index, threads,
+            shadow, d.ctypes.data, ga.ctypes.data, ge.ctypes.data, gl.ctypes.data, rgb.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"oracle adjoint failed ({rc})")
+    return ga, ge, gl[:nl], rgb

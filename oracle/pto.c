@@ -751,7 +751,15 @@ static real spot_attenuation(const PtoLight* l, v3 from_light_dir) {
 
 static inline real pow5(real v) { const real v2 = v * v; return v2 * v2 * v; }
 
-typedef struct { v3 radiance; int valid; } nee_t;
+/* radiance/valid: NeeResult (PathTracer.h:181-184). The adjoint also reads the partial
+ * derivatives of `radiance` by the albedo channel (d_albedo) and by the light colour (d_color)
+ * and the index of the sampled light. */
+typedef struct {
+  v3 radiance;
+  int valid;
+  v3 d_albedo, d_color;
+  uint32_t light;
+} nee_t;
 
 static int shadow_visible(const ctx_t* x, v3 o, v3 d, real dist, PtoCounters* c) {
   if (x->shadow_mode == PTO_SHADOW_ANYHIT) {
@@ -786,7 +794,8 @@ static nee_t sample_direct_light(const ctx_t* x, const hit_t* hit, v3 view_dir, 
   if (sc->light_count == 0) return r;
   const uint32_t nl = sc->light_count;
   const uint64_t li = (uint64_t)(next01(rng) * (real)nl);
-  const PtoLight* light = &sc->lights[li < nl - 1u ? li : nl - 1u];
+  r.light = (uint32_t)(li < nl - 1u ? li : nl - 1u);
+  const PtoLight* light = &sc->lights[r.light];
   v3 lp = v3f(light->position);
   if (light->radius > 1.0e-5f) {
     const real uz = RC(1.0) - RC(2.0) * next01(rng);
@@ -813,11 +822,16 @@ static nee_t sample_direct_light(const ctx_t* x, const hit_t* hit, v3 view_dir, 
   const v3 so = add(hit->position, muls(hit->normal, RC(0.002)));
   if (!shadow_visible(x, so, light_dir, dist, c)) return r;
   const v3 albedo = material_albedo(m);
-  const v3 irradiance = muls(v3f(light->color), ((real)light->intensity * spot) / (dist_sq + kEps));
+  const real kscale = ((real)light->intensity * spot) / (dist_sq + kEps);
+  const v3 irradiance = muls(v3f(light->color), kscale);
   v3 direct = mk(0, 0, 0);
   const real nlf = (real)nl;
+  r.d_albedo = mk(0, 0, 0);
+  r.d_color = mk(0, 0, 0);
   if (!isMirror && !isTransmissive) {
     direct = add(direct, muls(muls(mulv(muls(albedo, kInvPi), irradiance), cos_theta), nlf));
+    r.d_albedo = muls(irradiance, kInvPi * cos_theta * nlf);
+    r.d_color = muls(albedo, kInvPi * kscale * cos_theta * nlf);
   }
   if (isMirror || isMetallic || isClearcoat || isTransmissive || roughness < RC(0.65)) {
     const v3 half_dir = normalize(add(light_dir, normalize(view_dir)));
@@ -835,7 +849,10 @@ static nee_t sample_direct_light(const ctx_t* x, const hit_t* hit, v3 view_dir, 
                                       (isTransmissive ? fresnel : RC(0.0)));
     const v3 white = mk(RC(1.0), RC(1.0), RC(1.0));
     const v3 specTint = add(muls(white, isMetallic ? RC(0.15) : RC(0.88)), muls(albedo, isMetallic ? RC(0.85) : RC(0.12)));
-    direct = add(direct, muls(mulv(specTint, irradiance), spec * cos_theta * nlf * smax(RC(0.15), specStrength)));
+    const real s = spec * cos_theta * nlf * smax(RC(0.15), specStrength);
+    direct = add(direct, muls(mulv(specTint, irradiance), s));
+    r.d_albedo = add(r.d_albedo, muls(irradiance, (isMetallic ? RC(0.85) : RC(0.12)) * s));
+    r.d_color = add(r.d_color, muls(specTint, kscale * s));
   }
   r.radiance = direct;
   r.valid = 1;
@@ -917,8 +934,66 @@ static v3 reflect_about(v3 d, v3 n) { return sub(d, muls(n, RC(2.0) * dot(n, d))
 /* environment: SampleSceneEnvironment, PathTracer.cpp:213-272 (constant colour; maps are rejected at entry) */
 static v3 environment(const PtoScene* sc) { return v3f(sc->environment_color); }
 
-/* trace: ScalarCpuPathTracer.cpp:1148-1289 */
-static v3 trace(const ctx_t* x, v3 ro, v3 rd, uint64_t* rng, uint64_t* rays, PtoCounters* c) {
+/* ---- path-replay adjoint (no reference counterpart: the reference has no differentiation).
+ *
+ * The replay re-traces the path with the same random numbers and, knowing the path's total
+ * radiance L from the forward pass, derives the gradient of f = sum_c delta_c L_c with respect to
+ * material albedos, material emission and light colours (Vicini, Speierer & Jakob 2021, "Path
+ * replay backpropagation"):
+ *   - emission term T*e*w             -> d/de      = T*w
+ *   - NEE term T*D*w (if visible)     -> d/dalbedo = T*w*dD/dalbedo, d/dcolor = T*w*dD/dcolor
+ *   - bounce T' = T*bw/rr             -> d/dalbedo via d log bw * R, R = L - (radiance so far)
+ *   - the reference divides by rr at every depth (ScalarCpuPathTracer.cpp:1274-1280); below the
+ *     roulette start depth that division is deterministic and differentiated (rr = max channel of
+ *     T*bw when unclamped, which depends on every earlier albedo); from the start depth on the
+ *     roulette is unbiased (survival probability rr, weight 1/rr) and its expectation does not
+ *     depend on rr, so no derivative flows through it there.
+ * With x_j = log bw_j and rho_k = log rr_k the exact derivative of the realisation is
+ *   df/dx_{j,c} = delta_c R_{j,c} + sum_{k >= j active, c*_k = c} Psi_k,
+ *   Psi_k = -S_k + S_{next active after k},  S_k = sum_c delta_c R_{k,c},
+ * accumulated incrementally below with a history of the vertices before the roulette depth. */
+#define PTO_ADJ_HIST 8
+typedef struct {
+  v3 L_total;
+  double delta[3];
+  double *g_albedo, *g_emission, *g_light;
+  int pending;
+  uint32_t p_mat;
+  double p_dx[3];
+  int p_active, p_cstar;
+  int nhist;
+  uint32_t h_mat[PTO_ADJ_HIST];
+  double h_dx[PTO_ADJ_HIST][3];
+  int prev_active, prev_cstar, prev_nhist;
+} adj_t;
+
+static void adj_flush_pending(adj_t* a, v3 radiance) {
+  if (!a || !a->pending) return;
+  const double R[3] = {(double)(a->L_total.x - radiance.x), (double)(a->L_total.y - radiance.y),
+                       (double)(a->L_total.z - radiance.z)};
+  double S = 0.0;
+  for (int c = 0; c < 3; ++c) {
+    S += a->delta[c] * R[c];
+    a->g_albedo[3u * a->p_mat + c] += a->delta[c] * R[c] * a->p_dx[c];
+  }
+  if (a->p_active) {
+    for (int h = 0; h < a->nhist; ++h) a->g_albedo[3u * a->h_mat[h] + a->p_cstar] += -S * a->h_dx[h][a->p_cstar];
+    if (a->prev_active)
+      for (int h = 0; h < a->prev_nhist; ++h)
+        a->g_albedo[3u * a->h_mat[h] + a->prev_cstar] += S * a->h_dx[h][a->prev_cstar];
+    a->prev_active = 1;
+    a->prev_cstar = a->p_cstar;
+    a->prev_nhist = a->nhist;
+  }
+  a->pending = 0;
+}
+
+static inline double albedo_mask(const PtoMaterial* m, int c) {
+  return (m->albedo[c] > 0.0f && m->albedo[c] < 1.5f) ? 1.0 : 0.0;
+}
+
+/* trace: ScalarCpuPathTracer.cpp:1148-1289; adj != NULL replays the path for the adjoint. */
+static v3 trace(const ctx_t* x, v3 ro, v3 rd, uint64_t* rng, uint64_t* rays, PtoCounters* c, adj_t* adj) {
   const PtoScene* sc = x->sc;
   const PtoSettings* st = x->s;
   v3 radiance = mk(0, 0, 0);
@@ -927,6 +1002,7 @@ static v3 trace(const ctx_t* x, v3 ro, v3 rd, uint64_t* rng, uint64_t* rays, Pto
   for (uint32_t depth = 0; depth < x->max_depth; ++depth) {
     ++*rays;
     hit_t hit = intersect_scene(x, ro, rd, c);
+    adj_flush_pending(adj, radiance);
     if (!hit.hit) {
       radiance = add(radiance, mulv(throughput, environment(sc)));
       break;
@@ -947,13 +1023,22 @@ static v3 trace(const ctx_t* x, v3 ro, v3 rd, uint64_t* rng, uint64_t* rays, Pto
     const int isToon = m->material_model == 8u;
     const v3 emissive = v3f(m->emissive);
 
+    const uint32_t mat_index = sc->material_count ? (hit.material_index < sc->material_count - 1u
+                                                         ? hit.material_index : sc->material_count - 1u) : 0u;
+    real e_weight = RC(-1.0);
     if (!st->enable_nee || depth == 0) {
       radiance = add(radiance, mulv(throughput, emissive));
+      e_weight = RC(1.0);
     } else if (st->enable_mis && isEmissiveModel) {
       const real bsdf_pdf = smax(RC(0.0), dot(sN, neg(rd))) * kInvPi;
       const real l_pdf = light_pdf_for_direction(sc, ro, rd);
       const real w = mis_weight(bsdf_pdf, l_pdf);
       radiance = add(radiance, muls(mulv(throughput, emissive), w));
+      e_weight = w;
+    }
+    if (adj && e_weight >= RC(0.0) && sc->material_count) {
+      const real t3[3] = {throughput.x, throughput.y, throughput.z};
+      for (int ch = 0; ch < 3; ++ch) adj->g_emission[3u * mat_index + ch] += adj->delta[ch] * (double)t3[ch] * (double)e_weight;
     }
 
     if (st->enable_nee && depth < x->max_depth - 1u) {
@@ -961,16 +1046,27 @@ static v3 trace(const ctx_t* x, v3 ro, v3 rd, uint64_t* rng, uint64_t* rays, Pto
       lh.normal = sN;
       const nee_t nee = sample_direct_light(x, &lh, neg(rd), rng, c);
       if (nee.valid) {
+        real w = RC(1.0);
         if (st->enable_mis) {
           const v3 to_approx = sc->light_count == 0 ? mk(0, 0, 0) : sub(v3f(sc->lights[0].position), hit.position);
           const v3 l_dir = normalize(to_approx);
           const real cos_theta = smax(RC(0.0), dot(sN, l_dir));
           const real bsdf_pdf = cos_theta * kInvPi;
           const real l_pdf = sc->light_count == 0 ? RC(1.0) : (RC(1.0) / (real)sc->light_count);
-          const real w = mis_weight(l_pdf, bsdf_pdf);
+          w = mis_weight(l_pdf, bsdf_pdf);
           radiance = add(radiance, muls(mulv(throughput, nee.radiance), w));
         } else {
           radiance = add(radiance, mulv(throughput, nee.radiance));
+        }
+        if (adj) {
+          const real t3[3] = {throughput.x, throughput.y, throughput.z};
+          const real da[3] = {nee.d_albedo.x, nee.d_albedo.y, nee.d_albedo.z};
+          const real dc[3] = {nee.d_color.x, nee.d_color.y, nee.d_color.z};
+          for (int ch = 0; ch < 3; ++ch) {
+            const double tw = adj->delta[ch] * (double)t3[ch] * (double)w;
+            adj->g_albedo[3u * mat_index + ch] += tw * (double)da[ch] * albedo_mask(m, ch);
+            adj->g_light[3u * nee.light + ch] += tw * (double)dc[ch];
+          }
         }
       }
     }
@@ -1016,9 +1112,37 @@ static v3 trace(const ctx_t* x, v3 ro, v3 rd, uint64_t* rng, uint64_t* rays, Pto
     if (isToon) bw = muls(bw, dot(outDir, sN) > RC(0.55) ? RC(1.0) : RC(0.45));
     throughput = mulv(throughput, bw);
     if (vmax3(throughput) < RC(0.001)) break;
-    const real rr = smin(rr_max, smax(rr_min, vmax3(throughput)));
+    const real maxT = vmax3(throughput);
+    /* the channel std::max(x, std::max(y, z)) returns: x unless x < max(y, z); then z if y < z, else y */
+    const int cstar = (throughput.x < smax(throughput.y, throughput.z)) ? ((throughput.y < throughput.z) ? 2 : 1) : 0;
+    const real rr = smin(rr_max, smax(rr_min, maxT));
     if (depth >= st->rr_start_depth && next01(rng) > rr) break;
     throughput = divs(throughput, rr);
+    if (adj && sc->material_count) {
+      const real a3[3] = {albedo.x, albedo.y, albedo.z};
+      const real b3[3] = {bw.x, bw.y, bw.z};
+      real tint = RC(0.0);
+      if (isTransmissive && !(isMetallic || isMirror)) tint = clamp01((real)m->alpha) * (real)m->transmission;
+      adj->pending = 1;
+      adj->p_mat = mat_index;
+      for (int ch = 0; ch < 3; ++ch) {
+        double dx = 0.0;
+        if (isTransmissive && !(isMetallic || isMirror)) {
+          const real base = (RC(1.0) - tint) + a3[ch] * tint; /* bw before the refraction / toon factors */
+          dx = base > RC(0.0) ? (double)tint / (double)base : 0.0;
+        } else {
+          dx = a3[ch] > RC(0.0) ? 1.0 / (double)a3[ch] : 0.0;
+        }
+        adj->p_dx[ch] = b3[ch] > RC(0.0) ? dx * albedo_mask(m, ch) : 0.0;
+      }
+      adj->p_active = (depth < st->rr_start_depth && rr_min < maxT && maxT < rr_max) ? 1 : 0;
+      adj->p_cstar = cstar;
+      if (depth < st->rr_start_depth && adj->nhist < PTO_ADJ_HIST) {
+        adj->h_mat[adj->nhist] = mat_index;
+        for (int ch = 0; ch < 3; ++ch) adj->h_dx[adj->nhist][ch] = adj->p_dx[ch];
+        adj->nhist++;
+      }
+    }
     ro = add(hit.position, muls(outDir, RC(0.002)));
     rd = outDir;
   }
@@ -1102,7 +1226,7 @@ static void camera_ray(const camera_t* cam, const PtoSettings* s, uint32_t x, ui
 
 /* shade_pixel: ScalarCpuPathTracer.Render.cpp:228-284 */
 static int shade_pixel(const ctx_t* x, const camera_t* cam, uint32_t pixel, uint32_t sample, uint64_t frame,
-                       v3* out, uint64_t* rays, PtoCounters* c) {
+                       v3* out, uint64_t* rays, PtoCounters* c, adj_t* adj) {
   const PtoSettings* s = x->s;
   const uint32_t px = pixel % s->width, py = pixel / s->width;
   uint64_t seed = 0;
@@ -1111,7 +1235,7 @@ static int shade_pixel(const ctx_t* x, const camera_t* cam, uint32_t pixel, uint
   v3 ro, rd;
   camera_ray(cam, s, px, py, sample, frame, (uint32_t)path_id, &seed, &ro, &rd);
   uint64_t rng = pto_rng_seed(s->seed + sample_seed + seed, frame, sample, path_id, 0, 0, sample_seed);
-  const v3 v = trace(x, ro, rd, &rng, rays, c);
+  const v3 v = trace(x, ro, rd, &rng, rays, c, adj);
   *out = v;
   return is_finite_r(v.x) && is_finite_r(v.y) && is_finite_r(v.z);
 }
@@ -1162,7 +1286,7 @@ int pto_render_pixels(const PtoScene* scene, const pto_accel* a, const PtoSettin
       uint64_t rays = 0;
       const uint64_t total_px = (uint64_t)s->width * s->height;
       if ((uint64_t)pixels[i] >= total_px) { out_valid[i] = 0; continue; }
-      const int ok = shade_pixel(&x, &cam, pixels[i], sample_index, frame_index, &v, &rays, &lc);
+      const int ok = shade_pixel(&x, &cam, pixels[i], sample_index, frame_index, &v, &rays, &lc, NULL);
       rgb[3 * i] = v.x; rgb[3 * i + 1] = v.y; rgb[3 * i + 2] = v.z;
       out_valid[i] = (uint8_t)ok;
       lc.rays += rays;
@@ -1209,7 +1333,7 @@ int pto_render_tile(const PtoScene* scene, const pto_accel* a, const PtoSettings
         const uint32_t pixel = py * s->width + px;
         v3 v;
         uint64_t rays = 0;
-        const int ok = shade_pixel(&x, &cam, pixel, sample, frame_index, &v, &rays, &lc);
+        const int ok = shade_pixel(&x, &cam, pixel, sample, frame_index, &v, &rays, &lc, NULL);
         lc.rays += rays;
         if (!ok) continue; /* shade_pixel drops non-finite samples before FilmBuffer::add_sample */
         lc.samples++;
@@ -1224,5 +1348,66 @@ int pto_render_tile(const PtoScene* scene, const pto_accel* a, const PtoSettings
     }
   }
   if (counters) *counters = total;
+  return 0;
+}
+
+/* Path-replay adjoint for one sample of each listed pixel. delta: 3 floats per listed pixel,
+ * dLoss / d(the path's radiance) (= dLoss / d film sum at that pixel). The forward radiance is
+ * recomputed first (nothing is stored between the passes). Gradients are accumulated in double:
+ * g_albedo, g_emission (3 per material), g_light (3 per light colour); out_rgb (may be NULL)
+ * receives the forward radiance. Paths with non-finite radiance contribute nothing. */
+int pto_adjoint_pixels(const PtoScene* scene, const pto_accel* a, const PtoSettings* s,
+                       const uint32_t* pixels, uint32_t n, uint32_t sample_index, uint64_t frame_index,
+                       int threads, int shadow_mode, const float* delta, double* g_albedo, double* g_emission,
+                       double* g_light, void* out_rgb) {
+  if (!scene || !a || !s || s->width == 0 || s->height == 0) return -1;
+  const int sup = check_supported(scene);
+  if (sup) return sup;
+  if (s->rr_start_depth > PTO_ADJ_HIST) return -4;
+  if (shadow_mode == PTO_SHADOW_ANYHIT && any_transmissive(scene)) shadow_mode = PTO_SHADOW_CLOSEST;
+  ctx_t x = {scene, a, s, s->max_depth > 1u ? s->max_depth : 1u, shadow_mode};
+  const camera_t cam = make_camera(scene, s);
+  const uint32_t nm = scene->material_count, nl = scene->light_count;
+  real* rgb = (real*)out_rgb;
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#else
+  (void)threads;
+#endif
+#pragma omp parallel
+  {
+    double* ga = (double*)calloc(3u * (nm ? nm : 1u), sizeof(double));
+    double* ge = (double*)calloc(3u * (nm ? nm : 1u), sizeof(double));
+    double* gl = (double*)calloc(3u * (nl ? nl : 1u), sizeof(double));
+    PtoCounters lc;
+    memset(&lc, 0, sizeof(lc));
+#pragma omp for schedule(dynamic, 64)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+      v3 L;
+      uint64_t rays = 0;
+      const uint64_t total_px = (uint64_t)s->width * s->height;
+      if ((uint64_t)pixels[i] >= total_px) continue;
+      const int ok = shade_pixel(&x, &cam, pixels[i], sample_index, frame_index, &L, &rays, &lc, NULL);
+      if (rgb) { rgb[3 * i] = L.x; rgb[3 * i + 1] = L.y; rgb[3 * i + 2] = L.z; }
+      if (!ok) continue;
+      adj_t adj;
+      memset(&adj, 0, sizeof(adj));
+      adj.L_total = L;
+      for (int ch = 0; ch < 3; ++ch) adj.delta[ch] = (double)delta[3 * i + ch];
+      adj.g_albedo = ga;
+      adj.g_emission = ge;
+      adj.g_light = gl;
+      v3 L2;
+      shade_pixel(&x, &cam, pixels[i], sample_index, frame_index, &L2, &rays, &lc, &adj);
+    }
+#pragma omp critical
+    {
+      for (uint32_t k = 0; k < 3u * nm; ++k) { g_albedo[k] += ga[k]; g_emission[k] += ge[k]; }
+      for (uint32_t k = 0; k < 3u * nl; ++k) g_light[k] += gl[k];
+    }
+    free(ga);
+    free(ge);
+    free(gl);
+  }
   return 0;
 }

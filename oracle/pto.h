@@ -142,6 +142,13 @@ int pto_render_tile(const PtoScene* scene, const pto_accel* a, const PtoSettings
                     uint32_t sample_begin, uint32_t sample_count, uint64_t frame_index,
                     int threads, int shadow_mode, void* film_sum, uint32_t* film_count, PtoCounters* counters);
 
+/* Path-replay adjoint of one sample per listed pixel (no reference counterpart; see pto.c).
+ * delta: dLoss/d(path radiance), 3 floats per pixel. Gradients accumulate (+=) in double. */
+int pto_adjoint_pixels(const PtoScene* scene, const pto_accel* a, const PtoSettings* s,
+                       const uint32_t* pixels, uint32_t n, uint32_t sample_index, uint64_t frame_index,
+                       int threads, int shadow_mode, const float* delta, double* g_albedo, double* g_emission,
+                       double* g_light, void* out_rgb);
+
 #ifdef __cplusplus
 }
 #endif

@@ -111,6 +111,8 @@ SIGNATURES = {
     "pt_hip_read_kernel_stats": (C.c_int, [_P, C.POINTER(PtKernelStats)]),
     "pt_hip_reset_kernel_stats": (C.c_int, [_P]),
     "pt_bvh_build_host": (C.c_int, [C.POINTER(PtSceneView), C.POINTER(PtBvhParams), _P, _P, _P, _P, _P, _P, _P]),
+    "pt_hip_update_scene_delta": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint32]),
+    "pt_hip_render_adjoint": (C.c_int, [_P, C.POINTER(PtRenderTile), C.c_uint32, C.c_uint64, _P, _P, _P, _P]),
     "pt_hip_sizeof": (C.c_uint32, [C.c_int]),
     "pt_hip_version": (C.c_char_p, []),
 }

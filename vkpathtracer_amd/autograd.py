@@ -1,0 +1,88 @@
+"""Differentiable rendering: a torch.autograd.Function over the forward render and its path-replay adjoint.
+
+Parameters are per-material albedo and emission (M, 3) and per-light colour (L, 3). The forward pass
+uploads them with IPathTracer::update_scene_delta, renders the frame (one full-frame tile per sample,
+render_tiles) and returns the HDR image (resolve_hdr: film sum * 1/count). The backward pass hands
+dLoss/d(film sum) = dLoss/d(image) / count to pt_hip_render_adjoint, which replays every path with the
+same random numbers and accumulates the parameter gradients with fp32 atomics — nothing about the
+forward paths is stored.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from .scene import RenderSettings, Scene
+from .tracer import HipPathTracer
+
+
+class _RenderFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, albedo, emission, light_color, renderer):
+        renderer.upload(albedo, emission, light_color)
+        image, count = renderer.render()
+        ctx.renderer = renderer
+        ctx.save_for_backward(count)
+        return image
+
+    @staticmethod
+    def backward(ctx, grad_image):
+        (count,) = ctx.saved_tensors
+        r = ctx.renderer
+        inv = 1.0 / torch.clamp(count, min=1).to(torch.float32)
+        film_grad = (grad_image.to(torch.float32) * inv[..., None]).contiguous()
+        dev = film_grad.device
+        ga = torch.zeros((r.n_materials, 3), dtype=torch.float32, device=dev)
+        ge = torch.zeros((r.n_materials, 3), dtype=torch.float32, device=dev)
+        gl = torch.zeros((max(r.n_lights, 1), 3), dtype=torch.float32, device=dev)
+        r.tracer.render_adjoint(r.tiles, film_grad, ga, ge, gl[: r.n_lights], r.frame_index)
+        return ga, ge, gl[: r.n_lights], None
+
+
+class DifferentiableRenderer:
+    """Owns one HipPathTracer over `scene`; calling it with (albedo, emission, light_color) tensors
+    renders the image and makes it differentiable with respect to them."""
+
+    def __init__(self, scene: Scene, settings: RenderSettings, device: int = 0, frame_index: int = 0):
+        self.scene = scene
+        self.settings = settings
+        self.frame_index = frame_index
+        self.tracer = HipPathTracer(device)
+        for st in (self.tracer.configure(settings), self.tracer.load_scene_snapshot(scene),
+                   self.tracer.build_or_update_acceleration()):
+            if not st:
+                raise RuntimeError(st.message)
+        self.n_materials = len(scene.materials)
+        self.n_lights = len(scene.lights)
+        self.tiles = [(0, 0, settings.width, settings.height, s, 0, 0) for s in range(settings.spp)]
+        self.device = torch.device("cuda", device)
+
+    def initial_parameters(self):
+        m, l = self.scene.materials, self.scene.lights
+        t = lambda a: torch.tensor(np.asarray(a, np.float32), device=self.device)
+        return t(m["albedo"]), t(m["emissive"]), t(l["color"]) if len(l) else t(np.zeros((0, 3)))
+
+    def upload(self, albedo, emission, light_color):
+        mats = self.scene.materials.copy()
+        mats["albedo"] = albedo.detach().to("cpu", torch.float32).numpy()
+        mats["emissive"] = emission.detach().to("cpu", torch.float32).numpy()
+        lights = self.scene.lights.copy()
+        if len(lights):
+            lights["color"] = light_color.detach().to("cpu", torch.float32).numpy()
+        if not self.tracer.update_scene_delta(mats, lights if len(lights) else None):
+            raise RuntimeError(self.tracer.status().last_error)
+
+    def render(self):
+        if not self.tracer.reset_accumulation() or not self.tracer.render_tiles(self.tiles, self.frame_index):
+            raise RuntimeError(self.tracer.status().last_error)
+        rgb, cnt = self.tracer.film()
+        count = torch.from_numpy(cnt.astype(np.int32)).to(self.device)
+        inv = (1.0 / np.maximum(cnt, 1).astype(np.float32))[..., None]
+        image = torch.from_numpy(rgb * inv).to(self.device)
+        return image, count
+
+    def __call__(self, albedo, emission, light_color):
+        return _RenderFn.apply(albedo, emission, light_color, self)
+
+    def close(self):
+        self.tracer.close()

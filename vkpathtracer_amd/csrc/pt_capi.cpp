@@ -88,6 +88,9 @@ struct pt_tracer {
   DevArray<uint64_t> d_rng;
   DevArray<int32_t> d_stack_ref;
   DevArray<float> d_stack_t;
+  DevArray<float> d_adj_f;
+  DevArray<uint32_t> d_adj_u;
+  uint32_t adj_capacity_alloc = 0;
   uint32_t path_capacity_alloc = 0;
   uint32_t capacity = kDefaultCapacity;
 
@@ -191,6 +194,151 @@ void free_paths(pt_tracer* t) {
   t->d_path_u.release();
   t->d_rng.release();
   t->path_capacity_alloc = 0;
+  t->d_adj_f.release();
+  t->d_adj_u.release();
+  t->adj_capacity_alloc = 0;
+}
+
+// Adjoint state for `cap` paths: 27 float arrays and 8 u32 arrays (see ptd::DevAdjoint).
+int ensure_adjoint(pt_tracer* t, uint32_t cap, ptd::DevAdjoint& A) {
+  if (cap > t->adj_capacity_alloc) {
+    t->d_adj_f.release();
+    t->d_adj_u.release();
+    t->adj_capacity_alloc = 0;
+    PT_HIP(t, t->d_adj_f.ensure(static_cast<size_t>(27) * cap), "allocate adjoint state");
+    PT_HIP(t, t->d_adj_u.ensure(static_cast<size_t>(8) * cap), "allocate adjoint state");
+    t->adj_capacity_alloc = cap;
+  }
+  const size_t n = t->adj_capacity_alloc;
+  float* f = t->d_adj_f.ptr;
+  uint32_t* u = t->d_adj_u.ptr;
+  for (int c = 0; c < 3; ++c) {
+    A.ltot[c] = f + (0 + c) * n;
+    A.delta[c] = f + (3 + c) * n;
+    A.p_dx[c] = f + (6 + c) * n;
+    A.s_galb[c] = f + (9 + c) * n;
+    A.s_glight[c] = f + (12 + c) * n;
+    for (int h = 0; h < ptd::kAdjHist; ++h) A.h_dx[h][c] = f + (15 + 3 * h + c) * n;
+  }
+  A.p_mat = u;
+  A.flags = u + n;
+  A.s_mat = u + 2 * n;
+  A.s_light = u + 3 * n;
+  for (int h = 0; h < ptd::kAdjHist; ++h) A.h_mat[h] = u + (4 + h) * n;
+  return PT_OK;
+}
+
+int ensure_paths(pt_tracer* t, uint32_t cap);
+
+// Clamp tiles like render_tile_cancellable (ScalarCpuPathTracer.Render.cpp:172-180) and split
+// any tile larger than one pass into row bands.
+int build_work(pt_tracer* t, const PtRenderTile* tiles, uint32_t count, std::vector<ptd::DevTile>& work) {
+  const uint32_t W = t->settings.width, H = t->settings.height;
+  const uint32_t cap = std::max<uint32_t>(t->capacity, 1024u);
+  work.clear();
+  work.reserve(count);
+  for (uint32_t i = 0; i < count; ++i) {
+    const PtRenderTile& tl = tiles[i];
+    const uint32_t x0 = std::min(tl.x, W), y0 = std::min(tl.y, H);
+    const uint32_t x1 = static_cast<uint32_t>(std::min<uint64_t>(W, static_cast<uint64_t>(x0) + tl.width));
+    const uint32_t y1 = static_cast<uint32_t>(std::min<uint64_t>(H, static_cast<uint64_t>(y0) + tl.height));
+    if (x0 >= x1 || y0 >= y1) continue;
+    const uint32_t w = x1 - x0;
+    if (w > cap) return fail(t, PT_INVALID_ARGUMENT, "tile wider than the wavefront capacity");
+    const uint32_t rows_per = std::max(1u, cap / w);
+    for (uint32_t y = y0; y < y1; y += rows_per) {
+      const uint32_t h = std::min(rows_per, y1 - y);
+      work.push_back(ptd::DevTile{x0, y, w, h, tl.sample_index, 0u});
+    }
+  }
+  return PT_OK;
+}
+
+ptd::DevSettings dev_settings(const pt_tracer* t, uint64_t frame_index) {
+  ptd::DevSettings st{};
+  st.seed = t->settings.seed;
+  st.frame = frame_index;
+  st.width = t->settings.width;
+  st.height = t->settings.height;
+  st.max_depth = t->settings.max_depth;
+  st.enable_nee = t->settings.enable_nee;
+  st.enable_mis = t->settings.enable_mis;
+  st.rr_start = t->settings.rr_start_depth;
+  st.rr_min = t->settings.rr_min_survival;
+  st.rr_max = t->settings.rr_max_survival;
+  return st;
+}
+
+// The bounce loop of one wavefront pass (extend -> shade -> shadow -> advance per depth).
+void bounce_loop(pt_tracer* t, const ptd::DevPaths& P, const ptd::DevSettings& st, const ptd::DevAdjoint* adj) {
+  int buf = 0;
+  for (uint32_t depth = 0; depth < st.max_depth; ++depth) {
+    timed(t, PT_KERNEL_EXTEND, 0, [&] { ptd::launch_extend(t->stream, P, t->dscene, buf, t->grid_ext); });
+    timed(t, PT_KERNEL_SHADE, 0, [&] {
+      if (adj) ptd::launch_shade_adjoint(t->stream, P, *adj, t->dscene, st, buf, depth, t->grid_shade);
+      else ptd::launch_shade(t->stream, P, t->dscene, st, buf, depth, t->grid_shade);
+    });
+    if (st.enable_nee && depth + 1u < st.max_depth)
+      timed(t, PT_KERNEL_SHADOW, 0, [&] {
+        if (adj) ptd::launch_shadow_adjoint(t->stream, P, *adj, t->dscene, t->grid_ext);
+        else ptd::launch_shadow(t->stream, P, t->dscene, t->grid_ext);
+      });
+    ptd::launch_advance(t->stream, P);
+    buf ^= 1;
+  }
+}
+
+// Runs the wavefront over `work` in passes of at most `capacity` paths.
+// adj == nullptr: forward passes whose radiance is accumulated into the film in tile order.
+// adj != nullptr: per pass, a forward replay (radiance kept per path) then the adjoint replay;
+//                 the film and the render counters are left untouched.
+int run_passes(pt_tracer* t, std::vector<ptd::DevTile>& work, const ptd::DevSettings& st, ptd::DevAdjoint* adj) {
+  const uint32_t cap = std::max<uint32_t>(t->capacity, 1024u);
+  const ptd::DevFilm film{t->d_film_sum.ptr, t->d_film_count.ptr};
+  static_assert(kMaxTilesPerPass == ptd::kMaxTilesPerPass, "tile table size");
+  size_t next = 0;
+  while (next < work.size()) {
+    size_t end = next;
+    uint32_t paths = 0;
+    while (end < work.size() && end - next < kMaxTilesPerPass) {
+      const uint32_t n = work[end].w * work[end].h;
+      if (end > next && static_cast<uint64_t>(paths) + n > cap) break;
+      work[end].path_begin = paths;
+      paths += n;
+      ++end;
+    }
+    int rc = ensure_paths(t, std::max(paths, std::min(cap, 1u << 20)));
+    if (rc != PT_OK) return rc;
+    ptd::DevTileTable table{};
+    table.count = static_cast<uint32_t>(end - next);
+    for (size_t i = next; i < end; ++i) table.t[i - next] = work[i];
+    ptd::DevPaths P = t->paths;
+    if (adj) {
+      rc = ensure_adjoint(t, t->path_capacity_alloc, *adj);
+      if (rc != PT_OK) return rc;
+      P.counters = t->d_query_counters.ptr;
+    }
+    ptd::launch_init_queue(t->stream, P, paths);
+    timed(t, PT_KERNEL_RAYGEN, paths, [&] { ptd::launch_raygen(t->stream, P, t->dcam, st, table, paths); });
+    bounce_loop(t, P, st, nullptr);
+    if (adj) {
+      ptd::launch_copy_radiance(t->stream, P, *adj, paths);
+      ptd::launch_init_queue(t->stream, P, paths);
+      timed(t, PT_KERNEL_RAYGEN, paths,
+            [&] { ptd::launch_raygen_adjoint(t->stream, P, *adj, t->dcam, st, table, paths); });
+      bounce_loop(t, P, st, adj);
+    } else {
+      for (size_t i = next; i < end; ++i) {
+        const ptd::DevTile& tile = work[i];
+        timed(t, PT_KERNEL_FILM, tile.w * tile.h,
+              [&] { ptd::launch_film(t->stream, P, film, st.width, tile, tile.path_begin); });
+      }
+    }
+    const hipError_t le = hipGetLastError();
+    if (le != hipSuccess) return hip_fail(t, le, "kernel launch");
+    next = end;
+  }
+  return PT_OK;
 }
 
 // Path-state arrays for `cap` paths: 29 float arrays, 4 u32 arrays, 1 u64 array.
@@ -501,77 +649,11 @@ int pt_hip_render_tiles(pt_tracer* t, const PtRenderTile* tiles, uint32_t count,
   if (!t->accel_valid) return fail(t, PT_NOT_READY, "render_tile requires build_or_update_acceleration");
   if (count && !tiles) return fail(t, PT_INVALID_ARGUMENT, "render_tiles: null tiles");
   (void)hipSetDevice(t->device);
-  const uint32_t W = t->settings.width, H = t->settings.height;
-  // Clamp tiles like render_tile_cancellable (ScalarCpuPathTracer.Render.cpp:172-180) and split
-  // any tile larger than one pass into row bands.
   std::vector<ptd::DevTile> work;
-  work.reserve(count);
-  const uint32_t cap = std::max<uint32_t>(t->capacity, 1024u);
-  for (uint32_t i = 0; i < count; ++i) {
-    const PtRenderTile& tl = tiles[i];
-    const uint32_t x0 = std::min(tl.x, W), y0 = std::min(tl.y, H);
-    const uint32_t x1 = static_cast<uint32_t>(std::min<uint64_t>(W, static_cast<uint64_t>(x0) + tl.width));
-    const uint32_t y1 = static_cast<uint32_t>(std::min<uint64_t>(H, static_cast<uint64_t>(y0) + tl.height));
-    if (x0 >= x1 || y0 >= y1) continue;
-    const uint32_t w = x1 - x0;
-    if (w > cap) return fail(t, PT_INVALID_ARGUMENT, "tile wider than the wavefront capacity");
-    const uint32_t rows_per = std::max(1u, cap / w);
-    for (uint32_t y = y0; y < y1; y += rows_per) {
-      const uint32_t h = std::min(rows_per, y1 - y);
-      work.push_back(ptd::DevTile{x0, y, w, h, tl.sample_index, 0u});
-    }
-  }
-  ptd::DevSettings st{};
-  st.seed = t->settings.seed;
-  st.frame = frame_index;
-  st.width = W;
-  st.height = H;
-  st.max_depth = t->settings.max_depth;
-  st.enable_nee = t->settings.enable_nee;
-  st.enable_mis = t->settings.enable_mis;
-  st.rr_start = t->settings.rr_start_depth;
-  st.rr_min = t->settings.rr_min_survival;
-  st.rr_max = t->settings.rr_max_survival;
-  const ptd::DevFilm film{t->d_film_sum.ptr, t->d_film_count.ptr};
-
-  size_t next = 0;
-  while (next < work.size()) {
-    // Gather whole tiles into one pass.
-    size_t end = next;
-    uint32_t paths = 0;
-    while (end < work.size() && end - next < kMaxTilesPerPass) {
-      const uint32_t n = work[end].w * work[end].h;
-      if (end > next && static_cast<uint64_t>(paths) + n > cap) break;
-      work[end].path_begin = paths;
-      paths += n;
-      ++end;
-    }
-    const int rc = ensure_paths(t, std::max(paths, std::min(cap, 1u << 20)));
-    if (rc != PT_OK) return rc;
-    ptd::DevTileTable table{};
-    table.count = static_cast<uint32_t>(end - next);
-    for (size_t i = next; i < end; ++i) table.t[i - next] = work[i];
-    static_assert(kMaxTilesPerPass == ptd::kMaxTilesPerPass, "tile table size");
-    ptd::DevPaths& P = t->paths;
-    ptd::launch_init_queue(t->stream, P, paths);
-    timed(t, PT_KERNEL_RAYGEN, paths, [&] { ptd::launch_raygen(t->stream, P, t->dcam, st, table, paths); });
-    int buf = 0;
-    for (uint32_t depth = 0; depth < st.max_depth; ++depth) {
-      timed(t, PT_KERNEL_EXTEND, 0, [&] { ptd::launch_extend(t->stream, P, t->dscene, buf, t->grid_ext); });
-      timed(t, PT_KERNEL_SHADE, 0, [&] { ptd::launch_shade(t->stream, P, t->dscene, st, buf, depth, t->grid_shade); });
-      if (st.enable_nee && depth + 1u < st.max_depth)
-        timed(t, PT_KERNEL_SHADOW, 0, [&] { ptd::launch_shadow(t->stream, P, t->dscene, t->grid_ext); });
-      ptd::launch_advance(t->stream, P);
-      buf ^= 1;
-    }
-    for (size_t i = next; i < end; ++i) {
-      const ptd::DevTile& tile = work[i];
-      timed(t, PT_KERNEL_FILM, tile.w * tile.h, [&] { ptd::launch_film(t->stream, P, film, W, tile, tile.path_begin); });
-    }
-    const hipError_t le = hipGetLastError();
-    if (le != hipSuccess) return hip_fail(t, le, "kernel launch");
-    next = end;
-  }
+  int rc = build_work(t, tiles, count, work);
+  if (rc != PT_OK) return rc;
+  rc = run_passes(t, work, dev_settings(t, frame_index), nullptr);
+  if (rc != PT_OK) return rc;
   for (uint32_t i = 0; i < count; ++i) t->current_sample = std::max(t->current_sample, tiles[i].sample_index + 1u);
   if (t->profiling) {
     const int rc = sync_all(t);
@@ -809,5 +891,70 @@ extern "C" int pt_bvh_build_host(const PtSceneView* scene, const PtBvhParams* pa
         g[9 + k] = q.n[k];
       }
     }
+  return PT_OK;
+}
+
+extern "C" int pt_hip_update_scene_delta(pt_tracer* t, const PtMaterial* materials, uint32_t material_count,
+                                         const PtLight* lights, uint32_t light_count) {
+  if (!t) return PT_INVALID_ARGUMENT;
+  if (!t->accel_valid) return fail(t, PT_NOT_READY, "update_scene_delta requires an established acceleration");
+  (void)hipSetDevice(t->device);
+  if (materials && material_count != t->materials.size())
+    return fail(t, PT_INVALID_ARGUMENT, "update_scene_delta: material count differs from the loaded scene");
+  if (lights && light_count != t->lights.size())
+    return fail(t, PT_INVALID_ARGUMENT, "update_scene_delta: light count differs from the loaded scene");
+  if (materials)
+    for (uint32_t i = 0; i < material_count; ++i)
+      if (materials[i].material_effect != 0u)
+        return fail(t, PT_UNSUPPORTED, "material_effect != 0 (procedural albedo) is not supported on this path");
+  PT_HIP(t, hipStreamSynchronize(t->stream), "synchronize");
+  if (materials) {
+    t->materials.assign(materials, materials + material_count);
+    PT_HIP(t, hipMemcpy(t->d_materials.ptr, materials, sizeof(PtMaterial) * material_count, hipMemcpyHostToDevice),
+           "upload materials");
+    t->dscene.has_transmissive = 0;
+    for (const PtMaterial& m : t->materials)
+      if (m.material_model == 5u || m.transmission > 0.05f) t->dscene.has_transmissive = 1;
+  }
+  if (lights && light_count) {
+    t->lights.assign(lights, lights + light_count);
+    PT_HIP(t, hipMemcpy(t->d_lights.ptr, lights, sizeof(PtLight) * light_count, hipMemcpyHostToDevice),
+           "upload lights");
+  }
+  t->last_error.clear();
+  return PT_OK;
+}
+
+extern "C" int pt_hip_render_adjoint(pt_tracer* t, const PtRenderTile* tiles, uint32_t count, uint64_t frame_index,
+                                     const float* film_grad, float* grad_albedo, float* grad_emission,
+                                     float* grad_light) {
+  if (!t) return PT_INVALID_ARGUMENT;
+  if (!t->configured || !t->scene_loaded || !t->accel_valid)
+    return fail(t, PT_NOT_READY, "render_adjoint requires a built acceleration structure");
+  if (count && !tiles) return fail(t, PT_INVALID_ARGUMENT, "render_adjoint: null tiles");
+  if (!film_grad || !grad_albedo || !grad_emission || (!grad_light && !t->lights.empty()))
+    return fail(t, PT_INVALID_ARGUMENT, "render_adjoint: null gradient buffer");
+  if (t->settings.rr_start_depth > static_cast<uint32_t>(ptd::kAdjHist))
+    return fail(t, PT_UNSUPPORTED, "render_adjoint supports russian roulette start depths up to 4");
+  (void)hipSetDevice(t->device);
+  std::vector<ptd::DevTile> work;
+  int rc = build_work(t, tiles, count, work);
+  if (rc != PT_OK) return rc;
+  ptd::DevAdjoint A{};
+  A.g_albedo = grad_albedo;
+  A.g_emission = grad_emission;
+  A.g_light = grad_light;
+  A.film_grad = film_grad;
+  A.film_width = t->settings.width;
+  A.nmat = static_cast<uint32_t>(t->materials.size());
+  A.nlight = static_cast<uint32_t>(t->lights.size());
+  A.use_lds = (6u * A.nmat + 3u * A.nlight) <= static_cast<uint32_t>(ptd::kAdjLdsFloats) ? 1u : 0u;
+  rc = run_passes(t, work, dev_settings(t, frame_index), &A);
+  if (rc != PT_OK) return rc;
+  if (t->profiling) {
+    rc = sync_all(t);
+    if (rc != PT_OK) return rc;
+  }
+  t->last_error.clear();
   return PT_OK;
 }

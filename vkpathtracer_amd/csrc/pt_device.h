@@ -111,6 +111,32 @@ struct DevFilm {
   uint32_t* count;
 };
 
+// Path-replay adjoint state (no counterpart in the reference). Per path: forward radiance, the
+// loss adjoint of that radiance, the bounce awaiting its remainder radiance, and the history of
+// vertices before the roulette start depth (see k_shade). Per shadow record: the gradient pieces
+// that apply if the light turns out visible.
+constexpr int kAdjHist = 4;          // max roulette start depth supported by the adjoint
+constexpr int kAdjLdsFloats = 3072;  // block-staged gradient slots (12 KB of LDS)
+struct DevAdjoint {
+  float* ltot[3];
+  float* delta[3];
+  float* p_dx[3];
+  uint32_t* p_mat;
+  uint32_t* flags;  // bit0 pending, bit1 active, bits2-3 c*, bits4-6 nhist, bit7 prev active,
+                    // bits8-9 prev c*, bits10-12 prev nhist
+  float* h_dx[kAdjHist][3];
+  uint32_t* h_mat[kAdjHist];
+  float* s_galb[3];
+  uint32_t* s_mat;
+  float* s_glight[3];
+  uint32_t* s_light;
+  float* g_albedo;    // 3 per material
+  float* g_emission;  // 3 per material
+  float* g_light;     // 3 per light (colour)
+  const float* film_grad;  // 3 per pixel: dLoss / d(film sum)
+  uint32_t film_width, nmat, nlight, use_lds;
+};
+
 // Launchers (pt_kernels.hip). All launch on `stream`; none synchronizes.
 void launch_init_queue(hipStream_t stream, const DevPaths& paths, uint32_t live);
 void launch_raygen(hipStream_t stream, const DevPaths& paths, const DevCamera& cam, const DevSettings& st,
@@ -125,6 +151,15 @@ void launch_film(hipStream_t stream, const DevPaths& paths, const DevFilm& film,
 void launch_trace_rays(hipStream_t stream, const DevPaths& paths, const DevScene& scene, int any_hit,
                        const float* rays, const float* tmax, uint32_t count, float* out_t, int32_t* out_prim,
                        uint32_t grid);
+// Adjoint variants: raygen also gathers the per-path loss adjoint; shade and shadow accumulate
+// gradients; copy_radiance saves the forward radiance of a pass before the replay.
+void launch_raygen_adjoint(hipStream_t stream, const DevPaths& paths, const DevAdjoint& adj, const DevCamera& cam,
+                           const DevSettings& st, const DevTileTable& tiles, uint32_t path_count);
+void launch_shade_adjoint(hipStream_t stream, const DevPaths& paths, const DevAdjoint& adj, const DevScene& scene,
+                          const DevSettings& st, int in_buf, uint32_t depth, uint32_t grid);
+void launch_shadow_adjoint(hipStream_t stream, const DevPaths& paths, const DevAdjoint& adj, const DevScene& scene,
+                           uint32_t grid);
+void launch_copy_radiance(hipStream_t stream, const DevPaths& paths, const DevAdjoint& adj, uint32_t path_count);
 void launch_zero_counters(hipStream_t stream, unsigned long long* counters);
 void launch_clear_film(hipStream_t stream, const DevFilm& film, uint32_t pixels);
 

@@ -323,9 +323,11 @@ __global__ __launch_bounds__(kBlock) void k_clear_film(DevFilm film, uint32_t pi
   }
 }
 
-// camera_rays + shade_pixel seeding, ScalarCpuPathTracer.Render.cpp:55-119, :243-262
+// camera_rays + shade_pixel seeding, ScalarCpuPathTracer.Render.cpp:55-119, :243-262.
+// The adjoint variant also loads the path's loss adjoint from the film gradient.
+template <bool kAdj>
 __global__ __launch_bounds__(kBlock) void k_raygen(DevPaths P, DevRays out, DevCamera cam, DevSettings st,
-                                                  DevTileTable tiles, uint32_t path_count) {
+                                                  DevTileTable tiles, uint32_t path_count, DevAdjoint A) {
   const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
   if (p >= path_count) return;
   uint32_t lo = 0, hi = tiles.count;  // last tile with path_begin <= p
@@ -378,6 +380,81 @@ __global__ __launch_bounds__(kBlock) void k_raygen(DevPaths P, DevRays out, DevC
   P.tp[0][p] = 1.0f; P.tp[1][p] = 1.0f; P.tp[2][p] = 1.0f;
   P.rad[0][p] = 0.0f; P.rad[1][p] = 0.0f; P.rad[2][p] = 0.0f;
   P.rng[p] = prng;
+  if constexpr (kAdj) {
+    const size_t pix = static_cast<size_t>(y) * A.film_width + x;
+    A.delta[0][p] = A.film_grad[3 * pix];
+    A.delta[1][p] = A.film_grad[3 * pix + 1];
+    A.delta[2][p] = A.film_grad[3 * pix + 2];
+    A.flags[p] = 0u;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_copy_radiance(DevPaths P, DevAdjoint A, uint32_t n) {
+  const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= n) return;
+  A.ltot[0][p] = P.rad[0][p];
+  A.ltot[1][p] = P.rad[1][p];
+  A.ltot[2][p] = P.rad[2][p];
+}
+
+// ------------------------------------------------------------------ adjoint gradient accumulation
+// Gradient slots: albedo 3*nmat, emission 3*nmat, light colour 3*nlight. Staged in LDS per block
+// when they fit, else added straight to the global fp32 buffers.
+enum GradKind { kGradAlbedo = 0, kGradEmission = 1, kGradLight = 2 };
+
+__device__ __forceinline__ void gadd(const DevAdjoint& A, float* lds, GradKind kind, uint32_t idx, int c, float v) {
+  if (v == 0.0f) return;
+  const uint32_t slot = 3u * idx + static_cast<uint32_t>(c);
+  if (A.use_lds) {
+    const uint32_t off = kind == kGradAlbedo ? 0u : (kind == kGradEmission ? 3u * A.nmat : 6u * A.nmat);
+    atomicAdd(lds + off + slot, v);
+  } else {
+    float* g = kind == kGradAlbedo ? A.g_albedo : (kind == kGradEmission ? A.g_emission : A.g_light);
+    atomicAdd(g + slot, v);
+  }
+}
+
+__device__ __forceinline__ void grad_lds_clear(const DevAdjoint& A, float* lds) {
+  if (!A.use_lds) return;
+  const uint32_t n = 6u * A.nmat + 3u * A.nlight;
+  for (uint32_t i = threadIdx.x; i < n; i += kBlock) lds[i] = 0.0f;
+  __syncthreads();
+}
+
+__device__ __forceinline__ void grad_lds_flush(const DevAdjoint& A, const float* lds) {
+  if (!A.use_lds) return;
+  __syncthreads();
+  const uint32_t na = 3u * A.nmat, n = 6u * A.nmat + 3u * A.nlight;
+  for (uint32_t i = threadIdx.x; i < n; i += kBlock) {
+    const float v = lds[i];
+    if (v == 0.0f) continue;
+    if (i < na) atomicAdd(A.g_albedo + i, v);
+    else if (i < 2u * na) atomicAdd(A.g_emission + (i - na), v);
+    else atomicAdd(A.g_light + (i - 2u * na), v);
+  }
+}
+
+// Apply the bounce that waited for its remainder radiance R = L_total - L_so_far
+// (df/dx_j = delta*R_j plus the Russian-roulette terms; see the oracle's derivation).
+__device__ __forceinline__ void adj_flush(const DevAdjoint& A, float* lds, uint32_t p, V3 L, uint32_t& flags) {
+  if (!(flags & 1u)) return;
+  const float dl[3] = {A.delta[0][p], A.delta[1][p], A.delta[2][p]};
+  const float R[3] = {A.ltot[0][p] - L.x, A.ltot[1][p] - L.y, A.ltot[2][p] - L.z};
+  const float S = dl[0] * R[0] + dl[1] * R[1] + dl[2] * R[2];
+  const uint32_t m = A.p_mat[p];
+  for (int c = 0; c < 3; ++c) gadd(A, lds, kGradAlbedo, m, c, dl[c] * R[c] * A.p_dx[c][p]);
+  if (flags & 2u) {
+    const int cs = static_cast<int>((flags >> 2) & 3u);
+    const uint32_t nh = (flags >> 4) & 7u;
+    for (uint32_t h = 0; h < nh; ++h) gadd(A, lds, kGradAlbedo, A.h_mat[h][p], cs, -S * A.h_dx[h][cs][p]);
+    if (flags & 0x80u) {
+      const int pc = static_cast<int>((flags >> 8) & 3u);
+      const uint32_t pn = (flags >> 10) & 7u;
+      for (uint32_t h = 0; h < pn; ++h) gadd(A, lds, kGradAlbedo, A.h_mat[h][p], pc, S * A.h_dx[h][pc][p]);
+    }
+    flags = (flags & ~0x1F80u) | 0x80u | (static_cast<uint32_t>(cs) << 8) | (nh << 10);
+  }
+  flags &= ~1u;
 }
 
 __device__ __forceinline__ Stack make_stack(const DevPaths& P, int32_t* s_ref, float* s_t) {
@@ -536,17 +613,32 @@ struct ShadeOut {
   V3 no, nd;          // next extension ray
   V3 so, sd, contrib; // shadow ray and its radiance contribution if unoccluded
   float stmax;
+  V3 galb, glight;    // adjoint: gradient pieces applied if the light is visible
+  uint32_t gmat, glight_index;
 };
 
+__device__ __forceinline__ float albedo_mask(const PtMaterial& m, int c) {
+  return (m.albedo[c] > 0.0f && m.albedo[c] < 1.5f) ? 1.0f : 0.0f;
+}
+
 // One iteration of ScalarCpuPathTracer::trace (ScalarCpuPathTracer.cpp:1159-1287) for a path that
-// arrived at a hit (prim >= 0) or a miss (prim < 0).
+// arrived at a hit (prim >= 0) or a miss (prim < 0). kAdj replays the iteration for the adjoint.
+template <bool kAdj>
 __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc, const DevSettings& st, uint32_t depth,
                                            uint32_t p, V3 ro, V3 rd, float t, int32_t prim, ShadeOut& out,
-                                           uint32_t& shadow_tests) {
+                                           uint32_t& shadow_tests, const DevAdjoint& A, float* glds) {
   out.cont = false;
   out.shadow = false;
   V3 T = mk(P.tp[0][p], P.tp[1][p], P.tp[2][p]);
   V3 L = mk(P.rad[0][p], P.rad[1][p], P.rad[2][p]);
+  uint32_t aflags = 0;
+  V3 dl = mk(0.0f, 0.0f, 0.0f);
+  if constexpr (kAdj) {
+    if (!(finite(A.ltot[0][p]) && finite(A.ltot[1][p]) && finite(A.ltot[2][p]))) return;  // dropped sample
+    aflags = A.flags[p];
+    adj_flush(A, glds, p, L, aflags);
+    dl = mk(A.delta[0][p], A.delta[1][p], A.delta[2][p]);
+  }
   if (prim < 0) {
     L = add(L, mulv(T, ld3(sc.env)));
     P.rad[0][p] = L.x; P.rad[1][p] = L.y; P.rad[2][p] = L.z;
@@ -558,7 +650,9 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
   const V3 gN = mk(pn.x, pn.y, pn.z);
   const bool entering = dot(rd, gN) < 0.0f;
   const V3 sN = entering ? gN : neg(gN);
-  const PtMaterial& m = material_at(sc, static_cast<uint32_t>(__float_as_int(pn.w)));
+  const uint32_t raw_mat = static_cast<uint32_t>(__float_as_int(pn.w));
+  const uint32_t mat_index = raw_mat < sc.material_count - 1u ? raw_mat : sc.material_count - 1u;
+  const PtMaterial& m = sc.materials[mat_index];
   const V3 albedo = material_albedo(m);
   const float roughness = clamp01(m.roughness);
   MatFlags f;
@@ -572,13 +666,23 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
   f.toon = m.material_model == 8u;
   const V3 emissive = ld3(m.emissive);
 
+  float e_weight = -1.0f;
   if (!st.enable_nee || depth == 0u) {
     L = add(L, mulv(T, emissive));
+    e_weight = 1.0f;
   } else if (st.enable_mis && f.emissive_model) {
     const float bsdf_pdf = smax(0.0f, dot(sN, neg(rd))) * kInvPi;
     const float l_pdf = light_pdf_for_direction(sc, ro, rd);
     const float w = mis_weight(bsdf_pdf, l_pdf);
     L = add(L, muls(mulv(T, emissive), w));
+    e_weight = w;
+  }
+  if constexpr (kAdj) {
+    if (e_weight >= 0.0f) {
+      gadd(A, glds, kGradEmission, mat_index, 0, dl.x * T.x * e_weight);
+      gadd(A, glds, kGradEmission, mat_index, 1, dl.y * T.y * e_weight);
+      gadd(A, glds, kGradEmission, mat_index, 2, dl.z * T.z * e_weight);
+    }
   }
 
   const bool last = depth + 1u >= st.max_depth;
@@ -586,7 +690,8 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
     // sample_direct_light, ScalarCpuPathTracer.cpp:1022-1118
     const uint32_t nl = sc.light_count;
     const uint64_t li = static_cast<uint64_t>(next01(rng) * static_cast<float>(nl));
-    const PtLight& light = sc.lights[li < nl - 1u ? li : nl - 1u];
+    const uint32_t light_index = static_cast<uint32_t>(li < nl - 1u ? li : nl - 1u);
+    const PtLight& light = sc.lights[light_index];
     V3 lp = ld3(light.position);
     if (light.radius > 1.0e-5f) {
       const float uz = 1.0f - 2.0f * next01(rng);
@@ -607,11 +712,18 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
         if (spot > 0.0f) {
           ++shadow_tests;
           const V3 view_dir = neg(rd);
-          const V3 irradiance = muls(ld3(light.color), (light.intensity * spot) / (dist_sq + kEps));
+          const float kscale = (light.intensity * spot) / (dist_sq + kEps);
+          const V3 irradiance = muls(ld3(light.color), kscale);
           const float nlf = static_cast<float>(nl);
           V3 direct = mk(0.0f, 0.0f, 0.0f);
-          if (!f.mirror && !f.transmissive)
+          V3 d_alb = mk(0.0f, 0.0f, 0.0f), d_col = mk(0.0f, 0.0f, 0.0f);  // adjoint partials
+          if (!f.mirror && !f.transmissive) {
             direct = add(direct, muls(muls(mulv(muls(albedo, kInvPi), irradiance), cos_theta), nlf));
+            if constexpr (kAdj) {
+              d_alb = muls(irradiance, kInvPi * cos_theta * nlf);
+              d_col = muls(albedo, kInvPi * kscale * cos_theta * nlf);
+            }
+          }
           if (f.mirror || f.metallic || f.clearcoat || f.transmissive || roughness < 0.65f) {
             const V3 half_dir = normalize(add(light_dir, normalize(view_dir)));
             const float effR = smax(0.025f, roughness * (f.metallic ? 0.65f : 1.0f));
@@ -626,17 +738,31 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
                                                (f.mirror ? 0.75f : 0.0f) + (f.transmissive ? fresnel : 0.0f));
             const V3 specTint = add(muls(mk(1.0f, 1.0f, 1.0f), f.metallic ? 0.15f : 0.88f),
                                     muls(albedo, f.metallic ? 0.85f : 0.12f));
-            direct = add(direct, muls(mulv(specTint, irradiance), spec * cos_theta * nlf * smax(0.15f, specStrength)));
+            const float s = spec * cos_theta * nlf * smax(0.15f, specStrength);
+            direct = add(direct, muls(mulv(specTint, irradiance), s));
+            if constexpr (kAdj) {
+              d_alb = add(d_alb, muls(irradiance, (f.metallic ? 0.85f : 0.12f) * s));
+              d_col = add(d_col, muls(specTint, kscale * s));
+            }
           }
           V3 contrib;
+          float w = 1.0f;
           if (st.enable_mis) {
             const V3 l_dir = normalize(sub(ld3(sc.lights[0].position), pos));
             const float ct = smax(0.0f, dot(sN, l_dir));
             const float bsdf_pdf = ct * kInvPi;
             const float l_pdf = 1.0f / nlf;
-            contrib = muls(mulv(T, direct), mis_weight(l_pdf, bsdf_pdf));
+            w = mis_weight(l_pdf, bsdf_pdf);
+            contrib = muls(mulv(T, direct), w);
           } else {
             contrib = mulv(T, direct);
+          }
+          V3 galb = mk(0.0f, 0.0f, 0.0f), glight = mk(0.0f, 0.0f, 0.0f);
+          if constexpr (kAdj) {
+            const V3 tw = muls(mulv(dl, T), w);
+            galb = mk(tw.x * d_alb.x * albedo_mask(m, 0), tw.y * d_alb.y * albedo_mask(m, 1),
+                      tw.z * d_alb.z * albedo_mask(m, 2));
+            glight = mulv(tw, d_col);
           }
           const float remaining = dist - 0.004f;
           if (remaining > 0.004f) {
@@ -645,8 +771,20 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
             out.sd = light_dir;
             out.stmax = remaining;
             out.contrib = contrib;
+            out.galb = galb;
+            out.glight = glight;
+            out.gmat = mat_index;
+            out.glight_index = light_index;
           } else {
             L = add(L, contrib);
+            if constexpr (kAdj) {
+              gadd(A, glds, kGradAlbedo, mat_index, 0, galb.x);
+              gadd(A, glds, kGradAlbedo, mat_index, 1, galb.y);
+              gadd(A, glds, kGradAlbedo, mat_index, 2, galb.z);
+              gadd(A, glds, kGradLight, light_index, 0, glight.x);
+              gadd(A, glds, kGradLight, light_index, 1, glight.y);
+              gadd(A, glds, kGradLight, light_index, 2, glight.z);
+            }
           }
         }
       }
@@ -693,8 +831,39 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
   if (f.toon) bw = muls(bw, dot(outDir, sN) > 0.55f ? 1.0f : 0.45f);
   T = mulv(T, bw);
   if (vmax3(T) < 0.001f) return;
-  const float rr = smin(st.rr_max, smax(st.rr_min, vmax3(T)));
+  const float maxT = vmax3(T);
+  const float rr = smin(st.rr_max, smax(st.rr_min, maxT));
   if (depth >= st.rr_start && next01(rng) > rr) return;
+  if constexpr (kAdj) {
+    // d log bw / d albedo per channel, and the channel std::max picks for rr
+    const int cstar = (T.x < smax(T.y, T.z)) ? ((T.y < T.z) ? 2 : 1) : 0;
+    const bool trans_tint = f.transmissive && !(f.metallic || f.mirror);
+    const float tint = trans_tint ? clamp01(m.alpha) * m.transmission : 0.0f;
+    const float a3[3] = {albedo.x, albedo.y, albedo.z}, b3[3] = {bw.x, bw.y, bw.z};
+    float dx[3];
+    for (int c = 0; c < 3; ++c) {
+      float v;
+      if (trans_tint) {
+        const float base = (1.0f - tint) + a3[c] * tint;
+        v = base > 0.0f ? tint / base : 0.0f;
+      } else {
+        v = a3[c] > 0.0f ? 1.0f / a3[c] : 0.0f;
+      }
+      dx[c] = b3[c] > 0.0f ? v * albedo_mask(m, c) : 0.0f;
+      A.p_dx[c][p] = dx[c];
+    }
+    A.p_mat[p] = mat_index;
+    const bool active = depth < st.rr_start && st.rr_min < maxT && maxT < st.rr_max;
+    aflags = (aflags & ~0xFu) | 1u | (active ? 2u : 0u) | (static_cast<uint32_t>(cstar) << 2);
+    uint32_t nh = (aflags >> 4) & 7u;
+    if (depth < st.rr_start && nh < static_cast<uint32_t>(kAdjHist)) {
+      A.h_mat[nh][p] = mat_index;
+      for (int c = 0; c < 3; ++c) A.h_dx[nh][c][p] = dx[c];
+      ++nh;
+      aflags = (aflags & ~0x70u) | (nh << 4);
+    }
+    A.flags[p] = aflags;
+  }
   T = divs(T, rr);
   P.tp[0][p] = T.x; P.tp[1][p] = T.y; P.tp[2][p] = T.z;
   P.rng[p] = rng;
@@ -703,8 +872,11 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
   out.nd = outDir;
 }
 
+template <bool kAdj>
 __global__ __launch_bounds__(kBlock) void k_shade(DevPaths P, DevRays in, DevRays out, DevScene sc, DevSettings st,
-                                                 uint32_t depth) {
+                                                 uint32_t depth, DevAdjoint A) {
+  __shared__ float glds[kAdj ? kAdjLdsFloats : 1];
+  if constexpr (kAdj) grad_lds_clear(A, glds);
   const uint32_t n = P.queue[kQueueIn];
   const uint32_t stride = gridDim.x * kBlock;
   const uint32_t lane = threadIdx.x & 63u;
@@ -719,7 +891,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevPaths P, DevRays in, DevRay
       p = in.qpath[i];
       const V3 ro = mk(in.c[0][i], in.c[1][i], in.c[2][i]);
       const V3 rd = mk(in.c[3][i], in.c[4][i], in.c[5][i]);
-      shade_path(P, sc, st, depth, p, ro, rd, P.hit_t[i], P.hit_prim[i], so, shadow_tests);
+      shade_path<kAdj>(P, sc, st, depth, p, ro, rd, P.hit_t[i], P.hit_prim[i], so, shadow_tests, A, glds);
     }
     const uint32_t slot = wave_append(&P.queue[kQueueOut], so.cont);
     if (so.cont) {
@@ -734,11 +906,18 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevPaths P, DevRays in, DevRay
       P.stmax[sslot] = so.stmax;
       P.scontrib[0][sslot] = so.contrib.x; P.scontrib[1][sslot] = so.contrib.y; P.scontrib[2][sslot] = so.contrib.z;
       P.spath[sslot] = p;
+      if constexpr (kAdj) {
+        A.s_galb[0][sslot] = so.galb.x; A.s_galb[1][sslot] = so.galb.y; A.s_galb[2][sslot] = so.galb.z;
+        A.s_glight[0][sslot] = so.glight.x; A.s_glight[1][sslot] = so.glight.y; A.s_glight[2][sslot] = so.glight.z;
+        A.s_mat[sslot] = so.gmat;
+        A.s_light[sslot] = so.glight_index;
+      }
       ++shadow_rays;
     }
   }
   wave_add(P.counters + kShadowTests, shadow_tests);
   wave_add(P.counters + kShadowRays, shadow_rays);
+  if constexpr (kAdj) grad_lds_flush(A, glds);
 }
 
 // Visibility of one NEE sample. Opaque scenes: any-hit query with t_max = dist - 0.004.
@@ -768,9 +947,12 @@ __device__ __forceinline__ bool shadow_visible(const DevScene& sc, const Stack& 
   return true;
 }
 
-__global__ __launch_bounds__(kBlock) void k_shadow(DevPaths P, DevScene sc) {
+template <bool kAdj>
+__global__ __launch_bounds__(kBlock) void k_shadow(DevPaths P, DevScene sc, DevAdjoint A) {
   __shared__ int32_t s_ref[kStackLds * kBlock];
   __shared__ float s_t[kStackLds * kBlock];
+  __shared__ float glds[kAdj ? kAdjLdsFloats : 1];
+  if constexpr (kAdj) grad_lds_clear(A, glds);
   const Stack stk = make_stack(P, s_ref, s_t);
   const uint32_t n = P.queue[kQueueShadow];
   const uint32_t lane = threadIdx.x & 63u;
@@ -789,10 +971,18 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevPaths P, DevScene sc) {
         P.rad[0][p] = P.rad[0][p] + P.scontrib[0][i];
         P.rad[1][p] = P.rad[1][p] + P.scontrib[1][i];
         P.rad[2][p] = P.rad[2][p] + P.scontrib[2][i];
+        if constexpr (kAdj) {
+          const uint32_t mi = A.s_mat[i], li = A.s_light[i];
+          for (int c = 0; c < 3; ++c) {
+            gadd(A, glds, kGradAlbedo, mi, c, A.s_galb[c][i]);
+            gadd(A, glds, kGradLight, li, c, A.s_glight[c][i]);
+          }
+        }
       }
     }
   }
   flush_stats(P.counters, st, true);
+  if constexpr (kAdj) grad_lds_flush(A, glds);
 }
 
 // FilmBuffer::add_sample for one tile, after shade_pixel's non-finite drop
@@ -869,8 +1059,21 @@ void launch_clear_film(hipStream_t stream, const DevFilm& film, uint32_t pixels)
 void launch_raygen(hipStream_t stream, const DevPaths& paths, const DevCamera& cam, const DevSettings& st,
                    const DevTileTable& tiles, uint32_t path_count) {
   if (path_count == 0) return;
-  hipLaunchKernelGGL(k_raygen, dim3((path_count + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, paths,
-                     paths.rays[0], cam, st, tiles, path_count);
+  hipLaunchKernelGGL(k_raygen<false>, dim3((path_count + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, paths,
+                     paths.rays[0], cam, st, tiles, path_count, DevAdjoint{});
+}
+
+void launch_raygen_adjoint(hipStream_t stream, const DevPaths& paths, const DevAdjoint& adj, const DevCamera& cam,
+                           const DevSettings& st, const DevTileTable& tiles, uint32_t path_count) {
+  if (path_count == 0) return;
+  hipLaunchKernelGGL(k_raygen<true>, dim3((path_count + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, paths,
+                     paths.rays[0], cam, st, tiles, path_count, adj);
+}
+
+void launch_copy_radiance(hipStream_t stream, const DevPaths& paths, const DevAdjoint& adj, uint32_t path_count) {
+  if (path_count == 0) return;
+  hipLaunchKernelGGL(k_copy_radiance, dim3((path_count + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, paths, adj,
+                     path_count);
 }
 
 void launch_extend(hipStream_t stream, const DevPaths& paths, const DevScene& scene, int in_buf, uint32_t grid) {
@@ -879,12 +1082,23 @@ void launch_extend(hipStream_t stream, const DevPaths& paths, const DevScene& sc
 
 void launch_shade(hipStream_t stream, const DevPaths& paths, const DevScene& scene, const DevSettings& st, int in_buf,
                   uint32_t depth, uint32_t grid) {
-  hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kBlock), 0, stream, paths, paths.rays[in_buf], paths.rays[in_buf ^ 1],
-                     scene, st, depth);
+  hipLaunchKernelGGL(k_shade<false>, dim3(grid), dim3(kBlock), 0, stream, paths, paths.rays[in_buf],
+                     paths.rays[in_buf ^ 1], scene, st, depth, DevAdjoint{});
+}
+
+void launch_shade_adjoint(hipStream_t stream, const DevPaths& paths, const DevAdjoint& adj, const DevScene& scene,
+                          const DevSettings& st, int in_buf, uint32_t depth, uint32_t grid) {
+  hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(kBlock), 0, stream, paths, paths.rays[in_buf],
+                     paths.rays[in_buf ^ 1], scene, st, depth, adj);
 }
 
 void launch_shadow(hipStream_t stream, const DevPaths& paths, const DevScene& scene, uint32_t grid) {
-  hipLaunchKernelGGL(k_shadow, dim3(grid), dim3(kBlock), 0, stream, paths, scene);
+  hipLaunchKernelGGL(k_shadow<false>, dim3(grid), dim3(kBlock), 0, stream, paths, scene, DevAdjoint{});
+}
+
+void launch_shadow_adjoint(hipStream_t stream, const DevPaths& paths, const DevAdjoint& adj, const DevScene& scene,
+                           uint32_t grid) {
+  hipLaunchKernelGGL(k_shadow<true>, dim3(grid), dim3(kBlock), 0, stream, paths, scene, adj);
 }
 
 void launch_film(hipStream_t stream, const DevPaths& paths, const DevFilm& film, uint32_t film_width,
@@ -912,14 +1126,14 @@ static uint32_t occupancy_grid(int device, const void* fn) {
 
 uint32_t persistent_grid_extend(int device) {
   const uint32_t a = occupancy_grid(device, reinterpret_cast<const void*>(&k_extend));
-  const uint32_t b = occupancy_grid(device, reinterpret_cast<const void*>(&k_shadow));
+  const uint32_t b = occupancy_grid(device, reinterpret_cast<const void*>(&k_shadow<true>));
   const uint32_t c = occupancy_grid(device, reinterpret_cast<const void*>(&k_trace_rays));
   uint32_t m = a > b ? a : b;
   return m > c ? m : c;
 }
 
 uint32_t persistent_grid_shade(int device) {
-  return occupancy_grid(device, reinterpret_cast<const void*>(&k_shade));
+  return occupancy_grid(device, reinterpret_cast<const void*>(&k_shade<true>));
 }
 
 }  // namespace ptd

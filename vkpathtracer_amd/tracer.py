@@ -139,6 +139,40 @@ class HipPathTracer:
             arr[i] = abi.PtRenderTile(*[int(v) for v in (tuple(t) + (0, 0, 0, 0, 0, 0, 0))[:7]])
         return self._lib.pt_hip_render_tiles(self._h, arr, len(tiles), frame_index) == 0
 
+    @staticmethod
+    def _tile_array(tiles):
+        arr = (abi.PtRenderTile * max(1, len(tiles)))()
+        for i, t in enumerate(tiles):
+            if isinstance(t, dict):
+                t = (t.get("x", 0), t.get("y", 0), t["width"], t["height"], t.get("sample_index", 0),
+                     t.get("tile_id", 0), t.get("gpu_id", 0))
+            arr[i] = abi.PtRenderTile(*[int(v) for v in (tuple(t) + (0, 0, 0, 0, 0, 0, 0))[:7]])
+        return arr
+
+    def update_scene_delta(self, materials: np.ndarray | None = None, lights: np.ndarray | None = None) -> bool:
+        """IPathTracer::update_scene_delta: replace material and/or light records (same counts)."""
+        from .scene import LIGHT_DTYPE, MATERIAL_DTYPE
+
+        m = None if materials is None else np.ascontiguousarray(materials, MATERIAL_DTYPE)
+        l = None if lights is None else np.ascontiguousarray(lights, LIGHT_DTYPE)
+        rc = self._lib.pt_hip_update_scene_delta(self._h, None if m is None else m.ctypes.data, 0 if m is None else len(m),
+                                                 None if l is None else l.ctypes.data, 0 if l is None else len(l))
+        return rc == 0
+
+    def render_adjoint(self, tiles, film_grad, grad_albedo, grad_emission, grad_light, frame_index: int = 0):
+        """Path-replay adjoint of render_tiles(tiles): adds d(sum film_grad * film)/d(parameters) into the
+        device tensors grad_albedo / grad_emission (M, 3) and grad_light (L, 3). film_grad: (H, W, 3)
+        device tensor. Torch tensors must be contiguous float32 on this tracer's device."""
+        import torch
+
+        torch.cuda.synchronize(self.device)
+        arr = self._tile_array(tiles)
+        rc = self._lib.pt_hip_render_adjoint(self._h, arr, len(tiles), frame_index, film_grad.data_ptr(),
+                                             grad_albedo.data_ptr(), grad_emission.data_ptr(),
+                                             grad_light.data_ptr() if grad_light.numel() else None)
+        self._check(rc, "render_adjoint")
+        self.synchronize()
+
     def render_frame_samples(self, sample_begin: int, sample_count: int, frame_index: int = 0) -> bool:
         """Full-frame tiles for samples [begin, begin+count) in one batched call (render_tiles)."""
         s = self._settings
