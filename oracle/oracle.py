@@ -70,6 +70,8 @@ _SIGS = {
     "pto_sinf": (C.c_float, [C.c_float]),
     "pto_cosf": (C.c_float, [C.c_float]),
     "pto_powf": (C.c_float, [C.c_float, C.c_float]),
+    "pto_atan2f": (C.c_float, [C.c_float, C.c_float]),
+    "pto_acosf": (C.c_float, [C.c_float]),
     "pto_splitmix64": (C.c_uint64, [C.c_uint64]),
     "pto_rng_seed": (C.c_uint64, [C.c_uint64] * 7),
     "pto_rng_draws": (None, [C.c_uint64, C.c_uint32, _P, _P]),
@@ -124,7 +126,11 @@ class OracleScene:
         s.vertices, s.indices = _ptr(sc.vertices), _ptr(sc.indices)
         s.local_vertices, s.local_indices = _ptr(sc.local_vertices), _ptr(sc.local_indices)
         s.instances, s.materials, s.lights = _ptr(sc.instances), _ptr(sc.materials), _ptr(sc.lights)
-        s.environment_map = None
+        env = getattr(sc, "environment_map", None)
+        s.environment_map = _ptr(env)
+        if env is not None:
+            s.environment_map_height, s.environment_map_width = env.shape[0], env.shape[1]
+        s.environment_map_scale[:] = list(getattr(sc, "environment_map_scale", (0.0, 0.0, 0.0)))
         s.vertex_count, s.index_count = len(sc.vertices), len(sc.indices)
         s.local_vertex_count, s.local_index_count = len(sc.local_vertices), len(sc.local_indices)
         s.instance_count, s.material_count, s.light_count = len(sc.instances), len(sc.materials), len(sc.lights)

@@ -106,6 +106,25 @@ float pto_sinf(float x) { return (float)R_SIN((real)x); }
 float pto_cosf(float x) { return (float)R_COS((real)x); }
 float pto_powf(float x, float y) { return (float)R_POW((real)x, (real)y); }
 
+float pto_atan2f(float y, float x) {
+#if PTO_REAL_IS_DOUBLE
+  return (float)atan2((double)y, (double)x);
+#elif PTO_LIBM
+  return atan2f(y, x);
+#else
+  return pm_atan2f(y, x);
+#endif
+}
+float pto_acosf(float x) {
+#if PTO_REAL_IS_DOUBLE
+  return (float)acos((double)x);
+#elif PTO_LIBM
+  return acosf(x);
+#else
+  return pm_acosf(x);
+#endif
+}
+
 uint32_t pto_sizeof(int which) {
   switch (which) {
     case 0: return sizeof(PtoMaterial);
@@ -729,10 +748,189 @@ static const PtoMaterial* material_at(const PtoScene* sc, uint32_t index) {
   return &sc->materials[index < last ? index : last];
 }
 
-/* apply_material_effect for material_effect == 0: identity then clamp to [0, 1.5]
- * (ScalarCpuPathTracer.cpp:264-355). Other effects are rejected at entry. */
-static v3 material_albedo(const PtoMaterial* m) {
-  const v3 c = v3f(m->albedo);
+#if PTO_REAL_IS_DOUBLE
+#define R_FLOOR(x) floor(x)
+#define R_TRUNC(x) trunc(x)
+#define R_ATAN2(y, x) atan2((y), (x))
+#define R_ACOS(x) acos(x)
+#else
+#define R_FLOOR(x) floorf(x)
+#define R_TRUNC(x) truncf(x)
+#if PTO_LIBM
+#define R_ATAN2(y, x) atan2f((y), (x))
+#define R_ACOS(x) acosf(x)
+#else
+#define R_ATAN2(y, x) pm_atan2f((y), (x))
+#define R_ACOS(x) pm_acosf(x)
+#endif
+#endif
+
+/* std::fmod(x, 2) for the integer-valued sums the effects pass (exact). */
+static inline real fmod2(real x) { return x - RC(2.0) * R_TRUNC(x * RC(0.5)); }
+
+/* hash01 / smoothstep: ScalarCpuPathTracer.cpp:187-196 */
+static inline real hash01(real x, real y, real z, real seed) {
+  const real v = R_SIN(x * RC(12.9898) + y * RC(78.233) + z * RC(37.719) + seed * RC(19.19)) * RC(43758.5453);
+  return v - R_FLOOR(v);
+}
+static inline real smoothstep_r(real e0, real e1, real x) {
+  const real t = clamp01((x - e0) / smax(RC(1.0e-6), e1 - e0));
+  return t * t * (RC(3.0) - RC(2.0) * t);
+}
+
+/* procedural_wood_albedo: ScalarCpuPathTracer.cpp:198-262. Linear in `base`: *k receives the factor. */
+static v3 wood_albedo(v3 base, v3 p, real roughness, real clearcoat, uint32_t style, real* k) {
+  const real rough = clamp01(roughness), coat = clamp01(clearcoat);
+  const real polish = clamp01(coat * (RC(1.25) - rough * RC(0.75)));
+  const real parquet = style == 17u ? RC(1.0) : RC(0.0);
+  const real walnut = style == 18u ? RC(1.0) : RC(0.0);
+  const real sandal = style == 19u ? RC(1.0) : RC(0.0);
+  const real plank_width = RC(0.40) + RC(0.16) * parquet + RC(0.08) * polish - RC(0.18) * walnut;
+  const real plank_length = RC(1.15) + RC(0.85) * parquet + RC(0.55) * polish - RC(0.55) * walnut;
+  const real px = (p.x + sandal * p.y * RC(0.35)) / smax(RC(0.12), plank_width);
+  const real pz = (p.z + walnut * p.y * RC(0.70) + sandal * p.x * RC(0.22)) / smax(RC(0.30), plank_length);
+  const real cell_x = R_FLOOR(px), cell_z = R_FLOOR(pz);
+  const real local_x = px - cell_x, local_z = pz - cell_z;
+  const real seed = hash01(cell_x, cell_z, RC(0.0), RC(16.0));
+  const real macro_noise = hash01(R_FLOOR(p.x * RC(3.7)), R_FLOOR(p.y * RC(3.7)), R_FLOOR(p.z * RC(3.7)),
+                                  RC(31.0) + (real)style);
+  const real micro_noise = hash01(R_FLOOR(p.x * RC(34.0)), R_FLOOR(p.y * RC(34.0)), R_FLOOR(p.z * RC(34.0)),
+                                  RC(47.0) + (real)style);
+  const int rotate = parquet > RC(0.5) && fmod2(cell_x + cell_z) >= RC(1.0);
+  const real along = rotate ? local_z : local_x, across = rotate ? local_x : local_z;
+  const real grain_wobble = (macro_noise - RC(0.5)) * (RC(0.06) + rough * RC(0.12) + walnut * RC(0.08) + sandal * RC(0.04));
+  const real warped_along = along + grain_wobble + R_SIN(across * RC(6.0) + seed * RC(5.0)) * RC(0.018);
+  const real grain_freq = RC(12.0) + rough * RC(8.0) + walnut * RC(22.0) + sandal * RC(36.0);
+  const real wave = R_SIN((warped_along * grain_freq + seed * RC(6.2831853)) +
+                          R_SIN(across * (RC(6.0) + rough * RC(8.0)) + seed * RC(4.0)) *
+                              (RC(0.25) + rough * RC(0.45) + walnut * RC(0.45)));
+  const real fine = R_SIN(warped_along * (RC(55.0) + rough * RC(55.0) + sandal * RC(110.0)) +
+                          across * (RC(9.0) + walnut * RC(25.0)) + seed * RC(12.0));
+  const real fine_weight = RC(0.10) + rough * RC(0.18) + walnut * RC(0.18) + sandal * RC(0.22);
+  const real ring = smoothstep_r(RC(-0.35) + polish * RC(0.18), RC(0.82), wave) * (RC(1.0) - fine_weight) +
+                    (RC(0.5) + RC(0.5) * fine) * fine_weight;
+  const real seam_x = smin(local_x, RC(1.0) - local_x), seam_z = smin(local_z, RC(1.0) - local_z);
+  const real seam = (RC(1.0) - sandal) *
+                    (RC(1.0) - smoothstep_r(RC(0.0), RC(0.028) + polish * RC(0.018), smin(seam_x, seam_z)));
+  const real kd = RC(0.46) - rough * RC(0.10) - walnut * RC(0.18);
+  const real ka = RC(1.14) + coat * RC(0.22) + sandal * RC(0.18);
+  const v3 dark = muls(base, kd);
+  const v3 amber = add(muls(base, ka), mk(RC(0.08) + sandal * RC(0.18), RC(0.055) + sandal * RC(0.06), RC(0.025)));
+  v3 wood = add(muls(dark, RC(1.0) - ring), muls(amber, ring));
+  const real board_tint = RC(0.92) + (RC(0.08) + walnut * RC(0.12) + sandal * RC(0.06)) * seed +
+                          (macro_noise - RC(0.5)) * (RC(0.08) + rough * RC(0.06));
+  const real pores = smoothstep_r(RC(0.74) - walnut * RC(0.08), RC(1.0), micro_noise) *
+                     (RC(0.025) + rough * RC(0.055) + walnut * RC(0.055) + sandal * RC(0.018));
+  const real seam_f = RC(1.0) - seam * (RC(0.10) + rough * RC(0.35) + parquet * RC(0.18)) * (RC(1.0) - coat * RC(0.45));
+  wood = muls(wood, board_tint);
+  wood = muls(wood, RC(1.0) - pores);
+  wood = muls(wood, seam_f);
+  const real wood_mix = RC(0.30) + rough * RC(0.34) - polish * RC(0.14) + walnut * RC(0.26) + sandal * RC(0.18);
+  *k = (RC(1.0) - wood_mix) + (kd * (RC(1.0) - ring) + ka * ring) * board_tint * (RC(1.0) - pores) * seam_f * wood_mix;
+  return add(muls(base, RC(1.0) - wood_mix), muls(wood, wood_mix));
+}
+
+/* apply_material_effect: ScalarCpuPathTracer.cpp:264-355. Every effect is affine in the albedo
+ * (colour = k * albedo + b before the final [0, 1.5] clamp): *k receives k and pre[] the colour
+ * before the clamp, which the adjoint needs. */
+static v3 effect_albedo(const PtoMaterial* m, v3 pos, v3 n, v3 incoming, real* k_out, v3* pre_out) {
+  v3 c = v3f(m->albedo);
+  real k = RC(1.0);
+  const uint32_t e = m->material_effect;
+  if (e != 0u) {
+    const real h = hash01(pos.x, pos.y, pos.z, (real)e);
+    const real ndi = dot(n, neg(incoming));
+    switch (e) {
+      case 1u: {
+        const real r0 = smax(RC(0.0), RC(1.0) - R_FABS(ndi));
+        const real rim = r0 * r0;
+        k = RC(0.65) + RC(0.25) * h;
+        c = add(muls(c, k), muls(mk(RC(0.25), RC(0.22), RC(0.28)), rim * (RC(0.4) + (real)m->sheen)));
+        break;
+      }
+      case 2u: {
+        const real stripes = RC(0.5) + RC(0.5) * R_SIN(pos.x * RC(7.0) + pos.z * RC(5.0) + h * RC(6.0));
+        k = RC(0.45) + RC(0.55) * stripes;
+        c = muls(c, k);
+        break;
+      }
+      case 3u:
+        k = h > RC(0.72) ? RC(0.18) : RC(1.0);
+        c = muls(c, k);
+        break;
+      case 4u: {
+        const real vein = RC(0.5) + RC(0.5) * R_SIN((pos.x + pos.y * RC(0.4) + pos.z * RC(0.7)) * RC(9.0) + h * RC(3.0));
+        k = RC(0.55) + RC(0.45) * vein;
+        c = add(muls(c, k), muls(mk(RC(0.18), RC(0.20), RC(0.23)), RC(1.0) - vein));
+        break;
+      }
+      case 5u:
+        k = RC(0.55) + RC(0.35) * h;
+        c = add(muls(c, k), muls(mk(RC(0.45), RC(0.16), RC(0.04)), RC(0.25) + RC(0.25) * h));
+        break;
+      case 6u:
+        k = RC(0.65) + RC(0.35) * h;
+        c = muls(c, k);
+        break;
+      case 7u:
+        k = RC(0.78);
+        c = add(muls(c, k), muls(mk(RC(1.0), RC(0.62), RC(0.42)), RC(0.16)));
+        break;
+      case 8u:
+        k = RC(0.55);
+        c = add(muls(c, k), muls(mk(RC(0.35) + RC(0.45) * h, RC(0.25) + RC(0.35) * (RC(1.0) - h), RC(0.85)), RC(0.45)));
+        break;
+      case 9u: {
+        const real check = fmod2(R_FLOOR(pos.x * RC(4.0)) + R_FLOOR(pos.z * RC(4.0)));
+        k = check < RC(0.5) ? RC(1.0) : RC(0.28);
+        c = muls(c, k);
+        break;
+      }
+      case 10u:
+        k = RC(0.45);
+        c = add(muls(c, k), muls(mk(RC(1.0), RC(0.35) + RC(0.3) * h, RC(0.08)), RC(0.55)));
+        break;
+      case 11u: {
+        const real streak = RC(0.5) + RC(0.5) * R_SIN(pos.y * RC(18.0) + h * RC(5.0));
+        k = RC(0.55) + RC(0.45) * streak;
+        c = muls(c, k);
+        break;
+      }
+      case 12u: {
+        const real v = smax(RC(0.0), ndi);
+        const real v2 = v * v;
+        c = add(c, muls(mk(RC(0.55), RC(0.65), RC(0.95)), v2 * v2 * v2));
+        break;
+      }
+      case 13u:
+        k = RC(0.65) + RC(0.35) * R_FABS(R_SIN(pos.x * RC(10.0)) * R_COS(pos.z * RC(10.0)));
+        c = muls(c, k);
+        break;
+      case 14u: {
+        const real rim = R_POW(smax(RC(0.0), RC(1.0) - R_FABS(ndi)), RC(0.8));
+        k = RC(0.0);
+        c = muls(mk(RC(0.15), RC(0.75), RC(1.0)), RC(0.2) + RC(0.8) * rim);
+        break;
+      }
+      case 15u: {
+        const real bands = RC(0.5) + RC(0.5) * R_SIN(pos.y * RC(11.0) + pos.x * RC(3.0) + h * RC(7.0));
+        const real rim = R_POW(smax(RC(0.0), RC(1.0) - R_FABS(ndi)), RC(1.5));
+        k = RC(0.25) + RC(0.45) * bands;
+        c = add(muls(c, k), muls(mk(RC(0.48), RC(0.56), RC(0.68)), RC(0.18) + RC(0.32) * rim));
+        break;
+      }
+      case 16u:
+      case 17u:
+      case 18u:
+      case 19u:
+        c = wood_albedo(c, pos, (real)m->roughness, (real)m->clearcoat, e, &k);
+        break;
+      default:
+        break;
+    }
+  }
+  if (k_out) *k_out = k;
+  if (pre_out) *pre_out = c;
   return mk(smin(RC(1.5), smax(RC(0.0), c.x)), smin(RC(1.5), smax(RC(0.0), c.y)), smin(RC(1.5), smax(RC(0.0), c.z)));
 }
 
@@ -759,6 +957,8 @@ typedef struct {
   int valid;
   v3 d_albedo, d_color;
   uint32_t light;
+  real eff_k;  /* d(effect colour)/d(albedo) before the clamp, and that pre-clamp colour */
+  v3 eff_pre;
 } nee_t;
 
 static int shadow_visible(const ctx_t* x, v3 o, v3 d, real dist, PtoCounters* c) {
@@ -821,7 +1021,7 @@ static nee_t sample_direct_light(const ctx_t* x, const hit_t* hit, v3 view_dir, 
   c->shadow_tests++;
   const v3 so = add(hit->position, muls(hit->normal, RC(0.002)));
   if (!shadow_visible(x, so, light_dir, dist, c)) return r;
-  const v3 albedo = material_albedo(m);
+  const v3 albedo = effect_albedo(m, hit->position, hit->normal, neg(light_dir), &r.eff_k, &r.eff_pre);
   const real kscale = ((real)light->intensity * spot) / (dist_sq + kEps);
   const v3 irradiance = muls(v3f(light->color), kscale);
   v3 direct = mk(0, 0, 0);
@@ -931,8 +1131,37 @@ static v3 refract_dir(v3 incoming, v3 normal, real eta, int* tir) {
 
 static v3 reflect_about(v3 d, v3 n) { return sub(d, muls(n, RC(2.0) * dot(n, d))); }
 
-/* environment: SampleSceneEnvironment, PathTracer.cpp:213-272 (constant colour; maps are rejected at entry) */
-static v3 environment(const PtoScene* sc) { return v3f(sc->environment_color); }
+/* SampleSceneEnvironment, PathTracer.cpp:213-272: constant colour, or a bilinear lookup in an
+ * equirectangular RGB map. */
+static v3 environment(const PtoScene* sc, v3 d) {
+  const uint32_t w = sc->environment_map_width, h = sc->environment_map_height;
+  if (w == 0u || h == 0u || !sc->environment_map) return v3f(sc->environment_color);
+  const real len_sq = d.x * d.x + d.y * d.y + d.z * d.z;
+  if (len_sq <= RC(1.0e-12)) return v3f(sc->environment_color);
+  const real inv_len = RC(1.0) / R_SQRT(len_sq);
+  const v3 dir = mk(d.x * inv_len, d.y * inv_len, d.z * inv_len);
+  const real kInvTwoPi = RC(0.15915494309189533577), kInvPiE = RC(0.31830988618379067154);
+  real u = RC(0.5) + R_ATAN2(dir.z, dir.x) * kInvTwoPi;
+  u -= R_FLOOR(u);
+  const real v = R_ACOS(sclamp(dir.y, RC(-1.0), RC(1.0))) * kInvPiE;
+  const real xf = u * (real)w;
+  const real yf = sclamp(v, RC(0.0), RC(1.0)) * (real)(h - 1u);
+  const uint32_t x0 = (uint32_t)R_FLOOR(xf) % w, x1 = (x0 + 1u) % w;
+  const uint32_t y0 = (uint32_t)R_FLOOR(yf), y1 = y0 + 1u < h - 1u ? y0 + 1u : h - 1u;
+  const real tx = xf - R_FLOOR(xf), ty = yf - R_FLOOR(yf);
+  const float* m = sc->environment_map;
+  const v3 c00 = v3f(m + 3u * ((size_t)y0 * w + x0)), c10 = v3f(m + 3u * ((size_t)y0 * w + x1));
+  const v3 c01 = v3f(m + 3u * ((size_t)y1 * w + x0)), c11 = v3f(m + 3u * ((size_t)y1 * w + x1));
+  const v3 top = mk(c00.x + (c10.x - c00.x) * tx, c00.y + (c10.y - c00.y) * tx, c00.z + (c10.z - c00.z) * tx);
+  const v3 bot = mk(c01.x + (c11.x - c01.x) * tx, c01.y + (c11.y - c01.y) * tx, c01.z + (c11.z - c01.z) * tx);
+  v3 color = mk(top.x + (bot.x - top.x) * ty, top.y + (bot.y - top.y) * ty, top.z + (bot.z - top.z) * ty);
+  v3 scale = v3f(sc->environment_map_scale);
+  if (smax(scale.x, smax(scale.y, scale.z)) <= RC(1.0e-6)) scale = mk(RC(1.0), RC(1.0), RC(1.0));
+  color.x *= scale.x;
+  color.y *= scale.y;
+  color.z *= scale.z;
+  return color;
+}
 
 /* ---- path-replay adjoint (no reference counterpart: the reference has no differentiation).
  *
@@ -988,8 +1217,9 @@ static void adj_flush_pending(adj_t* a, v3 radiance) {
   a->pending = 0;
 }
 
-static inline double albedo_mask(const PtoMaterial* m, int c) {
-  return (m->albedo[c] > 0.0f && m->albedo[c] < 1.5f) ? 1.0 : 0.0;
+/* d(clamped effect colour)/d(albedo): k inside the (0, 1.5) clamp, 0 on or outside it. */
+static inline double eff_grad(real pre_c, real k) {
+  return (pre_c > RC(0.0) && pre_c < RC(1.5)) ? (double)k : 0.0;
 }
 
 /* trace: ScalarCpuPathTracer.cpp:1148-1289; adj != NULL replays the path for the adjoint. */
@@ -1004,14 +1234,16 @@ static v3 trace(const ctx_t* x, v3 ro, v3 rd, uint64_t* rng, uint64_t* rays, Pto
     hit_t hit = intersect_scene(x, ro, rd, c);
     adj_flush_pending(adj, radiance);
     if (!hit.hit) {
-      radiance = add(radiance, mulv(throughput, environment(sc)));
+      radiance = add(radiance, mulv(throughput, environment(sc, rd)));
       break;
     }
     const v3 gN = hit.normal;
     const int entering = dot(rd, gN) < RC(0.0);
     const v3 sN = entering ? gN : neg(gN);
     const PtoMaterial* m = material_at(sc, hit.material_index);
-    const v3 albedo = material_albedo(m);
+    real eff_k;
+    v3 eff_pre;
+    const v3 albedo = effect_albedo(m, hit.position, sN, rd, &eff_k, &eff_pre);
     const real roughness = clamp01((real)m->roughness);
     const int isEmissiveModel = m->material_model == 1u || is_emissive(m);
     const int isMirror = m->material_model == 2u || roughness <= RC(0.001);
@@ -1064,7 +1296,8 @@ static v3 trace(const ctx_t* x, v3 ro, v3 rd, uint64_t* rng, uint64_t* rays, Pto
           const real dc[3] = {nee.d_color.x, nee.d_color.y, nee.d_color.z};
           for (int ch = 0; ch < 3; ++ch) {
             const double tw = adj->delta[ch] * (double)t3[ch] * (double)w;
-            adj->g_albedo[3u * mat_index + ch] += tw * (double)da[ch] * albedo_mask(m, ch);
+            const real pre[3] = {nee.eff_pre.x, nee.eff_pre.y, nee.eff_pre.z};
+            adj->g_albedo[3u * mat_index + ch] += tw * (double)da[ch] * eff_grad(pre[ch], nee.eff_k);
             adj->g_light[3u * nee.light + ch] += tw * (double)dc[ch];
           }
         }
@@ -1125,6 +1358,7 @@ static v3 trace(const ctx_t* x, v3 ro, v3 rd, uint64_t* rng, uint64_t* rays, Pto
       if (isTransmissive && !(isMetallic || isMirror)) tint = clamp01((real)m->alpha) * (real)m->transmission;
       adj->pending = 1;
       adj->p_mat = mat_index;
+      const real pre3[3] = {eff_pre.x, eff_pre.y, eff_pre.z};
       for (int ch = 0; ch < 3; ++ch) {
         double dx = 0.0;
         if (isTransmissive && !(isMetallic || isMirror)) {
@@ -1133,7 +1367,7 @@ static v3 trace(const ctx_t* x, v3 ro, v3 rd, uint64_t* rng, uint64_t* rays, Pto
         } else {
           dx = a3[ch] > RC(0.0) ? 1.0 / (double)a3[ch] : 0.0;
         }
-        adj->p_dx[ch] = b3[ch] > RC(0.0) ? dx * albedo_mask(m, ch) : 0.0;
+        adj->p_dx[ch] = b3[ch] > RC(0.0) ? dx * eff_grad(pre3[ch], eff_k) : 0.0;
       }
       adj->p_active = (depth < st->rr_start_depth && rr_min < maxT && maxT < rr_max) ? 1 : 0;
       adj->p_cstar = cstar;
@@ -1183,12 +1417,6 @@ static camera_t make_camera(const PtoScene* sc, const PtoSettings* s) {
   cam.anamorphic = isfinite(sc->camera.anamorphic_squeeze) ? smax(RC(0.01), (real)sc->camera.anamorphic_squeeze) : RC(1.0);
   return cam;
 }
-
-#if PTO_REAL_IS_DOUBLE
-#define R_FLOOR(x) floor(x)
-#else
-#define R_FLOOR(x) floorf(x)
-#endif
 
 /* camera_rays: ScalarCpuPathTracer.Render.cpp:55-119 */
 static void camera_ray(const camera_t* cam, const PtoSettings* s, uint32_t x, uint32_t y, uint32_t sample,
@@ -1241,9 +1469,7 @@ static int shade_pixel(const ctx_t* x, const camera_t* cam, uint32_t pixel, uint
 }
 
 static int check_supported(const PtoScene* sc) {
-  for (uint32_t i = 0; i < sc->material_count; ++i)
-    if (sc->materials[i].material_effect != 0u) return -2;
-  if (sc->environment_map_width && sc->environment_map_height && sc->environment_map) return -3;
+  if (sc->environment_map && sc->environment_map_width && sc->environment_map_height == 0u) return -3;
   return 0;
 }
 

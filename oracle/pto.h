@@ -107,6 +107,8 @@ uint32_t pto_sizeof(int which); /* 0 material, 1 light, 2 instance, 3 camera, 4 
 float pto_sinf(float x);
 float pto_cosf(float x);
 float pto_powf(float x, float y);
+float pto_atan2f(float y, float x);
+float pto_acosf(float x);
 
 /* splitmix64 and the Rng of ScalarCpuPathTracer.cpp:36-41, :383-393. */
 uint64_t pto_splitmix64(uint64_t x);

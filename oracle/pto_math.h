@@ -205,4 +205,66 @@ static inline float pm_powf(float xf, float yf) {
   return (float)pm_exp(z);
 }
 
+/* atan(x) for x >= 0 in double: x > 1 folds to pi/2 - atan(1/x); then atan(x) = atan(c) +
+ * atan((x - c) / (1 + x c)) with c = k/16 the nearest sixteenth (|r| <= 1/32) and the odd Taylor
+ * series of the remainder through r^15 (error < 1e-24). atan(k/16) computed to 50 digits. */
+static const double pm_atan_k16[17] = {
+    0x0.0p+0, 0x1.ff55bb72cfdeap-5, 0x1.fd5ba9aac2f6ep-4, 0x1.7b97b4bce5b02p-3, 0x1.f5b75f92c80ddp-3,
+    0x1.362773707ebccp-2, 0x1.6f61941e4def1p-2, 0x1.a64eec3cc23fdp-2, 0x1.dac670561bb4fp-2,
+    0x1.0657e94db30d0p-1, 0x1.1e00babdefeb4p-1, 0x1.345f01cce37bbp-1, 0x1.4978fa3269ee1p-1,
+    0x1.5d58987169b18p-1, 0x1.700a7c5784634p-1, 0x1.819d0b7158a4dp-1, 0x1.921fb54442d18p-1};
+
+static inline double pm_atan_pos(double x) {
+  int inv = 0;
+  if (x > 1.0) {
+    x = 1.0 / x;
+    inv = 1;
+  }
+  const int k = (int)(x * 16.0 + 0.5);
+  const double c = (double)k * 0.0625;
+  const double r = (x - c) / (1.0 + x * c);
+  const double r2 = r * r;
+  double p = -0x1.1111111111111p-4;  /* -1/15 */
+  p = p * r2 + 0x1.3b13b13b13b14p-4; /* +1/13 */
+  p = p * r2 - 0x1.745d1745d1746p-4; /* -1/11 */
+  p = p * r2 + 0x1.c71c71c71c71cp-4; /* +1/9  */
+  p = p * r2 - 0x1.2492492492492p-3; /* -1/7  */
+  p = p * r2 + 0x1.999999999999ap-3; /* +1/5  */
+  p = p * r2 - 0x1.5555555555555p-2; /* -1/3  */
+  const double a = pm_atan_k16[k] + (r + r * (r2 * p));
+  return inv ? (0x1.921fb54442d18p+0 - a) + 0x1.1a62633145c07p-54 : a; /* pi/2 hi + lo */
+}
+
+/* atan2 in double for finite arguments (C99 signs for zeros). */
+static inline double pm_atan2(double y, double x) {
+  const double ay = y < 0.0 ? -y : y, ax = x < 0.0 ? -x : x;
+  double t;
+  if (ax == 0.0 && ay == 0.0) {
+    t = 0.0;
+  } else if (ay <= ax) {
+    t = pm_atan_pos(ay / ax);
+  } else {
+    t = (0x1.921fb54442d18p+0 - pm_atan_pos(ax / ay)) + 0x1.1a62633145c07p-54;
+  }
+  if (x < 0.0 || (x == 0.0 && 1.0 / x < 0.0)) t = (0x1.921fb54442d18p+1 - t) + 0x1.1a62633145c07p-53; /* pi - t */
+  const int yneg = y < 0.0 || (y == 0.0 && 1.0 / y < 0.0);
+  return yneg ? -t : t;
+}
+
+static inline float pm_atan2f(float y, float x) { return (float)pm_atan2((double)y, (double)x); }
+
+/* sqrt of a non-negative double: correctly rounded float sqrt, then one Newton step in double. */
+static inline double pm_sqrt_d(double v) {
+  if (!(v > 0.0)) return 0.0;
+  const double s0 = (double)sqrtf((float)v);
+  if (!(s0 > 0.0)) return 0.0;
+  return 0.5 * (s0 + v / s0);
+}
+
+/* acos(x) = atan2(sqrt((1 - x)(1 + x)), x) for x in [-1, 1]. */
+static inline float pm_acosf(float xf) {
+  const double x = (double)xf;
+  return (float)pm_atan2(pm_sqrt_d((1.0 - x) * (1.0 + x)), x);
+}
+
 #endif /* PTO_MATH_H */

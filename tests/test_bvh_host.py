@@ -116,3 +116,15 @@ def test_moller_trumbore_hand_cases():
     sc2 = S.Scene(v2, sc.indices, inst, sc.materials, sc.lights, S.Camera())
     t2, p2, _ = O.Oracle(sc2).trace_rays(np.array([[0.001, 0.002, 1, 0, 0, -1]], np.float32))
     assert p2[0] == -1
+
+
+def test_feature_scene_libm_and_portable_math_agree():
+    """The feature scene exercises sin/cos/pow/atan2/acos in effects, environment lookups and the lens:
+    the portable-math oracle and the glibc-math oracle (the reference's arithmetic) agree to 1e-3 relRMSE."""
+    sc = S.feature_scene()
+    st = S.RenderSettings(width=32, height=32, spp=4, max_depth=8)
+    a, ca, _ = O.Oracle(sc).render(st)
+    b, cb, _ = O.Oracle(sc, variant="f32_libm").render(st)
+    assert np.array_equal(ca, cb)
+    err = np.sqrt(((a.astype(np.float64) - b) ** 2).mean() / (b.astype(np.float64) ** 2).mean())
+    assert err < 1e-3

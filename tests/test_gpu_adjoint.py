@@ -84,7 +84,6 @@ def test_adjoint_matches_finite_differences():
         film, _ = pt.film()
         return float((fg.astype(np.float64) * film).sum())
 
-    h = 1e-3
     worst = 0.0
     cases = [("albedo", i, c) for i, c in [(0, 0), (2, 1), (4, 2), (6, 0), (9, 1), (12, 2), (22, 0), (31, 1)]]
     cases += [("emissive", 34, c) for c in range(3)] + [("color", 0, c) for c in range(3)]
@@ -93,6 +92,9 @@ def test_adjoint_matches_finite_differences():
         for sgn in (1.0, -1.0):
             mats, lights = sc.materials.copy(), sc.lights.copy()
             arr = lights if field == "color" else mats
+            # relative step h = 1e-3 * max(1, |p|): the film is fp32, so a fixed step on a parameter of
+            # magnitude 17 would be lost in rounding; everything here is linear or smooth in p
+            h = 1e-3 * max(1.0, abs(float(arr[idx][field][c])))
             arr[idx][field][c] = np.float32(arr[idx][field][c] + sgn * h)
             vals.append((f(mats, lights), float(arr[idx][field][c])))
         fd = (vals[0][0] - vals[1][0]) / (vals[0][1] - vals[1][1])

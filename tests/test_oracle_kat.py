@@ -105,3 +105,25 @@ def test_portable_math_agrees_with_glibc_builds():
         a = np.array([getattr(P, fn)(float(v)) for v in xs], np.float32)
         b = np.array([getattr(G, fn)(float(v)) for v in xs], np.float32)
         assert np.all(np.abs(a - b) <= np.spacing(np.abs(b)))
+
+
+def test_portable_atan2_acos_exact_to_one_ulp():
+    """atan2 over all quadrants (environment-map longitude) and acos over [-1, 1] (latitude)."""
+    L = O.lib("f32")
+    rng = np.random.default_rng(9)
+    ys = rng.normal(size=20000).astype(np.float32)
+    xs = rng.normal(size=20000).astype(np.float32)
+    xs[:50] = 0.0
+    got = np.array([L.pto_atan2f(float(y), float(x)) for y, x in zip(ys, xs)], np.float32)
+    exact = np.arctan2(ys.astype(np.float64), xs.astype(np.float64))
+    cr = exact.astype(np.float32)
+    assert (np.abs(got - exact) / np.spacing(np.abs(cr))).max() <= 1.0
+    assert np.mean(got == cr) > 0.999
+    a = np.concatenate([np.linspace(-1, 1, 20001), [-1.0, 1.0, 0.0]]).astype(np.float32)
+    g2 = np.array([L.pto_acosf(float(v)) for v in a], np.float32)
+    e2 = np.arccos(a.astype(np.float64))
+    c2 = e2.astype(np.float32)
+    nz = c2 != 0
+    assert (np.abs(g2[nz] - e2[nz]) / np.spacing(np.abs(c2[nz]))).max() <= 1.0
+    assert L.pto_acosf(1.0) == 0.0 and L.pto_atan2f(0.0, 1.0) == 0.0
+    assert L.pto_atan2f(1.0, 0.0) == np.float32(math.pi / 2) and L.pto_atan2f(0.0, -1.0) == np.float32(math.pi)

@@ -163,7 +163,12 @@ class SceneView:
         v.instances = _ptr(sc.instances)
         v.materials = _ptr(sc.materials)
         v.lights = _ptr(sc.lights)
-        v.environment_map = None
+        env = sc.environment_map
+        v.environment_map = _ptr(env)
+        if env is not None:
+            self.keep.append(env)
+            v.environment_map_height, v.environment_map_width = env.shape[0], env.shape[1]
+        v.environment_map_scale[:] = list(sc.environment_map_scale)
         v.vertex_count = len(sc.vertices)
         v.index_count = len(sc.indices)
         v.local_vertex_count = len(sc.local_vertices)

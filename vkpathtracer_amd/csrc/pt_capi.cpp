@@ -83,6 +83,7 @@ struct pt_tracer {
   DevArray<PtLight> d_lights;
   DevArray<uint32_t> d_film_count, d_queue, d_error;
   DevArray<unsigned long long> d_counters, d_query_counters;
+  DevArray<float> d_env;
   DevArray<float> d_path_f;
   DevArray<uint32_t> d_path_u;
   DevArray<uint64_t> d_rng;
@@ -187,6 +188,7 @@ void free_scene_device(pt_tracer* t) {
   t->d_prims.release();
   t->d_materials.release();
   t->d_lights.release();
+  t->d_env.release();
 }
 
 void free_paths(pt_tracer* t) {
@@ -472,11 +474,6 @@ int pt_hip_configure(pt_tracer* t, const PtRenderSettings* s) {
 int pt_hip_load_scene(pt_tracer* t, const PtSceneView* sc) {
   if (!t || !sc) return fail(t, PT_INVALID_ARGUMENT, "load_scene: null argument");
   if (!t->configured) return fail(t, PT_NOT_READY, "load_scene_snapshot called before configure");
-  for (uint32_t i = 0; i < sc->material_count; ++i)
-    if (sc->materials[i].material_effect != 0u)
-      return fail(t, PT_UNSUPPORTED, "material_effect != 0 (procedural albedo) is not supported on this path");
-  if (sc->environment_map && sc->environment_map_width && sc->environment_map_height)
-    return fail(t, PT_UNSUPPORTED, "environment maps are not supported on this path");
   auto copy = [](auto& dst, const auto* src, size_t n) {
     if (src && n) dst.assign(src, src + n);
     else dst.clear();
@@ -488,6 +485,9 @@ int pt_hip_load_scene(pt_tracer* t, const PtSceneView* sc) {
   copy(t->instances, sc->instances, sc->instance_count);
   copy(t->materials, sc->materials, sc->material_count);
   copy(t->lights, sc->lights, sc->light_count);
+  // SampleSceneEnvironment uses the map only when it holds width * height texels (PathTracer.cpp:216-219)
+  const size_t env_texels = static_cast<size_t>(sc->environment_map_width) * sc->environment_map_height;
+  copy(t->environment_map, sc->environment_map, env_texels * 3u);
   if (t->materials.empty()) {  // ScalarCpuPathTracer.cpp:469-471
     PtMaterial m{};
     m.albedo[0] = m.albedo[1] = m.albedo[2] = 1.0f;
@@ -554,6 +554,7 @@ int pt_hip_build_accel(pt_tracer* t, const PtBvhParams* params, PtBuildInfo* inf
   PT_HIP(t, upload(t->d_prims, L.prims.data(), L.prims.size() / 4u, sizeof(float4)), "upload primitives");
   PT_HIP(t, upload(t->d_materials, t->materials.data(), t->materials.size(), sizeof(PtMaterial)), "upload materials");
   PT_HIP(t, upload(t->d_lights, t->lights.data(), t->lights.size(), sizeof(PtLight)), "upload lights");
+  PT_HIP(t, upload(t->d_env, t->environment_map.data(), t->environment_map.size(), sizeof(float)), "upload environment");
 
   // Traversal stack overflow area: a lane holds at most max_depth entries.
   const uint32_t need = t->bvh.max_depth + 2u > static_cast<uint32_t>(ptd::kStackLds)
@@ -585,7 +586,14 @@ int pt_hip_build_accel(pt_tracer* t, const PtBvhParams* params, PtBuildInfo* inf
     if (m.material_model == 5u || m.transmission > 0.05f) S.has_transmissive = 1;
   S.root_ref = L.root_ref;
   for (int k = 0; k < 6; ++k) S.root_box[k] = L.root_box[k];
-  for (int k = 0; k < 3; ++k) S.env[k] = t->view.environment_color[k];
+  for (int k = 0; k < 3; ++k) {
+    S.env[k] = t->view.environment_color[k];
+    S.env_scale[k] = t->view.environment_map_scale[k];
+  }
+  const bool has_map = !t->environment_map.empty();
+  S.env_map = has_map ? t->d_env.ptr : nullptr;
+  S.env_w = has_map ? t->view.environment_map_width : 0u;
+  S.env_h = has_map ? t->view.environment_map_height : 0u;
 
   // compute_camera_basis, ScalarCpuPathTracer.cpp:139-152; camera_rays constants, Render.cpp:74-83
   const PtCamera& c = t->view.camera;
@@ -903,10 +911,6 @@ extern "C" int pt_hip_update_scene_delta(pt_tracer* t, const PtMaterial* materia
     return fail(t, PT_INVALID_ARGUMENT, "update_scene_delta: material count differs from the loaded scene");
   if (lights && light_count != t->lights.size())
     return fail(t, PT_INVALID_ARGUMENT, "update_scene_delta: light count differs from the loaded scene");
-  if (materials)
-    for (uint32_t i = 0; i < material_count; ++i)
-      if (materials[i].material_effect != 0u)
-        return fail(t, PT_UNSUPPORTED, "material_effect != 0 (procedural albedo) is not supported on this path");
   PT_HIP(t, hipStreamSynchronize(t->stream), "synchronize");
   if (materials) {
     t->materials.assign(materials, materials + material_count);

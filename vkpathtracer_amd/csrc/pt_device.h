@@ -49,6 +49,9 @@ struct DevScene {
   int32_t root_ref;
   float root_box[6];
   float env[3];
+  const float* env_map;  // rgb, env_w * env_h (nullptr: constant colour)
+  uint32_t env_w, env_h;
+  float env_scale[3];
 };
 
 struct DevCamera {

@@ -510,9 +510,200 @@ __device__ __forceinline__ const PtMaterial& material_at(const DevScene& sc, uin
   return sc.materials[idx < last ? idx : last];
 }
 
-__device__ __forceinline__ V3 material_albedo(const PtMaterial& m) {
-  return mk(smin(1.5f, smax(0.0f, m.albedo[0])), smin(1.5f, smax(0.0f, m.albedo[1])),
-            smin(1.5f, smax(0.0f, m.albedo[2])));
+// std::fmod(x, 2) for the integer-valued sums the effects pass (exact).
+__device__ __forceinline__ float fmod2(float x) { return x - 2.0f * __builtin_truncf(x * 0.5f); }
+
+// hash01 / smoothstep, ScalarCpuPathTracer.cpp:187-196
+__device__ __forceinline__ float hash01(float x, float y, float z, float seed) {
+  const float v = ptm::sinf_p(x * 12.9898f + y * 78.233f + z * 37.719f + seed * 19.19f) * 43758.5453f;
+  return v - __builtin_floorf(v);
+}
+__device__ __forceinline__ float smoothstep_f(float e0, float e1, float x) {
+  const float t = clamp01((x - e0) / smax(1.0e-6f, e1 - e0));
+  return t * t * (3.0f - 2.0f * t);
+}
+
+// procedural_wood_albedo, ScalarCpuPathTracer.cpp:198-262 (linear in base: k receives the factor).
+__device__ V3 wood_albedo(V3 base, V3 p, float roughness, float clearcoat, uint32_t style, float& k) {
+  const float rough = clamp01(roughness), coat = clamp01(clearcoat);
+  const float polish = clamp01(coat * (1.25f - rough * 0.75f));
+  const float parquet = style == 17u ? 1.0f : 0.0f;
+  const float walnut = style == 18u ? 1.0f : 0.0f;
+  const float sandal = style == 19u ? 1.0f : 0.0f;
+  const float plank_width = 0.40f + 0.16f * parquet + 0.08f * polish - 0.18f * walnut;
+  const float plank_length = 1.15f + 0.85f * parquet + 0.55f * polish - 0.55f * walnut;
+  const float px = (p.x + sandal * p.y * 0.35f) / smax(0.12f, plank_width);
+  const float pz = (p.z + walnut * p.y * 0.70f + sandal * p.x * 0.22f) / smax(0.30f, plank_length);
+  const float cell_x = __builtin_floorf(px), cell_z = __builtin_floorf(pz);
+  const float local_x = px - cell_x, local_z = pz - cell_z;
+  const float seed = hash01(cell_x, cell_z, 0.0f, 16.0f);
+  const float macro_noise = hash01(__builtin_floorf(p.x * 3.7f), __builtin_floorf(p.y * 3.7f),
+                                   __builtin_floorf(p.z * 3.7f), 31.0f + static_cast<float>(style));
+  const float micro_noise = hash01(__builtin_floorf(p.x * 34.0f), __builtin_floorf(p.y * 34.0f),
+                                   __builtin_floorf(p.z * 34.0f), 47.0f + static_cast<float>(style));
+  const bool rotate = parquet > 0.5f && fmod2(cell_x + cell_z) >= 1.0f;
+  const float along = rotate ? local_z : local_x, across = rotate ? local_x : local_z;
+  const float grain_wobble = (macro_noise - 0.5f) * (0.06f + rough * 0.12f + walnut * 0.08f + sandal * 0.04f);
+  const float warped_along = along + grain_wobble + ptm::sinf_p(across * 6.0f + seed * 5.0f) * 0.018f;
+  const float grain_freq = 12.0f + rough * 8.0f + walnut * 22.0f + sandal * 36.0f;
+  const float wave = ptm::sinf_p((warped_along * grain_freq + seed * 6.2831853f) +
+                                 ptm::sinf_p(across * (6.0f + rough * 8.0f) + seed * 4.0f) *
+                                     (0.25f + rough * 0.45f + walnut * 0.45f));
+  const float fine = ptm::sinf_p(warped_along * (55.0f + rough * 55.0f + sandal * 110.0f) +
+                                 across * (9.0f + walnut * 25.0f) + seed * 12.0f);
+  const float fine_weight = 0.10f + rough * 0.18f + walnut * 0.18f + sandal * 0.22f;
+  const float ring = smoothstep_f(-0.35f + polish * 0.18f, 0.82f, wave) * (1.0f - fine_weight) +
+                     (0.5f + 0.5f * fine) * fine_weight;
+  const float seam_x = smin(local_x, 1.0f - local_x), seam_z = smin(local_z, 1.0f - local_z);
+  const float seam = (1.0f - sandal) * (1.0f - smoothstep_f(0.0f, 0.028f + polish * 0.018f, smin(seam_x, seam_z)));
+  const float kd = 0.46f - rough * 0.10f - walnut * 0.18f;
+  const float ka = 1.14f + coat * 0.22f + sandal * 0.18f;
+  const V3 dark = muls(base, kd);
+  const V3 amber = add(muls(base, ka), mk(0.08f + sandal * 0.18f, 0.055f + sandal * 0.06f, 0.025f));
+  V3 wood = add(muls(dark, 1.0f - ring), muls(amber, ring));
+  const float board_tint = 0.92f + (0.08f + walnut * 0.12f + sandal * 0.06f) * seed +
+                           (macro_noise - 0.5f) * (0.08f + rough * 0.06f);
+  const float pores = smoothstep_f(0.74f - walnut * 0.08f, 1.0f, micro_noise) *
+                      (0.025f + rough * 0.055f + walnut * 0.055f + sandal * 0.018f);
+  const float seam_f = 1.0f - seam * (0.10f + rough * 0.35f + parquet * 0.18f) * (1.0f - coat * 0.45f);
+  wood = muls(wood, board_tint);
+  wood = muls(wood, 1.0f - pores);
+  wood = muls(wood, seam_f);
+  const float wood_mix = 0.30f + rough * 0.34f - polish * 0.14f + walnut * 0.26f + sandal * 0.18f;
+  k = (1.0f - wood_mix) + (kd * (1.0f - ring) + ka * ring) * board_tint * (1.0f - pores) * seam_f * wood_mix;
+  return add(muls(base, 1.0f - wood_mix), muls(wood, wood_mix));
+}
+
+// apply_material_effect, ScalarCpuPathTracer.cpp:264-355. Affine in the albedo before the final
+// [0, 1.5] clamp: k is d(colour)/d(albedo) there and pre the unclamped colour (for the adjoint).
+__device__ V3 effect_albedo(const PtMaterial& m, V3 pos, V3 n, V3 incoming, float& k, V3& pre) {
+  V3 c = ld3(m.albedo);
+  k = 1.0f;
+  const uint32_t e = m.material_effect;
+  if (e != 0u) {
+    const float h = hash01(pos.x, pos.y, pos.z, static_cast<float>(e));
+    const float ndi = dot(n, neg(incoming));
+    switch (e) {
+      case 1u: {
+        const float r0 = smax(0.0f, 1.0f - __builtin_fabsf(ndi));
+        const float rim = r0 * r0;
+        k = 0.65f + 0.25f * h;
+        c = add(muls(c, k), muls(mk(0.25f, 0.22f, 0.28f), rim * (0.4f + m.sheen)));
+        break;
+      }
+      case 2u: {
+        const float stripes = 0.5f + 0.5f * ptm::sinf_p(pos.x * 7.0f + pos.z * 5.0f + h * 6.0f);
+        k = 0.45f + 0.55f * stripes;
+        c = muls(c, k);
+        break;
+      }
+      case 3u:
+        k = h > 0.72f ? 0.18f : 1.0f;
+        c = muls(c, k);
+        break;
+      case 4u: {
+        const float vein = 0.5f + 0.5f * ptm::sinf_p((pos.x + pos.y * 0.4f + pos.z * 0.7f) * 9.0f + h * 3.0f);
+        k = 0.55f + 0.45f * vein;
+        c = add(muls(c, k), muls(mk(0.18f, 0.20f, 0.23f), 1.0f - vein));
+        break;
+      }
+      case 5u:
+        k = 0.55f + 0.35f * h;
+        c = add(muls(c, k), muls(mk(0.45f, 0.16f, 0.04f), 0.25f + 0.25f * h));
+        break;
+      case 6u:
+        k = 0.65f + 0.35f * h;
+        c = muls(c, k);
+        break;
+      case 7u:
+        k = 0.78f;
+        c = add(muls(c, k), muls(mk(1.0f, 0.62f, 0.42f), 0.16f));
+        break;
+      case 8u:
+        k = 0.55f;
+        c = add(muls(c, k), muls(mk(0.35f + 0.45f * h, 0.25f + 0.35f * (1.0f - h), 0.85f), 0.45f));
+        break;
+      case 9u: {
+        const float check = fmod2(__builtin_floorf(pos.x * 4.0f) + __builtin_floorf(pos.z * 4.0f));
+        k = check < 0.5f ? 1.0f : 0.28f;
+        c = muls(c, k);
+        break;
+      }
+      case 10u:
+        k = 0.45f;
+        c = add(muls(c, k), muls(mk(1.0f, 0.35f + 0.3f * h, 0.08f), 0.55f));
+        break;
+      case 11u: {
+        const float streak = 0.5f + 0.5f * ptm::sinf_p(pos.y * 18.0f + h * 5.0f);
+        k = 0.55f + 0.45f * streak;
+        c = muls(c, k);
+        break;
+      }
+      case 12u: {
+        const float v = smax(0.0f, ndi);
+        const float v2 = v * v;
+        c = add(c, muls(mk(0.55f, 0.65f, 0.95f), v2 * v2 * v2));
+        break;
+      }
+      case 13u:
+        k = 0.65f + 0.35f * __builtin_fabsf(ptm::sinf_p(pos.x * 10.0f) * ptm::cosf_p(pos.z * 10.0f));
+        c = muls(c, k);
+        break;
+      case 14u: {
+        const float rim = ptm::powf_p(smax(0.0f, 1.0f - __builtin_fabsf(ndi)), 0.8f);
+        k = 0.0f;
+        c = muls(mk(0.15f, 0.75f, 1.0f), 0.2f + 0.8f * rim);
+        break;
+      }
+      case 15u: {
+        const float bands = 0.5f + 0.5f * ptm::sinf_p(pos.y * 11.0f + pos.x * 3.0f + h * 7.0f);
+        const float rim = ptm::powf_p(smax(0.0f, 1.0f - __builtin_fabsf(ndi)), 1.5f);
+        k = 0.25f + 0.45f * bands;
+        c = add(muls(c, k), muls(mk(0.48f, 0.56f, 0.68f), 0.18f + 0.32f * rim));
+        break;
+      }
+      case 16u:
+      case 17u:
+      case 18u:
+      case 19u:
+        c = wood_albedo(c, pos, m.roughness, m.clearcoat, e, k);
+        break;
+      default:
+        break;
+    }
+  }
+  pre = c;
+  return mk(smin(1.5f, smax(0.0f, c.x)), smin(1.5f, smax(0.0f, c.y)), smin(1.5f, smax(0.0f, c.z)));
+}
+
+// SampleSceneEnvironment, PathTracer.cpp:213-272
+__device__ V3 environment(const DevScene& sc, V3 d) {
+  const uint32_t w = sc.env_w, h = sc.env_h;
+  if (w == 0u || h == 0u || !sc.env_map) return ld3(sc.env);
+  const float len_sq = d.x * d.x + d.y * d.y + d.z * d.z;
+  if (len_sq <= 1.0e-12f) return ld3(sc.env);
+  const float inv_len = 1.0f / __builtin_sqrtf(len_sq);
+  const V3 dir = mk(d.x * inv_len, d.y * inv_len, d.z * inv_len);
+  float u = 0.5f + ptm::atan2f_p(dir.z, dir.x) * 0.15915494309189533577f;
+  u -= __builtin_floorf(u);
+  const float v = ptm::acosf_p(sclamp(dir.y, -1.0f, 1.0f)) * 0.31830988618379067154f;
+  const float xf = u * static_cast<float>(w);
+  const float yf = sclamp(v, 0.0f, 1.0f) * static_cast<float>(h - 1u);
+  const uint32_t x0 = static_cast<uint32_t>(__builtin_floorf(xf)) % w, x1 = (x0 + 1u) % w;
+  const uint32_t y0 = static_cast<uint32_t>(__builtin_floorf(yf)), y1 = y0 + 1u < h - 1u ? y0 + 1u : h - 1u;
+  const float tx = xf - __builtin_floorf(xf), ty = yf - __builtin_floorf(yf);
+  const float* m = sc.env_map;
+  const V3 c00 = ld3(m + 3u * (static_cast<size_t>(y0) * w + x0)), c10 = ld3(m + 3u * (static_cast<size_t>(y0) * w + x1));
+  const V3 c01 = ld3(m + 3u * (static_cast<size_t>(y1) * w + x0)), c11 = ld3(m + 3u * (static_cast<size_t>(y1) * w + x1));
+  const V3 top = mk(c00.x + (c10.x - c00.x) * tx, c00.y + (c10.y - c00.y) * tx, c00.z + (c10.z - c00.z) * tx);
+  const V3 bot = mk(c01.x + (c11.x - c01.x) * tx, c01.y + (c11.y - c01.y) * tx, c01.z + (c11.z - c01.z) * tx);
+  V3 color = mk(top.x + (bot.x - top.x) * ty, top.y + (bot.y - top.y) * ty, top.z + (bot.z - top.z) * ty);
+  V3 scale = ld3(sc.env_scale);
+  if (smax(scale.x, smax(scale.y, scale.z)) <= 1.0e-6f) scale = mk(1.0f, 1.0f, 1.0f);
+  color.x *= scale.x;
+  color.y *= scale.y;
+  color.z *= scale.z;
+  return color;
 }
 
 // spot_attenuation, ScalarCpuPathTracer.cpp:168-185
@@ -617,9 +808,8 @@ struct ShadeOut {
   uint32_t gmat, glight_index;
 };
 
-__device__ __forceinline__ float albedo_mask(const PtMaterial& m, int c) {
-  return (m.albedo[c] > 0.0f && m.albedo[c] < 1.5f) ? 1.0f : 0.0f;
-}
+// d(clamped effect colour)/d(albedo): k inside the (0, 1.5) clamp, 0 on or outside it.
+__device__ __forceinline__ float eff_grad(float pre_c, float k) { return (pre_c > 0.0f && pre_c < 1.5f) ? k : 0.0f; }
 
 // One iteration of ScalarCpuPathTracer::trace (ScalarCpuPathTracer.cpp:1159-1287) for a path that
 // arrived at a hit (prim >= 0) or a miss (prim < 0). kAdj replays the iteration for the adjoint.
@@ -640,7 +830,7 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
     dl = mk(A.delta[0][p], A.delta[1][p], A.delta[2][p]);
   }
   if (prim < 0) {
-    L = add(L, mulv(T, ld3(sc.env)));
+    L = add(L, mulv(T, environment(sc, rd)));
     P.rad[0][p] = L.x; P.rad[1][p] = L.y; P.rad[2][p] = L.z;
     return;
   }
@@ -653,7 +843,9 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
   const uint32_t raw_mat = static_cast<uint32_t>(__float_as_int(pn.w));
   const uint32_t mat_index = raw_mat < sc.material_count - 1u ? raw_mat : sc.material_count - 1u;
   const PtMaterial& m = sc.materials[mat_index];
-  const V3 albedo = material_albedo(m);
+  float eff_k;
+  V3 eff_pre;
+  const V3 albedo = effect_albedo(m, pos, sN, rd, eff_k, eff_pre);
   const float roughness = clamp01(m.roughness);
   MatFlags f;
   f.emissive_model = m.material_model == 1u || m.emissive[0] > 0.0f || m.emissive[1] > 0.0f || m.emissive[2] > 0.0f;
@@ -712,16 +904,20 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
         if (spot > 0.0f) {
           ++shadow_tests;
           const V3 view_dir = neg(rd);
+          // the NEE albedo sees the light direction as the incoming one (ScalarCpuPathTracer.cpp:1089)
+          float nk = eff_k;
+          V3 npre = eff_pre;
+          const V3 nalb = m.material_effect != 0u ? effect_albedo(m, pos, sN, neg(light_dir), nk, npre) : albedo;
           const float kscale = (light.intensity * spot) / (dist_sq + kEps);
           const V3 irradiance = muls(ld3(light.color), kscale);
           const float nlf = static_cast<float>(nl);
           V3 direct = mk(0.0f, 0.0f, 0.0f);
           V3 d_alb = mk(0.0f, 0.0f, 0.0f), d_col = mk(0.0f, 0.0f, 0.0f);  // adjoint partials
           if (!f.mirror && !f.transmissive) {
-            direct = add(direct, muls(muls(mulv(muls(albedo, kInvPi), irradiance), cos_theta), nlf));
+            direct = add(direct, muls(muls(mulv(muls(nalb, kInvPi), irradiance), cos_theta), nlf));
             if constexpr (kAdj) {
               d_alb = muls(irradiance, kInvPi * cos_theta * nlf);
-              d_col = muls(albedo, kInvPi * kscale * cos_theta * nlf);
+              d_col = muls(nalb, kInvPi * kscale * cos_theta * nlf);
             }
           }
           if (f.mirror || f.metallic || f.clearcoat || f.transmissive || roughness < 0.65f) {
@@ -737,7 +933,7 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
             const float specStrength = clamp01((1.0f - roughness) * 0.8f + m.metallic * 0.45f + m.clearcoat * 0.35f +
                                                (f.mirror ? 0.75f : 0.0f) + (f.transmissive ? fresnel : 0.0f));
             const V3 specTint = add(muls(mk(1.0f, 1.0f, 1.0f), f.metallic ? 0.15f : 0.88f),
-                                    muls(albedo, f.metallic ? 0.85f : 0.12f));
+                                    muls(nalb, f.metallic ? 0.85f : 0.12f));
             const float s = spec * cos_theta * nlf * smax(0.15f, specStrength);
             direct = add(direct, muls(mulv(specTint, irradiance), s));
             if constexpr (kAdj) {
@@ -760,8 +956,8 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
           V3 galb = mk(0.0f, 0.0f, 0.0f), glight = mk(0.0f, 0.0f, 0.0f);
           if constexpr (kAdj) {
             const V3 tw = muls(mulv(dl, T), w);
-            galb = mk(tw.x * d_alb.x * albedo_mask(m, 0), tw.y * d_alb.y * albedo_mask(m, 1),
-                      tw.z * d_alb.z * albedo_mask(m, 2));
+            galb = mk(tw.x * d_alb.x * eff_grad(npre.x, nk), tw.y * d_alb.y * eff_grad(npre.y, nk),
+                      tw.z * d_alb.z * eff_grad(npre.z, nk));
             glight = mulv(tw, d_col);
           }
           const float remaining = dist - 0.004f;
@@ -849,7 +1045,8 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
       } else {
         v = a3[c] > 0.0f ? 1.0f / a3[c] : 0.0f;
       }
-      dx[c] = b3[c] > 0.0f ? v * albedo_mask(m, c) : 0.0f;
+      const float pre_c = c == 0 ? eff_pre.x : (c == 1 ? eff_pre.y : eff_pre.z);
+      dx[c] = b3[c] > 0.0f ? v * eff_grad(pre_c, eff_k) : 0.0f;
       A.p_dx[c][p] = dx[c];
     }
     A.p_mat[p] = mat_index;

@@ -165,4 +165,60 @@ PT_HD inline float powf_p(float xf, float yf) {
   return static_cast<float>(exp_p(z));
 }
 
+// atan(x), x >= 0: fold x > 1 to pi/2 - atan(1/x), then atan(x) = atan(k/16) + atan((x - c)/(1 + x c))
+// with the nearest sixteenth c (|r| <= 1/32) and the odd Taylor series through r^15.
+PT_HD inline double atan_pos(double x) {
+  constexpr double kAtanK16[17] = {
+      0x0.0p+0, 0x1.ff55bb72cfdeap-5, 0x1.fd5ba9aac2f6ep-4, 0x1.7b97b4bce5b02p-3, 0x1.f5b75f92c80ddp-3,
+      0x1.362773707ebccp-2, 0x1.6f61941e4def1p-2, 0x1.a64eec3cc23fdp-2, 0x1.dac670561bb4fp-2,
+      0x1.0657e94db30d0p-1, 0x1.1e00babdefeb4p-1, 0x1.345f01cce37bbp-1, 0x1.4978fa3269ee1p-1,
+      0x1.5d58987169b18p-1, 0x1.700a7c5784634p-1, 0x1.819d0b7158a4dp-1, 0x1.921fb54442d18p-1};
+  bool inv = false;
+  if (x > 1.0) {
+    x = 1.0 / x;
+    inv = true;
+  }
+  const int k = static_cast<int>(x * 16.0 + 0.5);
+  const double c = static_cast<double>(k) * 0.0625;
+  const double r = (x - c) / (1.0 + x * c);
+  const double r2 = r * r;
+  double p = -0x1.1111111111111p-4;
+  p = p * r2 + 0x1.3b13b13b13b14p-4;
+  p = p * r2 - 0x1.745d1745d1746p-4;
+  p = p * r2 + 0x1.c71c71c71c71cp-4;
+  p = p * r2 - 0x1.2492492492492p-3;
+  p = p * r2 + 0x1.999999999999ap-3;
+  p = p * r2 - 0x1.5555555555555p-2;
+  const double a = kAtanK16[k] + (r + r * (r2 * p));
+  return inv ? (0x1.921fb54442d18p+0 - a) + 0x1.1a62633145c07p-54 : a;
+}
+
+PT_HD inline double atan2_d(double y, double x) {
+  const double ay = y < 0.0 ? -y : y, ax = x < 0.0 ? -x : x;
+  double t;
+  if (ax == 0.0 && ay == 0.0) t = 0.0;
+  else if (ay <= ax) t = atan_pos(ay / ax);
+  else t = (0x1.921fb54442d18p+0 - atan_pos(ax / ay)) + 0x1.1a62633145c07p-54;
+  if (x < 0.0 || (x == 0.0 && 1.0 / x < 0.0)) t = (0x1.921fb54442d18p+1 - t) + 0x1.1a62633145c07p-53;
+  const bool yneg = y < 0.0 || (y == 0.0 && 1.0 / y < 0.0);
+  return yneg ? -t : t;
+}
+
+PT_HD inline float atan2f_p(float y, float x) {
+  return static_cast<float>(atan2_d(static_cast<double>(y), static_cast<double>(x)));
+}
+
+// sqrt of a non-negative double: correctly rounded fp32 sqrt, then one Newton step in double.
+PT_HD inline double sqrt_d(double v) {
+  if (!(v > 0.0)) return 0.0;
+  const double s0 = static_cast<double>(__builtin_sqrtf(static_cast<float>(v)));
+  if (!(s0 > 0.0)) return 0.0;
+  return 0.5 * (s0 + v / s0);
+}
+
+PT_HD inline float acosf_p(float xf) {
+  const double x = static_cast<double>(xf);
+  return static_cast<float>(atan2_d(sqrt_d((1.0 - x) * (1.0 + x)), x));
+}
+
 }  // namespace ptm

@@ -102,8 +102,13 @@ class Scene:
     local_indices: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0, np.uint32))
     environment_color: tuple = (0.0, 0.0, 0.0)
     name: str = "scene"
+    environment_map: np.ndarray | None = None  # (H, W, 3) f32 equirectangular, or None
+    environment_map_scale: tuple = (0.0, 0.0, 0.0)
 
     def __post_init__(self):
+        if self.environment_map is not None:
+            self.environment_map = np.ascontiguousarray(self.environment_map, np.float32).reshape(
+                self.environment_map.shape[0], self.environment_map.shape[1], 3)
         self.vertices = np.ascontiguousarray(self.vertices, np.float32).reshape(-1, 3)
         self.indices = np.ascontiguousarray(self.indices, np.uint32).reshape(-1)
         self.local_vertices = np.ascontiguousarray(self.local_vertices, np.float32).reshape(-1, 3)
@@ -205,6 +210,46 @@ def cornell_box() -> Scene:
     lights = np.array([make_light((0.0, 1.97, 0.0), (17.0, 12.0, 4.0), 0.0625, 0.005)], LIGHT_DTYPE)
     cam = Camera(position=(0.0, 1.0, 3.0), target=(0.0, 1.0, 0.0), up=(0.0, 1.0, 0.0), fov_deg=55.0)
     return Scene(vertices, indices, instances, materials, lights, cam, name="cornell")
+
+
+def feature_scene(seed: int = 5) -> Scene:
+    """The Cornell box with every material class and procedural effect of the integrator switched on:
+    per-triangle materials cycling through material_effect 0-19 and the models lambert / glossy /
+    mirror / metal / transmissive / sheen / clearcoat / toon, a spot light beside the area light, a
+    thin-lens camera with a six-blade iris and anamorphic squeeze, and a seeded 32x16 equirectangular
+    environment map seen through the open front of the box."""
+    rng = np.random.default_rng(seed)
+    sc = cornell_box()
+    mats = sc.materials.copy()
+    models = [MODEL_LAMBERT, MODEL_GLOSSY, MODEL_MIRROR, 4, 5, 6, 7, 8]
+    for i in range(34):
+        m = mats[i]
+        m["material_effect"] = i % 20
+        model = models[i % len(models)]
+        m["material_model"] = model
+        m["roughness"] = {MODEL_LAMBERT: 1.0, MODEL_GLOSSY: 0.3, MODEL_MIRROR: 0.0}.get(model, 0.5)
+        if model == 4:
+            m["metallic"] = 0.9
+        if model == 5:
+            m["transmission"] = 0.8
+            m["alpha"] = 0.7
+        if model == 6:
+            m["sheen"] = 0.6
+        if model == 7:
+            m["clearcoat"] = 0.5
+        m["albedo"] = rng.uniform(0.2, 0.9, 3)
+    spot = make_light((0.5, 1.9, 0.5), (3.0, 3.0, 2.5), 0.5, 0.02)
+    spot["direction"] = (0.0, -1.0, 0.0)
+    spot["spot_inner_cos"] = 0.95
+    spot["spot_outer_cos"] = 0.8
+    lights = np.concatenate([sc.lights, np.array([spot], LIGHT_DTYPE)])
+    cam = Camera(position=(0.0, 1.0, 3.0), target=(0.0, 1.0, 0.0), up=(0.0, 1.0, 0.0), fov_deg=55.0,
+                 aperture_radius=0.05, focus_distance=3.0, iris_blade_count=6, iris_rotation_degrees=10.0,
+                 iris_roundness=0.3, anamorphic_squeeze=1.3)
+    env = rng.uniform(0.0, 2.0, (16, 32, 3)).astype(np.float32)
+    return dataclasses.replace(sc, materials=mats, lights=lights, camera=cam, environment_map=env,
+                               environment_map_scale=(1.0, 0.9, 0.8), environment_color=(0.1, 0.1, 0.1),
+                               name="features")
 
 
 def perturbed_albedo_scene(scene: Scene, sigma: float = 0.1, seed: int = 7) -> Scene:
