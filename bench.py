@@ -106,6 +106,50 @@ def cpu_baseline(sc, st, target_seconds: float) -> dict:
                       f"{dt:.1f} s), oracle fp32 restatement, OpenMP over {cores} threads, {cpu_model()}"}
 
 
+def adjoint_ratio(log, reps: int = 3) -> dict:
+    """Adjoint/forward time on the inverse-rendering box: 512x512, 64 spp, depth 8, 35 albedos + emission
+    as parameters. Forward = render_tiles over the 64 sample tiles; adjoint = render_adjoint over the
+    same tiles (it replays each path forward, then backward, recomputing every vertex)."""
+    import torch
+
+    from vkpathtracer_amd import scene as S
+    from vkpathtracer_amd.tracer import HipPathTracer
+
+    sc = S.cornell_box()
+    st = S.RenderSettings(width=512, height=512, spp=64, max_depth=8)
+    tiles = [(0, 0, st.width, st.height, s, 0, 0) for s in range(st.spp)]
+    dev = torch.device("cuda", 0)
+    with HipPathTracer(0) as pt:
+        assert pt.configure(st) and pt.load_scene_snapshot(sc) and pt.build_or_update_acceleration()
+        g = torch.randn((st.height, st.width, 3), device=dev)
+        ga = torch.zeros((len(sc.materials), 3), device=dev)
+        ge = torch.zeros_like(ga)
+        gl = torch.zeros((len(sc.lights), 3), device=dev)
+
+        def fwd():
+            pt.reset_accumulation()
+            assert pt.render_tiles(tiles)
+            pt.synchronize()
+
+        def adj():
+            pt.render_adjoint(tiles, g, ga, ge, gl)
+
+        fwd()
+        adj()
+        times = {}
+        for name, fn in (("forward", fwd), ("adjoint", adj)):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                fn()
+            torch.cuda.synchronize()
+            times[name] = (time.perf_counter() - t0) * 1e3 / reps
+    log(f"adjoint {times['adjoint']:.1f} ms vs forward {times['forward']:.1f} ms")
+    return {"workload": "cornell box 512x512, 64 spp, depth 8, d/d(35 albedos, emission, light)",
+            "forward_ms": round(times["forward"], 2), "adjoint_ms": round(times["adjoint"], 2),
+            "ratio": round(times["adjoint"] / times["forward"], 3)}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +161,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="time without per-kernel events")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward time ratio")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="run a plain device copy of this size first (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build renders on a single MI355X (no multi-GPU path); "
@@ -129,6 +176,16 @@ def main():
 
     def log(msg):
         print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    if args.copy_probe_mib:
+        n = args.copy_probe_mib << 18  # float32 elements
+        a = torch.ones(n, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"copy probe: 3 x {args.copy_probe_mib} MiB device-to-device copies")
+        del a, b
 
     sc, st, desc = workload(args.workload, args.spp)
     log(f"scene {sc.name}: {sc.triangle_count} triangles")
@@ -198,6 +255,9 @@ def main():
                     "avg_launch_ms": round(k["total_ms"] / k["launches"], 4)}
 
     log(f"timed {args.steps} steps: {dt:.3f} s")
+    bvh_info = {k: pt.build_info[k] for k in ("node_count", "leaf_count", "max_depth", "device_bytes")}
+    pt.close()
+    adjoint = None if args.no_adjoint else adjoint_ratio(log)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(sc, st, args.cpu_seconds)
@@ -222,14 +282,14 @@ def main():
         "ref_paths_per_sec": round(ext_rays * args.steps / dt, 1),
         "rays_per_step": {"extension": ext_rays, "shadow": shadow_rays},
         "bvh_build_ms": round(build_wall_ms, 1),
-        "bvh": {k: pt.build_info[k] for k in ("node_count", "leaf_count", "max_depth", "device_bytes")},
+        "bvh": bvh_info,
         "roofline": roofline,
         "kernels": per_kernel,
         "cpu_baseline": cpu,
+        "adjoint": adjoint,
     }
     if cpu:
         out["speedup_vs_cpu"] = round(mpaths / cpu["value"], 1)
-    pt.close()
     print(json.dumps(out))
 
 

@@ -17,6 +17,7 @@
 namespace {
 
 constexpr uint32_t kDefaultCapacity = 8u * 1024u * 1024u;
+constexpr uint32_t kDefaultRefillMin = 16u;
 constexpr uint32_t kMaxTilesPerPass = 96u;
 
 template <typename T>
@@ -94,6 +95,7 @@ struct pt_tracer {
   uint32_t adj_capacity_alloc = 0;
   uint32_t path_capacity_alloc = 0;
   uint32_t capacity = kDefaultCapacity;
+  uint32_t refill_min = kDefaultRefillMin;
 
   ptd::DevScene dscene{};
   ptd::DevCamera dcam{};
@@ -182,6 +184,21 @@ int sync_all(pt_tracer* t) {
   return check_device_error(t);
 }
 
+// Sums the striped device counters (ptd::kStripes stripes of ptd::kCounterSlots) into out.
+int read_counter_block(pt_tracer* t, const unsigned long long* dev, unsigned long long out[ptd::kCounterSlots]) {
+  std::vector<unsigned long long> raw(ptd::kCounterWords);
+  PT_HIP(t, hipMemcpyAsync(raw.data(), dev, raw.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, t->stream),
+         "read counters");
+  const int rc = sync_all(t);
+  if (rc != PT_OK) return rc;
+  for (int k = 0; k < ptd::kCounterSlots; ++k) {
+    unsigned long long v = 0;
+    for (int s = 0; s < ptd::kStripes; ++s) v += raw[static_cast<size_t>(s) * ptd::kCounterSlots + k];
+    out[k] = v;
+  }
+  return PT_OK;
+}
+
 void free_scene_device(pt_tracer* t) {
   t->d_nodes.release();
   t->d_tris.release();
@@ -207,11 +224,13 @@ int ensure_adjoint(pt_tracer* t, uint32_t cap, ptd::DevAdjoint& A) {
     t->d_adj_f.release();
     t->d_adj_u.release();
     t->adj_capacity_alloc = 0;
-    PT_HIP(t, t->d_adj_f.ensure(static_cast<size_t>(27) * cap), "allocate adjoint state");
-    PT_HIP(t, t->d_adj_u.ensure(static_cast<size_t>(8) * cap), "allocate adjoint state");
+    const size_t m = std::max<size_t>(cap, static_cast<size_t>(ptd::kSegments) * ptd::segment_capacity(cap));
+    PT_HIP(t, t->d_adj_f.ensure(static_cast<size_t>(27) * m), "allocate adjoint state");
+    PT_HIP(t, t->d_adj_u.ensure(static_cast<size_t>(8) * m), "allocate adjoint state");
     t->adj_capacity_alloc = cap;
   }
-  const size_t n = t->adj_capacity_alloc;
+  const size_t n = std::max<size_t>(t->adj_capacity_alloc,
+                                    static_cast<size_t>(ptd::kSegments) * ptd::segment_capacity(t->adj_capacity_alloc));
   float* f = t->d_adj_f.ptr;
   uint32_t* u = t->d_adj_u.ptr;
   for (int c = 0; c < 3; ++c) {
@@ -330,11 +349,14 @@ int run_passes(pt_tracer* t, std::vector<ptd::DevTile>& work, const ptd::DevSett
             [&] { ptd::launch_raygen_adjoint(t->stream, P, *adj, t->dcam, st, table, paths); });
       bounce_loop(t, P, st, adj);
     } else {
-      for (size_t i = next; i < end; ++i) {
-        const ptd::DevTile& tile = work[i];
-        timed(t, PT_KERNEL_FILM, tile.w * tile.h,
-              [&] { ptd::launch_film(t->stream, P, film, st.width, tile, tile.path_begin); });
+      uint32_t rect[4] = {work[next].x, work[next].y, work[next].x + work[next].w, work[next].y + work[next].h};
+      for (size_t i = next + 1; i < end; ++i) {
+        rect[0] = std::min(rect[0], work[i].x);
+        rect[1] = std::min(rect[1], work[i].y);
+        rect[2] = std::max(rect[2], work[i].x + work[i].w);
+        rect[3] = std::max(rect[3], work[i].y + work[i].h);
       }
+      timed(t, PT_KERNEL_FILM, paths, [&] { ptd::launch_film(t->stream, P, film, st.width, table, rect); });
     }
     const hipError_t le = hipGetLastError();
     if (le != hipSuccess) return hip_fail(t, le, "kernel launch");
@@ -343,12 +365,14 @@ int run_passes(pt_tracer* t, std::vector<ptd::DevTile>& work, const ptd::DevSett
   return PT_OK;
 }
 
-// Path-state arrays for `cap` paths: 29 float arrays, 4 u32 arrays, 1 u64 array.
+// Path-state and queue arrays for `cap` paths: 29 float arrays, 4 u32 arrays, 1 u64 array, each
+// of max(cap, kSegments * seg_cap) entries (queue arrays are indexed by segment position).
 int ensure_paths(pt_tracer* t, uint32_t cap) {
   if (cap <= t->path_capacity_alloc) return PT_OK;
   free_paths(t);
   constexpr size_t kF = 29, kU = 4;
-  const size_t n = cap;
+  const uint32_t seg_cap = ptd::segment_capacity(cap);
+  const size_t n = std::max<size_t>(cap, static_cast<size_t>(ptd::kSegments) * seg_cap);
   PT_HIP(t, t->d_path_f.ensure(kF * n), "allocate path state");
   PT_HIP(t, t->d_path_u.ensure(kU * n), "allocate path state");
   PT_HIP(t, t->d_rng.ensure(n), "allocate path state");
@@ -371,6 +395,7 @@ int ensure_paths(pt_tracer* t, uint32_t cap) {
   P.spath = u + 3 * n;
   P.rng = t->d_rng.ptr;
   P.capacity = cap;
+  P.seg_cap = seg_cap;
   t->path_capacity_alloc = cap;
   return PT_OK;
 }
@@ -409,14 +434,18 @@ int pt_hip_create(int device, pt_tracer** out) {
     delete t;
     return PT_INTERNAL_ERROR;
   }
-  if (t->d_counters.ensure(ptd::kCounterSlots) != hipSuccess || t->d_query_counters.ensure(ptd::kCounterSlots) != hipSuccess ||
-      t->d_queue.ensure(ptd::kQueueSlots) != hipSuccess || t->d_error.ensure(1) != hipSuccess) {
+  if (t->d_counters.ensure(ptd::kCounterWords) != hipSuccess || t->d_query_counters.ensure(ptd::kCounterWords) != hipSuccess ||
+      t->d_queue.ensure(ptd::kQueueWords) != hipSuccess || t->d_error.ensure(1) != hipSuccess) {
     (void)hipStreamDestroy(t->stream);
     delete t;
     return PT_ALLOC_FAILED;
   }
-  (void)hipMemsetAsync(t->d_counters.ptr, 0, sizeof(unsigned long long) * ptd::kCounterSlots, t->stream);
-  (void)hipMemsetAsync(t->d_queue.ptr, 0, sizeof(uint32_t) * ptd::kQueueSlots, t->stream);
+  (void)hipMemsetAsync(t->d_counters.ptr, 0, sizeof(unsigned long long) * ptd::kCounterWords, t->stream);
+  (void)hipMemsetAsync(t->d_queue.ptr, 0, sizeof(uint32_t) * ptd::kQueueWords, t->stream);
+  t->paths.queue = t->d_queue.ptr;
+  t->paths.counters = t->d_counters.ptr;
+  t->paths.error = t->d_error.ptr;
+  t->paths.refill_min = t->refill_min;
   (void)hipMemsetAsync(t->d_error.ptr, 0, sizeof(uint32_t), t->stream);
   t->grid_ext = ptd::persistent_grid_extend(device);
   t->grid_shade = ptd::persistent_grid_shade(device);
@@ -744,8 +773,7 @@ int pt_hip_read_counters(pt_tracer* t, PtSampleCounters* out) {
   if (!t || !out) return fail(t, PT_INVALID_ARGUMENT, "read_counters: null argument");
   (void)hipSetDevice(t->device);
   unsigned long long c[ptd::kCounterSlots];
-  PT_HIP(t, hipMemcpyAsync(c, t->d_counters.ptr, sizeof(c), hipMemcpyDeviceToHost, t->stream), "read counters");
-  const int rc = sync_all(t);
+  const int rc = read_counter_block(t, t->d_counters.ptr, c);
   if (rc != PT_OK) return rc;
   to_counters(c, out);
   return PT_OK;
@@ -755,8 +783,7 @@ int pt_hip_read_kernel_stats(pt_tracer* t, PtKernelStats* out) {
   if (!t || !out) return fail(t, PT_INVALID_ARGUMENT, "read_kernel_stats: null argument");
   (void)hipSetDevice(t->device);
   unsigned long long c[ptd::kCounterSlots];
-  PT_HIP(t, hipMemcpyAsync(c, t->d_counters.ptr, sizeof(c), hipMemcpyDeviceToHost, t->stream), "read counters");
-  const int rc = sync_all(t);
+  const int rc = read_counter_block(t, t->d_counters.ptr, c);
   if (rc != PT_OK) return rc;
   *out = t->kstats;
   out->inner_node_fetches = c[ptd::kInnerFetches];
@@ -845,8 +872,7 @@ int pt_hip_trace_rays(pt_tracer* t, int mode, const float* rays, const float* tm
   const hipError_t le = hipGetLastError();
   if (le != hipSuccess) return hip_fail(t, le, "kernel launch");
   unsigned long long c[ptd::kCounterSlots];
-  PT_HIP(t, hipMemcpyAsync(c, q.counters, sizeof(c), hipMemcpyDeviceToHost, t->stream), "read counters");
-  const int rc = sync_all(t);
+  const int rc = read_counter_block(t, q.counters, c);
   if (rc != PT_OK) return rc;
   if (counters) to_counters(c, counters);
   return PT_OK;

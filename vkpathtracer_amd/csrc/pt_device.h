@@ -10,8 +10,19 @@
 namespace ptd {
 
 constexpr int kBlock = 256;        // threads per workgroup (4 waves of 64)
-constexpr int kStackLds = 16;      // traversal stack entries per lane held in LDS
+constexpr int kWavesPerBlock = kBlock / 64;
+#ifndef PT_STACK_LDS
+#define PT_STACK_LDS 12
+#endif
+constexpr int kStackLds = PT_STACK_LDS;  // traversal stack entries per lane held in LDS
 constexpr int kCounterSlots = 16;  // device u64 counters (see CounterSlot)
+
+// Every same-address device atomic is serialized (about 11 ns each on MI355X), so counters and
+// queue cursors are striped: counter slot k of stripe s lives at counters[s * kCounterSlots + k]
+// (one 128-byte line per stripe) and a wave adds into stripe (global wave id % kStripes). Readers
+// sum the stripes.
+constexpr int kStripes = 64;
+constexpr int kCounterWords = kStripes * kCounterSlots;
 
 enum CounterSlot : int {
   kSamples = 0,
@@ -29,15 +40,22 @@ enum CounterSlot : int {
   kShadowLeafVisits = 12,
 };
 
-// Queue counters (u32), one set per wavefront pass.
-enum QueueSlot : int {
-  kQueueIn = 0,       // live paths entering this bounce
-  kQueueOut = 1,      // paths appended for the next bounce
-  kQueueShadow = 2,   // shadow rays appended this bounce
-  kFetchExtend = 3,   // persistent-thread work cursor, extend
-  kFetchShadow = 4,   // persistent-thread work cursor, shadow
-  kQueueSlots = 8,
-};
+// Ray queues are split into kSegments segments of seg_cap entries. Queue entry j of a bounce
+// (0 <= j < total) lives in the segment s with prefix[s] <= j < prefix[s + 1], at position
+// s * seg_cap + (j - prefix[s]). Producers append a 64-entry input chunk c's outputs to segment
+// c % kSegments, so the append cursors are spread over kSegments cache lines; a segment never
+// receives more than seg_cap = 64 * ceil(ceil(capacity / 64) / kSegments) entries.
+constexpr int kSegments = 64;
+constexpr int kSegLine = 32;  // u32 per cursor line (128 bytes)
+// u32 layout of the queue block
+constexpr int kQueueInCount = 0;                                  // kSegments counts of this bounce's input
+constexpr int kQueueOutCursor = kSegments;                        // kSegments cursors, kSegLine apart
+constexpr int kQueueShadowCursor = kQueueOutCursor + kSegments * kSegLine;
+constexpr int kQueueWords = kQueueShadowCursor + kSegments * kSegLine;
+
+__host__ __device__ constexpr uint32_t segment_capacity(uint32_t capacity) {
+  return 64u * ((((capacity + 63u) / 64u) + kSegments - 1u) / kSegments);
+}
 
 struct DevScene {
   const float4* nodes;   // 4 x float4 per inner node
@@ -99,14 +117,16 @@ struct DevPaths {
   float* stmax;
   float* scontrib[3];
   uint32_t* spath;
-  uint32_t* queue;               // QueueSlot counters
-  unsigned long long* counters;  // CounterSlot counters
+  uint32_t* queue;               // kQueueWords: segment counts and append cursors
+  unsigned long long* counters;  // kCounterWords: striped CounterSlot counters
   uint32_t* error;               // nonzero: a kernel refused work (stack overflow area too small)
   int32_t* stack_ref;            // traversal stack overflow beyond kStackLds
   float* stack_t;
   uint32_t stack_overflow_depth; // entries per lane in the overflow area
   uint32_t stack_threads;        // lanes the overflow area is sized for (persistent grid * kBlock)
-  uint32_t capacity;
+  uint32_t capacity;             // paths per pass (path-state arrays)
+  uint32_t seg_cap;              // entries per queue segment (queue arrays hold kSegments * seg_cap)
+  uint32_t refill_min;           // idle lanes a traversal wave gathers before it takes new queries
 };
 
 struct DevFilm {
@@ -149,8 +169,10 @@ void launch_shade(hipStream_t stream, const DevPaths& paths, const DevScene& sce
                   int in_buf, uint32_t depth, uint32_t grid);
 void launch_shadow(hipStream_t stream, const DevPaths& paths, const DevScene& scene, uint32_t grid);
 void launch_advance(hipStream_t stream, const DevPaths& paths);
+// Adds the radiance of every tile of the pass into the film, pixel by pixel in tile order.
+// rect = {x0, y0, x1, y1}: the bounding rectangle of the pass's tiles.
 void launch_film(hipStream_t stream, const DevPaths& paths, const DevFilm& film, uint32_t film_width,
-                 const DevTile& tile, uint32_t path_offset_in_pass);
+                 const DevTileTable& tiles, const uint32_t rect[4]);
 void launch_trace_rays(hipStream_t stream, const DevPaths& paths, const DevScene& scene, int any_hit,
                        const float* rays, const float* tmax, uint32_t count, float* out_t, int32_t* out_prim,
                        uint32_t grid);
@@ -163,7 +185,7 @@ void launch_shade_adjoint(hipStream_t stream, const DevPaths& paths, const DevAd
 void launch_shadow_adjoint(hipStream_t stream, const DevPaths& paths, const DevAdjoint& adj, const DevScene& scene,
                            uint32_t grid);
 void launch_copy_radiance(hipStream_t stream, const DevPaths& paths, const DevAdjoint& adj, uint32_t path_count);
-void launch_zero_counters(hipStream_t stream, unsigned long long* counters);
+void launch_zero_counters(hipStream_t stream, unsigned long long* counters);  // kCounterWords
 void launch_clear_film(hipStream_t stream, const DevFilm& film, uint32_t pixels);
 
 // Occupancy-derived persistent grid sizes.

@@ -150,110 +150,174 @@ struct Stack {
   uint32_t* error;
 };
 
-// Closest hit (kAny = false) or first leaf holding an accepted hit below best_t (kAny = true),
-// following CpuBvhAccelerator::intersect's visit order: children tested at the parent,
-// the nearer pushed last (left on ties), every pop re-tested against the current best t.
+// One BVH query in flight on a lane.
+struct Trav {
+  RayQ q;
+  V3 d;
+  float best_t;
+  int32_t best_prim, cur;
+  uint32_t sp;
+};
+
+// Starts a query below t_max. False when it ends at once (no primitives, or the root box is
+// missed); best_prim is then -1.
+__device__ __forceinline__ bool trav_begin(const DevScene& sc, Trav& s, V3 o, V3 d, float tmax, TravStats& st) {
+  s.best_prim = -1;
+  s.best_t = tmax;
+  s.d = d;
+  s.sp = 0;
+  s.cur = sc.root_ref;
+  if (sc.prim_count == 0) return false;
+  s.q = make_query(o, d);
+  ++st.visits;
+  float tn;
+  return slab(s.q, sc.root_box[0], sc.root_box[1], sc.root_box[2], sc.root_box[3], sc.root_box[4], sc.root_box[5],
+              s.best_t, tn);
+}
+
+// One traversal step: an inner node (descend, or fall through to the pop when neither child is
+// hit) or a leaf followed by the pop. Returns true when the query is finished.
+// Closest hit (kAny = false) or first leaf batch holding an accepted hit below best_t
+// (kAny = true), following CpuBvhAccelerator::intersect's visit order: children tested at the
+// parent, the nearer pushed last (left on ties), every pop re-tested against the current best t.
 // A popped entry carries the t_near of its box; t_near <= best_t is the same test as the
 // reference's re-run of intersect_aabb with max_t = best_t.
 template <bool kAny>
-__device__ __forceinline__ void traverse(const DevScene& sc, const Stack& stk, V3 o, V3 d, float& best_t,
-                                         int32_t& best_prim, TravStats& st) {
-  best_prim = -1;
-  if (sc.prim_count == 0) return;
-  const RayQ q = make_query(o, d);
-  float tn;
-  ++st.visits;
-  if (!slab(q, sc.root_box[0], sc.root_box[1], sc.root_box[2], sc.root_box[3], sc.root_box[4], sc.root_box[5],
-            best_t, tn))
-    return;
-  int32_t cur = sc.root_ref;
-  uint32_t sp = 0;
-  while (true) {
-    if (cur >= 0) {
-      ++st.inner;
-      const float4* np = sc.nodes + 4u * static_cast<uint32_t>(cur);
-      const float4 a = np[0], b = np[1], c = np[2];
-      const int4 r = *reinterpret_cast<const int4*>(np + 3);
-      float t0, t1;
-      const bool h0 = slab(q, a.x, a.y, a.z, a.w, b.x, b.y, best_t, t0);
-      const bool h1 = slab(q, b.z, b.w, c.x, c.y, c.z, c.w, best_t, t1);
-      if (h0 && h1) {
-        const bool left_first = t0 <= t1;
-        const int32_t near_ref = left_first ? r.x : r.y;
-        const int32_t far_ref = left_first ? r.y : r.x;
-        const float far_t = left_first ? t1 : t0;
-        if (sp < static_cast<uint32_t>(kStackLds)) {
-          stk.lds_ref[sp * kBlock + stk.lane_slot] = far_ref;
-          stk.lds_t[sp * kBlock + stk.lane_slot] = far_t;
-        } else if (sp - kStackLds < stk.ovf_depth) {
-          const size_t o2 = static_cast<size_t>(sp - kStackLds) * stk.ovf_stride + stk.gid;
-          stk.ovf_ref[o2] = far_ref;
-          stk.ovf_t[o2] = far_t;
-        } else {
-          atomicOr(stk.error, 1u);  // overflow area smaller than the tree depth: refuse, never write out of bounds
-          return;
-        }
-        ++sp;
-        cur = near_ref;
-        ++st.visits;
-        continue;
-      }
-      if (h0 || h1) {
-        cur = h0 ? r.x : r.y;
-        ++st.visits;
-        continue;
-      }
-    } else {
-      ++st.leaves;
-      const uint32_t code = ~static_cast<uint32_t>(cur);
-      const uint32_t first = code >> 5, count = code & 31u;
-      uint32_t accepted = 0;
-      for (uint32_t k = 0; k < count; ++k) {
-        ++st.tests;
-        float t;
-        const float4* tp = sc.tris + 3u * (first + k);
-        if (tri_test(tp, o, d, best_t, t)) {  // commit_ordered_lane_hit, BvhTriangleIntersector.cpp:25-33
-          ++accepted;
-          best_t = t;
-          best_prim = __float_as_int(tp[0].w);
-        }
-        // The reference tests leaves in batches of 8 (kBvhTriangleBatchWidth); an any-hit
-        // query stops at the end of the first batch that accepted a hit.
-        if (kAny && accepted && ((k & 7u) == 7u || k + 1u == count)) {
-          st.hits += accepted;
-          return;
-        }
-      }
-      st.hits += accepted;
-    }
-    bool found = false;
-    while (sp > 0) {
-      --sp;
-      int32_t ref;
-      float t;
-      if (sp < static_cast<uint32_t>(kStackLds)) {
-        ref = stk.lds_ref[sp * kBlock + stk.lane_slot];
-        t = stk.lds_t[sp * kBlock + stk.lane_slot];
+__device__ __forceinline__ bool trav_step(const DevScene& sc, const Stack& stk, Trav& s, TravStats& st) {
+  if (s.cur >= 0) {
+    ++st.inner;
+    const float4* np = sc.nodes + 4u * static_cast<uint32_t>(s.cur);
+    const float4 a = np[0], b = np[1], c = np[2];
+    const int4 r = *reinterpret_cast<const int4*>(np + 3);
+    float t0, t1;
+    const bool h0 = slab(s.q, a.x, a.y, a.z, a.w, b.x, b.y, s.best_t, t0);
+    const bool h1 = slab(s.q, b.z, b.w, c.x, c.y, c.z, c.w, s.best_t, t1);
+    if (h0 && h1) {
+      const bool left_first = t0 <= t1;
+      const int32_t near_ref = left_first ? r.x : r.y;
+      const int32_t far_ref = left_first ? r.y : r.x;
+      const float far_t = left_first ? t1 : t0;
+      if (s.sp < static_cast<uint32_t>(kStackLds)) {
+        stk.lds_ref[s.sp * kBlock + stk.lane_slot] = far_ref;
+        stk.lds_t[s.sp * kBlock + stk.lane_slot] = far_t;
+      } else if (s.sp - kStackLds < stk.ovf_depth) {
+        const size_t o2 = static_cast<size_t>(s.sp - kStackLds) * stk.ovf_stride + stk.gid;
+        stk.ovf_ref[o2] = far_ref;
+        stk.ovf_t[o2] = far_t;
       } else {
-        const size_t o2 = static_cast<size_t>(sp - kStackLds) * stk.ovf_stride + stk.gid;
-        ref = stk.ovf_ref[o2];
-        t = stk.ovf_t[o2];
+        atomicOr(stk.error, 1u);  // overflow area smaller than the tree depth: refuse, never write out of bounds
+        return true;
       }
+      ++s.sp;
+      s.cur = near_ref;
       ++st.visits;
-      if (t <= best_t) {
-        cur = ref;
-        found = true;
-        break;
+      return false;
+    }
+    if (h0 || h1) {
+      s.cur = h0 ? r.x : r.y;
+      ++st.visits;
+      return false;
+    }
+  } else {
+    ++st.leaves;
+    const uint32_t code = ~static_cast<uint32_t>(s.cur);
+    const uint32_t first = code >> 5, count = code & 31u;
+    const V3 o = mk(s.q.o[0], s.q.o[1], s.q.o[2]);
+    uint32_t accepted = 0;
+    for (uint32_t k = 0; k < count; ++k) {
+      ++st.tests;
+      float t;
+      const float4* tp = sc.tris + 3u * (first + k);
+      if (tri_test(tp, o, s.d, s.best_t, t)) {  // commit_ordered_lane_hit, BvhTriangleIntersector.cpp:25-33
+        ++accepted;
+        s.best_t = t;
+        s.best_prim = __float_as_int(tp[0].w);
+      }
+      // The reference tests leaves in batches of 8 (kBvhTriangleBatchWidth); an any-hit
+      // query stops at the end of the first batch that accepted a hit.
+      if (kAny && accepted && ((k & 7u) == 7u || k + 1u == count)) {
+        st.hits += accepted;
+        return true;
       }
     }
-    if (!found) break;
+    st.hits += accepted;
+  }
+  while (s.sp > 0) {
+    --s.sp;
+    int32_t ref;
+    float t;
+    if (s.sp < static_cast<uint32_t>(kStackLds)) {
+      ref = stk.lds_ref[s.sp * kBlock + stk.lane_slot];
+      t = stk.lds_t[s.sp * kBlock + stk.lane_slot];
+    } else {
+      const size_t o2 = static_cast<size_t>(s.sp - kStackLds) * stk.ovf_stride + stk.gid;
+      ref = stk.ovf_ref[o2];
+      t = stk.ovf_t[o2];
+    }
+    ++st.visits;
+    if (t <= s.best_t) {
+      s.cur = ref;
+      return false;
+    }
+  }
+  return true;
+}
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+}
+
+__device__ __forceinline__ uint32_t global_wave() { return blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6); }
+
+// Persistent query driver with per-lane refill. The queries [0, n) are cut into 64-entry chunks;
+// wave w owns chunks w, w + W, w + 2W, ... (W = waves in the grid), so no cursor is shared.
+// A lane that finishes takes the next entry of its wave's chunk as soon as `refill_min` lanes
+// are idle, so a wave is not held by its slowest ray.
+//   begin(j, s): start query j on this lane; false if it ended at once.
+//   end(j, s)  : the query finished; true if the lane started a follow-up query on s.
+template <bool kAny, typename Begin, typename End>
+__device__ __forceinline__ void drive_queries(const DevScene& sc, const Stack& stk, uint32_t n, uint32_t refill_min,
+                                              TravStats& st, Begin begin, End end) {
+  const uint32_t waves = gridDim.x * kWavesPerBlock;
+  uint32_t chunk = global_wave();
+  uint32_t next = 0;  // entries of `chunk` already handed out
+  bool active = false;
+  uint32_t j = 0;
+  Trav s;
+  while (true) {
+    const uint32_t base = chunk * 64u;
+    if (base < n) {
+      const uint64_t idle = __ballot(!active);
+      const uint32_t k = static_cast<uint32_t>(__popcll(idle));
+      if (k >= refill_min || k == 64u) {
+        const uint32_t left = n - base - next;
+        const uint32_t avail = (64u - next) < left ? (64u - next) : left;
+        const uint32_t rank = lanes_below(idle);
+        if (!active && rank < avail) {
+          j = base + next + rank;
+          active = begin(j, s) || end(j, s);
+        }
+        next += k < avail ? k : avail;
+        if (next >= 64u || base + next >= n) {
+          chunk += waves;
+          next = 0;
+        }
+      }
+    }
+    if (__ballot(active) == 0u) {
+      if (chunk * 64u >= n) break;
+      continue;
+    }
+    if (active && trav_step<kAny>(sc, stk, s, st)) active = end(j, s);
   }
 }
 
-__device__ __forceinline__ void wave_add(unsigned long long* dst, uint64_t v) {
+__device__ __forceinline__ void wave_add(unsigned long long* counters, int slot, uint64_t v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(static_cast<unsigned long long>(v), off);
-  if ((threadIdx.x & 63u) == 0u && v != 0u) atomicAdd(dst, static_cast<unsigned long long>(v));
+  if ((threadIdx.x & 63u) == 0u && v != 0u)
+    atomicAdd(counters + (global_wave() & (kStripes - 1u)) * kCounterSlots + slot, static_cast<unsigned long long>(v));
 }
 
 // Shadow traversals add to the same SampleCounters fields as extension rays, as the
@@ -261,21 +325,16 @@ __device__ __forceinline__ void wave_add(unsigned long long* dst, uint64_t v) {
 // keep their share for the byte accounting.
 __device__ __forceinline__ void flush_stats(unsigned long long* counters, const TravStats& s, bool shadow) {
   if (shadow) {
-    wave_add(counters + kShadowNodeVisits, s.visits);
-    wave_add(counters + kShadowTriTests, s.tests);
-    wave_add(counters + kShadowInnerFetches, s.inner);
-    wave_add(counters + kShadowLeafVisits, s.leaves);
+    wave_add(counters, kShadowNodeVisits, s.visits);
+    wave_add(counters, kShadowTriTests, s.tests);
+    wave_add(counters, kShadowInnerFetches, s.inner);
+    wave_add(counters, kShadowLeafVisits, s.leaves);
   }
-  wave_add(counters + kNodeVisits, s.visits);
-  wave_add(counters + kLeafVisits, s.leaves);
-  wave_add(counters + kTriTests, s.tests);
-  wave_add(counters + kTriHits, s.hits);
-  wave_add(counters + kInnerFetches, s.inner);
-}
-
-__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
-  return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
-                                   __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+  wave_add(counters, kNodeVisits, s.visits);
+  wave_add(counters, kLeafVisits, s.leaves);
+  wave_add(counters, kTriTests, s.tests);
+  wave_add(counters, kTriHits, s.hits);
+  wave_add(counters, kInnerFetches, s.inner);
 }
 
 // Wave-wide append: returns this lane's slot (valid when pred is true).
@@ -290,29 +349,64 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
   return base + lanes_below(m);
 }
 
+// prefix[0..kSegments] of the kSegments counts counts[s * stride], into LDS. Every thread of the
+// block must call it (it ends with a barrier).
+__device__ __forceinline__ void load_prefix(const uint32_t* counts, int stride, uint32_t* prefix) {
+  if (threadIdx.x < 64u) {
+    uint32_t v = counts[threadIdx.x * static_cast<uint32_t>(stride)];
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t u = __shfl_up(v, off);
+      if (static_cast<int>(threadIdx.x) >= off) v += u;
+    }
+    prefix[threadIdx.x + 1u] = v;
+    if (threadIdx.x == 0u) prefix[0] = 0u;
+  }
+  __syncthreads();
+}
+
+// Position of queue entry j (< prefix[kSegments]).
+__device__ __forceinline__ uint32_t queue_pos(const uint32_t* prefix, uint32_t j, uint32_t seg_cap) {
+  uint32_t lo = 0, hi = kSegments;  // prefix[lo] <= j < prefix[hi]
+  while (hi - lo > 1u) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (prefix[mid] <= j) lo = mid;
+    else hi = mid;
+  }
+  return lo * seg_cap + (j - prefix[lo]);
+}
+
+// Appends pred lanes to segment `seg` of a segmented queue; returns the position.
+__device__ __forceinline__ uint32_t segment_append(uint32_t* cursors, uint32_t seg, uint32_t seg_cap, bool pred) {
+  return seg * seg_cap + wave_append(cursors + seg * kSegLine, pred);
+}
+
 // ------------------------------------------------------------------ kernels
-__global__ void k_init_queue(uint32_t* queue, uint32_t live) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    queue[kQueueIn] = live;
-    queue[kQueueOut] = 0;
-    queue[kQueueShadow] = 0;
-    queue[kFetchExtend] = 0;
-    queue[kFetchShadow] = 0;
+// Initial queue of a pass: entries 0..live-1 at positions 0..live-1 (segments filled in order).
+__global__ void k_init_queue(uint32_t* queue, uint32_t live, uint32_t seg_cap) {
+  const uint32_t s = threadIdx.x;
+  if (s < static_cast<uint32_t>(kSegments)) {
+    const uint64_t start = static_cast<uint64_t>(s) * seg_cap;
+    const uint64_t left = live > start ? live - start : 0u;
+    queue[kQueueInCount + s] = static_cast<uint32_t>(left < seg_cap ? left : seg_cap);
+    queue[kQueueOutCursor + s * kSegLine] = 0u;
+    queue[kQueueShadowCursor + s * kSegLine] = 0u;
   }
 }
 
+// Between bounces: the appended entries become the next input.
 __global__ void k_advance(uint32_t* queue) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    queue[kQueueIn] = queue[kQueueOut];
-    queue[kQueueOut] = 0;
-    queue[kQueueShadow] = 0;
-    queue[kFetchExtend] = 0;
-    queue[kFetchShadow] = 0;
+  const uint32_t s = threadIdx.x;
+  if (s < static_cast<uint32_t>(kSegments)) {
+    queue[kQueueInCount + s] = queue[kQueueOutCursor + s * kSegLine];
+    queue[kQueueOutCursor + s * kSegLine] = 0u;
+    queue[kQueueShadowCursor + s * kSegLine] = 0u;
   }
 }
 
 __global__ void k_zero_counters(unsigned long long* c) {
-  if (threadIdx.x < kCounterSlots && blockIdx.x == 0) c[threadIdx.x] = 0ull;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < static_cast<uint32_t>(kCounterWords)) c[i] = 0u;
 }
 
 __global__ __launch_bounds__(kBlock) void k_clear_film(DevFilm film, uint32_t pixels) {
@@ -471,34 +565,32 @@ __device__ __forceinline__ Stack make_stack(const DevPaths& P, int32_t* s_ref, f
   return s;
 }
 
-// Persistent closest-hit kernel: each wave takes 64 queue entries at a time.
+// Persistent closest-hit kernel over the bounce's segmented ray queue.
 __global__ __launch_bounds__(kBlock) void k_extend(DevPaths P, DevRays in, DevScene sc) {
   __shared__ int32_t s_ref[kStackLds * kBlock];
   __shared__ float s_t[kStackLds * kBlock];
+  __shared__ uint32_t prefix[kSegments + 1];
+  load_prefix(P.queue + kQueueInCount, 1, prefix);
   const Stack stk = make_stack(P, s_ref, s_t);
-  const uint32_t n = P.queue[kQueueIn];
-  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t n = prefix[kSegments];
   TravStats st = {0, 0, 0, 0, 0};
   uint32_t rays = 0;
-  while (true) {
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(&P.queue[kFetchExtend], 64u);
-    base = __shfl(base, 0);
-    if (base >= n) break;
-    const uint32_t i = base + lane;
-    if (i < n) {
-      const V3 o = mk(in.c[0][i], in.c[1][i], in.c[2][i]);
-      const V3 d = mk(in.c[3][i], in.c[4][i], in.c[5][i]);
-      float bt = kInf;
-      int32_t bp;
-      traverse<false>(sc, stk, o, d, bt, bp, st);
-      P.hit_t[i] = bt;
-      P.hit_prim[i] = bp;
-      ++rays;
-    }
-  }
+  uint32_t pos = 0;
+  auto begin = [&](uint32_t j, Trav& s) {
+    pos = queue_pos(prefix, j, P.seg_cap);
+    const V3 o = mk(in.c[0][pos], in.c[1][pos], in.c[2][pos]);
+    const V3 d = mk(in.c[3][pos], in.c[4][pos], in.c[5][pos]);
+    return trav_begin(sc, s, o, d, kInf, st);
+  };
+  auto end = [&](uint32_t, Trav& s) {
+    P.hit_t[pos] = s.best_t;
+    P.hit_prim[pos] = s.best_prim;
+    ++rays;
+    return false;
+  };
+  drive_queries<false>(sc, stk, n, P.refill_min, st, begin, end);
   flush_stats(P.counters, st, false);
-  wave_add(P.counters + kRays, rays);
+  wave_add(P.counters, kRays, rays);
 }
 
 struct MatFlags {
@@ -1073,8 +1165,10 @@ template <bool kAdj>
 __global__ __launch_bounds__(kBlock) void k_shade(DevPaths P, DevRays in, DevRays out, DevScene sc, DevSettings st,
                                                  uint32_t depth, DevAdjoint A) {
   __shared__ float glds[kAdj ? kAdjLdsFloats : 1];
+  __shared__ uint32_t prefix[kSegments + 1];
   if constexpr (kAdj) grad_lds_clear(A, glds);
-  const uint32_t n = P.queue[kQueueIn];
+  load_prefix(P.queue + kQueueInCount, 1, prefix);
+  const uint32_t n = prefix[kSegments];
   const uint32_t stride = gridDim.x * kBlock;
   const uint32_t lane = threadIdx.x & 63u;
   uint32_t shadow_tests = 0, shadow_rays = 0;
@@ -1085,18 +1179,20 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevPaths P, DevRays in, DevRay
     so.shadow = false;
     uint32_t p = 0;
     if (i < n) {
-      p = in.qpath[i];
-      const V3 ro = mk(in.c[0][i], in.c[1][i], in.c[2][i]);
-      const V3 rd = mk(in.c[3][i], in.c[4][i], in.c[5][i]);
-      shade_path<kAdj>(P, sc, st, depth, p, ro, rd, P.hit_t[i], P.hit_prim[i], so, shadow_tests, A, glds);
+      const uint32_t q = queue_pos(prefix, i, P.seg_cap);
+      p = in.qpath[q];
+      const V3 ro = mk(in.c[0][q], in.c[1][q], in.c[2][q]);
+      const V3 rd = mk(in.c[3][q], in.c[4][q], in.c[5][q]);
+      shade_path<kAdj>(P, sc, st, depth, p, ro, rd, P.hit_t[q], P.hit_prim[q], so, shadow_tests, A, glds);
     }
-    const uint32_t slot = wave_append(&P.queue[kQueueOut], so.cont);
+    const uint32_t seg = (w0 >> 6) & (kSegments - 1u);  // this chunk's output segment
+    const uint32_t slot = segment_append(P.queue + kQueueOutCursor, seg, P.seg_cap, so.cont);
     if (so.cont) {
       out.c[0][slot] = so.no.x; out.c[1][slot] = so.no.y; out.c[2][slot] = so.no.z;
       out.c[3][slot] = so.nd.x; out.c[4][slot] = so.nd.y; out.c[5][slot] = so.nd.z;
       out.qpath[slot] = p;
     }
-    const uint32_t sslot = wave_append(&P.queue[kQueueShadow], so.shadow);
+    const uint32_t sslot = segment_append(P.queue + kQueueShadowCursor, seg, P.seg_cap, so.shadow);
     if (so.shadow) {
       P.sray[0][sslot] = so.so.x; P.sray[1][sslot] = so.so.y; P.sray[2][sslot] = so.so.z;
       P.sray[3][sslot] = so.sd.x; P.sray[4][sslot] = so.sd.y; P.sray[5][sslot] = so.sd.z;
@@ -1112,36 +1208,28 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevPaths P, DevRays in, DevRay
       ++shadow_rays;
     }
   }
-  wave_add(P.counters + kShadowTests, shadow_tests);
-  wave_add(P.counters + kShadowRays, shadow_rays);
+  wave_add(P.counters, kShadowTests, shadow_tests);
+  wave_add(P.counters, kShadowRays, shadow_rays);
   if constexpr (kAdj) grad_lds_flush(A, glds);
 }
 
-// Visibility of one NEE sample. Opaque scenes: any-hit query with t_max = dist - 0.004.
-// Scenes with transmissive materials: the reference's closest-hit loop that skips up to
-// 8 transmissive blockers (ScalarCpuPathTracer.cpp:1066-1087).
-__device__ __forceinline__ bool shadow_visible(const DevScene& sc, const Stack& stk, V3 o, V3 d, float remaining,
-                                               TravStats& st) {
-  if (!sc.has_transmissive) {
-    float bt = remaining;
-    int32_t bp;
-    traverse<true>(sc, stk, o, d, bt, bp, st);
-    return bp < 0;
+// NEE visibility over the bounce's shadow queue. Opaque scenes: an any-hit query with
+// t_max = dist - 0.004. Scenes with transmissive materials: the reference's closest-hit loop
+// that steps through up to 8 transmissive blockers (ScalarCpuPathTracer.cpp:1066-1087), each
+// step a follow-up query on the same lane.
+template <bool kAdj>
+__device__ __forceinline__ void light_visible(const DevPaths& P, const DevAdjoint& A, float* glds, uint32_t q) {
+  const uint32_t p = P.spath[q];
+  P.rad[0][p] = P.rad[0][p] + P.scontrib[0][q];
+  P.rad[1][p] = P.rad[1][p] + P.scontrib[1][q];
+  P.rad[2][p] = P.rad[2][p] + P.scontrib[2][q];
+  if constexpr (kAdj) {
+    const uint32_t mi = A.s_mat[q], li = A.s_light[q];
+    for (int c = 0; c < 3; ++c) {
+      gadd(A, glds, kGradAlbedo, mi, c, A.s_galb[c][q]);
+      gadd(A, glds, kGradLight, li, c, A.s_glight[c][q]);
+    }
   }
-  V3 vo = o;
-  for (uint32_t skips = 0; skips < 8u && remaining > 0.004f; ++skips) {
-    float bt = kInf;
-    int32_t bp;
-    traverse<false>(sc, stk, vo, d, bt, bp, st);
-    if (bp < 0 || bt >= remaining) break;
-    const float4 pn = sc.prims[bp];
-    const PtMaterial& m = material_at(sc, static_cast<uint32_t>(__float_as_int(pn.w)));
-    if (!(m.material_model == 5u || m.transmission > 0.05f)) return false;
-    const float advance = bt + 0.004f;
-    vo = add(vo, muls(d, advance));
-    remaining -= advance;
-  }
-  return true;
 }
 
 template <bool kAdj>
@@ -1149,95 +1237,130 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevPaths P, DevScene sc, DevA
   __shared__ int32_t s_ref[kStackLds * kBlock];
   __shared__ float s_t[kStackLds * kBlock];
   __shared__ float glds[kAdj ? kAdjLdsFloats : 1];
+  __shared__ uint32_t prefix[kSegments + 1];
   if constexpr (kAdj) grad_lds_clear(A, glds);
+  load_prefix(P.queue + kQueueShadowCursor, kSegLine, prefix);
   const Stack stk = make_stack(P, s_ref, s_t);
-  const uint32_t n = P.queue[kQueueShadow];
-  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t n = prefix[kSegments];
   TravStats st = {0, 0, 0, 0, 0};
-  while (true) {
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(&P.queue[kFetchShadow], 64u);
-    base = __shfl(base, 0);
-    if (base >= n) break;
-    const uint32_t i = base + lane;
-    if (i < n) {
-      const V3 o = mk(P.sray[0][i], P.sray[1][i], P.sray[2][i]);
-      const V3 d = mk(P.sray[3][i], P.sray[4][i], P.sray[5][i]);
-      if (shadow_visible(sc, stk, o, d, P.stmax[i], st)) {
-        const uint32_t p = P.spath[i];
-        P.rad[0][p] = P.rad[0][p] + P.scontrib[0][i];
-        P.rad[1][p] = P.rad[1][p] + P.scontrib[1][i];
-        P.rad[2][p] = P.rad[2][p] + P.scontrib[2][i];
-        if constexpr (kAdj) {
-          const uint32_t mi = A.s_mat[i], li = A.s_light[i];
-          for (int c = 0; c < 3; ++c) {
-            gadd(A, glds, kGradAlbedo, mi, c, A.s_galb[c][i]);
-            gadd(A, glds, kGradLight, li, c, A.s_glight[c][i]);
-          }
-        }
+  uint32_t q = 0;
+  if (!sc.has_transmissive) {
+    auto begin = [&](uint32_t j, Trav& s) {
+      q = queue_pos(prefix, j, P.seg_cap);
+      const V3 o = mk(P.sray[0][q], P.sray[1][q], P.sray[2][q]);
+      const V3 d = mk(P.sray[3][q], P.sray[4][q], P.sray[5][q]);
+      return trav_begin(sc, s, o, d, P.stmax[q], st);
+    };
+    auto end = [&](uint32_t, Trav& s) {
+      if (s.best_prim < 0) light_visible<kAdj>(P, A, glds, q);
+      return false;
+    };
+    drive_queries<true>(sc, stk, n, P.refill_min, st, begin, end);
+  } else {
+    V3 o, d;
+    float remaining = 0.0f;
+    uint32_t skips = 0;
+    // Starts the next step of the loop; false when the loop is over (then `end` is called).
+    auto begin = [&](uint32_t j, Trav& s) {
+      q = queue_pos(prefix, j, P.seg_cap);
+      o = mk(P.sray[0][q], P.sray[1][q], P.sray[2][q]);
+      d = mk(P.sray[3][q], P.sray[4][q], P.sray[5][q]);
+      remaining = P.stmax[q];
+      skips = 0;
+      if (!(remaining > 0.004f)) {
+        s.best_prim = -1;  // the loop body never runs: visible
+        return false;
       }
-    }
+      return trav_begin(sc, s, o, d, kInf, st);
+    };
+    auto end = [&](uint32_t, Trav& s) {
+      while (true) {
+        if (s.best_prim < 0 || s.best_t >= remaining) break;  // nothing in the way
+        const float4 pn = sc.prims[s.best_prim];
+        const PtMaterial& m = material_at(sc, static_cast<uint32_t>(__float_as_int(pn.w)));
+        if (!(m.material_model == 5u || m.transmission > 0.05f)) return false;  // opaque blocker
+        const float advance = s.best_t + 0.004f;
+        o = add(o, muls(d, advance));
+        remaining -= advance;
+        ++skips;
+        if (!(skips < 8u && remaining > 0.004f)) break;  // loop exhausted: visible
+        if (trav_begin(sc, s, o, d, kInf, st)) return true;
+        // the follow-up query ended at once (root box missed): nothing in the way
+        break;
+      }
+      light_visible<kAdj>(P, A, glds, q);
+      return false;
+    };
+    drive_queries<false>(sc, stk, n, P.refill_min, st, begin, end);
   }
   flush_stats(P.counters, st, true);
   if constexpr (kAdj) grad_lds_flush(A, glds);
 }
 
-// FilmBuffer::add_sample for one tile, after shade_pixel's non-finite drop
-// (ScalarCpuPathTracer.Render.cpp:270-283, FilmBuffer.cpp:80-91).
-__global__ __launch_bounds__(kBlock) void k_film(DevPaths P, DevFilm film, uint32_t film_width, DevTile tile,
-                                                uint32_t path_offset) {
+// FilmBuffer::add_sample for every tile of a pass, after shade_pixel's non-finite drop
+// (ScalarCpuPathTracer.Render.cpp:270-283, FilmBuffer.cpp:80-91). One thread per pixel of the
+// tiles' bounding rectangle adds the tiles' samples in table order, as sequential
+// render_tile calls would.
+__global__ __launch_bounds__(kBlock) void k_film(DevPaths P, DevFilm film, uint32_t film_width, DevTileTable tiles,
+                                                uint32_t x0, uint32_t y0, uint32_t rw, uint32_t rh) {
   const uint32_t local = blockIdx.x * kBlock + threadIdx.x;
-  const uint32_t n = tile.w * tile.h;
-  bool ok = false;
-  if (local < n) {
-    const uint32_t p = path_offset + local;
-    const float r = P.rad[0][p], g = P.rad[1][p], b = P.rad[2][p];
-    if (finite(r) && finite(g) && finite(b)) {
-      const uint32_t px = (tile.y + local / tile.w) * film_width + tile.x + local % tile.w;
-      float4 s = film.sum[px];
+  uint32_t added = 0;
+  if (local < rw * rh) {
+    const uint32_t x = x0 + local % rw, y = y0 + local / rw;
+    const uint32_t px = y * film_width + x;
+    float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    uint32_t cnt = 0;
+    bool touched = false;
+    for (uint32_t k = 0; k < tiles.count; ++k) {
+      const DevTile& t = tiles.t[k];
+      if (x < t.x || y < t.y || x - t.x >= t.w || y - t.y >= t.h) continue;
+      const uint32_t p = t.path_begin + (y - t.y) * t.w + (x - t.x);
+      const float r = P.rad[0][p], g = P.rad[1][p], b = P.rad[2][p];
+      if (!(finite(r) && finite(g) && finite(b))) continue;
+      if (!touched) {
+        s = film.sum[px];
+        cnt = film.count[px];
+        touched = true;
+      }
       s.x = s.x + r;
       s.y = s.y + g;
       s.z = s.z + b;
+      ++cnt;
+      ++added;
+    }
+    if (touched) {
       film.sum[px] = s;
-      film.count[px] = film.count[px] + 1u;
-      ok = true;
+      film.count[px] = cnt;
     }
   }
-  wave_add(P.counters + kSamples, ok ? 1u : 0u);
+  wave_add(P.counters, kSamples, added);
 }
 
-// Batched IRayAccelerator::intersect for the ray-query entry point.
-__global__ __launch_bounds__(kBlock) void k_trace_rays(DevPaths P, DevScene sc, int any_hit, const float* rays,
-                                                      const float* tmax, uint32_t n, float* out_t, int32_t* out_prim) {
+// Batched IRayAccelerator::intersect for the ray-query entry point (rays are 6 floats each).
+template <bool kAny>
+__global__ __launch_bounds__(kBlock) void k_trace_rays(DevPaths P, DevScene sc, const float* rays, const float* tmax,
+                                                      uint32_t n, float* out_t, int32_t* out_prim) {
   __shared__ int32_t s_ref[kStackLds * kBlock];
   __shared__ float s_t[kStackLds * kBlock];
   const Stack stk = make_stack(P, s_ref, s_t);
-  const uint32_t lane = threadIdx.x & 63u;
   TravStats st = {0, 0, 0, 0, 0};
-  while (true) {
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(&P.queue[kFetchExtend], 64u);
-    base = __shfl(base, 0);
-    if (base >= n) break;
-    const uint32_t i = base + lane;
-    if (i < n) {
-      const float* r = rays + 6u * static_cast<size_t>(i);
-      const V3 o = mk(r[0], r[1], r[2]), d = mk(r[3], r[4], r[5]);
-      float bt = tmax ? tmax[i] : kInf;
-      int32_t bp;
-      if (any_hit) traverse<true>(sc, stk, o, d, bt, bp, st);
-      else traverse<false>(sc, stk, o, d, bt, bp, st);
-      out_t[i] = bp >= 0 ? bt : kInf;
-      out_prim[i] = bp;
-    }
-  }
+  auto begin = [&](uint32_t j, Trav& s) {
+    const float* r = rays + 6u * static_cast<size_t>(j);
+    return trav_begin(sc, s, mk(r[0], r[1], r[2]), mk(r[3], r[4], r[5]), tmax ? tmax[j] : kInf, st);
+  };
+  auto end = [&](uint32_t j, Trav& s) {
+    out_t[j] = s.best_prim >= 0 ? s.best_t : kInf;
+    out_prim[j] = s.best_prim;
+    return false;
+  };
+  drive_queries<kAny>(sc, stk, n, P.refill_min, st, begin, end);
   flush_stats(P.counters, st, false);
 }
 
 }  // namespace
 
 void launch_init_queue(hipStream_t stream, const DevPaths& paths, uint32_t live) {
-  hipLaunchKernelGGL(k_init_queue, dim3(1), dim3(64), 0, stream, paths.queue, live);
+  hipLaunchKernelGGL(k_init_queue, dim3(1), dim3(64), 0, stream, paths.queue, live, paths.seg_cap);
 }
 
 void launch_advance(hipStream_t stream, const DevPaths& paths) {
@@ -1245,7 +1368,7 @@ void launch_advance(hipStream_t stream, const DevPaths& paths) {
 }
 
 void launch_zero_counters(hipStream_t stream, unsigned long long* counters) {
-  hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, stream, counters);
+  hipLaunchKernelGGL(k_zero_counters, dim3((kCounterWords + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, counters);
 }
 
 void launch_clear_film(hipStream_t stream, const DevFilm& film, uint32_t pixels) {
@@ -1299,18 +1422,23 @@ void launch_shadow_adjoint(hipStream_t stream, const DevPaths& paths, const DevA
 }
 
 void launch_film(hipStream_t stream, const DevPaths& paths, const DevFilm& film, uint32_t film_width,
-                 const DevTile& tile, uint32_t path_offset_in_pass) {
-  const uint32_t n = tile.w * tile.h;
-  if (n == 0) return;
-  hipLaunchKernelGGL(k_film, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, paths, film, film_width, tile,
-                     path_offset_in_pass);
+                 const DevTileTable& tiles, const uint32_t rect[4]) {
+  if (rect[2] <= rect[0] || rect[3] <= rect[1]) return;
+  const uint32_t rw = rect[2] - rect[0], rh = rect[3] - rect[1];
+  const uint32_t n = rw * rh;
+  hipLaunchKernelGGL(k_film, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, paths, film, film_width, tiles,
+                     rect[0], rect[1], rw, rh);
 }
 
 void launch_trace_rays(hipStream_t stream, const DevPaths& paths, const DevScene& scene, int any_hit,
                        const float* rays, const float* tmax, uint32_t count, float* out_t, int32_t* out_prim,
                        uint32_t grid) {
-  hipLaunchKernelGGL(k_trace_rays, dim3(grid), dim3(kBlock), 0, stream, paths, scene, any_hit, rays, tmax, count,
-                     out_t, out_prim);
+  if (any_hit)
+    hipLaunchKernelGGL(k_trace_rays<true>, dim3(grid), dim3(kBlock), 0, stream, paths, scene, rays, tmax, count, out_t,
+                       out_prim);
+  else
+    hipLaunchKernelGGL(k_trace_rays<false>, dim3(grid), dim3(kBlock), 0, stream, paths, scene, rays, tmax, count,
+                       out_t, out_prim);
 }
 
 static uint32_t occupancy_grid(int device, const void* fn) {
@@ -1324,7 +1452,7 @@ static uint32_t occupancy_grid(int device, const void* fn) {
 uint32_t persistent_grid_extend(int device) {
   const uint32_t a = occupancy_grid(device, reinterpret_cast<const void*>(&k_extend));
   const uint32_t b = occupancy_grid(device, reinterpret_cast<const void*>(&k_shadow<true>));
-  const uint32_t c = occupancy_grid(device, reinterpret_cast<const void*>(&k_trace_rays));
+  const uint32_t c = occupancy_grid(device, reinterpret_cast<const void*>(&k_trace_rays<false>));
   uint32_t m = a > b ? a : b;
   return m > c ? m : c;
 }
