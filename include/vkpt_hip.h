@@ -234,6 +234,17 @@ int pt_hip_render_adjoint(pt_tracer* tracer, const PtRenderTile* tiles, uint32_t
 int pt_hip_synchronize(pt_tracer* tracer);
 /* Maximum paths in flight per wavefront pass (default 8M). */
 int pt_hip_set_wavefront_capacity(pt_tracer* tracer, uint32_t paths);
+/* Launch-shape tuning (no counterpart in the reference; chosen by vkpathtracer_amd.autotune).
+ * In pt_hip_set_tuning a zero wavefront_capacity or refill_min keeps the current value, and a zero
+ * block count restores the occupancy-derived default (also the maximum). */
+typedef struct PtTuning {
+  uint32_t wavefront_capacity; /* paths in flight per wavefront pass */
+  uint32_t refill_min;         /* 1..64: idle lanes a traversal wave gathers before it takes new rays */
+  uint32_t traversal_blocks;   /* persistent workgroups of the extend / shadow / ray-query kernels */
+  uint32_t shade_blocks;       /* workgroups of the grid-stride shade kernel */
+} PtTuning;
+int pt_hip_set_tuning(pt_tracer* tracer, const PtTuning* tuning);
+int pt_hip_get_tuning(pt_tracer* tracer, PtTuning* tuning);
 /* HIP-event timing per kernel (off by default). */
 int pt_hip_set_profiling(pt_tracer* tracer, int enabled);
 int pt_hip_read_kernel_stats(pt_tracer* tracer, PtKernelStats* stats);

@@ -53,6 +53,52 @@ def test_trace_rays_any_hit_matches(small_procedural, tracer_factory):
     assert c["bvh_node_visits"] == oc["bvh_node_visits"]
 
 
+def test_trace_rays_axis_parallel_and_nonfinite(cornell, small_procedural, tracer_factory):
+    """Rays with axis-parallel directions (|d_k| <= 1e-4: the slab test's parallel branch) and rays
+    with NaN / infinite components take the general slab form; results and counters still match."""
+    from vkpathtracer_amd import scene as S
+
+    for sc in (cornell, small_procedural):
+        pt = tracer_factory(sc, S.RenderSettings(width=8, height=8))
+        rays = S.random_rays(sc, 20_000, seed=17)
+        rng = np.random.default_rng(3)
+        d = rays[:, 3:6]
+        k = rng.integers(0, 3, len(rays))
+        d[np.arange(len(rays)), k] = rng.choice([0.0, 1e-4, -1e-4, 5e-5], len(rays))  # one parallel axis
+        d[:2000, (k[:2000] + 1) % 3] = 0.0                                              # two parallel axes
+        d[2000:2100] = [[0.0, 0.0, 1.0]]
+        rays[2100:2120, 0] = np.nan
+        rays[2120:2140, 4] = np.inf
+        rays[2140:2160, 3:6] = np.nan
+        t, p, c = pt.trace_rays(rays)
+        ot, op, oc = _oracle(sc).trace_rays(rays)
+        assert np.array_equal(p, op), f"{(p != op).sum()} primitive ids differ"
+        assert np.array_equal(t, ot, equal_nan=True)
+        for key in ("bvh_node_visits", "bvh_leaf_visits", "triangle_tests", "triangle_hits"):
+            assert c[key] == oc[key], (key, c[key], oc[key])
+
+
+@pytest.mark.parametrize("tuning", [dict(refill_min=1), dict(refill_min=64, traversal_blocks=7, shade_blocks=3),
+                                    dict(wavefront_capacity=5000, refill_min=33)])
+def test_launch_tuning_is_invisible(cornell, tracer_factory, tuning):
+    """Refill threshold, grid sizes and pass capacity change scheduling only: film and counters match."""
+    from vkpathtracer_amd import scene as S
+
+    st = S.RenderSettings(width=96, height=64, spp=3, max_depth=8)
+    a = tracer_factory(cornell, st)
+    assert a.render_frame_samples(0, st.spp)
+    fa, ca = a.film()
+    b = tracer_factory(cornell, st)
+    b.set_tuning(**tuning)
+    got = b.tuning()
+    for k, v in tuning.items():
+        assert got[k] == v
+    assert b.render_frame_samples(0, st.spp)
+    fb, cb = b.film()
+    assert np.array_equal(ca, cb) and np.array_equal(fa, fb)
+    assert a.read_counters() == b.read_counters()
+
+
 def test_render_cornell_small_bitexact(cornell, tracer_factory):
     from vkpathtracer_amd import scene as S
 

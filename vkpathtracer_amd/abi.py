@@ -77,6 +77,13 @@ class PtTracerStatus(C.Structure):
                 ("current_sample", C.c_uint32), ("pad0", C.c_uint32)]
 
 
+class PtTuning(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("wavefront_capacity", "refill_min", "traversal_blocks", "shade_blocks")]
+
+    def as_dict(self) -> dict:
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
 KERNELS = ("raygen", "extend", "shade", "shadow", "film")
 
 
@@ -108,6 +115,8 @@ SIGNATURES = {
     "pt_hip_synchronize": (C.c_int, [_P]),
     "pt_hip_set_wavefront_capacity": (C.c_int, [_P, C.c_uint32]),
     "pt_hip_set_profiling": (C.c_int, [_P, C.c_int]),
+    "pt_hip_set_tuning": (C.c_int, [_P, C.POINTER(PtTuning)]),
+    "pt_hip_get_tuning": (C.c_int, [_P, C.POINTER(PtTuning)]),
     "pt_hip_read_kernel_stats": (C.c_int, [_P, C.POINTER(PtKernelStats)]),
     "pt_hip_reset_kernel_stats": (C.c_int, [_P]),
     "pt_bvh_build_host": (C.c_int, [C.POINTER(PtSceneView), C.POINTER(PtBvhParams), _P, _P, _P, _P, _P, _P, _P]),

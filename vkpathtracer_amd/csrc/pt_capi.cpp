@@ -100,7 +100,8 @@ struct pt_tracer {
   ptd::DevScene dscene{};
   ptd::DevCamera dcam{};
   ptd::DevPaths paths{};
-  uint32_t grid_ext = 0, grid_shade = 0;
+  uint32_t grid_ext = 0, grid_shade = 0;          // launch grids in use
+  uint32_t grid_ext_max = 0, grid_shade_max = 0;  // occupancy-derived defaults
 
   bool profiling = false;
   std::vector<ProfiledLaunch> pending;
@@ -416,6 +417,7 @@ uint32_t pt_hip_sizeof(int which) {
     case 5: return sizeof(PtRenderSettings);
     case 6: return sizeof(PtSampleCounters);
     case 7: return sizeof(PtRenderTile);
+    case 8: return sizeof(PtTuning);
     default: return 0;
   }
 }
@@ -447,9 +449,8 @@ int pt_hip_create(int device, pt_tracer** out) {
   t->paths.error = t->d_error.ptr;
   t->paths.refill_min = t->refill_min;
   (void)hipMemsetAsync(t->d_error.ptr, 0, sizeof(uint32_t), t->stream);
-  t->grid_ext = ptd::persistent_grid_extend(device);
-  t->grid_shade = ptd::persistent_grid_shade(device);
-  if (t->grid_shade == 0) t->grid_shade = 1024;
+  t->grid_ext = t->grid_ext_max = ptd::persistent_grid_extend(device);
+  t->grid_shade = t->grid_shade_max = std::max(1u, ptd::persistent_grid_shade(device));
   *out = t;
   return PT_OK;
 }
@@ -589,7 +590,7 @@ int pt_hip_build_accel(pt_tracer* t, const PtBvhParams* params, PtBuildInfo* inf
   const uint32_t need = t->bvh.max_depth + 2u > static_cast<uint32_t>(ptd::kStackLds)
                             ? t->bvh.max_depth + 2u - static_cast<uint32_t>(ptd::kStackLds)
                             : 1u;
-  const uint32_t threads = t->grid_ext * static_cast<uint32_t>(ptd::kBlock);
+  const uint32_t threads = t->grid_ext_max * static_cast<uint32_t>(ptd::kBlock);
   PT_HIP(t, t->d_stack_ref.ensure(static_cast<size_t>(need) * threads), "allocate traversal stack");
   PT_HIP(t, t->d_stack_t.ensure(static_cast<size_t>(need) * threads), "allocate traversal stack");
   bytes += static_cast<uint64_t>(need) * threads * 8u;
@@ -615,6 +616,10 @@ int pt_hip_build_accel(pt_tracer* t, const PtBvhParams* params, PtBuildInfo* inf
     if (m.material_model == 5u || m.transmission > 0.05f) S.has_transmissive = 1;
   S.root_ref = L.root_ref;
   for (int k = 0; k < 6; ++k) S.root_box[k] = L.root_box[k];
+  S.general_boxes = 0;
+  for (size_t i = 0; i < L.nodes.size(); i += 16)  // 12 box floats, then the two child refs
+    for (size_t k = 0; k < 12; ++k)
+      if (!std::isfinite(L.nodes[i + k])) S.general_boxes = 1;
   for (int k = 0; k < 3; ++k) {
     S.env[k] = t->view.environment_color[k];
     S.env_scale[k] = t->view.environment_map_scale[k];
@@ -819,6 +824,35 @@ int pt_hip_set_wavefront_capacity(pt_tracer* t, uint32_t paths) {
   if (rc != PT_OK) return rc;
   t->capacity = paths;
   if (t->path_capacity_alloc > paths) free_paths(t);
+  return PT_OK;
+}
+
+int pt_hip_set_tuning(pt_tracer* t, const PtTuning* tu) {
+  if (!t || !tu) return fail(t, PT_INVALID_ARGUMENT, "set_tuning: null argument");
+  if (tu->wavefront_capacity && tu->wavefront_capacity < 1024u)
+    return fail(t, PT_INVALID_ARGUMENT, "wavefront capacity must be at least 1024 paths");
+  if (tu->refill_min > 64u) return fail(t, PT_INVALID_ARGUMENT, "refill_min must be in 1..64");
+  if (tu->traversal_blocks > t->grid_ext_max || tu->shade_blocks > t->grid_shade_max)
+    return fail(t, PT_INVALID_ARGUMENT, "set_tuning: block count above the occupancy-derived maximum");
+  (void)hipSetDevice(t->device);
+  const int rc = pt_hip_synchronize(t);
+  if (rc != PT_OK) return rc;
+  if (tu->wavefront_capacity) {
+    t->capacity = tu->wavefront_capacity;
+    if (t->path_capacity_alloc > t->capacity) free_paths(t);
+  }
+  if (tu->refill_min) t->refill_min = t->paths.refill_min = tu->refill_min;
+  t->grid_ext = tu->traversal_blocks ? tu->traversal_blocks : t->grid_ext_max;
+  t->grid_shade = tu->shade_blocks ? tu->shade_blocks : t->grid_shade_max;
+  return PT_OK;
+}
+
+int pt_hip_get_tuning(pt_tracer* t, PtTuning* tu) {
+  if (!t || !tu) return fail(t, PT_INVALID_ARGUMENT, "get_tuning: null argument");
+  tu->wavefront_capacity = t->capacity;
+  tu->refill_min = t->refill_min;
+  tu->traversal_blocks = t->grid_ext;
+  tu->shade_blocks = t->grid_shade;
   return PT_OK;
 }
 

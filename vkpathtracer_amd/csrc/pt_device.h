@@ -64,6 +64,7 @@ struct DevScene {
   const PtMaterial* materials;
   const PtLight* lights;
   uint32_t material_count, light_count, prim_count, has_transmissive;
+  uint32_t general_boxes;  // some node box coordinate is not finite: every ray takes the select form of slab
   int32_t root_ref;
   float root_box[6];
   float env[3];

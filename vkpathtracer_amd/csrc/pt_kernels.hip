@@ -77,6 +77,7 @@ __device__ __forceinline__ float next01(uint64_t& s) {
 struct RayQ {
   float o[3], inv[3];
   bool par[3];
+  bool general;  // some axis parallel, or a non-finite origin/direction: take the select form of slab
 };
 
 __device__ __forceinline__ RayQ make_query(V3 o, V3 d) {
@@ -88,6 +89,8 @@ __device__ __forceinline__ RayQ make_query(V3 o, V3 d) {
     q.par[k] = __builtin_fabsf(dd[k]) <= kEps;
     q.inv[k] = q.par[k] ? 0.0f : 1.0f / dd[k];
   }
+  q.general = q.par[0] || q.par[1] || q.par[2] || !(finite(o.x) && finite(o.y) && finite(o.z) && finite(d.x) &&
+                                                   finite(d.y) && finite(d.z));
   return q;
 }
 
@@ -112,6 +115,33 @@ __device__ __forceinline__ bool slab(const RayQ& q, float mnx, float mny, float 
   }
   t_near = tmin;
   return !outside && !(tmin > tmax) && tmax > kEps;
+}
+
+// The same test for a ray with no parallel axis and finite origin, direction and box: every t
+// is then a non-NaN float, so the std::min/std::max selects equal v_min/v_max except for the sign
+// of a zero, which cannot change tmin (>= kEps) nor the outcome (a zero tmax fails tmax > kEps).
+// Both children of a node in one pass.
+__device__ __forceinline__ void slab2_fast(const RayQ& q, const float4 a, const float4 b, const float4 c, float max_t,
+                                           bool& h0, float& t0, bool& h1, float& t1) {
+  const float ox = q.o[0], oy = q.o[1], oz = q.o[2], ix = q.inv[0], iy = q.inv[1], iz = q.inv[2];
+  {
+    const float x0 = (a.x - ox) * ix, x1 = (a.w - ox) * ix;
+    const float y0 = (a.y - oy) * iy, y1 = (b.x - oy) * iy;
+    const float z0 = (a.z - oz) * iz, z1 = (b.y - oz) * iz;
+    const float tmin = fmaxf(fmaxf(fmaxf(kEps, fminf(x0, x1)), fminf(y0, y1)), fminf(z0, z1));
+    const float tmax = fminf(fminf(fminf(max_t, fmaxf(x0, x1)), fmaxf(y0, y1)), fmaxf(z0, z1));
+    t0 = tmin;
+    h0 = tmin <= tmax && tmax > kEps;
+  }
+  {
+    const float x0 = (b.z - ox) * ix, x1 = (c.y - ox) * ix;
+    const float y0 = (b.w - oy) * iy, y1 = (c.z - oy) * iy;
+    const float z0 = (c.x - oz) * iz, z1 = (c.w - oz) * iz;
+    const float tmin = fmaxf(fmaxf(fmaxf(kEps, fminf(x0, x1)), fminf(y0, y1)), fminf(z0, z1));
+    const float tmax = fminf(fminf(fminf(max_t, fmaxf(x0, x1)), fmaxf(y0, y1)), fmaxf(z0, z1));
+    t1 = tmin;
+    h1 = tmin <= tmax && tmax > kEps;
+  }
 }
 
 struct TravStats {
@@ -190,8 +220,13 @@ __device__ __forceinline__ bool trav_step(const DevScene& sc, const Stack& stk, 
     const float4 a = np[0], b = np[1], c = np[2];
     const int4 r = *reinterpret_cast<const int4*>(np + 3);
     float t0, t1;
-    const bool h0 = slab(s.q, a.x, a.y, a.z, a.w, b.x, b.y, s.best_t, t0);
-    const bool h1 = slab(s.q, b.z, b.w, c.x, c.y, c.z, c.w, s.best_t, t1);
+    bool h0, h1;
+    if (!s.q.general && !sc.general_boxes) {
+      slab2_fast(s.q, a, b, c, s.best_t, h0, t0, h1, t1);
+    } else {
+      h0 = slab(s.q, a.x, a.y, a.z, a.w, b.x, b.y, s.best_t, t0);
+      h1 = slab(s.q, b.z, b.w, c.x, c.y, c.z, c.w, s.best_t, t1);
+    }
     if (h0 && h1) {
       const bool left_first = t0 <= t1;
       const int32_t near_ref = left_first ? r.x : r.y;
@@ -225,6 +260,12 @@ __device__ __forceinline__ bool trav_step(const DevScene& sc, const Stack& stk, 
     const V3 o = mk(s.q.o[0], s.q.o[1], s.q.o[2]);
     uint32_t accepted = 0;
     for (uint32_t k = 0; k < count; ++k) {
+      // IntersectBvhTriangleBatch skips a batch of 8 whose entry best t is not above kEps (only a
+      // NaN t accepted from a NaN ray gets here) but the triangles still count as tested.
+      if ((k & 7u) == 0u && !(s.best_t > kEps)) {
+        st.tests += count - k;
+        break;
+      }
       ++st.tests;
       float t;
       const float4* tp = sc.tris + 3u * (first + k);

@@ -215,6 +215,18 @@ class HipPathTracer:
     def set_wavefront_capacity(self, paths: int):
         self._check(self._lib.pt_hip_set_wavefront_capacity(self._h, int(paths)), "set_wavefront_capacity")
 
+    def set_tuning(self, wavefront_capacity: int = 0, refill_min: int = 0, traversal_blocks: int = 0,
+                   shade_blocks: int = 0):
+        """Launch shape (PtTuning): zero capacity / refill_min keep the current value, zero block
+        counts restore the occupancy-derived defaults."""
+        tu = abi.PtTuning(int(wavefront_capacity), int(refill_min), int(traversal_blocks), int(shade_blocks))
+        self._check(self._lib.pt_hip_set_tuning(self._h, C.byref(tu)), "set_tuning")
+
+    def tuning(self) -> dict:
+        tu = abi.PtTuning()
+        self._check(self._lib.pt_hip_get_tuning(self._h, C.byref(tu)), "get_tuning")
+        return tu.as_dict()
+
     def set_profiling(self, enabled: bool):
         self._check(self._lib.pt_hip_set_profiling(self._h, int(enabled)), "set_profiling")
 
