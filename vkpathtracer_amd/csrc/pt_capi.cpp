@@ -16,8 +16,8 @@
 
 namespace {
 
-constexpr uint32_t kDefaultCapacity = 8u * 1024u * 1024u;
-constexpr uint32_t kDefaultRefillMin = 16u;
+constexpr uint32_t kDefaultCapacity = 128u * 1024u * 1024u;  // paths per pass: 64 spp of 1080p in one pass (~20 GB)
+constexpr uint32_t kDefaultRefillMin = 8u;
 constexpr uint32_t kMaxTilesPerPass = 96u;
 
 template <typename T>

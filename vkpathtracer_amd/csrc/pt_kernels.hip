@@ -205,84 +205,93 @@ __device__ __forceinline__ bool trav_begin(const DevScene& sc, Trav& s, V3 o, V3
               s.best_t, tn);
 }
 
-// One traversal step: an inner node (descend, or fall through to the pop when neither child is
-// hit) or a leaf followed by the pop. Returns true when the query is finished.
-// Closest hit (kAny = false) or first leaf batch holding an accepted hit below best_t
-// (kAny = true), following CpuBvhAccelerator::intersect's visit order: children tested at the
-// parent, the nearer pushed last (left on ties), every pop re-tested against the current best t.
-// A popped entry carries the t_near of its box; t_near <= best_t is the same test as the
-// reference's re-run of intersect_aabb with max_t = best_t.
-template <bool kAny>
-__device__ __forceinline__ bool trav_step(const DevScene& sc, const Stack& stk, Trav& s, TravStats& st) {
-  if (s.cur >= 0) {
-    ++st.inner;
-    const float4* np = sc.nodes + 4u * static_cast<uint32_t>(s.cur);
-    const float4 a = np[0], b = np[1], c = np[2];
-    const int4 r = *reinterpret_cast<const int4*>(np + 3);
-    float t0, t1;
-    bool h0, h1;
-    if (!s.q.general && !sc.general_boxes) {
-      slab2_fast(s.q, a, b, c, s.best_t, h0, t0, h1, t1);
-    } else {
-      h0 = slab(s.q, a.x, a.y, a.z, a.w, b.x, b.y, s.best_t, t0);
-      h1 = slab(s.q, b.z, b.w, c.x, c.y, c.z, c.w, s.best_t, t1);
-    }
-    if (h0 && h1) {
-      const bool left_first = t0 <= t1;
-      const int32_t near_ref = left_first ? r.x : r.y;
-      const int32_t far_ref = left_first ? r.y : r.x;
-      const float far_t = left_first ? t1 : t0;
-      if (s.sp < static_cast<uint32_t>(kStackLds)) {
-        stk.lds_ref[s.sp * kBlock + stk.lane_slot] = far_ref;
-        stk.lds_t[s.sp * kBlock + stk.lane_slot] = far_t;
-      } else if (s.sp - kStackLds < stk.ovf_depth) {
-        const size_t o2 = static_cast<size_t>(s.sp - kStackLds) * stk.ovf_stride + stk.gid;
-        stk.ovf_ref[o2] = far_ref;
-        stk.ovf_t[o2] = far_t;
-      } else {
-        atomicOr(stk.error, 1u);  // overflow area smaller than the tree depth: refuse, never write out of bounds
-        return true;
-      }
-      ++s.sp;
-      s.cur = near_ref;
-      ++st.visits;
-      return false;
-    }
-    if (h0 || h1) {
-      s.cur = h0 ? r.x : r.y;
-      ++st.visits;
-      return false;
-    }
+// Traversal in three pieces, following CpuBvhAccelerator::intersect's visit order for closest hit
+// (kAny = false) or the first leaf batch holding an accepted hit below best_t (kAny = true):
+// children tested at the parent, the nearer pushed last (left on ties), every pop re-tested against
+// the current best t. A popped entry carries the t_near of its box; t_near <= best_t is the same
+// test as the reference's re-run of intersect_aabb with max_t = best_t.
+enum StepResult : int { kStepGo = 0, kStepPop = 1, kStepDone = 2 };
+
+// Inner node s.cur: descend into the nearer hit child (pushing the other when both are hit), or
+// ask for a pop when neither is hit.
+__device__ __forceinline__ int trav_inner(const DevScene& sc, const Stack& stk, Trav& s, TravStats& st) {
+  ++st.inner;
+  const float4* np = sc.nodes + 4u * static_cast<uint32_t>(s.cur);
+  const float4 a = np[0], b = np[1], c = np[2];
+  const int4 r = *reinterpret_cast<const int4*>(np + 3);
+  float t0, t1;
+  bool h0, h1;
+  if (!s.q.general && !sc.general_boxes) {
+    slab2_fast(s.q, a, b, c, s.best_t, h0, t0, h1, t1);
   } else {
-    ++st.leaves;
-    const uint32_t code = ~static_cast<uint32_t>(s.cur);
-    const uint32_t first = code >> 5, count = code & 31u;
-    const V3 o = mk(s.q.o[0], s.q.o[1], s.q.o[2]);
-    uint32_t accepted = 0;
-    for (uint32_t k = 0; k < count; ++k) {
-      // IntersectBvhTriangleBatch skips a batch of 8 whose entry best t is not above kEps (only a
-      // NaN t accepted from a NaN ray gets here) but the triangles still count as tested.
-      if ((k & 7u) == 0u && !(s.best_t > kEps)) {
-        st.tests += count - k;
-        break;
-      }
-      ++st.tests;
-      float t;
-      const float4* tp = sc.tris + 3u * (first + k);
-      if (tri_test(tp, o, s.d, s.best_t, t)) {  // commit_ordered_lane_hit, BvhTriangleIntersector.cpp:25-33
-        ++accepted;
-        s.best_t = t;
-        s.best_prim = __float_as_int(tp[0].w);
-      }
-      // The reference tests leaves in batches of 8 (kBvhTriangleBatchWidth); an any-hit
-      // query stops at the end of the first batch that accepted a hit.
-      if (kAny && accepted && ((k & 7u) == 7u || k + 1u == count)) {
-        st.hits += accepted;
-        return true;
-      }
-    }
-    st.hits += accepted;
+    h0 = slab(s.q, a.x, a.y, a.z, a.w, b.x, b.y, s.best_t, t0);
+    h1 = slab(s.q, b.z, b.w, c.x, c.y, c.z, c.w, s.best_t, t1);
   }
+  if (h0 && h1) {
+    const bool left_first = t0 <= t1;
+    const int32_t near_ref = left_first ? r.x : r.y;
+    const int32_t far_ref = left_first ? r.y : r.x;
+    const float far_t = left_first ? t1 : t0;
+    if (s.sp < static_cast<uint32_t>(kStackLds)) {
+      stk.lds_ref[s.sp * kBlock + stk.lane_slot] = far_ref;
+      stk.lds_t[s.sp * kBlock + stk.lane_slot] = far_t;
+    } else if (s.sp - kStackLds < stk.ovf_depth) {
+      const size_t o2 = static_cast<size_t>(s.sp - kStackLds) * stk.ovf_stride + stk.gid;
+      stk.ovf_ref[o2] = far_ref;
+      stk.ovf_t[o2] = far_t;
+    } else {
+      atomicOr(stk.error, 1u);  // overflow area smaller than the tree depth: refuse, never write out of bounds
+      return kStepDone;
+    }
+    ++s.sp;
+    s.cur = near_ref;
+    ++st.visits;
+    return kStepGo;
+  }
+  if (h0 || h1) {
+    s.cur = h0 ? r.x : r.y;
+    ++st.visits;
+    return kStepGo;
+  }
+  return kStepPop;
+}
+
+// Leaf s.cur: its triangles in batches of 8 (kBvhTriangleBatchWidth). kStepDone when an any-hit
+// query accepted a hit, else kStepPop.
+template <bool kAny>
+__device__ __forceinline__ int trav_leaf(const DevScene& sc, Trav& s, TravStats& st) {
+  ++st.leaves;
+  const uint32_t code = ~static_cast<uint32_t>(s.cur);
+  const uint32_t first = code >> 5, count = code & 31u;
+  const V3 o = mk(s.q.o[0], s.q.o[1], s.q.o[2]);
+  uint32_t accepted = 0;
+  for (uint32_t k = 0; k < count; ++k) {
+    // IntersectBvhTriangleBatch skips a batch of 8 whose entry best t is not above kEps (only a
+    // NaN t accepted from a NaN ray gets here) but the triangles still count as tested.
+    if ((k & 7u) == 0u && !(s.best_t > kEps)) {
+      st.tests += count - k;
+      break;
+    }
+    ++st.tests;
+    float t;
+    const float4* tp = sc.tris + 3u * (first + k);
+    if (tri_test(tp, o, s.d, s.best_t, t)) {  // commit_ordered_lane_hit, BvhTriangleIntersector.cpp:25-33
+      ++accepted;
+      s.best_t = t;
+      s.best_prim = __float_as_int(tp[0].w);
+    }
+    // an any-hit query stops at the end of the first batch that accepted a hit
+    if (kAny && accepted && ((k & 7u) == 7u || k + 1u == count)) {
+      st.hits += accepted;
+      return kStepDone;
+    }
+  }
+  st.hits += accepted;
+  return kStepPop;
+}
+
+// Pops until an entry passes t_near <= best_t (kStepGo, s.cur set) or the stack is empty (kStepDone).
+__device__ __forceinline__ int trav_pop(const Stack& stk, Trav& s, TravStats& st) {
   while (s.sp > 0) {
     --s.sp;
     int32_t ref;
@@ -298,10 +307,10 @@ __device__ __forceinline__ bool trav_step(const DevScene& sc, const Stack& stk, 
     ++st.visits;
     if (t <= s.best_t) {
       s.cur = ref;
-      return false;
+      return kStepGo;
     }
   }
-  return true;
+  return kStepDone;
 }
 
 __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
@@ -346,11 +355,23 @@ __device__ __forceinline__ void drive_queries(const DevScene& sc, const Stack& s
         }
       }
     }
-    if (__ballot(active) == 0u) {
+    // One step per iteration for the larger of the two groups of lanes, those at inner nodes or
+    // those at leaves; the others wait, so each step runs with most of its lanes busy.
+    const bool at_inner = active && s.cur >= 0;
+    const uint32_t n_inner = static_cast<uint32_t>(__popcll(__ballot(at_inner)));
+    const uint32_t n_leaf = static_cast<uint32_t>(__popcll(__ballot(active && !at_inner)));
+    if (n_inner + n_leaf == 0u) {
       if (chunk * 64u >= n) break;
       continue;
     }
-    if (active && trav_step<kAny>(sc, stk, s, st)) active = end(j, s);
+    int r = kStepGo;
+    if (n_inner >= n_leaf) {
+      if (at_inner) r = trav_inner(sc, stk, s, st);
+    } else if (active && !at_inner) {
+      r = trav_leaf<kAny>(sc, s, st);
+    }
+    if (r == kStepPop) r = trav_pop(stk, s, st);
+    if (r == kStepDone) active = end(j, s);
   }
 }
 
