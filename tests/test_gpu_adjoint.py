@@ -134,3 +134,49 @@ def test_autograd_function_inverse_rendering_step():
     print(f"loss {float(loss):.5f} -> {float(loss2):.5f}")
     assert float(loss2) < float(loss)
     r.close()
+
+
+def test_adjoint_reuses_forward_radiance_only_when_valid():
+    """A render_tiles of the same tiles leaves each path's radiance in place and the adjoint skips its
+    forward replay; after update_scene_delta it must replay. Gradients agree with a fresh tracer's
+    (up to the order of the fp32 atomic adds)."""
+    import torch
+
+    from vkpathtracer_amd import scene as S
+    from vkpathtracer_amd.tracer import HipPathTracer
+
+    sc = _scene()
+    st = S.RenderSettings(width=40, height=32, spp=3, max_depth=8)
+    fg = np.random.default_rng(5).normal(size=(st.height, st.width, 3)).astype(np.float32)
+    tiles = [(0, 0, st.width, st.height, s, 0, 0) for s in range(st.spp)]
+    dev = torch.device("cuda", 0)
+    m2 = sc.materials.copy()
+    m2["albedo"] = np.clip(m2["albedo"] * 0.8 + 0.1, 0.05, 0.95)
+    sc2 = dataclasses.replace(sc, materials=m2)
+
+    def grads(pt, nm, nl):
+        ga = torch.zeros((nm, 3), device=dev)
+        ge = torch.zeros((nm, 3), device=dev)
+        gl = torch.zeros((nl, 3), device=dev)
+        pt.render_adjoint(tiles, torch.from_numpy(fg).to(dev), ga, ge, gl)
+        return [x.cpu().numpy().astype(np.float64) for x in (ga, ge, gl)]
+
+    def close(a, b):
+        for x, y in zip(a, b):
+            assert np.abs(x - y).max() <= 1e-5 * max(np.abs(y).max(), 1e-6)
+
+    fresh1 = _gpu_adjoint(sc, st, fg)
+    fresh2 = _gpu_adjoint(sc2, st, fg)
+    with HipPathTracer(0) as pt:
+        assert pt.configure(st) and pt.load_scene_snapshot(sc) and pt.build_or_update_acceleration()
+        nm, nl = len(sc.materials), len(sc.lights)
+        assert pt.render_tiles(tiles)
+        close(grads(pt, nm, nl), fresh1)           # reuse
+        close(grads(pt, nm, nl), fresh1)           # no forward in between: replays
+        assert pt.render_tiles(tiles)
+        assert pt.update_scene_delta(materials=m2)
+        close(grads(pt, nm, nl), fresh2)           # scene changed after the render: replays
+        assert pt.render_tiles(tiles)
+        close(grads(pt, nm, nl), fresh2)           # reuse with the new scene
+        assert pt.render_tiles(tiles[:2])
+        close(grads(pt, nm, nl), fresh2)           # different tiles: replays

@@ -97,6 +97,14 @@ struct pt_tracer {
   uint32_t capacity = kDefaultCapacity;
   uint32_t refill_min = kDefaultRefillMin;
 
+  // Per-path radiance of the last forward render, still in the path arrays when that render was
+  // a single pass: an adjoint of the same tiles, frame and scene reuses it instead of replaying
+  // the forward pass. Any change of scene, settings or path arrays clears it.
+  bool fwd_rad_valid = false;
+  std::vector<ptd::DevTile> fwd_work;
+  uint64_t fwd_frame = 0;
+  uint64_t scene_version = 0, fwd_scene_version = 0;
+
   ptd::DevScene dscene{};
   ptd::DevCamera dcam{};
   ptd::DevPaths paths{};
@@ -210,6 +218,7 @@ void free_scene_device(pt_tracer* t) {
 }
 
 void free_paths(pt_tracer* t) {
+  t->fwd_rad_valid = false;
   t->d_path_f.release();
   t->d_path_u.release();
   t->d_rng.release();
@@ -310,10 +319,21 @@ void bounce_loop(pt_tracer* t, const ptd::DevPaths& P, const ptd::DevSettings& s
   }
 }
 
+bool same_work(const std::vector<ptd::DevTile>& a, const std::vector<ptd::DevTile>& b) {
+  if (a.size() != b.size()) return false;
+  for (size_t i = 0; i < a.size(); ++i)
+    if (a[i].x != b[i].x || a[i].y != b[i].y || a[i].w != b[i].w || a[i].h != b[i].h || a[i].sample != b[i].sample ||
+        a[i].path_begin != b[i].path_begin)
+      return false;
+  return true;
+}
+
 // Runs the wavefront over `work` in passes of at most `capacity` paths.
 // adj == nullptr: forward passes whose radiance is accumulated into the film in tile order.
 // adj != nullptr: per pass, a forward replay (radiance kept per path) then the adjoint replay;
-//                 the film and the render counters are left untouched.
+//                 the film and the render counters are left untouched. The forward replay is
+//                 skipped when the previous call was the single-pass forward render of the same
+//                 tiles, frame and scene (the autograd forward-then-backward pattern).
 int run_passes(pt_tracer* t, std::vector<ptd::DevTile>& work, const ptd::DevSettings& st, ptd::DevAdjoint* adj) {
   const uint32_t cap = std::max<uint32_t>(t->capacity, 1024u);
   const ptd::DevFilm film{t->d_film_sum.ptr, t->d_film_count.ptr};
@@ -329,8 +349,12 @@ int run_passes(pt_tracer* t, std::vector<ptd::DevTile>& work, const ptd::DevSett
       paths += n;
       ++end;
     }
+    const bool single_pass = next == 0 && end == work.size();
     int rc = ensure_paths(t, std::max(paths, std::min(cap, 1u << 20)));
     if (rc != PT_OK) return rc;
+    // the last forward render left exactly these paths' radiance in P.rad
+    const bool reuse = adj && single_pass && t->fwd_rad_valid && st.frame == t->fwd_frame &&
+                       t->scene_version == t->fwd_scene_version && same_work(work, t->fwd_work);
     ptd::DevTileTable table{};
     table.count = static_cast<uint32_t>(end - next);
     for (size_t i = next; i < end; ++i) table.t[i - next] = work[i];
@@ -340,10 +364,13 @@ int run_passes(pt_tracer* t, std::vector<ptd::DevTile>& work, const ptd::DevSett
       if (rc != PT_OK) return rc;
       P.counters = t->d_query_counters.ptr;
     }
-    ptd::launch_init_queue(t->stream, P, paths);
-    timed(t, PT_KERNEL_RAYGEN, paths, [&] { ptd::launch_raygen(t->stream, P, t->dcam, st, table, paths); });
-    bounce_loop(t, P, st, nullptr);
+    if (!reuse) {
+      ptd::launch_init_queue(t->stream, P, paths);
+      timed(t, PT_KERNEL_RAYGEN, paths, [&] { ptd::launch_raygen(t->stream, P, t->dcam, st, table, paths); });
+      bounce_loop(t, P, st, nullptr);
+    }
     if (adj) {
+      t->fwd_rad_valid = false;  // the replay below overwrites P.rad
       ptd::launch_copy_radiance(t->stream, P, *adj, paths);
       ptd::launch_init_queue(t->stream, P, paths);
       timed(t, PT_KERNEL_RAYGEN, paths,
@@ -358,6 +385,12 @@ int run_passes(pt_tracer* t, std::vector<ptd::DevTile>& work, const ptd::DevSett
         rect[3] = std::max(rect[3], work[i].y + work[i].h);
       }
       timed(t, PT_KERNEL_FILM, paths, [&] { ptd::launch_film(t->stream, P, film, st.width, table, rect); });
+      t->fwd_rad_valid = single_pass;
+      if (single_pass) {
+        t->fwd_work = work;
+        t->fwd_frame = st.frame;
+        t->fwd_scene_version = t->scene_version;
+      }
     }
     const hipError_t le = hipGetLastError();
     if (le != hipSuccess) return hip_fail(t, le, "kernel launch");
@@ -493,6 +526,7 @@ int pt_hip_configure(pt_tracer* t, const PtRenderSettings* s) {
   ptd::launch_clear_film(t->stream, ptd::DevFilm{t->d_film_sum.ptr, t->d_film_count.ptr}, static_cast<uint32_t>(pixels));
   ptd::launch_zero_counters(t->stream, t->d_counters.ptr);
   t->configured = true;
+  ++t->scene_version;
   t->scene_loaded = false;
   t->accel_valid = false;
   t->current_sample = 0;
@@ -545,6 +579,7 @@ int pt_hip_load_scene(pt_tracer* t, const PtSceneView* sc) {
   t->view.light_count = static_cast<uint32_t>(t->lights.size());
   t->scene_loaded = true;
   t->accel_valid = false;
+  ++t->scene_version;
   t->last_error.clear();
   return PT_OK;
 }
@@ -667,6 +702,7 @@ int pt_hip_build_accel(pt_tracer* t, const PtBvhParams* params, PtBuildInfo* inf
   t->info.deterministic = 1;
   if (info) *info = t->info;
   t->accel_valid = true;
+  ++t->scene_version;
   t->last_error.clear();
   return PT_OK;
 }
@@ -985,6 +1021,7 @@ extern "C" int pt_hip_update_scene_delta(pt_tracer* t, const PtMaterial* materia
     PT_HIP(t, hipMemcpy(t->d_lights.ptr, lights, sizeof(PtLight) * light_count, hipMemcpyHostToDevice),
            "upload lights");
   }
+  ++t->scene_version;
   t->last_error.clear();
   return PT_OK;
 }
