@@ -235,8 +235,9 @@ int pt_hip_synchronize(pt_tracer* tracer);
 /* Maximum paths in flight per wavefront pass (default 8M). */
 int pt_hip_set_wavefront_capacity(pt_tracer* tracer, uint32_t paths);
 /* Launch-shape tuning (no counterpart in the reference; chosen by vkpathtracer_amd.autotune).
- * In pt_hip_set_tuning a zero wavefront_capacity or refill_min keeps the current value, and a zero
- * block count restores the occupancy-derived default (also the maximum). */
+ * In pt_hip_set_tuning a zero wavefront_capacity or refill_min keeps the current value; a block
+ * count caps each kernel's occupancy-derived persistent grid, and zero restores that grid.
+ * pt_hip_get_tuning reports the extend and shade grids. */
 typedef struct PtTuning {
   uint32_t wavefront_capacity; /* paths in flight per wavefront pass */
   uint32_t refill_min;         /* 1..64: idle lanes a traversal wave gathers before it takes new rays */

@@ -108,8 +108,7 @@ struct pt_tracer {
   ptd::DevScene dscene{};
   ptd::DevCamera dcam{};
   ptd::DevPaths paths{};
-  uint32_t grid_ext = 0, grid_shade = 0;          // launch grids in use
-  uint32_t grid_ext_max = 0, grid_shade_max = 0;  // occupancy-derived defaults
+  ptd::Grids grids{}, grids_max{};  // launch grids in use, occupancy-derived maxima
 
   bool profiling = false;
   std::vector<ProfiledLaunch> pending;
@@ -304,15 +303,15 @@ ptd::DevSettings dev_settings(const pt_tracer* t, uint64_t frame_index) {
 void bounce_loop(pt_tracer* t, const ptd::DevPaths& P, const ptd::DevSettings& st, const ptd::DevAdjoint* adj) {
   int buf = 0;
   for (uint32_t depth = 0; depth < st.max_depth; ++depth) {
-    timed(t, PT_KERNEL_EXTEND, 0, [&] { ptd::launch_extend(t->stream, P, t->dscene, buf, t->grid_ext); });
+    timed(t, PT_KERNEL_EXTEND, 0, [&] { ptd::launch_extend(t->stream, P, t->dscene, buf, t->grids.extend); });
     timed(t, PT_KERNEL_SHADE, 0, [&] {
-      if (adj) ptd::launch_shade_adjoint(t->stream, P, *adj, t->dscene, st, buf, depth, t->grid_shade);
-      else ptd::launch_shade(t->stream, P, t->dscene, st, buf, depth, t->grid_shade);
+      if (adj) ptd::launch_shade_adjoint(t->stream, P, *adj, t->dscene, st, buf, depth, t->grids.shade_adj);
+      else ptd::launch_shade(t->stream, P, t->dscene, st, buf, depth, t->grids.shade);
     });
     if (st.enable_nee && depth + 1u < st.max_depth)
       timed(t, PT_KERNEL_SHADOW, 0, [&] {
-        if (adj) ptd::launch_shadow_adjoint(t->stream, P, *adj, t->dscene, t->grid_ext);
-        else ptd::launch_shadow(t->stream, P, t->dscene, t->grid_ext);
+        if (adj) ptd::launch_shadow_adjoint(t->stream, P, *adj, t->dscene, t->grids.shadow_adj);
+        else ptd::launch_shadow(t->stream, P, t->dscene, t->grids.shadow);
       });
     ptd::launch_advance(t->stream, P);
     buf ^= 1;
@@ -482,8 +481,7 @@ int pt_hip_create(int device, pt_tracer** out) {
   t->paths.error = t->d_error.ptr;
   t->paths.refill_min = t->refill_min;
   (void)hipMemsetAsync(t->d_error.ptr, 0, sizeof(uint32_t), t->stream);
-  t->grid_ext = t->grid_ext_max = ptd::persistent_grid_extend(device);
-  t->grid_shade = t->grid_shade_max = std::max(1u, ptd::persistent_grid_shade(device));
+  t->grids = t->grids_max = ptd::resident_grids(device);
   *out = t;
   return PT_OK;
 }
@@ -621,11 +619,12 @@ int pt_hip_build_accel(pt_tracer* t, const PtBvhParams* params, PtBuildInfo* inf
   PT_HIP(t, upload(t->d_lights, t->lights.data(), t->lights.size(), sizeof(PtLight)), "upload lights");
   PT_HIP(t, upload(t->d_env, t->environment_map.data(), t->environment_map.size(), sizeof(float)), "upload environment");
 
-  // Traversal stack overflow area: a lane holds at most max_depth entries.
-  const uint32_t need = t->bvh.max_depth + 2u > static_cast<uint32_t>(ptd::kStackLds)
-                            ? t->bvh.max_depth + 2u - static_cast<uint32_t>(ptd::kStackLds)
-                            : 1u;
-  const uint32_t threads = t->grid_ext_max * static_cast<uint32_t>(ptd::kBlock);
+  // Traversal stack spill area: a lane's stack never holds more than max_depth entries (one per
+  // level of the current path); entries older than the LDS ring's kStackLds go here.
+  const uint32_t need = t->bvh.max_depth + 2u;
+  const uint32_t threads =
+      std::max({t->grids_max.extend, t->grids_max.shadow, t->grids_max.shadow_adj, t->grids_max.query}) *
+      static_cast<uint32_t>(ptd::kBlock);
   PT_HIP(t, t->d_stack_ref.ensure(static_cast<size_t>(need) * threads), "allocate traversal stack");
   PT_HIP(t, t->d_stack_t.ensure(static_cast<size_t>(need) * threads), "allocate traversal stack");
   bytes += static_cast<uint64_t>(need) * threads * 8u;
@@ -868,8 +867,6 @@ int pt_hip_set_tuning(pt_tracer* t, const PtTuning* tu) {
   if (tu->wavefront_capacity && tu->wavefront_capacity < 1024u)
     return fail(t, PT_INVALID_ARGUMENT, "wavefront capacity must be at least 1024 paths");
   if (tu->refill_min > 64u) return fail(t, PT_INVALID_ARGUMENT, "refill_min must be in 1..64");
-  if (tu->traversal_blocks > t->grid_ext_max || tu->shade_blocks > t->grid_shade_max)
-    return fail(t, PT_INVALID_ARGUMENT, "set_tuning: block count above the occupancy-derived maximum");
   (void)hipSetDevice(t->device);
   const int rc = pt_hip_synchronize(t);
   if (rc != PT_OK) return rc;
@@ -878,8 +875,13 @@ int pt_hip_set_tuning(pt_tracer* t, const PtTuning* tu) {
     if (t->path_capacity_alloc > t->capacity) free_paths(t);
   }
   if (tu->refill_min) t->refill_min = t->paths.refill_min = tu->refill_min;
-  t->grid_ext = tu->traversal_blocks ? tu->traversal_blocks : t->grid_ext_max;
-  t->grid_shade = tu->shade_blocks ? tu->shade_blocks : t->grid_shade_max;
+  auto pick = [](uint32_t want, uint32_t max) { return want ? std::min(want, max) : max; };
+  t->grids.extend = pick(tu->traversal_blocks, t->grids_max.extend);
+  t->grids.shadow = pick(tu->traversal_blocks, t->grids_max.shadow);
+  t->grids.shadow_adj = pick(tu->traversal_blocks, t->grids_max.shadow_adj);
+  t->grids.query = pick(tu->traversal_blocks, t->grids_max.query);
+  t->grids.shade = pick(tu->shade_blocks, t->grids_max.shade);
+  t->grids.shade_adj = pick(tu->shade_blocks, t->grids_max.shade_adj);
   return PT_OK;
 }
 
@@ -887,8 +889,8 @@ int pt_hip_get_tuning(pt_tracer* t, PtTuning* tu) {
   if (!t || !tu) return fail(t, PT_INVALID_ARGUMENT, "get_tuning: null argument");
   tu->wavefront_capacity = t->capacity;
   tu->refill_min = t->refill_min;
-  tu->traversal_blocks = t->grid_ext;
-  tu->shade_blocks = t->grid_shade;
+  tu->traversal_blocks = t->grids.extend;
+  tu->shade_blocks = t->grids.shade;
   return PT_OK;
 }
 
@@ -938,7 +940,7 @@ int pt_hip_trace_rays(pt_tracer* t, int mode, const float* rays, const float* tm
   ptd::launch_zero_counters(t->stream, q.counters);
   ptd::launch_init_queue(t->stream, q, 0);
   if (count)
-    ptd::launch_trace_rays(t->stream, q, t->dscene, mode, rays, tmax, count, out_t, out_prim, t->grid_ext);
+    ptd::launch_trace_rays(t->stream, q, t->dscene, mode, rays, tmax, count, out_t, out_prim, t->grids.query);
   const hipError_t le = hipGetLastError();
   if (le != hipSuccess) return hip_fail(t, le, "kernel launch");
   unsigned long long c[ptd::kCounterSlots];

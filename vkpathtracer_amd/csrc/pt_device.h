@@ -12,9 +12,9 @@ namespace ptd {
 constexpr int kBlock = 256;        // threads per workgroup (4 waves of 64)
 constexpr int kWavesPerBlock = kBlock / 64;
 #ifndef PT_STACK_LDS
-#define PT_STACK_LDS 12
+#define PT_STACK_LDS 10
 #endif
-constexpr int kStackLds = PT_STACK_LDS;  // traversal stack entries per lane held in LDS
+constexpr int kStackLds = PT_STACK_LDS;  // newest traversal stack entries per lane held in LDS (a ring)
 constexpr int kCounterSlots = 16;  // device u64 counters (see CounterSlot)
 
 // Every same-address device atomic is serialized (about 11 ns each on MI355X), so counters and
@@ -189,8 +189,11 @@ void launch_copy_radiance(hipStream_t stream, const DevPaths& paths, const DevAd
 void launch_zero_counters(hipStream_t stream, unsigned long long* counters);  // kCounterWords
 void launch_clear_film(hipStream_t stream, const DevFilm& film, uint32_t pixels);
 
-// Occupancy-derived persistent grid sizes.
-uint32_t persistent_grid_extend(int device);
-uint32_t persistent_grid_shade(int device);
+// Workgroups of each kernel that are resident on the device at once (its persistent grid): a
+// persistent kernel launched with more would leave the extra waves' chunks to a late tail.
+struct Grids {
+  uint32_t extend, shadow, shadow_adj, query, shade, shade_adj;
+};
+Grids resident_grids(int device);
 
 }  // namespace ptd

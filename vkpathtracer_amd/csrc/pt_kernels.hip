@@ -171,6 +171,9 @@ __device__ __forceinline__ bool tri_test(const float4* tp, V3 o, V3 d, float bes
   return !(t <= kEps || t >= best_t);
 }
 
+// Traversal stack of one lane: the newest kStackLds entries live in an LDS ring (slot = index %
+// kStackLds); older entries are spilled to a per-lane global area (entry i at ovf[i * ovf_stride +
+// gid]) when the ring is full and read back from there when the pops reach them.
 struct Stack {
   int32_t* lds_ref;
   float* lds_t;
@@ -186,7 +189,8 @@ struct Trav {
   V3 d;
   float best_t;
   int32_t best_prim, cur;
-  uint32_t sp;
+  uint32_t sp;  // entries on the stack
+  uint32_t lo;  // entries [lo, sp) are in the LDS ring, [0, lo) in the global area
 };
 
 // Starts a query below t_max. False when it ends at once (no primitives, or the root box is
@@ -196,6 +200,7 @@ __device__ __forceinline__ bool trav_begin(const DevScene& sc, Trav& s, V3 o, V3
   s.best_t = tmax;
   s.d = d;
   s.sp = 0;
+  s.lo = 0;
   s.cur = sc.root_ref;
   if (sc.prim_count == 0) return false;
   s.q = make_query(o, d);
@@ -232,17 +237,20 @@ __device__ __forceinline__ int trav_inner(const DevScene& sc, const Stack& stk, 
     const int32_t near_ref = left_first ? r.x : r.y;
     const int32_t far_ref = left_first ? r.y : r.x;
     const float far_t = left_first ? t1 : t0;
-    if (s.sp < static_cast<uint32_t>(kStackLds)) {
-      stk.lds_ref[s.sp * kBlock + stk.lane_slot] = far_ref;
-      stk.lds_t[s.sp * kBlock + stk.lane_slot] = far_t;
-    } else if (s.sp - kStackLds < stk.ovf_depth) {
-      const size_t o2 = static_cast<size_t>(s.sp - kStackLds) * stk.ovf_stride + stk.gid;
-      stk.ovf_ref[o2] = far_ref;
-      stk.ovf_t[o2] = far_t;
-    } else {
-      atomicOr(stk.error, 1u);  // overflow area smaller than the tree depth: refuse, never write out of bounds
-      return kStepDone;
+    if (s.sp - s.lo == static_cast<uint32_t>(kStackLds)) {  // ring full: spill its oldest entry
+      if (s.lo >= stk.ovf_depth) {
+        atomicOr(stk.error, 1u);  // overflow area smaller than the tree depth: refuse, never write out of bounds
+        return kStepDone;
+      }
+      const uint32_t slot = (s.lo % kStackLds) * kBlock + stk.lane_slot;
+      const size_t o2 = static_cast<size_t>(s.lo) * stk.ovf_stride + stk.gid;
+      stk.ovf_ref[o2] = stk.lds_ref[slot];
+      stk.ovf_t[o2] = stk.lds_t[slot];
+      ++s.lo;
     }
+    const uint32_t slot = (s.sp % kStackLds) * kBlock + stk.lane_slot;
+    stk.lds_ref[slot] = far_ref;
+    stk.lds_t[slot] = far_t;
     ++s.sp;
     s.cur = near_ref;
     ++st.visits;
@@ -296,13 +304,15 @@ __device__ __forceinline__ int trav_pop(const Stack& stk, Trav& s, TravStats& st
     --s.sp;
     int32_t ref;
     float t;
-    if (s.sp < static_cast<uint32_t>(kStackLds)) {
-      ref = stk.lds_ref[s.sp * kBlock + stk.lane_slot];
-      t = stk.lds_t[s.sp * kBlock + stk.lane_slot];
-    } else {
-      const size_t o2 = static_cast<size_t>(s.sp - kStackLds) * stk.ovf_stride + stk.gid;
+    if (s.sp >= s.lo) {
+      const uint32_t slot = (s.sp % kStackLds) * kBlock + stk.lane_slot;
+      ref = stk.lds_ref[slot];
+      t = stk.lds_t[slot];
+    } else {  // below the ring: the ring is empty from here on
+      const size_t o2 = static_cast<size_t>(s.sp) * stk.ovf_stride + stk.gid;
       ref = stk.ovf_ref[o2];
       t = stk.ovf_t[o2];
+      s.lo = s.sp;
     }
     ++st.visits;
     if (t <= s.best_t) {
@@ -1511,16 +1521,17 @@ static uint32_t occupancy_grid(int device, const void* fn) {
   return static_cast<uint32_t>(cus) * static_cast<uint32_t>(blocks);
 }
 
-uint32_t persistent_grid_extend(int device) {
-  const uint32_t a = occupancy_grid(device, reinterpret_cast<const void*>(&k_extend));
-  const uint32_t b = occupancy_grid(device, reinterpret_cast<const void*>(&k_shadow<true>));
-  const uint32_t c = occupancy_grid(device, reinterpret_cast<const void*>(&k_trace_rays<false>));
-  uint32_t m = a > b ? a : b;
-  return m > c ? m : c;
-}
-
-uint32_t persistent_grid_shade(int device) {
-  return occupancy_grid(device, reinterpret_cast<const void*>(&k_shade<true>));
+Grids resident_grids(int device) {
+  Grids g;
+  g.extend = occupancy_grid(device, reinterpret_cast<const void*>(&k_extend));
+  g.shadow = occupancy_grid(device, reinterpret_cast<const void*>(&k_shadow<false>));
+  g.shadow_adj = occupancy_grid(device, reinterpret_cast<const void*>(&k_shadow<true>));
+  const uint32_t q0 = occupancy_grid(device, reinterpret_cast<const void*>(&k_trace_rays<false>));
+  const uint32_t q1 = occupancy_grid(device, reinterpret_cast<const void*>(&k_trace_rays<true>));
+  g.query = q0 < q1 ? q0 : q1;
+  g.shade = occupancy_grid(device, reinterpret_cast<const void*>(&k_shade<false>));
+  g.shade_adj = occupancy_grid(device, reinterpret_cast<const void*>(&k_shade<true>));
+  return g;
 }
 
 }  // namespace ptd
