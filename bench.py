@@ -18,6 +18,7 @@ stream and the kernels' own work counters.
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import platform
@@ -158,6 +159,68 @@ def adjoint_ratio(log, reps: int = 3) -> dict:
             "ratio": round(b / f, 3), "standalone_ratio": round(sa / f, 3)}
 
 
+def batched_comparison(log, count: int = 64, reps: int = 2) -> dict:
+    """Many scenes in one batch against the same scenes rendered one tracer at a time.
+
+    Workload: `count` seeded scenes (displaced icosphere + instanced spheres + 8 quad lights) with
+    triangle counts log-uniform in [1k, 500k], square images from {128, 256, 512, 1024}, spp from
+    {4, 16, 64}, depth 8, NEE + MIS. batched = one pt_batch_render of every scene's full frame (eager
+    launches, then HIP-graph replay); per_scene = one tracer per scene, each rendering its frame with
+    render_tiles, timed frame by frame and summed (scene upload and BVH builds are outside all timings)."""
+    import torch
+
+    from vkpathtracer_amd import scene as S
+    from vkpathtracer_amd.tracer import HipBatch, HipPathTracer
+
+    jobs = S.batch_scenes(seed=7, count=count)
+    settings = S.RenderSettings(width=1, height=1, spp=1, max_depth=8)
+    paths = sum(w * h * spp for _, w, h, spp in jobs)
+    tris = sum(sc.triangle_count for sc, _, _, _ in jobs)
+    log(f"batch: {count} scenes, {tris} triangles, {paths} paths per frame")
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3
+
+    out = {"workload": f"{count} seeded scenes, 1k-500k triangles (log-uniform), images 128^2..1024^2, spp 4/16/64, "
+                       "depth 8", "scenes": count, "triangles": tris, "paths_per_frame": paths}
+    with HipBatch(settings) as b:
+        for sc, w, h, spp in jobs:
+            b.add_scene(sc, w, h, spp)
+        info = b.build()
+        out["bvh_build_ms"] = round(info["build_ms"], 1)
+        for use_graph, key in ((False, "batched"), (True, "batched_graph")):
+            def frame():
+                b.reset_accumulation()
+                b.render(0, use_graph=use_graph)
+                b.synchronize()
+            frame()
+            ms = min(timed(frame) for _ in range(reps))
+            out[key + "_ms"] = round(ms, 2)
+            out[key + "_mpaths_per_s"] = round(paths / ms / 1e3, 2)
+    per_scene = 0.0
+    for sc, w, h, spp in jobs:
+        st = dataclasses.replace(settings, width=w, height=h, spp=spp)
+        tiles = [(0, 0, w, h, s, 0, 0) for s in range(spp)]
+        with HipPathTracer(0) as pt:
+            assert pt.configure(st) and pt.load_scene_snapshot(sc) and pt.build_or_update_acceleration("sah", 4, 8)
+
+            def frame():
+                pt.reset_accumulation()
+                assert pt.render_tiles(tiles)
+                pt.synchronize()
+            frame()
+            per_scene += min(timed(frame) for _ in range(reps))
+    out["per_scene_ms"] = round(per_scene, 2)
+    out["per_scene_mpaths_per_s"] = round(paths / per_scene / 1e3, 2)
+    out["speedup_batched_vs_per_scene"] = round(per_scene / min(out["batched_ms"], out["batched_graph_ms"]), 3)
+    log(f"batched {out['batched_ms']} ms, graph {out['batched_graph_ms']} ms, per-scene {out['per_scene_ms']} ms")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -170,6 +233,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="time without per-kernel events")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward time ratio")
+    ap.add_argument("--no-batched", action="store_true", help="skip the many-scene batch comparison")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run a plain device copy of this size first (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -266,6 +330,7 @@ def main():
     bvh_info = {k: pt.build_info[k] for k in ("node_count", "leaf_count", "max_depth", "device_bytes")}
     pt.close()
     adjoint = None if args.no_adjoint else adjoint_ratio(log)
+    batched = None if args.no_batched else batched_comparison(log)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(sc, st, args.cpu_seconds)
@@ -295,6 +360,7 @@ def main():
         "kernels": per_kernel,
         "cpu_baseline": cpu,
         "adjoint": adjoint,
+        "batched": batched,
     }
     if cpu:
         out["speedup_vs_cpu"] = round(mpaths / cpu["value"], 1)

@@ -251,6 +251,36 @@ int pt_hip_set_profiling(pt_tracer* tracer, int enabled);
 int pt_hip_read_kernel_stats(pt_tracer* tracer, PtKernelStats* stats);
 int pt_hip_reset_kernel_stats(pt_tracer* tracer);
 
+/* Batched rendering (no counterpart in the reference, whose tracers hold one scene): many scenes,
+ * each with its own image size, spp and BVH parameters, packed into one device scene pool and
+ * rendered by one wavefront launch sequence per frame (paths of every scene share the ray queues,
+ * so small scenes fill the GPU together). Integrator settings (seed, depth, NEE, MIS, roulette,
+ * lens defaults) are shared by the batch; each scene's film is its own. */
+typedef struct pt_batch pt_batch;
+int pt_batch_create(int device, const PtRenderSettings* settings, pt_batch** out);
+void pt_batch_destroy(pt_batch* batch);
+const char* pt_batch_last_error(const pt_batch* batch);
+/* Copies a scene (as pt_hip_load_scene does); index receives its position in the batch. params may be NULL. */
+int pt_batch_add_scene(pt_batch* batch, const PtSceneView* scene, uint32_t width, uint32_t height, uint32_t spp,
+                       const PtBvhParams* params, uint32_t* index);
+/* Builds every scene's tree on host threads, uploads the pool, allocates and clears the films.
+ * info (may be NULL) receives pool totals. */
+int pt_batch_build(pt_batch* batch, PtBuildInfo* info);
+int pt_batch_reset_accumulation(pt_batch* batch);
+/* Adds samples 0..spp-1 of every scene's full frame to its film. use_graph != 0 replays the frame's
+ * launch sequence from a HIP graph captured on first use (re-captured after a tuning change). */
+int pt_batch_render(pt_batch* batch, uint64_t frame_index, int use_graph);
+/* Film of one scene, as pt_hip_read_film. */
+int pt_batch_read_film(pt_batch* batch, uint32_t scene, float* rgb_sum, uint32_t* sample_count);
+/* Counters summed over the batch's scenes. */
+int pt_batch_read_counters(pt_batch* batch, PtSampleCounters* counters);
+int pt_batch_set_tuning(pt_batch* batch, const PtTuning* tuning);
+int pt_batch_get_tuning(pt_batch* batch, PtTuning* tuning);
+int pt_batch_set_profiling(pt_batch* batch, int enabled);
+int pt_batch_read_kernel_stats(pt_batch* batch, PtKernelStats* stats);
+int pt_batch_reset_kernel_stats(pt_batch* batch);
+int pt_batch_synchronize(pt_batch* batch);
+
 /* Host-only BVH construction (no device needed): the flattening and split decisions of
  * CpuBvhAccelerator::build (CpuBvhAccelerator.cpp:161-263) with the chosen builder. Call once with
  * NULL outputs to get the counts, then again with buffers of that size:

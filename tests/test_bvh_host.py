@@ -128,3 +128,12 @@ def test_feature_scene_libm_and_portable_math_agree():
     assert np.array_equal(ca, cb)
     err = np.sqrt(((a.astype(np.float64) - b) ** 2).mean() / (b.astype(np.float64) ** 2).mean())
     assert err < 1e-3
+
+
+def test_batch_scene_generator_is_seeded():
+    a = S.batch_scenes(seed=3, count=5, max_tris=20_000)
+    b = S.batch_scenes(seed=3, count=5, max_tris=20_000)
+    for (sa, wa, ha, pa), (sb, wb, hb, pb) in zip(a, b):
+        assert (wa, ha, pa) == (wb, hb, pb) and wa == ha and wa in (128, 256, 512, 1024) and pa in (4, 16, 64)
+        assert np.array_equal(sa.vertices, sb.vertices) and sa.triangle_count == sb.triangle_count
+        assert 500 <= sa.triangle_count <= 20_000

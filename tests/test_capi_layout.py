@@ -15,7 +15,7 @@ HEADER = pathlib.Path(__file__).resolve().parent.parent / "include" / "vkpt_hip.
 def declared_functions():
     text = HEADER.read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(pt_(?:hip|bvh)_\w+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(pt_(?:hip|bvh|batch)_\w+)\s*\(", text)))
 
 
 def test_every_declared_symbol_is_exported_and_bound():

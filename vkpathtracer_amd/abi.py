@@ -122,6 +122,22 @@ SIGNATURES = {
     "pt_bvh_build_host": (C.c_int, [C.POINTER(PtSceneView), C.POINTER(PtBvhParams), _P, _P, _P, _P, _P, _P, _P]),
     "pt_hip_update_scene_delta": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint32]),
     "pt_hip_render_adjoint": (C.c_int, [_P, C.POINTER(PtRenderTile), C.c_uint32, C.c_uint64, _P, _P, _P, _P]),
+    "pt_batch_create": (C.c_int, [C.c_int, C.POINTER(PtRenderSettings), C.POINTER(C.c_void_p)]),
+    "pt_batch_destroy": (None, [_P]),
+    "pt_batch_last_error": (C.c_char_p, [_P]),
+    "pt_batch_add_scene": (C.c_int, [_P, C.POINTER(PtSceneView), C.c_uint32, C.c_uint32, C.c_uint32,
+                                     C.POINTER(PtBvhParams), C.POINTER(C.c_uint32)]),
+    "pt_batch_build": (C.c_int, [_P, C.POINTER(PtBuildInfo)]),
+    "pt_batch_reset_accumulation": (C.c_int, [_P]),
+    "pt_batch_render": (C.c_int, [_P, C.c_uint64, C.c_int]),
+    "pt_batch_read_film": (C.c_int, [_P, C.c_uint32, _P, _P]),
+    "pt_batch_read_counters": (C.c_int, [_P, C.POINTER(PtSampleCounters)]),
+    "pt_batch_set_tuning": (C.c_int, [_P, C.POINTER(PtTuning)]),
+    "pt_batch_get_tuning": (C.c_int, [_P, C.POINTER(PtTuning)]),
+    "pt_batch_set_profiling": (C.c_int, [_P, C.c_int]),
+    "pt_batch_read_kernel_stats": (C.c_int, [_P, C.POINTER(PtKernelStats)]),
+    "pt_batch_reset_kernel_stats": (C.c_int, [_P]),
+    "pt_batch_synchronize": (C.c_int, [_P]),
     "pt_hip_sizeof": (C.c_uint32, [C.c_int]),
     "pt_hip_version": (C.c_char_p, []),
 }

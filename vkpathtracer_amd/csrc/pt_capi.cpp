@@ -1,13 +1,16 @@
-// pt_capi.cpp — the C ABI (include/vkpt_hip.h): IPathTracer state machine, device memory,
-// and the host loop that drives the wavefront kernels.
+// pt_capi.cpp — the C ABI (include/vkpt_hip.h): IPathTracer state machine, the scene pool in
+// device memory, and the host loop that drives the wavefront kernels. A pt_tracer renders one
+// scene; a pt_batch renders many scenes through the same pool and the same launches.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "pt_bvh.h"
@@ -18,7 +21,6 @@ namespace {
 
 constexpr uint32_t kDefaultCapacity = 128u * 1024u * 1024u;  // paths per pass: 64 spp of 1080p in one pass (~20 GB)
 constexpr uint32_t kDefaultRefillMin = 8u;
-constexpr uint32_t kMaxTilesPerPass = 96u;
 
 template <typename T>
 struct DevArray {
@@ -56,32 +58,51 @@ struct ProfiledLaunch {
   hipEvent_t start, stop;
 };
 
-}  // namespace
-
-struct pt_tracer {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  bool configured = false, scene_loaded = false, accel_valid = false;
-  PtRenderSettings settings{};
-  std::string last_error;
-  uint32_t current_sample = 0;
-  uint64_t accumulation_gen = 0;
-
-  // host copy of the snapshot
+// One scene: the host copy of its snapshot, its image, its tree.
+struct SceneSlot {
   std::vector<float> vertices, local_vertices, environment_map;
   std::vector<uint32_t> indices, local_indices;
   std::vector<PtInstance> instances;
   std::vector<PtMaterial> materials;
   std::vector<PtLight> lights;
   PtSceneView view{};
-
+  uint32_t width = 0, height = 0, spp = 0;  // spp: samples per batch frame
+  PtBvhParams bvh_params{PT_BVH_SAH, 4u, 8u, 0u};
   ptb::Bvh bvh;
   ptb::DeviceLayout layout;
   PtBuildInfo info{};
+  std::string error;
+};
+
+// One wavefront pass of a render: tiles [first, first + count) of the plan (grouped by scene,
+// render order kept within a scene), `paths` paths, and its film jobs.
+struct Pass {
+  size_t first = 0, count = 0;
+  uint32_t paths = 0;
+  size_t job_first = 0, job_count = 0;
+  uint32_t job_pixels = 0;
+};
+
+}  // namespace
+
+struct pt_tracer {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool configured = false, scene_loaded = false, accel_valid = false;
+  bool batch = false;
+  PtRenderSettings settings{};
+  std::string last_error;
+  uint32_t current_sample = 0;
+  uint64_t accumulation_gen = 0;
+
+  std::vector<SceneSlot> scenes;  // a tracer uses scenes[0]
+  PtBuildInfo info{};             // the pool's build (a tracer: its scene's)
+  uint32_t film_pixels = 0;
 
   DevArray<float4> d_nodes, d_tris, d_prims, d_film_sum;
   DevArray<PtMaterial> d_materials;
   DevArray<PtLight> d_lights;
+  DevArray<ptd::SceneRec> d_recs;
   DevArray<uint32_t> d_film_count, d_queue, d_error;
   DevArray<unsigned long long> d_counters, d_query_counters;
   DevArray<float> d_env;
@@ -92,10 +113,18 @@ struct pt_tracer {
   DevArray<float> d_stack_t;
   DevArray<float> d_adj_f;
   DevArray<uint32_t> d_adj_u;
+  DevArray<uint8_t> d_plan;     // tile tables and film jobs of the current render
+  DevArray<uint64_t> d_frame;   // frame index read by raygen in graph replays
   uint32_t adj_capacity_alloc = 0;
   uint32_t path_capacity_alloc = 0;
   uint32_t capacity = kDefaultCapacity;
   uint32_t refill_min = kDefaultRefillMin;
+
+  // pinned staging for host->device uploads that must follow the stream's earlier work
+  void* stage = nullptr;
+  size_t stage_bytes = 0;
+  hipEvent_t stage_done = nullptr;
+  bool stage_pending = false;
 
   // Per-path radiance of the last forward render, still in the path arrays when that render was
   // a single pass: an adjoint of the same tiles, frame and scene reuses it instead of replaying
@@ -105,8 +134,11 @@ struct pt_tracer {
   uint64_t fwd_frame = 0;
   uint64_t scene_version = 0, fwd_scene_version = 0;
 
+  // batch frame captured as a HIP graph (launch sequence of every pass)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+
   ptd::DevScene dscene{};
-  ptd::DevCamera dcam{};
   ptd::DevPaths paths{};
   ptd::Grids grids{}, grids_max{};  // launch grids in use, occupancy-derived maxima
 
@@ -114,6 +146,10 @@ struct pt_tracer {
   std::vector<ProfiledLaunch> pending;
   std::vector<hipEvent_t> event_pool;
   PtKernelStats kstats{};
+};
+
+struct pt_batch {
+  pt_tracer t;
 };
 
 namespace {
@@ -192,6 +228,35 @@ int sync_all(pt_tracer* t) {
   return check_device_error(t);
 }
 
+// Copies bytes to device memory in stream order through the pinned staging buffer (the copy runs
+// after the stream's earlier kernels, which may still read dst's previous contents).
+int stage_upload(pt_tracer* t, void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return PT_OK;
+  if (t->stage_pending) {
+    PT_HIP(t, hipEventSynchronize(t->stage_done), "wait for staging buffer");
+    t->stage_pending = false;
+  }
+  if (bytes > t->stage_bytes) {
+    if (t->stage) (void)hipHostFree(t->stage);
+    t->stage = nullptr;
+    t->stage_bytes = 0;
+    PT_HIP(t, hipHostMalloc(&t->stage, bytes, hipHostMallocDefault), "allocate staging buffer");
+    t->stage_bytes = bytes;
+  }
+  std::memcpy(t->stage, src, bytes);
+  PT_HIP(t, hipMemcpyAsync(dst, t->stage, bytes, hipMemcpyHostToDevice, t->stream), "upload");
+  PT_HIP(t, hipEventRecord(t->stage_done, t->stream), "record staging event");
+  t->stage_pending = true;
+  return PT_OK;
+}
+
+void drop_graph(pt_tracer* t) {
+  if (t->graph_exec) (void)hipGraphExecDestroy(t->graph_exec);
+  if (t->graph) (void)hipGraphDestroy(t->graph);
+  t->graph_exec = nullptr;
+  t->graph = nullptr;
+}
+
 // Sums the striped device counters (ptd::kStripes stripes of ptd::kCounterSlots) into out.
 int read_counter_block(pt_tracer* t, const unsigned long long* dev, unsigned long long out[ptd::kCounterSlots]) {
   std::vector<unsigned long long> raw(ptd::kCounterWords);
@@ -208,15 +273,18 @@ int read_counter_block(pt_tracer* t, const unsigned long long* dev, unsigned lon
 }
 
 void free_scene_device(pt_tracer* t) {
+  drop_graph(t);
   t->d_nodes.release();
   t->d_tris.release();
   t->d_prims.release();
   t->d_materials.release();
   t->d_lights.release();
   t->d_env.release();
+  t->d_recs.release();
 }
 
 void free_paths(pt_tracer* t) {
+  drop_graph(t);
   t->fwd_rad_valid = false;
   t->d_path_f.release();
   t->d_path_u.release();
@@ -258,15 +326,339 @@ int ensure_adjoint(pt_tracer* t, uint32_t cap, ptd::DevAdjoint& A) {
   return PT_OK;
 }
 
-int ensure_paths(pt_tracer* t, uint32_t cap);
+// Path-state and queue arrays for `cap` paths: 29 float arrays, 7 u32 arrays, 1 u64 array, each
+// of max(cap, kSegments * seg_cap) entries (queue arrays are indexed by segment position).
+int ensure_paths(pt_tracer* t, uint32_t cap) {
+  if (cap <= t->path_capacity_alloc) return PT_OK;
+  free_paths(t);
+  constexpr size_t kF = 29, kU = 7;
+  const uint32_t seg_cap = ptd::segment_capacity(cap);
+  const size_t n = std::max<size_t>(cap, static_cast<size_t>(ptd::kSegments) * seg_cap);
+  PT_HIP(t, t->d_path_f.ensure(kF * n), "allocate path state");
+  PT_HIP(t, t->d_path_u.ensure(kU * n), "allocate path state");
+  PT_HIP(t, t->d_rng.ensure(n), "allocate path state");
+  float* f = t->d_path_f.ptr;
+  uint32_t* u = t->d_path_u.ptr;
+  ptd::DevPaths& P = t->paths;
+  for (int b = 0; b < 2; ++b) {
+    for (int k = 0; k < 6; ++k) P.rays[b].c[k] = f + (size_t)(6 * b + k) * n;
+    P.rays[b].qpath = u + (size_t)b * n;
+    P.rays[b].qscene = u + (size_t)(4 + b) * n;
+  }
+  P.hit_t = f + 12 * n;
+  P.hit_prim = reinterpret_cast<int32_t*>(u + 2 * n);
+  for (int k = 0; k < 3; ++k) {
+    P.tp[k] = f + (size_t)(13 + k) * n;
+    P.rad[k] = f + (size_t)(16 + k) * n;
+    P.scontrib[k] = f + (size_t)(26 + k) * n;
+  }
+  for (int k = 0; k < 6; ++k) P.sray[k] = f + (size_t)(19 + k) * n;
+  P.stmax = f + 25 * n;
+  P.spath = u + 3 * n;
+  P.sscene = u + (size_t)6 * n;
+  P.rng = t->d_rng.ptr;
+  P.capacity = cap;
+  P.seg_cap = seg_cap;
+  t->path_capacity_alloc = cap;
+  return PT_OK;
+}
 
+bool same_work(const std::vector<ptd::DevTile>& a, const std::vector<ptd::DevTile>& b) {
+  if (a.size() != b.size()) return false;
+  for (size_t i = 0; i < a.size(); ++i)
+    if (a[i].x != b[i].x || a[i].y != b[i].y || a[i].w != b[i].w || a[i].h != b[i].h || a[i].sample != b[i].sample ||
+        a[i].path_begin != b[i].path_begin || a[i].scene != b[i].scene)
+      return false;
+  return true;
+}
+
+// ------------------------------------------------------------------ scenes
+// PathTracerSceneSnapshot copy (ScalarCpuPathTracer.cpp:458-477): an empty material list gets one
+// default material (:469-471); the environment map is used only when it holds width * height
+// texels (PathTracer.cpp:216-219).
+void copy_scene(SceneSlot& s, const PtSceneView* sc) {
+  auto copy = [](auto& dst, const auto* src, size_t n) {
+    if (src && n) dst.assign(src, src + n);
+    else dst.clear();
+  };
+  copy(s.vertices, sc->vertices, static_cast<size_t>(sc->vertex_count) * 3u);
+  copy(s.indices, sc->indices, sc->index_count);
+  copy(s.local_vertices, sc->local_vertices, static_cast<size_t>(sc->local_vertex_count) * 3u);
+  copy(s.local_indices, sc->local_indices, sc->local_index_count);
+  copy(s.instances, sc->instances, sc->instance_count);
+  copy(s.materials, sc->materials, sc->material_count);
+  copy(s.lights, sc->lights, sc->light_count);
+  const size_t env_texels = static_cast<size_t>(sc->environment_map_width) * sc->environment_map_height;
+  copy(s.environment_map, sc->environment_map, env_texels * 3u);
+  if (s.materials.empty()) {
+    PtMaterial m{};
+    m.albedo[0] = m.albedo[1] = m.albedo[2] = 1.0f;
+    m.roughness = 1.0f;
+    m.ior = 1.5f;
+    m.alpha = 1.0f;
+    m.base_color_texture_index = m.normal_texture_index = 0xFFFFFFFFu;
+    s.materials.push_back(m);
+  }
+  s.view = *sc;
+  s.view.vertices = s.vertices.data();
+  s.view.indices = s.indices.data();
+  s.view.local_vertices = s.local_vertices.data();
+  s.view.local_indices = s.local_indices.data();
+  s.view.instances = s.instances.data();
+  s.view.materials = s.materials.data();
+  s.view.lights = s.lights.data();
+  s.view.environment_map = nullptr;
+  s.view.vertex_count = static_cast<uint32_t>(s.vertices.size() / 3u);
+  s.view.index_count = static_cast<uint32_t>(s.indices.size());
+  s.view.local_vertex_count = static_cast<uint32_t>(s.local_vertices.size() / 3u);
+  s.view.local_index_count = static_cast<uint32_t>(s.local_indices.size());
+  s.view.instance_count = static_cast<uint32_t>(s.instances.size());
+  s.view.material_count = static_cast<uint32_t>(s.materials.size());
+  s.view.light_count = static_cast<uint32_t>(s.lights.size());
+  s.layout = ptb::DeviceLayout{};
+  s.bvh = ptb::Bvh{};
+}
+
+int check_bvh_params(pt_tracer* t, const PtBvhParams* params, PtBvhParams& p) {
+  p = PtBvhParams{PT_BVH_SAH, 4u, 8u, 0u};
+  if (params) p = *params;
+  if (p.leaf_size == 0) p.leaf_size = 4u;
+  if (p.bucket_count == 0) p.bucket_count = 8u;
+  if (p.leaf_size > ptb::kMaxLeafSize) return fail(t, PT_INVALID_ARGUMENT, "leaf_size exceeds 31");
+  if (p.builder != PT_BVH_SAH && p.builder != PT_BVH_MIDPOINT) return fail(t, PT_INVALID_ARGUMENT, "unknown builder");
+  return PT_OK;
+}
+
+// Host build of one scene's tree and device layout (thread-safe per slot).
+bool build_scene(SceneSlot& s) {
+  const auto t0 = std::chrono::steady_clock::now();
+  std::vector<float> raw;
+  s.error.clear();
+  if (!ptb::flatten(s.view, s.bvh.prims, raw, s.error)) return false;
+  if (s.bvh_params.builder == PT_BVH_SAH) ptb::build_sah(s.bvh, raw, s.bvh_params.leaf_size, s.bvh_params.bucket_count);
+  else ptb::build_midpoint(s.bvh, raw, s.bvh_params.leaf_size);
+  ptb::pack_device_layout(s.bvh, s.layout);
+  s.info = PtBuildInfo{};
+  s.info.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  s.info.built = 1;
+  s.info.primitive_count = static_cast<uint32_t>(s.bvh.prims.size());
+  s.info.node_count = static_cast<uint32_t>(s.bvh.nodes.size());
+  s.info.leaf_count = s.bvh.leaf_count;
+  s.info.max_depth = s.bvh.max_depth;
+  s.info.deterministic = 1;
+  return true;
+}
+
+// compute_camera_basis, ScalarCpuPathTracer.cpp:139-152; camera_rays constants, Render.cpp:74-83
+ptd::DevCamera make_camera(const PtCamera& c, uint32_t width, uint32_t height, float aperture_default,
+                           float focus_default) {
+  const V3h pos{c.position[0], c.position[1], c.position[2]};
+  const V3h fwd = normalize3(sub3(V3h{c.target[0], c.target[1], c.target[2]}, pos));
+  V3h right = normalize3(cross3(fwd, V3h{c.up[0], c.up[1], c.up[2]}));
+  if (dot3(right, right) <= 1e-4f * 1e-4f) right = {1.0f, 0.0f, 0.0f};
+  V3h up = normalize3(cross3(right, fwd));
+  if (dot3(up, up) <= 1e-4f * 1e-4f) up = {0.0f, 1.0f, 0.0f};
+  ptd::DevCamera C{};
+  const V3h* src[4] = {&pos, &fwd, &right, &up};
+  float* dst[4] = {C.pos, C.fwd, C.right, C.up};
+  for (int i = 0; i < 4; ++i) {
+    dst[i][0] = src[i]->x;
+    dst[i][1] = src[i]->y;
+    dst[i][2] = src[i]->z;
+  }
+  const float kPi = 3.14159265358979323846f;
+  C.aspect = static_cast<float>(width) / std::max(1.0f, static_cast<float>(height));
+  C.tan_half_fov = std::tan(0.5f * (c.fov_deg * (kPi / 180.0f)));
+  C.aperture = c.aperture_radius > 0.0f ? c.aperture_radius : aperture_default;
+  C.focus = c.focus_distance > 0.0f ? c.focus_distance : focus_default;
+  C.iris_blades = std::min(c.iris_blade_count, 64u);
+  C.iris_roundness = std::isfinite(c.iris_roundness) ? std::min(1.0f, std::max(0.0f, c.iris_roundness)) : 0.0f;
+  C.iris_rotation = c.iris_rotation_degrees * (kPi / 180.0f);
+  C.anamorphic = std::isfinite(c.anamorphic_squeeze) ? std::max(0.01f, c.anamorphic_squeeze) : 1.0f;
+  return C;
+}
+
+bool transmissive(const PtMaterial& m) { return m.material_model == 5u || m.transmission > 0.05f; }
+
+int32_t rebase_ref(int32_t ref, uint32_t node_base, uint32_t tri_base) {
+  if (ref >= 0) return ref + static_cast<int32_t>(node_base);
+  const uint32_t code = ~static_cast<uint32_t>(ref);
+  const uint32_t first = (code >> ptb::kLeafCountBits) + tri_base, count = code & ptb::kMaxLeafSize;
+  return ~static_cast<int32_t>((first << ptb::kLeafCountBits) | count);
+}
+
+// Packs every scene's tree, materials, lights and environment into the device pool, rebasing
+// node, triangle, primitive and material references, and writes the scene records. Films are
+// laid out scene after scene (film_base); a tracer's single scene starts at pixel 0.
+int upload_pool(pt_tracer* t) {
+  PT_HIP(t, hipStreamSynchronize(t->stream), "synchronize");
+  free_scene_device(t);
+  size_t n_nodes = 0, n_tris = 0, n_prims = 0, n_mats = 0, n_lights = 0, n_env = 0, film = 0;
+  uint32_t max_depth = 0;
+  for (const SceneSlot& s : t->scenes) {
+    n_nodes += s.layout.nodes.size() / 16u;
+    n_tris += s.layout.tris.size() / 12u;
+    n_prims += s.layout.prims.size() / 4u;
+    n_mats += s.materials.size();
+    n_lights += s.lights.size();
+    n_env += s.environment_map.size();
+    film += static_cast<size_t>(s.width) * s.height;
+    max_depth = std::max(max_depth, s.bvh.max_depth);
+  }
+  if (n_tris >= ptb::kMaxPrimitives || n_nodes >= 0x7fffffffu)
+    return fail(t, PT_UNSUPPORTED, "scene pool exceeds 2^26 triangles");
+  std::vector<float> nodes(16u * n_nodes), tris(12u * n_tris), prims(4u * n_prims), env(n_env);
+  std::vector<PtMaterial> mats;
+  std::vector<PtLight> lights;
+  std::vector<ptd::SceneRec> recs(t->scenes.size());
+  mats.reserve(n_mats);
+  lights.reserve(n_lights);
+  uint32_t node_base = 0, tri_base = 0, prim_base = 0, env_base = 0, film_base = 0;
+  bool general_boxes = false, any_transmissive = false;
+  for (size_t si = 0; si < t->scenes.size(); ++si) {
+    const SceneSlot& s = t->scenes[si];
+    const ptb::DeviceLayout& L = s.layout;
+    const uint32_t nn = static_cast<uint32_t>(L.nodes.size() / 16u), nt = static_cast<uint32_t>(L.tris.size() / 12u),
+                   np = static_cast<uint32_t>(L.prims.size() / 4u);
+    const uint32_t mat_base = static_cast<uint32_t>(mats.size()), light_base = static_cast<uint32_t>(lights.size());
+    for (uint32_t i = 0; i < nn; ++i) {
+      const float* src = &L.nodes[16u * i];
+      float* dst = &nodes[16u * (node_base + i)];
+      std::memcpy(dst, src, 12 * sizeof(float));
+      for (int k = 0; k < 12; ++k)
+        if (!std::isfinite(src[k])) general_boxes = true;
+      int32_t r[2];
+      std::memcpy(r, src + 12, sizeof(r));
+      for (int k = 0; k < 2; ++k) r[k] = rebase_ref(r[k], node_base, tri_base);
+      std::memcpy(dst + 12, r, sizeof(r));
+      std::memset(dst + 14, 0, 2 * sizeof(float));
+    }
+    for (uint32_t i = 0; i < nt; ++i) {
+      const float* src = &L.tris[12u * i];
+      float* dst = &tris[12u * (tri_base + i)];
+      std::memcpy(dst, src, 12 * sizeof(float));
+      uint32_t pid;
+      std::memcpy(&pid, src + 3, 4);
+      pid += prim_base;
+      std::memcpy(dst + 3, &pid, 4);
+    }
+    const uint32_t last_mat = static_cast<uint32_t>(s.materials.size()) - 1u;
+    for (uint32_t i = 0; i < np; ++i) {
+      const float* src = &L.prims[4u * i];
+      float* dst = &prims[4u * (prim_base + i)];
+      std::memcpy(dst, src, 3 * sizeof(float));
+      uint32_t m;
+      std::memcpy(&m, src + 3, 4);
+      m = std::min(m, last_mat) + mat_base;  // the shading clamp of a scene-local index, then the pool base
+      std::memcpy(dst + 3, &m, 4);
+    }
+    mats.insert(mats.end(), s.materials.begin(), s.materials.end());
+    lights.insert(lights.end(), s.lights.begin(), s.lights.end());
+    std::copy(s.environment_map.begin(), s.environment_map.end(), env.begin() + env_base);
+
+    ptd::SceneRec& R = recs[si];
+    R.root_ref = np ? rebase_ref(L.root_ref, node_base, tri_base) : 0;
+    R.prim_count = np;
+    for (int k = 0; k < 6; ++k) R.root_box[k] = L.root_box[k];
+    R.light_base = light_base;
+    R.light_count = static_cast<uint32_t>(s.lights.size());
+    R.has_transmissive = 0;
+    for (const PtMaterial& m : s.materials)
+      if (transmissive(m)) R.has_transmissive = 1;
+    any_transmissive = any_transmissive || R.has_transmissive;
+    R.env_base = env_base;
+    const bool has_map = !s.environment_map.empty();
+    R.env_w = has_map ? s.view.environment_map_width : 0u;
+    R.env_h = has_map ? s.view.environment_map_height : 0u;
+    for (int k = 0; k < 3; ++k) {
+      R.env[k] = s.view.environment_color[k];
+      R.env_scale[k] = s.view.environment_map_scale[k];
+    }
+    R.width = s.width;
+    R.height = s.height;
+    R.film_base = film_base;
+    R.cam = make_camera(s.view.camera, s.width, s.height, t->settings.camera_aperture_radius,
+                        t->settings.camera_focus_distance);
+    node_base += nn;
+    tri_base += nt;
+    prim_base += np;
+    env_base += static_cast<uint32_t>(s.environment_map.size());
+    film_base += s.width * s.height;
+  }
+
+  uint64_t bytes = 0;
+  auto upload = [&](auto& arr, const void* src, size_t count, size_t elem) -> hipError_t {
+    hipError_t e = arr.ensure(count);
+    if (e != hipSuccess) return e;
+    bytes += count * elem;
+    if (count) e = hipMemcpy(arr.ptr, src, count * elem, hipMemcpyHostToDevice);
+    return e;
+  };
+  PT_HIP(t, upload(t->d_nodes, nodes.data(), n_nodes * 4u, sizeof(float4)), "upload BVH nodes");
+  PT_HIP(t, upload(t->d_tris, tris.data(), n_tris * 3u, sizeof(float4)), "upload triangles");
+  PT_HIP(t, upload(t->d_prims, prims.data(), n_prims, sizeof(float4)), "upload primitives");
+  PT_HIP(t, upload(t->d_materials, mats.data(), mats.size(), sizeof(PtMaterial)), "upload materials");
+  PT_HIP(t, upload(t->d_lights, lights.data(), lights.size(), sizeof(PtLight)), "upload lights");
+  PT_HIP(t, upload(t->d_env, env.data(), env.size(), sizeof(float)), "upload environment");
+  PT_HIP(t, upload(t->d_recs, recs.data(), recs.size(), sizeof(ptd::SceneRec)), "upload scene records");
+
+  // Traversal stack spill area: a lane's stack never holds more than max_depth entries (one per
+  // level of the current path); entries older than the LDS ring's kStackLds go here.
+  const uint32_t need = max_depth + 2u;
+  const uint32_t threads =
+      std::max({t->grids_max.extend, t->grids_max.shadow, t->grids_max.shadow_adj, t->grids_max.query}) *
+      static_cast<uint32_t>(ptd::kBlock);
+  PT_HIP(t, t->d_stack_ref.ensure(static_cast<size_t>(need) * threads), "allocate traversal stack");
+  PT_HIP(t, t->d_stack_t.ensure(static_cast<size_t>(need) * threads), "allocate traversal stack");
+  bytes += static_cast<uint64_t>(need) * threads * 8u;
+  t->paths.stack_ref = t->d_stack_ref.ptr;
+  t->paths.stack_t = t->d_stack_t.ptr;
+  t->paths.stack_overflow_depth = need;
+  t->paths.stack_threads = threads;
+
+  ptd::DevScene& S = t->dscene;
+  S = ptd::DevScene{};
+  S.nodes = t->d_nodes.ptr;
+  S.tris = t->d_tris.ptr;
+  S.prims = t->d_prims.ptr;
+  S.materials = t->d_materials.ptr;
+  S.lights = t->d_lights.ptr;
+  S.material_count = static_cast<uint32_t>(mats.size());
+  S.light_count = static_cast<uint32_t>(lights.size());
+  S.has_transmissive = any_transmissive ? 1u : 0u;
+  S.general_boxes = general_boxes ? 1u : 0u;
+  S.env_map = t->d_env.ptr;
+  S.recs = t->d_recs.ptr;
+  S.scene_count = static_cast<uint32_t>(recs.size());
+
+  t->info = PtBuildInfo{};
+  t->info.device_bytes = bytes;
+  t->info.built = 1;
+  t->info.deterministic = 1;
+  for (const SceneSlot& s : t->scenes) {
+    t->info.build_ms += s.info.build_ms;
+    t->info.primitive_count += s.info.primitive_count;
+    t->info.node_count += s.info.node_count;
+    t->info.leaf_count += s.info.leaf_count;
+    t->info.max_depth = std::max(t->info.max_depth, s.info.max_depth);
+  }
+  t->film_pixels = film_base;
+  return PT_OK;
+}
+
+int alloc_film(pt_tracer* t, size_t pixels) {
+  PT_HIP(t, t->d_film_sum.ensure(pixels), "allocate film");
+  PT_HIP(t, t->d_film_count.ensure(pixels), "allocate film");
+  ptd::launch_clear_film(t->stream, ptd::DevFilm{t->d_film_sum.ptr, t->d_film_count.ptr}, static_cast<uint32_t>(pixels));
+  return PT_OK;
+}
+
+// ------------------------------------------------------------------ rendering
 // Clamp tiles like render_tile_cancellable (ScalarCpuPathTracer.Render.cpp:172-180) and split
 // any tile larger than one pass into row bands.
-int build_work(pt_tracer* t, const PtRenderTile* tiles, uint32_t count, std::vector<ptd::DevTile>& work) {
-  const uint32_t W = t->settings.width, H = t->settings.height;
+int build_work(pt_tracer* t, uint32_t scene, const PtRenderTile* tiles, uint32_t count, std::vector<ptd::DevTile>& work) {
+  const uint32_t W = t->scenes[scene].width, H = t->scenes[scene].height;
   const uint32_t cap = std::max<uint32_t>(t->capacity, 1024u);
-  work.clear();
-  work.reserve(count);
   for (uint32_t i = 0; i < count; ++i) {
     const PtRenderTile& tl = tiles[i];
     const uint32_t x0 = std::min(tl.x, W), y0 = std::min(tl.y, H);
@@ -278,7 +670,7 @@ int build_work(pt_tracer* t, const PtRenderTile* tiles, uint32_t count, std::vec
     const uint32_t rows_per = std::max(1u, cap / w);
     for (uint32_t y = y0; y < y1; y += rows_per) {
       const uint32_t h = std::min(rows_per, y1 - y);
-      work.push_back(ptd::DevTile{x0, y, w, h, tl.sample_index, 0u});
+      work.push_back(ptd::DevTile{x0, y, w, h, tl.sample_index, 0u, scene, 0u});
     }
   }
   return PT_OK;
@@ -288,8 +680,7 @@ ptd::DevSettings dev_settings(const pt_tracer* t, uint64_t frame_index) {
   ptd::DevSettings st{};
   st.seed = t->settings.seed;
   st.frame = frame_index;
-  st.width = t->settings.width;
-  st.height = t->settings.height;
+  st.frame_dev = nullptr;
   st.max_depth = t->settings.max_depth;
   st.enable_nee = t->settings.enable_nee;
   st.enable_mis = t->settings.enable_mis;
@@ -297,6 +688,76 @@ ptd::DevSettings dev_settings(const pt_tracer* t, uint64_t frame_index) {
   st.rr_min = t->settings.rr_min_survival;
   st.rr_max = t->settings.rr_max_survival;
   return st;
+}
+
+// Splits the work into passes of at most `capacity` paths (tiles in work order; inside a pass
+// the tiles are grouped by scene, keeping their order within a scene), numbers each pass's paths,
+// builds the film jobs and uploads tiles and jobs.
+int plan_passes(pt_tracer* t, std::vector<ptd::DevTile>& work, std::vector<Pass>& passes,
+                std::vector<ptd::FilmJob>& jobs) {
+  const uint32_t cap = std::max<uint32_t>(t->capacity, 1024u);
+  passes.clear();
+  jobs.clear();
+  size_t next = 0;
+  while (next < work.size()) {
+    Pass ps;
+    ps.first = next;
+    uint64_t paths = 0;
+    size_t end = next;
+    while (end < work.size()) {
+      const uint64_t n = static_cast<uint64_t>(work[end].w) * work[end].h;
+      if (end > next && paths + n > cap) break;
+      paths += n;
+      ++end;
+    }
+    std::stable_sort(work.begin() + static_cast<std::ptrdiff_t>(next), work.begin() + static_cast<std::ptrdiff_t>(end),
+                     [](const ptd::DevTile& a, const ptd::DevTile& b) { return a.scene < b.scene; });
+    uint32_t pb = 0;
+    for (size_t i = next; i < end; ++i) {
+      work[i].path_begin = pb;
+      pb += work[i].w * work[i].h;
+    }
+    ps.count = end - next;
+    ps.paths = pb;
+    ps.job_first = jobs.size();
+    uint32_t pix = 0;
+    for (size_t i = next; i < end;) {
+      size_t j = i;
+      ptd::FilmJob J{};
+      J.scene = work[i].scene;
+      uint32_t x0 = work[i].x, y0 = work[i].y, x1 = work[i].x + work[i].w, y1 = work[i].y + work[i].h;
+      while (j < end && work[j].scene == J.scene) {
+        x0 = std::min(x0, work[j].x);
+        y0 = std::min(y0, work[j].y);
+        x1 = std::max(x1, work[j].x + work[j].w);
+        y1 = std::max(y1, work[j].y + work[j].h);
+        ++j;
+      }
+      J.x0 = x0;
+      J.y0 = y0;
+      J.w = x1 - x0;
+      J.h = y1 - y0;
+      J.first_tile = static_cast<uint32_t>(i - next);
+      J.tile_count = static_cast<uint32_t>(j - i);
+      J.pixel_begin = pix;
+      pix += J.w * J.h;
+      jobs.push_back(J);
+      i = j;
+    }
+    ps.job_count = jobs.size() - ps.job_first;
+    ps.job_pixels = pix;
+    passes.push_back(ps);
+    next = end;
+  }
+  const size_t tile_bytes = work.size() * sizeof(ptd::DevTile), job_bytes = jobs.size() * sizeof(ptd::FilmJob);
+  std::vector<uint8_t> blob(tile_bytes + job_bytes);
+  if (tile_bytes) std::memcpy(blob.data(), work.data(), tile_bytes);
+  if (job_bytes) std::memcpy(blob.data() + tile_bytes, jobs.data(), job_bytes);
+  if (blob.size() > t->d_plan.n) {
+    PT_HIP(t, hipStreamSynchronize(t->stream), "synchronize");  // the old table may still be read
+    PT_HIP(t, t->d_plan.ensure(blob.size() + blob.size() / 2), "allocate tile table");
+  }
+  return stage_upload(t, t->d_plan.ptr, blob.data(), blob.size());
 }
 
 // The bounce loop of one wavefront pass (extend -> shade -> shadow -> advance per depth).
@@ -318,72 +779,56 @@ void bounce_loop(pt_tracer* t, const ptd::DevPaths& P, const ptd::DevSettings& s
   }
 }
 
-bool same_work(const std::vector<ptd::DevTile>& a, const std::vector<ptd::DevTile>& b) {
-  if (a.size() != b.size()) return false;
-  for (size_t i = 0; i < a.size(); ++i)
-    if (a[i].x != b[i].x || a[i].y != b[i].y || a[i].w != b[i].w || a[i].h != b[i].h || a[i].sample != b[i].sample ||
-        a[i].path_begin != b[i].path_begin)
-      return false;
-  return true;
+// Allocates what the passes need (outside any graph capture).
+int prepare_passes(pt_tracer* t, const std::vector<Pass>& passes, ptd::DevAdjoint* adj) {
+  const uint32_t cap = std::max<uint32_t>(t->capacity, 1024u);
+  uint32_t most = 0;
+  for (const Pass& p : passes) most = std::max(most, p.paths);
+  int rc = ensure_paths(t, std::max(most, std::min(cap, 1u << 20)));
+  if (rc != PT_OK) return rc;
+  if (adj) {
+    rc = ensure_adjoint(t, t->path_capacity_alloc, *adj);
+    if (rc != PT_OK) return rc;
+  }
+  return PT_OK;
 }
 
-// Runs the wavefront over `work` in passes of at most `capacity` paths.
-// adj == nullptr: forward passes whose radiance is accumulated into the film in tile order.
+// Launches every pass of a planned render (prepare_passes done).
+// adj == nullptr: forward passes whose radiance is accumulated into the films in tile order.
 // adj != nullptr: per pass, a forward replay (radiance kept per path) then the adjoint replay;
 //                 the film and the render counters are left untouched. The forward replay is
 //                 skipped when the previous call was the single-pass forward render of the same
 //                 tiles, frame and scene (the autograd forward-then-backward pattern).
-int run_passes(pt_tracer* t, std::vector<ptd::DevTile>& work, const ptd::DevSettings& st, ptd::DevAdjoint* adj) {
-  const uint32_t cap = std::max<uint32_t>(t->capacity, 1024u);
+int launch_passes(pt_tracer* t, const std::vector<ptd::DevTile>& work, const std::vector<Pass>& passes,
+                  const ptd::DevSettings& st, ptd::DevAdjoint* adj) {
   const ptd::DevFilm film{t->d_film_sum.ptr, t->d_film_count.ptr};
-  static_assert(kMaxTilesPerPass == ptd::kMaxTilesPerPass, "tile table size");
-  size_t next = 0;
-  while (next < work.size()) {
-    size_t end = next;
-    uint32_t paths = 0;
-    while (end < work.size() && end - next < kMaxTilesPerPass) {
-      const uint32_t n = work[end].w * work[end].h;
-      if (end > next && static_cast<uint64_t>(paths) + n > cap) break;
-      work[end].path_begin = paths;
-      paths += n;
-      ++end;
-    }
-    const bool single_pass = next == 0 && end == work.size();
-    int rc = ensure_paths(t, std::max(paths, std::min(cap, 1u << 20)));
-    if (rc != PT_OK) return rc;
-    // the last forward render left exactly these paths' radiance in P.rad
+  const ptd::DevTile* dtiles = reinterpret_cast<const ptd::DevTile*>(t->d_plan.ptr);
+  const ptd::FilmJob* djobs = reinterpret_cast<const ptd::FilmJob*>(t->d_plan.ptr + work.size() * sizeof(ptd::DevTile));
+  for (const Pass& ps : passes) {
+    const bool single_pass = passes.size() == 1;
     const bool reuse = adj && single_pass && t->fwd_rad_valid && st.frame == t->fwd_frame &&
                        t->scene_version == t->fwd_scene_version && same_work(work, t->fwd_work);
-    ptd::DevTileTable table{};
-    table.count = static_cast<uint32_t>(end - next);
-    for (size_t i = next; i < end; ++i) table.t[i - next] = work[i];
     ptd::DevPaths P = t->paths;
-    if (adj) {
-      rc = ensure_adjoint(t, t->path_capacity_alloc, *adj);
-      if (rc != PT_OK) return rc;
-      P.counters = t->d_query_counters.ptr;
-    }
+    if (adj) P.counters = t->d_query_counters.ptr;
+    const ptd::DevTile* tl = dtiles + ps.first;
+    const uint32_t nt = static_cast<uint32_t>(ps.count);
     if (!reuse) {
-      ptd::launch_init_queue(t->stream, P, paths);
-      timed(t, PT_KERNEL_RAYGEN, paths, [&] { ptd::launch_raygen(t->stream, P, t->dcam, st, table, paths); });
+      ptd::launch_init_queue(t->stream, P, ps.paths);
+      timed(t, PT_KERNEL_RAYGEN, ps.paths, [&] { ptd::launch_raygen(t->stream, P, t->dscene, st, tl, nt, ps.paths); });
       bounce_loop(t, P, st, nullptr);
     }
     if (adj) {
       t->fwd_rad_valid = false;  // the replay below overwrites P.rad
-      ptd::launch_copy_radiance(t->stream, P, *adj, paths);
-      ptd::launch_init_queue(t->stream, P, paths);
-      timed(t, PT_KERNEL_RAYGEN, paths,
-            [&] { ptd::launch_raygen_adjoint(t->stream, P, *adj, t->dcam, st, table, paths); });
+      ptd::launch_copy_radiance(t->stream, P, *adj, ps.paths);
+      ptd::launch_init_queue(t->stream, P, ps.paths);
+      timed(t, PT_KERNEL_RAYGEN, ps.paths,
+            [&] { ptd::launch_raygen_adjoint(t->stream, P, *adj, t->dscene, st, tl, nt, ps.paths); });
       bounce_loop(t, P, st, adj);
     } else {
-      uint32_t rect[4] = {work[next].x, work[next].y, work[next].x + work[next].w, work[next].y + work[next].h};
-      for (size_t i = next + 1; i < end; ++i) {
-        rect[0] = std::min(rect[0], work[i].x);
-        rect[1] = std::min(rect[1], work[i].y);
-        rect[2] = std::max(rect[2], work[i].x + work[i].w);
-        rect[3] = std::max(rect[3], work[i].y + work[i].h);
-      }
-      timed(t, PT_KERNEL_FILM, paths, [&] { ptd::launch_film(t->stream, P, film, st.width, table, rect); });
+      timed(t, PT_KERNEL_FILM, ps.paths, [&] {
+        ptd::launch_film(t->stream, P, t->dscene, film, tl, djobs + ps.job_first, static_cast<uint32_t>(ps.job_count),
+                         ps.job_pixels);
+      });
       t->fwd_rad_valid = single_pass;
       if (single_pass) {
         t->fwd_work = work;
@@ -393,51 +838,102 @@ int run_passes(pt_tracer* t, std::vector<ptd::DevTile>& work, const ptd::DevSett
     }
     const hipError_t le = hipGetLastError();
     if (le != hipSuccess) return hip_fail(t, le, "kernel launch");
-    next = end;
   }
   return PT_OK;
 }
 
-// Path-state and queue arrays for `cap` paths: 29 float arrays, 4 u32 arrays, 1 u64 array, each
-// of max(cap, kSegments * seg_cap) entries (queue arrays are indexed by segment position).
-int ensure_paths(pt_tracer* t, uint32_t cap) {
-  if (cap <= t->path_capacity_alloc) return PT_OK;
-  free_paths(t);
-  constexpr size_t kF = 29, kU = 4;
-  const uint32_t seg_cap = ptd::segment_capacity(cap);
-  const size_t n = std::max<size_t>(cap, static_cast<size_t>(ptd::kSegments) * seg_cap);
-  PT_HIP(t, t->d_path_f.ensure(kF * n), "allocate path state");
-  PT_HIP(t, t->d_path_u.ensure(kU * n), "allocate path state");
-  PT_HIP(t, t->d_rng.ensure(n), "allocate path state");
-  float* f = t->d_path_f.ptr;
-  uint32_t* u = t->d_path_u.ptr;
-  ptd::DevPaths& P = t->paths;
-  for (int b = 0; b < 2; ++b) {
-    for (int k = 0; k < 6; ++k) P.rays[b].c[k] = f + (size_t)(6 * b + k) * n;
-    P.rays[b].qpath = u + (size_t)b * n;
-  }
-  P.hit_t = f + 12 * n;
-  P.hit_prim = reinterpret_cast<int32_t*>(u + 2 * n);
-  for (int k = 0; k < 3; ++k) {
-    P.tp[k] = f + (size_t)(13 + k) * n;
-    P.rad[k] = f + (size_t)(16 + k) * n;
-    P.scontrib[k] = f + (size_t)(26 + k) * n;
-  }
-  for (int k = 0; k < 6; ++k) P.sray[k] = f + (size_t)(19 + k) * n;
-  P.stmax = f + 25 * n;
-  P.spath = u + 3 * n;
-  P.rng = t->d_rng.ptr;
-  P.capacity = cap;
-  P.seg_cap = seg_cap;
-  t->path_capacity_alloc = cap;
+int run_passes(pt_tracer* t, std::vector<ptd::DevTile>& work, const ptd::DevSettings& st, ptd::DevAdjoint* adj) {
+  std::vector<Pass> passes;
+  std::vector<ptd::FilmJob> jobs;
+  int rc = plan_passes(t, work, passes, jobs);
+  if (rc != PT_OK) return rc;
+  rc = prepare_passes(t, passes, adj);
+  if (rc != PT_OK) return rc;
+  return launch_passes(t, work, passes, st, adj);
+}
+
+int init_tracer(pt_tracer* t, int device) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return PT_UNSUPPORTED;
+  if (device < 0 || device >= count) return PT_INVALID_ARGUMENT;
+  if (hipSetDevice(device) != hipSuccess) return PT_INTERNAL_ERROR;
+  t->device = device;
+  if (hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking) != hipSuccess) return PT_INTERNAL_ERROR;
+  if (hipEventCreateWithFlags(&t->stage_done, hipEventDisableTiming) != hipSuccess) return PT_INTERNAL_ERROR;
+  if (t->d_counters.ensure(ptd::kCounterWords) != hipSuccess || t->d_query_counters.ensure(ptd::kCounterWords) != hipSuccess ||
+      t->d_queue.ensure(ptd::kQueueWords) != hipSuccess || t->d_error.ensure(1) != hipSuccess ||
+      t->d_frame.ensure(1) != hipSuccess)
+    return PT_ALLOC_FAILED;
+  (void)hipMemsetAsync(t->d_counters.ptr, 0, sizeof(unsigned long long) * ptd::kCounterWords, t->stream);
+  (void)hipMemsetAsync(t->d_queue.ptr, 0, sizeof(uint32_t) * ptd::kQueueWords, t->stream);
+  (void)hipMemsetAsync(t->d_error.ptr, 0, sizeof(uint32_t), t->stream);
+  t->paths.queue = t->d_queue.ptr;
+  t->paths.counters = t->d_counters.ptr;
+  t->paths.error = t->d_error.ptr;
+  t->paths.refill_min = t->refill_min;
+  t->grids = t->grids_max = ptd::resident_grids(device);
   return PT_OK;
+}
+
+void destroy_tracer(pt_tracer* t) {
+  (void)hipSetDevice(t->device);
+  if (t->stream) (void)hipStreamSynchronize(t->stream);
+  collect_profile(t);
+  for (hipEvent_t e : t->event_pool) (void)hipEventDestroy(e);
+  free_scene_device(t);
+  free_paths(t);
+  t->d_film_sum.release();
+  t->d_film_count.release();
+  t->d_queue.release();
+  t->d_error.release();
+  t->d_counters.release();
+  t->d_query_counters.release();
+  t->d_stack_ref.release();
+  t->d_stack_t.release();
+  t->d_plan.release();
+  t->d_frame.release();
+  if (t->stage) (void)hipHostFree(t->stage);
+  if (t->stage_done) (void)hipEventDestroy(t->stage_done);
+  if (t->stream) (void)hipStreamDestroy(t->stream);
+}
+
+int read_film_range(pt_tracer* t, size_t base, size_t pixels, float* rgb_sum, uint32_t* sample_count) {
+  std::vector<float4> sum(rgb_sum ? pixels : 0);
+  if (rgb_sum && pixels)
+    PT_HIP(t, hipMemcpyAsync(sum.data(), t->d_film_sum.ptr + base, pixels * sizeof(float4), hipMemcpyDeviceToHost,
+                             t->stream),
+           "read film");
+  if (sample_count && pixels)
+    PT_HIP(t, hipMemcpyAsync(sample_count, t->d_film_count.ptr + base, pixels * sizeof(uint32_t),
+                             hipMemcpyDeviceToHost, t->stream),
+           "read film");
+  const int rc = sync_all(t);
+  if (rc != PT_OK) return rc;
+  if (rgb_sum)
+    for (size_t i = 0; i < pixels; ++i) {
+      rgb_sum[3 * i] = sum[i].x;
+      rgb_sum[3 * i + 1] = sum[i].y;
+      rgb_sum[3 * i + 2] = sum[i].z;
+    }
+  return PT_OK;
+}
+
+void to_counters(const unsigned long long* c, PtSampleCounters* out) {
+  *out = PtSampleCounters{};
+  out->samples = c[ptd::kSamples];
+  out->rays = c[ptd::kRays];
+  out->triangle_tests = c[ptd::kTriTests];
+  out->triangle_hits = c[ptd::kTriHits];
+  out->bvh_node_visits = c[ptd::kNodeVisits];
+  out->bvh_leaf_visits = c[ptd::kLeafVisits];
+  out->shadow_tests = c[ptd::kShadowTests];
 }
 
 }  // namespace
 
 extern "C" {
 
-const char* pt_hip_version(void) { return "vkpathtracer_amd 0.1 (gfx950 wavefront)"; }
+const char* pt_hip_version(void) { return "vkpathtracer_amd 0.2 (gfx950 wavefront, scene pools)"; }
 
 uint32_t pt_hip_sizeof(int which) {
   switch (which) {
@@ -457,52 +953,21 @@ uint32_t pt_hip_sizeof(int which) {
 int pt_hip_create(int device, pt_tracer** out) {
   if (!out) return PT_INVALID_ARGUMENT;
   *out = nullptr;
-  int count = 0;
-  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return PT_UNSUPPORTED;
-  if (device < 0 || device >= count) return PT_INVALID_ARGUMENT;
-  if (hipSetDevice(device) != hipSuccess) return PT_INTERNAL_ERROR;
   pt_tracer* t = new (std::nothrow) pt_tracer();
   if (!t) return PT_ALLOC_FAILED;
-  t->device = device;
-  if (hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking) != hipSuccess) {
+  const int rc = init_tracer(t, device);
+  if (rc != PT_OK) {
+    destroy_tracer(t);
     delete t;
-    return PT_INTERNAL_ERROR;
+    return rc;
   }
-  if (t->d_counters.ensure(ptd::kCounterWords) != hipSuccess || t->d_query_counters.ensure(ptd::kCounterWords) != hipSuccess ||
-      t->d_queue.ensure(ptd::kQueueWords) != hipSuccess || t->d_error.ensure(1) != hipSuccess) {
-    (void)hipStreamDestroy(t->stream);
-    delete t;
-    return PT_ALLOC_FAILED;
-  }
-  (void)hipMemsetAsync(t->d_counters.ptr, 0, sizeof(unsigned long long) * ptd::kCounterWords, t->stream);
-  (void)hipMemsetAsync(t->d_queue.ptr, 0, sizeof(uint32_t) * ptd::kQueueWords, t->stream);
-  t->paths.queue = t->d_queue.ptr;
-  t->paths.counters = t->d_counters.ptr;
-  t->paths.error = t->d_error.ptr;
-  t->paths.refill_min = t->refill_min;
-  (void)hipMemsetAsync(t->d_error.ptr, 0, sizeof(uint32_t), t->stream);
-  t->grids = t->grids_max = ptd::resident_grids(device);
   *out = t;
   return PT_OK;
 }
 
 void pt_hip_destroy(pt_tracer* t) {
   if (!t) return;
-  (void)hipSetDevice(t->device);
-  (void)hipStreamSynchronize(t->stream);
-  collect_profile(t);
-  for (hipEvent_t e : t->event_pool) (void)hipEventDestroy(e);
-  free_scene_device(t);
-  free_paths(t);
-  t->d_film_sum.release();
-  t->d_film_count.release();
-  t->d_queue.release();
-  t->d_error.release();
-  t->d_counters.release();
-  t->d_query_counters.release();
-  t->d_stack_ref.release();
-  t->d_stack_t.release();
-  (void)hipStreamDestroy(t->stream);
+  destroy_tracer(t);
   delete t;
 }
 
@@ -510,6 +975,7 @@ const char* pt_hip_last_error(const pt_tracer* t) { return t ? t->last_error.c_s
 
 int pt_hip_configure(pt_tracer* t, const PtRenderSettings* s) {
   if (!t || !s) return fail(t, PT_INVALID_ARGUMENT, "configure: null argument");
+  if (t->batch) return fail(t, PT_INVALID_ARGUMENT, "configure: use the pt_batch entry points on a batch");
   (void)hipSetDevice(t->device);
   PT_HIP(t, hipStreamSynchronize(t->stream), "synchronize");
   if (!(s->rr_min_survival == s->rr_min_survival) || !(s->rr_max_survival == s->rr_max_survival))
@@ -518,15 +984,14 @@ int pt_hip_configure(pt_tracer* t, const PtRenderSettings* s) {
   t->settings.max_depth = std::max(1u, s->max_depth);  // MakePathTraceSettings, PathTracer.cpp:283
   t->settings.camera_aperture_radius = std::max(0.0f, s->camera_aperture_radius);
   t->settings.camera_focus_distance = std::max(0.0f, s->camera_focus_distance);
-  const size_t pixels = static_cast<size_t>(s->width) * s->height;
-  PT_HIP(t, t->d_film_sum.ensure(pixels), "allocate film");
-  PT_HIP(t, t->d_film_count.ensure(pixels), "allocate film");
-  ptd::launch_clear_film(t->stream, ptd::DevFilm{t->d_film_sum.ptr, t->d_film_count.ptr}, static_cast<uint32_t>(pixels));
+  const int rc = alloc_film(t, static_cast<size_t>(s->width) * s->height);
+  if (rc != PT_OK) return rc;
   ptd::launch_zero_counters(t->stream, t->d_counters.ptr);
   t->configured = true;
   ++t->scene_version;
   t->scene_loaded = false;
   t->accel_valid = false;
+  t->scenes.clear();
   t->current_sample = 0;
   t->accumulation_gen = 0;
   t->last_error.clear();
@@ -536,45 +1001,11 @@ int pt_hip_configure(pt_tracer* t, const PtRenderSettings* s) {
 int pt_hip_load_scene(pt_tracer* t, const PtSceneView* sc) {
   if (!t || !sc) return fail(t, PT_INVALID_ARGUMENT, "load_scene: null argument");
   if (!t->configured) return fail(t, PT_NOT_READY, "load_scene_snapshot called before configure");
-  auto copy = [](auto& dst, const auto* src, size_t n) {
-    if (src && n) dst.assign(src, src + n);
-    else dst.clear();
-  };
-  copy(t->vertices, sc->vertices, static_cast<size_t>(sc->vertex_count) * 3u);
-  copy(t->indices, sc->indices, sc->index_count);
-  copy(t->local_vertices, sc->local_vertices, static_cast<size_t>(sc->local_vertex_count) * 3u);
-  copy(t->local_indices, sc->local_indices, sc->local_index_count);
-  copy(t->instances, sc->instances, sc->instance_count);
-  copy(t->materials, sc->materials, sc->material_count);
-  copy(t->lights, sc->lights, sc->light_count);
-  // SampleSceneEnvironment uses the map only when it holds width * height texels (PathTracer.cpp:216-219)
-  const size_t env_texels = static_cast<size_t>(sc->environment_map_width) * sc->environment_map_height;
-  copy(t->environment_map, sc->environment_map, env_texels * 3u);
-  if (t->materials.empty()) {  // ScalarCpuPathTracer.cpp:469-471
-    PtMaterial m{};
-    m.albedo[0] = m.albedo[1] = m.albedo[2] = 1.0f;
-    m.roughness = 1.0f;
-    m.ior = 1.5f;
-    m.alpha = 1.0f;
-    m.base_color_texture_index = m.normal_texture_index = 0xFFFFFFFFu;
-    t->materials.push_back(m);
-  }
-  t->view = *sc;
-  t->view.vertices = t->vertices.data();
-  t->view.indices = t->indices.data();
-  t->view.local_vertices = t->local_vertices.data();
-  t->view.local_indices = t->local_indices.data();
-  t->view.instances = t->instances.data();
-  t->view.materials = t->materials.data();
-  t->view.lights = t->lights.data();
-  t->view.environment_map = nullptr;
-  t->view.vertex_count = static_cast<uint32_t>(t->vertices.size() / 3u);
-  t->view.index_count = static_cast<uint32_t>(t->indices.size());
-  t->view.local_vertex_count = static_cast<uint32_t>(t->local_vertices.size() / 3u);
-  t->view.local_index_count = static_cast<uint32_t>(t->local_indices.size());
-  t->view.instance_count = static_cast<uint32_t>(t->instances.size());
-  t->view.material_count = static_cast<uint32_t>(t->materials.size());
-  t->view.light_count = static_cast<uint32_t>(t->lights.size());
+  t->scenes.resize(1);
+  SceneSlot& s = t->scenes[0];
+  copy_scene(s, sc);
+  s.width = t->settings.width;
+  s.height = t->settings.height;
   t->scene_loaded = true;
   t->accel_valid = false;
   ++t->scene_version;
@@ -586,119 +1017,18 @@ int pt_hip_build_accel(pt_tracer* t, const PtBvhParams* params, PtBuildInfo* inf
   if (!t) return PT_INVALID_ARGUMENT;
   if (!t->scene_loaded) return fail(t, PT_NOT_READY, "build_or_update_acceleration called before load_scene_snapshot");
   (void)hipSetDevice(t->device);
-  PtBvhParams p{PT_BVH_SAH, 4u, 8u, 0u};
-  if (params) p = *params;
-  if (p.leaf_size == 0) p.leaf_size = 4u;
-  if (p.bucket_count == 0) p.bucket_count = 8u;
-  if (p.leaf_size > ptb::kMaxLeafSize) return fail(t, PT_INVALID_ARGUMENT, "leaf_size exceeds 31");
-  if (p.builder != PT_BVH_SAH && p.builder != PT_BVH_MIDPOINT) return fail(t, PT_INVALID_ARGUMENT, "unknown builder");
+  PtBvhParams p;
+  int rc = check_bvh_params(t, params, p);
+  if (rc != PT_OK) return rc;
   PT_HIP(t, hipStreamSynchronize(t->stream), "synchronize");
   t->accel_valid = false;
+  SceneSlot& s = t->scenes[0];
+  s.bvh_params = p;
   const auto t0 = std::chrono::steady_clock::now();
-  std::vector<float> raw;
-  std::string err;
-  if (!ptb::flatten(t->view, t->bvh.prims, raw, err)) return fail(t, PT_INVALID_ARGUMENT, err);
-  if (p.builder == PT_BVH_SAH) ptb::build_sah(t->bvh, raw, p.leaf_size, p.bucket_count);
-  else ptb::build_midpoint(t->bvh, raw, p.leaf_size);
-  ptb::pack_device_layout(t->bvh, t->layout);
-  const ptb::DeviceLayout& L = t->layout;
-
-  free_scene_device(t);
-  uint64_t bytes = 0;
-  auto upload = [&](auto& arr, const void* src, size_t count, size_t elem) -> hipError_t {
-    hipError_t e = arr.ensure(count);
-    if (e != hipSuccess) return e;
-    bytes += count * elem;
-    if (count) e = hipMemcpy(arr.ptr, src, count * elem, hipMemcpyHostToDevice);
-    return e;
-  };
-  PT_HIP(t, upload(t->d_nodes, L.nodes.data(), L.nodes.size() / 4u, sizeof(float4)), "upload BVH nodes");
-  PT_HIP(t, upload(t->d_tris, L.tris.data(), L.tris.size() / 4u, sizeof(float4)), "upload triangles");
-  PT_HIP(t, upload(t->d_prims, L.prims.data(), L.prims.size() / 4u, sizeof(float4)), "upload primitives");
-  PT_HIP(t, upload(t->d_materials, t->materials.data(), t->materials.size(), sizeof(PtMaterial)), "upload materials");
-  PT_HIP(t, upload(t->d_lights, t->lights.data(), t->lights.size(), sizeof(PtLight)), "upload lights");
-  PT_HIP(t, upload(t->d_env, t->environment_map.data(), t->environment_map.size(), sizeof(float)), "upload environment");
-
-  // Traversal stack spill area: a lane's stack never holds more than max_depth entries (one per
-  // level of the current path); entries older than the LDS ring's kStackLds go here.
-  const uint32_t need = t->bvh.max_depth + 2u;
-  const uint32_t threads =
-      std::max({t->grids_max.extend, t->grids_max.shadow, t->grids_max.shadow_adj, t->grids_max.query}) *
-      static_cast<uint32_t>(ptd::kBlock);
-  PT_HIP(t, t->d_stack_ref.ensure(static_cast<size_t>(need) * threads), "allocate traversal stack");
-  PT_HIP(t, t->d_stack_t.ensure(static_cast<size_t>(need) * threads), "allocate traversal stack");
-  bytes += static_cast<uint64_t>(need) * threads * 8u;
-  t->paths.stack_ref = t->d_stack_ref.ptr;
-  t->paths.stack_t = t->d_stack_t.ptr;
-  t->paths.stack_overflow_depth = need;
-  t->paths.stack_threads = threads;
-  t->paths.queue = t->d_queue.ptr;
-  t->paths.counters = t->d_counters.ptr;
-  t->paths.error = t->d_error.ptr;
-
-  ptd::DevScene& S = t->dscene;
-  S.nodes = t->d_nodes.ptr;
-  S.tris = t->d_tris.ptr;
-  S.prims = t->d_prims.ptr;
-  S.materials = t->d_materials.ptr;
-  S.lights = t->d_lights.ptr;
-  S.material_count = static_cast<uint32_t>(t->materials.size());
-  S.light_count = static_cast<uint32_t>(t->lights.size());
-  S.prim_count = static_cast<uint32_t>(t->bvh.prims.size());
-  S.has_transmissive = 0;
-  for (const PtMaterial& m : t->materials)
-    if (m.material_model == 5u || m.transmission > 0.05f) S.has_transmissive = 1;
-  S.root_ref = L.root_ref;
-  for (int k = 0; k < 6; ++k) S.root_box[k] = L.root_box[k];
-  S.general_boxes = 0;
-  for (size_t i = 0; i < L.nodes.size(); i += 16)  // 12 box floats, then the two child refs
-    for (size_t k = 0; k < 12; ++k)
-      if (!std::isfinite(L.nodes[i + k])) S.general_boxes = 1;
-  for (int k = 0; k < 3; ++k) {
-    S.env[k] = t->view.environment_color[k];
-    S.env_scale[k] = t->view.environment_map_scale[k];
-  }
-  const bool has_map = !t->environment_map.empty();
-  S.env_map = has_map ? t->d_env.ptr : nullptr;
-  S.env_w = has_map ? t->view.environment_map_width : 0u;
-  S.env_h = has_map ? t->view.environment_map_height : 0u;
-
-  // compute_camera_basis, ScalarCpuPathTracer.cpp:139-152; camera_rays constants, Render.cpp:74-83
-  const PtCamera& c = t->view.camera;
-  const V3h pos{c.position[0], c.position[1], c.position[2]};
-  const V3h fwd = normalize3(sub3(V3h{c.target[0], c.target[1], c.target[2]}, pos));
-  V3h right = normalize3(cross3(fwd, V3h{c.up[0], c.up[1], c.up[2]}));
-  if (dot3(right, right) <= 1e-4f * 1e-4f) right = {1.0f, 0.0f, 0.0f};
-  V3h up = normalize3(cross3(right, fwd));
-  if (dot3(up, up) <= 1e-4f * 1e-4f) up = {0.0f, 1.0f, 0.0f};
-  ptd::DevCamera& C = t->dcam;
-  const V3h* src[4] = {&pos, &fwd, &right, &up};
-  float* dst[4] = {C.pos, C.fwd, C.right, C.up};
-  for (int i = 0; i < 4; ++i) {
-    dst[i][0] = src[i]->x;
-    dst[i][1] = src[i]->y;
-    dst[i][2] = src[i]->z;
-  }
-  const float kPi = 3.14159265358979323846f;
-  C.aspect = static_cast<float>(t->settings.width) / std::max(1.0f, static_cast<float>(t->settings.height));
-  C.tan_half_fov = std::tan(0.5f * (c.fov_deg * (kPi / 180.0f)));
-  C.aperture = c.aperture_radius > 0.0f ? c.aperture_radius : t->settings.camera_aperture_radius;
-  C.focus = c.focus_distance > 0.0f ? c.focus_distance : t->settings.camera_focus_distance;
-  C.iris_blades = std::min(c.iris_blade_count, 64u);
-  C.iris_roundness = std::isfinite(c.iris_roundness) ? std::min(1.0f, std::max(0.0f, c.iris_roundness)) : 0.0f;
-  C.iris_rotation = c.iris_rotation_degrees * (kPi / 180.0f);
-  C.anamorphic = std::isfinite(c.anamorphic_squeeze) ? std::max(0.01f, c.anamorphic_squeeze) : 1.0f;
-
-  const auto t1 = std::chrono::steady_clock::now();
-  t->info = PtBuildInfo{};
-  t->info.build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-  t->info.device_bytes = bytes;
-  t->info.built = 1;
-  t->info.primitive_count = S.prim_count;
-  t->info.node_count = static_cast<uint32_t>(t->bvh.nodes.size());
-  t->info.leaf_count = t->bvh.leaf_count;
-  t->info.max_depth = t->bvh.max_depth;
-  t->info.deterministic = 1;
+  if (!build_scene(s)) return fail(t, PT_INVALID_ARGUMENT, s.error);
+  rc = upload_pool(t);
+  if (rc != PT_OK) return rc;
+  t->info.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (info) *info = t->info;
   t->accel_valid = true;
   ++t->scene_version;
@@ -710,7 +1040,7 @@ int pt_hip_reset_accumulation(pt_tracer* t) {
   if (!t) return PT_INVALID_ARGUMENT;
   if (!t->configured) return fail(t, PT_NOT_READY, "reset_accumulation called before configure");
   (void)hipSetDevice(t->device);
-  const uint32_t pixels = t->settings.width * t->settings.height;
+  const uint32_t pixels = t->batch ? t->film_pixels : t->settings.width * t->settings.height;
   ptd::launch_clear_film(t->stream, ptd::DevFilm{t->d_film_sum.ptr, t->d_film_count.ptr}, pixels);
   ptd::launch_zero_counters(t->stream, t->d_counters.ptr);
   t->current_sample = 0;
@@ -721,13 +1051,14 @@ int pt_hip_reset_accumulation(pt_tracer* t) {
 
 int pt_hip_render_tiles(pt_tracer* t, const PtRenderTile* tiles, uint32_t count, uint64_t frame_index) {
   if (!t) return PT_INVALID_ARGUMENT;
+  if (t->batch) return fail(t, PT_INVALID_ARGUMENT, "render_tiles: use pt_batch_render on a batch");
   if (!t->configured || !t->scene_loaded)
     return fail(t, PT_NOT_READY, "render_tile requires a configured tracer with a scene");
   if (!t->accel_valid) return fail(t, PT_NOT_READY, "render_tile requires build_or_update_acceleration");
   if (count && !tiles) return fail(t, PT_INVALID_ARGUMENT, "render_tiles: null tiles");
   (void)hipSetDevice(t->device);
   std::vector<ptd::DevTile> work;
-  int rc = build_work(t, tiles, count, work);
+  int rc = build_work(t, 0u, tiles, count, work);
   if (rc != PT_OK) return rc;
   rc = run_passes(t, work, dev_settings(t, frame_index), nullptr);
   if (rc != PT_OK) return rc;
@@ -755,24 +1086,7 @@ int pt_hip_read_film(pt_tracer* t, float* rgb_sum, uint32_t* sample_count) {
   if (!t) return PT_INVALID_ARGUMENT;
   if (!t->configured) return fail(t, PT_NOT_READY, "read_film called before configure");
   (void)hipSetDevice(t->device);
-  const size_t pixels = static_cast<size_t>(t->settings.width) * t->settings.height;
-  std::vector<float4> sum(rgb_sum ? pixels : 0);
-  if (rgb_sum && pixels)
-    PT_HIP(t, hipMemcpyAsync(sum.data(), t->d_film_sum.ptr, pixels * sizeof(float4), hipMemcpyDeviceToHost, t->stream),
-           "read film");
-  if (sample_count && pixels)
-    PT_HIP(t, hipMemcpyAsync(sample_count, t->d_film_count.ptr, pixels * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                             t->stream),
-           "read film");
-  const int rc = sync_all(t);
-  if (rc != PT_OK) return rc;
-  if (rgb_sum)
-    for (size_t i = 0; i < pixels; ++i) {
-      rgb_sum[3 * i] = sum[i].x;
-      rgb_sum[3 * i + 1] = sum[i].y;
-      rgb_sum[3 * i + 2] = sum[i].z;
-    }
-  return PT_OK;
+  return read_film_range(t, 0, static_cast<size_t>(t->settings.width) * t->settings.height, rgb_sum, sample_count);
 }
 
 int pt_hip_resolve_hdr(pt_tracer* t, float* rgb) {
@@ -796,17 +1110,6 @@ int pt_hip_resolve_hdr(pt_tracer* t, float* rgb) {
     rgb[3 * i + 2] = rgb[3 * i + 2] * inv;
   }
   return PT_OK;
-}
-
-static void to_counters(const unsigned long long* c, PtSampleCounters* out) {
-  *out = PtSampleCounters{};
-  out->samples = c[ptd::kSamples];
-  out->rays = c[ptd::kRays];
-  out->triangle_tests = c[ptd::kTriTests];
-  out->triangle_hits = c[ptd::kTriHits];
-  out->bvh_node_visits = c[ptd::kNodeVisits];
-  out->bvh_leaf_visits = c[ptd::kLeafVisits];
-  out->shadow_tests = c[ptd::kShadowTests];
 }
 
 int pt_hip_read_counters(pt_tracer* t, PtSampleCounters* out) {
@@ -858,6 +1161,7 @@ int pt_hip_set_wavefront_capacity(pt_tracer* t, uint32_t paths) {
   const int rc = pt_hip_synchronize(t);
   if (rc != PT_OK) return rc;
   t->capacity = paths;
+  drop_graph(t);
   if (t->path_capacity_alloc > paths) free_paths(t);
   return PT_OK;
 }
@@ -870,6 +1174,7 @@ int pt_hip_set_tuning(pt_tracer* t, const PtTuning* tu) {
   (void)hipSetDevice(t->device);
   const int rc = pt_hip_synchronize(t);
   if (rc != PT_OK) return rc;
+  drop_graph(t);
   if (tu->wavefront_capacity) {
     t->capacity = tu->wavefront_capacity;
     if (t->path_capacity_alloc > t->capacity) free_paths(t);
@@ -921,6 +1226,7 @@ int pt_hip_shutdown(pt_tracer* t) {
   free_paths(t);
   t->d_film_sum.release();
   t->d_film_count.release();
+  t->scenes.clear();
   t->configured = t->scene_loaded = t->accel_valid = false;
   t->current_sample = 0;
   t->accumulation_gen = 0;
@@ -931,6 +1237,7 @@ int pt_hip_shutdown(pt_tracer* t) {
 int pt_hip_trace_rays(pt_tracer* t, int mode, const float* rays, const float* tmax, uint32_t count, float* out_t,
                       int32_t* out_prim, PtSampleCounters* counters) {
   if (!t) return PT_INVALID_ARGUMENT;
+  if (t->batch) return fail(t, PT_INVALID_ARGUMENT, "trace_rays: not available on a batch");
   if (!t->accel_valid) return fail(t, PT_NOT_READY, "trace_rays requires build_or_update_acceleration");
   if (count && (!rays || !out_t || !out_prim)) return fail(t, PT_INVALID_ARGUMENT, "trace_rays: null buffer");
   if (mode != 0 && mode != 1) return fail(t, PT_INVALID_ARGUMENT, "trace_rays: mode must be 0 or 1");
@@ -938,7 +1245,6 @@ int pt_hip_trace_rays(pt_tracer* t, int mode, const float* rays, const float* tm
   ptd::DevPaths q = t->paths;
   q.counters = t->d_query_counters.ptr;
   ptd::launch_zero_counters(t->stream, q.counters);
-  ptd::launch_init_queue(t->stream, q, 0);
   if (count)
     ptd::launch_trace_rays(t->stream, q, t->dscene, mode, rays, tmax, count, out_t, out_prim, t->grids.query);
   const hipError_t le = hipGetLastError();
@@ -949,6 +1255,251 @@ int pt_hip_trace_rays(pt_tracer* t, int mode, const float* rays, const float* tm
   if (counters) to_counters(c, counters);
   return PT_OK;
 }
+
+int pt_hip_update_scene_delta(pt_tracer* t, const PtMaterial* materials, uint32_t material_count,
+                              const PtLight* lights, uint32_t light_count) {
+  if (!t) return PT_INVALID_ARGUMENT;
+  if (!t->accel_valid || t->scenes.size() != 1u)
+    return fail(t, PT_NOT_READY, "update_scene_delta requires an established acceleration");
+  (void)hipSetDevice(t->device);
+  SceneSlot& s = t->scenes[0];
+  if (materials && material_count != s.materials.size())
+    return fail(t, PT_INVALID_ARGUMENT, "update_scene_delta: material count differs from the loaded scene");
+  if (lights && light_count != s.lights.size())
+    return fail(t, PT_INVALID_ARGUMENT, "update_scene_delta: light count differs from the loaded scene");
+  PT_HIP(t, hipStreamSynchronize(t->stream), "synchronize");
+  if (materials) {
+    s.materials.assign(materials, materials + material_count);
+    PT_HIP(t, hipMemcpy(t->d_materials.ptr, materials, sizeof(PtMaterial) * material_count, hipMemcpyHostToDevice),
+           "upload materials");
+    uint32_t any = 0;
+    for (const PtMaterial& m : s.materials)
+      if (transmissive(m)) any = 1;
+    t->dscene.has_transmissive = any;
+    PT_HIP(t, hipMemcpy(&t->d_recs.ptr[0].has_transmissive, &any, sizeof(uint32_t), hipMemcpyHostToDevice),
+           "upload scene record");
+  }
+  if (lights && light_count) {
+    s.lights.assign(lights, lights + light_count);
+    PT_HIP(t, hipMemcpy(t->d_lights.ptr, lights, sizeof(PtLight) * light_count, hipMemcpyHostToDevice),
+           "upload lights");
+  }
+  ++t->scene_version;
+  t->last_error.clear();
+  return PT_OK;
+}
+
+int pt_hip_render_adjoint(pt_tracer* t, const PtRenderTile* tiles, uint32_t count, uint64_t frame_index,
+                          const float* film_grad, float* grad_albedo, float* grad_emission, float* grad_light) {
+  if (!t) return PT_INVALID_ARGUMENT;
+  if (t->batch) return fail(t, PT_INVALID_ARGUMENT, "render_adjoint: not available on a batch");
+  if (!t->configured || !t->scene_loaded || !t->accel_valid)
+    return fail(t, PT_NOT_READY, "render_adjoint requires a built acceleration structure");
+  if (count && !tiles) return fail(t, PT_INVALID_ARGUMENT, "render_adjoint: null tiles");
+  const SceneSlot& s = t->scenes[0];
+  if (!film_grad || !grad_albedo || !grad_emission || (!grad_light && !s.lights.empty()))
+    return fail(t, PT_INVALID_ARGUMENT, "render_adjoint: null gradient buffer");
+  if (t->settings.rr_start_depth > static_cast<uint32_t>(ptd::kAdjHist))
+    return fail(t, PT_UNSUPPORTED, "render_adjoint supports russian roulette start depths up to 4");
+  (void)hipSetDevice(t->device);
+  std::vector<ptd::DevTile> work;
+  int rc = build_work(t, 0u, tiles, count, work);
+  if (rc != PT_OK) return rc;
+  ptd::DevAdjoint A{};
+  A.g_albedo = grad_albedo;
+  A.g_emission = grad_emission;
+  A.g_light = grad_light;
+  A.film_grad = film_grad;
+  A.film_width = t->settings.width;
+  A.nmat = static_cast<uint32_t>(s.materials.size());
+  A.nlight = static_cast<uint32_t>(s.lights.size());
+  A.use_lds = (6u * A.nmat + 3u * A.nlight) <= static_cast<uint32_t>(ptd::kAdjLdsFloats) ? 1u : 0u;
+  rc = run_passes(t, work, dev_settings(t, frame_index), &A);
+  if (rc != PT_OK) return rc;
+  if (t->profiling) {
+    rc = sync_all(t);
+    if (rc != PT_OK) return rc;
+  }
+  t->last_error.clear();
+  return PT_OK;
+}
+
+// ------------------------------------------------------------------ batches
+int pt_batch_create(int device, const PtRenderSettings* settings, pt_batch** out) {
+  if (!out || !settings) return PT_INVALID_ARGUMENT;
+  *out = nullptr;
+  if (!(settings->rr_min_survival == settings->rr_min_survival) ||
+      !(settings->rr_max_survival == settings->rr_max_survival))
+    return PT_INVALID_ARGUMENT;
+  pt_batch* b = new (std::nothrow) pt_batch();
+  if (!b) return PT_ALLOC_FAILED;
+  pt_tracer* t = &b->t;
+  const int rc = init_tracer(t, device);
+  if (rc != PT_OK) {
+    destroy_tracer(t);
+    delete b;
+    return rc;
+  }
+  t->batch = true;
+  t->settings = *settings;
+  t->settings.max_depth = std::max(1u, settings->max_depth);
+  t->settings.camera_aperture_radius = std::max(0.0f, settings->camera_aperture_radius);
+  t->settings.camera_focus_distance = std::max(0.0f, settings->camera_focus_distance);
+  t->configured = true;
+  *out = b;
+  return PT_OK;
+}
+
+void pt_batch_destroy(pt_batch* b) {
+  if (!b) return;
+  destroy_tracer(&b->t);
+  delete b;
+}
+
+const char* pt_batch_last_error(const pt_batch* b) { return b ? b->t.last_error.c_str() : "null batch"; }
+
+int pt_batch_add_scene(pt_batch* b, const PtSceneView* scene, uint32_t width, uint32_t height, uint32_t spp,
+                       const PtBvhParams* params, uint32_t* index) {
+  if (!b) return PT_INVALID_ARGUMENT;
+  pt_tracer* t = &b->t;
+  if (!scene) return fail(t, PT_INVALID_ARGUMENT, "add_scene: null scene");
+  if (width == 0 || height == 0 || spp == 0) return fail(t, PT_INVALID_ARGUMENT, "add_scene: empty image or spp");
+  PtBvhParams p;
+  const int rc = check_bvh_params(t, params, p);
+  if (rc != PT_OK) return rc;
+  t->scenes.emplace_back();
+  SceneSlot& s = t->scenes.back();
+  copy_scene(s, scene);
+  s.width = width;
+  s.height = height;
+  s.spp = spp;
+  s.bvh_params = p;
+  t->scene_loaded = true;
+  t->accel_valid = false;
+  if (index) *index = static_cast<uint32_t>(t->scenes.size() - 1u);
+  t->last_error.clear();
+  return PT_OK;
+}
+
+int pt_batch_build(pt_batch* b, PtBuildInfo* info) {
+  if (!b) return PT_INVALID_ARGUMENT;
+  pt_tracer* t = &b->t;
+  if (t->scenes.empty()) return fail(t, PT_NOT_READY, "batch_build: no scenes");
+  (void)hipSetDevice(t->device);
+  const auto t0 = std::chrono::steady_clock::now();
+  // host builds in parallel, largest scenes first
+  std::vector<size_t> order(t->scenes.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+  std::sort(order.begin(), order.end(),
+            [&](size_t a, size_t c) { return t->scenes[a].indices.size() > t->scenes[c].indices.size(); });
+  const unsigned nthreads = std::max(1u, std::min<unsigned>(16u, std::thread::hardware_concurrency()));
+  std::vector<bool> ok(t->scenes.size(), true);
+  {
+    std::vector<std::thread> pool;
+    std::atomic<size_t> cursor{0};
+    for (unsigned w = 0; w < std::min<size_t>(nthreads, order.size()); ++w)
+      pool.emplace_back([&] {
+        for (size_t k = cursor++; k < order.size(); k = cursor++) ok[order[k]] = build_scene(t->scenes[order[k]]);
+      });
+    for (std::thread& th : pool) th.join();
+  }
+  for (size_t i = 0; i < ok.size(); ++i)
+    if (!ok[i]) return fail(t, PT_INVALID_ARGUMENT, "scene " + std::to_string(i) + ": " + t->scenes[i].error);
+  int rc = upload_pool(t);
+  if (rc != PT_OK) return rc;
+  rc = alloc_film(t, t->film_pixels);
+  if (rc != PT_OK) return rc;
+  ptd::launch_zero_counters(t->stream, t->d_counters.ptr);
+  t->info.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (info) *info = t->info;
+  t->accel_valid = true;
+  ++t->scene_version;
+  t->last_error.clear();
+  return PT_OK;
+}
+
+int pt_batch_reset_accumulation(pt_batch* b) { return b ? pt_hip_reset_accumulation(&b->t) : PT_INVALID_ARGUMENT; }
+
+int pt_batch_render(pt_batch* b, uint64_t frame_index, int use_graph) {
+  if (!b) return PT_INVALID_ARGUMENT;
+  pt_tracer* t = &b->t;
+  if (!t->accel_valid) return fail(t, PT_NOT_READY, "batch_render requires pt_batch_build");
+  (void)hipSetDevice(t->device);
+  std::vector<ptd::DevTile> work;
+  for (uint32_t si = 0; si < t->scenes.size(); ++si) {
+    const SceneSlot& s = t->scenes[si];
+    std::vector<PtRenderTile> tiles;
+    for (uint32_t k = 0; k < s.spp; ++k) tiles.push_back(PtRenderTile{0u, 0u, s.width, s.height, k, 0u, 0u});
+    const int rc = build_work(t, si, tiles.data(), static_cast<uint32_t>(tiles.size()), work);
+    if (rc != PT_OK) return rc;
+  }
+  ptd::DevSettings st = dev_settings(t, frame_index);
+  if (!use_graph) {
+    const int rc = run_passes(t, work, st, nullptr);
+    if (rc != PT_OK) return rc;
+  } else {
+    if (!t->graph_exec) {
+      std::vector<Pass> passes;
+      std::vector<ptd::FilmJob> jobs;
+      int rc = plan_passes(t, work, passes, jobs);
+      if (rc == PT_OK) rc = prepare_passes(t, passes, nullptr);
+      if (rc != PT_OK) return rc;
+      PT_HIP(t, hipStreamSynchronize(t->stream), "synchronize");  // staging copies done before capture
+      t->stage_pending = false;
+      st.frame_dev = t->d_frame.ptr;
+      const bool prof = t->profiling;
+      t->profiling = false;
+      PT_HIP(t, hipStreamBeginCapture(t->stream, hipStreamCaptureModeThreadLocal), "begin graph capture");
+      rc = launch_passes(t, work, passes, st, nullptr);
+      hipGraph_t g = nullptr;
+      const hipError_t ec = hipStreamEndCapture(t->stream, &g);
+      t->profiling = prof;
+      t->fwd_rad_valid = false;
+      if (rc != PT_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (ec != hipSuccess) return hip_fail(t, ec, "end graph capture");
+      t->graph = g;
+      PT_HIP(t, hipGraphInstantiate(&t->graph_exec, t->graph, nullptr, nullptr, 0), "instantiate graph");
+    }
+    const int rc = stage_upload(t, t->d_frame.ptr, &frame_index, sizeof(uint64_t));
+    if (rc != PT_OK) return rc;
+    PT_HIP(t, hipGraphLaunch(t->graph_exec, t->stream), "launch graph");
+  }
+  if (t->profiling) {
+    const int rc = sync_all(t);
+    if (rc != PT_OK) return rc;
+  }
+  t->last_error.clear();
+  return PT_OK;
+}
+
+int pt_batch_read_film(pt_batch* b, uint32_t scene, float* rgb_sum, uint32_t* sample_count) {
+  if (!b) return PT_INVALID_ARGUMENT;
+  pt_tracer* t = &b->t;
+  if (!t->accel_valid) return fail(t, PT_NOT_READY, "batch_read_film requires pt_batch_build");
+  if (scene >= t->scenes.size()) return fail(t, PT_INVALID_ARGUMENT, "batch_read_film: no such scene");
+  (void)hipSetDevice(t->device);
+  size_t base = 0;
+  for (uint32_t i = 0; i < scene; ++i) base += static_cast<size_t>(t->scenes[i].width) * t->scenes[i].height;
+  const SceneSlot& s = t->scenes[scene];
+  return read_film_range(t, base, static_cast<size_t>(s.width) * s.height, rgb_sum, sample_count);
+}
+
+int pt_batch_read_counters(pt_batch* b, PtSampleCounters* out) {
+  return b ? pt_hip_read_counters(&b->t, out) : PT_INVALID_ARGUMENT;
+}
+int pt_batch_set_tuning(pt_batch* b, const PtTuning* tu) { return b ? pt_hip_set_tuning(&b->t, tu) : PT_INVALID_ARGUMENT; }
+int pt_batch_get_tuning(pt_batch* b, PtTuning* tu) { return b ? pt_hip_get_tuning(&b->t, tu) : PT_INVALID_ARGUMENT; }
+int pt_batch_set_profiling(pt_batch* b, int enabled) {
+  return b ? pt_hip_set_profiling(&b->t, enabled) : PT_INVALID_ARGUMENT;
+}
+int pt_batch_read_kernel_stats(pt_batch* b, PtKernelStats* out) {
+  return b ? pt_hip_read_kernel_stats(&b->t, out) : PT_INVALID_ARGUMENT;
+}
+int pt_batch_reset_kernel_stats(pt_batch* b) { return b ? pt_hip_reset_kernel_stats(&b->t) : PT_INVALID_ARGUMENT; }
+int pt_batch_synchronize(pt_batch* b) { return b ? pt_hip_synchronize(&b->t) : PT_INVALID_ARGUMENT; }
 
 }  // extern "C"
 
@@ -997,67 +1548,5 @@ extern "C" int pt_bvh_build_host(const PtSceneView* scene, const PtBvhParams* pa
         g[9 + k] = q.n[k];
       }
     }
-  return PT_OK;
-}
-
-extern "C" int pt_hip_update_scene_delta(pt_tracer* t, const PtMaterial* materials, uint32_t material_count,
-                                         const PtLight* lights, uint32_t light_count) {
-  if (!t) return PT_INVALID_ARGUMENT;
-  if (!t->accel_valid) return fail(t, PT_NOT_READY, "update_scene_delta requires an established acceleration");
-  (void)hipSetDevice(t->device);
-  if (materials && material_count != t->materials.size())
-    return fail(t, PT_INVALID_ARGUMENT, "update_scene_delta: material count differs from the loaded scene");
-  if (lights && light_count != t->lights.size())
-    return fail(t, PT_INVALID_ARGUMENT, "update_scene_delta: light count differs from the loaded scene");
-  PT_HIP(t, hipStreamSynchronize(t->stream), "synchronize");
-  if (materials) {
-    t->materials.assign(materials, materials + material_count);
-    PT_HIP(t, hipMemcpy(t->d_materials.ptr, materials, sizeof(PtMaterial) * material_count, hipMemcpyHostToDevice),
-           "upload materials");
-    t->dscene.has_transmissive = 0;
-    for (const PtMaterial& m : t->materials)
-      if (m.material_model == 5u || m.transmission > 0.05f) t->dscene.has_transmissive = 1;
-  }
-  if (lights && light_count) {
-    t->lights.assign(lights, lights + light_count);
-    PT_HIP(t, hipMemcpy(t->d_lights.ptr, lights, sizeof(PtLight) * light_count, hipMemcpyHostToDevice),
-           "upload lights");
-  }
-  ++t->scene_version;
-  t->last_error.clear();
-  return PT_OK;
-}
-
-extern "C" int pt_hip_render_adjoint(pt_tracer* t, const PtRenderTile* tiles, uint32_t count, uint64_t frame_index,
-                                     const float* film_grad, float* grad_albedo, float* grad_emission,
-                                     float* grad_light) {
-  if (!t) return PT_INVALID_ARGUMENT;
-  if (!t->configured || !t->scene_loaded || !t->accel_valid)
-    return fail(t, PT_NOT_READY, "render_adjoint requires a built acceleration structure");
-  if (count && !tiles) return fail(t, PT_INVALID_ARGUMENT, "render_adjoint: null tiles");
-  if (!film_grad || !grad_albedo || !grad_emission || (!grad_light && !t->lights.empty()))
-    return fail(t, PT_INVALID_ARGUMENT, "render_adjoint: null gradient buffer");
-  if (t->settings.rr_start_depth > static_cast<uint32_t>(ptd::kAdjHist))
-    return fail(t, PT_UNSUPPORTED, "render_adjoint supports russian roulette start depths up to 4");
-  (void)hipSetDevice(t->device);
-  std::vector<ptd::DevTile> work;
-  int rc = build_work(t, tiles, count, work);
-  if (rc != PT_OK) return rc;
-  ptd::DevAdjoint A{};
-  A.g_albedo = grad_albedo;
-  A.g_emission = grad_emission;
-  A.g_light = grad_light;
-  A.film_grad = film_grad;
-  A.film_width = t->settings.width;
-  A.nmat = static_cast<uint32_t>(t->materials.size());
-  A.nlight = static_cast<uint32_t>(t->lights.size());
-  A.use_lds = (6u * A.nmat + 3u * A.nlight) <= static_cast<uint32_t>(ptd::kAdjLdsFloats) ? 1u : 0u;
-  rc = run_passes(t, work, dev_settings(t, frame_index), &A);
-  if (rc != PT_OK) return rc;
-  if (t->profiling) {
-    rc = sync_all(t);
-    if (rc != PT_OK) return rc;
-  }
-  t->last_error.clear();
   return PT_OK;
 }

@@ -57,22 +57,6 @@ __host__ __device__ constexpr uint32_t segment_capacity(uint32_t capacity) {
   return 64u * ((((capacity + 63u) / 64u) + kSegments - 1u) / kSegments);
 }
 
-struct DevScene {
-  const float4* nodes;   // 4 x float4 per inner node
-  const float4* tris;    // 3 x float4 per leaf-ordered triangle
-  const float4* prims;   // normal.xyz + material bits, by primitive index
-  const PtMaterial* materials;
-  const PtLight* lights;
-  uint32_t material_count, light_count, prim_count, has_transmissive;
-  uint32_t general_boxes;  // some node box coordinate is not finite: every ray takes the select form of slab
-  int32_t root_ref;
-  float root_box[6];
-  float env[3];
-  const float* env_map;  // rgb, env_w * env_h (nullptr: constant colour)
-  uint32_t env_w, env_h;
-  float env_scale[3];
-};
-
 struct DevCamera {
   float pos[3], fwd[3], right[3], up[3];
   float aspect, tan_half_fov, aperture, focus;
@@ -80,30 +64,68 @@ struct DevCamera {
   uint32_t iris_blades;
 };
 
+// One scene of the pool: where its tree, lights, environment and film live. A single-scene
+// tracer has one record; a batch has one per scene, and every path carries its scene's index.
+struct SceneRec {
+  int32_t root_ref;       // pool-rebased root reference
+  uint32_t prim_count;
+  float root_box[6];
+  uint32_t light_base, light_count, has_transmissive, env_base;  // env_base: float offset in the env pool
+  uint32_t env_w, env_h;  // 0 x 0: constant environment colour
+  uint32_t width, height, film_base;  // film pixels [film_base, film_base + width * height)
+  float env[3], env_scale[3];
+  DevCamera cam;
+};
+
+// The scene pool as kernels see it: all scenes' nodes, triangles and primitives (node and
+// primitive references rebased into the pool, material indices clamped per scene and rebased),
+// materials and lights. Inside shading a lane works on a copy whose lights, light_count,
+// has_transmissive and env fields are its own scene's (scene_view).
+struct DevScene {
+  const float4* nodes;   // 4 x float4 per inner node
+  const float4* tris;    // 3 x float4 per leaf-ordered triangle
+  const float4* prims;   // normal.xyz + pool material index, by pool primitive index
+  const PtMaterial* materials;
+  const PtLight* lights;
+  uint32_t material_count, light_count, has_transmissive;
+  uint32_t general_boxes;  // some node box coordinate is not finite: every ray takes the select form of slab
+  float env[3];
+  const float* env_map;  // rgb, env_w * env_h (nullptr: constant colour)
+  uint32_t env_w, env_h;
+  float env_scale[3];
+  const SceneRec* recs;
+  uint32_t scene_count;
+};
+
+// Integrator settings shared by every scene of a launch (image sizes are per scene: SceneRec).
 struct DevSettings {
   uint64_t seed;
   uint64_t frame;
-  uint32_t width, height, max_depth, enable_nee, enable_mis, rr_start;
+  const uint64_t* frame_dev;  // when set, raygen reads the frame index here (graph replays)
+  uint32_t max_depth, enable_nee, enable_mis, rr_start;
   float rr_min, rr_max;
 };
 
-// Tile table entry for one wavefront pass: paths [path_begin, path_begin + w*h) belong to it.
+// Tile of one wavefront pass: paths [path_begin, path_begin + w*h) of the pass render sample
+// `sample` of this rectangle of scene `scene`'s image.
 struct DevTile {
-  uint32_t x, y, w, h, sample, path_begin;
+  uint32_t x, y, w, h, sample, path_begin, scene, pad;
 };
 
-// The tiles of one pass travel as a kernel argument (no host->device copy per pass).
-constexpr uint32_t kMaxTilesPerPass = 96u;
-struct DevTileTable {
-  DevTile t[kMaxTilesPerPass];
-  uint32_t count;
+// Film accumulation of one scene in one pass: the bounding rectangle of the scene's tiles in the
+// pass and their range in the pass's tile table (in render order). Job pixels are numbered from
+// pixel_begin across the pass's jobs.
+struct FilmJob {
+  uint32_t scene, x0, y0, w, h, first_tile, tile_count, pixel_begin;
 };
 
-// One ray queue: origin xyz, direction xyz (SoA) and the path slot each queue entry serves.
+// One ray queue: origin xyz, direction xyz (SoA), the path slot each entry serves and its scene.
 struct DevRays {
   float* c[6];
   uint32_t* qpath;
+  uint32_t* qscene;
 };
+
 
 // Per-path wavefront state. Ray and hit buffers are in queue order (double-buffered rays);
 // throughput, radiance and RNG state are indexed by the path's slot in the pass.
@@ -118,6 +140,7 @@ struct DevPaths {
   float* stmax;
   float* scontrib[3];
   uint32_t* spath;
+  uint32_t* sscene;
   uint32_t* queue;               // kQueueWords: segment counts and append cursors
   unsigned long long* counters;  // kCounterWords: striped CounterSlot counters
   uint32_t* error;               // nonzero: a kernel refused work (stack overflow area too small)
@@ -163,24 +186,23 @@ struct DevAdjoint {
 
 // Launchers (pt_kernels.hip). All launch on `stream`; none synchronizes.
 void launch_init_queue(hipStream_t stream, const DevPaths& paths, uint32_t live);
-void launch_raygen(hipStream_t stream, const DevPaths& paths, const DevCamera& cam, const DevSettings& st,
-                   const DevTileTable& tiles, uint32_t path_count);
+void launch_raygen(hipStream_t stream, const DevPaths& paths, const DevScene& scene, const DevSettings& st,
+                   const DevTile* tiles, uint32_t tile_count, uint32_t path_count);
 void launch_extend(hipStream_t stream, const DevPaths& paths, const DevScene& scene, int in_buf, uint32_t grid);
 void launch_shade(hipStream_t stream, const DevPaths& paths, const DevScene& scene, const DevSettings& st,
                   int in_buf, uint32_t depth, uint32_t grid);
 void launch_shadow(hipStream_t stream, const DevPaths& paths, const DevScene& scene, uint32_t grid);
 void launch_advance(hipStream_t stream, const DevPaths& paths);
-// Adds the radiance of every tile of the pass into the film, pixel by pixel in tile order.
-// rect = {x0, y0, x1, y1}: the bounding rectangle of the pass's tiles.
-void launch_film(hipStream_t stream, const DevPaths& paths, const DevFilm& film, uint32_t film_width,
-                 const DevTileTable& tiles, const uint32_t rect[4]);
+// Adds the radiance of every tile of the pass into the films, pixel by pixel in tile order.
+void launch_film(hipStream_t stream, const DevPaths& paths, const DevScene& scene, const DevFilm& film,
+                 const DevTile* tiles, const FilmJob* jobs, uint32_t job_count, uint32_t job_pixels);
 void launch_trace_rays(hipStream_t stream, const DevPaths& paths, const DevScene& scene, int any_hit,
                        const float* rays, const float* tmax, uint32_t count, float* out_t, int32_t* out_prim,
                        uint32_t grid);
 // Adjoint variants: raygen also gathers the per-path loss adjoint; shade and shadow accumulate
 // gradients; copy_radiance saves the forward radiance of a pass before the replay.
-void launch_raygen_adjoint(hipStream_t stream, const DevPaths& paths, const DevAdjoint& adj, const DevCamera& cam,
-                           const DevSettings& st, const DevTileTable& tiles, uint32_t path_count);
+void launch_raygen_adjoint(hipStream_t stream, const DevPaths& paths, const DevAdjoint& adj, const DevScene& scene,
+                           const DevSettings& st, const DevTile* tiles, uint32_t tile_count, uint32_t path_count);
 void launch_shade_adjoint(hipStream_t stream, const DevPaths& paths, const DevAdjoint& adj, const DevScene& scene,
                           const DevSettings& st, int in_buf, uint32_t depth, uint32_t grid);
 void launch_shadow_adjoint(hipStream_t stream, const DevPaths& paths, const DevAdjoint& adj, const DevScene& scene,

@@ -195,18 +195,18 @@ struct Trav {
 
 // Starts a query below t_max. False when it ends at once (no primitives, or the root box is
 // missed); best_prim is then -1.
-__device__ __forceinline__ bool trav_begin(const DevScene& sc, Trav& s, V3 o, V3 d, float tmax, TravStats& st) {
+__device__ __forceinline__ bool trav_begin(const SceneRec& R, Trav& s, V3 o, V3 d, float tmax, TravStats& st) {
   s.best_prim = -1;
   s.best_t = tmax;
   s.d = d;
   s.sp = 0;
   s.lo = 0;
-  s.cur = sc.root_ref;
-  if (sc.prim_count == 0) return false;
+  s.cur = R.root_ref;
+  if (R.prim_count == 0) return false;
   s.q = make_query(o, d);
   ++st.visits;
   float tn;
-  return slab(s.q, sc.root_box[0], sc.root_box[1], sc.root_box[2], sc.root_box[3], sc.root_box[4], sc.root_box[5],
+  return slab(s.q, R.root_box[0], R.root_box[1], R.root_box[2], R.root_box[3], R.root_box[4], R.root_box[5],
               s.best_t, tn);
 }
 
@@ -492,27 +492,32 @@ __global__ __launch_bounds__(kBlock) void k_clear_film(DevFilm film, uint32_t pi
 // camera_rays + shade_pixel seeding, ScalarCpuPathTracer.Render.cpp:55-119, :243-262.
 // The adjoint variant also loads the path's loss adjoint from the film gradient.
 template <bool kAdj>
-__global__ __launch_bounds__(kBlock) void k_raygen(DevPaths P, DevRays out, DevCamera cam, DevSettings st,
-                                                  DevTileTable tiles, uint32_t path_count, DevAdjoint A) {
+__global__ __launch_bounds__(kBlock) void k_raygen(DevPaths P, DevRays out, DevScene sc, DevSettings st,
+                                                  const DevTile* tiles, uint32_t tile_count, uint32_t path_count,
+                                                  DevAdjoint A) {
   const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
   if (p >= path_count) return;
-  uint32_t lo = 0, hi = tiles.count;  // last tile with path_begin <= p
+  uint32_t lo = 0, hi = tile_count;  // last tile with path_begin <= p
   while (hi - lo > 1u) {
     const uint32_t mid = (lo + hi) >> 1;
-    if (tiles.t[mid].path_begin <= p) lo = mid; else hi = mid;
+    if (tiles[mid].path_begin <= p) lo = mid; else hi = mid;
   }
-  const DevTile tile = tiles.t[lo];
+  const DevTile tile = tiles[lo];
+  const SceneRec& rec = sc.recs[tile.scene];
+  const DevCamera& cam = rec.cam;
+  const uint32_t width = rec.width, height = rec.height;
   const uint32_t local = p - tile.path_begin;
   const uint32_t x = tile.x + local % tile.w;
   const uint32_t y = tile.y + local / tile.w;
   const uint32_t sample = tile.sample;
 
+  const uint64_t frame = st.frame_dev ? *st.frame_dev : st.frame;
   const uint64_t sample_seed = (static_cast<uint64_t>(y) << 32) | x;
-  const uint64_t path_id = sample_seed + static_cast<uint64_t>(sample) + st.frame;
-  uint64_t crng = rng_seed(st.seed, st.frame, sample, static_cast<uint32_t>(path_id),
-                           static_cast<uint64_t>(y) * st.width + x);
-  const float fx = (static_cast<float>(x) + next01(crng)) / static_cast<float>(st.width > 1u ? st.width : 1u);
-  const float fy = (static_cast<float>(y) + next01(crng)) / static_cast<float>(st.height > 1u ? st.height : 1u);
+  const uint64_t path_id = sample_seed + static_cast<uint64_t>(sample) + frame;
+  uint64_t crng = rng_seed(st.seed, frame, sample, static_cast<uint32_t>(path_id),
+                           static_cast<uint64_t>(y) * width + x);
+  const float fx = (static_cast<float>(x) + next01(crng)) / static_cast<float>(width > 1u ? width : 1u);
+  const float fy = (static_cast<float>(y) + next01(crng)) / static_cast<float>(height > 1u ? height : 1u);
   const uint64_t seed_mix = static_cast<uint64_t>(next01(crng) * static_cast<float>(0xffffu));
   const float nx = (2.0f * fx - 1.0f) * cam.aspect * cam.tan_half_fov;
   const float ny = (1.0f - 2.0f * fy) * cam.tan_half_fov;
@@ -539,10 +544,11 @@ __global__ __launch_bounds__(kBlock) void k_raygen(DevPaths P, DevRays out, DevC
     ro = add(C, offset);
     rd = normalize(sub(focus_point, ro));
   }
-  const uint64_t prng = rng_seed(st.seed + sample_seed + seed_mix, st.frame, sample, path_id, sample_seed);
+  const uint64_t prng = rng_seed(st.seed + sample_seed + seed_mix, frame, sample, path_id, sample_seed);
   out.c[0][p] = ro.x; out.c[1][p] = ro.y; out.c[2][p] = ro.z;
   out.c[3][p] = rd.x; out.c[4][p] = rd.y; out.c[5][p] = rd.z;
   out.qpath[p] = p;
+  out.qscene[p] = tile.scene;
   P.tp[0][p] = 1.0f; P.tp[1][p] = 1.0f; P.tp[2][p] = 1.0f;
   P.rad[0][p] = 0.0f; P.rad[1][p] = 0.0f; P.rad[2][p] = 0.0f;
   P.rng[p] = prng;
@@ -652,7 +658,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevPaths P, DevRays in, DevSc
     pos = queue_pos(prefix, j, P.seg_cap);
     const V3 o = mk(in.c[0][pos], in.c[1][pos], in.c[2][pos]);
     const V3 d = mk(in.c[3][pos], in.c[4][pos], in.c[5][pos]);
-    return trav_begin(sc, s, o, d, kInf, st);
+    return trav_begin(sc.recs[in.qscene[pos]], s, o, d, kInf, st);
   };
   auto end = [&](uint32_t, Trav& s) {
     P.hit_t[pos] = s.best_t;
@@ -663,6 +669,23 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevPaths P, DevRays in, DevSc
   drive_queries<false>(sc, stk, n, P.refill_min, st, begin, end);
   flush_stats(P.counters, st, false);
   wave_add(P.counters, kRays, rays);
+}
+
+// The pool as seen by one path of scene sid: that scene's lights and environment.
+__device__ __forceinline__ DevScene scene_view(const DevScene& pool, uint32_t sid) {
+  const SceneRec& R = pool.recs[sid];
+  DevScene v = pool;
+  v.lights = pool.lights + R.light_base;
+  v.light_count = R.light_count;
+  v.has_transmissive = R.has_transmissive;
+  for (int k = 0; k < 3; ++k) {
+    v.env[k] = R.env[k];
+    v.env_scale[k] = R.env_scale[k];
+  }
+  v.env_map = (R.env_w && R.env_h) ? pool.env_map + R.env_base : nullptr;
+  v.env_w = R.env_w;
+  v.env_h = R.env_h;
+  return v;
 }
 
 struct MatFlags {
@@ -1249,13 +1272,15 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevPaths P, DevRays in, DevRay
     ShadeOut so;
     so.cont = false;
     so.shadow = false;
-    uint32_t p = 0;
+    uint32_t p = 0, sid = 0;
     if (i < n) {
       const uint32_t q = queue_pos(prefix, i, P.seg_cap);
       p = in.qpath[q];
+      sid = in.qscene[q];
       const V3 ro = mk(in.c[0][q], in.c[1][q], in.c[2][q]);
       const V3 rd = mk(in.c[3][q], in.c[4][q], in.c[5][q]);
-      shade_path<kAdj>(P, sc, st, depth, p, ro, rd, P.hit_t[q], P.hit_prim[q], so, shadow_tests, A, glds);
+      shade_path<kAdj>(P, scene_view(sc, sid), st, depth, p, ro, rd, P.hit_t[q], P.hit_prim[q], so, shadow_tests, A,
+                       glds);
     }
     const uint32_t seg = (w0 >> 6) & (kSegments - 1u);  // this chunk's output segment
     const uint32_t slot = segment_append(P.queue + kQueueOutCursor, seg, P.seg_cap, so.cont);
@@ -1263,6 +1288,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevPaths P, DevRays in, DevRay
       out.c[0][slot] = so.no.x; out.c[1][slot] = so.no.y; out.c[2][slot] = so.no.z;
       out.c[3][slot] = so.nd.x; out.c[4][slot] = so.nd.y; out.c[5][slot] = so.nd.z;
       out.qpath[slot] = p;
+      out.qscene[slot] = sid;
     }
     const uint32_t sslot = segment_append(P.queue + kQueueShadowCursor, seg, P.seg_cap, so.shadow);
     if (so.shadow) {
@@ -1271,6 +1297,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevPaths P, DevRays in, DevRay
       P.stmax[sslot] = so.stmax;
       P.scontrib[0][sslot] = so.contrib.x; P.scontrib[1][sslot] = so.contrib.y; P.scontrib[2][sslot] = so.contrib.z;
       P.spath[sslot] = p;
+      P.sscene[sslot] = sid;
       if constexpr (kAdj) {
         A.s_galb[0][sslot] = so.galb.x; A.s_galb[1][sslot] = so.galb.y; A.s_galb[2][sslot] = so.galb.z;
         A.s_glight[0][sslot] = so.glight.x; A.s_glight[1][sslot] = so.glight.y; A.s_glight[2][sslot] = so.glight.z;
@@ -1321,7 +1348,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevPaths P, DevScene sc, DevA
       q = queue_pos(prefix, j, P.seg_cap);
       const V3 o = mk(P.sray[0][q], P.sray[1][q], P.sray[2][q]);
       const V3 d = mk(P.sray[3][q], P.sray[4][q], P.sray[5][q]);
-      return trav_begin(sc, s, o, d, P.stmax[q], st);
+      return trav_begin(sc.recs[P.sscene[q]], s, o, d, P.stmax[q], st);
     };
     auto end = [&](uint32_t, Trav& s) {
       if (s.best_prim < 0) light_visible<kAdj>(P, A, glds, q);
@@ -1331,7 +1358,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevPaths P, DevScene sc, DevA
   } else {
     V3 o, d;
     float remaining = 0.0f;
-    uint32_t skips = 0;
+    uint32_t skips = 0, sid = 0;
     // Starts the next step of the loop; false when the loop is over (then `end` is called).
     auto begin = [&](uint32_t j, Trav& s) {
       q = queue_pos(prefix, j, P.seg_cap);
@@ -1339,11 +1366,12 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevPaths P, DevScene sc, DevA
       d = mk(P.sray[3][q], P.sray[4][q], P.sray[5][q]);
       remaining = P.stmax[q];
       skips = 0;
+      sid = P.sscene[q];
       if (!(remaining > 0.004f)) {
         s.best_prim = -1;  // the loop body never runs: visible
         return false;
       }
-      return trav_begin(sc, s, o, d, kInf, st);
+      return trav_begin(sc.recs[sid], s, o, d, kInf, st);
     };
     auto end = [&](uint32_t, Trav& s) {
       while (true) {
@@ -1356,7 +1384,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevPaths P, DevScene sc, DevA
         remaining -= advance;
         ++skips;
         if (!(skips < 8u && remaining > 0.004f)) break;  // loop exhausted: visible
-        if (trav_begin(sc, s, o, d, kInf, st)) return true;
+        if (trav_begin(sc.recs[sid], s, o, d, kInf, st)) return true;
         // the follow-up query ended at once (root box missed): nothing in the way
         break;
       }
@@ -1370,32 +1398,40 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevPaths P, DevScene sc, DevA
 }
 
 // FilmBuffer::add_sample for every tile of a pass, after shade_pixel's non-finite drop
-// (ScalarCpuPathTracer.Render.cpp:270-283, FilmBuffer.cpp:80-91). One thread per pixel of the
-// tiles' bounding rectangle adds the tiles' samples in table order, as sequential
+// (ScalarCpuPathTracer.Render.cpp:270-283, FilmBuffer.cpp:80-91). One thread per pixel of each
+// scene's tile rectangle adds that scene's samples of the pass in tile order, as sequential
 // render_tile calls would.
-__global__ __launch_bounds__(kBlock) void k_film(DevPaths P, DevFilm film, uint32_t film_width, DevTileTable tiles,
-                                                uint32_t x0, uint32_t y0, uint32_t rw, uint32_t rh) {
-  const uint32_t local = blockIdx.x * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kBlock) void k_film(DevPaths P, DevScene sc, DevFilm film, const DevTile* tiles,
+                                                const FilmJob* jobs, uint32_t job_count, uint32_t job_pixels) {
+  const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
   uint32_t added = 0;
-  if (local < rw * rh) {
-    const uint32_t x = x0 + local % rw, y = y0 + local / rw;
-    const uint32_t px = y * film_width + x;
+  if (g < job_pixels) {
+    uint32_t lo = 0, hi = job_count;  // last job with pixel_begin <= g
+    while (hi - lo > 1u) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (jobs[mid].pixel_begin <= g) lo = mid; else hi = mid;
+    }
+    const FilmJob J = jobs[lo];
+    const uint32_t local = g - J.pixel_begin;
+    const uint32_t x = J.x0 + local % J.w, y = J.y0 + local / J.w;
+    const SceneRec& R = sc.recs[J.scene];
+    const uint32_t px = R.film_base + y * R.width + x;
     float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     uint32_t cnt = 0;
     bool touched = false;
-    for (uint32_t k = 0; k < tiles.count; ++k) {
-      const DevTile& t = tiles.t[k];
+    for (uint32_t k = 0; k < J.tile_count; ++k) {
+      const DevTile& t = tiles[J.first_tile + k];
       if (x < t.x || y < t.y || x - t.x >= t.w || y - t.y >= t.h) continue;
       const uint32_t p = t.path_begin + (y - t.y) * t.w + (x - t.x);
-      const float r = P.rad[0][p], g = P.rad[1][p], b = P.rad[2][p];
-      if (!(finite(r) && finite(g) && finite(b))) continue;
+      const float r = P.rad[0][p], gg = P.rad[1][p], b = P.rad[2][p];
+      if (!(finite(r) && finite(gg) && finite(b))) continue;
       if (!touched) {
         s = film.sum[px];
         cnt = film.count[px];
         touched = true;
       }
       s.x = s.x + r;
-      s.y = s.y + g;
+      s.y = s.y + gg;
       s.z = s.z + b;
       ++cnt;
       ++added;
@@ -1418,7 +1454,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_rays(DevPaths P, DevScene sc, 
   TravStats st = {0, 0, 0, 0, 0};
   auto begin = [&](uint32_t j, Trav& s) {
     const float* r = rays + 6u * static_cast<size_t>(j);
-    return trav_begin(sc, s, mk(r[0], r[1], r[2]), mk(r[3], r[4], r[5]), tmax ? tmax[j] : kInf, st);
+    return trav_begin(sc.recs[0], s, mk(r[0], r[1], r[2]), mk(r[3], r[4], r[5]), tmax ? tmax[j] : kInf, st);
   };
   auto end = [&](uint32_t j, Trav& s) {
     out_t[j] = s.best_prim >= 0 ? s.best_t : kInf;
@@ -1448,18 +1484,18 @@ void launch_clear_film(hipStream_t stream, const DevFilm& film, uint32_t pixels)
   hipLaunchKernelGGL(k_clear_film, dim3((pixels + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, film, pixels);
 }
 
-void launch_raygen(hipStream_t stream, const DevPaths& paths, const DevCamera& cam, const DevSettings& st,
-                   const DevTileTable& tiles, uint32_t path_count) {
+void launch_raygen(hipStream_t stream, const DevPaths& paths, const DevScene& scene, const DevSettings& st,
+                   const DevTile* tiles, uint32_t tile_count, uint32_t path_count) {
   if (path_count == 0) return;
   hipLaunchKernelGGL(k_raygen<false>, dim3((path_count + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, paths,
-                     paths.rays[0], cam, st, tiles, path_count, DevAdjoint{});
+                     paths.rays[0], scene, st, tiles, tile_count, path_count, DevAdjoint{});
 }
 
-void launch_raygen_adjoint(hipStream_t stream, const DevPaths& paths, const DevAdjoint& adj, const DevCamera& cam,
-                           const DevSettings& st, const DevTileTable& tiles, uint32_t path_count) {
+void launch_raygen_adjoint(hipStream_t stream, const DevPaths& paths, const DevAdjoint& adj, const DevScene& scene,
+                           const DevSettings& st, const DevTile* tiles, uint32_t tile_count, uint32_t path_count) {
   if (path_count == 0) return;
   hipLaunchKernelGGL(k_raygen<true>, dim3((path_count + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, paths,
-                     paths.rays[0], cam, st, tiles, path_count, adj);
+                     paths.rays[0], scene, st, tiles, tile_count, path_count, adj);
 }
 
 void launch_copy_radiance(hipStream_t stream, const DevPaths& paths, const DevAdjoint& adj, uint32_t path_count) {
@@ -1493,13 +1529,11 @@ void launch_shadow_adjoint(hipStream_t stream, const DevPaths& paths, const DevA
   hipLaunchKernelGGL(k_shadow<true>, dim3(grid), dim3(kBlock), 0, stream, paths, scene, adj);
 }
 
-void launch_film(hipStream_t stream, const DevPaths& paths, const DevFilm& film, uint32_t film_width,
-                 const DevTileTable& tiles, const uint32_t rect[4]) {
-  if (rect[2] <= rect[0] || rect[3] <= rect[1]) return;
-  const uint32_t rw = rect[2] - rect[0], rh = rect[3] - rect[1];
-  const uint32_t n = rw * rh;
-  hipLaunchKernelGGL(k_film, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, paths, film, film_width, tiles,
-                     rect[0], rect[1], rw, rh);
+void launch_film(hipStream_t stream, const DevPaths& paths, const DevScene& scene, const DevFilm& film,
+                 const DevTile* tiles, const FilmJob* jobs, uint32_t job_count, uint32_t job_pixels) {
+  if (job_pixels == 0) return;
+  hipLaunchKernelGGL(k_film, dim3((job_pixels + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, paths, scene, film,
+                     tiles, jobs, job_count, job_pixels);
 }
 
 void launch_trace_rays(hipStream_t stream, const DevPaths& paths, const DevScene& scene, int any_hit,

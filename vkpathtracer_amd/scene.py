@@ -496,3 +496,33 @@ def random_rays(scene: Scene, n: int, seed: int = 0) -> np.ndarray:
     d[k == 1, 1] = 0.0
     d /= np.linalg.norm(d, axis=1, keepdims=True)
     return np.concatenate([o, d], 1).astype(np.float32)
+
+
+def batch_scenes(seed: int = 7, count: int = 64, min_tris: int = 1_000, max_tris: int = 500_000,
+                 resolutions=(128, 256, 512, 1024), spps=(4, 16, 64)):
+    """`count` independent seeded scenes for batched rendering: (scene, width, height, spp) each.
+
+    Triangle counts are log-uniform in [min_tris, max_tris]: a displaced icosphere takes the largest
+    20*4^L (bisected: 40*4^L) face count not above 2/3 of the target, 168-triangle instanced spheres
+    and 8 quad lights fill the rest (the recipe of `procedural_scene`). Square resolutions and spp are
+    uniform over the given sets.
+    """
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        target = int(round(math.exp(rng.uniform(math.log(min_tris), math.log(max_tris)))))
+        level, bisect = 0, False
+        for lv in range(8):
+            for bi in (False, True):
+                n = 20 * 4 ** lv * (2 if bi else 1)
+                if n <= (2 * target) // 3:
+                    level, bisect = lv, bi
+        ico = 20 * 4 ** level * (2 if bisect else 1)
+        lights = 8
+        spheres = max(0, (target - ico - 2 * lights) // 168)
+        sc = procedural_scene(seed=seed * 1000 + i, ico_subdivisions=level, bisect=bisect, sphere_count=spheres,
+                              light_count=lights)
+        res = int(rng.choice(resolutions))
+        spp = int(rng.choice(spps))
+        out.append((dataclasses.replace(sc, name=f"batch-{seed}-{i}"), res, res, spp))
+    return out

@@ -267,3 +267,110 @@ class HipPathTracer:
                                          n, out_t.data_ptr(), out_p.data_ptr(), C.byref(c))
         self._check(rc, "trace_rays")
         return out_t.cpu().numpy(), out_p.cpu().numpy(), c.as_dict()
+
+
+class HipBatch:
+    """Many scenes rendered through one scene pool and one wavefront launch sequence per frame
+    (pt_batch_*; no counterpart in the reference). Integrator settings are shared; each scene has
+    its own image size, spp and BVH parameters."""
+
+    def __init__(self, settings: RenderSettings, device: int = 0, library=None):
+        self._lib = abi.load_library(library)
+        h = C.c_void_p()
+        s = abi.make_settings(settings)
+        rc = self._lib.pt_batch_create(device, C.byref(s), C.byref(h))
+        if rc != 0:
+            raise PathTracerError(Status(rc, "no HIP device available for the batch"), "pt_batch_create")
+        self._h = h
+        self.settings = settings
+        self.scenes: list[tuple[int, int, int]] = []  # (width, height, spp)
+        self.device = device
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.pt_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc: int, what: str):
+        if rc != 0:
+            msg = (self._lib.pt_batch_last_error(self._h) or b"").decode()
+            raise PathTracerError(Status(rc, msg), what)
+
+    def add_scene(self, scene: Scene, width: int, height: int, spp: int, builder: str = "sah", leaf_size: int = 4,
+                  bucket_count: int = 8) -> int:
+        view = abi.SceneView(scene)
+        p = abi.PtBvhParams({"sah": 0, "midpoint": 1}[builder], leaf_size, bucket_count, 0)
+        idx = C.c_uint32()
+        self._check(self._lib.pt_batch_add_scene(self._h, C.byref(view.view), width, height, spp, C.byref(p),
+                                                 C.byref(idx)), "add_scene")
+        self.scenes.append((width, height, spp))
+        return int(idx.value)
+
+    def build(self) -> dict:
+        info = abi.PtBuildInfo()
+        self._check(self._lib.pt_batch_build(self._h, C.byref(info)), "build")
+        self.build_info = info.as_dict()
+        return self.build_info
+
+    @property
+    def paths_per_frame(self) -> int:
+        return sum(w * h * s for w, h, s in self.scenes)
+
+    def reset_accumulation(self):
+        self._check(self._lib.pt_batch_reset_accumulation(self._h), "reset_accumulation")
+
+    def render(self, frame_index: int = 0, use_graph: bool = False):
+        self._check(self._lib.pt_batch_render(self._h, frame_index, int(use_graph)), "render")
+
+    def synchronize(self):
+        self._check(self._lib.pt_batch_synchronize(self._h), "synchronize")
+
+    def film(self, scene: int) -> tuple[np.ndarray, np.ndarray]:
+        w, h, _ = self.scenes[scene]
+        rgb = np.zeros((h, w, 3), np.float32)
+        cnt = np.zeros((h, w), np.uint32)
+        self._check(self._lib.pt_batch_read_film(self._h, scene, rgb.ctypes.data, cnt.ctypes.data), "read_film")
+        return rgb, cnt
+
+    def read_counters(self) -> dict:
+        c = abi.PtSampleCounters()
+        self._check(self._lib.pt_batch_read_counters(self._h, C.byref(c)), "read_counters")
+        return c.as_dict()
+
+    def set_tuning(self, wavefront_capacity: int = 0, refill_min: int = 0, traversal_blocks: int = 0,
+                   shade_blocks: int = 0):
+        tu = abi.PtTuning(int(wavefront_capacity), int(refill_min), int(traversal_blocks), int(shade_blocks))
+        self._check(self._lib.pt_batch_set_tuning(self._h, C.byref(tu)), "set_tuning")
+
+    def tuning(self) -> dict:
+        tu = abi.PtTuning()
+        self._check(self._lib.pt_batch_get_tuning(self._h, C.byref(tu)), "get_tuning")
+        return tu.as_dict()
+
+    def set_profiling(self, enabled: bool):
+        self._check(self._lib.pt_batch_set_profiling(self._h, int(enabled)), "set_profiling")
+
+    def reset_kernel_stats(self):
+        self._check(self._lib.pt_batch_reset_kernel_stats(self._h), "reset_kernel_stats")
+
+    def kernel_stats(self) -> dict:
+        k = abi.PtKernelStats()
+        self._check(self._lib.pt_batch_read_kernel_stats(self._h, C.byref(k)), "read_kernel_stats")
+        out = {name: {"total_ms": k.total_ms[i], "launches": int(k.launches[i]), "items": int(k.items[i])}
+               for i, name in enumerate(abi.KERNELS)}
+        for name in ("inner_node_fetches", "shadow_node_visits", "shadow_triangle_tests", "shadow_inner_node_fetches",
+                     "shadow_leaf_visits", "shadow_rays"):
+            out[name] = int(getattr(k, name))
+        return out
