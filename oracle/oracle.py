@@ -246,12 +246,13 @@ class Oracle:
 
 
 _SIGS_ADJ = (C.c_int, [C.POINTER(PtoScene), _P, C.POINTER(PtoSettings), _P, C.c_uint32, C.c_uint32, C.c_uint64, C.c_int,
-                       C.c_int, _P, _P, _P, _P, _P])
+                       C.c_int, _P, _P, _P, _P, _P, _P])
 
 
-def adjoint_pixels(orc: Oracle, settings, pixels, sample_index, delta, frame_index=0, threads=0,
-                   shadow=SHADOW_ANYHIT):
-    """Path-replay adjoint (pto_adjoint_pixels): returns (g_albedo (M,3), g_emission (M,3), g_light (Lt,3), rgb)."""
+def adjoint_gradients(orc: Oracle, settings, pixels, sample_index, delta, frame_index=0, threads=0,
+                      shadow=SHADOW_ANYHIT) -> dict:
+    """Path-replay adjoint (pto_adjoint_pixels) of one sample per listed pixel: a dict of fp64
+    gradients albedo (M,3), emission (M,3), light (Lt,3), roughness (M,) and the forward rgb."""
     fn = orc.L.pto_adjoint_pixels
     fn.restype, fn.argtypes = _SIGS_ADJ
     px = np.ascontiguousarray(pixels, np.uint32)
@@ -260,10 +261,32 @@ def adjoint_pixels(orc: Oracle, settings, pixels, sample_index, delta, frame_ind
     ga = np.zeros((nm, 3), np.float64)
     ge = np.zeros((nm, 3), np.float64)
     gl = np.zeros((max(nl, 1), 3), np.float64)
+    gr = np.zeros(max(nm, 1), np.float64)
     rgb = np.zeros((len(px), 3), orc.real)
     st = settings_struct(settings)
     rc = fn(C.byref(orc.scene.s), orc.accel, C.byref(st), px.ctypes.data, len(px), sample_index, frame_index, threads,
-            shadow, d.ctypes.data, ga.ctypes.data, ge.ctypes.data, gl.ctypes.data, rgb.ctypes.data)
+            shadow, d.ctypes.data, ga.ctypes.data, ge.ctypes.data, gl.ctypes.data, gr.ctypes.data, rgb.ctypes.data)
     if rc != 0:
         raise RuntimeError(f"oracle adjoint failed ({rc})")
-    return ga, ge, gl[:nl], rgb
+    return {"albedo": ga, "emission": ge, "light": gl[:nl], "roughness": gr[:nm], "rgb": rgb}
+
+
+def adjoint_pixels(orc: Oracle, settings, pixels, sample_index, delta, frame_index=0, threads=0,
+                   shadow=SHADOW_ANYHIT):
+    """(g_albedo (M,3), g_emission (M,3), g_light (Lt,3), rgb) of adjoint_gradients."""
+    g = adjoint_gradients(orc, settings, pixels, sample_index, delta, frame_index, threads, shadow)
+    return g["albedo"], g["emission"], g["light"], g["rgb"]
+
+
+def phong_samples(exponent: float, refl, normal, ps: float, state: int, count: int, variant: str = "f64"):
+    """pto_phong_samples: (directions (count, 3) float32, score (count,) float64)."""
+    L = lib(variant)
+    fn = L.pto_phong_samples
+    fn.restype = C.c_int
+    fn.argtypes = [C.c_float, _P, _P, C.c_float, C.c_uint64, C.c_uint32, _P, _P]
+    r = np.ascontiguousarray(refl, np.float32)
+    n = np.ascontiguousarray(normal, np.float32)
+    d = np.zeros((count, 3), np.float32)
+    sc = np.zeros(count, np.float64)
+    fn(exponent, r.ctypes.data, n.ctypes.data, ps, state & 0xFFFFFFFFFFFFFFFF, count, d.ctypes.data, sc.ctypes.data)
+    return d, sc

@@ -955,7 +955,7 @@ static inline real pow5(real v) { const real v2 = v * v; return v2 * v2 * v; }
 typedef struct {
   v3 radiance;
   int valid;
-  v3 d_albedo, d_color;
+  v3 d_albedo, d_color, d_rough;  /* d_rough: d radiance / d(material roughness parameter) */
   uint32_t light;
   real eff_k;  /* d(effect colour)/d(albedo) before the clamp, and that pre-clamp colour */
   v3 eff_pre;
@@ -1044,15 +1044,29 @@ static nee_t sample_direct_light(const ctx_t* x, const hit_t* hit, v3 view_dir, 
     real f0 = (RC(1.0) - ior) / (RC(1.0) + ior);
     f0 *= f0;
     const real fresnel = f0 + (RC(1.0) - f0) * pow5(RC(1.0) - cosView);
-    const real specStrength = clamp01((RC(1.0) - roughness) * RC(0.8) + (real)m->metallic * RC(0.45) +
-                                      (real)m->clearcoat * RC(0.35) + (isMirror ? RC(0.75) : RC(0.0)) +
-                                      (isTransmissive ? fresnel : RC(0.0)));
+    const real ss_raw = (RC(1.0) - roughness) * RC(0.8) + (real)m->metallic * RC(0.45) +
+                        (real)m->clearcoat * RC(0.35) + (isMirror ? RC(0.75) : RC(0.0)) +
+                        (isTransmissive ? fresnel : RC(0.0));
+    const real specStrength = clamp01(ss_raw);
     const v3 white = mk(RC(1.0), RC(1.0), RC(1.0));
     const v3 specTint = add(muls(white, isMetallic ? RC(0.15) : RC(0.88)), muls(albedo, isMetallic ? RC(0.85) : RC(0.12)));
     const real s = spec * cos_theta * nlf * smax(RC(0.15), specStrength);
     direct = add(direct, muls(mulv(specTint, irradiance), s));
     r.d_albedo = add(r.d_albedo, muls(irradiance, (isMetallic ? RC(0.85) : RC(0.12)) * s));
     r.d_color = add(r.d_color, muls(specTint, kscale * s));
+    /* d s / d roughness: through the Blinn-Phong exponent (2 / effR^4 - 2, unclamped in (4, 96))
+     * and through specStrength (unclamped in (0, 1) and above the 0.15 floor); times the clamp01
+     * derivative of the roughness parameter. */
+    const double nh = (double)smax(RC(0.0), dot(hit->normal, half_dir));
+    const double raw = 2.0 / fmax(0.0005, (double)a2 * (double)a2) - 2.0;
+    const double kM = isMetallic ? 0.65 : 1.0;
+    double dspec = 0.0;
+    if (nh > 0.0 && raw > 4.0 && raw < 96.0 && (double)roughness * kM > 0.025)
+      dspec = (double)spec * log(nh) * (-8.0 / pow((double)effR, 5.0)) * kM;
+    double ds = dspec * (double)smax(RC(0.15), specStrength);
+    if (specStrength > RC(0.15) && ss_raw > RC(0.0) && ss_raw < RC(1.0)) ds += (double)spec * -0.8;
+    ds *= (double)cos_theta * (double)nlf * ((m->roughness > 0.0f && m->roughness < 1.0f) ? 1.0 : 0.0);
+    r.d_rough = muls(mulv(specTint, irradiance), (real)ds);
   }
   r.radiance = direct;
   r.valid = 1;
@@ -1185,10 +1199,11 @@ static v3 environment(const PtoScene* sc, v3 d) {
 typedef struct {
   v3 L_total;
   double delta[3];
-  double *g_albedo, *g_emission, *g_light;
+  double *g_albedo, *g_emission, *g_light, *g_rough;
   int pending;
   uint32_t p_mat;
   double p_dx[3];
+  double p_score;  /* d log(density of the sampled direction) / d roughness of the pending bounce */
   int p_active, p_cstar;
   int nhist;
   uint32_t h_mat[PTO_ADJ_HIST];
@@ -1205,6 +1220,7 @@ static void adj_flush_pending(adj_t* a, v3 radiance) {
     S += a->delta[c] * R[c];
     a->g_albedo[3u * a->p_mat + c] += a->delta[c] * R[c] * a->p_dx[c];
   }
+  a->g_rough[a->p_mat] += S * a->p_score;
   if (a->p_active) {
     for (int h = 0; h < a->nhist; ++h) a->g_albedo[3u * a->h_mat[h] + a->p_cstar] += -S * a->h_dx[h][a->p_cstar];
     if (a->prev_active)
@@ -1220,6 +1236,63 @@ static void adj_flush_pending(adj_t* a, v3 radiance) {
 /* d(clamped effect colour)/d(albedo): k inside the (0, 1.5) clamp, 0 on or outside it. */
 static inline double eff_grad(real pre_c, real k) {
   return (pre_c > RC(0.0) && pre_c < RC(1.5)) ? (double)k : 0.0;
+}
+
+/* d log p(w) / d e for the direction the glossy branch returned: p is the density of
+ * sample_phong_lobe's output (the exponent-e lobe around refl, plus the lobe samples that fell
+ * below the surface and were mirrored into w) mixed with the sheen branch's cosine hemisphere of
+ * weight ps: p(w) = (1 - ps) [q(w) + q(w')] + ps cos_n / pi, q = (e+1)/(2 pi) max(0, w.refl)^e,
+ * w' = w mirrored through the surface plane. The hemisphere fallback for a mirrored sample still
+ * below the surface has probability zero. Evaluated in the log domain (e reaches 5e6). */
+static double phong_score(double e, v3 refl, v3 n, v3 w, double ps) {
+  const double wn = (double)dot(w, n);
+  const v3 wm = sub(w, muls(n, RC(2.0) * dot(w, n)));
+  const double c[2] = {(double)dot(w, refl), (double)dot(wm, refl)};
+  double l[2], a[2];
+  int any = 0;
+  double L = -INFINITY;
+  for (int i = 0; i < 2; ++i) {
+    if (c[i] > 0.0) {
+      l[i] = e * log(c[i]);
+      a[i] = 1.0 / (e + 1.0) + log(c[i]);
+      if (l[i] > L) L = l[i];
+      any = 1;
+    } else {
+      l[i] = -INFINITY;
+      a[i] = 0.0;
+    }
+  }
+  if (!any || ps >= 1.0) return 0.0;
+  double num = 0.0, den = 0.0;
+  for (int i = 0; i < 2; ++i)
+    if (c[i] > 0.0) {
+      const double wgt = exp(l[i] - L);
+      num += wgt * a[i];
+      den += wgt;
+    }
+  if (ps > 0.0 && wn > 0.0) {
+    const double lx = log(ps * wn / 3.14159265358979323846) - log((1.0 - ps) * (e + 1.0) / (2.0 * 3.14159265358979323846)) - L;
+    den += exp(fmin(lx, 700.0));
+  }
+  return num / den;
+}
+
+/* The glossy branch's direction sampling alone (sample_phong_lobe, then the sheen branch's
+ * hemisphere with probability ps), `count` times from one RNG stream, with each direction's
+ * phong_score. For pinning the score: its mean over the sampler's own directions must vanish and
+ * its covariance with a smooth function must equal that function's mean's derivative in e. */
+int pto_phong_samples(float exponent, const float refl_in[3], const float n_in[3], float ps, uint64_t state,
+                      uint32_t count, float* out_dirs, double* out_score) {
+  const v3 refl = v3f(refl_in), n = v3f(n_in);
+  for (uint32_t i = 0; i < count; ++i) {
+    v3 w = sample_phong_lobe(&state, refl, (real)exponent, n);
+    if (ps > 0.0f && next01(&state) < (real)ps) w = sample_hemisphere(&state, n);
+    out_dirs[3 * i] = (float)w.x;
+    out_dirs[3 * i + 1] = (float)w.y;
+    out_dirs[3 * i + 2] = (float)w.z;
+    out_score[i] = phong_score((double)exponent, refl, n, w, (double)ps);
+  }
+  return 0;
 }
 
 /* trace: ScalarCpuPathTracer.cpp:1148-1289; adj != NULL replays the path for the adjoint. */
@@ -1300,12 +1373,16 @@ static v3 trace(const ctx_t* x, v3 ro, v3 rd, uint64_t* rng, uint64_t* rays, Pto
             adj->g_albedo[3u * mat_index + ch] += tw * (double)da[ch] * eff_grad(pre[ch], nee.eff_k);
             adj->g_light[3u * nee.light + ch] += tw * (double)dc[ch];
           }
+          const real dr3[3] = {nee.d_rough.x, nee.d_rough.y, nee.d_rough.z};
+          for (int ch = 0; ch < 3; ++ch)
+            adj->g_rough[mat_index] += adj->delta[ch] * (double)t3[ch] * (double)w * (double)dr3[ch];
         }
       }
     }
 
     v3 outDir;
     int refracted = 0;
+    double score = 0.0;
     if (isMirror) {
       outDir = reflect_about(rd, sN);
     } else if (isTransmissive) {
@@ -1330,6 +1407,13 @@ static v3 trace(const ctx_t* x, v3 ro, v3 rd, uint64_t* rng, uint64_t* rays, Pto
       const v3 refl = reflect_about(rd, sN);
       outDir = sample_phong_lobe(rng, refl, expt, sN);
       if (isSheen && next01(rng) < (real)m->sheen * RC(0.35)) outDir = sample_hemisphere(rng, sN);
+      if (adj) {
+        const double kk = (isMetallic ? 0.75 : 1.0) * (isClearcoat ? 0.65 : 1.0);
+        const double ps = isSheen ? fmin(1.0, fmax(0.0, (double)((real)m->sheen * RC(0.35)))) : 0.0;
+        const double drp = (m->roughness > 0.0f && m->roughness < 1.0f) ? 1.0 : 0.0;
+        if (drp != 0.0 && (double)roughness * kk > 0.025 && expt > RC(0.0))
+          score = phong_score((double)expt, refl, sN, outDir, ps) * (-8.0 / pow((double)effR, 5.0)) * kk;
+      }
     }
 
     if (!isTransmissive && dot(outDir, sN) <= RC(0.0)) break;
@@ -1358,6 +1442,7 @@ static v3 trace(const ctx_t* x, v3 ro, v3 rd, uint64_t* rng, uint64_t* rays, Pto
       if (isTransmissive && !(isMetallic || isMirror)) tint = clamp01((real)m->alpha) * (real)m->transmission;
       adj->pending = 1;
       adj->p_mat = mat_index;
+      adj->p_score = score;
       const real pre3[3] = {eff_pre.x, eff_pre.y, eff_pre.z};
       for (int ch = 0; ch < 3; ++ch) {
         double dx = 0.0;
@@ -1580,12 +1665,13 @@ int pto_render_tile(const PtoScene* scene, const pto_accel* a, const PtoSettings
 /* Path-replay adjoint for one sample of each listed pixel. delta: 3 floats per listed pixel,
  * dLoss / d(the path's radiance) (= dLoss / d film sum at that pixel). The forward radiance is
  * recomputed first (nothing is stored between the passes). Gradients are accumulated in double:
- * g_albedo, g_emission (3 per material), g_light (3 per light colour); out_rgb (may be NULL)
+ * g_albedo, g_emission (3 per material), g_light (3 per light colour), g_rough (1 per material,
+ * the roughness parameter; may be NULL); out_rgb (may be NULL)
  * receives the forward radiance. Paths with non-finite radiance contribute nothing. */
 int pto_adjoint_pixels(const PtoScene* scene, const pto_accel* a, const PtoSettings* s,
                        const uint32_t* pixels, uint32_t n, uint32_t sample_index, uint64_t frame_index,
                        int threads, int shadow_mode, const float* delta, double* g_albedo, double* g_emission,
-                       double* g_light, void* out_rgb) {
+                       double* g_light, double* g_rough, void* out_rgb) {
   if (!scene || !a || !s || s->width == 0 || s->height == 0) return -1;
   const int sup = check_supported(scene);
   if (sup) return sup;
@@ -1605,6 +1691,7 @@ int pto_adjoint_pixels(const PtoScene* scene, const pto_accel* a, const PtoSetti
     double* ga = (double*)calloc(3u * (nm ? nm : 1u), sizeof(double));
     double* ge = (double*)calloc(3u * (nm ? nm : 1u), sizeof(double));
     double* gl = (double*)calloc(3u * (nl ? nl : 1u), sizeof(double));
+    double* gr = (double*)calloc(nm ? nm : 1u, sizeof(double));
     PtoCounters lc;
     memset(&lc, 0, sizeof(lc));
 #pragma omp for schedule(dynamic, 64)
@@ -1623,6 +1710,7 @@ int pto_adjoint_pixels(const PtoScene* scene, const pto_accel* a, const PtoSetti
       adj.g_albedo = ga;
       adj.g_emission = ge;
       adj.g_light = gl;
+      adj.g_rough = gr;
       v3 L2;
       shade_pixel(&x, &cam, pixels[i], sample_index, frame_index, &L2, &rays, &lc, &adj);
     }
@@ -1630,7 +1718,10 @@ int pto_adjoint_pixels(const PtoScene* scene, const pto_accel* a, const PtoSetti
     {
       for (uint32_t k = 0; k < 3u * nm; ++k) { g_albedo[k] += ga[k]; g_emission[k] += ge[k]; }
       for (uint32_t k = 0; k < 3u * nl; ++k) g_light[k] += gl[k];
+      if (g_rough)
+        for (uint32_t k = 0; k < nm; ++k) g_rough[k] += gr[k];
     }
+    free(gr);
     free(ga);
     free(ge);
     free(gl);

@@ -149,7 +149,11 @@ int pto_render_tile(const PtoScene* scene, const pto_accel* a, const PtoSettings
 int pto_adjoint_pixels(const PtoScene* scene, const pto_accel* a, const PtoSettings* s,
                        const uint32_t* pixels, uint32_t n, uint32_t sample_index, uint64_t frame_index,
                        int threads, int shadow_mode, const float* delta, double* g_albedo, double* g_emission,
-                       double* g_light, void* out_rgb);
+                       double* g_light, double* g_rough, void* out_rgb);
+
+/* Glossy-branch direction samples and their d log(density)/d exponent (see pto.c). */
+int pto_phong_samples(float exponent, const float refl[3], const float n[3], float ps, uint64_t state,
+                      uint32_t count, float* out_dirs, double* out_score);
 
 #ifdef __cplusplus
 }

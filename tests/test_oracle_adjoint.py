@@ -100,3 +100,87 @@ def test_adjoint_roulette_depth_consistent_in_expectation():
     A, F = np.array(A), np.array(F)
     se = np.sqrt(A.var() / len(A) + F.var() / len(F))
     assert abs(A.mean() - F.mean()) < 4 * se, (A.mean(), F.mean(), se)
+
+
+def _glossy_box(seed=3):
+    """The box with every surface glossy (model 3, roughness in [0.2, 0.6], so the Blinn-Phong NEE
+    term is on), metals and clearcoats among them, and the emissive quad dimmed to black: at depth 2
+    all radiance is depth-0 emission + NEE, a smooth function of roughness."""
+    sc = _sc()
+    m = sc.materials.copy()
+    rng = np.random.default_rng(seed)
+    for i in range(len(m)):
+        if m[i]["material_model"] == S.MODEL_EMISSIVE:
+            m[i]["emissive"] = (0.0, 0.0, 0.0)
+            m[i]["material_model"] = S.MODEL_LAMBERT
+            continue
+        m[i]["material_model"] = S.MODEL_GLOSSY
+        m[i]["roughness"] = rng.uniform(0.2, 0.6)
+        m[i]["metallic"] = 0.8 if i % 5 == 0 else 0.0
+        m[i]["clearcoat"] = 0.3 if i % 7 == 1 else 0.0
+    return dataclasses.replace(sc, materials=m)
+
+
+def test_roughness_nee_term_equals_central_differences():
+    """max_depth 2, no emitters: the radiance is the depth-0 NEE term (the bounce direction reaches
+    nothing that emits), deterministic in roughness; the adjoint's d/d roughness must equal the
+    central difference of the fp64 render."""
+    sc = _glossy_box()
+    st = S.RenderSettings(width=24, height=24, spp=1, max_depth=2)
+    px = np.arange(st.width * st.height, dtype=np.uint32)
+    delta = np.random.default_rng(1).normal(size=(len(px), 3)).astype(np.float32)
+    spp = 3
+    o = O.Oracle(sc, variant="f64")
+    G = sum(O.adjoint_gradients(o, st, px, s, delta, threads=4)["roughness"] for s in range(spp))
+    assert np.abs(G).max() > 0
+    h = 1e-4
+    for idx in (0, 5, 8, 12, 21, 33):
+        vals = []
+        for sgn in (1.0, -1.0):
+            m = sc.materials.copy()
+            m[idx]["roughness"] = np.float32(m[idx]["roughness"] + sgn * h)
+            s1 = dataclasses.replace(sc, materials=m)
+            vals.append((_f(s1, st, px, delta, spp), float(m[idx]["roughness"])))
+        fd = (vals[0][0] - vals[1][0]) / (vals[0][1] - vals[1][1])
+        assert abs(G[idx] - fd) <= 2e-4 * max(1.0, abs(fd)), (idx, G[idx], fd)
+
+
+def _normalize(v):
+    v = np.asarray(v, np.float64)
+    return (v / np.linalg.norm(v)).astype(np.float32)
+
+
+# (exponent, reflection direction, normal, sheen probability): narrow and wide lobes, a reflection
+# near grazing (many samples fall below the surface and are mirrored back), and a sheen mixture
+LOBES = [(2.88, _normalize([0.3, 0.8, 0.1]), _normalize([0, 1, 0]), 0.0),
+         (40.0, _normalize([0.9, 0.15, -0.2]), _normalize([0, 1, 0]), 0.0),
+         (1.2, _normalize([0.99, 0.05, 0.0]), _normalize([0, 1, 0]), 0.0),
+         (6.0, _normalize([0.2, 0.5, 0.8]), _normalize([0.1, 0.9, 0.2]), 0.3)]
+
+
+@pytest.mark.parametrize("e,refl,n,ps", LOBES)
+def test_phong_score_has_zero_mean_under_the_sampler(e, refl, n, ps):
+    """E[d log p / de] = 0 when p is the density the sampler actually draws from: a density that
+    missed the mirrored samples or the sheen mixture would fail this."""
+    _, score = O.phong_samples(e, refl, n, ps, 12345, 400_000)
+    se = score.std() / np.sqrt(len(score))
+    assert abs(score.mean()) <= 4 * se, (score.mean(), se)
+
+
+@pytest.mark.parametrize("e,refl,n,ps", LOBES)
+def test_phong_score_gives_the_derivative_of_expectations(e, refl, n, ps):
+    """E[score * f] = d E[f] / de for smooth f of the direction; the right side by a central
+    difference in e over the same random numbers (the sampled direction is continuous in e)."""
+    a = _normalize([0.4, 0.7, -0.6])
+    f = lambda d: (d.astype(np.float64) @ a.astype(np.float64)) ** 2 + d[:, 1].astype(np.float64)
+    n_s = 400_000
+    d, score = O.phong_samples(e, refl, n, ps, 777, n_s)
+    lhs = f(d) * score
+    h = 1e-3 * e
+    dp, _ = O.phong_samples(e + h, refl, n, ps, 777, n_s)
+    dm, _ = O.phong_samples(e - h, refl, n, ps, 777, n_s)
+    rhs = (f(dp) - f(dm)) / (2 * h)
+    diff = lhs - rhs
+    se = diff.std() / np.sqrt(n_s)
+    assert abs(lhs.mean()) > 6 * se, "the test direction must carry signal"
+    assert abs(diff.mean()) <= 4 * se, (lhs.mean(), rhs.mean(), se)
