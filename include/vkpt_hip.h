@@ -224,11 +224,15 @@ int pt_hip_update_scene_delta(pt_tracer* tracer, const PtMaterial* materials, ui
  * differentiation). For the same tiles and frame, replays every path with the same random
  * numbers and accumulates, with fp32 atomics, the gradient of
  *   loss = sum over pixels p, channels c of film_grad[p][c] * (film sum contributed by these tiles)[p][c]
- * into grad_albedo and grad_emission (3 floats per material) and grad_light (3 floats per light
- * colour). All four are device pointers; gradients are added to the buffers' contents. The film
- * and the render counters are not modified. Requires rr_start_depth <= 4. */
+ * into grad_albedo and grad_emission (3 floats per material), grad_light (3 floats per light
+ * colour) and grad_roughness (1 float per material: the Blinn-Phong NEE term plus the score of the
+ * glossy lobe's sampling density; may be NULL). All are device pointers; gradients are added to the
+ * buffers' contents. The film and the render counters are not modified. Requires rr_start_depth <= 4.
+ * Right after render_tiles of the same tiles and frame (no scene change between), the forward pass
+ * is not replayed: each path's radiance from that render is reused. */
 int pt_hip_render_adjoint(pt_tracer* tracer, const PtRenderTile* tiles, uint32_t count, uint64_t frame_index,
-                          const float* film_grad, float* grad_albedo, float* grad_emission, float* grad_light);
+                          const float* film_grad, float* grad_albedo, float* grad_emission, float* grad_light,
+                          float* grad_roughness);
 
 /* Block until all work on the handle's stream is done. */
 int pt_hip_synchronize(pt_tracer* tracer);

@@ -121,7 +121,7 @@ SIGNATURES = {
     "pt_hip_reset_kernel_stats": (C.c_int, [_P]),
     "pt_bvh_build_host": (C.c_int, [C.POINTER(PtSceneView), C.POINTER(PtBvhParams), _P, _P, _P, _P, _P, _P, _P]),
     "pt_hip_update_scene_delta": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint32]),
-    "pt_hip_render_adjoint": (C.c_int, [_P, C.POINTER(PtRenderTile), C.c_uint32, C.c_uint64, _P, _P, _P, _P]),
+    "pt_hip_render_adjoint": (C.c_int, [_P, C.POINTER(PtRenderTile), C.c_uint32, C.c_uint64, _P, _P, _P, _P, _P]),
     "pt_batch_create": (C.c_int, [C.c_int, C.POINTER(PtRenderSettings), C.POINTER(C.c_void_p)]),
     "pt_batch_destroy": (None, [_P]),
     "pt_batch_last_error": (C.c_char_p, [_P]),

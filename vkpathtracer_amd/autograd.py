@@ -1,6 +1,7 @@
 """Differentiable rendering: a torch.autograd.Function over the forward render and its path-replay adjoint.
 
-Parameters are per-material albedo and emission (M, 3) and per-light colour (L, 3). The forward pass
+Parameters are per-material albedo and emission (M, 3), per-light colour (L, 3) and, optionally,
+per-material roughness (M,). The forward pass
 uploads them with IPathTracer::update_scene_delta, renders the frame (one full-frame tile per sample,
 render_tiles) and returns the HDR image (resolve_hdr: film sum * 1/count). The backward pass hands
 dLoss/d(film sum) = dLoss/d(image) / count to pt_hip_render_adjoint, which replays every path with the
@@ -18,10 +19,11 @@ from .tracer import HipPathTracer
 
 class _RenderFn(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, albedo, emission, light_color, renderer):
-        renderer.upload(albedo, emission, light_color)
+    def forward(ctx, albedo, emission, light_color, roughness, renderer):
+        renderer.upload(albedo, emission, light_color, roughness)
         image, count = renderer.render()
         ctx.renderer = renderer
+        ctx.has_roughness = roughness is not None
         ctx.save_for_backward(count)
         return image
 
@@ -35,8 +37,9 @@ class _RenderFn(torch.autograd.Function):
         ga = torch.zeros((r.n_materials, 3), dtype=torch.float32, device=dev)
         ge = torch.zeros((r.n_materials, 3), dtype=torch.float32, device=dev)
         gl = torch.zeros((max(r.n_lights, 1), 3), dtype=torch.float32, device=dev)
-        r.tracer.render_adjoint(r.tiles, film_grad, ga, ge, gl[: r.n_lights], r.frame_index)
-        return ga, ge, gl[: r.n_lights], None
+        gr = torch.zeros(r.n_materials, dtype=torch.float32, device=dev) if ctx.has_roughness else None
+        r.tracer.render_adjoint(r.tiles, film_grad, ga, ge, gl[: r.n_lights], r.frame_index, grad_roughness=gr)
+        return ga, ge, gl[: r.n_lights], gr, None
 
 
 class DifferentiableRenderer:
@@ -62,10 +65,12 @@ class DifferentiableRenderer:
         t = lambda a: torch.tensor(np.asarray(a, np.float32), device=self.device)
         return t(m["albedo"]), t(m["emissive"]), t(l["color"]) if len(l) else t(np.zeros((0, 3)))
 
-    def upload(self, albedo, emission, light_color):
+    def upload(self, albedo, emission, light_color, roughness=None):
         mats = self.scene.materials.copy()
         mats["albedo"] = albedo.detach().to("cpu", torch.float32).numpy()
         mats["emissive"] = emission.detach().to("cpu", torch.float32).numpy()
+        if roughness is not None:
+            mats["roughness"] = roughness.detach().to("cpu", torch.float32).numpy()
         lights = self.scene.lights.copy()
         if len(lights):
             lights["color"] = light_color.detach().to("cpu", torch.float32).numpy()
@@ -81,8 +86,8 @@ class DifferentiableRenderer:
         image = torch.from_numpy(rgb * inv).to(self.device)
         return image, count
 
-    def __call__(self, albedo, emission, light_color):
-        return _RenderFn.apply(albedo, emission, light_color, self)
+    def __call__(self, albedo, emission, light_color, roughness=None):
+        return _RenderFn.apply(albedo, emission, light_color, roughness, self)
 
     def close(self):
         self.tracer.close()

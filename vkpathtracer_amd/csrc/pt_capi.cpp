@@ -295,14 +295,14 @@ void free_paths(pt_tracer* t) {
   t->adj_capacity_alloc = 0;
 }
 
-// Adjoint state for `cap` paths: 27 float arrays and 8 u32 arrays (see ptd::DevAdjoint).
+// Adjoint state for `cap` paths: 29 float arrays and 8 u32 arrays (see ptd::DevAdjoint).
 int ensure_adjoint(pt_tracer* t, uint32_t cap, ptd::DevAdjoint& A) {
   if (cap > t->adj_capacity_alloc) {
     t->d_adj_f.release();
     t->d_adj_u.release();
     t->adj_capacity_alloc = 0;
     const size_t m = std::max<size_t>(cap, static_cast<size_t>(ptd::kSegments) * ptd::segment_capacity(cap));
-    PT_HIP(t, t->d_adj_f.ensure(static_cast<size_t>(27) * m), "allocate adjoint state");
+    PT_HIP(t, t->d_adj_f.ensure(static_cast<size_t>(29) * m), "allocate adjoint state");
     PT_HIP(t, t->d_adj_u.ensure(static_cast<size_t>(8) * m), "allocate adjoint state");
     t->adj_capacity_alloc = cap;
   }
@@ -318,6 +318,8 @@ int ensure_adjoint(pt_tracer* t, uint32_t cap, ptd::DevAdjoint& A) {
     A.s_glight[c] = f + (12 + c) * n;
     for (int h = 0; h < ptd::kAdjHist; ++h) A.h_dx[h][c] = f + (15 + 3 * h + c) * n;
   }
+  A.p_score = f + 27 * n;
+  A.s_grough = f + 28 * n;
   A.p_mat = u;
   A.flags = u + n;
   A.s_mat = u + 2 * n;
@@ -1290,7 +1292,8 @@ int pt_hip_update_scene_delta(pt_tracer* t, const PtMaterial* materials, uint32_
 }
 
 int pt_hip_render_adjoint(pt_tracer* t, const PtRenderTile* tiles, uint32_t count, uint64_t frame_index,
-                          const float* film_grad, float* grad_albedo, float* grad_emission, float* grad_light) {
+                          const float* film_grad, float* grad_albedo, float* grad_emission, float* grad_light,
+                          float* grad_roughness) {
   if (!t) return PT_INVALID_ARGUMENT;
   if (t->batch) return fail(t, PT_INVALID_ARGUMENT, "render_adjoint: not available on a batch");
   if (!t->configured || !t->scene_loaded || !t->accel_valid)
@@ -1309,11 +1312,12 @@ int pt_hip_render_adjoint(pt_tracer* t, const PtRenderTile* tiles, uint32_t coun
   A.g_albedo = grad_albedo;
   A.g_emission = grad_emission;
   A.g_light = grad_light;
+  A.g_rough = grad_roughness;
   A.film_grad = film_grad;
   A.film_width = t->settings.width;
   A.nmat = static_cast<uint32_t>(s.materials.size());
   A.nlight = static_cast<uint32_t>(s.lights.size());
-  A.use_lds = (6u * A.nmat + 3u * A.nlight) <= static_cast<uint32_t>(ptd::kAdjLdsFloats) ? 1u : 0u;
+  A.use_lds = (7u * A.nmat + 3u * A.nlight) <= static_cast<uint32_t>(ptd::kAdjLdsFloats) ? 1u : 0u;
   rc = run_passes(t, work, dev_settings(t, frame_index), &A);
   if (rc != PT_OK) return rc;
   if (t->profiling) {

@@ -177,9 +177,12 @@ struct DevAdjoint {
   uint32_t* s_mat;
   float* s_glight[3];
   uint32_t* s_light;
+  float* p_score;   // d log(density of the pending bounce's direction) / d roughness
+  float* s_grough;  // per shadow record: d contribution / d roughness, weighted by the loss adjoint
   float* g_albedo;    // 3 per material
   float* g_emission;  // 3 per material
   float* g_light;     // 3 per light (colour)
+  float* g_rough;     // 1 per material (nullptr: not requested)
   const float* film_grad;  // 3 per pixel: dLoss / d(film sum)
   uint32_t film_width, nmat, nlight, use_lds;
 };

@@ -570,12 +570,18 @@ __global__ __launch_bounds__(kBlock) void k_copy_radiance(DevPaths P, DevAdjoint
 }
 
 // ------------------------------------------------------------------ adjoint gradient accumulation
-// Gradient slots: albedo 3*nmat, emission 3*nmat, light colour 3*nlight. Staged in LDS per block
-// when they fit, else added straight to the global fp32 buffers.
-enum GradKind { kGradAlbedo = 0, kGradEmission = 1, kGradLight = 2 };
+// Gradient slots: albedo 3*nmat, emission 3*nmat, light colour 3*nlight, roughness nmat. Staged in
+// LDS per block when they fit, else added straight to the global fp32 buffers.
+enum GradKind { kGradAlbedo = 0, kGradEmission = 1, kGradLight = 2, kGradRough = 3 };
 
 __device__ __forceinline__ void gadd(const DevAdjoint& A, float* lds, GradKind kind, uint32_t idx, int c, float v) {
   if (v == 0.0f) return;
+  if (kind == kGradRough) {
+    if (!A.g_rough) return;
+    if (A.use_lds) atomicAdd(lds + 6u * A.nmat + 3u * A.nlight + idx, v);
+    else atomicAdd(A.g_rough + idx, v);
+    return;
+  }
   const uint32_t slot = 3u * idx + static_cast<uint32_t>(c);
   if (A.use_lds) {
     const uint32_t off = kind == kGradAlbedo ? 0u : (kind == kGradEmission ? 3u * A.nmat : 6u * A.nmat);
@@ -588,7 +594,7 @@ __device__ __forceinline__ void gadd(const DevAdjoint& A, float* lds, GradKind k
 
 __device__ __forceinline__ void grad_lds_clear(const DevAdjoint& A, float* lds) {
   if (!A.use_lds) return;
-  const uint32_t n = 6u * A.nmat + 3u * A.nlight;
+  const uint32_t n = 7u * A.nmat + 3u * A.nlight;
   for (uint32_t i = threadIdx.x; i < n; i += kBlock) lds[i] = 0.0f;
   __syncthreads();
 }
@@ -596,13 +602,14 @@ __device__ __forceinline__ void grad_lds_clear(const DevAdjoint& A, float* lds) 
 __device__ __forceinline__ void grad_lds_flush(const DevAdjoint& A, const float* lds) {
   if (!A.use_lds) return;
   __syncthreads();
-  const uint32_t na = 3u * A.nmat, n = 6u * A.nmat + 3u * A.nlight;
+  const uint32_t na = 3u * A.nmat, nl = 3u * A.nlight, n = 7u * A.nmat + 3u * A.nlight;
   for (uint32_t i = threadIdx.x; i < n; i += kBlock) {
     const float v = lds[i];
     if (v == 0.0f) continue;
     if (i < na) atomicAdd(A.g_albedo + i, v);
     else if (i < 2u * na) atomicAdd(A.g_emission + (i - na), v);
-    else atomicAdd(A.g_light + (i - 2u * na), v);
+    else if (i < 2u * na + nl) atomicAdd(A.g_light + (i - 2u * na), v);
+    else if (A.g_rough) atomicAdd(A.g_rough + (i - 2u * na - nl), v);
   }
 }
 
@@ -615,6 +622,7 @@ __device__ __forceinline__ void adj_flush(const DevAdjoint& A, float* lds, uint3
   const float S = dl[0] * R[0] + dl[1] * R[1] + dl[2] * R[2];
   const uint32_t m = A.p_mat[p];
   for (int c = 0; c < 3; ++c) gadd(A, lds, kGradAlbedo, m, c, dl[c] * R[c] * A.p_dx[c][p]);
+  gadd(A, lds, kGradRough, m, 0, S * A.p_score[p]);
   if (flags & 2u) {
     const int cs = static_cast<int>((flags >> 2) & 3u);
     const uint32_t nh = (flags >> 4) & 7u;
@@ -986,12 +994,53 @@ __device__ __forceinline__ float pow5(float v) {
   return v2 * v2 * v;
 }
 
+// d log p(w) / d e for the direction the glossy branch returned: p is the density of
+// sample_phong_lobe's output (the exponent-e lobe around refl plus the lobe samples that fell below
+// the surface and were mirrored into w) mixed with the sheen branch's cosine hemisphere of weight
+// ps: p(w) = (1 - ps) [q(w) + q(w')] + ps cos_n / pi, q = (e+1)/(2 pi) max(0, w.refl)^e, w' = w
+// mirrored through the surface plane (the fallback for a mirrored sample still below the surface
+// has probability zero). In the log domain, in double: e reaches 5e6.
+__device__ double phong_score(double e, V3 refl, V3 n, V3 w, double ps) {
+  const double wn = static_cast<double>(dot(w, n));
+  const V3 wm = sub(w, muls(n, 2.0f * dot(w, n)));
+  const double c[2] = {static_cast<double>(dot(w, refl)), static_cast<double>(dot(wm, refl))};
+  double l[2], a[2];
+  bool any = false;
+  double L = -__builtin_huge_val();
+  for (int i = 0; i < 2; ++i) {
+    if (c[i] > 0.0) {
+      l[i] = e * log(c[i]);
+      a[i] = 1.0 / (e + 1.0) + log(c[i]);
+      if (l[i] > L) L = l[i];
+      any = true;
+    } else {
+      l[i] = -__builtin_huge_val();
+      a[i] = 0.0;
+    }
+  }
+  if (!any || ps >= 1.0) return 0.0;
+  double num = 0.0, den = 0.0;
+  for (int i = 0; i < 2; ++i)
+    if (c[i] > 0.0) {
+      const double wgt = exp(l[i] - L);
+      num += wgt * a[i];
+      den += wgt;
+    }
+  if (ps > 0.0 && wn > 0.0) {
+    const double kPiD = 3.14159265358979323846;
+    const double lx = log(ps * wn / kPiD) - log((1.0 - ps) * (e + 1.0) / (2.0 * kPiD)) - L;
+    den += exp(fmin(lx, 700.0));
+  }
+  return num / den;
+}
+
 struct ShadeOut {
   bool cont, shadow;
   V3 no, nd;          // next extension ray
   V3 so, sd, contrib; // shadow ray and its radiance contribution if unoccluded
   float stmax;
   V3 galb, glight;    // adjoint: gradient pieces applied if the light is visible
+  float grough;
   uint32_t gmat, glight_index;
 };
 
@@ -1099,7 +1148,7 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
           const V3 irradiance = muls(ld3(light.color), kscale);
           const float nlf = static_cast<float>(nl);
           V3 direct = mk(0.0f, 0.0f, 0.0f);
-          V3 d_alb = mk(0.0f, 0.0f, 0.0f), d_col = mk(0.0f, 0.0f, 0.0f);  // adjoint partials
+          V3 d_alb = mk(0.0f, 0.0f, 0.0f), d_col = mk(0.0f, 0.0f, 0.0f), d_rough = mk(0.0f, 0.0f, 0.0f);  // adjoint partials
           if (!f.mirror && !f.transmissive) {
             direct = add(direct, muls(muls(mulv(muls(nalb, kInvPi), irradiance), cos_theta), nlf));
             if constexpr (kAdj) {
@@ -1117,8 +1166,9 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
             float f0 = (1.0f - m.ior) / (1.0f + m.ior);
             f0 *= f0;
             const float fresnel = f0 + (1.0f - f0) * pow5(1.0f - cosView);
-            const float specStrength = clamp01((1.0f - roughness) * 0.8f + m.metallic * 0.45f + m.clearcoat * 0.35f +
-                                               (f.mirror ? 0.75f : 0.0f) + (f.transmissive ? fresnel : 0.0f));
+            const float ss_raw = (1.0f - roughness) * 0.8f + m.metallic * 0.45f + m.clearcoat * 0.35f +
+                                 (f.mirror ? 0.75f : 0.0f) + (f.transmissive ? fresnel : 0.0f);
+            const float specStrength = clamp01(ss_raw);
             const V3 specTint = add(muls(mk(1.0f, 1.0f, 1.0f), f.metallic ? 0.15f : 0.88f),
                                     muls(nalb, f.metallic ? 0.85f : 0.12f));
             const float s = spec * cos_theta * nlf * smax(0.15f, specStrength);
@@ -1126,6 +1176,20 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
             if constexpr (kAdj) {
               d_alb = add(d_alb, muls(irradiance, (f.metallic ? 0.85f : 0.12f) * s));
               d_col = add(d_col, muls(specTint, kscale * s));
+              // d s / d roughness: through the Blinn-Phong exponent 2 / effR^4 - 2 (unclamped in
+              // (4, 96)) and through specStrength (unclamped in (0, 1), above the 0.15 floor), times
+              // the clamp01 derivative of the roughness parameter
+              const double nh = static_cast<double>(smax(0.0f, dot(sN, half_dir)));
+              const double raw = 2.0 / fmax(0.0005, static_cast<double>(a2) * static_cast<double>(a2)) - 2.0;
+              const double kM = f.metallic ? 0.65 : 1.0;
+              double dspec = 0.0;
+              if (nh > 0.0 && raw > 4.0 && raw < 96.0 && static_cast<double>(roughness) * kM > 0.025)
+                dspec = static_cast<double>(spec) * log(nh) * (-8.0 / pow(static_cast<double>(effR), 5.0)) * kM;
+              double ds = dspec * static_cast<double>(smax(0.15f, specStrength));
+              if (specStrength > 0.15f && ss_raw > 0.0f && ss_raw < 1.0f) ds += static_cast<double>(spec) * -0.8;
+              ds *= static_cast<double>(cos_theta) * static_cast<double>(nlf) *
+                    ((m.roughness > 0.0f && m.roughness < 1.0f) ? 1.0 : 0.0);
+              d_rough = muls(mulv(specTint, irradiance), static_cast<float>(ds));
             }
           }
           V3 contrib;
@@ -1141,11 +1205,13 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
             contrib = mulv(T, direct);
           }
           V3 galb = mk(0.0f, 0.0f, 0.0f), glight = mk(0.0f, 0.0f, 0.0f);
+          float grough = 0.0f;
           if constexpr (kAdj) {
             const V3 tw = muls(mulv(dl, T), w);
             galb = mk(tw.x * d_alb.x * eff_grad(npre.x, nk), tw.y * d_alb.y * eff_grad(npre.y, nk),
                       tw.z * d_alb.z * eff_grad(npre.z, nk));
             glight = mulv(tw, d_col);
+            grough = tw.x * d_rough.x + tw.y * d_rough.y + tw.z * d_rough.z;
           }
           const float remaining = dist - 0.004f;
           if (remaining > 0.004f) {
@@ -1156,6 +1222,7 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
             out.contrib = contrib;
             out.galb = galb;
             out.glight = glight;
+            out.grough = grough;
             out.gmat = mat_index;
             out.glight_index = light_index;
           } else {
@@ -1167,6 +1234,7 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
               gadd(A, glds, kGradLight, light_index, 0, glight.x);
               gadd(A, glds, kGradLight, light_index, 1, glight.y);
               gadd(A, glds, kGradLight, light_index, 2, glight.z);
+              gadd(A, glds, kGradRough, mat_index, 0, grough);
             }
           }
         }
@@ -1178,6 +1246,7 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
 
   V3 outDir;
   bool refracted = false;
+  float score = 0.0f;  // adjoint: d log(density of outDir) / d roughness (glossy branch only)
   if (f.mirror) {
     outDir = reflect_about(rd, sN);
   } else if (f.transmissive) {
@@ -1199,8 +1268,16 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
     const float effR = smax(0.025f, roughness * (f.metallic ? 0.75f : 1.0f) * (f.clearcoat ? 0.65f : 1.0f));
     const float a2 = effR * effR;
     const float expt = smax(0.0f, 2.0f / (a2 * a2) - 2.0f);
-    outDir = sample_phong_lobe(rng, reflect_about(rd, sN), expt, sN);
+    const V3 refl = reflect_about(rd, sN);
+    outDir = sample_phong_lobe(rng, refl, expt, sN);
     if (f.sheen && next01(rng) < m.sheen * 0.35f) outDir = sample_hemisphere(rng, sN);
+    if constexpr (kAdj) {
+      const double kk = (f.metallic ? 0.75 : 1.0) * (f.clearcoat ? 0.65 : 1.0);
+      const double ps = f.sheen ? fmin(1.0, fmax(0.0, static_cast<double>(m.sheen * 0.35f))) : 0.0;
+      if (m.roughness > 0.0f && m.roughness < 1.0f && static_cast<double>(roughness) * kk > 0.025 && expt > 0.0f)
+        score = static_cast<float>(phong_score(static_cast<double>(expt), refl, sN, outDir, ps) *
+                                   (-8.0 / pow(static_cast<double>(effR), 5.0)) * kk);
+    }
   }
   if (!f.transmissive && dot(outDir, sN) <= 0.0f) return;
   V3 bw = albedo;
@@ -1237,6 +1314,7 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
       A.p_dx[c][p] = dx[c];
     }
     A.p_mat[p] = mat_index;
+    A.p_score[p] = score;
     const bool active = depth < st.rr_start && st.rr_min < maxT && maxT < st.rr_max;
     aflags = (aflags & ~0xFu) | 1u | (active ? 2u : 0u) | (static_cast<uint32_t>(cstar) << 2);
     uint32_t nh = (aflags >> 4) & 7u;
@@ -1303,6 +1381,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevPaths P, DevRays in, DevRay
         A.s_glight[0][sslot] = so.glight.x; A.s_glight[1][sslot] = so.glight.y; A.s_glight[2][sslot] = so.glight.z;
         A.s_mat[sslot] = so.gmat;
         A.s_light[sslot] = so.glight_index;
+        A.s_grough[sslot] = so.grough;
       }
       ++shadow_rays;
     }
@@ -1328,6 +1407,7 @@ __device__ __forceinline__ void light_visible(const DevPaths& P, const DevAdjoin
       gadd(A, glds, kGradAlbedo, mi, c, A.s_galb[c][q]);
       gadd(A, glds, kGradLight, li, c, A.s_glight[c][q]);
     }
+    gadd(A, glds, kGradRough, mi, 0, A.s_grough[q]);
   }
 }
 

@@ -159,17 +159,20 @@ class HipPathTracer:
                                                  None if l is None else l.ctypes.data, 0 if l is None else len(l))
         return rc == 0
 
-    def render_adjoint(self, tiles, film_grad, grad_albedo, grad_emission, grad_light, frame_index: int = 0):
+    def render_adjoint(self, tiles, film_grad, grad_albedo, grad_emission, grad_light, frame_index: int = 0,
+                       grad_roughness=None):
         """Path-replay adjoint of render_tiles(tiles): adds d(sum film_grad * film)/d(parameters) into the
-        device tensors grad_albedo / grad_emission (M, 3) and grad_light (L, 3). film_grad: (H, W, 3)
-        device tensor. Torch tensors must be contiguous float32 on this tracer's device."""
+        device tensors grad_albedo / grad_emission (M, 3), grad_light (L, 3) and, when given,
+        grad_roughness (M,). film_grad: (H, W, 3) device tensor. Torch tensors must be contiguous
+        float32 on this tracer's device."""
         import torch
 
         torch.cuda.synchronize(self.device)
         arr = self._tile_array(tiles)
         rc = self._lib.pt_hip_render_adjoint(self._h, arr, len(tiles), frame_index, film_grad.data_ptr(),
                                              grad_albedo.data_ptr(), grad_emission.data_ptr(),
-                                             grad_light.data_ptr() if grad_light.numel() else None)
+                                             grad_light.data_ptr() if grad_light.numel() else None,
+                                             grad_roughness.data_ptr() if grad_roughness is not None else None)
         self._check(rc, "render_adjoint")
         self.synchronize()
 
