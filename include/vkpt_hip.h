@@ -225,14 +225,37 @@ int pt_hip_update_scene_delta(pt_tracer* tracer, const PtMaterial* materials, ui
  * numbers and accumulates, with fp32 atomics, the gradient of
  *   loss = sum over pixels p, channels c of film_grad[p][c] * (film sum contributed by these tiles)[p][c]
  * into grad_albedo and grad_emission (3 floats per material), grad_light (3 floats per light
- * colour) and grad_roughness (1 float per material: the Blinn-Phong NEE term plus the score of the
- * glossy lobe's sampling density; may be NULL). All are device pointers; gradients are added to the
- * buffers' contents. The film and the render counters are not modified. Requires rr_start_depth <= 4.
- * Right after render_tiles of the same tiles and frame (no scene change between), the forward pass
- * is not replayed: each path's radiance from that render is reused. */
+ * colour), grad_roughness (1 float per material: the Blinn-Phong NEE term plus the score of the
+ * glossy lobe's sampling density; may be NULL) and grad_texels (3 floats per texel of the loaded
+ * textures, in pt_hip_load_textures order; may be NULL). A textured material's albedo gradient is
+ * taken with the texture held fixed and each texel's with the albedo held fixed. All are device
+ * pointers; gradients are added to the buffers' contents. The film and the render counters are not
+ * modified. Requires rr_start_depth <= 4. Right after render_tiles of the same tiles and frame (no
+ * scene change between), the forward pass is not replayed: each path's radiance from that render
+ * is reused. */
 int pt_hip_render_adjoint(pt_tracer* tracer, const PtRenderTile* tiles, uint32_t count, uint64_t frame_index,
                           const float* film_grad, float* grad_albedo, float* grad_emission, float* grad_light,
-                          float* grad_roughness);
+                          float* grad_roughness, float* grad_texels);
+
+/* Albedo textures (no counterpart in the reference's CPU path, whose hits carry no texture
+ * coordinates; SceneSnapshot.h:118 holds per-vertex texcoords that it never reads). A texture is
+ * width x height rgb fp32 texels, row-major with rows in increasing v, starting at texel `offset`
+ * of the texel array (3 floats per texel). A material whose base_color_texture_index names one of
+ * the scene's textures has base colour albedo * texture(u, v): bilinear, repeat addressing, texel
+ * centres at ((i + 0.5) / width, (j + 0.5) / height), uv interpolated from the hit triangle's
+ * vertex texcoords with the hit barycentrics. 16 bytes. */
+typedef struct PtTextureDesc {
+  uint32_t offset, width, height, pad;
+} PtTextureDesc;
+/* Texture coordinates (2 floats per vertex of PtSceneView.vertices; instanced geometry has none)
+ * and textures of the loaded scene. Host pointers, copied. Call after pt_hip_load_scene and before
+ * pt_hip_build_accel (a later call needs another build). texcoord_count must equal the scene's
+ * vertex count. */
+int pt_hip_load_textures(pt_tracer* tracer, const float* texcoords, uint32_t texcoord_count, const float* texels,
+                         uint64_t texel_count, const PtTextureDesc* textures, uint32_t texture_count);
+/* Replace every texel of the built scene (same count) without a rebuild. texels is a host or
+ * device pointer. */
+int pt_hip_update_texels(pt_tracer* tracer, const float* texels, uint64_t texel_count);
 
 /* Block until all work on the handle's stream is done. */
 int pt_hip_synchronize(pt_tracer* tracer);
@@ -267,6 +290,15 @@ const char* pt_batch_last_error(const pt_batch* batch);
 /* Copies a scene (as pt_hip_load_scene does); index receives its position in the batch. params may be NULL. */
 int pt_batch_add_scene(pt_batch* batch, const PtSceneView* scene, uint32_t width, uint32_t height, uint32_t spp,
                        const PtBvhParams* params, uint32_t* index);
+/* Another camera on scene `scene` (an index from pt_batch_add_scene): shares that scene's geometry,
+ * materials, lights and textures in the pool, renders its own image and film. index receives its
+ * position in the batch. */
+int pt_batch_add_view(pt_batch* batch, uint32_t scene, const PtCamera* camera, uint32_t width, uint32_t height,
+                      uint32_t spp, uint32_t* index);
+/* Textures of scene `scene`, as pt_hip_load_textures (before pt_batch_build). */
+int pt_batch_load_textures(pt_batch* batch, uint32_t scene, const float* texcoords, uint32_t texcoord_count,
+                           const float* texels, uint64_t texel_count, const PtTextureDesc* textures,
+                           uint32_t texture_count);
 /* Builds every scene's tree on host threads, uploads the pool, allocates and clears the films.
  * info (may be NULL) receives pool totals. */
 int pt_batch_build(pt_batch* batch, PtBuildInfo* info);
@@ -274,6 +306,18 @@ int pt_batch_reset_accumulation(pt_batch* batch);
 /* Adds samples 0..spp-1 of every scene's full frame to its film. use_graph != 0 replays the frame's
  * launch sequence from a HIP graph captured on first use (re-captured after a tuning change). */
 int pt_batch_render(pt_batch* batch, uint64_t frame_index, int use_graph);
+/* update_scene_delta over the whole pool: materials (lights) of every scene in the order the scenes
+ * were added, counts equal to the pool's. Host pointers. */
+int pt_batch_update_scene_delta(pt_batch* batch, const PtMaterial* materials, uint32_t material_count,
+                                const PtLight* lights, uint32_t light_count);
+/* Replace every texel of the pool (every scene's textures in order; same count). Host or device pointer. */
+int pt_batch_update_texels(pt_batch* batch, const float* texels, uint64_t texel_count);
+/* Path-replay adjoint of pt_batch_render (every scene's and view's full frame at its spp), as
+ * pt_hip_render_adjoint. film_grad holds 3 floats per pixel of every film, films in batch order;
+ * gradient buffers hold every scene's materials (lights, texels) in the order the scenes were added
+ * (views add none). Device pointers. */
+int pt_batch_render_adjoint(pt_batch* batch, uint64_t frame_index, const float* film_grad, float* grad_albedo,
+                            float* grad_emission, float* grad_light, float* grad_roughness, float* grad_texels);
 /* Film of one scene, as pt_hip_read_film. */
 int pt_batch_read_film(pt_batch* batch, uint32_t scene, float* rgb_sum, uint32_t* sample_count);
 /* Counters summed over the batch's scenes. */
@@ -297,7 +341,8 @@ int pt_bvh_build_host(const PtSceneView* scene, const PtBvhParams* params, uint3
                       uint32_t* prim_order, float* prim_geo);
 
 /* ABI sizes, for bindings to check their mirrors of the records above. */
-uint32_t pt_hip_sizeof(int which); /* 0 material, 1 light, 2 instance, 3 camera, 4 scene, 5 settings, 6 counters, 7 tile */
+uint32_t pt_hip_sizeof(int which); /* 0 material, 1 light, 2 instance, 3 camera, 4 scene, 5 settings, 6 counters,
+                                      7 tile, 8 tuning, 9 texture */
 const char* pt_hip_version(void);
 
 #ifdef __cplusplus

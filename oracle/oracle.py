@@ -77,6 +77,8 @@ _SIGS = {
     "pto_rng_draws": (None, [C.c_uint64, C.c_uint32, _P, _P]),
     "pto_accel_build": (_P, [C.POINTER(PtoScene), C.c_int, C.c_uint32, C.c_uint32]),
     "pto_accel_free": (None, [_P]),
+    "pto_accel_set_textures": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint64, _P, C.c_uint32]),
+    "pto_texture_sample": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_float, C.c_float, _P, _P, _P]),
     "pto_accel_node_count": (C.c_uint32, [_P]),
     "pto_accel_prim_count": (C.c_uint32, [_P]),
     "pto_accel_max_depth": (C.c_uint32, [_P]),
@@ -166,11 +168,30 @@ class Oracle:
         self.accel = self.L.pto_accel_build(C.byref(self.scene.s), BUILDER[builder], leaf_size, bucket_count)
         if not self.accel:
             raise RuntimeError("oracle accel build failed")
+        self.texel_count = 0
+        tc = getattr(scene, "texcoords", None)
+        if tc is not None:
+            texels, descs = scene.texture_arrays()
+            tc = np.ascontiguousarray(tc, np.float32)
+            rc = self.L.pto_accel_set_textures(self.accel, _ptr(tc), len(tc), _ptr(texels), len(texels), _ptr(descs),
+                                               len(descs))
+            if rc != 0:
+                raise RuntimeError("oracle textures rejected")
+            self.texel_count = len(texels)
 
     def __del__(self):
         if getattr(self, "accel", None):
             self.L.pto_accel_free(self.accel)
             self.accel = None
+
+    def texture_sample(self, tex: int, prim: int, bu: float, bv: float):
+        """pto_texture_sample: (rgb (3,), texel indices (4,), weights (4,)) of one bilinear tap."""
+        rgb = np.zeros(3, np.float64)
+        idx = np.zeros(4, np.uint32)
+        w = np.zeros(4, np.float64)
+        if self.L.pto_texture_sample(self.accel, tex, prim, bu, bv, rgb.ctypes.data, idx.ctypes.data, w.ctypes.data):
+            raise ValueError("no such texture or primitive")
+        return rgb, idx, w
 
     @property
     def prim_count(self):
@@ -246,13 +267,14 @@ class Oracle:
 
 
 _SIGS_ADJ = (C.c_int, [C.POINTER(PtoScene), _P, C.POINTER(PtoSettings), _P, C.c_uint32, C.c_uint32, C.c_uint64, C.c_int,
-                       C.c_int, _P, _P, _P, _P, _P, _P])
+                       C.c_int, _P, _P, _P, _P, _P, _P, _P])
 
 
 def adjoint_gradients(orc: Oracle, settings, pixels, sample_index, delta, frame_index=0, threads=0,
                       shadow=SHADOW_ANYHIT) -> dict:
     """Path-replay adjoint (pto_adjoint_pixels) of one sample per listed pixel: a dict of fp64
-    gradients albedo (M,3), emission (M,3), light (Lt,3), roughness (M,) and the forward rgb."""
+    gradients albedo (M,3), emission (M,3), light (Lt,3), roughness (M,), texture (N,3) per texel of the
+    scene's textures, and the forward rgb."""
     fn = orc.L.pto_adjoint_pixels
     fn.restype, fn.argtypes = _SIGS_ADJ
     px = np.ascontiguousarray(pixels, np.uint32)
@@ -262,13 +284,16 @@ def adjoint_gradients(orc: Oracle, settings, pixels, sample_index, delta, frame_
     ge = np.zeros((nm, 3), np.float64)
     gl = np.zeros((max(nl, 1), 3), np.float64)
     gr = np.zeros(max(nm, 1), np.float64)
+    gt = np.zeros((max(orc.texel_count, 1), 3), np.float64)
     rgb = np.zeros((len(px), 3), orc.real)
     st = settings_struct(settings)
     rc = fn(C.byref(orc.scene.s), orc.accel, C.byref(st), px.ctypes.data, len(px), sample_index, frame_index, threads,
-            shadow, d.ctypes.data, ga.ctypes.data, ge.ctypes.data, gl.ctypes.data, gr.ctypes.data, rgb.ctypes.data)
+            shadow, d.ctypes.data, ga.ctypes.data, ge.ctypes.data, gl.ctypes.data, gr.ctypes.data,
+            gt.ctypes.data if orc.texel_count else None, rgb.ctypes.data)
     if rc != 0:
         raise RuntimeError(f"oracle adjoint failed ({rc})")
-    return {"albedo": ga, "emission": ge, "light": gl[:nl], "roughness": gr[:nm], "rgb": rgb}
+    return {"albedo": ga, "emission": ge, "light": gl[:nl], "roughness": gr[:nm], "texture": gt[:orc.texel_count],
+            "rgb": rgb}
 
 
 def adjoint_pixels(orc: Oracle, settings, pixels, sample_index, delta, frame_index=0, threads=0,

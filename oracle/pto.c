@@ -134,6 +134,7 @@ uint32_t pto_sizeof(int which) {
     case 4: return sizeof(PtoScene);
     case 5: return sizeof(PtoSettings);
     case 6: return sizeof(PtoCounters);
+    case 7: return sizeof(PtoTextureDesc);
     default: return 0;
   }
 }
@@ -178,6 +179,7 @@ typedef struct {
 
 typedef struct {
   uint32_t material_index, primitive_index;
+  uint32_t vi[3]; /* vertex indices into PtoScene.vertices; ~0u for instance-local geometry */
   v3 v0, e1, e2, normal;
 } prim_t;
 
@@ -188,6 +190,13 @@ struct pto_accel {
   uint32_t node_count;
   uint32_t* order; /* leaf order -> primitive index */
   uint32_t max_depth;
+  /* textures (pto_accel_set_textures): texture coordinates of each primitive's three vertices
+   * (NULL: none), texels (rgb) and texture records */
+  float* prim_uv;
+  float* texels;
+  uint64_t texel_count;
+  PtoTextureDesc* textures;
+  uint32_t texture_count;
 };
 
 /* rotate_quat / transform_instance_vertex: CpuBvhAccelerator.cpp:48-74 */
@@ -224,6 +233,7 @@ static int flatten(const PtoScene* sc, prim_t** out_prims, uint32_t* out_n, v3**
     for (uint32_t tri = 0; tri < inst->triangle_count; ++tri) {
       v3 a = mk(0, 0, 0), b = a, c = a;
       int loaded = 0;
+      uint32_t vi[3] = {~0u, ~0u, ~0u};
       if (use_local) {
         const uint32_t local_base = inst->local_first_index + tri * 3u;
         const uint32_t local_end = inst->local_first_vertex + inst->local_vertex_count;
@@ -249,10 +259,16 @@ static int flatten(const PtoScene* sc, prim_t** out_prims, uint32_t* out_n, v3**
         a = v3f(sc->vertices + 3u * i0);
         b = v3f(sc->vertices + 3u * i1);
         c = v3f(sc->vertices + 3u * i2);
+        vi[0] = i0;
+        vi[1] = i1;
+        vi[2] = i2;
       }
       prim_t* p = &prims[n];
       p->material_index = inst->material_index;
       p->primitive_index = n;
+      p->vi[0] = vi[0];
+      p->vi[1] = vi[1];
+      p->vi[2] = vi[2];
       p->v0 = a;
       p->e1 = sub(b, a);
       p->e2 = sub(c, a);
@@ -513,7 +529,45 @@ void pto_accel_free(pto_accel* a) {
   free(a->prims);
   free(a->nodes);
   free(a->order);
+  free(a->prim_uv);
+  free(a->texels);
+  free(a->textures);
   free(a);
+}
+
+int pto_accel_set_textures(pto_accel* a, const float* texcoords, uint32_t texcoord_count, const float* texels,
+                           uint64_t texel_count, const PtoTextureDesc* textures, uint32_t texture_count) {
+  if (!a) return -1;
+  free(a->prim_uv);
+  free(a->texels);
+  free(a->textures);
+  a->prim_uv = NULL;
+  a->texels = NULL;
+  a->textures = NULL;
+  a->texture_count = 0;
+  a->texel_count = 0;
+  for (uint32_t i = 0; i < texture_count; ++i)
+    if (textures[i].width == 0 || textures[i].height == 0 ||
+        (uint64_t)textures[i].offset + (uint64_t)textures[i].width * textures[i].height > texel_count)
+      return -1;
+  if (!texcoords || !texcoord_count) return 0;
+  a->prim_uv = (float*)calloc(6u * (size_t)(a->prim_count ? a->prim_count : 1u), sizeof(float));
+  a->texels = (float*)malloc(sizeof(float) * 3u * (size_t)(texel_count ? texel_count : 1u));
+  a->textures = (PtoTextureDesc*)malloc(sizeof(PtoTextureDesc) * (texture_count ? texture_count : 1u));
+  if (!a->prim_uv || !a->texels || !a->textures) return -1;
+  for (uint32_t p = 0; p < a->prim_count; ++p)
+    for (int k = 0; k < 3; ++k) {
+      const uint32_t v = a->prims[p].vi[k];
+      if (v < texcoord_count) {
+        a->prim_uv[6u * p + 2u * k] = texcoords[2u * v];
+        a->prim_uv[6u * p + 2u * k + 1u] = texcoords[2u * v + 1u];
+      }
+    }
+  if (texel_count) memcpy(a->texels, texels, sizeof(float) * 3u * (size_t)texel_count);
+  if (texture_count) memcpy(a->textures, textures, sizeof(PtoTextureDesc) * texture_count);
+  a->texture_count = texture_count;
+  a->texel_count = texel_count;
+  return 0;
 }
 uint32_t pto_accel_node_count(const pto_accel* a) { return a ? a->node_count : 0; }
 uint32_t pto_accel_prim_count(const pto_accel* a) { return a ? a->prim_count : 0; }
@@ -579,12 +633,12 @@ static int slab(const aabb_t* b, const rayq_t* q, real max_t, real* t_near) {
   return tmax > kEps;
 }
 
-typedef struct { int hit; real t; uint32_t prim; } qhit_t;
+typedef struct { int hit; real t; uint32_t prim; real u, v; } qhit_t;
 
 /* Leaf batch: IntersectBvhTriangleBatch / intersect_scalar / commit_ordered_lane_hit,
  * BvhTriangleIntersector.cpp:25-74, :186-209. Returns accepted count; updates *best. */
 static uint32_t leaf_batch(const pto_accel* a, const uint32_t* prims, uint32_t cnt, v3 o, v3 d,
-                           real* best_t, int* lane_out) {
+                           real* best_t, int* lane_out, real* u_out, real* v_out) {
   uint32_t accepted = 0;
   *lane_out = -1;
   if (cnt == 0 || !(*best_t > kEps)) return 0;
@@ -609,6 +663,8 @@ static uint32_t leaf_batch(const pto_accel* a, const uint32_t* prims, uint32_t c
     if (t <= kEps || t >= out_t) continue;
     ++accepted;
     *lane_out = (int)lane;
+    *u_out = u;
+    *v_out = v;
     out_t = t;
   }
   *best_t = out_t;
@@ -618,7 +674,7 @@ static uint32_t leaf_batch(const pto_accel* a, const uint32_t* prims, uint32_t c
 /* Closest hit (any_hit == 0) or any hit with t < max_t (any_hit == 1). The any-hit form
  * is the GPU shadow query's contract: it stops at the first accepted triangle. */
 static qhit_t traverse(const pto_accel* a, v3 o, v3 d, real max_t, int any_hit, PtoCounters* c) {
-  qhit_t h = {0, RC(0.0), 0};
+  qhit_t h = {0, RC(0.0), 0, RC(0.0), RC(0.0)};
   if (a->node_count == 0 || a->prim_count == 0) return h;
   real best_t = max_t;
   const rayq_t q = make_query(o, d);
@@ -640,12 +696,15 @@ static qhit_t traverse(const pto_accel* a, v3 o, v3 d, real max_t, int any_hit, 
         while (i < node->count && cnt < 8u) lanes[cnt++] = a->order[node->first + i++];
         c->triangle_tests += cnt;
         int lane;
-        const uint32_t acc = leaf_batch(a, lanes, cnt, o, d, &best_t, &lane);
+        real bu = RC(0.0), bv = RC(0.0);
+        const uint32_t acc = leaf_batch(a, lanes, cnt, o, d, &best_t, &lane, &bu, &bv);
         if (lane < 0) continue;
         c->triangle_hits += acc;
         h.hit = 1;
         h.t = best_t;
         h.prim = lanes[lane];
+        h.u = bu;
+        h.v = bv;
         if (any_hit) goto done;
       }
       continue;
@@ -697,9 +756,9 @@ int pto_brute_force_rays(const pto_accel* a, const float* rays, uint32_t n, floa
     real best = (real)INFINITY;
     int32_t bp = -1;
     for (uint32_t p = 0; p < a->prim_count; ++p) {
-      real bt = best;
+      real bt = best, bu, bv;
       int lane;
-      leaf_batch(a, &p, 1, o, d, &bt, &lane);
+      leaf_batch(a, &p, 1, o, d, &bt, &lane, &bu, &bv);
       if (lane == 0) { best = bt; bp = (int32_t)p; }
     }
     out_t[i] = bp >= 0 ? (float)best : (float)INFINITY;
@@ -709,11 +768,24 @@ int pto_brute_force_rays(const pto_accel* a, const float* rays, uint32_t n, floa
 }
 
 /* ------------------------------------------------------------------ integrator */
+/* A bilinear tap of a texture: its four texels (indices into the texel array), weights and the
+ * filtered colour. tex = ~0u: no texture. */
+typedef struct {
+  uint32_t tex;
+  uint32_t idx[4];
+  real w[4];
+  v3 color;
+} tap_t;
+
 typedef struct {
   int hit;
   real t;
   v3 position, normal;
   uint32_t material_index;
+  uint32_t prim;   /* flattened primitive */
+  real bu, bv;     /* barycentrics of the hit */
+  v3 base;         /* base colour: material albedo times its texture (set by the shading code) */
+  tap_t tap;
 } hit_t;
 
 typedef struct {
@@ -736,9 +808,13 @@ static hit_t intersect_scene(const ctx_t* x, v3 o, v3 d, PtoCounters* c) {
     h.position = add(o, muls(d, q.t));
     h.normal = p->normal;
     h.material_index = p->material_index;
+    h.prim = q.prim;
+    h.bu = q.u;
+    h.bv = q.v;
   }
   return h;
 }
+
 
 static const PtoMaterial* material_at(const PtoScene* sc, uint32_t index) {
   static const PtoMaterial kDefault = {{1.0f, 1.0f, 1.0f}, {0.0f, 0.0f, 0.0f}, 1.0f, 0.0f, 1.5f, 0.0f, 0.0f, 0.0f,
@@ -764,6 +840,67 @@ static const PtoMaterial* material_at(const PtoScene* sc, uint32_t index) {
 #define R_ACOS(x) pm_acosf(x)
 #endif
 #endif
+
+/* ---- textures (no counterpart in the reference's CPU path, whose Hit carries no texture
+ * coordinates, PathTracer.h:1283-1289). uv interpolation at the hit barycentrics, then a bilinear
+ * tap with repeat addressing and texel centres at half-integers. The operation order is fixed so
+ * that fp32 builds reproduce the device bit for bit. */
+static tap_t texture_tap(const pto_accel* a, uint32_t ti, uint32_t prim, real bu, real bv) {
+  tap_t r;
+  const float* uvp = a->prim_uv + 6u * prim;
+  const real w = RC(1.0) - bu - bv;
+  const real uu = (real)uvp[0] * w + (real)uvp[2] * bu + (real)uvp[4] * bv;
+  const real vv = (real)uvp[1] * w + (real)uvp[3] * bu + (real)uvp[5] * bv;
+  const PtoTextureDesc* T = &a->textures[ti];
+  real uf = uu - R_FLOOR(uu), vf = vv - R_FLOOR(vv);
+  if (!(uf >= RC(0.0) && uf < RC(1.0))) uf = RC(0.0);
+  if (!(vf >= RC(0.0) && vf < RC(1.0))) vf = RC(0.0);
+  const real x = uf * (real)T->width - RC(0.5), y = vf * (real)T->height - RC(0.5);
+  const real x0 = R_FLOOR(x), y0 = R_FLOOR(y);
+  const real fx = x - x0, fy = y - y0;
+  int ix0 = (int)x0, iy0 = (int)y0;
+  if (ix0 < 0) ix0 += (int)T->width;
+  if (iy0 < 0) iy0 += (int)T->height;
+  const int ix1 = ix0 + 1 == (int)T->width ? 0 : ix0 + 1;
+  const int iy1 = iy0 + 1 == (int)T->height ? 0 : iy0 + 1;
+  r.tex = ti;
+  r.idx[0] = T->offset + (uint32_t)iy0 * T->width + (uint32_t)ix0;
+  r.idx[1] = T->offset + (uint32_t)iy0 * T->width + (uint32_t)ix1;
+  r.idx[2] = T->offset + (uint32_t)iy1 * T->width + (uint32_t)ix0;
+  r.idx[3] = T->offset + (uint32_t)iy1 * T->width + (uint32_t)ix1;
+  r.w[0] = (RC(1.0) - fx) * (RC(1.0) - fy);
+  r.w[1] = fx * (RC(1.0) - fy);
+  r.w[2] = (RC(1.0) - fx) * fy;
+  r.w[3] = fx * fy;
+  v3 col = mk(RC(0.0), RC(0.0), RC(0.0));
+  for (int k = 0; k < 4; ++k) col = add(col, muls(v3f(a->texels + 3u * r.idx[k]), r.w[k]));
+  r.color = col;
+  return r;
+}
+
+int pto_texture_sample(const pto_accel* a, uint32_t tex, uint32_t prim, float bu, float bv, double* out_rgb,
+                       uint32_t* out_idx, double* out_w) {
+  if (!a || !a->prim_uv || tex >= a->texture_count || prim >= a->prim_count) return -1;
+  const tap_t t = texture_tap(a, tex, prim, (real)bu, (real)bv);
+  out_rgb[0] = (double)t.color.x;
+  out_rgb[1] = (double)t.color.y;
+  out_rgb[2] = (double)t.color.z;
+  for (int k = 0; k < 4; ++k) {
+    out_idx[k] = t.idx[k];
+    out_w[k] = (double)t.w[k];
+  }
+  return 0;
+}
+
+/* The material's base colour at a hit: albedo, times the texture when the material names one. */
+static void hit_base_color(const pto_accel* a, const PtoMaterial* m, hit_t* h) {
+  h->base = v3f(m->albedo);
+  h->tap.tex = ~0u;
+  if (a->prim_uv && m->base_color_texture_index < a->texture_count) {
+    h->tap = texture_tap(a, m->base_color_texture_index, h->prim, h->bu, h->bv);
+    h->base = mulv(h->base, h->tap.color);
+  }
+}
 
 /* std::fmod(x, 2) for the integer-valued sums the effects pass (exact). */
 static inline real fmod2(real x) { return x - RC(2.0) * R_TRUNC(x * RC(0.5)); }
@@ -833,8 +970,8 @@ static v3 wood_albedo(v3 base, v3 p, real roughness, real clearcoat, uint32_t st
 /* apply_material_effect: ScalarCpuPathTracer.cpp:264-355. Every effect is affine in the albedo
  * (colour = k * albedo + b before the final [0, 1.5] clamp): *k receives k and pre[] the colour
  * before the clamp, which the adjoint needs. */
-static v3 effect_albedo(const PtoMaterial* m, v3 pos, v3 n, v3 incoming, real* k_out, v3* pre_out) {
-  v3 c = v3f(m->albedo);
+static v3 effect_albedo(const PtoMaterial* m, v3 base, v3 pos, v3 n, v3 incoming, real* k_out, v3* pre_out) {
+  v3 c = base;
   real k = RC(1.0);
   const uint32_t e = m->material_effect;
   if (e != 0u) {
@@ -1021,7 +1158,7 @@ static nee_t sample_direct_light(const ctx_t* x, const hit_t* hit, v3 view_dir, 
   c->shadow_tests++;
   const v3 so = add(hit->position, muls(hit->normal, RC(0.002)));
   if (!shadow_visible(x, so, light_dir, dist, c)) return r;
-  const v3 albedo = effect_albedo(m, hit->position, hit->normal, neg(light_dir), &r.eff_k, &r.eff_pre);
+  const v3 albedo = effect_albedo(m, hit->base, hit->position, hit->normal, neg(light_dir), &r.eff_k, &r.eff_pre);
   const real kscale = ((real)light->intensity * spot) / (dist_sq + kEps);
   const v3 irradiance = muls(v3f(light->color), kscale);
   v3 direct = mk(0, 0, 0);
@@ -1200,16 +1337,36 @@ typedef struct {
   v3 L_total;
   double delta[3];
   double *g_albedo, *g_emission, *g_light, *g_rough;
+  double* g_tex;  /* 3 per texel (NULL: not requested) */
+  const PtoScene* sc;
   int pending;
   uint32_t p_mat;
+  tap_t p_tap;  /* texture tap of the pending bounce's hit */
   double p_dx[3];
   double p_score;  /* d log(density of the sampled direction) / d roughness of the pending bounce */
   int p_active, p_cstar;
   int nhist;
   uint32_t h_mat[PTO_ADJ_HIST];
+  tap_t h_tap[PTO_ADJ_HIST];
   double h_dx[PTO_ADJ_HIST][3];
   int prev_active, prev_cstar, prev_nhist;
 } adj_t;
+
+static inline real v3c(v3 v, int c) { return c == 0 ? v.x : (c == 1 ? v.y : v.z); }
+
+/* Adds G = df/d(base colour channel c) of a vertex on material mat: base = albedo * texture tap
+ * (the tap's colour is 1 when untextured), so df/dalbedo_c = G * tap_c and
+ * df/dtexel_{j,c} = G * albedo_c * w_j for the tap's four texels. */
+static void adj_base(adj_t* a, uint32_t mat, const tap_t* tap, int c, double G) {
+  if (tap->tex == ~0u) {
+    a->g_albedo[3u * mat + c] += G;
+    return;
+  }
+  a->g_albedo[3u * mat + c] += G * (double)v3c(tap->color, c);
+  if (!a->g_tex) return;
+  const double alb = (double)material_at(a->sc, mat)->albedo[c];
+  for (int j = 0; j < 4; ++j) a->g_tex[3u * tap->idx[j] + c] += G * alb * (double)tap->w[j];
+}
 
 static void adj_flush_pending(adj_t* a, v3 radiance) {
   if (!a || !a->pending) return;
@@ -1218,14 +1375,15 @@ static void adj_flush_pending(adj_t* a, v3 radiance) {
   double S = 0.0;
   for (int c = 0; c < 3; ++c) {
     S += a->delta[c] * R[c];
-    a->g_albedo[3u * a->p_mat + c] += a->delta[c] * R[c] * a->p_dx[c];
+    adj_base(a, a->p_mat, &a->p_tap, c, a->delta[c] * R[c] * a->p_dx[c]);
   }
   a->g_rough[a->p_mat] += S * a->p_score;
   if (a->p_active) {
-    for (int h = 0; h < a->nhist; ++h) a->g_albedo[3u * a->h_mat[h] + a->p_cstar] += -S * a->h_dx[h][a->p_cstar];
+    for (int h = 0; h < a->nhist; ++h)
+      adj_base(a, a->h_mat[h], &a->h_tap[h], a->p_cstar, -S * a->h_dx[h][a->p_cstar]);
     if (a->prev_active)
       for (int h = 0; h < a->prev_nhist; ++h)
-        a->g_albedo[3u * a->h_mat[h] + a->prev_cstar] += S * a->h_dx[h][a->prev_cstar];
+        adj_base(a, a->h_mat[h], &a->h_tap[h], a->prev_cstar, S * a->h_dx[h][a->prev_cstar]);
     a->prev_active = 1;
     a->prev_cstar = a->p_cstar;
     a->prev_nhist = a->nhist;
@@ -1316,7 +1474,8 @@ static v3 trace(const ctx_t* x, v3 ro, v3 rd, uint64_t* rng, uint64_t* rays, Pto
     const PtoMaterial* m = material_at(sc, hit.material_index);
     real eff_k;
     v3 eff_pre;
-    const v3 albedo = effect_albedo(m, hit.position, sN, rd, &eff_k, &eff_pre);
+    hit_base_color(x->a, m, &hit);
+    const v3 albedo = effect_albedo(m, hit.base, hit.position, sN, rd, &eff_k, &eff_pre);
     const real roughness = clamp01((real)m->roughness);
     const int isEmissiveModel = m->material_model == 1u || is_emissive(m);
     const int isMirror = m->material_model == 2u || roughness <= RC(0.001);
@@ -1370,7 +1529,7 @@ static v3 trace(const ctx_t* x, v3 ro, v3 rd, uint64_t* rng, uint64_t* rays, Pto
           for (int ch = 0; ch < 3; ++ch) {
             const double tw = adj->delta[ch] * (double)t3[ch] * (double)w;
             const real pre[3] = {nee.eff_pre.x, nee.eff_pre.y, nee.eff_pre.z};
-            adj->g_albedo[3u * mat_index + ch] += tw * (double)da[ch] * eff_grad(pre[ch], nee.eff_k);
+            adj_base(adj, mat_index, &hit.tap, ch, tw * (double)da[ch] * eff_grad(pre[ch], nee.eff_k));
             adj->g_light[3u * nee.light + ch] += tw * (double)dc[ch];
           }
           const real dr3[3] = {nee.d_rough.x, nee.d_rough.y, nee.d_rough.z};
@@ -1442,6 +1601,7 @@ static v3 trace(const ctx_t* x, v3 ro, v3 rd, uint64_t* rng, uint64_t* rays, Pto
       if (isTransmissive && !(isMetallic || isMirror)) tint = clamp01((real)m->alpha) * (real)m->transmission;
       adj->pending = 1;
       adj->p_mat = mat_index;
+      adj->p_tap = hit.tap;
       adj->p_score = score;
       const real pre3[3] = {eff_pre.x, eff_pre.y, eff_pre.z};
       for (int ch = 0; ch < 3; ++ch) {
@@ -1458,6 +1618,7 @@ static v3 trace(const ctx_t* x, v3 ro, v3 rd, uint64_t* rng, uint64_t* rays, Pto
       adj->p_cstar = cstar;
       if (depth < st->rr_start_depth && adj->nhist < PTO_ADJ_HIST) {
         adj->h_mat[adj->nhist] = mat_index;
+        adj->h_tap[adj->nhist] = hit.tap;
         for (int ch = 0; ch < 3; ++ch) adj->h_dx[adj->nhist][ch] = adj->p_dx[ch];
         adj->nhist++;
       }
@@ -1671,7 +1832,7 @@ int pto_render_tile(const PtoScene* scene, const pto_accel* a, const PtoSettings
 int pto_adjoint_pixels(const PtoScene* scene, const pto_accel* a, const PtoSettings* s,
                        const uint32_t* pixels, uint32_t n, uint32_t sample_index, uint64_t frame_index,
                        int threads, int shadow_mode, const float* delta, double* g_albedo, double* g_emission,
-                       double* g_light, double* g_rough, void* out_rgb) {
+                       double* g_light, double* g_rough, double* g_tex, void* out_rgb) {
   if (!scene || !a || !s || s->width == 0 || s->height == 0) return -1;
   const int sup = check_supported(scene);
   if (sup) return sup;
@@ -1692,6 +1853,8 @@ int pto_adjoint_pixels(const PtoScene* scene, const pto_accel* a, const PtoSetti
     double* ge = (double*)calloc(3u * (nm ? nm : 1u), sizeof(double));
     double* gl = (double*)calloc(3u * (nl ? nl : 1u), sizeof(double));
     double* gr = (double*)calloc(nm ? nm : 1u, sizeof(double));
+    const size_t ntx = g_tex ? 3u * (size_t)a->texel_count : 0u;
+    double* gt = ntx ? (double*)calloc(ntx, sizeof(double)) : NULL;
     PtoCounters lc;
     memset(&lc, 0, sizeof(lc));
 #pragma omp for schedule(dynamic, 64)
@@ -1711,6 +1874,8 @@ int pto_adjoint_pixels(const PtoScene* scene, const pto_accel* a, const PtoSetti
       adj.g_emission = ge;
       adj.g_light = gl;
       adj.g_rough = gr;
+      adj.g_tex = gt;
+      adj.sc = scene;
       v3 L2;
       shade_pixel(&x, &cam, pixels[i], sample_index, frame_index, &L2, &rays, &lc, &adj);
     }
@@ -1720,7 +1885,9 @@ int pto_adjoint_pixels(const PtoScene* scene, const pto_accel* a, const PtoSetti
       for (uint32_t k = 0; k < 3u * nl; ++k) g_light[k] += gl[k];
       if (g_rough)
         for (uint32_t k = 0; k < nm; ++k) g_rough[k] += gr[k];
+      for (size_t k = 0; k < ntx; ++k) g_tex[k] += gt[k];
     }
+    free(gt);
     free(gr);
     free(ga);
     free(ge);

@@ -94,6 +94,11 @@ typedef struct {
   uint64_t sdf_hits, sdf_misses, bvh_node_visits, bvh_leaf_visits, shadow_tests;
 } PtoCounters;
 
+/* Texture record (include/vkpt_hip.h PtTextureDesc): width x height rgb texels from texel `offset`. */
+typedef struct {
+  uint32_t offset, width, height, pad;
+} PtoTextureDesc;
+
 enum { PTO_BUILDER_MIDPOINT = 0, PTO_BUILDER_SAH = 1 };
 enum { PTO_SHADOW_CLOSEST = 0, PTO_SHADOW_ANYHIT = 1 };
 
@@ -101,7 +106,8 @@ typedef struct pto_accel pto_accel;
 
 uint32_t pto_real_bytes(void);
 uint32_t pto_uses_libm(void);
-uint32_t pto_sizeof(int which); /* 0 material, 1 light, 2 instance, 3 camera, 4 scene, 5 settings, 6 counters */
+uint32_t pto_sizeof(int which); /* 0 material, 1 light, 2 instance, 3 camera, 4 scene, 5 settings, 6 counters,
+                                   7 texture */
 
 /* Math entry points (portable or libm, per build). */
 float pto_sinf(float x);
@@ -119,6 +125,16 @@ void pto_rng_draws(uint64_t state, uint32_t n, float* out, uint64_t* out_state);
 /* Acceleration structure over the flattened instance triangles. */
 pto_accel* pto_accel_build(const PtoScene* scene, int builder, uint32_t leaf_size, uint32_t bucket_count);
 void pto_accel_free(pto_accel* a);
+/* Albedo textures (no counterpart in the reference's CPU path): texcoords are 2 floats per scene
+ * vertex; a material whose base_color_texture_index is below texture_count has its albedo
+ * multiplied by the bilinear tap at the hit. Copied. Returns 0, or -1 for a texture outside the
+ * texels. */
+int pto_accel_set_textures(pto_accel* a, const float* texcoords, uint32_t texcoord_count, const float* texels,
+                           uint64_t texel_count, const PtoTextureDesc* textures, uint32_t texture_count);
+/* The bilinear tap of texture `tex` at barycentrics (bu, bv) of flattened primitive `prim`: colour,
+ * the four texel indices and their weights. Returns -1 without textures or for bad indices. */
+int pto_texture_sample(const pto_accel* a, uint32_t tex, uint32_t prim, float bu, float bv, double* out_rgb,
+                       uint32_t* out_idx, double* out_w);
 uint32_t pto_accel_node_count(const pto_accel* a);
 uint32_t pto_accel_prim_count(const pto_accel* a);
 uint32_t pto_accel_max_depth(const pto_accel* a);
@@ -149,7 +165,7 @@ int pto_render_tile(const PtoScene* scene, const pto_accel* a, const PtoSettings
 int pto_adjoint_pixels(const PtoScene* scene, const pto_accel* a, const PtoSettings* s,
                        const uint32_t* pixels, uint32_t n, uint32_t sample_index, uint64_t frame_index,
                        int threads, int shadow_mode, const float* delta, double* g_albedo, double* g_emission,
-                       double* g_light, double* g_rough, void* out_rgb);
+                       double* g_light, double* g_rough, double* g_tex, void* out_rgb);
 
 /* Glossy-branch direction samples and their d log(density)/d exponent (see pto.c). */
 int pto_phong_samples(float exponent, const float refl[3], const float n[3], float ps, uint64_t state,

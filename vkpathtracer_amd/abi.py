@@ -84,6 +84,10 @@ class PtTuning(C.Structure):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
 
 
+class PtTextureDesc(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("offset", "width", "height", "pad")]
+
+
 KERNELS = ("raygen", "extend", "shade", "shadow", "film")
 
 
@@ -121,12 +125,21 @@ SIGNATURES = {
     "pt_hip_reset_kernel_stats": (C.c_int, [_P]),
     "pt_bvh_build_host": (C.c_int, [C.POINTER(PtSceneView), C.POINTER(PtBvhParams), _P, _P, _P, _P, _P, _P, _P]),
     "pt_hip_update_scene_delta": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint32]),
-    "pt_hip_render_adjoint": (C.c_int, [_P, C.POINTER(PtRenderTile), C.c_uint32, C.c_uint64, _P, _P, _P, _P, _P]),
+    "pt_hip_render_adjoint": (C.c_int, [_P, C.POINTER(PtRenderTile), C.c_uint32, C.c_uint64, _P, _P, _P, _P, _P,
+                                        _P]),
+    "pt_batch_render_adjoint": (C.c_int, [_P, C.c_uint64, _P, _P, _P, _P, _P, _P]),
+    "pt_batch_update_scene_delta": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint32]),
+    "pt_batch_update_texels": (C.c_int, [_P, _P, C.c_uint64]),
+    "pt_hip_load_textures": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint64, _P, C.c_uint32]),
+    "pt_hip_update_texels": (C.c_int, [_P, _P, C.c_uint64]),
     "pt_batch_create": (C.c_int, [C.c_int, C.POINTER(PtRenderSettings), C.POINTER(C.c_void_p)]),
     "pt_batch_destroy": (None, [_P]),
     "pt_batch_last_error": (C.c_char_p, [_P]),
     "pt_batch_add_scene": (C.c_int, [_P, C.POINTER(PtSceneView), C.c_uint32, C.c_uint32, C.c_uint32,
                                      C.POINTER(PtBvhParams), C.POINTER(C.c_uint32)]),
+    "pt_batch_add_view": (C.c_int, [_P, C.c_uint32, C.POINTER(PtCamera), C.c_uint32, C.c_uint32, C.c_uint32,
+                                    C.POINTER(C.c_uint32)]),
+    "pt_batch_load_textures": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32, _P, C.c_uint64, _P, C.c_uint32]),
     "pt_batch_build": (C.c_int, [_P, C.POINTER(PtBuildInfo)]),
     "pt_batch_reset_accumulation": (C.c_int, [_P]),
     "pt_batch_render": (C.c_int, [_P, C.c_uint64, C.c_int]),
@@ -202,20 +215,36 @@ class SceneView:
         v.material_count = len(sc.materials)
         v.light_count = len(sc.lights)
         v.environment_color[:] = list(sc.environment_color)
-        c = sc.camera
-        cam = PtCamera()
-        cam.position[:] = list(c.position)
-        cam.target[:] = list(c.target)
-        cam.up[:] = list(c.up)
-        cam.fov_deg = c.fov_deg
-        cam.aperture_radius = c.aperture_radius
-        cam.focus_distance = c.focus_distance
-        cam.iris_blade_count = c.iris_blade_count
-        cam.iris_rotation_degrees = c.iris_rotation_degrees
-        cam.iris_roundness = c.iris_roundness
-        cam.anamorphic_squeeze = c.anamorphic_squeeze
-        v.camera = cam
+        v.camera = make_camera(sc.camera)
         self.view = v
+        # texture arrays (pt_hip_load_textures / pt_batch_load_textures)
+        self.texcoords = sc.texcoords
+        self.texels, self.texture_descs = sc.texture_arrays()
+
+    def texture_args(self) -> tuple:
+        """(texcoords, texcoord_count, texels, texel_count, descs, desc_count) for *_load_textures."""
+        tc = self.texcoords
+        return (_ptr(tc), 0 if tc is None else len(tc), _ptr(self.texels), len(self.texels),
+                _ptr(self.texture_descs), len(self.texture_descs))
+
+    @property
+    def has_textures(self) -> bool:
+        return self.texcoords is not None or len(self.texture_descs) > 0
+
+
+def make_camera(c: "_scene.Camera") -> PtCamera:
+    cam = PtCamera()
+    cam.position[:] = list(c.position)
+    cam.target[:] = list(c.target)
+    cam.up[:] = list(c.up)
+    cam.fov_deg = c.fov_deg
+    cam.aperture_radius = c.aperture_radius
+    cam.focus_distance = c.focus_distance
+    cam.iris_blade_count = c.iris_blade_count
+    cam.iris_rotation_degrees = c.iris_rotation_degrees
+    cam.iris_roundness = c.iris_roundness
+    cam.anamorphic_squeeze = c.anamorphic_squeeze
+    return cam
 
 
 def make_settings(s: "_scene.RenderSettings") -> PtRenderSettings:

@@ -237,7 +237,8 @@ struct MidBuilder {
 
 }  // namespace
 
-bool flatten(const PtSceneView& sc, std::vector<Prim>& prims, std::vector<float>& raw, std::string& err) {
+bool flatten(const PtSceneView& sc, std::vector<Prim>& prims, std::vector<float>& raw, std::string& err,
+             const float* texcoords) {
   uint64_t cap = 0;
   for (uint32_t i = 0; i < sc.instance_count; ++i) cap += sc.instances[i].triangle_count;
   if (cap >= kMaxPrimitives) {
@@ -253,6 +254,7 @@ bool flatten(const PtSceneView& sc, std::vector<Prim>& prims, std::vector<float>
     const bool use_local = (inst.flags & 1u) != 0u && inst.local_vertex_count > 0u && inst.local_index_count > 0u;
     for (uint32_t tri = 0; tri < inst.triangle_count; ++tri) {
       V3 a{0, 0, 0}, b{0, 0, 0}, c{0, 0, 0};
+      float uv[6] = {0, 0, 0, 0, 0, 0};
       bool loaded = false;
       if (use_local && sc.local_indices && sc.local_vertices) {
         const uint32_t local_base = inst.local_first_index + tri * 3u;
@@ -278,6 +280,13 @@ bool flatten(const PtSceneView& sc, std::vector<Prim>& prims, std::vector<float>
         a = load3(sc.vertices + 3u * i0);
         b = load3(sc.vertices + 3u * i1);
         c = load3(sc.vertices + 3u * i2);
+        if (texcoords) {
+          const uint32_t vi[3] = {i0, i1, i2};
+          for (int k = 0; k < 3; ++k) {
+            uv[2 * k] = texcoords[2u * vi[k]];
+            uv[2 * k + 1] = texcoords[2u * vi[k] + 1u];
+          }
+        }
       }
       Prim p;
       p.material = inst.material_index;
@@ -287,6 +296,7 @@ bool flatten(const PtSceneView& sc, std::vector<Prim>& prims, std::vector<float>
       p.e1[0] = e1.x; p.e1[1] = e1.y; p.e1[2] = e1.z;
       p.e2[0] = e2.x; p.e2[1] = e2.y; p.e2[2] = e2.z;
       p.n[0] = n.x; p.n[1] = n.y; p.n[2] = n.z;
+      std::memcpy(p.uv, uv, sizeof(uv));
       prims.push_back(p);
       const float r9[9] = {a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z};
       raw.insert(raw.end(), r9, r9 + 9);
@@ -342,10 +352,15 @@ void build_midpoint(Bvh& bvh, const std::vector<float>& raw, uint32_t leaf_size)
   bvh.nodes.resize(static_cast<size_t>(b.alloc));
 }
 
-void pack_device_layout(const Bvh& bvh, DeviceLayout& out) {
+void pack_device_layout(const Bvh& bvh, DeviceLayout& out, bool with_uv) {
   out.nodes.clear();
   out.tris.clear();
   out.prims.clear();
+  out.prim_uv.clear();
+  if (with_uv) {
+    out.prim_uv.resize(6u * bvh.prims.size());
+    for (size_t i = 0; i < bvh.prims.size(); ++i) std::memcpy(&out.prim_uv[6u * i], bvh.prims[i].uv, 6 * sizeof(float));
+  }
   out.inner_count = 0;
   const size_t nprim = bvh.prims.size();
   out.prims.resize(4u * nprim);

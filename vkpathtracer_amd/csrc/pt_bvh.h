@@ -14,6 +14,7 @@ namespace ptb {
 struct Prim {
   uint32_t material;
   float v0[3], e1[3], e2[3], n[3];
+  float uv[6];  // texture coordinates of v0, v1, v2 (zero without texcoords)
 };
 
 struct Node {
@@ -30,8 +31,10 @@ struct Bvh {
 };
 
 // Flatten instances into primitives; raw receives the 3 unpadded vertices of each primitive.
-// Returns false (with err set) on a malformed snapshot.
-bool flatten(const PtSceneView& scene, std::vector<Prim>& prims, std::vector<float>& raw, std::string& err);
+// texcoords (2 floats per vertex of scene.vertices, may be null) give the primitives of indexed
+// geometry their texture coordinates. Returns false (with err set) on a malformed snapshot.
+bool flatten(const PtSceneView& scene, std::vector<Prim>& prims, std::vector<float>& raw, std::string& err,
+             const float* texcoords = nullptr);
 
 // Bucketed SAH (D3D12GpuPathTracer.SceneBvh.cpp:73-238) and midpoint (ParallelBvhBuilder.cpp:61-163).
 void build_sah(Bvh& bvh, const std::vector<float>& raw, uint32_t leaf_size, uint32_t buckets);
@@ -47,6 +50,7 @@ struct DeviceLayout {
   std::vector<float> nodes;   // 16 floats per inner node
   std::vector<float> tris;    // 12 floats per leaf-ordered triangle
   std::vector<float> prims;   // 4 floats per primitive
+  std::vector<float> prim_uv; // 6 floats per primitive (texture coordinates), empty without texcoords
   float root_box[6] = {0, 0, 0, 0, 0, 0};
   int32_t root_ref = 0;
   uint32_t inner_count = 0;
@@ -56,6 +60,6 @@ constexpr int kLeafCountBits = 5;
 constexpr uint32_t kMaxLeafSize = (1u << kLeafCountBits) - 1u;
 constexpr uint32_t kMaxPrimitives = 1u << (31 - kLeafCountBits);
 
-void pack_device_layout(const Bvh& bvh, DeviceLayout& out);
+void pack_device_layout(const Bvh& bvh, DeviceLayout& out, bool with_uv = false);
 
 }  // namespace ptb

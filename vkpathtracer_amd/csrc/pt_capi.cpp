@@ -66,6 +66,10 @@ struct SceneSlot {
   std::vector<PtMaterial> materials;
   std::vector<PtLight> lights;
   PtSceneView view{};
+  std::vector<float> texcoords;  // 2 per vertex (may be empty)
+  std::vector<float> texels;     // rgb
+  std::vector<PtTextureDesc> textures;
+  int32_t view_of = -1;          // >= 0: another camera on that scene's geometry (batches)
   uint32_t width = 0, height = 0, spp = 0;  // spp: samples per batch frame
   PtBvhParams bvh_params{PT_BVH_SAH, 4u, 8u, 0u};
   ptb::Bvh bvh;
@@ -103,6 +107,9 @@ struct pt_tracer {
   DevArray<PtMaterial> d_materials;
   DevArray<PtLight> d_lights;
   DevArray<ptd::SceneRec> d_recs;
+  DevArray<ptd::TexRec> d_texrecs;
+  DevArray<float> d_texels, d_prim_uv;
+  size_t texel_count = 0;
   DevArray<uint32_t> d_film_count, d_queue, d_error;
   DevArray<unsigned long long> d_counters, d_query_counters;
   DevArray<float> d_env;
@@ -281,6 +288,9 @@ void free_scene_device(pt_tracer* t) {
   t->d_lights.release();
   t->d_env.release();
   t->d_recs.release();
+  t->d_texrecs.release();
+  t->d_texels.release();
+  t->d_prim_uv.release();
 }
 
 void free_paths(pt_tracer* t) {
@@ -296,13 +306,16 @@ void free_paths(pt_tracer* t) {
 }
 
 // Adjoint state for `cap` paths: 29 float arrays and 8 u32 arrays (see ptd::DevAdjoint).
+// Adjoint state: 33 + 2 * kAdjHist float arrays and 4 + kAdjHist u32 arrays of max(cap, kSegments *
+// seg_cap) entries.
+constexpr size_t kAdjFloats = 33 + 2 * ptd::kAdjHist;
 int ensure_adjoint(pt_tracer* t, uint32_t cap, ptd::DevAdjoint& A) {
   if (cap > t->adj_capacity_alloc) {
     t->d_adj_f.release();
     t->d_adj_u.release();
     t->adj_capacity_alloc = 0;
     const size_t m = std::max<size_t>(cap, static_cast<size_t>(ptd::kSegments) * ptd::segment_capacity(cap));
-    PT_HIP(t, t->d_adj_f.ensure(static_cast<size_t>(29) * m), "allocate adjoint state");
+    PT_HIP(t, t->d_adj_f.ensure(static_cast<size_t>(kAdjFloats) * m), "allocate adjoint state");
     PT_HIP(t, t->d_adj_u.ensure(static_cast<size_t>(8) * m), "allocate adjoint state");
     t->adj_capacity_alloc = cap;
   }
@@ -320,6 +333,11 @@ int ensure_adjoint(pt_tracer* t, uint32_t cap, ptd::DevAdjoint& A) {
   }
   A.p_score = f + 27 * n;
   A.s_grough = f + 28 * n;
+  for (int k = 0; k < 2; ++k) {
+    A.p_uv[k] = f + (29 + k) * n;
+    A.s_uv[k] = f + (31 + k) * n;
+    for (int h = 0; h < ptd::kAdjHist; ++h) A.h_uv[h][k] = f + (33 + 2 * h + k) * n;
+  }
   A.p_mat = u;
   A.flags = u + n;
   A.s_mat = u + 2 * n;
@@ -328,12 +346,12 @@ int ensure_adjoint(pt_tracer* t, uint32_t cap, ptd::DevAdjoint& A) {
   return PT_OK;
 }
 
-// Path-state and queue arrays for `cap` paths: 29 float arrays, 7 u32 arrays, 1 u64 array, each
+// Path-state and queue arrays for `cap` paths: 31 float arrays, 7 u32 arrays, 1 u64 array, each
 // of max(cap, kSegments * seg_cap) entries (queue arrays are indexed by segment position).
 int ensure_paths(pt_tracer* t, uint32_t cap) {
   if (cap <= t->path_capacity_alloc) return PT_OK;
   free_paths(t);
-  constexpr size_t kF = 29, kU = 7;
+  constexpr size_t kF = 31, kU = 7;
   const uint32_t seg_cap = ptd::segment_capacity(cap);
   const size_t n = std::max<size_t>(cap, static_cast<size_t>(ptd::kSegments) * seg_cap);
   PT_HIP(t, t->d_path_f.ensure(kF * n), "allocate path state");
@@ -356,6 +374,8 @@ int ensure_paths(pt_tracer* t, uint32_t cap) {
   }
   for (int k = 0; k < 6; ++k) P.sray[k] = f + (size_t)(19 + k) * n;
   P.stmax = f + 25 * n;
+  P.hit_u = f + 29 * n;
+  P.hit_v = f + 30 * n;
   P.spath = u + 3 * n;
   P.sscene = u + (size_t)6 * n;
   P.rng = t->d_rng.ptr;
@@ -419,6 +439,31 @@ void copy_scene(SceneSlot& s, const PtSceneView* sc) {
   s.view.light_count = static_cast<uint32_t>(s.lights.size());
   s.layout = ptb::DeviceLayout{};
   s.bvh = ptb::Bvh{};
+  s.texcoords.clear();
+  s.texels.clear();
+  s.textures.clear();
+  s.view_of = -1;
+}
+
+// Copies a scene's texture coordinates and textures after checking that every texture lies in
+// the texel array and the coordinates cover the scene's vertices.
+int copy_textures(pt_tracer* t, SceneSlot& s, const float* texcoords, uint32_t texcoord_count, const float* texels,
+                  uint64_t texel_count, const PtTextureDesc* textures, uint32_t texture_count) {
+  if (texcoord_count && texcoord_count != s.vertices.size() / 3u)
+    return fail(t, PT_INVALID_ARGUMENT, "load_textures: texcoord_count differs from the vertex count");
+  if ((texcoord_count && !texcoords) || (texel_count && !texels) || (texture_count && !textures))
+    return fail(t, PT_INVALID_ARGUMENT, "load_textures: null array");
+  for (uint32_t i = 0; i < texture_count; ++i) {
+    const PtTextureDesc& d = textures[i];
+    if (d.width == 0 || d.height == 0 || d.width > 65536u || d.height > 65536u)
+      return fail(t, PT_INVALID_ARGUMENT, "load_textures: texture " + std::to_string(i) + " has a bad size");
+    if (static_cast<uint64_t>(d.offset) + static_cast<uint64_t>(d.width) * d.height > texel_count)
+      return fail(t, PT_INVALID_ARGUMENT, "load_textures: texture " + std::to_string(i) + " exceeds the texels");
+  }
+  s.texcoords.assign(texcoords, texcoords + 2u * static_cast<size_t>(texcoord_count));
+  s.texels.assign(texels, texels + 3u * static_cast<size_t>(texel_count));
+  s.textures.assign(textures, textures + texture_count);
+  return PT_OK;
 }
 
 int check_bvh_params(pt_tracer* t, const PtBvhParams* params, PtBvhParams& p) {
@@ -436,10 +481,11 @@ bool build_scene(SceneSlot& s) {
   const auto t0 = std::chrono::steady_clock::now();
   std::vector<float> raw;
   s.error.clear();
-  if (!ptb::flatten(s.view, s.bvh.prims, raw, s.error)) return false;
+  if (!ptb::flatten(s.view, s.bvh.prims, raw, s.error, s.texcoords.empty() ? nullptr : s.texcoords.data()))
+    return false;
   if (s.bvh_params.builder == PT_BVH_SAH) ptb::build_sah(s.bvh, raw, s.bvh_params.leaf_size, s.bvh_params.bucket_count);
   else ptb::build_midpoint(s.bvh, raw, s.bvh_params.leaf_size);
-  ptb::pack_device_layout(s.bvh, s.layout);
+  ptb::pack_device_layout(s.bvh, s.layout, true);
   s.info = PtBuildInfo{};
   s.info.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   s.info.built = 1;
@@ -489,102 +535,138 @@ int32_t rebase_ref(int32_t ref, uint32_t node_base, uint32_t tri_base) {
   return ~static_cast<int32_t>((first << ptb::kLeafCountBits) | count);
 }
 
-// Packs every scene's tree, materials, lights and environment into the device pool, rebasing
-// node, triangle, primitive and material references, and writes the scene records. Films are
-// laid out scene after scene (film_base); a tracer's single scene starts at pixel 0.
+// A scene's materials as the device sees them: texture indices rebased into the texture pool. An
+// index beyond the scene's textures, or a scene without texture coordinates, means untextured.
+std::vector<PtMaterial> pool_materials(const SceneSlot& s, uint32_t tex_base) {
+  std::vector<PtMaterial> out = s.materials;
+  for (PtMaterial& m : out)
+    m.base_color_texture_index = m.base_color_texture_index < s.textures.size() && !s.texcoords.empty()
+                                     ? m.base_color_texture_index + tex_base
+                                     : 0xFFFFFFFFu;
+  return out;
+}
+
+// Packs every scene's tree, materials, lights, textures and environment into the device pool,
+// rebasing node, triangle, primitive, material and texture references, and writes the scene
+// records. Films are laid out scene after scene (film_base); a tracer's single scene starts at
+// pixel 0. A view shares its scene's pool entries and has its own camera and film.
 int upload_pool(pt_tracer* t) {
   PT_HIP(t, hipStreamSynchronize(t->stream), "synchronize");
   free_scene_device(t);
-  size_t n_nodes = 0, n_tris = 0, n_prims = 0, n_mats = 0, n_lights = 0, n_env = 0, film = 0;
+  size_t n_nodes = 0, n_tris = 0, n_prims = 0, n_mats = 0, n_lights = 0, n_env = 0, n_tex = 0, n_texels = 0;
   uint32_t max_depth = 0;
+  bool any_uv = false;
   for (const SceneSlot& s : t->scenes) {
+    if (s.view_of >= 0) continue;
     n_nodes += s.layout.nodes.size() / 16u;
     n_tris += s.layout.tris.size() / 12u;
     n_prims += s.layout.prims.size() / 4u;
     n_mats += s.materials.size();
     n_lights += s.lights.size();
     n_env += s.environment_map.size();
-    film += static_cast<size_t>(s.width) * s.height;
+    n_tex += s.textures.size();
+    n_texels += s.texels.size() / 3u;
     max_depth = std::max(max_depth, s.bvh.max_depth);
+    any_uv = any_uv || !s.texcoords.empty();
   }
   if (n_tris >= ptb::kMaxPrimitives || n_nodes >= 0x7fffffffu)
     return fail(t, PT_UNSUPPORTED, "scene pool exceeds 2^26 triangles");
+  if (n_texels >= (1ull << 32)) return fail(t, PT_UNSUPPORTED, "texture pool exceeds 2^32 texels");
   std::vector<float> nodes(16u * n_nodes), tris(12u * n_tris), prims(4u * n_prims), env(n_env);
+  std::vector<float> prim_uv(any_uv ? 6u * n_prims : 0u), texels;
+  std::vector<ptd::TexRec> texrecs;
   std::vector<PtMaterial> mats;
   std::vector<PtLight> lights;
   std::vector<ptd::SceneRec> recs(t->scenes.size());
   mats.reserve(n_mats);
   lights.reserve(n_lights);
+  texels.reserve(3u * n_texels);
+  texrecs.reserve(n_tex);
   uint32_t node_base = 0, tri_base = 0, prim_base = 0, env_base = 0, film_base = 0;
-  bool general_boxes = false, any_transmissive = false;
+  bool general_boxes = false, any_transmissive = false, textured = false;
   for (size_t si = 0; si < t->scenes.size(); ++si) {
     const SceneSlot& s = t->scenes[si];
-    const ptb::DeviceLayout& L = s.layout;
-    const uint32_t nn = static_cast<uint32_t>(L.nodes.size() / 16u), nt = static_cast<uint32_t>(L.tris.size() / 12u),
-                   np = static_cast<uint32_t>(L.prims.size() / 4u);
-    const uint32_t mat_base = static_cast<uint32_t>(mats.size()), light_base = static_cast<uint32_t>(lights.size());
-    for (uint32_t i = 0; i < nn; ++i) {
-      const float* src = &L.nodes[16u * i];
-      float* dst = &nodes[16u * (node_base + i)];
-      std::memcpy(dst, src, 12 * sizeof(float));
-      for (int k = 0; k < 12; ++k)
-        if (!std::isfinite(src[k])) general_boxes = true;
-      int32_t r[2];
-      std::memcpy(r, src + 12, sizeof(r));
-      for (int k = 0; k < 2; ++k) r[k] = rebase_ref(r[k], node_base, tri_base);
-      std::memcpy(dst + 12, r, sizeof(r));
-      std::memset(dst + 14, 0, 2 * sizeof(float));
-    }
-    for (uint32_t i = 0; i < nt; ++i) {
-      const float* src = &L.tris[12u * i];
-      float* dst = &tris[12u * (tri_base + i)];
-      std::memcpy(dst, src, 12 * sizeof(float));
-      uint32_t pid;
-      std::memcpy(&pid, src + 3, 4);
-      pid += prim_base;
-      std::memcpy(dst + 3, &pid, 4);
-    }
-    const uint32_t last_mat = static_cast<uint32_t>(s.materials.size()) - 1u;
-    for (uint32_t i = 0; i < np; ++i) {
-      const float* src = &L.prims[4u * i];
-      float* dst = &prims[4u * (prim_base + i)];
-      std::memcpy(dst, src, 3 * sizeof(float));
-      uint32_t m;
-      std::memcpy(&m, src + 3, 4);
-      m = std::min(m, last_mat) + mat_base;  // the shading clamp of a scene-local index, then the pool base
-      std::memcpy(dst + 3, &m, 4);
-    }
-    mats.insert(mats.end(), s.materials.begin(), s.materials.end());
-    lights.insert(lights.end(), s.lights.begin(), s.lights.end());
-    std::copy(s.environment_map.begin(), s.environment_map.end(), env.begin() + env_base);
-
     ptd::SceneRec& R = recs[si];
-    R.root_ref = np ? rebase_ref(L.root_ref, node_base, tri_base) : 0;
-    R.prim_count = np;
-    for (int k = 0; k < 6; ++k) R.root_box[k] = L.root_box[k];
-    R.light_base = light_base;
-    R.light_count = static_cast<uint32_t>(s.lights.size());
-    R.has_transmissive = 0;
-    for (const PtMaterial& m : s.materials)
-      if (transmissive(m)) R.has_transmissive = 1;
-    any_transmissive = any_transmissive || R.has_transmissive;
-    R.env_base = env_base;
-    const bool has_map = !s.environment_map.empty();
-    R.env_w = has_map ? s.view.environment_map_width : 0u;
-    R.env_h = has_map ? s.view.environment_map_height : 0u;
-    for (int k = 0; k < 3; ++k) {
-      R.env[k] = s.view.environment_color[k];
-      R.env_scale[k] = s.view.environment_map_scale[k];
+    if (s.view_of >= 0) {  // the scene's record, with this view's image, film and camera
+      R = recs[static_cast<size_t>(s.view_of)];
+    } else {
+      const ptb::DeviceLayout& L = s.layout;
+      const uint32_t nn = static_cast<uint32_t>(L.nodes.size() / 16u), nt = static_cast<uint32_t>(L.tris.size() / 12u),
+                     np = static_cast<uint32_t>(L.prims.size() / 4u);
+      const uint32_t mat_base = static_cast<uint32_t>(mats.size()), light_base = static_cast<uint32_t>(lights.size());
+      const uint32_t tex_base = static_cast<uint32_t>(texrecs.size());
+      for (uint32_t i = 0; i < nn; ++i) {
+        const float* src = &L.nodes[16u * i];
+        float* dst = &nodes[16u * (node_base + i)];
+        std::memcpy(dst, src, 12 * sizeof(float));
+        for (int k = 0; k < 12; ++k)
+          if (!std::isfinite(src[k])) general_boxes = true;
+        int32_t r[2];
+        std::memcpy(r, src + 12, sizeof(r));
+        for (int k = 0; k < 2; ++k) r[k] = rebase_ref(r[k], node_base, tri_base);
+        std::memcpy(dst + 12, r, sizeof(r));
+        std::memset(dst + 14, 0, 2 * sizeof(float));
+      }
+      for (uint32_t i = 0; i < nt; ++i) {
+        const float* src = &L.tris[12u * i];
+        float* dst = &tris[12u * (tri_base + i)];
+        std::memcpy(dst, src, 12 * sizeof(float));
+        uint32_t pid;
+        std::memcpy(&pid, src + 3, 4);
+        pid += prim_base;
+        std::memcpy(dst + 3, &pid, 4);
+      }
+      const uint32_t last_mat = static_cast<uint32_t>(s.materials.size()) - 1u;
+      for (uint32_t i = 0; i < np; ++i) {
+        const float* src = &L.prims[4u * i];
+        float* dst = &prims[4u * (prim_base + i)];
+        std::memcpy(dst, src, 3 * sizeof(float));
+        uint32_t m;
+        std::memcpy(&m, src + 3, 4);
+        m = std::min(m, last_mat) + mat_base;  // the shading clamp of a scene-local index, then the pool base
+        std::memcpy(dst + 3, &m, 4);
+        if (any_uv && !s.texcoords.empty())
+          std::memcpy(&prim_uv[6u * (prim_base + i)], &L.prim_uv[6u * i], 6 * sizeof(float));
+      }
+      const std::vector<PtMaterial> pm = pool_materials(s, tex_base);
+      for (const PtMaterial& m : pm)
+        if (m.base_color_texture_index != 0xFFFFFFFFu) textured = true;
+      mats.insert(mats.end(), pm.begin(), pm.end());
+      lights.insert(lights.end(), s.lights.begin(), s.lights.end());
+      std::copy(s.environment_map.begin(), s.environment_map.end(), env.begin() + env_base);
+      const uint32_t texel_base = static_cast<uint32_t>(texels.size() / 3u);
+      for (const PtTextureDesc& d : s.textures)
+        texrecs.push_back(ptd::TexRec{texel_base + d.offset, d.width, d.height, 0u});
+      texels.insert(texels.end(), s.texels.begin(), s.texels.end());
+
+      R = ptd::SceneRec{};
+      R.root_ref = np ? rebase_ref(L.root_ref, node_base, tri_base) : 0;
+      R.prim_count = np;
+      for (int k = 0; k < 6; ++k) R.root_box[k] = L.root_box[k];
+      R.light_base = light_base;
+      R.light_count = static_cast<uint32_t>(s.lights.size());
+      R.has_transmissive = 0;
+      for (const PtMaterial& m : s.materials)
+        if (transmissive(m)) R.has_transmissive = 1;
+      any_transmissive = any_transmissive || R.has_transmissive;
+      R.env_base = env_base;
+      const bool has_map = !s.environment_map.empty();
+      R.env_w = has_map ? s.view.environment_map_width : 0u;
+      R.env_h = has_map ? s.view.environment_map_height : 0u;
+      for (int k = 0; k < 3; ++k) {
+        R.env[k] = s.view.environment_color[k];
+        R.env_scale[k] = s.view.environment_map_scale[k];
+      }
+      node_base += nn;
+      tri_base += nt;
+      prim_base += np;
+      env_base += static_cast<uint32_t>(s.environment_map.size());
     }
     R.width = s.width;
     R.height = s.height;
     R.film_base = film_base;
     R.cam = make_camera(s.view.camera, s.width, s.height, t->settings.camera_aperture_radius,
                         t->settings.camera_focus_distance);
-    node_base += nn;
-    tri_base += nt;
-    prim_base += np;
-    env_base += static_cast<uint32_t>(s.environment_map.size());
     film_base += s.width * s.height;
   }
 
@@ -603,6 +685,9 @@ int upload_pool(pt_tracer* t) {
   PT_HIP(t, upload(t->d_lights, lights.data(), lights.size(), sizeof(PtLight)), "upload lights");
   PT_HIP(t, upload(t->d_env, env.data(), env.size(), sizeof(float)), "upload environment");
   PT_HIP(t, upload(t->d_recs, recs.data(), recs.size(), sizeof(ptd::SceneRec)), "upload scene records");
+  PT_HIP(t, upload(t->d_prim_uv, prim_uv.data(), prim_uv.size(), sizeof(float)), "upload texture coordinates");
+  PT_HIP(t, upload(t->d_texels, texels.data(), texels.size(), sizeof(float)), "upload textures");
+  PT_HIP(t, upload(t->d_texrecs, texrecs.data(), texrecs.size(), sizeof(ptd::TexRec)), "upload texture table");
 
   // Traversal stack spill area: a lane's stack never holds more than max_depth entries (one per
   // level of the current path); entries older than the LDS ring's kStackLds go here.
@@ -632,12 +717,19 @@ int upload_pool(pt_tracer* t) {
   S.env_map = t->d_env.ptr;
   S.recs = t->d_recs.ptr;
   S.scene_count = static_cast<uint32_t>(recs.size());
+  S.texels = t->d_texels.ptr;
+  S.textures = t->d_texrecs.ptr;
+  S.texture_count = static_cast<uint32_t>(texrecs.size());
+  S.textured = textured ? 1u : 0u;
+  S.prim_uv = any_uv ? t->d_prim_uv.ptr : nullptr;
+  t->texel_count = texels.size() / 3u;
 
   t->info = PtBuildInfo{};
   t->info.device_bytes = bytes;
   t->info.built = 1;
   t->info.deterministic = 1;
   for (const SceneSlot& s : t->scenes) {
+    if (s.view_of >= 0) continue;
     t->info.build_ms += s.info.build_ms;
     t->info.primitive_count += s.info.primitive_count;
     t->info.node_count += s.info.node_count;
@@ -948,6 +1040,7 @@ uint32_t pt_hip_sizeof(int which) {
     case 6: return sizeof(PtSampleCounters);
     case 7: return sizeof(PtRenderTile);
     case 8: return sizeof(PtTuning);
+    case 9: return sizeof(PtTextureDesc);
     default: return 0;
   }
 }
@@ -1010,6 +1103,35 @@ int pt_hip_load_scene(pt_tracer* t, const PtSceneView* sc) {
   s.height = t->settings.height;
   t->scene_loaded = true;
   t->accel_valid = false;
+  ++t->scene_version;
+  t->last_error.clear();
+  return PT_OK;
+}
+
+int pt_hip_load_textures(pt_tracer* t, const float* texcoords, uint32_t texcoord_count, const float* texels,
+                         uint64_t texel_count, const PtTextureDesc* textures, uint32_t texture_count) {
+  if (!t) return PT_INVALID_ARGUMENT;
+  if (t->batch) return fail(t, PT_INVALID_ARGUMENT, "load_textures: use pt_batch_load_textures on a batch");
+  if (!t->scene_loaded) return fail(t, PT_NOT_READY, "load_textures called before load_scene_snapshot");
+  const int rc = copy_textures(t, t->scenes[0], texcoords, texcoord_count, texels, texel_count, textures, texture_count);
+  if (rc != PT_OK) return rc;
+  t->accel_valid = false;
+  ++t->scene_version;
+  t->last_error.clear();
+  return PT_OK;
+}
+
+int pt_hip_update_texels(pt_tracer* t, const float* texels, uint64_t texel_count) {
+  if (!t) return PT_INVALID_ARGUMENT;
+  if (!t->accel_valid) return fail(t, PT_NOT_READY, "update_texels requires an established acceleration");
+  if (texel_count != t->texel_count) return fail(t, PT_INVALID_ARGUMENT, "update_texels: texel count differs");
+  if (texel_count && !texels) return fail(t, PT_INVALID_ARGUMENT, "update_texels: null texels");
+  (void)hipSetDevice(t->device);
+  if (texel_count)
+    PT_HIP(t,
+           hipMemcpyAsync(t->d_texels.ptr, texels, 3u * sizeof(float) * texel_count, hipMemcpyDefault, t->stream),
+           "upload texels");
+  PT_HIP(t, hipStreamSynchronize(t->stream), "synchronize");
   ++t->scene_version;
   t->last_error.clear();
   return PT_OK;
@@ -1272,11 +1394,15 @@ int pt_hip_update_scene_delta(pt_tracer* t, const PtMaterial* materials, uint32_
   PT_HIP(t, hipStreamSynchronize(t->stream), "synchronize");
   if (materials) {
     s.materials.assign(materials, materials + material_count);
-    PT_HIP(t, hipMemcpy(t->d_materials.ptr, materials, sizeof(PtMaterial) * material_count, hipMemcpyHostToDevice),
+    const std::vector<PtMaterial> pm = pool_materials(s, 0u);
+    PT_HIP(t, hipMemcpy(t->d_materials.ptr, pm.data(), sizeof(PtMaterial) * material_count, hipMemcpyHostToDevice),
            "upload materials");
-    uint32_t any = 0;
+    uint32_t any = 0, textured = 0;
     for (const PtMaterial& m : s.materials)
       if (transmissive(m)) any = 1;
+    for (const PtMaterial& m : pm)
+      if (m.base_color_texture_index != 0xFFFFFFFFu) textured = 1;
+    t->dscene.textured = textured;
     t->dscene.has_transmissive = any;
     PT_HIP(t, hipMemcpy(&t->d_recs.ptr[0].has_transmissive, &any, sizeof(uint32_t), hipMemcpyHostToDevice),
            "upload scene record");
@@ -1291,34 +1417,26 @@ int pt_hip_update_scene_delta(pt_tracer* t, const PtMaterial* materials, uint32_
   return PT_OK;
 }
 
-int pt_hip_render_adjoint(pt_tracer* t, const PtRenderTile* tiles, uint32_t count, uint64_t frame_index,
-                          const float* film_grad, float* grad_albedo, float* grad_emission, float* grad_light,
-                          float* grad_roughness) {
-  if (!t) return PT_INVALID_ARGUMENT;
-  if (t->batch) return fail(t, PT_INVALID_ARGUMENT, "render_adjoint: not available on a batch");
-  if (!t->configured || !t->scene_loaded || !t->accel_valid)
-    return fail(t, PT_NOT_READY, "render_adjoint requires a built acceleration structure");
-  if (count && !tiles) return fail(t, PT_INVALID_ARGUMENT, "render_adjoint: null tiles");
-  const SceneSlot& s = t->scenes[0];
-  if (!film_grad || !grad_albedo || !grad_emission || (!grad_light && !s.lights.empty()))
+// Path-replay adjoint of `work` (one tracer's tiles or a batch frame). Gradient buffers are indexed
+// by pool material, light and texel; film_grad by pool film pixel.
+static int run_adjoint(pt_tracer* t, std::vector<ptd::DevTile>& work, uint64_t frame_index, const float* film_grad,
+                float* grad_albedo, float* grad_emission, float* grad_light, float* grad_roughness,
+                float* grad_texels) {
+  if (!film_grad || !grad_albedo || !grad_emission || (!grad_light && t->dscene.light_count))
     return fail(t, PT_INVALID_ARGUMENT, "render_adjoint: null gradient buffer");
   if (t->settings.rr_start_depth > static_cast<uint32_t>(ptd::kAdjHist))
     return fail(t, PT_UNSUPPORTED, "render_adjoint supports russian roulette start depths up to 4");
-  (void)hipSetDevice(t->device);
-  std::vector<ptd::DevTile> work;
-  int rc = build_work(t, 0u, tiles, count, work);
-  if (rc != PT_OK) return rc;
   ptd::DevAdjoint A{};
   A.g_albedo = grad_albedo;
   A.g_emission = grad_emission;
   A.g_light = grad_light;
   A.g_rough = grad_roughness;
+  A.g_tex = t->texel_count ? grad_texels : nullptr;
   A.film_grad = film_grad;
-  A.film_width = t->settings.width;
-  A.nmat = static_cast<uint32_t>(s.materials.size());
-  A.nlight = static_cast<uint32_t>(s.lights.size());
+  A.nmat = t->dscene.material_count;
+  A.nlight = t->dscene.light_count;
   A.use_lds = (7u * A.nmat + 3u * A.nlight) <= static_cast<uint32_t>(ptd::kAdjLdsFloats) ? 1u : 0u;
-  rc = run_passes(t, work, dev_settings(t, frame_index), &A);
+  int rc = run_passes(t, work, dev_settings(t, frame_index), &A);
   if (rc != PT_OK) return rc;
   if (t->profiling) {
     rc = sync_all(t);
@@ -1326,6 +1444,22 @@ int pt_hip_render_adjoint(pt_tracer* t, const PtRenderTile* tiles, uint32_t coun
   }
   t->last_error.clear();
   return PT_OK;
+}
+
+int pt_hip_render_adjoint(pt_tracer* t, const PtRenderTile* tiles, uint32_t count, uint64_t frame_index,
+                          const float* film_grad, float* grad_albedo, float* grad_emission, float* grad_light,
+                          float* grad_roughness, float* grad_texels) {
+  if (!t) return PT_INVALID_ARGUMENT;
+  if (t->batch) return fail(t, PT_INVALID_ARGUMENT, "render_adjoint: use pt_batch_render_adjoint on a batch");
+  if (!t->configured || !t->scene_loaded || !t->accel_valid)
+    return fail(t, PT_NOT_READY, "render_adjoint requires a built acceleration structure");
+  if (count && !tiles) return fail(t, PT_INVALID_ARGUMENT, "render_adjoint: null tiles");
+  (void)hipSetDevice(t->device);
+  std::vector<ptd::DevTile> work;
+  const int rc = build_work(t, 0u, tiles, count, work);
+  if (rc != PT_OK) return rc;
+  return run_adjoint(t, work, frame_index, film_grad, grad_albedo, grad_emission, grad_light, grad_roughness,
+                     grad_texels);
 }
 
 // ------------------------------------------------------------------ batches
@@ -1385,6 +1519,44 @@ int pt_batch_add_scene(pt_batch* b, const PtSceneView* scene, uint32_t width, ui
   return PT_OK;
 }
 
+int pt_batch_add_view(pt_batch* b, uint32_t scene, const PtCamera* camera, uint32_t width, uint32_t height,
+                      uint32_t spp, uint32_t* index) {
+  if (!b) return PT_INVALID_ARGUMENT;
+  pt_tracer* t = &b->t;
+  if (!camera) return fail(t, PT_INVALID_ARGUMENT, "add_view: null camera");
+  if (scene >= t->scenes.size() || t->scenes[scene].view_of >= 0)
+    return fail(t, PT_INVALID_ARGUMENT, "add_view: not a scene of the batch");
+  if (width == 0 || height == 0 || spp == 0) return fail(t, PT_INVALID_ARGUMENT, "add_view: empty image or spp");
+  t->scenes.emplace_back();
+  SceneSlot& s = t->scenes.back();
+  const SceneSlot& base = t->scenes[scene];
+  s.view = base.view;
+  s.view.camera = *camera;
+  s.view_of = static_cast<int32_t>(scene);
+  s.width = width;
+  s.height = height;
+  s.spp = spp;
+  t->accel_valid = false;
+  if (index) *index = static_cast<uint32_t>(t->scenes.size() - 1u);
+  t->last_error.clear();
+  return PT_OK;
+}
+
+int pt_batch_load_textures(pt_batch* b, uint32_t scene, const float* texcoords, uint32_t texcoord_count,
+                           const float* texels, uint64_t texel_count, const PtTextureDesc* textures,
+                           uint32_t texture_count) {
+  if (!b) return PT_INVALID_ARGUMENT;
+  pt_tracer* t = &b->t;
+  if (scene >= t->scenes.size() || t->scenes[scene].view_of >= 0)
+    return fail(t, PT_INVALID_ARGUMENT, "load_textures: not a scene of the batch");
+  const int rc =
+      copy_textures(t, t->scenes[scene], texcoords, texcoord_count, texels, texel_count, textures, texture_count);
+  if (rc != PT_OK) return rc;
+  t->accel_valid = false;
+  t->last_error.clear();
+  return PT_OK;
+}
+
 int pt_batch_build(pt_batch* b, PtBuildInfo* info) {
   if (!b) return PT_INVALID_ARGUMENT;
   pt_tracer* t = &b->t;
@@ -1392,8 +1564,9 @@ int pt_batch_build(pt_batch* b, PtBuildInfo* info) {
   (void)hipSetDevice(t->device);
   const auto t0 = std::chrono::steady_clock::now();
   // host builds in parallel, largest scenes first
-  std::vector<size_t> order(t->scenes.size());
-  for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+  std::vector<size_t> order;
+  for (size_t i = 0; i < t->scenes.size(); ++i)
+    if (t->scenes[i].view_of < 0) order.push_back(i);
   std::sort(order.begin(), order.end(),
             [&](size_t a, size_t c) { return t->scenes[a].indices.size() > t->scenes[c].indices.size(); });
   const unsigned nthreads = std::max(1u, std::min<unsigned>(16u, std::thread::hardware_concurrency()));
@@ -1424,12 +1597,72 @@ int pt_batch_build(pt_batch* b, PtBuildInfo* info) {
 
 int pt_batch_reset_accumulation(pt_batch* b) { return b ? pt_hip_reset_accumulation(&b->t) : PT_INVALID_ARGUMENT; }
 
-int pt_batch_render(pt_batch* b, uint64_t frame_index, int use_graph) {
+int pt_batch_update_scene_delta(pt_batch* b, const PtMaterial* materials, uint32_t material_count,
+                                const PtLight* lights, uint32_t light_count) {
   if (!b) return PT_INVALID_ARGUMENT;
   pt_tracer* t = &b->t;
-  if (!t->accel_valid) return fail(t, PT_NOT_READY, "batch_render requires pt_batch_build");
+  if (!t->accel_valid) return fail(t, PT_NOT_READY, "batch_update_scene_delta requires pt_batch_build");
+  if (materials && material_count != t->dscene.material_count)
+    return fail(t, PT_INVALID_ARGUMENT, "batch_update_scene_delta: material count differs from the pool's");
+  if (lights && light_count != t->dscene.light_count)
+    return fail(t, PT_INVALID_ARGUMENT, "batch_update_scene_delta: light count differs from the pool's");
   (void)hipSetDevice(t->device);
-  std::vector<ptd::DevTile> work;
+  PT_HIP(t, hipStreamSynchronize(t->stream), "synchronize");
+  if (materials) {
+    std::vector<PtMaterial> pool;
+    pool.reserve(material_count);
+    std::vector<ptd::SceneRec> recs(t->scenes.size());
+    PT_HIP(t, hipMemcpy(recs.data(), t->d_recs.ptr, recs.size() * sizeof(ptd::SceneRec), hipMemcpyDeviceToHost),
+           "read scene records");
+    size_t at = 0;
+    uint32_t tex_base = 0, any = 0, textured = 0;
+    for (size_t si = 0; si < t->scenes.size(); ++si) {
+      SceneSlot& sl = t->scenes[si];
+      if (sl.view_of >= 0) {
+        recs[si].has_transmissive = recs[static_cast<size_t>(sl.view_of)].has_transmissive;
+        continue;
+      }
+      sl.materials.assign(materials + at, materials + at + sl.materials.size());
+      at += sl.materials.size();
+      const std::vector<PtMaterial> pm = pool_materials(sl, tex_base);
+      tex_base += static_cast<uint32_t>(sl.textures.size());
+      pool.insert(pool.end(), pm.begin(), pm.end());
+      recs[si].has_transmissive = 0;
+      for (const PtMaterial& m : sl.materials)
+        if (transmissive(m)) recs[si].has_transmissive = 1;
+      any |= recs[si].has_transmissive;
+      for (const PtMaterial& m : pm)
+        if (m.base_color_texture_index != 0xFFFFFFFFu) textured = 1;
+    }
+    PT_HIP(t, hipMemcpy(t->d_materials.ptr, pool.data(), sizeof(PtMaterial) * pool.size(), hipMemcpyHostToDevice),
+           "upload materials");
+    PT_HIP(t, hipMemcpy(t->d_recs.ptr, recs.data(), recs.size() * sizeof(ptd::SceneRec), hipMemcpyHostToDevice),
+           "upload scene records");
+    t->dscene.has_transmissive = any;
+    t->dscene.textured = textured;
+    drop_graph(t);  // the captured launches hold the scene flags by value
+  }
+  if (lights && light_count) {
+    size_t at = 0;
+    for (SceneSlot& sl : t->scenes) {
+      if (sl.view_of >= 0) continue;
+      sl.lights.assign(lights + at, lights + at + sl.lights.size());
+      at += sl.lights.size();
+    }
+    PT_HIP(t, hipMemcpy(t->d_lights.ptr, lights, sizeof(PtLight) * light_count, hipMemcpyHostToDevice),
+           "upload lights");
+  }
+  ++t->scene_version;
+  t->last_error.clear();
+  return PT_OK;
+}
+
+int pt_batch_update_texels(pt_batch* b, const float* texels, uint64_t texel_count) {
+  return b ? pt_hip_update_texels(&b->t, texels, texel_count) : PT_INVALID_ARGUMENT;
+}
+
+// A batch frame: samples 0..spp-1 of every scene's (and view's) full image, scene after scene.
+static int batch_frame_work(pt_tracer* t, std::vector<ptd::DevTile>& work) {
   for (uint32_t si = 0; si < t->scenes.size(); ++si) {
     const SceneSlot& s = t->scenes[si];
     std::vector<PtRenderTile> tiles;
@@ -1437,6 +1670,30 @@ int pt_batch_render(pt_batch* b, uint64_t frame_index, int use_graph) {
     const int rc = build_work(t, si, tiles.data(), static_cast<uint32_t>(tiles.size()), work);
     if (rc != PT_OK) return rc;
   }
+  return PT_OK;
+}
+
+int pt_batch_render_adjoint(pt_batch* b, uint64_t frame_index, const float* film_grad, float* grad_albedo,
+                            float* grad_emission, float* grad_light, float* grad_roughness, float* grad_texels) {
+  if (!b) return PT_INVALID_ARGUMENT;
+  pt_tracer* t = &b->t;
+  if (!t->accel_valid) return fail(t, PT_NOT_READY, "batch_render_adjoint requires pt_batch_build");
+  (void)hipSetDevice(t->device);
+  std::vector<ptd::DevTile> work;
+  const int rc = batch_frame_work(t, work);
+  if (rc != PT_OK) return rc;
+  return run_adjoint(t, work, frame_index, film_grad, grad_albedo, grad_emission, grad_light, grad_roughness,
+                     grad_texels);
+}
+
+int pt_batch_render(pt_batch* b, uint64_t frame_index, int use_graph) {
+  if (!b) return PT_INVALID_ARGUMENT;
+  pt_tracer* t = &b->t;
+  if (!t->accel_valid) return fail(t, PT_NOT_READY, "batch_render requires pt_batch_build");
+  (void)hipSetDevice(t->device);
+  std::vector<ptd::DevTile> work;
+  int wrc = batch_frame_work(t, work);
+  if (wrc != PT_OK) return wrc;
   ptd::DevSettings st = dev_settings(t, frame_index);
   if (!use_graph) {
     const int rc = run_passes(t, work, st, nullptr);

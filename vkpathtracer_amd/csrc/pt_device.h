@@ -77,6 +77,12 @@ struct SceneRec {
   DevCamera cam;
 };
 
+// One texture of the pool: width x height rgb fp32 texels starting at texel `offset` of
+// DevScene::texels (3 floats per texel), rows in increasing v.
+struct TexRec {
+  uint32_t offset, width, height, pad;
+};
+
 // The scene pool as kernels see it: all scenes' nodes, triangles and primitives (node and
 // primitive references rebased into the pool, material indices clamped per scene and rebased),
 // materials and lights. Inside shading a lane works on a copy whose lights, light_count,
@@ -95,6 +101,11 @@ struct DevScene {
   float env_scale[3];
   const SceneRec* recs;
   uint32_t scene_count;
+  const float* texels;     // texture pool (rgb)
+  const TexRec* textures;  // material base_color_texture_index values are pool texture indices
+  uint32_t texture_count;
+  uint32_t textured;       // some material has a texture: extend also writes the hit barycentrics
+  const float* prim_uv;    // 6 floats per pool primitive: texture coordinates of v0, v1, v2
 };
 
 // Integrator settings shared by every scene of a launch (image sizes are per scene: SceneRec).
@@ -133,6 +144,8 @@ struct DevPaths {
   DevRays rays[2];
   float* hit_t;
   int32_t* hit_prim;
+  float* hit_u;          // barycentrics of the hit (textured pools only)
+  float* hit_v;
   float* tp[3];          // throughput
   float* rad[3];         // radiance
   uint64_t* rng;
@@ -179,12 +192,16 @@ struct DevAdjoint {
   uint32_t* s_light;
   float* p_score;   // d log(density of the pending bounce's direction) / d roughness
   float* s_grough;  // per shadow record: d contribution / d roughness, weighted by the loss adjoint
+  float* p_uv[2];   // texture coordinates of the pending bounce's, the history's and the shadow
+  float* h_uv[kAdjHist][2];  // records' vertices (read for textured materials only)
+  float* s_uv[2];
   float* g_albedo;    // 3 per material
   float* g_emission;  // 3 per material
   float* g_light;     // 3 per light (colour)
   float* g_rough;     // 1 per material (nullptr: not requested)
-  const float* film_grad;  // 3 per pixel: dLoss / d(film sum)
-  uint32_t film_width, nmat, nlight, use_lds;
+  float* g_tex;       // 3 per texel of the texture pool (nullptr: not requested)
+  const float* film_grad;  // 3 per film pixel (all scenes' films in pool order): dLoss / d(film sum)
+  uint32_t nmat, nlight, use_lds;
 };
 
 // Launchers (pt_kernels.hip). All launch on `stream`; none synchronizes.

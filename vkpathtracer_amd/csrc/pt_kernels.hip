@@ -150,7 +150,8 @@ struct TravStats {
 
 // Moller-Trumbore against precomputed edges, intersect_scalar (BvhTriangleIntersector.cpp:35-74).
 // True when eps < t < best_t.
-__device__ __forceinline__ bool tri_test(const float4* tp, V3 o, V3 d, float best_t, float& t_out) {
+__device__ __forceinline__ bool tri_test(const float4* tp, V3 o, V3 d, float best_t, float& t_out, float& u_out,
+                                         float& v_out) {
   const float4 v0 = tp[0], e1 = tp[1], e2 = tp[2];
   const float hx = d.y * e2.z - d.z * e2.y;
   const float hy = d.z * e2.x - d.x * e2.z;
@@ -168,6 +169,8 @@ __device__ __forceinline__ bool tri_test(const float4* tp, V3 o, V3 d, float bes
   if (v < 0.0f || u + v > 1.0f) return false;
   const float t = (e2.x * qx + e2.y * qy + e2.z * qz) * inv_det;
   t_out = t;
+  u_out = u;
+  v_out = v;
   return !(t <= kEps || t >= best_t);
 }
 
@@ -189,6 +192,7 @@ struct Trav {
   V3 d;
   float best_t;
   int32_t best_prim, cur;
+  float best_u, best_v;  // barycentrics of the best hit
   uint32_t sp;  // entries on the stack
   uint32_t lo;  // entries [lo, sp) are in the LDS ring, [0, lo) in the global area
 };
@@ -198,6 +202,8 @@ struct Trav {
 __device__ __forceinline__ bool trav_begin(const SceneRec& R, Trav& s, V3 o, V3 d, float tmax, TravStats& st) {
   s.best_prim = -1;
   s.best_t = tmax;
+  s.best_u = 0.0f;
+  s.best_v = 0.0f;
   s.d = d;
   s.sp = 0;
   s.lo = 0;
@@ -281,11 +287,13 @@ __device__ __forceinline__ int trav_leaf(const DevScene& sc, Trav& s, TravStats&
       break;
     }
     ++st.tests;
-    float t;
+    float t, bu, bv;
     const float4* tp = sc.tris + 3u * (first + k);
-    if (tri_test(tp, o, s.d, s.best_t, t)) {  // commit_ordered_lane_hit, BvhTriangleIntersector.cpp:25-33
+    if (tri_test(tp, o, s.d, s.best_t, t, bu, bv)) {  // commit_ordered_lane_hit, BvhTriangleIntersector.cpp:25-33
       ++accepted;
       s.best_t = t;
+      s.best_u = bu;
+      s.best_v = bv;
       s.best_prim = __float_as_int(tp[0].w);
     }
     // an any-hit query stops at the end of the first batch that accepted a hit
@@ -553,7 +561,7 @@ __global__ __launch_bounds__(kBlock) void k_raygen(DevPaths P, DevRays out, DevS
   P.rad[0][p] = 0.0f; P.rad[1][p] = 0.0f; P.rad[2][p] = 0.0f;
   P.rng[p] = prng;
   if constexpr (kAdj) {
-    const size_t pix = static_cast<size_t>(y) * A.film_width + x;
+    const size_t pix = rec.film_base + static_cast<size_t>(y) * rec.width + x;
     A.delta[0][p] = A.film_grad[3 * pix];
     A.delta[1][p] = A.film_grad[3 * pix + 1];
     A.delta[2][p] = A.film_grad[3 * pix + 2];
@@ -613,30 +621,6 @@ __device__ __forceinline__ void grad_lds_flush(const DevAdjoint& A, const float*
   }
 }
 
-// Apply the bounce that waited for its remainder radiance R = L_total - L_so_far
-// (df/dx_j = delta*R_j plus the Russian-roulette terms; see the oracle's derivation).
-__device__ __forceinline__ void adj_flush(const DevAdjoint& A, float* lds, uint32_t p, V3 L, uint32_t& flags) {
-  if (!(flags & 1u)) return;
-  const float dl[3] = {A.delta[0][p], A.delta[1][p], A.delta[2][p]};
-  const float R[3] = {A.ltot[0][p] - L.x, A.ltot[1][p] - L.y, A.ltot[2][p] - L.z};
-  const float S = dl[0] * R[0] + dl[1] * R[1] + dl[2] * R[2];
-  const uint32_t m = A.p_mat[p];
-  for (int c = 0; c < 3; ++c) gadd(A, lds, kGradAlbedo, m, c, dl[c] * R[c] * A.p_dx[c][p]);
-  gadd(A, lds, kGradRough, m, 0, S * A.p_score[p]);
-  if (flags & 2u) {
-    const int cs = static_cast<int>((flags >> 2) & 3u);
-    const uint32_t nh = (flags >> 4) & 7u;
-    for (uint32_t h = 0; h < nh; ++h) gadd(A, lds, kGradAlbedo, A.h_mat[h][p], cs, -S * A.h_dx[h][cs][p]);
-    if (flags & 0x80u) {
-      const int pc = static_cast<int>((flags >> 8) & 3u);
-      const uint32_t pn = (flags >> 10) & 7u;
-      for (uint32_t h = 0; h < pn; ++h) gadd(A, lds, kGradAlbedo, A.h_mat[h][p], pc, S * A.h_dx[h][pc][p]);
-    }
-    flags = (flags & ~0x1F80u) | 0x80u | (static_cast<uint32_t>(cs) << 8) | (nh << 10);
-  }
-  flags &= ~1u;
-}
-
 __device__ __forceinline__ Stack make_stack(const DevPaths& P, int32_t* s_ref, float* s_t) {
   Stack s;
   s.lds_ref = s_ref;
@@ -671,6 +655,10 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevPaths P, DevRays in, DevSc
   auto end = [&](uint32_t, Trav& s) {
     P.hit_t[pos] = s.best_t;
     P.hit_prim[pos] = s.best_prim;
+    if (sc.textured) {
+      P.hit_u[pos] = s.best_u;
+      P.hit_v[pos] = s.best_v;
+    }
     ++rays;
     return false;
   };
@@ -694,6 +682,113 @@ __device__ __forceinline__ DevScene scene_view(const DevScene& pool, uint32_t si
   v.env_w = R.env_w;
   v.env_h = R.env_h;
   return v;
+}
+
+// ------------------------------------------------------------------ textures
+constexpr uint32_t kNoTexture = 0xFFFFFFFFu;
+
+// Texture coordinates at barycentrics (u, v) of a primitive with vertex coordinates uvp[0..5].
+__device__ __forceinline__ float2 uv_at(const float* uvp, float u, float v) {
+  const float w = 1.0f - u - v;
+  return make_float2(uvp[0] * w + uvp[2] * u + uvp[4] * v, uvp[1] * w + uvp[3] * u + uvp[5] * v);
+}
+
+// A bilinear tap (repeat addressing, texel centres at half-integers): its four texels (texel
+// indices into the pool, float offset / 3), their weights and the filtered colour.
+struct TexTap {
+  uint32_t tex;
+  uint32_t idx[4];
+  float w[4];
+  V3 color;
+};
+
+__device__ __forceinline__ TexTap tex_tap(const DevScene& sc, uint32_t ti, float2 uv) {
+  TexTap r;
+  r.tex = ti;
+  const TexRec T = sc.textures[ti];
+  float uf = uv.x - __builtin_floorf(uv.x), vf = uv.y - __builtin_floorf(uv.y);
+  if (!(uf >= 0.0f && uf < 1.0f)) uf = 0.0f;  // NaN, or a huge coordinate rounding to 1
+  if (!(vf >= 0.0f && vf < 1.0f)) vf = 0.0f;
+  const float x = uf * static_cast<float>(T.width) - 0.5f, y = vf * static_cast<float>(T.height) - 0.5f;
+  const float x0 = __builtin_floorf(x), y0 = __builtin_floorf(y);
+  const float fx = x - x0, fy = y - y0;
+  int ix0 = static_cast<int>(x0), iy0 = static_cast<int>(y0);
+  if (ix0 < 0) ix0 += static_cast<int>(T.width);
+  if (iy0 < 0) iy0 += static_cast<int>(T.height);
+  const int ix1 = ix0 + 1 == static_cast<int>(T.width) ? 0 : ix0 + 1;
+  const int iy1 = iy0 + 1 == static_cast<int>(T.height) ? 0 : iy0 + 1;
+  const uint32_t base = T.offset;
+  r.idx[0] = base + static_cast<uint32_t>(iy0) * T.width + static_cast<uint32_t>(ix0);
+  r.idx[1] = base + static_cast<uint32_t>(iy0) * T.width + static_cast<uint32_t>(ix1);
+  r.idx[2] = base + static_cast<uint32_t>(iy1) * T.width + static_cast<uint32_t>(ix0);
+  r.idx[3] = base + static_cast<uint32_t>(iy1) * T.width + static_cast<uint32_t>(ix1);
+  r.w[0] = (1.0f - fx) * (1.0f - fy);
+  r.w[1] = fx * (1.0f - fy);
+  r.w[2] = (1.0f - fx) * fy;
+  r.w[3] = fx * fy;
+  V3 c = mk(0.0f, 0.0f, 0.0f);
+  for (int k = 0; k < 4; ++k) c = add(c, muls(ld3(sc.texels + 3u * r.idx[k]), r.w[k]));
+  r.color = c;
+  return r;
+}
+
+// Adds G = df/d(base colour) of a vertex on pool material m whose texture coordinate is uv: base =
+// albedo * texture tap (tap colour 1 when untextured), so df/dalbedo_c = G_c * tap_c and
+// df/dtexel_{j,c} = G_c * albedo_c * w_j for the tap's four texels (global fp32 atomics).
+__device__ __forceinline__ void gadd_base(const DevAdjoint& A, const DevScene& sc, float* lds, uint32_t m, float2 uv,
+                                          V3 G) {
+  const PtMaterial& mat = sc.materials[m];
+  const float g[3] = {G.x, G.y, G.z};
+  if (mat.base_color_texture_index >= sc.texture_count) {
+    for (int c = 0; c < 3; ++c) gadd(A, lds, kGradAlbedo, m, c, g[c]);
+    return;
+  }
+  const TexTap tap = tex_tap(sc, mat.base_color_texture_index, uv);
+  const float tc[3] = {tap.color.x, tap.color.y, tap.color.z};
+  for (int c = 0; c < 3; ++c) gadd(A, lds, kGradAlbedo, m, c, g[c] * tc[c]);
+  if (!A.g_tex) return;
+  for (int c = 0; c < 3; ++c) {
+    const float ga = g[c] * mat.albedo[c];
+    if (ga == 0.0f) continue;
+    for (int j = 0; j < 4; ++j) {
+      const float v = ga * tap.w[j];
+      if (v != 0.0f) atomicAdd(A.g_tex + 3u * tap.idx[j] + static_cast<uint32_t>(c), v);
+    }
+  }
+}
+
+__device__ __forceinline__ V3 channel(int c, float v) {
+  return mk(c == 0 ? v : 0.0f, c == 1 ? v : 0.0f, c == 2 ? v : 0.0f);
+}
+
+// Apply the bounce that waited for its remainder radiance R = L_total - L_so_far
+// (df/dx_j = delta*R_j plus the Russian-roulette terms; see the oracle's derivation).
+__device__ __forceinline__ void adj_flush(const DevAdjoint& A, const DevScene& sc, float* lds, uint32_t p, V3 L,
+                                          uint32_t& flags) {
+  if (!(flags & 1u)) return;
+  const float dl[3] = {A.delta[0][p], A.delta[1][p], A.delta[2][p]};
+  const float R[3] = {A.ltot[0][p] - L.x, A.ltot[1][p] - L.y, A.ltot[2][p] - L.z};
+  const float S = dl[0] * R[0] + dl[1] * R[1] + dl[2] * R[2];
+  const uint32_t m = A.p_mat[p];
+  gadd_base(A, sc, lds, m, make_float2(A.p_uv[0][p], A.p_uv[1][p]),
+            mk(dl[0] * R[0] * A.p_dx[0][p], dl[1] * R[1] * A.p_dx[1][p], dl[2] * R[2] * A.p_dx[2][p]));
+  gadd(A, lds, kGradRough, m, 0, S * A.p_score[p]);
+  if (flags & 2u) {
+    const int cs = static_cast<int>((flags >> 2) & 3u);
+    const uint32_t nh = (flags >> 4) & 7u;
+    for (uint32_t h = 0; h < nh; ++h)
+      gadd_base(A, sc, lds, A.h_mat[h][p], make_float2(A.h_uv[h][0][p], A.h_uv[h][1][p]),
+                channel(cs, -S * A.h_dx[h][cs][p]));
+    if (flags & 0x80u) {
+      const int pc = static_cast<int>((flags >> 8) & 3u);
+      const uint32_t pn = (flags >> 10) & 7u;
+      for (uint32_t h = 0; h < pn; ++h)
+        gadd_base(A, sc, lds, A.h_mat[h][p], make_float2(A.h_uv[h][0][p], A.h_uv[h][1][p]),
+                  channel(pc, S * A.h_dx[h][pc][p]));
+    }
+    flags = (flags & ~0x1F80u) | 0x80u | (static_cast<uint32_t>(cs) << 8) | (nh << 10);
+  }
+  flags &= ~1u;
 }
 
 struct MatFlags {
@@ -771,8 +866,8 @@ __device__ V3 wood_albedo(V3 base, V3 p, float roughness, float clearcoat, uint3
 
 // apply_material_effect, ScalarCpuPathTracer.cpp:264-355. Affine in the albedo before the final
 // [0, 1.5] clamp: k is d(colour)/d(albedo) there and pre the unclamped colour (for the adjoint).
-__device__ V3 effect_albedo(const PtMaterial& m, V3 pos, V3 n, V3 incoming, float& k, V3& pre) {
-  V3 c = ld3(m.albedo);
+__device__ V3 effect_albedo(const PtMaterial& m, V3 base, V3 pos, V3 n, V3 incoming, float& k, V3& pre) {
+  V3 c = base;
   k = 1.0f;
   const uint32_t e = m.material_effect;
   if (e != 0u) {
@@ -1041,6 +1136,7 @@ struct ShadeOut {
   float stmax;
   V3 galb, glight;    // adjoint: gradient pieces applied if the light is visible
   float grough;
+  float2 guv;         // texture coordinate of the vertex (textured materials)
   uint32_t gmat, glight_index;
 };
 
@@ -1051,8 +1147,8 @@ __device__ __forceinline__ float eff_grad(float pre_c, float k) { return (pre_c 
 // arrived at a hit (prim >= 0) or a miss (prim < 0). kAdj replays the iteration for the adjoint.
 template <bool kAdj>
 __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc, const DevSettings& st, uint32_t depth,
-                                           uint32_t p, V3 ro, V3 rd, float t, int32_t prim, ShadeOut& out,
-                                           uint32_t& shadow_tests, const DevAdjoint& A, float* glds) {
+                                           uint32_t p, V3 ro, V3 rd, float t, int32_t prim, float hu, float hv,
+                                           ShadeOut& out, uint32_t& shadow_tests, const DevAdjoint& A, float* glds) {
   out.cont = false;
   out.shadow = false;
   V3 T = mk(P.tp[0][p], P.tp[1][p], P.tp[2][p]);
@@ -1062,7 +1158,7 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
   if constexpr (kAdj) {
     if (!(finite(A.ltot[0][p]) && finite(A.ltot[1][p]) && finite(A.ltot[2][p]))) return;  // dropped sample
     aflags = A.flags[p];
-    adj_flush(A, glds, p, L, aflags);
+    adj_flush(A, sc, glds, p, L, aflags);
     dl = mk(A.delta[0][p], A.delta[1][p], A.delta[2][p]);
   }
   if (prim < 0) {
@@ -1081,7 +1177,15 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
   const PtMaterial& m = sc.materials[mat_index];
   float eff_k;
   V3 eff_pre;
-  const V3 albedo = effect_albedo(m, pos, sN, rd, eff_k, eff_pre);
+  // the material's base colour, times its texture at the hit (no counterpart on the reference's
+  // CPU path, which renders untextured)
+  V3 base = ld3(m.albedo);
+  float2 huv = make_float2(0.0f, 0.0f);
+  if (m.base_color_texture_index < sc.texture_count) {
+    huv = uv_at(sc.prim_uv + 6u * static_cast<uint32_t>(prim), hu, hv);
+    base = mulv(base, tex_tap(sc, m.base_color_texture_index, huv).color);
+  }
+  const V3 albedo = effect_albedo(m, base, pos, sN, rd, eff_k, eff_pre);
   const float roughness = clamp01(m.roughness);
   MatFlags f;
   f.emissive_model = m.material_model == 1u || m.emissive[0] > 0.0f || m.emissive[1] > 0.0f || m.emissive[2] > 0.0f;
@@ -1143,7 +1247,7 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
           // the NEE albedo sees the light direction as the incoming one (ScalarCpuPathTracer.cpp:1089)
           float nk = eff_k;
           V3 npre = eff_pre;
-          const V3 nalb = m.material_effect != 0u ? effect_albedo(m, pos, sN, neg(light_dir), nk, npre) : albedo;
+          const V3 nalb = m.material_effect != 0u ? effect_albedo(m, base, pos, sN, neg(light_dir), nk, npre) : albedo;
           const float kscale = (light.intensity * spot) / (dist_sq + kEps);
           const V3 irradiance = muls(ld3(light.color), kscale);
           const float nlf = static_cast<float>(nl);
@@ -1223,14 +1327,13 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
             out.galb = galb;
             out.glight = glight;
             out.grough = grough;
+            out.guv = huv;
             out.gmat = mat_index;
             out.glight_index = light_index;
           } else {
             L = add(L, contrib);
             if constexpr (kAdj) {
-              gadd(A, glds, kGradAlbedo, mat_index, 0, galb.x);
-              gadd(A, glds, kGradAlbedo, mat_index, 1, galb.y);
-              gadd(A, glds, kGradAlbedo, mat_index, 2, galb.z);
+              gadd_base(A, sc, glds, mat_index, huv, galb);
               gadd(A, glds, kGradLight, light_index, 0, glight.x);
               gadd(A, glds, kGradLight, light_index, 1, glight.y);
               gadd(A, glds, kGradLight, light_index, 2, glight.z);
@@ -1314,12 +1417,16 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
       A.p_dx[c][p] = dx[c];
     }
     A.p_mat[p] = mat_index;
+    A.p_uv[0][p] = huv.x;
+    A.p_uv[1][p] = huv.y;
     A.p_score[p] = score;
     const bool active = depth < st.rr_start && st.rr_min < maxT && maxT < st.rr_max;
     aflags = (aflags & ~0xFu) | 1u | (active ? 2u : 0u) | (static_cast<uint32_t>(cstar) << 2);
     uint32_t nh = (aflags >> 4) & 7u;
     if (depth < st.rr_start && nh < static_cast<uint32_t>(kAdjHist)) {
       A.h_mat[nh][p] = mat_index;
+      A.h_uv[nh][0][p] = huv.x;
+      A.h_uv[nh][1][p] = huv.y;
       for (int c = 0; c < 3; ++c) A.h_dx[nh][c][p] = dx[c];
       ++nh;
       aflags = (aflags & ~0x70u) | (nh << 4);
@@ -1357,8 +1464,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevPaths P, DevRays in, DevRay
       sid = in.qscene[q];
       const V3 ro = mk(in.c[0][q], in.c[1][q], in.c[2][q]);
       const V3 rd = mk(in.c[3][q], in.c[4][q], in.c[5][q]);
-      shade_path<kAdj>(P, scene_view(sc, sid), st, depth, p, ro, rd, P.hit_t[q], P.hit_prim[q], so, shadow_tests, A,
-                       glds);
+      const float hu = sc.textured ? P.hit_u[q] : 0.0f, hv = sc.textured ? P.hit_v[q] : 0.0f;
+      shade_path<kAdj>(P, scene_view(sc, sid), st, depth, p, ro, rd, P.hit_t[q], P.hit_prim[q], hu, hv, so,
+                       shadow_tests, A, glds);
     }
     const uint32_t seg = (w0 >> 6) & (kSegments - 1u);  // this chunk's output segment
     const uint32_t slot = segment_append(P.queue + kQueueOutCursor, seg, P.seg_cap, so.cont);
@@ -1380,6 +1488,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevPaths P, DevRays in, DevRay
         A.s_galb[0][sslot] = so.galb.x; A.s_galb[1][sslot] = so.galb.y; A.s_galb[2][sslot] = so.galb.z;
         A.s_glight[0][sslot] = so.glight.x; A.s_glight[1][sslot] = so.glight.y; A.s_glight[2][sslot] = so.glight.z;
         A.s_mat[sslot] = so.gmat;
+        A.s_uv[0][sslot] = so.guv.x;
+        A.s_uv[1][sslot] = so.guv.y;
         A.s_light[sslot] = so.glight_index;
         A.s_grough[sslot] = so.grough;
       }
@@ -1396,17 +1506,17 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevPaths P, DevRays in, DevRay
 // that steps through up to 8 transmissive blockers (ScalarCpuPathTracer.cpp:1066-1087), each
 // step a follow-up query on the same lane.
 template <bool kAdj>
-__device__ __forceinline__ void light_visible(const DevPaths& P, const DevAdjoint& A, float* glds, uint32_t q) {
+__device__ __forceinline__ void light_visible(const DevPaths& P, const DevScene& sc, const DevAdjoint& A, float* glds,
+                                              uint32_t q) {
   const uint32_t p = P.spath[q];
   P.rad[0][p] = P.rad[0][p] + P.scontrib[0][q];
   P.rad[1][p] = P.rad[1][p] + P.scontrib[1][q];
   P.rad[2][p] = P.rad[2][p] + P.scontrib[2][q];
   if constexpr (kAdj) {
     const uint32_t mi = A.s_mat[q], li = A.s_light[q];
-    for (int c = 0; c < 3; ++c) {
-      gadd(A, glds, kGradAlbedo, mi, c, A.s_galb[c][q]);
-      gadd(A, glds, kGradLight, li, c, A.s_glight[c][q]);
-    }
+    gadd_base(A, sc, glds, mi, make_float2(A.s_uv[0][q], A.s_uv[1][q]),
+              mk(A.s_galb[0][q], A.s_galb[1][q], A.s_galb[2][q]));
+    for (int c = 0; c < 3; ++c) gadd(A, glds, kGradLight, li, c, A.s_glight[c][q]);
     gadd(A, glds, kGradRough, mi, 0, A.s_grough[q]);
   }
 }
@@ -1431,7 +1541,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevPaths P, DevScene sc, DevA
       return trav_begin(sc.recs[P.sscene[q]], s, o, d, P.stmax[q], st);
     };
     auto end = [&](uint32_t, Trav& s) {
-      if (s.best_prim < 0) light_visible<kAdj>(P, A, glds, q);
+      if (s.best_prim < 0) light_visible<kAdj>(P, sc, A, glds, q);
       return false;
     };
     drive_queries<true>(sc, stk, n, P.refill_min, st, begin, end);
@@ -1468,7 +1578,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevPaths P, DevScene sc, DevA
         // the follow-up query ended at once (root box missed): nothing in the way
         break;
       }
-      light_visible<kAdj>(P, A, glds, q);
+      light_visible<kAdj>(P, sc, A, glds, q);
       return false;
     };
     drive_queries<false>(sc, stk, n, P.refill_min, st, begin, end);

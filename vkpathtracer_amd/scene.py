@@ -104,6 +104,10 @@ class Scene:
     name: str = "scene"
     environment_map: np.ndarray | None = None  # (H, W, 3) f32 equirectangular, or None
     environment_map_scale: tuple = (0.0, 0.0, 0.0)
+    # albedo textures (no counterpart in the reference's CPU path): per-vertex (u, v) of `vertices`
+    # and (H, W, 3) f32 images, rows in increasing v; materials name them by base_color_texture_index
+    texcoords: np.ndarray | None = None
+    textures: list = dataclasses.field(default_factory=list)
 
     def __post_init__(self):
         if self.environment_map is not None:
@@ -116,6 +120,20 @@ class Scene:
         self.instances = np.ascontiguousarray(self.instances, INSTANCE_DTYPE).reshape(-1)
         self.materials = np.ascontiguousarray(self.materials, MATERIAL_DTYPE).reshape(-1)
         self.lights = np.ascontiguousarray(self.lights, LIGHT_DTYPE).reshape(-1)
+        if self.texcoords is not None:
+            self.texcoords = np.ascontiguousarray(self.texcoords, np.float32).reshape(-1, 2)
+        self.textures = [np.ascontiguousarray(t, np.float32).reshape(t.shape[0], t.shape[1], 3) for t in self.textures]
+
+    def texture_arrays(self) -> tuple[np.ndarray, np.ndarray]:
+        """All textures as one (N, 3) texel array and (K, 4) u32 records (offset, width, height, 0)."""
+        descs = np.zeros((len(self.textures), 4), np.uint32)
+        off = 0
+        for i, t in enumerate(self.textures):
+            descs[i] = (off, t.shape[1], t.shape[0], 0)
+            off += t.shape[0] * t.shape[1]
+        texels = (np.concatenate([t.reshape(-1, 3) for t in self.textures]) if self.textures
+                  else np.zeros((0, 3), np.float32))
+        return np.ascontiguousarray(texels, np.float32), descs
 
     @property
     def triangle_count(self) -> int:
@@ -526,3 +544,170 @@ def batch_scenes(seed: int = 7, count: int = 64, min_tris: int = 1_000, max_tris
         spp = int(rng.choice(spps))
         out.append((dataclasses.replace(sc, name=f"batch-{seed}-{i}"), res, res, spp))
     return out
+
+
+def geodesic_sphere(frequency: int):
+    """Unit geodesic sphere: every icosahedron face split into frequency^2 triangles on a barycentric
+    grid and projected to the sphere (20 * frequency^2 triangles). Returns the grid points (P, 3)
+    float64 and triangles (20 * frequency^2, 3) int64 indexing them; grid points on the icosahedron's
+    edges appear once per face."""
+    n = int(frequency)
+    v0, f0 = icosphere(0)
+    ii, jj = [], []
+    for i in range(n + 1):
+        for j in range(n + 1 - i):
+            ii.append(i)
+            jj.append(j)
+    ii = np.array(ii)
+    jj = np.array(jj)
+    idx = -np.ones((n + 1, n + 1), np.int64)
+    idx[ii, jj] = np.arange(len(ii))
+    up = [(idx[i, j], idx[i + 1, j], idx[i, j + 1]) for i in range(n) for j in range(n - i)]
+    down = [(idx[i + 1, j], idx[i + 1, j + 1], idx[i, j + 1]) for i in range(n) for j in range(n - i - 1)]
+    local = np.array(up + down, np.int64)
+    a, b, c = v0[f0[:, 0]], v0[f0[:, 1]], v0[f0[:, 2]]
+    wa = ((n - ii - jj) / n)[None, :, None]
+    pts = a[:, None] * wa + b[:, None] * (ii / n)[None, :, None] + c[:, None] * (jj / n)[None, :, None]
+    pts /= np.linalg.norm(pts, axis=2, keepdims=True)
+    npf = len(ii)
+    faces = (local[None] + (np.arange(20) * npf)[:, None, None]).reshape(-1, 3)
+    return pts.reshape(-1, 3), faces
+
+
+def value_noise_texture(size: int = 1024, seed: int = 3, cells: int = 16, octaves: int = 4, lo: float = 0.15,
+                        hi: float = 0.95) -> np.ndarray:
+    """Tileable seeded value noise, (size, size, 3) float32 in [lo, hi]: per octave a random lattice of
+    cells * 2^o cells per side (wrapping) interpolated with smoothstep weights, amplitudes halving."""
+    rng = np.random.default_rng(seed)
+    out = np.zeros((size, size, 3), np.float64)
+    amp, total = 1.0, 0.0
+    for o in range(octaves):
+        g = cells * 2 ** o
+        lat = rng.uniform(0.0, 1.0, (g, g, 3))
+        x = (np.arange(size) + 0.5) / size * g
+        i0 = np.floor(x).astype(np.int64) % g
+        i1 = (i0 + 1) % g
+        f = x - np.floor(x)
+        f = f * f * (3.0 - 2.0 * f)
+        rows = lat[i0] * (1.0 - f)[:, None, None] + lat[i1] * f[:, None, None]          # (size, g, 3)
+        out += amp * (rows[:, i0] * (1.0 - f)[None, :, None] + rows[:, i1] * f[None, :, None])
+        total += amp
+        amp *= 0.5
+    out /= total
+    mn, mx = out.min(), out.max()
+    return (lo + (hi - lo) * (out - mn) / max(mx - mn, 1e-12)).astype(np.float32)
+
+
+def ring_cameras(count: int = 16, radius: float = 13.0, height: float = 3.0, fov_deg: float = 42.0) -> list:
+    """`count` pinhole cameras evenly spaced on a horizontal ring, all looking at the origin."""
+    cams = []
+    for k in range(count):
+        a = 2.0 * math.pi * k / count
+        cams.append(Camera(position=(radius * math.cos(a), height, radius * math.sin(a)), target=(0.0, 0.0, 0.0),
+                           up=(0.0, 1.0, 0.0), fov_deg=fov_deg))
+    return cams
+
+
+def textured_sphere_scene(seed: int = 11, frequency: int = 100, texture_size: int = 1024, radius: float = 5.0,
+                          material_count: int = 8, light_count: int = 6, texture=None) -> Scene:
+    """Displaced geodesic sphere with a UV-mapped albedo texture (default 200,000 triangles and a
+    1024^2 rgb fp32 value-noise texture), on a ground quad, lit by emissive quads.
+
+    * Sphere: 20 * frequency^2 triangles, radius 5, displaced along the normal by 0.05 * radius *
+      (3-octave Perlin fBm), seeded. Vertices are per triangle corner, with equirectangular texture
+      coordinates u = 0.5 + atan2(z, x) / 2pi, v = acos(y) / pi of the undisplaced direction (a
+      triangle across the u seam has its small u values moved up by 1; the texture repeats).
+    * `material_count` glossy materials (model 3), one per longitude sector, albedo 1, roughness
+      U[0.15, 0.9], all using texture 0.
+    * Ground: a 40 x 40 Lambert quad (albedo 0.6) at y = -1.3 * radius, untextured.
+    * `light_count` downward emissive quads (side 1.5) on a ring of radius 6 at y = 9, emission
+      U[4, 12], each with an RTHitLight (intensity = quad area, radius 0.05) 0.1 below its centre.
+    Camera: the first of `ring_cameras()`.
+    """
+    rng = np.random.default_rng(seed)
+    pts, faces = geodesic_sphere(frequency)
+    noise = np.zeros(len(pts))
+    amp = 1.0
+    for octave in range(3):
+        noise += amp * perlin3(pts * (1.5 * 2 ** octave) + 17.0, seed * 131 + octave)
+        amp *= 0.5
+    disp = pts * (radius * (1.0 + 0.05 * noise))[:, None]
+    u = 0.5 + np.arctan2(pts[:, 2], pts[:, 0]) / (2.0 * math.pi)
+    v = np.arccos(np.clip(pts[:, 1], -1.0, 1.0)) / math.pi
+    tu = u[faces]                                              # (T, 3)
+    seam = (tu.max(1) - tu.min(1)) > 0.5
+    tu = np.where(seam[:, None] & (tu < 0.5), tu + 1.0, tu)
+    tv = v[faces]
+    # sectors by centroid longitude; triangles grouped by sector for contiguous instance ranges
+    sector = np.minimum((np.mod(tu.mean(1), 1.0) * material_count).astype(np.int64), material_count - 1)
+    order = np.argsort(sector, kind="stable")
+    faces, tu, tv, sector = faces[order], tu[order], tv[order], sector[order]
+    n_tri = len(faces)
+    sph_v = disp[faces].reshape(-1, 3)
+    sph_uv = np.stack([tu.reshape(-1), tv.reshape(-1)], 1)
+
+    g = 1.3 * radius
+    ground_v = np.array([[-20.0, -g, -20.0], [-20.0, -g, 20.0], [20.0, -g, 20.0], [20.0, -g, -20.0]])
+    side = 1.5
+    lv = []
+    light_pos = []
+    for k in range(light_count):
+        a = 2.0 * math.pi * (k + 0.5) / light_count
+        c = np.array([6.0 * math.cos(a), 9.0, 6.0 * math.sin(a)])
+        light_pos.append(c)
+        h = side / 2
+        lv.append(np.array([[c[0] - h, c[1], c[2] - h], [c[0] - h, c[1], c[2] + h], [c[0] + h, c[1], c[2] + h],
+                            [c[0] + h, c[1], c[2] - h]]))
+    quad = np.array([[0, 1, 2], [0, 2, 3]])
+    vertices = np.concatenate([sph_v, ground_v] + lv).astype(np.float32)
+    base = len(sph_v)
+    indices = np.concatenate([np.arange(3 * n_tri).reshape(-1, 3), quad + base] +
+                             [quad + base + 4 + 4 * k for k in range(light_count)]).astype(np.uint32).reshape(-1)
+    texcoords = np.zeros((len(vertices), 2), np.float32)
+    texcoords[:len(sph_uv)] = sph_uv
+
+    counts = np.bincount(sector, minlength=material_count)
+    n_inst = material_count + 1 + light_count
+    instances = np.zeros(n_inst, INSTANCE_DTYPE)
+    instances["rotation"] = (0.0, 0.0, 0.0, 1.0)
+    instances["scale"] = 1.0
+    instances["entity_id"] = np.arange(n_inst) + 1
+    first = 0
+    for k in range(material_count):
+        instances[k]["first_triangle"] = first
+        instances[k]["triangle_count"] = counts[k]
+        instances[k]["material_index"] = k
+        first += int(counts[k])
+    instances[material_count]["first_triangle"] = n_tri
+    instances[material_count]["triangle_count"] = 2
+    instances[material_count]["material_index"] = material_count
+    materials = make_materials(material_count + 1 + light_count)
+    rough = rng.uniform(0.15, 0.9, material_count)
+    for k in range(material_count):
+        materials[k]["albedo"] = (1.0, 1.0, 1.0)
+        materials[k]["material_model"] = MODEL_GLOSSY
+        materials[k]["roughness"] = rough[k]
+        materials[k]["base_color_texture_index"] = 0
+    materials[material_count]["albedo"] = (0.6, 0.6, 0.6)
+    materials[material_count]["material_model"] = MODEL_LAMBERT
+    emission = rng.uniform(4.0, 12.0, light_count)
+    lights = np.zeros(light_count, LIGHT_DTYPE)
+    for k in range(light_count):
+        i = material_count + 1 + k
+        instances[i]["first_triangle"] = n_tri + 2 + 2 * k
+        instances[i]["triangle_count"] = 2
+        instances[i]["material_index"] = i
+        materials[i]["albedo"] = (0.8, 0.8, 0.8)
+        materials[i]["emissive"] = (emission[k],) * 3
+        materials[i]["material_model"] = MODEL_EMISSIVE
+        c = light_pos[k]
+        lights[k] = make_light((c[0], c[1] - 0.1, c[2]), (emission[k],) * 3, side * side, 0.05)
+    tex = value_noise_texture(texture_size, seed=seed * 7 + 1) if texture is None else texture
+    return Scene(vertices, indices, instances, materials, lights, ring_cameras()[0], name=f"textured-{seed}",
+                 texcoords=texcoords, textures=[tex])
+
+
+def perturbed_texture(texture: np.ndarray, sigma: float = 0.1, seed: int = 9) -> np.ndarray:
+    """texture + N(0, sigma) per texel and channel, clipped to [0, 1]."""
+    rng = np.random.default_rng(seed)
+    return np.clip(texture + rng.normal(0.0, sigma, texture.shape), 0.0, 1.0).astype(np.float32)

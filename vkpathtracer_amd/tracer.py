@@ -59,6 +59,10 @@ class TracerStatus:
 _LIFECYCLE = ["Uninitialized", "Configured", "SceneLoaded", "Ready"]
 
 
+def _dptr(t):
+    return None if t is None or t.numel() == 0 else t.data_ptr()
+
+
 class HipPathTracer:
     """IPathTracer implemented by the gfx950 wavefront kernels."""
 
@@ -110,9 +114,24 @@ class HipPathTracer:
     def load_scene_snapshot(self, scene: Scene) -> Status:
         view = abi.SceneView(scene)
         st = self._status(self._lib.pt_hip_load_scene(self._h, C.byref(view.view)))
+        if st and view.has_textures:
+            st = self._status(self._lib.pt_hip_load_textures(self._h, *view.texture_args()))
         if st:
             self._view = view
         return st
+
+    def update_texels(self, texels) -> None:
+        """Replace every texel of the built scene (same count): a (N, 3) float32 numpy array or a
+        contiguous float32 device tensor on this tracer's device."""
+        if isinstance(texels, np.ndarray):
+            t = np.ascontiguousarray(texels, np.float32).reshape(-1, 3)
+            self._check(self._lib.pt_hip_update_texels(self._h, t.ctypes.data, len(t)), "update_texels")
+        else:
+            import torch
+
+            torch.cuda.synchronize(self.device)
+            self._check(self._lib.pt_hip_update_texels(self._h, texels.data_ptr(), texels.numel() // 3),
+                        "update_texels")
 
     def build_or_update_acceleration(self, builder: str = "sah", leaf_size: int = 4, bucket_count: int = 8) -> Status:
         p = abi.PtBvhParams()
@@ -160,11 +179,11 @@ class HipPathTracer:
         return rc == 0
 
     def render_adjoint(self, tiles, film_grad, grad_albedo, grad_emission, grad_light, frame_index: int = 0,
-                       grad_roughness=None):
+                       grad_roughness=None, grad_texels=None):
         """Path-replay adjoint of render_tiles(tiles): adds d(sum film_grad * film)/d(parameters) into the
         device tensors grad_albedo / grad_emission (M, 3), grad_light (L, 3) and, when given,
-        grad_roughness (M,). film_grad: (H, W, 3) device tensor. Torch tensors must be contiguous
-        float32 on this tracer's device."""
+        grad_roughness (M,) and grad_texels (N, 3) over the scene's textures. film_grad: (H, W, 3)
+        device tensor. Torch tensors must be contiguous float32 on this tracer's device."""
         import torch
 
         torch.cuda.synchronize(self.device)
@@ -172,7 +191,7 @@ class HipPathTracer:
         rc = self._lib.pt_hip_render_adjoint(self._h, arr, len(tiles), frame_index, film_grad.data_ptr(),
                                              grad_albedo.data_ptr(), grad_emission.data_ptr(),
                                              grad_light.data_ptr() if grad_light.numel() else None,
-                                             grad_roughness.data_ptr() if grad_roughness is not None else None)
+                                             _dptr(grad_roughness), _dptr(grad_texels))
         self._check(rc, "render_adjoint")
         self.synchronize()
 
@@ -318,6 +337,18 @@ class HipBatch:
         idx = C.c_uint32()
         self._check(self._lib.pt_batch_add_scene(self._h, C.byref(view.view), width, height, spp, C.byref(p),
                                                  C.byref(idx)), "add_scene")
+        if view.has_textures:
+            self._check(self._lib.pt_batch_load_textures(self._h, idx.value, *view.texture_args()), "load_textures")
+        self.scenes.append((width, height, spp))
+        return int(idx.value)
+
+    def add_view(self, scene: int, camera, width: int, height: int, spp: int) -> int:
+        """Another camera on batch scene `scene`: shares its geometry, materials and textures, has its own
+        image and film."""
+        cam = abi.make_camera(camera)
+        idx = C.c_uint32()
+        self._check(self._lib.pt_batch_add_view(self._h, scene, C.byref(cam), width, height, spp, C.byref(idx)),
+                    "add_view")
         self.scenes.append((width, height, spp))
         return int(idx.value)
 
@@ -339,6 +370,46 @@ class HipBatch:
 
     def synchronize(self):
         self._check(self._lib.pt_batch_synchronize(self._h), "synchronize")
+
+    def update_scene_delta(self, materials: np.ndarray | None = None, lights: np.ndarray | None = None):
+        """Replace the pool's materials and/or lights (every scene's, in the order they were added)."""
+        from .scene import LIGHT_DTYPE, MATERIAL_DTYPE
+
+        m = None if materials is None else np.ascontiguousarray(materials, MATERIAL_DTYPE)
+        l = None if lights is None else np.ascontiguousarray(lights, LIGHT_DTYPE)
+        self._check(self._lib.pt_batch_update_scene_delta(self._h, None if m is None else m.ctypes.data,
+                                                          0 if m is None else len(m),
+                                                          None if l is None else l.ctypes.data,
+                                                          0 if l is None else len(l)), "update_scene_delta")
+
+    def update_texels(self, texels) -> None:
+        """Replace every texel of the pool: (N, 3) float32 numpy array or contiguous float32 device tensor."""
+        if isinstance(texels, np.ndarray):
+            t = np.ascontiguousarray(texels, np.float32).reshape(-1, 3)
+            self._check(self._lib.pt_batch_update_texels(self._h, t.ctypes.data, len(t)), "update_texels")
+        else:
+            import torch
+
+            torch.cuda.synchronize(self.device)
+            self._check(self._lib.pt_batch_update_texels(self._h, texels.data_ptr(), texels.numel() // 3),
+                        "update_texels")
+
+    def render_adjoint(self, film_grad, grad_albedo, grad_emission, grad_light, frame_index: int = 0,
+                       grad_roughness=None, grad_texels=None):
+        """Path-replay adjoint of render(frame_index): film_grad is a device tensor of every film's pixels
+        (films in batch order, 3 floats each); gradients are pool-ordered device tensors (added to)."""
+        import torch
+
+        torch.cuda.synchronize(self.device)
+        rc = self._lib.pt_batch_render_adjoint(self._h, frame_index, film_grad.data_ptr(), grad_albedo.data_ptr(),
+                                               grad_emission.data_ptr(), _dptr(grad_light), _dptr(grad_roughness),
+                                               _dptr(grad_texels))
+        self._check(rc, "render_adjoint")
+        self.synchronize()
+
+    def films(self) -> list:
+        """Every film (rgb sum, count), in batch order."""
+        return [self.film(i) for i in range(len(self.scenes))]
 
     def film(self, scene: int) -> tuple[np.ndarray, np.ndarray]:
         w, h, _ = self.scenes[scene]
