@@ -159,6 +159,70 @@ def adjoint_ratio(log, reps: int = 3) -> dict:
             "ratio": round(b / f, 3), "standalone_ratio": round(sa / f, 3)}
 
 
+def textured_multiview(log, reps: int = 2, views: int = 16, size: int = 512, spp: int = 32) -> dict:
+    """Batched differentiable rendering: the 200,000-triangle displaced geodesic sphere with a 1024^2 rgb
+    fp32 value-noise albedo texture (8 glossy materials with their own roughness, a ground quad, 6 quad
+    lights), `views` cameras on a ring in one batch (the scene plus views-1 views), size^2 x spp each,
+    depth 6. forward = reset + pt_batch_render of every view's frame; backward = pt_batch_render_adjoint
+    of the same frame right after it (d/d albedo, emission, light colour, roughness and every texel,
+    fp32 atomics; the forward radiance of the single pass is reused); standalone = the adjoint with no
+    forward before it."""
+    import torch
+
+    from vkpathtracer_amd import scene as S
+    from vkpathtracer_amd.tracer import HipBatch
+
+    sc = S.textured_sphere_scene()
+    cams = S.ring_cameras(views)
+    settings = S.RenderSettings(width=1, height=1, spp=1, max_depth=6)
+    paths = views * size * size * spp
+    dev = torch.device("cuda", 0)
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3
+
+    with HipBatch(settings) as b:
+        b.add_scene(dataclasses.replace(sc, camera=cams[0]), size, size, spp)
+        for c in cams[1:]:
+            b.add_view(0, c, size, size, spp)
+        info = b.build()
+        nm, nl = len(sc.materials), len(sc.lights)
+        nt = sc.textures[0].shape[0] * sc.textures[0].shape[1]
+        g = torch.randn((views, size, size, 3), device=dev)
+        grads = [torch.zeros(shape, device=dev) for shape in ((nm, 3), (nm, 3), (nl, 3), (nm,), (nt, 3))]
+
+        def fwd():
+            b.reset_accumulation()
+            b.render(0)
+            b.synchronize()
+
+        def adj():
+            b.render_adjoint(g, grads[0], grads[1], grads[2], 0, grad_roughness=grads[3], grad_texels=grads[4])
+
+        fwd()
+        adj()
+        f_ms, b_ms, s_ms = [], [], []
+        for _ in range(reps):
+            f_ms.append(timed(fwd))
+            b_ms.append(timed(adj))
+            s_ms.append(timed(adj))
+        counters = b.read_counters()
+    f, bk, sa = min(f_ms), min(b_ms), min(s_ms)
+    log(f"textured multi-view: forward {f:.1f} ms, backward {bk:.1f} ms, standalone adjoint {sa:.1f} ms")
+    return {"workload": f"{sc.triangle_count}-triangle displaced textured sphere, 1024^2 rgb fp32 texture, {views} "
+                        f"views x {size}^2 x {spp} spp, depth 6, d/d(albedo, emission, light, roughness, "
+                        f"{nt} texels)",
+            "paths": paths, "bvh_build_ms": round(info["build_ms"], 1),
+            "forward_ms": round(f, 2), "forward_mpaths_per_s": round(paths / f / 1e3, 2),
+            "backward_ms": round(bk, 2), "standalone_adjoint_ms": round(sa, 2),
+            "ratio": round(bk / f, 3), "standalone_ratio": round(sa / f, 3),
+            "rays_per_frame": counters["rays"]}
+
+
 def batched_comparison(log, count: int = 64, reps: int = 2) -> dict:
     """Many scenes in one batch against the same scenes rendered one tracer at a time.
 
@@ -234,6 +298,7 @@ def main():
     ap.add_argument("--no-profile", action="store_true", help="time without per-kernel events")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward time ratio")
     ap.add_argument("--no-batched", action="store_true", help="skip the many-scene batch comparison")
+    ap.add_argument("--no-textured", action="store_true", help="skip the textured multi-view adjoint")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run a plain device copy of this size first (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -331,6 +396,7 @@ def main():
     pt.close()
     adjoint = None if args.no_adjoint else adjoint_ratio(log)
     batched = None if args.no_batched else batched_comparison(log)
+    textured = None if args.no_textured else textured_multiview(log)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(sc, st, args.cpu_seconds)
@@ -361,6 +427,7 @@ def main():
         "cpu_baseline": cpu,
         "adjoint": adjoint,
         "batched": batched,
+        "textured_multiview": textured,
     }
     if cpu:
         out["speedup_vs_cpu"] = round(mpaths / cpu["value"], 1)

@@ -168,6 +168,13 @@ def load_library(path: os.PathLike | None = None) -> C.CDLL:
     if _lib is not None and path is None:
         return _lib
     p = pathlib.Path(path) if path else LIB_PATH
+    # torch ships its own HIP runtime with the same sonames (libamdhip64.so.7, libhsa-runtime64.so.1).
+    # Loaded first, it satisfies this library's dependencies and the process has one runtime; loaded
+    # after this library, it would bring a second HSA runtime whose initialisation finds no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not p.exists():
         raise NativeLibraryMissing(f"{p} is missing: build the HIP library first (python -c 'import __graft_entry__ as g; g.build()')")
     lib = C.CDLL(str(p))

@@ -357,9 +357,12 @@ void pack_device_layout(const Bvh& bvh, DeviceLayout& out, bool with_uv) {
   out.tris.clear();
   out.prims.clear();
   out.prim_uv.clear();
+  out.prim_tri.clear();
   if (with_uv) {
     out.prim_uv.resize(6u * bvh.prims.size());
     for (size_t i = 0; i < bvh.prims.size(); ++i) std::memcpy(&out.prim_uv[6u * i], bvh.prims[i].uv, 6 * sizeof(float));
+    out.prim_tri.resize(bvh.prims.size());
+    for (size_t i = 0; i < bvh.order.size(); ++i) out.prim_tri[bvh.order[i]] = static_cast<uint32_t>(i);
   }
   out.inner_count = 0;
   const size_t nprim = bvh.prims.size();

@@ -51,6 +51,7 @@ struct DeviceLayout {
   std::vector<float> tris;    // 12 floats per leaf-ordered triangle
   std::vector<float> prims;   // 4 floats per primitive
   std::vector<float> prim_uv; // 6 floats per primitive (texture coordinates), empty without texcoords
+  std::vector<uint32_t> prim_tri; // per primitive: its position in leaf order (with prim_uv)
   float root_box[6] = {0, 0, 0, 0, 0, 0};
   int32_t root_ref = 0;
   uint32_t inner_count = 0;

@@ -109,6 +109,7 @@ struct pt_tracer {
   DevArray<ptd::SceneRec> d_recs;
   DevArray<ptd::TexRec> d_texrecs;
   DevArray<float> d_texels, d_prim_uv;
+  DevArray<uint32_t> d_prim_tri;
   size_t texel_count = 0;
   DevArray<uint32_t> d_film_count, d_queue, d_error;
   DevArray<unsigned long long> d_counters, d_query_counters;
@@ -291,6 +292,7 @@ void free_scene_device(pt_tracer* t) {
   t->d_texrecs.release();
   t->d_texels.release();
   t->d_prim_uv.release();
+  t->d_prim_tri.release();
 }
 
 void free_paths(pt_tracer* t) {
@@ -346,12 +348,12 @@ int ensure_adjoint(pt_tracer* t, uint32_t cap, ptd::DevAdjoint& A) {
   return PT_OK;
 }
 
-// Path-state and queue arrays for `cap` paths: 31 float arrays, 7 u32 arrays, 1 u64 array, each
+// Path-state and queue arrays for `cap` paths: 29 float arrays, 7 u32 arrays, 1 u64 array, each
 // of max(cap, kSegments * seg_cap) entries (queue arrays are indexed by segment position).
 int ensure_paths(pt_tracer* t, uint32_t cap) {
   if (cap <= t->path_capacity_alloc) return PT_OK;
   free_paths(t);
-  constexpr size_t kF = 31, kU = 7;
+  constexpr size_t kF = 29, kU = 7;
   const uint32_t seg_cap = ptd::segment_capacity(cap);
   const size_t n = std::max<size_t>(cap, static_cast<size_t>(ptd::kSegments) * seg_cap);
   PT_HIP(t, t->d_path_f.ensure(kF * n), "allocate path state");
@@ -374,8 +376,6 @@ int ensure_paths(pt_tracer* t, uint32_t cap) {
   }
   for (int k = 0; k < 6; ++k) P.sray[k] = f + (size_t)(19 + k) * n;
   P.stmax = f + 25 * n;
-  P.hit_u = f + 29 * n;
-  P.hit_v = f + 30 * n;
   P.spath = u + 3 * n;
   P.sscene = u + (size_t)6 * n;
   P.rng = t->d_rng.ptr;
@@ -574,6 +574,7 @@ int upload_pool(pt_tracer* t) {
   if (n_texels >= (1ull << 32)) return fail(t, PT_UNSUPPORTED, "texture pool exceeds 2^32 texels");
   std::vector<float> nodes(16u * n_nodes), tris(12u * n_tris), prims(4u * n_prims), env(n_env);
   std::vector<float> prim_uv(any_uv ? 6u * n_prims : 0u), texels;
+  std::vector<uint32_t> prim_tri(any_uv ? n_prims : 0u, 0u);
   std::vector<ptd::TexRec> texrecs;
   std::vector<PtMaterial> mats;
   std::vector<PtLight> lights;
@@ -625,8 +626,10 @@ int upload_pool(pt_tracer* t) {
         std::memcpy(&m, src + 3, 4);
         m = std::min(m, last_mat) + mat_base;  // the shading clamp of a scene-local index, then the pool base
         std::memcpy(dst + 3, &m, 4);
-        if (any_uv && !s.texcoords.empty())
+        if (any_uv && !s.texcoords.empty()) {
           std::memcpy(&prim_uv[6u * (prim_base + i)], &L.prim_uv[6u * i], 6 * sizeof(float));
+          prim_tri[prim_base + i] = tri_base + L.prim_tri[i];
+        }
       }
       const std::vector<PtMaterial> pm = pool_materials(s, tex_base);
       for (const PtMaterial& m : pm)
@@ -687,6 +690,7 @@ int upload_pool(pt_tracer* t) {
   PT_HIP(t, upload(t->d_recs, recs.data(), recs.size(), sizeof(ptd::SceneRec)), "upload scene records");
   PT_HIP(t, upload(t->d_prim_uv, prim_uv.data(), prim_uv.size(), sizeof(float)), "upload texture coordinates");
   PT_HIP(t, upload(t->d_texels, texels.data(), texels.size(), sizeof(float)), "upload textures");
+  PT_HIP(t, upload(t->d_prim_tri, prim_tri.data(), prim_tri.size(), sizeof(uint32_t)), "upload triangle map");
   PT_HIP(t, upload(t->d_texrecs, texrecs.data(), texrecs.size(), sizeof(ptd::TexRec)), "upload texture table");
 
   // Traversal stack spill area: a lane's stack never holds more than max_depth entries (one per
@@ -722,6 +726,7 @@ int upload_pool(pt_tracer* t) {
   S.texture_count = static_cast<uint32_t>(texrecs.size());
   S.textured = textured ? 1u : 0u;
   S.prim_uv = any_uv ? t->d_prim_uv.ptr : nullptr;
+  S.prim_tri = any_uv ? t->d_prim_tri.ptr : nullptr;
   t->texel_count = texels.size() / 3u;
 
   t->info = PtBuildInfo{};

@@ -104,8 +104,9 @@ struct DevScene {
   const float* texels;     // texture pool (rgb)
   const TexRec* textures;  // material base_color_texture_index values are pool texture indices
   uint32_t texture_count;
-  uint32_t textured;       // some material has a texture: extend also writes the hit barycentrics
+  uint32_t textured;       // some material has a texture
   const float* prim_uv;    // 6 floats per pool primitive: texture coordinates of v0, v1, v2
+  const uint32_t* prim_tri;  // per pool primitive: its triangle record in `tris`
 };
 
 // Integrator settings shared by every scene of a launch (image sizes are per scene: SceneRec).
@@ -144,8 +145,6 @@ struct DevPaths {
   DevRays rays[2];
   float* hit_t;
   int32_t* hit_prim;
-  float* hit_u;          // barycentrics of the hit (textured pools only)
-  float* hit_v;
   float* tp[3];          // throughput
   float* rad[3];         // radiance
   uint64_t* rng;

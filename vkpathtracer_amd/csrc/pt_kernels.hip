@@ -192,7 +192,6 @@ struct Trav {
   V3 d;
   float best_t;
   int32_t best_prim, cur;
-  float best_u, best_v;  // barycentrics of the best hit
   uint32_t sp;  // entries on the stack
   uint32_t lo;  // entries [lo, sp) are in the LDS ring, [0, lo) in the global area
 };
@@ -202,8 +201,6 @@ struct Trav {
 __device__ __forceinline__ bool trav_begin(const SceneRec& R, Trav& s, V3 o, V3 d, float tmax, TravStats& st) {
   s.best_prim = -1;
   s.best_t = tmax;
-  s.best_u = 0.0f;
-  s.best_v = 0.0f;
   s.d = d;
   s.sp = 0;
   s.lo = 0;
@@ -292,8 +289,6 @@ __device__ __forceinline__ int trav_leaf(const DevScene& sc, Trav& s, TravStats&
     if (tri_test(tp, o, s.d, s.best_t, t, bu, bv)) {  // commit_ordered_lane_hit, BvhTriangleIntersector.cpp:25-33
       ++accepted;
       s.best_t = t;
-      s.best_u = bu;
-      s.best_v = bv;
       s.best_prim = __float_as_int(tp[0].w);
     }
     // an any-hit query stops at the end of the first batch that accepted a hit
@@ -655,10 +650,6 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevPaths P, DevRays in, DevSc
   auto end = [&](uint32_t, Trav& s) {
     P.hit_t[pos] = s.best_t;
     P.hit_prim[pos] = s.best_prim;
-    if (sc.textured) {
-      P.hit_u[pos] = s.best_u;
-      P.hit_v[pos] = s.best_v;
-    }
     ++rays;
     return false;
   };
@@ -1147,7 +1138,7 @@ __device__ __forceinline__ float eff_grad(float pre_c, float k) { return (pre_c 
 // arrived at a hit (prim >= 0) or a miss (prim < 0). kAdj replays the iteration for the adjoint.
 template <bool kAdj>
 __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc, const DevSettings& st, uint32_t depth,
-                                           uint32_t p, V3 ro, V3 rd, float t, int32_t prim, float hu, float hv,
+                                           uint32_t p, V3 ro, V3 rd, float t, int32_t prim,
                                            ShadeOut& out, uint32_t& shadow_tests, const DevAdjoint& A, float* glds) {
   out.cont = false;
   out.shadow = false;
@@ -1182,6 +1173,10 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
   V3 base = ld3(m.albedo);
   float2 huv = make_float2(0.0f, 0.0f);
   if (m.base_color_texture_index < sc.texture_count) {
+    // the hit's barycentrics: the closest-hit test of the same ray against the same triangle record,
+    // recomputed here (bit-identical) rather than carried through the traversal
+    float ht = 0.0f, hu = 0.0f, hv = 0.0f;
+    tri_test(sc.tris + 3u * sc.prim_tri[prim], ro, rd, kInf, ht, hu, hv);
     huv = uv_at(sc.prim_uv + 6u * static_cast<uint32_t>(prim), hu, hv);
     base = mulv(base, tex_tap(sc, m.base_color_texture_index, huv).color);
   }
@@ -1464,8 +1459,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevPaths P, DevRays in, DevRay
       sid = in.qscene[q];
       const V3 ro = mk(in.c[0][q], in.c[1][q], in.c[2][q]);
       const V3 rd = mk(in.c[3][q], in.c[4][q], in.c[5][q]);
-      const float hu = sc.textured ? P.hit_u[q] : 0.0f, hv = sc.textured ? P.hit_v[q] : 0.0f;
-      shade_path<kAdj>(P, scene_view(sc, sid), st, depth, p, ro, rd, P.hit_t[q], P.hit_prim[q], hu, hv, so,
+      shade_path<kAdj>(P, scene_view(sc, sid), st, depth, p, ro, rd, P.hit_t[q], P.hit_prim[q], so,
                        shadow_tests, A, glds);
     }
     const uint32_t seg = (w0 >> 6) & (kSegments - 1u);  // this chunk's output segment
