@@ -230,13 +230,18 @@ def batched_comparison(log, count: int = 64, reps: int = 2) -> dict:
     triangle counts log-uniform in [1k, 500k], square images from {128, 256, 512, 1024}, spp from
     {4, 16, 64}, depth 8, NEE + MIS. batched = one pt_batch_render of every scene's full frame (eager
     launches, then HIP-graph replay); per_scene = one tracer per scene, each rendering its frame with
-    render_tiles, timed frame by frame and summed (scene upload and BVH builds are outside all timings)."""
+    render_tiles, timed frame by frame and summed (scene upload and BVH builds are outside all timings).
+    The batch uses the leaf width and launch shapes the autotuner chose for this workload (the
+    committed tuning table); the per-scene tracers use the defaults, which the tuner found within 1 %
+    of their best on the single-scene workload."""
     import torch
 
+    from vkpathtracer_amd import autotune
     from vkpathtracer_amd import scene as S
     from vkpathtracer_amd.tracer import HipBatch, HipPathTracer
 
     jobs = S.batch_scenes(seed=7, count=count)
+    tuned = autotune.tuned_params("batch64") if count == 64 else None
     settings = S.RenderSettings(width=1, height=1, spp=1, max_depth=8)
     paths = sum(w * h * spp for _, w, h, spp in jobs)
     tris = sum(sc.triangle_count for sc, _, _, _ in jobs)
@@ -251,11 +256,16 @@ def batched_comparison(log, count: int = 64, reps: int = 2) -> dict:
 
     out = {"workload": f"{count} seeded scenes, 1k-500k triangles (log-uniform), images 128^2..1024^2, spp 4/16/64, "
                        "depth 8", "scenes": count, "triangles": tris, "paths_per_frame": paths}
+    leaf = tuned["leaf_size"] if tuned else 4
+    out["tuning"] = {"source": "vkpathtracer_amd/tuning/mi355x.json (autotune.py)" if tuned else "defaults",
+                     "params": tuned}
     with HipBatch(settings) as b:
         for sc, w, h, spp in jobs:
-            b.add_scene(sc, w, h, spp)
+            b.add_scene(sc, w, h, spp, leaf_size=leaf)
         info = b.build()
         out["bvh_build_ms"] = round(info["build_ms"], 1)
+        if tuned:
+            autotune.apply_params(b, tuned)
         for use_graph, key in ((False, "batched"), (True, "batched_graph")):
             def frame():
                 b.reset_accumulation()
@@ -293,6 +303,8 @@ def main():
     ap.add_argument("--workload", choices=["procedural", "cornell"], default="procedural")
     ap.add_argument("--spp", type=int, default=None)
     ap.add_argument("--capacity", type=int, default=None, help="paths in flight per wavefront pass")
+    ap.add_argument("--tuning", choices=["table", "default"], default="table",
+                    help="leaf width and launch shapes: the autotuner's table for the workload, or the defaults")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="time without per-kernel events")
@@ -324,15 +336,21 @@ def main():
         log(f"copy probe: 3 x {args.copy_probe_mib} MiB device-to-device copies")
         del a, b
 
+    from vkpathtracer_amd import autotune
+
     sc, st, desc = workload(args.workload, args.spp)
     log(f"scene {sc.name}: {sc.triangle_count} triangles")
+    tuned = autotune.tuned_params(args.workload) if args.tuning == "table" else None
+    leaf = tuned["leaf_size"] if tuned else 4
     pt = HipPathTracer(0)
     assert pt.configure(st)
     assert pt.load_scene_snapshot(sc)
     t0 = time.perf_counter()
-    bst = pt.build_or_update_acceleration("sah", 4, 8)
+    bst = pt.build_or_update_acceleration("sah", leaf, 8)
     build_wall_ms = (time.perf_counter() - t0) * 1e3
     assert bst, bst.message
+    if tuned:
+        autotune.apply_params(pt, tuned)
     if args.capacity:
         pt.set_wavefront_capacity(args.capacity)
     tiles = [(0, 0, st.width, st.height, s, 0, 0) for s in range(st.spp)]
@@ -415,13 +433,16 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded procedural scene generated in-process)",
         "config": {"workload": desc, "width": st.width, "height": st.height, "spp": st.spp,
-                   "max_depth": st.max_depth, "triangles": sc.triangle_count, "bvh": "binned SAH, leaf<=4, 8 buckets",
+                   "max_depth": st.max_depth, "triangles": sc.triangle_count,
+                   "bvh": f"binned SAH, leaf<={leaf}, 8 buckets",
                    "nee": True, "mis": True, "rr_start_depth": 3, "seed": hex(st.seed)},
         "mrays_per_s": round(mrays, 3),
         "ref_paths_per_sec": round(ext_rays * args.steps / dt, 1),
         "rays_per_step": {"extension": ext_rays, "shadow": shadow_rays},
         "bvh_build_ms": round(build_wall_ms, 1),
         "bvh": bvh_info,
+        "tuning": {"source": "vkpathtracer_amd/tuning/mi355x.json (autotune.py)" if tuned else "defaults",
+                   "params": tuned},
         "roofline": roofline,
         "kernels": per_kernel,
         "cpu_baseline": cpu,

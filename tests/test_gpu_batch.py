@@ -107,3 +107,27 @@ def test_batch_graph_replays_match_eager_frames(cornell, small_procedural):
     for fa, fb in zip(eager, mixed):
         for a, c in zip(fa, fb):
             assert np.array_equal(a, c)
+
+
+def test_autotuned_batch_renders_the_same_films(cornell, small_procedural):
+    """The autotuner picks a leaf width and launch shapes by timing; whatever it picks, every film
+    equals that of a batch with the same leaf width and default launch shapes."""
+    from vkpathtracer_amd import scene as S
+    from vkpathtracer_amd.autotune import tune_batch
+    from vkpathtracer_amd.tracer import HipBatch
+
+    settings = S.RenderSettings(width=1, height=1, spp=1, max_depth=6)
+    jobs = [(sc, w, h, spp) for sc, w, h, spp, _ in _jobs(cornell, small_procedural)]
+    res, tb = tune_batch(jobs, settings, leaf_sizes=(2, 4), reps=1)
+    print(res.params, f"{res.default_ms:.2f} -> {res.ms:.2f} ms over {len(res.trials)} trials")
+    assert res.params["leaf_size"] in (2, 4) and res.ms <= res.default_ms
+    tb.reset_accumulation()
+    tb.render(0)
+    with HipBatch(settings) as b:
+        for sc, w, h, spp in jobs:
+            b.add_scene(sc, w, h, spp, leaf_size=res.params["leaf_size"])
+        b.build()
+        b.render(0)
+        for i in range(len(jobs)):
+            assert np.array_equal(tb.film(i)[0], b.film(i)[0]), i
+    tb.close()
