@@ -83,28 +83,62 @@ def kernel_bytes(ks: dict, counters: dict) -> dict:
     }
 
 
-def cpu_baseline(sc, st, target_seconds: float) -> dict:
-    """The oracle (CPU restatement, fp32, all host cores via OpenMP) on a bounded seeded pixel sample."""
+def _has_avx2() -> bool:
+    try:
+        with open("/proc/cpuinfo") as f:
+            return " avx2 " in f.read().replace("\n", " ")
+    except OSError:
+        return False
+
+
+def _cpu_rate(sc, st, variant: str, leaf: int, threads: int, target_seconds: float, rng_seed: int = 2024):
+    """Paths/s of the oracle `variant` (BVH leaves <= leaf) over a seeded pixel sample sized to take
+    about target_seconds for samples 0-3; returns (Mpaths/s, pixels, seconds)."""
     from oracle import oracle as O
 
-    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
-    o = O.Oracle(sc, builder="sah")
-    rng = np.random.default_rng(2024)
+    o = O.Oracle(sc, builder="sah", leaf_size=leaf, variant=variant)
+    rng = np.random.default_rng(rng_seed)
     n_px = st.width * st.height
-    probe = rng.choice(n_px, min(n_px, 20000), replace=False).astype(np.uint32)
-    t0 = time.perf_counter()
-    o.render_pixels(st, probe, 0, threads=cores)
-    rate = len(probe) / max(time.perf_counter() - t0, 1e-6)
+    rate = 0.0
+    for k in (2000, 8000):  # a warm-up probe, then the calibration probe
+        probe = rng.choice(n_px, min(n_px, k * threads), replace=False).astype(np.uint32)
+        t0 = time.perf_counter()
+        o.render_pixels(st, probe, 0, threads=threads)
+        rate = len(probe) / max(time.perf_counter() - t0, 1e-6)
     samples = 4
-    n = int(min(n_px, max(10000, rate * target_seconds / samples)))
+    n = int(min(n_px, max(2000, rate * target_seconds / samples)))
     px = rng.choice(n_px, n, replace=False).astype(np.uint32)
     t0 = time.perf_counter()
-    for s in range(samples):
-        o.render_pixels(st, px, s, threads=cores)
+    for s_ in range(samples):
+        o.render_pixels(st, px, s_, threads=threads)
     dt = time.perf_counter() - t0
-    return {"value": round(n * samples / dt / 1e6, 4), "unit": "Mpaths/s", "cores": cores, "kind": "port",
-            "sample": f"{n} seeded pixels x samples 0-{samples - 1} of the same workload ({n * samples} paths, "
-                      f"{dt:.1f} s), oracle fp32 restatement, OpenMP over {cores} threads, {cpu_model()}"}
+    return n * samples / dt / 1e6, n, dt
+
+
+def cpu_baseline(sc, st, target_seconds: float, log) -> dict:
+    """The CPU restatement (oracle, fp32) on the host cores, three ways, each on a bounded seeded pixel
+    sample of the same workload (samples 0-3 of n pixels): scalar = one thread, leaves of 4;
+    multicore = OpenMP over every core, leaves of 4; avx2 = every core, leaves of 8 intersected 8
+    triangles at a time with AVX2 (the reference's AVX2 configuration, CpuBvhAccelerator.cpp:252-256).
+    `value` is the AVX2 rate (multicore when the host lacks AVX2)."""
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    modes = {}
+    plan = [("scalar", "f32", 4, 1), ("multicore", "f32", 4, cores)]
+    if _has_avx2():
+        plan.append(("avx2", "f32_avx2", 8, cores))
+    for name, variant, leaf, threads in plan:
+        rate, n, dt = _cpu_rate(sc, st, variant, leaf, threads, target_seconds)
+        modes[name] = {"mpaths_per_s": round(rate, 4), "threads": threads, "leaf": leaf,
+                       "sample": f"{n} pixels x 4 samples, {dt:.1f} s"}
+        log(f"cpu {name}: {rate:.3f} Mpaths/s ({threads} threads)")
+    top = "avx2" if "avx2" in modes else "multicore"
+    m = modes[top]
+    return {"value": m["mpaths_per_s"], "unit": "Mpaths/s", "cores": m["threads"], "kind": "port",
+            "mode": top,
+            "sample": f"{m['sample']} of seeded pixels of the same workload, oracle fp32 restatement "
+                      f"({'AVX2 leaf-8 intersector, ' if top == 'avx2' else ''}OpenMP over {m['threads']} threads), "
+                      f"{cpu_model()}",
+            "modes": modes}
 
 
 def adjoint_ratio(log, reps: int = 3) -> dict:
@@ -305,7 +339,7 @@ def main():
     ap.add_argument("--capacity", type=int, default=None, help="paths in flight per wavefront pass")
     ap.add_argument("--tuning", choices=["table", "default"], default="table",
                     help="leaf width and launch shapes: the autotuner's table for the workload, or the defaults")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-seconds", type=float, default=8.0, help="target CPU time per baseline mode")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="time without per-kernel events")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward time ratio")
@@ -417,7 +451,7 @@ def main():
     textured = None if args.no_textured else textured_multiview(log)
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(sc, st, args.cpu_seconds)
+        cpu = cpu_baseline(sc, st, args.cpu_seconds, log)
 
     out = {
         "metric": "Mpaths/s & Mrays/s fwd at 1 GPU vs CPU AVX2; adjoint/forward time ratio",

@@ -6,6 +6,9 @@
  *
  * PTO_REAL    float (default) or double (the fp64 oracle)
  * PTO_LIBM    1: glibc sinf/cosf/powf like the reference; 0: pto_math.h
+ * PTO_AVX2    1: leaves are intersected 8 triangles at a time with AVX2 (fp32 only; the CPU
+ *             baseline's AVX2 mode, as the reference's leaf batch intersector when built with
+ *             PT_ENABLE_AVX2, BvhTriangleIntersector.cpp:156-171). Same results as the scalar loop.
  */
 #include "pto.h"
 
@@ -24,6 +27,15 @@
 #endif
 #ifndef PTO_LIBM
 #define PTO_LIBM 0
+#endif
+#ifndef PTO_AVX2
+#define PTO_AVX2 0
+#endif
+#if PTO_AVX2
+#if PTO_REAL_IS_DOUBLE
+#error "PTO_AVX2 is an fp32 build"
+#endif
+#include <immintrin.h>
 #endif
 
 #if PTO_REAL_IS_DOUBLE
@@ -195,6 +207,8 @@ struct pto_accel {
   float* prim_uv;
   float* texels;
   uint64_t texel_count;
+  float* soa;         /* PTO_AVX2: per leaf batch of 8, v0 xyz, e1 xyz, e2 xyz as 9 x 8 floats */
+  uint32_t* soa_base; /* PTO_AVX2: per node, its first batch in soa (leaves only) */
   PtoTextureDesc* textures;
   uint32_t texture_count;
 };
@@ -521,11 +535,35 @@ pto_accel* pto_accel_build(const PtoScene* scene, int builder, uint32_t leaf_siz
     free(tmp);
   }
   free(raw);
+#if PTO_AVX2
+  /* structure-of-arrays copies of every leaf's triangles, 8 per batch, unused lanes zero */
+  uint64_t batches = 0;
+  a->soa_base = (uint32_t*)calloc(a->node_count ? a->node_count : 1u, sizeof(uint32_t));
+  for (uint32_t i = 0; i < a->node_count; ++i)
+    if (a->nodes[i].count > 0) {
+      a->soa_base[i] = (uint32_t)batches;
+      batches += ((uint32_t)a->nodes[i].count + 7u) / 8u;
+    }
+  a->soa = (float*)aligned_alloc(32, sizeof(float) * 72u * (size_t)(batches ? batches : 1u));
+  memset(a->soa, 0, sizeof(float) * 72u * (size_t)(batches ? batches : 1u));
+  for (uint32_t i = 0; i < a->node_count; ++i) {
+    const node_t* nd = &a->nodes[i];
+    for (int32_t k = 0; k < nd->count; ++k) {
+      const prim_t* p = &a->prims[a->order[nd->first + k]];
+      float* b = a->soa + 72u * ((size_t)a->soa_base[i] + (uint32_t)k / 8u);
+      const uint32_t lane = (uint32_t)k & 7u;
+      const float f[9] = {p->v0.x, p->v0.y, p->v0.z, p->e1.x, p->e1.y, p->e1.z, p->e2.x, p->e2.y, p->e2.z};
+      for (int c = 0; c < 9; ++c) b[8 * c + lane] = f[c];
+    }
+  }
+#endif
   return a;
 }
 
 void pto_accel_free(pto_accel* a) {
   if (!a) return;
+  free(a->soa);
+  free(a->soa_base);
   free(a->prims);
   free(a->nodes);
   free(a->order);
@@ -671,6 +709,66 @@ static uint32_t leaf_batch(const pto_accel* a, const uint32_t* prims, uint32_t c
   return accepted;
 }
 
+#if PTO_AVX2
+/* leaf_batch for 8 triangles at once: the same Moller-Trumbore expressions lane-wise (mul / add /
+ * sub only, no fused ops), the same acceptance tests, then the lanes committed in order so the
+ * closest t, its lane (first lane on ties), the accepted count and u, v equal the scalar loop's. */
+static uint32_t leaf_batch_avx2(const float* b, uint32_t cnt, v3 o, v3 d, real* best_t, int* lane_out, real* u_out,
+                                real* v_out) {
+  *lane_out = -1;
+  if (cnt == 0 || !(*best_t > kEps)) return 0;
+  const __m256 v0x = _mm256_load_ps(b), v0y = _mm256_load_ps(b + 8), v0z = _mm256_load_ps(b + 16);
+  const __m256 e1x = _mm256_load_ps(b + 24), e1y = _mm256_load_ps(b + 32), e1z = _mm256_load_ps(b + 40);
+  const __m256 e2x = _mm256_load_ps(b + 48), e2y = _mm256_load_ps(b + 56), e2z = _mm256_load_ps(b + 64);
+  const __m256 dx = _mm256_set1_ps(d.x), dy = _mm256_set1_ps(d.y), dz = _mm256_set1_ps(d.z);
+  const __m256 hx = _mm256_sub_ps(_mm256_mul_ps(dy, e2z), _mm256_mul_ps(dz, e2y));
+  const __m256 hy = _mm256_sub_ps(_mm256_mul_ps(dz, e2x), _mm256_mul_ps(dx, e2z));
+  const __m256 hz = _mm256_sub_ps(_mm256_mul_ps(dx, e2y), _mm256_mul_ps(dy, e2x));
+  const __m256 det =
+      _mm256_add_ps(_mm256_add_ps(_mm256_mul_ps(e1x, hx), _mm256_mul_ps(e1y, hy)), _mm256_mul_ps(e1z, hz));
+  const __m256 inv = _mm256_div_ps(_mm256_set1_ps(1.0f), det);
+  const __m256 sx = _mm256_sub_ps(_mm256_set1_ps(o.x), v0x);
+  const __m256 sy = _mm256_sub_ps(_mm256_set1_ps(o.y), v0y);
+  const __m256 sz = _mm256_sub_ps(_mm256_set1_ps(o.z), v0z);
+  const __m256 u = _mm256_mul_ps(
+      _mm256_add_ps(_mm256_add_ps(_mm256_mul_ps(sx, hx), _mm256_mul_ps(sy, hy)), _mm256_mul_ps(sz, hz)), inv);
+  const __m256 qx = _mm256_sub_ps(_mm256_mul_ps(sy, e1z), _mm256_mul_ps(sz, e1y));
+  const __m256 qy = _mm256_sub_ps(_mm256_mul_ps(sz, e1x), _mm256_mul_ps(sx, e1z));
+  const __m256 qz = _mm256_sub_ps(_mm256_mul_ps(sx, e1y), _mm256_mul_ps(sy, e1x));
+  const __m256 v = _mm256_mul_ps(
+      _mm256_add_ps(_mm256_add_ps(_mm256_mul_ps(dx, qx), _mm256_mul_ps(dy, qy)), _mm256_mul_ps(dz, qz)), inv);
+  const __m256 t = _mm256_mul_ps(
+      _mm256_add_ps(_mm256_add_ps(_mm256_mul_ps(e2x, qx), _mm256_mul_ps(e2y, qy)), _mm256_mul_ps(e2z, qz)), inv);
+  const __m256 zero = _mm256_setzero_ps(), one = _mm256_set1_ps(1.0f), eps = _mm256_set1_ps(kEps);
+  const __m256 adet = _mm256_andnot_ps(_mm256_set1_ps(-0.0f), det);
+  /* the scalar loop's rejections, negated (unordered compares keep NaN lanes as the scalar loop does) */
+  __m256 ok = _mm256_cmp_ps(adet, eps, _CMP_NLT_UQ);
+  ok = _mm256_and_ps(ok, _mm256_cmp_ps(u, zero, _CMP_NLT_UQ));
+  ok = _mm256_and_ps(ok, _mm256_cmp_ps(u, one, _CMP_NGT_UQ));
+  ok = _mm256_and_ps(ok, _mm256_cmp_ps(v, zero, _CMP_NLT_UQ));
+  ok = _mm256_and_ps(ok, _mm256_cmp_ps(_mm256_add_ps(u, v), one, _CMP_NGT_UQ));
+  ok = _mm256_and_ps(ok, _mm256_cmp_ps(t, eps, _CMP_NLE_UQ));
+  uint32_t mask = (uint32_t)_mm256_movemask_ps(ok) & ((1u << cnt) - 1u);
+  if (!mask) return 0;
+  float tl[8], ul[8], vl[8];
+  _mm256_storeu_ps(tl, t);
+  _mm256_storeu_ps(ul, u);
+  _mm256_storeu_ps(vl, v);
+  uint32_t accepted = 0;
+  real out_t = *best_t;
+  for (uint32_t lane = 0; lane < cnt; ++lane) {
+    if (!(mask >> lane & 1u) || tl[lane] >= out_t) continue;
+    ++accepted;
+    out_t = tl[lane];
+    *lane_out = (int)lane;
+    *u_out = ul[lane];
+    *v_out = vl[lane];
+  }
+  *best_t = out_t;
+  return accepted;
+}
+#endif
+
 /* Closest hit (any_hit == 0) or any hit with t < max_t (any_hit == 1). The any-hit form
  * is the GPU shadow query's contract: it stops at the first accepted triangle. */
 static qhit_t traverse(const pto_accel* a, v3 o, v3 d, real max_t, int any_hit, PtoCounters* c) {
@@ -697,7 +795,13 @@ static qhit_t traverse(const pto_accel* a, v3 o, v3 d, real max_t, int any_hit, 
         c->triangle_tests += cnt;
         int lane;
         real bu = RC(0.0), bv = RC(0.0);
+#if PTO_AVX2
+        const uint32_t batch = (uint32_t)(i - (int32_t)cnt) / 8u;
+        const uint32_t acc = leaf_batch_avx2(a->soa + 72u * ((size_t)a->soa_base[ni] + batch), cnt, o, d, &best_t,
+                                             &lane, &bu, &bv);
+#else
         const uint32_t acc = leaf_batch(a, lanes, cnt, o, d, &best_t, &lane, &bu, &bv);
+#endif
         if (lane < 0) continue;
         c->triangle_hits += acc;
         h.hit = 1;
