@@ -135,6 +135,29 @@ class Scene:
                   else np.zeros((0, 3), np.float32))
         return np.ascontiguousarray(texels, np.float32), descs
 
+    def save_snapshot(self, path) -> None:
+        """Write the scene as a binary snapshot ("VKPTSNAP" version 1), the file format the C++ host
+        (vkpathtracer_amd/host, pt_hip_bench --scene) reads: magic, a header of counts, environment
+        colour and scale, the camera record, then every array in the C ABI's record layouts."""
+        import struct
+
+        env = self.environment_map
+        ew, eh = (0, 0) if env is None else (env.shape[1], env.shape[0])
+        texels, descs = self.texture_arrays()
+        tc = self.texcoords if self.texcoords is not None else np.zeros((0, 2), np.float32)
+        c = self.camera
+        cam = struct.pack("<12fI3f", *c.position, *c.target, *c.up, c.fov_deg, c.aperture_radius, c.focus_distance,
+                          c.iris_blade_count, c.iris_rotation_degrees, c.iris_roundness, c.anamorphic_squeeze)
+        head = struct.pack("<12IQ6f", 1, len(self.vertices), len(self.indices), len(self.local_vertices),
+                           len(self.local_indices), len(self.instances), len(self.materials), len(self.lights), ew, eh,
+                           len(tc), len(descs), len(texels), *self.environment_color, *self.environment_map_scale)
+        with open(path, "wb") as f:
+            f.write(b"VKPTSNAP" + head + cam)
+            for a in (self.vertices, self.indices, self.local_vertices, self.local_indices, self.instances,
+                      self.materials, self.lights, env if env is not None else np.zeros(0, np.float32), tc, texels,
+                      descs):
+                f.write(np.ascontiguousarray(a).tobytes())
+
     @property
     def triangle_count(self) -> int:
         return int(self.instances["triangle_count"].astype(np.int64).sum())
