@@ -84,7 +84,7 @@ struct Pass {
   size_t first = 0, count = 0;
   uint32_t paths = 0;
   size_t job_first = 0, job_count = 0;
-  uint32_t job_pixels = 0;
+  uint32_t job_waves = 0;
 };
 
 }  // namespace
@@ -389,7 +389,7 @@ bool same_work(const std::vector<ptd::DevTile>& a, const std::vector<ptd::DevTil
   if (a.size() != b.size()) return false;
   for (size_t i = 0; i < a.size(); ++i)
     if (a[i].x != b[i].x || a[i].y != b[i].y || a[i].w != b[i].w || a[i].h != b[i].h || a[i].sample != b[i].sample ||
-        a[i].path_begin != b[i].path_begin || a[i].scene != b[i].scene)
+        a[i].path_begin != b[i].path_begin || a[i].scene != b[i].scene || a[i].group != b[i].group)
       return false;
   return true;
 }
@@ -811,15 +811,25 @@ int plan_passes(pt_tracer* t, std::vector<ptd::DevTile>& work, std::vector<Pass>
     }
     std::stable_sort(work.begin() + static_cast<std::ptrdiff_t>(next), work.begin() + static_cast<std::ptrdiff_t>(end),
                      [](const ptd::DevTile& a, const ptd::DevTile& b) { return a.scene < b.scene; });
+    // path numbering: runs of tiles with the same scene and rectangle interleave (DevTile::group)
     uint32_t pb = 0;
-    for (size_t i = next; i < end; ++i) {
-      work[i].path_begin = pb;
-      pb += work[i].w * work[i].h;
+    for (size_t i = next; i < end;) {
+      size_t j = i + 1;
+      while (j < end && j - i < 0xFFFFu && work[j].scene == work[i].scene && work[j].x == work[i].x &&
+             work[j].y == work[i].y && work[j].w == work[i].w && work[j].h == work[i].h)
+        ++j;
+      const uint32_t G = static_cast<uint32_t>(j - i);
+      for (uint32_t k = 0; k < G; ++k) {
+        work[i + k].path_begin = pb;
+        work[i + k].group = (G << 16) | k;
+      }
+      pb += work[i].w * work[i].h * G;
+      i = j;
     }
     ps.count = end - next;
     ps.paths = pb;
     ps.job_first = jobs.size();
-    uint32_t pix = 0;
+    uint32_t waves = 0;
     for (size_t i = next; i < end;) {
       size_t j = i;
       ptd::FilmJob J{};
@@ -838,13 +848,14 @@ int plan_passes(pt_tracer* t, std::vector<ptd::DevTile>& work, std::vector<Pass>
       J.h = y1 - y0;
       J.first_tile = static_cast<uint32_t>(i - next);
       J.tile_count = static_cast<uint32_t>(j - i);
-      J.pixel_begin = pix;
-      pix += J.w * J.h;
+      J.wave_begin = waves;
+      waves += (J.w * J.h + 63u) / 64u;
+      J.group = (work[i].group >> 16) == J.tile_count ? J.tile_count : 1u;
       jobs.push_back(J);
       i = j;
     }
     ps.job_count = jobs.size() - ps.job_first;
-    ps.job_pixels = pix;
+    ps.job_waves = waves;
     passes.push_back(ps);
     next = end;
   }
@@ -926,7 +937,7 @@ int launch_passes(pt_tracer* t, const std::vector<ptd::DevTile>& work, const std
     } else {
       timed(t, PT_KERNEL_FILM, ps.paths, [&] {
         ptd::launch_film(t->stream, P, t->dscene, film, tl, djobs + ps.job_first, static_cast<uint32_t>(ps.job_count),
-                         ps.job_pixels);
+                         ps.job_waves);
       });
       t->fwd_rad_valid = single_pass;
       if (single_pass) {

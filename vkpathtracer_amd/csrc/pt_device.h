@@ -118,17 +118,25 @@ struct DevSettings {
   float rr_min, rr_max;
 };
 
-// Tile of one wavefront pass: paths [path_begin, path_begin + w*h) of the pass render sample
-// `sample` of this rectangle of scene `scene`'s image.
+// Tile of one wavefront pass: w*h paths render sample `sample` of this rectangle of scene `scene`'s
+// image. Consecutive tiles of a pass with the same scene and rectangle form a group of G tiles
+// (G = group >> 16; 1 for a lone tile) whose paths interleave: pixel i of the group's k-th tile
+// (k = group & 0xFFFF) is path path_begin + i * G + k. A pixel's samples are neighbours in the ray
+// queues, so a wave traces up to 64 samples of one pixel: nearly identical camera rays.
 struct DevTile {
-  uint32_t x, y, w, h, sample, path_begin, scene, pad;
+  uint32_t x, y, w, h, sample, path_begin, scene, group;
 };
 
+__host__ __device__ inline uint32_t tile_slot(const DevTile& t, uint32_t local) {
+  return t.path_begin + local * (t.group >> 16) + (t.group & 0xFFFFu);
+}
+
 // Film accumulation of one scene in one pass: the bounding rectangle of the scene's tiles in the
-// pass and their range in the pass's tile table (in render order). Job pixels are numbered from
-// pixel_begin across the pass's jobs.
+// pass and their range in the pass's tile table (in render order). The film kernel gives each job
+// ceil(w*h / 64) waves, numbered from wave_begin across the pass's jobs. group = G when the job's
+// tiles are exactly one interleaved group of G tiles (DevTile::group), else 1.
 struct FilmJob {
-  uint32_t scene, x0, y0, w, h, first_tile, tile_count, pixel_begin;
+  uint32_t scene, x0, y0, w, h, first_tile, tile_count, wave_begin, group, pad[3];
 };
 
 // One ray queue: origin xyz, direction xyz (SoA), the path slot each entry serves and its scene.
@@ -214,7 +222,7 @@ void launch_shadow(hipStream_t stream, const DevPaths& paths, const DevScene& sc
 void launch_advance(hipStream_t stream, const DevPaths& paths);
 // Adds the radiance of every tile of the pass into the films, pixel by pixel in tile order.
 void launch_film(hipStream_t stream, const DevPaths& paths, const DevScene& scene, const DevFilm& film,
-                 const DevTile* tiles, const FilmJob* jobs, uint32_t job_count, uint32_t job_pixels);
+                 const DevTile* tiles, const FilmJob* jobs, uint32_t job_count, uint32_t job_waves);
 void launch_trace_rays(hipStream_t stream, const DevPaths& paths, const DevScene& scene, int any_hit,
                        const float* rays, const float* tmax, uint32_t count, float* out_t, int32_t* out_prim,
                        uint32_t grid);

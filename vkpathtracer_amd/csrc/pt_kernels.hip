@@ -500,16 +500,18 @@ __global__ __launch_bounds__(kBlock) void k_raygen(DevPaths P, DevRays out, DevS
                                                   DevAdjoint A) {
   const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
   if (p >= path_count) return;
-  uint32_t lo = 0, hi = tile_count;  // last tile with path_begin <= p
+  uint32_t lo = 0, hi = tile_count;  // last tile with path_begin <= p: the last tile of p's group
   while (hi - lo > 1u) {
     const uint32_t mid = (lo + hi) >> 1;
     if (tiles[mid].path_begin <= p) lo = mid; else hi = mid;
   }
-  const DevTile tile = tiles[lo];
+  const uint32_t G = tiles[lo].group >> 16;
+  const uint32_t off = p - tiles[lo].path_begin;
+  const DevTile tile = tiles[lo - (G - 1u) + off % G];
   const SceneRec& rec = sc.recs[tile.scene];
   const DevCamera& cam = rec.cam;
   const uint32_t width = rec.width, height = rec.height;
-  const uint32_t local = p - tile.path_begin;
+  const uint32_t local = off / G;
   const uint32_t x = tile.x + local % tile.w;
   const uint32_t y = tile.y + local / tile.w;
   const uint32_t sample = tile.sample;
@@ -1585,45 +1587,82 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevPaths P, DevScene sc, DevA
 // (ScalarCpuPathTracer.Render.cpp:270-283, FilmBuffer.cpp:80-91). One thread per pixel of each
 // scene's tile rectangle adds that scene's samples of the pass in tile order, as sequential
 // render_tile calls would.
-__global__ __launch_bounds__(kBlock) void k_film(DevPaths P, DevScene sc, DevFilm film, const DevTile* tiles,
-                                                const FilmJob* jobs, uint32_t job_count, uint32_t job_pixels) {
-  const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
-  uint32_t added = 0;
-  if (g < job_pixels) {
-    uint32_t lo = 0, hi = job_count;  // last job with pixel_begin <= g
-    while (hi - lo > 1u) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (jobs[mid].pixel_begin <= g) lo = mid; else hi = mid;
+// Film accumulation, one wave (one 64-thread block) per 64 pixels of a job, each lane one pixel: the
+// pixel's samples are added in tile order (the reference's sample order), so the sums are those of
+// adding one sample per render_tile call. A job that is one interleaved group keeps a pixel's
+// samples in consecutive path slots; the wave stages them through LDS in chunks of 32 samples with
+// coalesced loads (lanes 0-31 and 32-63 read 32 samples of two pixels per instruction), then each
+// lane sums its pixel's chunk in order.
+constexpr uint32_t kFilmChunk = 32;
+__global__ __launch_bounds__(64) void k_film(DevPaths P, DevScene sc, DevFilm film, const DevTile* tiles,
+                                            const FilmJob* jobs, uint32_t job_count, uint32_t job_waves) {
+  __shared__ float stage[3][64 * (kFilmChunk + 1)];
+  const uint32_t w = blockIdx.x;
+  if (w >= job_waves) return;
+  uint32_t lo = 0, hi = job_count;  // last job with wave_begin <= w
+  while (hi - lo > 1u) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (jobs[mid].wave_begin <= w) lo = mid; else hi = mid;
+  }
+  const FilmJob J = jobs[lo];
+  const uint32_t lane = threadIdx.x;
+  const uint32_t jp = J.w * J.h;
+  const uint32_t first = (w - J.wave_begin) * 64u;
+  const uint32_t local = first + lane;
+  const bool valid = local < jp;
+  const uint32_t x = J.x0 + local % J.w, y = J.y0 + local / J.w;
+  const SceneRec& R = sc.recs[J.scene];
+  const uint32_t px = R.film_base + y * R.width + x;
+  float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  uint32_t cnt = 0, added = 0;
+  bool touched = false;
+  auto add = [&](float r, float g, float b) {
+    if (!(finite(r) && finite(g) && finite(b))) return;
+    if (!touched) {
+      s = film.sum[px];
+      cnt = film.count[px];
+      touched = true;
     }
-    const FilmJob J = jobs[lo];
-    const uint32_t local = g - J.pixel_begin;
-    const uint32_t x = J.x0 + local % J.w, y = J.y0 + local / J.w;
-    const SceneRec& R = sc.recs[J.scene];
-    const uint32_t px = R.film_base + y * R.width + x;
-    float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    uint32_t cnt = 0;
-    bool touched = false;
+    s.x = s.x + r;
+    s.y = s.y + g;
+    s.z = s.z + b;
+    ++cnt;
+    ++added;
+  };
+  if (J.group > 1u) {
+    const uint32_t G = J.group, base = tiles[J.first_tile].path_begin;
+    const uint32_t npx = jp - first < 64u ? jp - first : 64u;
+    for (uint32_t kc = 0; kc < G; kc += kFilmChunk) {
+      const uint32_t kn = G - kc < kFilmChunk ? G - kc : kFilmChunk;
+      const uint32_t k = lane & (kFilmChunk - 1u);
+      for (uint32_t i = 0; i < 64u; i += 2u) {
+        const uint32_t j = i + (lane >> 5);
+        if (j < npx && k < kn) {
+          const uint32_t slot = base + (first + j) * G + kc + k;
+          stage[0][j * (kFilmChunk + 1u) + k] = P.rad[0][slot];
+          stage[1][j * (kFilmChunk + 1u) + k] = P.rad[1][slot];
+          stage[2][j * (kFilmChunk + 1u) + k] = P.rad[2][slot];
+        }
+      }
+      __syncthreads();
+      if (valid)
+        for (uint32_t q = 0; q < kn; ++q) {
+          const uint32_t o = lane * (kFilmChunk + 1u) + q;
+          add(stage[0][o], stage[1][o], stage[2][o]);
+        }
+      __syncthreads();
+    }
+  } else if (valid) {
     for (uint32_t k = 0; k < J.tile_count; ++k) {
       const DevTile& t = tiles[J.first_tile + k];
       if (x < t.x || y < t.y || x - t.x >= t.w || y - t.y >= t.h) continue;
-      const uint32_t p = t.path_begin + (y - t.y) * t.w + (x - t.x);
-      const float r = P.rad[0][p], gg = P.rad[1][p], b = P.rad[2][p];
-      if (!(finite(r) && finite(gg) && finite(b))) continue;
-      if (!touched) {
-        s = film.sum[px];
-        cnt = film.count[px];
-        touched = true;
-      }
-      s.x = s.x + r;
-      s.y = s.y + gg;
-      s.z = s.z + b;
-      ++cnt;
-      ++added;
+      const uint32_t p = tile_slot(t, (y - t.y) * t.w + (x - t.x));
+      add(P.rad[0][p], P.rad[1][p], P.rad[2][p]);
     }
-    if (touched) {
-      film.sum[px] = s;
-      film.count[px] = cnt;
-    }
+  }
+  if (touched) {
+    film.sum[px] = s;
+    film.count[px] = cnt;
   }
   wave_add(P.counters, kSamples, added);
 }
@@ -1714,10 +1753,10 @@ void launch_shadow_adjoint(hipStream_t stream, const DevPaths& paths, const DevA
 }
 
 void launch_film(hipStream_t stream, const DevPaths& paths, const DevScene& scene, const DevFilm& film,
-                 const DevTile* tiles, const FilmJob* jobs, uint32_t job_count, uint32_t job_pixels) {
-  if (job_pixels == 0) return;
-  hipLaunchKernelGGL(k_film, dim3((job_pixels + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, paths, scene, film,
-                     tiles, jobs, job_count, job_pixels);
+                 const DevTile* tiles, const FilmJob* jobs, uint32_t job_count, uint32_t job_waves) {
+  if (job_waves == 0) return;
+  hipLaunchKernelGGL(k_film, dim3(job_waves), dim3(64), 0, stream, paths, scene, film, tiles, jobs, job_count,
+                     job_waves);
 }
 
 void launch_trace_rays(hipStream_t stream, const DevPaths& paths, const DevScene& scene, int any_hit,
