@@ -32,52 +32,34 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E
 
-# Algorithmic bytes per unit of work, by kernel: extension ray in (24) + hit out (8), one 64-byte
-# two-child BVH node per inner-node fetch, one 48-byte triangle record per test; shade reads the ray,
-# hit, path state, primitive and material and writes state and the next ray (204); shadow rays carry
-# origin, direction, t_max, contribution and path slot plus the radiance update (56); raygen writes
-# 60 bytes per path; film reads the radiance and updates sum and count (52).
+# Algorithmic bytes per unit of work (the per-unit figures of the hot-path measurement contract,
+# with reference SampleCounters semantics): 32 bytes per BVH node visit (one child box), 48 per
+# triangle test (v0, e1, e2 padded to float4 records), an extension ray 32 in + 16 out, a shadow ray
+# 32 in + 12 contribution + 4 result, a shade invocation 2 x 40 path state + 64 material, a film
+# sample 16 (read-modify-write); raygen writes 60 bytes per path (ray 24, throughput and radiance 24,
+# RNG 8, queue entry 4).
 BYTES = {
-    "ray_in": 24, "hit_out": 8, "node": 64, "triangle": 48,
-    "shade_path": 204, "shadow_record": 48, "shadow_ray": 56, "raygen_path": 60, "film_sample": 52,
+    "node_visit": 32, "triangle": 48, "ext_ray": 32 + 16, "shadow_ray": 32 + 12 + 4, "shade": 2 * 40 + 64,
+    "film_sample": 16, "raygen_path": 60,
 }
 
-
-def cpu_model() -> str:
-    try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
-    except OSError:
-        pass
-    return platform.processor() or "unknown"
-
-
-def workload(name: str, spp: int | None):
-    from vkpathtracer_amd import scene as S
-
-    if name == "procedural":
-        sc = S.procedural_scene(seed=1)
-        st = S.RenderSettings(width=1920, height=1080, spp=spp or 64, max_depth=8)
-        desc = "procedural 1M-tri (displaced icosphere 655360 + 2048x168 spheres + 16 quad lights), 1920x1080"
-    else:
-        sc = S.cornell_box()
-        st = S.RenderSettings(width=256, height=256, spp=spp or 16, max_depth=8)
-        desc = "cornell box 34 tris + light quad, 256x256"
-    return sc, st, desc
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this workload
+# (calibrated against a plain device copy: FETCH_SIZE x 2, WRITE_SIZE x 1), from
+# profiles/wavefront_procedural_1080p_64spp.md. They describe the current kernels; re-measure after
+# a kernel change.
+MEASURED_TRAFFIC = {"extend": 21.502e9, "shadow": 15.431e9, "shade": 8.163e9, "film": 1.677e9, "raygen": 8.494e9}
 
 
 def kernel_bytes(ks: dict, counters: dict) -> dict:
     ext_rays = counters["rays"]
     sh_rays = ks["shadow_rays"]
-    ext_nodes = ks["inner_node_fetches"] - ks["shadow_inner_node_fetches"]
+    ext_visits = counters["bvh_node_visits"] - ks["shadow_node_visits"]
     ext_tris = counters["triangle_tests"] - ks["shadow_triangle_tests"]
     return {
-        "extend": ext_rays * (BYTES["ray_in"] + BYTES["hit_out"]) + ext_nodes * BYTES["node"] + ext_tris * BYTES["triangle"],
-        "shadow": sh_rays * BYTES["shadow_ray"] + ks["shadow_inner_node_fetches"] * BYTES["node"]
+        "extend": ext_rays * BYTES["ext_ray"] + ext_visits * BYTES["node_visit"] + ext_tris * BYTES["triangle"],
+        "shadow": sh_rays * BYTES["shadow_ray"] + ks["shadow_node_visits"] * BYTES["node_visit"]
                   + ks["shadow_triangle_tests"] * BYTES["triangle"],
-        "shade": ext_rays * BYTES["shade_path"] + sh_rays * BYTES["shadow_record"],
+        "shade": ext_rays * BYTES["shade"],
         "raygen": ks["raygen"]["items"] * BYTES["raygen_path"],
         "film": ks["film"]["items"] * BYTES["film_sample"],
     }
@@ -438,10 +420,17 @@ def main():
         dom = max(per_kernel, key=lambda n: per_kernel[n]["total_ms"])
         k = ks[dom]
         achieved = kb[dom] / (k["total_ms"] * 1e-3) / 1e9
+        avg_ms = k["total_ms"] / k["launches"]
+        profiled = args.workload == "procedural" and args.spp is None and not args.capacity
+        traffic = MEASURED_TRAFFIC.get(dom) if profiled else None
         roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
-                    "bytes_per_launch": int(kb[dom] / k["launches"]),
-                    "avg_launch_ms": round(k["total_ms"] / k["launches"], 4)}
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                    "traffic": int(traffic) if traffic else None,
+                    "traffic_source": "profiles/wavefront_procedural_1080p_64spp.md (rocprofv3 FETCH_SIZE, WRITE_SIZE)"
+                    if traffic else None,
+                    "measured_hbm_gbps": round(traffic / (avg_ms * 1e6), 1) if traffic else None,
+                    "bytes_per_launch": int(kb[dom] / k["launches"]), "bytes_per_unit": BYTES,
+                    "avg_launch_ms": round(avg_ms, 4)}
 
     log(f"timed {args.steps} steps: {dt:.3f} s")
     bvh_info = {k: pt.build_info[k] for k in ("node_count", "leaf_count", "max_depth", "device_bytes")}
