@@ -50,6 +50,31 @@ BYTES = {
 MEASURED_TRAFFIC = {"extend": 21.502e9, "shadow": 15.431e9, "shade": 8.163e9, "film": 1.677e9, "raygen": 8.494e9}
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def workload(name: str, spp: int | None):
+    from vkpathtracer_amd import scene as S
+
+    if name == "procedural":
+        sc = S.procedural_scene(seed=1)
+        st = S.RenderSettings(width=1920, height=1080, spp=spp or 64, max_depth=8)
+        desc = "procedural 1M-tri (displaced icosphere 655360 + 2048x168 spheres + 16 quad lights), 1920x1080"
+    else:
+        sc = S.cornell_box()
+        st = S.RenderSettings(width=256, height=256, spp=spp or 16, max_depth=8)
+        desc = "cornell box 34 tris + light quad, 256x256"
+    return sc, st, desc
+
+
 def kernel_bytes(ks: dict, counters: dict) -> dict:
     ext_rays = counters["rays"]
     sh_rays = ks["shadow_rays"]
