@@ -110,6 +110,7 @@ struct pt_tracer {
   DevArray<ptd::TexRec> d_texrecs;
   DevArray<float> d_texels, d_prim_uv;
   DevArray<uint32_t> d_prim_tri;
+  DevArray<float> d_grad_partial;  // adjoint: per-block gradient rows
   size_t texel_count = 0;
   DevArray<uint32_t> d_film_count, d_queue, d_error;
   DevArray<unsigned long long> d_counters, d_query_counters;
@@ -304,10 +305,10 @@ void free_paths(pt_tracer* t) {
   t->path_capacity_alloc = 0;
   t->d_adj_f.release();
   t->d_adj_u.release();
+  t->d_grad_partial.release();
   t->adj_capacity_alloc = 0;
 }
 
-// Adjoint state for `cap` paths: 29 float arrays and 8 u32 arrays (see ptd::DevAdjoint).
 // Adjoint state: 33 + 2 * kAdjHist float arrays and 4 + kAdjHist u32 arrays of max(cap, kSegments *
 // seg_cap) entries.
 constexpr size_t kAdjFloats = 33 + 2 * ptd::kAdjHist;
@@ -340,6 +341,7 @@ int ensure_adjoint(pt_tracer* t, uint32_t cap, ptd::DevAdjoint& A) {
     A.s_uv[k] = f + (31 + k) * n;
     for (int h = 0; h < ptd::kAdjHist; ++h) A.h_uv[h][k] = f + (33 + 2 * h + k) * n;
   }
+  A.stride = static_cast<uint32_t>(n);
   A.p_mat = u;
   A.flags = u + n;
   A.s_mat = u + 2 * n;
@@ -1451,9 +1453,21 @@ static int run_adjoint(pt_tracer* t, std::vector<ptd::DevTile>& work, uint64_t f
   A.film_grad = film_grad;
   A.nmat = t->dscene.material_count;
   A.nlight = t->dscene.light_count;
-  A.use_lds = (7u * A.nmat + 3u * A.nlight) <= static_cast<uint32_t>(ptd::kAdjLdsFloats) ? 1u : 0u;
+  const uint32_t slots = 7u * A.nmat + 3u * A.nlight;
+  A.use_lds = slots <= static_cast<uint32_t>(ptd::kAdjLdsFloats) ? 1u : 0u;
+  A.lds_stride = slots | 1u;  // odd: the copies of a slot sit in different banks
+  A.lds_copies = 1;
+  while (A.lds_copies < 16u && 2u * A.lds_copies * A.lds_stride <= static_cast<uint32_t>(ptd::kAdjLdsFloats))
+    A.lds_copies *= 2u;
+  if (A.use_lds) {
+    PT_HIP(t, t->d_grad_partial.ensure(static_cast<size_t>(ptd::kGradRows) * slots), "allocate gradient rows");
+    A.partial = t->d_grad_partial.ptr;
+    A.partial_rows = ptd::kGradRows;
+  }
+  ptd::launch_grad_clear(t->stream, A);
   int rc = run_passes(t, work, dev_settings(t, frame_index), &A);
   if (rc != PT_OK) return rc;
+  ptd::launch_grad_reduce(t->stream, A);
   if (t->profiling) {
     rc = sync_all(t);
     if (rc != PT_OK) return rc;

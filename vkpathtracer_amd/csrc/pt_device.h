@@ -183,7 +183,7 @@ struct DevFilm {
 // vertices before the roulette start depth (see k_shade). Per shadow record: the gradient pieces
 // that apply if the light turns out visible.
 constexpr int kAdjHist = 4;          // max roulette start depth supported by the adjoint
-constexpr int kAdjLdsFloats = 3072;  // block-staged gradient slots (12 KB of LDS)
+constexpr int kAdjLdsFloats = 6144;  // block-staged gradient slots, all copies (24 KB of LDS)
 struct DevAdjoint {
   float* ltot[3];
   float* delta[3];
@@ -191,8 +191,8 @@ struct DevAdjoint {
   uint32_t* p_mat;
   uint32_t* flags;  // bit0 pending, bit1 active, bits2-3 c*, bits4-6 nhist, bit7 prev active,
                     // bits8-9 prev c*, bits10-12 prev nhist
-  float* h_dx[kAdjHist][3];
-  uint32_t* h_mat[kAdjHist];
+  float* h_dx[kAdjHist][3];   // h_dx[h][c] = h_dx[0][0] + (3h + c) * stride
+  uint32_t* h_mat[kAdjHist];  // h_mat[h] = h_mat[0] + h * stride
   float* s_galb[3];
   uint32_t* s_mat;
   float* s_glight[3];
@@ -200,7 +200,7 @@ struct DevAdjoint {
   float* p_score;   // d log(density of the pending bounce's direction) / d roughness
   float* s_grough;  // per shadow record: d contribution / d roughness, weighted by the loss adjoint
   float* p_uv[2];   // texture coordinates of the pending bounce's, the history's and the shadow
-  float* h_uv[kAdjHist][2];  // records' vertices (read for textured materials only)
+  float* h_uv[kAdjHist][2];  // records' vertices (read for textured materials only); h_uv[h][k] = h_uv[0][0] + (2h + k) * stride
   float* s_uv[2];
   float* g_albedo;    // 3 per material
   float* g_emission;  // 3 per material
@@ -209,6 +209,13 @@ struct DevAdjoint {
   float* g_tex;       // 3 per texel of the texture pool (nullptr: not requested)
   const float* film_grad;  // 3 per film pixel (all scenes' films in pool order): dLoss / d(film sum)
   uint32_t nmat, nlight, use_lds;
+  uint32_t stride;  // entries per adjoint state array
+  uint32_t lds_copies, lds_stride;  // LDS staging: copies (a power of two) of the slots, stride apart
+  // LDS staging: block b of every adjoint kernel adds its slots into row b of `partial` (plain
+  // read-modify-write: kernels of one stream never overlap, and rows are private to block indices);
+  // launch_grad_reduce adds the rows into the gradient buffers once at the end.
+  float* partial;
+  uint32_t partial_rows;
 };
 
 // Launchers (pt_kernels.hip). All launch on `stream`; none synchronizes.
@@ -235,6 +242,9 @@ void launch_shade_adjoint(hipStream_t stream, const DevPaths& paths, const DevAd
 void launch_shadow_adjoint(hipStream_t stream, const DevPaths& paths, const DevAdjoint& adj, const DevScene& scene,
                            uint32_t grid);
 void launch_copy_radiance(hipStream_t stream, const DevPaths& paths, const DevAdjoint& adj, uint32_t path_count);
+void launch_grad_clear(hipStream_t stream, const DevAdjoint& adj);   // zero the partial rows
+void launch_grad_reduce(hipStream_t stream, const DevAdjoint& adj);  // rows -> gradient buffers
+constexpr uint32_t kGradRows = 1024;  // partial rows: an upper bound on every adjoint kernel's grid
 void launch_zero_counters(hipStream_t stream, unsigned long long* counters);  // kCounterWords
 void launch_clear_film(hipStream_t stream, const DevFilm& film, uint32_t pixels);
 

@@ -579,18 +579,26 @@ __global__ __launch_bounds__(kBlock) void k_copy_radiance(DevPaths P, DevAdjoint
 // LDS per block when they fit, else added straight to the global fp32 buffers.
 enum GradKind { kGradAlbedo = 0, kGradEmission = 1, kGradLight = 2, kGradRough = 3 };
 
+// LDS staging keeps lds_copies copies of the slots, lds_stride floats apart (an odd stride puts the
+// copies of a slot in different banks); lane l adds into copy l mod lds_copies, so the lanes of a wave
+// that hit the same material (neighbouring pixels usually do) spread over the copies instead of
+// serialising on one address.
+__device__ __forceinline__ float* lds_copy(const DevAdjoint& A, float* lds) {
+  return lds + (threadIdx.x & (A.lds_copies - 1u)) * A.lds_stride;
+}
+
 __device__ __forceinline__ void gadd(const DevAdjoint& A, float* lds, GradKind kind, uint32_t idx, int c, float v) {
   if (v == 0.0f) return;
   if (kind == kGradRough) {
     if (!A.g_rough) return;
-    if (A.use_lds) atomicAdd(lds + 6u * A.nmat + 3u * A.nlight + idx, v);
+    if (A.use_lds) atomicAdd(lds_copy(A, lds) + 6u * A.nmat + 3u * A.nlight + idx, v);
     else atomicAdd(A.g_rough + idx, v);
     return;
   }
   const uint32_t slot = 3u * idx + static_cast<uint32_t>(c);
   if (A.use_lds) {
     const uint32_t off = kind == kGradAlbedo ? 0u : (kind == kGradEmission ? 3u * A.nmat : 6u * A.nmat);
-    atomicAdd(lds + off + slot, v);
+    atomicAdd(lds_copy(A, lds) + off + slot, v);
   } else {
     float* g = kind == kGradAlbedo ? A.g_albedo : (kind == kGradEmission ? A.g_emission : A.g_light);
     atomicAdd(g + slot, v);
@@ -599,23 +607,42 @@ __device__ __forceinline__ void gadd(const DevAdjoint& A, float* lds, GradKind k
 
 __device__ __forceinline__ void grad_lds_clear(const DevAdjoint& A, float* lds) {
   if (!A.use_lds) return;
-  const uint32_t n = 7u * A.nmat + 3u * A.nlight;
-  for (uint32_t i = threadIdx.x; i < n; i += kBlock) lds[i] = 0.0f;
+  const uint32_t n = A.lds_copies * A.lds_stride;
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) lds[i] = 0.0f;
   __syncthreads();
 }
 
 __device__ __forceinline__ void grad_lds_flush(const DevAdjoint& A, const float* lds) {
   if (!A.use_lds) return;
   __syncthreads();
-  const uint32_t na = 3u * A.nmat, nl = 3u * A.nlight, n = 7u * A.nmat + 3u * A.nlight;
-  for (uint32_t i = threadIdx.x; i < n; i += kBlock) {
-    const float v = lds[i];
-    if (v == 0.0f) continue;
-    if (i < na) atomicAdd(A.g_albedo + i, v);
-    else if (i < 2u * na) atomicAdd(A.g_emission + (i - na), v);
-    else if (i < 2u * na + nl) atomicAdd(A.g_light + (i - 2u * na), v);
-    else if (A.g_rough) atomicAdd(A.g_rough + (i - 2u * na - nl), v);
+  const uint32_t n = 7u * A.nmat + 3u * A.nlight;
+  float* row = A.partial + static_cast<size_t>(blockIdx.x) * n;
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    float v = 0.0f;
+    for (uint32_t k = 0; k < A.lds_copies; ++k) v += lds[k * A.lds_stride + i];
+    if (v != 0.0f) row[i] += v;
   }
+}
+
+__global__ __launch_bounds__(kBlock) void k_grad_clear(DevAdjoint A, uint32_t count) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < count) A.partial[i] = 0.0f;
+}
+
+// Thread (slot i, group y) sums rows y, y + 16, y + 32, ... of slot i and adds the sum to the
+// caller's buffer (16 atomic adds per slot).
+constexpr uint32_t kReduceGroups = 16;
+__global__ __launch_bounds__(kBlock) void k_grad_reduce(DevAdjoint A) {
+  const uint32_t n = 7u * A.nmat + 3u * A.nlight, na = 3u * A.nmat, nl = 3u * A.nlight;
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  float v = 0.0f;
+  for (uint32_t r = blockIdx.y; r < A.partial_rows; r += kReduceGroups) v += A.partial[static_cast<size_t>(r) * n + i];
+  if (v == 0.0f) return;
+  if (i < na) atomicAdd(A.g_albedo + i, v);
+  else if (i < 2u * na) atomicAdd(A.g_emission + (i - na), v);
+  else if (i < 2u * na + nl) atomicAdd(A.g_light + (i - 2u * na), v);
+  else if (A.g_rough) atomicAdd(A.g_rough + (i - 2u * na - nl), v);
 }
 
 __device__ __forceinline__ Stack make_stack(const DevPaths& P, int32_t* s_ref, float* s_t) {
@@ -750,6 +777,18 @@ __device__ __forceinline__ void gadd_base(const DevAdjoint& A, const DevScene& s
   }
 }
 
+// History slot h of the adjoint state. The arrays of all slots are laid out `stride` entries apart,
+// so a per-lane slot index is address arithmetic, not a per-lane choice among kernel-argument pointers.
+__device__ __forceinline__ float* hist_dx(const DevAdjoint& A, uint32_t h, int c) {
+  return A.h_dx[0][0] + static_cast<size_t>(3u * h + static_cast<uint32_t>(c)) * A.stride;
+}
+__device__ __forceinline__ uint32_t* hist_mat(const DevAdjoint& A, uint32_t h) {
+  return A.h_mat[0] + static_cast<size_t>(h) * A.stride;
+}
+__device__ __forceinline__ float* hist_uv(const DevAdjoint& A, uint32_t h, int k) {
+  return A.h_uv[0][0] + static_cast<size_t>(2u * h + static_cast<uint32_t>(k)) * A.stride;
+}
+
 __device__ __forceinline__ V3 channel(int c, float v) {
   return mk(c == 0 ? v : 0.0f, c == 1 ? v : 0.0f, c == 2 ? v : 0.0f);
 }
@@ -770,14 +809,14 @@ __device__ __forceinline__ void adj_flush(const DevAdjoint& A, const DevScene& s
     const int cs = static_cast<int>((flags >> 2) & 3u);
     const uint32_t nh = (flags >> 4) & 7u;
     for (uint32_t h = 0; h < nh; ++h)
-      gadd_base(A, sc, lds, A.h_mat[h][p], make_float2(A.h_uv[h][0][p], A.h_uv[h][1][p]),
-                channel(cs, -S * A.h_dx[h][cs][p]));
+      gadd_base(A, sc, lds, hist_mat(A, h)[p], make_float2(hist_uv(A, h, 0)[p], hist_uv(A, h, 1)[p]),
+                channel(cs, -S * hist_dx(A, h, cs)[p]));
     if (flags & 0x80u) {
       const int pc = static_cast<int>((flags >> 8) & 3u);
       const uint32_t pn = (flags >> 10) & 7u;
       for (uint32_t h = 0; h < pn; ++h)
-        gadd_base(A, sc, lds, A.h_mat[h][p], make_float2(A.h_uv[h][0][p], A.h_uv[h][1][p]),
-                  channel(pc, S * A.h_dx[h][pc][p]));
+        gadd_base(A, sc, lds, hist_mat(A, h)[p], make_float2(hist_uv(A, h, 0)[p], hist_uv(A, h, 1)[p]),
+                  channel(pc, S * hist_dx(A, h, pc)[p]));
     }
     flags = (flags & ~0x1F80u) | 0x80u | (static_cast<uint32_t>(cs) << 8) | (nh << 10);
   }
@@ -1421,10 +1460,10 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
     aflags = (aflags & ~0xFu) | 1u | (active ? 2u : 0u) | (static_cast<uint32_t>(cstar) << 2);
     uint32_t nh = (aflags >> 4) & 7u;
     if (depth < st.rr_start && nh < static_cast<uint32_t>(kAdjHist)) {
-      A.h_mat[nh][p] = mat_index;
-      A.h_uv[nh][0][p] = huv.x;
-      A.h_uv[nh][1][p] = huv.y;
-      for (int c = 0; c < 3; ++c) A.h_dx[nh][c][p] = dx[c];
+      hist_mat(A, nh)[p] = mat_index;
+      hist_uv(A, nh, 0)[p] = huv.x;
+      hist_uv(A, nh, 1)[p] = huv.y;
+      for (int c = 0; c < 3; ++c) hist_dx(A, nh, c)[p] = dx[c];
       ++nh;
       aflags = (aflags & ~0x70u) | (nh << 4);
     }
@@ -1439,9 +1478,8 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
 }
 
 template <bool kAdj>
-__global__ __launch_bounds__(kBlock) void k_shade(DevPaths P, DevRays in, DevRays out, DevScene sc, DevSettings st,
-                                                 uint32_t depth, DevAdjoint A) {
-  __shared__ float glds[kAdj ? kAdjLdsFloats : 1];
+__device__ __forceinline__ void shade_kernel(const DevPaths& P, const DevRays& in, const DevRays& out, const DevScene& sc,
+                                             const DevSettings& st, uint32_t depth, const DevAdjoint& A, float* glds) {
   __shared__ uint32_t prefix[kSegments + 1];
   if constexpr (kAdj) grad_lds_clear(A, glds);
   load_prefix(P.queue + kQueueInCount, 1, prefix);
@@ -1497,10 +1535,30 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevPaths P, DevRays in, DevRay
   if constexpr (kAdj) grad_lds_flush(A, glds);
 }
 
+__global__ __launch_bounds__(kBlock) void k_shade(DevPaths P, DevRays in, DevRays out, DevScene sc, DevSettings st,
+                                                 uint32_t depth) {
+  shade_kernel<false>(P, in, out, sc, st, depth, DevAdjoint{}, nullptr);
+}
+
+// The adjoint shade holds about twice the forward's live values; letting it run PT_SHADE_ADJ_WAVES
+// waves per SIMD (the compiler spills what does not fit) trades spills for latency hiding.
+#ifndef PT_SHADE_ADJ_WAVES
+#define PT_SHADE_ADJ_WAVES 2
+#endif
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PT_SHADE_ADJ_WAVES, 8))) void k_shade_adj(
+    DevPaths P, DevRays in, DevRays out, DevScene sc, DevSettings st, uint32_t depth, DevAdjoint A) {
+  __shared__ float glds[kAdjLdsFloats];
+  shade_kernel<true>(P, in, out, sc, st, depth, A, glds);
+}
+
 // NEE visibility over the bounce's shadow queue. Opaque scenes: an any-hit query with
 // t_max = dist - 0.004. Scenes with transmissive materials: the reference's closest-hit loop
 // that steps through up to 8 transmissive blockers (ScalarCpuPathTracer.cpp:1066-1087), each
 // step a follow-up query on the same lane.
+// Set in a shadow record's light index (adjoint passes) when the light turned out visible; the
+// record's radiance and gradients are then applied by k_shadow_grad.
+constexpr uint32_t kShadowVisible = 0x80000000u;
+
 template <bool kAdj>
 __device__ __forceinline__ void light_visible(const DevPaths& P, const DevScene& sc, const DevAdjoint& A, float* glds,
                                               uint32_t q) {
@@ -1509,7 +1567,7 @@ __device__ __forceinline__ void light_visible(const DevPaths& P, const DevScene&
   P.rad[1][p] = P.rad[1][p] + P.scontrib[1][q];
   P.rad[2][p] = P.rad[2][p] + P.scontrib[2][q];
   if constexpr (kAdj) {
-    const uint32_t mi = A.s_mat[q], li = A.s_light[q];
+    const uint32_t mi = A.s_mat[q], li = A.s_light[q] & ~kShadowVisible;
     gadd_base(A, sc, glds, mi, make_float2(A.s_uv[0][q], A.s_uv[1][q]),
               mk(A.s_galb[0][q], A.s_galb[1][q], A.s_galb[2][q]));
     for (int c = 0; c < 3; ++c) gadd(A, glds, kGradLight, li, c, A.s_glight[c][q]);
@@ -1521,10 +1579,14 @@ template <bool kAdj>
 __global__ __launch_bounds__(kBlock) void k_shadow(DevPaths P, DevScene sc, DevAdjoint A) {
   __shared__ int32_t s_ref[kStackLds * kBlock];
   __shared__ float s_t[kStackLds * kBlock];
-  __shared__ float glds[kAdj ? kAdjLdsFloats : 1];
   __shared__ uint32_t prefix[kSegments + 1];
-  if constexpr (kAdj) grad_lds_clear(A, glds);
   load_prefix(P.queue + kQueueShadowCursor, kSegLine, prefix);
+  // forward: a visible light adds its contribution now; adjoint: the record is marked and
+  // k_shadow_grad applies it (keeping this traversal kernel as lean as the forward one)
+  auto visible = [&](uint32_t q) {
+    if constexpr (kAdj) A.s_light[q] |= kShadowVisible;
+    else light_visible<false>(P, sc, A, nullptr, q);
+  };
   const Stack stk = make_stack(P, s_ref, s_t);
   const uint32_t n = prefix[kSegments];
   TravStats st = {0, 0, 0, 0, 0};
@@ -1537,7 +1599,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevPaths P, DevScene sc, DevA
       return trav_begin(sc.recs[P.sscene[q]], s, o, d, P.stmax[q], st);
     };
     auto end = [&](uint32_t, Trav& s) {
-      if (s.best_prim < 0) light_visible<kAdj>(P, sc, A, glds, q);
+      if (s.best_prim < 0) visible(q);
       return false;
     };
     drive_queries<true>(sc, stk, n, P.refill_min, st, begin, end);
@@ -1574,19 +1636,32 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevPaths P, DevScene sc, DevA
         // the follow-up query ended at once (root box missed): nothing in the way
         break;
       }
-      light_visible<kAdj>(P, sc, A, glds, q);
+      visible(q);
       return false;
     };
     drive_queries<false>(sc, stk, n, P.refill_min, st, begin, end);
   }
   flush_stats(P.counters, st, true);
-  if constexpr (kAdj) grad_lds_flush(A, glds);
+}
+
+// Adjoint passes: the radiance and gradient pieces of every shadow record whose light was visible.
+constexpr uint32_t kGradBlocks = 1024;
+static_assert(kGradBlocks <= kGradRows, "every block of an adjoint kernel owns a partial row");
+__global__ __launch_bounds__(kBlock) void k_shadow_grad(DevPaths P, DevScene sc, DevAdjoint A) {
+  __shared__ float glds[kAdjLdsFloats];
+  __shared__ uint32_t prefix[kSegments + 1];
+  grad_lds_clear(A, glds);
+  load_prefix(P.queue + kQueueShadowCursor, kSegLine, prefix);
+  const uint32_t n = prefix[kSegments];
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const uint32_t q = queue_pos(prefix, i, P.seg_cap);
+    if (A.s_light[q] & kShadowVisible) light_visible<true>(P, sc, A, glds, q);
+  }
+  grad_lds_flush(A, glds);
 }
 
 // FilmBuffer::add_sample for every tile of a pass, after shade_pixel's non-finite drop
-// (ScalarCpuPathTracer.Render.cpp:270-283, FilmBuffer.cpp:80-91). One thread per pixel of each
-// scene's tile rectangle adds that scene's samples of the pass in tile order, as sequential
-// render_tile calls would.
+// (ScalarCpuPathTracer.Render.cpp:270-283, FilmBuffer.cpp:80-91).
 // Film accumulation, one wave (one 64-thread block) per 64 pixels of a job, each lane one pixel: the
 // pixel's samples are added in tile order (the reference's sample order), so the sums are those of
 // adding one sample per render_tile call. A job that is one interleaved group keeps a pixel's
@@ -1721,6 +1796,18 @@ void launch_raygen_adjoint(hipStream_t stream, const DevPaths& paths, const DevA
                      paths.rays[0], scene, st, tiles, tile_count, path_count, adj);
 }
 
+void launch_grad_clear(hipStream_t stream, const DevAdjoint& adj) {
+  if (!adj.use_lds) return;
+  const uint32_t count = adj.partial_rows * (7u * adj.nmat + 3u * adj.nlight);
+  hipLaunchKernelGGL(k_grad_clear, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, adj, count);
+}
+
+void launch_grad_reduce(hipStream_t stream, const DevAdjoint& adj) {
+  if (!adj.use_lds) return;
+  const uint32_t n = 7u * adj.nmat + 3u * adj.nlight;
+  hipLaunchKernelGGL(k_grad_reduce, dim3((n + kBlock - 1) / kBlock, kReduceGroups), dim3(kBlock), 0, stream, adj);
+}
+
 void launch_copy_radiance(hipStream_t stream, const DevPaths& paths, const DevAdjoint& adj, uint32_t path_count) {
   if (path_count == 0) return;
   hipLaunchKernelGGL(k_copy_radiance, dim3((path_count + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, paths, adj,
@@ -1733,13 +1820,14 @@ void launch_extend(hipStream_t stream, const DevPaths& paths, const DevScene& sc
 
 void launch_shade(hipStream_t stream, const DevPaths& paths, const DevScene& scene, const DevSettings& st, int in_buf,
                   uint32_t depth, uint32_t grid) {
-  hipLaunchKernelGGL(k_shade<false>, dim3(grid), dim3(kBlock), 0, stream, paths, paths.rays[in_buf],
-                     paths.rays[in_buf ^ 1], scene, st, depth, DevAdjoint{});
+  hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kBlock), 0, stream, paths, paths.rays[in_buf],
+                     paths.rays[in_buf ^ 1], scene, st, depth);
 }
 
 void launch_shade_adjoint(hipStream_t stream, const DevPaths& paths, const DevAdjoint& adj, const DevScene& scene,
                           const DevSettings& st, int in_buf, uint32_t depth, uint32_t grid) {
-  hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(kBlock), 0, stream, paths, paths.rays[in_buf],
+  grid = grid < kGradRows ? grid : kGradRows;  // one partial row per block
+  hipLaunchKernelGGL(k_shade_adj, dim3(grid), dim3(kBlock), 0, stream, paths, paths.rays[in_buf],
                      paths.rays[in_buf ^ 1], scene, st, depth, adj);
 }
 
@@ -1750,6 +1838,7 @@ void launch_shadow(hipStream_t stream, const DevPaths& paths, const DevScene& sc
 void launch_shadow_adjoint(hipStream_t stream, const DevPaths& paths, const DevAdjoint& adj, const DevScene& scene,
                            uint32_t grid) {
   hipLaunchKernelGGL(k_shadow<true>, dim3(grid), dim3(kBlock), 0, stream, paths, scene, adj);
+  hipLaunchKernelGGL(k_shadow_grad, dim3(kGradBlocks), dim3(kBlock), 0, stream, paths, scene, adj);
 }
 
 void launch_film(hipStream_t stream, const DevPaths& paths, const DevScene& scene, const DevFilm& film,
@@ -1786,8 +1875,8 @@ Grids resident_grids(int device) {
   const uint32_t q0 = occupancy_grid(device, reinterpret_cast<const void*>(&k_trace_rays<false>));
   const uint32_t q1 = occupancy_grid(device, reinterpret_cast<const void*>(&k_trace_rays<true>));
   g.query = q0 < q1 ? q0 : q1;
-  g.shade = occupancy_grid(device, reinterpret_cast<const void*>(&k_shade<false>));
-  g.shade_adj = occupancy_grid(device, reinterpret_cast<const void*>(&k_shade<true>));
+  g.shade = occupancy_grid(device, reinterpret_cast<const void*>(&k_shade));
+  g.shade_adj = occupancy_grid(device, reinterpret_cast<const void*>(&k_shade_adj));
   return g;
 }
 
