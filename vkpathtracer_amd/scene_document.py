@@ -1,0 +1,1016 @@
+"""Scene documents (the reference's JSON scene files) converted to path-tracer scenes.
+
+Restates, for everything the CPU path tracer consumes:
+
+* the document loader, SceneDocument::load_from_text (src/scene/SceneDocumentLoad.cpp:118-551) with the
+  JSON readers of src/scene/Json.cpp:397-540 and the validation gate SceneDocument::validate
+  (SceneDocumentLoad.cpp:596-990);
+* the material family presets, ApplyMaterialFamilyPreset (src/scene/Scene.cpp:22-214);
+* world transforms of the ECS world the conversion builds, SceneDocument::to_world
+  (src/scene/Scene.cpp:275-383) and SceneWorld::compute_world_transform_unchecked
+  (src/scene/SceneWorldExtraction.cpp:93-146) with the matrix helpers of src/scene/SceneInternal.h:156-240;
+* BuildSceneDataFromDocumentAtFrame (src/pathtracer/SceneConversion.cpp:594-1711): materials, texture
+  indices, camera, triangle meshes (world-space vertices plus local geometry for transformable
+  instances), mesh culling, point / spot / environment lights and camera auto-placement.
+
+All arithmetic is float32 in the reference's operation order, so vertex positions, transforms and light
+records come out bit-identical to the reference's snapshot for the same document.
+
+Not converted (each is reported by :func:`unsupported_features`): particle emitters (rain / smoke
+geometry), SDF primitives, tessellation requests, skinning and animation clips, model imports
+(glTF / OBJ assets expanded by load_from_file), HDR environment images and texture images. The
+reference's CPU path tracer does not sample texture images, so materials keep their texture indices
+and render untextured, as they do there.
+"""
+from __future__ import annotations
+
+import json
+import math
+import pathlib
+
+import numpy as np
+
+from . import scene as S
+
+F = np.float32
+U32_MAX = 0xFFFFFFFF
+
+# RTInstanceFlags (src/pathtracer/PathTracer.h:264-269)
+INSTANCE_FLAG_PHYSICS = 1 << 1
+INSTANCE_FLAG_TRANSFORM_DIRTY = 1 << 2
+
+
+class SceneDocumentError(ValueError):
+    """The document fails the loader's structural checks or its validation gate."""
+
+
+# ------------------------------------------------------------------ JSON readers (Json.cpp:397-540)
+def _is_num(v) -> bool:
+    return isinstance(v, (int, float)) and not isinstance(v, bool)
+
+
+def _finite(v) -> bool:
+    return _is_num(v) and math.isfinite(float(v))
+
+
+def _get(obj, key):
+    return obj.get(key) if isinstance(obj, dict) else None
+
+
+def _read_float(obj, key, cur):
+    """read_float: a finite JSON number replaces the value (cast to float32)."""
+    v = _get(obj, key)
+    return F(v) if _finite(v) else cur
+
+
+def _read_u64(obj, key, cur):
+    """read_u64: a number in [0, 2^64] replaces the value (truncated)."""
+    v = _get(obj, key)
+    if not _finite(v) or v < 0 or v > 2.0 ** 64:
+        return cur
+    return int(v) & 0xFFFFFFFFFFFFFFFF
+
+
+def _read_u32(obj, key, cur):
+    v = _read_u64(obj, key, None)
+    return cur if v is None else v & U32_MAX
+
+
+def _read_str(obj, key, cur):
+    v = _get(obj, key)
+    return v if isinstance(v, str) else cur
+
+
+def _read_bool(obj, key, cur):
+    v = _get(obj, key)
+    return v if isinstance(v, bool) else cur
+
+
+def _read_vec(obj, key, cur, n):
+    """read_vec3 / read_quat: exactly n finite numbers, else unchanged."""
+    v = _get(obj, key)
+    if isinstance(v, list) and len(v) == n and all(_finite(x) for x in v):
+        return [F(x) for x in v]
+    return cur
+
+
+def _read_vec_list(obj, key, n):
+    """read_vec3_list / read_vec2_list: the elements before the first malformed one."""
+    out = []
+    v = _get(obj, key)
+    if isinstance(v, list):
+        for e in v:
+            if not (isinstance(e, list) and len(e) == n and all(_finite(x) for x in e)):
+                break
+            out.append([float(x) for x in e])
+    return np.asarray(out, np.float64).astype(np.float32).reshape(-1, n)
+
+
+def _read_u32_list(obj, key):
+    """read_u32_list: the elements before the first one that is not a number in [0, 2^32 - 1]."""
+    out = []
+    v = _get(obj, key)
+    if isinstance(v, list):
+        for e in v:
+            if not _finite(e) or e < 0 or e > U32_MAX:
+                break
+            out.append(int(e))
+    return np.asarray(out, np.int64)
+
+
+def _allocate_id(used) -> int:
+    """allocate_id (src/scene/SceneInternal.h): the smallest positive id not yet used."""
+    c = 1
+    while c in used:
+        c += 1
+    return c
+
+
+def _read_transform(obj):
+    """read_transform (SceneInternal.h:156-162) over TransformComponent's defaults (SceneTypes.h:76-81)."""
+    return {"t": _read_vec(obj, "translation", [F(0), F(0), F(0)], 3),
+            "r": _read_vec(obj, "rotation", [F(0), F(0), F(0), F(1)], 4),
+            "s": _read_vec(obj, "scale", [F(1), F(1), F(1)], 3), "dirty": True}
+
+
+_CAMERA_DEFAULTS = {"fov": 60.0, "near_plane": 0.1, "far_plane": 1000.0, "focal_length_mm": 35.0,
+                    "sensor_width_mm": 36.0, "sensor_height_mm": 24.0, "aperture_radius": 0.0,
+                    "focus_distance": 0.0, "f_stop": 0.0, "shutter_seconds": 0.0166666675, "iso": 100.0,
+                    "exposure_compensation": 0.0, "white_balance_kelvin": 6500.0, "iris_rotation_degrees": 0.0,
+                    "iris_roundness": 1.0, "anamorphic_squeeze": 1.0}
+
+
+def _read_camera(obj):
+    """read_camera_component (SceneInternal.h:35-53) over CameraComponent's defaults (SceneTypes.h:92-110)."""
+    c = {k: _read_float(obj, k, F(v)) for k, v in _CAMERA_DEFAULTS.items()}
+    c["iris_blade_count"] = _read_u32(obj, "iris_blade_count", 0)
+    return c
+
+
+def _read_light(obj):
+    """LightComponent (SceneTypes.h:112-120) as SceneDocumentLoad.cpp:351-359 reads it."""
+    return {"type": _read_str(obj, "type", "point"), "color": _read_vec(obj, "color", [F(1), F(1), F(1)], 3),
+            "intensity": _read_float(obj, "intensity", F(1)), "radius": _read_float(obj, "radius", F(0)),
+            "direction": _read_vec(obj, "direction", [F(0), F(-1), F(0)], 3),
+            "beam_angle": _read_float(obj, "beam_angle", F(35)), "blend": _read_float(obj, "blend", F(0.35))}
+
+
+# ------------------------------------------------------------------ material families
+def _fold_id(text: str, spaces: str) -> str:
+    out = []
+    for ch in text:
+        if ("A" <= ch <= "Z") or ("a" <= ch <= "z") or ("0" <= ch <= "9"):
+            out.append(ch.lower())
+        elif ch in "_-" or ch in spaces:
+            if out and out[-1] != "_":
+                out.append("_")
+    while out and out[-1] == "_":
+        out.pop()
+    return "".join(out)
+
+
+def normalize_family_id(text: str) -> str:
+    """NormalizeMaterialFamilyId (Scene.cpp:22-39): lower-case alphanumerics, runs of '_', '-' and
+    white space folded to one '_', trailing '_' dropped; empty becomes "diffuse"."""
+    return _fold_id(text, " \t\n\v\f\r") or "diffuse"
+
+
+def _normalize_material_id(text: str) -> str:
+    """normalize_material_id (SceneConversion.cpp:264-281): as above with only ' ' as white space and
+    no default."""
+    return _fold_id(text, " ")
+
+
+_EMISSIVE = ("emissive", "environment_emissive", "blackbody_emission", "fire_plasma", "fire_sparkle_emission",
+             "light_emitting_textile", "bokeh_motion_blur_stress")
+
+# ApplyMaterialFamilyPreset's family tests in their order (Scene.cpp:67-201): the first row whose
+# families hold the material's family applies. Each entry is (field, value, generic); a dict value maps
+# a family to its own value with None as the row's default.
+_PRESETS = [
+    (("mirror",), [("roughness", 0.0, 1.0), ("metallic", 1.0, 0.0)]),
+    (("glossy", "specular", "normal_mapped_pbr", "plastic"), [("roughness", 0.18, 1.0), ("metallic", 0.0, 0.0)]),
+    (("metallic_pbr", "ggx_rough_conductor", "anisotropic_ggx", "brushed_metal", "ground_metal"),
+     [("roughness", {"ggx_rough_conductor": 0.35, None: 0.24}, 1.0), ("metallic", 1.0, 0.0),
+      ("anisotropy", {"brushed_metal": 0.65, None: 0.0}, 0.0)]),
+    (("dielectric_glass", "ggx_rough_dielectric", "spectral_glass_approx", "resin", "epoxy", "gemstone",
+      "ice_crystal", "frosted_acrylic", "translucent_polymer"),
+     [("roughness", 0.02, 1.0), ("metallic", 0.0, 0.0),
+      ("ior", {"gemstone": 1.75, "ice_crystal": 1.31, None: 1.5}, 1.5), ("transmission", 1.0, 0.0),
+      ("alpha", 0.35, 1.0)]),
+    (("frosted_glass", "dirty_glass"),
+     [("roughness", 0.48, 1.0), ("metallic", 0.0, 0.0), ("ior", 1.45, 1.5), ("transmission", 0.85, 0.0),
+      ("alpha", 0.5, 1.0)]),
+    (("clearcoat", "paint", "car_paint", "porcelain_ceramic", "energy_conserving_layered"),
+     [("roughness", 0.22, 1.0), ("metallic", {"car_paint": 0.25, None: 0.0}, 0.0), ("clearcoat", 1.0, 0.0)]),
+    (("velvet", "fabric_cloth", "hair_fur_lobes"),
+     [("roughness", 0.86, 1.0), ("metallic", 0.0, 0.0), ("sheen", {"velvet": 0.85, None: 0.62}, 0.0)]),
+    (("toon_surface", "stylized_diffuse", "xray"), [("roughness", 1.0, 1.0), ("metallic", 0.0, 0.0)]),
+    (("subsurface_approx", "skin", "wax", "marble_scattering"),
+     [("roughness", {"marble_scattering": 0.72, None: 0.62}, 1.0), ("metallic", 0.0, 0.0),
+      ("sheen", {"skin": 0.2, None: 0.12}, 0.0), ("alpha", 0.92, 1.0)]),
+    (("volumetric_shafts", "volumetric_medium", "smoke", "chromatic_dust"),
+     [("roughness", 1.0, 1.0), ("metallic", 0.0, 0.0), ("alpha", 0.45, 1.0), ("transmission", 0.12, 0.0)]),
+    (_EMISSIVE, [("roughness", 1.0, 1.0), ("metallic", 0.0, 0.0), ("emission", "from_albedo", None),
+                 ("emission_intensity", {"blackbody_emission": 5.0, None: 3.0}, 0.0)]),
+    (("thin_film_iridescent", "holographic_coating", "pearl_lustre", "diffraction_grating"),
+     [("roughness", 0.2, 1.0), ("metallic", 0.0, 0.0), ("clearcoat", 0.8, 0.0), ("sheen", 0.35, 0.0)]),
+    (("alpha_mask",), [("roughness", 0.8, 1.0), ("metallic", 0.0, 0.0), ("alpha", 0.5, 1.0)]),
+    (("wet_surface", "water_fluid_surface"),
+     [("roughness", 0.08, 1.0), ("metallic", 0.0, 0.0), ("clearcoat", 0.9, 0.0),
+      ("transmission", {"water_fluid_surface": 0.55, None: 0.0}, 0.0)]),
+    (("retroreflector", "caustics_inspired_response"),
+     [("roughness", 0.12, 1.0), ("metallic", 0.0, 0.0), ("clearcoat", 0.75, 0.0)]),
+    (("plastic", "rubber"),
+     [("roughness", {"rubber": 0.55, None: 0.38}, 1.0), ("metallic", 0.0, 0.0),
+      ("ior", {"rubber": 1.35, None: 1.5}, 1.5)]),
+    (("wood", "oak", "walnut", "parquet", "sandalwood", "pine", "teak", "mahogany", "cedar"),
+     [("roughness", {"parquet": 0.28, "teak": 0.28, "walnut": 0.62, "mahogany": 0.62, "cedar": 0.62, None: 0.46}, 1.0),
+      ("metallic", 0.0, 0.0),
+      ("clearcoat", {"parquet": 0.55, "teak": 0.55, "walnut": 0.08, "mahogany": 0.08, "cedar": 0.08, None: 0.24}, 0.0),
+      ("anisotropy", {"parquet": 0.28, "teak": 0.28, None: 0.18}, 0.0)]),
+    (("stone", "concrete", "plaster", "sand", "mud", "terra_earth", "charcoal", "cardboard", "paper"),
+     [("roughness", 0.82, 1.0), ("metallic", 0.0, 0.0)]),
+]
+_PRESET_DEFAULT = [("roughness", 0.85, 1.0), ("metallic", 0.0, 0.0)]
+
+
+def apply_family_preset(m: dict, force: bool) -> None:
+    """ApplyMaterialFamilyPreset (Scene.cpp:41-214). force=True is the Override policy (the loader,
+    before explicit fields are read); force=False is FillGenericDefaults (the conversion), which only
+    replaces a field that still holds the generic default (within 1e-5) or is not finite."""
+    family = normalize_family_id(m["name"] if m["family"] == "" else m["family"])
+    m["family"] = family
+
+    def set_if_generic(key, value, generic):
+        cur = m[key]
+        if force or not math.isfinite(float(cur)) or abs(F(cur) - F(generic)) <= F(1.0e-5):
+            m[key] = F(value)
+
+    set_if_generic("alpha", 1.0, 1.0)
+    for key in ("transmission", "clearcoat", "sheen", "anisotropy"):
+        set_if_generic(key, 0.0, 0.0)
+    rows = next((r for fams, r in _PRESETS if family in fams), _PRESET_DEFAULT)
+    for key, value, generic in rows:
+        if value == "from_albedo":
+            e = m["emission"]
+            if force or (e[0] <= 0 and e[1] <= 0 and e[2] <= 0):
+                m["emission"] = [max(F(0.2), a) for a in m["albedo"]]
+            continue
+        if isinstance(value, dict):
+            value = value.get(family, value[None])
+        set_if_generic(key, value, generic)
+    if force and family not in _EMISSIVE:
+        m["emission"] = [F(0), F(0), F(0)]
+        m["emission_intensity"] = F(0)
+
+
+_MODELS = [
+    (1, _EMISSIVE), (2, ("mirror",)), (3, ("specular", "glossy", "normal_mapped_pbr", "plastic", "rubber")),
+    (4, ("ggx_rough_conductor", "metallic_pbr", "anisotropic_ggx", "brushed_metal", "ground_metal")),
+    (5, ("ggx_rough_dielectric", "dielectric_glass", "spectral_glass_approx", "frosted_glass", "dirty_glass",
+         "water_fluid_surface", "ice_crystal", "resin", "epoxy", "gemstone", "frosted_acrylic", "translucent_polymer")),
+    (6, ("velvet", "fabric_cloth", "hair_fur_lobes", "pearl_lustre")),
+    (7, ("clearcoat", "car_paint", "paint", "porcelain_ceramic", "wet_surface", "energy_conserving_layered",
+         "thin_film_iridescent", "diffraction_grating", "holographic_coating", "retroreflector",
+         "caustics_inspired_response")),
+    (8, ("toon_surface", "stylized_diffuse", "xray")),
+    (9, ("volumetric_medium", "volumetric_shafts", "smoke", "chromatic_dust")),
+]
+_EFFECTS = [
+    (1, ("velvet", "fabric_cloth", "hair_fur_lobes")), (2, ("procedural_material", "sdf_fractal_material")),
+    (3, ("voronoi_cracks", "charcoal", "cardboard")), (4, ("marble_scattering", "porcelain_ceramic", "ice_crystal")),
+    (5, ("corrosion_oxidation", "rust_progression", "terra_earth", "mud")),
+    (6, ("chromatic_dust", "sand", "stone", "concrete", "plaster")), (7, ("skin", "wax", "subsurface_approx", "paper")),
+    (8, ("thin_film_iridescent", "diffraction_grating", "holographic_coating", "pearl_lustre", "gemstone",
+         "spectral_glass_approx")),
+    (9, ("alpha_mask",)), (10, ("blackbody_emission", "fire_plasma", "fire_sparkle_emission")),
+    (11, ("wet_surface", "water_fluid_surface", "dirty_glass")),
+    (12, ("retroreflector", "caustics_inspired_response", "bokeh_motion_blur_stress")),
+    (13, ("normal_mapped_pbr",)), (14, ("xray",)), (15, ("volumetric_medium", "volumetric_shafts", "smoke")),
+    (17, ("parquet", "oak")), (18, ("walnut", "mahogany")), (19, ("sandalwood", "pine", "teak", "cedar")),
+    (16, ("wood",)),
+]
+
+
+def material_model_from_family(family: str) -> int:
+    """material_model_from_family (SceneConversion.cpp:517-560): the broad BSDF class id."""
+    i = _normalize_material_id(family)
+    return next((m for m, names in _MODELS if i in names), 0)
+
+
+def material_effect_from_family(family: str) -> int:
+    """material_effect_from_family (SceneConversion.cpp:562-586)."""
+    i = _normalize_material_id(family)
+    return next((e for e, names in _EFFECTS if i in names), 0)
+
+
+# ------------------------------------------------------------------ float32 transform helpers
+def _sqrtf(x):
+    return F(np.sqrt(F(x)))
+
+
+def _normalize_quat(q):
+    """normalize_quat (SceneInternal.h:173-184)."""
+    x, y, z, w = q
+    ls = x * x + y * y + z * z + w * w
+    if not (ls > F(0)) or not math.isfinite(float(ls)):
+        return [F(0), F(0), F(0), F(1)]
+    inv = F(1) / _sqrtf(ls)
+    return [x * inv, y * inv, z * inv, w * inv]
+
+
+def _multiply_quat(a, b):
+    """multiply_quat (SceneInternal.h:186-192), normalised."""
+    ax, ay, az, aw = a
+    bx, by, bz, bw = b
+    return _normalize_quat([aw * bx + ax * bw + ay * bz - az * by,
+                            aw * by - ax * bz + ay * bw + az * bx,
+                            aw * bz + ax * by - ay * bx + az * bw,
+                            aw * bw - ax * bx - ay * by - az * bz])
+
+
+def _identity():
+    m = [F(0)] * 16
+    m[0] = m[5] = m[10] = m[15] = F(1)
+    return m
+
+
+def _matrix(tr):
+    """make_transform_matrix (SceneInternal.h:208-237): column-major, rotation * scale, translation."""
+    x, y, z, w = _normalize_quat(tr["r"])
+    sx, sy, sz = tr["s"]
+    xx, yy, zz, xy, xz, yz, wx, wy, wz = x * x, y * y, z * z, x * y, x * z, y * z, w * x, w * y, w * z
+    one, two = F(1), F(2)
+    m = _identity()
+    m[0] = (one - two * (yy + zz)) * sx
+    m[1] = (two * (xy + wz)) * sx
+    m[2] = (two * (xz - wy)) * sx
+    m[4] = (two * (xy - wz)) * sy
+    m[5] = (one - two * (xx + zz)) * sy
+    m[6] = (two * (yz + wx)) * sy
+    m[8] = (two * (xz + wy)) * sz
+    m[9] = (two * (yz - wx)) * sz
+    m[10] = (one - two * (xx + yy)) * sz
+    m[12], m[13], m[14] = tr["t"]
+    return m
+
+
+def _mat_mul(a, b):
+    """multiply_matrix (SceneInternal.h:194-206)."""
+    out = [F(0)] * 16
+    for col in range(4):
+        for row in range(4):
+            v = F(0)
+            for k in range(4):
+                v = v + a[k * 4 + row] * b[col * 4 + k]
+            out[col * 4 + row] = v
+    return out
+
+
+def rotate_quat(v, q) -> np.ndarray:
+    """rotate_quat (SceneConversion.cpp:157-178) over float32 arrays of shape (..., 3)."""
+    x, y, z, w = (F(c) for c in q)
+    ln = _sqrtf(x * x + y * y + z * z + w * w)
+    if ln > F(1.0e-6):
+        inv = F(1) / ln
+        x, y, z, w = x * inv, y * inv, z * inv, w * inv
+    else:
+        x, y, z, w = F(0), F(0), F(0), F(1)
+    v = np.asarray(v, np.float32)
+    vx, vy, vz = v[..., 0], v[..., 1], v[..., 2]
+    two = F(2)
+    tx, ty, tz = (y * vz - z * vy) * two, (z * vx - x * vz) * two, (x * vy - y * vx) * two
+    return np.stack([(vx + tx * w) + (y * tz - z * ty),
+                     (vy + ty * w) + (z * tx - x * tz),
+                     (vz + tz * w) + (x * ty - y * tx)], -1).astype(np.float32)
+
+
+def transform_points(p, translation, rotation, scale) -> np.ndarray:
+    """transform_point (SceneConversion.cpp:180-187): scale, rotate, translate."""
+    p = np.asarray(p, np.float32)
+    r = rotate_quat(p * np.asarray(scale, np.float32), rotation)
+    return (r + np.asarray(translation, np.float32)).astype(np.float32)
+
+
+def _normalize3(v):
+    """normalize (SceneConversion.cpp:139-145): (0, 1, 0) for a length at or below 1e-4."""
+    v = np.asarray(v, np.float32)
+    ln = _sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2])
+    if ln <= F(1e-4):
+        return np.array([0, 1, 0], np.float32)
+    return (v / ln).astype(np.float32)
+
+
+def _dot(a, b):
+    """dot (SceneConversion.cpp:123-125), summed left to right."""
+    return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]
+
+
+# ------------------------------------------------------------------ the document
+_SECTIONS = {"schema": str, "metadata": dict, "assets": list, "materials": list, "geometry": list,
+             "sdf_primitives": list, "particle_emitters": list, "entities": list, "transforms": list,
+             "cameras": list, "lights": list, "scene_script": dict, "performance_culling": dict, "benchmark": dict}
+
+
+def read_document(root: dict) -> dict:
+    """SceneDocument::load_from_text (SceneDocumentLoad.cpp:118-551) for the sections the conversion reads,
+    then the validation gate. Raises SceneDocumentError where the reference's loader returns an error."""
+    if not isinstance(root, dict):
+        raise SceneDocumentError("scene document root is not a JSON object")
+    for key, kind in _SECTIONS.items():
+        if key in root and not isinstance(root[key], kind):
+            raise SceneDocumentError(f"section '{key}' has the wrong JSON kind")
+    d = {"schema": _read_str(root, "schema", ""), "assets": [], "materials": [], "geometry": [], "entities": [],
+         "cameras": [], "lights": [], "transforms": []}
+    if "metadata" in root:
+        d["schema"] = _read_str(root["metadata"], "schema", d["schema"])
+    used = set()
+    for item in root.get("assets", []):
+        a = {"id": _read_u64(item, "id", 0), "type": _read_str(item, "type", ""), "uri": _read_str(item, "uri", ""),
+             "parent": _read_u64(item, "parent", 0)}
+        if a["id"] == 0:
+            a["id"] = _allocate_id(used)
+        used.add(a["id"])
+        d["assets"].append(a)
+    used = set()
+    for item in root.get("materials", []):
+        m = {"id": _read_u64(item, "id", 0), "name": _read_str(item, "name", ""),
+             "family": _read_str(item, "family", "diffuse"), "albedo": [F(1), F(1), F(1)], "roughness": F(1),
+             "metallic": F(0), "ior": F(1.5), "transmission": F(0), "clearcoat": F(0), "sheen": F(0),
+             "anisotropy": F(0), "alpha": F(1), "double_sided": False, "base_color_texture": "",
+             "normal_texture": "", "emission": [F(0), F(0), F(0)], "emission_intensity": F(0)}
+        if m["id"] == 0:
+            m["id"] = _allocate_id(used)
+        used.add(m["id"])
+        apply_family_preset(m, force=True)
+        m["albedo"] = _read_vec(item, "albedo", m["albedo"], 3)
+        m["emission"] = _read_vec(item, "emission", m["emission"], 3)
+        m["emission_intensity"] = _read_float(item, "emission_intensity", m["emission_intensity"])
+        for k in ("roughness", "metallic", "ior", "transmission", "clearcoat", "sheen", "anisotropy", "alpha"):
+            m[k] = _read_float(item, k, m[k])
+        m["double_sided"] = _read_bool(item, "double_sided", m["double_sided"])
+        m["base_color_texture"] = _read_str(item, "base_color_texture", "")
+        m["normal_texture"] = _read_str(item, "normal_texture", "")
+        d["materials"].append(m)
+    for item in root.get("geometry", []):
+        d["geometry"].append({
+            "id": _read_u64(item, "id", 0), "primitive": _read_str(item, "primitive", ""),
+            "material_id": _read_u64(item, "material_id", 0), "vertices": _read_vec_list(item, "vertices", 3),
+            "texcoords": _read_vec_list(item, "texcoords", 2), "indices": _read_u32_list(item, "indices")})
+    used = set()
+    for item in root.get("entities", []):
+        e = {"id": _read_u64(item, "id", 0), "visible": True, "transform": None, "camera": None, "light": None,
+             "mesh": None, "parent": 0, "has_hierarchy": False, "physics": None, "scripted": False}
+        if e["id"] == 0:
+            e["id"] = _allocate_id(used)
+        used.add(e["id"])
+        e["visible"] = _read_bool(item, "visible", True)
+        if "transform" in item:
+            e["transform"] = _read_transform(item["transform"])
+        if "camera" in item:
+            e["camera"] = _read_camera(item["camera"])
+        if "light" in item:
+            e["light"] = _read_light(item["light"])
+        if "mesh" in item:
+            e["mesh"] = (_read_u64(item["mesh"], "mesh_id", 0), _read_u64(item["mesh"], "material_id", 0))
+        if "hierarchy" in item:
+            e["has_hierarchy"] = True
+            e["parent"] = _read_u64(item["hierarchy"], "parent", 0)
+        if "physics" in item:
+            p = item["physics"]
+            body = {"enabled": _read_bool(p, "enabled", True), "mass": _read_float(p, "mass", F(1)),
+                    "dynamic": _read_bool(p, "dynamic", False), "type": _read_str(p, "body_type", "static"),
+                    "shape": _read_str(p, "shape", "box"), "friction": _read_float(p, "friction", F(0.5)),
+                    "restitution": _read_float(p, "restitution", F(0))}
+            if body["type"] == "dynamic":
+                body["dynamic"] = True
+            elif body["type"] in ("static", "kinematic"):
+                body["dynamic"] = False
+            e["physics"] = body
+        script = _get(item, "script")
+        if isinstance(script, dict):  # ReadScriptComponent (SceneDocumentLoad.cpp:55-75)
+            src = _read_str(script, "source", "") or _read_str(script, "path", "")
+            e["scripted"] = _read_bool(script, "enabled", True) and src != ""
+        d["entities"].append(e)
+    for item in root.get("transforms", []):
+        d["transforms"].append({"id": _read_u64(item, "id", 0), "parent": _read_u64(item, "parent", 0),
+                                "transform": _read_transform(_get(item, "transform") or {})})
+    for item in root.get("cameras", []):
+        d["cameras"].append({"id": _read_u64(item, "id", 0), "camera": _read_camera(item)})
+    for item in root.get("lights", []):
+        d["lights"].append({"id": _read_u64(item, "id", 0), "light": _read_light(item)})
+    c = root.get("performance_culling", {})
+    d["culling"] = {"enabled": _read_bool(c, "enabled", False), "frustum": _read_bool(c, "frustum", True),
+                    "distance": _read_bool(c, "distance", True), "cull_dynamic": _read_bool(c, "cull_dynamic", True),
+                    "max_distance": _read_float(c, "max_distance", F(0)),
+                    "frustum_padding": _read_float(c, "frustum_padding", F(1.15)),
+                    "aspect_ratio": _read_float(c, "aspect_ratio", F(16.0) / F(9.0)),
+                    "min_instance_radius": _read_float(c, "min_instance_radius", F(0.05))}
+    issues = validate(d)
+    if issues:
+        raise SceneDocumentError("scene document failed validation: " + "; ".join(issues))
+    return d
+
+
+def _valid_transform(tr) -> bool:
+    """valid_transform_values (SceneInternal.h:361-364)."""
+    return all(math.isfinite(float(v)) for v in tr["t"] + tr["r"] + tr["s"]) and all(s != 0 for s in tr["s"])
+
+
+def _valid_camera(c) -> bool:
+    """valid_camera_values (SceneInternal.h:88-111)."""
+    return (c["fov"] > 0 and c["near_plane"] > 0 and c["far_plane"] > c["near_plane"] and c["focal_length_mm"] > 0
+            and c["sensor_width_mm"] > 0 and c["sensor_height_mm"] > 0 and c["aperture_radius"] >= 0
+            and c["focus_distance"] >= 0 and c["f_stop"] >= 0 and c["shutter_seconds"] > 0 and c["iso"] > 0
+            and 1000 <= c["white_balance_kelvin"] <= 40000 and c["iris_blade_count"] <= 64
+            and 0 <= c["iris_roundness"] <= 1 and c["anamorphic_squeeze"] > 0)
+
+
+def _valid_light(l) -> bool:
+    return l["intensity"] >= 0 and l["radius"] >= 0 and l["beam_angle"] > 0 and l["blend"] >= 0
+
+
+def validate(d: dict) -> list:
+    """SceneDocument::validate (SceneDocumentLoad.cpp:596-990) for the sections this module reads; an
+    empty list means the document passes. Checks of audio, UI panels, particle emitters, SDF primitives
+    and script parameters are not restated."""
+    issues = []
+    if d["schema"] != "1.0":
+        issues.append("metadata schema is empty" if d["schema"] == "" else f"unsupported schema {d['schema']}")
+    seen = set()
+    for a in d["assets"]:
+        if a["id"] in seen:
+            issues.append(f"duplicate asset id {a['id']}")
+        seen.add(a["id"])
+        if not a["type"] or not a["uri"]:
+            issues.append(f"asset type or uri is empty for {a['id']}")
+    mat_ids = set()
+    for m in d["materials"]:
+        if m["id"] in mat_ids:
+            issues.append(f"duplicate material id {m['id']}")
+        mat_ids.add(m["id"])
+        for k, lo, hi in (("roughness", 0, 1), ("metallic", 0, 1), ("transmission", 0, 1), ("clearcoat", 0, 1),
+                          ("sheen", 0, 1), ("anisotropy", -1, 1), ("alpha", 0, 1)):
+            if not lo <= m[k] <= hi:
+                issues.append(f"material {k} out of range {m['id']}")
+        if m["ior"] <= 0 or m["emission_intensity"] < 0:
+            issues.append(f"material ior or emission intensity out of range {m['id']}")
+    geo_ids = set()
+    for g in d["geometry"]:
+        if g["id"] == 0:
+            issues.append("geometry id is zero")
+        if g["id"] in geo_ids:
+            issues.append(f"duplicate geometry id {g['id']}")
+        geo_ids.add(g["id"])
+        if not g["primitive"]:
+            issues.append(f"geometry primitive is empty {g['id']}")
+        if g["material_id"] != 0 and mat_ids and g["material_id"] not in mat_ids:
+            issues.append(f"geometry references missing material {g['material_id']}")
+        if g["primitive"] == "triangle":
+            if len(g["vertices"]) == 0:
+                issues.append(f"triangle geometry has no vertices {g['id']}")
+            if len(g["indices"]) == 0 or len(g["indices"]) % 3:
+                issues.append(f"triangle geometry indices are not triangles {g['id']}")
+            if len(g["indices"]) and g["indices"].max() >= len(g["vertices"]):
+                issues.append(f"geometry index out of range {g['id']}")
+    ent_ids = set()
+    parent_of = {}
+    for e in d["entities"]:
+        if e["id"] in ent_ids:
+            issues.append(f"duplicate entity id {e['id']}")
+        ent_ids.add(e["id"])
+        if e["parent"] != 0:
+            if e["parent"] == e["id"]:
+                issues.append(f"entity has self parent {e['id']}")
+            parent_of[e["id"]] = e["parent"]
+        if e["transform"] is not None and not _valid_transform(e["transform"]):
+            issues.append(f"entity has invalid transform {e['id']}")
+        if e["mesh"] is not None:
+            if geo_ids and e["mesh"][0] not in geo_ids:
+                issues.append(f"entity references missing geometry {e['mesh'][0]}")
+            if e["mesh"][1] != 0 and mat_ids and e["mesh"][1] not in mat_ids:
+                issues.append(f"entity references missing material {e['mesh'][1]}")
+        p = e["physics"]
+        if p is not None and (not p["shape"] or p["mass"] <= 0 or p["friction"] < 0 or p["restitution"] < 0):
+            issues.append(f"entity physics values are invalid {e['id']}")
+        if e["camera"] is not None and not _valid_camera(e["camera"]):
+            issues.append(f"entity camera has invalid clip/fov {e['id']}")
+        if e["light"] is not None and not _valid_light(e["light"]):
+            issues.append(f"entity light has invalid values {e['id']}")
+    for t in d["transforms"]:
+        if t["id"] not in ent_ids:
+            issues.append(f"transform references missing entity {t['id']}")
+        if t["parent"] != 0:
+            if t["parent"] not in ent_ids:
+                issues.append(f"transform references missing parent {t['parent']}")
+            parent_of[t["id"]] = t["parent"]
+        if not _valid_transform(t["transform"]):
+            issues.append(f"transform has invalid values {t['id']}")
+    for l in d["lights"]:
+        if l["id"] not in ent_ids:
+            issues.append(f"light references missing entity {l['id']}")
+        if not _valid_light(l["light"]):
+            issues.append(f"light has invalid values {l['id']}")
+    for c in d["cameras"]:
+        if c["id"] not in ent_ids:
+            issues.append(f"camera references missing entity {c['id']}")
+        if not _valid_camera(c["camera"]):
+            issues.append(f"camera has invalid clip/fov {c['id']}")
+    for child, parent in parent_of.items():
+        if parent not in ent_ids:
+            issues.append(f"hierarchy references missing parent {parent}")
+        visited, cur = set(), child
+        while cur in parent_of:
+            if cur in visited:
+                issues.append(f"hierarchy cycle includes entity {child}")
+                break
+            visited.add(cur)
+            cur = parent_of[cur]
+    for a in d["assets"]:
+        if a["parent"] != 0 and a["parent"] not in ent_ids:
+            issues.append(f"asset import references missing parent {a['parent']}")
+    c = d["culling"]
+    if c["max_distance"] < 0 or c["frustum_padding"] <= 0 or c["aspect_ratio"] <= 0 or c["min_instance_radius"] < 0:
+        issues.append("performance_culling settings are invalid")
+    return issues
+
+
+def world_transforms(d: dict) -> dict:
+    """World transforms of the ECS world built by SceneDocument::to_world (Scene.cpp:275-383) after
+    recompute_world_transforms (SceneWorldExtraction.cpp:133-146): {entity id: {"t", "r", "s", "m"}} for
+    every entity that holds a transform (its own, or one from the document's "transforms" array, the last
+    entry winning). An entity's world transform composes its parent's; a parent without a transform
+    contributes the identity and ends the chain (compute_world_transform_unchecked,
+    SceneWorldExtraction.cpp:93-130)."""
+    local = {e["id"]: e["transform"] for e in d["entities"] if e["transform"] is not None}
+    parent = {e["id"]: e["parent"] for e in d["entities"] if e["has_hierarchy"]}
+    for t in d["transforms"]:
+        if t["id"] != 0:
+            if t["parent"] != 0:
+                parent[t["id"]] = t["parent"]
+            local[t["id"]] = t["transform"]
+    cache = {}
+
+    def world(i):
+        if i in cache:
+            return cache[i]
+        tr = local.get(i)
+        if tr is None:
+            return {"t": [F(0)] * 3, "r": [F(0), F(0), F(0), F(1)], "s": [F(1)] * 3, "m": _identity()}
+        out = {"t": list(tr["t"]), "r": _normalize_quat(tr["r"]), "s": list(tr["s"]), "m": _matrix(tr)}
+        p = parent.get(i, 0)
+        if p != 0:
+            pw = world(p)
+            out["m"] = _mat_mul(pw["m"], out["m"])
+            out["t"] = [out["m"][12], out["m"][13], out["m"][14]]
+            out["r"] = _multiply_quat(pw["r"], out["r"])
+            out["s"] = [pw["s"][k] * out["s"][k] for k in range(3)]
+        cache[i] = out
+        return out
+
+    return {i: world(i) for i in local}
+
+
+_ENV_TYPES = ("environment", "environment_sky", "environment_hdri", "environmenthdri", "hdri", "hdri_sky",
+              "open_sky", "sky")
+_TEXTURE_EXT = (".png", ".jpg", ".jpeg", ".tga", ".exr", ".hdr")
+
+
+def _is_texture_uri(uri: str) -> bool:
+    """is_texture_asset_uri (SceneConversion.cpp:253-262): the extension after the last '.'."""
+    dot = uri.rfind(".")
+    return dot >= 0 and uri[dot:].lower() in _TEXTURE_EXT
+
+
+def unsupported_features(root: dict) -> list:
+    """What :func:`document_to_scene` leaves out of this document (the module docstring says why)."""
+    out = []
+    if root.get("particle_emitters"):
+        out.append("particle emitters")
+    if root.get("sdf_primitives") or any("sdf_primitive" in e for e in root.get("entities", [])):
+        out.append("SDF primitives")
+    if any(isinstance(g.get("tessellation"), dict) and g["tessellation"].get("enabled") for g in root.get("geometry", [])):
+        out.append("tessellation requests")
+    kinds = [_normalize_material_id(a.get("type", "")) for a in root.get("assets", []) if isinstance(a, dict)]
+    if any(k.startswith("model") for k in kinds):
+        out.append("model imports")
+    if any(k in _ENV_TYPES for k in kinds):
+        out.append("environment images")
+    if any(k in ("texture", "image") for k in kinds) or any(
+            m.get("base_color_texture") or m.get("normal_texture") for m in root.get("materials", [])):
+        out.append("texture images")
+    return out
+
+
+# ------------------------------------------------------------------ conversion
+def document_to_scene(root: dict, name: str = "document") -> S.Scene:
+    """BuildSceneDataFromDocumentAtFrame (SceneConversion.cpp:594-1711) for materials, camera, triangle
+    meshes and lights, as a :class:`vkpathtracer_amd.scene.Scene`."""
+    d = read_document(root)
+
+    # texture indices: assets, then every material's textures, in order (SceneConversion.cpp:625-672)
+    tex = {}
+
+    def add_texture(uri):
+        if not uri or not _is_texture_uri(uri):
+            return U32_MAX
+        uri = uri.replace("\\", "/")
+        return tex.setdefault(uri, len(tex))
+
+    for a in d["assets"]:
+        kind = _normalize_material_id(a["type"])
+        if a["uri"] and (kind in ("texture", "image") or (kind not in _ENV_TYPES and _is_texture_uri(a["uri"]))):
+            add_texture(a["uri"])
+    for m in d["materials"]:
+        add_texture(m["base_color_texture"])
+        add_texture(m["normal_texture"])
+
+    # materials (SceneConversion.cpp:674-711)
+    mats = d["materials"] or [{"id": 0, "name": "", "family": "diffuse", "albedo": [F(1)] * 3, "roughness": F(1),
+                               "metallic": F(0), "ior": F(1.5), "transmission": F(0), "clearcoat": F(0),
+                               "sheen": F(0), "anisotropy": F(0), "alpha": F(1), "double_sided": False,
+                               "base_color_texture": "", "normal_texture": "", "emission": [F(0)] * 3,
+                               "emission_intensity": F(0)}]
+
+    def clamp01(v):
+        return F(0) if not math.isfinite(float(v)) else min(F(1), max(F(0), F(v)))
+
+    materials = S.make_materials(len(mats))
+    lookup = {}
+    for i, src in enumerate(mats):
+        m = dict(src)
+        apply_family_preset(m, force=False)
+        lookup[m["id"]] = i
+        r = materials[i]
+        r["albedo"] = m["albedo"]
+        r["emissive"] = [m["emission"][k] * m["emission_intensity"] for k in range(3)]
+        r["roughness"] = clamp01(m["roughness"])
+        r["metallic"] = clamp01(m["metallic"])
+        r["ior"] = max(F(1.01), m["ior"])
+        r["transmission"] = clamp01(m["transmission"])
+        r["clearcoat"] = clamp01(m["clearcoat"])
+        r["sheen"] = clamp01(m["sheen"])
+        r["anisotropy"] = min(F(1), max(F(-1), m["anisotropy"]))
+        r["alpha"] = clamp01(m["alpha"])
+        fam = m["name"] if m["family"] == "" else m["family"]
+        r["material_model"] = material_model_from_family(fam)
+        r["material_effect"] = material_effect_from_family(fam)
+        r["material_flags"] = 1 if m["double_sided"] else 0
+        r["base_color_texture_index"] = add_texture(m["base_color_texture"])
+        r["normal_texture_index"] = add_texture(m["normal_texture"])
+
+    worlds = world_transforms(d)
+    by_id = {e["id"]: e for e in d["entities"]}
+    identity = {"t": [F(0)] * 3, "r": [F(0), F(0), F(0), F(1)], "s": [F(1)] * 3}
+
+    def chain(e):
+        """The entity and its ancestors, stopping at a missing parent or a repeat."""
+        seen, cur = set(), e
+        while cur is not None and cur["id"] not in seen and len(seen) <= len(d["entities"]):
+            seen.add(cur["id"])
+            yield cur
+            if cur["parent"] == 0:
+                break
+            cur = by_id.get(cur["parent"])
+
+    def visible_path(e):  # entity_visible_path (SceneConversion.cpp:724-751)
+        return all(c["visible"] for c in chain(e))
+
+    def scripted_path(e):  # entity_has_scripted_transform_path (SceneConversion.cpp:755-781)
+        return any(c["scripted"] for c in chain(e))
+
+    def resolve(eid):
+        """resolve_entity_transform (SceneConversion.cpp:783-808): (transform, has_transform)."""
+        if eid in worlds:
+            return worlds[eid], True
+        e = by_id.get(eid)
+        if e is not None and e["transform"] is not None:
+            return e["transform"], True
+        return identity, False
+
+    # camera (SceneConversion.cpp:810-862)
+    cam = {"position": np.array([0, 1, 4], np.float32), "target": np.array([0, 1, 0], np.float32),
+           "up": np.array([0, 1, 0], np.float32), "fov": F(55), "near": F(0.01), "c": None}
+    has_cam = has_cam_tr = False
+
+    def apply_camera(cid, c):
+        tr, has_tr = resolve(cid)
+        cam["fov"], cam["c"] = c["fov"], c
+        cam["near"] = max(F(0.0001), c["near_plane"])
+        if has_tr:
+            cam["position"] = np.asarray(tr["t"], np.float32)
+            fwd = _normalize3(rotate_quat([0, 0, -1], tr["r"]))
+            up = _normalize3(rotate_quat([0, 1, 0], tr["r"]))
+            cam["target"] = (cam["position"] + fwd).astype(np.float32)
+            if _dot(up, up) > F(1e-4) * F(1e-4):
+                cam["up"] = up
+        return has_tr
+
+    for e in d["entities"]:
+        if e["camera"] is not None and visible_path(e):
+            has_cam, has_cam_tr = True, apply_camera(e["id"], e["camera"])
+            break
+    if not has_cam and d["cameras"]:
+        has_cam, has_cam_tr = True, apply_camera(d["cameras"][0]["id"], d["cameras"][0]["camera"])
+
+    cull = _culling(d["culling"], cam) if (d["culling"]["enabled"] and has_cam_tr) else None
+
+    # meshes (SceneConversion.cpp:1029-1193)
+    geom = {g["id"]: g for g in d["geometry"]}
+    verts, idx, lverts, lidx, insts = [], [], [], [], []
+    nv = nt = nlv = nli = 0
+    for e in d["entities"]:
+        if e["mesh"] is None or not visible_path(e):
+            continue
+        g = geom.get(e["mesh"][0])
+        if g is None or len(g["vertices"]) == 0 or len(g["indices"]) == 0 or len(g["indices"]) % 3:
+            continue
+        tr, _ = resolve(e["id"])
+        physics = e["physics"] is not None and e["physics"]["enabled"] and e["physics"]["dynamic"]
+        scripted = scripted_path(e)
+        dynamic = physics or scripted or e["transform"] is not None
+        if cull is not None and not cull(g, tr, physics or scripted):
+            continue
+        mat_id = e["mesh"][1] if e["mesh"][1] != 0 else g["material_id"]
+        first_vertex = nv
+        verts.append(transform_points(g["vertices"], tr["t"], tr["r"], tr["s"]))
+        nv += len(g["vertices"])
+        tri = g["indices"].reshape(-1, 3)
+        tri = tri[(tri < len(g["vertices"])).all(1)]
+        if len(tri) == 0:
+            continue
+        idx.append((tri + first_vertex).reshape(-1))
+        inst = np.zeros((), S.INSTANCE_DTYPE)
+        inst["entity_id"] = e["id"]
+        inst["geometry_id"] = e["mesh"][0] & U32_MAX
+        inst["first_triangle"] = nt
+        inst["triangle_count"] = len(tri)
+        inst["material_index"] = lookup.get(mat_id, 0)
+        flags = 0
+        if dynamic:
+            flags |= S.INSTANCE_FLAG_DYNAMIC_TRANSFORM | (INSTANCE_FLAG_PHYSICS if physics else 0)
+            inst["local_first_vertex"], inst["local_vertex_count"] = nlv, len(g["vertices"])
+            inst["local_first_index"], inst["local_index_count"] = nli, tri.size
+            lverts.append(g["vertices"])
+            lidx.append(tri.reshape(-1))
+            nlv += len(g["vertices"])
+            nli += tri.size
+        if e["transform"] is not None and e["transform"]["dirty"]:
+            flags |= INSTANCE_FLAG_TRANSFORM_DIRTY
+        inst["flags"] = flags
+        inst["transform_revision"] = len(insts) + 1
+        inst["translation"], inst["rotation"], inst["scale"] = tr["t"], tr["r"], tr["s"]
+        insts.append(inst)
+        nt += len(tri)
+
+    # lights (SceneConversion.cpp:1587-1647)
+    env = np.zeros(3, np.float32)
+    lights = []
+
+    def add_light(lid, l, enabled):
+        nonlocal env
+        eff = max(F(0), l["intensity"]) if enabled else F(0)
+        if _normalize_material_id(l["type"]) in _ENV_TYPES:
+            if eff > 0:
+                env = (env + np.asarray(l["color"], np.float32) * max(F(0), l["intensity"])).astype(np.float32)
+            return
+        tr, has_tr = resolve(lid)
+        pos = tr["t"] if has_tr else [F(0), F(1.8), F(0)]
+        direction = np.asarray(l["direction"], np.float32)
+        if _dot(direction, direction) <= F(1.0e-8):
+            direction = rotate_quat([0, 0, -1], tr["r"])
+        li = S.make_light(pos, l["color"], eff, max(F(0), l["radius"]))
+        li["direction"] = _normalize3(direction)
+        li["spot_inner_cos"], li["spot_outer_cos"] = _spot_cone(l)
+        lights.append(li)
+
+    ecs = set()
+    for e in d["entities"]:
+        if e["light"] is not None:
+            ecs.add(e["id"])
+            add_light(e["id"], e["light"], visible_path(e))
+    for l in d["lights"]:
+        if l["id"] not in ecs:
+            add_light(l["id"], l["light"], True)
+
+    vertices = np.concatenate(verts) if verts else np.zeros((0, 3), np.float32)
+    if has_cam and not has_cam_tr and len(vertices):  # SceneConversion.cpp:1657-1674
+        bmin, bmax = vertices.min(0), vertices.max(0)
+        center = ((bmin + bmax) * F(0.5)).astype(np.float32)
+        ext = (bmax - bmin).astype(np.float32)
+        radius = max(F(0.5), F(0.5) * max(ext))
+        cam["target"] = center
+        cam["position"] = (center + np.array([0, max(F(0.6), F(0.4) * radius), max(F(2), F(2.2) * radius)],
+                                             np.float32)).astype(np.float32)
+    c = cam["c"] or dict({k: F(v) for k, v in _CAMERA_DEFAULTS.items()}, iris_blade_count=0)
+    camera = S.Camera(position=tuple(float(x) for x in cam["position"]),
+                      target=tuple(float(x) for x in cam["target"]), up=tuple(float(x) for x in cam["up"]),
+                      fov_deg=float(cam["fov"]), aperture_radius=float(c["aperture_radius"]),
+                      focus_distance=float(c["focus_distance"]), iris_blade_count=int(c["iris_blade_count"]),
+                      iris_rotation_degrees=float(c["iris_rotation_degrees"]),
+                      iris_roundness=float(c["iris_roundness"]), anamorphic_squeeze=float(c["anamorphic_squeeze"]))
+    return S.Scene(vertices, np.concatenate(idx).astype(np.uint32) if idx else np.zeros(0, np.uint32),
+                   np.array(insts, S.INSTANCE_DTYPE) if insts else np.zeros(0, S.INSTANCE_DTYPE), materials,
+                   np.array(lights, S.LIGHT_DTYPE) if lights else np.zeros(0, S.LIGHT_DTYPE), camera,
+                   local_vertices=np.concatenate(lverts) if lverts else np.zeros((0, 3), np.float32),
+                   local_indices=np.concatenate(lidx).astype(np.uint32) if lidx else np.zeros(0, np.uint32),
+                   environment_color=tuple(float(x) for x in env), environment_map_scale=tuple(float(x) for x in env),
+                   name=name)
+
+
+def _spot_cone(l):
+    """spot_cone_cosines (SceneConversion.cpp:503-515)."""
+    if _normalize_material_id(l["type"]) not in ("spot", "spot_light", "spotlight"):
+        return F(-1), F(-1)
+    inner = min(F(89), max(F(1), l["beam_angle"] * F(0.5)))
+    outer = min(F(89.5), max(inner + F(0.25), inner * (F(1) + max(F(0), l["blend"]))))
+    pi = F(math.pi)
+    return F(math.cos(float(inner * pi / F(180)))), F(math.cos(float(outer * pi / F(180))))
+
+
+def _cross(a, b):
+    return np.array([a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]], np.float32)
+
+
+def _culling(s, cam):
+    """The mesh culling test of SceneConversion.cpp:895-994 (frustum and distance against per-geometry
+    bounding spheres), or None when the camera basis degenerates."""
+    eps2 = F(1e-4) * F(1e-4)
+    fwd_raw = (cam["target"] - cam["position"]).astype(np.float32)
+    if not _dot(fwd_raw, fwd_raw) > eps2:
+        return None
+    if not (s["frustum"] or (s["distance"] and s["max_distance"] > 0)):
+        return None
+    fwd = _normalize3(fwd_raw)
+    right = _cross(fwd, cam["up"])
+    for alt in ([0, 1, 0], [1, 0, 0]):
+        if _dot(right, right) <= eps2:
+            right = _cross(fwd, np.asarray(alt, np.float32))
+    if not _dot(right, right) > eps2:
+        return None
+    right = _normalize3(right)
+    up = _normalize3(_cross(right, fwd))
+    fov = min(F(175), max(F(1), cam["fov"]))
+    tan_y = F(math.tan(float(fov * F(math.pi) / F(360)))) * max(F(0.01), s["frustum_padding"])
+    tan_x = tan_y * max(F(0.01), s["aspect_ratio"])
+    near, pos = cam["near"], cam["position"]
+    bounds = {}
+
+    def sphere(g):  # compute_mesh_cull_bounds (SceneConversion.cpp:199-221)
+        if g["id"] not in bounds:
+            v = g["vertices"]
+            center = ((v.min(0) + v.max(0)) * F(0.5)).astype(np.float32)
+            dv = (v - center).astype(np.float32)
+            ls = (dv[:, 0] * dv[:, 0] + dv[:, 1] * dv[:, 1]) + dv[:, 2] * dv[:, 2]
+            bounds[g["id"]] = (center, F(np.sqrt(ls).max()))
+        return bounds[g["id"]]
+
+    def include(g, tr, runtime_dynamic):
+        if runtime_dynamic and not s["cull_dynamic"]:
+            return True
+        center, radius = sphere(g)
+        c = transform_points(center, tr["t"], tr["r"], tr["s"])
+        r = max(s["min_instance_radius"], radius * max(F(0.001), max(abs(x) for x in tr["s"])))
+        rel = (c - pos).astype(np.float32)
+        if s["distance"] and s["max_distance"] > 0:
+            md = s["max_distance"] + r
+            if _dot(rel, rel) > md * md:
+                return False
+        if not s["frustum"]:
+            return True
+        z = _dot(rel, fwd)
+        if z + r < near:
+            return False
+        plane_z = max(z, near)
+        if abs(_dot(rel, right)) > plane_z * tan_x + r:
+            return False
+        return not abs(_dot(rel, up)) > plane_z * tan_y + r
+
+    return include
+
+
+def load_scene_document(path) -> S.Scene:
+    """Read a scene JSON file and convert it (:func:`document_to_scene`)."""
+    p = pathlib.Path(path)
+    return document_to_scene(json.loads(p.read_text(encoding="utf-8")), name=p.stem)
+
+
+def main(argv=None):
+    """Convert a scene JSON file to a binary snapshot (.ptsnap) that `pt_hip_bench --scene` renders."""
+    import argparse
+
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("scene")
+    ap.add_argument("--out", required=True)
+    a = ap.parse_args(argv)
+    root = json.loads(pathlib.Path(a.scene).read_text(encoding="utf-8"))
+    sc = document_to_scene(root, name=pathlib.Path(a.scene).stem)
+    sc.save_snapshot(a.out)
+    skipped = unsupported_features(root)
+    print(f"{a.scene}: {sc.triangle_count} triangles, {len(sc.materials)} materials, {len(sc.lights)} lights"
+          + (f" (not converted: {', '.join(skipped)})" if skipped else "") + f" -> {a.out}")
+
+
+if __name__ == "__main__":
+    main()
