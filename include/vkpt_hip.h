@@ -115,6 +115,15 @@ typedef struct PtRenderSettings {
   float camera_aperture_radius, camera_focus_distance;
 } PtRenderSettings;
 
+/* FilmResolveSettings, PathTracer.h:67-74. 24 bytes. tone_map: ToneMapMode (0 linear, 1 reinhard,
+ * 2 filmic_approx, 3 aces_approx); output_transform: OutputTransformMode (0 gamma, 1 linear). */
+typedef struct PtFilmResolve {
+  float exposure, white_balance_kelvin;
+  uint32_t tone_map, output_transform;
+  float gamma;
+  uint32_t clamp_output;
+} PtFilmResolve;
+
 /* RenderTile, PathTracer.h:919-927. One tile adds one sample (sample_index) to each pixel. */
 typedef struct PtRenderTile {
   uint32_t x, y, width, height, sample_index, tile_id, gpu_id;
@@ -194,8 +203,18 @@ int pt_hip_render_tiles(pt_tracer* tracer, const PtRenderTile* tiles, uint32_t c
 /* Film readback: FilmBuffer::raw / sample_counts (PathTracer.h:955-958). Host pointers:
  * rgb_sum holds 3 floats per pixel, sample_count one u32 per pixel (either may be NULL). */
 int pt_hip_read_film(pt_tracer* tracer, float* rgb_sum, uint32_t* sample_count);
-/* FilmBuffer::resolve_hdr, FilmBuffer.cpp:160-214: 3 floats per pixel (mean of valid samples). */
+/* IPathTracer::replace_film_history, PathTracer.h:1096-1098: replaces the accumulated film (host
+ * pointers, 3 floats and one count per pixel) without touching scene or acceleration data. */
+int pt_hip_replace_film(pt_tracer* tracer, const float* rgb_sum, const uint32_t* sample_count);
+/* IPathTracer::resolve_hdr, PathTracer.h:1171 (FilmBuffer::resolve_hdr, FilmBuffer.cpp:160-214), evaluated
+ * on the GPU: 3 floats per pixel (mean of the pixel's samples). rgb may be host or device memory. */
 int pt_hip_resolve_hdr(pt_tracer* tracer, float* rgb);
+/* IPathTracer::resolve_ldr, PathTracer.h:1170 (FilmBuffer::resolve_ldr -> ApplyFilmResolve,
+ * FilmBuffer.cpp:156-158, :255-332), evaluated on the GPU: exposure, white balance, tone map, gamma,
+ * clamp, 4 bytes (r, g, b, 255) per pixel. settings NULL = FilmResolveSettings{}; a caller applying the
+ * camera's exposure controls passes CameraAdjustedFilmResolveSettings' result (FilmBuffer.cpp:226-253).
+ * rgba8 may be host or device memory. */
+int pt_hip_resolve_ldr(pt_tracer* tracer, const PtFilmResolve* settings, uint8_t* rgba8);
 /* IPathTracer::read_counters, PathTracer.h:1180. */
 int pt_hip_read_counters(pt_tracer* tracer, PtSampleCounters* counters);
 /* IPathTracer::status, PathTracer.h:1181. */
@@ -320,6 +339,9 @@ int pt_batch_render_adjoint(pt_batch* batch, uint64_t frame_index, const float* 
                             float* grad_emission, float* grad_light, float* grad_roughness, float* grad_texels);
 /* Film of one scene, as pt_hip_read_film. */
 int pt_batch_read_film(pt_batch* batch, uint32_t scene, float* rgb_sum, uint32_t* sample_count);
+/* Resolved image of one scene, as pt_hip_resolve_hdr / pt_hip_resolve_ldr. */
+int pt_batch_resolve_hdr(pt_batch* batch, uint32_t scene, float* rgb);
+int pt_batch_resolve_ldr(pt_batch* batch, uint32_t scene, const PtFilmResolve* settings, uint8_t* rgba8);
 /* Counters summed over the batch's scenes. */
 int pt_batch_read_counters(pt_batch* batch, PtSampleCounters* counters);
 int pt_batch_set_tuning(pt_batch* batch, const PtTuning* tuning);
@@ -342,7 +364,7 @@ int pt_bvh_build_host(const PtSceneView* scene, const PtBvhParams* params, uint3
 
 /* ABI sizes, for bindings to check their mirrors of the records above. */
 uint32_t pt_hip_sizeof(int which); /* 0 material, 1 light, 2 instance, 3 camera, 4 scene, 5 settings, 6 counters,
-                                      7 tile, 8 tuning, 9 texture */
+                                      7 tile, 8 tuning, 9 texture, 10 film resolve */
 const char* pt_hip_version(void);
 
 #ifdef __cplusplus

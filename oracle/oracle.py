@@ -90,7 +90,19 @@ _SIGS = {
     "pto_render_tile": (C.c_int, [C.POINTER(PtoScene), _P, C.POINTER(PtoSettings), C.c_uint32, C.c_uint32, C.c_uint32,
                                   C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.c_int, C.c_int, _P, _P,
                                   C.POINTER(PtoCounters)]),
+    "pto_color_temperature_rgb": (None, [C.c_float, _P]),
+    "pto_white_balance_scale": (None, [C.c_float, _P]),
+    "pto_resolve_hdr": (None, [_P, _P, C.c_uint64, _P]),
 }
+
+
+
+class PtoResolve(C.Structure):
+    _fields_ = [("exposure", C.c_float), ("white_balance", C.c_float * 3), ("inv_gamma", C.c_float),
+                ("tone_map", C.c_uint32), ("output_transform", C.c_uint32), ("clamp_output", C.c_uint32)]
+
+
+_SIGS["pto_resolve_ldr"] = (None, [_P, _P, C.c_uint64, C.POINTER(PtoResolve), _P])
 
 _libs: dict[str, C.CDLL] = {}
 
@@ -315,3 +327,46 @@ def phong_samples(exponent: float, refl, normal, ps: float, state: int, count: i
     sc = np.zeros(count, np.float64)
     fn(exponent, r.ctypes.data, n.ctypes.data, ps, state & 0xFFFFFFFFFFFFFFFF, count, d.ctypes.data, sc.ctypes.data)
     return d, sc
+
+
+# ------------------------------------------------------------------ film resolve
+TONE_MAPS = ("linear", "reinhard", "filmic_approx", "aces_approx")
+
+
+def color_temperature_rgb(kelvin: float, variant: str = "f32") -> np.ndarray:
+    out = np.zeros(3, np.float32)
+    lib(variant).pto_color_temperature_rgb(kelvin, out.ctypes.data)
+    return out
+
+
+def white_balance_scale(kelvin: float, variant: str = "f32") -> np.ndarray:
+    out = np.zeros(3, np.float32)
+    lib(variant).pto_white_balance_scale(kelvin, out.ctypes.data)
+    return out
+
+
+def resolve_hdr(rgb_sum, count, variant: str = "f32") -> np.ndarray:
+    """FilmBuffer::resolve_hdr of a film (rgb sums (H, W, 3), counts (H, W))."""
+    s = np.ascontiguousarray(rgb_sum, np.float32)
+    c = np.ascontiguousarray(count, np.uint32)
+    out = np.zeros(s.shape, np.float32)
+    lib(variant).pto_resolve_hdr(s.ctypes.data, c.ctypes.data, c.size, out.ctypes.data)
+    return out
+
+
+def resolve_ldr(rgb_sum, count, exposure=1.0, white_balance_kelvin=6500.0, tone_map="linear",
+                output_transform="gamma", gamma=2.2, clamp_output=True, variant: str = "f32") -> np.ndarray:
+    """ApplyFilmResolve of resolve_hdr's image: (H, W, 4) u8. The settings are FilmResolveSettings'
+    fields; the white-balance scale and inverse gamma are formed here as the reference's host does."""
+    s = np.ascontiguousarray(rgb_sum, np.float32)
+    c = np.ascontiguousarray(count, np.uint32)
+    rs = PtoResolve()
+    rs.exposure = exposure
+    rs.white_balance[:] = list(white_balance_scale(white_balance_kelvin, variant))
+    rs.inv_gamma = float(np.float32(1.0) / max(np.float32(0.01), np.float32(gamma)))
+    rs.tone_map = TONE_MAPS.index(tone_map)
+    rs.output_transform = 0 if output_transform == "gamma" else 1
+    rs.clamp_output = 1 if clamp_output else 0
+    out = np.zeros(c.shape + (4,), np.uint8)
+    lib(variant).pto_resolve_ldr(s.ctypes.data, c.ctypes.data, c.size, C.byref(rs), out.ctypes.data)
+    return out

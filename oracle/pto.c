@@ -1999,3 +1999,90 @@ int pto_adjoint_pixels(const PtoScene* scene, const pto_accel* a, const PtoSetti
   }
   return 0;
 }
+
+/* ------------------------------------------------------------------ film resolve
+ * FilmBuffer::resolve_hdr (FilmBuffer.cpp:160-214) and ApplyFilmResolve (FilmBuffer.cpp:255-332) with
+ * WhiteBalanceScale / ColorTemperatureToRgb (FilmBuffer.cpp:30-49, :216-224). The colour-temperature fit
+ * is the published Tanner Helland approximation (coefficients / 255), evaluated with libm as the
+ * reference's host code does; the per-pixel gamma power uses this build's powf (portable or libm). */
+#if PTO_LIBM || PTO_REAL_IS_DOUBLE
+#define RESOLVE_POW(x, y) powf((x), (y))
+#else
+#define RESOLVE_POW(x, y) pm_powf((x), (y))
+#endif
+
+/* std::max / std::min / std::clamp on floats, with their comparison semantics. */
+static inline float fmaxf_std(float a, float b) { return (a < b) ? b : a; }
+static inline float fminf_std(float a, float b) { return (b < a) ? b : a; }
+static inline float fclampf(float v, float lo, float hi) { return (v < lo) ? lo : ((hi < v) ? hi : v); }
+
+void pto_color_temperature_rgb(float kelvin, float out[3]) {
+  const float t = fclampf(kelvin, 1000.0f, 40000.0f) / 100.0f;
+  float r = 1.0f, g = 1.0f, b = 1.0f;
+  if (t <= 66.0f) {
+    g = fclampf(0.39008158f * logf(fmaxf_std(1.0f, t)) - 0.63184144f, 0.0f, 1.0f);
+    b = t <= 19.0f ? 0.0f : fclampf(0.5432068f * logf(fmaxf_std(1.0f, t - 10.0f)) - 1.1962541f, 0.0f, 1.0f);
+  } else {
+    r = fclampf(1.2929362f * powf(t - 60.0f, -0.13320476f), 0.0f, 1.0f);
+    g = fclampf(1.1298909f * powf(t - 60.0f, -0.075514846f), 0.0f, 1.0f);
+  }
+  out[0] = r;
+  out[1] = g;
+  out[2] = b;
+}
+
+void pto_white_balance_scale(float kelvin, float out[3]) {
+  float d65[3], tgt[3];
+  pto_color_temperature_rgb(6500.0f, d65);
+  pto_color_temperature_rgb(kelvin, tgt);
+  for (int k = 0; k < 3; ++k) out[k] = d65[k] / fmaxf_std(1.0e-3f, tgt[k]);
+}
+
+void pto_resolve_hdr(const float* sum3, const uint32_t* count, uint64_t pixels, float* rgb) {
+  for (uint64_t i = 0; i < pixels; ++i) {
+    const float inv = 1.0f / (float)(count[i] > 1u ? count[i] : 1u);
+    for (int k = 0; k < 3; ++k) rgb[3 * i + k] = sum3[3 * i + k] * inv;
+  }
+}
+
+static float resolve_tone(float x, uint32_t mode) {
+  switch (mode) {
+    case 1:
+      return x / (1.0f + x);
+    case 2: {
+      const float A = 0.15f, B = 0.50f, C = 0.10f, D = 0.20f, E = 0.02f, F = 0.30f, W = 11.2f;
+      const float fx = ((x * (A * x + C * B) + D * E) / (x * (A * x + B) + D * F)) - E / F;
+      const float fw = ((W * (A * W + C * B) + D * E) / (W * (A * W + B) + D * F)) - E / F;
+      return fx / fw;
+    }
+    case 3:
+      return (x * (2.51f * x + 0.03f)) / (x * (2.43f * x + 0.59f) + 0.14f);
+    default:
+      return x;
+  }
+}
+
+/* static_cast<uint8_t>(float) as x86-64 code evaluates it: truncation to int32 (INT_MIN for NaN and
+ * out-of-range values), then the low byte. */
+static uint8_t to_byte(float v) {
+  const int32_t i = (v > -2147483648.0f && v < 2147483648.0f) ? (int32_t)v : INT32_MIN;
+  return (uint8_t)(uint32_t)i;
+}
+
+void pto_resolve_ldr(const float* sum3, const uint32_t* count, uint64_t pixels, const PtoResolve* rs,
+                     uint8_t* rgba) {
+  for (uint64_t i = 0; i < pixels; ++i) {
+    const float inv = 1.0f / (float)(count[i] > 1u ? count[i] : 1u);
+    float c[3];
+    for (int k = 0; k < 3; ++k) {
+      float v = sum3[3 * i + k] * inv * rs->exposure * rs->white_balance[k];
+      if (!isfinite(v)) v = 0.0f;
+      v = resolve_tone(v, rs->tone_map);
+      if (rs->output_transform == 0) v = RESOLVE_POW(fmaxf_std(0.0f, v), rs->inv_gamma);
+      if (rs->clamp_output) v = fminf_std(1.0f, fmaxf_std(0.0f, v));
+      c[k] = v;
+    }
+    for (int k = 0; k < 3; ++k) rgba[4 * i + k] = to_byte(c[k] * 255.0f + 0.5f);
+    rgba[4 * i + 3] = 255u;
+  }
+}

@@ -171,6 +171,25 @@ int pto_adjoint_pixels(const PtoScene* scene, const pto_accel* a, const PtoSetti
 int pto_phong_samples(float exponent, const float refl[3], const float n[3], float ps, uint64_t state,
                       uint32_t count, float* out_dirs, double* out_score);
 
+/* FilmResolveSettings (PathTracer.h:67-74) after the host-side conversions: white-balance scale
+ * (WhiteBalanceScale) and inverse gamma 1 / max(0.01, gamma). tone_map: 0 linear, 1 reinhard,
+ * 2 filmic_approx, 3 aces_approx (ToneMapMode); output_transform: 0 gamma, 1 linear. */
+typedef struct PtoResolve {
+  float exposure;
+  float white_balance[3];
+  float inv_gamma;
+  uint32_t tone_map, output_transform, clamp_output;
+} PtoResolve;
+
+/* ColorTemperatureToRgb / WhiteBalanceScale (FilmBuffer.cpp:30-49, :216-224), libm. */
+void pto_color_temperature_rgb(float kelvin, float out[3]);
+void pto_white_balance_scale(float kelvin, float out[3]);
+/* FilmBuffer::resolve_hdr (FilmBuffer.cpp:160-214): sum3 holds 3 floats per pixel. */
+void pto_resolve_hdr(const float* sum3, const uint32_t* count, uint64_t pixels, float* rgb);
+/* ApplyFilmResolve (FilmBuffer.cpp:255-332) of resolve_hdr's image: 4 bytes (rgba) per pixel. */
+void pto_resolve_ldr(const float* sum3, const uint32_t* count, uint64_t pixels, const PtoResolve* rs,
+                     uint8_t* rgba);
+
 #ifdef __cplusplus
 }
 #endif

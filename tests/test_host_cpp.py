@@ -81,3 +81,132 @@ def test_pt_hip_bench_film_equals_oracle(tmp_path, per_sample):
     assert res["throughput"]["samples_per_sec"] > 0 and res["renderer_path"] == "hip-wavefront"
     pfm = (out / "beauty.pfm").read_bytes()
     assert pfm.startswith(b"PF\n48 32\n-1.0\n")
+
+
+def _artifact_json(dir_):
+    r = subprocess.run([str(BENCH), "--validate-artifacts", str(dir_)], capture_output=True, text=True, timeout=60)
+    return r.returncode, json.loads(r.stdout)
+
+
+def test_validate_artifacts_contract(tmp_path):
+    """--validate-artifacts restates ValidateBenchmarkArtifactsOnDisk: every required file of the contract
+    is reported missing in an empty directory; a complete set passes; an empty file, a results.json path
+    that resolves to nothing and a trace without events are each reported."""
+    rc, v = _artifact_json(tmp_path)
+    assert rc == 1 and not v["ok"]
+    assert set(v["missing_files"]) == {"results.json", "results.csv", "metadata.json", "scene_snapshot.json",
+                                       "shader_manifest.json", "asset_manifest.json", "beauty.png", "beauty.exr",
+                                       "logs.jsonl", "profiler_trace.json"}
+    for name in v["missing_files"]:
+        (tmp_path / name).write_text("x")
+    (tmp_path / "results.json").write_text(json.dumps({k: f"{tmp_path}/{f}" for k, f in (
+        ("beauty_png", "beauty.png"), ("beauty_exr", "beauty.exr"), ("logs_jsonl", "logs.jsonl"),
+        ("profiler_trace_json", "profiler_trace.json"))}))
+    (tmp_path / "metadata.json").write_text('{"benchmark_capabilities": {}}')
+    (tmp_path / "profiler_trace.json").write_text('{"events": []}')
+    rc, v = _artifact_json(tmp_path)
+    assert rc == 0 and v["ok"] and not v["invalid_files"], v
+    (tmp_path / "beauty.png").write_text("")
+    (tmp_path / "profiler_trace.json").write_text("{}")
+    res = json.loads((tmp_path / "results.json").read_text())
+    res["beauty_exr"] = "/nonexistent/beauty.exr"
+    (tmp_path / "results.json").write_text(json.dumps(res))
+    rc, v = _artifact_json(tmp_path)
+    assert rc == 1 and v["empty_files"] == ["beauty.png"]
+    assert "profiler_trace.json: missing events array" in v["invalid_files"]
+    assert "results.json: beauty_exr does not resolve to an artifact file" in v["invalid_files"]
+
+
+def _read_png(path):
+    """RGBA8 pixels of a PNG, decoded with zlib and the PNG specification's chunk rules (CRC checked)."""
+    import struct
+    import zlib
+
+    raw = pathlib.Path(path).read_bytes()
+    assert raw[:8] == b"\x89PNG\r\n\x1a\n"
+    pos, idat, w, h = 8, b"", 0, 0
+    while pos < len(raw):
+        n, = struct.unpack(">I", raw[pos:pos + 4])
+        typ, data = raw[pos + 4:pos + 8], raw[pos + 8:pos + 8 + n]
+        assert struct.unpack(">I", raw[pos + 8 + n:pos + 12 + n])[0] == zlib.crc32(typ + data)
+        if typ == b"IHDR":
+            w, h, depth, ctype = struct.unpack(">IIBB", data[:10])
+            assert depth == 8 and ctype == 6
+        elif typ == b"IDAT":
+            idat += data
+        pos += 12 + n
+    rows = np.frombuffer(zlib.decompress(idat), np.uint8).reshape(h, 4 * w + 1)
+    assert (rows[:, 0] == 0).all()
+    return rows[:, 1:].reshape(h, w, 4)
+
+
+def _read_exr(path):
+    """RGB float pixels of an uncompressed scanline OpenEXR file, read by the OpenEXR layout rules."""
+    import struct
+
+    raw = pathlib.Path(path).read_bytes()
+    assert raw[:4] == b"\x76\x2f\x31\x01" and raw[4] == 2
+    pos, attrs = 8, {}
+    while raw[pos] != 0:
+        name_end = raw.index(b"\0", pos)
+        type_end = raw.index(b"\0", name_end + 1)
+        size, = struct.unpack("<i", raw[type_end + 1:type_end + 5])
+        attrs[raw[pos:name_end].decode()] = (raw[name_end + 1:type_end].decode(), raw[type_end + 5:type_end + 5 + size])
+        pos = type_end + 5 + size
+    pos += 1
+    assert attrs["compression"][1] == b"\0"
+    x0, y0, x1, y1 = struct.unpack("<4i", attrs["dataWindow"][1])
+    w, h = x1 - x0 + 1, y1 - y0 + 1
+    chl, names, p = attrs["channels"][1], [], 0
+    while chl[p] != 0:
+        e = chl.index(b"\0", p)
+        names.append(chl[p:e].decode())
+        assert struct.unpack("<i", chl[e + 1:e + 5])[0] == 2  # FLOAT
+        p = e + 17
+    offsets = struct.unpack(f"<{h}Q", raw[pos:pos + 8 * h])
+    img = np.zeros((h, w, len(names)), np.float32)
+    for off in offsets:
+        y, size = struct.unpack("<ii", raw[off:off + 8])
+        data = np.frombuffer(raw[off + 8:off + 8 + size], "<f4").reshape(len(names), w)
+        img[y - y0] = data.T
+    return {n: img[..., i] for i, n in enumerate(names)}
+
+
+@pytest.mark.gpu
+def test_pt_hip_bench_artifact_set(tmp_path):
+    """A run writes ptbench's whole artifact set and passes its own contract check; beauty.png holds the
+    oracle's ApplyFilmResolve of the written film, beauty.exr its resolve_hdr, image_hash is FNV-1a of
+    the PNG bytes, and the trace carries the per-kernel GPU events."""
+    from oracle import oracle as O
+    from vkpathtracer_amd import scene as S
+
+    sc = S.cornell_box()
+    path = tmp_path / "cornell.ptsnap"
+    sc.save_snapshot(path)
+    out = tmp_path / "out"
+    cmd = [str(BENCH), "--scene", str(path), "--width", "40", "--height", "30", "--spp", "2", "--warmup", "0",
+           "--tone-map", "aces_approx", "--exposure", "1.5", "--white-balance", "4000", "--out", str(out)]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    rc, v = _artifact_json(out)
+    assert rc == 0 and v["ok"], v
+    rgb, cnt = _read_film(out)
+    png = _read_png(out / "beauty.png")
+    want = O.resolve_ldr(rgb, cnt, exposure=1.5, white_balance_kelvin=4000.0, tone_map="aces_approx")
+    assert np.array_equal(png, want)
+    exr = _read_exr(out / "beauty.exr")
+    hdr = O.resolve_hdr(rgb, cnt)
+    for i, ch in enumerate("RGB"):
+        assert np.array_equal(exr[ch], hdr[..., i]), ch
+    res = json.loads((out / "results.json").read_text())
+    data = (out / "beauty.png").read_bytes()
+    h = 1469598103934665603
+    for b in data:
+        h = ((h ^ b) * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+    assert res["image_hash"] == f"{h:016x}"
+    assert res["resolve"]["tone_map"] == "aces_approx"
+    rows = (out / "results.csv").read_text().splitlines()
+    assert rows[0].startswith("run_id,scene,backend") and rows[1].endswith(res["image_hash"])
+    ev = json.loads((out / "profiler_trace.json").read_text())["events"]
+    gpu = {e["name"]: e for e in ev if e["kind"] == "gpu"}
+    assert set(gpu) == {"raygen", "extend", "shade", "shadow", "film"} and gpu["extend"]["launches"] > 0

@@ -88,6 +88,26 @@ class PtTextureDesc(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in ("offset", "width", "height", "pad")]
 
 
+class PtFilmResolve(C.Structure):
+    _fields_ = [("exposure", C.c_float), ("white_balance_kelvin", C.c_float), ("tone_map", C.c_uint32),
+                ("output_transform", C.c_uint32), ("gamma", C.c_float), ("clamp_output", C.c_uint32)]
+
+
+def make_resolve(r: "_scene.FilmResolve | None") -> PtFilmResolve:
+    """PtFilmResolve of a FilmResolve (None: FilmResolveSettings defaults)."""
+    r = r or _scene.FilmResolve()
+    if r.tone_map not in _scene.TONE_MAPS or r.output_transform not in _scene.OUTPUT_TRANSFORMS:
+        raise ValueError(f"unknown tone map {r.tone_map!r} or output transform {r.output_transform!r}")
+    out = PtFilmResolve()
+    out.exposure = r.exposure
+    out.white_balance_kelvin = r.white_balance_kelvin
+    out.tone_map = _scene.TONE_MAPS.index(r.tone_map)
+    out.output_transform = _scene.OUTPUT_TRANSFORMS.index(r.output_transform)
+    out.gamma = r.gamma
+    out.clamp_output = 1 if r.clamp_output else 0
+    return out
+
+
 KERNELS = ("raygen", "extend", "shade", "shadow", "film")
 
 
@@ -111,6 +131,8 @@ SIGNATURES = {
     "pt_hip_render_tiles": (C.c_int, [_P, C.POINTER(PtRenderTile), C.c_uint32, C.c_uint64]),
     "pt_hip_read_film": (C.c_int, [_P, _P, _P]),
     "pt_hip_resolve_hdr": (C.c_int, [_P, _P]),
+    "pt_hip_resolve_ldr": (C.c_int, [_P, C.POINTER(PtFilmResolve), _P]),
+    "pt_hip_replace_film": (C.c_int, [_P, _P, _P]),
     "pt_hip_read_counters": (C.c_int, [_P, C.POINTER(PtSampleCounters)]),
     "pt_hip_status": (C.c_int, [_P, C.POINTER(PtTracerStatus)]),
     "pt_hip_shutdown": (C.c_int, [_P]),
@@ -144,6 +166,8 @@ SIGNATURES = {
     "pt_batch_reset_accumulation": (C.c_int, [_P]),
     "pt_batch_render": (C.c_int, [_P, C.c_uint64, C.c_int]),
     "pt_batch_read_film": (C.c_int, [_P, C.c_uint32, _P, _P]),
+    "pt_batch_resolve_hdr": (C.c_int, [_P, C.c_uint32, _P]),
+    "pt_batch_resolve_ldr": (C.c_int, [_P, C.c_uint32, C.POINTER(PtFilmResolve), _P]),
     "pt_batch_read_counters": (C.c_int, [_P, C.POINTER(PtSampleCounters)]),
     "pt_batch_set_tuning": (C.c_int, [_P, C.POINTER(PtTuning)]),
     "pt_batch_get_tuning": (C.c_int, [_P, C.POINTER(PtTuning)]),

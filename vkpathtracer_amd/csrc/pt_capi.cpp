@@ -17,6 +17,8 @@
 #include "pt_device.h"
 #include "vkpt_hip.h"
 
+static_assert(sizeof(PtFilmResolve) == 24, "PtFilmResolve layout");
+
 namespace {
 
 constexpr uint32_t kDefaultCapacity = 128u * 1024u * 1024u;  // paths per pass: 64 spp of 1080p in one pass (~20 GB)
@@ -113,6 +115,7 @@ struct pt_tracer {
   DevArray<float> d_grad_partial;  // adjoint: per-block gradient rows
   size_t texel_count = 0;
   DevArray<uint32_t> d_film_count, d_queue, d_error;
+  DevArray<uint32_t> d_resolve;  // resolve_hdr / resolve_ldr output before its copy to the caller
   DevArray<unsigned long long> d_counters, d_query_counters;
   DevArray<float> d_env;
   DevArray<float> d_path_f;
@@ -996,6 +999,7 @@ void destroy_tracer(pt_tracer* t) {
   free_paths(t);
   t->d_film_sum.release();
   t->d_film_count.release();
+  t->d_resolve.release();
   t->d_queue.release();
   t->d_error.release();
   t->d_counters.release();
@@ -1041,6 +1045,68 @@ void to_counters(const unsigned long long* c, PtSampleCounters* out) {
   out->shadow_tests = c[ptd::kShadowTests];
 }
 
+// ColorTemperatureToRgb (FilmBuffer.cpp:30-49): the Tanner Helland fit, with the host's libm as in the
+// reference.
+void color_temperature_rgb(float kelvin, float out[3]) {
+  const float t = std::clamp(kelvin, 1000.0f, 40000.0f) / 100.0f;
+  float r = 1.0f, g = 1.0f, b = 1.0f;
+  if (t <= 66.0f) {
+    g = std::clamp(0.39008158f * std::log(std::max(1.0f, t)) - 0.63184144f, 0.0f, 1.0f);
+    b = t <= 19.0f ? 0.0f : std::clamp(0.5432068f * std::log(std::max(1.0f, t - 10.0f)) - 1.1962541f, 0.0f, 1.0f);
+  } else {
+    r = std::clamp(1.2929362f * std::pow(t - 60.0f, -0.13320476f), 0.0f, 1.0f);
+    g = std::clamp(1.1298909f * std::pow(t - 60.0f, -0.075514846f), 0.0f, 1.0f);
+  }
+  out[0] = r;
+  out[1] = g;
+  out[2] = b;
+}
+
+// FilmResolveSettings -> the kernel's record: WhiteBalanceScale (FilmBuffer.cpp:216-224) and the
+// inverse gamma of ApplyFilmResolve (FilmBuffer.cpp:276).
+int make_resolve(pt_tracer* t, const PtFilmResolve* in, ptd::DevResolve& rs) {
+  const PtFilmResolve def{1.0f, 6500.0f, 0u, 0u, 2.2f, 1u};
+  const PtFilmResolve& s = in ? *in : def;
+  if (s.tone_map > 3u) return fail(t, PT_INVALID_ARGUMENT, "resolve_ldr: tone_map must be 0..3");
+  if (s.output_transform > 1u) return fail(t, PT_INVALID_ARGUMENT, "resolve_ldr: output_transform must be 0 or 1");
+  float d65[3], target[3];
+  color_temperature_rgb(6500.0f, d65);
+  color_temperature_rgb(s.white_balance_kelvin, target);
+  rs.exposure = s.exposure;
+  for (int k = 0; k < 3; ++k) rs.wb[k] = d65[k] / std::max(1.0e-3f, target[k]);
+  rs.inv_gamma = 1.0f / std::max(0.01f, s.gamma);
+  rs.tone_map = s.tone_map;
+  rs.gamma_out = s.output_transform == 0u ? 1u : 0u;
+  rs.clamp_out = s.clamp_output ? 1u : 0u;
+  return PT_OK;
+}
+
+// Resolves film pixels [base, base + pixels) on the device and copies the image to out (host or device
+// memory): 3 floats per pixel (rs == nullptr) or 4 bytes.
+int resolve_range(pt_tracer* t, size_t base, size_t pixels, const ptd::DevResolve* rs, void* out) {
+  if (pixels == 0) return PT_OK;
+  if (pixels > 0xFFFFFFFFull) return fail(t, PT_INVALID_ARGUMENT, "resolve: image too large");
+  const size_t words = rs ? pixels : 3u * pixels;
+  PT_HIP(t, t->d_resolve.ensure(words), "allocate resolve buffer");
+  const ptd::DevFilm film{t->d_film_sum.ptr + base, t->d_film_count.ptr + base};
+  ptd::launch_resolve(t->stream, film, static_cast<uint32_t>(pixels), rs, t->d_resolve.ptr);
+  PT_HIP(t, hipGetLastError(), "launch resolve");
+  PT_HIP(t, hipMemcpyAsync(out, t->d_resolve.ptr, words * sizeof(uint32_t), hipMemcpyDefault, t->stream),
+         "copy resolved image");
+  return sync_all(t);
+}
+
+// The film range of one batch scene: its first pixel in the pooled film and its pixel count.
+int batch_film_range(pt_batch* b, uint32_t scene, const char* what, size_t& base, size_t& pixels) {
+  pt_tracer* t = &b->t;
+  if (!t->accel_valid) return fail(t, PT_NOT_READY, std::string(what) + " requires pt_batch_build");
+  if (scene >= t->scenes.size()) return fail(t, PT_INVALID_ARGUMENT, std::string(what) + ": no such scene");
+  base = 0;
+  for (uint32_t i = 0; i < scene; ++i) base += static_cast<size_t>(t->scenes[i].width) * t->scenes[i].height;
+  pixels = static_cast<size_t>(t->scenes[scene].width) * t->scenes[scene].height;
+  return PT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1059,6 +1125,7 @@ uint32_t pt_hip_sizeof(int which) {
     case 7: return sizeof(PtRenderTile);
     case 8: return sizeof(PtTuning);
     case 9: return sizeof(PtTextureDesc);
+    case 10: return sizeof(PtFilmResolve);
     default: return 0;
   }
 }
@@ -1231,27 +1298,42 @@ int pt_hip_read_film(pt_tracer* t, float* rgb_sum, uint32_t* sample_count) {
   return read_film_range(t, 0, static_cast<size_t>(t->settings.width) * t->settings.height, rgb_sum, sample_count);
 }
 
+int pt_hip_replace_film(pt_tracer* t, const float* rgb_sum, const uint32_t* sample_count) {
+  if (!t || !rgb_sum || !sample_count) return fail(t, PT_INVALID_ARGUMENT, "replace_film_history: null argument");
+  if (!t->configured) return fail(t, PT_NOT_READY, "replace_film_history called before configure");
+  (void)hipSetDevice(t->device);
+  const size_t pixels = static_cast<size_t>(t->settings.width) * t->settings.height;
+  std::vector<float4> sum(pixels);
+  for (size_t i = 0; i < pixels; ++i)
+    sum[i] = make_float4(rgb_sum[3 * i], rgb_sum[3 * i + 1], rgb_sum[3 * i + 2], 0.0f);
+  PT_HIP(t, hipStreamSynchronize(t->stream), "synchronize");
+  if (pixels) {
+    PT_HIP(t, hipMemcpyAsync(t->d_film_sum.ptr, sum.data(), pixels * sizeof(float4), hipMemcpyHostToDevice, t->stream),
+           "upload film");
+    PT_HIP(t, hipMemcpyAsync(t->d_film_count.ptr, sample_count, pixels * sizeof(uint32_t), hipMemcpyHostToDevice,
+                             t->stream),
+           "upload film");
+  }
+  PT_HIP(t, hipStreamSynchronize(t->stream), "synchronize");
+  t->last_error.clear();
+  return PT_OK;
+}
+
 int pt_hip_resolve_hdr(pt_tracer* t, float* rgb) {
   if (!t || !rgb) return fail(t, PT_INVALID_ARGUMENT, "resolve_hdr: null argument");
-  const size_t pixels = static_cast<size_t>(t->settings.width) * t->settings.height;
-  std::vector<uint32_t> counts(pixels);
-  const int rc = pt_hip_read_film(t, rgb, counts.data());
+  if (!t->configured) return fail(t, PT_NOT_READY, "resolve_hdr called before configure");
+  (void)hipSetDevice(t->device);
+  return resolve_range(t, 0, static_cast<size_t>(t->settings.width) * t->settings.height, nullptr, rgb);
+}
+
+int pt_hip_resolve_ldr(pt_tracer* t, const PtFilmResolve* settings, uint8_t* rgba8) {
+  if (!t || !rgba8) return fail(t, PT_INVALID_ARGUMENT, "resolve_ldr: null argument");
+  if (!t->configured) return fail(t, PT_NOT_READY, "resolve_ldr called before configure");
+  ptd::DevResolve rs{};
+  const int rc = make_resolve(t, settings, rs);
   if (rc != PT_OK) return rc;
-  // FilmBuffer::resolve_hdr, FilmBuffer.cpp:160-214
-  const bool uniform = std::all_of(counts.begin(), counts.end(), [&](uint32_t c) { return c == counts.front(); });
-  if (uniform && pixels) {
-    const float inv = 1.0f / static_cast<float>(std::max(1u, counts.front()));
-    if (inv != 1.0f)
-      for (size_t i = 0; i < 3 * pixels; ++i) rgb[i] = rgb[i] * inv;
-    return PT_OK;
-  }
-  for (size_t i = 0; i < pixels; ++i) {
-    const float inv = 1.0f / static_cast<float>(std::max(1u, counts[i]));
-    rgb[3 * i] = rgb[3 * i] * inv;
-    rgb[3 * i + 1] = rgb[3 * i + 1] * inv;
-    rgb[3 * i + 2] = rgb[3 * i + 2] * inv;
-  }
-  return PT_OK;
+  (void)hipSetDevice(t->device);
+  return resolve_range(t, 0, static_cast<size_t>(t->settings.width) * t->settings.height, &rs, rgba8);
 }
 
 int pt_hip_read_counters(pt_tracer* t, PtSampleCounters* out) {
@@ -1368,6 +1450,7 @@ int pt_hip_shutdown(pt_tracer* t) {
   free_paths(t);
   t->d_film_sum.release();
   t->d_film_count.release();
+  t->d_resolve.release();
   t->scenes.clear();
   t->configured = t->scene_loaded = t->accel_valid = false;
   t->current_sample = 0;
@@ -1776,6 +1859,27 @@ int pt_batch_read_film(pt_batch* b, uint32_t scene, float* rgb_sum, uint32_t* sa
   for (uint32_t i = 0; i < scene; ++i) base += static_cast<size_t>(t->scenes[i].width) * t->scenes[i].height;
   const SceneSlot& s = t->scenes[scene];
   return read_film_range(t, base, static_cast<size_t>(s.width) * s.height, rgb_sum, sample_count);
+}
+
+int pt_batch_resolve_hdr(pt_batch* b, uint32_t scene, float* rgb) {
+  if (!b || !rgb) return b ? fail(&b->t, PT_INVALID_ARGUMENT, "batch_resolve_hdr: null argument") : PT_INVALID_ARGUMENT;
+  size_t base = 0, pixels = 0;
+  const int rc = batch_film_range(b, scene, "batch_resolve_hdr", base, pixels);
+  if (rc != PT_OK) return rc;
+  (void)hipSetDevice(b->t.device);
+  return resolve_range(&b->t, base, pixels, nullptr, rgb);
+}
+
+int pt_batch_resolve_ldr(pt_batch* b, uint32_t scene, const PtFilmResolve* settings, uint8_t* rgba8) {
+  if (!b || !rgba8) return b ? fail(&b->t, PT_INVALID_ARGUMENT, "batch_resolve_ldr: null argument") : PT_INVALID_ARGUMENT;
+  size_t base = 0, pixels = 0;
+  int rc = batch_film_range(b, scene, "batch_resolve_ldr", base, pixels);
+  if (rc != PT_OK) return rc;
+  ptd::DevResolve rs{};
+  rc = make_resolve(&b->t, settings, rs);
+  if (rc != PT_OK) return rc;
+  (void)hipSetDevice(b->t.device);
+  return resolve_range(&b->t, base, pixels, &rs, rgba8);
 }
 
 int pt_batch_read_counters(pt_batch* b, PtSampleCounters* out) {

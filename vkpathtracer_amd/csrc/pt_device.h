@@ -178,6 +178,17 @@ struct DevFilm {
   uint32_t* count;
 };
 
+// FilmResolveSettings (PathTracer.h:67-74) after the host-side conversions: the white-balance scale
+// (WhiteBalanceScale, FilmBuffer.cpp:216-224) and 1 / max(0.01, gamma).
+struct DevResolve {
+  float exposure;
+  float wb[3];
+  float inv_gamma;
+  uint32_t tone_map;   // ToneMapMode: 0 linear, 1 reinhard, 2 filmic_approx, 3 aces_approx
+  uint32_t gamma_out;  // OutputTransformMode::Gamma
+  uint32_t clamp_out;
+};
+
 // Path-replay adjoint state (no counterpart in the reference). Per path: forward radiance, the
 // loss adjoint of that radiance, the bounce awaiting its remainder radiance, and the history of
 // vertices before the roulette start depth (see k_shade). Per shadow record: the gradient pieces
@@ -247,6 +258,8 @@ void launch_grad_reduce(hipStream_t stream, const DevAdjoint& adj);  // rows -> 
 constexpr uint32_t kGradRows = 1024;  // partial rows: an upper bound on every adjoint kernel's grid
 void launch_zero_counters(hipStream_t stream, unsigned long long* counters);  // kCounterWords
 void launch_clear_film(hipStream_t stream, const DevFilm& film, uint32_t pixels);
+// resolve_hdr (rs == nullptr: 3 floats per pixel into out) or resolve_ldr (rgba8 per pixel into out).
+void launch_resolve(hipStream_t stream, const DevFilm& film, uint32_t pixels, const DevResolve* rs, void* out);
 
 // Workgroups of each kernel that are resident on the device at once (its persistent grid): a
 // persistent kernel launched with more would leave the extra waves' chunks to a late tail.

@@ -492,6 +492,62 @@ __global__ __launch_bounds__(kBlock) void k_clear_film(DevFilm film, uint32_t pi
   }
 }
 
+// FilmBuffer::resolve_hdr (FilmBuffer.cpp:160-214) and ApplyFilmResolve (FilmBuffer.cpp:255-332), one
+// thread per pixel: the mean of the pixel's samples, then exposure, white balance, tone map, gamma,
+// clamp and 8-bit quantisation in the reference's operation order (HBM-bound: 20 bytes read, 4 or 12
+// written per pixel).
+__device__ inline float resolve_tone(float x, uint32_t mode) {
+  switch (mode) {
+    case 1:
+      return x / (1.0f + x);
+    case 2: {
+      const float A = 0.15f, B = 0.50f, C = 0.10f, D = 0.20f, E = 0.02f, F = 0.30f, W = 11.2f;
+      const float fx = ((x * (A * x + C * B) + D * E) / (x * (A * x + B) + D * F)) - E / F;
+      const float fw = ((W * (A * W + C * B) + D * E) / (W * (A * W + B) + D * F)) - E / F;
+      return fx / fw;
+    }
+    case 3:
+      return (x * (2.51f * x + 0.03f)) / (x * (2.43f * x + 0.59f) + 0.14f);
+    default:
+      return x;
+  }
+}
+
+// static_cast<uint8_t>(float) as x86-64 code evaluates it (truncation to int32, INT_MIN for NaN and
+// out-of-range values, then the low byte), so unclamped output quantises as on the reference's host.
+__device__ inline uint32_t resolve_byte(float v) {
+  const int32_t i = (v > -2147483648.0f && v < 2147483648.0f) ? static_cast<int32_t>(v) : INT32_MIN;
+  return static_cast<uint32_t>(i) & 0xFFu;
+}
+
+template <bool kLdr>
+__global__ __launch_bounds__(kBlock) void k_resolve(DevFilm film, uint32_t pixels, DevResolve rs, float* hdr,
+                                                   uint32_t* ldr) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= pixels) return;
+  const float4 s = film.sum[i];
+  const uint32_t c = film.count[i];
+  const float inv = 1.0f / static_cast<float>(c > 1u ? c : 1u);
+  const float m[3] = {s.x * inv, s.y * inv, s.z * inv};
+  if constexpr (!kLdr) {
+    hdr[3u * i] = m[0];
+    hdr[3u * i + 1u] = m[1];
+    hdr[3u * i + 2u] = m[2];
+  } else {
+    uint32_t packed = 0xFF000000u;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      float v = m[k] * rs.exposure * rs.wb[k];
+      if (!isfinite(v)) v = 0.0f;
+      v = resolve_tone(v, rs.tone_map);
+      if (rs.gamma_out) v = ptm::powf_p(smax(0.0f, v), rs.inv_gamma);
+      if (rs.clamp_out) v = smin(1.0f, smax(0.0f, v));
+      packed |= resolve_byte(v * 255.0f + 0.5f) << (8 * k);
+    }
+    ldr[i] = packed;
+  }
+}
+
 // camera_rays + shade_pixel seeding, ScalarCpuPathTracer.Render.cpp:55-119, :243-262.
 // The adjoint variant also loads the path's loss adjoint from the film gradient.
 template <bool kAdj>
@@ -1780,6 +1836,17 @@ void launch_zero_counters(hipStream_t stream, unsigned long long* counters) {
 void launch_clear_film(hipStream_t stream, const DevFilm& film, uint32_t pixels) {
   if (pixels == 0) return;
   hipLaunchKernelGGL(k_clear_film, dim3((pixels + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, film, pixels);
+}
+
+void launch_resolve(hipStream_t stream, const DevFilm& film, uint32_t pixels, const DevResolve* rs, void* out) {
+  if (pixels == 0) return;
+  const dim3 grid((pixels + kBlock - 1) / kBlock);
+  if (rs)
+    hipLaunchKernelGGL(k_resolve<true>, grid, dim3(kBlock), 0, stream, film, pixels, *rs, nullptr,
+                       static_cast<uint32_t*>(out));
+  else
+    hipLaunchKernelGGL(k_resolve<false>, grid, dim3(kBlock), 0, stream, film, pixels, DevResolve{},
+                       static_cast<float*>(out), nullptr);
 }
 
 void launch_raygen(hipStream_t stream, const DevPaths& paths, const DevScene& scene, const DevSettings& st,

@@ -249,6 +249,25 @@ FilmHdr HipPathTracer::resolve_hdr() const {
   return out;
 }
 
+FilmLdr HipPathTracer::resolve_ldr(const PtFilmResolve* settings) const {
+  FilmLdr out;
+  if (!handle_) return out;
+  out.width = settings_.width;
+  out.height = settings_.height;
+  out.rgba8.resize(4ull * out.width * out.height);
+  if (!fail_bool(pt_hip_resolve_ldr(handle_, settings, out.rgba8.data()))) out = FilmLdr{};
+  return out;
+}
+
+bool HipPathTracer::replace_film_history(const Film& film) {
+  if (!handle_) return false;
+  if (film.width != settings_.width || film.height != settings_.height ||
+      film.rgb_sum.size() != 3ull * film.width * film.height ||
+      film.counts.size() != static_cast<size_t>(film.width) * film.height)
+    return false;
+  return fail_bool(pt_hip_replace_film(handle_, film.rgb_sum.data(), film.counts.data()));
+}
+
 Film HipPathTracer::film() const {
   Film out;
   if (!handle_) return out;

@@ -66,6 +66,11 @@ struct FilmHdr {  // FilmBuffer::resolve_hdr (FilmBuffer.cpp:160-214)
   std::vector<float> rgb;  // 3 per pixel
 };
 
+struct FilmLdr {  // FilmLdr / ApplyFilmResolve (FilmBuffer.cpp:255-332)
+  uint32_t width = 0, height = 0;
+  std::vector<uint8_t> rgba8;  // 4 per pixel
+};
+
 struct TracerStatus {  // PathTracerStatus (PathTracer.h:830-851) and the last error message
   PtTracerStatus state{};
   std::string last_error;
@@ -93,6 +98,10 @@ class HipPathTracer {
   bool supports_tile_batching() const { return true; }
   bool update_scene_delta(std::span<const PtMaterial> materials, std::span<const PtLight> lights);
   FilmHdr resolve_hdr() const;
+  // IPathTracer::resolve_ldr with the given FilmResolveSettings (nullptr: the defaults).
+  FilmLdr resolve_ldr(const PtFilmResolve* settings = nullptr) const;
+  // IPathTracer::replace_film_history (PathTracer.h:1096-1098).
+  bool replace_film_history(const Film& film);
   Film film() const;
   PtSampleCounters read_counters() const;
   TracerStatus status() const;

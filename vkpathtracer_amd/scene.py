@@ -181,6 +181,47 @@ class RenderSettings:
     camera_focus_distance: float = 0.0
 
 
+TONE_MAPS = ("linear", "reinhard", "filmic_approx", "aces_approx")  # ToneMapMode, PathTracer.h:55-60
+OUTPUT_TRANSFORMS = ("gamma", "linear")                              # OutputTransformMode, PathTracer.h:62-65
+
+
+@dataclasses.dataclass
+class FilmResolve:
+    """FilmResolveSettings (PathTracer.h:67-74): the LDR resolve's exposure, white balance, tone map,
+    output transform and clamp."""
+    exposure: float = 1.0
+    white_balance_kelvin: float = 6500.0
+    tone_map: str = "linear"
+    output_transform: str = "gamma"
+    gamma: float = 2.2
+    clamp_output: bool = True
+
+
+def camera_adjusted_resolve(base: FilmResolve, f_stop: float = 0.0, shutter_seconds: float = 0.0166666675,
+                            iso: float = 100.0, exposure_compensation: float = 0.0,
+                            white_balance_kelvin: float = 6500.0) -> FilmResolve:
+    """CameraAdjustedFilmResolveSettings (FilmBuffer.cpp:226-253) in float32: the camera's f-stop,
+    shutter and ISO relative to f/2.8, 1/60 s, ISO 100 scale the exposure (when all three are positive),
+    exposure compensation multiplies it by 2^EV, and an in-range white balance replaces the base's. The
+    defaults are PathTracerSceneSnapshot's (PathTracer.h:400-404)."""
+    f32 = np.float32
+    exposure = f32(base.exposure)
+    fs, sh, iso_ = f32(f_stop), f32(shutter_seconds), f32(iso)
+    if all(math.isfinite(v) for v in (fs, sh, iso_)) and fs > 0 and sh > 0 and iso_ > 0:
+        reference = (f32(1.0) / f32(60.0) * f32(100.0)) / (f32(2.8) * f32(2.8))
+        physical = (sh * iso_) / (fs * fs)
+        ratio = physical / max(f32(1.0e-6), reference)
+        exposure = f32(exposure * min(f32(1.0e6), max(f32(1.0e-6), ratio)))
+    ec = f32(exposure_compensation)
+    if math.isfinite(ec):
+        exposure = f32(exposure * np.power(f32(2.0), min(f32(32.0), max(f32(-32.0), ec))))
+    kelvin = base.white_balance_kelvin
+    wb = f32(white_balance_kelvin)
+    if math.isfinite(wb) and 1000.0 <= wb <= 40000.0:
+        kelvin = float(wb)
+    return dataclasses.replace(base, exposure=float(exposure), white_balance_kelvin=kelvin)
+
+
 # ---------------------------------------------------------------------- geometry helpers
 def _quad(p0, p1, p2, p3):
     return np.array([p0, p1, p2, p3], np.float64), np.array([[0, 1, 2], [0, 2, 3]], np.int64)

@@ -212,11 +212,31 @@ class HipPathTracer:
         self._check(self._lib.pt_hip_read_film(self._h, rgb.ctypes.data, cnt.ctypes.data), "read_film")
         return rgb, cnt
 
-    def resolve_hdr(self) -> np.ndarray:
+    def replace_film_history(self, rgb_sum, counts) -> None:
+        """IPathTracer::replace_film_history: the film becomes (rgb sums (H, W, 3), counts (H, W))."""
         s = self._settings
-        rgb = np.zeros((s.height, s.width, 3), np.float32)
-        self._check(self._lib.pt_hip_resolve_hdr(self._h, rgb.ctypes.data), "resolve_hdr")
+        rgb = np.ascontiguousarray(rgb_sum, np.float32).reshape(s.height, s.width, 3)
+        cnt = np.ascontiguousarray(counts, np.uint32).reshape(s.height, s.width)
+        self._check(self._lib.pt_hip_replace_film(self._h, rgb.ctypes.data, cnt.ctypes.data), "replace_film_history")
+
+    def resolve_hdr(self, out=None):
+        """FilmBuffer::resolve_hdr on the GPU: (H, W, 3) f32 mean radiance. out: an optional device
+        tensor of H*W*3 floats that receives the image instead of a new numpy array."""
+        s = self._settings
+        rgb = out if out is not None else np.zeros((s.height, s.width, 3), np.float32)
+        ptr = _dptr(out) if out is not None else rgb.ctypes.data
+        self._check(self._lib.pt_hip_resolve_hdr(self._h, ptr), "resolve_hdr")
         return rgb
+
+    def resolve_ldr(self, settings=None, out=None):
+        """IPathTracer::resolve_ldr on the GPU (ApplyFilmResolve): (H, W, 4) u8 rgba. settings: a
+        FilmResolve (None: defaults); out: an optional device tensor of H*W*4 bytes."""
+        s = self._settings
+        rs = abi.make_resolve(settings)
+        img = out if out is not None else np.zeros((s.height, s.width, 4), np.uint8)
+        ptr = _dptr(out) if out is not None else img.ctypes.data
+        self._check(self._lib.pt_hip_resolve_ldr(self._h, C.byref(rs), ptr), "resolve_ldr")
+        return img
 
     def read_counters(self) -> dict:
         c = abi.PtSampleCounters()
@@ -417,6 +437,19 @@ class HipBatch:
         cnt = np.zeros((h, w), np.uint32)
         self._check(self._lib.pt_batch_read_film(self._h, scene, rgb.ctypes.data, cnt.ctypes.data), "read_film")
         return rgb, cnt
+
+    def resolve_hdr(self, scene: int) -> np.ndarray:
+        w, h, _ = self.scenes[scene]
+        rgb = np.zeros((h, w, 3), np.float32)
+        self._check(self._lib.pt_batch_resolve_hdr(self._h, scene, rgb.ctypes.data), "resolve_hdr")
+        return rgb
+
+    def resolve_ldr(self, scene: int, settings=None) -> np.ndarray:
+        w, h, _ = self.scenes[scene]
+        img = np.zeros((h, w, 4), np.uint8)
+        rs = abi.make_resolve(settings)
+        self._check(self._lib.pt_batch_resolve_ldr(self._h, scene, C.byref(rs), img.ctypes.data), "resolve_ldr")
+        return img
 
     def read_counters(self) -> dict:
         c = abi.PtSampleCounters()
