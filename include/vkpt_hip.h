@@ -124,6 +124,20 @@ typedef struct PtFilmResolve {
   uint32_t clamp_output;
 } PtFilmResolve;
 
+/* FilmReadbackToken, PathTracer.h:894-901 (only the latest request's token is valid), and
+ * FilmReadbackState, PathTracer.h:903-909. */
+typedef struct PtFilmReadbackToken {
+  uint64_t id, fence_value;
+  uint32_t width, height;
+} PtFilmReadbackToken;
+enum {
+  PT_READBACK_INVALID = 0,
+  PT_READBACK_PENDING = 1,
+  PT_READBACK_READY = 2,
+  PT_READBACK_FAILED = 3,
+  PT_READBACK_UNSUPPORTED = 4
+};
+
 /* RenderTile, PathTracer.h:919-927. One tile adds one sample (sample_index) to each pixel. */
 typedef struct PtRenderTile {
   uint32_t x, y, width, height, sample_index, tile_id, gpu_id;
@@ -193,6 +207,14 @@ int pt_hip_load_scene(pt_tracer* tracer, const PtSceneView* scene);
 /* IPathTracer::build_or_update_acceleration, PathTracer.h:1095 (ScalarCpuPathTracer.cpp:479-520)
  * and IRayAccelerator::build / build_info, PathTracer.h:1023, :1034. params may be NULL. */
 int pt_hip_build_accel(pt_tracer* tracer, const PtBvhParams* params, PtBuildInfo* info);
+/* IPathTracer::update_render_settings, PathTracer.h:1083-1087: new integrator settings (seed, spp,
+ * depth, NEE / MIS, roulette, lens defaults) without touching the scene or BVH. A different resolution
+ * returns PT_UNSUPPORTED: the caller falls back to configure / load / build, as the contract says. */
+int pt_hip_update_render_settings(pt_tracer* tracer, const PtRenderSettings* settings);
+/* IPathTracer::update_camera / update_camera_state, PathTracer.h:1100-1106 (ScalarCpuPathTracer.cpp:522-546):
+ * the scene's camera changes without re-uploading geometry or rebuilding the BVH. The accumulated film
+ * is kept (the caller resets it, as with the reference). */
+int pt_hip_update_camera(pt_tracer* tracer, const PtCamera* camera);
 /* IPathTracer::reset_accumulation, PathTracer.h:1099 (ScalarCpuPathTracer.cpp:669-688). */
 int pt_hip_reset_accumulation(pt_tracer* tracer);
 /* IPathTracer::render_tile, PathTracer.h:1141 (ScalarCpuPathTracer.Render.cpp:140-226). */
@@ -203,6 +225,14 @@ int pt_hip_render_tiles(pt_tracer* tracer, const PtRenderTile* tiles, uint32_t c
 /* Film readback: FilmBuffer::raw / sample_counts (PathTracer.h:955-958). Host pointers:
  * rgb_sum holds 3 floats per pixel, sample_count one u32 per pixel (either may be NULL). */
 int pt_hip_read_film(pt_tracer* tracer, float* rgb_sum, uint32_t* sample_count);
+/* IPathTracer::request_film_readback / poll_film, PathTracer.h:1172-1179: the request enqueues the GPU
+ * resolve of the film as it stands after all work queued so far (settings as pt_hip_resolve_ldr) and its
+ * copy into pinned host memory on a second stream, and returns at once; rendering queued afterwards
+ * overlaps the copy. poll_film sets *state to PT_READBACK_PENDING until the copy has landed, then to
+ * PT_READBACK_READY and copies the image (4 bytes per pixel) into rgba8 (may be NULL); a token other
+ * than the latest is PT_READBACK_INVALID. */
+int pt_hip_request_film_readback(pt_tracer* tracer, const PtFilmResolve* settings, PtFilmReadbackToken* token);
+int pt_hip_poll_film(pt_tracer* tracer, const PtFilmReadbackToken* token, uint8_t* rgba8, uint32_t* state);
 /* IPathTracer::replace_film_history, PathTracer.h:1096-1098: replaces the accumulated film (host
  * pointers, 3 floats and one count per pixel) without touching scene or acceleration data. */
 int pt_hip_replace_film(pt_tracer* tracer, const float* rgb_sum, const uint32_t* sample_count);
@@ -339,6 +369,8 @@ int pt_batch_render_adjoint(pt_batch* batch, uint64_t frame_index, const float* 
                             float* grad_emission, float* grad_light, float* grad_roughness, float* grad_texels);
 /* Film of one scene, as pt_hip_read_film. */
 int pt_batch_read_film(pt_batch* batch, uint32_t scene, float* rgb_sum, uint32_t* sample_count);
+/* Camera of one scene or view, as pt_hip_update_camera (graph replays see the new camera). */
+int pt_batch_update_camera(pt_batch* batch, uint32_t scene, const PtCamera* camera);
 /* Resolved image of one scene, as pt_hip_resolve_hdr / pt_hip_resolve_ldr. */
 int pt_batch_resolve_hdr(pt_batch* batch, uint32_t scene, float* rgb);
 int pt_batch_resolve_ldr(pt_batch* batch, uint32_t scene, const PtFilmResolve* settings, uint8_t* rgba8);

@@ -93,6 +93,13 @@ class PtFilmResolve(C.Structure):
                 ("output_transform", C.c_uint32), ("gamma", C.c_float), ("clamp_output", C.c_uint32)]
 
 
+class PtFilmReadbackToken(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("fence_value", C.c_uint64), ("width", C.c_uint32), ("height", C.c_uint32)]
+
+
+READBACK_STATES = ("invalid", "pending", "ready", "failed", "unsupported")  # FilmReadbackState
+
+
 def make_resolve(r: "_scene.FilmResolve | None") -> PtFilmResolve:
     """PtFilmResolve of a FilmResolve (None: FilmResolveSettings defaults)."""
     r = r or _scene.FilmResolve()
@@ -133,6 +140,10 @@ SIGNATURES = {
     "pt_hip_resolve_hdr": (C.c_int, [_P, _P]),
     "pt_hip_resolve_ldr": (C.c_int, [_P, C.POINTER(PtFilmResolve), _P]),
     "pt_hip_replace_film": (C.c_int, [_P, _P, _P]),
+    "pt_hip_request_film_readback": (C.c_int, [_P, C.POINTER(PtFilmResolve), C.POINTER(PtFilmReadbackToken)]),
+    "pt_hip_poll_film": (C.c_int, [_P, C.POINTER(PtFilmReadbackToken), _P, C.POINTER(C.c_uint32)]),
+    "pt_hip_update_camera": (C.c_int, [_P, C.POINTER(PtCamera)]),
+    "pt_hip_update_render_settings": (C.c_int, [_P, C.POINTER(PtRenderSettings)]),
     "pt_hip_read_counters": (C.c_int, [_P, C.POINTER(PtSampleCounters)]),
     "pt_hip_status": (C.c_int, [_P, C.POINTER(PtTracerStatus)]),
     "pt_hip_shutdown": (C.c_int, [_P]),
@@ -167,6 +178,7 @@ SIGNATURES = {
     "pt_batch_render": (C.c_int, [_P, C.c_uint64, C.c_int]),
     "pt_batch_read_film": (C.c_int, [_P, C.c_uint32, _P, _P]),
     "pt_batch_resolve_hdr": (C.c_int, [_P, C.c_uint32, _P]),
+    "pt_batch_update_camera": (C.c_int, [_P, C.c_uint32, C.POINTER(PtCamera)]),
     "pt_batch_resolve_ldr": (C.c_int, [_P, C.c_uint32, C.POINTER(PtFilmResolve), _P]),
     "pt_batch_read_counters": (C.c_int, [_P, C.POINTER(PtSampleCounters)]),
     "pt_batch_set_tuning": (C.c_int, [_P, C.POINTER(PtTuning)]),

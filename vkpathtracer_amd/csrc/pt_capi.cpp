@@ -132,6 +132,17 @@ struct pt_tracer {
   uint32_t capacity = kDefaultCapacity;
   uint32_t refill_min = kDefaultRefillMin;
 
+  // split film readback: the resolved image is copied on copy_stream into pinned host memory while
+  // the main stream goes on rendering; rb_copied marks the copy's completion
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t rb_resolved = nullptr, rb_copied = nullptr;
+  DevArray<uint32_t> d_readback;
+  uint8_t* rb_host = nullptr;
+  size_t rb_host_bytes = 0;
+  bool rb_inflight = false;
+  uint64_t rb_next_id = 1;
+  PtFilmReadbackToken rb_latest{};
+
   // pinned staging for host->device uploads that must follow the stream's earlier work
   void* stage = nullptr;
   size_t stage_bytes = 0;
@@ -1008,6 +1019,12 @@ void destroy_tracer(pt_tracer* t) {
   t->d_stack_t.release();
   t->d_plan.release();
   t->d_frame.release();
+  if (t->copy_stream) (void)hipStreamSynchronize(t->copy_stream);
+  t->d_readback.release();
+  if (t->rb_host) (void)hipHostFree(t->rb_host);
+  if (t->rb_resolved) (void)hipEventDestroy(t->rb_resolved);
+  if (t->rb_copied) (void)hipEventDestroy(t->rb_copied);
+  if (t->copy_stream) (void)hipStreamDestroy(t->copy_stream);
   if (t->stage) (void)hipHostFree(t->stage);
   if (t->stage_done) (void)hipEventDestroy(t->stage_done);
   if (t->stream) (void)hipStreamDestroy(t->stream);
@@ -1104,6 +1121,19 @@ int batch_film_range(pt_batch* b, uint32_t scene, const char* what, size_t& base
   base = 0;
   for (uint32_t i = 0; i < scene; ++i) base += static_cast<size_t>(t->scenes[i].width) * t->scenes[i].height;
   pixels = static_cast<size_t>(t->scenes[scene].width) * t->scenes[scene].height;
+  return PT_OK;
+}
+
+// IPathTracer::update_camera_state for scene slot `idx`: the slot's camera and, once the pool is on the
+// device, its scene record's camera (the geometry and BVH stay).
+int update_slot_camera(pt_tracer* t, uint32_t idx, const PtCamera& cam) {
+  SceneSlot& s = t->scenes[idx];
+  s.view.camera = cam;
+  if (!t->accel_valid) return PT_OK;
+  const ptd::DevCamera dc = make_camera(cam, s.width, s.height, t->settings.camera_aperture_radius,
+                                        t->settings.camera_focus_distance);
+  PT_HIP(t, hipStreamSynchronize(t->stream), "synchronize");
+  PT_HIP(t, hipMemcpy(&t->d_recs.ptr[idx].cam, &dc, sizeof(dc), hipMemcpyHostToDevice), "upload camera");
   return PT_OK;
 }
 
@@ -1479,6 +1509,109 @@ int pt_hip_trace_rays(pt_tracer* t, int mode, const float* rays, const float* tm
   if (rc != PT_OK) return rc;
   if (counters) to_counters(c, counters);
   return PT_OK;
+}
+
+int pt_hip_update_camera(pt_tracer* t, const PtCamera* camera) {
+  if (!t || !camera) return fail(t, PT_INVALID_ARGUMENT, "update_camera: null argument");
+  if (!t->configured || !t->scene_loaded || t->scenes.size() != 1u)
+    return fail(t, PT_NOT_READY, "update_camera_state requires a configured tracer with a scene");
+  (void)hipSetDevice(t->device);
+  const int rc = update_slot_camera(t, 0u, *camera);
+  if (rc != PT_OK) return rc;
+  ++t->scene_version;
+  t->last_error.clear();
+  return PT_OK;
+}
+
+int pt_hip_update_render_settings(pt_tracer* t, const PtRenderSettings* s) {
+  if (!t || !s) return fail(t, PT_INVALID_ARGUMENT, "update_render_settings: null argument");
+  if (!t->configured || t->batch) return fail(t, PT_NOT_READY, "update_render_settings requires a configured tracer");
+  if (s->width != t->settings.width || s->height != t->settings.height)
+    return fail(t, PT_UNSUPPORTED, "update_render_settings: a new resolution needs configure");
+  if (!(s->rr_min_survival == s->rr_min_survival) || !(s->rr_max_survival == s->rr_max_survival))
+    return fail(t, PT_INVALID_ARGUMENT, "update_render_settings: russian roulette bounds are NaN");
+  (void)hipSetDevice(t->device);
+  PT_HIP(t, hipStreamSynchronize(t->stream), "synchronize");
+  const bool lens = s->camera_aperture_radius != t->settings.camera_aperture_radius ||
+                    s->camera_focus_distance != t->settings.camera_focus_distance;
+  t->settings = *s;
+  t->settings.max_depth = std::max(1u, s->max_depth);
+  t->settings.camera_aperture_radius = std::max(0.0f, s->camera_aperture_radius);
+  t->settings.camera_focus_distance = std::max(0.0f, s->camera_focus_distance);
+  if (lens && t->scene_loaded) {
+    const int rc = update_slot_camera(t, 0u, t->scenes[0].view.camera);
+    if (rc != PT_OK) return rc;
+  }
+  ++t->scene_version;
+  t->last_error.clear();
+  return PT_OK;
+}
+
+int pt_hip_request_film_readback(pt_tracer* t, const PtFilmResolve* settings, PtFilmReadbackToken* token) {
+  if (!t || !token) return fail(t, PT_INVALID_ARGUMENT, "request_film_readback: null argument");
+  *token = PtFilmReadbackToken{};
+  if (!t->configured) return fail(t, PT_NOT_READY, "request_film_readback called before configure");
+  ptd::DevResolve rs{};
+  int rc = make_resolve(t, settings, rs);
+  if (rc != PT_OK) return rc;
+  (void)hipSetDevice(t->device);
+  const size_t pixels = static_cast<size_t>(t->settings.width) * t->settings.height;
+  if (pixels == 0 || pixels > 0xFFFFFFFFull) return fail(t, PT_INVALID_ARGUMENT, "request_film_readback: image size");
+  if (!t->copy_stream) {
+    PT_HIP(t, hipStreamCreateWithFlags(&t->copy_stream, hipStreamNonBlocking), "create copy stream");
+    PT_HIP(t, hipEventCreateWithFlags(&t->rb_resolved, hipEventDisableTiming), "create event");
+    PT_HIP(t, hipEventCreateWithFlags(&t->rb_copied, hipEventDisableTiming), "create event");
+  }
+  if (t->rb_host_bytes < 4 * pixels) {
+    PT_HIP(t, hipStreamSynchronize(t->copy_stream), "synchronize copy stream");
+    if (t->rb_host) (void)hipHostFree(t->rb_host);
+    t->rb_host = nullptr;
+    t->rb_host_bytes = 0;
+    PT_HIP(t, hipHostMalloc(reinterpret_cast<void**>(&t->rb_host), 4 * pixels, hipHostMallocDefault),
+           "allocate pinned readback buffer");
+    t->rb_host_bytes = 4 * pixels;
+  }
+  PT_HIP(t, t->d_readback.ensure(pixels), "allocate readback buffer");
+  // the previous request's copy reads d_readback: the new resolve waits for it
+  if (t->rb_inflight) PT_HIP(t, hipStreamWaitEvent(t->stream, t->rb_copied, 0), "order readback");
+  const ptd::DevFilm film{t->d_film_sum.ptr, t->d_film_count.ptr};
+  ptd::launch_resolve(t->stream, film, static_cast<uint32_t>(pixels), &rs, t->d_readback.ptr);
+  PT_HIP(t, hipGetLastError(), "launch resolve");
+  PT_HIP(t, hipEventRecord(t->rb_resolved, t->stream), "record event");
+  PT_HIP(t, hipStreamWaitEvent(t->copy_stream, t->rb_resolved, 0), "order readback");
+  PT_HIP(t, hipMemcpyAsync(t->rb_host, t->d_readback.ptr, 4 * pixels, hipMemcpyDeviceToHost, t->copy_stream),
+         "copy readback");
+  PT_HIP(t, hipEventRecord(t->rb_copied, t->copy_stream), "record event");
+  t->rb_inflight = true;
+  token->id = t->rb_next_id++;
+  token->fence_value = token->id;
+  token->width = t->settings.width;
+  token->height = t->settings.height;
+  t->rb_latest = *token;
+  t->last_error.clear();
+  return PT_OK;
+}
+
+int pt_hip_poll_film(pt_tracer* t, const PtFilmReadbackToken* token, uint8_t* rgba8, uint32_t* state) {
+  if (!t || !token || !state) return fail(t, PT_INVALID_ARGUMENT, "poll_film: null argument");
+  *state = PT_READBACK_INVALID;
+  if (token->id == 0 || token->id != t->rb_latest.id || token->width != t->rb_latest.width ||
+      token->height != t->rb_latest.height)
+    return PT_OK;
+  (void)hipSetDevice(t->device);
+  const hipError_t q = hipEventQuery(t->rb_copied);
+  if (q == hipErrorNotReady) {
+    *state = PT_READBACK_PENDING;
+    return PT_OK;
+  }
+  if (q != hipSuccess) {
+    *state = PT_READBACK_FAILED;
+    return fail(t, PT_INTERNAL_ERROR, std::string("poll_film: ") + hipGetErrorString(q));
+  }
+  t->rb_inflight = false;
+  if (rgba8) std::memcpy(rgba8, t->rb_host, 4ull * token->width * token->height);
+  *state = PT_READBACK_READY;
+  return PT_OK;  // no wait on the main stream: rendering queued after the request keeps running
 }
 
 int pt_hip_update_scene_delta(pt_tracer* t, const PtMaterial* materials, uint32_t material_count,
@@ -1859,6 +1992,17 @@ int pt_batch_read_film(pt_batch* b, uint32_t scene, float* rgb_sum, uint32_t* sa
   for (uint32_t i = 0; i < scene; ++i) base += static_cast<size_t>(t->scenes[i].width) * t->scenes[i].height;
   const SceneSlot& s = t->scenes[scene];
   return read_film_range(t, base, static_cast<size_t>(s.width) * s.height, rgb_sum, sample_count);
+}
+
+int pt_batch_update_camera(pt_batch* b, uint32_t scene, const PtCamera* camera) {
+  if (!b || !camera) return b ? fail(&b->t, PT_INVALID_ARGUMENT, "batch_update_camera: null argument") : PT_INVALID_ARGUMENT;
+  pt_tracer* t = &b->t;
+  if (scene >= t->scenes.size()) return fail(t, PT_INVALID_ARGUMENT, "batch_update_camera: no such scene");
+  (void)hipSetDevice(t->device);
+  const int rc = update_slot_camera(t, scene, *camera);
+  if (rc != PT_OK) return rc;
+  t->last_error.clear();
+  return PT_OK;
 }
 
 int pt_batch_resolve_hdr(pt_batch* b, uint32_t scene, float* rgb) {

@@ -16,6 +16,7 @@ import dataclasses
 import numpy as np
 
 from . import abi
+from .scene import Camera as _Camera
 from .scene import RenderSettings, Scene
 
 
@@ -77,6 +78,7 @@ class HipPathTracer:
         self._h = h
         self._settings: RenderSettings | None = None
         self._view: abi.SceneView | None = None
+        self._camera = None
         self.device = device
 
     # -- lifetime
@@ -118,7 +120,47 @@ class HipPathTracer:
             st = self._status(self._lib.pt_hip_load_textures(self._h, *view.texture_args()))
         if st:
             self._view = view
+            self._camera = scene.camera
         return st
+
+    def update_render_settings(self, settings: RenderSettings) -> bool:
+        """IPathTracer::update_render_settings: False when the backend needs configure (a new size)."""
+        s = abi.make_settings(settings)
+        ok = self._lib.pt_hip_update_render_settings(self._h, C.byref(s)) == 0
+        if ok:
+            self._settings = settings
+        return ok
+
+    def update_camera_state(self, camera) -> bool:
+        """IPathTracer::update_camera_state: a new camera (scene.Camera) without re-upload or rebuild."""
+        c = abi.make_camera(camera)
+        ok = self._lib.pt_hip_update_camera(self._h, C.byref(c)) == 0
+        if ok:
+            self._camera = camera
+        return ok
+
+    def update_camera(self, position, target, up, fov_deg: float) -> bool:
+        """IPathTracer::update_camera: pose and field of view; the lens fields stay."""
+        cam = dataclasses.replace(self._camera or _Camera(), position=tuple(position), target=tuple(target),
+                                  up=tuple(up), fov_deg=fov_deg)
+        return self.update_camera_state(cam)
+
+    def request_film_readback(self, settings=None) -> "abi.PtFilmReadbackToken":
+        """IPathTracer::request_film_readback: the LDR resolve of the film as queued so far, copied
+        to host memory while later rendering proceeds. Poll it with poll_film."""
+        tok = abi.PtFilmReadbackToken()
+        rs = abi.make_resolve(settings)
+        self._check(self._lib.pt_hip_request_film_readback(self._h, C.byref(rs), C.byref(tok)),
+                    "request_film_readback")
+        return tok
+
+    def poll_film(self, token) -> tuple[str, np.ndarray | None]:
+        """IPathTracer::poll_film: ("pending" | "ready" | "invalid" | "failed", (H, W, 4) u8 when ready)."""
+        state = C.c_uint32()
+        img = np.zeros((max(token.height, 1), max(token.width, 1), 4), np.uint8)
+        self._check(self._lib.pt_hip_poll_film(self._h, C.byref(token), img.ctypes.data, C.byref(state)), "poll_film")
+        name = abi.READBACK_STATES[state.value]
+        return name, (img if name == "ready" else None)
 
     def update_texels(self, texels) -> None:
         """Replace every texel of the built scene (same count): a (N, 3) float32 numpy array or a
@@ -361,6 +403,11 @@ class HipBatch:
             self._check(self._lib.pt_batch_load_textures(self._h, idx.value, *view.texture_args()), "load_textures")
         self.scenes.append((width, height, spp))
         return int(idx.value)
+
+    def update_camera(self, scene: int, camera) -> None:
+        """The camera of scene or view `scene` (scene.Camera); graph replays see it."""
+        c = abi.make_camera(camera)
+        self._check(self._lib.pt_batch_update_camera(self._h, scene, C.byref(c)), "update_camera")
 
     def add_view(self, scene: int, camera, width: int, height: int, spp: int) -> int:
         """Another camera on batch scene `scene`: shares its geometry, materials and textures, has its own
