@@ -138,6 +138,35 @@ enum {
   PT_READBACK_UNSUPPORTED = 4
 };
 
+/* RTInstanceTransformUpdate, PathTracer.h:298-306. 64 bytes. An instance_index out of range (e.g.
+ * 0xFFFFFFFF) selects the instance whose entity_id matches. */
+typedef struct PtInstanceTransformUpdate {
+  uint64_t entity_id;
+  uint32_t instance_index, flags, transform_revision;
+  float translation[3];
+  float rotation[4]; /* quaternion x, y, z, w */
+  float scale[3];
+  uint32_t pad0;
+} PtInstanceTransformUpdate;
+
+/* InstanceTransformPlan / InstanceTransformUpdateResult, PathTracer.h:565-615 (status values in
+ * InstanceTransformUpdateStatus order). 24 bytes. */
+typedef struct PtInstanceTransformResult {
+  uint32_t status, requested_count, matched_count, applied_count;
+  double refit_ms;
+} PtInstanceTransformResult;
+enum {
+  PT_XFORM_APPLIED_METADATA_ONLY = 0,
+  PT_XFORM_APPLIED_INSTANCE_BUFFER_ONLY = 1,
+  PT_XFORM_APPLIED_DYNAMIC_ACCEL_UPDATE = 2,
+  PT_XFORM_APPLIED_FULL_STATIC_ACCEL_REBUILD = 3,
+  PT_XFORM_APPLIED_FULL_SCENE_RELOAD = 4,
+  PT_XFORM_BLOCKED_NEEDS_FULL_STATIC_ACCEL_REBUILD = 5,
+  PT_XFORM_BLOCKED_NEEDS_FULL_SCENE_RELOAD = 6,
+  PT_XFORM_UNSUPPORTED = 7,
+  PT_XFORM_FAILED = 8
+};
+
 /* RenderTile, PathTracer.h:919-927. One tile adds one sample (sample_index) to each pixel. */
 typedef struct PtRenderTile {
   uint32_t x, y, width, height, sample_index, tile_id, gpu_id;
@@ -215,6 +244,16 @@ int pt_hip_update_render_settings(pt_tracer* tracer, const PtRenderSettings* set
  * the scene's camera changes without re-uploading geometry or rebuilding the BVH. The accumulated film
  * is kept (the caller resets it, as with the reference). */
 int pt_hip_update_camera(pt_tracer* tracer, const PtCamera* camera);
+/* IPathTracer / IRayAccelerator plan_instance_transform_update and apply_instance_transform_update,
+ * PathTracer.h:1009-1032, :1115-1137 (CpuBvhAccelerator.cpp:265-358): the plan is a pure query; apply
+ * recomputes the updated dynamic instances' triangles from their local geometry on the GPU and refits
+ * every node box bottom-up (the topology stays), then renders see the moved geometry. Instances
+ * without dynamic local geometry block with PT_XFORM_BLOCKED_NEEDS_FULL_STATIC_ACCEL_REBUILD (the
+ * caller rebuilds). The call returns PT_OK with the outcome in the result's status. */
+int pt_hip_plan_instance_transform_update(pt_tracer* tracer, const PtInstanceTransformUpdate* updates,
+                                          uint32_t count, PtInstanceTransformResult* plan);
+int pt_hip_apply_instance_transform_update(pt_tracer* tracer, const PtInstanceTransformUpdate* updates,
+                                           uint32_t count, PtInstanceTransformResult* result);
 /* IPathTracer::reset_accumulation, PathTracer.h:1099 (ScalarCpuPathTracer.cpp:669-688). */
 int pt_hip_reset_accumulation(pt_tracer* tracer);
 /* IPathTracer::render_tile, PathTracer.h:1141 (ScalarCpuPathTracer.Render.cpp:140-226). */
@@ -396,7 +435,8 @@ int pt_bvh_build_host(const PtSceneView* scene, const PtBvhParams* params, uint3
 
 /* ABI sizes, for bindings to check their mirrors of the records above. */
 uint32_t pt_hip_sizeof(int which); /* 0 material, 1 light, 2 instance, 3 camera, 4 scene, 5 settings, 6 counters,
-                                      7 tile, 8 tuning, 9 texture, 10 film resolve */
+                                      7 tile, 8 tuning, 9 texture, 10 film resolve,
+                                      11 instance transform update */
 const char* pt_hip_version(void);
 
 #ifdef __cplusplus

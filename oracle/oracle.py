@@ -90,6 +90,7 @@ _SIGS = {
     "pto_render_tile": (C.c_int, [C.POINTER(PtoScene), _P, C.POINTER(PtoSettings), C.c_uint32, C.c_uint32, C.c_uint32,
                                   C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.c_int, C.c_int, _P, _P,
                                   C.POINTER(PtoCounters)]),
+    "pto_accel_refit": (C.c_int, [_P, C.POINTER(PtoScene)]),
     "pto_color_temperature_rgb": (None, [C.c_float, _P]),
     "pto_white_balance_scale": (None, [C.c_float, _P]),
     "pto_resolve_hdr": (None, [_P, _P, C.c_uint64, _P]),
@@ -195,6 +196,14 @@ class Oracle:
         if getattr(self, "accel", None):
             self.L.pto_accel_free(self.accel)
             self.accel = None
+
+    def refit(self, scene) -> None:
+        """CpuBvhAccelerator's dynamic-instance refit (pto_accel_refit): `scene` is this scene with
+        updated instance transforms; primitives and node boxes follow them, the topology stays."""
+        self.scene = OracleScene(scene)
+        rc = self.L.pto_accel_refit(self.accel, C.byref(self.scene.s))
+        if rc != 0:
+            raise RuntimeError(f"oracle refit failed ({rc})")
 
     def texture_sample(self, tex: int, prim: int, bu: float, bv: float):
         """pto_texture_sample: (rgb (3,), texel indices (4,), weights (4,)) of one bilinear tap."""

@@ -493,6 +493,33 @@ static uint32_t sah_build(sah_ctx* c, uint32_t start, uint32_t count, uint32_t d
   return ni;
 }
 
+#if PTO_AVX2
+/* structure-of-arrays copies of every leaf's triangles, 8 per batch, unused lanes zero */
+static void soa_fill(pto_accel* a) {
+  free(a->soa);
+  free(a->soa_base);
+  uint64_t batches = 0;
+  a->soa_base = (uint32_t*)calloc(a->node_count ? a->node_count : 1u, sizeof(uint32_t));
+  for (uint32_t i = 0; i < a->node_count; ++i)
+    if (a->nodes[i].count > 0) {
+      a->soa_base[i] = (uint32_t)batches;
+      batches += ((uint32_t)a->nodes[i].count + 7u) / 8u;
+    }
+  a->soa = (float*)aligned_alloc(32, sizeof(float) * 72u * (size_t)(batches ? batches : 1u));
+  memset(a->soa, 0, sizeof(float) * 72u * (size_t)(batches ? batches : 1u));
+  for (uint32_t i = 0; i < a->node_count; ++i) {
+    const node_t* nd = &a->nodes[i];
+    for (int32_t k = 0; k < nd->count; ++k) {
+      const prim_t* p = &a->prims[a->order[nd->first + k]];
+      float* b = a->soa + 72u * ((size_t)a->soa_base[i] + (uint32_t)k / 8u);
+      const uint32_t lane = (uint32_t)k & 7u;
+      const float f[9] = {p->v0.x, p->v0.y, p->v0.z, p->e1.x, p->e1.y, p->e1.z, p->e2.x, p->e2.y, p->e2.z};
+      for (int c = 0; c < 9; ++c) b[8 * c + lane] = f[c];
+    }
+  }
+}
+#endif
+
 pto_accel* pto_accel_build(const PtoScene* scene, int builder, uint32_t leaf_size, uint32_t bucket_count) {
   pto_accel* a = (pto_accel*)calloc(1, sizeof(pto_accel));
   if (!a) return NULL;
@@ -536,26 +563,7 @@ pto_accel* pto_accel_build(const PtoScene* scene, int builder, uint32_t leaf_siz
   }
   free(raw);
 #if PTO_AVX2
-  /* structure-of-arrays copies of every leaf's triangles, 8 per batch, unused lanes zero */
-  uint64_t batches = 0;
-  a->soa_base = (uint32_t*)calloc(a->node_count ? a->node_count : 1u, sizeof(uint32_t));
-  for (uint32_t i = 0; i < a->node_count; ++i)
-    if (a->nodes[i].count > 0) {
-      a->soa_base[i] = (uint32_t)batches;
-      batches += ((uint32_t)a->nodes[i].count + 7u) / 8u;
-    }
-  a->soa = (float*)aligned_alloc(32, sizeof(float) * 72u * (size_t)(batches ? batches : 1u));
-  memset(a->soa, 0, sizeof(float) * 72u * (size_t)(batches ? batches : 1u));
-  for (uint32_t i = 0; i < a->node_count; ++i) {
-    const node_t* nd = &a->nodes[i];
-    for (int32_t k = 0; k < nd->count; ++k) {
-      const prim_t* p = &a->prims[a->order[nd->first + k]];
-      float* b = a->soa + 72u * ((size_t)a->soa_base[i] + (uint32_t)k / 8u);
-      const uint32_t lane = (uint32_t)k & 7u;
-      const float f[9] = {p->v0.x, p->v0.y, p->v0.z, p->e1.x, p->e1.y, p->e1.z, p->e2.x, p->e2.y, p->e2.z};
-      for (int c = 0; c < 9; ++c) b[8 * c + lane] = f[c];
-    }
-  }
+  soa_fill(a);
 #endif
   return a;
 }
@@ -607,6 +615,63 @@ int pto_accel_set_textures(pto_accel* a, const float* texcoords, uint32_t texcoo
   a->texel_count = texel_count;
   return 0;
 }
+
+/* ------------------------------------------------------------------ dynamic-instance refit
+ * CpuBvhAccelerator::apply_instance_transform_update (CpuBvhAccelerator.cpp:310-358): the primitives of
+ * the updated instances are recomputed from their local geometry through the new transforms
+ * (refit_instance_primitives, :527-584), then every node box is refit bottom-up as the union of its
+ * children's, leaves from their padded primitive boxes (refit_node, :586-603). The topology stays.
+ * Here the scene passed in already holds the updated instance records; re-flattening it recomputes
+ * exactly the updated instances' primitives (the others come out as they were). */
+static aabb_t union_box(aabb_t l, aabb_t r) {
+  aabb_t o;
+  for (int k = 0; k < 3; ++k) {
+    o.mn[k] = smin(l.mn[k], r.mn[k]);
+    o.mx[k] = smax(l.mx[k], r.mx[k]);
+  }
+  return o;
+}
+
+static aabb_t refit_node(pto_accel* a, const aabb_t* boxes, int32_t ni) {
+  node_t* nd = &a->nodes[ni];
+  if (nd->count > 0) {
+    aabb_t b = boxes[a->order[nd->first]];
+    for (int32_t i = 1; i < nd->count; ++i) b = union_box(b, boxes[a->order[nd->first + i]]);
+    nd->box = b;
+    return b;
+  }
+  const aabb_t l = refit_node(a, boxes, nd->left);
+  const aabb_t r = refit_node(a, boxes, nd->right);
+  nd->box = union_box(l, r);
+  return nd->box;
+}
+
+int pto_accel_refit(pto_accel* a, const PtoScene* scene) {
+  if (!a) return -1;
+  prim_t* prims = NULL;
+  uint32_t n = 0;
+  v3* raw = NULL;
+  if (flatten(scene, &prims, &n, &raw) != 0) return -1;
+  if (n != a->prim_count) {
+    free(prims);
+    free(raw);
+    return -2;
+  }
+  free(a->prims);
+  a->prims = prims;
+  if (a->node_count) {
+    aabb_t* boxes = (aabb_t*)malloc(sizeof(aabb_t) * n);
+    for (uint32_t i = 0; i < n; ++i) boxes[i] = padded_box(raw + 3 * i);
+    refit_node(a, boxes, 0);
+    free(boxes);
+  }
+  free(raw);
+#if PTO_AVX2
+  soa_fill(a);
+#endif
+  return 0;
+}
+
 uint32_t pto_accel_node_count(const pto_accel* a) { return a ? a->node_count : 0; }
 uint32_t pto_accel_prim_count(const pto_accel* a) { return a ? a->prim_count : 0; }
 uint32_t pto_accel_max_depth(const pto_accel* a) { return a ? a->max_depth : 0; }

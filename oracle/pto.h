@@ -171,6 +171,11 @@ int pto_adjoint_pixels(const PtoScene* scene, const pto_accel* a, const PtoSetti
 int pto_phong_samples(float exponent, const float refl[3], const float n[3], float ps, uint64_t state,
                       uint32_t count, float* out_dirs, double* out_score);
 
+/* CpuBvhAccelerator's dynamic-instance refit (CpuBvhAccelerator.cpp:310-358, :527-603): scene holds the
+ * updated instance transforms; primitives are recomputed and node boxes refit bottom-up, the topology
+ * kept. Returns -2 when the primitive count changed (a rebuild is needed). */
+int pto_accel_refit(pto_accel* a, const PtoScene* scene);
+
 /* FilmResolveSettings (PathTracer.h:67-74) after the host-side conversions: white-balance scale
  * (WhiteBalanceScale) and inverse gamma 1 / max(0.01, gamma). tone_map: 0 linear, 1 reinhard,
  * 2 filmic_approx, 3 aces_approx (ToneMapMode); output_transform: 0 gamma, 1 linear. */

@@ -39,6 +39,8 @@ def test_record_sizes_match_everywhere():
     assert lib.pt_hip_sizeof(7) == C.sizeof(abi.PtRenderTile) == 28
     assert lib.pt_hip_sizeof(8) == C.sizeof(abi.PtTuning) == 16
     assert lib.pt_hip_sizeof(10) == C.sizeof(abi.PtFilmResolve) == 24
+    assert lib.pt_hip_sizeof(11) == C.sizeof(abi.PtInstanceTransformUpdate) == 64
+    assert C.sizeof(abi.PtInstanceTransformResult) == 24
     assert C.sizeof(O.PtoScene) == C.sizeof(abi.PtSceneView)
     assert C.sizeof(O.PtoSettings) == C.sizeof(abi.PtRenderSettings) == 56
 

@@ -100,6 +100,38 @@ class PtFilmReadbackToken(C.Structure):
 READBACK_STATES = ("invalid", "pending", "ready", "failed", "unsupported")  # FilmReadbackState
 
 
+class PtInstanceTransformUpdate(C.Structure):
+    _fields_ = [("entity_id", C.c_uint64), ("instance_index", C.c_uint32), ("flags", C.c_uint32),
+                ("transform_revision", C.c_uint32), ("translation", C.c_float * 3), ("rotation", C.c_float * 4),
+                ("scale", C.c_float * 3), ("pad0", C.c_uint32)]
+
+
+class PtInstanceTransformResult(C.Structure):
+    _fields_ = [("status", C.c_uint32), ("requested_count", C.c_uint32), ("matched_count", C.c_uint32),
+                ("applied_count", C.c_uint32), ("refit_ms", C.c_double)]
+
+
+# InstanceTransformUpdateStatus (PathTracer.h:565-575)
+TRANSFORM_STATUS = ("applied_metadata_only", "applied_instance_buffer_only", "applied_dynamic_accel_update",
+                    "applied_full_static_accel_rebuild", "applied_full_scene_reload",
+                    "blocked_needs_full_static_accel_rebuild", "blocked_needs_full_scene_reload", "unsupported",
+                    "failed")
+
+
+def make_transform_updates(updates) -> C.Array:
+    """PtInstanceTransformUpdate array of scene.InstanceTransformUpdate records."""
+    arr = (PtInstanceTransformUpdate * max(1, len(updates)))()
+    for a, u in zip(arr, updates):
+        a.entity_id = u.entity_id
+        a.instance_index = u.instance_index
+        a.flags = u.flags
+        a.transform_revision = u.transform_revision
+        a.translation[:] = list(u.translation)
+        a.rotation[:] = list(u.rotation)
+        a.scale[:] = list(u.scale)
+    return arr
+
+
 def make_resolve(r: "_scene.FilmResolve | None") -> PtFilmResolve:
     """PtFilmResolve of a FilmResolve (None: FilmResolveSettings defaults)."""
     r = r or _scene.FilmResolve()
@@ -143,6 +175,8 @@ SIGNATURES = {
     "pt_hip_request_film_readback": (C.c_int, [_P, C.POINTER(PtFilmResolve), C.POINTER(PtFilmReadbackToken)]),
     "pt_hip_poll_film": (C.c_int, [_P, C.POINTER(PtFilmReadbackToken), _P, C.POINTER(C.c_uint32)]),
     "pt_hip_update_camera": (C.c_int, [_P, C.POINTER(PtCamera)]),
+    "pt_hip_plan_instance_transform_update": (C.c_int, [_P, _P, C.c_uint32, C.POINTER(PtInstanceTransformResult)]),
+    "pt_hip_apply_instance_transform_update": (C.c_int, [_P, _P, C.c_uint32, C.POINTER(PtInstanceTransformResult)]),
     "pt_hip_update_render_settings": (C.c_int, [_P, C.POINTER(PtRenderSettings)]),
     "pt_hip_read_counters": (C.c_int, [_P, C.POINTER(PtSampleCounters)]),
     "pt_hip_status": (C.c_int, [_P, C.POINTER(PtTracerStatus)]),

@@ -238,7 +238,7 @@ struct MidBuilder {
 }  // namespace
 
 bool flatten(const PtSceneView& sc, std::vector<Prim>& prims, std::vector<float>& raw, std::string& err,
-             const float* texcoords) {
+             const float* texcoords, std::vector<uint32_t>* inst_first) {
   uint64_t cap = 0;
   for (uint32_t i = 0; i < sc.instance_count; ++i) cap += sc.instances[i].triangle_count;
   if (cap >= kMaxPrimitives) {
@@ -249,7 +249,9 @@ bool flatten(const PtSceneView& sc, std::vector<Prim>& prims, std::vector<float>
   raw.clear();
   prims.reserve(cap);
   raw.reserve(cap * 9u);
+  if (inst_first) inst_first->clear();
   for (uint32_t ii = 0; ii < sc.instance_count; ++ii) {
+    if (inst_first) inst_first->push_back(static_cast<uint32_t>(prims.size()));
     const PtInstance& inst = sc.instances[ii];
     const bool use_local = (inst.flags & 1u) != 0u && inst.local_vertex_count > 0u && inst.local_index_count > 0u;
     for (uint32_t tri = 0; tri < inst.triangle_count; ++tri) {
@@ -302,6 +304,7 @@ bool flatten(const PtSceneView& sc, std::vector<Prim>& prims, std::vector<float>
       raw.insert(raw.end(), r9, r9 + 9);
     }
   }
+  if (inst_first) inst_first->push_back(static_cast<uint32_t>(prims.size()));
   return true;
 }
 

@@ -32,9 +32,11 @@ struct Bvh {
 
 // Flatten instances into primitives; raw receives the 3 unpadded vertices of each primitive.
 // texcoords (2 floats per vertex of scene.vertices, may be null) give the primitives of indexed
-// geometry their texture coordinates. Returns false (with err set) on a malformed snapshot.
+// geometry their texture coordinates. inst_first (may be null) receives each instance's first
+// primitive and, last, the primitive count (CpuBvhAccelerator's instance ranges). Returns false (with
+// err set) on a malformed snapshot.
 bool flatten(const PtSceneView& scene, std::vector<Prim>& prims, std::vector<float>& raw, std::string& err,
-             const float* texcoords = nullptr);
+             const float* texcoords = nullptr, std::vector<uint32_t>* inst_first = nullptr);
 
 // Bucketed SAH (D3D12GpuPathTracer.SceneBvh.cpp:73-238) and midpoint (ParallelBvhBuilder.cpp:61-163).
 void build_sah(Bvh& bvh, const std::vector<float>& raw, uint32_t leaf_size, uint32_t buckets);

@@ -18,6 +18,8 @@
 #include "vkpt_hip.h"
 
 static_assert(sizeof(PtFilmResolve) == 24, "PtFilmResolve layout");
+static_assert(sizeof(PtInstanceTransformUpdate) == 64, "PtInstanceTransformUpdate layout");
+static_assert(sizeof(PtInstanceTransformResult) == 24, "PtInstanceTransformResult layout");
 
 namespace {
 
@@ -76,6 +78,7 @@ struct SceneSlot {
   PtBvhParams bvh_params{PT_BVH_SAH, 4u, 8u, 0u};
   ptb::Bvh bvh;
   ptb::DeviceLayout layout;
+  std::vector<uint32_t> inst_first;  // per instance its first primitive, then the primitive count
   PtBuildInfo info{};
   std::string error;
 };
@@ -131,6 +134,14 @@ struct pt_tracer {
   uint32_t path_capacity_alloc = 0;
   uint32_t capacity = kDefaultCapacity;
   uint32_t refill_min = kDefaultRefillMin;
+
+  // dynamic-instance refit: padded triangle boxes in leaf order, local geometry, inner nodes ordered by
+  // height (refit_level[h] = first node of height h + 1), updated instance records, non-finite flag
+  bool refit_ready = false;
+  DevArray<float> d_tri_box, d_local_v;
+  DevArray<uint32_t> d_local_i, d_leaf_pos, d_refit_order, d_refit_flag;
+  DevArray<ptd::RefitInst> d_refit_insts;
+  std::vector<uint32_t> refit_level;
 
   // split film readback: the resolved image is copied on copy_stream into pinned host memory while
   // the main stream goes on rendering; rb_copied marks the copy's completion
@@ -296,6 +307,14 @@ int read_counter_block(pt_tracer* t, const unsigned long long* dev, unsigned lon
 }
 
 void free_scene_device(pt_tracer* t) {
+  t->refit_ready = false;
+  t->d_tri_box.release();
+  t->d_local_v.release();
+  t->d_local_i.release();
+  t->d_leaf_pos.release();
+  t->d_refit_order.release();
+  t->d_refit_flag.release();
+  t->d_refit_insts.release();
   drop_graph(t);
   t->d_nodes.release();
   t->d_tris.release();
@@ -497,7 +516,8 @@ bool build_scene(SceneSlot& s) {
   const auto t0 = std::chrono::steady_clock::now();
   std::vector<float> raw;
   s.error.clear();
-  if (!ptb::flatten(s.view, s.bvh.prims, raw, s.error, s.texcoords.empty() ? nullptr : s.texcoords.data()))
+  if (!ptb::flatten(s.view, s.bvh.prims, raw, s.error, s.texcoords.empty() ? nullptr : s.texcoords.data(),
+                    &s.inst_first))
     return false;
   if (s.bvh_params.builder == PT_BVH_SAH) ptb::build_sah(s.bvh, raw, s.bvh_params.leaf_size, s.bvh_params.bucket_count);
   else ptb::build_midpoint(s.bvh, raw, s.bvh_params.leaf_size);
@@ -567,6 +587,7 @@ std::vector<PtMaterial> pool_materials(const SceneSlot& s, uint32_t tex_base) {
 // records. Films are laid out scene after scene (film_base); a tracer's single scene starts at
 // pixel 0. A view shares its scene's pool entries and has its own camera and film.
 int upload_pool(pt_tracer* t) {
+  t->refit_ready = false;
   PT_HIP(t, hipStreamSynchronize(t->stream), "synchronize");
   free_scene_device(t);
   size_t n_nodes = 0, n_tris = 0, n_prims = 0, n_mats = 0, n_lights = 0, n_env = 0, n_tex = 0, n_texels = 0;
@@ -1137,6 +1158,118 @@ int update_slot_camera(pt_tracer* t, uint32_t idx, const PtCamera& cam) {
   return PT_OK;
 }
 
+// resolve_instance_index (CpuBvhAccelerator.cpp:507-525): the update's instance, by index or entity id.
+bool resolve_instance(const SceneSlot& s, const PtInstanceTransformUpdate& u, uint32_t& out) {
+  uint32_t i = u.instance_index;
+  if (i >= s.instances.size() && u.entity_id != 0u)
+    for (size_t k = 0; k < s.instances.size(); ++k)
+      if (s.instances[k].entity_id == u.entity_id) {
+        i = static_cast<uint32_t>(k);
+        break;
+      }
+  if (i >= s.instances.size()) return false;
+  out = i;
+  return true;
+}
+
+// plan_instance_transform_update (CpuBvhAccelerator.cpp:265-306).
+PtInstanceTransformResult plan_refit(const pt_tracer* t, const PtInstanceTransformUpdate* ups, uint32_t n,
+                                     const char*& message) {
+  PtInstanceTransformResult r{PT_XFORM_FAILED, n, 0u, 0u, 0.0};
+  if (!t->accel_valid || t->batch || t->scenes.size() != 1u || t->scenes[0].layout.tris.empty()) {
+    message = "the BVH is not built";
+    return r;
+  }
+  const SceneSlot& s = t->scenes[0];
+  for (uint32_t k = 0; k < n; ++k) {
+    uint32_t i = 0;
+    if (!resolve_instance(s, ups[k], i)) {
+      message = "transform update references an unknown instance";
+      return r;
+    }
+    const PtInstance& inst = s.instances[i];
+    if ((inst.flags & 1u) == 0u || inst.local_vertex_count == 0u || inst.local_index_count == 0u ||
+        i + 1u >= s.inst_first.size()) {
+      r.status = PT_XFORM_BLOCKED_NEEDS_FULL_STATIC_ACCEL_REBUILD;
+      message = "transform update needs dynamic local geometry metadata";
+      return r;
+    }
+    ++r.matched_count;
+  }
+  r.status = PT_XFORM_APPLIED_DYNAMIC_ACCEL_UPDATE;
+  message = "the BVH can refit dynamic instance transforms";
+  return r;
+}
+
+// One-time set-up of the refit data: padded boxes of every triangle in leaf order (from a flattening of
+// the scene as it now stands, which is what the device holds), the local geometry, and the inner nodes
+// grouped by height so that each launch only reads children already refit.
+int prepare_refit(pt_tracer* t) {
+  SceneSlot& s = t->scenes[0];
+  std::vector<ptb::Prim> prims;
+  std::vector<float> raw;
+  std::string err;
+  if (!ptb::flatten(s.view, prims, raw, err, nullptr, nullptr)) return fail(t, PT_INTERNAL_ERROR, "refit: " + err);
+  const size_t n = prims.size();
+  if (n != s.layout.prim_tri.size()) return fail(t, PT_INTERNAL_ERROR, "refit: primitive count changed");
+  std::vector<float> box(6 * n);
+  auto mn3 = [](float a, float b, float c) { float m = a; if (b < m) m = b; if (c < m) m = c; return m; };
+  auto mx3 = [](float a, float b, float c) { float m = a; if (m < b) m = b; if (m < c) m = c; return m; };
+  for (size_t pid = 0; pid < n; ++pid) {
+    const float* v = &raw[9 * pid];
+    float* b = &box[6u * s.layout.prim_tri[pid]];
+    for (int k = 0; k < 3; ++k) {
+      b[k] = mn3(v[k], v[3 + k], v[6 + k]) - 1e-4f;
+      b[3 + k] = mx3(v[k], v[3 + k], v[6 + k]) + 1e-4f;
+    }
+  }
+  // node heights: children before parents (inner nodes are numbered in pre-order, so a reverse sweep
+  // sees every child first)
+  const uint32_t inner = s.layout.inner_count;
+  std::vector<uint32_t> height(inner, 0);
+  uint32_t max_h = 0;
+  for (uint32_t i = inner; i-- > 0;) {
+    int32_t ref[2];
+    std::memcpy(ref, &s.layout.nodes[16u * i + 12u], sizeof(ref));
+    uint32_t h = 1;
+    for (int c = 0; c < 2; ++c)
+      if (ref[c] >= 0) h = std::max(h, height[static_cast<uint32_t>(ref[c])] + 1u);
+    height[i] = h;
+    max_h = std::max(max_h, h);
+  }
+  t->refit_level.assign(max_h + 1u, 0u);
+  for (uint32_t i = 0; i < inner; ++i) ++t->refit_level[height[i]];
+  uint32_t acc = 0;  // refit_level[h - 1] = first slot of height h after the prefix sum
+  for (uint32_t h = 1; h <= max_h; ++h) {
+    const uint32_t c = t->refit_level[h];
+    t->refit_level[h - 1] = acc;
+    acc += c;
+  }
+  t->refit_level[max_h] = acc;
+  std::vector<uint32_t> order(inner), fill(t->refit_level.begin(), t->refit_level.end());
+  for (uint32_t i = 0; i < inner; ++i) order[fill[height[i] - 1u]++] = i;
+  PT_HIP(t, t->d_tri_box.ensure(box.size()), "allocate refit boxes");
+  PT_HIP(t, hipMemcpy(t->d_tri_box.ptr, box.data(), box.size() * sizeof(float), hipMemcpyHostToDevice), "upload boxes");
+  PT_HIP(t, t->d_leaf_pos.ensure(n), "allocate refit map");
+  PT_HIP(t, hipMemcpy(t->d_leaf_pos.ptr, s.layout.prim_tri.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice),
+         "upload refit map");
+  PT_HIP(t, t->d_refit_order.ensure(order.size()), "allocate refit order");
+  if (!order.empty())
+    PT_HIP(t, hipMemcpy(t->d_refit_order.ptr, order.data(), order.size() * sizeof(uint32_t), hipMemcpyHostToDevice),
+           "upload refit order");
+  PT_HIP(t, t->d_local_v.ensure(s.local_vertices.size()), "allocate local vertices");
+  if (!s.local_vertices.empty())
+    PT_HIP(t, hipMemcpy(t->d_local_v.ptr, s.local_vertices.data(), s.local_vertices.size() * sizeof(float),
+                        hipMemcpyHostToDevice), "upload local vertices");
+  PT_HIP(t, t->d_local_i.ensure(s.local_indices.size()), "allocate local indices");
+  if (!s.local_indices.empty())
+    PT_HIP(t, hipMemcpy(t->d_local_i.ptr, s.local_indices.data(), s.local_indices.size() * sizeof(uint32_t),
+                        hipMemcpyHostToDevice), "upload local indices");
+  PT_HIP(t, t->d_refit_flag.ensure(1), "allocate refit flag");
+  t->refit_ready = true;
+  return PT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1156,6 +1289,7 @@ uint32_t pt_hip_sizeof(int which) {
     case 8: return sizeof(PtTuning);
     case 9: return sizeof(PtTextureDesc);
     case 10: return sizeof(PtFilmResolve);
+    case 11: return sizeof(PtInstanceTransformUpdate);
     default: return 0;
   }
 }
@@ -1612,6 +1746,123 @@ int pt_hip_poll_film(pt_tracer* t, const PtFilmReadbackToken* token, uint8_t* rg
   if (rgba8) std::memcpy(rgba8, t->rb_host, 4ull * token->width * token->height);
   *state = PT_READBACK_READY;
   return PT_OK;  // no wait on the main stream: rendering queued after the request keeps running
+}
+
+int pt_hip_plan_instance_transform_update(pt_tracer* t, const PtInstanceTransformUpdate* updates, uint32_t count,
+                                          PtInstanceTransformResult* plan) {
+  if (!t || !plan || (count && !updates)) return fail(t, PT_INVALID_ARGUMENT, "plan_instance_transform_update: null argument");
+  const char* message = "";
+  *plan = plan_refit(t, updates, count, message);
+  if (plan->status != PT_XFORM_APPLIED_DYNAMIC_ACCEL_UPDATE) t->last_error = message;
+  return PT_OK;
+}
+
+int pt_hip_apply_instance_transform_update(pt_tracer* t, const PtInstanceTransformUpdate* updates, uint32_t count,
+                                           PtInstanceTransformResult* result) {
+  if (!t || !result || (count && !updates))
+    return fail(t, PT_INVALID_ARGUMENT, "apply_instance_transform_update: null argument");
+  const char* message = "";
+  *result = plan_refit(t, updates, count, message);
+  if (result->status != PT_XFORM_APPLIED_DYNAMIC_ACCEL_UPDATE) {
+    t->last_error = message;
+    return PT_OK;
+  }
+  const auto t0 = std::chrono::steady_clock::now();
+  (void)hipSetDevice(t->device);
+  PT_HIP(t, hipStreamSynchronize(t->stream), "synchronize");
+  if (!t->refit_ready) {
+    const int rc = prepare_refit(t);
+    if (rc != PT_OK) {
+      result->status = PT_XFORM_FAILED;
+      return rc;
+    }
+  }
+  SceneSlot& s = t->scenes[0];
+  // the instances touched, each once; refit_instance_primitives' checks (CpuBvhAccelerator.cpp:531-566)
+  // run on all of them before anything changes
+  std::vector<int32_t> slot_of(s.instances.size(), -1);
+  std::vector<uint32_t> inst_of;
+  for (uint32_t k = 0; k < count; ++k) {
+    uint32_t i = 0;
+    (void)resolve_instance(s, updates[k], i);
+    if (slot_of[i] < 0) {
+      slot_of[i] = static_cast<int32_t>(inst_of.size());
+      inst_of.push_back(i);
+    }
+  }
+  for (uint32_t i : inst_of) {
+    const PtInstance& inst = s.instances[i];
+    bool ok = s.inst_first[i + 1u] - s.inst_first[i] == inst.triangle_count;
+    const uint64_t local_end = static_cast<uint64_t>(inst.local_first_vertex) + inst.local_vertex_count;
+    for (uint32_t tri = 0; ok && tri < inst.triangle_count; ++tri) {
+      const uint64_t base = static_cast<uint64_t>(inst.local_first_index) + 3u * tri;
+      if (base + 2u >= s.local_indices.size()) {
+        ok = false;
+        break;
+      }
+      for (int c = 0; c < 3; ++c) {
+        const uint64_t v = static_cast<uint64_t>(inst.local_first_vertex) + s.local_indices[base + c];
+        if (v >= local_end || 3u * v + 2u >= s.local_vertices.size()) ok = false;
+      }
+    }
+    if (!ok) {
+      result->status = PT_XFORM_FAILED;
+      t->last_error = "failed while refitting instance primitives";
+      return PT_OK;
+    }
+  }
+  // ApplyInstanceTransformUpdates on the slot's records, in update order (a later update of the same
+  // instance wins, as sequential refits leave it); the next full build sees them too
+  for (uint32_t k = 0; k < count; ++k) {
+    uint32_t i = 0;
+    (void)resolve_instance(s, updates[k], i);
+    PtInstance& inst = s.instances[i];
+    std::memcpy(inst.translation, updates[k].translation, sizeof(inst.translation));
+    std::memcpy(inst.rotation, updates[k].rotation, sizeof(inst.rotation));
+    std::memcpy(inst.scale, updates[k].scale, sizeof(inst.scale));
+    inst.flags |= updates[k].flags;
+    if (updates[k].transform_revision != 0u) inst.transform_revision = updates[k].transform_revision;
+  }
+  std::vector<ptd::RefitInst> recs(inst_of.size());
+  uint32_t tri_total = 0;
+  for (size_t r = 0; r < recs.size(); ++r) {
+    const PtInstance& inst = s.instances[inst_of[r]];
+    ptd::RefitInst& R = recs[r];
+    R.prim_first = s.inst_first[inst_of[r]];
+    R.tri_count = inst.triangle_count;
+    R.local_first_vertex = inst.local_first_vertex;
+    R.local_vertex_count = inst.local_vertex_count;
+    R.local_first_index = inst.local_first_index;
+    R.tri_begin = tri_total;
+    std::memcpy(R.t, inst.translation, sizeof(R.t));
+    std::memcpy(R.q, inst.rotation, sizeof(R.q));
+    std::memcpy(R.s, inst.scale, sizeof(R.s));
+    tri_total += inst.triangle_count;
+  }
+  PT_HIP(t, t->d_refit_insts.ensure(recs.size()), "allocate refit records");
+  if (!recs.empty())
+    PT_HIP(t, hipMemcpy(t->d_refit_insts.ptr, recs.data(), recs.size() * sizeof(ptd::RefitInst), hipMemcpyHostToDevice),
+           "upload refit records");
+  PT_HIP(t, hipMemsetAsync(t->d_refit_flag.ptr, 0, sizeof(uint32_t), t->stream), "clear refit flag");
+  ptd::launch_refit_tris(t->stream, t->d_refit_insts.ptr, static_cast<uint32_t>(recs.size()), tri_total,
+                         t->d_local_v.ptr, t->d_local_i.ptr, t->d_leaf_pos.ptr, t->d_tris.ptr, t->d_prims.ptr,
+                         t->d_tri_box.ptr);
+  for (size_t h = 0; h + 1 < t->refit_level.size(); ++h)
+    ptd::launch_refit_nodes(t->stream, t->d_nodes.ptr, t->d_refit_order.ptr, t->refit_level[h],
+                            t->refit_level[h + 1] - t->refit_level[h], t->d_tri_box.ptr, t->d_refit_flag.ptr);
+  ptd::launch_refit_root(t->stream, t->d_nodes.ptr, s.layout.root_ref, t->d_tri_box.ptr, t->d_recs.ptr);
+  PT_HIP(t, hipGetLastError(), "launch refit");
+  uint32_t nonfinite = 0;
+  PT_HIP(t, hipMemcpyAsync(&nonfinite, t->d_refit_flag.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost, t->stream),
+         "read refit flag");
+  PT_HIP(t, hipStreamSynchronize(t->stream), "synchronize");
+  if (nonfinite) t->dscene.general_boxes = 1u;
+  t->fwd_rad_valid = false;
+  ++t->scene_version;
+  result->applied_count = count;
+  result->refit_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  t->last_error.clear();
+  return PT_OK;
 }
 
 int pt_hip_update_scene_delta(pt_tracer* t, const PtMaterial* materials, uint32_t material_count,

@@ -258,6 +258,23 @@ void launch_grad_reduce(hipStream_t stream, const DevAdjoint& adj);  // rows -> 
 constexpr uint32_t kGradRows = 1024;  // partial rows: an upper bound on every adjoint kernel's grid
 void launch_zero_counters(hipStream_t stream, unsigned long long* counters);  // kCounterWords
 void launch_clear_film(hipStream_t stream, const DevFilm& film, uint32_t pixels);
+// Dynamic-instance refit (CpuBvhAccelerator.cpp:310-358, :527-603). One record per updated instance:
+// its primitive range, local geometry and new TRS; tri_begin is the prefix of triangle counts.
+struct RefitInst {
+  uint32_t prim_first, tri_count, local_first_vertex, local_vertex_count;
+  uint32_t local_first_index, tri_begin, pad0, pad1;
+  float t[3], q[4], s[3], pad2[2];
+};
+// Recompute the updated instances' triangles (leaf-order records, primitive normals, padded boxes).
+void launch_refit_tris(hipStream_t stream, const RefitInst* insts, uint32_t inst_count, uint32_t tri_count,
+                       const float* local_v, const uint32_t* local_i, const uint32_t* leaf_pos, float4* tris,
+                       float4* prims, float* tri_box);
+// Refit inner nodes order[begin, begin + count) (their children already refit); flags non-finite boxes.
+void launch_refit_nodes(hipStream_t stream, float4* nodes, const uint32_t* order, uint32_t begin, uint32_t count,
+                        const float* tri_box, uint32_t* nonfinite);
+// The scene record's root box from the refit root (inner node or leaf).
+void launch_refit_root(hipStream_t stream, const float4* nodes, int32_t root_ref, const float* tri_box,
+                       SceneRec* rec);
 // resolve_hdr (rs == nullptr: 3 floats per pixel into out) or resolve_ldr (rgba8 per pixel into out).
 void launch_resolve(hipStream_t stream, const DevFilm& film, uint32_t pixels, const DevResolve* rs, void* out);
 

@@ -492,6 +492,140 @@ __global__ __launch_bounds__(kBlock) void k_clear_film(DevFilm film, uint32_t pi
   }
 }
 
+// ------------------------------------------------------------------ dynamic-instance refit
+// CpuBvhAccelerator::apply_instance_transform_update (CpuBvhAccelerator.cpp:310-358): the updated
+// instances' triangles are recomputed from local geometry (refit_instance_primitives, :527-584), then the
+// node boxes bottom-up, one launch per tree height (refit_node, :586-603). The tree's topology and leaf
+// order stay; every box is the union of padded primitive boxes, so it equals a build's box.
+
+// rotate_quat / transform_instance_vertex / normalize_vec3 (CpuBvhAccelerator.cpp:48-82), evaluated as
+// the host builder evaluates them.
+__device__ inline V3 refit_vertex(V3 local, const RefitInst& r) {
+  const V3 v = mk(local.x * r.s[0], local.y * r.s[1], local.z * r.s[2]);
+  V3 o = v;
+  const float len_sq = r.q[0] * r.q[0] + r.q[1] * r.q[1] + r.q[2] * r.q[2] + r.q[3] * r.q[3];
+  if (!(len_sq <= kEps * kEps)) {
+    const float inv_len = 1.0f / __builtin_sqrtf(len_sq);
+    const V3 qv = mk(r.q[0] * inv_len, r.q[1] * inv_len, r.q[2] * inv_len);
+    const float qw = r.q[3] * inv_len;
+    const V3 t = muls(cross(qv, v), 2.0f);
+    o = add(add(v, muls(t, qw)), cross(qv, t));
+  }
+  return add(o, mk(r.t[0], r.t[1], r.t[2]));
+}
+
+__device__ inline V3 refit_normal(V3 v) {
+  const float len_sq = dot(v, v);
+  if (len_sq <= kEps * kEps) return mk(0.0f, 1.0f, 0.0f);
+  return muls(v, 1.0f / __builtin_sqrtf(len_sq));
+}
+
+__device__ inline float first_min3(float a, float b, float c) {  // std::min({a, b, c})
+  float m = a;
+  if (b < m) m = b;
+  if (c < m) m = c;
+  return m;
+}
+__device__ inline float first_max3(float a, float b, float c) {  // std::max({a, b, c})
+  float m = a;
+  if (m < b) m = b;
+  if (m < c) m = c;
+  return m;
+}
+
+__global__ __launch_bounds__(kBlock) void k_refit_tris(const RefitInst* insts, uint32_t inst_count,
+                                                       uint32_t tri_count, const float* lv, const uint32_t* li,
+                                                       const uint32_t* leaf_pos, float4* tris, float4* prims,
+                                                       float* tri_box) {
+  const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
+  if (g >= tri_count) return;
+  uint32_t lo = 0, hi = inst_count;  // last instance with tri_begin <= g
+  while (hi - lo > 1u) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (insts[mid].tri_begin <= g) lo = mid;
+    else hi = mid;
+  }
+  const RefitInst& r = insts[lo];
+  const uint32_t tri = g - r.tri_begin;
+  const uint32_t base = r.local_first_index + 3u * tri;
+  V3 v[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) v[k] = refit_vertex(ld3(lv + 3u * (r.local_first_vertex + li[base + k])), r);
+  const V3 e1 = sub(v[1], v[0]), e2 = sub(v[2], v[0]);
+  const V3 n = refit_normal(cross(e1, e2));
+  const uint32_t pid = r.prim_first + tri;
+  const uint32_t lp = leaf_pos[pid];
+  tris[3u * lp] = make_float4(v[0].x, v[0].y, v[0].z, __uint_as_float(pid));
+  tris[3u * lp + 1u] = make_float4(e1.x, e1.y, e1.z, 0.0f);
+  tris[3u * lp + 2u] = make_float4(e2.x, e2.y, e2.z, 0.0f);
+  const float4 old = prims[pid];
+  prims[pid] = make_float4(n.x, n.y, n.z, old.w);
+  float* b = tri_box + 6u * lp;
+  b[0] = first_min3(v[0].x, v[1].x, v[2].x) - kEps;
+  b[1] = first_min3(v[0].y, v[1].y, v[2].y) - kEps;
+  b[2] = first_min3(v[0].z, v[1].z, v[2].z) - kEps;
+  b[3] = first_max3(v[0].x, v[1].x, v[2].x) + kEps;
+  b[4] = first_max3(v[0].y, v[1].y, v[2].y) + kEps;
+  b[5] = first_max3(v[0].z, v[1].z, v[2].z) + kEps;
+}
+
+// The box of child reference `ref`: a leaf's is the union of its triangles' padded boxes (in leaf order,
+// starting from the first as refit_node does), an inner node's the union of its two child boxes.
+__device__ inline void refit_child_box(const float4* nodes, int32_t ref, const float* tri_box, float* mn,
+                                       float* mx) {
+  if (ref < 0) {
+    const uint32_t code = ~static_cast<uint32_t>(ref);
+    const uint32_t first = code >> 5, count = code & 31u;
+    const float* b = tri_box + 6u * first;
+    for (int k = 0; k < 3; ++k) {
+      mn[k] = b[k];
+      mx[k] = b[3 + k];
+    }
+    for (uint32_t i = 1; i < count; ++i) {
+      b += 6;
+      for (int k = 0; k < 3; ++k) {
+        mn[k] = smin(mn[k], b[k]);
+        mx[k] = smax(mx[k], b[3 + k]);
+      }
+    }
+    return;
+  }
+  const float4 f0 = nodes[4u * ref], f1 = nodes[4u * ref + 1u], f2 = nodes[4u * ref + 2u];
+  const float c0mn[3] = {f0.x, f0.y, f0.z}, c0mx[3] = {f0.w, f1.x, f1.y};
+  const float c1mn[3] = {f1.z, f1.w, f2.x}, c1mx[3] = {f2.y, f2.z, f2.w};
+  for (int k = 0; k < 3; ++k) {
+    mn[k] = smin(c0mn[k], c1mn[k]);
+    mx[k] = smax(c0mx[k], c1mx[k]);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_refit_nodes(float4* nodes, const uint32_t* order, uint32_t begin,
+                                                        uint32_t count, const float* tri_box, uint32_t* nonfinite) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= count) return;
+  const uint32_t ni = order[begin + i];
+  const int4 refs = reinterpret_cast<const int4*>(nodes)[4u * ni + 3u];
+  float a0[3], b0[3], a1[3], b1[3];
+  refit_child_box(nodes, refs.x, tri_box, a0, b0);
+  refit_child_box(nodes, refs.y, tri_box, a1, b1);
+  nodes[4u * ni] = make_float4(a0[0], a0[1], a0[2], b0[0]);
+  nodes[4u * ni + 1u] = make_float4(b0[1], b0[2], a1[0], a1[1]);
+  nodes[4u * ni + 2u] = make_float4(a1[2], b1[0], b1[1], b1[2]);
+  bool fin = true;
+  for (int k = 0; k < 3; ++k) fin = fin && finite(a0[k]) && finite(b0[k]) && finite(a1[k]) && finite(b1[k]);
+  if (!fin) atomicOr(nonfinite, 1u);
+}
+
+__global__ void k_refit_root(const float4* nodes, int32_t root_ref, const float* tri_box, SceneRec* rec) {
+  if (threadIdx.x != 0) return;
+  float mn[3], mx[3];
+  refit_child_box(nodes, root_ref, tri_box, mn, mx);
+  for (int k = 0; k < 3; ++k) {
+    rec->root_box[k] = mn[k];
+    rec->root_box[3 + k] = mx[k];
+  }
+}
+
 // FilmBuffer::resolve_hdr (FilmBuffer.cpp:160-214) and ApplyFilmResolve (FilmBuffer.cpp:255-332), one
 // thread per pixel: the mean of the pixel's samples, then exposure, white balance, tone map, gamma,
 // clamp and 8-bit quantisation in the reference's operation order (HBM-bound: 20 bytes read, 4 or 12
@@ -1836,6 +1970,26 @@ void launch_zero_counters(hipStream_t stream, unsigned long long* counters) {
 void launch_clear_film(hipStream_t stream, const DevFilm& film, uint32_t pixels) {
   if (pixels == 0) return;
   hipLaunchKernelGGL(k_clear_film, dim3((pixels + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, film, pixels);
+}
+
+void launch_refit_tris(hipStream_t stream, const RefitInst* insts, uint32_t inst_count, uint32_t tri_count,
+                       const float* local_v, const uint32_t* local_i, const uint32_t* leaf_pos, float4* tris,
+                       float4* prims, float* tri_box) {
+  if (tri_count == 0 || inst_count == 0) return;
+  hipLaunchKernelGGL(k_refit_tris, dim3((tri_count + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, insts,
+                     inst_count, tri_count, local_v, local_i, leaf_pos, tris, prims, tri_box);
+}
+
+void launch_refit_nodes(hipStream_t stream, float4* nodes, const uint32_t* order, uint32_t begin, uint32_t count,
+                        const float* tri_box, uint32_t* nonfinite) {
+  if (count == 0) return;
+  hipLaunchKernelGGL(k_refit_nodes, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, nodes, order,
+                     begin, count, tri_box, nonfinite);
+}
+
+void launch_refit_root(hipStream_t stream, const float4* nodes, int32_t root_ref, const float* tri_box,
+                       SceneRec* rec) {
+  hipLaunchKernelGGL(k_refit_root, dim3(1), dim3(64), 0, stream, nodes, root_ref, tri_box, rec);
 }
 
 void launch_resolve(hipStream_t stream, const DevFilm& film, uint32_t pixels, const DevResolve* rs, void* out) {

@@ -181,6 +181,48 @@ class RenderSettings:
     camera_focus_distance: float = 0.0
 
 
+@dataclasses.dataclass
+class InstanceTransformUpdate:
+    """RTInstanceTransformUpdate (PathTracer.h:298-306): a new TRS for the instance at instance_index,
+    or, when that index is out of range, for the instance whose entity_id matches."""
+    translation: tuple = (0.0, 0.0, 0.0)
+    rotation: tuple = (0.0, 0.0, 0.0, 1.0)
+    scale: tuple = (1.0, 1.0, 1.0)
+    instance_index: int = 0xFFFFFFFF
+    entity_id: int = 0
+    flags: int = 0
+    transform_revision: int = 0
+
+
+def resolve_instance_index(scene: "Scene", u: InstanceTransformUpdate) -> int | None:
+    """resolve_instance_index (CpuBvhAccelerator.cpp:507-525)."""
+    i = u.instance_index
+    if i >= len(scene.instances) and u.entity_id != 0:
+        hits = np.nonzero(scene.instances["entity_id"] == np.uint64(u.entity_id))[0]
+        if len(hits):
+            i = int(hits[0])
+    return i if i < len(scene.instances) else None
+
+
+def with_instance_transforms(scene: "Scene", updates) -> "Scene":
+    """ApplyInstanceTransformUpdates in its metadata mode (PathTracer.cpp:364-390): a copy of the scene
+    whose instance records carry the updates' TRS, flags (or-ed in) and revisions. Dynamic instances
+    take their triangles from local geometry through these transforms."""
+    inst = scene.instances.copy()
+    probe = dataclasses.replace(scene, instances=inst)
+    for u in updates:
+        i = resolve_instance_index(probe, u)
+        if i is None:
+            continue
+        inst[i]["translation"] = u.translation
+        inst[i]["rotation"] = u.rotation
+        inst[i]["scale"] = u.scale
+        inst[i]["flags"] |= np.uint32(u.flags)
+        if u.transform_revision:
+            inst[i]["transform_revision"] = u.transform_revision
+    return dataclasses.replace(scene, instances=inst)
+
+
 TONE_MAPS = ("linear", "reinhard", "filmic_approx", "aces_approx")  # ToneMapMode, PathTracer.h:55-60
 OUTPUT_TRANSFORMS = ("gamma", "linear")                              # OutputTransformMode, PathTracer.h:62-65
 

@@ -145,6 +145,28 @@ class HipPathTracer:
                                   up=tuple(up), fov_deg=fov_deg)
         return self.update_camera_state(cam)
 
+    def plan_instance_transform_update(self, updates) -> tuple[str, int]:
+        """plan_instance_transform_update: (InstanceTransformUpdateStatus name, matched count); a pure
+        query."""
+        r = abi.PtInstanceTransformResult()
+        arr = abi.make_transform_updates(updates)
+        self._check(self._lib.pt_hip_plan_instance_transform_update(self._h, arr, len(updates), C.byref(r)),
+                    "plan_instance_transform_update")
+        return abi.TRANSFORM_STATUS[r.status], r.matched_count
+
+    def apply_instance_transform_update(self, updates) -> tuple[str, int, float]:
+        """apply_instance_transform_update: the updated dynamic instances move and the BVH is refit on
+        the GPU. (status name, applied count, refit milliseconds)."""
+        r = abi.PtInstanceTransformResult()
+        arr = abi.make_transform_updates(updates)
+        self._check(self._lib.pt_hip_apply_instance_transform_update(self._h, arr, len(updates), C.byref(r)),
+                    "apply_instance_transform_update")
+        return abi.TRANSFORM_STATUS[r.status], r.applied_count, r.refit_ms
+
+    def update_instance_transforms(self, updates) -> bool:
+        """IPathTracer::update_instance_transforms: True when the transforms were applied."""
+        return self.apply_instance_transform_update(updates)[0].startswith("applied")
+
     def request_film_readback(self, settings=None) -> "abi.PtFilmReadbackToken":
         """IPathTracer::request_film_readback: the LDR resolve of the film as queued so far, copied
         to host memory while later rendering proceeds. Poll it with poll_film."""
