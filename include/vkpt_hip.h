@@ -138,6 +138,15 @@ enum {
   PT_READBACK_UNSUPPORTED = 4
 };
 
+/* RTSdfPrimitive, PathTracer.h:330-339. 56 bytes. shape: SdfShape (0 sphere, 1 box, 2 rounded box,
+ * 3 plane, 4 torus, 5 capsule, 6 unknown); rotation: Euler angles in radians (x, y, z). */
+typedef struct PtSdfPrimitive {
+  uint32_t shape;
+  float position[3], scale[3], rotation[3];
+  uint32_t material_index;
+  float radius, param_a, param_b;
+} PtSdfPrimitive;
+
 /* RTInstanceTransformUpdate, PathTracer.h:298-306. 64 bytes. An instance_index out of range (e.g.
  * 0xFFFFFFFF) selects the instance whose entity_id matches. */
 typedef struct PtInstanceTransformUpdate {
@@ -264,6 +273,11 @@ int pt_hip_render_tiles(pt_tracer* tracer, const PtRenderTile* tiles, uint32_t c
 /* Film readback: FilmBuffer::raw / sample_counts (PathTracer.h:955-958). Host pointers:
  * rgb_sum holds 3 floats per pixel, sample_count one u32 per pixel (either may be NULL). */
 int pt_hip_read_film(pt_tracer* tracer, float* rgb_sum, uint32_t* sample_count);
+/* PathTracerSceneSnapshot::sdf_primitives (PathTracer.h:358-409), after pt_hip_load_scene and before
+ * pt_hip_build_accel: primitives intersect_scene tests after the triangles
+ * (ScalarCpuPathTracer.Intersect.cpp:91-129; intersectors ScalarCpuPathTracer.cpp:809-962), on
+ * extension and shadow rays, counted in the sdf_* counters. Unknown shapes are dropped (never hit). */
+int pt_hip_load_sdf_primitives(pt_tracer* tracer, const PtSdfPrimitive* prims, uint32_t count);
 /* IPathTracer::request_film_readback / poll_film, PathTracer.h:1172-1179: the request enqueues the GPU
  * resolve of the film as it stands after all work queued so far (settings as pt_hip_resolve_ldr) and its
  * copy into pinned host memory on a second stream, and returns at once; rendering queued afterwards
@@ -408,6 +422,8 @@ int pt_batch_render_adjoint(pt_batch* batch, uint64_t frame_index, const float* 
                             float* grad_emission, float* grad_light, float* grad_roughness, float* grad_texels);
 /* Film of one scene, as pt_hip_read_film. */
 int pt_batch_read_film(pt_batch* batch, uint32_t scene, float* rgb_sum, uint32_t* sample_count);
+/* SDF primitives of one scene (not a view), as pt_hip_load_sdf_primitives; before pt_batch_build. */
+int pt_batch_load_sdf_primitives(pt_batch* batch, uint32_t scene, const PtSdfPrimitive* prims, uint32_t count);
 /* Camera of one scene or view, as pt_hip_update_camera (graph replays see the new camera). */
 int pt_batch_update_camera(pt_batch* batch, uint32_t scene, const PtCamera* camera);
 /* Resolved image of one scene, as pt_hip_resolve_hdr / pt_hip_resolve_ldr. */
@@ -436,7 +452,7 @@ int pt_bvh_build_host(const PtSceneView* scene, const PtBvhParams* params, uint3
 /* ABI sizes, for bindings to check their mirrors of the records above. */
 uint32_t pt_hip_sizeof(int which); /* 0 material, 1 light, 2 instance, 3 camera, 4 scene, 5 settings, 6 counters,
                                       7 tile, 8 tuning, 9 texture, 10 film resolve,
-                                      11 instance transform update */
+                                      11 instance transform update, 12 SDF primitive */
 const char* pt_hip_version(void);
 
 #ifdef __cplusplus

@@ -91,6 +91,9 @@ _SIGS = {
                                   C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.c_int, C.c_int, _P, _P,
                                   C.POINTER(PtoCounters)]),
     "pto_accel_refit": (C.c_int, [_P, C.POINTER(PtoScene)]),
+    "pto_accel_set_sdfs": (C.c_int, [_P, _P, C.c_uint32]),
+    "pto_intersect_scene_rays": (C.c_int, [C.POINTER(PtoScene), _P, _P, C.c_uint32, _P, _P, _P, _P,
+                                           C.POINTER(PtoCounters)]),
     "pto_color_temperature_rgb": (None, [C.c_float, _P]),
     "pto_white_balance_scale": (None, [C.c_float, _P]),
     "pto_resolve_hdr": (None, [_P, _P, C.c_uint64, _P]),
@@ -182,6 +185,11 @@ class Oracle:
         if not self.accel:
             raise RuntimeError("oracle accel build failed")
         self.texel_count = 0
+        sdf = getattr(scene, "sdf_primitives", None)
+        if sdf is not None and len(sdf):
+            self._sdf = np.ascontiguousarray(sdf)
+            if self.L.pto_accel_set_sdfs(self.accel, self._sdf.ctypes.data, len(self._sdf)) != 0:
+                raise RuntimeError("oracle SDF primitives rejected")
         tc = getattr(scene, "texcoords", None)
         if tc is not None:
             texels, descs = scene.texture_arrays()
@@ -196,6 +204,19 @@ class Oracle:
         if getattr(self, "accel", None):
             self.L.pto_accel_free(self.accel)
             self.accel = None
+
+    def intersect_scene(self, rays):
+        """intersect_scene for (N, 6) rays: (t, normal (N, 3), material, is_sdf, counters)."""
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = len(rays)
+        t = np.zeros(n, np.float32)
+        nrm = np.zeros((n, 3), np.float32)
+        mat = np.zeros(n, np.uint32)
+        sdf = np.zeros(n, np.uint8)
+        c = PtoCounters()
+        self.L.pto_intersect_scene_rays(C.byref(self.scene.s), self.accel, rays.ctypes.data, n, t.ctypes.data,
+                                        nrm.ctypes.data, mat.ctypes.data, sdf.ctypes.data, C.byref(c))
+        return t, nrm, mat, sdf.astype(bool), c.as_dict()
 
     def refit(self, scene) -> None:
         """CpuBvhAccelerator's dynamic-instance refit (pto_accel_refit): `scene` is this scene with

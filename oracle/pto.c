@@ -211,6 +211,8 @@ struct pto_accel {
   uint32_t* soa_base; /* PTO_AVX2: per node, its first batch in soa (leaves only) */
   PtoTextureDesc* textures;
   uint32_t texture_count;
+  PtoSdf* sdfs; /* pto_accel_set_sdfs: the SDF primitives intersect_scene tests after the triangles */
+  uint32_t sdf_count;
 };
 
 /* rotate_quat / transform_instance_vertex: CpuBvhAccelerator.cpp:48-74 */
@@ -578,6 +580,7 @@ void pto_accel_free(pto_accel* a) {
   free(a->prim_uv);
   free(a->texels);
   free(a->textures);
+  free(a->sdfs);
   free(a);
 }
 
@@ -965,10 +968,118 @@ typedef struct {
   int shadow_mode;
 } ctx_t;
 
-/* intersect_scene with the accelerator: ScalarCpuPathTracer.Intersect.cpp:28-50 */
+/* ---- SDF primitives: sdf_distance / sdf_normal / intersect_sphere / intersect_box and its aliases
+ * (ScalarCpuPathTracer.cpp:809-962), rotate_x/y/z and rotate_euler(_inv) (:113-137), divide_by_scale
+ * (:154-159). */
+#define MIN_MARCH_STEP RC(1.0e-3) /* ScalarCpuPathTracer.cpp:17-19 */
+#define MAX_MARCH_STEPS 192u
+#define MAX_MARCH_DISTANCE RC(10000.0)
+
+static v3 rot_x(v3 v, real a) { const real s = R_SIN(a), c = R_COS(a); return mk(v.x, c * v.y - s * v.z, s * v.y + c * v.z); }
+static v3 rot_y(v3 v, real a) { const real s = R_SIN(a), c = R_COS(a); return mk(c * v.x + s * v.z, v.y, -s * v.x + c * v.z); }
+static v3 rot_z(v3 v, real a) { const real s = R_SIN(a), c = R_COS(a); return mk(c * v.x - s * v.y, s * v.x + c * v.y, v.z); }
+static v3 euler_inv(v3 v, const PtoSdf* p) {
+  return rot_z(rot_y(rot_x(v, -(real)p->rotation[0]), -(real)p->rotation[1]), -(real)p->rotation[2]);
+}
+static v3 euler_fwd(v3 v, const PtoSdf* p) {
+  return rot_x(rot_y(rot_z(v, (real)p->rotation[2]), (real)p->rotation[1]), (real)p->rotation[0]);
+}
+static v3 div_scale(v3 v, const float* s) {
+  return mk(v.x / (s[0] != 0.0f ? (real)s[0] : RC(1.0)), v.y / (s[1] != 0.0f ? (real)s[1] : RC(1.0)),
+            v.z / (s[2] != 0.0f ? (real)s[2] : RC(1.0)));
+}
+
+static real sdf_distance(const PtoSdf* p, v3 w) {
+  const v3 l = div_scale(euler_inv(sub(w, v3f(p->position)), p), p->scale);
+  const real sx = (real)p->scale[0], sy = (real)p->scale[1], sz = (real)p->scale[2];
+  switch (p->shape) {
+    case 1u: {
+      const v3 q = mk(R_FABS(l.x) - sx, R_FABS(l.y) - sy, R_FABS(l.z) - sz);
+      const v3 mq = mk(smax(RC(0.0), q.x), smax(RC(0.0), q.y), smax(RC(0.0), q.z));
+      return length(mq) + smin(smax(q.x, smax(q.y, q.z)), RC(0.0));
+    }
+    case 2u: {
+      const real r = (real)p->param_a;
+      const v3 q = mk(R_FABS(l.x) - sx + r, R_FABS(l.y) - sy + r, R_FABS(l.z) - sz + r);
+      const v3 mq = mk(smax(RC(0.0), q.x), smax(RC(0.0), q.y), smax(RC(0.0), q.z));
+      return length(mq) + smin(smax(q.x, smax(q.y, q.z)), RC(0.0)) - (real)p->param_a;
+    }
+    case 3u:
+      return l.y + (real)p->param_b;
+    case 4u: {
+      const real a = smax(RC(0.05), (real)p->param_a);
+      const real b = smax(RC(0.02), (real)p->radius);
+      const real qx = length(mk(l.x, RC(0.0), l.z)) - a;
+      return length(mk(qx, l.y, RC(0.0))) - b;
+    }
+    case 5u: {
+      const v3 a = mk(RC(0.0), -(real)p->param_a, RC(0.0));
+      const v3 b = mk(RC(0.0), (real)p->param_a, RC(0.0));
+      const v3 ab = sub(b, a);
+      const real t = dot(sub(l, a), ab) / smax(kEps, dot(ab, ab));
+      const real c = smax(RC(0.0), smin(RC(1.0), t));
+      return length(sub(l, add(a, muls(ab, c)))) - (real)p->radius;
+    }
+    default:
+      return length(l) - (real)p->radius;
+  }
+}
+
+static v3 sdf_normal(const PtoSdf* p, v3 w) {
+  const real e = RC(0.0005);
+  const real dx = sdf_distance(p, add(w, mk(e, RC(0.0), RC(0.0)))) - sdf_distance(p, sub(w, mk(e, RC(0.0), RC(0.0))));
+  const real dy = sdf_distance(p, add(w, mk(RC(0.0), e, RC(0.0)))) - sdf_distance(p, sub(w, mk(RC(0.0), e, RC(0.0))));
+  const real dz = sdf_distance(p, add(w, mk(RC(0.0), RC(0.0), e))) - sdf_distance(p, sub(w, mk(RC(0.0), RC(0.0), e)));
+  return normalize(muls(mk(dx, dy, dz), RC(0.5)));
+}
+
+static int sdf_test(const PtoSdf* p, v3 o, v3 d, real* t_out, v3* n_out, PtoCounters* c) {
+  ++c->sdf_tests;
+  if (p->shape == 0u) {
+    const real radius = smax(RC(0.01), (real)p->radius);
+    const v3 lo = div_scale(euler_inv(sub(o, v3f(p->position)), p), p->scale);
+    const v3 ld = div_scale(euler_inv(d, p), p->scale);
+    const real a = dot(ld, ld);
+    ++c->sdf_steps;
+    if (a <= kEps * kEps) { ++c->sdf_misses; return 0; }
+    const real hb = dot(lo, ld);
+    const real cc = dot(lo, lo) - radius * radius;
+    const real disc = hb * hb - a * cc;
+    if (disc < RC(0.0)) { ++c->sdf_misses; return 0; }
+    const real root = R_SQRT(disc);
+    real cand = (-hb - root) / a;
+    if (cand <= kEps) cand = (-hb + root) / a;
+    if (cand <= kEps || !isfinite(cand)) { ++c->sdf_misses; return 0; }
+    *t_out = cand;
+    const v3 lhit = add(lo, muls(ld, cand));
+    *n_out = normalize(euler_fwd(div_scale(normalize(lhit), p->scale), p));
+    ++c->sdf_hits;
+    return 1;
+  }
+  real t = RC(0.0);
+  for (uint32_t i = 0; i < MAX_MARCH_STEPS; ++i) {
+    ++c->sdf_steps;
+    const v3 pt = add(o, muls(d, t));
+    const real dd = sdf_distance(p, pt);
+    if (dd <= kEps) {
+      *n_out = sdf_normal(p, pt);
+      *t_out = t;
+      ++c->sdf_hits;
+      return 1;
+    }
+    t += smax(MIN_MARCH_STEP, dd);
+    if (t > MAX_MARCH_DISTANCE) { ++c->sdf_misses; return 0; }
+  }
+  ++c->sdf_misses;
+  return 0;
+}
+
+/* intersect_scene: ScalarCpuPathTracer.Intersect.cpp:28-132 (the accelerator, then every SDF primitive;
+ * an SDF hit replaces the surface when it is nearer). SDF hits carry prim = ~0u. */
 static hit_t intersect_scene(const ctx_t* x, v3 o, v3 d, PtoCounters* c) {
   hit_t h;
   memset(&h, 0, sizeof(h));
+  real best = (real)INFINITY;
   const qhit_t q = traverse(x->a, o, d, (real)INFINITY, 0, c);
   if (q.hit) {
     const prim_t* p = &x->a->prims[q.prim];
@@ -980,10 +1091,58 @@ static hit_t intersect_scene(const ctx_t* x, v3 o, v3 d, PtoCounters* c) {
     h.prim = q.prim;
     h.bu = q.u;
     h.bv = q.v;
+    best = q.t;
+  }
+  for (uint32_t k = 0; k < x->a->sdf_count; ++k) {
+    const PtoSdf* p = &x->a->sdfs[k];
+    real t = RC(0.0);
+    v3 n = mk(RC(0.0), RC(0.0), RC(0.0));
+    if (!sdf_test(p, o, d, &t, &n, c) || t >= best) continue;
+    best = t;
+    h.hit = 1;
+    h.t = t;
+    h.position = add(o, muls(d, t));
+    h.normal = normalize(n);
+    h.material_index = p->material_index;
+    h.prim = ~0u;
+    h.bu = h.bv = RC(0.0);
   }
   return h;
 }
 
+/* intersect_scene for a batch of rays (o xyz, d xyz): t (inf on a miss), the reported normal, the
+ * material index and whether the surface is an SDF primitive. */
+int pto_intersect_scene_rays(const PtoScene* sc, const pto_accel* a, const float* rays, uint32_t n, float* out_t,
+                             float* out_normal, uint32_t* out_material, uint8_t* out_sdf, PtoCounters* c) {
+  if (!sc || !a || (n && !rays)) return -1;
+  PtoSettings st;
+  memset(&st, 0, sizeof(st));
+  ctx_t x = {sc, a, &st, 1u, PTO_SHADOW_CLOSEST};
+  for (uint32_t i = 0; i < n; ++i) {
+    const float* r = rays + 6u * i;
+    const hit_t h = intersect_scene(&x, v3f(r), v3f(r + 3), c);
+    out_t[i] = h.hit ? (float)h.t : INFINITY;
+    out_normal[3 * i] = (float)h.normal.x;
+    out_normal[3 * i + 1] = (float)h.normal.y;
+    out_normal[3 * i + 2] = (float)h.normal.z;
+    out_material[i] = h.material_index;
+    out_sdf[i] = (uint8_t)(h.hit && h.prim == ~0u);
+  }
+  return 0;
+}
+
+int pto_accel_set_sdfs(pto_accel* a, const PtoSdf* sdfs, uint32_t count) {
+  if (!a || (count && !sdfs)) return -1;
+  free(a->sdfs);
+  a->sdfs = NULL;
+  a->sdf_count = 0;
+  if (!count) return 0;
+  a->sdfs = (PtoSdf*)malloc(sizeof(PtoSdf) * count);
+  if (!a->sdfs) return -1;
+  for (uint32_t i = 0; i < count; ++i)
+    if (sdfs[i].shape <= 5u) a->sdfs[a->sdf_count++] = sdfs[i]; /* Unknown shapes are never tested */
+  return 0;
+}
 
 static const PtoMaterial* material_at(const PtoScene* sc, uint32_t index) {
   static const PtoMaterial kDefault = {{1.0f, 1.0f, 1.0f}, {0.0f, 0.0f, 0.0f}, 1.0f, 0.0f, 1.5f, 0.0f, 0.0f, 0.0f,
@@ -1065,7 +1224,7 @@ int pto_texture_sample(const pto_accel* a, uint32_t tex, uint32_t prim, float bu
 static void hit_base_color(const pto_accel* a, const PtoMaterial* m, hit_t* h) {
   h->base = v3f(m->albedo);
   h->tap.tex = ~0u;
-  if (a->prim_uv && m->base_color_texture_index < a->texture_count) {
+  if (a->prim_uv && m->base_color_texture_index < a->texture_count && h->prim != ~0u) {
     h->tap = texture_tap(a, m->base_color_texture_index, h->prim, h->bu, h->bv);
     h->base = mulv(h->base, h->tap.color);
   }
@@ -1895,6 +2054,8 @@ static int any_transmissive(const PtoScene* sc) {
 }
 
 static void add_counters(PtoCounters* d, const PtoCounters* s) {
+  d->sdf_tests += s->sdf_tests; d->sdf_steps += s->sdf_steps; d->sdf_hits += s->sdf_hits;
+  d->sdf_misses += s->sdf_misses;
   d->samples += s->samples; d->rays += s->rays; d->triangle_tests += s->triangle_tests;
   d->triangle_hits += s->triangle_hits; d->bvh_node_visits += s->bvh_node_visits;
   d->bvh_leaf_visits += s->bvh_leaf_visits; d->shadow_tests += s->shadow_tests;
@@ -1906,7 +2067,7 @@ int pto_render_pixels(const PtoScene* scene, const pto_accel* a, const PtoSettin
   if (!scene || !a || !s || s->width == 0 || s->height == 0) return -1;
   const int sup = check_supported(scene);
   if (sup) return sup;
-  if (shadow_mode == PTO_SHADOW_ANYHIT && any_transmissive(scene)) shadow_mode = PTO_SHADOW_CLOSEST;
+  if (shadow_mode == PTO_SHADOW_ANYHIT && (any_transmissive(scene) || a->sdf_count)) shadow_mode = PTO_SHADOW_CLOSEST;
   ctx_t x = {scene, a, s, s->max_depth > 1u ? s->max_depth : 1u, shadow_mode};
   const camera_t cam = make_camera(scene, s);
   real* rgb = (real*)out_rgb;
@@ -1947,7 +2108,7 @@ int pto_render_tile(const PtoScene* scene, const pto_accel* a, const PtoSettings
   if (!scene || !a || !s || s->width == 0 || s->height == 0) return -1;
   const int sup = check_supported(scene);
   if (sup) return sup;
-  if (shadow_mode == PTO_SHADOW_ANYHIT && any_transmissive(scene)) shadow_mode = PTO_SHADOW_CLOSEST;
+  if (shadow_mode == PTO_SHADOW_ANYHIT && (any_transmissive(scene) || a->sdf_count)) shadow_mode = PTO_SHADOW_CLOSEST;
   ctx_t x = {scene, a, s, s->max_depth > 1u ? s->max_depth : 1u, shadow_mode};
   const camera_t cam = make_camera(scene, s);
   real* film = (real*)film_sum;
@@ -2006,7 +2167,7 @@ int pto_adjoint_pixels(const PtoScene* scene, const pto_accel* a, const PtoSetti
   const int sup = check_supported(scene);
   if (sup) return sup;
   if (s->rr_start_depth > PTO_ADJ_HIST) return -4;
-  if (shadow_mode == PTO_SHADOW_ANYHIT && any_transmissive(scene)) shadow_mode = PTO_SHADOW_CLOSEST;
+  if (shadow_mode == PTO_SHADOW_ANYHIT && (any_transmissive(scene) || a->sdf_count)) shadow_mode = PTO_SHADOW_CLOSEST;
   ctx_t x = {scene, a, s, s->max_depth > 1u ? s->max_depth : 1u, shadow_mode};
   const camera_t cam = make_camera(scene, s);
   const uint32_t nm = scene->material_count, nl = scene->light_count;

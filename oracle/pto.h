@@ -102,6 +102,14 @@ typedef struct {
 enum { PTO_BUILDER_MIDPOINT = 0, PTO_BUILDER_SAH = 1 };
 enum { PTO_SHADOW_CLOSEST = 0, PTO_SHADOW_ANYHIT = 1 };
 
+/* SDF primitive (include/vkpt_hip.h PtSdfPrimitive; RTSdfPrimitive, PathTracer.h:330-339). */
+typedef struct {
+  uint32_t shape;
+  float position[3], scale[3], rotation[3];
+  uint32_t material_index;
+  float radius, param_a, param_b;
+} PtoSdf;
+
 typedef struct pto_accel pto_accel;
 
 uint32_t pto_real_bytes(void);
@@ -170,6 +178,13 @@ int pto_adjoint_pixels(const PtoScene* scene, const pto_accel* a, const PtoSetti
 /* Glossy-branch direction samples and their d log(density)/d exponent (see pto.c). */
 int pto_phong_samples(float exponent, const float refl[3], const float n[3], float ps, uint64_t state,
                       uint32_t count, float* out_dirs, double* out_score);
+
+/* The scene's SDF primitives (intersect_scene tests them after the triangles; Unknown shapes dropped). */
+int pto_accel_set_sdfs(pto_accel* a, const PtoSdf* sdfs, uint32_t count);
+/* intersect_scene (ScalarCpuPathTracer.Intersect.cpp:28-132) for rays (o xyz, d xyz): t (inf: miss),
+ * normal, material index, SDF flag; counters accumulate. */
+int pto_intersect_scene_rays(const PtoScene* sc, const pto_accel* a, const float* rays, uint32_t n, float* out_t,
+                             float* out_normal, uint32_t* out_material, uint8_t* out_sdf, PtoCounters* c);
 
 /* CpuBvhAccelerator's dynamic-instance refit (CpuBvhAccelerator.cpp:310-358, :527-603): scene holds the
  * updated instance transforms; primitives are recomputed and node boxes refit bottom-up, the topology

@@ -410,7 +410,7 @@ def test_fixture_triangle_counts(name):
 
     want = sum(geom.get(e["mesh"]["mesh_id"], 0) for e in doc["entities"] if "mesh" in e and visible(e))
     assert sc.triangle_count == want
-    assert len(sc.vertices) and np.isfinite(sc.vertices).all()
+    assert (len(sc.vertices) or len(sc.sdf_primitives)) and np.isfinite(sc.vertices).all()
     assert (sc.instances["material_index"] < len(sc.materials)).all()
 
 
@@ -460,3 +460,35 @@ def test_host_reads_converted_snapshot(tmp_path):
     assert got["local_vertices"] == len(sc.local_vertices) and got["materials"] == len(sc.materials)
     vsum = float(np.sum(sc.vertices, dtype=np.float64))
     assert abs(got["vertex_sum"] - vsum) <= 1e-6 * max(1.0, abs(vsum))
+
+
+def test_sdf_primitives_conversion():
+    """SDF entities become primitives with their world translation and scale, the world quaternion's
+    x, y, z as Euler angles (parse_shape_rotation), radius at least 0.01 and the entity's material override;
+    document-level entries add shapes unless an entity has their id; unknown shapes are dropped."""
+    from vkpathtracer_amd import scene as S
+    from vkpathtracer_amd import scene_document as SD
+
+    sc = SD.document_to_scene(_load("sdf_complexity.json"))
+    p = sc.sdf_primitives
+    assert [S.SDF_SHAPES[k] for k in p["shape"]] == ["sphere", "box"]
+    np.testing.assert_array_equal(p["position"], np.float32([[0, 1, 0], [1.2, 0.5, 0]]))
+    np.testing.assert_array_equal(p["scale"], np.float32([[0.8, 0.8, 0.8], [0.4, 0.6, 0.4]]))
+    assert p["radius"][0] == np.float32(0.8) and p["param_a"][1] == np.float32(0.4)
+    ents = [{"id": 1, "sdf_primitive": {"shape": "torus", "radius": 0.001, "param_a": 2.0},
+             "transform": {"translation": [1, 2, 3], "rotation": [0.1, 0.2, 0.3, 0.9]}, "material": {"id": 2}},
+            {"id": 2, "sdf_primitive": {"shape": "cone"}}]
+    doc = _quad_doc(ents, materials=[{"id": 1}, {"id": 2, "albedo": [1, 0, 0]}],
+                    sdf_primitives=[{"id": 1, "shape": "sphere"}, {"id": 7, "shape": "capsule",
+                                     "transform": {"translation": [0, 5, 0]}, "primitive": {"radius": 0.3}}])
+    p = SD.document_to_scene(doc).sdf_primitives
+    assert [S.SDF_SHAPES[k] for k in p["shape"]] == ["torus", "capsule"]
+    assert p["radius"][0] == np.float32(0.01) and p["material_index"][0] == 1
+    # the world transform's (normalised) quaternion x, y, z
+    np.testing.assert_allclose(p["rotation"][0], np.array([0.1, 0.2, 0.3]) / math.sqrt(0.95), rtol=1e-6)
+    np.testing.assert_array_equal(p["position"][1], [0, 5, 0])
+    assert p["radius"][1] == np.float32(0.3) and p["material_index"][1] == 0
+    # a document-level entry names its shape at the top level; without one the loader rejects it
+    doc["sdf_primitives"].append({"id": 8, "primitive": {"shape": "sphere"}})
+    with pytest.raises(SD.SceneDocumentError, match="sdf primitive is invalid 8"):
+        SD.document_to_scene(doc)

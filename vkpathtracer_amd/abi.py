@@ -175,6 +175,7 @@ SIGNATURES = {
     "pt_hip_request_film_readback": (C.c_int, [_P, C.POINTER(PtFilmResolve), C.POINTER(PtFilmReadbackToken)]),
     "pt_hip_poll_film": (C.c_int, [_P, C.POINTER(PtFilmReadbackToken), _P, C.POINTER(C.c_uint32)]),
     "pt_hip_update_camera": (C.c_int, [_P, C.POINTER(PtCamera)]),
+    "pt_hip_load_sdf_primitives": (C.c_int, [_P, _P, C.c_uint32]),
     "pt_hip_plan_instance_transform_update": (C.c_int, [_P, _P, C.c_uint32, C.POINTER(PtInstanceTransformResult)]),
     "pt_hip_apply_instance_transform_update": (C.c_int, [_P, _P, C.c_uint32, C.POINTER(PtInstanceTransformResult)]),
     "pt_hip_update_render_settings": (C.c_int, [_P, C.POINTER(PtRenderSettings)]),
@@ -213,6 +214,7 @@ SIGNATURES = {
     "pt_batch_read_film": (C.c_int, [_P, C.c_uint32, _P, _P]),
     "pt_batch_resolve_hdr": (C.c_int, [_P, C.c_uint32, _P]),
     "pt_batch_update_camera": (C.c_int, [_P, C.c_uint32, C.POINTER(PtCamera)]),
+    "pt_batch_load_sdf_primitives": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32]),
     "pt_batch_resolve_ldr": (C.c_int, [_P, C.c_uint32, C.POINTER(PtFilmResolve), _P]),
     "pt_batch_read_counters": (C.c_int, [_P, C.POINTER(PtSampleCounters)]),
     "pt_batch_set_tuning": (C.c_int, [_P, C.POINTER(PtTuning)]),

@@ -15,9 +15,12 @@
 
 #include "pt_bvh.h"
 #include "pt_device.h"
+#include "pt_math.h"
 #include "vkpt_hip.h"
 
 static_assert(sizeof(PtFilmResolve) == 24, "PtFilmResolve layout");
+static_assert(sizeof(PtSdfPrimitive) == 56, "PtSdfPrimitive layout");
+static_assert(sizeof(ptd::DevSdf) == 96, "DevSdf layout");
 static_assert(sizeof(PtInstanceTransformUpdate) == 64, "PtInstanceTransformUpdate layout");
 static_assert(sizeof(PtInstanceTransformResult) == 24, "PtInstanceTransformResult layout");
 
@@ -73,6 +76,7 @@ struct SceneSlot {
   std::vector<float> texcoords;  // 2 per vertex (may be empty)
   std::vector<float> texels;     // rgb
   std::vector<PtTextureDesc> textures;
+  std::vector<PtSdfPrimitive> sdfs;  // shapes the reference intersects (Unknown dropped)
   int32_t view_of = -1;          // >= 0: another camera on that scene's geometry (batches)
   uint32_t width = 0, height = 0, spp = 0;  // spp: samples per batch frame
   PtBvhParams bvh_params{PT_BVH_SAH, 4u, 8u, 0u};
@@ -113,6 +117,7 @@ struct pt_tracer {
   DevArray<PtLight> d_lights;
   DevArray<ptd::SceneRec> d_recs;
   DevArray<ptd::TexRec> d_texrecs;
+  DevArray<ptd::DevSdf> d_sdfs;
   DevArray<float> d_texels, d_prim_uv;
   DevArray<uint32_t> d_prim_tri;
   DevArray<float> d_grad_partial;  // adjoint: per-block gradient rows
@@ -308,6 +313,7 @@ int read_counter_block(pt_tracer* t, const unsigned long long* dev, unsigned lon
 
 void free_scene_device(pt_tracer* t) {
   t->refit_ready = false;
+  t->d_sdfs.release();
   t->d_tri_box.release();
   t->d_local_v.release();
   t->d_local_i.release();
@@ -434,6 +440,7 @@ bool same_work(const std::vector<ptd::DevTile>& a, const std::vector<ptd::DevTil
 // default material (:469-471); the environment map is used only when it holds width * height
 // texels (PathTracer.cpp:216-219).
 void copy_scene(SceneSlot& s, const PtSceneView* sc) {
+  s.sdfs.clear();
   auto copy = [](auto& dst, const auto* src, size_t n) {
     if (src && n) dst.assign(src, src + n);
     else dst.clear();
@@ -616,6 +623,8 @@ int upload_pool(pt_tracer* t) {
   std::vector<PtMaterial> mats;
   std::vector<PtLight> lights;
   std::vector<ptd::SceneRec> recs(t->scenes.size());
+  std::vector<ptd::DevSdf> sdfs;
+  std::vector<uint32_t> slot_mat_base(t->scenes.size(), 0u);
   mats.reserve(n_mats);
   lights.reserve(n_lights);
   texels.reserve(3u * n_texels);
@@ -632,6 +641,7 @@ int upload_pool(pt_tracer* t) {
       const uint32_t nn = static_cast<uint32_t>(L.nodes.size() / 16u), nt = static_cast<uint32_t>(L.tris.size() / 12u),
                      np = static_cast<uint32_t>(L.prims.size() / 4u);
       const uint32_t mat_base = static_cast<uint32_t>(mats.size()), light_base = static_cast<uint32_t>(lights.size());
+      slot_mat_base[si] = mat_base;
       const uint32_t tex_base = static_cast<uint32_t>(texrecs.size());
       for (uint32_t i = 0; i < nn; ++i) {
         const float* src = &L.nodes[16u * i];
@@ -702,6 +712,26 @@ int upload_pool(pt_tracer* t) {
       prim_base += np;
       env_base += static_cast<uint32_t>(s.environment_map.size());
     }
+    if (s.view_of < 0) {  // SDF primitives: pool material index (scene clamp, then base), trig of the Euler angles
+      R.sdf_base = static_cast<uint32_t>(sdfs.size());
+      R.sdf_count = static_cast<uint32_t>(s.sdfs.size());
+      const uint32_t last = static_cast<uint32_t>(s.materials.size()) - 1u;
+      for (const PtSdfPrimitive& q : s.sdfs) {
+        ptd::DevSdf d{};
+        d.shape = q.shape;
+        d.material = std::min(q.material_index, last) + slot_mat_base[si];
+        for (int k = 0; k < 3; ++k) {
+          d.pos[k] = q.position[k];
+          d.scale[k] = q.scale[k];
+          ptm::sincosf_p(-q.rotation[k], d.inv_s[k], d.inv_c[k]);
+          ptm::sincosf_p(q.rotation[k], d.fwd_s[k], d.fwd_c[k]);
+        }
+        d.radius = q.radius;
+        d.pa = q.param_a;
+        d.pb = q.param_b;
+        sdfs.push_back(d);
+      }
+    }
     R.width = s.width;
     R.height = s.height;
     R.film_base = film_base;
@@ -729,6 +759,7 @@ int upload_pool(pt_tracer* t) {
   PT_HIP(t, upload(t->d_texels, texels.data(), texels.size(), sizeof(float)), "upload textures");
   PT_HIP(t, upload(t->d_prim_tri, prim_tri.data(), prim_tri.size(), sizeof(uint32_t)), "upload triangle map");
   PT_HIP(t, upload(t->d_texrecs, texrecs.data(), texrecs.size(), sizeof(ptd::TexRec)), "upload texture table");
+  PT_HIP(t, upload(t->d_sdfs, sdfs.data(), sdfs.size(), sizeof(ptd::DevSdf)), "upload SDF primitives");
 
   // Traversal stack spill area: a lane's stack never holds more than max_depth entries (one per
   // level of the current path); entries older than the LDS ring's kStackLds go here.
@@ -764,6 +795,8 @@ int upload_pool(pt_tracer* t) {
   S.textured = textured ? 1u : 0u;
   S.prim_uv = any_uv ? t->d_prim_uv.ptr : nullptr;
   S.prim_tri = any_uv ? t->d_prim_tri.ptr : nullptr;
+  S.sdfs = sdfs.empty() ? nullptr : t->d_sdfs.ptr;
+  S.has_sdf = sdfs.empty() ? 0u : 1u;
   t->texel_count = texels.size() / 3u;
 
   t->info = PtBuildInfo{};
@@ -1081,6 +1114,10 @@ void to_counters(const unsigned long long* c, PtSampleCounters* out) {
   out->bvh_node_visits = c[ptd::kNodeVisits];
   out->bvh_leaf_visits = c[ptd::kLeafVisits];
   out->shadow_tests = c[ptd::kShadowTests];
+  out->sdf_tests = c[ptd::kSdfTests];
+  out->sdf_steps = c[ptd::kSdfSteps];
+  out->sdf_hits = c[ptd::kSdfHits];
+  out->sdf_misses = c[ptd::kSdfTests] - c[ptd::kSdfHits];  // every test ends in a hit or a miss
 }
 
 // ColorTemperatureToRgb (FilmBuffer.cpp:30-49): the Tanner Helland fit, with the host's libm as in the
@@ -1270,6 +1307,15 @@ int prepare_refit(pt_tracer* t) {
   return PT_OK;
 }
 
+// The SDF list of one slot (Unknown shapes dropped: intersect_scene never tests them).
+int set_sdfs(pt_tracer* t, SceneSlot& s, const PtSdfPrimitive* prims, uint32_t count) {
+  if (count && !prims) return fail(t, PT_INVALID_ARGUMENT, "load_sdf_primitives: null primitives");
+  s.sdfs.clear();
+  for (uint32_t i = 0; i < count; ++i)
+    if (prims[i].shape <= 5u) s.sdfs.push_back(prims[i]);
+  return PT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1290,6 +1336,7 @@ uint32_t pt_hip_sizeof(int which) {
     case 9: return sizeof(PtTextureDesc);
     case 10: return sizeof(PtFilmResolve);
     case 11: return sizeof(PtInstanceTransformUpdate);
+    case 12: return sizeof(PtSdfPrimitive);
     default: return 0;
   }
 }
@@ -1365,6 +1412,18 @@ int pt_hip_load_textures(pt_tracer* t, const float* texcoords, uint32_t texcoord
   const int rc = copy_textures(t, t->scenes[0], texcoords, texcoord_count, texels, texel_count, textures, texture_count);
   if (rc != PT_OK) return rc;
   t->accel_valid = false;
+  ++t->scene_version;
+  t->last_error.clear();
+  return PT_OK;
+}
+
+int pt_hip_load_sdf_primitives(pt_tracer* t, const PtSdfPrimitive* prims, uint32_t count) {
+  if (!t) return PT_INVALID_ARGUMENT;
+  if (!t->scene_loaded || t->scenes.size() != 1u)
+    return fail(t, PT_NOT_READY, "load_sdf_primitives called before load_scene_snapshot");
+  const int rc = set_sdfs(t, t->scenes[0], prims, count);
+  if (rc != PT_OK) return rc;
+  t->accel_valid = false;  // the next build uploads them
   ++t->scene_version;
   t->last_error.clear();
   return PT_OK;
@@ -2243,6 +2302,18 @@ int pt_batch_read_film(pt_batch* b, uint32_t scene, float* rgb_sum, uint32_t* sa
   for (uint32_t i = 0; i < scene; ++i) base += static_cast<size_t>(t->scenes[i].width) * t->scenes[i].height;
   const SceneSlot& s = t->scenes[scene];
   return read_film_range(t, base, static_cast<size_t>(s.width) * s.height, rgb_sum, sample_count);
+}
+
+int pt_batch_load_sdf_primitives(pt_batch* b, uint32_t scene, const PtSdfPrimitive* prims, uint32_t count) {
+  if (!b) return PT_INVALID_ARGUMENT;
+  pt_tracer* t = &b->t;
+  if (scene >= t->scenes.size() || t->scenes[scene].view_of >= 0)
+    return fail(t, PT_INVALID_ARGUMENT, "batch_load_sdf_primitives: no such scene (views share their scene's)");
+  const int rc = set_sdfs(t, t->scenes[scene], prims, count);
+  if (rc != PT_OK) return rc;
+  t->accel_valid = false;
+  t->last_error.clear();
+  return PT_OK;
 }
 
 int pt_batch_update_camera(pt_batch* b, uint32_t scene, const PtCamera* camera) {

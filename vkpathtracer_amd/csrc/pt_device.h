@@ -38,6 +38,9 @@ enum CounterSlot : int {
   kShadowRays = 10,
   kShadowInnerFetches = 11,
   kShadowLeafVisits = 12,
+  kSdfTests = 13,  // SampleCounters::sdf_tests / sdf_steps / sdf_hits (sdf_misses = tests - hits)
+  kSdfSteps = 14,
+  kSdfHits = 15,
 };
 
 // Ray queues are split into kSegments segments of seg_cap entries. Queue entry j of a bounce
@@ -66,6 +69,19 @@ struct DevCamera {
 
 // One scene of the pool: where its tree, lights, environment and film live. A single-scene
 // tracer has one record; a batch has one per scene, and every path carries its scene's index.
+// RTSdfPrimitive (PathTracer.h:330-339) as the kernels use it: the pool material index, and the sines
+// and cosines of -rotation (rotate_euler_inv) and +rotation (rotate_euler) per axis, formed once on the
+// host with the same sin / cos the kernels use (pure functions, so the values are those a per-step
+// evaluation would give).
+struct DevSdf {
+  uint32_t shape, material;
+  float pos[3], scale[3];
+  float radius, pa, pb;
+  float inv_s[3], inv_c[3];
+  float fwd_s[3], fwd_c[3];
+  float pad;
+};
+
 struct SceneRec {
   int32_t root_ref;       // pool-rebased root reference
   uint32_t prim_count;
@@ -73,6 +89,7 @@ struct SceneRec {
   uint32_t light_base, light_count, has_transmissive, env_base;  // env_base: float offset in the env pool
   uint32_t env_w, env_h;  // 0 x 0: constant environment colour
   uint32_t width, height, film_base;  // film pixels [film_base, film_base + width * height)
+  uint32_t sdf_base, sdf_count;       // the scene's SDF primitives in DevScene::sdfs
   float env[3], env_scale[3];
   DevCamera cam;
 };
@@ -107,6 +124,9 @@ struct DevScene {
   uint32_t textured;       // some material has a texture
   const float* prim_uv;    // 6 floats per pool primitive: texture coordinates of v0, v1, v2
   const uint32_t* prim_tri;  // per pool primitive: its triangle record in `tris`
+  const DevSdf* sdfs;      // SDF primitives of all scenes (SceneRec::sdf_base / sdf_count)
+  uint32_t has_sdf;        // some scene has SDF primitives: extension rays take k_sdf_extend, shadow
+                           // rays the closest-hit loop with SDF tests
 };
 
 // Integrator settings shared by every scene of a launch (image sizes are per scene: SceneRec).

@@ -850,6 +850,172 @@ __device__ __forceinline__ Stack make_stack(const DevPaths& P, int32_t* s_ref, f
 }
 
 // Persistent closest-hit kernel over the bounce's segmented ray queue.
+// ------------------------------------------------------------------ SDF primitives
+// ScalarCpuPathTracer's analytic / sphere-traced SDF intersectors (ScalarCpuPathTracer.cpp:809-962) and
+// the SDF loop of intersect_scene (ScalarCpuPathTracer.Intersect.cpp:91-129), in the reference's float
+// expressions. Euler rotations use the primitive's precomputed sines and cosines.
+constexpr float kMinMarchStep = 1.0e-3f;     // ScalarCpuPathTracer.cpp:17-19
+constexpr uint32_t kMaxMarchSteps = 192u;
+constexpr float kMaxMarchDistance = 10000.0f;
+
+__device__ __forceinline__ V3 rot_x(V3 v, float s, float c) { return mk(v.x, c * v.y - s * v.z, s * v.y + c * v.z); }
+__device__ __forceinline__ V3 rot_y(V3 v, float s, float c) { return mk(c * v.x + s * v.z, v.y, -s * v.x + c * v.z); }
+__device__ __forceinline__ V3 rot_z(V3 v, float s, float c) { return mk(c * v.x - s * v.y, s * v.x + c * v.y, v.z); }
+__device__ __forceinline__ V3 euler_inv(V3 v, const DevSdf& p) {  // rotate_euler_inv, :131-133
+  return rot_z(rot_y(rot_x(v, p.inv_s[0], p.inv_c[0]), p.inv_s[1], p.inv_c[1]), p.inv_s[2], p.inv_c[2]);
+}
+__device__ __forceinline__ V3 euler_fwd(V3 v, const DevSdf& p) {  // rotate_euler, :135-137
+  return rot_x(rot_y(rot_z(v, p.fwd_s[2], p.fwd_c[2]), p.fwd_s[1], p.fwd_c[1]), p.fwd_s[0], p.fwd_c[0]);
+}
+__device__ __forceinline__ V3 div_scale(V3 v, const float* s) {  // divide_by_scale, :154-159
+  return mk(v.x / (s[0] != 0.0f ? s[0] : 1.0f), v.y / (s[1] != 0.0f ? s[1] : 1.0f), v.z / (s[2] != 0.0f ? s[2] : 1.0f));
+}
+__device__ __forceinline__ float vlen(V3 v) { return __builtin_sqrtf(dot(v, v)); }
+
+__device__ float sdf_distance(const DevSdf& p, V3 w) {  // sdf_distance, :809-846
+  const V3 l = div_scale(euler_inv(sub(w, mk(p.pos[0], p.pos[1], p.pos[2])), p), p.scale);
+  switch (p.shape) {
+    case 1u: {  // box
+      const V3 q = mk(__builtin_fabsf(l.x) - p.scale[0], __builtin_fabsf(l.y) - p.scale[1], __builtin_fabsf(l.z) - p.scale[2]);
+      const V3 mq = mk(smax(0.0f, q.x), smax(0.0f, q.y), smax(0.0f, q.z));
+      return vlen(mq) + smin(smax(q.x, smax(q.y, q.z)), 0.0f);
+    }
+    case 2u: {  // rounded box
+      const float r = p.pa;
+      const V3 q = mk(__builtin_fabsf(l.x) - p.scale[0] + r, __builtin_fabsf(l.y) - p.scale[1] + r,
+                      __builtin_fabsf(l.z) - p.scale[2] + r);
+      const V3 mq = mk(smax(0.0f, q.x), smax(0.0f, q.y), smax(0.0f, q.z));
+      return vlen(mq) + smin(smax(q.x, smax(q.y, q.z)), 0.0f) - p.pa;
+    }
+    case 3u:  // plane
+      return l.y + p.pb;
+    case 4u: {  // torus
+      const float a = smax(0.05f, p.pa);
+      const float b = smax(0.02f, p.radius);
+      const float qx = vlen(mk(l.x, 0.0f, l.z)) - a;
+      return vlen(mk(qx, l.y, 0.0f)) - b;
+    }
+    case 5u: {  // capsule
+      const V3 a = mk(0.0f, -p.pa, 0.0f);
+      const V3 b = mk(0.0f, p.pa, 0.0f);
+      const V3 ab = sub(b, a);
+      const float t = dot(sub(l, a), ab) / smax(kEps, dot(ab, ab));
+      const float c = smax(0.0f, smin(1.0f, t));
+      const V3 closest = add(a, muls(ab, c));
+      return vlen(sub(l, closest)) - p.radius;
+    }
+    default:  // sphere
+      return vlen(l) - p.radius;
+  }
+}
+
+__device__ V3 sdf_normal(const DevSdf& p, V3 w) {  // sdf_normal, :848-854
+  const float e = 0.0005f;
+  const float dx = sdf_distance(p, add(w, mk(e, 0.0f, 0.0f))) - sdf_distance(p, sub(w, mk(e, 0.0f, 0.0f)));
+  const float dy = sdf_distance(p, add(w, mk(0.0f, e, 0.0f))) - sdf_distance(p, sub(w, mk(0.0f, e, 0.0f)));
+  const float dz = sdf_distance(p, add(w, mk(0.0f, 0.0f, e))) - sdf_distance(p, sub(w, mk(0.0f, 0.0f, e)));
+  return normalize(muls(mk(dx, dy, dz), 0.5f));
+}
+
+struct SdfStats {
+  uint32_t tests, steps, hits;
+};
+
+// One primitive's test (intersect_sphere :855-903, intersect_box and its aliases :905-962): true with
+// the hit's t.
+__device__ bool sdf_test(const DevSdf& p, V3 o, V3 d, float& t_out, SdfStats& st) {
+  ++st.tests;
+  if (p.shape == 0u) {
+    const float radius = smax(0.01f, p.radius);
+    const V3 lo = div_scale(euler_inv(sub(o, mk(p.pos[0], p.pos[1], p.pos[2])), p), p.scale);
+    const V3 ld = div_scale(euler_inv(d, p), p.scale);
+    const float a = dot(ld, ld);
+    ++st.steps;
+    if (a <= kEps * kEps) return false;
+    const float hb = dot(lo, ld);
+    const float c = dot(lo, lo) - radius * radius;
+    const float disc = hb * hb - a * c;
+    if (disc < 0.0f) return false;
+    const float root = __builtin_sqrtf(disc);
+    float cand = (-hb - root) / a;
+    if (cand <= kEps) cand = (-hb + root) / a;
+    if (cand <= kEps || !finite(cand)) return false;
+    t_out = cand;
+    ++st.hits;
+    return true;
+  }
+  float t = 0.0f;
+  for (uint32_t i = 0; i < kMaxMarchSteps; ++i) {
+    ++st.steps;
+    const float dd = sdf_distance(p, add(o, muls(d, t)));
+    if (dd <= kEps) {
+      t_out = t;
+      ++st.hits;
+      return true;
+    }
+    t += smax(kMinMarchStep, dd);
+    if (t > kMaxMarchDistance) return false;
+  }
+  return false;
+}
+
+// The surface normal intersect_scene reports for an SDF hit at t (the sphere's analytic normal, the
+// others' central differences at the hit point), normalised once more as intersect_scene does.
+__device__ V3 sdf_hit_normal(const DevSdf& p, V3 o, V3 d, float t) {
+  V3 n;
+  if (p.shape == 0u) {
+    const V3 lo = div_scale(euler_inv(sub(o, mk(p.pos[0], p.pos[1], p.pos[2])), p), p.scale);
+    const V3 ld = div_scale(euler_inv(d, p), p.scale);
+    const V3 lhit = add(lo, muls(ld, t));
+    n = normalize(euler_fwd(div_scale(normalize(lhit), p.scale), p));
+  } else {
+    n = sdf_normal(p, add(o, muls(d, t)));
+  }
+  return normalize(n);
+}
+
+// The SDF loop of intersect_scene: the nearest SDF hit below best_t (index into the pool, or -1).
+__device__ int32_t sdf_closest(const DevScene& sc, const SceneRec& R, V3 o, V3 d, float& best_t, SdfStats& st) {
+  int32_t best = -1;
+  for (uint32_t k = 0; k < R.sdf_count; ++k) {
+    float t = 0.0f;
+    if (sdf_test(sc.sdfs[R.sdf_base + k], o, d, t, st) && !(t >= best_t)) {
+      best_t = t;
+      best = static_cast<int32_t>(R.sdf_base + k);
+    }
+  }
+  return best;
+}
+
+__device__ __forceinline__ void flush_sdf(unsigned long long* counters, const SdfStats& st) {
+  wave_add(counters, kSdfTests, st.tests);
+  wave_add(counters, kSdfSteps, st.steps);
+  wave_add(counters, kSdfHits, st.hits);
+}
+
+// After k_extend, when some scene has SDF primitives: each extension ray's triangle hit is replaced
+// by a nearer SDF hit, encoded as hit_prim = -2 - (pool SDF index).
+__global__ __launch_bounds__(kBlock) void k_sdf_extend(DevPaths P, DevRays in, DevScene sc) {
+  __shared__ uint32_t prefix[kSegments + 1];
+  load_prefix(P.queue + kQueueInCount, 1, prefix);
+  const uint32_t n = prefix[kSegments];
+  SdfStats st = {0, 0, 0};
+  for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
+    const uint32_t pos = queue_pos(prefix, j, P.seg_cap);
+    const SceneRec& R = sc.recs[in.qscene[pos]];
+    if (R.sdf_count == 0u) continue;
+    const V3 o = mk(in.c[0][pos], in.c[1][pos], in.c[2][pos]);
+    const V3 d = mk(in.c[3][pos], in.c[4][pos], in.c[5][pos]);
+    float bt = P.hit_prim[pos] >= 0 ? P.hit_t[pos] : kInf;
+    const int32_t k = sdf_closest(sc, R, o, d, bt, st);
+    if (k >= 0) {
+      P.hit_t[pos] = bt;
+      P.hit_prim[pos] = -2 - k;
+    }
+  }
+  flush_sdf(P.counters, st);
+}
+
 __global__ __launch_bounds__(kBlock) void k_extend(DevPaths P, DevRays in, DevScene sc) {
   __shared__ int32_t s_ref[kStackLds * kBlock];
   __shared__ float s_t[kStackLds * kBlock];
@@ -1383,18 +1549,26 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
     adj_flush(A, sc, glds, p, L, aflags);
     dl = mk(A.delta[0][p], A.delta[1][p], A.delta[2][p]);
   }
-  if (prim < 0) {
+  if (prim == -1) {
     L = add(L, mulv(T, environment(sc, rd)));
     P.rad[0][p] = L.x; P.rad[1][p] = L.y; P.rad[2][p] = L.z;
     return;
   }
   uint64_t rng = P.rng[p];
   const V3 pos = add(ro, muls(rd, t));
-  const float4 pn = sc.prims[prim];
-  const V3 gN = mk(pn.x, pn.y, pn.z);
+  V3 gN;
+  uint32_t raw_mat;
+  if (prim >= 0) {
+    const float4 pn = sc.prims[prim];
+    gN = mk(pn.x, pn.y, pn.z);
+    raw_mat = static_cast<uint32_t>(__float_as_int(pn.w));
+  } else {  // an SDF primitive (k_sdf_extend)
+    const DevSdf& sp = sc.sdfs[-2 - prim];
+    gN = sdf_hit_normal(sp, ro, rd, t);
+    raw_mat = sp.material;
+  }
   const bool entering = dot(rd, gN) < 0.0f;
   const V3 sN = entering ? gN : neg(gN);
-  const uint32_t raw_mat = static_cast<uint32_t>(__float_as_int(pn.w));
   const uint32_t mat_index = raw_mat < sc.material_count - 1u ? raw_mat : sc.material_count - 1u;
   const PtMaterial& m = sc.materials[mat_index];
   float eff_k;
@@ -1403,7 +1577,7 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
   // CPU path, which renders untextured)
   V3 base = ld3(m.albedo);
   float2 huv = make_float2(0.0f, 0.0f);
-  if (m.base_color_texture_index < sc.texture_count) {
+  if (m.base_color_texture_index < sc.texture_count && prim >= 0) {
     // the hit's barycentrics: the closest-hit test of the same ray against the same triangle record,
     // recomputed here (bit-identical) rather than carried through the traversal
     float ht = 0.0f, hu = 0.0f, hv = 0.0f;
@@ -1765,7 +1939,7 @@ __device__ __forceinline__ void light_visible(const DevPaths& P, const DevScene&
   }
 }
 
-template <bool kAdj>
+template <bool kAdj, bool kSdf>
 __global__ __launch_bounds__(kBlock) void k_shadow(DevPaths P, DevScene sc, DevAdjoint A) {
   __shared__ int32_t s_ref[kStackLds * kBlock];
   __shared__ float s_t[kStackLds * kBlock];
@@ -1781,7 +1955,8 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevPaths P, DevScene sc, DevA
   const uint32_t n = prefix[kSegments];
   TravStats st = {0, 0, 0, 0, 0};
   uint32_t q = 0;
-  if (!sc.has_transmissive) {
+  SdfStats sst = {0, 0, 0};
+  if (!kSdf && !sc.has_transmissive) {
     auto begin = [&](uint32_t j, Trav& s) {
       q = queue_pos(prefix, j, P.seg_cap);
       const V3 o = mk(P.sray[0][q], P.sray[1][q], P.sray[2][q]);
@@ -1813,18 +1988,30 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevPaths P, DevScene sc, DevA
     };
     auto end = [&](uint32_t, Trav& s) {
       while (true) {
-        if (s.best_prim < 0 || s.best_t >= remaining) break;  // nothing in the way
-        const float4 pn = sc.prims[s.best_prim];
-        const PtMaterial& m = material_at(sc, static_cast<uint32_t>(__float_as_int(pn.w)));
+        // intersect_scene's nearest surface: the triangle hit, then any nearer SDF primitive
+        float bt = s.best_prim >= 0 ? s.best_t : kInf;
+        uint32_t raw_mat = 0u;
+        bool hit = s.best_prim >= 0;
+        if (hit) raw_mat = static_cast<uint32_t>(__float_as_int(sc.prims[s.best_prim].w));
+        if constexpr (kSdf) {
+          const int32_t k = sdf_closest(sc, sc.recs[sid], o, d, bt, sst);
+          if (k >= 0) {
+            hit = true;
+            raw_mat = sc.sdfs[k].material;
+          }
+        }
+        if (!hit || bt >= remaining) break;  // nothing in the way
+        const PtMaterial& m = material_at(sc, raw_mat);
         if (!(m.material_model == 5u || m.transmission > 0.05f)) return false;  // opaque blocker
-        const float advance = s.best_t + 0.004f;
+        const float advance = bt + 0.004f;
         o = add(o, muls(d, advance));
         remaining -= advance;
         ++skips;
         if (!(skips < 8u && remaining > 0.004f)) break;  // loop exhausted: visible
         if (trav_begin(sc.recs[sid], s, o, d, kInf, st)) return true;
-        // the follow-up query ended at once (root box missed): nothing in the way
-        break;
+        // the follow-up query ended at once (root box missed): no triangle in the way
+        s.best_prim = -1;
+        if constexpr (!kSdf) break;
       }
       visible(q);
       return false;
@@ -1832,6 +2019,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevPaths P, DevScene sc, DevA
     drive_queries<false>(sc, stk, n, P.refill_min, st, begin, end);
   }
   flush_stats(P.counters, st, true);
+  if constexpr (kSdf) flush_sdf(P.counters, sst);
 }
 
 // Adjoint passes: the radiance and gradient pieces of every shadow record whose light was visible.
@@ -2037,6 +2225,8 @@ void launch_copy_radiance(hipStream_t stream, const DevPaths& paths, const DevAd
 
 void launch_extend(hipStream_t stream, const DevPaths& paths, const DevScene& scene, int in_buf, uint32_t grid) {
   hipLaunchKernelGGL(k_extend, dim3(grid), dim3(kBlock), 0, stream, paths, paths.rays[in_buf], scene);
+  if (scene.has_sdf)
+    hipLaunchKernelGGL(k_sdf_extend, dim3(grid), dim3(kBlock), 0, stream, paths, paths.rays[in_buf], scene);
 }
 
 void launch_shade(hipStream_t stream, const DevPaths& paths, const DevScene& scene, const DevSettings& st, int in_buf,
@@ -2053,12 +2243,18 @@ void launch_shade_adjoint(hipStream_t stream, const DevPaths& paths, const DevAd
 }
 
 void launch_shadow(hipStream_t stream, const DevPaths& paths, const DevScene& scene, uint32_t grid) {
-  hipLaunchKernelGGL(k_shadow<false>, dim3(grid), dim3(kBlock), 0, stream, paths, scene, DevAdjoint{});
+  if (scene.has_sdf)
+    hipLaunchKernelGGL((k_shadow<false, true>), dim3(grid), dim3(kBlock), 0, stream, paths, scene, DevAdjoint{});
+  else
+    hipLaunchKernelGGL((k_shadow<false, false>), dim3(grid), dim3(kBlock), 0, stream, paths, scene, DevAdjoint{});
 }
 
 void launch_shadow_adjoint(hipStream_t stream, const DevPaths& paths, const DevAdjoint& adj, const DevScene& scene,
                            uint32_t grid) {
-  hipLaunchKernelGGL(k_shadow<true>, dim3(grid), dim3(kBlock), 0, stream, paths, scene, adj);
+  if (scene.has_sdf)
+    hipLaunchKernelGGL((k_shadow<true, true>), dim3(grid), dim3(kBlock), 0, stream, paths, scene, adj);
+  else
+    hipLaunchKernelGGL((k_shadow<true, false>), dim3(grid), dim3(kBlock), 0, stream, paths, scene, adj);
   hipLaunchKernelGGL(k_shadow_grad, dim3(kGradBlocks), dim3(kBlock), 0, stream, paths, scene, adj);
 }
 
@@ -2091,8 +2287,8 @@ static uint32_t occupancy_grid(int device, const void* fn) {
 Grids resident_grids(int device) {
   Grids g;
   g.extend = occupancy_grid(device, reinterpret_cast<const void*>(&k_extend));
-  g.shadow = occupancy_grid(device, reinterpret_cast<const void*>(&k_shadow<false>));
-  g.shadow_adj = occupancy_grid(device, reinterpret_cast<const void*>(&k_shadow<true>));
+  g.shadow = occupancy_grid(device, reinterpret_cast<const void*>(&k_shadow<false, false>));
+  g.shadow_adj = occupancy_grid(device, reinterpret_cast<const void*>(&k_shadow<true, false>));
   const uint32_t q0 = occupancy_grid(device, reinterpret_cast<const void*>(&k_trace_rays<false>));
   const uint32_t q1 = occupancy_grid(device, reinterpret_cast<const void*>(&k_trace_rays<true>));
   g.query = q0 < q1 ? q0 : q1;

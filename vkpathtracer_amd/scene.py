@@ -36,7 +36,14 @@ INSTANCE_DTYPE = np.dtype([
     ("local_index_count", "<u4"), ("translation", "<f4", (3,)), ("rotation", "<f4", (4,)),
     ("scale", "<f4", (3,)), ("pad", "<u4", (2,)),
 ])
+# RTSdfPrimitive (PathTracer.h:330-339); shape: SdfShape (PathTracer.h:232-240)
+SDF_DTYPE = np.dtype([
+    ("shape", "<u4"), ("position", "<f4", (3,)), ("scale", "<f4", (3,)), ("rotation", "<f4", (3,)),
+    ("material_index", "<u4"), ("radius", "<f4"), ("param_a", "<f4"), ("param_b", "<f4"),
+])
+SDF_SHAPES = ("sphere", "box", "rounded_box", "plane", "torus", "capsule", "unknown")
 assert MATERIAL_DTYPE.itemsize == 80 and LIGHT_DTYPE.itemsize == 64 and INSTANCE_DTYPE.itemsize == 96
+assert SDF_DTYPE.itemsize == 56
 
 # RTInstance flag (PathTracer.h:266): vertices come from local_* arrays through the TRS transform.
 INSTANCE_FLAG_DYNAMIC_TRANSFORM = 1
@@ -61,6 +68,16 @@ def make_materials(n: int) -> np.ndarray:
     out = np.zeros(n, MATERIAL_DTYPE)
     out[:] = default_material()
     return out
+
+
+def make_sdf(shape: str, position, scale=(1.0, 1.0, 1.0), rotation=(0.0, 0.0, 0.0), material_index: int = 0,
+             radius: float = 0.5, param_a: float = 0.0, param_b: float = 0.0) -> np.ndarray:
+    """An RTSdfPrimitive record (PathTracer.h:330-339); rotation: Euler angles in radians."""
+    p = np.zeros((), SDF_DTYPE)
+    p["shape"] = SDF_SHAPES.index(shape)
+    p["position"], p["scale"], p["rotation"] = position, scale, rotation
+    p["material_index"], p["radius"], p["param_a"], p["param_b"] = material_index, radius, param_a, param_b
+    return p
 
 
 def make_light(position, color, intensity, radius=0.0) -> np.ndarray:
@@ -108,6 +125,8 @@ class Scene:
     # and (H, W, 3) f32 images, rows in increasing v; materials name them by base_color_texture_index
     texcoords: np.ndarray | None = None
     textures: list = dataclasses.field(default_factory=list)
+    # SDF primitives intersect_scene tests after the triangles (SDF_DTYPE)
+    sdf_primitives: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0, SDF_DTYPE))
 
     def __post_init__(self):
         if self.environment_map is not None:
@@ -123,6 +142,7 @@ class Scene:
         if self.texcoords is not None:
             self.texcoords = np.ascontiguousarray(self.texcoords, np.float32).reshape(-1, 2)
         self.textures = [np.ascontiguousarray(t, np.float32).reshape(t.shape[0], t.shape[1], 3) for t in self.textures]
+        self.sdf_primitives = np.ascontiguousarray(self.sdf_primitives, SDF_DTYPE).reshape(-1)
 
     def texture_arrays(self) -> tuple[np.ndarray, np.ndarray]:
         """All textures as one (N, 3) texel array and (K, 4) u32 records (offset, width, height, 0)."""

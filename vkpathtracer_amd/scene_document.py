@@ -11,13 +11,13 @@ Restates, for everything the CPU path tracer consumes:
   (src/scene/SceneWorldExtraction.cpp:93-146) with the matrix helpers of src/scene/SceneInternal.h:156-240;
 * BuildSceneDataFromDocumentAtFrame (src/pathtracer/SceneConversion.cpp:594-1711): materials, texture
   indices, camera, triangle meshes (world-space vertices plus local geometry for transformable
-  instances), mesh culling, point / spot / environment lights and camera auto-placement.
+  instances), mesh culling, SDF primitives, point / spot / environment lights and camera auto-placement.
 
 All arithmetic is float32 in the reference's operation order, so vertex positions, transforms and light
 records come out bit-identical to the reference's snapshot for the same document.
 
 Not converted (each is reported by :func:`unsupported_features`): particle emitters (rain / smoke
-geometry), SDF primitives, tessellation requests, skinning and animation clips, model imports
+geometry), tessellation requests, skinning and animation clips, model imports
 (glTF / OBJ assets expanded by load_from_file), HDR environment images and texture images. The
 reference's CPU path tracer does not sample texture images, so materials keep their texture indices
 and render untextured, as they do there.
@@ -461,7 +461,8 @@ def read_document(root: dict) -> dict:
     used = set()
     for item in root.get("entities", []):
         e = {"id": _read_u64(item, "id", 0), "visible": True, "transform": None, "camera": None, "light": None,
-             "mesh": None, "parent": 0, "has_hierarchy": False, "physics": None, "scripted": False}
+             "mesh": None, "parent": 0, "has_hierarchy": False, "physics": None, "scripted": False, "sdf": None,
+             "material_override": 0}
         if e["id"] == 0:
             e["id"] = _allocate_id(used)
         used.add(e["id"])
@@ -477,6 +478,12 @@ def read_document(root: dict) -> dict:
         if "hierarchy" in item:
             e["has_hierarchy"] = True
             e["parent"] = _read_u64(item["hierarchy"], "parent", 0)
+        if "sdf_primitive" in item:  # SdfPrimitiveComponent (SceneTypes.h:127-132)
+            n = item["sdf_primitive"]
+            e["sdf"] = {"shape": _read_str(n, "shape", "sphere"), "radius": _read_float(n, "radius", F(1)),
+                        "param_a": _read_float(n, "param_a", F(0)), "param_b": _read_float(n, "param_b", F(0))}
+        if "material" in item:
+            e["material_override"] = _read_u64(item["material"], "id", 0)
         if "physics" in item:
             p = item["physics"]
             body = {"enabled": _read_bool(p, "enabled", True), "mass": _read_float(p, "mass", F(1)),
@@ -493,6 +500,18 @@ def read_document(root: dict) -> dict:
             src = _read_str(script, "source", "") or _read_str(script, "path", "")
             e["scripted"] = _read_bool(script, "enabled", True) and src != ""
         d["entities"].append(e)
+    used = set()
+    d["sdf_primitives"] = []
+    for item in root.get("sdf_primitives", []):  # SceneDocumentLoad.cpp:262-285
+        n = _get(item, "primitive") or {}
+        q = {"id": _read_u64(item, "id", 0), "shape": _read_str(item, "shape", ""),
+             "transform": _read_transform(_get(item, "transform") or {}),
+             "radius": _read_float(n, "radius", F(1)), "param_a": _read_float(n, "param_a", F(0)),
+             "param_b": _read_float(n, "param_b", F(0))}
+        if q["id"] == 0:
+            q["id"] = _allocate_id(used)
+        used.add(q["id"])
+        d["sdf_primitives"].append(q)
     for item in root.get("transforms", []):
         d["transforms"].append({"id": _read_u64(item, "id", 0), "parent": _read_u64(item, "parent", 0),
                                 "transform": _read_transform(_get(item, "transform") or {})})
@@ -598,6 +617,17 @@ def validate(d: dict) -> list:
             issues.append(f"entity camera has invalid clip/fov {e['id']}")
         if e["light"] is not None and not _valid_light(e["light"]):
             issues.append(f"entity light has invalid values {e['id']}")
+        if e["sdf"] is not None and (not e["sdf"]["shape"] or e["sdf"]["radius"] < 0):
+            issues.append(f"entity sdf primitive is invalid {e['id']}")
+        if e["material_override"] != 0 and mat_ids and e["material_override"] not in mat_ids:
+            issues.append(f"entity material override references missing material {e['material_override']}")
+    sdf_ids = set()
+    for q in d.get("sdf_primitives", []):
+        if q["id"] in sdf_ids:
+            issues.append(f"duplicate sdf primitive id {q['id']}")
+        sdf_ids.add(q["id"])
+        if not q["shape"] or not _valid_transform(q["transform"]) or q["radius"] < 0:
+            issues.append(f"sdf primitive is invalid {q['id']}")
     for t in d["transforms"]:
         if t["id"] not in ent_ids:
             issues.append(f"transform references missing entity {t['id']}")
@@ -688,8 +718,6 @@ def unsupported_features(root: dict) -> list:
     out = []
     if root.get("particle_emitters"):
         out.append("particle emitters")
-    if root.get("sdf_primitives") or any("sdf_primitive" in e for e in root.get("entities", [])):
-        out.append("SDF primitives")
     if any(isinstance(g.get("tessellation"), dict) and g["tessellation"].get("enabled") for g in root.get("geometry", [])):
         out.append("tessellation requests")
     kinds = [_normalize_material_id(a.get("type", "")) for a in root.get("assets", []) if isinstance(a, dict)]
@@ -864,6 +892,35 @@ def document_to_scene(root: dict, name: str = "document") -> S.Scene:
         insts.append(inst)
         nt += len(tri)
 
+    # SDF primitives (SceneConversion.cpp:1221-1270): ECS entities (world transform; the quaternion's x, y, z
+    # serve as the Euler angles, as parse_shape_rotation does), then document entries not already seen
+    sdf_names = {"sphere": 0, "box": 1, "rounded_box": 2, "plane": 3, "torus": 4, "capsule": 5}
+    sdfs = []
+
+    def add_sdf(shape, tr, radius, pa, pb, material):
+        if shape not in sdf_names:
+            return
+        r = np.zeros((), S.SDF_DTYPE)
+        r["shape"] = sdf_names[shape]
+        r["position"], r["rotation"], r["scale"] = tr["t"], tr["r"][:3], tr["s"]
+        r["radius"], r["param_a"], r["param_b"] = max(F(0.01), radius), pa, pb
+        r["material_index"] = material
+        sdfs.append(r)
+
+    ecs_sdf = set()
+    for e in d["entities"]:
+        if e["sdf"] is None:
+            continue
+        ecs_sdf.add(e["id"])
+        if not visible_path(e):
+            continue
+        tr, _ = resolve(e["id"])
+        mat = lookup.get(e["material_override"], 0) if e["material_override"] != 0 else 0
+        add_sdf(e["sdf"]["shape"], tr, e["sdf"]["radius"], e["sdf"]["param_a"], e["sdf"]["param_b"], mat)
+    for q in d["sdf_primitives"]:
+        if q["id"] not in ecs_sdf:
+            add_sdf(q["shape"], q["transform"], q["radius"], q["param_a"], q["param_b"], 0)
+
     # lights (SceneConversion.cpp:1587-1647)
     env = np.zeros(3, np.float32)
     lights = []
@@ -916,7 +973,7 @@ def document_to_scene(root: dict, name: str = "document") -> S.Scene:
                    local_vertices=np.concatenate(lverts) if lverts else np.zeros((0, 3), np.float32),
                    local_indices=np.concatenate(lidx).astype(np.uint32) if lidx else np.zeros(0, np.uint32),
                    environment_color=tuple(float(x) for x in env), environment_map_scale=tuple(float(x) for x in env),
-                   name=name)
+                   name=name, sdf_primitives=np.array(sdfs, S.SDF_DTYPE) if sdfs else np.zeros(0, S.SDF_DTYPE))
 
 
 def _spot_cone(l):

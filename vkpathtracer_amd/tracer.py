@@ -118,6 +118,11 @@ class HipPathTracer:
         st = self._status(self._lib.pt_hip_load_scene(self._h, C.byref(view.view)))
         if st and view.has_textures:
             st = self._status(self._lib.pt_hip_load_textures(self._h, *view.texture_args()))
+        sdf = getattr(scene, "sdf_primitives", None)
+        if st and sdf is not None and len(sdf):
+            sdf = np.ascontiguousarray(sdf)
+            view.keep.append(sdf)
+            st = self._status(self._lib.pt_hip_load_sdf_primitives(self._h, sdf.ctypes.data, len(sdf)))
         if st:
             self._view = view
             self._camera = scene.camera
@@ -423,6 +428,11 @@ class HipBatch:
                                                  C.byref(idx)), "add_scene")
         if view.has_textures:
             self._check(self._lib.pt_batch_load_textures(self._h, idx.value, *view.texture_args()), "load_textures")
+        sdf = getattr(scene, "sdf_primitives", None)
+        if sdf is not None and len(sdf):
+            sdf = np.ascontiguousarray(sdf)
+            self._check(self._lib.pt_batch_load_sdf_primitives(self._h, idx.value, sdf.ctypes.data, len(sdf)),
+                        "load_sdf_primitives")
         self.scenes.append((width, height, spp))
         return int(idx.value)
 
