@@ -90,6 +90,21 @@ def kernel_bytes(ks: dict, counters: dict) -> dict:
     }
 
 
+def _work_per_ray(ks: dict, counters: dict) -> dict:
+    """Traversal work per query of one step, extension and shadow apart (SampleCounters semantics:
+    node visits count every box test, leaves every leaf entered, tests every triangle tested)."""
+    ext = {"visits": counters["bvh_node_visits"] - ks["shadow_node_visits"],
+           "leaves": counters["bvh_leaf_visits"] - ks["shadow_leaf_visits"],
+           "tests": counters["triangle_tests"] - ks["shadow_triangle_tests"],
+           "inner_fetches": ks["inner_node_fetches"] - ks["shadow_inner_node_fetches"]}
+    sh = {"visits": ks["shadow_node_visits"], "leaves": ks["shadow_leaf_visits"],
+          "tests": ks["shadow_triangle_tests"], "inner_fetches": ks["shadow_inner_node_fetches"]}
+    out = {}
+    for name, work, rays in (("extension", ext, counters["rays"]), ("shadow", sh, ks["shadow_rays"])):
+        out[name] = {k: round(v / rays, 2) if rays else None for k, v in work.items()}
+    return out
+
+
 def _has_avx2() -> bool:
     try:
         with open("/proc/cpuinfo") as f:
@@ -487,6 +502,7 @@ def main():
         "mrays_per_s": round(mrays, 3),
         "ref_paths_per_sec": round(ext_rays * args.steps / dt, 1),
         "rays_per_step": {"extension": ext_rays, "shadow": shadow_rays},
+        "work_per_ray": _work_per_ray(ks, counters),
         "bvh_build_ms": round(build_wall_ms, 1),
         "bvh": bvh_info,
         "tuning": {"source": "vkpathtracer_amd/tuning/mi355x.json (autotune.py)" if tuned else "defaults",

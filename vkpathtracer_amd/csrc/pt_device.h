@@ -54,7 +54,11 @@ constexpr int kSegLine = 32;  // u32 per cursor line (128 bytes)
 constexpr int kQueueInCount = 0;                                  // kSegments counts of this bounce's input
 constexpr int kQueueOutCursor = kSegments;                        // kSegments cursors, kSegLine apart
 constexpr int kQueueShadowCursor = kQueueOutCursor + kSegments * kSegLine;
-constexpr int kQueueWords = kQueueShadowCursor + kSegments * kSegLine;
+// Traversal scheduling: the next chunk of queries to hand out, and the waves of the running launch
+// that have finished; the launch's last wave sets both back to zero.
+constexpr int kQueueChunkCursor = kQueueShadowCursor + kSegments * kSegLine;
+constexpr int kQueueWavesDone = kQueueChunkCursor + kSegLine;
+constexpr int kQueueWords = kQueueWavesDone + kSegLine;
 
 __host__ __device__ constexpr uint32_t segment_capacity(uint32_t capacity) {
   return 64u * ((((capacity + 63u) / 64u) + kSegments - 1u) / kSegments);

@@ -331,17 +331,45 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
                                    __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
 }
 
-__device__ __forceinline__ uint32_t global_wave() { return blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6); }
+// The wave's index in the grid, read from lane 0 so that the compiler keeps it (and the chunk
+// bookkeeping derived from it) in scalar registers.
+__device__ __forceinline__ uint32_t global_wave() {
+  return __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
+}
+
+// Ballot of a lane predicate straight from its compare mask (no select to a VGPR and back).
+__device__ __forceinline__ uint64_t wave_ballot(bool pred) { return __builtin_amdgcn_ballot_w64(pred); }
+
+// The first active lane takes the next chunk index from the shared cursor; every lane gets it.
+// Called with the wave converged.
+__device__ __forceinline__ uint32_t wave_grab(uint32_t* cursor) {
+  uint32_t v = 0;
+  if (lanes_below(wave_ballot(true)) == 0u) v = atomicAdd(cursor, 1u);
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
+// A wave leaves drive_queries. The grid's last wave resets the chunk cursor and the count for the
+// next launch on the stream: every other wave's last grab returned before it counted itself here.
+__device__ __forceinline__ void wave_finish(uint32_t* queue, uint32_t waves) {
+  if (lanes_below(wave_ballot(true)) == 0u) {
+    if (atomicAdd(queue + kQueueWavesDone, 1u) == waves - 1u) {
+      atomicExch(queue + kQueueChunkCursor, 0u);
+      atomicExch(queue + kQueueWavesDone, 0u);
+    }
+  }
+}
 
 // Persistent query driver with per-lane refill. The queries [0, n) are cut into 64-entry chunks;
-// wave w owns chunks w, w + W, w + 2W, ... (W = waves in the grid), so no cursor is shared.
-// A lane that finishes takes the next entry of its wave's chunk as soon as `refill_min` lanes
-// are idle, so a wave is not held by its slowest ray.
+// wave w starts on chunk w, and a wave that has handed out its chunk takes the next one from a
+// shared cursor (chunk W + cursor, W = waves in the grid), so waves that drew cheap rays take more
+// chunks and the launch ends evenly. A lane that finishes takes the next entry of its wave's chunk
+// as soon as `refill_min` lanes are idle, so a wave is not held by its slowest ray. Every wave of
+// the grid must call it exactly once (the last one resets the shared cursor).
 //   begin(j, s): start query j on this lane; false if it ended at once.
 //   end(j, s)  : the query finished; true if the lane started a follow-up query on s.
 template <bool kAny, typename Begin, typename End>
-__device__ __forceinline__ void drive_queries(const DevScene& sc, const Stack& stk, uint32_t n, uint32_t refill_min,
-                                              TravStats& st, Begin begin, End end) {
+__device__ __forceinline__ void drive_queries(const DevScene& sc, const Stack& stk, uint32_t* queue, uint32_t n,
+                                              uint32_t refill_min, TravStats& st, Begin begin, End end) {
   const uint32_t waves = gridDim.x * kWavesPerBlock;
   uint32_t chunk = global_wave();
   uint32_t next = 0;  // entries of `chunk` already handed out
@@ -351,7 +379,7 @@ __device__ __forceinline__ void drive_queries(const DevScene& sc, const Stack& s
   while (true) {
     const uint32_t base = chunk * 64u;
     if (base < n) {
-      const uint64_t idle = __ballot(!active);
+      const uint64_t idle = wave_ballot(!active);
       const uint32_t k = static_cast<uint32_t>(__popcll(idle));
       if (k >= refill_min || k == 64u) {
         const uint32_t left = n - base - next;
@@ -363,7 +391,7 @@ __device__ __forceinline__ void drive_queries(const DevScene& sc, const Stack& s
         }
         next += k < avail ? k : avail;
         if (next >= 64u || base + next >= n) {
-          chunk += waves;
+          chunk = waves + wave_grab(queue + kQueueChunkCursor);
           next = 0;
         }
       }
@@ -371,8 +399,8 @@ __device__ __forceinline__ void drive_queries(const DevScene& sc, const Stack& s
     // One step per iteration for the larger of the two groups of lanes, those at inner nodes or
     // those at leaves; the others wait, so each step runs with most of its lanes busy.
     const bool at_inner = active && s.cur >= 0;
-    const uint32_t n_inner = static_cast<uint32_t>(__popcll(__ballot(at_inner)));
-    const uint32_t n_leaf = static_cast<uint32_t>(__popcll(__ballot(active && !at_inner)));
+    const uint32_t n_inner = static_cast<uint32_t>(__popcll(wave_ballot(at_inner)));
+    const uint32_t n_leaf = static_cast<uint32_t>(__popcll(wave_ballot(active && !at_inner)));
     if (n_inner + n_leaf == 0u) {
       if (chunk * 64u >= n) break;
       continue;
@@ -386,6 +414,7 @@ __device__ __forceinline__ void drive_queries(const DevScene& sc, const Stack& s
     if (r == kStepPop) r = trav_pop(stk, s, st);
     if (r == kStepDone) active = end(j, s);
   }
+  wave_finish(queue, waves);
 }
 
 __device__ __forceinline__ void wave_add(unsigned long long* counters, int slot, uint64_t v) {
@@ -414,7 +443,7 @@ __device__ __forceinline__ void flush_stats(unsigned long long* counters, const 
 
 // Wave-wide append: returns this lane's slot (valid when pred is true).
 __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
-  const uint64_t m = __ballot(pred);
+  const uint64_t m = wave_ballot(pred);
   uint32_t base = 0;
   if (m != 0u) {
     const uint32_t leader = static_cast<uint32_t>(__builtin_ctzll(m));
@@ -466,6 +495,10 @@ __global__ void k_init_queue(uint32_t* queue, uint32_t live, uint32_t seg_cap) {
     queue[kQueueInCount + s] = static_cast<uint32_t>(left < seg_cap ? left : seg_cap);
     queue[kQueueOutCursor + s * kSegLine] = 0u;
     queue[kQueueShadowCursor + s * kSegLine] = 0u;
+  }
+  if (s == 0u) {
+    queue[kQueueChunkCursor] = 0u;
+    queue[kQueueWavesDone] = 0u;
   }
 }
 
@@ -1038,7 +1071,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevPaths P, DevRays in, DevSc
     ++rays;
     return false;
   };
-  drive_queries<false>(sc, stk, n, P.refill_min, st, begin, end);
+  drive_queries<false>(sc, stk, P.queue, n, P.refill_min, st, begin, end);
   flush_stats(P.counters, st, false);
   wave_add(P.counters, kRays, rays);
 }
@@ -1967,7 +2000,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevPaths P, DevScene sc, DevA
       if (s.best_prim < 0) visible(q);
       return false;
     };
-    drive_queries<true>(sc, stk, n, P.refill_min, st, begin, end);
+    drive_queries<true>(sc, stk, P.queue, n, P.refill_min, st, begin, end);
   } else {
     V3 o, d;
     float remaining = 0.0f;
@@ -2016,7 +2049,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevPaths P, DevScene sc, DevA
       visible(q);
       return false;
     };
-    drive_queries<false>(sc, stk, n, P.refill_min, st, begin, end);
+    drive_queries<false>(sc, stk, P.queue, n, P.refill_min, st, begin, end);
   }
   flush_stats(P.counters, st, true);
   if constexpr (kSdf) flush_sdf(P.counters, sst);
@@ -2137,7 +2170,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_rays(DevPaths P, DevScene sc, 
     out_prim[j] = s.best_prim;
     return false;
   };
-  drive_queries<kAny>(sc, stk, n, P.refill_min, st, begin, end);
+  drive_queries<kAny>(sc, stk, P.queue, n, P.refill_min, st, begin, end);
   flush_stats(P.counters, st, false);
 }
 
