@@ -75,9 +75,11 @@ def test_documents_in_one_batch(use_graph):
             assert np.array_equal(film, rf), (name, float(np.abs(film - rf).max()))
 
 
-def test_host_renders_converted_snapshot(tmp_path):
+@pytest.mark.parametrize("name", ["cornell_native", "sdf_complexity"])
+def test_host_renders_converted_snapshot(tmp_path, name):
     """python -m vkpathtracer_amd.scene_document writes a snapshot that pt_hip_bench --scene renders:
-    its film equals the oracle's render of the converted document."""
+    its film equals the oracle's render of the converted document (SDF primitives travel in a
+    version-2 snapshot)."""
     import json
 
     from oracle import oracle as O
@@ -85,9 +87,9 @@ def test_host_renders_converted_snapshot(tmp_path):
 
     exe = ROOT / "vkpathtracer_amd" / "bin" / "pt_hip_bench"
     assert exe.exists(), "pt_hip_bench is not built"
-    snap = tmp_path / "cornell.ptsnap"
+    snap = tmp_path / f"{name}.ptsnap"
     out = tmp_path / "out"
-    subprocess.run(["python", "-m", "vkpathtracer_amd.scene_document", str(GOLDEN / "cornell_native.json"),
+    subprocess.run(["python", "-m", "vkpathtracer_amd.scene_document", str(GOLDEN / f"{name}.json"),
                     "--out", str(snap)], check=True, cwd=ROOT, timeout=120)
     st = S.RenderSettings(width=40, height=30, spp=3, max_depth=6)
     r = subprocess.run([str(exe), "--scene", str(snap), "--width", "40", "--height", "30", "--spp", "3",
@@ -98,9 +100,9 @@ def test_host_renders_converted_snapshot(tmp_path):
     w, h = np.frombuffer(raw[:8], np.uint32)
     rgb = np.frombuffer(raw[8:8 + 12 * w * h], np.float32).reshape(h, w, 3)
     cnt = np.frombuffer(raw[8 + 12 * w * h:], np.uint32).reshape(h, w)
-    rf, rc, rcnt = O.Oracle(_scene("cornell_native.json")).render(st)
+    rf, rc, rcnt = O.Oracle(_scene(f"{name}.json")).render(st)
     assert np.array_equal(cnt, rc)
     assert np.array_equal(rgb, rf), float(np.abs(rgb - rf).max())
     res = json.loads((out / "results.json").read_text())
-    for k in OPAQUE:
+    for k in (ALWAYS if _transmissive(_scene(f"{name}.json")) else OPAQUE):
         assert res["counters"][k] == rcnt[k], k

@@ -492,3 +492,38 @@ def test_sdf_primitives_conversion():
     doc["sdf_primitives"].append({"id": 8, "primitive": {"shape": "sphere"}})
     with pytest.raises(SD.SceneDocumentError, match="sdf primitive is invalid 8"):
         SD.document_to_scene(doc)
+
+
+def test_snapshot_version_2_carries_sdf_primitives(tmp_path):
+    """A scene with SDF primitives is written as snapshot version 2 (version 1 plus the SDF records),
+    which the C++ host reads back field for field; a scene without them stays version 1."""
+    import struct
+    import subprocess
+
+    from vkpathtracer_amd import scene as S
+    from vkpathtracer_amd import scene_document as SD
+
+    bench = GOLDEN.parent.parent.parent / "vkpathtracer_amd" / "bin" / "pt_hip_bench"
+    if not bench.exists():
+        pytest.skip("pt_hip_bench is not built")
+    sc = SD.document_to_scene(_load("sdf_complexity.json"))
+    assert len(sc.sdf_primitives) > 0
+    path = tmp_path / "sdf.ptsnap"
+    sc.save_snapshot(path)
+    raw = path.read_bytes()
+    assert raw[:8] == b"VKPTSNAP" and struct.unpack("<I", raw[8:12])[0] == 2
+    n = len(sc.sdf_primitives)
+    assert struct.unpack("<I", raw[-(4 + 56 * n):-56 * n])[0] == n
+    assert raw[-56 * n:] == np.ascontiguousarray(sc.sdf_primitives, S.SDF_DTYPE).tobytes()
+    got = json.loads(subprocess.run([str(bench), "--check-snapshot", str(path)], check=True, capture_output=True,
+                                    text=True, timeout=60).stdout)
+    p = sc.sdf_primitives
+    want = float(sum(np.sum(p[k], dtype=np.float64) for k in ("shape", "material_index", "radius", "param_a",
+                                                                  "param_b", "position", "scale", "rotation")))
+    assert got["sdf_primitives"] == n and abs(got["sdf_sum"] - want) <= 1e-9 * max(1.0, abs(want))
+    plain = tmp_path / "plain.ptsnap"
+    SD.document_to_scene(_load("cornell_native.json")).save_snapshot(plain)
+    assert struct.unpack("<I", plain.read_bytes()[8:12])[0] == 1
+    got = json.loads(subprocess.run([str(bench), "--check-snapshot", str(plain)], check=True, capture_output=True,
+                                    text=True, timeout=60).stdout)
+    assert got["sdf_primitives"] == 0

@@ -51,7 +51,7 @@ uint64_t SceneSnapshot::triangle_count() const {
 namespace {
 
 constexpr char kMagic[8] = {'V', 'K', 'P', 'T', 'S', 'N', 'A', 'P'};
-constexpr uint32_t kVersion = 1;
+constexpr uint32_t kVersion = 1, kVersionSdf = 2;
 
 // Header after the magic: version, then the element counts of each array.
 struct SnapHeader {
@@ -90,12 +90,12 @@ bool load_snapshot_file(const std::string& path, SceneSnapshot& out, std::string
   SnapHeader h{};
   f.read(magic, sizeof(magic));
   f.read(reinterpret_cast<char*>(&h), sizeof(h));
-  if (!f || std::memcmp(magic, kMagic, sizeof(kMagic)) != 0 || h.version != kVersion) {
-    error = path + ": not a version-1 scene snapshot";
+  if (!f || std::memcmp(magic, kMagic, sizeof(kMagic)) != 0 || (h.version != kVersion && h.version != kVersionSdf)) {
+    error = path + ": not a version-1 or version-2 scene snapshot";
     return false;
   }
   SceneSnapshot s;
-  const bool ok = read_array(f, s.vertices, 3ull * h.vertex_count) && read_array(f, s.indices, h.index_count) &&
+  bool ok = read_array(f, s.vertices, 3ull * h.vertex_count) && read_array(f, s.indices, h.index_count) &&
                   read_array(f, s.local_vertices, 3ull * h.local_vertex_count) &&
                   read_array(f, s.local_indices, h.local_index_count) &&
                   read_array(f, s.instances, h.instance_count) && read_array(f, s.materials, h.material_count) &&
@@ -104,6 +104,11 @@ bool load_snapshot_file(const std::string& path, SceneSnapshot& out, std::string
                              3ull * h.environment_map_width * static_cast<uint64_t>(h.environment_map_height)) &&
                   read_array(f, s.texcoords, 2ull * h.texcoord_count) && read_array(f, s.texels, 3ull * h.texel_count) &&
                   read_array(f, s.textures, h.texture_count);
+  uint32_t sdf_count = 0;
+  if (ok && h.version == kVersionSdf) {
+    f.read(reinterpret_cast<char*>(&sdf_count), sizeof(sdf_count));
+    ok = static_cast<bool>(f) && read_array(f, s.sdf_primitives, sdf_count);
+  }
   if (!ok) {
     error = path + ": truncated scene snapshot";
     return false;
@@ -126,7 +131,7 @@ bool save_snapshot_file(const std::string& path, const SceneSnapshot& s, std::st
     return false;
   }
   SnapHeader h{};
-  h.version = kVersion;
+  h.version = s.sdf_primitives.empty() ? kVersion : kVersionSdf;
   h.vertex_count = static_cast<uint32_t>(s.vertices.size() / 3u);
   h.index_count = static_cast<uint32_t>(s.indices.size());
   h.local_vertex_count = static_cast<uint32_t>(s.local_vertices.size() / 3u);
@@ -157,6 +162,11 @@ bool save_snapshot_file(const std::string& path, const SceneSnapshot& s, std::st
   write_array(f, s.texcoords);
   write_array(f, s.texels);
   write_array(f, s.textures);
+  if (h.version == kVersionSdf) {
+    const uint32_t sdf_count = static_cast<uint32_t>(s.sdf_primitives.size());
+    f.write(reinterpret_cast<const char*>(&sdf_count), sizeof(sdf_count));
+    write_array(f, s.sdf_primitives);
+  }
   if (!f) {
     error = "write failed: " + path;
     return false;
@@ -206,6 +216,9 @@ Status HipPathTracer::load_scene_snapshot(const SceneSnapshot& scene) {
                               scene.texels.empty() ? nullptr : scene.texels.data(), scene.texels.size() / 3u,
                               scene.textures.empty() ? nullptr : scene.textures.data(),
                               static_cast<uint32_t>(scene.textures.size()));
+  if (rc == PT_OK && !scene.sdf_primitives.empty())
+    rc = pt_hip_load_sdf_primitives(handle_, scene.sdf_primitives.data(),
+                                    static_cast<uint32_t>(scene.sdf_primitives.size()));
   return to_status(rc);
 }
 

@@ -44,14 +44,17 @@ struct SceneSnapshot {
   PtCamera camera{};
   std::vector<float> texcoords, texels;
   std::vector<PtTextureDesc> textures;
+  std::vector<PtSdfPrimitive> sdf_primitives;  // loaded after the triangles (pt_hip_load_sdf_primitives)
 
   // The C view of the arrays (valid while the snapshot is alive and unchanged).
   PtSceneView view() const;
   uint64_t triangle_count() const;
 };
 
-// Binary snapshot files ("VKPTSNAP", version 1, little-endian): a header of counts, then every array
-// in SceneSnapshot's order. vkpathtracer_amd.scene.Scene.save_snapshot writes the same format.
+// Binary snapshot files ("VKPTSNAP", little-endian): a header of counts, then every array in
+// SceneSnapshot's order (version 1); version 2 appends a u32 count and the SDF primitive records.
+// Files without SDF primitives are written as version 1. vkpathtracer_amd.scene.Scene.save_snapshot
+// writes the same format.
 bool load_snapshot_file(const std::string& path, SceneSnapshot& out, std::string& error);
 bool save_snapshot_file(const std::string& path, const SceneSnapshot& snapshot, std::string& error);
 

@@ -167,14 +167,20 @@ int check_snapshot(const std::string& path) {
   for (float v : s.vertices) vsum += v;
   double tsum = 0.0;
   for (float v : s.texels) tsum += v;
+  double ssum = 0.0;  // every float field of the SDF records, plus shape and material index
+  for (const PtSdfPrimitive& p : s.sdf_primitives) {
+    ssum += p.shape + p.material_index + p.radius + p.param_a + p.param_b;
+    for (int k = 0; k < 3; ++k) ssum += p.position[k] + p.scale[k] + p.rotation[k];
+  }
   std::printf(
       "{\"vertices\": %zu, \"indices\": %zu, \"local_vertices\": %zu, \"local_indices\": %zu, \"instances\": %zu, "
       "\"materials\": %zu, \"lights\": %zu, \"environment_map\": [%u, %u], \"texcoords\": %zu, \"texels\": %zu, "
-      "\"textures\": %zu, \"triangles\": %llu, \"vertex_sum\": %.9g, \"texel_sum\": %.9g, \"fov_deg\": %.9g}\n",
+      "\"textures\": %zu, \"triangles\": %llu, \"vertex_sum\": %.9g, \"texel_sum\": %.9g, \"fov_deg\": %.9g, "
+      "\"sdf_primitives\": %zu, \"sdf_sum\": %.9g}\n",
       s.vertices.size() / 3, s.indices.size(), s.local_vertices.size() / 3, s.local_indices.size(), s.instances.size(),
       s.materials.size(), s.lights.size(), s.environment_map_width, s.environment_map_height, s.texcoords.size() / 2,
       s.texels.size() / 3, s.textures.size(), static_cast<unsigned long long>(s.triangle_count()), vsum, tsum,
-      static_cast<double>(s.camera.fov_deg));
+      static_cast<double>(s.camera.fov_deg), s.sdf_primitives.size(), ssum);
   return 0;
 }
 
@@ -371,7 +377,8 @@ int main(int argc, char** argv) {
   std::ofstream meta(o.out + "/metadata.json", std::ios::trunc);
   meta << "{\"run_id\":\"" << run_id << "\",\"command\":\"pt_hip_bench\",\"scene_path\":\"" << json_escape(o.scene)
        << "\",\"backend\":\"hip\",\"renderer_path\":\"hip-wavefront\",\"scene_name\":\"" << json_escape(scene_name)
-       << "\",\"schema\":\"VKPTSNAP 1\",\"material_count\":" << scene.materials.size()
+       << "\",\"schema\":\"VKPTSNAP " << (scene.sdf_primitives.empty() ? 1 : 2)
+       << "\",\"material_count\":" << scene.materials.size()
        << ",\"instance_count\":" << scene.instances.size() << ",\"triangle_count\":" << scene.triangle_count()
        << ",\"benchmark_capabilities\":{\"backends\":[\"hip\"],\"renderer_paths\":[\"hip-wavefront\"],"
           "\"tile_rendering\":true,\"tile_batching\":true,\"deterministic\":true,\"film_resolve\":\"gpu\","

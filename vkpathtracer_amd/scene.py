@@ -156,19 +156,23 @@ class Scene:
         return np.ascontiguousarray(texels, np.float32), descs
 
     def save_snapshot(self, path) -> None:
-        """Write the scene as a binary snapshot ("VKPTSNAP" version 1), the file format the C++ host
+        """Write the scene as a binary snapshot ("VKPTSNAP"), the file format the C++ host
         (vkpathtracer_amd/host, pt_hip_bench --scene) reads: magic, a header of counts, environment
-        colour and scale, the camera record, then every array in the C ABI's record layouts."""
+        colour and scale, the camera record, then every array in the C ABI's record layouts. A scene
+        with SDF primitives is written as version 2: version 1 followed by a u32 count and the
+        PtSdfPrimitive records; without them the file is version 1."""
         import struct
 
         env = self.environment_map
         ew, eh = (0, 0) if env is None else (env.shape[1], env.shape[0])
         texels, descs = self.texture_arrays()
         tc = self.texcoords if self.texcoords is not None else np.zeros((0, 2), np.float32)
+        sdf = self.sdf_primitives if self.sdf_primitives is not None else np.zeros(0, SDF_DTYPE)
+        version = 2 if len(sdf) else 1
         c = self.camera
         cam = struct.pack("<12fI3f", *c.position, *c.target, *c.up, c.fov_deg, c.aperture_radius, c.focus_distance,
                           c.iris_blade_count, c.iris_rotation_degrees, c.iris_roundness, c.anamorphic_squeeze)
-        head = struct.pack("<12IQ6f", 1, len(self.vertices), len(self.indices), len(self.local_vertices),
+        head = struct.pack("<12IQ6f", version, len(self.vertices), len(self.indices), len(self.local_vertices),
                            len(self.local_indices), len(self.instances), len(self.materials), len(self.lights), ew, eh,
                            len(tc), len(descs), len(texels), *self.environment_color, *self.environment_map_scale)
         with open(path, "wb") as f:
@@ -177,6 +181,8 @@ class Scene:
                       self.materials, self.lights, env if env is not None else np.zeros(0, np.float32), tc, texels,
                       descs):
                 f.write(np.ascontiguousarray(a).tobytes())
+            if version == 2:
+                f.write(struct.pack("<I", len(sdf)) + np.ascontiguousarray(sdf, SDF_DTYPE).tobytes())
 
     @property
     def triangle_count(self) -> int:
