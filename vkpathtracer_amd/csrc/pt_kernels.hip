@@ -1972,8 +1972,11 @@ __device__ __forceinline__ void light_visible(const DevPaths& P, const DevScene&
   }
 }
 
-template <bool kAdj, bool kSdf>
+// kAny: the scene has neither transmissive materials nor SDF primitives, so a shadow ray is one
+// any-hit query; compiled apart from the skip loop so that it keeps extend-like register use.
+template <bool kAdj, bool kSdf, bool kAny>
 __global__ __launch_bounds__(kBlock) void k_shadow(DevPaths P, DevScene sc, DevAdjoint A) {
+  static_assert(!(kAny && kSdf), "SDF shadows need the closest-hit skip loop");
   __shared__ int32_t s_ref[kStackLds * kBlock];
   __shared__ float s_t[kStackLds * kBlock];
   __shared__ uint32_t prefix[kSegments + 1];
@@ -1989,7 +1992,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevPaths P, DevScene sc, DevA
   TravStats st = {0, 0, 0, 0, 0};
   uint32_t q = 0;
   SdfStats sst = {0, 0, 0};
-  if (!kSdf && !sc.has_transmissive) {
+  if constexpr (kAny) {
     auto begin = [&](uint32_t j, Trav& s) {
       q = queue_pos(prefix, j, P.seg_cap);
       const V3 o = mk(P.sray[0][q], P.sray[1][q], P.sray[2][q]);
@@ -2277,17 +2280,22 @@ void launch_shade_adjoint(hipStream_t stream, const DevPaths& paths, const DevAd
 
 void launch_shadow(hipStream_t stream, const DevPaths& paths, const DevScene& scene, uint32_t grid) {
   if (scene.has_sdf)
-    hipLaunchKernelGGL((k_shadow<false, true>), dim3(grid), dim3(kBlock), 0, stream, paths, scene, DevAdjoint{});
+    hipLaunchKernelGGL((k_shadow<false, true, false>), dim3(grid), dim3(kBlock), 0, stream, paths, scene, DevAdjoint{});
+  else if (scene.has_transmissive)
+    hipLaunchKernelGGL((k_shadow<false, false, false>), dim3(grid), dim3(kBlock), 0, stream, paths, scene,
+                       DevAdjoint{});
   else
-    hipLaunchKernelGGL((k_shadow<false, false>), dim3(grid), dim3(kBlock), 0, stream, paths, scene, DevAdjoint{});
+    hipLaunchKernelGGL((k_shadow<false, false, true>), dim3(grid), dim3(kBlock), 0, stream, paths, scene, DevAdjoint{});
 }
 
 void launch_shadow_adjoint(hipStream_t stream, const DevPaths& paths, const DevAdjoint& adj, const DevScene& scene,
                            uint32_t grid) {
   if (scene.has_sdf)
-    hipLaunchKernelGGL((k_shadow<true, true>), dim3(grid), dim3(kBlock), 0, stream, paths, scene, adj);
+    hipLaunchKernelGGL((k_shadow<true, true, false>), dim3(grid), dim3(kBlock), 0, stream, paths, scene, adj);
+  else if (scene.has_transmissive)
+    hipLaunchKernelGGL((k_shadow<true, false, false>), dim3(grid), dim3(kBlock), 0, stream, paths, scene, adj);
   else
-    hipLaunchKernelGGL((k_shadow<true, false>), dim3(grid), dim3(kBlock), 0, stream, paths, scene, adj);
+    hipLaunchKernelGGL((k_shadow<true, false, true>), dim3(grid), dim3(kBlock), 0, stream, paths, scene, adj);
   hipLaunchKernelGGL(k_shadow_grad, dim3(kGradBlocks), dim3(kBlock), 0, stream, paths, scene, adj);
 }
 
@@ -2320,8 +2328,8 @@ static uint32_t occupancy_grid(int device, const void* fn) {
 Grids resident_grids(int device) {
   Grids g;
   g.extend = occupancy_grid(device, reinterpret_cast<const void*>(&k_extend));
-  g.shadow = occupancy_grid(device, reinterpret_cast<const void*>(&k_shadow<false, false>));
-  g.shadow_adj = occupancy_grid(device, reinterpret_cast<const void*>(&k_shadow<true, false>));
+  g.shadow = occupancy_grid(device, reinterpret_cast<const void*>(&k_shadow<false, false, true>));
+  g.shadow_adj = occupancy_grid(device, reinterpret_cast<const void*>(&k_shadow<true, false, true>));
   const uint32_t q0 = occupancy_grid(device, reinterpret_cast<const void*>(&k_trace_rays<false>));
   const uint32_t q1 = occupancy_grid(device, reinterpret_cast<const void*>(&k_trace_rays<true>));
   g.query = q0 < q1 ? q0 : q1;
