@@ -1974,6 +1974,9 @@ __device__ __forceinline__ void light_visible(const DevPaths& P, const DevScene&
 
 // kAny: the scene has neither transmissive materials nor SDF primitives, so a shadow ray is one
 // any-hit query; compiled apart from the skip loop so that it keeps extend-like register use.
+// Top bit of a shadow record's scene index: the forward any-hit query found the light unoccluded.
+constexpr uint32_t kShadowReached = 0x80000000u;
+
 template <bool kAdj, bool kSdf, bool kAny>
 __global__ __launch_bounds__(kBlock) void k_shadow(DevPaths P, DevScene sc, DevAdjoint A) {
   static_assert(!(kAny && kSdf), "SDF shadows need the closest-hit skip loop");
@@ -1993,14 +1996,21 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevPaths P, DevScene sc, DevA
   uint32_t q = 0;
   SdfStats sst = {0, 0, 0};
   if constexpr (kAny) {
+    uint32_t sid = 0;
     auto begin = [&](uint32_t j, Trav& s) {
       q = queue_pos(prefix, j, P.seg_cap);
+      sid = P.sscene[q];
       const V3 o = mk(P.sray[0][q], P.sray[1][q], P.sray[2][q]);
       const V3 d = mk(P.sray[3][q], P.sray[4][q], P.sray[5][q]);
-      return trav_begin(sc.recs[P.sscene[q]], s, o, d, P.stmax[q], st);
+      return trav_begin(sc.recs[sid], s, o, d, P.stmax[q], st);
     };
+    // forward: only mark the record (a store, no read-modify-write of the path inside the
+    // traversal loop); k_shadow_apply adds the contribution right after this kernel
     auto end = [&](uint32_t, Trav& s) {
-      if (s.best_prim < 0) visible(q);
+      if (s.best_prim < 0) {
+        if constexpr (kAdj) visible(q);
+        else P.sscene[q] = sid | kShadowReached;
+      }
       return false;
     };
     drive_queries<true>(sc, stk, P.queue, n, P.refill_min, st, begin, end);
@@ -2060,7 +2070,21 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevPaths P, DevScene sc, DevA
 
 // Adjoint passes: the radiance and gradient pieces of every shadow record whose light was visible.
 constexpr uint32_t kGradBlocks = 1024;
+constexpr uint32_t kShadowApplyBlocks = 1024;  // grid-stride blocks of k_shadow_apply
 static_assert(kGradBlocks <= kGradRows, "every block of an adjoint kernel owns a partial row");
+// Adds the light contribution of every shadow record k_shadow's forward any-hit query marked. A path
+// has at most one shadow record per bounce and this runs before the next shade, so every radiance
+// sum sees its additions in the same order as when the traversal kernel added them itself.
+__global__ __launch_bounds__(kBlock) void k_shadow_apply(DevPaths P, DevScene sc) {
+  __shared__ uint32_t prefix[kSegments + 1];
+  load_prefix(P.queue + kQueueShadowCursor, kSegLine, prefix);
+  const uint32_t n = prefix[kSegments];
+  for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
+    const uint32_t q = queue_pos(prefix, j, P.seg_cap);
+    if (P.sscene[q] & kShadowReached) light_visible<false>(P, sc, DevAdjoint{}, nullptr, q);
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_shadow_grad(DevPaths P, DevScene sc, DevAdjoint A) {
   __shared__ float glds[kAdjLdsFloats];
   __shared__ uint32_t prefix[kSegments + 1];
@@ -2284,8 +2308,10 @@ void launch_shadow(hipStream_t stream, const DevPaths& paths, const DevScene& sc
   else if (scene.has_transmissive)
     hipLaunchKernelGGL((k_shadow<false, false, false>), dim3(grid), dim3(kBlock), 0, stream, paths, scene,
                        DevAdjoint{});
-  else
+  else {
     hipLaunchKernelGGL((k_shadow<false, false, true>), dim3(grid), dim3(kBlock), 0, stream, paths, scene, DevAdjoint{});
+    hipLaunchKernelGGL(k_shadow_apply, dim3(kShadowApplyBlocks), dim3(kBlock), 0, stream, paths, scene);
+  }
 }
 
 void launch_shadow_adjoint(hipStream_t stream, const DevPaths& paths, const DevAdjoint& adj, const DevScene& scene,
