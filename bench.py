@@ -47,7 +47,7 @@ BYTES = {
 # (calibrated against a plain device copy: FETCH_SIZE x 2, WRITE_SIZE x 1), from
 # profiles/wavefront_procedural_1080p_64spp.md. They describe the current kernels; re-measure after
 # a kernel change.
-MEASURED_TRAFFIC = {"extend": 20.587e9, "shadow": 15.640e9, "shade": 8.163e9, "film": 1.677e9, "raygen": 8.494e9}
+MEASURED_TRAFFIC = {"extend": 20.585e9, "shadow": 10.763e9, "shade": 8.163e9, "film": 1.677e9, "raygen": 8.494e9}
 
 
 def cpu_model() -> str:
