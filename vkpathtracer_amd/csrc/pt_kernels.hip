@@ -1932,7 +1932,12 @@ __device__ __forceinline__ void shade_kernel(const DevPaths& P, const DevRays& i
   if constexpr (kAdj) grad_lds_flush(A, glds);
 }
 
-__global__ __launch_bounds__(kBlock) void k_shade(DevPaths P, DevRays in, DevRays out, DevScene sc, DevSettings st,
+// Waves per SIMD the forward shade is compiled for (its register budget).
+#ifndef PT_SHADE_WAVES
+#define PT_SHADE_WAVES 4
+#endif
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PT_SHADE_WAVES, 8))) void k_shade(
+    DevPaths P, DevRays in, DevRays out, DevScene sc, DevSettings st,
                                                  uint32_t depth) {
   shade_kernel<false>(P, in, out, sc, st, depth, DevAdjoint{}, nullptr);
 }
