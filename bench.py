@@ -47,6 +47,12 @@ BYTES = {
 # (calibrated against a plain device copy: FETCH_SIZE x 2, WRITE_SIZE x 1), from
 # profiles/wavefront_procedural_1080p_64spp.md. They describe the current kernels; re-measure after
 # a kernel change.
+# VALU issue of the same kernels from the same profile: VALU busy = SQ_INSTS_VALU x 4 cycles over the
+# SIMD cycles of the launch (gfx950 peak: one wave64 VALU instruction per SIMD every 4 cycles), lane
+# utilisation = SQ_THREAD_CYCLES_VALU / (64 x SQ_ACTIVE_INST_VALU), L2 hit = TCC_HIT / (TCC_HIT + TCC_MISS).
+MEASURED_VALU = {"extend": {"valu_busy": 0.99, "lane_utilisation": 0.54, "l2_hit": 0.791},
+                 "shadow": {"valu_busy": 0.89, "lane_utilisation": 0.46, "l2_hit": 0.817},
+                 "shade": {"valu_busy": 0.61, "lane_utilisation": 0.33, "l2_hit": 0.468}}
 MEASURED_TRAFFIC = {"extend": 20.585e9, "shadow": 10.763e9, "shade": 8.163e9, "film": 1.677e9, "raygen": 8.494e9}
 
 
@@ -470,7 +476,10 @@ def main():
                     if traffic else None,
                     "measured_hbm_gbps": round(traffic / (avg_ms * 1e6), 1) if traffic else None,
                     "bytes_per_launch": int(kb[dom] / k["launches"]), "bytes_per_unit": BYTES,
-                    "avg_launch_ms": round(avg_ms, 4)}
+                    "avg_launch_ms": round(avg_ms, 4),
+                    "valu": {n: MEASURED_VALU[n] for n in ("extend", "shadow", "shade")} if profiled else None,
+                    "valu_source": "profiles/wavefront_procedural_1080p_64spp.md (rocprofv3 SQ_* and TCC_* counters)"
+                    if profiled else None}
 
     log(f"timed {args.steps} steps: {dt:.3f} s")
     bvh_info = {k: pt.build_info[k] for k in ("node_count", "leaf_count", "max_depth", "device_bytes")}
