@@ -267,3 +267,30 @@ def test_state_machine_errors(cornell):
         assert pt.reset_accumulation()
         _, cnt = pt.film()
         assert cnt.sum() == 0 and pt.status().accumulation_gen == 1
+
+
+def test_chunk_cursor_resets_between_launches(small_procedural, tracer_factory):
+    """Persistent traversal waves take 64-query chunks from a shared cursor that the grid's last wave
+    resets. Back-to-back launches with empty, ragged and large query counts, closest and any hit, and
+    grids smaller and larger than the resident count must each see every query exactly once."""
+    from vkpathtracer_amd import scene as S
+
+    sc = small_procedural
+    pt = tracer_factory(sc, S.RenderSettings(width=8, height=8))
+    o = _oracle(sc)
+    rays = S.random_rays(sc, 70_000, seed=23)
+    tmax = np.random.default_rng(4).uniform(0.01, 8.0, len(rays)).astype(np.float32)
+    plan = [(0, 0), (1, 0), (63, 7), (64, 0), (65, 3), (70_000, 0), (0, 1), (129, 1), (20_000, 4096), (5, 2)]
+    for i, (n, blocks) in enumerate(plan):
+        pt.set_tuning(traversal_blocks=blocks)
+        any_hit = i % 2 == 1
+        r, tm = rays[:n], (tmax[:n] if any_hit else None)
+        print(f"chunk cursor launch {i}: {n} queries, blocks {blocks}, any_hit {any_hit}", flush=True)
+        t, p, c = pt.trace_rays(r, any_hit=any_hit, tmax=tm)
+        if n == 0:
+            assert len(t) == 0 and len(p) == 0
+            continue
+        ot, op, oc = o.trace_rays(r, any_hit=any_hit, tmax=tm)
+        assert np.array_equal(p, op) and np.array_equal(t, ot), i
+        assert c["bvh_node_visits"] == oc["bvh_node_visits"], i
+    pt.set_tuning(traversal_blocks=0)
