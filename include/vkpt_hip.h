@@ -187,6 +187,20 @@ typedef struct PtSampleCounters {
   uint64_t sdf_hits, sdf_misses, bvh_node_visits, bvh_leaf_visits, shadow_tests;
 } PtSampleCounters;
 
+/* RayQueryHit, PathTracer.h:976-983, as CpuBvhAccelerator::intersect fills it
+ * (CpuBvhAccelerator.cpp:446-452): on a hit, position = origin + direction * t, normal = the
+ * primitive's face normal normalize(e1 x e2), material_index = its instance's material index (not
+ * clamped to the material list) and primitive_index = its index in instance-flattening order; on a miss
+ * every field is zero (`out = {}`). hit is the bool as a 32-bit word. 40 bytes. */
+typedef struct PtRayQueryHit {
+  uint32_t hit;
+  float t;
+  float position[3];
+  float normal[3];
+  uint32_t material_index;
+  uint32_t primitive_index;
+} PtRayQueryHit;
+
 /* BVH build parameters. builder 0 = bucketed SAH (BvhBuildConfig,
  * src/gpu/D3D12GpuPathTracer.SceneBvh.h:15-19; defaults leaf 4, 8 buckets),
  * builder 1 = midpoint split (ParallelBvhBuilder, src/cpu/ParallelBvhBuilder.cpp:61-163). */
@@ -316,11 +330,57 @@ const char* pt_hip_last_error(const pt_tracer* tracer);
 int pt_hip_trace_rays(pt_tracer* tracer, int mode, const float* rays, const float* tmax, uint32_t count,
                       float* out_t, int32_t* out_prim, PtSampleCounters* counters);
 
+/* IRayAccelerator::intersect, PathTracer.h:1032 (CpuBvhAccelerator.cpp:360-458), with the whole
+ * RayQueryHit per ray (PtRayQueryHit). rays (6 floats each), tmax (per-ray limit or NULL = infinity)
+ * and hits may be host or device memory; they are staged through device buffers the tracer owns.
+ * mode 0 = closest hit (the reference's query), mode 1 = the first leaf batch with an accepted hit
+ * below tmax. stats (host, may be NULL) receive RayQueryStats' four counters (PathTracer.h:985-990)
+ * summed over the call, in the triangle_tests / triangle_hits / bvh_node_visits / bvh_leaf_visits
+ * fields. */
+int pt_hip_intersect(pt_tracer* tracer, int mode, const float* rays, const float* tmax, uint32_t count,
+                     PtRayQueryHit* hits, PtSampleCounters* stats);
+
 /* IPathTracer::update_scene_delta (PathTracer.h:1139; ApplySceneDeltaUpdate): replace the material
  * and / or light records without rebuilding geometry. Counts must equal the loaded scene's. Either
  * pointer may be NULL. Host pointers. */
 int pt_hip_update_scene_delta(pt_tracer* tracer, const PtMaterial* materials, uint32_t material_count,
                               const PtLight* lights, uint32_t light_count);
+
+/* RTSceneDeltaUpdate, PathTracer.h:433-461: per-index material and light records, per-instance material
+ * indices (an instance named by index, or by entity_id when the index is out of range) and the
+ * environment colour. Host pointers; a count of 0 with a NULL pointer is an empty list. */
+typedef struct PtMaterialUpdate {
+  uint32_t material_index, pad;
+  PtMaterial material;
+} PtMaterialUpdate;
+typedef struct PtLightUpdate {
+  uint32_t light_index;
+  PtLight light;
+} PtLightUpdate;
+typedef struct PtInstanceMaterialUpdate {
+  uint64_t entity_id;
+  uint32_t instance_index; /* 0xFFFFFFFF (kInvalidRTInstanceIndex): look the instance up by entity_id */
+  uint32_t material_index;
+} PtInstanceMaterialUpdate;
+typedef struct PtSceneDeltaUpdate {
+  const PtMaterialUpdate* materials;
+  const PtLightUpdate* lights;
+  const PtInstanceMaterialUpdate* instance_materials;
+  uint32_t material_count, light_count, instance_material_count;
+  uint32_t environment_color_changed;
+  float environment_color[3];
+  uint32_t pad;
+} PtSceneDeltaUpdate;
+
+/* IPathTracer::update_scene_delta, PathTracer.h:1139, as ScalarCpuPathTracer::update_scene_delta
+ * (ScalarCpuPathTracer.cpp:701-720) applies ApplySceneDeltaUpdate (PathTracer.cpp:427-482): every index
+ * is checked first (PT_INVALID_ARGUMENT, nothing changed), then the records are replaced. Materials,
+ * lights and the environment colour reach the next render; instance materials are baked into the
+ * acceleration structure's primitives, as the reference's accelerator bakes them at build
+ * (CpuBvhAccelerator.cpp:232-238), so they reach renders after the next pt_hip_build_accel. An update
+ * that changes nothing (no records, and instance materials equal to the current ones) returns
+ * PT_INVALID_ARGUMENT with "scene delta did not apply", as the reference returns false. */
+int pt_hip_apply_scene_delta(pt_tracer* tracer, const PtSceneDeltaUpdate* update);
 
 /* Path-replay adjoint of render_tiles (no counterpart in the reference, which has no
  * differentiation). For the same tiles and frame, replays every path with the same random
@@ -452,7 +512,9 @@ int pt_bvh_build_host(const PtSceneView* scene, const PtBvhParams* params, uint3
 /* ABI sizes, for bindings to check their mirrors of the records above. */
 uint32_t pt_hip_sizeof(int which); /* 0 material, 1 light, 2 instance, 3 camera, 4 scene, 5 settings, 6 counters,
                                       7 tile, 8 tuning, 9 texture, 10 film resolve,
-                                      11 instance transform update, 12 SDF primitive */
+                                      11 instance transform update, 12 SDF primitive, 13 ray query hit,
+                                      14 scene delta update, 15 material update, 16 light update,
+                                      17 instance material update */
 const char* pt_hip_version(void);
 
 #ifdef __cplusplus

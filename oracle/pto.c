@@ -2091,8 +2091,11 @@ int pto_render_pixels(const PtoScene* scene, const pto_accel* a, const PtoSettin
       const int ok = shade_pixel(&x, &cam, pixels[i], sample_index, frame_index, &v, &rays, &lc, NULL);
       rgb[3 * i] = v.x; rgb[3 * i + 1] = v.y; rgb[3 * i + 2] = v.z;
       out_valid[i] = (uint8_t)ok;
-      lc.rays += rays;
-      if (ok) lc.samples++;
+      /* shade_pixel adds ray_count only for a finite sample (Render.cpp:270-281) */
+      if (ok) {
+        lc.rays += rays;
+        lc.samples++;
+      }
     }
 #pragma omp critical
     add_counters(&total, &lc);
@@ -2136,8 +2139,10 @@ int pto_render_tile(const PtoScene* scene, const pto_accel* a, const PtoSettings
         v3 v;
         uint64_t rays = 0;
         const int ok = shade_pixel(&x, &cam, pixel, sample, frame_index, &v, &rays, &lc, NULL);
+        /* shade_pixel drops a non-finite sample before FilmBuffer::add_sample and before it adds the
+         * sample's rays to ray_count (Render.cpp:270-281) */
+        if (!ok) continue;
         lc.rays += rays;
-        if (!ok) continue; /* shade_pixel drops non-finite samples before FilmBuffer::add_sample */
         lc.samples++;
         real* f = film + 3u * (uint64_t)pixel;
         f[0] = f[0] + v.x;

@@ -131,3 +131,36 @@ def test_autotuned_batch_renders_the_same_films(cornell, small_procedural):
         for i in range(len(jobs)):
             assert np.array_equal(tb.film(i)[0], b.film(i)[0]), i
     tb.close()
+
+
+def test_graph_replay_invalidates_reused_forward_radiance(cornell):
+    """A graph captured on frame 1, an eager render of frame 0, a replay of the graph for frame 2, then
+    the adjoint of frame 0: the replay rewrote every path's radiance, so the adjoint must replay frame 0
+    itself. Its gradients equal those of a batch
+    that only runs the adjoint of frame 0 (up to the order of the fp32 atomic adds)."""
+    import torch
+
+    from vkpathtracer_amd import scene as S
+    from vkpathtracer_amd.tracer import HipBatch
+
+    settings = S.RenderSettings(width=1, height=1, spp=1, max_depth=8)
+    w, h, spp = 40, 32, 2
+    nm, nl = len(cornell.materials), len(cornell.lights)
+    dev = torch.device("cuda", 0)
+    fg = torch.from_numpy(np.random.default_rng(11).normal(size=(h * w, 3)).astype(np.float32)).to(dev)
+
+    def grads(sequence):
+        with HipBatch(settings) as b:
+            b.add_scene(cornell, w, h, spp)
+            b.build()
+            for f, g in sequence:
+                b.render(f, use_graph=g)
+            b.synchronize()
+            out = [torch.zeros((n, 3), device=dev) for n in (nm, nm, nl)]
+            b.render_adjoint(fg, *out, frame_index=0)
+            return [x.cpu().numpy().astype(np.float64) for x in out]
+
+    ref = grads([])
+    got = grads([(1, True), (0, False), (2, True)])
+    for a, r in zip(got, ref):
+        assert np.abs(a - r).max() <= 1e-5 * max(np.abs(r).max(), 1e-6)

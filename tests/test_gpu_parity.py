@@ -294,3 +294,68 @@ def test_chunk_cursor_resets_between_launches(small_procedural, tracer_factory):
         assert np.array_equal(p, op) and np.array_equal(t, ot), i
         assert c["bvh_node_visits"] == oc["bvh_node_visits"], i
     pt.set_tuning(traversal_blocks=0)
+
+
+def test_dropped_nonfinite_samples_add_no_rays(tracer_factory):
+    """Infinite environment texels make many samples non-finite. shade_pixel drops them before it adds
+    their rays (ScalarCpuPathTracer.Render.cpp:270-281): film, counts and every counter, rays included,
+    equal the oracle, which follows that rule (tests/test_oracle_counters.py)."""
+    from vkpathtracer_amd import scene as S
+    from test_oracle_counters import nonfinite_env_scene
+
+    sc = nonfinite_env_scene()
+    st = S.RenderSettings(width=48, height=36, spp=4, max_depth=8)
+    pt = tracer_factory(sc, st)
+    assert pt.render_frame_samples(0, st.spp), pt.status().last_error
+    film, cnt = pt.film()
+    counters = pt.read_counters()
+    rf, rc, rcnt = _oracle(sc).render(st)
+    assert int(rc.sum()) < st.width * st.height * st.spp  # some samples were dropped
+    assert np.array_equal(cnt, rc)
+    assert np.array_equal(film, rf)
+    for k in ("samples", "rays", "shadow_tests", "bvh_node_visits", "bvh_leaf_visits", "triangle_tests",
+              "triangle_hits"):
+        assert counters[k] == rcnt[k], (k, counters[k], rcnt[k])
+
+
+def _misnumbered(scene):
+    """The scene with its first instance naming a material beyond the material list: shading clamps
+    the index to the last material, RayQueryHit::material_index keeps it as given."""
+    import dataclasses
+
+    inst = scene.instances.copy()
+    inst[0]["material_index"] = len(scene.materials) + 5
+    return dataclasses.replace(scene, instances=inst)
+
+
+@pytest.mark.parametrize("builder", ["midpoint", "sah"])
+def test_intersect_ray_query_hits_equal_oracle(small_procedural, tracer_factory, builder):
+    """pt_hip_intersect fills RayQueryHit as CpuBvhAccelerator::intersect does (CpuBvhAccelerator.cpp:
+    446-452): hit, t, position = o + d t, the primitive's face normal, its instance's unclamped material
+    index and its primitive index; a miss leaves every field zero. RayQueryStats equal the oracle's."""
+    from vkpathtracer_amd import scene as S
+
+    sc = _misnumbered(small_procedural)
+    pt = tracer_factory(sc, S.RenderSettings(width=8, height=8), builder=builder)
+    rays = S.random_rays(sc, 60_000, seed=21)
+    rays[::7, 3:] *= -1.0  # some rays leave the scene
+    orc = _oracle(sc, builder=builder)
+    geo = orc.export()
+    for any_hit in (False, True):
+        hits, stats = pt.intersect(rays, any_hit=any_hit)
+        t, p, c = orc.trace_rays(rays, any_hit=any_hit)
+        hit = p >= 0
+        assert hit.any() and (~hit).any()
+        assert np.array_equal(hits["hit"].astype(bool), hit)
+        assert np.array_equal(hits["primitive_index"][hit], p[hit].astype(np.uint32))
+        assert np.array_equal(hits["t"][hit], t[hit])
+        pos = rays[hit, :3] + rays[hit, 3:] * t[hit, None]  # float32, unfused: add_vec3(o, mul_vec3(d, t))
+        assert np.array_equal(hits["position"][hit], pos)
+        assert np.array_equal(hits["normal"][hit], geo["geo"][p[hit], 9:12].astype(np.float32))
+        assert np.array_equal(hits["material_index"][hit], geo["material"][p[hit]])
+        assert (hits["material_index"][hit] >= len(sc.materials)).any()  # the unclamped index travels
+        miss = hits[~hit]
+        for f in ("t", "position", "normal", "material_index", "primitive_index"):
+            assert not np.any(miss[f]), f
+        for k in ("triangle_tests", "triangle_hits", "bvh_node_visits", "bvh_leaf_visits"):
+            assert stats[k] == c[k], (any_hit, k, stats[k], c[k])

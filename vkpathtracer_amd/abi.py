@@ -147,6 +147,26 @@ def make_resolve(r: "_scene.FilmResolve | None") -> PtFilmResolve:
     return out
 
 
+# PtRayQueryHit (RayQueryHit, PathTracer.h:976-983): 40 bytes.
+RAY_QUERY_HIT_DTYPE = np.dtype([("hit", np.uint32), ("t", np.float32), ("position", np.float32, 3),
+                                ("normal", np.float32, 3), ("material_index", np.uint32),
+                                ("primitive_index", np.uint32)])
+# RTMaterialUpdate / RTLightUpdate / RTInstanceMaterialUpdate (PathTracer.h:438-452).
+MATERIAL_UPDATE_DTYPE = np.dtype([("material_index", np.uint32), ("pad", np.uint32),
+                                  ("material", _scene.MATERIAL_DTYPE)])
+LIGHT_UPDATE_DTYPE = np.dtype([("light_index", np.uint32), ("light", _scene.LIGHT_DTYPE)])
+INSTANCE_MATERIAL_UPDATE_DTYPE = np.dtype([("entity_id", np.uint64), ("instance_index", np.uint32),
+                                           ("material_index", np.uint32)])
+INVALID_INSTANCE_INDEX = 0xFFFFFFFF  # kInvalidRTInstanceIndex
+
+
+class PtSceneDeltaUpdate(C.Structure):
+    _fields_ = [("materials", C.c_void_p), ("lights", C.c_void_p), ("instance_materials", C.c_void_p),
+                ("material_count", C.c_uint32), ("light_count", C.c_uint32),
+                ("instance_material_count", C.c_uint32), ("environment_color_changed", C.c_uint32),
+                ("environment_color", C.c_float * 3), ("pad", C.c_uint32)]
+
+
 KERNELS = ("raygen", "extend", "shade", "shadow", "film")
 
 
@@ -184,6 +204,8 @@ SIGNATURES = {
     "pt_hip_shutdown": (C.c_int, [_P]),
     "pt_hip_last_error": (C.c_char_p, [_P]),
     "pt_hip_trace_rays": (C.c_int, [_P, C.c_int, _P, _P, C.c_uint32, _P, _P, C.POINTER(PtSampleCounters)]),
+    "pt_hip_intersect": (C.c_int, [_P, C.c_int, _P, _P, C.c_uint32, _P, C.POINTER(PtSampleCounters)]),
+    "pt_hip_apply_scene_delta": (C.c_int, [_P, C.POINTER(PtSceneDeltaUpdate)]),
     "pt_hip_synchronize": (C.c_int, [_P]),
     "pt_hip_set_wavefront_capacity": (C.c_int, [_P, C.c_uint32]),
     "pt_hip_set_profiling": (C.c_int, [_P, C.c_int]),

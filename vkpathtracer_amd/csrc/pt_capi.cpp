@@ -125,6 +125,11 @@ struct pt_tracer {
   DevArray<uint32_t> d_film_count, d_queue, d_error;
   DevArray<uint32_t> d_resolve;  // resolve_hdr / resolve_ldr output before its copy to the caller
   DevArray<unsigned long long> d_counters, d_query_counters;
+  // pt_hip_intersect: staging for rays, limits and RayQueryHit records, and the instance material of
+  // every primitive as the last build baked it (rebuilt after each build)
+  DevArray<uint8_t> d_iq;
+  DevArray<uint32_t> d_inst_mat;
+  bool inst_mat_ready = false;
   DevArray<float> d_env;
   DevArray<float> d_path_f;
   DevArray<uint32_t> d_path_u;
@@ -594,6 +599,7 @@ std::vector<PtMaterial> pool_materials(const SceneSlot& s, uint32_t tex_base) {
 // records. Films are laid out scene after scene (film_base); a tracer's single scene starts at
 // pixel 0. A view shares its scene's pool entries and has its own camera and film.
 int upload_pool(pt_tracer* t) {
+  t->inst_mat_ready = false;
   t->refit_ready = false;
   PT_HIP(t, hipStreamSynchronize(t->stream), "synchronize");
   free_scene_device(t);
@@ -1337,6 +1343,11 @@ uint32_t pt_hip_sizeof(int which) {
     case 10: return sizeof(PtFilmResolve);
     case 11: return sizeof(PtInstanceTransformUpdate);
     case 12: return sizeof(PtSdfPrimitive);
+    case 13: return sizeof(PtRayQueryHit);
+    case 14: return sizeof(PtSceneDeltaUpdate);
+    case 15: return sizeof(PtMaterialUpdate);
+    case 16: return sizeof(PtLightUpdate);
+    case 17: return sizeof(PtInstanceMaterialUpdate);
     default: return 0;
   }
 }
@@ -1704,6 +1715,64 @@ int pt_hip_trace_rays(pt_tracer* t, int mode, const float* rays, const float* tm
   return PT_OK;
 }
 
+int pt_hip_intersect(pt_tracer* t, int mode, const float* rays, const float* tmax, uint32_t count, PtRayQueryHit* hits,
+                     PtSampleCounters* stats) {
+  if (!t) return PT_INVALID_ARGUMENT;
+  if (t->batch) return fail(t, PT_INVALID_ARGUMENT, "intersect: not available on a batch");
+  if (!t->accel_valid) return fail(t, PT_NOT_READY, "intersect requires build_or_update_acceleration");
+  if (count && (!rays || !hits)) return fail(t, PT_INVALID_ARGUMENT, "intersect: null buffer");
+  if (mode != 0 && mode != 1) return fail(t, PT_INVALID_ARGUMENT, "intersect: mode must be 0 or 1");
+  (void)hipSetDevice(t->device);
+  if (!t->inst_mat_ready) {  // RayQueryHit::material_index: the instance material baked at build
+    const std::vector<ptb::Prim>& prims = t->scenes[0].bvh.prims;
+    std::vector<uint32_t> m(prims.size());
+    for (size_t i = 0; i < prims.size(); ++i) m[i] = prims[i].material;
+    PT_HIP(t, hipStreamSynchronize(t->stream), "synchronize");
+    PT_HIP(t, t->d_inst_mat.ensure(m.size()), "allocate instance materials");
+    if (!m.empty())
+      PT_HIP(t, hipMemcpy(t->d_inst_mat.ptr, m.data(), m.size() * sizeof(uint32_t), hipMemcpyHostToDevice),
+             "upload instance materials");
+    t->inst_mat_ready = true;
+  }
+  const size_t ray_bytes = 24u * static_cast<size_t>(count), tmax_bytes = tmax ? 4u * static_cast<size_t>(count) : 0u;
+  const size_t hit_off = (ray_bytes + tmax_bytes + 255u) & ~static_cast<size_t>(255u);
+  const size_t total = hit_off + sizeof(PtRayQueryHit) * static_cast<size_t>(count);
+  if (count) {
+    // the previous call's copies read d_iq on this stream: the stream's order keeps them first
+    PT_HIP(t, t->d_iq.ensure(total), "allocate ray query staging");
+    PT_HIP(t, hipMemcpyAsync(t->d_iq.ptr, rays, ray_bytes, hipMemcpyDefault, t->stream), "stage rays");
+    if (tmax)
+      PT_HIP(t, hipMemcpyAsync(t->d_iq.ptr + ray_bytes, tmax, tmax_bytes, hipMemcpyDefault, t->stream), "stage limits");
+  }
+  ptd::DevPaths q = t->paths;
+  q.counters = t->d_query_counters.ptr;
+  ptd::launch_zero_counters(t->stream, q.counters);
+  if (count) {
+    const float* drays = reinterpret_cast<const float*>(t->d_iq.ptr);
+    const float* dtmax = tmax ? reinterpret_cast<const float*>(t->d_iq.ptr + ray_bytes) : nullptr;
+    PtRayQueryHit* dhits = reinterpret_cast<PtRayQueryHit*>(t->d_iq.ptr + hit_off);
+    ptd::launch_intersect(t->stream, q, t->dscene, mode, drays, dtmax, count, dhits, t->d_inst_mat.ptr,
+                          t->grids.query);
+    const hipError_t le = hipGetLastError();
+    if (le != hipSuccess) return hip_fail(t, le, "kernel launch");
+    PT_HIP(t, hipMemcpyAsync(hits, dhits, sizeof(PtRayQueryHit) * static_cast<size_t>(count), hipMemcpyDefault,
+                             t->stream),
+           "copy ray query hits");
+  }
+  unsigned long long c[ptd::kCounterSlots];
+  const int rc = read_counter_block(t, q.counters, c);
+  if (rc != PT_OK) return rc;
+  if (stats) {
+    *stats = PtSampleCounters{};
+    stats->triangle_tests = c[ptd::kTriTests];
+    stats->triangle_hits = c[ptd::kTriHits];
+    stats->bvh_node_visits = c[ptd::kNodeVisits];
+    stats->bvh_leaf_visits = c[ptd::kLeafVisits];
+  }
+  t->last_error.clear();
+  return PT_OK;
+}
+
 int pt_hip_update_camera(pt_tracer* t, const PtCamera* camera) {
   if (!t || !camera) return fail(t, PT_INVALID_ARGUMENT, "update_camera: null argument");
   if (!t->configured || !t->scene_loaded || t->scenes.size() != 1u)
@@ -1956,6 +2025,88 @@ int pt_hip_update_scene_delta(pt_tracer* t, const PtMaterial* materials, uint32_
     PT_HIP(t, hipMemcpy(t->d_lights.ptr, lights, sizeof(PtLight) * light_count, hipMemcpyHostToDevice),
            "upload lights");
   }
+  ++t->scene_version;
+  t->last_error.clear();
+  return PT_OK;
+}
+
+int pt_hip_apply_scene_delta(pt_tracer* t, const PtSceneDeltaUpdate* u) {
+  if (!t || !u) return fail(t, PT_INVALID_ARGUMENT, "update_scene_delta: null argument");
+  if (!t->configured || !t->scene_loaded || t->scenes.size() != 1u)
+    return fail(t, PT_NOT_READY, "update_scene_delta requires a configured tracer with a scene");
+  if ((u->material_count && !u->materials) || (u->light_count && !u->lights) ||
+      (u->instance_material_count && !u->instance_materials))
+    return fail(t, PT_INVALID_ARGUMENT, "update_scene_delta: null list");
+  SceneSlot& s = t->scenes[0];
+  // ApplySceneDeltaUpdate (PathTracer.cpp:427-482): check every index before changing anything
+  auto instance_of = [&](const PtInstanceMaterialUpdate& im) -> uint32_t {
+    uint32_t idx = im.instance_index;
+    if (idx >= s.instances.size() && im.entity_id != 0u)
+      for (size_t k = 0; k < s.instances.size(); ++k)
+        if (s.instances[k].entity_id == im.entity_id) {
+          idx = static_cast<uint32_t>(k);
+          break;
+        }
+    return idx;
+  };
+  for (uint32_t i = 0; i < u->material_count; ++i)
+    if (u->materials[i].material_index >= s.materials.size())
+      return fail(t, PT_INVALID_ARGUMENT, "scene delta did not apply: material index out of range");
+  for (uint32_t i = 0; i < u->light_count; ++i)
+    if (u->lights[i].light_index >= s.lights.size())
+      return fail(t, PT_INVALID_ARGUMENT, "scene delta did not apply: light index out of range");
+  for (uint32_t i = 0; i < u->instance_material_count; ++i)
+    if (instance_of(u->instance_materials[i]) >= s.instances.size() ||
+        u->instance_materials[i].material_index >= s.materials.size())
+      return fail(t, PT_INVALID_ARGUMENT, "scene delta did not apply: instance or material index out of range");
+  bool changed = false;
+  for (uint32_t i = 0; i < u->material_count; ++i) {
+    s.materials[u->materials[i].material_index] = u->materials[i].material;
+    changed = true;
+  }
+  for (uint32_t i = 0; i < u->light_count; ++i) {
+    s.lights[u->lights[i].light_index] = u->lights[i].light;
+    changed = true;
+  }
+  for (uint32_t i = 0; i < u->instance_material_count; ++i) {
+    PtInstance& inst = s.instances[instance_of(u->instance_materials[i])];
+    if (inst.material_index != u->instance_materials[i].material_index) {
+      inst.material_index = u->instance_materials[i].material_index;  // baked at the next build
+      changed = true;
+    }
+  }
+  if (u->environment_color_changed) {
+    for (int k = 0; k < 3; ++k) s.view.environment_color[k] = u->environment_color[k];
+    changed = true;
+  }
+  if (!changed) return fail(t, PT_INVALID_ARGUMENT, "scene delta did not apply");
+  if (!t->accel_valid) {  // nothing on the device yet: the next build uploads the snapshot as it now is
+    ++t->scene_version;
+    t->last_error.clear();
+    return PT_OK;
+  }
+  (void)hipSetDevice(t->device);
+  PT_HIP(t, hipStreamSynchronize(t->stream), "synchronize");
+  if (u->material_count) {
+    const std::vector<PtMaterial> pm = pool_materials(s, 0u);
+    PT_HIP(t, hipMemcpy(t->d_materials.ptr, pm.data(), sizeof(PtMaterial) * pm.size(), hipMemcpyHostToDevice),
+           "upload materials");
+    uint32_t any = 0, textured = 0;
+    for (const PtMaterial& m : s.materials)
+      if (transmissive(m)) any = 1;
+    for (const PtMaterial& m : pm)
+      if (m.base_color_texture_index != 0xFFFFFFFFu) textured = 1;
+    t->dscene.textured = textured;
+    t->dscene.has_transmissive = any;
+    PT_HIP(t, hipMemcpy(&t->d_recs.ptr[0].has_transmissive, &any, sizeof(uint32_t), hipMemcpyHostToDevice),
+           "upload scene record");
+  }
+  if (u->light_count)
+    PT_HIP(t, hipMemcpy(t->d_lights.ptr, s.lights.data(), sizeof(PtLight) * s.lights.size(), hipMemcpyHostToDevice),
+           "upload lights");
+  if (u->environment_color_changed)
+    PT_HIP(t, hipMemcpy(t->d_recs.ptr[0].env, s.view.environment_color, 3 * sizeof(float), hipMemcpyHostToDevice),
+           "upload environment colour");
   ++t->scene_version;
   t->last_error.clear();
   return PT_OK;
@@ -2282,6 +2433,9 @@ int pt_batch_render(pt_batch* b, uint64_t frame_index, int use_graph) {
     }
     const int rc = stage_upload(t, t->d_frame.ptr, &frame_index, sizeof(uint64_t));
     if (rc != PT_OK) return rc;
+    // the replay rewrites every path's radiance: an adjoint must not take it for an earlier eager
+    // render's
+    t->fwd_rad_valid = false;
     PT_HIP(t, hipGraphLaunch(t->graph_exec, t->stream), "launch graph");
   }
   if (t->profiling) {

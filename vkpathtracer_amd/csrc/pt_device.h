@@ -268,6 +268,9 @@ void launch_film(hipStream_t stream, const DevPaths& paths, const DevScene& scen
 void launch_trace_rays(hipStream_t stream, const DevPaths& paths, const DevScene& scene, int any_hit,
                        const float* rays, const float* tmax, uint32_t count, float* out_t, int32_t* out_prim,
                        uint32_t grid);
+// The same queries, each storing a whole RayQueryHit (inst_mat: the instance material of each primitive).
+void launch_intersect(hipStream_t stream, const DevPaths& paths, const DevScene& scene, int any_hit, const float* rays,
+                      const float* tmax, uint32_t count, PtRayQueryHit* hits, const uint32_t* inst_mat, uint32_t grid);
 // Adjoint variants: raygen also gathers the per-path loss adjoint; shade and shadow accumulate
 // gradients; copy_radiance saves the forward radiance of a pass before the replay.
 void launch_raygen_adjoint(hipStream_t stream, const DevPaths& paths, const DevAdjoint& adj, const DevScene& scene,

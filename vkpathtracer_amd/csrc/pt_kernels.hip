@@ -1898,6 +1898,11 @@ __device__ __forceinline__ void shade_kernel(const DevPaths& P, const DevRays& i
       const V3 rd = mk(in.c[3][q], in.c[4][q], in.c[5][q]);
       shade_path<kAdj>(P, scene_view(sc, sid), st, depth, p, ro, rd, P.hit_t[q], P.hit_prim[q], so,
                        shadow_tests, A, glds);
+      // A path that ends here traced depth + 1 extension rays. Its throughput is dead, so its first
+      // word keeps that count for k_film, which takes the rays of a dropped (non-finite) sample back
+      // out of the ray counter (shade_pixel adds ray_count only for finite samples).
+      if constexpr (!kAdj)
+        if (!so.cont) P.tp[0][p] = __uint_as_float(depth + 1u);
     }
     const uint32_t seg = (w0 >> 6) & (kSegments - 1u);  // this chunk's output segment
     const uint32_t slot = segment_append(P.queue + kQueueOutCursor, seg, P.seg_cap, so.cont);
@@ -2132,10 +2137,13 @@ __global__ __launch_bounds__(64) void k_film(DevPaths P, DevScene sc, DevFilm fi
   const SceneRec& R = sc.recs[J.scene];
   const uint32_t px = R.film_base + y * R.width + x;
   float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  uint32_t cnt = 0, added = 0;
+  uint32_t cnt = 0, added = 0, dropped_rays = 0;
   bool touched = false;
-  auto add = [&](float r, float g, float b) {
-    if (!(finite(r) && finite(g) && finite(b))) return;
+  auto add = [&](float r, float g, float b, uint32_t slot) {
+    if (!(finite(r) && finite(g) && finite(b))) {  // shade_pixel drops it, rays included (Render.cpp:270-281)
+      dropped_rays += __float_as_uint(P.tp[0][slot]);
+      return;
+    }
     if (!touched) {
       s = film.sum[px];
       cnt = film.count[px];
@@ -2166,7 +2174,7 @@ __global__ __launch_bounds__(64) void k_film(DevPaths P, DevScene sc, DevFilm fi
       if (valid)
         for (uint32_t q = 0; q < kn; ++q) {
           const uint32_t o = lane * (kFilmChunk + 1u) + q;
-          add(stage[0][o], stage[1][o], stage[2][o]);
+          add(stage[0][o], stage[1][o], stage[2][o], base + local * G + kc + q);
         }
       __syncthreads();
     }
@@ -2175,7 +2183,7 @@ __global__ __launch_bounds__(64) void k_film(DevPaths P, DevScene sc, DevFilm fi
       const DevTile& t = tiles[J.first_tile + k];
       if (x < t.x || y < t.y || x - t.x >= t.w || y - t.y >= t.h) continue;
       const uint32_t p = tile_slot(t, (y - t.y) * t.w + (x - t.x));
-      add(P.rad[0][p], P.rad[1][p], P.rad[2][p]);
+      add(P.rad[0][p], P.rad[1][p], P.rad[2][p], p);
     }
   }
   if (touched) {
@@ -2183,12 +2191,50 @@ __global__ __launch_bounds__(64) void k_film(DevPaths P, DevScene sc, DevFilm fi
     film.count[px] = cnt;
   }
   wave_add(P.counters, kSamples, added);
+  // k_extend counted every extension ray; the u64 counter wraps back to the finite samples' rays
+  wave_add(P.counters, kRays, 0ull - static_cast<uint64_t>(dropped_rays));
 }
 
-// Batched IRayAccelerator::intersect for the ray-query entry point (rays are 6 floats each).
-template <bool kAny>
+// Batched IRayAccelerator::intersect for the ray-query entry points (rays are 6 floats each). The
+// output policy stores one finished query: t and primitive arrays, or whole RayQueryHit records.
+struct HitArrays {
+  float* t;
+  int32_t* prim;
+  __device__ __forceinline__ void put(uint32_t j, const float*, const Trav& s, const DevScene&) const {
+    t[j] = s.best_prim >= 0 ? s.best_t : kInf;
+    prim[j] = s.best_prim;
+  }
+};
+
+// RayQueryHit as CpuBvhAccelerator::intersect fills it (CpuBvhAccelerator.cpp:446-452): position =
+// add_vec3(origin, mul_vec3(direction, t)), the primitive's face normal, its instance's material
+// index (inst_mat, unclamped) and primitive index; `out = {}` on a miss.
+struct HitRecords {
+  PtRayQueryHit* out;
+  const uint32_t* inst_mat;  // per primitive
+  __device__ __forceinline__ void put(uint32_t j, const float* r, const Trav& s, const DevScene& sc) const {
+    PtRayQueryHit h = {};
+    if (s.best_prim >= 0) {
+      const float t = s.best_t;
+      const float4 pn = sc.prims[s.best_prim];
+      h.hit = 1u;
+      h.t = t;
+      h.position[0] = r[0] + r[3] * t;
+      h.position[1] = r[1] + r[4] * t;
+      h.position[2] = r[2] + r[5] * t;
+      h.normal[0] = pn.x;
+      h.normal[1] = pn.y;
+      h.normal[2] = pn.z;
+      h.material_index = inst_mat[s.best_prim];
+      h.primitive_index = static_cast<uint32_t>(s.best_prim);
+    }
+    out[j] = h;
+  }
+};
+
+template <bool kAny, typename Out>
 __global__ __launch_bounds__(kBlock) void k_trace_rays(DevPaths P, DevScene sc, const float* rays, const float* tmax,
-                                                      uint32_t n, float* out_t, int32_t* out_prim) {
+                                                      uint32_t n, Out out) {
   __shared__ int32_t s_ref[kStackLds * kBlock];
   __shared__ float s_t[kStackLds * kBlock];
   const Stack stk = make_stack(P, s_ref, s_t);
@@ -2198,8 +2244,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_rays(DevPaths P, DevScene sc, 
     return trav_begin(sc.recs[0], s, mk(r[0], r[1], r[2]), mk(r[3], r[4], r[5]), tmax ? tmax[j] : kInf, st);
   };
   auto end = [&](uint32_t j, Trav& s) {
-    out_t[j] = s.best_prim >= 0 ? s.best_t : kInf;
-    out_prim[j] = s.best_prim;
+    out.put(j, rays + 6u * static_cast<size_t>(j), s, sc);
     return false;
   };
   drive_queries<kAny>(sc, stk, P.queue, n, P.refill_min, st, begin, end);
@@ -2340,12 +2385,24 @@ void launch_film(hipStream_t stream, const DevPaths& paths, const DevScene& scen
 void launch_trace_rays(hipStream_t stream, const DevPaths& paths, const DevScene& scene, int any_hit,
                        const float* rays, const float* tmax, uint32_t count, float* out_t, int32_t* out_prim,
                        uint32_t grid) {
+  const HitArrays out{out_t, out_prim};
   if (any_hit)
-    hipLaunchKernelGGL(k_trace_rays<true>, dim3(grid), dim3(kBlock), 0, stream, paths, scene, rays, tmax, count, out_t,
-                       out_prim);
+    hipLaunchKernelGGL((k_trace_rays<true, HitArrays>), dim3(grid), dim3(kBlock), 0, stream, paths, scene, rays, tmax,
+                       count, out);
   else
-    hipLaunchKernelGGL(k_trace_rays<false>, dim3(grid), dim3(kBlock), 0, stream, paths, scene, rays, tmax, count,
-                       out_t, out_prim);
+    hipLaunchKernelGGL((k_trace_rays<false, HitArrays>), dim3(grid), dim3(kBlock), 0, stream, paths, scene, rays, tmax,
+                       count, out);
+}
+
+void launch_intersect(hipStream_t stream, const DevPaths& paths, const DevScene& scene, int any_hit, const float* rays,
+                      const float* tmax, uint32_t count, PtRayQueryHit* hits, const uint32_t* inst_mat, uint32_t grid) {
+  const HitRecords out{hits, inst_mat};
+  if (any_hit)
+    hipLaunchKernelGGL((k_trace_rays<true, HitRecords>), dim3(grid), dim3(kBlock), 0, stream, paths, scene, rays, tmax,
+                       count, out);
+  else
+    hipLaunchKernelGGL((k_trace_rays<false, HitRecords>), dim3(grid), dim3(kBlock), 0, stream, paths, scene, rays,
+                       tmax, count, out);
 }
 
 static uint32_t occupancy_grid(int device, const void* fn) {
@@ -2361,9 +2418,15 @@ Grids resident_grids(int device) {
   g.extend = occupancy_grid(device, reinterpret_cast<const void*>(&k_extend));
   g.shadow = occupancy_grid(device, reinterpret_cast<const void*>(&k_shadow<false, false, true>));
   g.shadow_adj = occupancy_grid(device, reinterpret_cast<const void*>(&k_shadow<true, false, true>));
-  const uint32_t q0 = occupancy_grid(device, reinterpret_cast<const void*>(&k_trace_rays<false>));
-  const uint32_t q1 = occupancy_grid(device, reinterpret_cast<const void*>(&k_trace_rays<true>));
-  g.query = q0 < q1 ? q0 : q1;
+  const void* queries[4] = {reinterpret_cast<const void*>(&k_trace_rays<false, HitArrays>),
+                            reinterpret_cast<const void*>(&k_trace_rays<true, HitArrays>),
+                            reinterpret_cast<const void*>(&k_trace_rays<false, HitRecords>),
+                            reinterpret_cast<const void*>(&k_trace_rays<true, HitRecords>)};
+  g.query = ~0u;
+  for (const void* fn : queries) {
+    const uint32_t q = occupancy_grid(device, fn);
+    g.query = q < g.query ? q : g.query;
+  }
   g.shade = occupancy_grid(device, reinterpret_cast<const void*>(&k_shade));
   g.shade_adj = occupancy_grid(device, reinterpret_cast<const void*>(&k_shade_adj));
   return g;

@@ -252,6 +252,20 @@ bool HipPathTracer::update_scene_delta(std::span<const PtMaterial> materials, st
                                              static_cast<uint32_t>(lights.size())));
 }
 
+bool HipPathTracer::update_scene_delta(const SceneDeltaUpdate& update) {
+  if (!handle_) return false;
+  PtSceneDeltaUpdate u{};
+  u.materials = update.materials.empty() ? nullptr : update.materials.data();
+  u.lights = update.lights.empty() ? nullptr : update.lights.data();
+  u.instance_materials = update.instance_materials.empty() ? nullptr : update.instance_materials.data();
+  u.material_count = static_cast<uint32_t>(update.materials.size());
+  u.light_count = static_cast<uint32_t>(update.lights.size());
+  u.instance_material_count = static_cast<uint32_t>(update.instance_materials.size());
+  u.environment_color_changed = update.environment_color_changed ? 1u : 0u;
+  for (int k = 0; k < 3; ++k) u.environment_color[k] = update.environment_color[k];
+  return fail_bool(pt_hip_apply_scene_delta(handle_, &u));
+}
+
 FilmHdr HipPathTracer::resolve_hdr() const {
   FilmHdr out;
   if (!handle_) return out;
@@ -322,7 +336,13 @@ PtTuning HipPathTracer::tuning() const {
 }
 
 // ------------------------------------------------------------------ HipRayAccelerator
-Status HipRayAccelerator::build(const SceneSnapshot& scene, const PtBvhParams* params) {
+bool HipRayAccelerator::build(const SceneSnapshot& scene, bool deterministic) {
+  return build(scene, deterministic, PtBvhParams{PT_BVH_MIDPOINT, 4u, 8u, 0u});
+}
+
+bool HipRayAccelerator::build(const SceneSnapshot& scene, bool deterministic, const PtBvhParams& params) {
+  reset();
+  info_.deterministic = deterministic;
   PtRenderSettings st{};
   st.width = 1;
   st.height = 1;
@@ -332,17 +352,72 @@ Status HipRayAccelerator::build(const SceneSnapshot& scene, const PtBvhParams* p
   st.rr_max_survival = 0.99f;
   Status s = tracer_.configure(st);
   if (s) s = tracer_.load_scene_snapshot(scene);
-  if (s) s = tracer_.build_or_update_acceleration(params);
-  return s;
+  if (s) s = tracer_.build_or_update_acceleration(&params);
+  if (!s) {
+    error_ = s.message;
+    return false;
+  }
+  const PtBuildInfo& b = tracer_.build_info();
+  info_.built = b.built != 0u;
+  info_.primitive_count = b.primitive_count;
+  info_.node_count = b.node_count;
+  info_.leaf_count = b.leaf_count;
+  info_.build_ms = b.build_ms;
+  error_.clear();
+  return info_.built && (b.node_count > 0u || b.primitive_count == 0u);
 }
 
-Status HipRayAccelerator::intersect(bool any_hit, const float* rays, const float* tmax, uint32_t count, float* out_t,
-                                    int32_t* out_prim, PtSampleCounters* counters) {
+bool HipRayAccelerator::intersect(const Ray& ray, RayQueryHit& out, RayQueryStats* stats) const {
+  return intersect(std::span<const Ray>(&ray, 1), std::span<RayQueryHit>(&out, 1), stats) && out.hit;
+}
+
+bool HipRayAccelerator::intersect(std::span<const Ray> rays, std::span<RayQueryHit> out, RayQueryStats* stats) const {
+  for (RayQueryHit& h : out) h = RayQueryHit{};
+  if (!tracer_.handle() || !info_.built || out.size() != rays.size()) {
+    error_ = out.size() != rays.size() ? "intersect: output and ray counts differ" : "intersect before build";
+    return false;
+  }
+  if (rays.empty()) return true;
+  static_assert(sizeof(Ray) == 24 && sizeof(Vec3) == 12, "Ray must be 6 packed floats");
+  std::vector<PtRayQueryHit> hits(rays.size());
+  PtSampleCounters c{};
+  const int rc = pt_hip_intersect(tracer_.handle(), 0, reinterpret_cast<const float*>(rays.data()), nullptr,
+                                  static_cast<uint32_t>(rays.size()), hits.data(), &c);
+  if (rc != PT_OK) {
+    error_ = pt_hip_last_error(tracer_.handle());
+    return false;
+  }
+  for (std::size_t i = 0; i < hits.size(); ++i) {
+    const PtRayQueryHit& h = hits[i];
+    RayQueryHit& o = out[i];
+    o.hit = h.hit != 0u;
+    o.t = h.t;
+    o.position = Vec3{h.position[0], h.position[1], h.position[2]};
+    o.normal = Vec3{h.normal[0], h.normal[1], h.normal[2]};
+    o.material_index = h.material_index;
+    o.primitive_index = h.primitive_index;
+  }
+  if (stats) {  // the reference's intersect adds to the caller's stats
+    stats->triangle_tests += c.triangle_tests;
+    stats->triangle_hits += c.triangle_hits;
+    stats->bvh_node_visits += c.bvh_node_visits;
+    stats->bvh_leaf_visits += c.bvh_leaf_visits;
+  }
+  return true;
+}
+
+Status HipRayAccelerator::trace_rays(bool any_hit, const float* rays, const float* tmax, uint32_t count, float* out_t,
+                                     int32_t* out_prim, PtSampleCounters* counters) {
   if (!tracer_.handle()) return Status{PT_NOT_READY, tracer_.status().last_error};
   Status s;
   s.code = pt_hip_trace_rays(tracer_.handle(), any_hit ? 1 : 0, rays, tmax, count, out_t, out_prim, counters);
   if (!s) s.message = pt_hip_last_error(tracer_.handle());
   return s;
+}
+
+void HipRayAccelerator::reset() {
+  tracer_.shutdown();
+  info_ = RayAcceleratorBuildInfo{};
 }
 
 }  // namespace vkpt_hip

@@ -9,6 +9,7 @@
 // any GPU code: every call goes through libvkpt_hip.so.
 #pragma once
 
+#include <cstddef>
 #include <cstdint>
 #include <span>
 #include <stop_token>
@@ -74,6 +75,16 @@ struct FilmLdr {  // FilmLdr / ApplyFilmResolve (FilmBuffer.cpp:255-332)
   std::vector<uint8_t> rgba8;  // 4 per pixel
 };
 
+// RTSceneDeltaUpdate (PathTracer.h:455-461): per-index material and light records, per-instance material
+// indices and the environment colour.
+struct SceneDeltaUpdate {
+  std::vector<PtMaterialUpdate> materials;
+  std::vector<PtLightUpdate> lights;
+  std::vector<PtInstanceMaterialUpdate> instance_materials;
+  bool environment_color_changed = false;
+  float environment_color[3] = {0.0f, 0.0f, 0.0f};
+};
+
 struct TracerStatus {  // PathTracerStatus (PathTracer.h:830-851) and the last error message
   PtTracerStatus state{};
   std::string last_error;
@@ -100,6 +111,9 @@ class HipPathTracer {
   bool supports_tile_rendering() const { return true; }
   bool supports_tile_batching() const { return true; }
   bool update_scene_delta(std::span<const PtMaterial> materials, std::span<const PtLight> lights);
+  // IPathTracer::update_scene_delta(const RTSceneDeltaUpdate&) (PathTracer.h:1139): false when an index is
+  // out of range or nothing changed, as ScalarCpuPathTracer::update_scene_delta.
+  bool update_scene_delta(const SceneDeltaUpdate& update);
   FilmHdr resolve_hdr() const;
   // IPathTracer::resolve_ldr with the given FilmResolveSettings (nullptr: the defaults).
   FilmLdr resolve_ldr(const PtFilmResolve* settings = nullptr) const;
@@ -125,20 +139,63 @@ class HipPathTracer {
   mutable std::string error_;
 };
 
-// IRayAccelerator (PathTracer.h:1001-1036): build over a snapshot, then batched closest-hit or
-// any-hit queries on device buffers (pt_hip_trace_rays).
+// Ray, RayQueryHit, RayQueryStats and RayAcceleratorBuildInfo (PathTracer.h:30-53, :976-999), with the
+// reference's field names so that an IRayAccelerator adapter copies them member for member.
+struct Vec3 {
+  float x = 0.0f, y = 0.0f, z = 0.0f;
+};
+struct Ray {
+  Vec3 origin{};
+  Vec3 direction{};
+};
+struct RayQueryHit {
+  bool hit = false;
+  float t = 0.0f;
+  Vec3 position{};
+  Vec3 normal{};
+  uint32_t material_index = 0;
+  uint32_t primitive_index = 0;
+};
+struct RayQueryStats {
+  uint64_t triangle_tests = 0;
+  uint64_t triangle_hits = 0;
+  uint64_t bvh_node_visits = 0;
+  uint64_t bvh_leaf_visits = 0;
+};
+struct RayAcceleratorBuildInfo {
+  bool built = false;
+  std::size_t primitive_count = 0;
+  std::size_t node_count = 0;
+  std::size_t leaf_count = 0;
+  double build_ms = 0.0;
+  bool deterministic = false;
+};
+
+// IRayAccelerator (PathTracer.h:1001-1036) over the GPU traversal. build() takes CpuBvhAccelerator's
+// tree (midpoint splits, leaves of 4: ParallelBvhBuilder, CpuBvhAccelerator.cpp:252-256) unless a
+// PtBvhParams names another; intersect() answers one query as the reference does (RayQueryHit filled,
+// stats added to) and the span overload answers a batch in one launch; trace_rays() is the device-buffer
+// form (t and primitive only). The builds are deterministic whatever the flag says; build_info()
+// reports the flag as given, as the reference does.
 class HipRayAccelerator {
  public:
   explicit HipRayAccelerator(int device = 0) : tracer_(device) {}
-  Status build(const SceneSnapshot& scene, const PtBvhParams* params = nullptr);
+  bool build(const SceneSnapshot& scene, bool deterministic);
+  bool build(const SceneSnapshot& scene, bool deterministic, const PtBvhParams& params);
+  bool intersect(const Ray& ray, RayQueryHit& out, RayQueryStats* stats = nullptr) const;
+  // One launch for all rays; out.size() must equal rays.size(). False (and every hit cleared) on an error.
+  bool intersect(std::span<const Ray> rays, std::span<RayQueryHit> out, RayQueryStats* stats = nullptr) const;
   // rays: 6 floats per ray (origin, direction); tmax: per-ray limit or nullptr; all device pointers.
-  Status intersect(bool any_hit, const float* rays, const float* tmax, uint32_t count, float* out_t,
-                   int32_t* out_prim, PtSampleCounters* counters = nullptr);
-  const PtBuildInfo& build_info() const { return tracer_.build_info(); }
-  void reset() { tracer_.shutdown(); }
+  Status trace_rays(bool any_hit, const float* rays, const float* tmax, uint32_t count, float* out_t,
+                    int32_t* out_prim, PtSampleCounters* counters = nullptr);
+  RayAcceleratorBuildInfo build_info() const { return info_; }
+  void reset();
+  const std::string& last_error() const { return error_; }
 
  private:
   HipPathTracer tracer_;
+  RayAcceleratorBuildInfo info_{};
+  mutable std::string error_;
 };
 
 }  // namespace vkpt_hip

@@ -19,6 +19,7 @@
 //                [--output-transform gamma|linear] [--gamma G] [--no-clamp]
 //   pt_hip_bench --check-snapshot FILE     (no device: print the snapshot's counts as JSON)
 //   pt_hip_bench --validate-artifacts DIR  (no device: ValidateBenchmarkArtifactsOnDisk's report)
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -26,6 +27,8 @@
 #include <cstring>
 #include <filesystem>
 #include <fstream>
+#include <iterator>
+#include <span>
 #include <string>
 #include <vector>
 
@@ -35,7 +38,8 @@
 namespace {
 
 struct Options {
-  std::string scene, out = "pt_hip_bench_out", check, validate;
+  std::string scene, out = "pt_hip_bench_out", check, validate, ray_queries;
+  uint32_t single = 256;  // --ray-queries: rays also answered one intersect() call each
   PtFilmResolve resolve{1.0f, 6500.0f, 0u, 0u, 2.2f, 1u};  // FilmResolveSettings{}
   uint32_t width = 256, height = 256, spp = 16, max_depth = 8, rr_start = 3, leaf = 4, warmup = 1, frames = 1;
   uint64_t seed = 0xBADC0FFEEull;
@@ -50,6 +54,8 @@ int usage() {
                "                    [--leaf L] [--warmup F] [--frames F] [--per-sample-launches] [--device I]\n"
                "                    [--out DIR] [--exposure E] [--white-balance K] [--tone-map NAME]\n"
                "                    [--output-transform gamma|linear] [--gamma G] [--no-clamp]\n"
+               "       pt_hip_bench --scene FILE.ptsnap --ray-queries RAYS.f32 [--single K] [--builder B]\n"
+               "                    [--leaf L] [--out DIR]\n"
                "       pt_hip_bench --check-snapshot FILE\n"
                "       pt_hip_bench --validate-artifacts DIR\n");
   return 2;
@@ -117,6 +123,8 @@ bool parse(int argc, char** argv, Options& o) {
     else if (a == "--warmup") ok = num("--warmup", o.warmup);
     else if (a == "--frames") ok = num("--frames", o.frames);
     else if (a == "--device") ok = num("--device", o.device);
+    else if (a == "--ray-queries") { const char* v = next("--ray-queries"); ok = v; if (v) o.ray_queries = v; }
+    else if (a == "--single") ok = num("--single", o.single);
     else if (a == "--no-nee") o.nee = false;
     else if (a == "--no-mis") o.mis = false;
     else if (a == "--per-sample-launches") o.per_sample = true;
@@ -206,6 +214,77 @@ bool write_film(const std::string& dir, const vkpt_hip::Film& film, const vkpt_h
   return static_cast<bool>(f);
 }
 
+// IRayAccelerator queries (PathTracer.h:1001-1036) through HipRayAccelerator: the rays of a raw file
+// (6 little-endian floats each) answered in one batched intersect, the first `single` of them again by
+// one intersect(Ray, RayQueryHit&, RayQueryStats*) call each. Writes DIR/hits.bin (PtRayQueryHit records
+// of the batch) and DIR/ray_queries.json (build info, summed stats of both forms, and how many of the
+// single-ray answers differ from the batch's).
+int run_ray_queries(const Options& o, const vkpt_hip::SceneSnapshot& scene) {
+  std::ifstream f(o.ray_queries, std::ios::binary);
+  std::vector<char> raw((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+  if (!f.good() && !f.eof()) {
+    std::fprintf(stderr, "cannot read %s\n", o.ray_queries.c_str());
+    return 2;
+  }
+  if (raw.size() % sizeof(vkpt_hip::Ray) != 0) {
+    std::fprintf(stderr, "%s: size is not a multiple of 24 bytes\n", o.ray_queries.c_str());
+    return 2;
+  }
+  std::vector<vkpt_hip::Ray> rays(raw.size() / sizeof(vkpt_hip::Ray));
+  if (!rays.empty()) std::memcpy(rays.data(), raw.data(), raw.size());
+  vkpt_hip::HipRayAccelerator accel(o.device);
+  const PtBvhParams bp{o.midpoint ? PT_BVH_MIDPOINT : PT_BVH_SAH, o.leaf, 8u, 0u};
+  if (!accel.build(scene, true, bp)) {
+    std::fprintf(stderr, "build failed: %s\n", accel.last_error().c_str());
+    return 2;
+  }
+  std::vector<vkpt_hip::RayQueryHit> hits(rays.size());
+  vkpt_hip::RayQueryStats batch_stats{}, single_stats{};
+  if (!accel.intersect(std::span<const vkpt_hip::Ray>(rays), std::span<vkpt_hip::RayQueryHit>(hits), &batch_stats)) {
+    std::fprintf(stderr, "intersect failed: %s\n", accel.last_error().c_str());
+    return 2;
+  }
+  const size_t single = std::min<size_t>(o.single, rays.size());
+  size_t mismatched = 0, single_hits = 0;
+  for (size_t i = 0; i < single; ++i) {
+    vkpt_hip::RayQueryHit h;
+    const bool hit = accel.intersect(rays[i], h, &single_stats);
+    single_hits += hit ? 1u : 0u;
+    const vkpt_hip::RayQueryHit& b = hits[i];
+    if (hit != b.hit || std::memcmp(&h.t, &b.t, sizeof(float)) != 0 || h.primitive_index != b.primitive_index ||
+        h.material_index != b.material_index || std::memcmp(&h.position, &b.position, sizeof(h.position)) != 0 ||
+        std::memcmp(&h.normal, &b.normal, sizeof(h.normal)) != 0)
+      ++mismatched;
+  }
+  std::filesystem::create_directories(o.out);
+  std::vector<PtRayQueryHit> recs(hits.size());
+  for (size_t i = 0; i < hits.size(); ++i) {
+    const vkpt_hip::RayQueryHit& h = hits[i];
+    recs[i] = PtRayQueryHit{h.hit ? 1u : 0u, h.t, {h.position.x, h.position.y, h.position.z},
+                            {h.normal.x, h.normal.y, h.normal.z}, h.material_index, h.primitive_index};
+  }
+  std::ofstream hb(std::filesystem::path(o.out) / "hits.bin", std::ios::binary);
+  hb.write(reinterpret_cast<const char*>(recs.data()), static_cast<std::streamsize>(recs.size() * sizeof(PtRayQueryHit)));
+  const vkpt_hip::RayAcceleratorBuildInfo bi = accel.build_info();
+  std::ofstream js(std::filesystem::path(o.out) / "ray_queries.json");
+  auto stats_json = [](const vkpt_hip::RayQueryStats& s) {
+    return "{\"triangle_tests\": " + std::to_string(s.triangle_tests) + ", \"triangle_hits\": " +
+           std::to_string(s.triangle_hits) + ", \"bvh_node_visits\": " + std::to_string(s.bvh_node_visits) +
+           ", \"bvh_leaf_visits\": " + std::to_string(s.bvh_leaf_visits) + "}";
+  };
+  js << "{\"rays\": " << rays.size() << ", \"single\": " << single << ", \"single_hits\": " << single_hits
+     << ", \"single_mismatched\": " << mismatched << ", \"build_info\": {\"built\": " << (bi.built ? "true" : "false")
+     << ", \"primitive_count\": " << bi.primitive_count << ", \"node_count\": " << bi.node_count
+     << ", \"leaf_count\": " << bi.leaf_count << ", \"deterministic\": " << (bi.deterministic ? "true" : "false")
+     << "}, \"batch_stats\": " << stats_json(batch_stats) << ", \"single_stats\": " << stats_json(single_stats)
+     << "}\n";
+  if (!hb || !js) {
+    std::fprintf(stderr, "cannot write %s\n", o.out.c_str());
+    return 2;
+  }
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -225,6 +304,7 @@ int main(int argc, char** argv) {
     std::fprintf(stderr, "%s\n", err.c_str());
     return 2;
   }
+  if (!o.ray_queries.empty()) return run_ray_queries(o, scene);
   vkpt_hip::HipPathTracer tracer(o.device);
   if (!tracer.valid()) {
     std::fprintf(stderr, "%s\n", tracer.status().last_error.c_str());

@@ -247,6 +247,37 @@ class HipPathTracer:
                                                  None if l is None else l.ctypes.data, 0 if l is None else len(l))
         return rc == 0
 
+    def apply_scene_delta(self, materials=None, lights=None, instance_materials=None,
+                          environment_color=None) -> bool:
+        """IPathTracer::update_scene_delta with an RTSceneDeltaUpdate (PathTracer.h:455-461):
+        materials = [(index, record)], lights = [(index, record)], instance_materials =
+        [(instance_index, material_index)] or [(entity_id, INVALID_INSTANCE_INDEX, material_index)],
+        environment_color = rgb or None. False (status message kept) when an index is out of range or
+        nothing changed, as ScalarCpuPathTracer::update_scene_delta returns false."""
+        mu = np.zeros(len(materials or ()), abi.MATERIAL_UPDATE_DTYPE)
+        for i, (k, rec) in enumerate(materials or ()):
+            mu[i]["material_index"] = k
+            mu[i]["material"] = rec
+        lu = np.zeros(len(lights or ()), abi.LIGHT_UPDATE_DTYPE)
+        for i, (k, rec) in enumerate(lights or ()):
+            lu[i]["light_index"] = k
+            lu[i]["light"] = rec
+        iu = np.zeros(len(instance_materials or ()), abi.INSTANCE_MATERIAL_UPDATE_DTYPE)
+        for i, e in enumerate(instance_materials or ()):
+            if len(e) == 2:
+                iu[i]["instance_index"], iu[i]["material_index"] = e
+            else:
+                iu[i]["entity_id"], iu[i]["instance_index"], iu[i]["material_index"] = e
+        u = abi.PtSceneDeltaUpdate()
+        u.materials = mu.ctypes.data if len(mu) else None
+        u.lights = lu.ctypes.data if len(lu) else None
+        u.instance_materials = iu.ctypes.data if len(iu) else None
+        u.material_count, u.light_count, u.instance_material_count = len(mu), len(lu), len(iu)
+        if environment_color is not None:
+            u.environment_color_changed = 1
+            u.environment_color[:] = [float(c) for c in environment_color]
+        return self._lib.pt_hip_apply_scene_delta(self._h, C.byref(u)) == 0
+
     def render_adjoint(self, tiles, film_grad, grad_albedo, grad_emission, grad_light, frame_index: int = 0,
                        grad_roughness=None, grad_texels=None):
         """Path-replay adjoint of render_tiles(tiles): adds d(sum film_grad * film)/d(parameters) into the
@@ -378,6 +409,19 @@ class HipPathTracer:
                                          n, out_t.data_ptr(), out_p.data_ptr(), C.byref(c))
         self._check(rc, "trace_rays")
         return out_t.cpu().numpy(), out_p.cpu().numpy(), c.as_dict()
+
+    def intersect(self, rays, any_hit: bool = False, tmax=None):
+        """IRayAccelerator::intersect over an (N, 6) numpy array of rays (host memory, staged by the
+        library): (RAY_QUERY_HIT_DTYPE records, RayQueryStats as a dict)."""
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+        tm = None if tmax is None else np.ascontiguousarray(tmax, np.float32)
+        hits = np.zeros(len(r), abi.RAY_QUERY_HIT_DTYPE)
+        c = abi.PtSampleCounters()
+        rc = self._lib.pt_hip_intersect(self._h, 1 if any_hit else 0, r.ctypes.data,
+                                        None if tm is None else tm.ctypes.data, len(r), hits.ctypes.data, C.byref(c))
+        self._check(rc, "intersect")
+        d = c.as_dict()
+        return hits, {k: d[k] for k in ("triangle_tests", "triangle_hits", "bvh_node_visits", "bvh_leaf_visits")}
 
 
 class HipBatch:
