@@ -330,6 +330,19 @@ const char* pt_hip_last_error(const pt_tracer* tracer);
 int pt_hip_trace_rays(pt_tracer* tracer, int mode, const float* rays, const float* tmax, uint32_t count,
                       float* out_t, int32_t* out_prim, PtSampleCounters* counters);
 
+/* pt_hip_trace_rays without any allocation (no counterpart in the reference: for callers that cannot
+ * allocate device memory — code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments, results and counters as pt_hip_trace_rays; rays, tmax, out_t and out_prim
+ * are device pointers. The traversal's scratch (counter stripes, queue cursor, error flag and the stack
+ * spill area of a 512-workgroup grid) lives in one static 24 MiB device array per GPU, set up when
+ * the first tracer on that GPU is created and held until the process ends. Trees deeper than
+ * PT_NOALLOC_MAX_TREE_DEPTH levels are rejected with PT_UNSUPPORTED. Every call on one GPU shares that
+ * GPU's array: calls that overlap on one GPU (two streams, two threads) race and must be serialised by
+ * the caller. */
+#define PT_NOALLOC_MAX_TREE_DEPTH 32
+int pt_hip_trace_rays_noalloc(pt_tracer* tracer, int mode, const float* rays, const float* tmax, uint32_t count,
+                              float* out_t, int32_t* out_prim, PtSampleCounters* counters);
+
 /* IRayAccelerator::intersect, PathTracer.h:1032 (CpuBvhAccelerator.cpp:360-458), with the whole
  * RayQueryHit per ray (PtRayQueryHit). rays (6 floats each), tmax (per-ray limit or NULL = infinity)
  * and hits may be host or device memory; they are staged through device buffers the tracer owns.

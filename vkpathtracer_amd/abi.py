@@ -204,6 +204,7 @@ SIGNATURES = {
     "pt_hip_shutdown": (C.c_int, [_P]),
     "pt_hip_last_error": (C.c_char_p, [_P]),
     "pt_hip_trace_rays": (C.c_int, [_P, C.c_int, _P, _P, C.c_uint32, _P, _P, C.POINTER(PtSampleCounters)]),
+    "pt_hip_trace_rays_noalloc": (C.c_int, [_P, C.c_int, _P, _P, C.c_uint32, _P, _P, C.POINTER(PtSampleCounters)]),
     "pt_hip_intersect": (C.c_int, [_P, C.c_int, _P, _P, C.c_uint32, _P, C.POINTER(PtSampleCounters)]),
     "pt_hip_apply_scene_delta": (C.c_int, [_P, C.POINTER(PtSceneDeltaUpdate)]),
     "pt_hip_synchronize": (C.c_int, [_P]),

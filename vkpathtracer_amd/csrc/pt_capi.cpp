@@ -128,6 +128,7 @@ struct pt_tracer {
   // pt_hip_intersect: staging for rays, limits and RayQueryHit records, and the instance material of
   // every primitive as the last build baked it (rebuilt after each build)
   DevArray<uint8_t> d_iq;
+  hipError_t noalloc_status = hipErrorNotInitialized;  // pt_hip_trace_rays_noalloc's scratch lookup
   DevArray<uint32_t> d_inst_mat;
   bool inst_mat_ready = false;
   DevArray<float> d_env;
@@ -1058,6 +1059,10 @@ int init_tracer(pt_tracer* t, int device) {
   t->paths.error = t->d_error.ptr;
   t->paths.refill_min = t->refill_min;
   t->grids = t->grids_max = ptd::resident_grids(device);
+  // the allocation-free ray queries' static scratch on this device (looked up once per device)
+  t->noalloc_status = ptd::noalloc_setup(device);
+  // error messages of the allocation-free entry point fit without growing the string
+  t->last_error.reserve(256);
   return PT_OK;
 }
 
@@ -1712,6 +1717,42 @@ int pt_hip_trace_rays(pt_tracer* t, int mode, const float* rays, const float* tm
   const int rc = read_counter_block(t, q.counters, c);
   if (rc != PT_OK) return rc;
   if (counters) to_counters(c, counters);
+  return PT_OK;
+}
+
+int pt_hip_trace_rays_noalloc(pt_tracer* t, int mode, const float* rays, const float* tmax, uint32_t count,
+                              float* out_t, int32_t* out_prim, PtSampleCounters* counters) {
+  if (!t) return PT_INVALID_ARGUMENT;
+  if (t->batch) return fail(t, PT_INVALID_ARGUMENT, "trace_rays_noalloc: not available on a batch");
+  if (!t->accel_valid) return fail(t, PT_NOT_READY, "trace_rays_noalloc requires build_or_update_acceleration");
+  if (count && (!rays || !out_t || !out_prim)) return fail(t, PT_INVALID_ARGUMENT, "trace_rays_noalloc: null buffer");
+  if (mode != 0 && mode != 1) return fail(t, PT_INVALID_ARGUMENT, "trace_rays_noalloc: mode must be 0 or 1");
+  if (t->info.max_depth > PT_NOALLOC_MAX_TREE_DEPTH)
+    return fail(t, PT_UNSUPPORTED, "trace_rays_noalloc: tree deeper than PT_NOALLOC_MAX_TREE_DEPTH levels");
+  if (t->noalloc_status != hipSuccess) return hip_fail(t, t->noalloc_status, "trace_rays_noalloc: scratch lookup");
+  (void)hipSetDevice(t->device);
+  const ptd::NoAllocScratch sc = ptd::noalloc_scratch(t->device);
+  if (!sc.counters) return fail(t, PT_INTERNAL_ERROR, "trace_rays_noalloc: no scratch on this device");
+  ptd::launch_trace_rays_noalloc(t->stream, sc, t->paths.refill_min, t->dscene, mode, rays, tmax, count, out_t,
+                                 out_prim, t->grids.query);
+  const hipError_t le = hipGetLastError();
+  if (le != hipSuccess) return hip_fail(t, le, "kernel launch");
+  unsigned long long raw[ptd::kCounterWords];
+  uint32_t error = 0;
+  PT_HIP(t, hipMemcpyAsync(raw, sc.counters, sizeof(raw), hipMemcpyDeviceToHost, t->stream), "read counters");
+  PT_HIP(t, hipMemcpyAsync(&error, sc.error, sizeof(error), hipMemcpyDeviceToHost, t->stream), "read error flag");
+  PT_HIP(t, hipStreamSynchronize(t->stream), "synchronize");
+  if (error) return fail(t, PT_INTERNAL_ERROR, "trace_rays_noalloc: traversal stack exceeded its spill area");
+  if (counters) {
+    unsigned long long c[ptd::kCounterSlots];
+    for (int k = 0; k < ptd::kCounterSlots; ++k) {
+      unsigned long long v = 0;
+      for (int st = 0; st < ptd::kStripes; ++st) v += raw[static_cast<size_t>(st) * ptd::kCounterSlots + k];
+      c[k] = v;
+    }
+    to_counters(c, counters);
+  }
+  t->last_error.clear();
   return PT_OK;
 }
 

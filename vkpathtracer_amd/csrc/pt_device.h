@@ -312,4 +312,24 @@ struct Grids {
 };
 Grids resident_grids(int device);
 
+// Allocation-free ray queries (pt_hip_trace_rays_noalloc): their scratch is one static device array per
+// GPU. noalloc_setup (once per device, with that device current) looks its address up; noalloc_scratch
+// returns the pieces (all null before a successful setup on that device).
+constexpr uint32_t kNoAllocBlocks = 512;  // largest persistent grid the spill area is sized for
+constexpr uint32_t kNoAllocSpill = 24;    // spilled stack entries per lane: trees of up to 24 + kStackLds - 2 levels
+struct NoAllocScratch {
+  unsigned long long* counters;  // kCounterWords
+  uint32_t* queue;               // kQueueWords
+  uint32_t* error;
+  int32_t* stack_ref;
+  float* stack_t;
+};
+hipError_t noalloc_setup(int device);
+NoAllocScratch noalloc_scratch(int device);
+// Clears the scratch's counters, queue block and error flag, then runs the queries (launch_trace_rays)
+// on a grid of min(grid, kNoAllocBlocks) workgroups over the scratch.
+void launch_trace_rays_noalloc(hipStream_t stream, const NoAllocScratch& sc, uint32_t refill_min, const DevScene& scene,
+                               int any_hit, const float* rays, const float* tmax, uint32_t count, float* out_t,
+                               int32_t* out_prim, uint32_t grid);
+
 }  // namespace ptd

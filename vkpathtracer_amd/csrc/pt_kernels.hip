@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 
 #include "pt_device.h"
 #include "pt_math.h"
@@ -2411,6 +2412,88 @@ static uint32_t occupancy_grid(int device, const void* fn) {
   int blocks = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fn, kBlock, 0) != hipSuccess || blocks <= 0) blocks = 4;
   return static_cast<uint32_t>(cus) * static_cast<uint32_t>(blocks);
+}
+
+// ------------------------------------------------------------------ allocation-free ray queries
+// pt_hip_trace_rays_noalloc keeps all of its traversal scratch in one static device array (each GPU
+// holds its own copy from module load until the process ends): the counter stripes, the queue block
+// (chunk cursor and waves-done count), the error flag, and the stack spill area of a grid of at most
+// kNoAllocBlocks workgroups with kNoAllocSpill spilled entries per lane (one int32 reference array, then
+// one float t_near array).
+constexpr size_t kNoAllocQueueOff = sizeof(unsigned long long) * kCounterWords;          // 8 KiB of counters first
+constexpr size_t kNoAllocErrorOff = kNoAllocQueueOff + ((sizeof(uint32_t) * kQueueWords + 255u) & ~size_t(255u));
+constexpr size_t kNoAllocSpillOff = 32768;
+constexpr size_t kNoAllocLanes = static_cast<size_t>(kNoAllocBlocks) * kBlock;
+constexpr size_t kNoAllocSpillEntries = static_cast<size_t>(kNoAllocSpill) * kNoAllocLanes;
+constexpr size_t kNoAllocBytes = kNoAllocSpillOff + 8u * kNoAllocSpillEntries;
+static_assert(kNoAllocErrorOff + 256u <= kNoAllocSpillOff, "scratch header overlaps the spill area");
+static_assert(kNoAllocBytes == 32768u + 24u * 1024u * 1024u, "the documented scratch size is 24 MiB + 32 KiB");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+
+// Clears what the queries read before writing: counters, queue block, error flag.
+__global__ __launch_bounds__(kBlock) void k_noalloc_clear(unsigned long long* counters, uint32_t* queue,
+                                                          uint32_t* error) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < static_cast<uint32_t>(kCounterWords)) counters[i] = 0u;
+  if (i < static_cast<uint32_t>(kQueueWords)) queue[i] = 0u;
+  if (i == 0u) *error = 0u;
+}
+
+constexpr int kMaxNoAllocDevices = 64;
+std::mutex g_noalloc_mutex;
+unsigned char* g_noalloc_base[kMaxNoAllocDevices] = {};
+hipError_t g_noalloc_status[kMaxNoAllocDevices] = {};
+bool g_noalloc_tried[kMaxNoAllocDevices] = {};
+
+}  // namespace
+
+hipError_t noalloc_setup(int device) {
+  if (device < 0 || device >= kMaxNoAllocDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+  if (!g_noalloc_tried[device]) {
+    // `scratch` names a host-side stand-in; the device copy's address comes from the runtime, per GPU
+    void* p = nullptr;
+    g_noalloc_status[device] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_noalloc_base[device] = g_noalloc_status[device] == hipSuccess ? static_cast<unsigned char*>(p) : nullptr;
+    g_noalloc_tried[device] = true;
+  }
+  return g_noalloc_status[device];
+}
+
+NoAllocScratch noalloc_scratch(int device) {
+  NoAllocScratch s{};
+  if (device < 0 || device >= kMaxNoAllocDevices) return s;
+  std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+  unsigned char* b = g_noalloc_base[device];
+  if (!b) return s;
+  s.counters = reinterpret_cast<unsigned long long*>(b);
+  s.queue = reinterpret_cast<uint32_t*>(b + kNoAllocQueueOff);
+  s.error = reinterpret_cast<uint32_t*>(b + kNoAllocErrorOff);
+  s.stack_ref = reinterpret_cast<int32_t*>(b + kNoAllocSpillOff);
+  s.stack_t = reinterpret_cast<float*>(b + kNoAllocSpillOff + 4u * kNoAllocSpillEntries);
+  return s;
+}
+
+void launch_trace_rays_noalloc(hipStream_t stream, const NoAllocScratch& sc, uint32_t refill_min, const DevScene& scene,
+                               int any_hit, const float* rays, const float* tmax, uint32_t count, float* out_t,
+                               int32_t* out_prim, uint32_t grid) {
+  grid = grid < kNoAllocBlocks ? grid : kNoAllocBlocks;
+  DevPaths P{};
+  P.queue = sc.queue;
+  P.counters = sc.counters;
+  P.error = sc.error;
+  P.stack_ref = sc.stack_ref;
+  P.stack_t = sc.stack_t;
+  P.stack_overflow_depth = kNoAllocSpill;  // entry i of lane g at i * stack_threads + g < kNoAllocSpillEntries
+  P.stack_threads = grid * kBlock;
+  P.refill_min = refill_min;
+  constexpr uint32_t clear_words = kQueueWords > kCounterWords ? kQueueWords : kCounterWords;
+  hipLaunchKernelGGL(k_noalloc_clear, dim3((clear_words + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, sc.counters,
+                     sc.queue, sc.error);
+  if (count) launch_trace_rays(stream, P, scene, any_hit, rays, tmax, count, out_t, out_prim, grid);
 }
 
 Grids resident_grids(int device) {

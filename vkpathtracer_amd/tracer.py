@@ -386,11 +386,12 @@ class HipPathTracer:
         return out
 
     # -- IRayAccelerator::intersect, batched
-    def trace_rays(self, rays, any_hit: bool = False, tmax=None):
+    def trace_rays(self, rays, any_hit: bool = False, tmax=None, noalloc: bool = False):
         """Closest-hit (or any-hit below tmax) for an (N, 6) array of rays.
 
         Accepts numpy arrays or torch device tensors; returns (t, primitive_index, counters)
-        as numpy arrays (t = inf and primitive_index = -1 on a miss)."""
+        as numpy arrays (t = inf and primitive_index = -1 on a miss). noalloc: through
+        pt_hip_trace_rays_noalloc (the library's static scratch instead of its own allocations)."""
         import torch
 
         dev = torch.device("cuda", self.device)
@@ -405,9 +406,10 @@ class HipPathTracer:
         out_p = torch.empty(n, dtype=torch.int32, device=dev)
         torch.cuda.synchronize(dev)
         c = abi.PtSampleCounters()
-        rc = self._lib.pt_hip_trace_rays(self._h, 1 if any_hit else 0, r.data_ptr(), tm.data_ptr() if tm is not None else None,
-                                         n, out_t.data_ptr(), out_p.data_ptr(), C.byref(c))
-        self._check(rc, "trace_rays")
+        fn = self._lib.pt_hip_trace_rays_noalloc if noalloc else self._lib.pt_hip_trace_rays
+        rc = fn(self._h, 1 if any_hit else 0, r.data_ptr(), tm.data_ptr() if tm is not None else None, n,
+                out_t.data_ptr(), out_p.data_ptr(), C.byref(c))
+        self._check(rc, "trace_rays_noalloc" if noalloc else "trace_rays")
         return out_t.cpu().numpy(), out_p.cpu().numpy(), c.as_dict()
 
     def intersect(self, rays, any_hit: bool = False, tmax=None):
