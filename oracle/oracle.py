@@ -108,6 +108,13 @@ class PtoResolve(C.Structure):
 
 _SIGS["pto_resolve_ldr"] = (None, [_P, _P, C.c_uint64, C.POINTER(PtoResolve), _P])
 
+_SIGS["pto_camera_rays"] = (C.c_int, [C.POINTER(PtoScene), C.POINTER(PtoSettings), _P, C.c_uint32, C.c_uint32,
+                                       C.c_uint64, _P, _P])
+_SIGS["pto_camera_dir"] = (C.c_int, [C.POINTER(PtoScene), C.POINTER(PtoSettings), C.c_uint32, C.c_uint32, C.c_float,
+                                     C.c_float, _P])
+_SIGS["pto_hemisphere_samples"] = (C.c_int, [_P, C.c_uint64, C.c_uint32, _P])
+_SIGS["pto_light_pdf"] = (C.c_double, [C.POINTER(PtoScene), _P, _P])
+
 _libs: dict[str, C.CDLL] = {}
 
 
@@ -277,6 +284,31 @@ class Oracle:
         self.L.pto_brute_force_rays(self.accel, rays.ctypes.data, n, t.ctypes.data, p.ctypes.data)
         return t, p
 
+    def camera_rays(self, settings, pixels: np.ndarray, sample_index: int, frame_index: int = 0):
+        """The primary rays shade_pixel makes for one sample of each pixel: (rays (N, 6), jitter (N, 2))."""
+        px = np.ascontiguousarray(pixels, np.uint32)
+        rays = np.zeros((len(px), 6), np.float32)
+        jit = np.zeros((len(px), 2), np.float32)
+        st = settings_struct(settings)
+        if self.L.pto_camera_rays(C.byref(self.scene.s), C.byref(st), px.ctypes.data, len(px), sample_index,
+                                  frame_index, rays.ctypes.data, jit.ctypes.data) != 0:
+            raise RuntimeError("oracle camera_rays failed")
+        return rays, jit
+
+    def camera_dir(self, settings, x: int, y: int, u: float, v: float) -> np.ndarray:
+        """The pinhole direction through pixel (x, y) at jitter (u, v)."""
+        out = np.zeros(3, np.float32)
+        st = settings_struct(settings)
+        if self.L.pto_camera_dir(C.byref(self.scene.s), C.byref(st), x, y, u, v, out.ctypes.data) != 0:
+            raise RuntimeError("oracle camera_dir failed")
+        return out
+
+    def light_pdf(self, pos, direction) -> float:
+        """light_pdf_for_direction at pos for a direction."""
+        p = np.ascontiguousarray(pos, np.float32)
+        d = np.ascontiguousarray(direction, np.float32)
+        return float(self.L.pto_light_pdf(C.byref(self.scene.s), p.ctypes.data, d.ctypes.data))
+
     def render_pixels(self, settings, pixels: np.ndarray, sample_index: int, frame_index: int = 0, threads: int = 0,
                       shadow: int = SHADOW_ANYHIT):
         px = np.ascontiguousarray(pixels, np.uint32)
@@ -357,6 +389,14 @@ def phong_samples(exponent: float, refl, normal, ps: float, state: int, count: i
     sc = np.zeros(count, np.float64)
     fn(exponent, r.ctypes.data, n.ctypes.data, ps, state & 0xFFFFFFFFFFFFFFFF, count, d.ctypes.data, sc.ctypes.data)
     return d, sc
+
+
+def hemisphere_samples(normal, state: int, count: int, variant: str = "f32") -> np.ndarray:
+    """pto_hemisphere_samples: sample_hemisphere directions (count, 3) about `normal`."""
+    n = np.ascontiguousarray(normal, np.float32)
+    d = np.zeros((count, 3), np.float32)
+    lib(variant).pto_hemisphere_samples(n.ctypes.data, state & 0xFFFFFFFFFFFFFFFF, count, d.ctypes.data)
+    return d
 
 
 # ------------------------------------------------------------------ film resolve

@@ -9,6 +9,21 @@
  * PTO_AVX2    1: leaves are intersected 8 triangles at a time with AVX2 (fp32 only; the CPU
  *             baseline's AVX2 mode, as the reference's leaf batch intersector when built with
  *             PT_ENABLE_AVX2, BvhTriangleIntersector.cpp:156-171). Same results as the scalar loop.
+ *
+ * How each part is pinned (the reference cannot be built here and ships no vectors for this path):
+ *   splitmix64 / Rng / next01 ........ published SplitMix64 known answers (tests/test_oracle_kat.py)
+ *   sin cos pow atan2 acos ............ double-precision values, and the glibc build (test_oracle_kat.py,
+ *                                        test_oracle_integrator.py: portable vs glibc on every config)
+ *   Moller-Trumbore, traversal ........ hand cases, brute force over all triangles (test_bvh_host.py)
+ *   midpoint / SAH builders ........... invariants, costs (test_bvh_host.py)
+ *   camera_rays, RNG key .............. pinhole geometry, key layout (test_oracle_integrator.py)
+ *   trace at depth 1 / 2, NEE, spot,
+ *   MIS with lights[0], light pdf,
+ *   roulette, samplers, effects, env .. closed forms, sampler moments, a second transcription of
+ *                                        the effects (test_oracle_integrator.py)
+ *   dropped samples and ray counts .... the reference's counting rule (test_oracle_counters.py)
+ *   film resolve ...................... closed forms and the libm build (test_oracle_resolve.py)
+ *   adjoint ........................... finite differences (test_oracle_adjoint.py, test_oracle_textures.py)
  */
 #include "pto.h"
 
@@ -1992,16 +2007,24 @@ static camera_t make_camera(const PtoScene* sc, const PtoSettings* s) {
   return cam;
 }
 
-/* camera_rays: ScalarCpuPathTracer.Render.cpp:55-119 */
-static void camera_ray(const camera_t* cam, const PtoSettings* s, uint32_t x, uint32_t y, uint32_t sample,
-                       uint64_t frame, uint32_t path_id, uint64_t* sample_seed, v3* ro, v3* rd) {
-  uint64_t rng = pto_rng_seed(s->seed, frame, sample, path_id, 0, 0, (uint64_t)y * s->width + x);
-  const real fx = ((real)x + next01(&rng)) / (real)(s->width > 1u ? s->width : 1u);
-  const real fy = ((real)y + next01(&rng)) / (real)(s->height > 1u ? s->height : 1u);
-  *sample_seed ^= (uint64_t)(next01(&rng) * (real)0xffffu);
+/* The pinhole direction of camera_rays (ScalarCpuPathTracer.Render.cpp:74-81) for the pixel jitter (u, v). */
+static v3 camera_dir(const camera_t* cam, const PtoSettings* s, uint32_t x, uint32_t y, real u, real v) {
+  const real fx = ((real)x + u) / (real)(s->width > 1u ? s->width : 1u);
+  const real fy = ((real)y + v) / (real)(s->height > 1u ? s->height : 1u);
   const real nx = (RC(2.0) * fx - RC(1.0)) * cam->aspect * cam->tan_half_fov;
   const real ny = (RC(1.0) - RC(2.0) * fy) * cam->tan_half_fov;
-  const v3 dir = normalize(add(add(cam->fwd, muls(cam->right, nx)), muls(cam->up, ny)));
+  return normalize(add(add(cam->fwd, muls(cam->right, nx)), muls(cam->up, ny)));
+}
+
+/* camera_rays: ScalarCpuPathTracer.Render.cpp:55-119. jitter (may be NULL) receives the two pixel draws. */
+static void camera_ray(const camera_t* cam, const PtoSettings* s, uint32_t x, uint32_t y, uint32_t sample,
+                       uint64_t frame, uint32_t path_id, uint64_t* sample_seed, v3* ro, v3* rd, real* jitter) {
+  uint64_t rng = pto_rng_seed(s->seed, frame, sample, path_id, 0, 0, (uint64_t)y * s->width + x);
+  const real u = next01(&rng);
+  const real v = next01(&rng);
+  if (jitter) { jitter[0] = u; jitter[1] = v; }
+  *sample_seed ^= (uint64_t)(next01(&rng) * (real)0xffffu);
+  const v3 dir = camera_dir(cam, s, x, y, u, v);
   if (cam->aperture > RC(0.0) && cam->focus > kEps) {
     const real lens_radius_sample = R_SQRT(next01(&rng));
     const real lens_phi = RC(2.0) * kPi * next01(&rng);
@@ -2035,11 +2058,61 @@ static int shade_pixel(const ctx_t* x, const camera_t* cam, uint32_t pixel, uint
   const uint64_t sample_seed = ((uint64_t)py << 32) | px;
   const uint64_t path_id = sample_seed + (uint64_t)sample + frame;
   v3 ro, rd;
-  camera_ray(cam, s, px, py, sample, frame, (uint32_t)path_id, &seed, &ro, &rd);
+  camera_ray(cam, s, px, py, sample, frame, (uint32_t)path_id, &seed, &ro, &rd, NULL);
   uint64_t rng = pto_rng_seed(s->seed + sample_seed + seed, frame, sample, path_id, 0, 0, sample_seed);
   const v3 v = trace(x, ro, rd, &rng, rays, c, adj);
   *out = v;
   return is_finite_r(v.x) && is_finite_r(v.y) && is_finite_r(v.z);
+}
+
+/* Probes of single oracle functions, for tests that check them against closed forms. */
+
+/* The primary ray shade_pixel makes for one sample of each listed pixel (camera_rays with the path id
+ * and RNG key of Render.cpp:246-251): origin and direction (6 per pixel) and the two pixel jitter draws
+ * (2 per pixel, may be NULL). */
+int pto_camera_rays(const PtoScene* scene, const PtoSettings* s, const uint32_t* pixels, uint32_t n,
+                    uint32_t sample, uint64_t frame, float* out_rays, float* out_jitter) {
+  if (!scene || !s || s->width == 0 || s->height == 0) return -1;
+  const camera_t cam = make_camera(scene, s);
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t px = pixels[i] % s->width, py = pixels[i] / s->width;
+    const uint64_t sample_seed = ((uint64_t)py << 32) | px;
+    const uint64_t path_id = sample_seed + (uint64_t)sample + frame;
+    uint64_t seed = 0;
+    v3 ro, rd;
+    real jit[2];
+    camera_ray(&cam, s, px, py, sample, frame, (uint32_t)path_id, &seed, &ro, &rd, jit);
+    float* r = out_rays + 6u * i;
+    r[0] = (float)ro.x; r[1] = (float)ro.y; r[2] = (float)ro.z;
+    r[3] = (float)rd.x; r[4] = (float)rd.y; r[5] = (float)rd.z;
+    if (out_jitter) { out_jitter[2u * i] = (float)jit[0]; out_jitter[2u * i + 1u] = (float)jit[1]; }
+  }
+  return 0;
+}
+
+/* The pinhole direction of camera_rays through pixel (x, y) at jitter (u, v). */
+int pto_camera_dir(const PtoScene* scene, const PtoSettings* s, uint32_t x, uint32_t y, float u, float v,
+                   float out_dir[3]) {
+  if (!scene || !s || s->width == 0 || s->height == 0) return -1;
+  const camera_t cam = make_camera(scene, s);
+  const v3 d = camera_dir(&cam, s, x, y, (real)u, (real)v);
+  out_dir[0] = (float)d.x; out_dir[1] = (float)d.y; out_dir[2] = (float)d.z;
+  return 0;
+}
+
+/* sample_hemisphere directions about n from the RNG state `state` (each draw advances it). */
+int pto_hemisphere_samples(const float n_in[3], uint64_t state, uint32_t count, float* out_dirs) {
+  const v3 n = v3f(n_in);
+  for (uint32_t i = 0; i < count; ++i) {
+    const v3 w = sample_hemisphere(&state, n);
+    out_dirs[3 * i] = (float)w.x; out_dirs[3 * i + 1] = (float)w.y; out_dirs[3 * i + 2] = (float)w.z;
+  }
+  return 0;
+}
+
+/* light_pdf_for_direction (ScalarCpuPathTracer.cpp:1120-1138) at a position for a direction. */
+double pto_light_pdf(const PtoScene* scene, const float pos[3], const float dir[3]) {
+  return (double)light_pdf_for_direction(scene, v3f(pos), v3f(dir));
 }
 
 static int check_supported(const PtoScene* sc) {

@@ -179,6 +179,15 @@ int pto_adjoint_pixels(const PtoScene* scene, const pto_accel* a, const PtoSetti
 int pto_phong_samples(float exponent, const float refl[3], const float n[3], float ps, uint64_t state,
                       uint32_t count, float* out_dirs, double* out_score);
 
+/* Probes for closed-form checks (see pto.c): the primary rays and pixel jitter shade_pixel uses, the
+ * pinhole direction at a given jitter, cosine-hemisphere samples, and light_pdf_for_direction. */
+int pto_camera_rays(const PtoScene* scene, const PtoSettings* s, const uint32_t* pixels, uint32_t n,
+                    uint32_t sample, uint64_t frame, float* out_rays, float* out_jitter);
+int pto_camera_dir(const PtoScene* scene, const PtoSettings* s, uint32_t x, uint32_t y, float u, float v,
+                   float out_dir[3]);
+int pto_hemisphere_samples(const float n[3], uint64_t state, uint32_t count, float* out_dirs);
+double pto_light_pdf(const PtoScene* scene, const float pos[3], const float dir[3]);
+
 /* The scene's SDF primitives (intersect_scene tests them after the triangles; Unknown shapes dropped). */
 int pto_accel_set_sdfs(pto_accel* a, const PtoSdf* sdfs, uint32_t count);
 /* intersect_scene (ScalarCpuPathTracer.Intersect.cpp:28-132) for rays (o xyz, d xyz): t (inf: miss),
