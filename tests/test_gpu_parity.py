@@ -359,3 +359,29 @@ def test_intersect_ray_query_hits_equal_oracle(small_procedural, tracer_factory,
             assert not np.any(miss[f]), f
         for k in ("triangle_tests", "triangle_hits", "bvh_node_visits", "bvh_leaf_visits"):
             assert stats[k] == c[k], (any_hit, k, stats[k], c[k])
+
+
+def test_deep_tree_spills_the_traversal_stack(tracer_factory):
+    """A 36-level tree (triangles in planes x = 2^k, midpoint splits) drives the traversal stack past
+    its LDS ring into the global spill area and back: t, primitive ids and counters equal the oracle's
+    for closest and any hit."""
+    from test_zz_no_alloc import _chain_scene
+    from vkpathtracer_amd import scene as S
+
+    sc = _chain_scene(48)
+    pt = tracer_factory(sc, S.RenderSettings(width=8, height=8), builder="midpoint")
+    assert pt.build_info["max_depth"] >= 30
+    rng = np.random.default_rng(2)
+    n = 40_000
+    o = np.stack([np.full(n, -1.0), rng.uniform(0.0, 1.0, n), rng.uniform(0.0, 1.0, n)], 1)
+    d = np.stack([np.ones(n), rng.uniform(-0.3, 0.3, n) * 1e-6, rng.uniform(-0.3, 0.3, n) * 1e-6], 1)
+    rays = np.concatenate([o, d], 1).astype(np.float32)
+    rays[::3, 3] = -1.0  # from beyond the far end back through every plane
+    rays[::3, 0] = 3.0e14
+    orc = _oracle(sc, builder="midpoint")
+    for any_hit in (False, True):
+        t, p, c = pt.trace_rays(rays, any_hit=any_hit)
+        rt, rp, rc = orc.trace_rays(rays, any_hit=any_hit)
+        assert np.array_equal(p, rp) and np.array_equal(t, rt)
+        for k in ("bvh_node_visits", "bvh_leaf_visits", "triangle_tests", "triangle_hits"):
+            assert c[k] == rc[k], (any_hit, k)

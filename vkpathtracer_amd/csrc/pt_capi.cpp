@@ -774,6 +774,8 @@ int upload_pool(pt_tracer* t) {
   const uint32_t threads =
       std::max({t->grids_max.extend, t->grids_max.shadow, t->grids_max.shadow_adj, t->grids_max.query}) *
       static_cast<uint32_t>(ptd::kBlock);
+  if (static_cast<uint64_t>(need) * threads >= (1ull << 32))  // the kernels index the area with 32-bit offsets
+    return fail(t, PT_UNSUPPORTED, "traversal stack area exceeds 2^32 entries (tree too deep)");
   PT_HIP(t, t->d_stack_ref.ensure(static_cast<size_t>(need) * threads), "allocate traversal stack");
   PT_HIP(t, t->d_stack_t.ensure(static_cast<size_t>(need) * threads), "allocate traversal stack");
   bytes += static_cast<uint64_t>(need) * threads * 8u;

@@ -175,14 +175,20 @@ __device__ __forceinline__ bool tri_test(const float4* tp, V3 o, V3 d, float bes
   return !(t <= kEps || t >= best_t);
 }
 
-// Traversal stack of one lane: the newest kStackLds entries live in an LDS ring (slot = index %
-// kStackLds); older entries are spilled to a per-lane global area (entry i at ovf[i * ovf_stride +
-// gid]) when the ring is full and read back from there when the pops reach them.
+// Traversal stack of one lane: the newest kStackLds entries live in an LDS ring; older entries are
+// spilled to a per-lane global area (entry i at ovf[i * ovf_stride + gid], 32-bit offsets: the host
+// sizes the area below 2^32 entries) when the ring is full and read back from there when the pops
+// reach them.
+// The ring pointers carry the LDS address space and the spill pointers the global one, so that no
+// load or store can be formed that addresses either memory (a generic access and its wait on both
+// counters on every pop).
+#define PT_LDS __attribute__((address_space(3)))
+#define PT_GLOBAL __attribute__((address_space(1)))
 struct Stack {
-  int32_t* lds_ref;
-  float* lds_t;
-  int32_t* ovf_ref;
-  float* ovf_t;
+  PT_LDS int32_t* lds_ref;
+  PT_LDS float* lds_t;
+  PT_GLOBAL int32_t* ovf_ref;
+  PT_GLOBAL float* ovf_t;
   uint32_t ovf_depth, ovf_stride, lane_slot, gid;
   uint32_t* error;
 };
@@ -193,8 +199,9 @@ struct Trav {
   V3 d;
   float best_t;
   int32_t best_prim, cur;
-  uint32_t sp;  // entries on the stack
-  uint32_t lo;  // entries [lo, sp) are in the LDS ring, [0, lo) in the global area
+  uint32_t sp;   // entries on the stack
+  uint32_t lo;   // entries [lo, sp) are in the LDS ring, [0, lo) in the global area
+  uint32_t top;  // ring slot of the next push (entry sp's slot); the oldest ring entry's when it is full
 };
 
 // Starts a query below t_max. False when it ends at once (no primitives, or the root box is
@@ -205,6 +212,7 @@ __device__ __forceinline__ bool trav_begin(const SceneRec& R, Trav& s, V3 o, V3 
   s.d = d;
   s.sp = 0;
   s.lo = 0;
+  s.top = 0;
   s.cur = R.root_ref;
   if (R.prim_count == 0) return false;
   s.q = make_query(o, d);
@@ -241,20 +249,20 @@ __device__ __forceinline__ int trav_inner(const DevScene& sc, const Stack& stk, 
     const int32_t near_ref = left_first ? r.x : r.y;
     const int32_t far_ref = left_first ? r.y : r.x;
     const float far_t = left_first ? t1 : t0;
-    if (s.sp - s.lo == static_cast<uint32_t>(kStackLds)) {  // ring full: spill its oldest entry
+    const uint32_t slot = s.top * kBlock + stk.lane_slot;
+    if (s.sp - s.lo == static_cast<uint32_t>(kStackLds)) {  // ring full: spill its oldest entry (slot top)
       if (s.lo >= stk.ovf_depth) {
         atomicOr(stk.error, 1u);  // overflow area smaller than the tree depth: refuse, never write out of bounds
         return kStepDone;
       }
-      const uint32_t slot = (s.lo % kStackLds) * kBlock + stk.lane_slot;
-      const size_t o2 = static_cast<size_t>(s.lo) * stk.ovf_stride + stk.gid;
+      const uint32_t o2 = s.lo * stk.ovf_stride + stk.gid;
       stk.ovf_ref[o2] = stk.lds_ref[slot];
       stk.ovf_t[o2] = stk.lds_t[slot];
       ++s.lo;
     }
-    const uint32_t slot = (s.sp % kStackLds) * kBlock + stk.lane_slot;
     stk.lds_ref[slot] = far_ref;
     stk.lds_t[slot] = far_t;
+    s.top = s.top + 1u == static_cast<uint32_t>(kStackLds) ? 0u : s.top + 1u;
     ++s.sp;
     s.cur = near_ref;
     ++st.visits;
@@ -303,17 +311,20 @@ __device__ __forceinline__ int trav_leaf(const DevScene& sc, Trav& s, TravStats&
 }
 
 // Pops until an entry passes t_near <= best_t (kStepGo, s.cur set) or the stack is empty (kStepDone).
+// The ring slot below top is read unconditionally (an LDS read; stale when the ring is empty) and
+// replaced by the spilled entry in that rare case, so the common pop stays an LDS access instead of
+// one generic load that could address either memory.
 __device__ __forceinline__ int trav_pop(const Stack& stk, Trav& s, TravStats& st) {
   while (s.sp > 0) {
     --s.sp;
-    int32_t ref;
-    float t;
+    const uint32_t below = (s.top == 0u ? static_cast<uint32_t>(kStackLds) : s.top) - 1u;
+    const uint32_t slot = below * kBlock + stk.lane_slot;
+    int32_t ref = stk.lds_ref[slot];
+    float t = stk.lds_t[slot];
     if (s.sp >= s.lo) {
-      const uint32_t slot = (s.sp % kStackLds) * kBlock + stk.lane_slot;
-      ref = stk.lds_ref[slot];
-      t = stk.lds_t[slot];
-    } else {  // below the ring: the ring is empty from here on
-      const size_t o2 = static_cast<size_t>(s.sp) * stk.ovf_stride + stk.gid;
+      s.top = below;
+    } else {  // below the ring: the ring is empty from here on (top stays where the next push goes)
+      const uint32_t o2 = s.sp * stk.ovf_stride + stk.gid;
       ref = stk.ovf_ref[o2];
       t = stk.ovf_t[o2];
       s.lo = s.sp;
@@ -871,10 +882,10 @@ __global__ __launch_bounds__(kBlock) void k_grad_reduce(DevAdjoint A) {
 
 __device__ __forceinline__ Stack make_stack(const DevPaths& P, int32_t* s_ref, float* s_t) {
   Stack s;
-  s.lds_ref = s_ref;
-  s.lds_t = s_t;
-  s.ovf_ref = P.stack_ref;
-  s.ovf_t = P.stack_t;
+  s.lds_ref = (PT_LDS int32_t*)s_ref;
+  s.lds_t = (PT_LDS float*)s_t;
+  s.ovf_ref = (PT_GLOBAL int32_t*)P.stack_ref;
+  s.ovf_t = (PT_GLOBAL float*)P.stack_t;
   s.ovf_depth = P.stack_overflow_depth;
   s.ovf_stride = P.stack_threads;
   s.lane_slot = threadIdx.x;
