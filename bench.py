@@ -175,7 +175,8 @@ def adjoint_ratio(log, reps: int = 3) -> dict:
     tiles. backward = render_adjoint of the same tiles right after that forward, as in an autograd
     step: it reuses each path's forward radiance and replays the paths once, recomputing every vertex.
     standalone = render_adjoint with no forward before it (forward replay + adjoint replay).
-    ratio = backward / forward."""
+    ratio = (forward + backward) / forward: the time of a forward and its adjoint over the forward's
+    alone; standalone_ratio = standalone / forward; backward_over_forward = backward / forward."""
     import torch
 
     from vkpathtracer_amd import scene as S
@@ -218,7 +219,8 @@ def adjoint_ratio(log, reps: int = 3) -> dict:
     log(f"forward {f:.1f} ms, backward {b:.1f} ms, standalone adjoint {sa:.1f} ms")
     return {"workload": "cornell box 512x512, 64 spp, depth 8, d/d(35 albedos, 35 emissions, light colour)",
             "forward_ms": round(f, 2), "backward_ms": round(b, 2), "standalone_adjoint_ms": round(sa, 2),
-            "ratio": round(b / f, 3), "standalone_ratio": round(sa / f, 3)}
+            "ratio": round((f + b) / f, 3), "standalone_ratio": round(sa / f, 3),
+            "backward_over_forward": round(b / f, 3)}
 
 
 def textured_multiview(log, reps: int = 2, views: int = 16, size: int = 512, spp: int = 32) -> dict:
@@ -228,7 +230,7 @@ def textured_multiview(log, reps: int = 2, views: int = 16, size: int = 512, spp
     depth 6. forward = reset + pt_batch_render of every view's frame; backward = pt_batch_render_adjoint
     of the same frame right after it (d/d albedo, emission, light colour, roughness and every texel,
     fp32 atomics; the forward radiance of the single pass is reused); standalone = the adjoint with no
-    forward before it."""
+    forward before it. ratio = (forward + backward) / forward; standalone_ratio = standalone / forward."""
     import torch
 
     from vkpathtracer_amd import scene as S
@@ -281,8 +283,8 @@ def textured_multiview(log, reps: int = 2, views: int = 16, size: int = 512, spp
             "paths": paths, "bvh_build_ms": round(info["build_ms"], 1),
             "forward_ms": round(f, 2), "forward_mpaths_per_s": round(paths / f / 1e3, 2),
             "backward_ms": round(bk, 2), "standalone_adjoint_ms": round(sa, 2),
-            "ratio": round(bk / f, 3), "standalone_ratio": round(sa / f, 3),
-            "rays_per_frame": counters["rays"]}
+            "ratio": round((f + bk) / f, 3), "standalone_ratio": round(sa / f, 3),
+            "backward_over_forward": round(bk / f, 3), "rays_per_frame": counters["rays"]}
 
 
 def batched_comparison(log, count: int = 64, reps: int = 2) -> dict:

@@ -18,6 +18,11 @@
 #include "pt_device.h"
 #include "pt_math.h"
 
+// Pointer qualifiers for the LDS and global address spaces, where an access must be issued as that
+// memory's instruction (a pointer that may address either compiles to a generic access).
+#define PT_LDS __attribute__((address_space(3)))
+#define PT_GLOBAL __attribute__((address_space(1)))
+
 namespace ptd {
 namespace {
 
@@ -182,8 +187,6 @@ __device__ __forceinline__ bool tri_test(const float4* tp, V3 o, V3 d, float bes
 // The ring pointers carry the LDS address space and the spill pointers the global one, so that no
 // load or store can be formed that addresses either memory (a generic access and its wait on both
 // counters on every pop).
-#define PT_LDS __attribute__((address_space(3)))
-#define PT_GLOBAL __attribute__((address_space(1)))
 struct Stack {
   PT_LDS int32_t* lds_ref;
   PT_LDS float* lds_t;
@@ -822,21 +825,30 @@ __device__ __forceinline__ float* lds_copy(const DevAdjoint& A, float* lds) {
   return lds + (threadIdx.x & (A.lds_copies - 1u)) * A.lds_stride;
 }
 
+// The staged add is an LDS atomic and the direct one a global atomic: each is issued through a pointer
+// of its own address space, so the two branches cannot fold into one generic (flat) atomic.
+__device__ __forceinline__ void lds_fadd(float* p, float v) {
+  __hip_atomic_fetch_add((PT_LDS float*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void global_fadd(float* p, float v) {
+  __hip_atomic_fetch_add((PT_GLOBAL float*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ __forceinline__ void gadd(const DevAdjoint& A, float* lds, GradKind kind, uint32_t idx, int c, float v) {
   if (v == 0.0f) return;
   if (kind == kGradRough) {
     if (!A.g_rough) return;
-    if (A.use_lds) atomicAdd(lds_copy(A, lds) + 6u * A.nmat + 3u * A.nlight + idx, v);
-    else atomicAdd(A.g_rough + idx, v);
+    if (A.use_lds) lds_fadd(lds_copy(A, lds) + 6u * A.nmat + 3u * A.nlight + idx, v);
+    else global_fadd(A.g_rough + idx, v);
     return;
   }
   const uint32_t slot = 3u * idx + static_cast<uint32_t>(c);
   if (A.use_lds) {
     const uint32_t off = kind == kGradAlbedo ? 0u : (kind == kGradEmission ? 3u * A.nmat : 6u * A.nmat);
-    atomicAdd(lds_copy(A, lds) + off + slot, v);
+    lds_fadd(lds_copy(A, lds) + off + slot, v);
   } else {
     float* g = kind == kGradAlbedo ? A.g_albedo : (kind == kGradEmission ? A.g_emission : A.g_light);
-    atomicAdd(g + slot, v);
+    global_fadd(g + slot, v);
   }
 }
 
@@ -1173,7 +1185,7 @@ __device__ __forceinline__ void gadd_base(const DevAdjoint& A, const DevScene& s
     if (ga == 0.0f) continue;
     for (int j = 0; j < 4; ++j) {
       const float v = ga * tap.w[j];
-      if (v != 0.0f) atomicAdd(A.g_tex + 3u * tap.idx[j] + static_cast<uint32_t>(c), v);
+      if (v != 0.0f) global_fadd(A.g_tex + 3u * tap.idx[j] + static_cast<uint32_t>(c), v);
     }
   }
 }
