@@ -859,6 +859,59 @@ __device__ __forceinline__ void grad_lds_clear(const DevAdjoint& A, float* lds) 
   __syncthreads();
 }
 
+// Texel gradients through a per-wave LDS table. A wave's lanes usually hold samples of one pixel, so
+// their bilinear taps add into the same few texels; adding them straight to global memory serialises
+// those atomics in L2. Each wave owns kTexSlots entries {texel, r, g, b} after the gradient slots of
+// the block's LDS array: a tap claims the entry its texel hashes to (or finds it already holding that
+// texel) and adds there with LDS atomics; a tap whose entry holds another texel adds to global memory
+// directly. tex_grad_flush, run by the converged wave, moves every claimed entry to global memory
+// (one atomic per channel) and frees it. The sums are those of the direct adds, in another order.
+constexpr uint32_t kTexSlots = 64;
+constexpr uint32_t kTexEmpty = 0xFFFFFFFFu;
+constexpr int kTexTableFloats = kWavesPerBlock * static_cast<int>(kTexSlots) * 4;
+
+__device__ __forceinline__ float* tex_table(float* lds) {
+  return lds + kAdjLdsFloats + (threadIdx.x >> 6) * (kTexSlots * 4u);
+}
+
+__device__ __forceinline__ void tex_table_init(float* lds) {
+  float* e = tex_table(lds) + 4u * (threadIdx.x & 63u);
+  e[0] = __uint_as_float(kTexEmpty);
+  e[1] = e[2] = e[3] = 0.0f;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+}
+
+__device__ __forceinline__ void tex_grad_add(const DevAdjoint& A, float* lds, uint32_t texel, const float v[3]) {
+  float* e = tex_table(lds) + 4u * ((texel * 2654435761u) >> 26);  // 64 entries
+  uint32_t held = kTexEmpty;
+  const bool claimed = __hip_atomic_compare_exchange_strong((PT_LDS uint32_t*)e, &held, texel, __ATOMIC_RELAXED,
+                                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  if (claimed || held == texel) {
+    for (int c = 0; c < 3; ++c)
+      if (v[c] != 0.0f) lds_fadd(e + 1 + c, v[c]);
+  } else {
+    for (int c = 0; c < 3; ++c)
+      if (v[c] != 0.0f) global_fadd(A.g_tex + 3u * texel + static_cast<uint32_t>(c), v[c]);
+  }
+}
+
+// Called by every lane of the wave at a point where the wave is converged.
+__device__ __forceinline__ void tex_grad_flush(const DevAdjoint& A, float* lds) {
+  if (!A.g_tex) return;
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");  // the wave's table adds are complete
+  float* e = tex_table(lds) + 4u * (threadIdx.x & 63u);
+  const uint32_t texel = __float_as_uint(e[0]);
+  if (texel != kTexEmpty) {
+    for (int c = 0; c < 3; ++c) {
+      const float v = e[1 + c];
+      if (v != 0.0f) global_fadd(A.g_tex + 3u * texel + static_cast<uint32_t>(c), v);
+      e[1 + c] = 0.0f;
+    }
+    e[0] = __uint_as_float(kTexEmpty);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+}
+
 __device__ __forceinline__ void grad_lds_flush(const DevAdjoint& A, const float* lds) {
   if (!A.use_lds) return;
   __syncthreads();
@@ -1180,13 +1233,11 @@ __device__ __forceinline__ void gadd_base(const DevAdjoint& A, const DevScene& s
   const float tc[3] = {tap.color.x, tap.color.y, tap.color.z};
   for (int c = 0; c < 3; ++c) gadd(A, lds, kGradAlbedo, m, c, g[c] * tc[c]);
   if (!A.g_tex) return;
-  for (int c = 0; c < 3; ++c) {
-    const float ga = g[c] * mat.albedo[c];
-    if (ga == 0.0f) continue;
-    for (int j = 0; j < 4; ++j) {
-      const float v = ga * tap.w[j];
-      if (v != 0.0f) global_fadd(A.g_tex + 3u * tap.idx[j] + static_cast<uint32_t>(c), v);
-    }
+  const float ga[3] = {g[0] * mat.albedo[0], g[1] * mat.albedo[1], g[2] * mat.albedo[2]};
+  if (ga[0] == 0.0f && ga[1] == 0.0f && ga[2] == 0.0f) return;
+  for (int j = 0; j < 4; ++j) {
+    const float v[3] = {ga[0] * tap.w[j], ga[1] * tap.w[j], ga[2] * tap.w[j]};
+    tex_grad_add(A, lds, tap.idx[j], v);
   }
 }
 
@@ -1902,7 +1953,10 @@ template <bool kAdj>
 __device__ __forceinline__ void shade_kernel(const DevPaths& P, const DevRays& in, const DevRays& out, const DevScene& sc,
                                              const DevSettings& st, uint32_t depth, const DevAdjoint& A, float* glds) {
   __shared__ uint32_t prefix[kSegments + 1];
-  if constexpr (kAdj) grad_lds_clear(A, glds);
+  if constexpr (kAdj) {
+    tex_table_init(glds);
+    grad_lds_clear(A, glds);
+  }
   load_prefix(P.queue + kQueueInCount, 1, prefix);
   const uint32_t n = prefix[kSegments];
   const uint32_t stride = gridDim.x * kBlock;
@@ -1955,6 +2009,7 @@ __device__ __forceinline__ void shade_kernel(const DevPaths& P, const DevRays& i
       }
       ++shadow_rays;
     }
+    if constexpr (kAdj) tex_grad_flush(A, glds);  // the wave is converged here
   }
   wave_add(P.counters, kShadowTests, shadow_tests);
   wave_add(P.counters, kShadowRays, shadow_rays);
@@ -1978,7 +2033,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PT_SHADE
 #endif
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PT_SHADE_ADJ_WAVES, 8))) void k_shade_adj(
     DevPaths P, DevRays in, DevRays out, DevScene sc, DevSettings st, uint32_t depth, DevAdjoint A) {
-  __shared__ float glds[kAdjLdsFloats];
+  __shared__ float glds[kAdjLdsFloats + kTexTableFloats];
   shade_kernel<true>(P, in, out, sc, st, depth, A, glds);
 }
 
@@ -2120,14 +2175,20 @@ __global__ __launch_bounds__(kBlock) void k_shadow_apply(DevPaths P, DevScene sc
 }
 
 __global__ __launch_bounds__(kBlock) void k_shadow_grad(DevPaths P, DevScene sc, DevAdjoint A) {
-  __shared__ float glds[kAdjLdsFloats];
+  __shared__ float glds[kAdjLdsFloats + kTexTableFloats];
   __shared__ uint32_t prefix[kSegments + 1];
+  tex_table_init(glds);
   grad_lds_clear(A, glds);
   load_prefix(P.queue + kQueueShadowCursor, kSegLine, prefix);
   const uint32_t n = prefix[kSegments];
-  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-    const uint32_t q = queue_pos(prefix, i, P.seg_cap);
-    if (A.s_light[q] & kShadowVisible) light_visible<true>(P, sc, A, glds, q);
+  const uint32_t lane = threadIdx.x & 63u;
+  for (uint32_t w0 = blockIdx.x * kBlock + (threadIdx.x & ~63u); w0 < n; w0 += gridDim.x * kBlock) {
+    const uint32_t i = w0 + lane;
+    if (i < n) {
+      const uint32_t q = queue_pos(prefix, i, P.seg_cap);
+      if (A.s_light[q] & kShadowVisible) light_visible<true>(P, sc, A, glds, q);
+    }
+    tex_grad_flush(A, glds);  // the wave is converged here
   }
   grad_lds_flush(A, glds);
 }
