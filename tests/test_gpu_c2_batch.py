@@ -68,7 +68,7 @@ def test_batched_64_scenes_bitexact_vs_oracle():
                 for (fa, ca), (fb, cb) in zip(films["eager"], cur):
                     assert np.array_equal(ca, cb) and np.array_equal(fa, fb)
         print("C2: eager frame and two graph replays rendered", flush=True)
-    gpu32, ref32, gpu64, ref64 = [], [], [], []
+    gpu64, ref64 = [], []
     for i, (sc, w, h, spp) in enumerate(jobs):
         st = dataclasses.replace(settings, width=w, height=h, spp=spp)
         px = _subset(w, h, PIXELS_PER_SCENE, 1000 + i)
