@@ -54,6 +54,21 @@ MEASURED_VALU = {"extend": {"valu_busy": 0.99, "lane_utilisation": 0.54, "l2_hit
                  "shadow": {"valu_busy": 0.89, "lane_utilisation": 0.46, "l2_hit": 0.817},
                  "shade": {"valu_busy": 0.61, "lane_utilisation": 0.33, "l2_hit": 0.468}}
 MEASURED_TRAFFIC = {"extend": 20.585e9, "shadow": 10.763e9, "shade": 8.163e9, "film": 1.677e9, "raygen": 8.494e9}
+# The two tables above describe the kernels built from sources whose sha256 (over KERNEL_SOURCES, in
+# order) is PROFILED_KERNEL_SHA256; when the sources differ they are dropped from the line.
+KERNEL_SOURCES = ("vkpathtracer_amd/csrc/pt_kernels.hip", "vkpathtracer_amd/csrc/pt_device.h",
+                  "vkpathtracer_amd/csrc/pt_math.h")
+PROFILED_KERNEL_SHA256 = None
+
+
+def kernel_source_sha256() -> str:
+    import hashlib
+
+    h = hashlib.sha256()
+    for rel in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, rel), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()
 
 
 def cpu_model() -> str:
@@ -119,54 +134,96 @@ def _has_avx2() -> bool:
         return False
 
 
-def _cpu_rate(sc, st, variant: str, leaf: int, threads: int, target_seconds: float, rng_seed: int = 2024):
-    """Paths/s of the oracle `variant` (BVH leaves <= leaf) over a seeded pixel sample sized to take
-    about target_seconds for samples 0-3; returns (Mpaths/s, pixels, seconds)."""
+def _cpu_rate(sc, st, variant: str, leaf: int, threads: int, target_seconds: float, band_rows: int = 16,
+              samples: int = 4):
+    """Paths/s of the oracle `variant` (BVH leaves <= leaf) with the reference's tiled CPU schedule:
+    per sample, every band of band_rows whole rows is one job for one thread (pto_render_bands).
+    Every band_stride-th band is rendered, the stride chosen from a one-band probe so that samples
+    0..samples-1 take about target_seconds; returns (Mpaths/s, rows, bands, seconds)."""
     from oracle import oracle as O
 
     o = O.Oracle(sc, builder="sah", leaf_size=leaf, variant=variant)
-    rng = np.random.default_rng(rng_seed)
-    n_px = st.width * st.height
-    rate = 0.0
-    for k in (2000, 8000):  # a warm-up probe, then the calibration probe
-        probe = rng.choice(n_px, min(n_px, k * threads), replace=False).astype(np.uint32)
-        t0 = time.perf_counter()
-        o.render_pixels(st, probe, 0, threads=threads)
-        rate = len(probe) / max(time.perf_counter() - t0, 1e-6)
-    samples = 4
-    n = int(min(n_px, max(2000, rate * target_seconds / samples)))
-    px = rng.choice(n_px, n, replace=False).astype(np.uint32)
+    n_bands = (st.height + band_rows - 1) // band_rows
+    mid = n_bands // 2
+    o.render_bands(st, band_rows, mid, n_bands, 0, 1, threads=1)  # warm-up
     t0 = time.perf_counter()
-    for s_ in range(samples):
-        o.render_pixels(st, px, s_, threads=threads)
+    o.render_bands(st, band_rows, mid, n_bands, 0, 1, threads=1)
+    per_band = max(time.perf_counter() - t0, 1e-6)  # one thread, one band, one sample
+    bands_wanted = target_seconds * threads / (per_band * samples)
+    stride = max(1, int(n_bands // max(bands_wanted, 1.0)))
+    first = (stride // 2) if stride > 1 else 0
+    bands = len(range(first, n_bands, stride))
+    rows = sum(min(band_rows, st.height - b * band_rows) for b in range(first, n_bands, stride))
+    t0 = time.perf_counter()
+    o.render_bands(st, band_rows, first, stride, 0, samples, threads=threads)
     dt = time.perf_counter() - t0
-    return n * samples / dt / 1e6, n, dt
+    return rows * st.width * samples / dt / 1e6, rows, bands, dt
 
 
 def cpu_baseline(sc, st, target_seconds: float, log) -> dict:
-    """The CPU restatement (oracle, fp32) on the host cores, three ways, each on a bounded seeded pixel
-    sample of the same workload (samples 0-3 of n pixels): scalar = one thread, leaves of 4;
-    multicore = OpenMP over every core, leaves of 4; avx2 = every core, leaves of 8 intersected 8
-    triangles at a time with AVX2 (the reference's AVX2 configuration, CpuBvhAccelerator.cpp:252-256).
-    `value` is the AVX2 rate (multicore when the host lacks AVX2)."""
+    """The CPU restatement (oracle, fp32, glibc sinf / cosf / powf as the reference is built) on the host
+    cores with TiledCpuPathTracer's schedule (16-row bands, one band per job per sample), three ways, each
+    on a bounded set of bands of the same frame, samples 0-3: scalar = one thread, leaves of 4;
+    multicore = every core, leaves of 4; avx2 = every core, leaves of 8 intersected 8 triangles at a time
+    with AVX2 (the reference's AVX2 configuration, CpuBvhAccelerator.cpp:252-256). `value` is the AVX2 rate
+    (multicore when the host lacks AVX2)."""
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     modes = {}
-    plan = [("scalar", "f32", 4, 1), ("multicore", "f32", 4, cores)]
+    plan = [("scalar", "f32_libm", 4, 1), ("multicore", "f32_libm", 4, cores)]
     if _has_avx2():
-        plan.append(("avx2", "f32_avx2", 8, cores))
+        plan.append(("avx2", "f32_avx2_libm", 8, cores))
     for name, variant, leaf, threads in plan:
-        rate, n, dt = _cpu_rate(sc, st, variant, leaf, threads, target_seconds)
-        modes[name] = {"mpaths_per_s": round(rate, 4), "threads": threads, "leaf": leaf,
-                       "sample": f"{n} pixels x 4 samples, {dt:.1f} s"}
-        log(f"cpu {name}: {rate:.3f} Mpaths/s ({threads} threads)")
+        rate, rows, bands, dt = _cpu_rate(sc, st, variant, leaf, threads, target_seconds)
+        modes[name] = {"mpaths_per_s": round(rate, 4), "threads": threads, "leaf": leaf, "oracle_build": variant,
+                       "sample": f"{bands} bands of 16 rows ({rows} x {st.width} pixels) x 4 samples, {dt:.1f} s"}
+        log(f"cpu {name}: {rate:.3f} Mpaths/s ({threads} threads, {bands} bands)")
     top = "avx2" if "avx2" in modes else "multicore"
     m = modes[top]
     return {"value": m["mpaths_per_s"], "unit": "Mpaths/s", "cores": m["threads"], "kind": "port",
             "mode": top,
-            "sample": f"{m['sample']} of seeded pixels of the same workload, oracle fp32 restatement "
-                      f"({'AVX2 leaf-8 intersector, ' if top == 'avx2' else ''}OpenMP over {m['threads']} threads), "
-                      f"{cpu_model()}",
+            "sample": f"{m['sample']} of the same frame; oracle fp32 restatement with glibc math "
+                      f"({'AVX2 leaf-8 intersector, ' if top == 'avx2' else ''}16-row band jobs as TiledCpuPathTracer, "
+                      f"OpenMP over {m['threads']} threads), {cpu_model()}",
             "modes": modes}
+
+
+def parity_report(pt, sc, st, leaf: int, log, pixels: int = 1500, rays: int = 250_000) -> dict:
+    """The benchmark frame against the CPU oracle (fp32, the arithmetic the kernels restate), outside
+    the timed region: the film's per-pixel mean radiance on `pixels` seeded pixels against the oracle's
+    over the same samples (relative RMSE and the fraction of pixels bit-identical); closest-hit primitive
+    ids and t of `rays` seeded random rays (mismatch counts); and the fraction of their triangle tests
+    that the |det| < 1e-4 cut-off rejects (BvhTriangleIntersector.cpp:48)."""
+    from oracle import oracle as O
+    from vkpathtracer_amd import scene as S
+
+    o = O.Oracle(sc, builder="sah", leaf_size=leaf)
+    film, cnt = pt.film()
+    px = np.sort(np.random.default_rng(7).choice(st.width * st.height, pixels, replace=False)).astype(np.uint32)
+    acc = np.zeros((len(px), 3), np.float32)
+    n = np.zeros(len(px), np.uint32)
+    for s_ in range(st.spp):
+        rgb, valid, _ = o.render_pixels(st, px, s_)
+        rgb = np.asarray(rgb, np.float32)
+        acc[valid] = acc[valid] + rgb[valid]
+        n += valid
+    g = film.reshape(-1, 3)[px]
+    gm = g.astype(np.float64) / np.maximum(cnt.reshape(-1)[px], 1)[:, None]
+    om = acc.astype(np.float64) / np.maximum(n, 1)[:, None]
+    den = float(np.sqrt(np.mean(om * om))) or 1.0
+    relrmse = float(np.sqrt(np.mean((gm - om) ** 2))) / den
+    same = float(np.mean(np.all(g == acc, axis=1) & (cnt.reshape(-1)[px] == n)))
+    r = S.random_rays(sc, rays, seed=11)
+    t, p, _ = pt.trace_rays(r)
+    rt, rp, _ = o.trace_rays(r)
+    tests, rejects = o.det_rejections(r)
+    out = {"relrmse": relrmse, "pixels": len(px), "spp": st.spp, "bitexact_pixel_fraction": round(same, 6),
+           "rays": rays, "primid_mismatch": int(np.count_nonzero(p != rp)),
+           "t_mismatch": int(np.count_nonzero((t != rt) & ~(np.isnan(t) & np.isnan(rt)))),
+           "det_reject_fraction": rejects / tests if tests else None, "triangle_tests": tests,
+           "oracle": "fp32 restatement (oracle/pto.c, portable math)"}
+    log(f"parity: relRMSE {relrmse:.3e} over {len(px)} pixels x {st.spp} spp, {same:.4f} of pixels bit-identical, "
+        f"{out['primid_mismatch']} primitive-id mismatches in {rays} rays, |det| rejections {rejects}/{tests}")
+    return out
 
 
 def adjoint_ratio(log, reps: int = 3) -> dict:
@@ -371,6 +428,7 @@ def main():
                     help="leaf width and launch shapes: the autotuner's table for the workload, or the defaults")
     ap.add_argument("--cpu-seconds", type=float, default=8.0, help="target CPU time per baseline mode")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-parity", action="store_true", help="skip the oracle comparison of the benchmark frame")
     ap.add_argument("--no-profile", action="store_true", help="time without per-kernel events")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward time ratio")
     ap.add_argument("--no-batched", action="store_true", help="skip the many-scene batch comparison")
@@ -469,7 +527,9 @@ def main():
         k = ks[dom]
         achieved = kb[dom] / (k["total_ms"] * 1e-3) / 1e9
         avg_ms = k["total_ms"] / k["launches"]
-        profiled = args.workload == "procedural" and args.spp is None and not args.capacity
+        sha = kernel_source_sha256()
+        same_shape = args.workload == "procedural" and args.spp is None and not args.capacity
+        profiled = same_shape and sha == PROFILED_KERNEL_SHA256
         traffic = MEASURED_TRAFFIC.get(dom) if profiled else None
         roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
@@ -481,10 +541,16 @@ def main():
                     "avg_launch_ms": round(avg_ms, 4),
                     "valu": {n: MEASURED_VALU[n] for n in ("extend", "shadow", "shade")} if profiled else None,
                     "valu_source": "profiles/wavefront_procedural_1080p_64spp.md (rocprofv3 SQ_* and TCC_* counters)"
-                    if profiled else None}
+                    if profiled else None,
+                    "kernel_source_sha256": sha,
+                    "profiled_kernel_sha256": PROFILED_KERNEL_SHA256,
+                    "profile_note": None if profiled else
+                    ("kernel sources differ from the profiled ones: traffic and valu omitted" if same_shape
+                     else "not the profiled workload shape: traffic and valu omitted")}
 
     log(f"timed {args.steps} steps: {dt:.3f} s")
     bvh_info = {k: pt.build_info[k] for k in ("node_count", "leaf_count", "max_depth", "device_bytes")}
+    parity = None if args.no_parity else parity_report(pt, sc, st, leaf, log)
     pt.close()
     adjoint = None if args.no_adjoint else adjoint_ratio(log)
     batched = None if args.no_batched else batched_comparison(log)
@@ -520,6 +586,7 @@ def main():
                    "params": tuned},
         "roofline": roofline,
         "kernels": per_kernel,
+        "parity": parity,
         "cpu_baseline": cpu,
         "adjoint": adjoint,
         "batched": batched,

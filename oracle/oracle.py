@@ -19,7 +19,7 @@ import numpy as np
 
 HERE = pathlib.Path(__file__).resolve().parent
 LIB_DIR = HERE / "lib"
-VARIANTS = ("f32", "f32_libm", "f64")
+VARIANTS = ("f32", "f32_libm", "f64", "f32_avx2", "f32_avx2_libm")
 
 
 class PtoCamera(C.Structure):
@@ -114,6 +114,11 @@ _SIGS["pto_camera_dir"] = (C.c_int, [C.POINTER(PtoScene), C.POINTER(PtoSettings)
                                      C.c_float, _P])
 _SIGS["pto_hemisphere_samples"] = (C.c_int, [_P, C.c_uint64, C.c_uint32, _P])
 _SIGS["pto_light_pdf"] = (C.c_double, [C.POINTER(PtoScene), _P, _P])
+
+_SIGS["pto_det_rejections"] = (C.c_int, [_P, _P, C.c_uint32, C.c_int, _P])
+_SIGS["pto_render_bands"] = (C.c_int, [C.POINTER(PtoScene), _P, C.POINTER(PtoSettings), C.c_uint32, C.c_uint32,
+                                       C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.c_int, C.c_int, _P, _P,
+                                       C.POINTER(PtoCounters)])
 
 _libs: dict[str, C.CDLL] = {}
 
@@ -283,6 +288,31 @@ class Oracle:
         p = np.zeros(n, np.int32)
         self.L.pto_brute_force_rays(self.accel, rays.ctypes.data, n, t.ctypes.data, p.ctypes.data)
         return t, p
+
+    def det_rejections(self, rays: np.ndarray, any_hit: bool = False) -> tuple[int, int]:
+        """(triangle tests, tests rejected by |det| < 1e-4) of closest- (or any-) hit queries."""
+        rays = np.ascontiguousarray(rays, np.float32)
+        out = np.zeros(2, np.uint64)
+        self.L.pto_det_rejections(self.accel, rays.ctypes.data, len(rays), int(any_hit), out.ctypes.data)
+        return int(out[0]), int(out[1])
+
+    def render_bands(self, settings, band_rows: int = 16, band_first: int = 0, band_stride: int = 1,
+                     sample_begin: int = 0, sample_count: int | None = None, frame_index: int = 0, threads: int = 0,
+                     shadow: int = SHADOW_ANYHIT):
+        """render_tile over whole-width bands of band_rows rows (bands band_first + k * band_stride), one
+        band per job per sample: (film (H, W, 3), counts (H, W), counters)."""
+        W, H = settings.width, settings.height
+        film = np.zeros((H, W, 3), self.real)
+        cnt = np.zeros((H, W), np.uint32)
+        c = PtoCounters()
+        st = settings_struct(settings)
+        count = settings.spp if sample_count is None else sample_count
+        rc = self.L.pto_render_bands(C.byref(self.scene.s), self.accel, C.byref(st), band_rows, band_first, band_stride,
+                                     sample_begin, count, frame_index, threads, shadow, film.ctypes.data,
+                                     cnt.ctypes.data, C.byref(c))
+        if rc != 0:
+            raise RuntimeError(f"oracle render_bands failed ({rc})")
+        return film, cnt, c.as_dict()
 
     def camera_rays(self, settings, pixels: np.ndarray, sample_index: int, frame_index: int = 0):
         """The primary rays shade_pixel makes for one sample of each pixel: (rays (N, 6), jitter (N, 2))."""

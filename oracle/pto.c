@@ -756,6 +756,10 @@ static int slab(const aabb_t* b, const rayq_t* q, real max_t, real* t_near) {
 
 typedef struct { int hit; real t; uint32_t prim; real u, v; } qhit_t;
 
+/* Triangle tests of this thread rejected by |det| < 1e-4 (BvhTriangleIntersector.cpp:48), for the
+ * benchmark's report of how often the scale-dependent cut-off fires (pto_det_rejections). */
+static _Thread_local uint64_t tls_det_rejects;
+
 /* Leaf batch: IntersectBvhTriangleBatch / intersect_scalar / commit_ordered_lane_hit,
  * BvhTriangleIntersector.cpp:25-74, :186-209. Returns accepted count; updates *best. */
 static uint32_t leaf_batch(const pto_accel* a, const uint32_t* prims, uint32_t cnt, v3 o, v3 d,
@@ -770,7 +774,10 @@ static uint32_t leaf_batch(const pto_accel* a, const uint32_t* prims, uint32_t c
     const real hy = d.z * p->e2.x - d.x * p->e2.z;
     const real hz = d.x * p->e2.y - d.y * p->e2.x;
     const real det = p->e1.x * hx + p->e1.y * hy + p->e1.z * hz;
-    if (R_FABS(det) < kEps) continue;
+    if (R_FABS(det) < kEps) {
+      ++tls_det_rejects;
+      continue;
+    }
     const real inv_det = RC(1.0) / det;
     const real sx = o.x - p->v0.x, sy = o.y - p->v0.y, sz = o.z - p->v0.z;
     const real u = (sx * hx + sy * hy + sz * hz) * inv_det;
@@ -933,6 +940,21 @@ int pto_trace_rays(const pto_accel* a, const float* rays, uint32_t n, int any_hi
     out_prim[i] = h.hit ? (int32_t)a->prims[h.prim].primitive_index : -1;
   }
   if (counters) *counters = c;
+  return 0;
+}
+
+/* pto_trace_rays (closest or any hit) that also reports how many of the triangle tests the |det| <
+ * 1e-4 cut-off rejected: out[0] = triangle tests, out[1] = rejections. Single-threaded. */
+int pto_det_rejections(const pto_accel* a, const float* rays, uint32_t n, int any_hit, uint64_t out[2]) {
+  PtoCounters c;
+  memset(&c, 0, sizeof(c));
+  tls_det_rejects = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    const v3 o = v3f(rays + 6 * i), d = v3f(rays + 6 * i + 3);
+    (void)traverse(a, o, d, (real)INFINITY, any_hit, &c);
+  }
+  out[0] = c.triangle_tests;
+  out[1] = tls_det_rejects;
   return 0;
 }
 
@@ -2222,6 +2244,66 @@ int pto_render_tile(const PtoScene* scene, const pto_accel* a, const PtoSettings
         f[1] = f[1] + v.y;
         f[2] = f[2] + v.z;
         film_count[pixel] += 1u;
+      }
+#pragma omp critical
+      add_counters(&total, &lc);
+    }
+  }
+  if (counters) *counters = total;
+  return 0;
+}
+
+/* TiledCpuPathTracer's schedule (TiledCpuPathTracer.cpp:279-323, :663-773): for each sample, the
+ * image is cut into bands of band_rows rows and each band is one job rendered by one thread in
+ * row-major order (ScalarCpuPathTracer::render_tile on the band), jobs taken dynamically by the
+ * workers. Bands band_first, band_first + band_stride, ... below the image height are rendered;
+ * samples [sample_begin, sample_begin + sample_count) are added to the film in order, so the film
+ * equals pto_render_tile's over the same rows. Used to time the CPU baseline. */
+int pto_render_bands(const PtoScene* scene, const pto_accel* a, const PtoSettings* s, uint32_t band_rows,
+                     uint32_t band_first, uint32_t band_stride, uint32_t sample_begin, uint32_t sample_count,
+                     uint64_t frame_index, int threads, int shadow_mode, void* film_sum, uint32_t* film_count,
+                     PtoCounters* counters) {
+  if (!scene || !a || !s || s->width == 0 || s->height == 0 || band_rows == 0 || band_stride == 0) return -1;
+  const int sup = check_supported(scene);
+  if (sup) return sup;
+  if (shadow_mode == PTO_SHADOW_ANYHIT && (any_transmissive(scene) || a->sdf_count)) shadow_mode = PTO_SHADOW_CLOSEST;
+  ctx_t x = {scene, a, s, s->max_depth > 1u ? s->max_depth : 1u, shadow_mode};
+  const camera_t cam = make_camera(scene, s);
+  real* film = (real*)film_sum;
+  const uint32_t n_bands = (s->height + band_rows - 1u) / band_rows;
+  const int64_t jobs = band_first < n_bands ? (int64_t)((n_bands - 1u - band_first) / band_stride + 1u) : 0;
+  PtoCounters total;
+  memset(&total, 0, sizeof(total));
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#else
+  (void)threads;
+#endif
+  for (uint32_t si = 0; si < sample_count; ++si) {
+    const uint32_t sample = sample_begin + si;
+#pragma omp parallel
+    {
+      PtoCounters lc;
+      memset(&lc, 0, sizeof(lc));
+#pragma omp for schedule(dynamic, 1)
+      for (int64_t j = 0; j < jobs; ++j) {
+        const uint32_t y0 = (band_first + (uint32_t)j * band_stride) * band_rows;
+        const uint32_t y1 = y0 + band_rows < s->height ? y0 + band_rows : s->height;
+        for (uint32_t py = y0; py < y1; ++py)
+          for (uint32_t px = 0; px < s->width; ++px) {
+            const uint32_t pixel = py * s->width + px;
+            v3 v;
+            uint64_t rays = 0;
+            const int ok = shade_pixel(&x, &cam, pixel, sample, frame_index, &v, &rays, &lc, NULL);
+            if (!ok) continue;
+            lc.rays += rays;
+            lc.samples++;
+            real* f = film + 3u * (uint64_t)pixel;
+            f[0] = f[0] + v.x;
+            f[1] = f[1] + v.y;
+            f[2] = f[2] + v.z;
+            film_count[pixel] += 1u;
+          }
       }
 #pragma omp critical
       add_counters(&total, &lc);

@@ -179,6 +179,15 @@ int pto_adjoint_pixels(const PtoScene* scene, const pto_accel* a, const PtoSetti
 int pto_phong_samples(float exponent, const float refl[3], const float n[3], float ps, uint64_t state,
                       uint32_t count, float* out_dirs, double* out_score);
 
+/* pto_trace_rays' traversal, counting the triangle tests the |det| < 1e-4 cut-off rejects:
+ * out[0] = triangle tests, out[1] = rejections. */
+int pto_det_rejections(const pto_accel* a, const float* rays, uint32_t n, int any_hit, uint64_t out[2]);
+/* render_tile with TiledCpuPathTracer's band jobs (see pto.c); the CPU baseline's schedule. */
+int pto_render_bands(const PtoScene* scene, const pto_accel* a, const PtoSettings* s, uint32_t band_rows,
+                     uint32_t band_first, uint32_t band_stride, uint32_t sample_begin, uint32_t sample_count,
+                     uint64_t frame_index, int threads, int shadow_mode, void* film_sum, uint32_t* film_count,
+                     PtoCounters* counters);
+
 /* Probes for closed-form checks (see pto.c): the primary rays and pixel jitter shade_pixel uses, the
  * pinhole direction at a given jitter, cosine-hemisphere samples, and light_pdf_for_direction. */
 int pto_camera_rays(const PtoScene* scene, const PtoSettings* s, const uint32_t* pixels, uint32_t n,

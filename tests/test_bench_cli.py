@@ -34,3 +34,13 @@ def test_cpu_baseline_modes_run():
     out = bench.cpu_baseline(S.cornell_box(), st, 0.05, lambda m: None)
     assert out["kind"] == "port" and out["value"] > 0 and {"scalar", "multicore"} <= set(out["modes"])
     assert out["modes"]["scalar"]["threads"] == 1
+
+
+def test_profiled_constants_are_stamped_with_the_kernel_sources():
+    """The traffic and VALU figures pasted from the committed rocprofv3 profile are reported only while
+    the kernel sources hash to the profiled ones."""
+    import bench
+
+    sha = bench.kernel_source_sha256()
+    assert len(sha) == 64 and sha == bench.kernel_source_sha256()
+    assert bench.PROFILED_KERNEL_SHA256 is None or len(bench.PROFILED_KERNEL_SHA256) == 64
