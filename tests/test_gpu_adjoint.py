@@ -136,6 +136,35 @@ def test_autograd_function_inverse_rendering_step():
     r.close()
 
 
+def test_autograd_backward_uses_its_own_forward_parameters():
+    """Two forward calls with different parameters before the first one's backward: that backward
+    equals the backward of a lone forward at the first parameters (the renderer uploads them again)."""
+    import torch
+
+    from vkpathtracer_amd import scene as S
+    from vkpathtracer_amd.autograd import DifferentiableRenderer
+
+    st = S.RenderSettings(width=24, height=24, spp=2, max_depth=4)
+    r = DifferentiableRenderer(_scene(), st)
+    a0, e0, l0 = r.initial_parameters()
+    g = torch.randn((st.height, st.width, 3), device=a0.device)
+
+    def grads_of(a, e, l, other=None):
+        a, e, l = [x.clone().requires_grad_(True) for x in (a, e, l)]
+        img = r(a, e, l)
+        if other is not None:
+            with torch.no_grad():
+                r(*other)  # the renderer now holds other parameters
+        (img * g).sum().backward()
+        return [x.grad.cpu().numpy().astype(np.float64) for x in (a, e, l)]
+
+    ref = grads_of(a0, e0, l0)
+    got = grads_of(a0, e0, l0, other=(torch.clamp(a0 * 0.5 + 0.2, 0.05, 0.95), e0 * 2.0, l0 * 0.5))
+    for x, y in zip(got, ref):
+        assert np.abs(x - y).max() <= 1e-5 * max(np.abs(y).max(), 1e-6)
+    r.close()
+
+
 def test_adjoint_reuses_forward_radiance_only_when_valid():
     """A render_tiles of the same tiles leaves each path's radiance in place and the adjoint skips its
     forward replay; after update_scene_delta it must replay. Gradients agree with a fresh tracer's

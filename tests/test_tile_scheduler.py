@@ -91,3 +91,33 @@ def test_batches_share_a_gpu_and_respect_the_cap():
     out = [("keep",)]
     s.begin_sample(0, 2)
     assert s.next_tile_batch(2, out) == 2 and out[0] == ("keep",) and len(out) == 3  # appends
+
+
+def test_order_is_kept_while_feedback_stays_within_its_buckets():
+    """TileScheduler.cpp:184-223: feedback that moves a band's variance within its logarithmic bucket
+    (or its sample count within a doubling bucket) keeps the previous order, even where the exact
+    comparator would now sort differently; leaving the bucket re-sorts."""
+    from vkpathtracer_amd.tile_scheduler import (TilePriorityFeedback, TileScheduler, TileSchedulerConfig,
+                                                 sample_count_bucket, variance_bucket)
+
+    s = TileScheduler(TileSchedulerConfig(width=8, height=48, tile_height=16))
+    s.set_feedback([TilePriorityFeedback(0, 1.00, 4), TilePriorityFeedback(1, 1.01, 4), TilePriorityFeedback(2, 0.1, 4)])
+    s.begin_sample(1, 0)
+    first = [s.next_tile()[5] for _ in range(3)]
+    assert first == [1, 0, 2]
+    assert variance_bucket(1.02) == variance_bucket(1.01) == variance_bucket(1.00)
+    s.set_feedback([TilePriorityFeedback(0, 1.02, 4)])  # band 0 now has the larger variance, same bucket
+    s.begin_sample(1, 1)
+    assert [s.next_tile()[5] for _ in range(3)] == first
+    # band 2's sample count leaves its bucket (4 -> 33): re-sorted with the exact comparator
+    s.set_feedback([TilePriorityFeedback(2, 0.1, 33)])
+    s.begin_sample(1, 2)
+    assert [s.next_tile()[5] for _ in range(3)] == [0, 1, 2]
+    # within buckets again (33 -> 40, variance 1.01 -> 1.03): the order stays although band 1 now leads
+    assert sample_count_bucket(40) == sample_count_bucket(33) and variance_bucket(1.03) == variance_bucket(1.01)
+    s.set_feedback([TilePriorityFeedback(2, 0.1, 40), TilePriorityFeedback(1, 1.03, 4)])
+    s.begin_sample(1, 3)
+    assert [s.next_tile()[5] for _ in range(3)] == [0, 1, 2]
+    s.set_feedback([TilePriorityFeedback(1, 5.0, 4)])  # another bucket: re-sorted
+    s.begin_sample(1, 4)
+    assert [s.next_tile()[5] for _ in range(3)] == [1, 0, 2]

@@ -21,20 +21,27 @@ from .tracer import HipPathTracer
 
 
 class _RenderFn(torch.autograd.Function):
+    """The backward pass replays the paths at the parameters of its own forward pass: each upload bumps
+    the renderer's version, the forward keeps its parameters and version, and a backward that finds
+    the renderer holding other parameters (another forward ran in between) uploads its own again."""
+
     @staticmethod
     def forward(ctx, albedo, emission, light_color, roughness, texture, renderer):
         renderer.upload(albedo, emission, light_color, roughness, texture)
         image, count = renderer.render()
         ctx.renderer = renderer
+        ctx.version = renderer.version
         ctx.has_roughness = roughness is not None
         ctx.texture_shape = None if texture is None else texture.shape
-        ctx.save_for_backward(count)
+        ctx.save_for_backward(count, albedo, emission, light_color, roughness, texture)
         return image
 
     @staticmethod
     def backward(ctx, grad_image):
-        (count,) = ctx.saved_tensors
+        count, albedo, emission, light_color, roughness, texture = ctx.saved_tensors
         r = ctx.renderer
+        if r.version != ctx.version:
+            r.upload(albedo, emission, light_color, roughness, texture)
         inv = 1.0 / torch.clamp(count, min=1).to(torch.float32)
         film_grad = (grad_image.to(torch.float32) * inv[..., None]).contiguous()
         dev = film_grad.device
@@ -66,6 +73,7 @@ class DifferentiableRenderer:
         self.n_lights = len(scene.lights)
         self.tiles = [(0, 0, settings.width, settings.height, s, 0, 0) for s in range(settings.spp)]
         self.device = torch.device("cuda", device)
+        self.version = 0  # bumped by every upload of parameters
 
     def initial_parameters(self):
         m, l = self.scene.materials, self.scene.lights
@@ -94,6 +102,7 @@ class DifferentiableRenderer:
             raise RuntimeError(self.tracer.status().last_error)
         if texture is not None:
             self.tracer.update_texels(texture.detach().to(self.device, torch.float32).contiguous())
+        self.version += 1
 
     def adjoint(self, film_grad, ga, ge, gl, gr, gt):
         self.tracer.render_adjoint(self.tiles, film_grad, ga, ge, gl, self.frame_index, grad_roughness=gr,
@@ -138,12 +147,14 @@ class MultiViewRenderer(DifferentiableRenderer):
         self.n_materials = len(scene.materials)
         self.n_lights = len(scene.lights)
         self.device = torch.device("cuda", device)
+        self.version = 0
 
     def upload(self, albedo, emission, light_color, roughness=None, texture=None):
         mats, lights = self._records(albedo, emission, light_color, roughness)
         self.batch.update_scene_delta(mats, lights if len(lights) else None)
         if texture is not None:
             self.batch.update_texels(texture.detach().to(self.device, torch.float32).contiguous())
+        self.version += 1
 
     def render(self):
         self.batch.reset_accumulation()

@@ -15,8 +15,11 @@ render coordinator does (RenderCoordinator.cpp:1350-1391: drain up to 64 same-GP
   index of a slice is `sample_index * (k + 1) + offset` so every slice renders a distinct sample
   (TileScheduler.cpp:104-129, :287-321).
 
-The reference caches the sorted order behind a fingerprint of its inputs; that only skips work, so
-it is not restated. Tiles are the 7-tuples `(x, y, width, height, sample_index, tile_id, gpu_id)`
+The order is rebuilt only when a fingerprint of its inputs changes (TileScheduler.cpp:184-223): an
+FNV-1a mix of the band count, the foveated extra samples, tile height, image height and GPU count, and
+per band its dirty flag, its variance quantised to about 256 logarithmic buckets and its sample count
+bucket (exact below 16, then one bucket per power of two). Feedback that changes only within a bucket
+therefore keeps the previous order, as in the reference. Tiles are the 7-tuples `(x, y, width, height, sample_index, tile_id, gpu_id)`
 that `HipPathTracer.render_tiles` takes (the fields of `PtRenderTile`).
 """
 from __future__ import annotations
@@ -25,6 +28,38 @@ import dataclasses
 
 TILE_BATCH_MAX = 64  # RenderCoordinator.cpp:1350
 _U32_MAX = 0xFFFFFFFF
+_FNV_OFFSET = 0xCBF29CE484222325  # FNV-1a 64 (TileScheduler.cpp:18-27)
+_FNV_PRIME = 0x100000001B3
+_U64 = (1 << 64) - 1
+
+
+def _fnv_mix(h: int, v: int) -> int:
+    """Mixes the 8 bytes of v, low byte first."""
+    for i in range(8):
+        h ^= (v >> (8 * i)) & 0xFF
+        h = (h * _FNV_PRIME) & _U64
+    return h
+
+
+def variance_bucket(variance: float) -> int:
+    """VarianceBucket (TileScheduler.cpp:35-46): about 256 bins over log10 of [1e-9, 1e9]."""
+    import math
+
+    if not (variance > 0.0):
+        return 0
+    scaled = (math.log10(min(max(variance, 1e-9), 1e9)) + 9.0) * (256.0 / 18.0)
+    return int(min(max(scaled, 0.0), 255.0))
+
+
+def sample_count_bucket(samples: int) -> int:
+    """SampleCountBucket (TileScheduler.cpp:48-58): exact below 16, then one bucket per doubling."""
+    if samples < 16:
+        return samples
+    bucket, v = 16, samples
+    while v >= 32:
+        v >>= 1
+        bucket += 1
+    return bucket
 
 
 @dataclasses.dataclass
@@ -67,6 +102,7 @@ class TileScheduler:
         self._tile_count = 0
         self._feedback: list[TilePriorityFeedback] = []
         self._order: list[tuple[int, int]] = []  # (tile_id, sample_offset)
+        self._fingerprint: int | None = None  # of the inputs the current order was built from
         if config is not None:
             self.configure(config)
 
@@ -81,6 +117,7 @@ class TileScheduler:
         self._tile_count = 0 if c.height == 0 else (c.height + c.tile_height - 1) // c.tile_height
         self._feedback = [TilePriorityFeedback(tile_id=i) for i in range(self._tile_count)]
         self._order = []
+        self._fingerprint = None
         self._next = 0
 
     def set_feedback(self, feedback) -> None:
@@ -142,7 +179,23 @@ class TileScheduler:
         center = (float(y) + float(h) * 0.5) / float(c.height)
         return abs(center - 0.5) <= c.foveated_center_radius
 
+    def _fingerprint_of_inputs(self) -> int:
+        c = self._config
+        h = _FNV_OFFSET
+        for v in (self._tile_count, c.foveated_center_extra_samples, c.tile_height, c.height, c.gpu_count):
+            h = _fnv_mix(h, int(v) & _U64)
+        for t in range(self._tile_count):
+            f = self._feedback[t]
+            packed = ((1 if f.dirty else 0) << 56) | ((variance_bucket(f.variance) & 0xFF) << 48) | \
+                     ((sample_count_bucket(int(f.sample_count)) & 0xFFFF) << 32) | t
+            h = _fnv_mix(h, packed)
+        return h
+
     def _rebuild_order(self) -> None:
+        fingerprint = self._fingerprint_of_inputs()
+        if self._order and fingerprint == self._fingerprint:
+            return  # the bucketed sort keys are unchanged: keep the previous order
+        self._fingerprint = fingerprint
         fb = self._feedback
         dirty = sum(1 for f in fb if f.dirty)
         mixed = 0 < dirty < self._tile_count
