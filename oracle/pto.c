@@ -21,6 +21,9 @@
  *   MIS with lights[0], light pdf,
  *   roulette, samplers, effects, env .. closed forms, sampler moments, a second transcription of
  *                                        the effects (test_oracle_integrator.py)
+ *   trace at depth 8, every branch .... a second, float64 transcription of camera_rays, trace,
+ *                                        sample_direct_light and the shadow skip loop: every sample
+ *                                        within 1e-4 (test_oracle_trace_transcription.py)
  *   dropped samples and ray counts .... the reference's counting rule (test_oracle_counters.py)
  *   film resolve ...................... closed forms and the libm build (test_oracle_resolve.py)
  *   adjoint ........................... finite differences (test_oracle_adjoint.py, test_oracle_textures.py)
