@@ -334,7 +334,7 @@ int pt_hip_trace_rays(pt_tracer* tracer, int mode, const float* rays, const floa
  * allocate device memory — code under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc). Same arguments, results and counters as pt_hip_trace_rays; rays, tmax, out_t and out_prim
  * are device pointers. The traversal's scratch (counter stripes, queue cursor, error flag and the stack
- * spill area of a 512-workgroup grid) lives in one static 24 MiB device array per GPU, set up when
+ * spill area of a 512-workgroup grid) lives in one static 26 MiB device array per GPU, set up when
  * the first tracer on that GPU is created and held until the process ends. Trees deeper than
  * PT_NOALLOC_MAX_TREE_DEPTH levels are rejected with PT_UNSUPPORTED. Every call on one GPU shares that
  * GPU's array: calls that overlap on one GPU (two streams, two threads) race and must be serialised by

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstring>
 #include <new>
+#include <queue>
 #include <string>
 #include <thread>
 #include <vector>
@@ -120,6 +121,7 @@ struct pt_tracer {
   DevArray<ptd::DevSdf> d_sdfs;
   DevArray<float> d_texels, d_prim_uv;
   DevArray<uint32_t> d_prim_tri;
+  DevArray<uint32_t> d_hot_src;    // the pool node each hot copy duplicates (refreshed after a refit)
   DevArray<float> d_grad_partial;  // adjoint: per-block gradient rows
   size_t texel_count = 0;
   DevArray<uint32_t> d_film_count, d_queue, d_error;
@@ -339,6 +341,7 @@ void free_scene_device(pt_tracer* t) {
   t->d_texels.release();
   t->d_prim_uv.release();
   t->d_prim_tri.release();
+  t->d_hot_src.release();
 }
 
 void free_paths(pt_tracer* t) {
@@ -747,6 +750,61 @@ int upload_pool(pt_tracer* t) {
     film_base += s.width * s.height;
   }
 
+  // Hot copies for the traversal kernels' LDS: in each tree (the hot budget split evenly between the
+  // pool's trees, in pool order) the inner nodes of largest box surface area, taken from the root
+  // down (a child's box lies in its parent's, so the set is a top subtree), copied after the pool's
+  // nodes with their hot children's references pointing at the copies. Scene records whose root is
+  // hot start at its copy. Traversal order and results do not depend on where a node is stored.
+  std::vector<uint32_t> hot_src;
+  {
+    std::vector<size_t> trees;
+    for (size_t si = 0; si < t->scenes.size(); ++si)
+      if (t->scenes[si].view_of < 0 && recs[si].prim_count && recs[si].root_ref >= 0) trees.push_back(si);
+    const size_t per = trees.empty() ? 0 : std::max<size_t>(1, ptd::kHotNodes / trees.size());
+    auto area = [](const float* mn, const float* mx) {  // non-finite boxes rank last
+      const float ex = mx[0] - mn[0], ey = mx[1] - mn[1], ez = mx[2] - mn[2];
+      const float a = ex * ey + ey * ez + ez * ex;
+      return std::isfinite(a) ? a : -1.0f;
+    };
+    for (size_t si : trees) {
+      const size_t quota = std::min(per, static_cast<size_t>(ptd::kHotNodes) - hot_src.size());
+      if (quota == 0) break;
+      std::priority_queue<std::pair<float, int32_t>> frontier;  // (area, node): largest first, then larger index
+      frontier.emplace(area(recs[si].root_box, recs[si].root_box + 3), recs[si].root_ref);
+      size_t taken = 0;
+      while (!frontier.empty() && taken < quota) {
+        const int32_t ni = frontier.top().second;
+        frontier.pop();
+        hot_src.push_back(static_cast<uint32_t>(ni));
+        ++taken;
+        const float* f = &nodes[16u * static_cast<uint32_t>(ni)];
+        int32_t r[2];
+        std::memcpy(r, f + 12, sizeof(r));
+        for (int k = 0; k < 2; ++k)
+          if (r[k] >= 0) frontier.emplace(area(f + 6 * k, f + 6 * k + 3), r[k]);
+      }
+    }
+  }
+  const uint32_t hot_base = static_cast<uint32_t>(n_nodes), hot_count = static_cast<uint32_t>(hot_src.size());
+  if (hot_count) {
+    std::vector<int32_t> hot_slot(n_nodes, -1);
+    for (uint32_t h = 0; h < hot_count; ++h) hot_slot[hot_src[h]] = static_cast<int32_t>(h);
+    nodes.resize(16u * (n_nodes + hot_count));
+    for (uint32_t h = 0; h < hot_count; ++h) {
+      float* dst = &nodes[16u * (hot_base + h)];
+      std::memcpy(dst, &nodes[16u * hot_src[h]], 16 * sizeof(float));
+      int32_t r[2];
+      std::memcpy(r, dst + 12, sizeof(r));
+      for (int k = 0; k < 2; ++k)
+        if (r[k] >= 0 && hot_slot[static_cast<uint32_t>(r[k])] >= 0)
+          r[k] = static_cast<int32_t>(hot_base) + hot_slot[static_cast<uint32_t>(r[k])];
+      std::memcpy(dst + 12, r, sizeof(r));
+    }
+    for (ptd::SceneRec& R : recs)
+      if (R.prim_count && R.root_ref >= 0 && hot_slot[static_cast<uint32_t>(R.root_ref)] >= 0)
+        R.root_ref = static_cast<int32_t>(hot_base) + hot_slot[static_cast<uint32_t>(R.root_ref)];
+  }
+
   uint64_t bytes = 0;
   auto upload = [&](auto& arr, const void* src, size_t count, size_t elem) -> hipError_t {
     hipError_t e = arr.ensure(count);
@@ -755,7 +813,8 @@ int upload_pool(pt_tracer* t) {
     if (count) e = hipMemcpy(arr.ptr, src, count * elem, hipMemcpyHostToDevice);
     return e;
   };
-  PT_HIP(t, upload(t->d_nodes, nodes.data(), n_nodes * 4u, sizeof(float4)), "upload BVH nodes");
+  PT_HIP(t, upload(t->d_nodes, nodes.data(), (n_nodes + hot_count) * 4u, sizeof(float4)), "upload BVH nodes");
+  PT_HIP(t, upload(t->d_hot_src, hot_src.data(), hot_count, sizeof(uint32_t)), "upload hot node table");
   PT_HIP(t, upload(t->d_tris, tris.data(), n_tris * 3u, sizeof(float4)), "upload triangles");
   PT_HIP(t, upload(t->d_prims, prims.data(), n_prims, sizeof(float4)), "upload primitives");
   PT_HIP(t, upload(t->d_materials, mats.data(), mats.size(), sizeof(PtMaterial)), "upload materials");
@@ -787,6 +846,8 @@ int upload_pool(pt_tracer* t) {
   ptd::DevScene& S = t->dscene;
   S = ptd::DevScene{};
   S.nodes = t->d_nodes.ptr;
+  S.hot_base = hot_base;
+  S.hot_count = hot_count;
   S.tris = t->d_tris.ptr;
   S.prims = t->d_prims.ptr;
   S.materials = t->d_materials.ptr;
@@ -2022,6 +2083,7 @@ int pt_hip_apply_instance_transform_update(pt_tracer* t, const PtInstanceTransfo
     ptd::launch_refit_nodes(t->stream, t->d_nodes.ptr, t->d_refit_order.ptr, t->refit_level[h],
                             t->refit_level[h + 1] - t->refit_level[h], t->d_tri_box.ptr, t->d_refit_flag.ptr);
   ptd::launch_refit_root(t->stream, t->d_nodes.ptr, s.layout.root_ref, t->d_tri_box.ptr, t->d_recs.ptr);
+  ptd::launch_hot_refresh(t->stream, t->d_nodes.ptr, t->d_hot_src.ptr, t->dscene.hot_base, t->dscene.hot_count);
   PT_HIP(t, hipGetLastError(), "launch refit");
   uint32_t nonfinite = 0;
   PT_HIP(t, hipMemcpyAsync(&nonfinite, t->d_refit_flag.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost, t->stream),

@@ -12,9 +12,15 @@ namespace ptd {
 constexpr int kBlock = 256;        // threads per workgroup (4 waves of 64)
 constexpr int kWavesPerBlock = kBlock / 64;
 #ifndef PT_STACK_LDS
-#define PT_STACK_LDS 10
+#define PT_STACK_LDS 8
 #endif
 constexpr int kStackLds = PT_STACK_LDS;  // newest traversal stack entries per lane held in LDS (a ring)
+#ifndef PT_HOT_NODES
+#define PT_HOT_NODES 64
+#endif
+// Inner nodes a traversal workgroup keeps in LDS: the pool's hot copies (DevScene::hot_base). With
+// the stack ring this is 20 KiB per workgroup, seven workgroups per CU.
+constexpr int kHotNodes = PT_HOT_NODES;
 constexpr int kCounterSlots = 16;  // device u64 counters (see CounterSlot)
 
 // Every same-address device atomic is serialized (about 11 ns each on MI355X), so counters and
@@ -131,6 +137,10 @@ struct DevScene {
   const DevSdf* sdfs;      // SDF primitives of all scenes (SceneRec::sdf_base / sdf_count)
   uint32_t has_sdf;        // some scene has SDF primitives: extension rays take k_sdf_extend, shadow
                            // rays the closest-hit loop with SDF tests
+  // Copies of the most visited inner nodes at node indices [hot_base, hot_base + hot_count) (after
+  // the pool's own nodes; hot_count <= kHotNodes): a hot copy's children that are hot are hot
+  // references, and a scene whose root is hot starts there. Traversal kernels read them from LDS.
+  uint32_t hot_base, hot_count;
 };
 
 // Integrator settings shared by every scene of a launch (image sizes are per scene: SceneRec).
@@ -302,6 +312,9 @@ void launch_refit_nodes(hipStream_t stream, float4* nodes, const uint32_t* order
 // The scene record's root box from the refit root (inner node or leaf).
 void launch_refit_root(hipStream_t stream, const float4* nodes, int32_t root_ref, const float* tri_box,
                        SceneRec* rec);
+// Copies the refit boxes of the hot nodes' originals (src[h], h < count) into their hot copies at
+// node indices hot_base + h (child references stay).
+void launch_hot_refresh(hipStream_t stream, float4* nodes, const uint32_t* src, uint32_t hot_base, uint32_t count);
 // resolve_hdr (rs == nullptr: 3 floats per pixel into out) or resolve_ldr (rgba8 per pixel into out).
 void launch_resolve(hipStream_t stream, const DevFilm& film, uint32_t pixels, const DevResolve* rs, void* out);
 
@@ -316,7 +329,8 @@ Grids resident_grids(int device);
 // GPU. noalloc_setup (once per device, with that device current) looks its address up; noalloc_scratch
 // returns the pieces (all null before a successful setup on that device).
 constexpr uint32_t kNoAllocBlocks = 512;  // largest persistent grid the spill area is sized for
-constexpr uint32_t kNoAllocSpill = 24;    // spilled stack entries per lane: trees of up to 24 + kStackLds - 2 levels
+// spilled stack entries per lane: a tree of PT_NOALLOC_MAX_TREE_DEPTH levels needs that + 2 entries
+constexpr uint32_t kNoAllocSpill = PT_NOALLOC_MAX_TREE_DEPTH + 2u - static_cast<uint32_t>(kStackLds);
 struct NoAllocScratch {
   unsigned long long* counters;  // kCounterWords
   uint32_t* queue;               // kQueueWords

@@ -150,15 +150,21 @@ __device__ __forceinline__ void slab2_fast(const RayQ& q, const float4 a, const 
   }
 }
 
+// Per-lane counts (visits, tests, hits) and per-wave counts (inner, leaves: the lanes that took an
+// inner-node or a leaf step, added once per step from the step's lane count).
 struct TravStats {
-  uint32_t visits, leaves, tests, hits, inner;
+  uint32_t visits, tests, hits;
+  uint32_t inner, leaves;
 };
 
 // Moller-Trumbore against precomputed edges, intersect_scalar (BvhTriangleIntersector.cpp:35-74).
 // True when eps < t < best_t.
+// The record is v0 + primitive id, e1, e2 in three 16-byte slots; the unused fourth floats of e1 and e2
+// are not read (40 bytes per test).
 __device__ __forceinline__ bool tri_test(const float4* tp, V3 o, V3 d, float best_t, float& t_out, float& u_out,
                                          float& v_out) {
-  const float4 v0 = tp[0], e1 = tp[1], e2 = tp[2];
+  const float4 v0 = tp[0];
+  const float3 e1 = *reinterpret_cast<const float3*>(tp + 1), e2 = *reinterpret_cast<const float3*>(tp + 2);
   const float hx = d.y * e2.z - d.z * e2.y;
   const float hy = d.z * e2.x - d.x * e2.z;
   const float hz = d.x * e2.y - d.y * e2.x;
@@ -194,6 +200,8 @@ struct Stack {
   PT_GLOBAL float* ovf_t;
   uint32_t ovf_depth, ovf_stride, lane_slot, gid;
   uint32_t* error;
+  const PT_LDS float* hot;  // the workgroup's LDS copy of the hot nodes (node indices hot_base + h), 16 floats each
+  uint32_t hot_base, hot_count;
 };
 
 // One BVH query in flight on a lane.
@@ -233,12 +241,27 @@ __device__ __forceinline__ bool trav_begin(const SceneRec& R, Trav& s, V3 o, V3 
 enum StepResult : int { kStepGo = 0, kStepPop = 1, kStepDone = 2 };
 
 // Inner node s.cur: descend into the nearer hit child (pushing the other when both are hit), or
-// ask for a pop when neither is hit.
+// ask for a pop when neither is hit. A hot copy's record comes from the workgroup's LDS copy.
 __device__ __forceinline__ int trav_inner(const DevScene& sc, const Stack& stk, Trav& s, TravStats& st) {
-  ++st.inner;
-  const float4* np = sc.nodes + 4u * static_cast<uint32_t>(s.cur);
-  const float4 a = np[0], b = np[1], c = np[2];
-  const int4 r = *reinterpret_cast<const int4*>(np + 3);
+  float4 a, b, c;
+  int32_t rx, ry;
+  const uint32_t h = static_cast<uint32_t>(s.cur) - stk.hot_base;
+  if (h < stk.hot_count) {
+    const PT_LDS float* hp = stk.hot + 16u * h;
+    a = make_float4(hp[0], hp[1], hp[2], hp[3]);
+    b = make_float4(hp[4], hp[5], hp[6], hp[7]);
+    c = make_float4(hp[8], hp[9], hp[10], hp[11]);
+    rx = __float_as_int(hp[12]);
+    ry = __float_as_int(hp[13]);
+  } else {
+    const float4* np = sc.nodes + 4u * static_cast<uint32_t>(s.cur);
+    a = np[0];
+    b = np[1];
+    c = np[2];
+    const int2 r = *reinterpret_cast<const int2*>(np + 3);  // the two child references (8 of the slot's 16 bytes)
+    rx = r.x;
+    ry = r.y;
+  }
   float t0, t1;
   bool h0, h1;
   if (!s.q.general && !sc.general_boxes) {
@@ -249,8 +272,8 @@ __device__ __forceinline__ int trav_inner(const DevScene& sc, const Stack& stk, 
   }
   if (h0 && h1) {
     const bool left_first = t0 <= t1;
-    const int32_t near_ref = left_first ? r.x : r.y;
-    const int32_t far_ref = left_first ? r.y : r.x;
+    const int32_t near_ref = left_first ? rx : ry;
+    const int32_t far_ref = left_first ? ry : rx;
     const float far_t = left_first ? t1 : t0;
     const uint32_t slot = s.top * kBlock + stk.lane_slot;
     if (s.sp - s.lo == static_cast<uint32_t>(kStackLds)) {  // ring full: spill its oldest entry (slot top)
@@ -272,7 +295,7 @@ __device__ __forceinline__ int trav_inner(const DevScene& sc, const Stack& stk, 
     return kStepGo;
   }
   if (h0 || h1) {
-    s.cur = h0 ? r.x : r.y;
+    s.cur = h0 ? rx : ry;
     ++st.visits;
     return kStepGo;
   }
@@ -283,7 +306,6 @@ __device__ __forceinline__ int trav_inner(const DevScene& sc, const Stack& stk, 
 // query accepted a hit, else kStepPop.
 template <bool kAny>
 __device__ __forceinline__ int trav_leaf(const DevScene& sc, Trav& s, TravStats& st) {
-  ++st.leaves;
   const uint32_t code = ~static_cast<uint32_t>(s.cur);
   const uint32_t first = code >> 5, count = code & 31u;
   const V3 o = mk(s.q.o[0], s.q.o[1], s.q.o[2]);
@@ -355,6 +377,12 @@ __device__ __forceinline__ uint32_t global_wave() {
 // Ballot of a lane predicate straight from its compare mask (no select to a VGPR and back).
 __device__ __forceinline__ uint64_t wave_ballot(bool pred) { return __builtin_amdgcn_ballot_w64(pred); }
 
+// Lanes whose predicate holds, as a 32-bit scalar (compares of two counts then stay scalar instead of
+// being widened to a 64-bit vector compare).
+__device__ __forceinline__ uint32_t wave_count(bool pred) {
+  return __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(__popcll(wave_ballot(pred))));
+}
+
 // The first active lane takes the next chunk index from the shared cursor; every lane gets it.
 // Called with the wave converged.
 __device__ __forceinline__ uint32_t wave_grab(uint32_t* cursor) {
@@ -413,21 +441,40 @@ __device__ __forceinline__ void drive_queries(const DevScene& sc, const Stack& s
     }
     // One step per iteration for the larger of the two groups of lanes, those at inner nodes or
     // those at leaves; the others wait, so each step runs with most of its lanes busy.
-    const bool at_inner = active && s.cur >= 0;
-    const uint32_t n_inner = static_cast<uint32_t>(__popcll(wave_ballot(at_inner)));
-    const uint32_t n_leaf = static_cast<uint32_t>(__popcll(wave_ballot(active && !at_inner)));
+    bool at_inner = active && s.cur >= 0;
+    uint32_t n_inner = wave_count(at_inner);
+    const uint32_t n_leaf = wave_count(active && !at_inner);
     if (n_inner + n_leaf == 0u) {
       if (chunk * 64u >= n) break;
       continue;
     }
-    int r = kStepGo;
-    if (n_inner >= n_leaf) {
-      if (at_inner) r = trav_inner(sc, stk, s, st);
-    } else if (active && !at_inner) {
-      r = trav_leaf<kAny>(sc, s, st);
+    if (static_cast<int32_t>(n_inner - n_leaf) >= 0) {  // n_inner >= n_leaf (both <= 64)
+      // Inner steps back to back for as long as the schedule above would choose them again: no
+      // refill due (fewer idle lanes than refill_min, or no queries left to hand out) and inner
+      // lanes at least as many as leaf lanes.
+      const bool can_refill = chunk * 64u < n;
+      while (true) {
+        st.inner += n_inner;
+        if (at_inner) {
+          int r = trav_inner(sc, stk, s, st);
+          if (r == kStepPop) r = trav_pop(stk, s, st);
+          if (r == kStepDone) active = end(j, s);
+        }
+        at_inner = active && s.cur >= 0;
+        n_inner = wave_count(at_inner);
+        const uint32_t n_left = wave_count(active && !at_inner);
+        if (n_inner == 0u || static_cast<int32_t>(n_inner - n_left) < 0 ||
+            (can_refill && 64u - n_inner - n_left >= refill_min))
+          break;
+      }
+      continue;
     }
-    if (r == kStepPop) r = trav_pop(stk, s, st);
-    if (r == kStepDone) active = end(j, s);
+    st.leaves += n_leaf;
+    if (active && !at_inner) {
+      int r = trav_leaf<kAny>(sc, s, st);
+      if (r == kStepPop) r = trav_pop(stk, s, st);
+      if (r == kStepDone) active = end(j, s);
+    }
   }
   wave_finish(queue, waves);
 }
@@ -439,6 +486,12 @@ __device__ __forceinline__ void wave_add(unsigned long long* counters, int slot,
     atomicAdd(counters + (global_wave() & (kStripes - 1u)) * kCounterSlots + slot, static_cast<unsigned long long>(v));
 }
 
+// Adds a value every lane of the wave holds (the same on all of them) once.
+__device__ __forceinline__ void wave_add_uniform(unsigned long long* counters, int slot, uint32_t v) {
+  if ((threadIdx.x & 63u) == 0u && v != 0u)
+    atomicAdd(counters + (global_wave() & (kStripes - 1u)) * kCounterSlots + slot, static_cast<unsigned long long>(v));
+}
+
 // Shadow traversals add to the same SampleCounters fields as extension rays, as the
 // reference's shadow query does (ScalarCpuPathTracer.cpp:1071); the shadow-only slots
 // keep their share for the byte accounting.
@@ -446,14 +499,14 @@ __device__ __forceinline__ void flush_stats(unsigned long long* counters, const 
   if (shadow) {
     wave_add(counters, kShadowNodeVisits, s.visits);
     wave_add(counters, kShadowTriTests, s.tests);
-    wave_add(counters, kShadowInnerFetches, s.inner);
-    wave_add(counters, kShadowLeafVisits, s.leaves);
+    wave_add_uniform(counters, kShadowInnerFetches, s.inner);
+    wave_add_uniform(counters, kShadowLeafVisits, s.leaves);
   }
   wave_add(counters, kNodeVisits, s.visits);
-  wave_add(counters, kLeafVisits, s.leaves);
+  wave_add_uniform(counters, kLeafVisits, s.leaves);
   wave_add(counters, kTriTests, s.tests);
   wave_add(counters, kTriHits, s.hits);
-  wave_add(counters, kInnerFetches, s.inner);
+  wave_add_uniform(counters, kInnerFetches, s.inner);
 }
 
 // Wave-wide append: returns this lane's slot (valid when pred is true).
@@ -662,6 +715,14 @@ __global__ __launch_bounds__(kBlock) void k_refit_nodes(float4* nodes, const uin
   bool fin = true;
   for (int k = 0; k < 3; ++k) fin = fin && finite(a0[k]) && finite(b0[k]) && finite(a1[k]) && finite(b1[k]);
   if (!fin) atomicOr(nonfinite, 1u);
+}
+
+__global__ __launch_bounds__(kBlock) void k_hot_refresh(float4* nodes, const uint32_t* src, uint32_t hot_base,
+                                                        uint32_t count) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= 3u * count) return;
+  const uint32_t h = i / 3u, k = i % 3u;
+  nodes[4u * (hot_base + h) + k] = nodes[4u * src[h] + k];
 }
 
 __global__ void k_refit_root(const float4* nodes, int32_t root_ref, const float* tri_box, SceneRec* rec) {
@@ -945,8 +1006,18 @@ __global__ __launch_bounds__(kBlock) void k_grad_reduce(DevAdjoint A) {
   else if (A.g_rough) atomicAdd(A.g_rough + (i - 2u * na - nl), v);
 }
 
-__device__ __forceinline__ Stack make_stack(const DevPaths& P, int32_t* s_ref, float* s_t) {
+// Copies the hot nodes into the workgroup's LDS array (kHotNodes records); the caller's next barrier
+// (load_prefix's) publishes them.
+__device__ __forceinline__ void load_hot(const DevScene& sc, float4* s_hot) {
+  for (uint32_t i = threadIdx.x; i < 4u * sc.hot_count; i += kBlock) s_hot[i] = sc.nodes[4u * sc.hot_base + i];
+}
+
+__device__ __forceinline__ Stack make_stack(const DevPaths& P, const DevScene& sc, int32_t* s_ref, float* s_t,
+                                            const float4* s_hot) {
   Stack s;
+  s.hot = (const PT_LDS float*)s_hot;
+  s.hot_base = sc.hot_base;
+  s.hot_count = sc.hot_count;
   s.lds_ref = (PT_LDS int32_t*)s_ref;
   s.lds_t = (PT_LDS float*)s_t;
   s.ovf_ref = (PT_GLOBAL int32_t*)P.stack_ref;
@@ -1129,9 +1200,11 @@ __global__ __launch_bounds__(kBlock) void k_sdf_extend(DevPaths P, DevRays in, D
 __global__ __launch_bounds__(kBlock) void k_extend(DevPaths P, DevRays in, DevScene sc) {
   __shared__ int32_t s_ref[kStackLds * kBlock];
   __shared__ float s_t[kStackLds * kBlock];
+  __shared__ float4 s_hot[4 * (kHotNodes > 0 ? kHotNodes : 1)];
+  load_hot(sc, s_hot);
   __shared__ uint32_t prefix[kSegments + 1];
   load_prefix(P.queue + kQueueInCount, 1, prefix);
-  const Stack stk = make_stack(P, s_ref, s_t);
+  const Stack stk = make_stack(P, sc, s_ref, s_t, s_hot);
   const uint32_t n = prefix[kSegments];
   TravStats st = {0, 0, 0, 0, 0};
   uint32_t rays = 0;
@@ -2071,6 +2144,8 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevPaths P, DevScene sc, DevA
   static_assert(!(kAny && kSdf), "SDF shadows need the closest-hit skip loop");
   __shared__ int32_t s_ref[kStackLds * kBlock];
   __shared__ float s_t[kStackLds * kBlock];
+  __shared__ float4 s_hot[4 * (kHotNodes > 0 ? kHotNodes : 1)];
+  load_hot(sc, s_hot);
   __shared__ uint32_t prefix[kSegments + 1];
   load_prefix(P.queue + kQueueShadowCursor, kSegLine, prefix);
   // forward: a visible light adds its contribution now; adjoint: the record is marked and
@@ -2079,7 +2154,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevPaths P, DevScene sc, DevA
     if constexpr (kAdj) A.s_light[q] |= kShadowVisible;
     else light_visible<false>(P, sc, A, nullptr, q);
   };
-  const Stack stk = make_stack(P, s_ref, s_t);
+  const Stack stk = make_stack(P, sc, s_ref, s_t, s_hot);
   const uint32_t n = prefix[kSegments];
   TravStats st = {0, 0, 0, 0, 0};
   uint32_t q = 0;
@@ -2198,10 +2273,10 @@ __global__ __launch_bounds__(kBlock) void k_shadow_grad(DevPaths P, DevScene sc,
 // Film accumulation, one wave (one 64-thread block) per 64 pixels of a job, each lane one pixel: the
 // pixel's samples are added in tile order (the reference's sample order), so the sums are those of
 // adding one sample per render_tile call. A job that is one interleaved group keeps a pixel's
-// samples in consecutive path slots; the wave stages them through LDS in chunks of 32 samples with
-// coalesced loads (lanes 0-31 and 32-63 read 32 samples of two pixels per instruction), then each
-// lane sums its pixel's chunk in order.
-constexpr uint32_t kFilmChunk = 32;
+// samples in consecutive path slots; the wave stages them through LDS in chunks of kFilmChunk samples
+// with coalesced loads (each group of kFilmChunk lanes reads one pixel's chunk, 64 / kFilmChunk pixels
+// per instruction), then each lane sums its pixel's chunk in order.
+constexpr uint32_t kFilmChunk = 16;  // 13 KiB of LDS per wave: twelve waves per CU
 __global__ __launch_bounds__(64) void k_film(DevPaths P, DevScene sc, DevFilm film, const DevTile* tiles,
                                             const FilmJob* jobs, uint32_t job_count, uint32_t job_waves) {
   __shared__ float stage[3][64 * (kFilmChunk + 1)];
@@ -2224,21 +2299,25 @@ __global__ __launch_bounds__(64) void k_film(DevPaths P, DevScene sc, DevFilm fi
   float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   uint32_t cnt = 0, added = 0, dropped_rays = 0;
   bool touched = false;
+  // Both tallies are updated on every sample (a select, not a branch between the two variables, which
+  // the compiler would otherwise keep in a scratch array).
   auto add = [&](float r, float g, float b, uint32_t slot) {
-    if (!(finite(r) && finite(g) && finite(b))) {  // shade_pixel drops it, rays included (Render.cpp:270-281)
-      dropped_rays += __float_as_uint(P.tp[0][slot]);
-      return;
+    const bool ok = finite(r) && finite(g) && finite(b);  // shade_pixel drops it, rays included (Render.cpp:270-281)
+    uint32_t drop = 0;
+    if (!ok) drop = __float_as_uint(P.tp[0][slot]);
+    dropped_rays += drop;
+    added += ok ? 1u : 0u;
+    if (ok) {
+      if (!touched) {
+        s = film.sum[px];
+        cnt = film.count[px];
+        touched = true;
+      }
+      s.x = s.x + r;
+      s.y = s.y + g;
+      s.z = s.z + b;
+      ++cnt;
     }
-    if (!touched) {
-      s = film.sum[px];
-      cnt = film.count[px];
-      touched = true;
-    }
-    s.x = s.x + r;
-    s.y = s.y + g;
-    s.z = s.z + b;
-    ++cnt;
-    ++added;
   };
   if (J.group > 1u) {
     const uint32_t G = J.group, base = tiles[J.first_tile].path_begin;
@@ -2246,8 +2325,8 @@ __global__ __launch_bounds__(64) void k_film(DevPaths P, DevScene sc, DevFilm fi
     for (uint32_t kc = 0; kc < G; kc += kFilmChunk) {
       const uint32_t kn = G - kc < kFilmChunk ? G - kc : kFilmChunk;
       const uint32_t k = lane & (kFilmChunk - 1u);
-      for (uint32_t i = 0; i < 64u; i += 2u) {
-        const uint32_t j = i + (lane >> 5);
+      for (uint32_t i = 0; i < 64u; i += 64u / kFilmChunk) {  // 64 / kFilmChunk pixels per load
+        const uint32_t j = i + lane / kFilmChunk;
         if (j < npx && k < kn) {
           const uint32_t slot = base + (first + j) * G + kc + k;
           stage[0][j * (kFilmChunk + 1u) + k] = P.rad[0][slot];
@@ -2322,7 +2401,10 @@ __global__ __launch_bounds__(kBlock) void k_trace_rays(DevPaths P, DevScene sc, 
                                                       uint32_t n, Out out) {
   __shared__ int32_t s_ref[kStackLds * kBlock];
   __shared__ float s_t[kStackLds * kBlock];
-  const Stack stk = make_stack(P, s_ref, s_t);
+  __shared__ float4 s_hot[4 * (kHotNodes > 0 ? kHotNodes : 1)];
+  load_hot(sc, s_hot);
+  __syncthreads();
+  const Stack stk = make_stack(P, sc, s_ref, s_t, s_hot);
   TravStats st = {0, 0, 0, 0, 0};
   auto begin = [&](uint32_t j, Trav& s) {
     const float* r = rays + 6u * static_cast<size_t>(j);
@@ -2373,6 +2455,12 @@ void launch_refit_nodes(hipStream_t stream, float4* nodes, const uint32_t* order
 void launch_refit_root(hipStream_t stream, const float4* nodes, int32_t root_ref, const float* tri_box,
                        SceneRec* rec) {
   hipLaunchKernelGGL(k_refit_root, dim3(1), dim3(64), 0, stream, nodes, root_ref, tri_box, rec);
+}
+
+void launch_hot_refresh(hipStream_t stream, float4* nodes, const uint32_t* src, uint32_t hot_base, uint32_t count) {
+  if (count == 0) return;
+  hipLaunchKernelGGL(k_hot_refresh, dim3((3u * count + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, nodes, src,
+                     hot_base, count);
 }
 
 void launch_resolve(hipStream_t stream, const DevFilm& film, uint32_t pixels, const DevResolve* rs, void* out) {
@@ -2511,7 +2599,9 @@ constexpr size_t kNoAllocLanes = static_cast<size_t>(kNoAllocBlocks) * kBlock;
 constexpr size_t kNoAllocSpillEntries = static_cast<size_t>(kNoAllocSpill) * kNoAllocLanes;
 constexpr size_t kNoAllocBytes = kNoAllocSpillOff + 8u * kNoAllocSpillEntries;
 static_assert(kNoAllocErrorOff + 256u <= kNoAllocSpillOff, "scratch header overlaps the spill area");
-static_assert(kNoAllocBytes == 32768u + 24u * 1024u * 1024u, "the documented scratch size is 24 MiB + 32 KiB");
+#if PT_STACK_LDS == 8  // the default ring
+static_assert(kNoAllocBytes == 32768u + 26u * 1024u * 1024u, "the documented scratch size is 26 MiB + 32 KiB");
+#endif
 
 static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 
