@@ -47,18 +47,18 @@ BYTES = {
 # (calibrated against a plain device copy: FETCH_SIZE x 2, WRITE_SIZE x 1), from
 # profiles/wavefront_procedural_1080p_64spp.md. They describe the current kernels; re-measure after
 # a kernel change.
-# VALU issue of the same kernels from the same profile: VALU busy = SQ_INSTS_VALU x 4 cycles over the
-# SIMD cycles of the launch (gfx950 peak: one wave64 VALU instruction per SIMD every 4 cycles), lane
-# utilisation = SQ_THREAD_CYCLES_VALU / (64 x SQ_ACTIVE_INST_VALU), L2 hit = TCC_HIT / (TCC_HIT + TCC_MISS).
-MEASURED_VALU = {"extend": {"valu_busy": 0.99, "lane_utilisation": 0.54, "l2_hit": 0.791},
-                 "shadow": {"valu_busy": 0.89, "lane_utilisation": 0.46, "l2_hit": 0.817},
-                 "shade": {"valu_busy": 0.61, "lane_utilisation": 0.33, "l2_hit": 0.468}}
-MEASURED_TRAFFIC = {"extend": 20.585e9, "shadow": 10.763e9, "shade": 8.163e9, "film": 1.677e9, "raygen": 8.494e9}
+# VALU issue of the same kernels from the same profile: VALU busy = SQ_ACTIVE_INST_VALU / (256 CUs x the
+# launch's cycles on one XCD; rocprof's VALUBusy), lane utilisation = SQ_THREAD_CYCLES_VALU /
+# (64 x SQ_ACTIVE_INST_VALU), L2 hit = TCC_HIT / (TCC_HIT + TCC_MISS).
+MEASURED_VALU = {"extend": {"valu_busy": 0.85, "lane_utilisation": 0.47, "l2_hit": 0.78},
+                 "shadow": {"valu_busy": 0.79, "lane_utilisation": 0.4, "l2_hit": 0.806},
+                 "shade": {"valu_busy": 0.63, "lane_utilisation": 0.33, "l2_hit": 0.475}}
+MEASURED_TRAFFIC = {"extend": 20.56e9, "shadow": 10.94e9, "shade": 8.243e9, "film": 1.677e9, "raygen": 8.494e9}
 # The two tables above describe the kernels built from sources whose sha256 (over KERNEL_SOURCES, in
 # order) is PROFILED_KERNEL_SHA256; when the sources differ they are dropped from the line.
 KERNEL_SOURCES = ("vkpathtracer_amd/csrc/pt_kernels.hip", "vkpathtracer_amd/csrc/pt_device.h",
                   "vkpathtracer_amd/csrc/pt_math.h")
-PROFILED_KERNEL_SHA256 = None
+PROFILED_KERNEL_SHA256 = "8ce77ac15445e061c3ada884e86c063eb474d8f10c53e40bd83e2a795ce8d134"
 
 
 def kernel_source_sha256() -> str:

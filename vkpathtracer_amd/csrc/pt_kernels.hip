@@ -2274,12 +2274,14 @@ __global__ __launch_bounds__(kBlock) void k_shadow_grad(DevPaths P, DevScene sc,
 // pixel's samples are added in tile order (the reference's sample order), so the sums are those of
 // adding one sample per render_tile call. A job that is one interleaved group keeps a pixel's
 // samples in consecutive path slots; the wave stages them through LDS in chunks of kFilmChunk samples
-// with coalesced loads (each group of kFilmChunk lanes reads one pixel's chunk, 64 / kFilmChunk pixels
-// per instruction), then each lane sums its pixel's chunk in order.
-constexpr uint32_t kFilmChunk = 16;  // 13 KiB of LDS per wave: twelve waves per CU
+// for half its pixels at a time, with coalesced loads (lanes 0-31 and 32-63 each read one pixel's
+// 128-byte chunk per instruction), then those pixels' lanes sum their chunks in order. Half the
+// pixels per stage keeps the LDS at 12.7 KiB per wave (twelve waves per CU) with whole-line loads.
+constexpr uint32_t kFilmChunk = 32;
+constexpr uint32_t kFilmHalf = 32;  // pixels staged at a time
 __global__ __launch_bounds__(64) void k_film(DevPaths P, DevScene sc, DevFilm film, const DevTile* tiles,
                                             const FilmJob* jobs, uint32_t job_count, uint32_t job_waves) {
-  __shared__ float stage[3][64 * (kFilmChunk + 1)];
+  __shared__ float stage[3][kFilmHalf * (kFilmChunk + 1)];
   const uint32_t w = blockIdx.x;
   if (w >= job_waves) return;
   uint32_t lo = 0, hi = job_count;  // last job with wave_begin <= w
@@ -2325,22 +2327,24 @@ __global__ __launch_bounds__(64) void k_film(DevPaths P, DevScene sc, DevFilm fi
     for (uint32_t kc = 0; kc < G; kc += kFilmChunk) {
       const uint32_t kn = G - kc < kFilmChunk ? G - kc : kFilmChunk;
       const uint32_t k = lane & (kFilmChunk - 1u);
-      for (uint32_t i = 0; i < 64u; i += 64u / kFilmChunk) {  // 64 / kFilmChunk pixels per load
-        const uint32_t j = i + lane / kFilmChunk;
-        if (j < npx && k < kn) {
-          const uint32_t slot = base + (first + j) * G + kc + k;
-          stage[0][j * (kFilmChunk + 1u) + k] = P.rad[0][slot];
-          stage[1][j * (kFilmChunk + 1u) + k] = P.rad[1][slot];
-          stage[2][j * (kFilmChunk + 1u) + k] = P.rad[2][slot];
+      for (uint32_t h = 0; h < 64u; h += kFilmHalf) {  // pixels [h, h + kFilmHalf) of the wave
+        for (uint32_t i = 0; i < kFilmHalf; i += 64u / kFilmChunk) {  // 64 / kFilmChunk pixels per load
+          const uint32_t j = i + lane / kFilmChunk;
+          if (h + j < npx && k < kn) {
+            const uint32_t slot = base + (first + h + j) * G + kc + k;
+            stage[0][j * (kFilmChunk + 1u) + k] = P.rad[0][slot];
+            stage[1][j * (kFilmChunk + 1u) + k] = P.rad[1][slot];
+            stage[2][j * (kFilmChunk + 1u) + k] = P.rad[2][slot];
+          }
         }
+        __syncthreads();
+        if (valid && lane - h < kFilmHalf)
+          for (uint32_t q = 0; q < kn; ++q) {
+            const uint32_t o = (lane - h) * (kFilmChunk + 1u) + q;
+            add(stage[0][o], stage[1][o], stage[2][o], base + local * G + kc + q);
+          }
+        __syncthreads();
       }
-      __syncthreads();
-      if (valid)
-        for (uint32_t q = 0; q < kn; ++q) {
-          const uint32_t o = lane * (kFilmChunk + 1u) + q;
-          add(stage[0][o], stage[1][o], stage[2][o], base + local * G + kc + q);
-        }
-      __syncthreads();
     }
   } else if (valid) {
     for (uint32_t k = 0; k < J.tile_count; ++k) {
