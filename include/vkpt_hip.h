@@ -203,21 +203,26 @@ typedef struct PtRayQueryHit {
 
 /* BVH build parameters. builder 0 = bucketed SAH (BvhBuildConfig,
  * src/gpu/D3D12GpuPathTracer.SceneBvh.h:15-19; defaults leaf 4, 8 buckets),
- * builder 1 = midpoint split (ParallelBvhBuilder, src/cpu/ParallelBvhBuilder.cpp:61-163). */
+ * builder 1 = midpoint split (ParallelBvhBuilder, src/cpu/ParallelBvhBuilder.cpp:61-163).
+ * The SAH tree is built on the GPU (the host builder's tree, node for node) unless flags has
+ * PT_BVH_FLAG_HOST_BUILD or a vertex is not finite; the midpoint tree is built on the host. */
 typedef struct PtBvhParams {
   uint32_t builder;
   uint32_t leaf_size;
   uint32_t bucket_count;
-  uint32_t reserved;
+  uint32_t flags;
 } PtBvhParams;
 
 enum { PT_BVH_SAH = 0, PT_BVH_MIDPOINT = 1 };
+enum { PT_BVH_FLAG_HOST_BUILD = 1 };
 
-/* RayAcceleratorBuildInfo, PathTracer.h:992-999, plus depth and device bytes. */
+/* RayAcceleratorBuildInfo, PathTracer.h:992-999, plus depth, device bytes and whether every tree of
+ * the build was built on the GPU (device_built; a tree built on the host clears it). */
 typedef struct PtBuildInfo {
   double build_ms;
   uint64_t device_bytes;
   uint32_t built, primitive_count, node_count, leaf_count, max_depth, deterministic;
+  uint32_t device_built, pad0;
 } PtBuildInfo;
 
 /* PathTracerStatus subset (PathTracer.h:854-865). */
@@ -522,12 +527,21 @@ int pt_bvh_build_host(const PtSceneView* scene, const PtBvhParams* params, uint3
                       uint32_t* prim_count, uint32_t* max_depth, float* bounds, int32_t* meta,
                       uint32_t* prim_order, float* prim_geo);
 
+/* The same export for the tree pt_hip_build_accel builds: the SAH tree built on the current HIP device
+ * (built_on_device, may be NULL, is 1; 0 when a non-finite vertex sent the build to the host, or for
+ * the midpoint builder). The outputs equal pt_bvh_build_host's for the same scene and parameters.
+ * No counterpart in the reference, whose builders run on the CPU (D3D12GpuPathTracer.SceneBvh.cpp:73-238
+ * for the bucketed SAH). */
+int pt_bvh_build_device(const PtSceneView* scene, const PtBvhParams* params, uint32_t* node_count,
+                        uint32_t* prim_count, uint32_t* max_depth, float* bounds, int32_t* meta,
+                        uint32_t* prim_order, float* prim_geo, uint32_t* built_on_device);
+
 /* ABI sizes, for bindings to check their mirrors of the records above. */
 uint32_t pt_hip_sizeof(int which); /* 0 material, 1 light, 2 instance, 3 camera, 4 scene, 5 settings, 6 counters,
                                       7 tile, 8 tuning, 9 texture, 10 film resolve,
                                       11 instance transform update, 12 SDF primitive, 13 ray query hit,
                                       14 scene delta update, 15 material update, 16 light update,
-                                      17 instance material update */
+                                      17 instance material update, 18 build info */
 const char* pt_hip_version(void);
 
 #ifdef __cplusplus

@@ -459,7 +459,11 @@ static uint32_t sah_build(sah_ctx* c, uint32_t start, uint32_t count, uint32_t d
       for (int k = 0; k < 3; ++k) { bmn_b[b][k] = RC(1e30); bmx_b[b][k] = RC(-1e30); }
     }
     for (uint32_t i = start; i < start + count; ++i) {
-      int b = (int)((c->refs[i].c[axis] - cmin) / extent * (real)B);
+      /* the reference's unchecked int conversion is undefined for a NaN or infinite value (a
+       * non-finite vertex): those primitives take bucket 0, or the last for +inf, as the product does */
+      real x = (c->refs[i].c[axis] - cmin) / extent * (real)B;
+      if (!(x >= RC(0.0))) x = RC(0.0);
+      int b = x < (real)B ? (int)x : B - 1;
       if (B - 1 < b) b = B - 1;
       bn[b]++;
       for (int k = 0; k < 3; ++k) {

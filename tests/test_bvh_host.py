@@ -137,3 +137,24 @@ def test_batch_scene_generator_is_seeded():
         assert (wa, ha, pa) == (wb, hb, pb) and wa == ha and wa in (128, 256, 512, 1024) and pa in (4, 16, 64)
         assert np.array_equal(sa.vertices, sb.vertices) and sa.triangle_count == sb.triangle_count
         assert 500 <= sa.triangle_count <= 20_000
+
+
+def test_non_finite_vertex_builds_and_matches_oracle():
+    """A NaN or infinite vertex puts a NaN or infinite centroid into the bucket index, whose int
+    conversion the reference leaves undefined; the product and the oracle send such primitives to
+    bucket 0 (the last for +inf) and build the same tree instead of writing out of bounds."""
+    rng = np.random.default_rng(5)
+    t = rng.uniform(-1, 1, (100, 3, 3)).astype(np.float32)
+    t[17, 1, 2] = np.inf
+    t[40, 2, 0] = -np.inf
+    t[3, 0, 0] = np.nan
+    inst = np.zeros(1, S.INSTANCE_DTYPE)
+    inst["triangle_count"] = len(t)
+    inst["rotation"] = (0, 0, 0, 1)
+    inst["scale"] = 1
+    sc = S.Scene(t.reshape(-1, 3), np.arange(3 * len(t), dtype=np.uint32), inst, S.make_materials(1),
+                 np.zeros(0, S.LIGHT_DTYPE), S.Camera())
+    a = bvh.build_host(sc, "sah", 4, 8)
+    o = O.Oracle(sc, builder="sah", leaf_size=4, bucket_count=8).export()
+    assert np.array_equal(a["meta"], o["meta"]) and np.array_equal(a["order"], o["order"])
+    assert sorted(a["order"].tolist()) == list(range(len(t)))

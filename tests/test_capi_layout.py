@@ -42,6 +42,7 @@ def test_record_sizes_match_everywhere():
     assert lib.pt_hip_sizeof(11) == C.sizeof(abi.PtInstanceTransformUpdate) == 64
     assert C.sizeof(abi.PtInstanceTransformResult) == 24
     assert lib.pt_hip_sizeof(13) == abi.RAY_QUERY_HIT_DTYPE.itemsize == 40
+    assert lib.pt_hip_sizeof(18) == C.sizeof(abi.PtBuildInfo) == 48
     assert lib.pt_hip_sizeof(14) == C.sizeof(abi.PtSceneDeltaUpdate) == 56
     assert lib.pt_hip_sizeof(15) == abi.MATERIAL_UPDATE_DTYPE.itemsize == 88
     assert lib.pt_hip_sizeof(16) == abi.LIGHT_UPDATE_DTYPE.itemsize == 68

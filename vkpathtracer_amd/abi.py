@@ -59,13 +59,14 @@ class PtSampleCounters(C.Structure):
 
 class PtBvhParams(C.Structure):
     _fields_ = [("builder", C.c_uint32), ("leaf_size", C.c_uint32), ("bucket_count", C.c_uint32),
-                ("reserved", C.c_uint32)]
+                ("flags", C.c_uint32)]  # PT_BVH_FLAG_HOST_BUILD = 1
 
 
 class PtBuildInfo(C.Structure):
     _fields_ = [("build_ms", C.c_double), ("device_bytes", C.c_uint64), ("built", C.c_uint32),
                 ("primitive_count", C.c_uint32), ("node_count", C.c_uint32), ("leaf_count", C.c_uint32),
-                ("max_depth", C.c_uint32), ("deterministic", C.c_uint32)]
+                ("max_depth", C.c_uint32), ("deterministic", C.c_uint32), ("device_built", C.c_uint32),
+                ("pad0", C.c_uint32)]
 
     def as_dict(self) -> dict:
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -215,6 +216,7 @@ SIGNATURES = {
     "pt_hip_read_kernel_stats": (C.c_int, [_P, C.POINTER(PtKernelStats)]),
     "pt_hip_reset_kernel_stats": (C.c_int, [_P]),
     "pt_bvh_build_host": (C.c_int, [C.POINTER(PtSceneView), C.POINTER(PtBvhParams), _P, _P, _P, _P, _P, _P, _P]),
+    "pt_bvh_build_device": (C.c_int, [C.POINTER(PtSceneView), C.POINTER(PtBvhParams), _P, _P, _P, _P, _P, _P, _P, _P]),
     "pt_hip_update_scene_delta": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint32]),
     "pt_hip_render_adjoint": (C.c_int, [_P, C.POINTER(PtRenderTile), C.c_uint32, C.c_uint64, _P, _P, _P, _P, _P,
                                         _P]),

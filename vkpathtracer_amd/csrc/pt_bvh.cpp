@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <thread>
 
 namespace ptb {
 namespace {
@@ -49,6 +50,22 @@ V3 rotate_quat(V3 value, const float* q) {
 V3 transform_vertex(V3 local, const PtInstance& inst) {
   const V3 scaled{local.x * inst.scale[0], local.y * inst.scale[1], local.z * inst.scale[2]};
   return add(rotate_quat(scaled, inst.rotation), load3(inst.translation));
+}
+
+// fn(begin, end) over [0, n) split between up to `threads` threads (one below 64k elements).
+template <typename F>
+void parallel_for(size_t n, unsigned threads, F fn) {
+  if (threads <= 1u || n < 65536u) {
+    fn(size_t{0}, n);
+    return;
+  }
+  const size_t chunk = (n + threads - 1u) / threads;
+  std::vector<std::thread> pool;
+  for (unsigned t = 0; t < threads; ++t) {
+    const size_t a = t * chunk, b = std::min(n, a + chunk);
+    if (a < b) pool.emplace_back(fn, a, b);
+  }
+  for (std::thread& th : pool) th.join();
 }
 
 struct Ref {
@@ -116,7 +133,11 @@ struct SahBuilder {
       }
       const float fB = static_cast<float>(B);
       for (uint32_t i = start; i < start + count; ++i) {
-        int b = static_cast<int>((refs[i].c[axis] - cmin) / extent * fB);
+        // the reference converts this to int unchecked; a non-finite vertex makes it NaN or infinite,
+        // where the conversion is undefined: such primitives go to bucket 0 (or the last, for +inf)
+        float x = (refs[i].c[axis] - cmin) / extent * fB;
+        if (!(x >= 0.0f)) x = 0.0f;
+        int b = x < fB ? static_cast<int>(x) : B - 1;
         if (B - 1 < b) b = B - 1;
         bn[b]++;
         for (int k = 0; k < 3; ++k) {
@@ -355,7 +376,7 @@ void build_midpoint(Bvh& bvh, const std::vector<float>& raw, uint32_t leaf_size)
   bvh.nodes.resize(static_cast<size_t>(b.alloc));
 }
 
-void pack_device_layout(const Bvh& bvh, DeviceLayout& out, bool with_uv) {
+void pack_device_layout(const Bvh& bvh, DeviceLayout& out, bool with_uv, unsigned threads) {
   out.nodes.clear();
   out.tris.clear();
   out.prims.clear();
@@ -363,30 +384,38 @@ void pack_device_layout(const Bvh& bvh, DeviceLayout& out, bool with_uv) {
   out.prim_tri.clear();
   if (with_uv) {
     out.prim_uv.resize(6u * bvh.prims.size());
-    for (size_t i = 0; i < bvh.prims.size(); ++i) std::memcpy(&out.prim_uv[6u * i], bvh.prims[i].uv, 6 * sizeof(float));
+    parallel_for(bvh.prims.size(), threads, [&](size_t a, size_t b) {
+      for (size_t i = a; i < b; ++i) std::memcpy(&out.prim_uv[6u * i], bvh.prims[i].uv, 6 * sizeof(float));
+    });
     out.prim_tri.resize(bvh.prims.size());
-    for (size_t i = 0; i < bvh.order.size(); ++i) out.prim_tri[bvh.order[i]] = static_cast<uint32_t>(i);
+    parallel_for(bvh.order.size(), threads, [&](size_t a, size_t b) {
+      for (size_t i = a; i < b; ++i) out.prim_tri[bvh.order[i]] = static_cast<uint32_t>(i);
+    });
   }
   out.inner_count = 0;
   const size_t nprim = bvh.prims.size();
   out.prims.resize(4u * nprim);
-  for (size_t i = 0; i < nprim; ++i) {
-    const Prim& p = bvh.prims[i];
-    out.prims[4 * i + 0] = p.n[0];
-    out.prims[4 * i + 1] = p.n[1];
-    out.prims[4 * i + 2] = p.n[2];
-    std::memcpy(&out.prims[4 * i + 3], &p.material, 4);
-  }
+  parallel_for(nprim, threads, [&](size_t a, size_t b) {
+    for (size_t i = a; i < b; ++i) {
+      const Prim& p = bvh.prims[i];
+      out.prims[4 * i + 0] = p.n[0];
+      out.prims[4 * i + 1] = p.n[1];
+      out.prims[4 * i + 2] = p.n[2];
+      std::memcpy(&out.prims[4 * i + 3], &p.material, 4);
+    }
+  });
   out.tris.resize(12u * bvh.order.size());
-  for (size_t i = 0; i < bvh.order.size(); ++i) {
-    const uint32_t pid = bvh.order[i];
-    const Prim& p = bvh.prims[pid];
-    float* t = &out.tris[12 * i];
-    t[0] = p.v0[0]; t[1] = p.v0[1]; t[2] = p.v0[2];
-    std::memcpy(&t[3], &pid, 4);
-    t[4] = p.e1[0]; t[5] = p.e1[1]; t[6] = p.e1[2]; t[7] = 0.0f;
-    t[8] = p.e2[0]; t[9] = p.e2[1]; t[10] = p.e2[2]; t[11] = 0.0f;
-  }
+  parallel_for(bvh.order.size(), threads, [&](size_t a, size_t b) {
+    for (size_t i = a; i < b; ++i) {
+      const uint32_t pid = bvh.order[i];
+      const Prim& p = bvh.prims[pid];
+      float* t = &out.tris[12 * i];
+      t[0] = p.v0[0]; t[1] = p.v0[1]; t[2] = p.v0[2];
+      std::memcpy(&t[3], &pid, 4);
+      t[4] = p.e1[0]; t[5] = p.e1[1]; t[6] = p.e1[2]; t[7] = 0.0f;
+      t[8] = p.e2[0]; t[9] = p.e2[1]; t[10] = p.e2[2]; t[11] = 0.0f;
+    }
+  });
   if (bvh.nodes.empty()) {
     out.root_ref = 0;
     return;
@@ -406,21 +435,23 @@ void pack_device_layout(const Bvh& bvh, DeviceLayout& out, bool with_uv) {
   };
   out.inner_count = inner;
   out.nodes.resize(16u * inner);
-  for (size_t i = 0; i < bvh.nodes.size(); ++i) {
-    const Node& nd = bvh.nodes[i];
-    if (nd.count > 0) continue;
-    const Node& c0 = bvh.nodes[static_cast<size_t>(nd.left)];
-    const Node& c1 = bvh.nodes[static_cast<size_t>(nd.right)];
-    float* g = &out.nodes[16u * static_cast<size_t>(remap[i])];
-    g[0] = c0.mn[0]; g[1] = c0.mn[1]; g[2] = c0.mn[2]; g[3] = c0.mx[0];
-    g[4] = c0.mx[1]; g[5] = c0.mx[2]; g[6] = c1.mn[0]; g[7] = c1.mn[1];
-    g[8] = c1.mn[2]; g[9] = c1.mx[0]; g[10] = c1.mx[1]; g[11] = c1.mx[2];
-    const int32_t r0 = ref_of(nd.left), r1 = ref_of(nd.right), z = 0;
-    std::memcpy(&g[12], &r0, 4);
-    std::memcpy(&g[13], &r1, 4);
-    std::memcpy(&g[14], &z, 4);
-    std::memcpy(&g[15], &z, 4);
-  }
+  parallel_for(bvh.nodes.size(), threads, [&](size_t a, size_t b) {
+    for (size_t i = a; i < b; ++i) {
+      const Node& nd = bvh.nodes[i];
+      if (nd.count > 0) continue;
+      const Node& c0 = bvh.nodes[static_cast<size_t>(nd.left)];
+      const Node& c1 = bvh.nodes[static_cast<size_t>(nd.right)];
+      float* g = &out.nodes[16u * static_cast<size_t>(remap[i])];
+      g[0] = c0.mn[0]; g[1] = c0.mn[1]; g[2] = c0.mn[2]; g[3] = c0.mx[0];
+      g[4] = c0.mx[1]; g[5] = c0.mx[2]; g[6] = c1.mn[0]; g[7] = c1.mn[1];
+      g[8] = c1.mn[2]; g[9] = c1.mx[0]; g[10] = c1.mx[1]; g[11] = c1.mx[2];
+      const int32_t r0 = ref_of(nd.left), r1 = ref_of(nd.right), z = 0;
+      std::memcpy(&g[12], &r0, 4);
+      std::memcpy(&g[13], &r1, 4);
+      std::memcpy(&g[14], &z, 4);
+      std::memcpy(&g[15], &z, 4);
+    }
+  });
   const Node& root = bvh.nodes[0];
   for (int k = 0; k < 3; ++k) {
     out.root_box[k] = root.mn[k];

@@ -63,6 +63,7 @@ constexpr int kLeafCountBits = 5;
 constexpr uint32_t kMaxLeafSize = (1u << kLeafCountBits) - 1u;
 constexpr uint32_t kMaxPrimitives = 1u << (31 - kLeafCountBits);
 
-void pack_device_layout(const Bvh& bvh, DeviceLayout& out, bool with_uv = false);
+// threads: host threads for the element-wise passes (1: the calling thread only).
+void pack_device_layout(const Bvh& bvh, DeviceLayout& out, bool with_uv = false, unsigned threads = 1);
 
 }  // namespace ptb

@@ -14,6 +14,7 @@
 #include <thread>
 #include <vector>
 
+#include "pt_build.h"
 #include "pt_bvh.h"
 #include "pt_device.h"
 #include "pt_math.h"
@@ -82,6 +83,7 @@ struct SceneSlot {
   uint32_t width = 0, height = 0, spp = 0;  // spp: samples per batch frame
   PtBvhParams bvh_params{PT_BVH_SAH, 4u, 8u, 0u};
   ptb::Bvh bvh;
+  bool built_on_device = false;  // the last build ran on the GPU
   ptb::DeviceLayout layout;
   std::vector<uint32_t> inst_first;  // per instance its first primitive, then the primitive count
   PtBuildInfo info{};
@@ -528,16 +530,28 @@ int check_bvh_params(pt_tracer* t, const PtBvhParams* params, PtBvhParams& p) {
 }
 
 // Host build of one scene's tree and device layout (thread-safe per slot).
-bool build_scene(SceneSlot& s) {
+// The scene's tree: the SAH tree on the GPU (on `stream`; the host builder's tree node for node)
+// unless the parameters ask for the host or a vertex is not finite, the midpoint tree on the host.
+// `threads` host threads pack the device layout.
+bool build_scene(SceneSlot& s, hipStream_t stream, unsigned threads) {
   const auto t0 = std::chrono::steady_clock::now();
   std::vector<float> raw;
   s.error.clear();
   if (!ptb::flatten(s.view, s.bvh.prims, raw, s.error, s.texcoords.empty() ? nullptr : s.texcoords.data(),
                     &s.inst_first))
     return false;
-  if (s.bvh_params.builder == PT_BVH_SAH) ptb::build_sah(s.bvh, raw, s.bvh_params.leaf_size, s.bvh_params.bucket_count);
-  else ptb::build_midpoint(s.bvh, raw, s.bvh_params.leaf_size);
-  ptb::pack_device_layout(s.bvh, s.layout, true);
+  if (s.bvh_params.builder == PT_BVH_SAH) {
+    int rc = 1;
+    if (!(s.bvh_params.flags & PT_BVH_FLAG_HOST_BUILD))
+      rc = ptg::build_sah_device(stream, s.bvh, raw, s.bvh_params.leaf_size, s.bvh_params.bucket_count, s.error);
+    if (rc < 0) return false;
+    if (rc > 0) ptb::build_sah(s.bvh, raw, s.bvh_params.leaf_size, s.bvh_params.bucket_count);
+    s.built_on_device = rc == 0;
+  } else {
+    ptb::build_midpoint(s.bvh, raw, s.bvh_params.leaf_size);
+    s.built_on_device = false;
+  }
+  ptb::pack_device_layout(s.bvh, s.layout, true, threads);
   s.info = PtBuildInfo{};
   s.info.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   s.info.built = 1;
@@ -546,6 +560,7 @@ bool build_scene(SceneSlot& s) {
   s.info.leaf_count = s.bvh.leaf_count;
   s.info.max_depth = s.bvh.max_depth;
   s.info.deterministic = 1;
+  s.info.device_built = s.built_on_device ? 1u : 0u;
   return true;
 }
 
@@ -873,8 +888,10 @@ int upload_pool(pt_tracer* t) {
   t->info.device_bytes = bytes;
   t->info.built = 1;
   t->info.deterministic = 1;
+  t->info.device_built = 1;
   for (const SceneSlot& s : t->scenes) {
     if (s.view_of >= 0) continue;
+    t->info.device_built &= s.info.device_built;
     t->info.build_ms += s.info.build_ms;
     t->info.primitive_count += s.info.primitive_count;
     t->info.node_count += s.info.node_count;
@@ -1416,6 +1433,7 @@ uint32_t pt_hip_sizeof(int which) {
     case 15: return sizeof(PtMaterialUpdate);
     case 16: return sizeof(PtLightUpdate);
     case 17: return sizeof(PtInstanceMaterialUpdate);
+    case 18: return sizeof(PtBuildInfo);
     default: return 0;
   }
 }
@@ -1536,7 +1554,8 @@ int pt_hip_build_accel(pt_tracer* t, const PtBvhParams* params, PtBuildInfo* inf
   SceneSlot& s = t->scenes[0];
   s.bvh_params = p;
   const auto t0 = std::chrono::steady_clock::now();
-  if (!build_scene(s)) return fail(t, PT_INVALID_ARGUMENT, s.error);
+  if (!build_scene(s, t->stream, std::max(1u, std::min(16u, std::thread::hardware_concurrency()))))
+    return fail(t, PT_INVALID_ARGUMENT, s.error);
   rc = upload_pool(t);
   if (rc != PT_OK) return rc;
   t->info.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -2375,20 +2394,31 @@ int pt_batch_build(pt_batch* b, PtBuildInfo* info) {
   if (t->scenes.empty()) return fail(t, PT_NOT_READY, "batch_build: no scenes");
   (void)hipSetDevice(t->device);
   const auto t0 = std::chrono::steady_clock::now();
-  // host builds in parallel, largest scenes first
+  // scene builds in parallel host threads, largest scenes first, each thread with its own stream for
+  // the GPU part
   std::vector<size_t> order;
   for (size_t i = 0; i < t->scenes.size(); ++i)
     if (t->scenes[i].view_of < 0) order.push_back(i);
   std::sort(order.begin(), order.end(),
             [&](size_t a, size_t c) { return t->scenes[a].indices.size() > t->scenes[c].indices.size(); });
   const unsigned nthreads = std::max(1u, std::min<unsigned>(16u, std::thread::hardware_concurrency()));
-  std::vector<bool> ok(t->scenes.size(), true);
+  std::vector<char> ok(t->scenes.size(), 1);  // one byte per scene: threads write different elements
   {
     std::vector<std::thread> pool;
     std::atomic<size_t> cursor{0};
     for (unsigned w = 0; w < std::min<size_t>(nthreads, order.size()); ++w)
       pool.emplace_back([&] {
-        for (size_t k = cursor++; k < order.size(); k = cursor++) ok[order[k]] = build_scene(t->scenes[order[k]]);
+        hipStream_t stream = nullptr;
+        if (hipSetDevice(t->device) != hipSuccess || hipStreamCreate(&stream) != hipSuccess) {
+          for (size_t k = cursor++; k < order.size(); k = cursor++) {
+            t->scenes[order[k]].error = "could not create a stream for the scene build";
+            ok[order[k]] = 0;
+          }
+          return;
+        }
+        for (size_t k = cursor++; k < order.size(); k = cursor++)
+          ok[order[k]] = build_scene(t->scenes[order[k]], stream, 1u);
+        (void)hipStreamDestroy(stream);
       });
     for (std::thread& th : pool) th.join();
   }
@@ -2623,9 +2653,12 @@ int pt_batch_synchronize(pt_batch* b) { return b ? pt_hip_synchronize(&b->t) : P
 
 }  // extern "C"
 
-extern "C" int pt_bvh_build_host(const PtSceneView* scene, const PtBvhParams* params, uint32_t* node_count,
-                                 uint32_t* prim_count, uint32_t* max_depth, float* bounds, int32_t* meta,
-                                 uint32_t* prim_order, float* prim_geo) {
+namespace {
+// The tree pt_hip_build_accel builds for `scene`, exported: on the host, or (device) the SAH tree on
+// the current HIP device through a stream of its own.
+int bvh_build_export(const PtSceneView* scene, const PtBvhParams* params, bool device, uint32_t* node_count,
+                     uint32_t* prim_count, uint32_t* max_depth, float* bounds, int32_t* meta, uint32_t* prim_order,
+                     float* prim_geo, uint32_t* built_on_device) {
   if (!scene) return PT_INVALID_ARGUMENT;
   PtBvhParams p{PT_BVH_SAH, 4u, 8u, 0u};
   if (params) p = *params;
@@ -2637,8 +2670,19 @@ extern "C" int pt_bvh_build_host(const PtSceneView* scene, const PtBvhParams* pa
   std::vector<float> raw;
   std::string err;
   if (!ptb::flatten(*scene, bvh.prims, raw, err)) return PT_INVALID_ARGUMENT;
-  if (p.builder == PT_BVH_SAH) ptb::build_sah(bvh, raw, p.leaf_size, p.bucket_count);
-  else ptb::build_midpoint(bvh, raw, p.leaf_size);
+  int rc = 1;
+  if (device && p.builder == PT_BVH_SAH) {
+    hipStream_t stream = nullptr;
+    if (hipStreamCreate(&stream) != hipSuccess) return PT_INTERNAL_ERROR;
+    rc = ptg::build_sah_device(stream, bvh, raw, p.leaf_size, p.bucket_count, err);
+    (void)hipStreamDestroy(stream);
+    if (rc < 0) return PT_INTERNAL_ERROR;
+  }
+  if (built_on_device) *built_on_device = rc == 0 ? 1u : 0u;
+  if (rc > 0) {
+    if (p.builder == PT_BVH_SAH) ptb::build_sah(bvh, raw, p.leaf_size, p.bucket_count);
+    else ptb::build_midpoint(bvh, raw, p.leaf_size);
+  }
   if (node_count) *node_count = static_cast<uint32_t>(bvh.nodes.size());
   if (prim_count) *prim_count = static_cast<uint32_t>(bvh.prims.size());
   if (max_depth) *max_depth = bvh.max_depth;
@@ -2669,4 +2713,19 @@ extern "C" int pt_bvh_build_host(const PtSceneView* scene, const PtBvhParams* pa
       }
     }
   return PT_OK;
+}
+}  // namespace
+
+extern "C" int pt_bvh_build_host(const PtSceneView* scene, const PtBvhParams* params, uint32_t* node_count,
+                                 uint32_t* prim_count, uint32_t* max_depth, float* bounds, int32_t* meta,
+                                 uint32_t* prim_order, float* prim_geo) {
+  return bvh_build_export(scene, params, false, node_count, prim_count, max_depth, bounds, meta, prim_order, prim_geo,
+                          nullptr);
+}
+
+extern "C" int pt_bvh_build_device(const PtSceneView* scene, const PtBvhParams* params, uint32_t* node_count,
+                                   uint32_t* prim_count, uint32_t* max_depth, float* bounds, int32_t* meta,
+                                   uint32_t* prim_order, float* prim_geo, uint32_t* built_on_device) {
+  return bvh_build_export(scene, params, true, node_count, prim_count, max_depth, bounds, meta, prim_order, prim_geo,
+                          built_on_device);
 }
