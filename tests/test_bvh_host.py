@@ -158,3 +158,32 @@ def test_non_finite_vertex_builds_and_matches_oracle():
     o = O.Oracle(sc, builder="sah", leaf_size=4, bucket_count=8).export()
     assert np.array_equal(a["meta"], o["meta"]) and np.array_equal(a["order"], o["order"])
     assert sorted(a["order"].tolist()) == list(range(len(t)))
+
+
+def test_parallel_flatten_numbers_primitives_like_the_sequential_loop():
+    """Above 64k triangles the flattening runs in several host threads, each triangle into its own
+    slot, and squeezes out the triangles the reference skips (an index beyond the vertex array, a
+    range beyond the index array). Empty instances, skipped triangles at chunk edges and an instance
+    transform must leave the primitive numbering, the tree and the instance ranges as the oracle's
+    one-thread loop makes them."""
+    rng = np.random.default_rng(9)
+    nv = 150_000
+    v = rng.uniform(-20, 20, (nv, 3)).astype(np.float32)
+    idx = rng.integers(0, nv, 3 * 140_000).astype(np.uint32)
+    bad = rng.choice(140_000, 300, replace=False)
+    idx[3 * bad + rng.integers(0, 3, 300)] = nv + 7  # out of range: the triangle is skipped
+    counts = [0, 50_000, 0, 0, 60_000, 1, 29_000, 0, 1_500]  # the last runs past the index array
+    inst = np.zeros(len(counts), S.INSTANCE_DTYPE)
+    inst["triangle_count"] = counts
+    inst["first_triangle"] = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    inst["rotation"] = (0, 0, 0, 1)
+    inst["scale"] = 1
+    inst["material_index"] = np.arange(len(counts)) % 3
+    inst[4]["translation"] = (1.5, -2.0, 0.25)  # instance transforms do not apply to shared geometry
+    sc = S.Scene(v, idx, inst, S.make_materials(3), np.zeros(0, S.LIGHT_DTYPE), S.Camera())
+    a = bvh.build_host(sc, "sah", 4, 8)
+    o = O.Oracle(sc, builder="sah", leaf_size=4, bucket_count=8).export()
+    assert len(a["order"]) == len(o["order"]) < sum(counts)
+    assert np.array_equal(a["geo"], o["geo"])
+    assert np.array_equal(a["meta"], o["meta"]) and np.array_equal(a["order"], o["order"])
+    assert np.array_equal(a["bounds"], o["bounds"])

@@ -33,7 +33,7 @@ def _jobs(cornell, small_procedural):
 
 @pytest.mark.parametrize("use_graph", [False, True])
 def test_batch_films_and_counters_equal_single_scene_renders(cornell, small_procedural, use_graph):
-    from vkpathtracer_amd import scene as S
+    from vkpathtracer_amd import bvh, scene as S
     from vkpathtracer_amd.tracer import HipBatch
 
     settings = S.RenderSettings(width=1, height=1, spp=1, max_depth=8)
@@ -42,7 +42,14 @@ def test_batch_films_and_counters_equal_single_scene_renders(cornell, small_proc
     with HipBatch(settings) as b:
         for sc, w, h, spp, leaf in jobs:
             b.add_scene(sc, w, h, spp, leaf_size=leaf)
-        b.build()
+        info = b.build()
+        # the scenes' trees are built together on the device (one level-synchronous build, each scene
+        # with its own leaf size): together they hold the host builder's nodes, leaves and depth
+        host = [bvh.build_host(sc, "sah", leaf, 8) for sc, *_, leaf in jobs]
+        assert info["device_built"] == 1
+        assert info["node_count"] == sum(len(h["meta"]) for h in host)
+        assert info["leaf_count"] == sum(int((h["meta"][:, 3] > 0).sum()) for h in host)
+        assert info["max_depth"] == max(h["max_depth"] for h in host)
         for f in frames:
             b.render(f, use_graph=use_graph)
         total = {k: 0 for k in COUNTERS}

@@ -26,6 +26,7 @@ constexpr uint32_t kMaxBuckets = 16;
 constexpr uint32_t kBigSegment = 2048;    // segments above this use 1024-thread workgroups
 constexpr uint32_t kHugeSegment = 65536;  // segments above this are split by many workgroups
 constexpr uint32_t kHugeChunk = 4096;     // references per workgroup of a huge segment's passes
+constexpr uint64_t kMaxForestRefs = 64ull << 20;  // references per level-synchronous build of several scenes
 
 // Primitive references, structure of arrays: centroid, box, original index.
 struct Refs {
@@ -35,8 +36,21 @@ struct Refs {
   uint32_t* orig;
 };
 
+// A node still to be split: its references [start, start + count), its node index, depth and scene.
 struct Seg {
-  uint32_t start, count, node, depth;
+  uint32_t start, count, node, depth, scene;
+};
+
+// Per scene of a build: its leaf size and bucket count, and where its references start in the
+// concatenated reference arrays.
+struct SceneTab {
+  uint32_t leaf, nb, ref_base, pad;
+};
+
+// Per scene, written by the build: the host builder's max_depth and leaf_count, and where the scene's
+// nodes start (and how many there are) in the renumbered node array.
+struct SceneOut {
+  uint32_t max_depth, leaf_count, node_base, node_count;
 };
 
 struct DevNode {  // ptb::Node
@@ -44,9 +58,14 @@ struct DevNode {  // ptb::Node
   int32_t left, right, first, count;
 };
 
-enum Counter : int {
-  kNextBig = 0, kNextSmall = 1, kNextHuge = 2, kNodeCount = 3, kMaxDepth = 4, kLeafCount = 5, kCounters = 8
-};
+enum Counter : int { kNextBig = 0, kNextHuge = 1, kInner = 2, kCounters = 4 };
+
+// Node numbering without shared counters. A level's segments are listed in node order (the segment at
+// position i is node level_base + i); an exclusive prefix count of its inner segments (those above
+// their scene's leaf size) gives the segment at i its rank r among them, its children the nodes
+// next_base + 2r and next_base + 2r + 1, and their places 2r and 2r + 1 in the next level's list.
+// The count runs in chunks of kScanChunk segments: rank = chunk-local rank + the chunk's offset.
+constexpr uint32_t kScanChunk = 1024;
 
 // std::min / std::max as the host builder writes them
 __device__ __forceinline__ float smin(float a, float b) { return (b < a) ? b : a; }
@@ -119,13 +138,173 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t x, uint32_t* red) {
   return s;
 }
 
+// The host builder's cost scan over the bins, one candidate split (axis, bucket boundary s) per lane of
+// one wave. Each lane evaluates its candidate with the scan's own float expressions; the scan keeps the
+// first candidate, in axis-major order, whose cost is below both 1e30 and every earlier cost, which is the
+// least cost with ties going to the earliest candidate. Costs are non-negative, so their bit patterns
+// order like the values and one 64-bit minimum (cost bits, candidate index) finds it. All 64 lanes of the
+// wave must call this; bin_n is [3][kMaxBuckets], bin_b [3][kMaxBuckets][6] (ordered uints), in LDS
+// or global memory.
+__device__ __forceinline__ void scan_splits(const uint32_t* bin_n, const uint32_t* bin_b, uint32_t nb,
+                                            const float* bmn, const float* bmx, const float* cmin,
+                                            const float* extent, int& found, int& best_axis, float& best_split) {
+  const uint32_t lane = threadIdx.x & 63u, per_axis = nb - 1u;
+  const float fB = static_cast<float>(nb);
+  const float node_sa = smax(1e-9f, bvh_sa(bmn, bmx));
+  uint64_t key = ~0ull;
+  if (lane < 3u * per_axis) {
+    const uint32_t axis = lane / per_axis, s = lane % per_axis + 1u;
+    if (!(extent[axis] < 1e-6f)) {
+      float lmn[3] = {1e30f, 1e30f, 1e30f}, lmx[3] = {-1e30f, -1e30f, -1e30f};
+      float rmn[3] = {1e30f, 1e30f, 1e30f}, rmx[3] = {-1e30f, -1e30f, -1e30f};
+      uint32_t lc = 0, rc = 0;
+      for (uint32_t b = 0; b < nb; ++b) {
+        const uint32_t c = bin_n[axis * kMaxBuckets + b];
+        if (!c) continue;
+        const uint32_t* q = bin_b + (axis * kMaxBuckets + b) * 6u;
+        if (b < s) {
+          lc += c;
+          for (int k = 0; k < 3; ++k) {
+            lmn[k] = smin(lmn[k], u2f(q[k]));
+            lmx[k] = smax(lmx[k], u2f(q[3 + k]));
+          }
+        } else {
+          rc += c;
+          for (int k = 0; k < 3; ++k) {
+            rmn[k] = smin(rmn[k], u2f(q[k]));
+            rmx[k] = smax(rmx[k], u2f(q[3 + k]));
+          }
+        }
+      }
+      if (lc && rc) {
+        const float cost =
+            (bvh_sa(lmn, lmx) * static_cast<float>(lc) + bvh_sa(rmn, rmx) * static_cast<float>(rc)) / node_sa;
+        if (cost < 1e30f) key = (static_cast<uint64_t>(__float_as_uint(cost)) << 32) | lane;
+      }
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint64_t other = __shfl_xor(key, off);
+    key = other < key ? other : key;
+  }
+  found = key != ~0ull ? 1 : 0;
+  best_axis = 0;
+  best_split = 0.0f;
+  if (found) {
+    const uint32_t c = static_cast<uint32_t>(key & 0xffffffffu);
+    best_axis = static_cast<int>(c / per_axis);
+    best_split = cmin[best_axis] + static_cast<float>(c % per_axis + 1u) / fB * extent[best_axis];
+  }
+}
+
+// Chunk-local exclusive ranks of the inner segments (256 threads, four segments each) and each chunk's
+// inner count.
+__global__ __launch_bounds__(256) void k_seg_rank(const Seg* segs, uint32_t m, const SceneTab* tab,
+                                                  uint32_t* rank_local, uint32_t* chunk_sum) {
+  __shared__ uint32_t wsum[4];
+  const uint32_t base = blockIdx.x * kScanChunk + threadIdx.x * 4u;
+  uint32_t f[4], t = 0;
+  for (uint32_t k = 0; k < 4u; ++k) {
+    const uint32_t i = base + k;
+    f[k] = 0u;
+    if (i < m) {
+      const Seg sg = segs[i];
+      f[k] = sg.count > tab[sg.scene].leaf ? 1u : 0u;
+    }
+    t += f[k];
+  }
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  uint32_t x = t;  // inclusive scan over the wave
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off);
+    if (lane >= static_cast<uint32_t>(off)) x += y;
+  }
+  if (lane == 63u) wsum[w] = x;
+  __syncthreads();
+  uint32_t run = x - t, tot = 0;
+  for (uint32_t j = 0; j < 4u; ++j) {
+    if (j < w) run += wsum[j];
+    tot += wsum[j];
+  }
+  for (uint32_t k = 0; k < 4u; ++k) {
+    const uint32_t i = base + k;
+    if (i < m) rank_local[i] = run;
+    run += f[k];
+  }
+  if (threadIdx.x == 0u) chunk_sum[blockIdx.x] = tot;
+}
+
+// The chunks' counts become exclusive offsets in place (one workgroup); the level's inner count goes to
+// counters[kInner].
+__global__ __launch_bounds__(1024) void k_chunk_offsets(uint32_t* chunk_sum, uint32_t chunks, uint32_t* counters) {
+  __shared__ uint32_t wsum[16];
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  uint32_t carry = 0;
+  for (uint32_t c0 = 0; c0 < chunks; c0 += 1024u) {
+    const uint32_t i = c0 + threadIdx.x;
+    const uint32_t t = i < chunks ? chunk_sum[i] : 0u;
+    uint32_t x = t;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t y = __shfl_up(x, off);
+      if (lane >= static_cast<uint32_t>(off)) x += y;
+    }
+    if (lane == 63u) wsum[w] = x;
+    __syncthreads();
+    uint32_t pre = x - t, tot = 0;
+    for (uint32_t j = 0; j < 16u; ++j) {
+      if (j < w) pre += wsum[j];
+      tot += wsum[j];
+    }
+    if (i < chunks) chunk_sum[i] = carry + pre;
+    carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0u) counters[kInner] = carry;
+}
+
+// Where an inner segment's children go (ranks as above).
+struct Numbering {
+  const uint32_t* rank_local;
+  const uint32_t* chunk_off;
+  uint32_t level_base, next_base;
+  Seg* next_all;  // the next level's segments, in node order
+  Seg* next_big;  // and, appended, those of them above kBigSegment (and at most kHugeSegment)
+  Seg* next_huge; // those above kHugeSegment
+  uint32_t* counters;
+};
+
+// Writes the inner node sg (its box in `node`) and its children [start, start + mid) and the rest.
+__device__ void emit_inner(const Numbering& N, const Seg& sg, DevNode node, uint32_t mid, DevNode* nodes,
+                           uint32_t* node_scene) {
+  const uint32_t i = sg.node - N.level_base;
+  const uint32_t r = N.rank_local[i] + N.chunk_off[i / kScanChunk];
+  const uint32_t base = N.next_base + 2u * r;
+  node.left = static_cast<int32_t>(base);
+  node.right = static_cast<int32_t>(base + 1u);
+  node.first = -1;
+  node.count = 0;
+  nodes[sg.node] = node;
+  node_scene[sg.node] = sg.scene;
+  const Seg kids[2] = {{sg.start, mid, base, sg.depth + 1u, sg.scene},
+                       {sg.start + mid, sg.count - mid, base + 1u, sg.depth + 1u, sg.scene}};
+  for (uint32_t c = 0; c < 2u; ++c) {
+    N.next_all[2u * r + c] = kids[c];
+    if (kids[c].count > kHugeSegment) N.next_huge[atomicAdd(N.counters + kNextHuge, 1u)] = kids[c];
+    else if (kids[c].count > kBigSegment) N.next_big[atomicAdd(N.counters + kNextBig, 1u)] = kids[c];
+  }
+}
+
 // One node per workgroup: its bounds, then a leaf, or the bucketed SAH split (binning of all three
-// axes into LDS, the host builder's cost scan by one thread) and the stable partition of its
-// references from A into B, and its two children appended to the next level.
+// axes into LDS, the host builder's cost scan on one wave) and the stable partition of its
+// references from A into B, and its two children. The 64-thread form runs over a level's whole list
+// and leaves the segments above kBigSegment to the 1024-thread form (run over the big list) and the
+// huge-segment passes.
 template <uint32_t NT>
-__global__ __launch_bounds__(NT) void k_split(const Seg* segs, Refs A, Refs B, DevNode* nodes, uint32_t* order,
-                                               Seg* next_big, Seg* next_small, uint32_t* counters, uint32_t leaf,
-                                               uint32_t nb) {
+__global__ __launch_bounds__(NT) void k_split(const Seg* segs, Refs A, Refs B, DevNode* nodes, uint32_t* node_scene,
+                                               uint32_t* order, Numbering N, const SceneTab* tab) {
   __shared__ float red[12][NT / 64u];
   __shared__ uint32_t ured[NT / 64u];
   __shared__ uint32_t bin_n[3][kMaxBuckets];
@@ -133,6 +312,9 @@ __global__ __launch_bounds__(NT) void k_split(const Seg* segs, Refs A, Refs B, D
   __shared__ int s_axis, s_found;
   __shared__ float s_split;
   const Seg sg = segs[blockIdx.x];
+  if (NT == 64u && sg.count > kBigSegment) return;
+  const SceneTab P = tab[sg.scene];
+  const uint32_t leaf = P.leaf, nb = P.nb;
   const uint32_t tid = threadIdx.x;
 
   // bounds of the boxes and of the centroids (the host starts from +-1e30)
@@ -160,15 +342,14 @@ __global__ __launch_bounds__(NT) void k_split(const Seg* segs, Refs A, Refs B, D
     node.mn[k] = bmn[k] - kEps;
     node.mx[k] = bmx[k] + kEps;
   }
-  if (sg.count <= leaf) {  // a leaf: its references in order are its slice of the leaf order
-    for (uint32_t i = tid; i < sg.count; i += NT) order[sg.start + i] = A.orig[sg.start + i];
+  if (sg.count <= leaf) {  // a leaf: its references in order are its slice of the scene's leaf order
+    for (uint32_t i = tid; i < sg.count; i += NT) order[sg.start + i] = A.orig[sg.start + i] - P.ref_base;
     if (tid == 0u) {
       node.left = node.right = -1;
-      node.first = static_cast<int32_t>(sg.start);
+      node.first = static_cast<int32_t>(sg.start - P.ref_base);
       node.count = static_cast<int32_t>(sg.count);
       nodes[sg.node] = node;
-      atomicMax(counters + kMaxDepth, sg.depth);
-      atomicAdd(counters + kLeafCount, 1u);
+      node_scene[sg.node] = sg.scene;
     }
     return;
   }
@@ -209,46 +390,15 @@ __global__ __launch_bounds__(NT) void k_split(const Seg* segs, Refs A, Refs B, D
     }
   }
   __syncthreads();
-  if (tid == 0u) {  // the host builder's cost scan, in its order and float expressions
-    const float node_sa = smax(1e-9f, bvh_sa(bmn, bmx));
-    int best_axis = 0;
-    float best_split = 0.0f, best_cost = 1e30f;
-    int found = 0;
-    for (int axis = 0; axis < 3; ++axis) {
-      if (extent[axis] < 1e-6f) continue;
-      for (uint32_t s = 1; s < nb; ++s) {
-        float lmn[3] = {1e30f, 1e30f, 1e30f}, lmx[3] = {-1e30f, -1e30f, -1e30f};
-        float rmn[3] = {1e30f, 1e30f, 1e30f}, rmx[3] = {-1e30f, -1e30f, -1e30f};
-        uint32_t lc = 0, rc = 0;
-        for (uint32_t b = 0; b < s; ++b) {
-          if (!bin_n[axis][b]) continue;
-          lc += bin_n[axis][b];
-          for (int k = 0; k < 3; ++k) {
-            lmn[k] = smin(lmn[k], u2f(bin_b[axis][b][k]));
-            lmx[k] = smax(lmx[k], u2f(bin_b[axis][b][3 + k]));
-          }
-        }
-        for (uint32_t b = s; b < nb; ++b) {
-          if (!bin_n[axis][b]) continue;
-          rc += bin_n[axis][b];
-          for (int k = 0; k < 3; ++k) {
-            rmn[k] = smin(rmn[k], u2f(bin_b[axis][b][k]));
-            rmx[k] = smax(rmx[k], u2f(bin_b[axis][b][3 + k]));
-          }
-        }
-        if (!lc || !rc) continue;
-        const float cost = (bvh_sa(lmn, lmx) * static_cast<float>(lc) + bvh_sa(rmn, rmx) * static_cast<float>(rc)) / node_sa;
-        if (cost < best_cost) {
-          best_cost = cost;
-          best_axis = axis;
-          best_split = cmin[axis] + static_cast<float>(s) / fB * extent[axis];
-          found = 1;
-        }
-      }
+  if (tid < 64u) {  // the host builder's cost scan, on the first wave
+    int found, best_axis;
+    float best_split;
+    scan_splits(&bin_n[0][0], &bin_b[0][0][0], nb, bmn, bmx, cmin, extent, found, best_axis, best_split);
+    if (tid == 0u) {
+      s_axis = best_axis;
+      s_split = best_split;
+      s_found = found;
     }
-    s_axis = best_axis;
-    s_split = best_split;
-    s_found = found;
   }
   __syncthreads();
   const int axis = s_axis;
@@ -304,17 +454,7 @@ __global__ __launch_bounds__(NT) void k_split(const Seg* segs, Refs A, Refs B, D
   if (tid == 0u) {
     uint32_t mid = found ? nl : sg.count / 2u;
     if (mid == 0u || mid >= sg.count) mid = sg.count / 2u;
-    const uint32_t base = atomicAdd(counters + kNodeCount, 2u);
-    node.left = static_cast<int32_t>(base);
-    node.right = static_cast<int32_t>(base + 1u);
-    node.first = -1;
-    node.count = 0;
-    nodes[sg.node] = node;
-    const Seg kids[2] = {{sg.start, mid, base, sg.depth + 1u}, {sg.start + mid, sg.count - mid, base + 1u, sg.depth + 1u}};
-    for (int c = 0; c < 2; ++c) {
-      if (kids[c].count > kBigSegment) next_big[atomicAdd(counters + kNextBig, 1u)] = kids[c];
-      else next_small[atomicAdd(counters + kNextSmall, 1u)] = kids[c];
-    }
+    emit_inner(N, sg, node, mid, nodes, node_scene);
   }
 }
 
@@ -382,13 +522,14 @@ __global__ __launch_bounds__(256) void k_huge_bounds(const Seg* huge, Refs A, Hu
   }
 }
 
-__global__ __launch_bounds__(256) void k_huge_bins(const Seg* huge, Refs A, HugeAcc* accs, uint32_t nb) {
+__global__ __launch_bounds__(256) void k_huge_bins(const Seg* huge, Refs A, HugeAcc* accs, const SceneTab* tab) {
   __shared__ uint32_t bin_n[3][kMaxBuckets];
   __shared__ uint32_t bin_b[3][kMaxBuckets][6];
   Seg sg;
   uint32_t c0, c1;
   if (!huge_chunk(huge, sg, c0, c1)) return;
   HugeAcc* acc = accs + blockIdx.y;
+  const uint32_t nb = tab[sg.scene].nb;
   const float fB = static_cast<float>(nb);
   float cmin[3], extent[3];
   for (int k = 0; k < 3; ++k) {
@@ -435,11 +576,10 @@ __global__ __launch_bounds__(256) void k_huge_bins(const Seg* huge, Refs A, Huge
   }
 }
 
-// k_split's cost scan over the combined bins, one thread per segment.
-__global__ void k_huge_decide(HugeAcc* accs, uint32_t nb) {
-  if (threadIdx.x != 0u) return;
+// k_split's cost scan over the combined bins, one wave per segment.
+__global__ __launch_bounds__(64) void k_huge_decide(const Seg* huge, HugeAcc* accs, const SceneTab* tab) {
   HugeAcc* acc = accs + blockIdx.x;
-  const float fB = static_cast<float>(nb);
+  const uint32_t nb = tab[huge[blockIdx.x].scene].nb;
   float bmn[3], bmx[3], cmin[3], extent[3];
   for (int k = 0; k < 3; ++k) {
     bmn[k] = u2f(acc->bounds[k]);
@@ -447,45 +587,14 @@ __global__ void k_huge_decide(HugeAcc* accs, uint32_t nb) {
     cmin[k] = u2f(acc->bounds[6 + k]);
     extent[k] = u2f(acc->bounds[9 + k]) - cmin[k];
   }
-  const float node_sa = smax(1e-9f, bvh_sa(bmn, bmx));
-  int best_axis = 0;
-  float best_split = 0.0f, best_cost = 1e30f;
-  int found = 0;
-  for (int axis = 0; axis < 3; ++axis) {
-    if (extent[axis] < 1e-6f) continue;
-    for (uint32_t s = 1; s < nb; ++s) {
-      float lmn[3] = {1e30f, 1e30f, 1e30f}, lmx[3] = {-1e30f, -1e30f, -1e30f};
-      float rmn[3] = {1e30f, 1e30f, 1e30f}, rmx[3] = {-1e30f, -1e30f, -1e30f};
-      uint32_t lc = 0, rc = 0;
-      for (uint32_t b = 0; b < s; ++b) {
-        if (!acc->bin_n[axis][b]) continue;
-        lc += acc->bin_n[axis][b];
-        for (int k = 0; k < 3; ++k) {
-          lmn[k] = smin(lmn[k], u2f(acc->bin_b[axis][b][k]));
-          lmx[k] = smax(lmx[k], u2f(acc->bin_b[axis][b][3 + k]));
-        }
-      }
-      for (uint32_t b = s; b < nb; ++b) {
-        if (!acc->bin_n[axis][b]) continue;
-        rc += acc->bin_n[axis][b];
-        for (int k = 0; k < 3; ++k) {
-          rmn[k] = smin(rmn[k], u2f(acc->bin_b[axis][b][k]));
-          rmx[k] = smax(rmx[k], u2f(acc->bin_b[axis][b][3 + k]));
-        }
-      }
-      if (!lc || !rc) continue;
-      const float cost = (bvh_sa(lmn, lmx) * static_cast<float>(lc) + bvh_sa(rmn, rmx) * static_cast<float>(rc)) / node_sa;
-      if (cost < best_cost) {
-        best_cost = cost;
-        best_axis = axis;
-        best_split = cmin[axis] + static_cast<float>(s) / fB * extent[axis];
-        found = 1;
-      }
-    }
+  int found, best_axis;
+  float best_split;
+  scan_splits(&acc->bin_n[0][0], &acc->bin_b[0][0][0], nb, bmn, bmx, cmin, extent, found, best_axis, best_split);
+  if (threadIdx.x == 0u) {
+    acc->found = found;
+    acc->axis = best_axis;
+    acc->split = best_split;
   }
-  acc->found = found;
-  acc->axis = best_axis;
-  acc->split = best_split;
 }
 
 __global__ __launch_bounds__(256) void k_huge_count(const Seg* huge, Refs A, const HugeAcc* accs, uint32_t* chunk_left,
@@ -508,7 +617,7 @@ __global__ __launch_bounds__(256) void k_huge_count(const Seg* huge, Refs A, con
 // Per segment (one thread): the chunks' left counts become exclusive offsets in place, then the node
 // record and the two children, as k_split writes them.
 __global__ void k_huge_finish(const Seg* huge, HugeAcc* accs, uint32_t* chunk_left, uint32_t max_chunks,
-                              DevNode* nodes, Seg* next_huge, Seg* next_big, Seg* next_small, uint32_t* counters) {
+                              DevNode* nodes, uint32_t* node_scene, Numbering N) {
   if (threadIdx.x != 0u) return;
   const Seg sg = huge[blockIdx.x];
   HugeAcc* acc = accs + blockIdx.x;
@@ -528,18 +637,7 @@ __global__ void k_huge_finish(const Seg* huge, HugeAcc* accs, uint32_t* chunk_le
     node.mn[k] = u2f(acc->bounds[k]) - kEps;
     node.mx[k] = u2f(acc->bounds[3 + k]) + kEps;
   }
-  const uint32_t base = atomicAdd(counters + kNodeCount, 2u);
-  node.left = static_cast<int32_t>(base);
-  node.right = static_cast<int32_t>(base + 1u);
-  node.first = -1;
-  node.count = 0;
-  nodes[sg.node] = node;
-  const Seg kids[2] = {{sg.start, mid, base, sg.depth + 1u}, {sg.start + mid, sg.count - mid, base + 1u, sg.depth + 1u}};
-  for (int c = 0; c < 2; ++c) {
-    if (kids[c].count > kHugeSegment) next_huge[atomicAdd(counters + kNextHuge, 1u)] = kids[c];
-    else if (kids[c].count > kBigSegment) next_big[atomicAdd(counters + kNextBig, 1u)] = kids[c];
-    else next_small[atomicAdd(counters + kNextSmall, 1u)] = kids[c];
-  }
+  emit_inner(N, sg, node, mid, nodes, node_scene);
 }
 
 // k_split's stable partition, each chunk starting from the offsets of the chunks before it (every
@@ -595,14 +693,17 @@ __global__ __launch_bounds__(256) void k_huge_scatter(const Seg* huge, Refs A, R
   }
 }
 
-// Renumbering into the host builder's pre-order. The nodes created while level L is split are level
-// L + 1, a contiguous range of indices, so each pass is one launch per level: subtree sizes from the
-// deepest level up, pre-order positions from the root down, then the scatter.
-__global__ void k_subtree_sizes(const DevNode* nodes, uint32_t* size, uint32_t lo, uint32_t hi) {
+// Renumbering into the host builder's pre-order, per scene. The nodes created while level L is split
+// are level L + 1, a contiguous range of indices, so each pass is one launch per level: subtree sizes
+// from the deepest level up, pre-order positions from the roots down (scene s's root is node s), then
+// the scatter of each scene's nodes to its own range of the output.
+// Subtree sizes and heights (a root's height is its scene's max_depth: the host builder's deepest node).
+__global__ void k_subtree_sizes(const DevNode* nodes, uint32_t* size, uint32_t* height, uint32_t lo, uint32_t hi) {
   const uint32_t b = lo + blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= hi) return;
   const DevNode& d = nodes[b];
   size[b] = d.count > 0 ? 1u : 1u + size[d.left] + size[d.right];
+  height[b] = d.count > 0 ? 0u : 1u + max(height[d.left], height[d.right]);
 }
 
 __global__ void k_preorder(const DevNode* nodes, const uint32_t* size, uint32_t* pre, uint32_t lo, uint32_t hi) {
@@ -614,7 +715,22 @@ __global__ void k_preorder(const DevNode* nodes, const uint32_t* size, uint32_t*
   pre[d.right] = pre[b] + 1u + size[d.left];
 }
 
-__global__ void k_scatter_preorder(const DevNode* nodes, const uint32_t* pre, DevNode* out, uint32_t n) {
+// The scenes' node ranges of the output, depths and leaf counts (one thread; scene s's root is node s,
+// and a tree whose inner nodes all have two children has (nodes + 1) / 2 leaves).
+__global__ void k_scene_bases(const uint32_t* size, const uint32_t* height, SceneOut* sout, uint32_t scenes) {
+  if (blockIdx.x != 0u || threadIdx.x != 0u) return;
+  uint32_t base = 0;
+  for (uint32_t s = 0; s < scenes; ++s) {
+    sout[s].node_base = base;
+    sout[s].node_count = size[s];
+    sout[s].max_depth = height[s];
+    sout[s].leaf_count = (size[s] + 1u) / 2u;
+    base += size[s];
+  }
+}
+
+__global__ void k_scatter_preorder(const DevNode* nodes, const uint32_t* node_scene, const uint32_t* pre,
+                                   const SceneOut* sout, DevNode* out, uint32_t n) {
   const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= n) return;
   DevNode d = nodes[b];
@@ -622,7 +738,7 @@ __global__ void k_scatter_preorder(const DevNode* nodes, const uint32_t* pre, De
     d.left = static_cast<int32_t>(pre[d.left]);
     d.right = static_cast<int32_t>(pre[d.right]);
   }
-  out[pre[b]] = d;
+  out[sout[node_scene[b]].node_base + pre[b]] = d;
 }
 
 // Stream-ordered scratch: allocation and release follow the build's stream, so builds of several
@@ -641,49 +757,55 @@ struct DevBuf {
   }
 };
 
-}  // namespace
-
-int build_sah_device(hipStream_t stream, ptb::Bvh& bvh, const std::vector<float>& raw, uint32_t leaf_size,
-                     uint32_t buckets, std::string& err) {
-  const uint32_t n = static_cast<uint32_t>(bvh.prims.size());
-  bvh.nodes.clear();
-  bvh.order.clear();
-  bvh.max_depth = 0;
-  bvh.leaf_count = 0;
-  if (n == 0) return 0;
-  // NaN boxes make min/max order-dependent, and coordinates beyond the builder's 1e30 sentinels can
-  // overflow a centroid: the host builds such scenes
-  for (float x : raw)
-    if (!(std::fabs(x) <= 1e30f)) return 1;
-  const uint32_t leaf = leaf_size ? leaf_size : 4u;
-  const uint32_t nb = std::clamp(buckets ? buckets : 8u, 2u, kMaxBuckets);
+// One level-synchronous build of several scenes (a forest): their references are concatenated and
+// every scene's root is a segment of the first level, so the scenes share each level's launches and
+// its one synchronisation.
+int build_forest(hipStream_t stream, std::vector<DeviceBuild>& jobs, const std::vector<uint32_t>& sel,
+                 std::string& err) {
+  const uint32_t S = static_cast<uint32_t>(sel.size());
+  std::vector<SceneTab> tab(S);
+  uint32_t n = 0;
+  for (uint32_t s = 0; s < S; ++s) {
+    const DeviceBuild& d = jobs[sel[s]];
+    tab[s].leaf = d.leaf_size ? d.leaf_size : 4u;
+    tab[s].nb = std::clamp(d.buckets ? d.buckets : 8u, 2u, kMaxBuckets);
+    tab[s].ref_base = n;
+    tab[s].pad = 0u;
+    n += static_cast<uint32_t>(d.bvh->prims.size());
+  }
   DevBuf<float> d_raw, d_ref[2];
-  DevBuf<uint32_t> d_orig[2], d_order, d_counters;
-  DevBuf<Seg> d_seg[4];   // big and small lists of the current and the next level
-  DevBuf<Seg> d_huge[2];  // huge lists of the current and the next level
+  DevBuf<uint32_t> d_orig[2], d_order, d_counters, d_node_scene, d_rank, d_chunk;
+  DevBuf<Seg> d_all[2];   // a level's segments in node order: this level's and the next
+  DevBuf<Seg> d_big[2];   // of those, the ones above kBigSegment (and at most kHugeSegment)
+  DevBuf<Seg> d_huge[2];  // and the ones above kHugeSegment
   DevBuf<HugeAcc> d_acc;
   DevBuf<uint32_t> d_chunk_left;
   DevBuf<DevNode> d_nodes, d_pre_nodes;
+  DevBuf<SceneTab> d_tab;
+  DevBuf<SceneOut> d_sout;
   auto check = [&](hipError_t e, const char* what) {
     if (e != hipSuccess) err = std::string("device BVH build: ") + what + ": " + hipGetErrorString(e);
     return e == hipSuccess;
   };
+  // segments of one level are disjoint: at most n of them, n / kBigSegment above kBigSegment and
+  // n / kHugeSegment above kHugeSegment
+  const uint32_t max_big = n / kBigSegment + 1u, max_huge = n / kHugeSegment + 1u;
+  const uint32_t max_chunks = (n + kHugeChunk - 1u) / kHugeChunk, max_scan = (n + kScanChunk - 1u) / kScanChunk;
   const size_t n9 = 9u * static_cast<size_t>(n), n2 = 2u * static_cast<size_t>(n);
   if (!check(d_raw.alloc(n9, stream), "allocate") || !check(d_ref[0].alloc(n9, stream), "allocate") ||
       !check(d_ref[1].alloc(n9, stream), "allocate") || !check(d_orig[0].alloc(n, stream), "allocate") ||
       !check(d_orig[1].alloc(n, stream), "allocate") || !check(d_order.alloc(n, stream), "allocate") ||
       !check(d_counters.alloc(kCounters, stream), "allocate") || !check(d_nodes.alloc(n2, stream), "allocate") ||
-      !check(d_pre_nodes.alloc(n2, stream), "allocate"))
+      !check(d_node_scene.alloc(n2, stream), "allocate") || !check(d_pre_nodes.alloc(n2, stream), "allocate") ||
+      !check(d_rank.alloc(n, stream), "allocate") || !check(d_chunk.alloc(max_scan, stream), "allocate") ||
+      !check(d_acc.alloc(max_huge, stream), "allocate") ||
+      !check(d_chunk_left.alloc(static_cast<size_t>(max_huge) * max_chunks, stream), "allocate") ||
+      !check(d_tab.alloc(S, stream), "allocate") || !check(d_sout.alloc(S, stream), "allocate"))
     return -1;
-  for (auto& s : d_seg)
-    if (!check(s.alloc(n, stream), "allocate")) return -1;
-  // huge segments are disjoint and each holds more than kHugeSegment references
-  const uint32_t max_huge = n / kHugeSegment + 1u, max_chunks = (n + kHugeChunk - 1u) / kHugeChunk;
-  for (auto& s : d_huge)
-    if (!check(s.alloc(max_huge, stream), "allocate")) return -1;
-  if (!check(d_acc.alloc(max_huge, stream), "allocate") ||
-      !check(d_chunk_left.alloc(static_cast<size_t>(max_huge) * max_chunks, stream), "allocate"))
-    return -1;
+  for (int k = 0; k < 2; ++k)
+    if (!check(d_all[k].alloc(n, stream), "allocate") || !check(d_big[k].alloc(max_big, stream), "allocate") ||
+        !check(d_huge[k].alloc(max_huge, stream), "allocate"))
+      return -1;
   Refs R[2];
   for (int b = 0; b < 2; ++b) {
     for (int k = 0; k < 3; ++k) {
@@ -693,30 +815,46 @@ int build_sah_device(hipStream_t stream, ptb::Bvh& bvh, const std::vector<float>
     }
     R[b].orig = d_orig[b].p;
   }
-  if (!check(hipMemcpyAsync(d_raw.p, raw.data(), raw.size() * sizeof(float), hipMemcpyHostToDevice, stream), "upload"))
-    return -1;
+  for (uint32_t s = 0; s < S; ++s) {
+    const ptb::HostVec<float>& raw = *jobs[sel[s]].raw;
+    if (!check(hipMemcpyAsync(d_raw.p + 9u * static_cast<size_t>(tab[s].ref_base), raw.data(),
+                              raw.size() * sizeof(float), hipMemcpyHostToDevice, stream), "upload"))
+      return -1;
+  }
   hipLaunchKernelGGL(k_build_refs, dim3((n + 255u) / 256u), dim3(256), 0, stream, d_raw.p, n, R[0]);
-  uint32_t counters[kCounters] = {};
-  counters[kNodeCount] = 1u;  // the root is node 0
-  const Seg root{0u, n, 0u, 0u};
-  uint32_t nhuge = n > kHugeSegment ? 1u : 0u, nbig = !nhuge && n > kBigSegment ? 1u : 0u;
-  uint32_t nsmall = nhuge + nbig ? 0u : 1u;
-  Seg* root_list = nhuge ? d_huge[0].p : d_seg[nbig ? 0 : 1].p;
-  std::vector<Seg> huge(max_huge);
-  if (nhuge) huge[0] = root;
-  if (!check(hipMemcpyAsync(root_list, &root, sizeof(Seg), hipMemcpyHostToDevice, stream), "upload") ||
-      !check(hipMemcpyAsync(d_counters.p, counters, sizeof(counters), hipMemcpyHostToDevice, stream), "upload"))
+  // level 0: the roots (scene s is node s)
+  std::vector<Seg> roots(S), big0, huge;
+  for (uint32_t s = 0; s < S; ++s) {
+    const uint32_t c = static_cast<uint32_t>(jobs[sel[s]].bvh->prims.size());
+    roots[s] = Seg{tab[s].ref_base, c, s, 0u, s};
+    if (c > kHugeSegment) huge.push_back(roots[s]);
+    else if (c > kBigSegment) big0.push_back(roots[s]);
+  }
+  uint32_t nhuge = static_cast<uint32_t>(huge.size()), nbig = static_cast<uint32_t>(big0.size());
+  huge.resize(max_huge);
+  if (!check(hipMemcpyAsync(d_all[0].p, roots.data(), S * sizeof(Seg), hipMemcpyHostToDevice, stream), "upload") ||
+      (nhuge && !check(hipMemcpyAsync(d_huge[0].p, huge.data(), nhuge * sizeof(Seg), hipMemcpyHostToDevice, stream),
+                       "upload")) ||
+      (nbig && !check(hipMemcpyAsync(d_big[0].p, big0.data(), nbig * sizeof(Seg), hipMemcpyHostToDevice, stream),
+                      "upload")) ||
+      !check(hipMemcpyAsync(d_tab.p, tab.data(), S * sizeof(SceneTab), hipMemcpyHostToDevice, stream), "upload"))
     return -1;
-  int cur = 0;  // lists 2 * cur (big), 2 * cur + 1 (small), d_huge[cur]; references in R[level % 2]
-  std::vector<uint32_t> level_start{0u, 1u};  // nodes of level L: [level_start[L], level_start[L + 1])
-  for (uint32_t level = 0; nhuge + nbig + nsmall > 0u; ++level) {
-    const uint32_t zero3[3] = {0u, 0u, 0u};  // kNextBig, kNextSmall, kNextHuge
-    if (!check(hipMemcpyAsync(d_counters.p, zero3, sizeof(zero3), hipMemcpyHostToDevice, stream), "reset"))
+  int cur = 0;  // the level's lists are d_all/d_big/d_huge[cur]; its references are in R[level % 2]
+  std::vector<uint32_t> level_start{0u};  // nodes of level L: [level_start[L], level_start[L + 1])
+  uint32_t level_base = 0, m = S;
+  uint32_t counters[kCounters] = {};
+  for (uint32_t level = 0; m > 0u; ++level) {
+    const uint32_t zero2[2] = {0u, 0u};  // kNextBig, kNextHuge
+    if (!check(hipMemcpyAsync(d_counters.p, zero2, sizeof(zero2), hipMemcpyHostToDevice, stream), "reset"))
       return -1;
     const Refs& A = R[level % 2u];
     const Refs& B = R[(level + 1u) % 2u];
-    Seg* nb_list = d_seg[2 * (1 - cur)].p;
-    Seg* ns_list = d_seg[2 * (1 - cur) + 1].p;
+    const uint32_t scan_chunks = (m + kScanChunk - 1u) / kScanChunk;
+    hipLaunchKernelGGL(k_seg_rank, dim3(scan_chunks), dim3(256), 0, stream, d_all[cur].p, m, d_tab.p, d_rank.p,
+                       d_chunk.p);
+    hipLaunchKernelGGL(k_chunk_offsets, dim3(1), dim3(1024), 0, stream, d_chunk.p, scan_chunks, d_counters.p);
+    const Numbering N{d_rank.p, d_chunk.p, level_base, level_base + m, d_all[1 - cur].p, d_big[1 - cur].p,
+                      d_huge[1 - cur].p, d_counters.p};
     if (nhuge) {
       uint32_t grid_x = 0;
       for (uint32_t h = 0; h < nhuge; ++h) grid_x = std::max(grid_x, (huge[h].count + kHugeChunk - 1u) / kHugeChunk);
@@ -724,19 +862,18 @@ int build_sah_device(hipStream_t stream, ptb::Bvh& bvh, const std::vector<float>
       const dim3 grid(grid_x, nhuge);
       hipLaunchKernelGGL(k_huge_init, dim3(nhuge), dim3(256), 0, stream, d_acc.p);
       hipLaunchKernelGGL(k_huge_bounds, grid, dim3(256), 0, stream, hl, A, d_acc.p);
-      hipLaunchKernelGGL(k_huge_bins, grid, dim3(256), 0, stream, hl, A, d_acc.p, nb);
-      hipLaunchKernelGGL(k_huge_decide, dim3(nhuge), dim3(64), 0, stream, d_acc.p, nb);
+      hipLaunchKernelGGL(k_huge_bins, grid, dim3(256), 0, stream, hl, A, d_acc.p, d_tab.p);
+      hipLaunchKernelGGL(k_huge_decide, dim3(nhuge), dim3(64), 0, stream, hl, d_acc.p, d_tab.p);
       hipLaunchKernelGGL(k_huge_count, grid, dim3(256), 0, stream, hl, A, d_acc.p, d_chunk_left.p, max_chunks);
       hipLaunchKernelGGL(k_huge_finish, dim3(nhuge), dim3(64), 0, stream, hl, d_acc.p, d_chunk_left.p, max_chunks,
-                         d_nodes.p, d_huge[1 - cur].p, nb_list, ns_list, d_counters.p);
+                         d_nodes.p, d_node_scene.p, N);
       hipLaunchKernelGGL(k_huge_scatter, grid, dim3(256), 0, stream, hl, A, B, d_acc.p, d_chunk_left.p, max_chunks);
     }
     if (nbig)
-      hipLaunchKernelGGL(k_split<1024>, dim3(nbig), dim3(1024), 0, stream, d_seg[2 * cur].p, A, B, d_nodes.p,
-                         d_order.p, nb_list, ns_list, d_counters.p, leaf, nb);
-    if (nsmall)
-      hipLaunchKernelGGL(k_split<64>, dim3(nsmall), dim3(64), 0, stream, d_seg[2 * cur + 1].p, A, B, d_nodes.p,
-                         d_order.p, nb_list, ns_list, d_counters.p, leaf, nb);
+      hipLaunchKernelGGL(k_split<1024>, dim3(nbig), dim3(1024), 0, stream, d_big[cur].p, A, B, d_nodes.p,
+                         d_node_scene.p, d_order.p, N, d_tab.p);
+    hipLaunchKernelGGL(k_split<64>, dim3(m), dim3(64), 0, stream, d_all[cur].p, A, B, d_nodes.p, d_node_scene.p,
+                       d_order.p, N, d_tab.p);
     // the next huge list travels with the counters (it is short: at most n / kHugeSegment entries)
     if (!check(hipGetLastError(), "launch") ||
         !check(hipMemcpyAsync(counters, d_counters.p, sizeof(counters), hipMemcpyDeviceToHost, stream), "read") ||
@@ -746,43 +883,97 @@ int build_sah_device(hipStream_t stream, ptb::Bvh& bvh, const std::vector<float>
       return -1;
     nhuge = counters[kNextHuge];
     nbig = counters[kNextBig];
-    nsmall = counters[kNextSmall];
-    level_start.push_back(counters[kNodeCount]);
+    level_base += m;
+    level_start.push_back(level_base);
+    m = 2u * counters[kInner];
     cur = 1 - cur;
   }
-  const uint32_t node_count = counters[kNodeCount];
-  // pre-order renumbering on the device (the arrays of the finished lists are reused)
-  uint32_t* size = reinterpret_cast<uint32_t*>(d_seg[0].p);  // node_count <= 2n - 1 < 4n words
-  uint32_t* pre = reinterpret_cast<uint32_t*>(d_seg[2].p);
+  const uint32_t node_count = level_base;
+  // pre-order renumbering on the device (the arrays of the finished lists are reused: each holds 5n
+  // words, and node_count < 2n)
+  static_assert(sizeof(Seg) == 20 && sizeof(DevNode) == 40 && sizeof(ptb::Node) == 40, "record sizes");
+  uint32_t* size = reinterpret_cast<uint32_t*>(d_all[0].p);
+  uint32_t* height = size + n2;
+  uint32_t* pre = reinterpret_cast<uint32_t*>(d_all[1].p);
   DevNode* out = d_pre_nodes.p;
-  static_assert(sizeof(Seg) == 16 && sizeof(DevNode) == 40 && sizeof(ptb::Node) == 40, "record sizes");
   const uint32_t levels = static_cast<uint32_t>(level_start.size()) - 1u;
   for (uint32_t L = levels; L-- > 0;) {
     const uint32_t lo = level_start[L], hi = level_start[L + 1];
-    if (hi > lo)
-      hipLaunchKernelGGL(k_subtree_sizes, dim3((hi - lo + 255u) / 256u), dim3(256), 0, stream, d_nodes.p, size, lo, hi);
+    hipLaunchKernelGGL(k_subtree_sizes, dim3((hi - lo + 255u) / 256u), dim3(256), 0, stream, d_nodes.p, size, height,
+                       lo, hi);
   }
-  const uint32_t zero = 0u;
-  if (!check(hipMemcpyAsync(pre, &zero, sizeof(zero), hipMemcpyHostToDevice, stream), "upload")) return -1;
+  if (!check(hipMemsetAsync(pre, 0, S * sizeof(uint32_t), stream), "clear")) return -1;  // every root is 0
   for (uint32_t L = 0; L < levels; ++L) {
     const uint32_t lo = level_start[L], hi = level_start[L + 1];
-    if (hi > lo)
-      hipLaunchKernelGGL(k_preorder, dim3((hi - lo + 255u) / 256u), dim3(256), 0, stream, d_nodes.p, size, pre, lo, hi);
+    hipLaunchKernelGGL(k_preorder, dim3((hi - lo + 255u) / 256u), dim3(256), 0, stream, d_nodes.p, size, pre, lo, hi);
   }
-  hipLaunchKernelGGL(k_scatter_preorder, dim3((node_count + 255u) / 256u), dim3(256), 0, stream, d_nodes.p, pre, out,
-                     node_count);
-  bvh.nodes.resize(node_count);
-  bvh.order.resize(n);
+  hipLaunchKernelGGL(k_scene_bases, dim3(1), dim3(64), 0, stream, size, height, d_sout.p, S);
+  hipLaunchKernelGGL(k_scatter_preorder, dim3((node_count + 255u) / 256u), dim3(256), 0, stream, d_nodes.p,
+                     d_node_scene.p, pre, d_sout.p, out, node_count);
+  std::vector<SceneOut> sout(S);
   if (!check(hipGetLastError(), "launch") ||
-      !check(hipMemcpyAsync(bvh.nodes.data(), out, node_count * sizeof(DevNode), hipMemcpyDeviceToHost, stream),
-             "download") ||
-      !check(hipMemcpyAsync(bvh.order.data(), d_order.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, stream),
-             "download") ||
+      !check(hipMemcpyAsync(sout.data(), d_sout.p, S * sizeof(SceneOut), hipMemcpyDeviceToHost, stream), "read") ||
       !check(hipStreamSynchronize(stream), "synchronize"))
     return -1;
-  bvh.max_depth = counters[kMaxDepth];
-  bvh.leaf_count = counters[kLeafCount];
+  for (uint32_t s = 0; s < S; ++s) {
+    ptb::Bvh& bvh = *jobs[sel[s]].bvh;
+    const size_t prims = bvh.prims.size();
+    bvh.nodes.resize(sout[s].node_count);
+    bvh.order.resize(prims);
+    if (!check(hipMemcpyAsync(bvh.nodes.data(), out + sout[s].node_base, sout[s].node_count * sizeof(DevNode),
+                              hipMemcpyDeviceToHost, stream), "download") ||
+        !check(hipMemcpyAsync(bvh.order.data(), d_order.p + tab[s].ref_base, prims * sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, stream), "download"))
+      return -1;
+    bvh.max_depth = sout[s].max_depth;
+    bvh.leaf_count = sout[s].leaf_count;
+  }
+  return check(hipStreamSynchronize(stream), "synchronize") ? 0 : -1;
+}
+
+}  // namespace
+
+int build_sah_device_many(hipStream_t stream, std::vector<DeviceBuild>& jobs, std::string& err) {
+  std::vector<uint32_t> sel;
+  uint64_t refs = 0;
+  for (uint32_t j = 0; j < jobs.size(); ++j) {
+    DeviceBuild& d = jobs[j];
+    ptb::Bvh& bvh = *d.bvh;
+    bvh.nodes.clear();
+    bvh.order.clear();
+    bvh.max_depth = 0;
+    bvh.leaf_count = 0;
+    d.rc = 0;
+    if (bvh.prims.empty()) continue;
+    // NaN boxes make min/max order-dependent, and coordinates beyond the builder's 1e30 sentinels can
+    // overflow a centroid: the host builds such scenes
+    bool finite = true;
+    for (float x : *d.raw)
+      if (!(std::fabs(x) <= 1e30f)) {
+        finite = false;
+        break;
+      }
+    if (!finite) {
+      d.rc = 1;
+      continue;
+    }
+    // one forest holds at most kMaxForestRefs references (device scratch of about 370 bytes each)
+    if (!sel.empty() && refs + bvh.prims.size() > kMaxForestRefs) {
+      if (build_forest(stream, jobs, sel, err) < 0) return -1;
+      sel.clear();
+      refs = 0;
+    }
+    sel.push_back(j);
+    refs += bvh.prims.size();
+  }
+  if (!sel.empty() && build_forest(stream, jobs, sel, err) < 0) return -1;
   return 0;
+}
+
+int build_sah_device(hipStream_t stream, ptb::Bvh& bvh, const ptb::HostVec<float>& raw, uint32_t leaf_size,
+                     uint32_t buckets, std::string& err) {
+  std::vector<DeviceBuild> jobs{DeviceBuild{&bvh, &raw, leaf_size, buckets, 0}};
+  return build_sah_device_many(stream, jobs, err) < 0 ? -1 : jobs[0].rc;
 }
 
 }  // namespace ptg

@@ -52,22 +52,6 @@ V3 transform_vertex(V3 local, const PtInstance& inst) {
   return add(rotate_quat(scaled, inst.rotation), load3(inst.translation));
 }
 
-// fn(begin, end) over [0, n) split between up to `threads` threads (one below 64k elements).
-template <typename F>
-void parallel_for(size_t n, unsigned threads, F fn) {
-  if (threads <= 1u || n < 65536u) {
-    fn(size_t{0}, n);
-    return;
-  }
-  const size_t chunk = (n + threads - 1u) / threads;
-  std::vector<std::thread> pool;
-  for (unsigned t = 0; t < threads; ++t) {
-    const size_t a = t * chunk, b = std::min(n, a + chunk);
-    if (a < b) pool.emplace_back(fn, a, b);
-  }
-  for (std::thread& th : pool) th.join();
-}
-
 struct Ref {
   uint32_t orig;
   float c[3], mn[3], mx[3];
@@ -258,78 +242,103 @@ struct MidBuilder {
 
 }  // namespace
 
-bool flatten(const PtSceneView& sc, std::vector<Prim>& prims, std::vector<float>& raw, std::string& err,
-             const float* texcoords, std::vector<uint32_t>* inst_first) {
-  uint64_t cap = 0;
-  for (uint32_t i = 0; i < sc.instance_count; ++i) cap += sc.instances[i].triangle_count;
+// One triangle of an instance (CpuBvhAccelerator.cpp:497-560's primitive record). Returns false, and
+// writes nothing, for a triangle whose indices are out of range (the reference skips it).
+static bool make_prim(const PtSceneView& sc, const PtInstance& inst, uint32_t tri, const float* texcoords, Prim& p,
+               float* r9) {
+  const bool use_local = (inst.flags & 1u) != 0u && inst.local_vertex_count > 0u && inst.local_index_count > 0u;
+  V3 a{0, 0, 0}, b{0, 0, 0}, c{0, 0, 0};
+  float uv[6] = {0, 0, 0, 0, 0, 0};
+  bool loaded = false;
+  if (use_local && sc.local_indices && sc.local_vertices) {
+    const uint32_t local_base = inst.local_first_index + tri * 3u;
+    const uint32_t local_end = inst.local_first_vertex + inst.local_vertex_count;
+    if (static_cast<uint64_t>(local_base) + 2u < sc.local_index_count) {
+      const uint32_t f = inst.local_first_vertex;
+      const uint32_t l0 = f + sc.local_indices[local_base], l1 = f + sc.local_indices[local_base + 1u],
+                     l2 = f + sc.local_indices[local_base + 2u];
+      if (l0 < local_end && l1 < local_end && l2 < local_end && l0 < sc.local_vertex_count &&
+          l1 < sc.local_vertex_count && l2 < sc.local_vertex_count) {
+        a = transform_vertex(load3(sc.local_vertices + 3u * l0), inst);
+        b = transform_vertex(load3(sc.local_vertices + 3u * l1), inst);
+        c = transform_vertex(load3(sc.local_vertices + 3u * l2), inst);
+        loaded = true;
+      }
+    }
+  }
+  if (!loaded) {
+    const uint64_t base = (static_cast<uint64_t>(inst.first_triangle) + tri) * 3u;
+    if (base + 2u >= sc.index_count) return false;
+    const uint32_t i0 = sc.indices[base], i1 = sc.indices[base + 1], i2 = sc.indices[base + 2];
+    if (i0 >= sc.vertex_count || i1 >= sc.vertex_count || i2 >= sc.vertex_count) return false;
+    a = load3(sc.vertices + 3u * i0);
+    b = load3(sc.vertices + 3u * i1);
+    c = load3(sc.vertices + 3u * i2);
+    if (texcoords) {
+      const uint32_t vi[3] = {i0, i1, i2};
+      for (int k = 0; k < 3; ++k) {
+        uv[2 * k] = texcoords[2u * vi[k]];
+        uv[2 * k + 1] = texcoords[2u * vi[k] + 1u];
+      }
+    }
+  }
+  p.material = inst.material_index;
+  const V3 e1 = sub(b, a), e2 = sub(c, a);
+  const V3 n = normalize_accel(cross(e1, e2));
+  p.v0[0] = a.x; p.v0[1] = a.y; p.v0[2] = a.z;
+  p.e1[0] = e1.x; p.e1[1] = e1.y; p.e1[2] = e1.z;
+  p.e2[0] = e2.x; p.e2[1] = e2.y; p.e2[2] = e2.z;
+  p.n[0] = n.x; p.n[1] = n.y; p.n[2] = n.z;
+  std::memcpy(p.uv, uv, sizeof(uv));
+  const float v9[9] = {a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z};
+  std::memcpy(r9, v9, sizeof(v9));
+  return true;
+}
+
+// Every (instance, triangle) slot is made in parallel into its own place; slots the reference skips
+// are then squeezed out in order, so the primitives are numbered as the sequential loop numbers them.
+bool flatten(const PtSceneView& sc, HostVec<Prim>& prims, HostVec<float>& raw, std::string& err,
+             const float* texcoords, std::vector<uint32_t>* inst_first, unsigned threads) {
+  std::vector<uint64_t> first(static_cast<size_t>(sc.instance_count) + 1u, 0u);
+  for (uint32_t i = 0; i < sc.instance_count; ++i) first[i + 1u] = first[i] + sc.instances[i].triangle_count;
+  const uint64_t cap = first.back();
   if (cap >= kMaxPrimitives) {
     err = "scene has more triangles than the device BVH encoding supports";
     return false;
   }
-  prims.clear();
-  raw.clear();
-  prims.reserve(cap);
-  raw.reserve(cap * 9u);
+  prims.resize(cap);
+  raw.resize(9u * cap);
+  HostVec<uint8_t> valid(cap);
+  parallel_for(cap, threads, [&](size_t lo, size_t hi) {
+    // the instance holding slot lo: the last one starting at or before it (empty instances skipped)
+    size_t ii = static_cast<size_t>(std::upper_bound(first.begin(), first.end(), lo) - first.begin()) - 1u;
+    for (size_t g = lo; g < hi; ++g) {
+      while (g >= first[ii + 1u]) ++ii;
+      valid[g] = make_prim(sc, sc.instances[ii], static_cast<uint32_t>(g - first[ii]), texcoords, prims[g], &raw[9u * g])
+                     ? 1u
+                     : 0u;
+    }
+  });
+  size_t w = 0;
   if (inst_first) inst_first->clear();
   for (uint32_t ii = 0; ii < sc.instance_count; ++ii) {
-    if (inst_first) inst_first->push_back(static_cast<uint32_t>(prims.size()));
-    const PtInstance& inst = sc.instances[ii];
-    const bool use_local = (inst.flags & 1u) != 0u && inst.local_vertex_count > 0u && inst.local_index_count > 0u;
-    for (uint32_t tri = 0; tri < inst.triangle_count; ++tri) {
-      V3 a{0, 0, 0}, b{0, 0, 0}, c{0, 0, 0};
-      float uv[6] = {0, 0, 0, 0, 0, 0};
-      bool loaded = false;
-      if (use_local && sc.local_indices && sc.local_vertices) {
-        const uint32_t local_base = inst.local_first_index + tri * 3u;
-        const uint32_t local_end = inst.local_first_vertex + inst.local_vertex_count;
-        if (static_cast<uint64_t>(local_base) + 2u < sc.local_index_count) {
-          const uint32_t f = inst.local_first_vertex;
-          const uint32_t l0 = f + sc.local_indices[local_base], l1 = f + sc.local_indices[local_base + 1u],
-                         l2 = f + sc.local_indices[local_base + 2u];
-          if (l0 < local_end && l1 < local_end && l2 < local_end && l0 < sc.local_vertex_count &&
-              l1 < sc.local_vertex_count && l2 < sc.local_vertex_count) {
-            a = transform_vertex(load3(sc.local_vertices + 3u * l0), inst);
-            b = transform_vertex(load3(sc.local_vertices + 3u * l1), inst);
-            c = transform_vertex(load3(sc.local_vertices + 3u * l2), inst);
-            loaded = true;
-          }
-        }
+    if (inst_first) inst_first->push_back(static_cast<uint32_t>(w));
+    for (uint64_t g = first[ii]; g < first[ii + 1u]; ++g) {
+      if (!valid[g]) continue;
+      if (w != g) {
+        prims[w] = prims[g];
+        std::memcpy(&raw[9u * w], &raw[9u * g], 9 * sizeof(float));
       }
-      if (!loaded) {
-        const uint64_t base = (static_cast<uint64_t>(inst.first_triangle) + tri) * 3u;
-        if (base + 2u >= sc.index_count) continue;
-        const uint32_t i0 = sc.indices[base], i1 = sc.indices[base + 1], i2 = sc.indices[base + 2];
-        if (i0 >= sc.vertex_count || i1 >= sc.vertex_count || i2 >= sc.vertex_count) continue;
-        a = load3(sc.vertices + 3u * i0);
-        b = load3(sc.vertices + 3u * i1);
-        c = load3(sc.vertices + 3u * i2);
-        if (texcoords) {
-          const uint32_t vi[3] = {i0, i1, i2};
-          for (int k = 0; k < 3; ++k) {
-            uv[2 * k] = texcoords[2u * vi[k]];
-            uv[2 * k + 1] = texcoords[2u * vi[k] + 1u];
-          }
-        }
-      }
-      Prim p;
-      p.material = inst.material_index;
-      const V3 e1 = sub(b, a), e2 = sub(c, a);
-      const V3 n = normalize_accel(cross(e1, e2));
-      p.v0[0] = a.x; p.v0[1] = a.y; p.v0[2] = a.z;
-      p.e1[0] = e1.x; p.e1[1] = e1.y; p.e1[2] = e1.z;
-      p.e2[0] = e2.x; p.e2[1] = e2.y; p.e2[2] = e2.z;
-      p.n[0] = n.x; p.n[1] = n.y; p.n[2] = n.z;
-      std::memcpy(p.uv, uv, sizeof(uv));
-      prims.push_back(p);
-      const float r9[9] = {a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z};
-      raw.insert(raw.end(), r9, r9 + 9);
+      ++w;
     }
   }
-  if (inst_first) inst_first->push_back(static_cast<uint32_t>(prims.size()));
+  prims.resize(w);
+  raw.resize(9u * w);
+  if (inst_first) inst_first->push_back(static_cast<uint32_t>(w));
   return true;
 }
 
-void build_sah(Bvh& bvh, const std::vector<float>& raw, uint32_t leaf_size, uint32_t buckets) {
+void build_sah(Bvh& bvh, const HostVec<float>& raw, uint32_t leaf_size, uint32_t buckets) {
   const uint32_t n = static_cast<uint32_t>(bvh.prims.size());
   bvh.nodes.clear();
   bvh.order.clear();
@@ -353,7 +362,7 @@ void build_sah(Bvh& bvh, const std::vector<float>& raw, uint32_t leaf_size, uint
   b.build(0, n, 0);
 }
 
-void build_midpoint(Bvh& bvh, const std::vector<float>& raw, uint32_t leaf_size) {
+void build_midpoint(Bvh& bvh, const HostVec<float>& raw, uint32_t leaf_size) {
   const uint32_t n = static_cast<uint32_t>(bvh.prims.size());
   bvh.nodes.clear();
   bvh.order.clear();

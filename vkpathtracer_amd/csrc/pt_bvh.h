@@ -2,12 +2,52 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "vkpt_hip.h"
 
 namespace ptb {
+
+// An allocator whose resize leaves new elements of trivial types unwritten: the large host arrays of a
+// build are written once, element by element, by several threads, so their pages are first touched
+// in parallel rather than zeroed by one thread.
+template <typename T>
+struct NoInitAlloc : std::allocator<T> {
+  template <typename U>
+  struct rebind {
+    using other = NoInitAlloc<U>;
+  };
+  NoInitAlloc() = default;
+  template <typename U>
+  NoInitAlloc(const NoInitAlloc<U>&) noexcept {}
+  template <typename U>
+  void construct(U*) noexcept {}
+  template <typename U, typename... A>
+  void construct(U* p, A&&... a) {
+    ::new (static_cast<void*>(p)) U(std::forward<A>(a)...);
+  }
+};
+template <typename T>
+using HostVec = std::vector<T, NoInitAlloc<T>>;
+
+// fn(begin, end) over [0, n) split between up to `threads` threads (one below 64k elements).
+template <typename F>
+void parallel_for(size_t n, unsigned threads, F fn) {
+  if (threads <= 1u || n < 65536u) {
+    fn(size_t{0}, n);
+    return;
+  }
+  const size_t chunk = (n + threads - 1u) / threads;
+  std::vector<std::thread> pool;
+  for (unsigned t = 0; t < threads; ++t) {
+    const size_t a = t * chunk, b = std::min(n, a + chunk);
+    if (a < b) pool.emplace_back(fn, a, b);
+  }
+  for (std::thread& th : pool) th.join();
+}
 
 // One flattened triangle (CpuBvhAccelerator::Primitive, CpuBvhAccelerator.cpp:497-505):
 // primitive_index is its position in the flattened array.
@@ -23,9 +63,9 @@ struct Node {
 };
 
 struct Bvh {
-  std::vector<Prim> prims;
-  std::vector<Node> nodes;     // node 0 is the root
-  std::vector<uint32_t> order; // leaf-ordered primitive indices
+  HostVec<Prim> prims;
+  HostVec<Node> nodes;      // node 0 is the root
+  HostVec<uint32_t> order;  // leaf-ordered primitive indices
   uint32_t max_depth = 0;
   uint32_t leaf_count = 0;
 };
@@ -34,13 +74,13 @@ struct Bvh {
 // texcoords (2 floats per vertex of scene.vertices, may be null) give the primitives of indexed
 // geometry their texture coordinates. inst_first (may be null) receives each instance's first
 // primitive and, last, the primitive count (CpuBvhAccelerator's instance ranges). Returns false (with
-// err set) on a malformed snapshot.
-bool flatten(const PtSceneView& scene, std::vector<Prim>& prims, std::vector<float>& raw, std::string& err,
-             const float* texcoords = nullptr, std::vector<uint32_t>* inst_first = nullptr);
+// err set) on a malformed snapshot. `threads` host threads share the triangles.
+bool flatten(const PtSceneView& scene, HostVec<Prim>& prims, HostVec<float>& raw, std::string& err,
+             const float* texcoords = nullptr, std::vector<uint32_t>* inst_first = nullptr, unsigned threads = 1);
 
 // Bucketed SAH (D3D12GpuPathTracer.SceneBvh.cpp:73-238) and midpoint (ParallelBvhBuilder.cpp:61-163).
-void build_sah(Bvh& bvh, const std::vector<float>& raw, uint32_t leaf_size, uint32_t buckets);
-void build_midpoint(Bvh& bvh, const std::vector<float>& raw, uint32_t leaf_size);
+void build_sah(Bvh& bvh, const HostVec<float>& raw, uint32_t leaf_size, uint32_t buckets);
+void build_midpoint(Bvh& bvh, const HostVec<float>& raw, uint32_t leaf_size);
 
 // Device layout.
 //   node (64 B):  float4 {c0.min.xyz, c0.max.x}, {c0.max.yz, c1.min.xy}, {c1.min.z, c1.max.xyz},
@@ -49,11 +89,11 @@ void build_midpoint(Bvh& bvh, const std::vector<float>& raw, uint32_t leaf_size)
 //   tri  (48 B):  float4 {v0.xyz, prim id bits}, {e1.xyz, 0}, {e2.xyz, 0} in leaf order
 //   prim (16 B):  float4 {normal.xyz, material bits} by primitive index
 struct DeviceLayout {
-  std::vector<float> nodes;   // 16 floats per inner node
-  std::vector<float> tris;    // 12 floats per leaf-ordered triangle
-  std::vector<float> prims;   // 4 floats per primitive
-  std::vector<float> prim_uv; // 6 floats per primitive (texture coordinates), empty without texcoords
-  std::vector<uint32_t> prim_tri; // per primitive: its position in leaf order (with prim_uv)
+  HostVec<float> nodes;       // 16 floats per inner node
+  HostVec<float> tris;        // 12 floats per leaf-ordered triangle
+  HostVec<float> prims;       // 4 floats per primitive
+  HostVec<float> prim_uv;     // 6 floats per primitive (texture coordinates), empty without texcoords
+  HostVec<uint32_t> prim_tri; // per primitive: its position in leaf order (with prim_uv)
   float root_box[6] = {0, 0, 0, 0, 0, 0};
   int32_t root_ref = 0;
   uint32_t inner_count = 0;

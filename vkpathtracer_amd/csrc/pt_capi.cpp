@@ -529,31 +529,33 @@ int check_bvh_params(pt_tracer* t, const PtBvhParams* params, PtBvhParams& p) {
   return PT_OK;
 }
 
-// Host build of one scene's tree and device layout (thread-safe per slot).
-// The scene's tree: the SAH tree on the GPU (on `stream`; the host builder's tree node for node)
-// unless the parameters ask for the host or a vertex is not finite, the midpoint tree on the host.
-// `threads` host threads pack the device layout.
-bool build_scene(SceneSlot& s, hipStream_t stream, unsigned threads) {
-  const auto t0 = std::chrono::steady_clock::now();
-  std::vector<float> raw;
+// A scene's build in three steps (each thread-safe per slot): flatten its triangles; its tree, the SAH
+// tree on the GPU (the host builder's tree node for node; several scenes at once in one device build)
+// unless the parameters ask for the host or a vertex is not finite, the midpoint tree on the host;
+// then the device layout, packed by `threads` host threads.
+bool flatten_scene(SceneSlot& s, ptb::HostVec<float>& raw, unsigned threads) {
   s.error.clear();
-  if (!ptb::flatten(s.view, s.bvh.prims, raw, s.error, s.texcoords.empty() ? nullptr : s.texcoords.data(),
-                    &s.inst_first))
-    return false;
-  if (s.bvh_params.builder == PT_BVH_SAH) {
-    int rc = 1;
-    if (!(s.bvh_params.flags & PT_BVH_FLAG_HOST_BUILD))
-      rc = ptg::build_sah_device(stream, s.bvh, raw, s.bvh_params.leaf_size, s.bvh_params.bucket_count, s.error);
-    if (rc < 0) return false;
-    if (rc > 0) ptb::build_sah(s.bvh, raw, s.bvh_params.leaf_size, s.bvh_params.bucket_count);
-    s.built_on_device = rc == 0;
-  } else {
-    ptb::build_midpoint(s.bvh, raw, s.bvh_params.leaf_size);
-    s.built_on_device = false;
+  return ptb::flatten(s.view, s.bvh.prims, raw, s.error, s.texcoords.empty() ? nullptr : s.texcoords.data(),
+                      &s.inst_first, threads);
+}
+
+bool wants_device_build(const SceneSlot& s) {
+  return s.bvh_params.builder == PT_BVH_SAH && !(s.bvh_params.flags & PT_BVH_FLAG_HOST_BUILD);
+}
+
+// `device_rc`: 0 when the device built the tree, 1 when it is still to be built here.
+void finish_scene(SceneSlot& s, const ptb::HostVec<float>& raw, int device_rc, unsigned threads, double shared_ms,
+                  std::chrono::steady_clock::duration own) {
+  const auto t0 = std::chrono::steady_clock::now();
+  if (device_rc != 0) {
+    if (s.bvh_params.builder == PT_BVH_SAH) ptb::build_sah(s.bvh, raw, s.bvh_params.leaf_size, s.bvh_params.bucket_count);
+    else ptb::build_midpoint(s.bvh, raw, s.bvh_params.leaf_size);
   }
+  s.built_on_device = device_rc == 0;
   ptb::pack_device_layout(s.bvh, s.layout, true, threads);
   s.info = PtBuildInfo{};
-  s.info.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  s.info.build_ms = shared_ms + std::chrono::duration<double, std::milli>(own + (std::chrono::steady_clock::now() - t0))
+                                    .count();
   s.info.built = 1;
   s.info.primitive_count = static_cast<uint32_t>(s.bvh.prims.size());
   s.info.node_count = static_cast<uint32_t>(s.bvh.nodes.size());
@@ -561,6 +563,18 @@ bool build_scene(SceneSlot& s, hipStream_t stream, unsigned threads) {
   s.info.max_depth = s.bvh.max_depth;
   s.info.deterministic = 1;
   s.info.device_built = s.built_on_device ? 1u : 0u;
+}
+
+bool build_scene(SceneSlot& s, hipStream_t stream, unsigned threads) {
+  const auto t0 = std::chrono::steady_clock::now();
+  ptb::HostVec<float> raw;
+  if (!flatten_scene(s, raw, threads)) return false;
+  int rc = 1;
+  if (wants_device_build(s)) {
+    rc = ptg::build_sah_device(stream, s.bvh, raw, s.bvh_params.leaf_size, s.bvh_params.bucket_count, s.error);
+    if (rc < 0) return false;
+  }
+  finish_scene(s, raw, rc, threads, 0.0, std::chrono::steady_clock::now() - t0);
   return true;
 }
 
@@ -641,7 +655,8 @@ int upload_pool(pt_tracer* t) {
   if (n_tris >= ptb::kMaxPrimitives || n_nodes >= 0x7fffffffu)
     return fail(t, PT_UNSUPPORTED, "scene pool exceeds 2^26 triangles");
   if (n_texels >= (1ull << 32)) return fail(t, PT_UNSUPPORTED, "texture pool exceeds 2^32 texels");
-  std::vector<float> nodes(16u * n_nodes), tris(12u * n_tris), prims(4u * n_prims), env(n_env);
+  ptb::HostVec<float> nodes(16u * n_nodes), tris(12u * n_tris), prims(4u * n_prims);  // every element written
+  std::vector<float> env(n_env);
   std::vector<float> prim_uv(any_uv ? 6u * n_prims : 0u), texels;
   std::vector<uint32_t> prim_tri(any_uv ? n_prims : 0u, 0u);
   std::vector<ptd::TexRec> texrecs;
@@ -655,7 +670,9 @@ int upload_pool(pt_tracer* t) {
   texels.reserve(3u * n_texels);
   texrecs.reserve(n_tex);
   uint32_t node_base = 0, tri_base = 0, prim_base = 0, env_base = 0, film_base = 0;
-  bool general_boxes = false, any_transmissive = false, textured = false;
+  bool any_transmissive = false, textured = false;
+  std::atomic<bool> general_boxes{false};
+  const unsigned host_threads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
   for (size_t si = 0; si < t->scenes.size(); ++si) {
     const SceneSlot& s = t->scenes[si];
     ptd::SceneRec& R = recs[si];
@@ -668,41 +685,49 @@ int upload_pool(pt_tracer* t) {
       const uint32_t mat_base = static_cast<uint32_t>(mats.size()), light_base = static_cast<uint32_t>(lights.size());
       slot_mat_base[si] = mat_base;
       const uint32_t tex_base = static_cast<uint32_t>(texrecs.size());
-      for (uint32_t i = 0; i < nn; ++i) {
-        const float* src = &L.nodes[16u * i];
-        float* dst = &nodes[16u * (node_base + i)];
-        std::memcpy(dst, src, 12 * sizeof(float));
-        for (int k = 0; k < 12; ++k)
-          if (!std::isfinite(src[k])) general_boxes = true;
-        int32_t r[2];
-        std::memcpy(r, src + 12, sizeof(r));
-        for (int k = 0; k < 2; ++k) r[k] = rebase_ref(r[k], node_base, tri_base);
-        std::memcpy(dst + 12, r, sizeof(r));
-        std::memset(dst + 14, 0, 2 * sizeof(float));
-      }
-      for (uint32_t i = 0; i < nt; ++i) {
-        const float* src = &L.tris[12u * i];
-        float* dst = &tris[12u * (tri_base + i)];
-        std::memcpy(dst, src, 12 * sizeof(float));
-        uint32_t pid;
-        std::memcpy(&pid, src + 3, 4);
-        pid += prim_base;
-        std::memcpy(dst + 3, &pid, 4);
-      }
-      const uint32_t last_mat = static_cast<uint32_t>(s.materials.size()) - 1u;
-      for (uint32_t i = 0; i < np; ++i) {
-        const float* src = &L.prims[4u * i];
-        float* dst = &prims[4u * (prim_base + i)];
-        std::memcpy(dst, src, 3 * sizeof(float));
-        uint32_t m;
-        std::memcpy(&m, src + 3, 4);
-        m = std::min(m, last_mat) + mat_base;  // the shading clamp of a scene-local index, then the pool base
-        std::memcpy(dst + 3, &m, 4);
-        if (any_uv && !s.texcoords.empty()) {
-          std::memcpy(&prim_uv[6u * (prim_base + i)], &L.prim_uv[6u * i], 6 * sizeof(float));
-          prim_tri[prim_base + i] = tri_base + L.prim_tri[i];
+      ptb::parallel_for(nn, host_threads, [&, node_base, tri_base](size_t a, size_t b) {
+        bool general = false;
+        for (size_t i = a; i < b; ++i) {
+          const float* src = &L.nodes[16u * i];
+          float* dst = &nodes[16u * (node_base + i)];
+          std::memcpy(dst, src, 12 * sizeof(float));
+          for (int k = 0; k < 12; ++k)
+            if (!std::isfinite(src[k])) general = true;
+          int32_t r[2];
+          std::memcpy(r, src + 12, sizeof(r));
+          for (int k = 0; k < 2; ++k) r[k] = rebase_ref(r[k], node_base, tri_base);
+          std::memcpy(dst + 12, r, sizeof(r));
+          std::memset(dst + 14, 0, 2 * sizeof(float));
         }
-      }
+        if (general) general_boxes.store(true, std::memory_order_relaxed);
+      });
+      ptb::parallel_for(nt, host_threads, [&, tri_base, prim_base](size_t a, size_t b) {
+        for (size_t i = a; i < b; ++i) {
+          const float* src = &L.tris[12u * i];
+          float* dst = &tris[12u * (tri_base + i)];
+          std::memcpy(dst, src, 12 * sizeof(float));
+          uint32_t pid;
+          std::memcpy(&pid, src + 3, 4);
+          pid += prim_base;
+          std::memcpy(dst + 3, &pid, 4);
+        }
+      });
+      const uint32_t last_mat = static_cast<uint32_t>(s.materials.size()) - 1u;
+      ptb::parallel_for(np, host_threads, [&, tri_base, prim_base, mat_base](size_t a, size_t b) {
+        for (size_t i = a; i < b; ++i) {
+          const float* src = &L.prims[4u * i];
+          float* dst = &prims[4u * (prim_base + i)];
+          std::memcpy(dst, src, 3 * sizeof(float));
+          uint32_t m;
+          std::memcpy(&m, src + 3, 4);
+          m = std::min(m, last_mat) + mat_base;  // the shading clamp of a scene-local index, then the pool base
+          std::memcpy(dst + 3, &m, 4);
+          if (any_uv && !s.texcoords.empty()) {
+            std::memcpy(&prim_uv[6u * (prim_base + i)], &L.prim_uv[6u * i], 6 * sizeof(float));
+            prim_tri[prim_base + i] = tri_base + L.prim_tri[i];
+          }
+        }
+      });
       const std::vector<PtMaterial> pm = pool_materials(s, tex_base);
       for (const PtMaterial& m : pm)
         if (m.base_color_texture_index != 0xFFFFFFFFu) textured = true;
@@ -870,7 +895,7 @@ int upload_pool(pt_tracer* t) {
   S.material_count = static_cast<uint32_t>(mats.size());
   S.light_count = static_cast<uint32_t>(lights.size());
   S.has_transmissive = any_transmissive ? 1u : 0u;
-  S.general_boxes = general_boxes ? 1u : 0u;
+  S.general_boxes = general_boxes.load() ? 1u : 0u;
   S.env_map = t->d_env.ptr;
   S.recs = t->d_recs.ptr;
   S.scene_count = static_cast<uint32_t>(recs.size());
@@ -1334,8 +1359,8 @@ PtInstanceTransformResult plan_refit(const pt_tracer* t, const PtInstanceTransfo
 // grouped by height so that each launch only reads children already refit.
 int prepare_refit(pt_tracer* t) {
   SceneSlot& s = t->scenes[0];
-  std::vector<ptb::Prim> prims;
-  std::vector<float> raw;
+  ptb::HostVec<ptb::Prim> prims;
+  ptb::HostVec<float> raw;
   std::string err;
   if (!ptb::flatten(s.view, prims, raw, err, nullptr, nullptr)) return fail(t, PT_INTERNAL_ERROR, "refit: " + err);
   const size_t n = prims.size();
@@ -1847,7 +1872,7 @@ int pt_hip_intersect(pt_tracer* t, int mode, const float* rays, const float* tma
   if (mode != 0 && mode != 1) return fail(t, PT_INVALID_ARGUMENT, "intersect: mode must be 0 or 1");
   (void)hipSetDevice(t->device);
   if (!t->inst_mat_ready) {  // RayQueryHit::material_index: the instance material baked at build
-    const std::vector<ptb::Prim>& prims = t->scenes[0].bvh.prims;
+    const ptb::HostVec<ptb::Prim>& prims = t->scenes[0].bvh.prims;
     std::vector<uint32_t> m(prims.size());
     for (size_t i = 0; i < prims.size(); ++i) m[i] = prims[i].material;
     PT_HIP(t, hipStreamSynchronize(t->stream), "synchronize");
@@ -2394,8 +2419,8 @@ int pt_batch_build(pt_batch* b, PtBuildInfo* info) {
   if (t->scenes.empty()) return fail(t, PT_NOT_READY, "batch_build: no scenes");
   (void)hipSetDevice(t->device);
   const auto t0 = std::chrono::steady_clock::now();
-  // scene builds in parallel host threads, largest scenes first, each thread with its own stream for
-  // the GPU part
+  // the scenes are flattened in parallel host threads (largest first), their SAH trees built together
+  // in one device build, then the host trees and device layouts again in parallel threads
   std::vector<size_t> order;
   for (size_t i = 0; i < t->scenes.size(); ++i)
     if (t->scenes[i].view_of < 0) order.push_back(i);
@@ -2403,27 +2428,44 @@ int pt_batch_build(pt_batch* b, PtBuildInfo* info) {
             [&](size_t a, size_t c) { return t->scenes[a].indices.size() > t->scenes[c].indices.size(); });
   const unsigned nthreads = std::max(1u, std::min<unsigned>(16u, std::thread::hardware_concurrency()));
   std::vector<char> ok(t->scenes.size(), 1);  // one byte per scene: threads write different elements
-  {
+  std::vector<ptb::HostVec<float>> raws(t->scenes.size());
+  std::vector<std::chrono::steady_clock::duration> own(t->scenes.size());
+  auto each_scene = [&](auto fn) {
     std::vector<std::thread> pool;
     std::atomic<size_t> cursor{0};
     for (unsigned w = 0; w < std::min<size_t>(nthreads, order.size()); ++w)
       pool.emplace_back([&] {
-        hipStream_t stream = nullptr;
-        if (hipSetDevice(t->device) != hipSuccess || hipStreamCreate(&stream) != hipSuccess) {
-          for (size_t k = cursor++; k < order.size(); k = cursor++) {
-            t->scenes[order[k]].error = "could not create a stream for the scene build";
-            ok[order[k]] = 0;
-          }
-          return;
+        for (size_t k = cursor++; k < order.size(); k = cursor++) {
+          const auto s0 = std::chrono::steady_clock::now();
+          fn(order[k]);
+          own[order[k]] += std::chrono::steady_clock::now() - s0;
         }
-        for (size_t k = cursor++; k < order.size(); k = cursor++)
-          ok[order[k]] = build_scene(t->scenes[order[k]], stream, 1u);
-        (void)hipStreamDestroy(stream);
       });
     for (std::thread& th : pool) th.join();
-  }
+  };
+  each_scene([&](size_t i) { ok[i] = flatten_scene(t->scenes[i], raws[i], 1u); });
   for (size_t i = 0; i < ok.size(); ++i)
     if (!ok[i]) return fail(t, PT_INVALID_ARGUMENT, "scene " + std::to_string(i) + ": " + t->scenes[i].error);
+  std::vector<ptg::DeviceBuild> jobs;
+  std::vector<size_t> job_scene;
+  for (size_t i : order) {
+    SceneSlot& s = t->scenes[i];
+    if (!wants_device_build(s)) continue;
+    jobs.push_back(ptg::DeviceBuild{&s.bvh, &raws[i], s.bvh_params.leaf_size, s.bvh_params.bucket_count, 1});
+    job_scene.push_back(i);
+  }
+  std::vector<int> device_rc(t->scenes.size(), 1);
+  const auto d0 = std::chrono::steady_clock::now();
+  if (!jobs.empty()) {
+    std::string err;
+    if (ptg::build_sah_device_many(t->stream, jobs, err) < 0) return fail(t, PT_INTERNAL_ERROR, err);
+    for (size_t j = 0; j < jobs.size(); ++j) device_rc[job_scene[j]] = jobs[j].rc;
+  }
+  const double device_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - d0).count();
+  each_scene([&](size_t i) {
+    finish_scene(t->scenes[i], raws[i], device_rc[i], 1u, device_ms, own[i]);
+    ptb::HostVec<float>().swap(raws[i]);
+  });
   int rc = upload_pool(t);
   if (rc != PT_OK) return rc;
   rc = alloc_film(t, t->film_pixels);
@@ -2667,9 +2709,10 @@ int bvh_build_export(const PtSceneView* scene, const PtBvhParams* params, bool d
   if (p.leaf_size > ptb::kMaxLeafSize || (p.builder != PT_BVH_SAH && p.builder != PT_BVH_MIDPOINT))
     return PT_INVALID_ARGUMENT;
   ptb::Bvh bvh;
-  std::vector<float> raw;
+  ptb::HostVec<float> raw;
   std::string err;
-  if (!ptb::flatten(*scene, bvh.prims, raw, err)) return PT_INVALID_ARGUMENT;
+  const unsigned threads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (!ptb::flatten(*scene, bvh.prims, raw, err, nullptr, nullptr, threads)) return PT_INVALID_ARGUMENT;
   int rc = 1;
   if (device && p.builder == PT_BVH_SAH) {
     hipStream_t stream = nullptr;
