@@ -13,10 +13,13 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
 #include "pt_build.h"
+#include "pt_scan.h"
 
 namespace ptg {
 namespace {
@@ -64,8 +67,6 @@ enum Counter : int { kNextBig = 0, kNextHuge = 1, kInner = 2, kCounters = 4 };
 // position i is node level_base + i); an exclusive prefix count of its inner segments (those above
 // their scene's leaf size) gives the segment at i its rank r among them, its children the nodes
 // next_base + 2r and next_base + 2r + 1, and their places 2r and 2r + 1 in the next level's list.
-// The count runs in chunks of kScanChunk segments: rank = chunk-local rank + the chunk's offset.
-constexpr uint32_t kScanChunk = 1024;
 
 // std::min / std::max as the host builder writes them
 __device__ __forceinline__ float smin(float a, float b) { return (b < a) ? b : a; }
@@ -198,71 +199,16 @@ __device__ __forceinline__ void scan_splits(const uint32_t* bin_n, const uint32_
   }
 }
 
-// Chunk-local exclusive ranks of the inner segments (256 threads, four segments each) and each chunk's
-// inner count.
+// Chunk-local exclusive ranks of the inner segments and each chunk's inner count.
 __global__ __launch_bounds__(256) void k_seg_rank(const Seg* segs, uint32_t m, const SceneTab* tab,
                                                   uint32_t* rank_local, uint32_t* chunk_sum) {
-  __shared__ uint32_t wsum[4];
-  const uint32_t base = blockIdx.x * kScanChunk + threadIdx.x * 4u;
-  uint32_t f[4], t = 0;
-  for (uint32_t k = 0; k < 4u; ++k) {
-    const uint32_t i = base + k;
-    f[k] = 0u;
-    if (i < m) {
-      const Seg sg = segs[i];
-      f[k] = sg.count > tab[sg.scene].leaf ? 1u : 0u;
-    }
-    t += f[k];
-  }
-  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
-  uint32_t x = t;  // inclusive scan over the wave
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const uint32_t y = __shfl_up(x, off);
-    if (lane >= static_cast<uint32_t>(off)) x += y;
-  }
-  if (lane == 63u) wsum[w] = x;
-  __syncthreads();
-  uint32_t run = x - t, tot = 0;
-  for (uint32_t j = 0; j < 4u; ++j) {
-    if (j < w) run += wsum[j];
-    tot += wsum[j];
-  }
-  for (uint32_t k = 0; k < 4u; ++k) {
-    const uint32_t i = base + k;
-    if (i < m) rank_local[i] = run;
-    run += f[k];
-  }
-  if (threadIdx.x == 0u) chunk_sum[blockIdx.x] = tot;
-}
-
-// The chunks' counts become exclusive offsets in place (one workgroup); the level's inner count goes to
-// counters[kInner].
-__global__ __launch_bounds__(1024) void k_chunk_offsets(uint32_t* chunk_sum, uint32_t chunks, uint32_t* counters) {
-  __shared__ uint32_t wsum[16];
-  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
-  uint32_t carry = 0;
-  for (uint32_t c0 = 0; c0 < chunks; c0 += 1024u) {
-    const uint32_t i = c0 + threadIdx.x;
-    const uint32_t t = i < chunks ? chunk_sum[i] : 0u;
-    uint32_t x = t;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const uint32_t y = __shfl_up(x, off);
-      if (lane >= static_cast<uint32_t>(off)) x += y;
-    }
-    if (lane == 63u) wsum[w] = x;
-    __syncthreads();
-    uint32_t pre = x - t, tot = 0;
-    for (uint32_t j = 0; j < 16u; ++j) {
-      if (j < w) pre += wsum[j];
-      tot += wsum[j];
-    }
-    if (i < chunks) chunk_sum[i] = carry + pre;
-    carry += tot;
-    __syncthreads();
-  }
-  if (threadIdx.x == 0u) counters[kInner] = carry;
+  chunk_rank(
+      m,
+      [&](uint32_t i) {
+        const Seg sg = segs[i];
+        return sg.count > tab[sg.scene].leaf ? 1u : 0u;
+      },
+      rank_local, chunk_sum);
 }
 
 // Where an inner segment's children go (ranks as above).
@@ -279,8 +225,7 @@ struct Numbering {
 // Writes the inner node sg (its box in `node`) and its children [start, start + mid) and the rest.
 __device__ void emit_inner(const Numbering& N, const Seg& sg, DevNode node, uint32_t mid, DevNode* nodes,
                            uint32_t* node_scene) {
-  const uint32_t i = sg.node - N.level_base;
-  const uint32_t r = N.rank_local[i] + N.chunk_off[i / kScanChunk];
+  const uint32_t r = rank_of(N.rank_local, N.chunk_off, sg.node - N.level_base);
   const uint32_t base = N.next_base + 2u * r;
   node.left = static_cast<int32_t>(base);
   node.right = static_cast<int32_t>(base + 1u);
@@ -755,13 +700,26 @@ struct DevBuf {
     s = stream;
     return hipMallocAsync(reinterpret_cast<void**>(&p), std::max<size_t>(n, 1) * sizeof(T), stream);
   }
+  // hands the buffer to a shared owner that releases it on the same stream
+  std::shared_ptr<void> share() {
+    T* q = p;
+    p = nullptr;
+    const hipStream_t st = s;
+    return std::shared_ptr<void>(q, [st](void* x) { (void)hipFreeAsync(x, st); });
+  }
 };
+
+// Each scene's root record (its node 0 in the renumbered order).
+__global__ void k_gather_roots(const DevNode* out, const SceneOut* sout, DevNode* roots, uint32_t scenes) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < scenes) roots[s] = out[sout[s].node_base];
+}
 
 // One level-synchronous build of several scenes (a forest): their references are concatenated and
 // every scene's root is a segment of the first level, so the scenes share each level's launches and
 // its one synchronisation.
 int build_forest(hipStream_t stream, std::vector<DeviceBuild>& jobs, const std::vector<uint32_t>& sel,
-                 std::string& err) {
+                 std::string& err, std::vector<DeviceForest>* keep) {
   const uint32_t S = static_cast<uint32_t>(sel.size());
   std::vector<SceneTab> tab(S);
   uint32_t n = 0;
@@ -783,10 +741,18 @@ int build_forest(hipStream_t stream, std::vector<DeviceBuild>& jobs, const std::
   DevBuf<DevNode> d_nodes, d_pre_nodes;
   DevBuf<SceneTab> d_tab;
   DevBuf<SceneOut> d_sout;
+  DevBuf<uint32_t> d_mat;   // kept trees: the primitives' instance materials
+  DevBuf<DevNode> d_roots;  // kept trees: each scene's root record
   auto check = [&](hipError_t e, const char* what) {
     if (e != hipSuccess) err = std::string("device BVH build: ") + what + ": " + hipGetErrorString(e);
     return e == hipSuccess;
   };
+  if (keep)
+    for (uint32_t s = 0; s < S; ++s)
+      if (!jobs[sel[s]].material || jobs[sel[s]].material->size() != jobs[sel[s]].bvh->prims.size()) {
+        err = "device BVH build: a kept tree needs its primitives' materials";
+        return -1;
+      }
   // segments of one level are disjoint: at most n of them, n / kBigSegment above kBigSegment and
   // n / kHugeSegment above kHugeSegment
   const uint32_t max_big = n / kBigSegment + 1u, max_huge = n / kHugeSegment + 1u;
@@ -815,11 +781,19 @@ int build_forest(hipStream_t stream, std::vector<DeviceBuild>& jobs, const std::
     }
     R[b].orig = d_orig[b].p;
   }
+  if (keep && (!check(d_mat.alloc(n, stream), "allocate") || !check(d_roots.alloc(S, stream), "allocate")))
+    return -1;
   for (uint32_t s = 0; s < S; ++s) {
     const ptb::HostVec<float>& raw = *jobs[sel[s]].raw;
     if (!check(hipMemcpyAsync(d_raw.p + 9u * static_cast<size_t>(tab[s].ref_base), raw.data(),
                               raw.size() * sizeof(float), hipMemcpyHostToDevice, stream), "upload"))
       return -1;
+    if (keep) {
+      const ptb::HostVec<uint32_t>& mat = *jobs[sel[s]].material;
+      if (!check(hipMemcpyAsync(d_mat.p + tab[s].ref_base, mat.data(), mat.size() * sizeof(uint32_t),
+                                hipMemcpyHostToDevice, stream), "upload"))
+        return -1;
+    }
   }
   hipLaunchKernelGGL(k_build_refs, dim3((n + 255u) / 256u), dim3(256), 0, stream, d_raw.p, n, R[0]);
   // level 0: the roots (scene s is node s)
@@ -852,7 +826,8 @@ int build_forest(hipStream_t stream, std::vector<DeviceBuild>& jobs, const std::
     const uint32_t scan_chunks = (m + kScanChunk - 1u) / kScanChunk;
     hipLaunchKernelGGL(k_seg_rank, dim3(scan_chunks), dim3(256), 0, stream, d_all[cur].p, m, d_tab.p, d_rank.p,
                        d_chunk.p);
-    hipLaunchKernelGGL(k_chunk_offsets, dim3(1), dim3(1024), 0, stream, d_chunk.p, scan_chunks, d_counters.p);
+    hipLaunchKernelGGL(k_chunk_offsets, dim3(1), dim3(1024), 0, stream, d_chunk.p, scan_chunks,
+                       d_counters.p + kInner);
     const Numbering N{d_rank.p, d_chunk.p, level_base, level_base + m, d_all[1 - cur].p, d_big[1 - cur].p,
                       d_huge[1 - cur].p, d_counters.p};
     if (nhuge) {
@@ -911,10 +886,37 @@ int build_forest(hipStream_t stream, std::vector<DeviceBuild>& jobs, const std::
   hipLaunchKernelGGL(k_scatter_preorder, dim3((node_count + 255u) / 256u), dim3(256), 0, stream, d_nodes.p,
                      d_node_scene.p, pre, d_sout.p, out, node_count);
   std::vector<SceneOut> sout(S);
+  std::vector<DevNode> root_nodes(keep ? S : 0u);
+  if (keep) hipLaunchKernelGGL(k_gather_roots, dim3((S + 63u) / 64u), dim3(64), 0, stream, out, d_sout.p, d_roots.p, S);
   if (!check(hipGetLastError(), "launch") ||
       !check(hipMemcpyAsync(sout.data(), d_sout.p, S * sizeof(SceneOut), hipMemcpyDeviceToHost, stream), "read") ||
+      (keep && !check(hipMemcpyAsync(root_nodes.data(), d_roots.p, S * sizeof(DevNode), hipMemcpyDeviceToHost, stream),
+                      "read")) ||
       !check(hipStreamSynchronize(stream), "synchronize"))
     return -1;
+  if (keep) {  // the trees stay where they are
+    DeviceForest f;
+    f.nodes = reinterpret_cast<const ptb::Node*>(d_pre_nodes.p);
+    f.order = d_order.p;
+    f.raw = d_raw.p;
+    f.material = d_mat.p;
+    f.node_total = node_count;
+    f.ref_total = n;
+    f.storage = {d_pre_nodes.share(), d_order.share(), d_raw.share(), d_mat.share()};
+    for (uint32_t s = 0; s < S; ++s) {
+      DeviceBuild& d = jobs[sel[s]];
+      ForestScene fs{sout[s].node_base, sout[s].node_count, tab[s].ref_base,
+                     static_cast<uint32_t>(d.bvh->prims.size()), ptb::Node{}};
+      std::memcpy(&fs.root, &root_nodes[s], sizeof(ptb::Node));
+      f.scenes.push_back(fs);
+      d.forest = static_cast<int>(keep->size());
+      d.scene = s;
+      d.bvh->max_depth = sout[s].max_depth;
+      d.bvh->leaf_count = sout[s].leaf_count;
+    }
+    keep->push_back(std::move(f));
+    return 0;
+  }
   for (uint32_t s = 0; s < S; ++s) {
     ptb::Bvh& bvh = *jobs[sel[s]].bvh;
     const size_t prims = bvh.prims.size();
@@ -933,7 +935,8 @@ int build_forest(hipStream_t stream, std::vector<DeviceBuild>& jobs, const std::
 
 }  // namespace
 
-int build_sah_device_many(hipStream_t stream, std::vector<DeviceBuild>& jobs, std::string& err) {
+int build_sah_device_many(hipStream_t stream, std::vector<DeviceBuild>& jobs, std::string& err,
+                          std::vector<DeviceForest>* keep) {
   std::vector<uint32_t> sel;
   uint64_t refs = 0;
   for (uint32_t j = 0; j < jobs.size(); ++j) {
@@ -947,26 +950,29 @@ int build_sah_device_many(hipStream_t stream, std::vector<DeviceBuild>& jobs, st
     if (bvh.prims.empty()) continue;
     // NaN boxes make min/max order-dependent, and coordinates beyond the builder's 1e30 sentinels can
     // overflow a centroid: the host builds such scenes
-    bool finite = true;
-    for (float x : *d.raw)
-      if (!(std::fabs(x) <= 1e30f)) {
-        finite = false;
-        break;
-      }
+    bool finite = d.bounded == 1;
+    if (d.bounded < 0) {
+      finite = true;
+      for (float x : *d.raw)
+        if (!(std::fabs(x) <= 1e30f)) {
+          finite = false;
+          break;
+        }
+    }
     if (!finite) {
       d.rc = 1;
       continue;
     }
     // one forest holds at most kMaxForestRefs references (device scratch of about 370 bytes each)
     if (!sel.empty() && refs + bvh.prims.size() > kMaxForestRefs) {
-      if (build_forest(stream, jobs, sel, err) < 0) return -1;
+      if (build_forest(stream, jobs, sel, err, keep) < 0) return -1;
       sel.clear();
       refs = 0;
     }
     sel.push_back(j);
     refs += bvh.prims.size();
   }
-  if (!sel.empty() && build_forest(stream, jobs, sel, err) < 0) return -1;
+  if (!sel.empty() && build_forest(stream, jobs, sel, err, keep) < 0) return -1;
   return 0;
 }
 

@@ -6,6 +6,7 @@
 #include "pt_bvh.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -298,7 +299,8 @@ static bool make_prim(const PtSceneView& sc, const PtInstance& inst, uint32_t tr
 // Every (instance, triangle) slot is made in parallel into its own place; slots the reference skips
 // are then squeezed out in order, so the primitives are numbered as the sequential loop numbers them.
 bool flatten(const PtSceneView& sc, HostVec<Prim>& prims, HostVec<float>& raw, std::string& err,
-             const float* texcoords, std::vector<uint32_t>* inst_first, unsigned threads) {
+             const float* texcoords, std::vector<uint32_t>* inst_first, unsigned threads, HostVec<uint32_t>* materials,
+             bool* bounded) {
   std::vector<uint64_t> first(static_cast<size_t>(sc.instance_count) + 1u, 0u);
   for (uint32_t i = 0; i < sc.instance_count; ++i) first[i + 1u] = first[i] + sc.instances[i].triangle_count;
   const uint64_t cap = first.back();
@@ -308,17 +310,25 @@ bool flatten(const PtSceneView& sc, HostVec<Prim>& prims, HostVec<float>& raw, s
   }
   prims.resize(cap);
   raw.resize(9u * cap);
+  if (materials) materials->resize(cap);
   HostVec<uint8_t> valid(cap);
+  std::atomic<bool> unbounded{false};
   parallel_for(cap, threads, [&](size_t lo, size_t hi) {
     // the instance holding slot lo: the last one starting at or before it (empty instances skipped)
     size_t ii = static_cast<size_t>(std::upper_bound(first.begin(), first.end(), lo) - first.begin()) - 1u;
+    bool big = false;
     for (size_t g = lo; g < hi; ++g) {
       while (g >= first[ii + 1u]) ++ii;
-      valid[g] = make_prim(sc, sc.instances[ii], static_cast<uint32_t>(g - first[ii]), texcoords, prims[g], &raw[9u * g])
-                     ? 1u
-                     : 0u;
+      const bool ok = make_prim(sc, sc.instances[ii], static_cast<uint32_t>(g - first[ii]), texcoords, prims[g],
+                                &raw[9u * g]);
+      valid[g] = ok ? 1u : 0u;
+      if (!ok) continue;
+      if (materials) (*materials)[g] = prims[g].material;
+      for (int k = 0; k < 9; ++k) big = big || !(std::fabs(raw[9u * g + k]) <= 1e30f);
     }
+    if (big) unbounded.store(true, std::memory_order_relaxed);
   });
+  if (bounded) *bounded = !unbounded.load();
   size_t w = 0;
   if (inst_first) inst_first->clear();
   for (uint32_t ii = 0; ii < sc.instance_count; ++ii) {
@@ -328,12 +338,14 @@ bool flatten(const PtSceneView& sc, HostVec<Prim>& prims, HostVec<float>& raw, s
       if (w != g) {
         prims[w] = prims[g];
         std::memcpy(&raw[9u * w], &raw[9u * g], 9 * sizeof(float));
+        if (materials) (*materials)[w] = (*materials)[g];
       }
       ++w;
     }
   }
   prims.resize(w);
   raw.resize(9u * w);
+  if (materials) materials->resize(w);
   if (inst_first) inst_first->push_back(static_cast<uint32_t>(w));
   return true;
 }

@@ -74,9 +74,12 @@ struct Bvh {
 // texcoords (2 floats per vertex of scene.vertices, may be null) give the primitives of indexed
 // geometry their texture coordinates. inst_first (may be null) receives each instance's first
 // primitive and, last, the primitive count (CpuBvhAccelerator's instance ranges). Returns false (with
-// err set) on a malformed snapshot. `threads` host threads share the triangles.
+// err set) on a malformed snapshot. `threads` host threads share the triangles. materials (may be null)
+// receives each primitive's instance material; bounded (may be null) whether every vertex coordinate is
+// finite and within 1e30 (what the device build needs).
 bool flatten(const PtSceneView& scene, HostVec<Prim>& prims, HostVec<float>& raw, std::string& err,
-             const float* texcoords = nullptr, std::vector<uint32_t>* inst_first = nullptr, unsigned threads = 1);
+             const float* texcoords = nullptr, std::vector<uint32_t>* inst_first = nullptr, unsigned threads = 1,
+             HostVec<uint32_t>* materials = nullptr, bool* bounded = nullptr);
 
 // Bucketed SAH (D3D12GpuPathTracer.SceneBvh.cpp:73-238) and midpoint (ParallelBvhBuilder.cpp:61-163).
 void build_sah(Bvh& bvh, const HostVec<float>& raw, uint32_t leaf_size, uint32_t buckets);

@@ -84,7 +84,14 @@ struct SceneSlot {
   PtBvhParams bvh_params{PT_BVH_SAH, 4u, 8u, 0u};
   ptb::Bvh bvh;
   bool built_on_device = false;  // the last build ran on the GPU
-  ptb::DeviceLayout layout;
+  ptb::DeviceLayout layout;      // the host layout of a tree built on the host (empty for device trees)
+  // the tree as the pool sees it: inner nodes, root reference (scene-local) and root box; during a
+  // build, where the device kept the tree (forest, scene; forest -1: the host layout holds it)
+  uint32_t inner_count = 0;
+  int32_t root_ref = 0;
+  float root_box[6] = {0, 0, 0, 0, 0, 0};
+  int forest = -1;
+  uint32_t forest_scene = 0;
   std::vector<uint32_t> inst_first;  // per instance its first primitive, then the primitive count
   PtBuildInfo info{};
   std::string error;
@@ -492,6 +499,9 @@ void copy_scene(SceneSlot& s, const PtSceneView* sc) {
   s.view.light_count = static_cast<uint32_t>(s.lights.size());
   s.layout = ptb::DeviceLayout{};
   s.bvh = ptb::Bvh{};
+  s.inner_count = 0;
+  s.root_ref = 0;
+  s.forest = -1;
   s.texcoords.clear();
   s.texels.clear();
   s.textures.clear();
@@ -530,51 +540,98 @@ int check_bvh_params(pt_tracer* t, const PtBvhParams* params, PtBvhParams& p) {
 }
 
 // A scene's build in three steps (each thread-safe per slot): flatten its triangles; its tree, the SAH
-// tree on the GPU (the host builder's tree node for node; several scenes at once in one device build)
-// unless the parameters ask for the host or a vertex is not finite, the midpoint tree on the host;
-// then the device layout, packed by `threads` host threads.
-bool flatten_scene(SceneSlot& s, ptb::HostVec<float>& raw, unsigned threads) {
+// tree on the GPU (the host builder's tree node for node; several scenes at once in one device build,
+// kept on the device for the pool) unless the parameters ask for the host or a vertex is not finite, the
+// midpoint tree on the host; then, for a tree built on the host, the device layout, packed by `threads`
+// host threads.
+struct FlatScene {
+  ptb::HostVec<float> raw;
+  ptb::HostVec<uint32_t> material;
+  bool bounded = false;
+};
+
+bool flatten_scene(SceneSlot& s, FlatScene& f, unsigned threads) {
   s.error.clear();
-  return ptb::flatten(s.view, s.bvh.prims, raw, s.error, s.texcoords.empty() ? nullptr : s.texcoords.data(),
-                      &s.inst_first, threads);
+  return ptb::flatten(s.view, s.bvh.prims, f.raw, s.error, s.texcoords.empty() ? nullptr : s.texcoords.data(),
+                      &s.inst_first, threads, &f.material, &f.bounded);
 }
 
 bool wants_device_build(const SceneSlot& s) {
   return s.bvh_params.builder == PT_BVH_SAH && !(s.bvh_params.flags & PT_BVH_FLAG_HOST_BUILD);
 }
 
-// `device_rc`: 0 when the device built the tree, 1 when it is still to be built here.
-void finish_scene(SceneSlot& s, const ptb::HostVec<float>& raw, int device_rc, unsigned threads, double shared_ms,
+ptg::DeviceBuild device_job(SceneSlot& s, const FlatScene& f) {
+  ptg::DeviceBuild d{&s.bvh, &f.raw, s.bvh_params.leaf_size, s.bvh_params.bucket_count, 1};
+  d.material = &f.material;
+  d.bounded = f.bounded ? 1 : 0;
+  return d;
+}
+
+// `kept`: the device built the tree and kept it (its forest scene); otherwise it is built here.
+void finish_scene(SceneSlot& s, const FlatScene& f, const ptg::ForestScene* kept, unsigned threads, double shared_ms,
                   std::chrono::steady_clock::duration own) {
   const auto t0 = std::chrono::steady_clock::now();
-  if (device_rc != 0) {
-    if (s.bvh_params.builder == PT_BVH_SAH) ptb::build_sah(s.bvh, raw, s.bvh_params.leaf_size, s.bvh_params.bucket_count);
-    else ptb::build_midpoint(s.bvh, raw, s.bvh_params.leaf_size);
+  s.built_on_device = kept != nullptr;
+  uint32_t node_count = 0;
+  if (kept) {
+    s.layout = ptb::DeviceLayout{};
+    node_count = kept->node_count;
+    s.inner_count = (node_count - 1u) / 2u;  // every inner node has two children
+    const ptb::Node& r = kept->root;
+    s.root_ref = r.count > 0 ? ~static_cast<int32_t>(static_cast<uint32_t>(r.count)) : 0;  // leaf: first 0
+    for (int k = 0; k < 3; ++k) {
+      s.root_box[k] = r.mn[k];
+      s.root_box[3 + k] = r.mx[k];
+    }
+  } else {
+    s.forest = -1;
+    if (s.bvh_params.builder == PT_BVH_SAH) ptb::build_sah(s.bvh, f.raw, s.bvh_params.leaf_size, s.bvh_params.bucket_count);
+    else ptb::build_midpoint(s.bvh, f.raw, s.bvh_params.leaf_size);
+    ptb::pack_device_layout(s.bvh, s.layout, true, threads);
+    node_count = static_cast<uint32_t>(s.bvh.nodes.size());
+    s.inner_count = s.layout.inner_count;
+    s.root_ref = s.layout.root_ref;
+    std::memcpy(s.root_box, s.layout.root_box, sizeof(s.root_box));
   }
-  s.built_on_device = device_rc == 0;
-  ptb::pack_device_layout(s.bvh, s.layout, true, threads);
   s.info = PtBuildInfo{};
   s.info.build_ms = shared_ms + std::chrono::duration<double, std::milli>(own + (std::chrono::steady_clock::now() - t0))
                                     .count();
   s.info.built = 1;
   s.info.primitive_count = static_cast<uint32_t>(s.bvh.prims.size());
-  s.info.node_count = static_cast<uint32_t>(s.bvh.nodes.size());
+  s.info.node_count = node_count;
   s.info.leaf_count = s.bvh.leaf_count;
   s.info.max_depth = s.bvh.max_depth;
   s.info.deterministic = 1;
   s.info.device_built = s.built_on_device ? 1u : 0u;
 }
 
-bool build_scene(SceneSlot& s, hipStream_t stream, unsigned threads) {
-  const auto t0 = std::chrono::steady_clock::now();
-  ptb::HostVec<float> raw;
-  if (!flatten_scene(s, raw, threads)) return false;
-  int rc = 1;
-  if (wants_device_build(s)) {
-    rc = ptg::build_sah_device(stream, s.bvh, raw, s.bvh_params.leaf_size, s.bvh_params.bucket_count, s.error);
-    if (rc < 0) return false;
+// Builds jobs[i] for scenes[i] (those that want the device) into `forests`; records where each landed.
+bool build_on_device(hipStream_t stream, std::vector<SceneSlot*>& scenes, std::vector<ptg::DeviceBuild>& jobs,
+                     std::vector<ptg::DeviceForest>& forests, std::string& err) {
+  if (jobs.empty()) return true;
+  if (ptg::build_sah_device_many(stream, jobs, err, &forests) < 0) return false;
+  for (size_t j = 0; j < jobs.size(); ++j) {
+    scenes[j]->forest = jobs[j].rc == 0 ? jobs[j].forest : -1;
+    scenes[j]->forest_scene = jobs[j].scene;
   }
-  finish_scene(s, raw, rc, threads, 0.0, std::chrono::steady_clock::now() - t0);
+  return true;
+}
+
+const ptg::ForestScene* kept_tree(const SceneSlot& s, const std::vector<ptg::DeviceForest>& forests) {
+  return s.forest >= 0 ? &forests[static_cast<size_t>(s.forest)].scenes[s.forest_scene] : nullptr;
+}
+
+bool build_scene(SceneSlot& s, hipStream_t stream, unsigned threads, std::vector<ptg::DeviceForest>& forests) {
+  const auto t0 = std::chrono::steady_clock::now();
+  FlatScene f;
+  s.forest = -1;
+  if (!flatten_scene(s, f, threads)) return false;
+  if (wants_device_build(s)) {
+    std::vector<SceneSlot*> one{&s};
+    std::vector<ptg::DeviceBuild> jobs{device_job(s, f)};
+    if (!build_on_device(stream, one, jobs, forests, s.error)) return false;
+  }
+  finish_scene(s, f, kept_tree(s, forests), threads, 0.0, std::chrono::steady_clock::now() - t0);
   return true;
 }
 
@@ -631,7 +688,7 @@ std::vector<PtMaterial> pool_materials(const SceneSlot& s, uint32_t tex_base) {
 // rebasing node, triangle, primitive, material and texture references, and writes the scene
 // records. Films are laid out scene after scene (film_base); a tracer's single scene starts at
 // pixel 0. A view shares its scene's pool entries and has its own camera and film.
-int upload_pool(pt_tracer* t) {
+int upload_pool(pt_tracer* t, const std::vector<ptg::DeviceForest>& forests) {
   t->inst_mat_ready = false;
   t->refit_ready = false;
   PT_HIP(t, hipStreamSynchronize(t->stream), "synchronize");
@@ -641,9 +698,9 @@ int upload_pool(pt_tracer* t) {
   bool any_uv = false;
   for (const SceneSlot& s : t->scenes) {
     if (s.view_of >= 0) continue;
-    n_nodes += s.layout.nodes.size() / 16u;
-    n_tris += s.layout.tris.size() / 12u;
-    n_prims += s.layout.prims.size() / 4u;
+    n_nodes += s.inner_count;
+    n_tris += s.info.primitive_count;
+    n_prims += s.info.primitive_count;
     n_mats += s.materials.size();
     n_lights += s.lights.size();
     n_env += s.environment_map.size();
@@ -655,16 +712,28 @@ int upload_pool(pt_tracer* t) {
   if (n_tris >= ptb::kMaxPrimitives || n_nodes >= 0x7fffffffu)
     return fail(t, PT_UNSUPPORTED, "scene pool exceeds 2^26 triangles");
   if (n_texels >= (1ull << 32)) return fail(t, PT_UNSUPPORTED, "texture pool exceeds 2^32 texels");
-  ptb::HostVec<float> nodes(16u * n_nodes), tris(12u * n_tris), prims(4u * n_prims);  // every element written
-  std::vector<float> env(n_env);
-  std::vector<float> prim_uv(any_uv ? 6u * n_prims : 0u), texels;
-  std::vector<uint32_t> prim_tri(any_uv ? n_prims : 0u, 0u);
+  // The trees' arrays are written in place: trees the device build kept are packed on the device,
+  // host-built trees uploaded from their host layouts; the hot copies go after the nodes.
+  PT_HIP(t, t->d_nodes.ensure((n_nodes + ptd::kHotNodes) * 4u), "allocate BVH nodes");
+  PT_HIP(t, t->d_tris.ensure(n_tris * 3u), "allocate triangles");
+  PT_HIP(t, t->d_prims.ensure(n_prims), "allocate primitives");
+  PT_HIP(t, t->d_hot_src.ensure(std::max<size_t>(ptd::kHotNodes, 1)), "allocate hot node table");
+  if (any_uv) {  // scenes without texture coordinates keep zeros here
+    PT_HIP(t, t->d_prim_uv.ensure(6u * n_prims), "allocate texture coordinates");
+    PT_HIP(t, t->d_prim_tri.ensure(n_prims), "allocate triangle map");
+    PT_HIP(t, hipMemsetAsync(t->d_prim_uv.ptr, 0, 6u * n_prims * sizeof(float), t->stream), "clear");
+    PT_HIP(t, hipMemsetAsync(t->d_prim_tri.ptr, 0, n_prims * sizeof(uint32_t), t->stream), "clear");
+    PT_HIP(t, hipStreamSynchronize(t->stream), "synchronize");  // before the host uploads below
+  }
+  std::vector<float> env(n_env), texels;
   std::vector<ptd::TexRec> texrecs;
   std::vector<PtMaterial> mats;
   std::vector<PtLight> lights;
   std::vector<ptd::SceneRec> recs(t->scenes.size());
   std::vector<ptd::DevSdf> sdfs;
   std::vector<uint32_t> slot_mat_base(t->scenes.size(), 0u);
+  std::vector<std::vector<ptg::PoolDest>> dests(forests.size());
+  for (size_t f = 0; f < forests.size(); ++f) dests[f].resize(forests[f].scenes.size());
   mats.reserve(n_mats);
   lights.reserve(n_lights);
   texels.reserve(3u * n_texels);
@@ -679,55 +748,79 @@ int upload_pool(pt_tracer* t) {
     if (s.view_of >= 0) {  // the scene's record, with this view's image, film and camera
       R = recs[static_cast<size_t>(s.view_of)];
     } else {
-      const ptb::DeviceLayout& L = s.layout;
-      const uint32_t nn = static_cast<uint32_t>(L.nodes.size() / 16u), nt = static_cast<uint32_t>(L.tris.size() / 12u),
-                     np = static_cast<uint32_t>(L.prims.size() / 4u);
+      const uint32_t nn = s.inner_count, np = s.info.primitive_count, nt = np;
       const uint32_t mat_base = static_cast<uint32_t>(mats.size()), light_base = static_cast<uint32_t>(lights.size());
       slot_mat_base[si] = mat_base;
       const uint32_t tex_base = static_cast<uint32_t>(texrecs.size());
-      ptb::parallel_for(nn, host_threads, [&, node_base, tri_base](size_t a, size_t b) {
-        bool general = false;
-        for (size_t i = a; i < b; ++i) {
-          const float* src = &L.nodes[16u * i];
-          float* dst = &nodes[16u * (node_base + i)];
-          std::memcpy(dst, src, 12 * sizeof(float));
-          for (int k = 0; k < 12; ++k)
-            if (!std::isfinite(src[k])) general = true;
-          int32_t r[2];
-          std::memcpy(r, src + 12, sizeof(r));
-          for (int k = 0; k < 2; ++k) r[k] = rebase_ref(r[k], node_base, tri_base);
-          std::memcpy(dst + 12, r, sizeof(r));
-          std::memset(dst + 14, 0, 2 * sizeof(float));
-        }
-        if (general) general_boxes.store(true, std::memory_order_relaxed);
-      });
-      ptb::parallel_for(nt, host_threads, [&, tri_base, prim_base](size_t a, size_t b) {
-        for (size_t i = a; i < b; ++i) {
-          const float* src = &L.tris[12u * i];
-          float* dst = &tris[12u * (tri_base + i)];
-          std::memcpy(dst, src, 12 * sizeof(float));
-          uint32_t pid;
-          std::memcpy(&pid, src + 3, 4);
-          pid += prim_base;
-          std::memcpy(dst + 3, &pid, 4);
-        }
-      });
       const uint32_t last_mat = static_cast<uint32_t>(s.materials.size()) - 1u;
-      ptb::parallel_for(np, host_threads, [&, tri_base, prim_base, mat_base](size_t a, size_t b) {
-        for (size_t i = a; i < b; ++i) {
-          const float* src = &L.prims[4u * i];
-          float* dst = &prims[4u * (prim_base + i)];
-          std::memcpy(dst, src, 3 * sizeof(float));
-          uint32_t m;
-          std::memcpy(&m, src + 3, 4);
-          m = std::min(m, last_mat) + mat_base;  // the shading clamp of a scene-local index, then the pool base
-          std::memcpy(dst + 3, &m, 4);
-          if (any_uv && !s.texcoords.empty()) {
-            std::memcpy(&prim_uv[6u * (prim_base + i)], &L.prim_uv[6u * i], 6 * sizeof(float));
-            prim_tri[prim_base + i] = tri_base + L.prim_tri[i];
-          }
+      const bool with_uv = any_uv && !s.texcoords.empty();
+      if (s.forest >= 0) {  // packed on the device below; texture coordinates from the flattening
+        dests[static_cast<size_t>(s.forest)][s.forest_scene] =
+            ptg::PoolDest{node_base, tri_base, prim_base, mat_base, last_mat, with_uv ? 1u : 0u};
+        if (with_uv) {
+          ptb::HostVec<float> uv(6u * np);
+          ptb::parallel_for(np, host_threads, [&](size_t a, size_t b) {
+            for (size_t i = a; i < b; ++i) std::memcpy(&uv[6u * i], s.bvh.prims[i].uv, 6 * sizeof(float));
+          });
+          PT_HIP(t, hipMemcpy(t->d_prim_uv.ptr + 6u * static_cast<size_t>(prim_base), uv.data(),
+                              uv.size() * sizeof(float), hipMemcpyHostToDevice), "upload texture coordinates");
         }
-      });
+      } else {  // the host layout, rebased into the pool
+        const ptb::DeviceLayout& L = s.layout;
+        ptb::HostVec<float> nodes(16u * nn), tris(12u * nt), prims(4u * np);  // every element written
+        ptb::parallel_for(nn, host_threads, [&, node_base, tri_base](size_t a, size_t b) {
+          bool general = false;
+          for (size_t i = a; i < b; ++i) {
+            const float* src = &L.nodes[16u * i];
+            float* dst = &nodes[16u * i];
+            std::memcpy(dst, src, 12 * sizeof(float));
+            for (int k = 0; k < 12; ++k)
+              if (!std::isfinite(src[k])) general = true;
+            int32_t r[2];
+            std::memcpy(r, src + 12, sizeof(r));
+            for (int k = 0; k < 2; ++k) r[k] = rebase_ref(r[k], node_base, tri_base);
+            std::memcpy(dst + 12, r, sizeof(r));
+            std::memset(dst + 14, 0, 2 * sizeof(float));
+          }
+          if (general) general_boxes.store(true, std::memory_order_relaxed);
+        });
+        ptb::parallel_for(nt, host_threads, [&, prim_base](size_t a, size_t b) {
+          for (size_t i = a; i < b; ++i) {
+            const float* src = &L.tris[12u * i];
+            float* dst = &tris[12u * i];
+            std::memcpy(dst, src, 12 * sizeof(float));
+            uint32_t pid;
+            std::memcpy(&pid, src + 3, 4);
+            pid += prim_base;
+            std::memcpy(dst + 3, &pid, 4);
+          }
+        });
+        ptb::parallel_for(np, host_threads, [&, mat_base, last_mat](size_t a, size_t b) {
+          for (size_t i = a; i < b; ++i) {
+            const float* src = &L.prims[4u * i];
+            float* dst = &prims[4u * i];
+            std::memcpy(dst, src, 3 * sizeof(float));
+            uint32_t m;
+            std::memcpy(&m, src + 3, 4);
+            m = std::min(m, last_mat) + mat_base;  // the shading clamp of a scene-local index, then the pool base
+            std::memcpy(dst + 3, &m, 4);
+          }
+        });
+        PT_HIP(t, hipMemcpy(t->d_nodes.ptr + 4u * static_cast<size_t>(node_base), nodes.data(),
+                            nodes.size() * sizeof(float), hipMemcpyHostToDevice), "upload BVH nodes");
+        PT_HIP(t, hipMemcpy(t->d_tris.ptr + 3u * static_cast<size_t>(tri_base), tris.data(), tris.size() * sizeof(float),
+                            hipMemcpyHostToDevice), "upload triangles");
+        PT_HIP(t, hipMemcpy(t->d_prims.ptr + prim_base, prims.data(), prims.size() * sizeof(float),
+                            hipMemcpyHostToDevice), "upload primitives");
+        if (with_uv) {
+          std::vector<uint32_t> prim_tri(np);
+          for (uint32_t i = 0; i < np; ++i) prim_tri[i] = tri_base + L.prim_tri[i];
+          PT_HIP(t, hipMemcpy(t->d_prim_uv.ptr + 6u * static_cast<size_t>(prim_base), L.prim_uv.data(),
+                              L.prim_uv.size() * sizeof(float), hipMemcpyHostToDevice), "upload texture coordinates");
+          PT_HIP(t, hipMemcpy(t->d_prim_tri.ptr + prim_base, prim_tri.data(), np * sizeof(uint32_t),
+                              hipMemcpyHostToDevice), "upload triangle map");
+        }
+      }
       const std::vector<PtMaterial> pm = pool_materials(s, tex_base);
       for (const PtMaterial& m : pm)
         if (m.base_color_texture_index != 0xFFFFFFFFu) textured = true;
@@ -740,9 +833,9 @@ int upload_pool(pt_tracer* t) {
       texels.insert(texels.end(), s.texels.begin(), s.texels.end());
 
       R = ptd::SceneRec{};
-      R.root_ref = np ? rebase_ref(L.root_ref, node_base, tri_base) : 0;
+      R.root_ref = np ? rebase_ref(s.root_ref, node_base, tri_base) : 0;
       R.prim_count = np;
-      for (int k = 0; k < 6; ++k) R.root_box[k] = L.root_box[k];
+      for (int k = 0; k < 6; ++k) R.root_box[k] = s.root_box[k];
       R.light_base = light_base;
       R.light_count = static_cast<uint32_t>(s.lights.size());
       R.has_transmissive = 0;
@@ -790,6 +883,14 @@ int upload_pool(pt_tracer* t) {
     film_base += s.width * s.height;
   }
 
+  for (size_t f = 0; f < forests.size(); ++f) {
+    std::string err;
+    if (ptg::pack_pool(t->stream, forests[f], dests[f],
+                       ptg::PoolArrays{t->d_nodes.ptr, t->d_tris.ptr, t->d_prims.ptr, any_uv ? t->d_prim_tri.ptr : nullptr},
+                       err) < 0)
+      return fail(t, PT_INTERNAL_ERROR, err);
+  }
+
   // Hot copies for the traversal kernels' LDS: in each tree (the hot budget split evenly between the
   // pool's trees, in pool order) the inner nodes of largest box surface area, taken from the root
   // down (a child's box lies in its parent's, so the set is a top subtree), copied after the pool's
@@ -797,55 +898,30 @@ int upload_pool(pt_tracer* t) {
   // hot start at its copy. Traversal order and results do not depend on where a node is stored.
   std::vector<uint32_t> hot_src;
   {
-    std::vector<size_t> trees;
+    std::vector<int32_t> roots;
+    std::vector<float> boxes;
     for (size_t si = 0; si < t->scenes.size(); ++si)
-      if (t->scenes[si].view_of < 0 && recs[si].prim_count && recs[si].root_ref >= 0) trees.push_back(si);
-    const size_t per = trees.empty() ? 0 : std::max<size_t>(1, ptd::kHotNodes / trees.size());
-    auto area = [](const float* mn, const float* mx) {  // non-finite boxes rank last
-      const float ex = mx[0] - mn[0], ey = mx[1] - mn[1], ez = mx[2] - mn[2];
-      const float a = ex * ey + ey * ez + ez * ex;
-      return std::isfinite(a) ? a : -1.0f;
-    };
-    for (size_t si : trees) {
-      const size_t quota = std::min(per, static_cast<size_t>(ptd::kHotNodes) - hot_src.size());
-      if (quota == 0) break;
-      std::priority_queue<std::pair<float, int32_t>> frontier;  // (area, node): largest first, then larger index
-      frontier.emplace(area(recs[si].root_box, recs[si].root_box + 3), recs[si].root_ref);
-      size_t taken = 0;
-      while (!frontier.empty() && taken < quota) {
-        const int32_t ni = frontier.top().second;
-        frontier.pop();
-        hot_src.push_back(static_cast<uint32_t>(ni));
-        ++taken;
-        const float* f = &nodes[16u * static_cast<uint32_t>(ni)];
-        int32_t r[2];
-        std::memcpy(r, f + 12, sizeof(r));
-        for (int k = 0; k < 2; ++k)
-          if (r[k] >= 0) frontier.emplace(area(f + 6 * k, f + 6 * k + 3), r[k]);
+      if (t->scenes[si].view_of < 0 && recs[si].prim_count && recs[si].root_ref >= 0) {
+        roots.push_back(recs[si].root_ref);
+        boxes.insert(boxes.end(), recs[si].root_box, recs[si].root_box + 6);
       }
-    }
+    const uint32_t per = roots.empty() ? 0u : std::max<uint32_t>(1u, ptd::kHotNodes / static_cast<uint32_t>(roots.size()));
+    std::string err;
+    if (ptg::select_hot_nodes(t->stream, t->d_nodes.ptr, static_cast<uint32_t>(n_nodes), roots, boxes, per,
+                              ptd::kHotNodes, t->d_hot_src.ptr, hot_src, err) < 0)
+      return fail(t, PT_INTERNAL_ERROR, err);
   }
   const uint32_t hot_base = static_cast<uint32_t>(n_nodes), hot_count = static_cast<uint32_t>(hot_src.size());
-  if (hot_count) {
-    std::vector<int32_t> hot_slot(n_nodes, -1);
-    for (uint32_t h = 0; h < hot_count; ++h) hot_slot[hot_src[h]] = static_cast<int32_t>(h);
-    nodes.resize(16u * (n_nodes + hot_count));
-    for (uint32_t h = 0; h < hot_count; ++h) {
-      float* dst = &nodes[16u * (hot_base + h)];
-      std::memcpy(dst, &nodes[16u * hot_src[h]], 16 * sizeof(float));
-      int32_t r[2];
-      std::memcpy(r, dst + 12, sizeof(r));
-      for (int k = 0; k < 2; ++k)
-        if (r[k] >= 0 && hot_slot[static_cast<uint32_t>(r[k])] >= 0)
-          r[k] = static_cast<int32_t>(hot_base) + hot_slot[static_cast<uint32_t>(r[k])];
-      std::memcpy(dst + 12, r, sizeof(r));
-    }
-    for (ptd::SceneRec& R : recs)
-      if (R.prim_count && R.root_ref >= 0 && hot_slot[static_cast<uint32_t>(R.root_ref)] >= 0)
-        R.root_ref = static_cast<int32_t>(hot_base) + hot_slot[static_cast<uint32_t>(R.root_ref)];
-  }
+  for (ptd::SceneRec& R : recs)
+    if (R.prim_count && R.root_ref >= 0)
+      for (uint32_t h = 0; h < hot_count; ++h)
+        if (hot_src[h] == static_cast<uint32_t>(R.root_ref)) {
+          R.root_ref = static_cast<int32_t>(hot_base + h);
+          break;
+        }
 
-  uint64_t bytes = 0;
+  uint64_t bytes = (n_nodes + hot_count) * 64u + hot_count * 4u + n_tris * 48u + n_prims * 16u +
+                   (any_uv ? n_prims * 28u : 0u);
   auto upload = [&](auto& arr, const void* src, size_t count, size_t elem) -> hipError_t {
     hipError_t e = arr.ensure(count);
     if (e != hipSuccess) return e;
@@ -853,19 +929,14 @@ int upload_pool(pt_tracer* t) {
     if (count) e = hipMemcpy(arr.ptr, src, count * elem, hipMemcpyHostToDevice);
     return e;
   };
-  PT_HIP(t, upload(t->d_nodes, nodes.data(), (n_nodes + hot_count) * 4u, sizeof(float4)), "upload BVH nodes");
-  PT_HIP(t, upload(t->d_hot_src, hot_src.data(), hot_count, sizeof(uint32_t)), "upload hot node table");
-  PT_HIP(t, upload(t->d_tris, tris.data(), n_tris * 3u, sizeof(float4)), "upload triangles");
-  PT_HIP(t, upload(t->d_prims, prims.data(), n_prims, sizeof(float4)), "upload primitives");
   PT_HIP(t, upload(t->d_materials, mats.data(), mats.size(), sizeof(PtMaterial)), "upload materials");
   PT_HIP(t, upload(t->d_lights, lights.data(), lights.size(), sizeof(PtLight)), "upload lights");
   PT_HIP(t, upload(t->d_env, env.data(), env.size(), sizeof(float)), "upload environment");
   PT_HIP(t, upload(t->d_recs, recs.data(), recs.size(), sizeof(ptd::SceneRec)), "upload scene records");
-  PT_HIP(t, upload(t->d_prim_uv, prim_uv.data(), prim_uv.size(), sizeof(float)), "upload texture coordinates");
   PT_HIP(t, upload(t->d_texels, texels.data(), texels.size(), sizeof(float)), "upload textures");
-  PT_HIP(t, upload(t->d_prim_tri, prim_tri.data(), prim_tri.size(), sizeof(uint32_t)), "upload triangle map");
   PT_HIP(t, upload(t->d_texrecs, texrecs.data(), texrecs.size(), sizeof(ptd::TexRec)), "upload texture table");
   PT_HIP(t, upload(t->d_sdfs, sdfs.data(), sdfs.size(), sizeof(ptd::DevSdf)), "upload SDF primitives");
+  PT_HIP(t, hipStreamSynchronize(t->stream), "synchronize");
 
   // Traversal stack spill area: a lane's stack never holds more than max_depth entries (one per
   // level of the current path); entries older than the LDS ring's kStackLds go here.
@@ -1329,7 +1400,7 @@ bool resolve_instance(const SceneSlot& s, const PtInstanceTransformUpdate& u, ui
 PtInstanceTransformResult plan_refit(const pt_tracer* t, const PtInstanceTransformUpdate* ups, uint32_t n,
                                      const char*& message) {
   PtInstanceTransformResult r{PT_XFORM_FAILED, n, 0u, 0u, 0.0};
-  if (!t->accel_valid || t->batch || t->scenes.size() != 1u || t->scenes[0].layout.tris.empty()) {
+  if (!t->accel_valid || t->batch || t->scenes.size() != 1u || t->scenes[0].info.primitive_count == 0u) {
     message = "the BVH is not built";
     return r;
   }
@@ -1364,13 +1435,32 @@ int prepare_refit(pt_tracer* t) {
   std::string err;
   if (!ptb::flatten(s.view, prims, raw, err, nullptr, nullptr)) return fail(t, PT_INTERNAL_ERROR, "refit: " + err);
   const size_t n = prims.size();
-  if (n != s.layout.prim_tri.size()) return fail(t, PT_INTERNAL_ERROR, "refit: primitive count changed");
+  if (n != s.info.primitive_count) return fail(t, PT_INTERNAL_ERROR, "refit: primitive count changed");
+  // each primitive's leaf position and the inner nodes' references: from the host layout of a tree
+  // built on the host, else from the pool (one scene: its indices are the scene's own)
+  const uint32_t inner = s.inner_count;
+  std::vector<uint32_t> leaf_pos(s.layout.prim_tri.begin(), s.layout.prim_tri.end());
+  std::vector<int32_t> refs(2u * inner);
+  PT_HIP(t, t->d_leaf_pos.ensure(n), "allocate refit map");
+  if (s.layout.prim_tri.size() != n) {
+    ptg::launch_leaf_positions(t->stream, t->d_tris.ptr, static_cast<uint32_t>(n), t->d_leaf_pos.ptr);
+    leaf_pos.resize(n);
+    std::vector<float4> pool_nodes(4u * static_cast<size_t>(inner));
+    PT_HIP(t, hipMemcpyAsync(leaf_pos.data(), t->d_leaf_pos.ptr, n * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                             t->stream), "read refit map");
+    PT_HIP(t, hipMemcpyAsync(pool_nodes.data(), t->d_nodes.ptr, pool_nodes.size() * sizeof(float4),
+                             hipMemcpyDeviceToHost, t->stream), "read BVH nodes");
+    PT_HIP(t, hipStreamSynchronize(t->stream), "synchronize");
+    for (uint32_t i = 0; i < inner; ++i) std::memcpy(&refs[2u * i], &pool_nodes[4u * i + 3u], 2 * sizeof(int32_t));
+  } else {
+    for (uint32_t i = 0; i < inner; ++i) std::memcpy(&refs[2u * i], &s.layout.nodes[16u * i + 12u], 2 * sizeof(int32_t));
+  }
   std::vector<float> box(6 * n);
   auto mn3 = [](float a, float b, float c) { float m = a; if (b < m) m = b; if (c < m) m = c; return m; };
   auto mx3 = [](float a, float b, float c) { float m = a; if (m < b) m = b; if (m < c) m = c; return m; };
   for (size_t pid = 0; pid < n; ++pid) {
     const float* v = &raw[9 * pid];
-    float* b = &box[6u * s.layout.prim_tri[pid]];
+    float* b = &box[6u * leaf_pos[pid]];
     for (int k = 0; k < 3; ++k) {
       b[k] = mn3(v[k], v[3 + k], v[6 + k]) - 1e-4f;
       b[3 + k] = mx3(v[k], v[3 + k], v[6 + k]) + 1e-4f;
@@ -1378,12 +1468,10 @@ int prepare_refit(pt_tracer* t) {
   }
   // node heights: children before parents (inner nodes are numbered in pre-order, so a reverse sweep
   // sees every child first)
-  const uint32_t inner = s.layout.inner_count;
   std::vector<uint32_t> height(inner, 0);
   uint32_t max_h = 0;
   for (uint32_t i = inner; i-- > 0;) {
-    int32_t ref[2];
-    std::memcpy(ref, &s.layout.nodes[16u * i + 12u], sizeof(ref));
+    const int32_t* ref = &refs[2u * i];
     uint32_t h = 1;
     for (int c = 0; c < 2; ++c)
       if (ref[c] >= 0) h = std::max(h, height[static_cast<uint32_t>(ref[c])] + 1u);
@@ -1403,8 +1491,7 @@ int prepare_refit(pt_tracer* t) {
   for (uint32_t i = 0; i < inner; ++i) order[fill[height[i] - 1u]++] = i;
   PT_HIP(t, t->d_tri_box.ensure(box.size()), "allocate refit boxes");
   PT_HIP(t, hipMemcpy(t->d_tri_box.ptr, box.data(), box.size() * sizeof(float), hipMemcpyHostToDevice), "upload boxes");
-  PT_HIP(t, t->d_leaf_pos.ensure(n), "allocate refit map");
-  PT_HIP(t, hipMemcpy(t->d_leaf_pos.ptr, s.layout.prim_tri.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice),
+  PT_HIP(t, hipMemcpy(t->d_leaf_pos.ptr, leaf_pos.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice),
          "upload refit map");
   PT_HIP(t, t->d_refit_order.ensure(order.size()), "allocate refit order");
   if (!order.empty())
@@ -1579,9 +1666,10 @@ int pt_hip_build_accel(pt_tracer* t, const PtBvhParams* params, PtBuildInfo* inf
   SceneSlot& s = t->scenes[0];
   s.bvh_params = p;
   const auto t0 = std::chrono::steady_clock::now();
-  if (!build_scene(s, t->stream, std::max(1u, std::min(16u, std::thread::hardware_concurrency()))))
+  std::vector<ptg::DeviceForest> forests;
+  if (!build_scene(s, t->stream, std::max(1u, std::min(16u, std::thread::hardware_concurrency())), forests))
     return fail(t, PT_INVALID_ARGUMENT, s.error);
-  rc = upload_pool(t);
+  rc = upload_pool(t, forests);
   if (rc != PT_OK) return rc;
   t->info.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (info) *info = t->info;
@@ -2126,7 +2214,7 @@ int pt_hip_apply_instance_transform_update(pt_tracer* t, const PtInstanceTransfo
   for (size_t h = 0; h + 1 < t->refit_level.size(); ++h)
     ptd::launch_refit_nodes(t->stream, t->d_nodes.ptr, t->d_refit_order.ptr, t->refit_level[h],
                             t->refit_level[h + 1] - t->refit_level[h], t->d_tri_box.ptr, t->d_refit_flag.ptr);
-  ptd::launch_refit_root(t->stream, t->d_nodes.ptr, s.layout.root_ref, t->d_tri_box.ptr, t->d_recs.ptr);
+  ptd::launch_refit_root(t->stream, t->d_nodes.ptr, s.root_ref, t->d_tri_box.ptr, t->d_recs.ptr);
   ptd::launch_hot_refresh(t->stream, t->d_nodes.ptr, t->d_hot_src.ptr, t->dscene.hot_base, t->dscene.hot_count);
   PT_HIP(t, hipGetLastError(), "launch refit");
   uint32_t nonfinite = 0;
@@ -2428,7 +2516,7 @@ int pt_batch_build(pt_batch* b, PtBuildInfo* info) {
             [&](size_t a, size_t c) { return t->scenes[a].indices.size() > t->scenes[c].indices.size(); });
   const unsigned nthreads = std::max(1u, std::min<unsigned>(16u, std::thread::hardware_concurrency()));
   std::vector<char> ok(t->scenes.size(), 1);  // one byte per scene: threads write different elements
-  std::vector<ptb::HostVec<float>> raws(t->scenes.size());
+  std::vector<FlatScene> flat(t->scenes.size());
   std::vector<std::chrono::steady_clock::duration> own(t->scenes.size());
   auto each_scene = [&](auto fn) {
     std::vector<std::thread> pool;
@@ -2443,30 +2531,30 @@ int pt_batch_build(pt_batch* b, PtBuildInfo* info) {
       });
     for (std::thread& th : pool) th.join();
   };
-  each_scene([&](size_t i) { ok[i] = flatten_scene(t->scenes[i], raws[i], 1u); });
+  for (size_t i : order) t->scenes[i].forest = -1;
+  each_scene([&](size_t i) { ok[i] = flatten_scene(t->scenes[i], flat[i], 1u); });
   for (size_t i = 0; i < ok.size(); ++i)
     if (!ok[i]) return fail(t, PT_INVALID_ARGUMENT, "scene " + std::to_string(i) + ": " + t->scenes[i].error);
   std::vector<ptg::DeviceBuild> jobs;
-  std::vector<size_t> job_scene;
+  std::vector<SceneSlot*> job_scene;
   for (size_t i : order) {
     SceneSlot& s = t->scenes[i];
     if (!wants_device_build(s)) continue;
-    jobs.push_back(ptg::DeviceBuild{&s.bvh, &raws[i], s.bvh_params.leaf_size, s.bvh_params.bucket_count, 1});
-    job_scene.push_back(i);
+    jobs.push_back(device_job(s, flat[i]));
+    job_scene.push_back(&s);
   }
-  std::vector<int> device_rc(t->scenes.size(), 1);
+  std::vector<ptg::DeviceForest> forests;
   const auto d0 = std::chrono::steady_clock::now();
-  if (!jobs.empty()) {
+  {
     std::string err;
-    if (ptg::build_sah_device_many(t->stream, jobs, err) < 0) return fail(t, PT_INTERNAL_ERROR, err);
-    for (size_t j = 0; j < jobs.size(); ++j) device_rc[job_scene[j]] = jobs[j].rc;
+    if (!build_on_device(t->stream, job_scene, jobs, forests, err)) return fail(t, PT_INTERNAL_ERROR, err);
   }
   const double device_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - d0).count();
   each_scene([&](size_t i) {
-    finish_scene(t->scenes[i], raws[i], device_rc[i], 1u, device_ms, own[i]);
-    ptb::HostVec<float>().swap(raws[i]);
+    finish_scene(t->scenes[i], flat[i], kept_tree(t->scenes[i], forests), 1u, device_ms, own[i]);
+    flat[i] = FlatScene{};
   });
-  int rc = upload_pool(t);
+  int rc = upload_pool(t, forests);
   if (rc != PT_OK) return rc;
   rc = alloc_film(t, t->film_pixels);
   if (rc != PT_OK) return rc;
