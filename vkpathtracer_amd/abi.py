@@ -57,9 +57,12 @@ class PtSampleCounters(C.Structure):
         return {n: int(getattr(self, n)) for n in COUNTER_FIELDS}
 
 
+PT_BVH_FLAG_HOST_BUILD = 1  # PtBvhParams.flags: build the SAH tree on the host
+
+
 class PtBvhParams(C.Structure):
     _fields_ = [("builder", C.c_uint32), ("leaf_size", C.c_uint32), ("bucket_count", C.c_uint32),
-                ("flags", C.c_uint32)]  # PT_BVH_FLAG_HOST_BUILD = 1
+                ("flags", C.c_uint32)]
 
 
 class PtBuildInfo(C.Structure):

@@ -202,11 +202,14 @@ class HipPathTracer:
             self._check(self._lib.pt_hip_update_texels(self._h, texels.data_ptr(), texels.numel() // 3),
                         "update_texels")
 
-    def build_or_update_acceleration(self, builder: str = "sah", leaf_size: int = 4, bucket_count: int = 8) -> Status:
+    def build_or_update_acceleration(self, builder: str = "sah", leaf_size: int = 4, bucket_count: int = 8,
+                                     host_build: bool = False) -> Status:
+        """host_build: build the SAH tree on the host (PT_BVH_FLAG_HOST_BUILD) instead of the GPU."""
         p = abi.PtBvhParams()
         p.builder = {"sah": 0, "midpoint": 1}[builder]
         p.leaf_size = leaf_size
         p.bucket_count = bucket_count
+        p.flags = abi.PT_BVH_FLAG_HOST_BUILD if host_build else 0
         info = abi.PtBuildInfo()
         st = self._status(self._lib.pt_hip_build_accel(self._h, C.byref(p), C.byref(info)))
         self.build_info = info.as_dict()
@@ -466,9 +469,10 @@ class HipBatch:
             raise PathTracerError(Status(rc, msg), what)
 
     def add_scene(self, scene: Scene, width: int, height: int, spp: int, builder: str = "sah", leaf_size: int = 4,
-                  bucket_count: int = 8) -> int:
+                  bucket_count: int = 8, host_build: bool = False) -> int:
         view = abi.SceneView(scene)
-        p = abi.PtBvhParams({"sah": 0, "midpoint": 1}[builder], leaf_size, bucket_count, 0)
+        p = abi.PtBvhParams({"sah": 0, "midpoint": 1}[builder], leaf_size, bucket_count,
+                            abi.PT_BVH_FLAG_HOST_BUILD if host_build else 0)
         idx = C.c_uint32()
         self._check(self._lib.pt_batch_add_scene(self._h, C.byref(view.view), width, height, spp, C.byref(p),
                                                  C.byref(idx)), "add_scene")
