@@ -407,17 +407,19 @@ int ensure_adjoint(pt_tracer* t, uint32_t cap, ptd::DevAdjoint& A) {
   return PT_OK;
 }
 
-// Path-state and queue arrays for `cap` paths: 29 float arrays, 7 u32 arrays, 1 u64 array, each
+// Path-state and queue arrays for `cap` paths: 41 float arrays, 7 u32 arrays, 3 u64 arrays, each
 // of max(cap, kSegments * seg_cap) entries (queue arrays are indexed by segment position).
 int ensure_paths(pt_tracer* t, uint32_t cap) {
   if (cap <= t->path_capacity_alloc) return PT_OK;
   free_paths(t);
-  constexpr size_t kF = 29, kU = 7;
+  constexpr size_t kF = 41, kU = 7;
   const uint32_t seg_cap = ptd::segment_capacity(cap);
   const size_t n = std::max<size_t>(cap, static_cast<size_t>(ptd::kSegments) * seg_cap);
+  // shadow records name a queue entry with the top bit set (ptd::kRadQueued)
+  if (n >= static_cast<size_t>(ptd::kRadQueued)) return fail(t, PT_INVALID_ARGUMENT, "wavefront capacity too large");
   PT_HIP(t, t->d_path_f.ensure(kF * n), "allocate path state");
   PT_HIP(t, t->d_path_u.ensure(kU * n), "allocate path state");
-  PT_HIP(t, t->d_rng.ensure(n), "allocate path state");
+  PT_HIP(t, t->d_rng.ensure(3 * n), "allocate path state");
   float* f = t->d_path_f.ptr;
   uint32_t* u = t->d_path_u.ptr;
   ptd::DevPaths& P = t->paths;
@@ -425,6 +427,11 @@ int ensure_paths(pt_tracer* t, uint32_t cap) {
     for (int k = 0; k < 6; ++k) P.rays[b].c[k] = f + (size_t)(6 * b + k) * n;
     P.rays[b].qpath = u + (size_t)b * n;
     P.rays[b].qscene = u + (size_t)(4 + b) * n;
+    for (int k = 0; k < 3; ++k) {
+      P.rays[b].tp[k] = f + (size_t)(29 + 6 * b + k) * n;
+      P.rays[b].rad[k] = f + (size_t)(32 + 6 * b + k) * n;
+    }
+    P.rays[b].rng = t->d_rng.ptr + (size_t)(1 + b) * n;
   }
   P.hit_t = f + 12 * n;
   P.hit_prim = reinterpret_cast<int32_t*>(u + 2 * n);
@@ -1135,7 +1142,7 @@ void bounce_loop(pt_tracer* t, const ptd::DevPaths& P, const ptd::DevSettings& s
     if (st.enable_nee && depth + 1u < st.max_depth)
       timed(t, PT_KERNEL_SHADOW, 0, [&] {
         if (adj) ptd::launch_shadow_adjoint(t->stream, P, *adj, t->dscene, t->grids.shadow_adj);
-        else ptd::launch_shadow(t->stream, P, t->dscene, t->grids.shadow);
+        else ptd::launch_shadow(t->stream, P, t->dscene, buf, t->grids.shadow);
       });
     ptd::launch_advance(t->stream, P);
     buf ^= 1;

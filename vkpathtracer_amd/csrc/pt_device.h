@@ -174,15 +174,32 @@ struct FilmJob {
 };
 
 // One ray queue: origin xyz, direction xyz (SoA), the path slot each entry serves and its scene.
+// Forward passes keep a live path's throughput, radiance and RNG state with its queue entry (tp, rad,
+// rng in queue order), so shading reads and writes them with the entry's coalesced loads and stores;
+// a path's radiance reaches DevPaths::rad (indexed by path slot) once, when the path ends. Adjoint
+// replays keep all three in DevPaths, by path slot, beside the rest of their per-path state.
 struct DevRays {
   float* c[6];
   uint32_t* qpath;
   uint32_t* qscene;
+  float* tp[3];
+  float* rad[3];
+  uint64_t* rng;
+};
+
+// Where a forward shadow record's light contribution goes (DevPaths::spath): the radiance of the
+// path's queued continuation ray (kRadQueued | its entry in the output queue of the bounce), or, when
+// the path ends at this bounce, DevPaths::rad at its path slot.
+constexpr uint32_t kRadQueued = 0x80000000u;
+struct RadTarget {
+  float* rad[3];  // the output queue's radiance arrays
 };
 
 
 // Per-path wavefront state. Ray and hit buffers are in queue order (double-buffered rays);
-// throughput, radiance and RNG state are indexed by the path's slot in the pass.
+// throughput, radiance and RNG state indexed by the path's slot in the pass serve the adjoint replay,
+// and in forward passes hold only what a path leaves when it ends: its radiance, and in tp[0] the
+// number of extension rays it traced (for k_film's ray count of dropped samples).
 struct DevPaths {
   DevRays rays[2];
   float* hit_t;
@@ -270,7 +287,9 @@ void launch_raygen(hipStream_t stream, const DevPaths& paths, const DevScene& sc
 void launch_extend(hipStream_t stream, const DevPaths& paths, const DevScene& scene, int in_buf, uint32_t grid);
 void launch_shade(hipStream_t stream, const DevPaths& paths, const DevScene& scene, const DevSettings& st,
                   int in_buf, uint32_t depth, uint32_t grid);
-void launch_shadow(hipStream_t stream, const DevPaths& paths, const DevScene& scene, uint32_t grid);
+// in_buf: the bounce's shade input; forward contributions of paths that continue go to the radiance of
+// their entries in the other buffer.
+void launch_shadow(hipStream_t stream, const DevPaths& paths, const DevScene& scene, int in_buf, uint32_t grid);
 void launch_advance(hipStream_t stream, const DevPaths& paths);
 // Adds the radiance of every tile of the pass into the films, pixel by pixel in tile order.
 void launch_film(hipStream_t stream, const DevPaths& paths, const DevScene& scene, const DevFilm& film,

@@ -853,9 +853,19 @@ __global__ __launch_bounds__(kBlock) void k_raygen(DevPaths P, DevRays out, DevS
   out.c[3][p] = rd.x; out.c[4][p] = rd.y; out.c[5][p] = rd.z;
   out.qpath[p] = p;
   out.qscene[p] = tile.scene;
-  P.tp[0][p] = 1.0f; P.tp[1][p] = 1.0f; P.tp[2][p] = 1.0f;
-  P.rad[0][p] = 0.0f; P.rad[1][p] = 0.0f; P.rad[2][p] = 0.0f;
-  P.rng[p] = prng;
+  if constexpr (kAdj) {
+    P.tp[0][p] = 1.0f; P.tp[1][p] = 1.0f; P.tp[2][p] = 1.0f;
+    P.rad[0][p] = 0.0f; P.rad[1][p] = 0.0f; P.rad[2][p] = 0.0f;
+    P.rng[p] = prng;
+  } else {
+    // the initial queue is the identity (entry p serves path p)
+    out.tp[0][p] = 1.0f; out.tp[1][p] = 1.0f; out.tp[2][p] = 1.0f;
+    out.rad[0][p] = 0.0f; out.rad[1][p] = 0.0f; out.rad[2][p] = 0.0f;
+    out.rng[p] = prng;
+    if (st.max_depth == 0u) {  // no bounce will end the path: its radiance stays 0
+      P.rad[0][p] = 0.0f; P.rad[1][p] = 0.0f; P.rad[2][p] = 0.0f;
+    }
+  }
   if constexpr (kAdj) {
     const size_t pix = rec.film_base + static_cast<size_t>(y) * rec.width + x;
     A.delta[0][p] = A.film_grad[3 * pix];
@@ -1244,7 +1254,6 @@ __device__ __forceinline__ DevScene scene_view(const DevScene& pool, uint32_t si
 }
 
 // ------------------------------------------------------------------ textures
-constexpr uint32_t kNoTexture = 0xFFFFFFFFu;
 
 // Texture coordinates at barycentrics (u, v) of a primitive with vertex coordinates uvp[0..5].
 __device__ __forceinline__ float2 uv_at(const float* uvp, float u, float v) {
@@ -1714,14 +1723,16 @@ __device__ __forceinline__ float eff_grad(float pre_c, float k) { return (pre_c 
 
 // One iteration of ScalarCpuPathTracer::trace (ScalarCpuPathTracer.cpp:1159-1287) for a path that
 // arrived at a hit (prim >= 0) or a miss (prim < 0). kAdj replays the iteration for the adjoint.
+// One iteration of trace's loop for the path in slot p whose extension ray (ro, rd) hit prim at t (-1:
+// missed). T, L and rng are the path's throughput, radiance and RNG state, updated in place; the caller
+// loads and stores them (the radiance is final when out.cont is false).
 template <bool kAdj>
 __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc, const DevSettings& st, uint32_t depth,
-                                           uint32_t p, V3 ro, V3 rd, float t, int32_t prim,
-                                           ShadeOut& out, uint32_t& shadow_tests, const DevAdjoint& A, float* glds) {
+                                           uint32_t p, V3 ro, V3 rd, float t, int32_t prim, V3& T, V3& L,
+                                           uint64_t& rng, ShadeOut& out, uint32_t& shadow_tests, const DevAdjoint& A,
+                                           float* glds) {
   out.cont = false;
   out.shadow = false;
-  V3 T = mk(P.tp[0][p], P.tp[1][p], P.tp[2][p]);
-  V3 L = mk(P.rad[0][p], P.rad[1][p], P.rad[2][p]);
   uint32_t aflags = 0;
   V3 dl = mk(0.0f, 0.0f, 0.0f);
   if constexpr (kAdj) {
@@ -1732,10 +1743,8 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
   }
   if (prim == -1) {
     L = add(L, mulv(T, environment(sc, rd)));
-    P.rad[0][p] = L.x; P.rad[1][p] = L.y; P.rad[2][p] = L.z;
     return;
   }
-  uint64_t rng = P.rng[p];
   const V3 pos = add(ro, muls(rd, t));
   V3 gN;
   uint32_t raw_mat;
@@ -1925,7 +1934,6 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
       }
     }
   }
-  P.rad[0][p] = L.x; P.rad[1][p] = L.y; P.rad[2][p] = L.z;
   if (last) return;  // the loop ends after this iteration; nothing below reaches the radiance
 
   V3 outDir;
@@ -2015,8 +2023,6 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
     A.flags[p] = aflags;
   }
   T = divs(T, rr);
-  P.tp[0][p] = T.x; P.tp[1][p] = T.y; P.tp[2][p] = T.z;
-  P.rng[p] = rng;
   out.cont = true;
   out.no = add(pos, muls(outDir, 0.002f));
   out.nd = outDir;
@@ -2041,19 +2047,39 @@ __device__ __forceinline__ void shade_kernel(const DevPaths& P, const DevRays& i
     so.cont = false;
     so.shadow = false;
     uint32_t p = 0, sid = 0;
+    V3 T = mk(0.0f, 0.0f, 0.0f), L = T;
+    uint64_t rng = 0;
     if (i < n) {
       const uint32_t q = queue_pos(prefix, i, P.seg_cap);
       p = in.qpath[q];
       sid = in.qscene[q];
       const V3 ro = mk(in.c[0][q], in.c[1][q], in.c[2][q]);
       const V3 rd = mk(in.c[3][q], in.c[4][q], in.c[5][q]);
-      shade_path<kAdj>(P, scene_view(sc, sid), st, depth, p, ro, rd, P.hit_t[q], P.hit_prim[q], so,
+      if constexpr (kAdj) {
+        T = mk(P.tp[0][p], P.tp[1][p], P.tp[2][p]);
+        L = mk(P.rad[0][p], P.rad[1][p], P.rad[2][p]);
+        rng = P.rng[p];
+      } else {
+        T = mk(in.tp[0][q], in.tp[1][q], in.tp[2][q]);
+        L = mk(in.rad[0][q], in.rad[1][q], in.rad[2][q]);
+        rng = in.rng[q];
+      }
+      shade_path<kAdj>(P, scene_view(sc, sid), st, depth, p, ro, rd, P.hit_t[q], P.hit_prim[q], T, L, rng, so,
                        shadow_tests, A, glds);
-      // A path that ends here traced depth + 1 extension rays. Its throughput is dead, so its first
-      // word keeps that count for k_film, which takes the rays of a dropped (non-finite) sample back
-      // out of the ray counter (shade_pixel adds ray_count only for finite samples).
-      if constexpr (!kAdj)
-        if (!so.cont) P.tp[0][p] = __uint_as_float(depth + 1u);
+      if constexpr (kAdj) {
+        P.rad[0][p] = L.x; P.rad[1][p] = L.y; P.rad[2][p] = L.z;
+        if (so.cont) {
+          P.tp[0][p] = T.x; P.tp[1][p] = T.y; P.tp[2][p] = T.z;
+          P.rng[p] = rng;
+        }
+      } else if (!so.cont) {
+        // The path ends here: its radiance goes to its slot. It traced depth + 1 extension rays; its
+        // throughput is dead, so the slot's first throughput word keeps that count for k_film, which
+        // takes the rays of a dropped (non-finite) sample back out of the ray counter (shade_pixel
+        // adds ray_count only for finite samples).
+        P.rad[0][p] = L.x; P.rad[1][p] = L.y; P.rad[2][p] = L.z;
+        P.tp[0][p] = __uint_as_float(depth + 1u);
+      }
     }
     const uint32_t seg = (w0 >> 6) & (kSegments - 1u);  // this chunk's output segment
     const uint32_t slot = segment_append(P.queue + kQueueOutCursor, seg, P.seg_cap, so.cont);
@@ -2062,6 +2088,11 @@ __device__ __forceinline__ void shade_kernel(const DevPaths& P, const DevRays& i
       out.c[3][slot] = so.nd.x; out.c[4][slot] = so.nd.y; out.c[5][slot] = so.nd.z;
       out.qpath[slot] = p;
       out.qscene[slot] = sid;
+      if constexpr (!kAdj) {
+        out.tp[0][slot] = T.x; out.tp[1][slot] = T.y; out.tp[2][slot] = T.z;
+        out.rad[0][slot] = L.x; out.rad[1][slot] = L.y; out.rad[2][slot] = L.z;
+        out.rng[slot] = rng;
+      }
     }
     const uint32_t sslot = segment_append(P.queue + kQueueShadowCursor, seg, P.seg_cap, so.shadow);
     if (so.shadow) {
@@ -2069,7 +2100,8 @@ __device__ __forceinline__ void shade_kernel(const DevPaths& P, const DevRays& i
       P.sray[3][sslot] = so.sd.x; P.sray[4][sslot] = so.sd.y; P.sray[5][sslot] = so.sd.z;
       P.stmax[sslot] = so.stmax;
       P.scontrib[0][sslot] = so.contrib.x; P.scontrib[1][sslot] = so.contrib.y; P.scontrib[2][sslot] = so.contrib.z;
-      P.spath[sslot] = p;
+      if constexpr (kAdj) P.spath[sslot] = p;
+      else P.spath[sslot] = so.cont ? (kRadQueued | slot) : p;
       P.sscene[sslot] = sid;
       if constexpr (kAdj) {
         A.s_galb[0][sslot] = so.galb.x; A.s_galb[1][sslot] = so.galb.y; A.s_galb[2][sslot] = so.galb.z;
@@ -2120,11 +2152,19 @@ constexpr uint32_t kShadowVisible = 0x80000000u;
 
 template <bool kAdj>
 __device__ __forceinline__ void light_visible(const DevPaths& P, const DevScene& sc, const DevAdjoint& A, float* glds,
-                                              uint32_t q) {
-  const uint32_t p = P.spath[q];
-  P.rad[0][p] = P.rad[0][p] + P.scontrib[0][q];
-  P.rad[1][p] = P.rad[1][p] + P.scontrib[1][q];
-  P.rad[2][p] = P.rad[2][p] + P.scontrib[2][q];
+                                              const RadTarget& rq, uint32_t q) {
+  const uint32_t target = P.spath[q];
+  float* const* rad = P.rad;
+  uint32_t k = target;
+  if constexpr (!kAdj) {
+    if (target & kRadQueued) {
+      rad = rq.rad;
+      k = target & ~kRadQueued;
+    }
+  }
+  rad[0][k] = rad[0][k] + P.scontrib[0][q];
+  rad[1][k] = rad[1][k] + P.scontrib[1][q];
+  rad[2][k] = rad[2][k] + P.scontrib[2][q];
   if constexpr (kAdj) {
     const uint32_t mi = A.s_mat[q], li = A.s_light[q] & ~kShadowVisible;
     gadd_base(A, sc, glds, mi, make_float2(A.s_uv[0][q], A.s_uv[1][q]),
@@ -2140,7 +2180,7 @@ __device__ __forceinline__ void light_visible(const DevPaths& P, const DevScene&
 constexpr uint32_t kShadowReached = 0x80000000u;
 
 template <bool kAdj, bool kSdf, bool kAny>
-__global__ __launch_bounds__(kBlock) void k_shadow(DevPaths P, DevScene sc, DevAdjoint A) {
+__global__ __launch_bounds__(kBlock) void k_shadow(DevPaths P, DevScene sc, DevAdjoint A, RadTarget rq) {
   static_assert(!(kAny && kSdf), "SDF shadows need the closest-hit skip loop");
   __shared__ int32_t s_ref[kStackLds * kBlock];
   __shared__ float s_t[kStackLds * kBlock];
@@ -2152,7 +2192,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevPaths P, DevScene sc, DevA
   // k_shadow_grad applies it (keeping this traversal kernel as lean as the forward one)
   auto visible = [&](uint32_t q) {
     if constexpr (kAdj) A.s_light[q] |= kShadowVisible;
-    else light_visible<false>(P, sc, A, nullptr, q);
+    else light_visible<false>(P, sc, A, nullptr, rq, q);
   };
   const Stack stk = make_stack(P, sc, s_ref, s_t, s_hot);
   const uint32_t n = prefix[kSegments];
@@ -2239,13 +2279,13 @@ static_assert(kGradBlocks <= kGradRows, "every block of an adjoint kernel owns a
 // Adds the light contribution of every shadow record k_shadow's forward any-hit query marked. A path
 // has at most one shadow record per bounce and this runs before the next shade, so every radiance
 // sum sees its additions in the same order as when the traversal kernel added them itself.
-__global__ __launch_bounds__(kBlock) void k_shadow_apply(DevPaths P, DevScene sc) {
+__global__ __launch_bounds__(kBlock) void k_shadow_apply(DevPaths P, DevScene sc, RadTarget rq) {
   __shared__ uint32_t prefix[kSegments + 1];
   load_prefix(P.queue + kQueueShadowCursor, kSegLine, prefix);
   const uint32_t n = prefix[kSegments];
   for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
     const uint32_t q = queue_pos(prefix, j, P.seg_cap);
-    if (P.sscene[q] & kShadowReached) light_visible<false>(P, sc, DevAdjoint{}, nullptr, q);
+    if (P.sscene[q] & kShadowReached) light_visible<false>(P, sc, DevAdjoint{}, nullptr, rq, q);
   }
 }
 
@@ -2261,7 +2301,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow_grad(DevPaths P, DevScene sc,
     const uint32_t i = w0 + lane;
     if (i < n) {
       const uint32_t q = queue_pos(prefix, i, P.seg_cap);
-      if (A.s_light[q] & kShadowVisible) light_visible<true>(P, sc, A, glds, q);
+      if (A.s_light[q] & kShadowVisible) light_visible<true>(P, sc, A, glds, RadTarget{}, q);
     }
     tex_grad_flush(A, glds);  // the wave is converged here
   }
@@ -2529,26 +2569,33 @@ void launch_shade_adjoint(hipStream_t stream, const DevPaths& paths, const DevAd
                      paths.rays[in_buf ^ 1], scene, st, depth, adj);
 }
 
-void launch_shadow(hipStream_t stream, const DevPaths& paths, const DevScene& scene, uint32_t grid) {
+void launch_shadow(hipStream_t stream, const DevPaths& paths, const DevScene& scene, int in_buf, uint32_t grid) {
+  const DevRays& o = paths.rays[in_buf ^ 1];
+  const RadTarget rq{{o.rad[0], o.rad[1], o.rad[2]}};
   if (scene.has_sdf)
-    hipLaunchKernelGGL((k_shadow<false, true, false>), dim3(grid), dim3(kBlock), 0, stream, paths, scene, DevAdjoint{});
+    hipLaunchKernelGGL((k_shadow<false, true, false>), dim3(grid), dim3(kBlock), 0, stream, paths, scene, DevAdjoint{},
+                       rq);
   else if (scene.has_transmissive)
     hipLaunchKernelGGL((k_shadow<false, false, false>), dim3(grid), dim3(kBlock), 0, stream, paths, scene,
-                       DevAdjoint{});
+                       DevAdjoint{}, rq);
   else {
-    hipLaunchKernelGGL((k_shadow<false, false, true>), dim3(grid), dim3(kBlock), 0, stream, paths, scene, DevAdjoint{});
-    hipLaunchKernelGGL(k_shadow_apply, dim3(kShadowApplyBlocks), dim3(kBlock), 0, stream, paths, scene);
+    hipLaunchKernelGGL((k_shadow<false, false, true>), dim3(grid), dim3(kBlock), 0, stream, paths, scene, DevAdjoint{},
+                       rq);
+    hipLaunchKernelGGL(k_shadow_apply, dim3(kShadowApplyBlocks), dim3(kBlock), 0, stream, paths, scene, rq);
   }
 }
 
 void launch_shadow_adjoint(hipStream_t stream, const DevPaths& paths, const DevAdjoint& adj, const DevScene& scene,
                            uint32_t grid) {
   if (scene.has_sdf)
-    hipLaunchKernelGGL((k_shadow<true, true, false>), dim3(grid), dim3(kBlock), 0, stream, paths, scene, adj);
+    hipLaunchKernelGGL((k_shadow<true, true, false>), dim3(grid), dim3(kBlock), 0, stream, paths, scene, adj,
+                       RadTarget{});
   else if (scene.has_transmissive)
-    hipLaunchKernelGGL((k_shadow<true, false, false>), dim3(grid), dim3(kBlock), 0, stream, paths, scene, adj);
+    hipLaunchKernelGGL((k_shadow<true, false, false>), dim3(grid), dim3(kBlock), 0, stream, paths, scene, adj,
+                       RadTarget{});
   else
-    hipLaunchKernelGGL((k_shadow<true, false, true>), dim3(grid), dim3(kBlock), 0, stream, paths, scene, adj);
+    hipLaunchKernelGGL((k_shadow<true, false, true>), dim3(grid), dim3(kBlock), 0, stream, paths, scene, adj,
+                       RadTarget{});
   hipLaunchKernelGGL(k_shadow_grad, dim3(kGradBlocks), dim3(kBlock), 0, stream, paths, scene, adj);
 }
 
