@@ -1621,26 +1621,6 @@ __device__ __forceinline__ V3 sample_hemisphere(uint64_t& rng, V3 n) {
   return normalize(add(add(muls(tangent, local.x), muls(bitangent, local.y)), muls(n, local.z)));
 }
 
-// sample_phong_lobe, ScalarCpuPathTracer.cpp:996-1020
-__device__ __forceinline__ V3 sample_phong_lobe(uint64_t& rng, V3 refl, float exponent, V3 n) {
-  const float u1 = next01(rng);
-  const float u2 = next01(rng);
-  const float cosT = ptm::powf_p(smax(0.0f, u1), 1.0f / (exponent + 1.0f));
-  const float sinT = __builtin_sqrtf(smax(0.0f, 1.0f - cosT * cosT));
-  const float phi = 2.0f * kPi * u2;
-  float sp, cp;
-  ptm::sincosf_p(phi, sp, cp);
-  const V3 ref_t = (__builtin_fabsf(refl.z) < 0.999f) ? mk(0.0f, 0.0f, 1.0f) : mk(0.0f, 1.0f, 0.0f);
-  const V3 tang = normalize(cross(ref_t, refl));
-  const V3 btan = cross(refl, tang);
-  V3 out = normalize(add(add(muls(tang, sinT * cp), muls(btan, sinT * sp)), muls(refl, cosT)));
-  if (dot(out, n) <= 0.0f) {
-    out = sub(out, muls(n, 2.0f * dot(out, n)));
-    if (dot(out, n) <= 0.0f) return sample_hemisphere(rng, n);
-  }
-  return out;
-}
-
 __device__ __forceinline__ V3 reflect_about(V3 d, V3 n) { return sub(d, muls(n, 2.0f * dot(n, d))); }
 
 __device__ __forceinline__ float schlick_fresnel(float cosTheta, float ior) {
@@ -1954,19 +1934,52 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
       if (tir || dot(outDir, outDir) <= 1.0e-8f) outDir = reflect_about(rd, sN);
       else refracted = true;
     }
-  } else if (f.diffuse || f.toon) {
-    outDir = sample_hemisphere(rng, sN);
   } else {
-    const float effR = smax(0.025f, roughness * (f.metallic ? 0.75f : 1.0f) * (f.clearcoat ? 0.65f : 1.0f));
-    const float a2 = effR * effR;
-    const float expt = smax(0.0f, 2.0f / (a2 * a2) - 2.0f);
-    const V3 refl = reflect_about(rd, sN);
-    outDir = sample_phong_lobe(rng, refl, expt, sN);
-    if (f.sheen && next01(rng) < m.sheen * 0.35f) outDir = sample_hemisphere(rng, sN);
+    // sample_hemisphere (diffuse, toon; ScalarCpuPathTracer.cpp:978-988) or sample_phong_lobe (the
+    // rest; :996-1020). Both draw u1 then u2, turn u2
+    // into the angle 2 pi u2 and place (s cos, s sin, z) in a frame around an axis (the normal, or the
+    // mirror direction) with the same operations, so the angle's sine and cosine and the frame are
+    // evaluated once for lanes of either kind.
+    const bool hemi = f.diffuse || f.toon;
+    float effR = 0.0f, expt = 0.0f;
+    V3 refl = sN;
+    const float u1 = next01(rng);
+    const float u2 = next01(rng);
+    float radial, lz;
+    V3 axis;
+    if (hemi) {
+      radial = __builtin_sqrtf(smax(0.0f, 1.0f - u1));
+      lz = __builtin_sqrtf(smax(0.0f, u1));
+      axis = sN;
+    } else {
+      effR = smax(0.025f, roughness * (f.metallic ? 0.75f : 1.0f) * (f.clearcoat ? 0.65f : 1.0f));
+      const float a2 = effR * effR;
+      expt = smax(0.0f, 2.0f / (a2 * a2) - 2.0f);
+      refl = reflect_about(rd, sN);
+      lz = ptm::powf_p(smax(0.0f, u1), 1.0f / (expt + 1.0f));  // cos theta of the lobe
+      radial = __builtin_sqrtf(smax(0.0f, 1.0f - lz * lz));
+      axis = refl;
+    }
+    const float phi = 2.0f * kPi * u2;
+    float sp, cp;
+    ptm::sincosf_p(phi, sp, cp);
+    const V3 ref_t = (__builtin_fabsf(axis.z) < 0.999f) ? mk(0.0f, 0.0f, 1.0f) : mk(0.0f, 1.0f, 0.0f);
+    const V3 tang = normalize(cross(ref_t, axis));
+    const V3 btan = cross(axis, tang);
+    outDir = normalize(add(add(muls(tang, radial * cp), muls(btan, radial * sp)), muls(axis, lz)));
+    if (!hemi) {
+      // the rest of sample_phong_lobe: a sample below the surface is mirrored, else resampled
+      if (dot(outDir, sN) <= 0.0f) {
+        outDir = sub(outDir, muls(sN, 2.0f * dot(outDir, sN)));
+        if (dot(outDir, sN) <= 0.0f) outDir = sample_hemisphere(rng, sN);
+      }
+      if (f.sheen && next01(rng) < m.sheen * 0.35f) outDir = sample_hemisphere(rng, sN);
+    }
     if constexpr (kAdj) {
       const double kk = (f.metallic ? 0.75 : 1.0) * (f.clearcoat ? 0.65 : 1.0);
       const double ps = f.sheen ? fmin(1.0, fmax(0.0, static_cast<double>(m.sheen * 0.35f))) : 0.0;
-      if (m.roughness > 0.0f && m.roughness < 1.0f && static_cast<double>(roughness) * kk > 0.025 && expt > 0.0f)
+      if (!hemi && m.roughness > 0.0f && m.roughness < 1.0f && static_cast<double>(roughness) * kk > 0.025 &&
+          expt > 0.0f)
         score = static_cast<float>(phong_score(static_cast<double>(expt), refl, sN, outDir, ps) *
                                    (-8.0 / pow(static_cast<double>(effR), 5.0)) * kk);
     }
