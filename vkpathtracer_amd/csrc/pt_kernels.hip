@@ -858,13 +858,11 @@ __global__ __launch_bounds__(kBlock) void k_raygen(DevPaths P, DevRays out, DevS
     P.rad[0][p] = 0.0f; P.rad[1][p] = 0.0f; P.rad[2][p] = 0.0f;
     P.rng[p] = prng;
   } else {
-    // the initial queue is the identity (entry p serves path p)
+    // the initial queue is the identity (entry p serves path p); settings keep max_depth >= 1, so
+    // every path ends in some bounce's shade, which writes its radiance to its slot
     out.tp[0][p] = 1.0f; out.tp[1][p] = 1.0f; out.tp[2][p] = 1.0f;
     out.rad[0][p] = 0.0f; out.rad[1][p] = 0.0f; out.rad[2][p] = 0.0f;
     out.rng[p] = prng;
-    if (st.max_depth == 0u) {  // no bounce will end the path: its radiance stays 0
-      P.rad[0][p] = 0.0f; P.rad[1][p] = 0.0f; P.rad[2][p] = 0.0f;
-    }
   }
   if constexpr (kAdj) {
     const size_t pix = rec.film_base + static_cast<size_t>(y) * rec.width + x;
