@@ -231,7 +231,8 @@ def adjoint_ratio(log, reps: int = 3) -> dict:
     emission and the light colour as parameters. forward = reset + render_tiles over the 64 sample
     tiles. backward = render_adjoint of the same tiles right after that forward, as in an autograd
     step: it reuses each path's forward radiance and replays the paths once, recomputing every vertex.
-    standalone = render_adjoint with no forward before it (forward replay + adjoint replay).
+    standalone = render_adjoint of a frame that no forward rendered (frame 1: forward replay + adjoint
+    replay; an adjoint keeps the forward radiance it reused, so a repeated call would reuse it again).
     ratio = (forward + backward) / forward: the time of a forward and its adjoint over the forward's
     alone; standalone_ratio = standalone / forward; backward_over_forward = backward / forward."""
     import torch
@@ -255,8 +256,8 @@ def adjoint_ratio(log, reps: int = 3) -> dict:
             assert pt.render_tiles(tiles)
             pt.synchronize()
 
-        def adj():
-            pt.render_adjoint(tiles, g, ga, ge, gl)
+        def adj(frame=0):
+            pt.render_adjoint(tiles, g, ga, ge, gl, frame)
 
         def timed(fn):
             torch.cuda.synchronize()
@@ -271,7 +272,7 @@ def adjoint_ratio(log, reps: int = 3) -> dict:
         for _ in range(reps):
             f_ms.append(timed(fwd))
             b_ms.append(timed(adj))   # right after the forward: reuses its radiance
-            s_ms.append(timed(adj))   # nothing to reuse: replays forward, then adjoint
+            s_ms.append(timed(lambda: adj(1)))  # nothing to reuse: replays forward, then adjoint
     f, b, sa = min(f_ms), min(b_ms), min(s_ms)
     log(f"forward {f:.1f} ms, backward {b:.1f} ms, standalone adjoint {sa:.1f} ms")
     return {"workload": "cornell box 512x512, 64 spp, depth 8, d/d(35 albedos, 35 emissions, light colour)",
@@ -286,8 +287,9 @@ def textured_multiview(log, reps: int = 2, views: int = 16, size: int = 512, spp
     lights), `views` cameras on a ring in one batch (the scene plus views-1 views), size^2 x spp each,
     depth 6. forward = reset + pt_batch_render of every view's frame; backward = pt_batch_render_adjoint
     of the same frame right after it (d/d albedo, emission, light colour, roughness and every texel,
-    fp32 atomics; the forward radiance of the single pass is reused); standalone = the adjoint with no
-    forward before it. ratio = (forward + backward) / forward; standalone_ratio = standalone / forward."""
+    fp32 atomics; the forward radiance of the single pass is reused); standalone = the adjoint of a frame
+    no forward rendered (frame 1). ratio = (forward + backward) / forward; standalone_ratio = standalone /
+    forward."""
     import torch
 
     from vkpathtracer_amd import scene as S
@@ -321,8 +323,8 @@ def textured_multiview(log, reps: int = 2, views: int = 16, size: int = 512, spp
             b.render(0)
             b.synchronize()
 
-        def adj():
-            b.render_adjoint(g, grads[0], grads[1], grads[2], 0, grad_roughness=grads[3], grad_texels=grads[4])
+        def adj(frame=0):
+            b.render_adjoint(g, grads[0], grads[1], grads[2], frame, grad_roughness=grads[3], grad_texels=grads[4])
 
         fwd()
         adj()
@@ -330,7 +332,7 @@ def textured_multiview(log, reps: int = 2, views: int = 16, size: int = 512, spp
         for _ in range(reps):
             f_ms.append(timed(fwd))
             b_ms.append(timed(adj))
-            s_ms.append(timed(adj))
+            s_ms.append(timed(lambda: adj(1)))
         counters = b.read_counters()
     f, bk, sa = min(f_ms), min(b_ms), min(s_ms)
     log(f"textured multi-view: forward {f:.1f} ms, backward {bk:.1f} ms, standalone adjoint {sa:.1f} ms")

@@ -167,8 +167,10 @@ def test_autograd_backward_uses_its_own_forward_parameters():
 
 def test_adjoint_reuses_forward_radiance_only_when_valid():
     """A render_tiles of the same tiles leaves each path's radiance in place and the adjoint skips its
-    forward replay; after update_scene_delta it must replay. Gradients agree with a fresh tracer's
-    (up to the order of the fp32 atomic adds)."""
+    forward replay, as does a second adjoint (the replay leaves that radiance alone); after
+    update_scene_delta, or after an adjoint of another frame replayed that frame's forward in the same
+    slots, it must replay. Gradients agree with a fresh tracer's (up to the order of the fp32 atomic
+    adds)."""
     import torch
 
     from vkpathtracer_amd import scene as S
@@ -183,11 +185,11 @@ def test_adjoint_reuses_forward_radiance_only_when_valid():
     m2["albedo"] = np.clip(m2["albedo"] * 0.8 + 0.1, 0.05, 0.95)
     sc2 = dataclasses.replace(sc, materials=m2)
 
-    def grads(pt, nm, nl):
+    def grads(pt, nm, nl, frame=0):
         ga = torch.zeros((nm, 3), device=dev)
         ge = torch.zeros((nm, 3), device=dev)
         gl = torch.zeros((nl, 3), device=dev)
-        pt.render_adjoint(tiles, torch.from_numpy(fg).to(dev), ga, ge, gl)
+        pt.render_adjoint(tiles, torch.from_numpy(fg).to(dev), ga, ge, gl, frame)
         return [x.cpu().numpy().astype(np.float64) for x in (ga, ge, gl)]
 
     def close(a, b):
@@ -201,7 +203,10 @@ def test_adjoint_reuses_forward_radiance_only_when_valid():
         nm, nl = len(sc.materials), len(sc.lights)
         assert pt.render_tiles(tiles)
         close(grads(pt, nm, nl), fresh1)           # reuse
-        close(grads(pt, nm, nl), fresh1)           # no forward in between: replays
+        close(grads(pt, nm, nl), fresh1)           # no forward in between: reuses again
+        g1 = grads(pt, nm, nl, 1)                  # frame 1: replays its forward into the slots
+        assert not all(np.array_equal(x, y) for x, y in zip(g1, fresh1))
+        close(grads(pt, nm, nl), fresh1)           # frame 0 again: replays
         assert pt.render_tiles(tiles)
         assert pt.update_scene_delta(materials=m2)
         close(grads(pt, nm, nl), fresh2)           # scene changed after the render: replays

@@ -368,58 +368,59 @@ void free_paths(pt_tracer* t) {
 
 // Adjoint state: 33 + 2 * kAdjHist float arrays and 4 + kAdjHist u32 arrays of max(cap, kSegments *
 // seg_cap) entries.
-constexpr size_t kAdjFloats = 33 + 2 * ptd::kAdjHist;
+// Adjoint state: two buffers of queue-ordered items, then the shadow-record and history arrays.
+constexpr size_t kAdjFloats = 2 * ptd::kItemFloats + 3 + 3 + 3 * ptd::kAdjHist + 1 + 2 + 2 * ptd::kAdjHist;
+constexpr size_t kAdjWords = 2 * ptd::kItemWords + 2 + ptd::kAdjHist;
 int ensure_adjoint(pt_tracer* t, uint32_t cap, ptd::DevAdjoint& A) {
   if (cap > t->adj_capacity_alloc) {
     t->d_adj_f.release();
     t->d_adj_u.release();
     t->adj_capacity_alloc = 0;
     const size_t m = std::max<size_t>(cap, static_cast<size_t>(ptd::kSegments) * ptd::segment_capacity(cap));
-    PT_HIP(t, t->d_adj_f.ensure(static_cast<size_t>(kAdjFloats) * m), "allocate adjoint state");
-    PT_HIP(t, t->d_adj_u.ensure(static_cast<size_t>(8) * m), "allocate adjoint state");
+    PT_HIP(t, t->d_adj_f.ensure(kAdjFloats * m), "allocate adjoint state");
+    PT_HIP(t, t->d_adj_u.ensure(kAdjWords * m), "allocate adjoint state");
     t->adj_capacity_alloc = cap;
   }
   const size_t n = std::max<size_t>(t->adj_capacity_alloc,
                                     static_cast<size_t>(ptd::kSegments) * ptd::segment_capacity(t->adj_capacity_alloc));
   float* f = t->d_adj_f.ptr;
   uint32_t* u = t->d_adj_u.ptr;
-  for (int c = 0; c < 3; ++c) {
-    A.ltot[c] = f + (0 + c) * n;
-    A.delta[c] = f + (3 + c) * n;
-    A.p_dx[c] = f + (6 + c) * n;
-    A.s_galb[c] = f + (9 + c) * n;
-    A.s_glight[c] = f + (12 + c) * n;
-    for (int h = 0; h < ptd::kAdjHist; ++h) A.h_dx[h][c] = f + (15 + 3 * h + c) * n;
+  for (int b = 0; b < 2; ++b) {
+    A.items[b].f = f + static_cast<size_t>(b) * ptd::kItemFloats * n;
+    A.items[b].u = u + static_cast<size_t>(b) * ptd::kItemWords * n;
   }
-  A.p_score = f + 27 * n;
-  A.s_grough = f + 28 * n;
+  float* fs = f + 2 * ptd::kItemFloats * n;  // shadow-record and history floats
+  for (int c = 0; c < 3; ++c) {
+    A.s_galb[c] = fs + (0 + c) * n;
+    A.s_glight[c] = fs + (3 + c) * n;
+    for (int h = 0; h < ptd::kAdjHist; ++h) A.h_dx[h][c] = fs + (6 + 3 * h + c) * n;
+  }
+  A.s_grough = fs + (6 + 3 * ptd::kAdjHist) * n;
   for (int k = 0; k < 2; ++k) {
-    A.p_uv[k] = f + (29 + k) * n;
-    A.s_uv[k] = f + (31 + k) * n;
-    for (int h = 0; h < ptd::kAdjHist; ++h) A.h_uv[h][k] = f + (33 + 2 * h + k) * n;
+    A.s_uv[k] = fs + (7 + 3 * ptd::kAdjHist + k) * n;
+    for (int h = 0; h < ptd::kAdjHist; ++h) A.h_uv[h][k] = fs + (9 + 3 * ptd::kAdjHist + 2 * h + k) * n;
   }
   A.stride = static_cast<uint32_t>(n);
-  A.p_mat = u;
-  A.flags = u + n;
-  A.s_mat = u + 2 * n;
-  A.s_light = u + 3 * n;
-  for (int h = 0; h < ptd::kAdjHist; ++h) A.h_mat[h] = u + (4 + h) * n;
+  uint32_t* us = u + 2 * ptd::kItemWords * n;
+  A.s_mat = us;
+  A.s_light = us + n;
+  for (int h = 0; h < ptd::kAdjHist; ++h) A.h_mat[h] = us + (2 + h) * n;
   return PT_OK;
 }
 
-// Path-state and queue arrays for `cap` paths: 41 float arrays, 7 u32 arrays, 3 u64 arrays, each
+// Path-state and queue arrays for `cap` paths: 38 float arrays, 8 u32 arrays, 2 u64 arrays, each
 // of max(cap, kSegments * seg_cap) entries (queue arrays are indexed by segment position).
 int ensure_paths(pt_tracer* t, uint32_t cap) {
   if (cap <= t->path_capacity_alloc) return PT_OK;
   free_paths(t);
-  constexpr size_t kF = 41, kU = 7;
+  constexpr size_t kF = 38, kU = 8;
   const uint32_t seg_cap = ptd::segment_capacity(cap);
   const size_t n = std::max<size_t>(cap, static_cast<size_t>(ptd::kSegments) * seg_cap);
   // shadow records name a queue entry with the top bit set (ptd::kRadQueued)
   if (n >= static_cast<size_t>(ptd::kRadQueued)) return fail(t, PT_INVALID_ARGUMENT, "wavefront capacity too large");
   PT_HIP(t, t->d_path_f.ensure(kF * n), "allocate path state");
   PT_HIP(t, t->d_path_u.ensure(kU * n), "allocate path state");
-  PT_HIP(t, t->d_rng.ensure(3 * n), "allocate path state");
+  PT_HIP(t, t->d_rng.ensure(2 * n), "allocate path state");
   float* f = t->d_path_f.ptr;
   uint32_t* u = t->d_path_u.ptr;
   ptd::DevPaths& P = t->paths;
@@ -428,23 +429,22 @@ int ensure_paths(pt_tracer* t, uint32_t cap) {
     P.rays[b].qpath = u + (size_t)b * n;
     P.rays[b].qscene = u + (size_t)(4 + b) * n;
     for (int k = 0; k < 3; ++k) {
-      P.rays[b].tp[k] = f + (size_t)(29 + 6 * b + k) * n;
-      P.rays[b].rad[k] = f + (size_t)(32 + 6 * b + k) * n;
+      P.rays[b].tp[k] = f + (size_t)(26 + 6 * b + k) * n;
+      P.rays[b].rad[k] = f + (size_t)(29 + 6 * b + k) * n;
     }
-    P.rays[b].rng = t->d_rng.ptr + (size_t)(1 + b) * n;
+    P.rays[b].rng = t->d_rng.ptr + (size_t)b * n;
   }
   P.hit_t = f + 12 * n;
   P.hit_prim = reinterpret_cast<int32_t*>(u + 2 * n);
   for (int k = 0; k < 3; ++k) {
-    P.tp[k] = f + (size_t)(13 + k) * n;
-    P.rad[k] = f + (size_t)(16 + k) * n;
-    P.scontrib[k] = f + (size_t)(26 + k) * n;
+    P.rad[k] = f + (size_t)(13 + k) * n;
+    P.scontrib[k] = f + (size_t)(23 + k) * n;
   }
-  for (int k = 0; k < 6; ++k) P.sray[k] = f + (size_t)(19 + k) * n;
-  P.stmax = f + 25 * n;
+  for (int k = 0; k < 6; ++k) P.sray[k] = f + (size_t)(16 + k) * n;
+  P.stmax = f + 22 * n;
   P.spath = u + 3 * n;
   P.sscene = u + (size_t)6 * n;
-  P.rng = t->d_rng.ptr;
+  P.ended_rays = u + (size_t)7 * n;
   P.capacity = cap;
   P.seg_cap = seg_cap;
   t->path_capacity_alloc = cap;
@@ -1141,7 +1141,7 @@ void bounce_loop(pt_tracer* t, const ptd::DevPaths& P, const ptd::DevSettings& s
     });
     if (st.enable_nee && depth + 1u < st.max_depth)
       timed(t, PT_KERNEL_SHADOW, 0, [&] {
-        if (adj) ptd::launch_shadow_adjoint(t->stream, P, *adj, t->dscene, t->grids.shadow_adj);
+        if (adj) ptd::launch_shadow_adjoint(t->stream, P, *adj, t->dscene, buf, t->grids.shadow_adj);
         else ptd::launch_shadow(t->stream, P, t->dscene, buf, t->grids.shadow);
       });
     ptd::launch_advance(t->stream, P);
@@ -1188,8 +1188,9 @@ int launch_passes(pt_tracer* t, const std::vector<ptd::DevTile>& work, const std
       bounce_loop(t, P, st, nullptr);
     }
     if (adj) {
-      t->fwd_rad_valid = false;  // the replay below overwrites P.rad
-      ptd::launch_copy_radiance(t->stream, P, *adj, ps.paths);
+      // The adjoint replay leaves the path slots' radiance alone; a forward replay above has replaced it
+      // with this pass's, which belongs to no recorded forward render.
+      if (!reuse) t->fwd_rad_valid = false;
       ptd::launch_init_queue(t->stream, P, ps.paths);
       timed(t, PT_KERNEL_RAYGEN, ps.paths,
             [&] { ptd::launch_raygen_adjoint(t->stream, P, *adj, t->dscene, st, tl, nt, ps.paths); });

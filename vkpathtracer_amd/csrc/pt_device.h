@@ -191,22 +191,23 @@ struct DevRays {
 // path's queued continuation ray (kRadQueued | its entry in the output queue of the bounce), or, when
 // the path ends at this bounce, DevPaths::rad at its path slot.
 constexpr uint32_t kRadQueued = 0x80000000u;
+// Adjoint replays: the path ended at this bounce and its replayed radiance is no longer needed.
+constexpr uint32_t kRadNone = 0xFFFFFFFFu;
 struct RadTarget {
   float* rad[3];  // the output queue's radiance arrays
 };
 
 
-// Per-path wavefront state. Ray and hit buffers are in queue order (double-buffered rays);
-// throughput, radiance and RNG state indexed by the path's slot in the pass serve the adjoint replay,
-// and in forward passes hold only what a path leaves when it ends: its radiance, and in tp[0] the
-// number of extension rays it traced (for k_film's ray count of dropped samples).
+// Per-path wavefront state. Ray and hit buffers, and a live path's state, are in queue order
+// (double-buffered rays); indexed by the path's slot in the pass is only what a forward path leaves
+// when it ends: its radiance and the number of extension rays it traced (for k_film's ray count of
+// dropped samples).
 struct DevPaths {
   DevRays rays[2];
   float* hit_t;
   int32_t* hit_prim;
-  float* tp[3];          // throughput
-  float* rad[3];         // radiance
-  uint64_t* rng;
+  float* rad[3];          // radiance of ended paths
+  uint32_t* ended_rays;   // extension rays of ended paths
   float* sray[6];        // shadow queue: origin xyz, direction xyz
   float* stmax;
   float* scontrib[3];
@@ -240,29 +241,36 @@ struct DevResolve {
   uint32_t clamp_out;
 };
 
-// Path-replay adjoint state (no counterpart in the reference). Per path: forward radiance, the
-// loss adjoint of that radiance, the bounce awaiting its remainder radiance, and the history of
-// vertices before the roulette start depth (see k_shade). Per shadow record: the gradient pieces
-// that apply if the light turns out visible.
+// Path-replay adjoint state (no counterpart in the reference). Per queue entry (items): forward
+// radiance, the loss adjoint of that radiance and the bounce awaiting its remainder radiance; per path
+// slot: the history of vertices before the roulette start depth (see k_shade). Per shadow record: the
+// gradient pieces that apply if the light turns out visible.
 constexpr int kAdjHist = 4;          // max roulette start depth supported by the adjoint
 constexpr int kAdjLdsFloats = 6144;  // block-staged gradient slots, all copies (24 KB of LDS)
+// A path's per-bounce adjoint state travels with its queue entry, double-buffered like the ray queues
+// (items[b] belongs to DevPaths::rays[b]): `stride` entries apart, the float arrays forward radiance
+// ltot xyz (kItemLtot), loss adjoint delta xyz (kItemDelta), the pending bounce's d log(bounce weight)
+// / d albedo xyz (kItemPdx), its roughness score (kItemScore) and texture coordinate (kItemUv), and the
+// u32 arrays pending material (kItemMat) and flags (kItemFlags: bit0 pending, bit1 active, bits2-3 c*,
+// bits4-6 nhist, bit7 prev active, bits8-9 prev c*, bits10-12 prev nhist).
+enum : uint32_t { kItemLtot = 0, kItemDelta = 3, kItemPdx = 6, kItemScore = 9, kItemUv = 10, kItemFloats = 12 };
+enum : uint32_t { kItemMat = 0, kItemFlags = 1, kItemWords = 2 };
+struct AdjItems {
+  float* f;
+  uint32_t* u;
+};
 struct DevAdjoint {
-  float* ltot[3];
-  float* delta[3];
-  float* p_dx[3];
-  uint32_t* p_mat;
-  uint32_t* flags;  // bit0 pending, bit1 active, bits2-3 c*, bits4-6 nhist, bit7 prev active,
-                    // bits8-9 prev c*, bits10-12 prev nhist
+  AdjItems items[2];
   float* h_dx[kAdjHist][3];   // h_dx[h][c] = h_dx[0][0] + (3h + c) * stride
   uint32_t* h_mat[kAdjHist];  // h_mat[h] = h_mat[0] + h * stride
   float* s_galb[3];
   uint32_t* s_mat;
   float* s_glight[3];
   uint32_t* s_light;
-  float* p_score;   // d log(density of the pending bounce's direction) / d roughness
   float* s_grough;  // per shadow record: d contribution / d roughness, weighted by the loss adjoint
-  float* p_uv[2];   // texture coordinates of the pending bounce's, the history's and the shadow
-  float* h_uv[kAdjHist][2];  // records' vertices (read for textured materials only); h_uv[h][k] = h_uv[0][0] + (2h + k) * stride
+  // texture coordinates of the history's and the shadow records' vertices (read for textured materials
+  // only); h_uv[h][k] = h_uv[0][0] + (2h + k) * stride
+  float* h_uv[kAdjHist][2];
   float* s_uv[2];
   float* g_albedo;    // 3 per material
   float* g_emission;  // 3 per material
@@ -271,7 +279,7 @@ struct DevAdjoint {
   float* g_tex;       // 3 per texel of the texture pool (nullptr: not requested)
   const float* film_grad;  // 3 per film pixel (all scenes' films in pool order): dLoss / d(film sum)
   uint32_t nmat, nlight, use_lds;
-  uint32_t stride;  // entries per adjoint state array
+  uint32_t stride;  // entries per adjoint state array (history arrays by path slot, items by queue entry)
   uint32_t lds_copies, lds_stride;  // LDS staging: copies (a power of two) of the slots, stride apart
   // LDS staging: block b of every adjoint kernel adds its slots into row b of `partial` (plain
   // read-modify-write: kernels of one stream never overlap, and rows are private to block indices);
@@ -300,15 +308,14 @@ void launch_trace_rays(hipStream_t stream, const DevPaths& paths, const DevScene
 // The same queries, each storing a whole RayQueryHit (inst_mat: the instance material of each primitive).
 void launch_intersect(hipStream_t stream, const DevPaths& paths, const DevScene& scene, int any_hit, const float* rays,
                       const float* tmax, uint32_t count, PtRayQueryHit* hits, const uint32_t* inst_mat, uint32_t grid);
-// Adjoint variants: raygen also gathers the per-path loss adjoint; shade and shadow accumulate
-// gradients; copy_radiance saves the forward radiance of a pass before the replay.
+// Adjoint variants: raygen also gathers each path's loss adjoint and the forward radiance the pass left
+// in the path slots; shade and shadow accumulate gradients.
 void launch_raygen_adjoint(hipStream_t stream, const DevPaths& paths, const DevAdjoint& adj, const DevScene& scene,
                            const DevSettings& st, const DevTile* tiles, uint32_t tile_count, uint32_t path_count);
 void launch_shade_adjoint(hipStream_t stream, const DevPaths& paths, const DevAdjoint& adj, const DevScene& scene,
                           const DevSettings& st, int in_buf, uint32_t depth, uint32_t grid);
 void launch_shadow_adjoint(hipStream_t stream, const DevPaths& paths, const DevAdjoint& adj, const DevScene& scene,
-                           uint32_t grid);
-void launch_copy_radiance(hipStream_t stream, const DevPaths& paths, const DevAdjoint& adj, uint32_t path_count);
+                           int in_buf, uint32_t grid);
 void launch_grad_clear(hipStream_t stream, const DevAdjoint& adj);   // zero the partial rows
 void launch_grad_reduce(hipStream_t stream, const DevAdjoint& adj);  // rows -> gradient buffers
 constexpr uint32_t kGradRows = 1024;  // partial rows: an upper bound on every adjoint kernel's grid

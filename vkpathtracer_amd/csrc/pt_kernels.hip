@@ -853,32 +853,23 @@ __global__ __launch_bounds__(kBlock) void k_raygen(DevPaths P, DevRays out, DevS
   out.c[3][p] = rd.x; out.c[4][p] = rd.y; out.c[5][p] = rd.z;
   out.qpath[p] = p;
   out.qscene[p] = tile.scene;
+  // The initial queue is the identity (entry p serves path p). Settings keep max_depth >= 1, so every
+  // forward path ends in some bounce's shade, which writes its radiance to its slot.
+  out.tp[0][p] = 1.0f; out.tp[1][p] = 1.0f; out.tp[2][p] = 1.0f;
+  out.rad[0][p] = 0.0f; out.rad[1][p] = 0.0f; out.rad[2][p] = 0.0f;
+  out.rng[p] = prng;
   if constexpr (kAdj) {
-    P.tp[0][p] = 1.0f; P.tp[1][p] = 1.0f; P.tp[2][p] = 1.0f;
-    P.rad[0][p] = 0.0f; P.rad[1][p] = 0.0f; P.rad[2][p] = 0.0f;
-    P.rng[p] = prng;
-  } else {
-    // the initial queue is the identity (entry p serves path p); settings keep max_depth >= 1, so
-    // every path ends in some bounce's shade, which writes its radiance to its slot
-    out.tp[0][p] = 1.0f; out.tp[1][p] = 1.0f; out.tp[2][p] = 1.0f;
-    out.rad[0][p] = 0.0f; out.rad[1][p] = 0.0f; out.rad[2][p] = 0.0f;
-    out.rng[p] = prng;
-  }
-  if constexpr (kAdj) {
+    // the replay's items (buffer 0, beside the initial queue): the forward pass's radiance of the path,
+    // which it left in the path's slot, and the loss adjoint of its pixel
     const size_t pix = rec.film_base + static_cast<size_t>(y) * rec.width + x;
-    A.delta[0][p] = A.film_grad[3 * pix];
-    A.delta[1][p] = A.film_grad[3 * pix + 1];
-    A.delta[2][p] = A.film_grad[3 * pix + 2];
-    A.flags[p] = 0u;
+    float* f = A.items[0].f;
+    const size_t S = A.stride;
+    for (int c = 0; c < 3; ++c) {
+      f[(kItemLtot + c) * S + p] = P.rad[c][p];
+      f[(kItemDelta + c) * S + p] = A.film_grad[3 * pix + c];
+    }
+    A.items[0].u[kItemFlags * S + p] = 0u;
   }
-}
-
-__global__ __launch_bounds__(kBlock) void k_copy_radiance(DevPaths P, DevAdjoint A, uint32_t n) {
-  const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
-  if (p >= n) return;
-  A.ltot[0][p] = P.rad[0][p];
-  A.ltot[1][p] = P.rad[1][p];
-  A.ltot[2][p] = P.rad[2][p];
 }
 
 // ------------------------------------------------------------------ adjoint gradient accumulation
@@ -1337,18 +1328,54 @@ __device__ __forceinline__ V3 channel(int c, float v) {
   return mk(c == 0 ? v : 0.0f, c == 1 ? v : 0.0f, c == 2 ? v : 0.0f);
 }
 
-// Apply the bounce that waited for its remainder radiance R = L_total - L_so_far
-// (df/dx_j = delta*R_j plus the Russian-roulette terms; see the oracle's derivation).
+// A path's adjoint item, loaded from its queue entry (AdjItems) for one shade.
+struct AdjItem {
+  float ltot[3], delta[3], pdx[3], score, uv[2];
+  uint32_t mat, flags;
+};
+
+__device__ __forceinline__ AdjItem load_item(const AdjItems& it, size_t stride, uint32_t q) {
+  AdjItem a;
+  for (int c = 0; c < 3; ++c) {
+    a.ltot[c] = it.f[(kItemLtot + c) * stride + q];
+    a.delta[c] = it.f[(kItemDelta + c) * stride + q];
+    a.pdx[c] = it.f[(kItemPdx + c) * stride + q];
+  }
+  a.score = it.f[kItemScore * stride + q];
+  a.uv[0] = it.f[kItemUv * stride + q];
+  a.uv[1] = it.f[(kItemUv + 1) * stride + q];
+  a.mat = it.u[kItemMat * stride + q];
+  a.flags = it.u[kItemFlags * stride + q];
+  return a;
+}
+
+__device__ __forceinline__ void store_item(const AdjItems& it, size_t stride, uint32_t q, const AdjItem& a) {
+  for (int c = 0; c < 3; ++c) {
+    it.f[(kItemLtot + c) * stride + q] = a.ltot[c];
+    it.f[(kItemDelta + c) * stride + q] = a.delta[c];
+    it.f[(kItemPdx + c) * stride + q] = a.pdx[c];
+  }
+  it.f[kItemScore * stride + q] = a.score;
+  it.f[kItemUv * stride + q] = a.uv[0];
+  it.f[(kItemUv + 1) * stride + q] = a.uv[1];
+  it.u[kItemMat * stride + q] = a.mat;
+  it.u[kItemFlags * stride + q] = a.flags;
+}
+
+// Applies the bounce of item `a` (path slot p) that waited for its remainder radiance R = L_total -
+// L_so_far: df/dx_j = delta * R_j for its albedo / texels and roughness, plus the Russian-roulette terms
+// of the history vertices through rr's channel while the roulette derivative is active.
 __device__ __forceinline__ void adj_flush(const DevAdjoint& A, const DevScene& sc, float* lds, uint32_t p, V3 L,
-                                          uint32_t& flags) {
+                                          AdjItem& a) {
+  uint32_t& flags = a.flags;
   if (!(flags & 1u)) return;
-  const float dl[3] = {A.delta[0][p], A.delta[1][p], A.delta[2][p]};
-  const float R[3] = {A.ltot[0][p] - L.x, A.ltot[1][p] - L.y, A.ltot[2][p] - L.z};
+  const float* dl = a.delta;
+  const float R[3] = {a.ltot[0] - L.x, a.ltot[1] - L.y, a.ltot[2] - L.z};
   const float S = dl[0] * R[0] + dl[1] * R[1] + dl[2] * R[2];
-  const uint32_t m = A.p_mat[p];
-  gadd_base(A, sc, lds, m, make_float2(A.p_uv[0][p], A.p_uv[1][p]),
-            mk(dl[0] * R[0] * A.p_dx[0][p], dl[1] * R[1] * A.p_dx[1][p], dl[2] * R[2] * A.p_dx[2][p]));
-  gadd(A, lds, kGradRough, m, 0, S * A.p_score[p]);
+  const uint32_t m = a.mat;
+  gadd_base(A, sc, lds, m, make_float2(a.uv[0], a.uv[1]),
+            mk(dl[0] * R[0] * a.pdx[0], dl[1] * R[1] * a.pdx[1], dl[2] * R[2] * a.pdx[2]));
+  gadd(A, lds, kGradRough, m, 0, S * a.score);
   if (flags & 2u) {
     const int cs = static_cast<int>((flags >> 2) & 3u);
     const uint32_t nh = (flags >> 4) & 7u;
@@ -1708,16 +1735,16 @@ template <bool kAdj>
 __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc, const DevSettings& st, uint32_t depth,
                                            uint32_t p, V3 ro, V3 rd, float t, int32_t prim, V3& T, V3& L,
                                            uint64_t& rng, ShadeOut& out, uint32_t& shadow_tests, const DevAdjoint& A,
-                                           float* glds) {
+                                           float* glds, AdjItem& ai) {
   out.cont = false;
   out.shadow = false;
   uint32_t aflags = 0;
   V3 dl = mk(0.0f, 0.0f, 0.0f);
   if constexpr (kAdj) {
-    if (!(finite(A.ltot[0][p]) && finite(A.ltot[1][p]) && finite(A.ltot[2][p]))) return;  // dropped sample
-    aflags = A.flags[p];
-    adj_flush(A, sc, glds, p, L, aflags);
-    dl = mk(A.delta[0][p], A.delta[1][p], A.delta[2][p]);
+    if (!(finite(ai.ltot[0]) && finite(ai.ltot[1]) && finite(ai.ltot[2]))) return;  // dropped sample
+    adj_flush(A, sc, glds, p, L, ai);
+    aflags = ai.flags;
+    dl = mk(ai.delta[0], ai.delta[1], ai.delta[2]);
   }
   if (prim == -1) {
     L = add(L, mulv(T, environment(sc, rd)));
@@ -2014,12 +2041,12 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
       }
       const float pre_c = c == 0 ? eff_pre.x : (c == 1 ? eff_pre.y : eff_pre.z);
       dx[c] = b3[c] > 0.0f ? v * eff_grad(pre_c, eff_k) : 0.0f;
-      A.p_dx[c][p] = dx[c];
+      ai.pdx[c] = dx[c];
     }
-    A.p_mat[p] = mat_index;
-    A.p_uv[0][p] = huv.x;
-    A.p_uv[1][p] = huv.y;
-    A.p_score[p] = score;
+    ai.mat = mat_index;
+    ai.uv[0] = huv.x;
+    ai.uv[1] = huv.y;
+    ai.score = score;
     const bool active = depth < st.rr_start && st.rr_min < maxT && maxT < st.rr_max;
     aflags = (aflags & ~0xFu) | 1u | (active ? 2u : 0u) | (static_cast<uint32_t>(cstar) << 2);
     uint32_t nh = (aflags >> 4) & 7u;
@@ -2031,7 +2058,7 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
       ++nh;
       aflags = (aflags & ~0x70u) | (nh << 4);
     }
-    A.flags[p] = aflags;
+    ai.flags = aflags;
   }
   T = divs(T, rr);
   out.cont = true;
@@ -2041,7 +2068,8 @@ __device__ __forceinline__ void shade_path(const DevPaths& P, const DevScene& sc
 
 template <bool kAdj>
 __device__ __forceinline__ void shade_kernel(const DevPaths& P, const DevRays& in, const DevRays& out, const DevScene& sc,
-                                             const DevSettings& st, uint32_t depth, const DevAdjoint& A, float* glds) {
+                                             const DevSettings& st, uint32_t depth, const DevAdjoint& A, float* glds,
+                                             uint32_t in_buf) {
   __shared__ uint32_t prefix[kSegments + 1];
   if constexpr (kAdj) {
     tex_table_init(glds);
@@ -2060,36 +2088,25 @@ __device__ __forceinline__ void shade_kernel(const DevPaths& P, const DevRays& i
     uint32_t p = 0, sid = 0;
     V3 T = mk(0.0f, 0.0f, 0.0f), L = T;
     uint64_t rng = 0;
+    AdjItem ai;
     if (i < n) {
       const uint32_t q = queue_pos(prefix, i, P.seg_cap);
       p = in.qpath[q];
       sid = in.qscene[q];
       const V3 ro = mk(in.c[0][q], in.c[1][q], in.c[2][q]);
       const V3 rd = mk(in.c[3][q], in.c[4][q], in.c[5][q]);
-      if constexpr (kAdj) {
-        T = mk(P.tp[0][p], P.tp[1][p], P.tp[2][p]);
-        L = mk(P.rad[0][p], P.rad[1][p], P.rad[2][p]);
-        rng = P.rng[p];
-      } else {
-        T = mk(in.tp[0][q], in.tp[1][q], in.tp[2][q]);
-        L = mk(in.rad[0][q], in.rad[1][q], in.rad[2][q]);
-        rng = in.rng[q];
-      }
+      T = mk(in.tp[0][q], in.tp[1][q], in.tp[2][q]);
+      L = mk(in.rad[0][q], in.rad[1][q], in.rad[2][q]);
+      rng = in.rng[q];
+      if constexpr (kAdj) ai = load_item(A.items[in_buf], A.stride, q);
       shade_path<kAdj>(P, scene_view(sc, sid), st, depth, p, ro, rd, P.hit_t[q], P.hit_prim[q], T, L, rng, so,
-                       shadow_tests, A, glds);
-      if constexpr (kAdj) {
+                       shadow_tests, A, glds, ai);
+      if (!kAdj && !so.cont) {
+        // The path ends here: its radiance goes to its slot, and so does the number of extension rays
+        // it traced (depth + 1), which k_film takes back out of the ray counter for a dropped
+        // (non-finite) sample (shade_pixel adds ray_count only for finite samples).
         P.rad[0][p] = L.x; P.rad[1][p] = L.y; P.rad[2][p] = L.z;
-        if (so.cont) {
-          P.tp[0][p] = T.x; P.tp[1][p] = T.y; P.tp[2][p] = T.z;
-          P.rng[p] = rng;
-        }
-      } else if (!so.cont) {
-        // The path ends here: its radiance goes to its slot. It traced depth + 1 extension rays; its
-        // throughput is dead, so the slot's first throughput word keeps that count for k_film, which
-        // takes the rays of a dropped (non-finite) sample back out of the ray counter (shade_pixel
-        // adds ray_count only for finite samples).
-        P.rad[0][p] = L.x; P.rad[1][p] = L.y; P.rad[2][p] = L.z;
-        P.tp[0][p] = __uint_as_float(depth + 1u);
+        P.ended_rays[p] = depth + 1u;
       }
     }
     const uint32_t seg = (w0 >> 6) & (kSegments - 1u);  // this chunk's output segment
@@ -2099,11 +2116,10 @@ __device__ __forceinline__ void shade_kernel(const DevPaths& P, const DevRays& i
       out.c[3][slot] = so.nd.x; out.c[4][slot] = so.nd.y; out.c[5][slot] = so.nd.z;
       out.qpath[slot] = p;
       out.qscene[slot] = sid;
-      if constexpr (!kAdj) {
-        out.tp[0][slot] = T.x; out.tp[1][slot] = T.y; out.tp[2][slot] = T.z;
-        out.rad[0][slot] = L.x; out.rad[1][slot] = L.y; out.rad[2][slot] = L.z;
-        out.rng[slot] = rng;
-      }
+      out.tp[0][slot] = T.x; out.tp[1][slot] = T.y; out.tp[2][slot] = T.z;
+      out.rad[0][slot] = L.x; out.rad[1][slot] = L.y; out.rad[2][slot] = L.z;
+      out.rng[slot] = rng;
+      if constexpr (kAdj) store_item(A.items[in_buf ^ 1u], A.stride, slot, ai);
     }
     const uint32_t sslot = segment_append(P.queue + kQueueShadowCursor, seg, P.seg_cap, so.shadow);
     if (so.shadow) {
@@ -2111,8 +2127,9 @@ __device__ __forceinline__ void shade_kernel(const DevPaths& P, const DevRays& i
       P.sray[3][sslot] = so.sd.x; P.sray[4][sslot] = so.sd.y; P.sray[5][sslot] = so.sd.z;
       P.stmax[sslot] = so.stmax;
       P.scontrib[0][sslot] = so.contrib.x; P.scontrib[1][sslot] = so.contrib.y; P.scontrib[2][sslot] = so.contrib.z;
-      if constexpr (kAdj) P.spath[sslot] = p;
-      else P.spath[sslot] = so.cont ? (kRadQueued | slot) : p;
+      // where the contribution goes if the light is visible: an adjoint replay needs the radiance of
+      // continuing paths only (the remainder of their pending bounce)
+      P.spath[sslot] = so.cont ? (kRadQueued | slot) : (kAdj ? kRadNone : p);
       P.sscene[sslot] = sid;
       if constexpr (kAdj) {
         A.s_galb[0][sslot] = so.galb.x; A.s_galb[1][sslot] = so.galb.y; A.s_galb[2][sslot] = so.galb.z;
@@ -2139,7 +2156,7 @@ __device__ __forceinline__ void shade_kernel(const DevPaths& P, const DevRays& i
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PT_SHADE_WAVES, 8))) void k_shade(
     DevPaths P, DevRays in, DevRays out, DevScene sc, DevSettings st,
                                                  uint32_t depth) {
-  shade_kernel<false>(P, in, out, sc, st, depth, DevAdjoint{}, nullptr);
+  shade_kernel<false>(P, in, out, sc, st, depth, DevAdjoint{}, nullptr, 0u);
 }
 
 // The adjoint shade holds about twice the forward's live values; letting it run PT_SHADE_ADJ_WAVES
@@ -2148,9 +2165,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PT_SHADE
 #define PT_SHADE_ADJ_WAVES 2
 #endif
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PT_SHADE_ADJ_WAVES, 8))) void k_shade_adj(
-    DevPaths P, DevRays in, DevRays out, DevScene sc, DevSettings st, uint32_t depth, DevAdjoint A) {
+    DevPaths P, DevRays in, DevRays out, DevScene sc, DevSettings st, uint32_t depth, DevAdjoint A, uint32_t in_buf) {
   __shared__ float glds[kAdjLdsFloats + kTexTableFloats];
-  shade_kernel<true>(P, in, out, sc, st, depth, A, glds);
+  shade_kernel<true>(P, in, out, sc, st, depth, A, glds, in_buf);
 }
 
 // NEE visibility over the bounce's shadow queue. Opaque scenes: an any-hit query with
@@ -2165,17 +2182,17 @@ template <bool kAdj>
 __device__ __forceinline__ void light_visible(const DevPaths& P, const DevScene& sc, const DevAdjoint& A, float* glds,
                                               const RadTarget& rq, uint32_t q) {
   const uint32_t target = P.spath[q];
-  float* const* rad = P.rad;
-  uint32_t k = target;
-  if constexpr (!kAdj) {
+  if (target != kRadNone) {
+    float* const* rad = P.rad;
+    uint32_t k = target;
     if (target & kRadQueued) {
       rad = rq.rad;
       k = target & ~kRadQueued;
     }
+    rad[0][k] = rad[0][k] + P.scontrib[0][q];
+    rad[1][k] = rad[1][k] + P.scontrib[1][q];
+    rad[2][k] = rad[2][k] + P.scontrib[2][q];
   }
-  rad[0][k] = rad[0][k] + P.scontrib[0][q];
-  rad[1][k] = rad[1][k] + P.scontrib[1][q];
-  rad[2][k] = rad[2][k] + P.scontrib[2][q];
   if constexpr (kAdj) {
     const uint32_t mi = A.s_mat[q], li = A.s_light[q] & ~kShadowVisible;
     gadd_base(A, sc, glds, mi, make_float2(A.s_uv[0][q], A.s_uv[1][q]),
@@ -2300,7 +2317,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow_apply(DevPaths P, DevScene sc
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_shadow_grad(DevPaths P, DevScene sc, DevAdjoint A) {
+__global__ __launch_bounds__(kBlock) void k_shadow_grad(DevPaths P, DevScene sc, DevAdjoint A, RadTarget rq) {
   __shared__ float glds[kAdjLdsFloats + kTexTableFloats];
   __shared__ uint32_t prefix[kSegments + 1];
   tex_table_init(glds);
@@ -2312,7 +2329,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow_grad(DevPaths P, DevScene sc,
     const uint32_t i = w0 + lane;
     if (i < n) {
       const uint32_t q = queue_pos(prefix, i, P.seg_cap);
-      if (A.s_light[q] & kShadowVisible) light_visible<true>(P, sc, A, glds, RadTarget{}, q);
+      if (A.s_light[q] & kShadowVisible) light_visible<true>(P, sc, A, glds, rq, q);
     }
     tex_grad_flush(A, glds);  // the wave is converged here
   }
@@ -2357,7 +2374,7 @@ __global__ __launch_bounds__(64) void k_film(DevPaths P, DevScene sc, DevFilm fi
   auto add = [&](float r, float g, float b, uint32_t slot) {
     const bool ok = finite(r) && finite(g) && finite(b);  // shade_pixel drops it, rays included (Render.cpp:270-281)
     uint32_t drop = 0;
-    if (!ok) drop = __float_as_uint(P.tp[0][slot]);
+    if (!ok) drop = P.ended_rays[slot];
     dropped_rays += drop;
     added += ok ? 1u : 0u;
     if (ok) {
@@ -2555,12 +2572,6 @@ void launch_grad_reduce(hipStream_t stream, const DevAdjoint& adj) {
   hipLaunchKernelGGL(k_grad_reduce, dim3((n + kBlock - 1) / kBlock, kReduceGroups), dim3(kBlock), 0, stream, adj);
 }
 
-void launch_copy_radiance(hipStream_t stream, const DevPaths& paths, const DevAdjoint& adj, uint32_t path_count) {
-  if (path_count == 0) return;
-  hipLaunchKernelGGL(k_copy_radiance, dim3((path_count + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, paths, adj,
-                     path_count);
-}
-
 void launch_extend(hipStream_t stream, const DevPaths& paths, const DevScene& scene, int in_buf, uint32_t grid) {
   hipLaunchKernelGGL(k_extend, dim3(grid), dim3(kBlock), 0, stream, paths, paths.rays[in_buf], scene);
   if (scene.has_sdf)
@@ -2577,7 +2588,7 @@ void launch_shade_adjoint(hipStream_t stream, const DevPaths& paths, const DevAd
                           const DevSettings& st, int in_buf, uint32_t depth, uint32_t grid) {
   grid = grid < kGradRows ? grid : kGradRows;  // one partial row per block
   hipLaunchKernelGGL(k_shade_adj, dim3(grid), dim3(kBlock), 0, stream, paths, paths.rays[in_buf],
-                     paths.rays[in_buf ^ 1], scene, st, depth, adj);
+                     paths.rays[in_buf ^ 1], scene, st, depth, adj, static_cast<uint32_t>(in_buf));
 }
 
 void launch_shadow(hipStream_t stream, const DevPaths& paths, const DevScene& scene, int in_buf, uint32_t grid) {
@@ -2597,17 +2608,16 @@ void launch_shadow(hipStream_t stream, const DevPaths& paths, const DevScene& sc
 }
 
 void launch_shadow_adjoint(hipStream_t stream, const DevPaths& paths, const DevAdjoint& adj, const DevScene& scene,
-                           uint32_t grid) {
+                           int in_buf, uint32_t grid) {
+  const DevRays& o = paths.rays[in_buf ^ 1];
+  const RadTarget rq{{o.rad[0], o.rad[1], o.rad[2]}};
   if (scene.has_sdf)
-    hipLaunchKernelGGL((k_shadow<true, true, false>), dim3(grid), dim3(kBlock), 0, stream, paths, scene, adj,
-                       RadTarget{});
+    hipLaunchKernelGGL((k_shadow<true, true, false>), dim3(grid), dim3(kBlock), 0, stream, paths, scene, adj, rq);
   else if (scene.has_transmissive)
-    hipLaunchKernelGGL((k_shadow<true, false, false>), dim3(grid), dim3(kBlock), 0, stream, paths, scene, adj,
-                       RadTarget{});
+    hipLaunchKernelGGL((k_shadow<true, false, false>), dim3(grid), dim3(kBlock), 0, stream, paths, scene, adj, rq);
   else
-    hipLaunchKernelGGL((k_shadow<true, false, true>), dim3(grid), dim3(kBlock), 0, stream, paths, scene, adj,
-                       RadTarget{});
-  hipLaunchKernelGGL(k_shadow_grad, dim3(kGradBlocks), dim3(kBlock), 0, stream, paths, scene, adj);
+    hipLaunchKernelGGL((k_shadow<true, false, true>), dim3(grid), dim3(kBlock), 0, stream, paths, scene, adj, rq);
+  hipLaunchKernelGGL(k_shadow_grad, dim3(kGradBlocks), dim3(kBlock), 0, stream, paths, scene, adj, rq);
 }
 
 void launch_film(hipStream_t stream, const DevPaths& paths, const DevScene& scene, const DevFilm& film,
