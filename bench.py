@@ -36,11 +36,11 @@ HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E
 # with reference SampleCounters semantics): 32 bytes per BVH node visit (one child box), 48 per
 # triangle test (v0, e1, e2 padded to float4 records), an extension ray 32 in + 16 out, a shadow ray
 # 32 in + 12 contribution + 4 result, a shade invocation 2 x 40 path state + 64 material, a film
-# sample 16 (read-modify-write); raygen writes 60 bytes per path (ray 24, throughput and radiance 24,
-# RNG 8, queue entry 4).
+# sample 16 (read-modify-write); raygen writes 36 bytes per path (ray 24, RNG 8, scene index 4; the
+# first bounce takes throughput 1, radiance 0 and the identity path slot without reading them).
 BYTES = {
     "node_visit": 32, "triangle": 48, "ext_ray": 32 + 16, "shadow_ray": 32 + 12 + 4, "shade": 2 * 40 + 64,
-    "film_sample": 16, "raygen_path": 60,
+    "film_sample": 16, "raygen_path": 36,
 }
 
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this workload

@@ -851,12 +851,10 @@ __global__ __launch_bounds__(kBlock) void k_raygen(DevPaths P, DevRays out, DevS
   const uint64_t prng = rng_seed(st.seed + sample_seed + seed_mix, frame, sample, path_id, sample_seed);
   out.c[0][p] = ro.x; out.c[1][p] = ro.y; out.c[2][p] = ro.z;
   out.c[3][p] = rd.x; out.c[4][p] = rd.y; out.c[5][p] = rd.z;
-  out.qpath[p] = p;
   out.qscene[p] = tile.scene;
-  // The initial queue is the identity (entry p serves path p). Settings keep max_depth >= 1, so every
-  // forward path ends in some bounce's shade, which writes its radiance to its slot.
-  out.tp[0][p] = 1.0f; out.tp[1][p] = 1.0f; out.tp[2][p] = 1.0f;
-  out.rad[0][p] = 0.0f; out.rad[1][p] = 0.0f; out.rad[2][p] = 0.0f;
+  // The initial queue is the identity (entry p serves path p, so its path slot is not stored).
+  // Throughput 1 and radiance 0 are not stored either: the first bounce's shade starts from them. Settings keep max_depth >= 1, so every forward
+  // path ends in some bounce's shade, which writes its radiance to its slot.
   out.rng[p] = prng;
   if constexpr (kAdj) {
     // the replay's items (buffer 0, beside the initial queue): the forward pass's radiance of the path,
@@ -2091,12 +2089,16 @@ __device__ __forceinline__ void shade_kernel(const DevPaths& P, const DevRays& i
     AdjItem ai;
     if (i < n) {
       const uint32_t q = queue_pos(prefix, i, P.seg_cap);
-      p = in.qpath[q];
+      p = depth > 0u ? in.qpath[q] : q;  // the first bounce's queue is the identity
       sid = in.qscene[q];
       const V3 ro = mk(in.c[0][q], in.c[1][q], in.c[2][q]);
       const V3 rd = mk(in.c[3][q], in.c[4][q], in.c[5][q]);
-      T = mk(in.tp[0][q], in.tp[1][q], in.tp[2][q]);
-      L = mk(in.rad[0][q], in.rad[1][q], in.rad[2][q]);
+      if (depth > 0u) {  // camera rays start with throughput 1 and radiance 0 (k_raygen stores neither)
+        T = mk(in.tp[0][q], in.tp[1][q], in.tp[2][q]);
+        L = mk(in.rad[0][q], in.rad[1][q], in.rad[2][q]);
+      } else {
+        T = mk(1.0f, 1.0f, 1.0f);
+      }
       rng = in.rng[q];
       if constexpr (kAdj) ai = load_item(A.items[in_buf], A.stride, q);
       shade_path<kAdj>(P, scene_view(sc, sid), st, depth, p, ro, rd, P.hit_t[q], P.hit_prim[q], T, L, rng, so,
