@@ -346,6 +346,71 @@ def textured_multiview(log, reps: int = 2, views: int = 16, size: int = 512, spp
             "backward_over_forward": round(bk / f, 3), "rays_per_frame": counters["rays"]}
 
 
+# The reference's published render times for this path, all on its Cornell scene
+# assets/scenes/cornell_native.json (48 triangles) with ptbench's defaults: max depth 6, NEE + MIS,
+# seed 0xBADC0FFEE. Samples per pixel = spp x rays per pixel per dispatch. Each entry: published
+# render time (ms) of a frame, and where it is quoted.
+REFERENCE_PUBLISHED = [
+    {"width": 96, "height": 96, "samples": 1, "published": {
+        "cpu_scalar_ms": (5.4613, "experiments/pathtracer_optimization_graph/NOTES.md:165"),
+        "vulkan_compute_arc_b580_ms": (4.5266, "experiments/pathtracer_optimization_graph/NOTES.md:166-167")}},
+    {"width": 64, "height": 64, "samples": 2, "published": {
+        "cpu_scalar_ms": (6.0917, "experiments/d3d12_dxr_optimization_graph/NOTES.md:36"),
+        "d3d12_compute_ms": (1.6897, "experiments/d3d12_dxr_optimization_graph/NOTES.md:36")}},
+    {"width": 64, "height": 64, "samples": 16, "published": {
+        "d3d12_compute_rpp8_best_bvh_ms": (4.11, "experiments/d3d12_dxr_algorithm_graph/NOTES.md:22,53-55")}},
+    {"width": 128, "height": 128, "samples": 16, "published": {
+        "d3d12_compute_rpp8_ms": (5.04725, "experiments/d3d12_dxr_algorithm_graph/NOTES.md:95")}},
+]
+
+
+def reference_configs(log, reps: int = 30) -> dict:
+    """The reference's own published configurations: its Cornell scene (the committed copy of
+    cornell_native.json, converted by the scene-document ingest) at each published resolution and sample
+    count, ptbench's protocol (BVH built before timing; a frame = reset_accumulation + the frame's samples,
+    here one render_tiles call), the median of `reps` frames, beside the published render time of the same
+    frame on the reference's CPU and GPU backends."""
+    import torch
+
+    from vkpathtracer_amd import scene as S
+    from vkpathtracer_amd import scene_document as SD
+    from vkpathtracer_amd.tracer import HipPathTracer
+
+    sc = SD.load_scene_document(os.path.join(ROOT, "tests", "golden", "scenes", "cornell_native.json"))
+    rows = []
+    for cfg in REFERENCE_PUBLISHED:
+        st = S.RenderSettings(width=cfg["width"], height=cfg["height"], spp=cfg["samples"], max_depth=6)
+        tiles = [(0, 0, st.width, st.height, s, 0, 0) for s in range(st.spp)]
+        with HipPathTracer(0) as pt:
+            assert pt.configure(st) and pt.load_scene_snapshot(sc) and pt.build_or_update_acceleration()
+
+            def frame():
+                pt.reset_accumulation()
+                assert pt.render_tiles(tiles)
+
+            for _ in range(3):
+                frame()
+            pt.synchronize()
+            times = []
+            for _ in range(reps):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                frame()
+                pt.synchronize()
+                times.append((time.perf_counter() - t0) * 1e3)
+        ms = float(np.median(times))
+        paths = st.width * st.height * st.spp
+        best = min(v[0] for v in cfg["published"].values())
+        rows.append({"resolution": f"{st.width}x{st.height}", "samples_per_pixel": st.spp, "ms": round(ms, 4),
+                     "mpaths_per_s": round(paths / ms / 1e3, 2),
+                     "published_ms": {k: v[0] for k, v in cfg["published"].items()},
+                     "published_source": {k: v[1] for k, v in cfg["published"].items()},
+                     "speedup_vs_best_published": round(best / ms, 2)})
+        log(f"cornell_native {st.width}x{st.height}x{st.spp}: {ms:.3f} ms (best published {best} ms)")
+    return {"scene": "cornell_native.json (48 triangles), max depth 6, NEE + MIS, seed 0xbadc0ffee, SAH leaf 4",
+            "configs": rows}
+
+
 def batched_comparison(log, count: int = 64, reps: int = 2) -> dict:
     """Many scenes in one batch against the same scenes rendered one tracer at a time.
 
@@ -435,6 +500,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward time ratio")
     ap.add_argument("--no-batched", action="store_true", help="skip the many-scene batch comparison")
     ap.add_argument("--no-textured", action="store_true", help="skip the textured multi-view adjoint")
+    ap.add_argument("--no-reference-configs", action="store_true",
+                    help="skip the reference's published Cornell configurations")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run a plain device copy of this size first (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -557,6 +624,7 @@ def main():
     adjoint = None if args.no_adjoint else adjoint_ratio(log)
     batched = None if args.no_batched else batched_comparison(log)
     textured = None if args.no_textured else textured_multiview(log)
+    refcfg = None if args.no_reference_configs else reference_configs(log)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(sc, st, args.cpu_seconds, log)
@@ -593,6 +661,7 @@ def main():
         "adjoint": adjoint,
         "batched": batched,
         "textured_multiview": textured,
+        "reference_configs": refcfg,
     }
     if cpu:
         out["speedup_vs_cpu"] = round(mpaths / cpu["value"], 1)

@@ -64,7 +64,12 @@ constexpr int kQueueShadowCursor = kQueueOutCursor + kSegments * kSegLine;
 // that have finished; the launch's last wave sets both back to zero.
 constexpr int kQueueChunkCursor = kQueueShadowCursor + kSegments * kSegLine;
 constexpr int kQueueWavesDone = kQueueChunkCursor + kSegLine;
-constexpr int kQueueWords = kQueueWavesDone + kSegLine;
+// The finished waves are counted in kDoneStripes stripes (wave w in stripe w % kDoneStripes, one line
+// each); kQueueWavesDone counts the completed stripes. One counter for all waves would serialise every
+// wave of a launch on one address (tens of microseconds for a resident grid).
+constexpr int kDoneStripes = 32;
+constexpr int kQueueDoneStripe = kQueueWavesDone + kSegLine;
+constexpr int kQueueWords = kQueueDoneStripe + kDoneStripes * kSegLine;
 
 __host__ __device__ constexpr uint32_t segment_capacity(uint32_t capacity) {
   return 64u * ((((capacity + 63u) / 64u) + kSegments - 1u) / kSegments);

@@ -391,12 +391,19 @@ __device__ __forceinline__ uint32_t wave_grab(uint32_t* cursor) {
   return __builtin_amdgcn_readfirstlane(v);
 }
 
-// A wave leaves drive_queries. The grid's last wave resets the chunk cursor and the count for the
-// next launch on the stream: every other wave's last grab returned before it counted itself here.
+// A wave leaves drive_queries. It counts itself in its stripe; a stripe's last wave counts the stripe,
+// and the last stripe's resets the chunk cursor and every count for the next launch on the stream:
+// every other wave's last grab returned before it counted itself here.
 __device__ __forceinline__ void wave_finish(uint32_t* queue, uint32_t waves) {
   if (lanes_below(wave_ballot(true)) == 0u) {
-    if (atomicAdd(queue + kQueueWavesDone, 1u) == waves - 1u) {
+    constexpr uint32_t D = static_cast<uint32_t>(kDoneStripes);
+    const uint32_t st = global_wave() % D;
+    const uint32_t in_stripe = waves / D + (st < waves % D ? 1u : 0u);  // waves w < waves with w % D == st
+    const uint32_t stripes = waves < D ? waves : D;                      // stripes holding a wave
+    if (atomicAdd(queue + kQueueDoneStripe + st * kSegLine, 1u) == in_stripe - 1u &&
+        atomicAdd(queue + kQueueWavesDone, 1u) == stripes - 1u) {
       atomicExch(queue + kQueueChunkCursor, 0u);
+      for (uint32_t k = 0; k < D; ++k) atomicExch(queue + kQueueDoneStripe + k * kSegLine, 0u);
       atomicExch(queue + kQueueWavesDone, 0u);
     }
   }
@@ -564,6 +571,7 @@ __global__ void k_init_queue(uint32_t* queue, uint32_t live, uint32_t seg_cap) {
     queue[kQueueOutCursor + s * kSegLine] = 0u;
     queue[kQueueShadowCursor + s * kSegLine] = 0u;
   }
+  if (s < static_cast<uint32_t>(kDoneStripes)) queue[kQueueDoneStripe + s * kSegLine] = 0u;
   if (s == 0u) {
     queue[kQueueChunkCursor] = 0u;
     queue[kQueueWavesDone] = 0u;
