@@ -1130,19 +1130,24 @@ int plan_passes(pt_tracer* t, std::vector<ptd::DevTile>& work, std::vector<Pass>
   return stage_upload(t, t->d_plan.ptr, blob.data(), blob.size());
 }
 
-// The bounce loop of one wavefront pass (extend -> shade -> shadow -> advance per depth).
-void bounce_loop(pt_tracer* t, const ptd::DevPaths& P, const ptd::DevSettings& st, const ptd::DevAdjoint* adj) {
+// The bounce loop of one wavefront pass (extend -> shade -> shadow -> advance per depth) of `paths`
+// paths. No queue of the pass holds more than `paths` entries, so no launch needs more than one
+// workgroup per 256 of them (a wave takes 64 entries at a time): small passes launch small grids.
+void bounce_loop(pt_tracer* t, const ptd::DevPaths& P, const ptd::DevSettings& st, const ptd::DevAdjoint* adj,
+                 uint32_t paths) {
+  const uint32_t need = std::max<uint32_t>(1u, (paths + 255u) / 256u);
+  const auto grid = [need](uint32_t full) { return std::min(full, need); };
   int buf = 0;
   for (uint32_t depth = 0; depth < st.max_depth; ++depth) {
-    timed(t, PT_KERNEL_EXTEND, 0, [&] { ptd::launch_extend(t->stream, P, t->dscene, buf, t->grids.extend); });
+    timed(t, PT_KERNEL_EXTEND, 0, [&] { ptd::launch_extend(t->stream, P, t->dscene, buf, grid(t->grids.extend)); });
     timed(t, PT_KERNEL_SHADE, 0, [&] {
-      if (adj) ptd::launch_shade_adjoint(t->stream, P, *adj, t->dscene, st, buf, depth, t->grids.shade_adj);
-      else ptd::launch_shade(t->stream, P, t->dscene, st, buf, depth, t->grids.shade);
+      if (adj) ptd::launch_shade_adjoint(t->stream, P, *adj, t->dscene, st, buf, depth, grid(t->grids.shade_adj));
+      else ptd::launch_shade(t->stream, P, t->dscene, st, buf, depth, grid(t->grids.shade));
     });
     if (st.enable_nee && depth + 1u < st.max_depth)
       timed(t, PT_KERNEL_SHADOW, 0, [&] {
-        if (adj) ptd::launch_shadow_adjoint(t->stream, P, *adj, t->dscene, buf, t->grids.shadow_adj);
-        else ptd::launch_shadow(t->stream, P, t->dscene, buf, t->grids.shadow);
+        if (adj) ptd::launch_shadow_adjoint(t->stream, P, *adj, t->dscene, buf, grid(t->grids.shadow_adj));
+        else ptd::launch_shadow(t->stream, P, t->dscene, buf, grid(t->grids.shadow));
       });
     ptd::launch_advance(t->stream, P);
     buf ^= 1;
@@ -1185,7 +1190,7 @@ int launch_passes(pt_tracer* t, const std::vector<ptd::DevTile>& work, const std
     if (!reuse) {
       ptd::launch_init_queue(t->stream, P, ps.paths);
       timed(t, PT_KERNEL_RAYGEN, ps.paths, [&] { ptd::launch_raygen(t->stream, P, t->dscene, st, tl, nt, ps.paths); });
-      bounce_loop(t, P, st, nullptr);
+      bounce_loop(t, P, st, nullptr, ps.paths);
     }
     if (adj) {
       // The adjoint replay leaves the path slots' radiance alone; a forward replay above has replaced it
@@ -1194,7 +1199,7 @@ int launch_passes(pt_tracer* t, const std::vector<ptd::DevTile>& work, const std
       ptd::launch_init_queue(t->stream, P, ps.paths);
       timed(t, PT_KERNEL_RAYGEN, ps.paths,
             [&] { ptd::launch_raygen_adjoint(t->stream, P, *adj, t->dscene, st, tl, nt, ps.paths); });
-      bounce_loop(t, P, st, adj);
+      bounce_loop(t, P, st, adj, ps.paths);
     } else {
       timed(t, PT_KERNEL_FILM, ps.paths, [&] {
         ptd::launch_film(t->stream, P, t->dscene, film, tl, djobs + ps.job_first, static_cast<uint32_t>(ps.job_count),

@@ -2613,7 +2613,8 @@ void launch_shadow(hipStream_t stream, const DevPaths& paths, const DevScene& sc
   else {
     hipLaunchKernelGGL((k_shadow<false, false, true>), dim3(grid), dim3(kBlock), 0, stream, paths, scene, DevAdjoint{},
                        rq);
-    hipLaunchKernelGGL(k_shadow_apply, dim3(kShadowApplyBlocks), dim3(kBlock), 0, stream, paths, scene, rq);
+    hipLaunchKernelGGL(k_shadow_apply, dim3(grid < kShadowApplyBlocks ? grid : kShadowApplyBlocks), dim3(kBlock), 0,
+                       stream, paths, scene, rq);
   }
 }
 
@@ -2627,7 +2628,8 @@ void launch_shadow_adjoint(hipStream_t stream, const DevPaths& paths, const DevA
     hipLaunchKernelGGL((k_shadow<true, false, false>), dim3(grid), dim3(kBlock), 0, stream, paths, scene, adj, rq);
   else
     hipLaunchKernelGGL((k_shadow<true, false, true>), dim3(grid), dim3(kBlock), 0, stream, paths, scene, adj, rq);
-  hipLaunchKernelGGL(k_shadow_grad, dim3(kGradBlocks), dim3(kBlock), 0, stream, paths, scene, adj, rq);
+  hipLaunchKernelGGL(k_shadow_grad, dim3(grid < kGradBlocks ? grid : kGradBlocks), dim3(kBlock), 0, stream, paths,
+                     scene, adj, rq);
 }
 
 void launch_film(hipStream_t stream, const DevPaths& paths, const DevScene& scene, const DevFilm& film,
