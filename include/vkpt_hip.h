@@ -411,8 +411,8 @@ int pt_hip_apply_scene_delta(pt_tracer* tracer, const PtSceneDeltaUpdate* update
  * taken with the texture held fixed and each texel's with the albedo held fixed. All are device
  * pointers; gradients are added to the buffers' contents. The film and the render counters are not
  * modified. Requires rr_start_depth <= 4. Right after render_tiles of the same tiles and frame (no
- * scene change between), the forward pass is not replayed: each path's radiance from that render
- * is reused. */
+ * scene change between; other adjoints of that render may come between), the forward pass is not
+ * replayed: each path's radiance from that render is reused. */
 int pt_hip_render_adjoint(pt_tracer* tracer, const PtRenderTile* tiles, uint32_t count, uint64_t frame_index,
                           const float* film_grad, float* grad_albedo, float* grad_emission, float* grad_light,
                           float* grad_roughness, float* grad_texels);
