@@ -346,7 +346,7 @@ def textured_multiview(log, reps: int = 2, views: int = 16, size: int = 512, spp
             "backward_over_forward": round(bk / f, 3), "rays_per_frame": counters["rays"]}
 
 
-# The reference's published render times for this path, all on its Cornell scene
+# The render times the reference publishes for this path, all on its Cornell scene
 # assets/scenes/cornell_native.json (48 triangles) with ptbench's defaults: max depth 6, NEE + MIS,
 # seed 0xBADC0FFEE. Samples per pixel = spp x rays per pixel per dispatch. Each entry: published
 # render time (ms) of a frame, and where it is quoted.
