@@ -88,14 +88,28 @@ def _timed(frame: Callable[[], None], sync: Callable[[], None], reps: int) -> fl
     return best
 
 
-def _capacities(paths: int) -> list:
-    """The default (the whole frame in one pass, up to 128 Mi paths), twice that for frames larger
-    than the default, and 1/2 and 1/4 of the frame. At most 256 Mi paths (about 40 GB of path state)."""
+PATH_STATE_BYTES = 200  # device bytes per path of wavefront capacity (forward path state and queues)
+
+
+def _capacities(paths: int, free_bytes: int | None = None) -> list:
+    """The default (the whole frame in one pass, up to 128 Mi paths), the whole frame, twice the
+    default, and 1/2 and 1/4 of the frame. At most 1 Gi paths (about 200 GB of path state, which an
+    MI355X's 288 GB hold) and at most what 80 % of the device's free memory holds."""
+    limit = 1 << 30
+    if free_bytes is None:
+        try:
+            import torch
+
+            free_bytes = torch.cuda.mem_get_info()[0]
+        except Exception:  # no device: the static limit only
+            free_bytes = None
+    if free_bytes is not None:
+        limit = min(limit, int(0.8 * free_bytes) // PATH_STATE_BYTES)
     default = max(1024, min(paths, 128 << 20))
-    cands = [default, min(paths, 256 << 20), paths // 2, paths // 4]
+    cands = [default, paths, min(paths, 256 << 20), paths // 2, paths // 4]
     out = []
     for c in cands:
-        c = max(1024, min(int(c), 256 << 20))
+        c = max(1024, min(int(c), limit))
         if c not in out:
             out.append(c)
     return out
