@@ -1,5 +1,5 @@
 """The autotuner's search (coordinate descent over a discrete space) on synthetic cost functions."""
-from vkpathtracer_amd.autotune import tune
+from vkpathtracer_amd.autotune import PATH_STATE_BYTES, fitting_capacity, tune
 
 
 def test_finds_the_minimum_of_a_separable_cost():
@@ -29,3 +29,15 @@ def test_each_point_is_measured_once_and_noise_does_not_move():
     r = tune(cost, {"a": [0, 1, 2]}, {"a": 0})
     assert r.params == {"a": 0}
     assert len(calls) == len(set(calls)) == 3
+
+
+def test_tuned_capacity_is_limited_to_free_device_memory():
+    """A whole-frame capacity is kept when its path state fits in 80 % of the free memory, cut to what
+    fits otherwise, and never below 1 Mi paths (or the capacity itself when smaller)."""
+    cap = 874971136
+    assert fitting_capacity(cap, 288 << 30) == cap
+    small = fitting_capacity(cap, 64 << 30)
+    assert small < cap and small * PATH_STATE_BYTES <= 0.8 * (64 << 30)
+    assert small == int(0.8 * (64 << 30)) // PATH_STATE_BYTES
+    assert fitting_capacity(cap, 1 << 20) == 1 << 20
+    assert fitting_capacity(5000, 1 << 20) == 5000

@@ -214,10 +214,24 @@ def tuned_params(workload: str) -> dict | None:
         return None
 
 
-def apply_params(target, params: dict) -> None:
+def fitting_capacity(capacity: int, free_bytes: int | None = None) -> int:
+    """`capacity` paths, or fewer when their path state would not fit in 80 % of the device's free
+    memory (at least 1 Mi paths): a tuned whole-frame pass on a device with less free memory renders in
+    more, smaller passes instead of failing to allocate."""
+    if free_bytes is None:
+        try:
+            import torch
+
+            free_bytes = torch.cuda.mem_get_info()[0]
+        except Exception:  # no device to ask
+            return capacity
+    return max(min(capacity, 1 << 20), min(capacity, int(0.8 * free_bytes) // PATH_STATE_BYTES))
+
+
+def apply_params(target, params: dict, free_bytes: int | None = None) -> None:
     """Apply the launch-shape part of `params` to a HipPathTracer or HipBatch (the leaf width is a
-    build parameter)."""
-    _apply(target, params)
+    build parameter), with the wavefront capacity limited to what the device's free memory holds."""
+    _apply(target, dict(params, wavefront_capacity=fitting_capacity(params["wavefront_capacity"], free_bytes)))
 
 
 def main(argv=None):
