@@ -62,8 +62,11 @@ constexpr int kQueueOutCursor = kSegments;                        // kSegments c
 constexpr int kQueueShadowCursor = kQueueOutCursor + kSegments * kSegLine;
 // Traversal scheduling: the next chunk of queries to hand out, and the waves of the running launch
 // that have finished; the launch's last wave sets both back to zero.
+// Traversal chunks are handed out from one cursor per region of the queue (kRegions regions, one per
+// XCD, each cursor on its own line).
+constexpr int kRegions = 8;
 constexpr int kQueueChunkCursor = kQueueShadowCursor + kSegments * kSegLine;
-constexpr int kQueueWavesDone = kQueueChunkCursor + kSegLine;
+constexpr int kQueueWavesDone = kQueueChunkCursor + kRegions * kSegLine;
 // The finished waves are counted in kDoneStripes stripes (wave w in stripe w % kDoneStripes, one line
 // each); kQueueWavesDone counts the completed stripes. One counter for all waves would serialise every
 // wave of a launch on one address (tens of microseconds for a resident grid).

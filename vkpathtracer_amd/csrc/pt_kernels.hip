@@ -374,6 +374,13 @@ __device__ __forceinline__ uint32_t global_wave() {
   return __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
 }
 
+// The XCD (accelerator complex die) the wave runs on, read from the XCC_ID hardware register.
+__device__ __forceinline__ uint32_t xcc_id() {
+  uint32_t v;
+  asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
 // Ballot of a lane predicate straight from its compare mask (no select to a VGPR and back).
 __device__ __forceinline__ uint64_t wave_ballot(bool pred) { return __builtin_amdgcn_ballot_w64(pred); }
 
@@ -402,26 +409,44 @@ __device__ __forceinline__ void wave_finish(uint32_t* queue, uint32_t waves) {
     const uint32_t stripes = waves < D ? waves : D;                      // stripes holding a wave
     if (atomicAdd(queue + kQueueDoneStripe + st * kSegLine, 1u) == in_stripe - 1u &&
         atomicAdd(queue + kQueueWavesDone, 1u) == stripes - 1u) {
-      atomicExch(queue + kQueueChunkCursor, 0u);
+      for (int r = 0; r < kRegions; ++r) atomicExch(queue + kQueueChunkCursor + r * kSegLine, 0u);
       for (uint32_t k = 0; k < D; ++k) atomicExch(queue + kQueueDoneStripe + k * kSegLine, 0u);
       atomicExch(queue + kQueueWavesDone, 0u);
     }
   }
 }
 
-// Persistent query driver with per-lane refill. The queries [0, n) are cut into 64-entry chunks;
-// wave w starts on chunk w, and a wave that has handed out its chunk takes the next one from a
-// shared cursor (chunk W + cursor, W = waves in the grid), so waves that drew cheap rays take more
-// chunks and the launch ends evenly. A lane that finishes takes the next entry of its wave's chunk
-// as soon as `refill_min` lanes are idle, so a wave is not held by its slowest ray. Every wave of
-// the grid must call it exactly once (the last one resets the shared cursor).
+// Persistent query driver with per-lane refill. The queries [0, n) are cut into 64-entry chunks, and
+// the chunks into one contiguous region per XCD; a wave takes its chunks one at a time from a cursor
+// of its own XCD's region (then of the other regions, once its own is handed out), so waves that drew
+// cheap rays take more chunks and the launch ends evenly. A lane that finishes takes the next entry of
+// its wave's chunk as soon as `refill_min` lanes are idle, so a wave is not held by its slowest ray.
+// Every wave of the grid must call it exactly once (the last one resets the cursors).
 //   begin(j, s): start query j on this lane; false if it ended at once.
 //   end(j, s)  : the query finished; true if the lane started a follow-up query on s.
 template <bool kAny, typename Begin, typename End>
 __device__ __forceinline__ void drive_queries(const DevScene& sc, const Stack& stk, uint32_t* queue, uint32_t n,
                                               uint32_t refill_min, TravStats& st, Begin begin, End end) {
   const uint32_t waves = gridDim.x * kWavesPerBlock;
-  uint32_t chunk = global_wave();
+  // The queue's chunks form `regions` contiguous regions of R chunks (kRegions, or one when the grid
+  // has a wave for every chunk). A wave takes chunks of its own XCD's region from that region's
+  // cursor, and once that region is exhausted, of the following regions in turn; it leaves when every
+  // region is exhausted.
+  const uint32_t nchunks = (n + 63u) / 64u;
+  const uint32_t regions = nchunks > waves ? static_cast<uint32_t>(kRegions) : 1u;
+  const uint32_t R = (nchunks + regions - 1u) / regions;
+  uint32_t region = xcc_id() % regions;
+  uint32_t exhausted = 0;  // regions whose cursor has passed their end
+  auto grab = [&]() -> uint32_t {
+    while (true) {
+      const uint32_t c = wave_grab(queue + kQueueChunkCursor + region * kSegLine);
+      if (c < R && region * R + c < nchunks) return region * R + c;
+      exhausted |= 1u << region;
+      if (exhausted == (1u << regions) - 1u) return nchunks;  // every chunk handed out
+      while (exhausted & (1u << region)) region = (region + 1u) % regions;
+    }
+  };
+  uint32_t chunk = grab();
   uint32_t next = 0;  // entries of `chunk` already handed out
   bool active = false;
   uint32_t j = 0;
@@ -441,7 +466,7 @@ __device__ __forceinline__ void drive_queries(const DevScene& sc, const Stack& s
         }
         next += k < avail ? k : avail;
         if (next >= 64u || base + next >= n) {
-          chunk = waves + wave_grab(queue + kQueueChunkCursor);
+          chunk = grab();
           next = 0;
         }
       }
@@ -573,7 +598,7 @@ __global__ void k_init_queue(uint32_t* queue, uint32_t live, uint32_t seg_cap) {
   }
   if (s < static_cast<uint32_t>(kDoneStripes)) queue[kQueueDoneStripe + s * kSegLine] = 0u;
   if (s == 0u) {
-    queue[kQueueChunkCursor] = 0u;
+    for (int r = 0; r < kRegions; ++r) queue[kQueueChunkCursor + r * kSegLine] = 0u;
     queue[kQueueWavesDone] = 0u;
   }
 }
